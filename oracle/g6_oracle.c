@@ -1,0 +1,923 @@
+/*
+ * CPU oracle (test infrastructure only): FP64 serial restatement of the
+ * Genesis6 rigid-water interaction step.  Compiled with gcc -O2
+ * -ffp-contract=off so that every + - * / sqrt is a single IEEE operation in
+ * the same order as the reference's Fortran, which flang -O2 compiles for
+ * baseline x86-64 without FMA.  See g6_oracle.h for the per-function
+ * reference citations.  Not part of the product: the product library never
+ * links or calls it.
+ */
+#include "g6_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ---- physical constants (PhysConst.F90:5-62) --------------------------- */
+static const double Gear5_0 = 3.0 / 16.0, Gear5_1 = 251.0 / 360.0,
+                    Gear5_3 = 11.0 / 18.0, Gear5_4 = 1.0 / 6.0,
+                    Gear5_5 = 1.0 / 60.0;
+static const double Gear4_0 = 251.0 / 720.0, Gear4_2 = 11.0 / 12.0,
+                    Gear4_3 = 1.0 / 3.0, Gear4_4 = 1.0 / 24.0;
+#define PI_T 3.1415926535897932
+#define NA_ 6.0225e23
+#define KB_ 1.380066e-23
+#define CA_ 4.184
+static double rgas(void) { return NA_ * KB_; }
+static double coeff(void) { return CA_ * 332.17752; }
+static const double I2J = 10.0;
+static double J2I(void) { return 1.0 / 10.0; }
+static double KX2X(void) { return 1.0 / 1e-3; }
+
+/* x**k for small integer k as repeated multiplication */
+static double ipow(double x, int k) {
+  double r = x;
+  for (int i = 1; i < k; i++) r = r * x;
+  return r;
+}
+
+/* ---- models ------------------------------------------------------------ */
+int g6o_model_get(int model, g6o_model *m) {
+  memset(m, 0, sizeof(*m));
+  if (model == G6O_TIP4P) {
+    /* Rigid_TIP4P_Constructor, TIP4PModule.F90:70-97 (moly(5) unset = 0) */
+    m->nsite = 5;
+    m->Ixx = 1.75618411509879202e+00;
+    m->Iyy = 6.10236519209664374e-01;
+    m->Izz = 1.14594759588912742e+00;
+    m->mass = 18.0;
+    m->by[2] = 7.56950327263661182e-01;
+    m->by[3] = -7.56950327263661182e-01;
+    m->bz[0] = -6.50980307353661025e-02;
+    m->bz[1] = 8.49019692646338919e-02;
+    m->bz[2] = 5.20784245882928820e-01;
+    m->bz[3] = 5.20784245882928820e-01;
+    /* tip4p_setinteraction, TIP4PModule.F90:43-68 */
+    m->q[1] = -1.04;
+    m->q[2] = m->q[3] = +0.52;
+    m->eps[0] = 0.15504166666666666666666 * CA_;
+    m->sig[0] = 3.15357794197649532464;
+    return m->nsite;
+  }
+  if (model == G6O_TIP5P) {
+    /* Rigid_TIP5P_Constructor, RigidModule.F90:429-477 */
+    const double theta = 104.52 * PI_T / 180.0, phi = 109.47 * PI_T / 180.0;
+    const double l1 = 0.9572, l2 = 0.70, hmass = 1.0, omass = 16.0;
+    m->nsite = 6;
+    m->by[1] = l1 * sin(0.5 * theta);
+    m->bz[1] = l1 * cos(0.5 * theta);
+    m->by[2] = -l1 * sin(0.5 * theta);
+    m->bz[2] = l1 * cos(0.5 * theta);
+    m->bx[3] = l2 * sin(0.5 * phi);
+    m->bz[3] = -l2 * cos(0.5 * phi);
+    m->bx[4] = -l2 * sin(0.5 * phi);
+    m->bz[4] = -l2 * cos(0.5 * phi);
+    double zshift = m->bz[1] * 2.0 * hmass / (2.0 * hmass + omass);
+    for (int i = 0; i < 5; i++) m->bz[i] = m->bz[i] - zshift;
+    m->Ixx = omass * (m->by[0] * m->by[0] + m->bz[0] * m->bz[0]) +
+             hmass * (m->by[1] * m->by[1] + m->bz[1] * m->bz[1]) * 2.0;
+    m->Iyy = omass * (m->bz[0] * m->bz[0] + m->bx[0] * m->bx[0]) +
+             hmass * (m->bz[1] * m->bz[1] + m->bx[1] * m->bx[1]) * 2.0;
+    m->Izz = omass * (m->bx[0] * m->bx[0] + m->by[0] * m->by[0]) +
+             hmass * (m->bx[1] * m->bx[1] + m->by[1] * m->by[1]) * 2.0;
+    m->mass = 2.0 * hmass + omass;
+    /* tip5p_setinteraction, TIP5PModule.F90:57-83 */
+    m->q[1] = m->q[2] = 0.2410;
+    m->q[3] = m->q[4] = -0.2410;
+    m->eps[0] = 0.6694;
+    m->sig[0] = 3.12;
+    return m->nsite;
+  }
+  if (model == G6O_ST2) {
+    /* Rigid_ST2_Constructor, ST2Module.F90:60-100 */
+    const double theta = 109.47 * PI_T / 180.0, l1 = 1.0, l2 = 0.8;
+    const double hmass = 1.0, omass = 16.0;
+    m->nsite = 6;
+    m->by[1] = l1 * sin(0.5 * theta);
+    m->bz[1] = l1 * cos(0.5 * theta);
+    m->by[2] = -l1 * sin(0.5 * theta);
+    m->bz[2] = l1 * cos(0.5 * theta);
+    m->bx[3] = l2 * sin(0.5 * theta);
+    m->bz[3] = -l2 * cos(0.5 * theta);
+    m->bx[4] = -l2 * sin(0.5 * theta);
+    m->bz[4] = -l2 * cos(0.5 * theta);
+    double zshift = m->bz[1] * 2.0 * hmass / (2.0 * hmass + omass);
+    for (int i = 0; i < 5; i++) m->bz[i] = m->bz[i] - zshift;
+    m->Ixx = omass * (m->by[0] * m->by[0] + m->bz[0] * m->bz[0]) +
+             hmass * (m->by[1] * m->by[1] + m->bz[1] * m->bz[1]) * 2.0;
+    m->Iyy = omass * (m->bz[0] * m->bz[0] + m->bx[0] * m->bx[0]) +
+             hmass * (m->bz[1] * m->bz[1] + m->bx[1] * m->bx[1]) * 2.0;
+    m->Izz = omass * (m->bx[0] * m->bx[0] + m->by[0] * m->by[0]) +
+             hmass * (m->bx[1] * m->bx[1] + m->by[1] * m->by[1]) * 2.0;
+    m->mass = 2.0 * hmass + omass;
+    /* st2_setinteraction, ST2Module.F90:32-58.  The charges and eps are
+     * default-REAL (single precision) literals in the reference
+     * (ST2Module.F90:15-17), widened to REAL(8).                           */
+    m->q[1] = m->q[2] = (double)0.2357f;
+    m->q[3] = m->q[4] = (double)(-0.2357f);
+    m->eps[0] = (double)0.52605e-21f * NA_ * 1e-3;
+    m->sig[0] = 3.100;
+    return m->nsite;
+  }
+  return 0;
+}
+
+/* [WWMM, WWMH, WWHH, AA, BB, QMM, QHM, QHH, AAst2, BBst2, RL, RU] */
+void g6o_constants(double *o) {
+  const double C = coeff();
+  o[0] = -1.04 * -1.04 * C * 100.0;   /* TIP4PConst.F90:11 */
+  o[1] = -1.04 * 0.52 * C * 100.0;    /* TIP4PConst.F90:12 */
+  o[2] = 0.52 * 0.52 * C * 100.0;     /* TIP4PConst.F90:13 */
+  o[3] = 600000.0 * CA_ * 100.0;      /* TIP4PConst.F90:4 */
+  o[4] = -610.0 * CA_ * 100.0;        /* TIP4PConst.F90:5 */
+  g6o_model st2;
+  g6o_model_get(G6O_ST2, &st2);
+  const double qm = st2.q[3], qh = st2.q[1], eps = st2.eps[0], sig = st2.sig[0];
+  o[5] = qm * qm * C * 100.0;         /* ST2Module.F90:121-123 */
+  o[6] = qh * qm * C * 100.0;
+  o[7] = qh * qh * C * 100.0;
+  o[8] = 4.0 * eps * ipow(sig, 12) * 100.0;
+  o[9] = -4.0 * eps * ipow(sig, 6) * 100.0;
+  o[10] = 2.0160;                     /* ST2Module.F90:21-22 */
+  o[11] = 3.1287;
+}
+
+/* ---- grid (GridModule.F90) --------------------------------------------- */
+int g6o_grid_ndiv(double L, double req) { return (int)(L / req); }
+
+int g6o_stencil_homo(const int nd[3], int *sx, int *sy, int *sz, int *n0p,
+                     int *sx0, int *sy0, int *sz0) {
+  int nmax[3], nmin[3];
+  for (int a = 0; a < 3; a++) {
+    nmax[a] = 1;
+    nmin[a] = -1;
+    if (nmax[a] > (nd[a] - 1) / 2) nmax[a] = (nd[a] - 1) / 2;
+    if (nmin[a] < -(nd[a] / 2)) nmin[a] = -(nd[a] / 2);
+  }
+  int *mark = calloc((size_t)nd[0] * nd[1] * nd[2], sizeof(int));
+#define MK(x, y, z) mark[((z) * nd[1] + (y)) * nd[0] + (x)]
+  int n = 0, n0 = 0;
+  for (int jz = nmin[2]; jz <= nmax[2]; jz++) {
+    int pz = jz < 0 ? jz + nd[2] : jz, mz = pz == 0 ? 0 : nd[2] - pz;
+    for (int jy = nmin[1]; jy <= nmax[1]; jy++) {
+      int py = jy < 0 ? jy + nd[1] : jy, my = py == 0 ? 0 : nd[1] - py;
+      for (int jx = nmin[0]; jx <= nmax[0]; jx++) {
+        int px = jx < 0 ? jx + nd[0] : jx, mx = px == 0 ? 0 : nd[0] - px;
+        if (MK(px, py, pz) == 0) {
+          MK(mx, my, mz) += 1;
+          MK(px, py, pz) += 1;
+          if (MK(px, py, pz) != 1) {
+            sx0[n0] = px; sy0[n0] = py; sz0[n0] = pz; n0++;
+          } else {
+            sx[n] = px; sy[n] = py; sz[n] = pz; n++;
+          }
+        }
+      }
+    }
+  }
+#undef MK
+  free(mark);
+  *n0p = n0;
+  return n;
+}
+
+void g6o_address_assign(int n, const double *com, double L, const int nd[3],
+                        int *cell) {
+  const double invL = 1.0 / L;
+  double ci[3];
+  for (int a = 0; a < 3; a++) ci[a] = 1.0 / (L / nd[a]);
+  for (int i = 0; i < n; i++) {
+    int j[3];
+    for (int a = 0; a < 3; a++) {
+      double d = com[3 * i + a];
+      double c = round(d * invL) * L;       /* box_renormalize */
+      d = d - c;
+      j[a] = (int)round(d * ci[a]);
+      if (j[a] < 0) j[a] += nd[a];
+    }
+    cell[i] = (j[2] * nd[1] + j[1]) * nd[0] + j[0];
+  }
+}
+
+/* minimum-image FP64 test of one candidate (InteractionModule.F90:454-474) */
+static int compress_one(const double *com, int i, int j, double L, double invL,
+                        double rr, double *o) {
+  double d[3];
+  for (int a = 0; a < 3; a++) {
+    d[a] = com[3 * i + a] - com[3 * j + a];
+    o[a] = round(d[a] * invL) * L;
+    d[a] = d[a] - o[a];
+  }
+  double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  return dd < rr;
+}
+
+long g6o_pairlist(int n, const double *com, double L, double rc, double req,
+                  long cap, long *npair0, int *pi, int *pj, double *ox,
+                  double *oy, double *oz) {
+  int nd[3];
+  nd[0] = nd[1] = nd[2] = g6o_grid_ndiv(L, req);
+  int sx[27], sy[27], sz[27], sx0[27], sy0[27], sz0[27], n0;
+  int ns = g6o_stencil_homo(nd, sx, sy, sz, &n0, sx0, sy0, sz0);
+  const int ng = nd[0] * nd[1] * nd[2];
+  int *cell = malloc(sizeof(int) * n);
+  g6o_address_assign(n, com, L, nd, cell);
+  /* residents in molecule order (GridModule.F90:538-548) */
+  int *cnt = calloc(ng + 1, sizeof(int)), *start = calloc(ng + 1, sizeof(int));
+  int *res = malloc(sizeof(int) * (n > 0 ? n : 1));
+  for (int i = 0; i < n; i++) cnt[cell[i]]++;
+  for (int c = 0; c < ng; c++) start[c + 1] = start[c] + cnt[c];
+  memset(cnt, 0, sizeof(int) * (ng + 1));
+  for (int i = 0; i < n; i++) res[start[cell[i]] + cnt[cell[i]]++] = i;
+  const double invL = 1.0 / L, rr = rc * rc;
+  long k0 = 0, k = 0;
+  for (int part = 0; part < 2; part++) {
+    const int nst = part == 0 ? ns : n0;
+    const int *tx = part == 0 ? sx : sx0, *ty = part == 0 ? sy : sy0,
+              *tz = part == 0 ? sz : sz0;
+    for (int c1 = 0; c1 < ng; c1++) {
+      const int ix = c1 % nd[0], iy = (c1 / nd[0]) % nd[1], iz = c1 / (nd[0] * nd[1]);
+      for (int ii = start[c1]; ii < start[c1 + 1]; ii++) {
+        const int i = res[ii];
+        for (int s = 0; s < nst; s++) {
+          const int c2 = (((iz + tz[s]) % nd[2]) * nd[1] + (iy + ty[s]) % nd[1]) * nd[0] +
+                         (ix + tx[s]) % nd[0];
+          for (int jj = start[c2]; jj < start[c2 + 1]; jj++) {
+            const int j = res[jj];
+            if (part == 1 && !(j > i)) continue;
+            k0++;
+            double o[3];
+            if (compress_one(com, i, j, L, invL, rr, o)) {
+              if (k >= cap) { k = -1; goto done; }
+              pi[k] = i; pj[k] = j; ox[k] = o[0]; oy[k] = o[1]; oz[k] = o[2];
+              k++;
+            }
+          }
+        }
+      }
+    }
+  }
+done:
+  *npair0 = k0;
+  free(cell); free(cnt); free(start); free(res);
+  return k;
+}
+
+long g6o_pairlist_alltoall(int n, const double *com, double L, double rc,
+                           long cap, int *pi, int *pj, double *ox, double *oy,
+                           double *oz) {
+  const double invL = 1.0 / L, rr = rc * rc;
+  long k = 0;
+  for (int i = 0; i < n; i++)
+    for (int j = i + 1; j < n; j++) {
+      double o[3];
+      if (compress_one(com, i, j, L, invL, rr, o)) {
+        if (k >= cap) return -1;
+        pi[k] = i; pj[k] = j; ox[k] = o[0]; oy[k] = o[1]; oz[k] = o[2];
+        k++;
+      }
+    }
+  return k;
+}
+
+/* ---- cutoff (CutoffModule.F90:25-59, 220-278) --------------------------- */
+void g6o_cutoff_func(double rin, double rout, double dd, double *hh, double *dh) {
+  const double c1 = 1.0 / ipow(rin - rout, 5), c2 = c1 * 30.0;
+  *hh = 0.0;
+  *dh = 0.0;
+  if (dd < rout * rout) {
+    if (dd > rin * rin) {
+      double dr = sqrt(dd), u = dr - rout, l = dr - rin, uu = u * u, ll = l * l;
+      *hh = c1 * uu * u * (10.0 * ll - 5.0 * u * l + uu);
+      *dh = -(c2 * uu * ll / dr);
+    } else {
+      *hh = 1.0;
+      *dh = 0.0;
+    }
+  }
+}
+
+void g6o_cutoff_dimmer(long np, const int *pi, const int *pj, const double *com,
+                       const double *ox, const double *oy, const double *oz,
+                       double rin, double rout, double *er, double *fr) {
+  for (long k = 0; k < np; k++) {
+    const int i = pi[k], j = pj[k];
+    double dx = com[3 * i] - com[3 * j], dy = com[3 * i + 1] - com[3 * j + 1],
+           dz = com[3 * i + 2] - com[3 * j + 2];
+    dx = dx - ox[k]; dy = dy - oy[k]; dz = dz - oz[k];
+    g6o_cutoff_func(rin, rout, dx * dx + dy * dy + dz * dz, &er[k], &fr[k]);
+  }
+}
+
+/* ---- TIP4P custom kernel (TIP4PModule.F90:187-785) ---------------------- */
+void g6o_force_tip4p(long np, const int *pi, const int *pj, const double *ox,
+                     const double *oy, const double *oz, const double *er,
+                     const double *fr, const double *x, const double *y,
+                     const double *z, double *fx, double *fy, double *fz,
+                     double *ep1, double *vr) {
+  double K[12];
+  g6o_constants(K);
+  const double WWMM = K[0], WWMH = K[1], WWHH = K[2], AA = K[3], BB = K[4];
+  /* the nine Coulomb pairs in the order of the live (non-TEST12) code */
+  static const int ci[9] = {1, 1, 1, 2, 3, 2, 2, 3, 3};
+  static const int cj[9] = {1, 2, 3, 1, 1, 2, 3, 2, 3};
+  const double cq[9] = {WWMM, WWMH, WWMH, WWMH, WWMH, WWHH, WWHH, WWHH, WWHH};
+  for (int a = 0; a < 9; a++) vr[a] = 0.0;
+  for (long k = 0; k < np; k++) {
+    const int ii = pi[k] * 5, jj = pj[k] * 5;
+    double dx00 = x[ii + 4], dy00 = y[ii + 4], dz00 = z[ii + 4];
+    dx00 = dx00 - x[jj + 4]; dy00 = dy00 - y[jj + 4]; dz00 = dz00 - z[jj + 4];
+    dx00 = dx00 - ox[k]; dy00 = dy00 - oy[k]; dz00 = dz00 - oz[k];
+    const double hh = er[k], dh = fr[k];
+    double dhx = dh * dx00, dhy = dh * dy00, dhz = dh * dz00;
+    const double cx = -ox[k], cy = -oy[k], cz = -oz[k];
+    double fi[5][3] = {{0}}, fj[5][3] = {{0}};
+    double ep = 0.0;
+    for (int t = 0; t < 9; t++) {
+      const int a = ci[t], b = cj[t];
+      double dx = x[ii + a] - x[jj + b] + cx, dy = y[ii + a] - y[jj + b] + cy,
+             dz = z[ii + a] - z[jj + b] + cz;
+      double dd = 1.0 / (dx * dx + dy * dy + dz * dz);
+      double ri = sqrt(dd);
+      double e0 = cq[t] * ri, f0 = cq[t] * ri * dd;
+      ep = ep + e0;
+      double gx = dx * f0, gy = dy * f0, gz = dz * f0;
+      fi[a][0] += gx; fi[a][1] += gy; fi[a][2] += gz;
+      fj[b][0] += gx; fj[b][1] += gy; fj[b][2] += gz;
+    }
+    {
+      double dx = x[ii] - x[jj] + cx, dy = y[ii] - y[jj] + cy, dz = z[ii] - z[jj] + cz;
+      double dd = 1.0 / (dx * dx + dy * dy + dz * dz);
+      double d6 = dd * dd * dd, d12 = d6 * d6;
+      double e0 = AA * d12 + BB * d6;
+      double f0 = dd * (AA * 12.0 * d12 + BB * 6.0 * d6);
+      ep = ep + e0;
+      double gx = dx * f0, gy = dy * f0, gz = dz * f0;
+      fi[0][0] += gx; fi[0][1] += gy; fi[0][2] += gz;
+      fj[0][0] += gx; fj[0][1] += gy; fj[0][2] += gz;
+    }
+    for (int s = 0; s < 5; s++)
+      for (int a = 0; a < 3; a++) { fi[s][a] = fi[s][a] * hh; fj[s][a] = fj[s][a] * hh; }
+    dhx = dhx * ep; dhy = dhy * ep; dhz = dhz * ep;
+    fi[4][0] += dhx; fi[4][1] += dhy; fi[4][2] += dhz;
+    fj[4][0] += dhx; fj[4][1] += dhy; fj[4][2] += dhz;
+    ep1[k] = ep * hh;
+    double v[3];
+    for (int a = 0; a < 3; a++)
+      v[a] = fi[0][a] + fi[1][a] + fi[2][a] + fi[3][a] + fi[4][a];
+    const double d00[3] = {dx00, dy00, dz00};
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) vr[3 * a + b] = vr[3 * a + b] + d00[a] * v[b];
+    for (int s = 0; s < 5; s++) {
+      fx[ii + s] += fi[s][0]; fy[ii + s] += fi[s][1]; fz[ii + s] += fi[s][2];
+      fx[jj + s] -= fj[s][0]; fy[jj + s] -= fj[s][1]; fz[jj + s] -= fj[s][2];
+    }
+  }
+}
+
+/* ---- generic StdForce (StdInteraction.F90:63-630) ----------------------- */
+typedef struct { int a, b, kind; double qq, aa, bb; } stdterm;
+/* site-pair terms in the isite-outer, jsite-inner order with the reference's
+ * coefficient arithmetic (StdInteraction.F90:201-221); kind 1=LJ+C, 2=C, 3=LJ */
+static int std_terms(const g6o_model *m, stdterm *t) {
+  int nt = 0;
+  const double scale = 1.0;
+  for (int a = 0; a < m->nsite; a++)
+    for (int b = 0; b < m->nsite; b++) {
+      double qq = m->q[a] * m->q[b] * coeff() * KX2X() * J2I() * scale;
+      double aa = m->eps[a] * m->eps[b];
+      stdterm s = {a, b, 0, qq, 0.0, 0.0};
+      if (qq != 0.0 || aa != 0.0) {
+        if (aa != 0.0) {
+          double ljeps = sqrt(aa);
+          double ljsig = (m->sig[a] + m->sig[b]) * 0.5;
+          ljsig = ipow(ljsig, 6);
+          s.aa = 4.0 * ljeps * ljsig * ljsig * KX2X() * J2I() * scale;
+          s.bb = -4.0 * ljeps * ljsig * KX2X() * J2I() * scale;
+        }
+        s.kind = qq != 0.0 ? (aa != 0.0 ? 1 : 2) : 3;
+        t[nt++] = s;
+      }
+    }
+  return nt;
+}
+
+void g6o_force_std(int model, long np, const int *pi, const int *pj,
+                   const double *ox, const double *oy, const double *oz,
+                   const double *er, const double *fr, const double *x,
+                   const double *y, const double *z, double *fx, double *fy,
+                   double *fz, double *ep, double *vr) {
+  g6o_model m;
+  g6o_model_get(model, &m);
+  const int ns = m.nsite;
+  stdterm t[G6O_MAXSITE * G6O_MAXSITE];
+  const int nt = std_terms(&m, t);
+  double *fi = calloc((size_t)np * ns * 3 + 1, sizeof(double));
+  double *fj = calloc((size_t)np * ns * 3 + 1, sizeof(double));
+#define FI(k, s, a) fi[((size_t)(s) * np + (k)) * 3 + (a)]
+#define FJ(k, s, a) fj[((size_t)(s) * np + (k)) * 3 + (a)]
+  for (long k = 0; k < np; k++) ep[k] = 0.0;
+  for (int u = 0; u < nt; u++) {
+    const stdterm s = t[u];
+    for (long k = 0; k < np; k++) {
+      const int ii = pi[k] * ns + s.a, jj = pj[k] * ns + s.b;
+      double dx = x[ii] - x[jj], dy = y[ii] - y[jj], dz = z[ii] - z[jj];
+      dx = dx - ox[k]; dy = dy - oy[k]; dz = dz - oz[k];
+      double dd = 1.0 / (dx * dx + dy * dy + dz * dz);
+      double e0, f0;
+      if (s.kind == 1) {
+        double ri = sqrt(dd);
+        e0 = s.qq * ri;
+        f0 = s.qq * ri * dd;
+        double d6 = dd * dd * dd, d12 = d6 * d6;
+        e0 = e0 + s.aa * d12 + s.bb * d6;
+        f0 = f0 + dd * (s.aa * 12.0 * d12 + s.bb * 6.0 * d6);
+      } else if (s.kind == 2) {
+        double ri = sqrt(dd);
+        e0 = s.qq * ri;
+        f0 = s.qq * ri * dd;
+      } else {
+        double d6 = dd * dd * dd, d12 = d6 * d6;
+        e0 = s.aa * d12 + s.bb * d6;
+        f0 = dd * (s.aa * 12.0 * d12 + s.bb * 6.0 * d6);
+      }
+      double gx = dx * f0, gy = dy * f0, gz = dz * f0;
+      FI(k, s.a, 0) += gx; FI(k, s.a, 1) += gy; FI(k, s.a, 2) += gz;
+      FJ(k, s.b, 0) += gx; FJ(k, s.b, 1) += gy; FJ(k, s.b, 2) += gz;
+      ep[k] = ep[k] + e0;
+    }
+  }
+  /* smoothing (491-514), CoM term (515-536) */
+  for (int l = 0; l < ns; l++)
+    for (long k = 0; k < np; k++)
+      for (int a = 0; a < 3; a++) FI(k, l, a) = FI(k, l, a) * er[k];
+  for (int l = 0; l < ns; l++)
+    for (long k = 0; k < np; k++)
+      for (int a = 0; a < 3; a++) FJ(k, l, a) = FJ(k, l, a) * er[k];
+  const int c = ns - 1;
+  for (long k = 0; k < np; k++) {
+    const int ii = pi[k] * ns + c, jj = pj[k] * ns + c;
+    double d0 = x[ii] - x[jj] - ox[k], d1 = y[ii] - y[jj] - oy[k],
+           d2 = z[ii] - z[jj] - oz[k];
+    double dhx = fr[k] * d0 * ep[k], dhy = fr[k] * d1 * ep[k], dhz = fr[k] * d2 * ep[k];
+    FI(k, c, 0) += dhx; FI(k, c, 1) += dhy; FI(k, c, 2) += dhz;
+    FJ(k, c, 0) += dhx; FJ(k, c, 1) += dhy; FJ(k, c, 2) += dhz;
+  }
+  /* scatter, TEST22 order (537-560) */
+  for (int l = 0; l < ns; l++)
+    for (long k = 0; k < np; k++) {
+      const int ii = pi[k] * ns + l;
+      fx[ii] = fx[ii] + FI(k, l, 0); fy[ii] = fy[ii] + FI(k, l, 1); fz[ii] = fz[ii] + FI(k, l, 2);
+    }
+  for (int l = 0; l < ns; l++)
+    for (long k = 0; k < np; k++) {
+      const int jj = pj[k] * ns + l;
+      fx[jj] = fx[jj] - FJ(k, l, 0); fy[jj] = fy[jj] - FJ(k, l, 1); fz[jj] = fz[jj] - FJ(k, l, 2);
+    }
+  for (long k = 0; k < np; k++) ep[k] = ep[k] * er[k];
+  /* virial (582-604): vrmat(j,i) += d00(j)*vr(i) */
+  for (int a = 0; a < 9; a++) vr[a] = 0.0;
+  for (long k = 0; k < np; k++) {
+    const int ii = pi[k] * ns + c, jj = pj[k] * ns + c;
+    double d00[3] = {x[ii] - x[jj] - ox[k], y[ii] - y[jj] - oy[k], z[ii] - z[jj] - oz[k]};
+    double v[3] = {0, 0, 0};
+    for (int l = 0; l < ns; l++)
+      for (int a = 0; a < 3; a++) v[a] = v[a] + FI(k, l, a);
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) vr[3 * j + i] = vr[3 * j + i] + d00[j] * v[i];
+  }
+#undef FI
+#undef FJ
+  free(fi);
+  free(fj);
+}
+
+void g6o_energy_std(int model, long np, const int *pi, const int *pj,
+                    const double *ox, const double *oy, const double *oz,
+                    const double *er, const double *x, const double *y,
+                    const double *z, double *ep) {
+  g6o_model m;
+  g6o_model_get(model, &m);
+  const int ns = m.nsite;
+  stdterm t[G6O_MAXSITE * G6O_MAXSITE];
+  const int nt = std_terms(&m, t);
+  for (long k = 0; k < np; k++) ep[k] = 0.0;
+  for (int u = 0; u < nt; u++) {
+    const stdterm s = t[u];
+    for (long k = 0; k < np; k++) {
+      const int ii = pi[k] * ns + s.a, jj = pj[k] * ns + s.b;
+      double dx = x[ii] - x[jj], dy = y[ii] - y[jj], dz = z[ii] - z[jj];
+      dx = dx - ox[k]; dy = dy - oy[k]; dz = dz - oz[k];
+      double dd = 1.0 / (dx * dx + dy * dy + dz * dz), e0;
+      if (s.kind == 1) {
+        double ri = sqrt(dd);
+        e0 = s.qq * ri;
+        double d6 = dd * dd * dd, d12 = d6 * d6;
+        e0 = e0 + s.aa * d12 + s.bb * d6;
+      } else if (s.kind == 2) {
+        e0 = s.qq * sqrt(dd);
+      } else {
+        double d6 = dd * dd * dd, d12 = d6 * d6;
+        e0 = s.aa * d12 + s.bb * d6;
+      }
+      ep[k] = ep[k] + e0;
+    }
+  }
+  for (long k = 0; k < np; k++) ep[k] = ep[k] * er[k];
+}
+
+/* ---- ST2 custom kernel with the short-range switch (ST2Module.F90:112-897) */
+void g6o_force_st2(long np, const int *pi, const int *pj, const double *ox,
+                   const double *oy, const double *oz, const double *er,
+                   const double *fr, const double *x, const double *y,
+                   const double *z, double *fx, double *fy, double *fz,
+                   double *ep1, double *vr) {
+  double K[12];
+  g6o_constants(K);
+  const double QMM = K[5], QHM = K[6], QHH = K[7], AA = K[8], BB = K[9];
+  const double rl = K[10], ru = K[11];
+  /* Coulomb terms: sites 2..5 (1-based) = H,H,M,M; 16 pairs */
+  for (int a = 0; a < 9; a++) vr[a] = 0.0;
+  for (long k = 0; k < np; k++) {
+    const int ii = pi[k] * 6, jj = pj[k] * 6;
+    double dx00 = x[ii + 5] - x[jj + 5], dy00 = y[ii + 5] - y[jj + 5],
+           dz00 = z[ii + 5] - z[jj + 5];
+    dx00 = dx00 - ox[k]; dy00 = dy00 - oy[k]; dz00 = dz00 - oz[k];
+    const double hh = er[k], dh = fr[k];
+    double d12 = dx00 * dx00 + dy00 * dy00 + dz00 * dz00, s12, ds12;
+    if (d12 < ru * ru) {
+      if (d12 < rl * rl) { s12 = 0.0; ds12 = 0.0; }
+      else {
+        double r12 = sqrt(d12);
+        s12 = (r12 - rl) * (r12 - rl) * (3.0 * ru - rl - 2.0 * r12) / ipow(ru - rl, 3);
+        ds12 = 6.0 * (r12 - rl) * (ru - r12) / ipow(ru - rl, 3);
+        ds12 = -ds12 / r12;
+      }
+    } else { s12 = 1.0; ds12 = 0.0; }
+    const double dhc = dh * s12 + hh * ds12, hhc = hh * s12;
+    double dhcx = dhc * dx00, dhcy = dhc * dy00, dhcz = dhc * dz00;
+    double dhx = dh * dx00, dhy = dh * dy00, dhz = dh * dz00;
+    const double cx = -ox[k], cy = -oy[k], cz = -oz[k];
+    double fi[6][3] = {{0}}, fj[6][3] = {{0}};
+    double ep = 0.0, eplj;
+    for (int a = 1; a <= 4; a++)
+      for (int b = 1; b <= 4; b++) {
+        const double qq = (a <= 2) == (b <= 2) ? (a <= 2 ? QHH : QMM) : QHM;
+        double dx = x[ii + a] - x[jj + b] + cx, dy = y[ii + a] - y[jj + b] + cy,
+               dz = z[ii + a] - z[jj + b] + cz;
+        double dd = 1.0 / (dx * dx + dy * dy + dz * dz), ri = sqrt(dd);
+        double e0 = qq * ri, f0 = qq * ri * dd;
+        ep = ep + e0;
+        fi[a][0] += dx * f0; fi[a][1] += dy * f0; fi[a][2] += dz * f0;
+        fj[b][0] += dx * f0; fj[b][1] += dy * f0; fj[b][2] += dz * f0;
+      }
+    {
+      double dx = x[ii] - x[jj] + cx, dy = y[ii] - y[jj] + cy, dz = z[ii] - z[jj] + cz;
+      double dd = 1.0 / (dx * dx + dy * dy + dz * dz);
+      double d6 = dd * dd * dd, dl12 = d6 * d6;
+      eplj = AA * dl12 + BB * d6;
+      double f0 = dd * (AA * 12.0 * dl12 + BB * 6.0 * d6);
+      fi[0][0] += dx * f0; fi[0][1] += dy * f0; fi[0][2] += dz * f0;
+      fj[0][0] += dx * f0; fj[0][1] += dy * f0; fj[0][2] += dz * f0;
+    }
+    for (int s = 0; s < 6; s++) {
+      const double sc = (s == 0 || s == 5) ? hh : hhc;
+      for (int a = 0; a < 3; a++) { fi[s][a] *= sc; fj[s][a] *= sc; }
+    }
+    dhx *= eplj; dhy *= eplj; dhz *= eplj;
+    dhcx *= ep; dhcy *= ep; dhcz *= ep;
+    fi[5][0] += dhx + dhcx; fi[5][1] += dhy + dhcy; fi[5][2] += dhz + dhcz;
+    fj[5][0] += dhx + dhcx; fj[5][1] += dhy + dhcy; fj[5][2] += dhz + dhcz;
+    ep1[k] = ep * hhc + eplj * hh;
+    double v[3];
+    for (int a = 0; a < 3; a++)
+      v[a] = fi[0][a] + fi[1][a] + fi[2][a] + fi[3][a] + fi[4][a] + fi[5][a];
+    const double d00[3] = {dx00, dy00, dz00};
+    for (int a = 0; a < 3; a++)
+      for (int b = 0; b < 3; b++) vr[3 * a + b] += d00[a] * v[b];
+    for (int s = 0; s < 6; s++) {
+      fx[ii + s] += fi[s][0]; fy[ii + s] += fi[s][1]; fz[ii + s] += fi[s][2];
+      fx[jj + s] -= fj[s][0]; fy[jj + s] -= fj[s][1]; fz[jj + s] -= fj[s][2];
+    }
+  }
+}
+
+/* ---- rigid bodies (RigidModule.F90) ------------------------------------- */
+/* rigid_setmatrix (62-83) + rigid_setsiteposition0 serial path (110-128).
+ * tmat[9n] row-major t11..t33; intra[3*nsite*n] per molecule per site.      */
+void g6o_setsites(int model, int n, const double *com, const double *quat,
+                  double *x, double *y, double *z, double *tmat, double *intra) {
+  g6o_model m;
+  g6o_model_get(model, &m);
+  const int ns = m.nsite;
+  for (int i = 0; i < n; i++) {
+    const double qa = quat[0 * n + i], qb = quat[1 * n + i], qc = quat[2 * n + i],
+                 qd = quat[3 * n + i];
+    double *t = tmat + 9 * i;
+    t[0] = (qa * qa + qb * qb - (qc * qc + qd * qd));
+    t[1] = -2.0 * (qa * qd + qb * qc);
+    t[2] = 2.0 * (qb * qd - qa * qc);
+    t[3] = 2.0 * (qa * qd - qb * qc);
+    t[4] = qa * qa + qc * qc - (qb * qb + qd * qd);
+    t[5] = -2.0 * (qa * qb + qc * qd);
+    t[6] = 2.0 * (qa * qc + qb * qd);
+    t[7] = 2.0 * (qa * qb - qc * qd);
+    t[8] = qa * qa + qd * qd - (qb * qb + qc * qc);
+    for (int s = 0; s < ns; s++) {
+      double *r = intra + 3 * (i * ns + s);
+      r[0] = t[0] * m.bx[s] + t[1] * m.by[s] + t[2] * m.bz[s];
+      r[1] = t[3] * m.bx[s] + t[4] * m.by[s] + t[5] * m.bz[s];
+      r[2] = t[6] * m.bx[s] + t[7] * m.by[s] + t[8] * m.bz[s];
+      x[i * ns + s] = com[3 * i] + r[0];
+      y[i * ns + s] = com[3 * i + 1] + r[1];
+      z[i * ns + s] = com[3 * i + 2] + r[2];
+    }
+  }
+}
+
+/* rigid_collectforce serial path (275-317); torque returned in body frame */
+void g6o_collect(int model, int n, const double *fx, const double *fy,
+                 const double *fz, const double *tmat, const double *intra,
+                 double *force, double *torque) {
+  g6o_model m;
+  g6o_model_get(model, &m);
+  const int ns = m.nsite;
+  for (int i = 0; i < n; i++) {
+    double F[3] = {0, 0, 0}, tq[3] = {0, 0, 0};
+    for (int s = 0; s < ns; s++) {
+      const int k = i * ns + s;
+      const double *r = intra + 3 * k;
+      F[0] = F[0] + fx[k]; F[1] = F[1] + fy[k]; F[2] = F[2] + fz[k];
+      tq[0] = tq[0] + r[1] * fz[k] - r[2] * fy[k];
+      tq[1] = tq[1] + r[2] * fx[k] - r[0] * fz[k];
+      tq[2] = tq[2] + r[0] * fy[k] - r[1] * fx[k];
+    }
+    const double *t = tmat + 9 * i;
+    force[3 * i] = F[0]; force[3 * i + 1] = F[1]; force[3 * i + 2] = F[2];
+    torque[3 * i] = t[0] * tq[0] + t[3] * tq[1] + t[6] * tq[2];
+    torque[3 * i + 1] = t[1] * tq[0] + t[4] * tq[1] + t[7] * tq[2];
+    torque[3 * i + 2] = t[2] * tq[0] + t[5] * tq[1] + t[8] * tq[2];
+  }
+}
+
+#define DX(s, a, i) dx[((s) * 3 + (a)) * n + (i)]
+#define W(s, a, i) w[((s) * 3 + (a)) * n + (i)]
+#define Q(s, a, i) quat[((s) * 4 + (a)) * n + (i)]
+/* rigid_predict (480-559) */
+void g6o_predict(int n, double *com, double *dx, double *w, double *quat) {
+  for (int i = 0; i < n; i++) {
+    for (int a = 0; a < 3; a++) {
+      com[3 * i + a] = com[3 * i + a] + DX(0, a, i) + DX(1, a, i) + DX(2, a, i) +
+                       DX(3, a, i) + DX(4, a, i);
+      DX(0, a, i) = DX(0, a, i) + 2.0 * DX(1, a, i) + 3.0 * DX(2, a, i) +
+                    4.0 * DX(3, a, i) + 5.0 * DX(4, a, i);
+      DX(1, a, i) = DX(1, a, i) + 3.0 * DX(2, a, i) + 6.0 * DX(3, a, i) + 10.0 * DX(4, a, i);
+      DX(2, a, i) = DX(2, a, i) + 4.0 * DX(3, a, i) + 10.0 * DX(4, a, i);
+      DX(3, a, i) = DX(3, a, i) + 5.0 * DX(4, a, i);
+    }
+    for (int a = 0; a < 3; a++) {
+      W(0, a, i) = W(0, a, i) + W(1, a, i) + W(2, a, i) + W(3, a, i) + W(4, a, i);
+      W(1, a, i) = W(1, a, i) + 2.0 * W(2, a, i) + 3.0 * W(3, a, i) + 4.0 * W(4, a, i);
+      W(2, a, i) = W(2, a, i) + 3.0 * W(3, a, i) + 6.0 * W(4, a, i);
+      W(3, a, i) = W(3, a, i) + 4.0 * W(4, a, i);
+    }
+    for (int a = 0; a < 4; a++) {
+      Q(0, a, i) = Q(0, a, i) + Q(1, a, i) + Q(2, a, i) + Q(3, a, i) + Q(4, a, i);
+      Q(1, a, i) = Q(1, a, i) + 2.0 * Q(2, a, i) + 3.0 * Q(3, a, i) + 4.0 * Q(4, a, i);
+    }
+    const double re = 1.0 / sqrt(Q(0, 0, i) * Q(0, 0, i) + Q(0, 1, i) * Q(0, 1, i) +
+                                 Q(0, 2, i) * Q(0, 2, i) + Q(0, 3, i) * Q(0, 3, i));
+    for (int a = 0; a < 4; a++) Q(0, a, i) = Q(0, a, i) * re;
+  }
+}
+
+/* rigid_correct (1255-1487), Nose drag only (no Andersen) */
+void g6o_correct(int model, int n, double dt, double zeta0, const double *force,
+                 const double *torque, double *com, double *dx, double *w,
+                 double *quat) {
+  g6o_model m;
+  g6o_model_get(model, &m);
+  const double massi = 1.0 / m.mass, Ixxi = 1.0 / m.Ixx, Iyyi = 1.0 / m.Iyy,
+               Izzi = 1.0 / m.Izz;
+  const double nc = zeta0 * dt, ext = nc * 0.5, exr = nc;
+  const double h = dt * 0.5, tf = dt * dt * 0.5 * massi;
+  for (int i = 0; i < n; i++) {
+    for (int a = 0; a < 3; a++) {
+      double nf = force[3 * i + a] * tf;
+      nf = nf - DX(0, a, i) * ext;
+      const double d = nf - DX(1, a, i);
+      com[3 * i + a] = com[3 * i + a] + Gear5_0 * d;
+      DX(0, a, i) = DX(0, a, i) + Gear5_1 * d;
+      DX(1, a, i) = nf;
+      DX(2, a, i) = DX(2, a, i) + Gear5_3 * d;
+      DX(3, a, i) = DX(3, a, i) + Gear5_4 * d;
+      DX(4, a, i) = DX(4, a, i) + Gear5_5 * d;
+    }
+    double wx = W(0, 0, i), wy = W(0, 1, i), wz = W(0, 2, i);
+    double qa = Q(0, 0, i), qb = Q(0, 1, i), qc = Q(0, 2, i), qd = Q(0, 3, i);
+    const double nwa = (-wx * qb + wy * qc - wz * qd) * dt * 0.5;
+    const double nwb = (wx * qa - wy * qd - wz * qc) * dt * 0.5;
+    const double nwc = (-wx * qd - wy * qa + wz * qb) * dt * 0.5;
+    const double nwd = (wx * qc + wy * qb + wz * qa) * dt * 0.5;
+    const double dwa = nwa - Q(1, 0, i), dwb = nwb - Q(1, 1, i), dwc = nwc - Q(1, 2, i),
+                 dwd = nwd - Q(1, 3, i);
+    Q(0, 0, i) = Q(0, 0, i) + Gear4_0 * dwa;
+    Q(0, 1, i) = Q(0, 1, i) + Gear4_0 * dwb;
+    Q(0, 2, i) = Q(0, 2, i) + Gear4_0 * dwc;
+    Q(0, 3, i) = Q(0, 3, i) + Gear4_0 * dwd;
+    double nf[3];
+    nf[0] = (torque[3 * i] + (m.Iyy - m.Izz) * wy * wz) * dt * Ixxi;
+    nf[1] = (torque[3 * i + 1] + (m.Izz - m.Ixx) * wz * wx) * dt * Iyyi;
+    nf[2] = (torque[3 * i + 2] + (m.Ixx - m.Iyy) * wx * wy) * dt * Izzi;
+    for (int a = 0; a < 3; a++) {
+      nf[a] = nf[a] - W(0, a, i) * exr;
+      const double d = nf[a] - W(1, a, i);
+      W(0, a, i) = W(0, a, i) + Gear4_0 * d;
+      W(1, a, i) = nf[a];
+      W(2, a, i) = W(2, a, i) + Gear4_2 * d;
+      W(3, a, i) = W(3, a, i) + Gear4_3 * d;
+      W(4, a, i) = W(4, a, i) + Gear4_4 * d;
+    }
+    /* quaternion derivatives 2..5 from the corrected omega derivatives */
+    double w1x = W(0, 0, i), w1y = W(0, 1, i), w1z = W(0, 2, i);
+    double w2x = W(1, 0, i), w2y = W(1, 1, i), w2z = W(1, 2, i);
+    double w3x = W(2, 0, i), w3y = W(2, 1, i), w3z = W(2, 2, i);
+    double w4x = W(3, 0, i), w4y = W(3, 1, i), w4z = W(3, 2, i);
+    qa = Q(0, 0, i); qb = Q(0, 1, i); qc = Q(0, 2, i); qd = Q(0, 3, i);
+    Q(1, 2, i) = (-w1x * qd - w1y * qa + w1z * qb) * dt * 0.5;
+    Q(1, 1, i) = (w1x * qa - w1y * qd - w1z * qc) * dt * 0.5;
+    Q(1, 3, i) = (w1x * qc + w1y * qb + w1z * qa) * dt * 0.5;
+    Q(1, 0, i) = (-w1x * qb + w1y * qc - w1z * qd) * dt * 0.5;
+    const double pa = Q(1, 0, i), pb = Q(1, 1, i), pc = Q(1, 2, i), pd = Q(1, 3, i);
+    Q(2, 2, i) = (-w2x * qd - w2y * qa + w2z * qb - w1x * pd - w1y * pa + w1z * pb) * dt * 0.25;
+    Q(2, 1, i) = (w2x * qa - w2y * qd - w2z * qc + w1x * pa - w1y * pd - w1z * pc) * dt * 0.25;
+    Q(2, 3, i) = (w2x * qc + w2y * qb + w2z * qa + w1x * pc + w1y * pb + w1z * pa) * dt * 0.25;
+    Q(2, 0, i) = (-w2x * qb + w2y * qc - w2z * qd - w1x * pb + w1y * pc - w1z * pd) * dt * 0.25;
+    const double ra = Q(2, 0, i), rb = Q(2, 1, i), rc_ = Q(2, 2, i), rd = Q(2, 3, i);
+    Q(3, 2, i) = (-w3x * qd - w3y * qa + w3z * qb - w1x * rd - w1y * ra + w1z * rb -
+                  w2x * pd - w2y * pa + w2z * pb) * dt * (1.0 / 6.0);
+    Q(3, 1, i) = (w3x * qa - w3y * qd - w3z * qc + w1x * ra - w1y * rd - w1z * rc_ +
+                  w2x * pa - w2y * pd - w2z * pc) * dt * (1.0 / 6.0);
+    Q(3, 3, i) = (w3x * qc + w3y * qb + w3z * qa + w1x * rc_ + w1y * rb + w1z * ra +
+                  w2x * pc + w2y * pb + w2z * pa) * dt * (1.0 / 6.0);
+    Q(3, 0, i) = (-w3x * qb + w3y * qc - w3z * qd - w1x * rb + w1y * rc_ - w1z * rd -
+                  w2x * pb + w2y * pc - w2z * pd) * dt * (1.0 / 6.0);
+    const double sa = Q(3, 0, i), sb = Q(3, 1, i), sc = Q(3, 2, i), sd = Q(3, 3, i);
+    Q(4, 2, i) = (-w4x * qd - w4y * qa + w4z * qb - w3x * pd - w3y * pa + w3z * pb -
+                  w2x * rd - w2y * ra + w2z * rb - w1x * sd - w1y * sa + w1z * sb) * dt * (1.0 / 8.0);
+    Q(4, 1, i) = (w4x * qa - w4y * qd - w4z * qc + w3x * pa - w3y * pd - w3z * pc +
+                  w2x * ra - w2y * rd - w2z * rc_ + w1x * sa - w1y * sd - w1z * sc) * dt * (1.0 / 8.0);
+    Q(4, 3, i) = (w4x * qc + w4y * qb + w4z * qa + w3x * pc + w3y * pb + w3z * pa +
+                  w2x * rc_ + w2y * rb + w2z * ra + w1x * sc + w1y * sb + w1z * sa) * dt * (1.0 / 8.0);
+    Q(4, 0, i) = (-w4x * qb + w4y * qc - w4z * qd - w3x * pb + w3y * pc - w3z * pd -
+                  w2x * rb + w2y * rc_ - w2z * rd - w1x * sb + w1y * sc - w1z * sd) * dt * (1.0 / 8.0);
+  }
+  (void)h;
+}
+
+/* Rigid_Ek (576-597) */
+void g6o_ek(int model, int n, double dt, const double *dx, const double *w,
+            double *ekt, double *ekr) {
+  g6o_model m;
+  g6o_model_get(model, &m);
+  double t = 0, rx = 0, ry = 0, rz = 0;
+  for (int i = 0; i < n; i++) {
+    t = t + DX(0, 0, i) * DX(0, 0, i) + DX(0, 1, i) * DX(0, 1, i) + DX(0, 2, i) * DX(0, 2, i);
+    rx = rx + W(0, 0, i) * W(0, 0, i);
+    ry = ry + W(0, 1, i) * W(0, 1, i);
+    rz = rz + W(0, 2, i) * W(0, 2, i);
+  }
+  *ekr = (rx * m.Ixx + ry * m.Iyy + rz * m.Izz) * 0.5;
+  *ekt = t * m.mass * 0.5 * (1.0 / (dt * dt));
+}
+#undef DX
+#undef W
+#undef Q
+
+/* Nose_Predict / Nose_Correct (NoseModule.F90:43-87) */
+void g6o_nose_predict(double *s) {
+  s[3] = s[3] + s[4] + s[5] + s[6] + s[7];
+  s[4] = s[4] + 2.0 * s[5] + 3.0 * s[6] + 4.0 * s[7];
+  s[5] = s[5] + 3.0 * s[6] + 6.0 * s[7];
+  s[6] = s[6] + 4.0 * s[7];
+}
+void g6o_nose_correct(double *s, double temp) {
+  const double zetaf = (temp - s[0]) / s[1] - s[2] * s[3];
+  const double dz2 = zetaf - s[4];
+  s[3] = s[3] + Gear4_0 * dz2;
+  s[4] = zetaf;
+  s[5] = s[5] + Gear4_2 * dz2;
+  s[6] = s[6] + Gear4_3 * dz2;
+  s[7] = s[7] + Gear4_4 * dz2;
+  s[8] = s[8] + s[3];
+}
+
+/* ---- MD step in Mixture.F90 order (1789-2286) --------------------------- */
+long g6o_md_step(g6o_md *s) {
+  g6o_model m;
+  g6o_model_get(s->model, &m);
+  const int n = s->n, ns = m.nsite;
+  g6o_predict(n, s->com, s->dx, s->w, s->quat);
+  if (s->nose_active) g6o_nose_predict(s->nose);
+  double *x = malloc(sizeof(double) * n * ns * 6), *y = x + n * ns, *z = y + n * ns;
+  double *fx = z + n * ns, *fy = fx + n * ns, *fz = fy + n * ns;
+  double *tmat = malloc(sizeof(double) * 9 * n), *intra = malloc(sizeof(double) * 3 * ns * n);
+  g6o_setsites(s->model, n, s->com, s->quat, x, y, z, tmat, intra);
+  const long cap = (long)n * 800;
+  int *pi = malloc(sizeof(int) * cap), *pj = malloc(sizeof(int) * cap);
+  double *o = malloc(sizeof(double) * cap * 6);
+  double *ox = o, *oy = o + cap, *oz = o + 2 * cap, *er = o + 3 * cap, *fr = o + 4 * cap,
+         *ep1 = o + 5 * cap;
+  long np0;
+  long np = g6o_pairlist(n, s->com, s->L, s->rc, s->req, cap, &np0, pi, pj, ox, oy, oz);
+  if (np >= 0) {
+    memset(fx, 0, sizeof(double) * 3 * n * ns);
+    g6o_cutoff_dimmer(np, pi, pj, s->com, ox, oy, oz, s->rc - s->taper, s->rc, er, fr);
+    if (s->model == G6O_TIP4P && s->fast)
+      g6o_force_tip4p(np, pi, pj, ox, oy, oz, er, fr, x, y, z, fx, fy, fz, ep1, s->vir);
+    else if (s->model == G6O_ST2 && s->fast)
+      g6o_force_st2(np, pi, pj, ox, oy, oz, er, fr, x, y, z, fx, fy, fz, ep1, s->vir);
+    else
+      g6o_force_std(s->model, np, pi, pj, ox, oy, oz, er, fr, x, y, z, fx, fy, fz, ep1, s->vir);
+    double ep = 0.0;
+    for (long k = 0; k < np; k++) ep = ep + ep1[k];
+    s->ep = ep;
+    double *F = malloc(sizeof(double) * 6 * n), *T = F + 3 * n;
+    g6o_collect(s->model, n, fx, fy, fz, tmat, intra, F, T);
+    g6o_correct(s->model, n, s->dt, s->nose_active ? s->nose[3] : 0.0, F, T, s->com,
+                s->dx, s->w, s->quat);
+    free(F);
+    g6o_ek(s->model, n, s->dt, s->dx, s->w, &s->ekt, &s->ekr);
+    s->temperature = I2J * 2.0 * (s->ekt + s->ekr) / (rgas() * (double)(6 * n));
+    if (s->nose_active) g6o_nose_correct(s->nose, s->temperature);
+  }
+  s->npair = np;
+  free(x); free(tmat); free(intra); free(pi); free(pj); free(o);
+  return np;
+}
+
+double g6o_md_hamiltonian(const g6o_md *s) {
+  const double dof = 6.0 * s->n;
+  double h = (s->ep + s->ekt + s->ekr) * I2J;
+  if (s->nose_active) {
+    const double *z = s->nose;
+    const double nose_ek = (z[3] * z[3] * z[1] * s->dt * dof * 0.5) * rgas();
+    const double nose_ep = (dof * z[0] * (z[8] - 0.5 * z[3]) * s->dt) * rgas();
+    h = h + nose_ep + nose_ek;
+  }
+  return h;
+}
+
+/* ---- MT19937 (Matsumoto & Nishimura 1998, init_genrand of 2002) --------- */
+static uint32_t mt[624];
+static int mti = 625;
+void g6o_mt_seed(uint32_t seed) {
+  mt[0] = seed;
+  for (int i = 1; i < 624; i++)
+    mt[i] = 1812433253u * (mt[i - 1] ^ (mt[i - 1] >> 30)) + (uint32_t)i;
+  mti = 624;
+}
+uint32_t g6o_mt_next(void) {
+  if (mti >= 624) {
+    if (mti == 625) g6o_mt_seed(5489u);
+    for (int k = 0; k < 624; k++) {
+      uint32_t y = (mt[k] & 0x80000000u) | (mt[(k + 1) % 624] & 0x7fffffffu);
+      mt[k] = mt[(k + 397) % 624] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+    }
+    mti = 0;
+  }
+  uint32_t y = mt[mti++];
+  y ^= y >> 11;
+  y ^= (y << 7) & 0x9d2c5680u;
+  y ^= (y << 15) & 0xefc60000u;
+  y ^= y >> 18;
+  return y;
+}
+double g6o_random_getnext(void) {
+  const double x = (double)(g6o_mt_next() >> 1) * (1.0 / 2147483648.0);
+  return 1.0 - x;
+}
+
+/* Random_Normal (RandomModule.F90:28-36) */
+static double random_normal(double avg, double var, double r1, double r2) {
+  double r = sqrt(-2.0 * log(r1));
+  double theta = 2.0 * PI_T * r2;
+  return r * sin(theta) * sqrt(var) + avg;
+}
+
+void g6o_vreset(int model, int n, double dt, double temp, uint32_t seed,
+                double *dx, double *w) {
+  g6o_model m;
+  g6o_model_get(model, &m);
+  const double massi = 1.0 / m.mass, iin[3] = {1.0 / m.Ixx, 1.0 / m.Iyy, 1.0 / m.Izz};
+  g6o_mt_seed(seed);
+  for (int i = 0; i < n; i++) {
+    for (int a = 0; a < 3; a++) {
+      double r1 = g6o_random_getnext(), r2 = g6o_random_getnext();
+      dx[a * n + i] = random_normal(0.0, temp * rgas() * J2I() * massi, r1, r2) * dt;
+    }
+    for (int a = 0; a < 3; a++) {
+      double r1 = g6o_random_getnext(), r2 = g6o_random_getnext();
+      w[a * n + i] = random_normal(0.0, temp * rgas() * J2I() * iin[a], r1, r2);
+    }
+  }
+}

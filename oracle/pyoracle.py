@@ -1,0 +1,264 @@
+"""ctypes front end of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure only: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.  Every wrapped
+function restates a reference routine; the citations live in g6_oracle.h.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+TIP4P, TIP5P, ST2 = 0, 1, 2
+MODEL_IDS = {TIP4P: "TIP4P   ", TIP5P: "TIP5P   ", ST2: "ST2_____"}
+
+_lib = None
+
+
+def build(quiet=True):
+    """Compile the oracle (gcc, no FMA contraction) next to its source."""
+    import subprocess
+    cmd = ["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-o", LIB,
+           os.path.join(HERE, "g6_oracle.c"), "-lm"]
+    subprocess.run(cmd, check=True, capture_output=quiet)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        _lib = C.CDLL(LIB)
+        _declare(_lib)
+    return _lib
+
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+class Model(C.Structure):
+    _fields_ = [("nsite", C.c_int)] + [
+        (k, C.c_double * 6) for k in ("q", "eps", "sig", "bx", "by", "bz")] + [
+        ("Ixx", C.c_double), ("Iyy", C.c_double), ("Izz", C.c_double),
+        ("mass", C.c_double)]
+
+
+class MD(C.Structure):
+    _fields_ = [("model", C.c_int), ("n", C.c_int), ("fast", C.c_int),
+                ("nose_active", C.c_int), ("L", C.c_double),
+                ("rc", C.c_double), ("taper", C.c_double), ("req", C.c_double),
+                ("dt", C.c_double), ("nose", C.c_double * 9),
+                ("com", C.POINTER(C.c_double)), ("dx", C.POINTER(C.c_double)),
+                ("w", C.POINTER(C.c_double)), ("quat", C.POINTER(C.c_double)),
+                ("ep", C.c_double), ("ekt", C.c_double), ("ekr", C.c_double),
+                ("temperature", C.c_double), ("vir", C.c_double * 9),
+                ("npair", C.c_long)]
+
+
+def _declare(L):
+    L.g6o_model_get.argtypes = [C.c_int, C.POINTER(Model)]
+    L.g6o_constants.argtypes = [_dp]
+    L.g6o_grid_ndiv.argtypes = [C.c_double, C.c_double]
+    L.g6o_grid_ndiv.restype = C.c_int
+    L.g6o_stencil_homo.argtypes = [_ip, _ip, _ip, _ip, C.POINTER(C.c_int), _ip, _ip, _ip]
+    L.g6o_stencil_homo.restype = C.c_int
+    L.g6o_address_assign.argtypes = [C.c_int, _dp, C.c_double, _ip, _ip]
+    L.g6o_pairlist.argtypes = [C.c_int, _dp, C.c_double, C.c_double, C.c_double,
+                               C.c_long, C.POINTER(C.c_long), _ip, _ip, _dp, _dp, _dp]
+    L.g6o_pairlist.restype = C.c_long
+    L.g6o_pairlist_alltoall.argtypes = [C.c_int, _dp, C.c_double, C.c_double,
+                                        C.c_long, _ip, _ip, _dp, _dp, _dp]
+    L.g6o_pairlist_alltoall.restype = C.c_long
+    L.g6o_cutoff_dimmer.argtypes = [C.c_long, _ip, _ip, _dp, _dp, _dp, _dp,
+                                    C.c_double, C.c_double, _dp, _dp]
+    L.g6o_cutoff_func.argtypes = [C.c_double, C.c_double, C.c_double,
+                                  C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    fsig = [C.c_long, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+            _dp, _dp, _dp]
+    L.g6o_force_tip4p.argtypes = fsig
+    L.g6o_force_st2.argtypes = fsig
+    L.g6o_force_std.argtypes = [C.c_int] + fsig
+    L.g6o_energy_std.argtypes = [C.c_int, C.c_long, _ip, _ip, _dp, _dp, _dp, _dp,
+                                 _dp, _dp, _dp, _dp]
+    L.g6o_setsites.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+    L.g6o_collect.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp]
+    L.g6o_predict.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
+    L.g6o_correct.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, _dp, _dp,
+                              _dp, _dp, _dp, _dp]
+    L.g6o_ek.argtypes = [C.c_int, C.c_int, C.c_double, _dp, _dp,
+                         C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.g6o_nose_predict.argtypes = [_dp]
+    L.g6o_nose_correct.argtypes = [_dp, C.c_double]
+    L.g6o_md_step.argtypes = [C.POINTER(MD)]
+    L.g6o_md_step.restype = C.c_long
+    L.g6o_md_hamiltonian.argtypes = [C.POINTER(MD)]
+    L.g6o_md_hamiltonian.restype = C.c_double
+    L.g6o_mt_seed.argtypes = [C.c_uint32]
+    L.g6o_mt_next.restype = C.c_uint32
+    L.g6o_random_getnext.restype = C.c_double
+    L.g6o_vreset.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_uint32,
+                             _dp, _dp]
+
+
+def model(mid):
+    m = Model()
+    lib().g6o_model_get(mid, C.byref(m))
+    return m
+
+
+def constants():
+    out = np.zeros(16)
+    lib().g6o_constants(out)
+    return out
+
+
+def stencil(ndiv):
+    nd = np.array([ndiv] * 3, dtype=np.int32)
+    a = [np.zeros(27, np.int32) for _ in range(6)]
+    n0 = C.c_int()
+    n = lib().g6o_stencil_homo(nd, a[0], a[1], a[2], C.byref(n0), a[3], a[4], a[5])
+    return (np.stack(a[:3], 1)[:n], np.stack(a[3:], 1)[:n0.value])
+
+
+def address_assign(com, L, ndiv):
+    n = com.shape[0]
+    cell = np.zeros(n, np.int32)
+    lib().g6o_address_assign(n, np.ascontiguousarray(com, np.float64), L,
+                             np.array([ndiv] * 3, np.int32), cell)
+    return cell
+
+
+def pairlist(com, L, rc, req=None, alltoall=False):
+    """Returns (npair0, pi, pj, shifts[np,3]) in the reference's order."""
+    com = np.ascontiguousarray(com, np.float64)
+    n = com.shape[0]
+    cap = max(16, n * 800)
+    pi = np.zeros(cap, np.int32); pj = np.zeros(cap, np.int32)
+    o = [np.zeros(cap) for _ in range(3)]
+    np0 = C.c_long(0)
+    if alltoall:
+        k = lib().g6o_pairlist_alltoall(n, com, L, rc, cap, pi, pj, *o)
+        np0.value = n * (n - 1) // 2
+    else:
+        k = lib().g6o_pairlist(n, com, L, rc, req if req else rc, cap,
+                               C.byref(np0), pi, pj, *o)
+    assert k >= 0
+    return np0.value, pi[:k].copy(), pj[:k].copy(), np.stack([x[:k] for x in o], 1)
+
+
+def cutoff(pi, pj, com, shifts, rin, rout):
+    k = len(pi)
+    er = np.zeros(max(k, 1)); fr = np.zeros(max(k, 1))
+    sh = np.ascontiguousarray(shifts.T)
+    lib().g6o_cutoff_dimmer(k, pi, pj, np.ascontiguousarray(com), sh[0].copy(),
+                            sh[1].copy(), sh[2].copy(), rin, rout, er, fr)
+    return er[:k], fr[:k]
+
+
+def setsites(mid, com, quat):
+    """quat: (4, n) array of quat(1) components. Returns sites (n*ns,3), tmat, intra."""
+    m = model(mid)
+    n = com.shape[0]
+    ns = m.nsite
+    x = np.zeros(n * ns); y = np.zeros(n * ns); z = np.zeros(n * ns)
+    t = np.zeros(9 * n); intra = np.zeros(3 * ns * n)
+    q = np.zeros((5, 4, n)); q[0] = quat
+    lib().g6o_setsites(mid, n, np.ascontiguousarray(com), q.ravel(), x, y, z, t, intra)
+    return np.stack([x, y, z], 1), t, intra
+
+
+def force(mid, kind, pi, pj, shifts, er, fr, sites):
+    """kind: 'std' (StdForce), 'fast' (custom TIP4P/ST2 kernel)."""
+    k = len(pi)
+    ns = sites.shape[0]
+    f = [np.zeros(ns) for _ in range(3)]
+    ep = np.zeros(max(k, 1)); vr = np.zeros(9)
+    sh = [np.ascontiguousarray(shifts[:, a]) for a in range(3)]
+    s = [np.ascontiguousarray(sites[:, a]) for a in range(3)]
+    args = [k, pi, pj, *sh, er, fr, *s, *f, ep, vr]
+    if kind == "std":
+        lib().g6o_force_std(mid, *args)
+    elif mid == TIP4P:
+        lib().g6o_force_tip4p(*args)
+    elif mid == ST2:
+        lib().g6o_force_st2(*args)
+    else:
+        raise ValueError("no custom kernel for this model")
+    return np.stack(f, 1), ep[:k], vr.reshape(3, 3)
+
+
+def energy(mid, pi, pj, shifts, er, sites):
+    k = len(pi)
+    ep = np.zeros(max(k, 1))
+    sh = [np.ascontiguousarray(shifts[:, a]) for a in range(3)]
+    s = [np.ascontiguousarray(sites[:, a]) for a in range(3)]
+    lib().g6o_energy_std(mid, k, pi, pj, *sh, er, *s, ep)
+    return ep[:k]
+
+
+def collect(mid, forces, tmat, intra):
+    n = tmat.size // 9
+    F = np.zeros(3 * n); T = np.zeros(3 * n)
+    f = [np.ascontiguousarray(forces[:, a]) for a in range(3)]
+    lib().g6o_collect(mid, n, *f, tmat, intra, F, T)
+    return F.reshape(n, 3), T.reshape(n, 3)
+
+
+class MDRun:
+    """The reference MD step (Mixture.F90 order) on numpy-owned state.
+
+    State arrays: com (n,3); quat (5,4,n); dx (5,3,n); w (5,3,n), exactly the
+    reference's Nordsieck layout with dx(1)=v*dt and w in rad/ps.
+    """
+
+    def __init__(self, mid, com, quat, L, rc, taper, dt, req=None, fast=False,
+                 nose=None):
+        n = com.shape[0]
+        self.n = n
+        self.com = np.ascontiguousarray(com, np.float64).copy()
+        self.quat = np.zeros((5, 4, n)); self.quat[0] = quat
+        self.dx = np.zeros((5, 3, n)); self.w = np.zeros((5, 3, n))
+        self.s = MD()
+        self.s.model = mid; self.s.n = n; self.s.fast = int(fast)
+        self.s.L = L; self.s.rc = rc; self.s.taper = taper
+        self.s.req = req if req else rc; self.s.dt = dt
+        if nose is not None:
+            # nose = (T0 [K], Q as in @NOS2, dump): q internal = Q*dt (Mixture.F90:1206)
+            self.s.nose_active = 1
+            for i, v in enumerate([nose[0], nose[1] * dt, nose[2]] + [0.0] * 6):
+                self.s.nose[i] = v
+        self._bind()
+
+    def _bind(self):
+        P = C.POINTER(C.c_double)
+        self.s.com = self.com.ctypes.data_as(P)
+        self.s.dx = self.dx.ctypes.data_as(P)
+        self.s.w = self.w.ctypes.data_as(P)
+        self.s.quat = self.quat.ctypes.data_as(P)
+
+    def vreset(self, temp, seed):
+        dx1 = np.zeros(3 * self.n); w1 = np.zeros(3 * self.n)
+        lib().g6o_vreset(self.s.model, self.n, self.s.dt, temp, seed, dx1, w1)
+        self.dx[0] = dx1.reshape(3, self.n)
+        self.w[0] = w1.reshape(3, self.n)
+
+    def step(self, nsteps=1):
+        for _ in range(nsteps):
+            k = lib().g6o_md_step(C.byref(self.s))
+            assert k >= 0
+        return self
+
+    def hamiltonian(self):
+        return lib().g6o_md_hamiltonian(C.byref(self.s))
+
+    @property
+    def nose(self):
+        return np.array(self.s.nose[:])
+
+
+def mt_stream(seed, count):
+    lib().g6o_mt_seed(seed)
+    return np.array([lib().g6o_mt_next() for _ in range(count)], dtype=np.uint64)

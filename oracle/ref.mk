@@ -1,0 +1,63 @@
+# Builds the reference's own Fortran modules, unmodified and in place under
+# /root/reference, into oracle/_ref/ (objects + .mod files), then links the
+# fixture dumper refdump.F90 (our code); `make -f oracle/ref.mk mixture`
+# also links the reference MD driver Mixture.
+# Only ever run in the build container; nothing here travels to the GPU box.
+#
+#   make -f oracle/ref.mk            (from the repository root)
+#
+REF     ?= /root/reference
+OUT     ?= oracle/_ref
+REFFC   ?= flang
+# -O2 on baseline x86-64: no FMA instructions, so no contraction.
+FFLAGS  ?= -cpp -DSTDERR=0 -O2
+FFLAGS  += -J$(OUT)
+
+# Module dependency order of the objects the Mixture link line names, plus
+# PairInfo/Neighbor/MonteCarloModule (derived from the reference's .depend).
+OBJS = Umbrella Error PhysConst CommonModule RandomModule Graph Replica MDReplica \
+  MatrixModule VectorModule Monitor QuatModule SiteModule MolModule BoxModule \
+  InteractionModule CutoffModule NoseModule AverageModule TimeModule ExternalModule \
+  AndersenModule GridModule BookModule MonteCarloModule StdInteraction Argon FlexModule \
+  RattleModule Stack TIP4PConst OPLSMeOHModule RigidModule NvdEModule SJBenzene \
+  TIP4PModule Triplet SWSilicon ST2Module Interaction2 FlexSetup EPM2CO2Module \
+  SPCEModule FlatFallerTHFModule SSPKMethaneModule CJTHFModule TIP5PModule PiWater \
+  Dumbbell RigidSetup PairInfo Neighbor
+
+OBJFILES = $(addprefix $(OUT)/,$(addsuffix .o,$(OBJS))) $(OUT)/mtrnd.o
+
+all: $(OUT)/refdump
+
+# The reference MD driver (about 300 MB of static arrays); built only on
+# demand by tests/golden/make_golden.py, which removes it afterwards.
+mixture: $(OUT)/Mixture
+
+$(OUT):
+	mkdir -p $(OUT)
+
+# Serial build: each object needs the .mod files of those before it.
+$(OUT)/objs.stamp: | $(OUT)
+	set -e; for o in $(OBJS); do \
+	  $(REFFC) $(FFLAGS) -c $(REF)/$$o.F90 -o $(OUT)/$$o.o; done
+	$(REFFC) -O2 -c $(REF)/mtrnd.f -o $(OUT)/mtrnd.o
+	touch $@
+
+# The reference's C helpers (process transport, timer) are needed only at
+# link time: MDReplica references them.
+COBJS = $(OUT)/FakeMPI.o $(OUT)/FakeMPIF.o $(OUT)/Timer.o
+$(OUT)/%.o: $(REF)/%.c | $(OUT)
+	gcc -O2 -c $< -o $@
+
+$(OUT)/refdump: oracle/refdump.F90 $(OUT)/objs.stamp $(COBJS)
+	$(REFFC) $(FFLAGS) oracle/refdump.F90 $(OBJFILES) $(COBJS) -o $@
+
+# The reference MD driver itself.
+$(OUT)/Mixture: $(OUT)/objs.stamp $(COBJS)
+	$(REFFC) $(FFLAGS) -c $(REF)/Mixture.F90 -o $(OUT)/Mixture.o
+	$(REFFC) $(FFLAGS) $(OUT)/Mixture.o $(OBJFILES) $(COBJS) -o $@
+	rm -f $(OUT)/Mixture.o
+
+clean:
+	rm -rf $(OUT)
+
+.PHONY: all clean mixture

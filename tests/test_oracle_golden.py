@@ -1,0 +1,172 @@
+"""The oracle against the reference's own outputs (tests/golden/, written by
+running the reference built from its sources) and against closed forms."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as O
+from tests.golden.make_golden import lattice_start
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+CASES = ["tip4p_216", "tip5p_216", "st2_216", "tip4p_125_ndiv3"]
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_grid_and_pairlist_bit_exact(golden, name):
+    g = golden(name)
+    com, L, rc = g["com"], float(g["L"]), float(g["rc"])
+    ndiv = O.lib().g6o_grid_ndiv(L, float(g["req"]))
+    assert ndiv == g["ndiv"][0]
+    st, st0 = O.stencil(ndiv)
+    np.testing.assert_array_equal(st, g["stencil"].reshape(-1, 3))
+    np.testing.assert_array_equal(st0, g["stencil0"].reshape(-1, 3))
+    np.testing.assert_array_equal(O.address_assign(com, L, ndiv), g["cell"])
+    np0, pi, pj, sh = O.pairlist(com, L, rc, float(g["req"]))
+    assert np0 == int(g["npair0"])
+    np.testing.assert_array_equal(pi, g["pi"])
+    np.testing.assert_array_equal(pj, g["pj"])
+    assert np.array_equal(sh, g["shift"])      # FP64 shifts bit-exact
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_cutoff_sites_forces(golden, name):
+    g = golden(name)
+    mid = int(g["model"])
+    rc, taper = float(g["rc"]), float(g["taper"])
+    er, fr = O.cutoff(g["pi"], g["pj"], g["com"], g["shift"], rc - taper, rc)
+    assert np.array_equal(er, g["eratio"]) and np.array_equal(fr, g["fratio"])
+    sites, tm, intra = O.setsites(mid, g["com"], g["quat"].T.copy())
+    assert np.array_equal(sites, g["sites"])
+    F, ep, vr = O.force(mid, "std", g["pi"], g["pj"], g["shift"], er, fr, sites)
+    assert np.array_equal(F, g["force_std"])
+    assert np.array_equal(ep, g["ep_std"])
+    assert np.array_equal(vr, g["vir_std"])
+    Fm, T = O.collect(mid, F, tm, intra)
+    assert np.array_equal(Fm, g["mol_force"]) and np.array_equal(T, g["mol_torque"])
+    assert np.array_equal(O.energy(mid, g["pi"], g["pj"], g["shift"], er, sites), g["epot"])
+    if "force_fast" in g:
+        F2, ep2, vr2 = O.force(mid, "fast", g["pi"], g["pj"], g["shift"], er, fr, sites)
+        # TIP4P custom kernel: same operation order -> bit-exact; ST2: the oracle
+        # loops the 16 Coulomb terms in a different order (<= 1e-14 relative).
+        tol = 0.0 if mid == O.TIP4P else 1e-14
+        scale = np.abs(g["force_fast"]).max()
+        assert np.abs(F2 - g["force_fast"]).max() <= tol * scale
+        assert np.abs(ep2 - g["ep_fast"]).max() <= tol * np.abs(g["ep_fast"]).max()
+
+
+def test_pairlist_digests_larger_grids():
+    dig = json.load(open(os.path.join(GOLD, "pairlist_digests.json")))
+    import hashlib
+    sha = lambda a: hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+    for key, d in dig.items():
+        com, q = lattice_start(d["n"], d["L"], d["seed"])
+        np0, pi, pj, sh = O.pairlist(com, d["L"], d["rc"], d["rc"])
+        assert (np0, len(pi)) == (d["npair0"], d["npair"]), key
+        assert sha(pi) == d["sha_pi"] and sha(pj) == d["sha_pj"], key
+        assert sha(sh) == d["sha_shift"], key
+        assert sha(O.address_assign(com, d["L"], d["ndiv"])) == d["sha_cell"]
+
+
+@pytest.mark.parametrize("name", ["md_tip4p_216_nve", "md_tip4p_216_nvt"])
+def test_md_trajectory_matches_reference_driver(golden, name):
+    g = golden(name)
+    nose = None if g["nose"][1] == 0 else tuple(g["nose"])
+    md = O.MDRun(O.TIP4P, g["com"], g["quat"].T.copy(), float(g["L"]), float(g["rc"]),
+                 float(g["taper"]), float(g["dt"]), fast=False, nose=nose)
+    md.vreset(300.0, int(g["seed"]))
+    for s in range(int(g["nsteps"])):
+        md.step()
+        row = g["logd"][s]
+        assert md.s.temperature == row[1]                 # @LOGD column 2
+        assert md.s.ep * 10 * 1e-3 == row[4]              # column 5, kJ/mol
+        assert abs(md.hamiltonian() - row[12]) <= 1e-12 * abs(row[12])
+    n = md.n
+    assert np.array_equal(md.com.T, g["com_final"])
+    assert np.array_equal(md.dx, g["dxyz_final"].transpose(1, 0, 2))
+    assert np.array_equal(md.quat, g["quat_final"].transpose(1, 0, 2))
+    assert np.array_equal(md.w * float(g["dt"]), g["w_final_ext"].transpose(1, 0, 2))
+
+
+def test_mt19937_known_answers():
+    # std::mt19937 default seed 5489: first output, and the 10000th output
+    # required by the C++ standard ([rand.predef]).
+    s = O.mt_stream(5489, 10000)
+    assert s[0] == 3499211612
+    assert s[9999] == 4123659995
+
+
+def test_cutoff_closed_form():
+    rin, rout = 8.5, 9.0
+    hh, dh = np.zeros(1), np.zeros(1)
+    import ctypes as C
+    def f(r):
+        a, b = C.c_double(), C.c_double()
+        O.lib().g6o_cutoff_func(rin, rout, r * r, C.byref(a), C.byref(b))
+        return a.value, b.value
+    assert f(8.0) == (1.0, 0.0) and f(9.0) == (0.0, 0.0)
+    # S continuous at both ends, S' = -dh*r matches a central difference
+    for r in np.linspace(8.51, 8.99, 17):
+        e = 1e-6
+        d = (f(r + e)[0] - f(r - e)[0]) / (2 * e)
+        assert abs(d - (-f(r)[1] * r)) < 1e-7
+    assert abs(f(rin + 1e-7)[0] - 1) < 1e-12 and abs(f(rout - 1e-7)[0]) < 1e-12
+
+
+@pytest.mark.parametrize("mid", [O.TIP4P, O.TIP5P, O.ST2])
+def test_force_is_minus_gradient(mid):
+    """F = -dE/dx by central differences on one molecule's COM; sum F = 0."""
+    L, rc, taper = 14.0, 6.5, 1.0
+    com, q = lattice_start(27, L, 7 + mid)
+    def energy_and_force(c):
+        _, pi, pj, sh = O.pairlist(c, L, rc, alltoall=True)
+        er, fr = O.cutoff(pi, pj, c, sh, rc - taper, rc)
+        sites, tm, intra = O.setsites(mid, c, q.T.copy())
+        kind = "fast" if mid != O.TIP5P else "std"
+        F, ep, _ = O.force(mid, kind, pi, pj, sh, er, fr, sites)
+        Fm, _ = O.collect(mid, F, tm, intra)
+        return ep.sum(), Fm
+    E0, F0 = energy_and_force(com)
+    assert np.abs(F0.sum(0)).max() < 1e-9 * np.abs(F0).max()
+    h = 1e-6
+    for i in (0, 13):
+        for a in range(3):
+            cp = com.copy(); cp[i, a] += h
+            cm = com.copy(); cm[i, a] -= h
+            num = -(energy_and_force(cp)[0] - energy_and_force(cm)[0]) / (2 * h)
+            assert abs(num - F0[i, a]) < 1e-5 * max(1.0, abs(F0[i, a]))
+
+
+def test_rotation_matrix_orthonormal():
+    com, q = lattice_start(50, 20.0, 3)
+    _, tm, _ = O.setsites(O.TIP4P, com, q.T.copy())
+    for t in tm.reshape(-1, 3, 3):
+        assert np.allclose(t @ t.T, np.eye(3), atol=1e-14)
+        assert abs(np.linalg.det(t) - 1) < 1e-14
+    _, tm, _ = O.setsites(O.TIP4P, com[:1], np.array([[1.0], [0.0], [0.0], [0.0]]))
+    assert np.array_equal(tm.reshape(3, 3), np.eye(3))
+
+
+def test_lj_coulomb_dimer_closed_form():
+    """Two TIP4P molecules with parallel orientation: energy from textbook
+    Coulomb + LJ sums over the site table equals the kernel's pair energy."""
+    m = O.model(O.TIP4P)
+    K = O.constants()
+    com = np.array([[0.0, 0.0, 0.0], [3.0, 0.4, -0.2]])
+    q = np.array([[1.0, 0.0, 0.0, 0.0], [1.0, 0.0, 0.0, 0.0]])
+    sites, tm, intra = O.setsites(O.TIP4P, com, q.T.copy())
+    _, pi, pj, sh = O.pairlist(com, 100.0, 9.0, alltoall=True)
+    er, fr = O.cutoff(pi, pj, com, sh, 8.5, 9.0)
+    _, ep, _ = O.force(O.TIP4P, "fast", pi, pj, sh, er, fr, sites)
+    body = np.stack([m.bx[:5], m.by[:5], m.bz[:5]], 1)
+    e = 0.0
+    qs = [0.0, -1.04, 0.52, 0.52]
+    C = 4.184 * 332.17752 * 100.0
+    for a in range(1, 4):
+        for b in range(1, 4):
+            r = np.linalg.norm(body[a] - (body[b] + com[1]))
+            e += qs[a] * qs[b] * C / r
+    r = np.linalg.norm(body[0] - (body[0] + com[1]))
+    e += K[3] / r ** 12 + K[4] / r ** 6
+    assert abs(ep[0] - e) < 1e-12 * abs(e)
