@@ -1,0 +1,634 @@
+// C ABI of g6gpu (include/g6gpu.h): handles, host-side tables, state
+// transfer, the MD step driver (stream launches captured into a hipGraph),
+// MT19937 for the Maxwell-Boltzmann start, and the synthetic lattice start.
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
+#include "g6_internal.h"
+
+namespace g6 {
+
+// ---- reference constants (PhysConst.F90, TIP4PConst.F90, ST2Module.F90) ---
+static const double kPI = 3.1415926535897932;  // PItanaka
+static const double kNA = 6.0225e23, kKB = 1.380066e-23, kCA = 4.184;
+static double coeff() { return kCA * 332.17752; }
+static double ipow(double x, int k) {
+  double r = x;
+  for (int i = 1; i < k; i++) r = r * x;
+  return r;
+}
+
+bool make_model_table(int model, int prec, ModelTable *m) {
+  (void)prec;
+  memset(m, 0, sizeof(*m));
+  m->model = model;
+  m->lj_site = 0;
+  if (model == G6_TIP4P) {
+    // Rigid_TIP4P_Constructor (TIP4PModule.F90:70-97)
+    m->nsite = 5;
+    m->Ixx = 1.75618411509879202e+00;
+    m->Iyy = 6.10236519209664374e-01;
+    m->Izz = 1.14594759588912742e+00;
+    m->mass = 18.0;
+    m->by[2] = 7.56950327263661182e-01;
+    m->by[3] = -7.56950327263661182e-01;
+    m->bz[0] = -6.50980307353661025e-02;
+    m->bz[1] = 8.49019692646338919e-02;
+    m->bz[2] = 5.20784245882928820e-01;
+    m->bz[3] = 5.20784245882928820e-01;
+    // custom-kernel constants WWMM, WWMH, WWHH, AA, BB (TIP4PConst.F90:4-13)
+    const double C = coeff();
+    const double wwmm = -1.04 * -1.04 * C * 100.0, wwmh = -1.04 * 0.52 * C * 100.0,
+                 wwhh = 0.52 * 0.52 * C * 100.0;
+    const int ci[9] = {1, 1, 1, 2, 3, 2, 2, 3, 3}, cj[9] = {1, 2, 3, 1, 1, 2, 3, 2, 3};
+    m->ncoul = 9;
+    for (int t = 0; t < 9; t++) {
+      m->ci[t] = ci[t];
+      m->cj[t] = cj[t];
+      m->qq[t] = (ci[t] == 1 && cj[t] == 1) ? wwmm
+                 : (ci[t] == 1 || cj[t] == 1) ? wwmh : wwhh;
+    }
+    m->lj_a = 600000.0 * kCA * 100.0;
+    m->lj_b = -610.0 * kCA * 100.0;
+    return true;
+  }
+  if (model == G6_TIP5P || model == G6_ST2) {
+    const bool st2 = model == G6_ST2;
+    // Rigid_TIP5P_Constructor (RigidModule.F90:429-477),
+    // Rigid_ST2_Constructor (ST2Module.F90:60-100)
+    const double theta = (st2 ? 109.47 : 104.52) * kPI / 180.0;
+    const double phi = 109.47 * kPI / 180.0;
+    const double l1 = st2 ? 1.0 : 0.9572, l2 = st2 ? 0.8 : 0.70;
+    const double hmass = 1.0, omass = 16.0;
+    const double ang2 = st2 ? theta : phi;
+    m->nsite = 6;
+    m->by[1] = l1 * sin(0.5 * theta);
+    m->bz[1] = l1 * cos(0.5 * theta);
+    m->by[2] = -l1 * sin(0.5 * theta);
+    m->bz[2] = l1 * cos(0.5 * theta);
+    m->bx[3] = l2 * sin(0.5 * ang2);
+    m->bz[3] = -l2 * cos(0.5 * ang2);
+    m->bx[4] = -l2 * sin(0.5 * ang2);
+    m->bz[4] = -l2 * cos(0.5 * ang2);
+    const double zshift = m->bz[1] * 2.0 * hmass / (2.0 * hmass + omass);
+    for (int i = 0; i < 5; i++) m->bz[i] = m->bz[i] - zshift;
+    m->Ixx = omass * (m->by[0] * m->by[0] + m->bz[0] * m->bz[0]) +
+             hmass * (m->by[1] * m->by[1] + m->bz[1] * m->bz[1]) * 2.0;
+    m->Iyy = omass * (m->bz[0] * m->bz[0] + m->bx[0] * m->bx[0]) +
+             hmass * (m->bz[1] * m->bz[1] + m->bx[1] * m->bx[1]) * 2.0;
+    m->Izz = omass * (m->bx[0] * m->bx[0] + m->by[0] * m->by[0]) +
+             hmass * (m->bx[1] * m->bx[1] + m->by[1] * m->by[1]) * 2.0;
+    m->mass = 2.0 * hmass + omass;
+    m->ncoul = 16;
+    if (!st2) {
+      // StdForce coefficients (StdInteraction.F90:206-221), the live TIP5P path
+      const double q[6] = {0.0, 0.2410, 0.2410, -0.2410, -0.2410, 0.0};
+      const double KX2X = 1.0 / 1e-3, J2I = 1.0 / 10.0;
+      for (int t = 0; t < 16; t++) {
+        const int a = 1 + t / 4, b = 1 + t % 4;
+        m->ci[t] = a;
+        m->cj[t] = b;
+        m->qq[t] = q[a] * q[b] * coeff() * KX2X * J2I * 1.0;
+      }
+      const double eps = 0.6694, sig = 3.12;
+      const double ljeps = sqrt(eps * eps);
+      const double s6 = ipow((sig + sig) * 0.5, 6);
+      m->lj_a = 4.0 * ljeps * s6 * s6 * KX2X * J2I * 1.0;
+      m->lj_b = -4.0 * ljeps * s6 * KX2X * J2I * 1.0;
+    } else {
+      // force_st2 constants (ST2Module.F90:15-22, 121-125); the charges and
+      // eps are single-precision literals in the reference
+      const double qh = (double)0.2357f, qm = (double)(-0.2357f);
+      const double eps = (double)0.52605e-21f * kNA * 1e-3, sig = 3.100;
+      const double C = coeff();
+      for (int t = 0; t < 16; t++) {
+        const int a = 1 + t / 4, b = 1 + t % 4;
+        m->ci[t] = a;
+        m->cj[t] = b;
+        const bool ha = a <= 2, hb = b <= 2;
+        m->qq[t] = (ha && hb) ? qh * qh * C * 100.0
+                   : (!ha && !hb) ? qm * qm * C * 100.0 : qh * qm * C * 100.0;
+      }
+      m->lj_a = 4.0 * eps * ipow(sig, 12) * 100.0;
+      m->lj_b = -4.0 * eps * ipow(sig, 6) * 100.0;
+      m->rl = 2.0160;
+      m->ru = 3.1287;
+    }
+    return true;
+  }
+  return false;
+}
+
+// half-shell stencil, maxradius = 0 branch (NeighborCell_ListHomo,
+// GridModule.F90:279-374); offsets packed ox | oy<<8 | oz<<16
+static void make_stencil(int nd, Geom *g) {
+  int nmax = 1, nmin = -1;
+  if (nmax > (nd - 1) / 2) nmax = (nd - 1) / 2;
+  if (nmin < -(nd / 2)) nmin = -(nd / 2);
+  std::vector<int> mark((size_t)nd * nd * nd, 0);
+  auto MK = [&](int x, int y, int z) -> int & { return mark[((size_t)z * nd + y) * nd + x]; };
+  g->nst = g->nst0 = 0;
+  for (int jz = nmin; jz <= nmax; jz++) {
+    const int pz = jz < 0 ? jz + nd : jz, mz = pz == 0 ? 0 : nd - pz;
+    for (int jy = nmin; jy <= nmax; jy++) {
+      const int py = jy < 0 ? jy + nd : jy, my = py == 0 ? 0 : nd - py;
+      for (int jx = nmin; jx <= nmax; jx++) {
+        const int px = jx < 0 ? jx + nd : jx, mx = px == 0 ? 0 : nd - px;
+        if (MK(px, py, pz) == 0) {
+          MK(mx, my, mz) += 1;
+          MK(px, py, pz) += 1;
+          const int packed = px | (py << 8) | (pz << 16);
+          if (MK(px, py, pz) != 1) g->st0[g->nst0++] = packed;
+          else g->st[g->nst++] = packed;
+        }
+      }
+    }
+  }
+}
+
+// ---- MT19937 (Matsumoto & Nishimura 1998; init_genrand of 2002) ---------
+struct MT {
+  uint32_t s[624];
+  int k = 624;
+  explicit MT(uint32_t seed) {
+    s[0] = seed;
+    for (int i = 1; i < 624; i++) s[i] = 1812433253u * (s[i - 1] ^ (s[i - 1] >> 30)) + (uint32_t)i;
+  }
+  uint32_t next() {
+    if (k >= 624) {
+      for (int i = 0; i < 624; i++) {
+        const uint32_t y = (s[i] & 0x80000000u) | (s[(i + 1) % 624] & 0x7fffffffu);
+        s[i] = s[(i + 397) % 624] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+      }
+      k = 0;
+    }
+    uint32_t y = s[k++];
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= y >> 18;
+    return y;
+  }
+  // 31-bit uniform in [0,1) as mtrndv (mtrnd.f:280-306)
+  double uniform() { return (double)(next() >> 1) * (1.0 / 2147483648.0); }
+  // Random_GetNext returns 1 - x (RandomModule.F90:123-130)
+  double getnext() { return 1.0 - uniform(); }
+};
+
+static std::mutex g_mu;
+static std::vector<System *> g_sys;
+
+static System *get(g6_handle h) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  if (h <= 0 || h > (g6_handle)g_sys.size()) return nullptr;
+  return g_sys[h - 1];
+}
+
+static int alloc_all(System &s) {
+  const int n = s.n, ns = s.ns, nc = s.geo.ncell;
+  auto A = [](void **p, size_t b) { return hipMalloc(p, b > 0 ? b : 8); };
+#define AL(ptr, bytes) G6_CHECK(A((void **)&(ptr), (bytes)))
+  AL(s.com, sizeof(double) * 3 * n);
+  AL(s.dx, sizeof(double) * 15 * n);
+  AL(s.w, sizeof(double) * 15 * n);
+  AL(s.quat, sizeof(double) * 20 * n);
+  AL(s.tmat, sizeof(double) * 9 * n);
+  AL(s.intra, sizeof(double) * 3 * ns * n);
+  AL(s.intraf, sizeof(float) * 3 * ns * n);
+  AL(s.com_ref, sizeof(double) * 3 * n);
+  AL(s.cell, sizeof(int) * n);
+  AL(s.cell_count, sizeof(int) * (nc + 1));
+  AL(s.cell_start, sizeof(int) * (nc + 1));
+  AL(s.cell_fill, sizeof(int) * (nc + 1));
+  AL(s.sorted_tmp, sizeof(int) * n);
+  AL(s.sorted, sizeof(int) * n);
+  AL(s.cnt, sizeof(int) * 2 * n);
+  AL(s.off, sizeof(int) * (2 * n + 1));
+  AL(s.pi, sizeof(int) * (size_t)s.pcap);
+  AL(s.pj, sizeof(int) * (size_t)s.pcap);
+  AL(s.scode, sizeof(int) * (size_t)s.pcap);
+  AL(s.facc, sizeof(double) * 12 * n);
+  AL(s.rowpart, sizeof(double) * 11 * n);
+  AL(s.blockpart, sizeof(double) * 4 * s.nblk_mol);
+  AL(s.nose, sizeof(double) * 9);
+  AL(s.thermo, sizeof(double) * G6_THERMO_LEN);
+  AL(s.flags, sizeof(int) * 8);
+#undef AL
+  G6_CHECK(hipMemset(s.com, 0, sizeof(double) * 3 * n));
+  G6_CHECK(hipMemset(s.dx, 0, sizeof(double) * 15 * n));
+  G6_CHECK(hipMemset(s.w, 0, sizeof(double) * 15 * n));
+  G6_CHECK(hipMemset(s.quat, 0, sizeof(double) * 20 * n));
+  G6_CHECK(hipMemset(s.off, 0, sizeof(int) * (2 * n + 1)));
+  G6_CHECK(hipMemset(s.nose, 0, sizeof(double) * 9));
+  G6_CHECK(hipMemset(s.thermo, 0, sizeof(double) * G6_THERMO_LEN));
+  return 0;
+}
+
+// list and accumulators ready for a fresh first step
+static int reset_step_state(System &s) {
+  int fl[8] = {0, 1, 0, 0, 0, 0, 0, 0};
+  G6_CHECK(hipMemcpy(s.flags, fl, sizeof(fl), hipMemcpyHostToDevice));
+  G6_CHECK(hipMemset(s.cell_count, 0, sizeof(int) * (s.geo.ncell + 1)));
+  G6_CHECK(hipMemset(s.facc, 0, sizeof(double) * 12 * s.n));
+  return 0;
+}
+
+static void free_all(System &s) {
+  void *ps[] = {s.com, s.dx, s.w, s.quat, s.tmat, s.intra, s.intraf, s.com_ref, s.cell,
+                s.cell_count, s.cell_start, s.cell_fill, s.sorted_tmp, s.sorted, s.cnt,
+                s.off, s.pi, s.pj, s.scode, s.facc, s.rowpart, s.blockpart, s.nose,
+                s.thermo, s.flags};
+  for (void *p : ps)
+    if (p) (void)hipFree(p);
+  if (s.graph) (void)hipGraphExecDestroy(s.graph);
+  for (auto &e : s.ev) (void)hipEventDestroy(e);
+  if (s.stream) (void)hipStreamDestroy(s.stream);
+}
+
+static void launch_step(System &s) {
+  launch_predict_sites_bin(s, true);
+  launch_pairlist(s);
+  launch_force_rows(s);
+  launch_correct(s);
+  launch_thermo(s);
+}
+
+static int graph_steps() {
+  const char *e = getenv("G6_GRAPH_STEPS");
+  int v = e ? atoi(e) : 20;
+  return v < 1 ? 1 : (v > 1000 ? 1000 : v);
+}
+
+}  // namespace g6
+
+using namespace g6;
+
+extern "C" {
+
+int g6_model_nsite(int model) {
+  ModelTable m;
+  return make_model_table(model, 0, &m) ? m.nsite : 0;
+}
+
+int g6_init(g6_handle *h, int model, int n, double box_l, double rc, double taper,
+            double voxa, double dt, int prec, double skin) {
+  if (!h || n < 2 || !(box_l > 0) || !(rc > 0) || !(taper >= 0) || taper > rc || !(dt > 0) ||
+      (prec != G6_FP64 && prec != G6_FP32) || !(skin >= 0) || !(voxa > 0))
+    return G6_EARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return G6_ENODEVICE;
+  System *s = new System();
+  if (!make_model_table(model, prec, &s->mt)) {
+    delete s;
+    return G6_EARG;
+  }
+  s->model = model;
+  s->n = n;
+  s->ns = s->mt.nsite;
+  s->prec = prec;
+  s->L = box_l;
+  s->rc = rc;
+  s->taper = taper;
+  s->req = voxa;
+  s->dt = dt;
+  s->skin = skin;
+  Geom &g = s->geo;
+  g.n = n;
+  // Grid_ResetDivision: integer part of L/req (GridModule.F90:497-499)
+  g.ndiv = (int)(box_l / voxa);
+  if (g.ndiv < 1) g.ndiv = 1;
+  if (rc + skin > 0.5 * box_l * (1.0 + 1e-12) && g.ndiv > 2) {
+    delete s;  // the half-shell stencil of 27 cells needs rc + skin <= L/2
+    return G6_EARG;
+  }
+  g.ncell = g.ndiv * g.ndiv * g.ndiv;
+  make_stencil(g.ndiv, &g);
+  g.L = box_l;
+  g.invL = 1.0 / box_l;                  // Box_ReadBXLA
+  g.cellinv = 1.0 / (box_l / g.ndiv);    // Grid_ResetCellSize + invert
+  const double rl = rc + skin;
+  g.rr = rl * rl;
+  g.rc2 = rc * rc;
+  g.rin = rc - taper;
+  g.rout = rc;
+  g.c1 = taper > 0 ? 1.0 / ipow(g.rin - g.rout, 5) : 0.0;  // cutoff_initialize
+  g.c2 = g.c1 * 30.0;
+  // pair capacity: 1.6x the ideal-gas half-shell count within rc+skin, +64/molecule
+  const double per = 0.5 * 4.0 / 3.0 * kPI * rl * rl * rl * n / (box_l * box_l * box_l);
+  const double capd = (double)n * (per * 1.6 + 64.0);
+  const double all = 0.5 * (double)n * (double)(n - 1);
+  s->pcap = (int)fmin(fmin(capd, all + 64.0), 2.0e9);
+  s->nblk_mol = (n + 255) / 256;
+  int rcode = alloc_all(*s);
+  if (rcode == 0 && hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
+    rcode = G6_EHIP;
+  for (int k = 0; rcode == 0 && k < 8; k++)
+    if (hipEventCreate(&s->ev[k]) != hipSuccess) rcode = G6_EHIP;
+  if (rcode == 0) rcode = reset_step_state(*s);
+  if (rcode != 0) {
+    free_all(*s);
+    delete s;
+    return rcode;
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_sys.push_back(s);
+  *h = (g6_handle)g_sys.size();
+  return 0;
+}
+
+int g6_finalize(g6_handle h) {
+  System *s = get(h);
+  if (!s) return G6_EARG;
+  (void)hipStreamSynchronize(s->stream);
+  free_all(*s);
+  delete s;
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_sys[h - 1] = nullptr;
+  return 0;
+}
+
+int g6_upload_state(g6_handle h, const double *com, const double *quat, const double *dx,
+                    const double *w) {
+  System *s = get(h);
+  if (!s || !com || !quat || !dx || !w) return G6_EARG;
+  const int n = s->n;
+  G6_CHECK(hipMemcpy(s->com, com, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+  G6_CHECK(hipMemcpy(s->quat, quat, sizeof(double) * 20 * n, hipMemcpyHostToDevice));
+  G6_CHECK(hipMemcpy(s->dx, dx, sizeof(double) * 15 * n, hipMemcpyHostToDevice));
+  G6_CHECK(hipMemcpy(s->w, w, sizeof(double) * 15 * n, hipMemcpyHostToDevice));
+  return reset_step_state(*s);
+}
+
+int g6_download_state(g6_handle h, double *com, double *quat, double *dx, double *w) {
+  System *s = get(h);
+  if (!s) return G6_EARG;
+  const int n = s->n;
+  G6_CHECK(hipStreamSynchronize(s->stream));
+  if (com) G6_CHECK(hipMemcpy(com, s->com, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+  if (quat) G6_CHECK(hipMemcpy(quat, s->quat, sizeof(double) * 20 * n, hipMemcpyDeviceToHost));
+  if (dx) G6_CHECK(hipMemcpy(dx, s->dx, sizeof(double) * 15 * n, hipMemcpyDeviceToHost));
+  if (w) G6_CHECK(hipMemcpy(w, s->w, sizeof(double) * 15 * n, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int g6_set_nose(g6_handle h, int active, const double *nose) {
+  System *s = get(h);
+  if (!s) return G6_EARG;
+  if (active && (!nose || !(nose[1] > 0))) return G6_EARG;
+  s->nose_active = active != 0;
+  double z[9] = {0};
+  if (nose) memcpy(z, nose, sizeof(z));
+  G6_CHECK(hipStreamSynchronize(s->stream));
+  G6_CHECK(hipMemcpy(s->nose, z, sizeof(z), hipMemcpyHostToDevice));
+  if (s->graph) {  // the captured launches embed the Nose flag
+    (void)hipGraphExecDestroy(s->graph);
+    s->graph = nullptr;
+  }
+  return 0;
+}
+
+int g6_get_nose(g6_handle h, double *nose) {
+  System *s = get(h);
+  if (!s || !nose) return G6_EARG;
+  G6_CHECK(hipStreamSynchronize(s->stream));
+  G6_CHECK(hipMemcpy(nose, s->nose, sizeof(double) * 9, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int g6_vreset(g6_handle h, double temp, uint32_t seed) {
+  // Rigid_VReset2 (RigidModule.F90:1016-1053) with Random_Normal
+  // (RandomModule.F90:28-36): dx(1) = N(0, T*rgas*J2I/m)*dt, w(1) = N(0, T*rgas*J2I/I)
+  System *s = get(h);
+  if (!s || !(temp >= 0)) return G6_EARG;
+  const int n = s->n;
+  std::vector<double> dx(15 * (size_t)n), w(15 * (size_t)n);
+  G6_CHECK(hipStreamSynchronize(s->stream));
+  G6_CHECK(hipMemcpy(dx.data(), s->dx, sizeof(double) * 15 * n, hipMemcpyDeviceToHost));
+  G6_CHECK(hipMemcpy(w.data(), s->w, sizeof(double) * 15 * n, hipMemcpyDeviceToHost));
+  MT mt(seed);
+  const double rgas = kNA * kKB, J2I = 1.0 / 10.0;
+  const double massi = 1.0 / s->mt.mass;
+  const double ii[3] = {1.0 / s->mt.Ixx, 1.0 / s->mt.Iyy, 1.0 / s->mt.Izz};
+  auto normal = [](double var, double r1, double r2) {
+    const double r = sqrt(-2.0 * log(r1));
+    const double theta = 2.0 * kPI * r2;
+    return r * sin(theta) * sqrt(var) + 0.0;
+  };
+  for (int i = 0; i < n; i++) {
+    for (int a = 0; a < 3; a++) {
+      const double r1 = mt.getnext(), r2 = mt.getnext();
+      dx[(size_t)a * n + i] = normal(temp * rgas * J2I * massi, r1, r2) * s->dt;
+    }
+    for (int a = 0; a < 3; a++) {
+      const double r1 = mt.getnext(), r2 = mt.getnext();
+      w[(size_t)a * n + i] = normal(temp * rgas * J2I * ii[a], r1, r2);
+    }
+  }
+  G6_CHECK(hipMemcpy(s->dx, dx.data(), sizeof(double) * 15 * n, hipMemcpyHostToDevice));
+  G6_CHECK(hipMemcpy(s->w, w.data(), sizeof(double) * 15 * n, hipMemcpyHostToDevice));
+  return 0;
+}
+
+static int check_flags(System &s) {
+  int fl[2] = {0, 0};
+  G6_CHECK(hipMemcpyAsync(fl, s.flags, sizeof(fl), hipMemcpyDeviceToHost, s.stream));
+  G6_CHECK(hipStreamSynchronize(s.stream));
+  if (fl[0] == 1) return G6_ECAPACITY;
+  if (fl[0] == 2) return G6_EARG;
+  return 0;
+}
+
+int g6_step(g6_handle h, int nsteps) {
+  System *s = get(h);
+  if (!s || nsteps < 0) return G6_EARG;
+  const int G = graph_steps();
+  if (nsteps >= G) {
+    if (!s->graph || s->graph_steps != G) {
+      if (s->graph) {
+        (void)hipGraphExecDestroy(s->graph);
+        s->graph = nullptr;
+      }
+      hipGraph_t gr;
+      G6_CHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+      for (int k = 0; k < G; k++) launch_step(*s);
+      G6_CHECK(hipStreamEndCapture(s->stream, &gr));
+      G6_CHECK(hipGraphInstantiate(&s->graph, gr, nullptr, nullptr, 0));
+      G6_CHECK(hipGraphDestroy(gr));
+      s->graph_steps = G;
+    }
+    for (int k = 0; k + G <= nsteps; k += G) G6_CHECK(hipGraphLaunch(s->graph, s->stream));
+  }
+  for (int k = 0; k < nsteps % G; k++) launch_step(*s);
+  G6_CHECK(hipGetLastError());
+  s->steps_done += nsteps;
+  return check_flags(*s);
+}
+
+int g6_step_timed(g6_handle h, int nsteps, double *t) {
+  System *s = get(h);
+  if (!s || nsteps < 1 || !t) return G6_EARG;
+  double acc[4] = {0, 0, 0, 0};
+  for (int k = 0; k < nsteps; k++) {
+    G6_CHECK(hipEventRecord(s->ev[0], s->stream));
+    launch_predict_sites_bin(*s, true);
+    G6_CHECK(hipEventRecord(s->ev[1], s->stream));
+    launch_pairlist(*s);
+    G6_CHECK(hipEventRecord(s->ev[2], s->stream));
+    launch_force_rows(*s);
+    G6_CHECK(hipEventRecord(s->ev[3], s->stream));
+    launch_correct(*s);
+    launch_thermo(*s);
+    G6_CHECK(hipEventRecord(s->ev[4], s->stream));
+    G6_CHECK(hipEventSynchronize(s->ev[4]));
+    float a, b, c, d, e;
+    G6_CHECK(hipEventElapsedTime(&a, s->ev[0], s->ev[1]));
+    G6_CHECK(hipEventElapsedTime(&b, s->ev[1], s->ev[2]));
+    G6_CHECK(hipEventElapsedTime(&c, s->ev[2], s->ev[3]));
+    G6_CHECK(hipEventElapsedTime(&d, s->ev[3], s->ev[4]));
+    G6_CHECK(hipEventElapsedTime(&e, s->ev[0], s->ev[4]));
+    acc[0] += c;
+    acc[1] += b;
+    acc[2] += a + d;
+    acc[3] += e;
+  }
+  for (int q = 0; q < 4; q++) t[q] = acc[q] / nsteps;
+  s->steps_done += nsteps;
+  return check_flags(*s);
+}
+
+int g6_eval_forces(g6_handle h, double *force, double *torque, double *thermo) {
+  System *s = get(h);
+  if (!s || !force || !torque) return G6_EARG;
+  const int n = s->n;
+  double *out = nullptr;
+  G6_CHECK(hipMalloc(&out, sizeof(double) * 6 * n));
+  // the state is unchanged: a list build is forced and the thermostat is
+  // left alone (k_thermo gets nose_active = 0 through a copy of the system)
+  int one = 1;
+  G6_CHECK(hipMemcpyAsync(s->flags + 1, &one, sizeof(int), hipMemcpyHostToDevice, s->stream));
+  launch_predict_sites_bin(*s, false);
+  launch_pairlist(*s);
+  launch_force_rows(*s);
+  launch_collect_acc(*s, out);
+  System tmp = *s;
+  tmp.nose_active = false;
+  tmp.skin = 0;  // keeps the build gate set for the next step
+  launch_thermo(tmp);
+  G6_CHECK(hipGetLastError());
+  G6_CHECK(hipMemcpyAsync(force, out, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s->stream));
+  G6_CHECK(hipMemcpyAsync(torque, out + 3 * n, sizeof(double) * 3 * n, hipMemcpyDeviceToHost,
+                          s->stream));
+  G6_CHECK(hipStreamSynchronize(s->stream));
+  G6_CHECK(hipFree(out));
+  if (thermo) G6_CHECK(hipMemcpy(thermo, s->thermo, sizeof(double) * G6_THERMO_LEN,
+                                 hipMemcpyDeviceToHost));
+  // the next step must rebuild the list from the predicted positions
+  G6_CHECK(hipMemcpy(s->flags + 1, &one, sizeof(int), hipMemcpyHostToDevice));
+  return check_flags(*s);
+}
+
+int g6_get_thermo(g6_handle h, double *thermo) {
+  System *s = get(h);
+  if (!s || !thermo) return G6_EARG;
+  G6_CHECK(hipStreamSynchronize(s->stream));
+  G6_CHECK(hipMemcpy(thermo, s->thermo, sizeof(double) * G6_THERMO_LEN, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int g6_pairlist(g6_handle h, const double *com, int64_t cap, int64_t *npair0, int64_t *npair,
+                int32_t *pair_i, int32_t *pair_j, double *ox, double *oy, double *oz) {
+  System *s = get(h);
+  if (!s || !com || !npair0 || !npair || cap < 0) return G6_EARG;
+  if (s->skin != 0) return G6_EARG;  // the reference list is the skin-0 list
+  return pairlist_once(*s, com, cap, npair0, npair, pair_i, pair_j, ox, oy, oz);
+}
+
+int g6_cells(g6_handle h, const double *com, int32_t *cell, int32_t *ndiv) {
+  System *s = get(h);
+  if (!s || !com || !cell || !ndiv) return G6_EARG;
+  *ndiv = s->geo.ndiv;
+  return cells_once(*s, com, cell);
+}
+
+int g6_cutoff(g6_handle h, int64_t npair, const int32_t *pi, const int32_t *pj,
+              const double *com, const double *ox, const double *oy, const double *oz,
+              double *eratio, double *fratio) {
+  System *s = get(h);
+  if (!s || npair < 0) return G6_EARG;
+  return cutoff_once(*s, npair, pi, pj, com, ox, oy, oz, eratio, fratio);
+}
+
+int g6_force(g6_handle h, int64_t npair, const int32_t *pi, const int32_t *pj,
+             const double *ox, const double *oy, const double *oz, const double *er,
+             const double *fr, const double *x, const double *y, const double *z, double *fx,
+             double *fy, double *fz, double *ep, double *vrmat) {
+  System *s = get(h);
+  if (!s || npair < 0 || !vrmat) return G6_EARG;
+  return force_pairs(*s, npair, pi, pj, ox, oy, oz, er, fr, x, y, z, fx, fy, fz, ep, vrmat,
+                     false);
+}
+
+int g6_pair_energy(g6_handle h, int64_t npair, const int32_t *pi, const int32_t *pj,
+                   const double *ox, const double *oy, const double *oz, const double *er,
+                   const double *x, const double *y, const double *z, double *ep) {
+  System *s = get(h);
+  if (!s || npair < 0) return G6_EARG;
+  double v[9];
+  return force_pairs(*s, npair, pi, pj, ox, oy, oz, er, nullptr, x, y, z, nullptr, nullptr,
+                     nullptr, ep, v, true);
+}
+
+int g6_site_positions(g6_handle h, const double *com, const double *quat1, double *x,
+                      double *y, double *z) {
+  System *s = get(h);
+  if (!s || !com || !quat1 || !x || !y || !z) return G6_EARG;
+  return sites_once(*s, com, quat1, x, y, z);
+}
+
+int g6_collect(g6_handle h, const double *quat1, const double *fx, const double *fy,
+               const double *fz, double *force, double *torque) {
+  System *s = get(h);
+  if (!s || !quat1 || !fx || !force || !torque) return G6_EARG;
+  return collect_once(*s, quat1, fx, fy, fz, force, torque);
+}
+
+int g6_lattice_start(int n, double box_l, uint32_t seed, double *com, double *quat1) {
+  if (n < 1 || !(box_l > 0) || !com || !quat1) return G6_EARG;
+  int m = 1;
+  while ((long long)m * m * m < n) m++;
+  const long long sites = (long long)m * m * m;
+  std::vector<int> perm((size_t)sites);
+  for (long long k = 0; k < sites; k++) perm[k] = (int)k;
+  MT mt(seed);
+  // Fisher-Yates with the MT stream; the first n entries, sorted, are occupied
+  for (long long k = sites - 1; k > 0; k--) {
+    const long long r = (long long)(mt.uniform() * (double)(k + 1));
+    const int t = perm[k];
+    perm[k] = perm[r];
+    perm[r] = t;
+  }
+  std::vector<int> occ(perm.begin(), perm.begin() + n);
+  std::sort(occ.begin(), occ.end());
+  const double a = box_l / m;
+  for (int i = 0; i < n; i++) {
+    const int k = occ[i];
+    const int ix = k / (m * m), iy = (k / m) % m, iz = k % m;
+    com[i] = (ix + 0.5) * a;
+    com[n + i] = (iy + 0.5) * a;
+    com[2 * n + i] = (iz + 0.5) * a;
+    // Shoemake (1992): uniform on S^3 from three uniforms
+    const double u1 = mt.uniform(), u2 = mt.uniform(), u3 = mt.uniform();
+    quat1[i] = sqrt(1 - u1) * sin(2 * kPI * u2);
+    quat1[n + i] = sqrt(1 - u1) * cos(2 * kPI * u2);
+    quat1[2 * n + i] = sqrt(u1) * sin(2 * kPI * u3);
+    quat1[3 * n + i] = sqrt(u1) * cos(2 * kPI * u3);
+  }
+  return 0;
+}
+
+}  // extern "C"

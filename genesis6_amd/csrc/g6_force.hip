@@ -1,0 +1,289 @@
+// Force kernels.
+//
+// k_force_rows: the MD hot path.  One wavefront per row of the pair list
+// (molecule i in cell order, both parts of its candidate list); each lane
+// evaluates one pair (i, j) with pair_sites (g6_pair.h), reduces the site
+// forces to the molecule force and lab-frame torque of i and j (the work of
+// rigid_collectforce, RigidModule.F90:275-317, done per pair), keeps i's
+// share in registers and sends j's share with FP64 atomics.  i's totals,
+// the pair energy and the virial vrmat(a,b) += d00(a)*F_i(b)
+// (TIP4PModule.F90:682-693) are reduced across the wave with shuffles.
+//
+// k_force_pairs / k_energy_pairs: the module-seam entry points force() and
+// PairPotential() on caller-given site coordinates and ratios (one thread
+// per pair, site forces scattered with atomics).
+#include "g6_pair.h"
+
+namespace g6 {
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
+__device__ inline void unpack_shift(int code, double L, double &ox, double &oy,
+                                    double &oz) {
+  ox = (double)((code & 1023) - 512) * L;
+  oy = (double)(((code >> 10) & 1023) - 512) * L;
+  oz = (double)(((code >> 20) & 1023) - 512) * L;
+}
+
+template <int M, typename R>
+__global__ __launch_bounds__(256) void k_force_rows(
+    ModelTable mt, Geom g, const double *__restrict__ com,
+    const R *__restrict__ intra, const int *__restrict__ sorted,
+    const int *__restrict__ off, const int *__restrict__ pj,
+    const int *__restrict__ scode, double *__restrict__ facc,
+    double *__restrict__ rowpart, int pcap, int filter) {
+  using T = Traits<M>;
+  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r = blockIdx.x * 4 + wid;
+  if (r >= g.n) return;
+  const int n = g.n;
+  const int i = sorted[r];
+  R ri[T::NS][3];
+#pragma unroll
+  for (int s = 0; s < T::NS; s++)
+#pragma unroll
+    for (int a = 0; a < 3; a++) ri[s][a] = intra[(s * 3 + a) * n + i];
+  const double xi = com[i], yi = com[n + i], zi = com[2 * n + i];
+  double F[3] = {0, 0, 0}, Ti[3] = {0, 0, 0}, ep = 0, v[9] = {0}, npi = 0;
+  for (int part = 0; part < 2; part++) {
+    const int kb = off[part * n + r];
+    const int ke = min(off[part * n + r + 1], pcap);
+    for (int k = kb + lane; k < ke; k += 64) {
+      const int j = pj[k];
+      double ox, oy, oz;
+      unpack_shift(scode[k], g.L, ox, oy, oz);
+      double d0 = xi - com[j], d1 = yi - com[n + j], d2 = zi - com[2 * n + j];
+      d0 = d0 - ox;
+      d1 = d1 - oy;
+      d2 = d2 - oz;
+      const double dd64 = d0 * d0 + d1 * d1 + d2 * d2;
+      if (filter && !(dd64 < g.rc2)) continue;
+      R rj[T::NS][3];
+#pragma unroll
+      for (int s = 0; s < T::NS; s++)
+#pragma unroll
+        for (int a = 0; a < 3; a++) rj[s][a] = intra[(s * 3 + a) * n + j];
+      R hh, dh;
+      cutoff_sw<R>(g, R(dd64), hh, dh);
+      R fi[T::NS][3], fj[T::NS][3];
+      const R e = pair_sites<M, R>(mt, hh, dh, R(d0), R(d1), R(d2), ri, rj, fi, fj);
+      R Fx = 0, Fy = 0, Fz = 0, tix = 0, tiy = 0, tiz = 0, tjx = 0, tjy = 0, tjz = 0;
+#pragma unroll
+      for (int s = 0; s < T::NS; s++) {
+        Fx += fi[s][0]; Fy += fi[s][1]; Fz += fi[s][2];
+        tix += ri[s][1] * fi[s][2] - ri[s][2] * fi[s][1];
+        tiy += ri[s][2] * fi[s][0] - ri[s][0] * fi[s][2];
+        tiz += ri[s][0] * fi[s][1] - ri[s][1] * fi[s][0];
+        tjx += rj[s][1] * fj[s][2] - rj[s][2] * fj[s][1];
+        tjy += rj[s][2] * fj[s][0] - rj[s][0] * fj[s][2];
+        tjz += rj[s][0] * fj[s][1] - rj[s][1] * fj[s][0];
+      }
+      F[0] += Fx; F[1] += Fy; F[2] += Fz;
+      Ti[0] += tix; Ti[1] += tiy; Ti[2] += tiz;
+      unsafeAtomicAdd(&facc[0 * n + j], -(double)Fx);
+      unsafeAtomicAdd(&facc[1 * n + j], -(double)Fy);
+      unsafeAtomicAdd(&facc[2 * n + j], -(double)Fz);
+      unsafeAtomicAdd(&facc[3 * n + j], -(double)tjx);
+      unsafeAtomicAdd(&facc[4 * n + j], -(double)tjy);
+      unsafeAtomicAdd(&facc[5 * n + j], -(double)tjz);
+      ep += (double)e;
+      npi += 1.0;
+      const double dv[3] = {d0, d1, d2}, fv[3] = {(double)Fx, (double)Fy, (double)Fz};
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) v[3 * a + b] += dv[a] * fv[b];
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    F[a] = wave_sum(F[a]);
+    Ti[a] = wave_sum(Ti[a]);
+  }
+  ep = wave_sum(ep);
+  npi = wave_sum(npi);
+#pragma unroll
+  for (int a = 0; a < 9; a++) v[a] = wave_sum(v[a]);
+  if (lane == 0) {
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      facc[(6 + a) * n + i] = F[a];
+      facc[(9 + a) * n + i] = Ti[a];
+    }
+    rowpart[r] = ep;
+#pragma unroll
+    for (int a = 0; a < 9; a++) rowpart[(1 + a) * n + r] = v[a];
+    rowpart[10 * n + r] = npi;
+  }
+}
+
+template <int M, typename R>
+static void launch_rows_t(System &s) {
+  const int nb = (s.n + 3) / 4;
+  k_force_rows<M, R><<<nb, 256, 0, s.stream>>>(
+      s.mt, s.geo, s.com, reinterpret_cast<const R *>(sizeof(R) == 4 ? (void *)s.intraf : (void *)s.intra),
+      s.sorted, s.off, s.pj, s.scode, s.facc, s.rowpart, s.pcap, s.skin > 0 ? 1 : 0);
+}
+
+void launch_force_rows(System &s) {
+  if (s.prec == G6_FP32) {
+    if (s.model == G6_TIP4P) launch_rows_t<G6_TIP4P, float>(s);
+    else if (s.model == G6_TIP5P) launch_rows_t<G6_TIP5P, float>(s);
+    else launch_rows_t<G6_ST2, float>(s);
+  } else {
+    if (s.model == G6_TIP4P) launch_rows_t<G6_TIP4P, double>(s);
+    else if (s.model == G6_TIP5P) launch_rows_t<G6_TIP5P, double>(s);
+    else launch_rows_t<G6_ST2, double>(s);
+  }
+}
+
+// ---- module seam: force() / PairPotential() on host-given arrays ---------
+template <int M, typename R>
+__global__ __launch_bounds__(256) void k_force_pairs(
+    ModelTable mt, int64_t np, const int *pi, const int *pj, const double *ox,
+    const double *oy, const double *oz, const double *er, const double *fr,
+    const double *x, const double *y, const double *z, double *fx, double *fy,
+    double *fz, double *ep, double *vir, int energy_only) {
+  using T = Traits<M>;
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double v[9] = {0};
+  if (k < np) {
+    const int ii = pi[k] * T::NS, jj = pj[k] * T::NS;
+    const int c = T::COM;
+    double d0 = x[ii + c], d1 = y[ii + c], d2 = z[ii + c];
+    d0 = d0 - x[jj + c];
+    d1 = d1 - y[jj + c];
+    d2 = d2 - z[jj + c];
+    d0 = d0 - ox[k];
+    d1 = d1 - oy[k];
+    d2 = d2 - oz[k];
+    R ri[T::NS][3], rj[T::NS][3];
+#pragma unroll
+    for (int s = 0; s < T::NS; s++) {
+      ri[s][0] = R(x[ii + s] - x[ii + c]);
+      ri[s][1] = R(y[ii + s] - y[ii + c]);
+      ri[s][2] = R(z[ii + s] - z[ii + c]);
+      rj[s][0] = R(x[jj + s] - x[jj + c]);
+      rj[s][1] = R(y[jj + s] - y[jj + c]);
+      rj[s][2] = R(z[jj + s] - z[jj + c]);
+    }
+    if (energy_only) {
+      ep[k] = (double)pair_energy<M, R>(mt, R(er[k]), R(d0), R(d1), R(d2), ri, rj);
+    } else {
+      R fi[T::NS][3], fj[T::NS][3];
+      ep[k] = (double)pair_sites<M, R>(mt, R(er[k]), R(fr[k]), R(d0), R(d1), R(d2), ri,
+                                       rj, fi, fj);
+      double Fs[3] = {0, 0, 0};
+#pragma unroll
+      for (int s = 0; s < T::NS; s++) {
+        unsafeAtomicAdd(&fx[ii + s], (double)fi[s][0]);
+        unsafeAtomicAdd(&fy[ii + s], (double)fi[s][1]);
+        unsafeAtomicAdd(&fz[ii + s], (double)fi[s][2]);
+        unsafeAtomicAdd(&fx[jj + s], -(double)fj[s][0]);
+        unsafeAtomicAdd(&fy[jj + s], -(double)fj[s][1]);
+        unsafeAtomicAdd(&fz[jj + s], -(double)fj[s][2]);
+        Fs[0] += (double)fi[s][0]; Fs[1] += (double)fi[s][1]; Fs[2] += (double)fi[s][2];
+      }
+      const double dv[3] = {d0, d1, d2};
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) v[3 * a + b] = dv[a] * Fs[b];
+    }
+  }
+  if (!energy_only) {
+    __shared__ double red[4][9];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int a = 0; a < 9; a++) {
+      const double t = wave_sum(v[a]);
+      if (lane == 0) red[wid][a] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x < 9)
+      unsafeAtomicAdd(&vir[threadIdx.x], red[0][threadIdx.x] + red[1][threadIdx.x] +
+                                             red[2][threadIdx.x] + red[3][threadIdx.x]);
+  }
+}
+
+template <int M, typename R>
+static void launch_pairs_t(System &s, int64_t np, double **d, double *vir,
+                           const int *pi, const int *pj, int eo) {
+  const int nb = (int)((np + 255) / 256);
+  k_force_pairs<M, R><<<nb, 256, 0, s.stream>>>(s.mt, np, pi, pj, d[0], d[1], d[2], d[3],
+                                                 d[4], d[5], d[6], d[7], d[8], d[9], d[10],
+                                                 d[11], vir, eo);
+}
+
+int force_pairs(System &s, int64_t np, const int32_t *pi_h, const int32_t *pj_h,
+                const double *ox, const double *oy, const double *oz,
+                const double *er, const double *fr, const double *x,
+                const double *y, const double *z, double *fx, double *fy,
+                double *fz, double *ep, double *vrmat, bool energy_only) {
+  if (np < 0) return G6_EARG;
+  for (int64_t k = 0; k < np; k++)  // indices come from the caller: check them
+    if (pi_h[k] < 0 || pi_h[k] >= s.n || pj_h[k] < 0 || pj_h[k] >= s.n) return G6_EARG;
+  const int nsites = s.n * s.ns;
+  const size_t pb = sizeof(double) * (size_t)(np > 0 ? np : 1);
+  const size_t sb = sizeof(double) * (size_t)nsites;
+  int *pi = nullptr, *pj = nullptr;
+  double *d[12];
+  double *vir = nullptr;
+  G6_CHECK(hipMalloc(&pi, sizeof(int) * (size_t)(np > 0 ? np : 1)));
+  G6_CHECK(hipMalloc(&pj, sizeof(int) * (size_t)(np > 0 ? np : 1)));
+  for (int a = 0; a < 5; a++) G6_CHECK(hipMalloc(&d[a], pb));
+  for (int a = 5; a < 11; a++) G6_CHECK(hipMalloc(&d[a], sb));
+  G6_CHECK(hipMalloc(&d[11], pb));
+  G6_CHECK(hipMalloc(&vir, sizeof(double) * 9));
+  hipStream_t st = s.stream;
+  if (np > 0) {
+    G6_CHECK(hipMemcpyAsync(pi, pi_h, sizeof(int) * np, hipMemcpyHostToDevice, st));
+    G6_CHECK(hipMemcpyAsync(pj, pj_h, sizeof(int) * np, hipMemcpyHostToDevice, st));
+    const double *src[5] = {ox, oy, oz, er, fr};
+    for (int a = 0; a < 5; a++) {
+      if (energy_only && a == 4) continue;
+      G6_CHECK(hipMemcpyAsync(d[a], src[a], pb, hipMemcpyHostToDevice, st));
+    }
+  }
+  const double *ssrc[3] = {x, y, z};
+  for (int a = 0; a < 3; a++)
+    G6_CHECK(hipMemcpyAsync(d[5 + a], ssrc[a], sb, hipMemcpyHostToDevice, st));
+  if (!energy_only) {
+    const double *fsrc[3] = {fx, fy, fz};
+    for (int a = 0; a < 3; a++)
+      G6_CHECK(hipMemcpyAsync(d[8 + a], fsrc[a], sb, hipMemcpyHostToDevice, st));
+  }
+  G6_CHECK(hipMemsetAsync(vir, 0, sizeof(double) * 9, st));
+  if (np > 0) {
+    const int eo = energy_only ? 1 : 0;
+    if (s.prec == G6_FP32) {
+      if (s.model == G6_TIP4P) launch_pairs_t<G6_TIP4P, float>(s, np, d, vir, pi, pj, eo);
+      else if (s.model == G6_TIP5P) launch_pairs_t<G6_TIP5P, float>(s, np, d, vir, pi, pj, eo);
+      else launch_pairs_t<G6_ST2, float>(s, np, d, vir, pi, pj, eo);
+    } else {
+      if (s.model == G6_TIP4P) launch_pairs_t<G6_TIP4P, double>(s, np, d, vir, pi, pj, eo);
+      else if (s.model == G6_TIP5P) launch_pairs_t<G6_TIP5P, double>(s, np, d, vir, pi, pj, eo);
+      else launch_pairs_t<G6_ST2, double>(s, np, d, vir, pi, pj, eo);
+    }
+    G6_CHECK(hipGetLastError());
+    G6_CHECK(hipMemcpyAsync(ep, d[11], sizeof(double) * np, hipMemcpyDeviceToHost, st));
+  }
+  if (!energy_only) {
+    double *fdst[3] = {fx, fy, fz};
+    for (int a = 0; a < 3; a++)
+      G6_CHECK(hipMemcpyAsync(fdst[a], d[8 + a], sb, hipMemcpyDeviceToHost, st));
+    G6_CHECK(hipMemcpyAsync(vrmat, vir, sizeof(double) * 9, hipMemcpyDeviceToHost, st));
+  }
+  G6_CHECK(hipStreamSynchronize(st));
+  G6_CHECK(hipFree(pi));
+  G6_CHECK(hipFree(pj));
+  for (int a = 0; a < 12; a++) G6_CHECK(hipFree(d[a]));
+  G6_CHECK(hipFree(vir));
+  return 0;
+}
+
+}  // namespace g6

@@ -1,0 +1,211 @@
+// Site-site interaction of one molecule pair, templated on the model and on
+// the pair arithmetic type R (float: FP32-pair mode, double: FP64 mode).
+//
+// Restates, per pair k of the reference loops:
+//   TIP4P  interaction_force_tip4p, TIP4PModule.F90:187-785 (9 Coulomb + O-O LJ,
+//          scale by S, CoM term dh*d00*ep, ep1 = ep*S)
+//   TIP5P  StdForce, StdInteraction.F90:201-576 (16 Coulomb + O-O LJ)
+//   ST2    force_st2, ST2Module.F90:187-738 (16 Coulomb scaled by S*s12 with
+//          the Stillinger-Rahman switch s12 on the COM distance, LJ by S)
+// Cutoff ratios hh = S(r), dh = -S'(r)/r on the COM distance are inputs
+// (cutoff_func, CutoffModule.F90:38-59; cutoff_sw below).
+// Inputs are the COM separation d00 (shift already removed) and the site
+// offsets from each COM; outputs are the per-site forces on i (fi) and the
+// per-site forces that the reference subtracts from j (fj), and ep1.
+#pragma once
+#include "g6_internal.h"
+
+namespace g6 {
+
+template <int M>
+struct Traits;
+template <>
+struct Traits<G6_TIP4P> {
+  static constexpr int NS = 5, COM = 4, NC = 9;
+  static constexpr bool ST2SW = false;
+  __device__ static constexpr int ci(int t) {
+    constexpr int a[9] = {1, 1, 1, 2, 3, 2, 2, 3, 3};
+    return a[t];
+  }
+  __device__ static constexpr int cj(int t) {
+    constexpr int b[9] = {1, 2, 3, 1, 1, 2, 3, 2, 3};
+    return b[t];
+  }
+};
+template <>
+struct Traits<G6_TIP5P> {
+  static constexpr int NS = 6, COM = 5, NC = 16;
+  static constexpr bool ST2SW = false;
+  __device__ static constexpr int ci(int t) { return 1 + t / 4; }
+  __device__ static constexpr int cj(int t) { return 1 + t % 4; }
+};
+template <>
+struct Traits<G6_ST2> {
+  static constexpr int NS = 6, COM = 5, NC = 16;
+  static constexpr bool ST2SW = true;
+  __device__ static constexpr int ci(int t) { return 1 + t / 4; }
+  __device__ static constexpr int cj(int t) { return 1 + t % 4; }
+};
+
+__device__ inline float g6_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
+__device__ inline float g6_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// Switching function S and dh = -S'/r between rin and rout.
+template <typename R>
+__device__ inline void cutoff_sw(const Geom &g, R dd, R &hh, R &dh) {
+  hh = R(0);
+  dh = R(0);
+  if (dd < R(g.rout * g.rout)) {
+    if (dd > R(g.rin * g.rin)) {
+      const R dr = sqrt(dd);
+      const R u = dr - R(g.rout), l = dr - R(g.rin);
+      const R uu = u * u, ll = l * l;
+      hh = R(g.c1) * uu * u * (R(10) * ll - R(5) * u * l + uu);
+      dh = -(R(g.c2) * uu * ll / dr);
+    } else {
+      hh = R(1);
+    }
+  }
+}
+
+// 1/r^2 and 1/r of one site pair
+template <typename R>
+__device__ inline void inv_r(R r2, R &inv2, R &inv1);
+template <>
+__device__ inline void inv_r<double>(double r2, double &inv2, double &inv1) {
+  inv2 = 1.0 / r2;
+  inv1 = sqrt(inv2);
+}
+template <>
+__device__ inline void inv_r<float>(float r2, float &inv2, float &inv1) {
+  inv1 = g6_rsqrt(r2);
+  inv2 = inv1 * inv1;
+}
+
+template <int M, typename R>
+__device__ inline R pair_sites(const ModelTable &mt, R hh, R dh, R d0, R d1,
+                               R d2, const R (&ri)[Traits<M>::NS][3],
+                               const R (&rj)[Traits<M>::NS][3],
+                               R (&fi)[Traits<M>::NS][3],
+                               R (&fj)[Traits<M>::NS][3]) {
+  using T = Traits<M>;
+#pragma unroll
+  for (int s = 0; s < T::NS; s++)
+#pragma unroll
+    for (int a = 0; a < 3; a++) fi[s][a] = fj[s][a] = R(0);
+  R epc = R(0);
+#pragma unroll
+  for (int t = 0; t < T::NC; t++) {
+    const int a = T::ci(t), b = T::cj(t);
+    const R dx = d0 + (ri[a][0] - rj[b][0]);
+    const R dy = d1 + (ri[a][1] - rj[b][1]);
+    const R dz = d2 + (ri[a][2] - rj[b][2]);
+    R inv2, inv1;
+    inv_r<R>(dx * dx + dy * dy + dz * dz, inv2, inv1);
+    const R e0 = R(mt.qq[t]) * inv1;
+    const R f0 = e0 * inv2;
+    epc += e0;
+    fi[a][0] += dx * f0; fi[a][1] += dy * f0; fi[a][2] += dz * f0;
+    fj[b][0] += dx * f0; fj[b][1] += dy * f0; fj[b][2] += dz * f0;
+  }
+  R eplj;
+  {
+    const R dx = d0 + (ri[0][0] - rj[0][0]);
+    const R dy = d1 + (ri[0][1] - rj[0][1]);
+    const R dz = d2 + (ri[0][2] - rj[0][2]);
+    R inv2, inv1;
+    inv_r<R>(dx * dx + dy * dy + dz * dz, inv2, inv1);
+    const R d6 = inv2 * inv2 * inv2, d12 = d6 * d6;
+    eplj = R(mt.lj_a) * d12 + R(mt.lj_b) * d6;
+    const R f0 = inv2 * (R(mt.lj_a * 12.0) * d12 + R(mt.lj_b * 6.0) * d6);
+    fi[0][0] += dx * f0; fi[0][1] += dy * f0; fi[0][2] += dz * f0;
+    fj[0][0] += dx * f0; fj[0][1] += dy * f0; fj[0][2] += dz * f0;
+  }
+  R ep1;
+  if constexpr (T::ST2SW) {
+    const R dd = d0 * d0 + d1 * d1 + d2 * d2;
+    R s12 = R(1), ds12 = R(0);
+    const R rl = R(mt.rl), ru = R(mt.ru);
+    if (dd < ru * ru) {
+      if (dd < rl * rl) {
+        s12 = R(0);
+      } else {
+        const R r12 = sqrt(dd);
+        const R w3 = (ru - rl) * (ru - rl) * (ru - rl);
+        s12 = (r12 - rl) * (r12 - rl) * (R(3) * ru - rl - R(2) * r12) / w3;
+        ds12 = -(R(6) * (r12 - rl) * (ru - r12) / w3) / r12;
+      }
+    }
+    const R hhc = hh * s12, dhc = dh * s12 + hh * ds12;
+#pragma unroll
+    for (int s = 0; s < T::NS; s++) {
+      const R sc = (s == 0 || s == T::COM) ? hh : hhc;
+#pragma unroll
+      for (int a = 0; a < 3; a++) { fi[s][a] *= sc; fj[s][a] *= sc; }
+    }
+    const R cc = dh * eplj + dhc * epc;
+    fi[T::COM][0] += cc * d0; fi[T::COM][1] += cc * d1; fi[T::COM][2] += cc * d2;
+    fj[T::COM][0] += cc * d0; fj[T::COM][1] += cc * d1; fj[T::COM][2] += cc * d2;
+    ep1 = epc * hhc + eplj * hh;
+  } else {
+#pragma unroll
+    for (int s = 0; s < T::NS; s++)
+#pragma unroll
+      for (int a = 0; a < 3; a++) { fi[s][a] *= hh; fj[s][a] *= hh; }
+    const R ep = epc + eplj;
+    const R cc = dh * ep;
+    fi[T::COM][0] += cc * d0; fi[T::COM][1] += cc * d1; fi[T::COM][2] += cc * d2;
+    fj[T::COM][0] += cc * d0; fj[T::COM][1] += cc * d1; fj[T::COM][2] += cc * d2;
+    ep1 = ep * hh;
+  }
+  return ep1;
+}
+
+// Energy only (PairPotential path): sum of site terms times S (and s12 for
+// the ST2 Coulomb part, as force_st2's ep1).
+template <int M, typename R>
+__device__ inline R pair_energy(const ModelTable &mt, R hh, R d0, R d1,
+                                R d2, const R (&ri)[Traits<M>::NS][3],
+                                const R (&rj)[Traits<M>::NS][3]) {
+  using T = Traits<M>;
+  R epc = R(0);
+#pragma unroll
+  for (int t = 0; t < T::NC; t++) {
+    const int a = T::ci(t), b = T::cj(t);
+    const R dx = d0 + (ri[a][0] - rj[b][0]);
+    const R dy = d1 + (ri[a][1] - rj[b][1]);
+    const R dz = d2 + (ri[a][2] - rj[b][2]);
+    R inv2, inv1;
+    inv_r<R>(dx * dx + dy * dy + dz * dz, inv2, inv1);
+    epc += R(mt.qq[t]) * inv1;
+  }
+  R eplj;
+  {
+    const R dx = d0 + (ri[0][0] - rj[0][0]);
+    const R dy = d1 + (ri[0][1] - rj[0][1]);
+    const R dz = d2 + (ri[0][2] - rj[0][2]);
+    R inv2, inv1;
+    inv_r<R>(dx * dx + dy * dy + dz * dz, inv2, inv1);
+    const R d6 = inv2 * inv2 * inv2;
+    eplj = R(mt.lj_a) * d6 * d6 + R(mt.lj_b) * d6;
+  }
+  if constexpr (T::ST2SW) {
+    const R dd = d0 * d0 + d1 * d1 + d2 * d2;
+    R s12 = R(1);
+    const R rl = R(mt.rl), ru = R(mt.ru);
+    if (dd < ru * ru) {
+      if (dd < rl * rl) {
+        s12 = R(0);
+      } else {
+        const R r12 = sqrt(dd);
+        s12 = (r12 - rl) * (r12 - rl) * (R(3) * ru - rl - R(2) * r12) /
+              ((ru - rl) * (ru - rl) * (ru - rl));
+      }
+    }
+    return epc * hh * s12 + eplj * hh;
+  } else {
+    return (epc + eplj) * hh;
+  }
+}
+
+}  // namespace g6

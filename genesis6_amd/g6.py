@@ -1,0 +1,216 @@
+"""ctypes binding of the g6gpu C ABI (include/g6gpu.h).
+
+Mirrors the reference's module entry points on the Python side: the method
+names follow the reference routines they stand for (force, pair_energy,
+pairlist, collect, site_positions, step).  There is no CPU fallback: if the
+HIP library is missing or no device is present, every call raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "lib", "libg6gpu.so")
+TIP4P, TIP5P, ST2 = 0, 1, 2
+FP64, FP32 = 0, 1
+THERMO_LEN = 16
+ERRORS = {-1: "HIP runtime error", -2: "invalid argument",
+          -3: "pair capacity exceeded", -4: "no HIP device"}
+
+_lib = None
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_i64p = C.POINTER(C.c_int64)
+_H = C.c_int64
+
+
+class G6Error(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise G6Error("libg6gpu.so is not built: run python -m genesis6_amd.build")
+        L = C.CDLL(LIB)
+        L.g6_init.argtypes = [C.POINTER(_H), C.c_int, C.c_int, C.c_double, C.c_double,
+                              C.c_double, C.c_double, C.c_double, C.c_int, C.c_double]
+        L.g6_finalize.argtypes = [_H]
+        L.g6_upload_state.argtypes = [_H, _dp, _dp, _dp, _dp]
+        L.g6_download_state.argtypes = [_H, _dp, _dp, _dp, _dp]
+        L.g6_set_nose.argtypes = [_H, C.c_int, _dp]
+        L.g6_get_nose.argtypes = [_H, _dp]
+        L.g6_vreset.argtypes = [_H, C.c_double, C.c_uint32]
+        L.g6_step.argtypes = [_H, C.c_int]
+        L.g6_step_timed.argtypes = [_H, C.c_int, _dp]
+        L.g6_get_thermo.argtypes = [_H, _dp]
+        L.g6_eval_forces.argtypes = [_H, _dp, _dp, _dp]
+        L.g6_pairlist.argtypes = [_H, _dp, C.c_int64, _i64p, _i64p, _ip, _ip, _dp, _dp, _dp]
+        L.g6_cells.argtypes = [_H, _dp, _ip, _ip]
+        L.g6_cutoff.argtypes = [_H, C.c_int64, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.g6_force.argtypes = [_H, C.c_int64, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                               _dp, _dp, _dp, _dp, _dp, _dp]
+        L.g6_pair_energy.argtypes = [_H, C.c_int64, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp,
+                                     _dp, _dp]
+        L.g6_site_positions.argtypes = [_H, _dp, _dp, _dp, _dp, _dp]
+        L.g6_collect.argtypes = [_H, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.g6_lattice_start.argtypes = [C.c_int, C.c_double, C.c_uint32, _dp, _dp]
+        L.g6_model_nsite.argtypes = [C.c_int]
+        _lib = L
+    return _lib
+
+
+def check(rc, what):
+    if rc != 0:
+        raise G6Error("%s failed: %d (%s)" % (what, rc, ERRORS.get(rc, "reference error code")))
+
+
+def _c(a, dt=np.float64):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def lattice_start(n, L, seed):
+    """Synthetic start: returns com (n,3) and quat1 (n,4)."""
+    com = np.zeros(3 * n); q = np.zeros(4 * n)
+    check(lib().g6_lattice_start(n, L, seed, com, q), "g6_lattice_start")
+    return com.reshape(3, n).T.copy(), q.reshape(4, n).T.copy()
+
+
+class System:
+    """One rigid-water system resident on the GPU (a g6_handle)."""
+
+    def __init__(self, model, n, L, rc, taper, dt, voxa=None, prec=FP64, skin=0.0):
+        self.model, self.n, self.L, self.rc, self.taper, self.dt = model, n, L, rc, taper, dt
+        self.nsite = lib().g6_model_nsite(model)
+        h = _H(0)
+        check(lib().g6_init(C.byref(h), model, n, L, rc, taper, voxa if voxa else rc, dt,
+                            prec, skin), "g6_init")
+        self.h = h.value
+
+    def close(self):
+        if self.h:
+            lib().g6_finalize(self.h)
+            self.h = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- state ----------------------------------------------------------
+    def upload(self, com, quat1, dx=None, w=None, quat=None):
+        """com (n,3); quat1 (n,4) order-1 quaternions (higher orders zero,
+        as Rigid_ReadNX4A leaves them) unless the full quat (5,4,n) is given;
+        dx, w: Nordsieck (5,3,n) or None for zero."""
+        n = self.n
+        q = np.zeros((5, 4, n)) if quat is None else _c(quat).reshape(5, 4, n)
+        if quat is None:
+            q[0] = _c(quat1).T
+        dxa = np.zeros((5, 3, n)) if dx is None else _c(dx).reshape(5, 3, n)
+        wa = np.zeros((5, 3, n)) if w is None else _c(w).reshape(5, 3, n)
+        check(lib().g6_upload_state(self.h, _c(com.T).ravel(), _c(q).ravel(),
+                                    _c(dxa).ravel(), _c(wa).ravel()), "g6_upload_state")
+
+    def download(self):
+        n = self.n
+        com = np.zeros(3 * n); q = np.zeros(20 * n); dx = np.zeros(15 * n); w = np.zeros(15 * n)
+        check(lib().g6_download_state(self.h, com, q, dx, w), "g6_download_state")
+        return (com.reshape(3, n).T.copy(), q.reshape(5, 4, n), dx.reshape(5, 3, n),
+                w.reshape(5, 3, n))
+
+    def set_nose(self, temp, q_nos2, dump=0.0, zeta=(0, 0, 0, 0, 0), zetasum=0.0,
+                 active=True):
+        """@NOS2 semantics: q_nos2 is the tag's mass; internally q*dt."""
+        z = np.array([temp, q_nos2 * self.dt, dump] + list(zeta) + [zetasum], float)
+        check(lib().g6_set_nose(self.h, 1 if active else 0, z), "g6_set_nose")
+
+    def nose(self):
+        z = np.zeros(9)
+        check(lib().g6_get_nose(self.h, z), "g6_get_nose")
+        return z
+
+    def vreset(self, temp, seed):
+        check(lib().g6_vreset(self.h, temp, seed), "g6_vreset")
+
+    def step(self, nsteps=1):
+        check(lib().g6_step(self.h, nsteps), "g6_step")
+
+    def step_timed(self, nsteps):
+        t = np.zeros(4)
+        check(lib().g6_step_timed(self.h, nsteps, t), "g6_step_timed")
+        return t
+
+    def eval_forces(self):
+        """Molecule forces (n,3), body torques (n,3) and the thermo record of
+        the current state through the MD kernels (state unchanged)."""
+        n = self.n
+        F = np.zeros(3 * n); T = np.zeros(3 * n); th = np.zeros(THERMO_LEN)
+        check(lib().g6_eval_forces(self.h, F, T, th), "g6_eval_forces")
+        return F.reshape(3, n).T.copy(), T.reshape(3, n).T.copy(), th
+
+    def thermo(self):
+        t = np.zeros(THERMO_LEN)
+        check(lib().g6_get_thermo(self.h, t), "g6_get_thermo")
+        return t
+
+    # ---- module-seam entry points ----------------------------------------
+    def pairlist(self, com, cap=None):
+        n = self.n
+        cap = cap or n * 800
+        pi = np.zeros(cap, np.int32); pj = np.zeros(cap, np.int32)
+        ox = np.zeros(cap); oy = np.zeros(cap); oz = np.zeros(cap)
+        np0 = C.c_int64(0); np_ = C.c_int64(0)
+        check(lib().g6_pairlist(self.h, _c(com.T).ravel(), cap, C.byref(np0), C.byref(np_),
+                                pi, pj, ox, oy, oz), "g6_pairlist")
+        k = np_.value
+        return np0.value, pi[:k].copy(), pj[:k].copy(), np.stack([ox[:k], oy[:k], oz[:k]], 1)
+
+    def cells(self, com):
+        cell = np.zeros(self.n, np.int32); nd = np.zeros(1, np.int32)
+        check(lib().g6_cells(self.h, _c(com.T).ravel(), cell, nd), "g6_cells")
+        return cell, int(nd[0])
+
+    def cutoff(self, pi, pj, com, shifts):
+        k = len(pi)
+        er = np.zeros(max(k, 1)); fr = np.zeros(max(k, 1))
+        sh = _c(shifts).reshape(-1, 3)
+        check(lib().g6_cutoff(self.h, k, _c(pi, np.int32), _c(pj, np.int32), _c(com.T).ravel(),
+                              _c(sh[:, 0]), _c(sh[:, 1]), _c(sh[:, 2]), er, fr), "g6_cutoff")
+        return er[:k], fr[:k]
+
+    def site_positions(self, com, quat1):
+        m = self.n * self.nsite
+        x = np.zeros(m); y = np.zeros(m); z = np.zeros(m)
+        check(lib().g6_site_positions(self.h, _c(com.T).ravel(), _c(quat1.T).ravel(), x, y, z),
+              "g6_site_positions")
+        return np.stack([x, y, z], 1)
+
+    def force(self, pi, pj, shifts, er, fr, sites):
+        k = len(pi)
+        m = sites.shape[0]
+        f = [np.zeros(m) for _ in range(3)]
+        ep = np.zeros(max(k, 1)); vr = np.zeros(9)
+        sh = _c(shifts).reshape(-1, 3)
+        check(lib().g6_force(self.h, k, _c(pi, np.int32), _c(pj, np.int32), _c(sh[:, 0]),
+                             _c(sh[:, 1]), _c(sh[:, 2]), _c(er), _c(fr), _c(sites[:, 0]),
+                             _c(sites[:, 1]), _c(sites[:, 2]), *f, ep, vr), "g6_force")
+        return np.stack(f, 1), ep[:k], vr.reshape(3, 3)
+
+    def pair_energy(self, pi, pj, shifts, er, sites):
+        k = len(pi)
+        ep = np.zeros(max(k, 1))
+        sh = _c(shifts).reshape(-1, 3)
+        check(lib().g6_pair_energy(self.h, k, _c(pi, np.int32), _c(pj, np.int32), _c(sh[:, 0]),
+                                   _c(sh[:, 1]), _c(sh[:, 2]), _c(er), _c(sites[:, 0]),
+                                   _c(sites[:, 1]), _c(sites[:, 2]), ep), "g6_pair_energy")
+        return ep[:k]
+
+    def collect(self, quat1, forces):
+        n = self.n
+        F = np.zeros(3 * n); T = np.zeros(3 * n)
+        check(lib().g6_collect(self.h, _c(quat1.T).ravel(), _c(forces[:, 0]), _c(forces[:, 1]),
+                               _c(forces[:, 2]), F, T), "g6_collect")
+        return F.reshape(3, n).T.copy(), T.reshape(3, n).T.copy()

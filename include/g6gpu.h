@@ -1,0 +1,147 @@
+/*
+ * g6gpu: C ABI of the MI355X rigid-water interaction step.
+ *
+ * The drop-in seam is the reference's Fortran module interface for the MD
+ * step (Mixture.F90:1789-2286) and for the MC trial energies (mc2.F90:544-862).
+ * Each entry point below names the reference routine it replaces.  Calling
+ * conventions (so that ISO_C_BINDING interfaces bind them directly):
+ *   - every function returns int32: 0 = error_none; > 0 = the reference's
+ *     error codes (Error.F90:2-7); < 0 = GPU/HIP or capacity errors (G6_E*);
+ *   - scalars by value, arrays as plain host pointers to contiguous memory;
+ *     no struct, no Fortran derived type crosses the boundary;
+ *   - molecule/pair indices are 0-based; Fortran callers add 1;
+ *   - per-molecule arrays are component-major ("SoA"): com[c*n + i] is
+ *     component c of molecule i, i.e. Fortran com(n,3); Nordsieck arrays are
+ *     [order][component][molecule], i.e. Fortran dx(n,3,5), quat(n,4,5);
+ *   - all calls are synchronous: results are in the caller's arrays on return.
+ * Units are the reference's internal units (A, ps, amu; energy 10 J/mol).
+ */
+#ifndef G6GPU_H
+#define G6GPU_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int64_t g6_handle;
+
+/* models: the reference ids "TIP4P   ", "TIP5P   ", "ST2_____"
+ * (RigidSetup.F90:33-62) */
+enum { G6_TIP4P = 0, G6_TIP5P = 1, G6_ST2 = 2 };
+/* pair arithmetic precision; accumulation of forces, torques, energy and
+ * virial is FP64 in both */
+enum { G6_FP64 = 0, G6_FP32 = 1 };
+/* error codes (< 0) */
+enum {
+  G6_EHIP = -1,       /* a HIP runtime call failed                        */
+  G6_EARG = -2,       /* invalid argument (size, model, handle)           */
+  G6_ECAPACITY = -3,  /* pair capacity exceeded: nothing truncated        */
+  G6_ENODEVICE = -4   /* no HIP device                                    */
+};
+/* thermo record of the last step, written by g6_get_thermo:
+ * [0] ep (sum of pair energies), [1] ekt, [2] ekr, [3] temperature (K),
+ * [4..12] virial vrmat(3,3) row-major, [13] conserved H (J/mol,
+ * Mixture.F90:2431-2468), [14] interacting pairs (dd < rc^2),
+ * [15] pairs in the current list (differs from [14] only when skin > 0) */
+#define G6_THERMO_LEN 16
+
+/* ---- system lifecycle (Mixture.F90:1141-1347 setup) -------------------- */
+/* model: G6_*; n molecules in a cubic box of edge box_l (@BXLA); cutoff rc
+ * with switch width taper (@RCOA rc taper); voxa = requested cell edge
+ * (@VOXA); dt in ps (@DTPS); prec: G6_FP64/G6_FP32.  skin >= 0: pair-list
+ * margin in A; 0 rebuilds the reference list every step (the reference's
+ * @BOOK interval 1); > 0 rebuilds it when any molecule moved more than
+ * skin/2 and filters dd < rc^2 in FP64 every step (same interacting set). */
+int g6_init(g6_handle *h, int model, int n, double box_l, double rc,
+            double taper, double voxa, double dt, int prec, double skin);
+int g6_finalize(g6_handle h);
+
+/* Rigid state (RigidModule.F90:14-33 sRigid1, internal units).  com[3n];
+ * quat[20n] = quat(n,4,5); dx[15n] = (dx,dy,dz)(n,3,5) Nordsieck, dx(1)=v*dt;
+ * w[15n] = (wx,wy,wz)(n,3,5), body-frame omega in rad/ps. */
+int g6_upload_state(g6_handle h, const double *com, const double *quat,
+                    const double *dx, const double *w);
+int g6_download_state(g6_handle h, double *com, double *quat, double *dx,
+                      double *w);
+
+/* Nose thermostat (NoseModule.F90:14-21).  nose[9] = temp (K), q (internal,
+ * i.e. the @NOS2 value times dt, Mixture.F90:1206), dump, zeta0..zeta4,
+ * zetasum.  active = 0 switches it off (NVE). */
+int g6_set_nose(g6_handle h, int active, const double *nose);
+int g6_get_nose(g6_handle h, double *nose);
+
+/* Maxwell-Boltzmann velocities, Rigid_VReset2 (RigidModule.F90:1016-1053)
+ * with the reference's MT19937 stream (@VSET temp seed). */
+int g6_vreset(g6_handle h, double temp, uint32_t seed);
+
+/* nsteps MD steps in the reference order (Mixture.F90:1789-2286): predict,
+ * Nose_Predict, site placement, grid + pair list, cutoff ratios, forces,
+ * collect, correct, Ek, Nose_Correct.  Replaces that loop body. */
+int g6_step(g6_handle h, int nsteps);
+int g6_get_thermo(g6_handle h, double *thermo);
+/* Forces of the current state through the MD kernels, without moving it:
+ * site placement, pair list, forces and rigid_collectforce
+ * (RigidModule.F90:275-317).  force[3n] lab frame, torque[3n] body frame;
+ * thermo as g6_get_thermo with ep, virial and pair counts of this state. */
+int g6_eval_forces(g6_handle h, double *force, double *torque, double *thermo);
+
+/* ---- the reference's module entry points, one call each --------------- */
+/* Grid_Update + Address_Assign + Grid_NeighborList + Interaction_Compress2
+ * (GridModule.F90:95-125, 503-752; InteractionModule.F90:398-605), on the
+ * COMs com[3n] of the handle's box.  Writes the compressed list in the
+ * reference's order; *npair0 = candidates.  Returns G6_ECAPACITY if more than
+ * cap pairs pass (nothing is written past cap). */
+int g6_pairlist(g6_handle h, const double *com, int64_t cap, int64_t *npair0,
+                int64_t *npair, int32_t *pair_i, int32_t *pair_j, double *ox,
+                double *oy, double *oz);
+/* Address_Assign cell of each molecule (0-based mAddr-1) and the grid. */
+int g6_cells(g6_handle h, const double *com, int32_t *cell, int32_t *ndiv);
+/* CutOff_Dimmer2 (CutoffModule.F90:220-278) for a given pair list. */
+int g6_cutoff(g6_handle h, int64_t npair, const int32_t *pair_i,
+              const int32_t *pair_j, const double *com, const double *ox,
+              const double *oy, const double *oz, double *eratio,
+              double *fratio);
+/* force(iv,mi,mj,site,ep,vrmat,custom,si,sj) (Interaction2.F90:19-68):
+ * site coordinates x,y,z[n*nsite] (site index i*nsite+s, CoM last);
+ * site forces are ADDED to fx,fy,fz; ep[npair] and vrmat[9] (row-major
+ * vrmat(a,b)) are overwritten.  Uses the handle's model and precision. */
+int g6_force(g6_handle h, int64_t npair, const int32_t *pair_i,
+             const int32_t *pair_j, const double *ox, const double *oy,
+             const double *oz, const double *eratio, const double *fratio,
+             const double *x, const double *y, const double *z, double *fx,
+             double *fy, double *fz, double *ep, double *vrmat);
+/* PairPotential / StdPairPotential energy-only path
+ * (Interaction2.F90:71-121, StdInteraction.F90:633-762). */
+int g6_pair_energy(g6_handle h, int64_t npair, const int32_t *pair_i,
+                   const int32_t *pair_j, const double *ox, const double *oy,
+                   const double *oz, const double *eratio, const double *x,
+                   const double *y, const double *z, double *ep);
+/* rigid_setsiteposition (RigidModule.F90:146-153): site coordinates from
+ * com[3n] and quat1[4n] (quat(n,4) of order 1). */
+int g6_site_positions(g6_handle h, const double *com, const double *quat1,
+                      double *x, double *y, double *z);
+/* rigid_collectforce (RigidModule.F90:275-317): molecule force[3n] and
+ * body-frame torque[3n] from site forces and quat1[4n]. */
+int g6_collect(g6_handle h, const double *quat1, const double *fx,
+               const double *fy, const double *fz, double *force,
+               double *torque);
+
+/* ---- synthetic start (no reference counterpart: the reference reads
+ * @NX4A files) --------------------------------------------------------- */
+/* Simple-cubic lattice, m = ceil(n^(1/3)) per edge, occupied sites chosen by
+ * an MT19937-seeded shuffle, Shoemake-uniform quaternions.  com[3n], quat1[4n].*/
+int g6_lattice_start(int n, double box_l, uint32_t seed, double *com,
+                     double *quat1);
+
+/* ---- info -------------------------------------------------------------- */
+/* nsteps MD steps launched one kernel at a time (no graph) with HIP events
+ * on the library's stream around each phase; t[0..3] = mean ms per step of
+ * the force kernel, the pair-list build, the per-molecule kernels and the
+ * whole step.  Same results as g6_step. */
+int g6_step_timed(g6_handle h, int nsteps, double *t);
+int g6_model_nsite(int model);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,185 @@
+"""HIP path against the oracle and the reference's golden fixtures (GPU).
+
+Bars: integer/index work bit-exact (cells, pair list order and shifts); FP64
+mode <= 1e-12 relative RMS force error; FP32-pair mode <= 1e-5 relative RMS
+force error, both against the FP64 oracle.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from genesis6_amd import g6
+from oracle import pyoracle as O
+from tests.golden.make_golden import lattice_start
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+CASES = ["tip4p_216", "tip5p_216", "st2_216", "tip4p_125_ndiv3"]
+
+
+def relrms(a, b):
+    return float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
+
+
+def system_for(g, prec=g6.FP64, skin=0.0, dt=0.002):
+    print("case model=%d n=%d L=%g prec=%d" % (int(g["model"]), len(g["com"]), float(g["L"]), prec),
+          flush=True)
+    return g6.System(int(g["model"]), len(g["com"]), float(g["L"]), float(g["rc"]),
+                     float(g["taper"]), dt, voxa=float(g["req"]), prec=prec, skin=skin)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_cells_and_pairlist_bit_exact(golden, name):
+    g = golden(name)
+    s = system_for(g)
+    cell, nd = s.cells(g["com"])
+    assert nd == int(g["ndiv"][0])
+    np.testing.assert_array_equal(cell, g["cell"])
+    np0, pi, pj, sh = s.pairlist(g["com"])
+    assert np0 == int(g["npair0"])
+    np.testing.assert_array_equal(pi, g["pi"])
+    np.testing.assert_array_equal(pj, g["pj"])
+    assert np.array_equal(sh, g["shift"])
+
+
+def test_pairlist_digests_and_full_size():
+    """Reference digests at 1024/2048 molecules; oracle at 4096 (C2 size)."""
+    import hashlib
+    sha = lambda a: hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+    dig = json.load(open(os.path.join(GOLD, "pairlist_digests.json")))
+    for key, d in dig.items():
+        com, _ = lattice_start(d["n"], d["L"], d["seed"])
+        s = g6.System(g6.TIP4P, d["n"], d["L"], d["rc"], 0.5, 0.002)
+        np0, pi, pj, sh = s.pairlist(com)
+        assert (np0, len(pi)) == (d["npair0"], d["npair"]), key
+        assert sha(pi) == d["sha_pi"] and sha(pj) == d["sha_pj"] and sha(sh) == d["sha_shift"]
+    n, L, rc = 4096, 49.669, 9.0
+    com, _ = lattice_start(n, L, 20261202 + 2)
+    com = com + np.random.default_rng(5).normal(0, 0.3, com.shape) + 3.0 * L  # unwrapped
+    s = g6.System(g6.TIP4P, n, L, rc, 0.5, 0.002)
+    got = s.pairlist(com)
+    ref = O.pairlist(com, L, rc, rc)
+    assert got[0] == ref[0]
+    np.testing.assert_array_equal(got[1], ref[1])
+    np.testing.assert_array_equal(got[2], ref[2])
+    assert np.array_equal(got[3], ref[3])
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_sites_cutoff_collect(golden, name):
+    g = golden(name)
+    s = system_for(g)
+    sites = s.site_positions(g["com"], g["quat"])
+    assert np.abs(sites - g["sites"]).max() <= 1e-14 * np.abs(g["sites"]).max()
+    er, fr = s.cutoff(g["pi"], g["pj"], g["com"], g["shift"])
+    assert np.abs(er - g["eratio"]).max() <= 1e-14
+    assert np.abs(fr - g["fratio"]).max() <= 1e-14 * max(1e-300, np.abs(g["fratio"]).max())
+    F, T = s.collect(g["quat"], g["force_std"])
+    assert relrms(F, g["mol_force"]) <= 1e-13
+    assert relrms(T, g["mol_torque"]) <= 1e-13
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("prec,tol", [(g6.FP64, 1e-12), (g6.FP32, 1e-5)])
+def test_force_seam(golden, name, prec, tol):
+    """force() drop-in: site forces, pair energies, virial vs the reference."""
+    g = golden(name)
+    s = system_for(g, prec)
+    ref = g["force_fast"] if "force_fast" in g else g["force_std"]
+    ref_ep = g["ep_fast"] if "ep_fast" in g else g["ep_std"]
+    ref_vr = g["vir_fast"] if "vir_fast" in g else g["vir_std"]
+    F, ep, vr = s.force(g["pi"], g["pj"], g["shift"], g["eratio"], g["fratio"], g["sites"])
+    assert relrms(F, ref) <= tol, relrms(F, ref)
+    assert relrms(ep, ref_ep) <= tol
+    assert relrms(vr, ref_vr) <= tol
+    e = s.pair_energy(g["pi"], g["pj"], g["shift"], g["eratio"], g["sites"])
+    assert relrms(e, ref_ep) <= tol
+
+
+@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("prec,tol", [(g6.FP64, 1e-12), (g6.FP32, 1e-5)])
+def test_md_kernels_forces(golden, name, prec, tol):
+    """The hot path (row kernel + fused collect) vs the reference's
+    rigid_collectforce output on the same state."""
+    g = golden(name)
+    s = system_for(g, prec)
+    s.upload(g["com"], g["quat"])
+    F, T, th = s.eval_forces()
+    assert relrms(F, g["mol_force"]) <= tol, relrms(F, g["mol_force"])
+    assert relrms(T, g["mol_torque"]) <= tol, relrms(T, g["mol_torque"])
+    ep_ref = (g["ep_fast"] if "ep_fast" in g else g["ep_std"]).sum()
+    assert abs(th[0] - ep_ref) <= tol * np.abs(g["ep_std"]).sum()
+    assert int(th[14]) == len(g["pi"])
+
+
+@pytest.mark.parametrize("skin", [0.0, 1.0])
+def test_md_kernels_forces_c2_size(skin):
+    """4096 TIP4P at C2 density: FP64 and FP32 hot path vs the FP64 oracle."""
+    n, L, rc, taper = 4096, 49.669, 9.0, 0.5
+    com, q = lattice_start(n, L, 20261202 + 2)
+    np0, pi, pj, sh = O.pairlist(com, L, rc, rc)
+    er, fr = O.cutoff(pi, pj, com, sh, rc - taper, rc)
+    sites, tm, intra = O.setsites(O.TIP4P, com, q.T.copy())
+    Fs, ep, vr = O.force(O.TIP4P, "fast", pi, pj, sh, er, fr, sites)
+    Fm, Tm = O.collect(O.TIP4P, Fs, tm, intra)
+    for prec, tol in [(g6.FP64, 1e-12), (g6.FP32, 1e-5)]:
+        s = g6.System(g6.TIP4P, n, L, rc, taper, 0.002, prec=prec, skin=skin)
+        s.upload(com, q)
+        F, T, th = s.eval_forces()
+        assert relrms(F, Fm) <= tol and relrms(T, Tm) <= tol, (prec, relrms(F, Fm), relrms(T, Tm))
+        assert int(th[14]) == len(pi)
+        assert abs(th[0] - ep.sum()) <= tol * np.abs(ep).sum()
+        assert relrms(th[4:13].reshape(3, 3), vr) <= tol
+
+
+@pytest.mark.parametrize("name", ["md_tip4p_216_nve", "md_tip4p_216_nvt"])
+def test_md_trajectory_vs_reference_driver(golden, name):
+    """20 FP64 steps against the reference Mixture run (the same MB start
+    from MT19937): temperature and energy per step, final Gear state."""
+    g = golden(name)
+    n = len(g["com"])
+    s = g6.System(g6.TIP4P, n, float(g["L"]), float(g["rc"]), float(g["taper"]), float(g["dt"]))
+    s.upload(g["com"], g["quat"])
+    if g["nose"][1] != 0:
+        s.set_nose(*g["nose"])
+    s.vreset(300.0, int(g["seed"]))
+    for k in range(int(g["nsteps"])):
+        s.step(1)
+        th = s.thermo()
+        row = g["logd"][k]
+        assert abs(th[3] - row[1]) <= 1e-10 * row[1], (k, th[3], row[1])
+        assert abs(th[0] * 1e-2 - row[4]) <= 1e-9 * abs(row[4]) + 1e-9, (k, th[0] * 1e-2, row[4])
+    com, quat, dx, w = s.download()
+    assert np.abs(com.T - g["com_final"]).max() < 1e-9
+    assert np.abs(quat - g["quat_final"].transpose(1, 0, 2)).max() < 1e-9
+    assert np.abs(dx - g["dxyz_final"].transpose(1, 0, 2)).max() < 1e-12
+
+
+def test_graph_replay_equals_eager():
+    """hipGraph-captured steps give the same trajectory as eager launches."""
+    n, L = 512, 24.83
+    com, q = lattice_start(n, L, 11)
+    a = g6.System(g6.TIP4P, n, L, 9.0, 0.5, 0.002)
+    b = g6.System(g6.TIP4P, n, L, 9.0, 0.5, 0.002)
+    for s in (a, b):
+        s.upload(com, q)
+        s.vreset(300.0, 7)
+    a.step(40)            # two graph replays of 20 steps
+    for _ in range(40):
+        b.step(1)         # eager launches
+    ca, qa, _, _ = a.download()
+    cb, qb, _, _ = b.download()
+    # atomics make the FP64 sums order-dependent: agreement to round-off growth
+    assert np.abs(ca - cb).max() < 1e-8 and np.abs(qa - qb).max() < 1e-8
+    ta, tb = a.thermo(), b.thermo()
+    assert abs(ta[0] - tb[0]) < 1e-8 * abs(tb[0])
+
+
+def test_capacity_error_is_reported():
+    n, L = 216, 18.626
+    com, q = lattice_start(n, L, 3)
+    s = g6.System(g6.TIP4P, n, L, 8.5, 0.5, 0.002)
+    with pytest.raises(g6.G6Error):
+        s.pairlist(com, cap=10)
