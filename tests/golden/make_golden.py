@@ -17,29 +17,13 @@ import tempfile
 
 import numpy as np
 
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+from configs import lattice_start  # noqa: E402
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 REF = os.path.join(ROOT, "oracle", "_ref")
 IDS = {0: "TIP4P   ", 1: "TIP5P   ", 2: "ST2_____"}
-
-
-def lattice_start(n, L, seed):
-    """Simple-cubic lattice, m = ceil(n^(1/3)) per edge, sites chosen by a
-    seeded permutation, small seeded jitter, Shoemake-uniform quaternions."""
-    rng = np.random.default_rng(seed)
-    m = int(np.ceil(round(n ** (1.0 / 3.0), 9)))
-    while m ** 3 < n:
-        m += 1
-    g = (np.arange(m) + 0.5) * L / m
-    pts = np.array(np.meshgrid(g, g, g, indexing="ij")).reshape(3, -1).T
-    idx = np.sort(rng.permutation(len(pts))[:n])
-    com = pts[idx] + rng.normal(0.0, 0.05, (n, 3))
-    u = rng.random((n, 3))
-    q = np.stack([np.sqrt(1 - u[:, 0]) * np.sin(2 * np.pi * u[:, 1]),
-                  np.sqrt(1 - u[:, 0]) * np.cos(2 * np.pi * u[:, 1]),
-                  np.sqrt(u[:, 0]) * np.sin(2 * np.pi * u[:, 2]),
-                  np.sqrt(u[:, 0]) * np.cos(2 * np.pi * u[:, 2])], 1)
-    return com, q
 
 
 def run(cmd, cwd, stdin_text):

@@ -12,7 +12,7 @@ import pytest
 
 from genesis6_amd import g6
 from oracle import pyoracle as O
-from tests.golden.make_golden import lattice_start
+from tests.golden.configs import lattice_start
 
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(__file__), "golden")

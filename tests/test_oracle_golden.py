@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from oracle import pyoracle as O
-from tests.golden.make_golden import lattice_start
+from tests.golden.configs import lattice_start
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
 CASES = ["tip4p_216", "tip5p_216", "st2_216", "tip4p_125_ndiv3"]
