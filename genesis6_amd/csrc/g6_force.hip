@@ -24,9 +24,11 @@ __device__ inline double wave_sum(double v) {
 
 __device__ inline void unpack_shift(int code, double L, double &ox, double &oy,
                                     double &oz) {
-  ox = (double)((code & 1023) - 512) * L;
-  oy = (double)(((code >> 10) & 1023) - 512) * L;
-  oz = (double)(((code >> 20) & 1023) - 512) * L;
+  // field 0 encodes -0.0 (see shift_code in g6_pairlist.hip)
+  const int a = code & 1023, b = (code >> 10) & 1023, c = (code >> 20) & 1023;
+  ox = a == 0 ? -0.0 : (double)(a - 512) * L;
+  oy = b == 0 ? -0.0 : (double)(b - 512) * L;
+  oz = c == 0 ? -0.0 : (double)(c - 512) * L;
 }
 
 template <int M, typename R>

@@ -13,7 +13,8 @@
 //     GridModule.F90:557-752); pairs in [off[r], off[r+1])
 //   pj[pcap], pi[pcap], scode[pcap]: compressed pair list; scode packs the
 //     image shift ox = kx*L etc. as 10-bit fields (kx+512) | (ky+512)<<10 |
-//     (kz+512)<<20 (COMs are unwrapped, so |k| can exceed 1)
+//     (kz+512)<<20 (COMs are unwrapped, so |k| can exceed 1); field value 0
+//     encodes the -0.0 shift that dnint gives for small negative differences
 //   facc[12][n]       FP64 accumulators: [0..5] force and torque (lab) that
 //     molecule i receives as the j of other rows (atomics), [6..11] its own
 //     row's share (plain stores, one writer)

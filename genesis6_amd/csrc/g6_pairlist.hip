@@ -102,9 +102,17 @@ __device__ inline int nbr_cell(const Geom &g, int c, int packed) {
   return (((cz + oz) % nd) * nd + (cy + oy) % nd) * nd + (cx + ox) % nd;
 }
 
-// Shift code: three 10-bit fields holding k + 512 for ox = k*L.
-__device__ inline int shift_code(int kx, int ky, int kz) {
-  return (kx + 512) | ((ky + 512) << 10) | ((kz + 512) << 20);
+// Shift code: three 10-bit fields holding k + 512 for ox = k*L, k in
+// [-511, 511]; the field value 0 stands for -0.0 (dnint of a small negative
+// difference), which the reference's FP64 shifts carry.
+__device__ inline int shift_field(double k) {
+  return (k == 0.0 && signbit(k)) ? 0 : (int)k + 512;
+}
+__device__ inline int shift_code(double kx, double ky, double kz) {
+  return shift_field(kx) | (shift_field(ky) << 10) | (shift_field(kz) << 20);
+}
+__device__ __host__ inline double shift_value(int field, double L) {
+  return field == 0 ? -0.0 : (double)(field - 512) * L;
 }
 
 // One wave per row r (r-th molecule in cell order).  mode 0 counts the
@@ -165,7 +173,7 @@ __global__ __launch_bounds__(256) void k_rows(Geom g, const double *com,
           pass = dd < g.rr;
           if (MODE == 1 && pass) {
             if (fabs(k0) > 511.0 || fabs(k1) > 511.0 || fabs(k2) > 511.0) flags[0] = 2;
-            code = shift_code((int)k0, (int)k1, (int)k2);
+            code = shift_code(k0, k1, k2);
           }
         }
       }
@@ -252,9 +260,9 @@ __global__ void k_unpack_shifts(const int *code, int np, double L, double *ox,
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= np) return;
   const int v = code[k];
-  ox[k] = (double)((v & 1023) - 512) * L;
-  oy[k] = (double)(((v >> 10) & 1023) - 512) * L;
-  oz[k] = (double)(((v >> 20) & 1023) - 512) * L;
+  ox[k] = shift_value(v & 1023, L);
+  oy[k] = shift_value((v >> 10) & 1023, L);
+  oz[k] = shift_value((v >> 20) & 1023, L);
 }
 
 int pairlist_once(System &s, const double *com_h, int64_t cap, int64_t *npair0,

@@ -28,6 +28,7 @@ def test_grid_and_pairlist_bit_exact(golden, name):
     np.testing.assert_array_equal(pi, g["pi"])
     np.testing.assert_array_equal(pj, g["pj"])
     assert np.array_equal(sh, g["shift"])      # FP64 shifts bit-exact
+    assert np.array_equal(np.signbit(sh), np.signbit(g["shift"]))
 
 
 @pytest.mark.parametrize("name", CASES)
