@@ -1,0 +1,162 @@
+"""Benchmark: TIP4P-4096 rigid-water MD step on one MI355X.
+
+Workload (the headline configuration): 4096 TIP4P molecules at 1.0 g/cm3
+(L = 49.669 A), Nose NVT at 300 K, cutoff 9.0 A switched over 0.5 A, dt 2 fs,
+FP32 pair arithmetic with FP64 accumulation; synthetic seeded lattice start
+with uniform random orientations and Maxwell-Boltzmann velocities.  One
+"step" is one full MD step (predict, cell grid + pair list, forces,
+collect, correct, thermostat), all resident in HBM and replayed as hipGraphs.
+
+Prints ONE JSON line (rank 0, N=1).  The FP64 GPU path on the same workload
+is timed too and reported beside it.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N, L, RC, TAPER, DT, TEMP = 4096, 49.669, 9.0, 0.5, 0.002, 300.0
+NOSE_Q = 7.5e5          # @NOS2 mass: T0*tau^2/dt^2 with tau = 0.1 ps
+SEED = 20261204
+FLOP_PER_PAIR = 257     # TIP4P: 9 Coulomb x 21 + LJ 28 + 40 per molecule pair
+FP32_PEAK = 157.3e12    # MI355X vector FP32 (vendor)
+HBM_PEAK = 8.0e12
+
+
+def make_system(prec, skin):
+    from genesis6_amd import g6
+    com, q = g6.lattice_start(N, L, SEED)
+    s = g6.System(g6.TIP4P, N, L, RC, TAPER, DT, prec=prec, skin=skin)
+    s.upload(com, q)
+    s.set_nose(TEMP, NOSE_Q)
+    s.vreset(TEMP, SEED)
+    return s, com, q
+
+
+def timed_run(s, steps, warmup):
+    s.step(warmup)
+    s.thermo()                       # synchronizes the stream
+    t0 = time.perf_counter()
+    s.step(steps)
+    th = s.thermo()                  # synchronizes the stream
+    t1 = time.perf_counter()
+    return t1 - t0, th
+
+
+def cpu_baseline(com, q, budget_s):
+    """The FP64 oracle (serial C restatement of the reference step) on the
+    same start, on one host core, for a bounded number of steps."""
+    from oracle import pyoracle as O
+    md = O.MDRun(O.TIP4P, com, q.T.copy(), L, RC, TAPER, DT, fast=True,
+                 nose=(TEMP, NOSE_Q, 0.0))
+    md.vreset(TEMP, SEED)
+    md.step(1)
+    t0 = time.perf_counter()
+    k = 0
+    while True:
+        md.step(1)
+        k += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s:
+            break
+    return N * k / el, k, el
+
+
+def traffic_from_profile():
+    """HBM bytes per force-kernel launch from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if present."""
+    p = os.path.join(ROOT, "profiles", "force_rows_tip4p4096_traffic.json")
+    try:
+        return json.load(open(p))["bytes_per_launch"]
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--skin", type=float, default=float(os.environ.get("G6_SKIN", "0")))
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the system is not sharded across GPUs "
+              "(see DESIGN.md, 'More than one GPU'); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    from genesis6_amd import g6
+    print("bench: TIP4P-%d FP32-pair, %d warmup + %d steps" % (N, a.warmup, a.steps),
+          file=sys.stderr, flush=True)
+    s, com, q = make_system(g6.FP32, a.skin)
+    el, th = timed_run(s, a.steps, a.warmup)
+    value = N * a.steps / el
+    # per-kernel timing with HIP events on the library stream (eager launches)
+    nt = min(200, max(20, a.steps // 50))
+    t = s.step_timed(nt)
+    t_force_s = t[0] * 1e-3
+    npair = t[4]
+    achieved = npair * FLOP_PER_PAIR / t_force_s
+    print("bench: FP32 %.4g molecule-steps/s, T=%.1f K, force kernel %.1f us, npair %.0f"
+          % (value, th[3], t[0] * 1e3, npair), file=sys.stderr, flush=True)
+    s.close()
+    fp64 = None
+    if not a.no_fp64:
+        s64, _, _ = make_system(g6.FP64, a.skin)
+        k64 = max(100, a.steps // 5)
+        el64, _ = timed_run(s64, k64, a.warmup)
+        t64 = s64.step_timed(nt)
+        fp64 = dict(value=N * k64 / el64, steps=k64, force_kernel_us=t64[0] * 1e3)
+        s64.close()
+        print("bench: FP64 %.4g molecule-steps/s" % fp64["value"], file=sys.stderr, flush=True)
+    cpu = None
+    if not a.no_cpu:
+        v, k, e = cpu_baseline(com, q, a.cpu_seconds)
+        cpu = {"value": v, "unit": "molecule-steps/s", "cores": 1, "kind": "port",
+               "sample": "TIP4P-4096 Nose NVT, %d MD steps (%.1f s) from the same start, "
+                         "FP64 oracle (serial C restatement of the reference step, "
+                         "custom TIP4P kernel)" % (k, e)}
+    out = {
+        "metric": "TIP4P-4096 molecule-steps/s",
+        "value": value,
+        "unit": "molecule-steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": el / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic: seeded lattice start, uniform random orientations, "
+                "Maxwell-Boltzmann velocities (MT19937 seed %d)" % SEED,
+        "config": {"workload": "TIP4P rigid water, 4096 molecules, 1.0 g/cm3, L=49.669 A, "
+                               "Nose NVT 300 K, rc 9.0 A switched over 0.5 A, dt 2 fs, "
+                               "FP32 pair arithmetic / FP64 accumulation",
+                   "n_molecules": N, "pair_list_skin_A": a.skin,
+                   "final_temperature_K": float(th[3]),
+                   "interacting_pairs_per_step": float(npair)},
+        "fp64_gpu": fp64,
+        "speedup_vs_fp64_gpu": (value / fp64["value"]) if fp64 else None,
+        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": FP32_PEAK / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK,
+                     "traffic": traffic_from_profile(),
+                     "kernel": "k_force_rows<TIP4P,float>",
+                     "kernel_ms": t[0], "flop_per_pair": FLOP_PER_PAIR,
+                     "pairs_per_launch": float(npair),
+                     "pairlist_ms": t[1], "per_molecule_kernels_ms": t[2],
+                     "eager_step_ms": t[3]},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -472,7 +472,7 @@ int g6_step(g6_handle h, int nsteps) {
 int g6_step_timed(g6_handle h, int nsteps, double *t) {
   System *s = get(h);
   if (!s || nsteps < 1 || !t) return G6_EARG;
-  double acc[4] = {0, 0, 0, 0};
+  double acc[5] = {0, 0, 0, 0, 0};
   for (int k = 0; k < nsteps; k++) {
     G6_CHECK(hipEventRecord(s->ev[0], s->stream));
     launch_predict_sites_bin(*s, true);
@@ -495,8 +495,11 @@ int g6_step_timed(g6_handle h, int nsteps, double *t) {
     acc[1] += b;
     acc[2] += a + d;
     acc[3] += e;
+    double th[G6_THERMO_LEN];
+    G6_CHECK(hipMemcpy(th, s->thermo, sizeof(th), hipMemcpyDeviceToHost));
+    acc[4] += th[14];
   }
-  for (int q = 0; q < 4; q++) t[q] = acc[q] / nsteps;
+  for (int q = 0; q < 5; q++) t[q] = acc[q] / nsteps;
   s->steps_done += nsteps;
   return check_flags(*s);
 }

@@ -139,7 +139,7 @@ class System:
         check(lib().g6_step(self.h, nsteps), "g6_step")
 
     def step_timed(self, nsteps):
-        t = np.zeros(4)
+        t = np.zeros(5)
         check(lib().g6_step_timed(self.h, nsteps, t), "g6_step_timed")
         return t
 

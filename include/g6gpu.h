@@ -137,7 +137,8 @@ int g6_lattice_start(int n, double box_l, uint32_t seed, double *com,
 /* nsteps MD steps launched one kernel at a time (no graph) with HIP events
  * on the library's stream around each phase; t[0..3] = mean ms per step of
  * the force kernel, the pair-list build, the per-molecule kernels and the
- * whole step.  Same results as g6_step. */
+ * whole step; t[4] = mean interacting pairs per step.  Same results as
+ * g6_step. */
 int g6_step_timed(g6_handle h, int nsteps, double *t);
 int g6_model_nsite(int model);
 
