@@ -190,17 +190,34 @@ static System *get(g6_handle h) {
 }
 
 static int alloc_all(System &s) {
-  const int n = s.n, ns = s.ns, nc = s.geo.ncell;
+  const int n = s.n, ns = s.ns, nc = s.geo.ncell, ncm = s.geo_md.ncell;
   auto A = [](void **p, size_t b) { return hipMalloc(p, b > 0 ? b : 8); };
 #define AL(ptr, bytes) G6_CHECK(A((void **)&(ptr), (bytes)))
-  AL(s.com, sizeof(double) * 3 * n);
-  AL(s.dx, sizeof(double) * 15 * n);
-  AL(s.w, sizeof(double) * 15 * n);
-  AL(s.quat, sizeof(double) * 20 * n);
-  AL(s.tmat, sizeof(double) * 9 * n);
-  AL(s.intra, sizeof(double) * 3 * ns * n);
-  AL(s.intraf, sizeof(float) * 3 * ns * n);
-  AL(s.com_ref, sizeof(double) * 3 * n);
+  MdBuf &m = s.md;
+  AL(m.com, sizeof(double) * 3 * n);
+  AL(m.com_p, sizeof(double) * 3 * n);
+  AL(m.dx, sizeof(double) * 15 * n);
+  AL(m.w, sizeof(double) * 15 * n);
+  AL(m.quat, sizeof(double) * 20 * n);
+  AL(m.tmat, sizeof(double) * 9 * n);
+  AL(m.intra, sizeof(double) * 3 * ns * n);
+  AL(m.intraf, sizeof(float) * 3 * ns * n);
+  AL(m.com_ref, sizeof(double) * 3 * n);
+  AL(m.cell, sizeof(int) * n);
+  AL(m.cellcnt, sizeof(int) * (ncm + 1));
+  AL(m.cellres, sizeof(int) * (size_t)ncm * m.cell_cap);
+  AL(m.nbr_j, sizeof(int) * (size_t)n * m.maxnb);
+  AL(m.nbr_code, sizeof(int) * (size_t)n * m.maxnb);
+  AL(m.nbr_cnt, sizeof(int) * n);
+  AL(m.blockpart, sizeof(double) * 16 * s.nblk_md);
+  AL(m.done, sizeof(int) * 4);
+  AL(m.nose, sizeof(double) * 9);
+  AL(m.thermo, sizeof(double) * G6_THERMO_LEN);
+  AL(m.flags, sizeof(int) * 8);
+  s.nose = m.nose;
+  s.thermo = m.thermo;
+  s.flags = m.flags;
+  // reference-list entry point (g6_pairlist)
   AL(s.cell, sizeof(int) * n);
   AL(s.cell_count, sizeof(int) * (nc + 1));
   AL(s.cell_start, sizeof(int) * (nc + 1));
@@ -212,37 +229,37 @@ static int alloc_all(System &s) {
   AL(s.pi, sizeof(int) * (size_t)s.pcap);
   AL(s.pj, sizeof(int) * (size_t)s.pcap);
   AL(s.scode, sizeof(int) * (size_t)s.pcap);
-  AL(s.facc, sizeof(double) * 12 * n);
-  AL(s.rowpart, sizeof(double) * 11 * n);
-  AL(s.blockpart, sizeof(double) * 4 * s.nblk_mol);
-  AL(s.nose, sizeof(double) * 9);
-  AL(s.thermo, sizeof(double) * G6_THERMO_LEN);
-  AL(s.flags, sizeof(int) * 8);
+  AL(s.com_ref, sizeof(double) * 3 * n);
+  AL(s.lflags, sizeof(int) * 8);
 #undef AL
-  G6_CHECK(hipMemset(s.com, 0, sizeof(double) * 3 * n));
-  G6_CHECK(hipMemset(s.dx, 0, sizeof(double) * 15 * n));
-  G6_CHECK(hipMemset(s.w, 0, sizeof(double) * 15 * n));
-  G6_CHECK(hipMemset(s.quat, 0, sizeof(double) * 20 * n));
-  G6_CHECK(hipMemset(s.off, 0, sizeof(int) * (2 * n + 1)));
-  G6_CHECK(hipMemset(s.nose, 0, sizeof(double) * 9));
-  G6_CHECK(hipMemset(s.thermo, 0, sizeof(double) * G6_THERMO_LEN));
+  G6_CHECK(hipMemset(m.com, 0, sizeof(double) * 3 * n));
+  G6_CHECK(hipMemset(m.dx, 0, sizeof(double) * 15 * n));
+  G6_CHECK(hipMemset(m.w, 0, sizeof(double) * 15 * n));
+  G6_CHECK(hipMemset(m.quat, 0, sizeof(double) * 20 * n));
+  G6_CHECK(hipMemset(m.nbr_cnt, 0, sizeof(int) * n));
+  G6_CHECK(hipMemset(m.com_ref, 0, sizeof(double) * 3 * n));
+  G6_CHECK(hipMemset(m.done, 0, sizeof(int) * 4));
+  G6_CHECK(hipMemset(m.nose, 0, sizeof(double) * 9));
+  G6_CHECK(hipMemset(m.thermo, 0, sizeof(double) * G6_THERMO_LEN));
   return 0;
 }
 
-// list and accumulators ready for a fresh first step
+// neighbour rows and cell lists ready for a fresh first step
 static int reset_step_state(System &s) {
   int fl[8] = {0, 1, 0, 0, 0, 0, 0, 0};
-  G6_CHECK(hipMemcpy(s.flags, fl, sizeof(fl), hipMemcpyHostToDevice));
-  G6_CHECK(hipMemset(s.cell_count, 0, sizeof(int) * (s.geo.ncell + 1)));
-  G6_CHECK(hipMemset(s.facc, 0, sizeof(double) * 12 * s.n));
+  G6_CHECK(hipMemcpy(s.md.flags, fl, sizeof(fl), hipMemcpyHostToDevice));
+  G6_CHECK(hipMemset(s.md.cellcnt, 0, sizeof(int) * (s.geo_md.ncell + 1)));
+  G6_CHECK(hipMemset(s.md.done, 0, sizeof(int) * 4));
   return 0;
 }
 
 static void free_all(System &s) {
-  void *ps[] = {s.com, s.dx, s.w, s.quat, s.tmat, s.intra, s.intraf, s.com_ref, s.cell,
-                s.cell_count, s.cell_start, s.cell_fill, s.sorted_tmp, s.sorted, s.cnt,
-                s.off, s.pi, s.pj, s.scode, s.facc, s.rowpart, s.blockpart, s.nose,
-                s.thermo, s.flags};
+  const MdBuf &m = s.md;
+  void *ps[] = {m.com, m.com_p, m.dx, m.w, m.quat, m.tmat, m.intra, m.intraf, m.com_ref,
+                m.cell, m.cellcnt, m.cellres, m.nbr_j, m.nbr_code, m.nbr_cnt, m.blockpart,
+                m.done, m.nose, m.thermo, m.flags, s.cell, s.cell_count, s.cell_start,
+                s.cell_fill, s.sorted_tmp, s.sorted, s.cnt, s.off, s.pi, s.pj, s.scode,
+                s.com_ref, s.lflags};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   if (s.graph) (void)hipGraphExecDestroy(s.graph);
@@ -251,17 +268,15 @@ static void free_all(System &s) {
 }
 
 static void launch_step(System &s) {
-  launch_predict_sites_bin(s, true);
-  launch_pairlist(s);
-  launch_force_rows(s);
-  launch_correct(s);
-  launch_thermo(s);
+  md_launch_pre(s, true);
+  md_launch_main(s, nullptr);
 }
 
+// steps per captured hipGraph (G6_GRAPH_STEPS; 0 disables graphs)
 static int graph_steps() {
   const char *e = getenv("G6_GRAPH_STEPS");
-  int v = e ? atoi(e) : 20;
-  return v < 1 ? 1 : (v > 1000 ? 1000 : v);
+  int v = e ? atoi(e) : 50;
+  return v < 0 ? 0 : (v > 1000 ? 1000 : v);
 }
 
 }  // namespace g6
@@ -323,7 +338,36 @@ int g6_init(g6_handle *h, int model, int n, double box_l, double rc, double tape
   const double capd = (double)n * (per * 1.6 + 64.0);
   const double all = 0.5 * (double)n * (double)(n - 1);
   s->pcap = (int)fmin(fmin(capd, all + 64.0), 2.0e9);
-  s->nblk_mol = (n + 255) / 256;
+  // the MD grid: cells no smaller than rc + skin, unique 27-neighbourhood
+  {
+    Geom &m = s->geo_md;
+    m = g;
+    m.ndiv = (int)(box_l / fmax(voxa, rl));
+    if (m.ndiv < 1) m.ndiv = 1;
+    m.ncell = m.ndiv * m.ndiv * m.ndiv;
+    m.cellinv = 1.0 / (box_l / m.ndiv);
+    MdBuf &b = s->md;
+    b.nfull = 0;
+    for (int dz = -1; dz <= 1; dz++)
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+          const int ox = (dx + m.ndiv) % m.ndiv, oy = (dy + m.ndiv) % m.ndiv,
+                    oz = (dz + m.ndiv) % m.ndiv;
+          const int packed = ox | (oy << 8) | (oz << 16);
+          bool seen = false;
+          for (int k = 0; k < b.nfull; k++) seen = seen || b.full[k] == packed;
+          if (!seen) b.full[b.nfull++] = packed;
+        }
+    const double rho = n / (box_l * box_l * box_l);
+    const double percell = (double)n / m.ncell;
+    b.cell_cap = ((int)(2.0 * percell + 32.0) + 31) / 32 * 32;
+    if (b.cell_cap > n) b.cell_cap = n;
+    const double nb = 4.0 / 3.0 * kPI * rl * rl * rl * rho;
+    b.maxnb = ((int)(1.5 * nb + 64.0) + 63) / 64 * 64;
+    if (b.maxnb > n) b.maxnb = n;
+    b.half_skin2 = skin > 0 ? 0.25 * skin * skin : 0.0;
+    s->nblk_md = (n + 3) / 4;
+  }
   int rcode = alloc_all(*s);
   if (rcode == 0 && hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
     rcode = G6_EHIP;
@@ -357,10 +401,10 @@ int g6_upload_state(g6_handle h, const double *com, const double *quat, const do
   System *s = get(h);
   if (!s || !com || !quat || !dx || !w) return G6_EARG;
   const int n = s->n;
-  G6_CHECK(hipMemcpy(s->com, com, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
-  G6_CHECK(hipMemcpy(s->quat, quat, sizeof(double) * 20 * n, hipMemcpyHostToDevice));
-  G6_CHECK(hipMemcpy(s->dx, dx, sizeof(double) * 15 * n, hipMemcpyHostToDevice));
-  G6_CHECK(hipMemcpy(s->w, w, sizeof(double) * 15 * n, hipMemcpyHostToDevice));
+  G6_CHECK(hipMemcpy(s->md.com, com, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+  G6_CHECK(hipMemcpy(s->md.quat, quat, sizeof(double) * 20 * n, hipMemcpyHostToDevice));
+  G6_CHECK(hipMemcpy(s->md.dx, dx, sizeof(double) * 15 * n, hipMemcpyHostToDevice));
+  G6_CHECK(hipMemcpy(s->md.w, w, sizeof(double) * 15 * n, hipMemcpyHostToDevice));
   return reset_step_state(*s);
 }
 
@@ -369,10 +413,10 @@ int g6_download_state(g6_handle h, double *com, double *quat, double *dx, double
   if (!s) return G6_EARG;
   const int n = s->n;
   G6_CHECK(hipStreamSynchronize(s->stream));
-  if (com) G6_CHECK(hipMemcpy(com, s->com, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
-  if (quat) G6_CHECK(hipMemcpy(quat, s->quat, sizeof(double) * 20 * n, hipMemcpyDeviceToHost));
-  if (dx) G6_CHECK(hipMemcpy(dx, s->dx, sizeof(double) * 15 * n, hipMemcpyDeviceToHost));
-  if (w) G6_CHECK(hipMemcpy(w, s->w, sizeof(double) * 15 * n, hipMemcpyDeviceToHost));
+  if (com) G6_CHECK(hipMemcpy(com, s->md.com, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+  if (quat) G6_CHECK(hipMemcpy(quat, s->md.quat, sizeof(double) * 20 * n, hipMemcpyDeviceToHost));
+  if (dx) G6_CHECK(hipMemcpy(dx, s->md.dx, sizeof(double) * 15 * n, hipMemcpyDeviceToHost));
+  if (w) G6_CHECK(hipMemcpy(w, s->md.w, sizeof(double) * 15 * n, hipMemcpyDeviceToHost));
   return 0;
 }
 
@@ -408,8 +452,8 @@ int g6_vreset(g6_handle h, double temp, uint32_t seed) {
   const int n = s->n;
   std::vector<double> dx(15 * (size_t)n), w(15 * (size_t)n);
   G6_CHECK(hipStreamSynchronize(s->stream));
-  G6_CHECK(hipMemcpy(dx.data(), s->dx, sizeof(double) * 15 * n, hipMemcpyDeviceToHost));
-  G6_CHECK(hipMemcpy(w.data(), s->w, sizeof(double) * 15 * n, hipMemcpyDeviceToHost));
+  G6_CHECK(hipMemcpy(dx.data(), s->md.dx, sizeof(double) * 15 * n, hipMemcpyDeviceToHost));
+  G6_CHECK(hipMemcpy(w.data(), s->md.w, sizeof(double) * 15 * n, hipMemcpyDeviceToHost));
   MT mt(seed);
   const double rgas = kNA * kKB, J2I = 1.0 / 10.0;
   const double massi = 1.0 / s->mt.mass;
@@ -429,16 +473,18 @@ int g6_vreset(g6_handle h, double temp, uint32_t seed) {
       w[(size_t)a * n + i] = normal(temp * rgas * J2I * ii[a], r1, r2);
     }
   }
-  G6_CHECK(hipMemcpy(s->dx, dx.data(), sizeof(double) * 15 * n, hipMemcpyHostToDevice));
-  G6_CHECK(hipMemcpy(s->w, w.data(), sizeof(double) * 15 * n, hipMemcpyHostToDevice));
+  G6_CHECK(hipMemcpy(s->md.dx, dx.data(), sizeof(double) * 15 * n, hipMemcpyHostToDevice));
+  G6_CHECK(hipMemcpy(s->md.w, w.data(), sizeof(double) * 15 * n, hipMemcpyHostToDevice));
   return 0;
 }
 
+// flags[0]: 1 pair capacity, 2 image shift beyond 511 box lengths,
+// 3 cell capacity, 4 neighbour-row capacity
 static int check_flags(System &s) {
   int fl[2] = {0, 0};
-  G6_CHECK(hipMemcpyAsync(fl, s.flags, sizeof(fl), hipMemcpyDeviceToHost, s.stream));
+  G6_CHECK(hipMemcpyAsync(fl, s.md.flags, sizeof(fl), hipMemcpyDeviceToHost, s.stream));
   G6_CHECK(hipStreamSynchronize(s.stream));
-  if (fl[0] == 1) return G6_ECAPACITY;
+  if (fl[0] == 1 || fl[0] == 3 || fl[0] == 4) return G6_ECAPACITY;
   if (fl[0] == 2) return G6_EARG;
   return 0;
 }
@@ -447,7 +493,7 @@ int g6_step(g6_handle h, int nsteps) {
   System *s = get(h);
   if (!s || nsteps < 0) return G6_EARG;
   const int G = graph_steps();
-  if (nsteps >= G) {
+  if (G > 0 && nsteps >= G) {
     if (!s->graph || s->graph_steps != G) {
       if (s->graph) {
         (void)hipGraphExecDestroy(s->graph);
@@ -463,7 +509,8 @@ int g6_step(g6_handle h, int nsteps) {
     }
     for (int k = 0; k + G <= nsteps; k += G) G6_CHECK(hipGraphLaunch(s->graph, s->stream));
   }
-  for (int k = 0; k < nsteps % G; k++) launch_step(*s);
+  const int rest = G > 0 ? nsteps % G : nsteps;
+  for (int k = 0; k < rest; k++) launch_step(*s);
   G6_CHECK(hipGetLastError());
   s->steps_done += nsteps;
   return check_flags(*s);
@@ -475,25 +522,18 @@ int g6_step_timed(g6_handle h, int nsteps, double *t) {
   double acc[5] = {0, 0, 0, 0, 0};
   for (int k = 0; k < nsteps; k++) {
     G6_CHECK(hipEventRecord(s->ev[0], s->stream));
-    launch_predict_sites_bin(*s, true);
+    md_launch_pre(*s, true);
     G6_CHECK(hipEventRecord(s->ev[1], s->stream));
-    launch_pairlist(*s);
+    md_launch_main(*s, nullptr);
     G6_CHECK(hipEventRecord(s->ev[2], s->stream));
-    launch_force_rows(*s);
-    G6_CHECK(hipEventRecord(s->ev[3], s->stream));
-    launch_correct(*s);
-    launch_thermo(*s);
-    G6_CHECK(hipEventRecord(s->ev[4], s->stream));
-    G6_CHECK(hipEventSynchronize(s->ev[4]));
-    float a, b, c, d, e;
+    G6_CHECK(hipEventSynchronize(s->ev[2]));
+    float a, b, e;
     G6_CHECK(hipEventElapsedTime(&a, s->ev[0], s->ev[1]));
     G6_CHECK(hipEventElapsedTime(&b, s->ev[1], s->ev[2]));
-    G6_CHECK(hipEventElapsedTime(&c, s->ev[2], s->ev[3]));
-    G6_CHECK(hipEventElapsedTime(&d, s->ev[3], s->ev[4]));
-    G6_CHECK(hipEventElapsedTime(&e, s->ev[0], s->ev[4]));
-    acc[0] += c;
-    acc[1] += b;
-    acc[2] += a + d;
+    G6_CHECK(hipEventElapsedTime(&e, s->ev[0], s->ev[2]));
+    acc[0] += b;   // force kernel (with the fused row rebuild and corrector)
+    acc[1] += 0;   // the list build is fused into the force kernel
+    acc[2] += a;   // predictor / sites / binning
     acc[3] += e;
     double th[G6_THERMO_LEN];
     G6_CHECK(hipMemcpy(th, s->thermo, sizeof(th), hipMemcpyDeviceToHost));
@@ -510,18 +550,12 @@ int g6_eval_forces(g6_handle h, double *force, double *torque, double *thermo) {
   const int n = s->n;
   double *out = nullptr;
   G6_CHECK(hipMalloc(&out, sizeof(double) * 6 * n));
-  // the state is unchanged: a list build is forced and the thermostat is
-  // left alone (k_thermo gets nose_active = 0 through a copy of the system)
+  // rows are rebuilt from the current positions; the state is not moved and
+  // the thermostat is not advanced
   int one = 1;
-  G6_CHECK(hipMemcpyAsync(s->flags + 1, &one, sizeof(int), hipMemcpyHostToDevice, s->stream));
-  launch_predict_sites_bin(*s, false);
-  launch_pairlist(*s);
-  launch_force_rows(*s);
-  launch_collect_acc(*s, out);
-  System tmp = *s;
-  tmp.nose_active = false;
-  tmp.skin = 0;  // keeps the build gate set for the next step
-  launch_thermo(tmp);
+  G6_CHECK(hipMemcpyAsync(s->md.flags + 1, &one, sizeof(int), hipMemcpyHostToDevice, s->stream));
+  md_launch_pre(*s, false);
+  md_launch_main(*s, out);
   G6_CHECK(hipGetLastError());
   G6_CHECK(hipMemcpyAsync(force, out, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s->stream));
   G6_CHECK(hipMemcpyAsync(torque, out + 3 * n, sizeof(double) * 3 * n, hipMemcpyDeviceToHost,
@@ -530,8 +564,6 @@ int g6_eval_forces(g6_handle h, double *force, double *torque, double *thermo) {
   G6_CHECK(hipFree(out));
   if (thermo) G6_CHECK(hipMemcpy(thermo, s->thermo, sizeof(double) * G6_THERMO_LEN,
                                  hipMemcpyDeviceToHost));
-  // the next step must rebuild the list from the predicted positions
-  G6_CHECK(hipMemcpy(s->flags + 1, &one, sizeof(int), hipMemcpyHostToDevice));
   return check_flags(*s);
 }
 

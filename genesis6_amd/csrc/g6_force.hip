@@ -1,13 +1,4 @@
-// Force kernels.
-//
-// k_force_rows: the MD hot path.  One wavefront per row of the pair list
-// (molecule i in cell order, both parts of its candidate list); each lane
-// evaluates one pair (i, j) with pair_sites (g6_pair.h), reduces the site
-// forces to the molecule force and lab-frame torque of i and j (the work of
-// rigid_collectforce, RigidModule.F90:275-317, done per pair), keeps i's
-// share in registers and sends j's share with FP64 atomics.  i's totals,
-// the pair energy and the virial vrmat(a,b) += d00(a)*F_i(b)
-// (TIP4PModule.F90:682-693) are reduced across the wave with shuffles.
+// Force kernels of the module-seam entry points (the MD path is g6_md.hip).
 //
 // k_force_pairs / k_energy_pairs: the module-seam entry points force() and
 // PairPotential() on caller-given site coordinates and ratios (one thread
@@ -29,118 +20,6 @@ __device__ inline void unpack_shift(int code, double L, double &ox, double &oy,
   ox = a == 0 ? -0.0 : (double)(a - 512) * L;
   oy = b == 0 ? -0.0 : (double)(b - 512) * L;
   oz = c == 0 ? -0.0 : (double)(c - 512) * L;
-}
-
-template <int M, typename R>
-__global__ __launch_bounds__(256) void k_force_rows(
-    ModelTable mt, Geom g, const double *__restrict__ com,
-    const R *__restrict__ intra, const int *__restrict__ sorted,
-    const int *__restrict__ off, const int *__restrict__ pj,
-    const int *__restrict__ scode, double *__restrict__ facc,
-    double *__restrict__ rowpart, int pcap, int filter) {
-  using T = Traits<M>;
-  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int r = blockIdx.x * 4 + wid;
-  if (r >= g.n) return;
-  const int n = g.n;
-  const int i = sorted[r];
-  R ri[T::NS][3];
-#pragma unroll
-  for (int s = 0; s < T::NS; s++)
-#pragma unroll
-    for (int a = 0; a < 3; a++) ri[s][a] = intra[(s * 3 + a) * n + i];
-  const double xi = com[i], yi = com[n + i], zi = com[2 * n + i];
-  double F[3] = {0, 0, 0}, Ti[3] = {0, 0, 0}, ep = 0, v[9] = {0}, npi = 0;
-  for (int part = 0; part < 2; part++) {
-    const int kb = off[part * n + r];
-    const int ke = min(off[part * n + r + 1], pcap);
-    for (int k = kb + lane; k < ke; k += 64) {
-      const int j = pj[k];
-      double ox, oy, oz;
-      unpack_shift(scode[k], g.L, ox, oy, oz);
-      double d0 = xi - com[j], d1 = yi - com[n + j], d2 = zi - com[2 * n + j];
-      d0 = d0 - ox;
-      d1 = d1 - oy;
-      d2 = d2 - oz;
-      const double dd64 = d0 * d0 + d1 * d1 + d2 * d2;
-      if (filter && !(dd64 < g.rc2)) continue;
-      R rj[T::NS][3];
-#pragma unroll
-      for (int s = 0; s < T::NS; s++)
-#pragma unroll
-        for (int a = 0; a < 3; a++) rj[s][a] = intra[(s * 3 + a) * n + j];
-      R hh, dh;
-      cutoff_sw<R>(g, R(dd64), hh, dh);
-      R fi[T::NS][3], fj[T::NS][3];
-      const R e = pair_sites<M, R>(mt, hh, dh, R(d0), R(d1), R(d2), ri, rj, fi, fj);
-      R Fx = 0, Fy = 0, Fz = 0, tix = 0, tiy = 0, tiz = 0, tjx = 0, tjy = 0, tjz = 0;
-#pragma unroll
-      for (int s = 0; s < T::NS; s++) {
-        Fx += fi[s][0]; Fy += fi[s][1]; Fz += fi[s][2];
-        tix += ri[s][1] * fi[s][2] - ri[s][2] * fi[s][1];
-        tiy += ri[s][2] * fi[s][0] - ri[s][0] * fi[s][2];
-        tiz += ri[s][0] * fi[s][1] - ri[s][1] * fi[s][0];
-        tjx += rj[s][1] * fj[s][2] - rj[s][2] * fj[s][1];
-        tjy += rj[s][2] * fj[s][0] - rj[s][0] * fj[s][2];
-        tjz += rj[s][0] * fj[s][1] - rj[s][1] * fj[s][0];
-      }
-      F[0] += Fx; F[1] += Fy; F[2] += Fz;
-      Ti[0] += tix; Ti[1] += tiy; Ti[2] += tiz;
-      unsafeAtomicAdd(&facc[0 * n + j], -(double)Fx);
-      unsafeAtomicAdd(&facc[1 * n + j], -(double)Fy);
-      unsafeAtomicAdd(&facc[2 * n + j], -(double)Fz);
-      unsafeAtomicAdd(&facc[3 * n + j], -(double)tjx);
-      unsafeAtomicAdd(&facc[4 * n + j], -(double)tjy);
-      unsafeAtomicAdd(&facc[5 * n + j], -(double)tjz);
-      ep += (double)e;
-      npi += 1.0;
-      const double dv[3] = {d0, d1, d2}, fv[3] = {(double)Fx, (double)Fy, (double)Fz};
-#pragma unroll
-      for (int a = 0; a < 3; a++)
-#pragma unroll
-        for (int b = 0; b < 3; b++) v[3 * a + b] += dv[a] * fv[b];
-    }
-  }
-#pragma unroll
-  for (int a = 0; a < 3; a++) {
-    F[a] = wave_sum(F[a]);
-    Ti[a] = wave_sum(Ti[a]);
-  }
-  ep = wave_sum(ep);
-  npi = wave_sum(npi);
-#pragma unroll
-  for (int a = 0; a < 9; a++) v[a] = wave_sum(v[a]);
-  if (lane == 0) {
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-      facc[(6 + a) * n + i] = F[a];
-      facc[(9 + a) * n + i] = Ti[a];
-    }
-    rowpart[r] = ep;
-#pragma unroll
-    for (int a = 0; a < 9; a++) rowpart[(1 + a) * n + r] = v[a];
-    rowpart[10 * n + r] = npi;
-  }
-}
-
-template <int M, typename R>
-static void launch_rows_t(System &s) {
-  const int nb = (s.n + 3) / 4;
-  k_force_rows<M, R><<<nb, 256, 0, s.stream>>>(
-      s.mt, s.geo, s.com, reinterpret_cast<const R *>(sizeof(R) == 4 ? (void *)s.intraf : (void *)s.intra),
-      s.sorted, s.off, s.pj, s.scode, s.facc, s.rowpart, s.pcap, s.skin > 0 ? 1 : 0);
-}
-
-void launch_force_rows(System &s) {
-  if (s.prec == G6_FP32) {
-    if (s.model == G6_TIP4P) launch_rows_t<G6_TIP4P, float>(s);
-    else if (s.model == G6_TIP5P) launch_rows_t<G6_TIP5P, float>(s);
-    else launch_rows_t<G6_ST2, float>(s);
-  } else {
-    if (s.model == G6_TIP4P) launch_rows_t<G6_TIP4P, double>(s);
-    else if (s.model == G6_TIP5P) launch_rows_t<G6_TIP5P, double>(s);
-    else launch_rows_t<G6_ST2, double>(s);
-  }
 }
 
 // ---- module seam: force() / PairPotential() on host-given arrays ---------

@@ -1,24 +1,26 @@
 // Internal definitions shared by the g6gpu translation units.
 //
-// Device data layout (one system per handle, all resident in HBM):
+// Device data layout (one system per handle, all resident in HBM).
+//
+// MD path (g6_md.hip), structure MdBuf:
 //   rigid state, FP64, component-major:  com[3][n], dx[5][3][n], w[5][3][n],
 //     quat[5][4][n]  (the Nordsieck arrays of RigidModule.F90:14-33 sRigid1)
+//   com_p[3][n]       predicted COMs, read by all waves of the force kernel
 //   tmat[9][n]        rotation matrices (rigid_setmatrix, RigidModule.F90:62-83)
 //   intra[ns][3][n]   site offsets from the COM, FP64; intraf the FP32 copy
-//   cell[n], cell_count/start/fill[ncell+1], sorted[n]: cell residents in
-//     molecule order (Address_Assign, GridModule.F90:503-552)
-//   row counts cnt[2n] and offsets off[2n+1]: row r = r-th molecule in cell
-//     order; rows 0..n-1 hold the half-shell part of the candidate list,
-//     rows n..2n-1 the self-mirror part (Grid_NeighborList,
-//     GridModule.F90:557-752); pairs in [off[r], off[r+1])
-//   pj[pcap], pi[pcap], scode[pcap]: compressed pair list; scode packs the
-//     image shift ox = kx*L etc. as 10-bit fields (kx+512) | (ky+512)<<10 |
-//     (kz+512)<<20 (COMs are unwrapped, so |k| can exceed 1); field value 0
-//     encodes the -0.0 shift that dnint gives for small negative differences
-//   facc[12][n]       FP64 accumulators: [0..5] force and torque (lab) that
-//     molecule i receives as the j of other rows (atomics), [6..11] its own
-//     row's share (plain stores, one writer)
-//   rowpart[11][n]    per-row energy, virial, interacting-pair count (FP64)
+//   cellcnt[ncell+1], cellres[ncell][cell_cap]: fixed-capacity cell lists of
+//     the MD grid (cell edge >= rc + skin)
+//   nbr_j/nbr_code[n][maxnb], nbr_cnt[n]: full (both-direction) Verlet rows
+//     within rc + skin; nbr_code packs the image shift ox = kx*L etc. as
+//     10-bit fields (kx+512) | (ky+512)<<10 | (kz+512)<<20 (COMs are
+//     unwrapped, so |k| can exceed 1); field value 0 encodes the -0.0 shift
+//     that dnint gives for small negative differences
+//   blockpart[nblk][16], done: per-block energy/virial/kinetic partials and
+//     the completion counter of the last-block reduction
+// Reference-list entry point (g6_pairlist.hip): cell residents in molecule
+//   order and the half-shell candidate rows of Grid_NeighborList
+//   (GridModule.F90:557-752) with counts cnt[2n], offsets off[2n+1], and the
+//   compressed list pi/pj/scode[pcap] in the reference's order.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -56,11 +58,30 @@ struct Geom {
   double rin, rout, c1, c2; // switch: S between rin and rout
 };
 
+// Device pointers and sizes of the MD path, passed to kernels by value.
+struct MdBuf {
+  double *com, *com_p, *dx, *w, *quat, *tmat, *intra, *com_ref;
+  float *intraf;
+  int *cell, *cellcnt, *cellres;
+  int cell_cap;
+  int *nbr_j, *nbr_code, *nbr_cnt;
+  int maxnb;
+  int nfull;
+  int full[27];  // unique neighbour-cell offsets (packed ox | oy<<8 | oz<<16)
+  double *blockpart;
+  int *done;
+  double *nose, *thermo;
+  int *flags;  // [0] error code, [1] rebuild rows this step
+  double half_skin2;
+};
+
 struct System {
   int model = 0, n = 0, ns = 0, prec = 0;
   double L = 0, rc = 0, taper = 0, req = 0, dt = 0, skin = 0;
   ModelTable mt{};
-  Geom geo{};
+  Geom geo{};     // the reference grid (ndiv = int(L/voxa)), entry points
+  Geom geo_md{};  // the MD grid (cell edge >= rc + skin)
+  MdBuf md{};
   int pcap = 0;
   bool nose_active = false;
   double nose_h[9] = {0};
@@ -74,10 +95,10 @@ struct System {
   int *cnt = nullptr, *off = nullptr;
   int *pi = nullptr, *pj = nullptr;
   int *scode = nullptr;
-  double *facc = nullptr, *rowpart = nullptr, *blockpart = nullptr;
   double *nose = nullptr, *thermo = nullptr;
-  int *flags = nullptr;  // [0] overflow, [1] rebuild request, [2] builds
-  int nblk_mol = 0;
+  int *flags = nullptr;   // = md.flags
+  int *lflags = nullptr;  // g6_pairlist: [0] error code, [1] build gate
+  int nblk_md = 0;
   hipStream_t stream = nullptr;
   hipGraphExec_t graph = nullptr;
   int graph_steps = 0;
@@ -89,13 +110,9 @@ struct System {
 // model tables (g6_api.hip)
 bool make_model_table(int model, int prec, ModelTable *mt);
 
-// launchers (stream-ordered, graph-capturable: no memset, no host sync)
-void launch_predict_sites_bin(System &s, bool bin);
-void launch_pairlist(System &s);
-void launch_force_rows(System &s);
-void launch_correct(System &s);
-void launch_thermo(System &s);
-void launch_collect_acc(System &s, double *out);
+// MD launchers (stream-ordered, graph-capturable: no memset, no host sync)
+void md_launch_pre(System &s, bool predict);
+void md_launch_main(System &s, double *out);
 // host-array entry points of the module seam
 int pairlist_once(System &s, const double *com_h, int64_t cap, int64_t *npair0,
                   int64_t *npair, int32_t *pi, int32_t *pj, double *ox,

@@ -1,8 +1,8 @@
-// Cell grid and pair list on the GPU, reproducing the reference's list
-// element for element: the same pairs in the same order with the same FP64
-// image shifts.
+// The reference's cell grid and pair list on the GPU (the g6_pairlist entry
+// point), reproducing the list element for element: the same pairs in the
+// same order with the same FP64 image shifts.
 //
-//   Address_Assign      GridModule.F90:503-552   (k_bin in g6_integrate.hip)
+//   Address_Assign      GridModule.F90:503-552   k_bin_only
 //   residents in molecule order                  k_scan_cells, k_scatter,
 //                                                k_sort_cells
 //   Grid_NeighborList   GridModule.F90:557-752   k_rows (count / write)
@@ -195,26 +195,6 @@ __global__ __launch_bounds__(256) void k_rows(Geom g, const double *com,
   }
 }
 
-void launch_pairlist(System &s) {
-  const Geom &g = s.geo;
-  const int *gate = s.flags + 1;
-  k_exscan<<<1, 1024, 0, s.stream>>>(s.cell_count, s.cell_start, s.cell_fill,
-                                     g.ncell, gate, 0, s.flags);
-  k_scatter<<<(g.n + 255) / 256, 256, 0, s.stream>>>(s.cell, s.cell_fill, s.sorted_tmp,
-                                                      s.com, s.com_ref, g.n, gate);
-  k_sort_cells<<<(g.ncell + 3) / 4, 256, 0, s.stream>>>(s.cell_start, s.sorted_tmp,
-                                                         s.sorted, g.ncell, gate);
-  const int nb = (g.n + 3) / 4;
-  k_rows<0><<<nb, 256, 0, s.stream>>>(g, s.com, s.cell, s.cell_start, s.sorted, s.cnt,
-                                      s.off, s.pi, s.pj, s.scode,
-                                      s.pcap, gate, s.flags);
-  k_exscan<<<1, 1024, 0, s.stream>>>(s.cnt, s.off, nullptr, 2 * g.n, gate, s.pcap,
-                                     s.flags);
-  k_rows<1><<<nb, 256, 0, s.stream>>>(g, s.com, s.cell, s.cell_start, s.sorted, s.cnt,
-                                      s.off, s.pi, s.pj, s.scode,
-                                      s.pcap, gate, s.flags);
-}
-
 // Host-array entry: the reference's list for given COMs, written to caller
 // arrays as 0-based indices and FP64 shifts.
 __global__ void k_bin_only(Geom g, const double *com, int *cell, int *count) {
@@ -278,13 +258,13 @@ int pairlist_once(System &s, const double *com_h, int64_t cap, int64_t *npair0,
                           s.stream));
   G6_CHECK(hipMemsetAsync(s.cell_count, 0, sizeof(int) * (g.ncell + 1), s.stream));
   int one = 1;
-  G6_CHECK(hipMemcpyAsync(s.flags + 1, &one, sizeof(int), hipMemcpyHostToDevice, s.stream));
-  G6_CHECK(hipMemsetAsync(s.flags, 0, sizeof(int), s.stream));
+  G6_CHECK(hipMemcpyAsync(s.lflags + 1, &one, sizeof(int), hipMemcpyHostToDevice, s.stream));
+  G6_CHECK(hipMemsetAsync(s.lflags, 0, sizeof(int), s.stream));
   k_bin_only<<<(n + 255) / 256, 256, 0, s.stream>>>(g, com, s.cell, s.cell_count);
-  const int *gate = s.flags + 1;
+  const int *gate = s.lflags + 1;
   Geom g2 = g;
   k_exscan<<<1, 1024, 0, s.stream>>>(s.cell_count, s.cell_start, s.cell_fill, g.ncell,
-                                     gate, 0, s.flags);
+                                     gate, 0, s.lflags);
   k_scatter<<<(n + 255) / 256, 256, 0, s.stream>>>(s.cell, s.cell_fill, s.sorted_tmp, com,
                                                     ref, n, gate);
   k_sort_cells<<<(g.ncell + 3) / 4, 256, 0, s.stream>>>(s.cell_start, s.sorted_tmp,
@@ -292,15 +272,15 @@ int pairlist_once(System &s, const double *com_h, int64_t cap, int64_t *npair0,
   const int nb = (n + 3) / 4;
   k_rows<0><<<nb, 256, 0, s.stream>>>(g2, com, s.cell, s.cell_start, s.sorted, s.cnt,
                                       s.off, s.pi, s.pj, s.scode,
-                                      s.pcap, gate, s.flags);
-  k_exscan<<<1, 1024, 0, s.stream>>>(s.cnt, s.off, nullptr, 2 * n, gate, s.pcap, s.flags);
+                                      s.pcap, gate, s.lflags);
+  k_exscan<<<1, 1024, 0, s.stream>>>(s.cnt, s.off, nullptr, 2 * n, gate, s.pcap, s.lflags);
   k_rows<1><<<nb, 256, 0, s.stream>>>(g2, com, s.cell, s.cell_start, s.sorted, s.cnt,
                                       s.off, s.pi, s.pj, s.scode,
-                                      s.pcap, gate, s.flags);
+                                      s.pcap, gate, s.lflags);
   int total = 0, flag = 0;
   G6_CHECK(hipMemcpyAsync(&total, s.off + 2 * n, sizeof(int), hipMemcpyDeviceToHost,
                           s.stream));
-  G6_CHECK(hipMemcpyAsync(&flag, s.flags, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+  G6_CHECK(hipMemcpyAsync(&flag, s.lflags, sizeof(int), hipMemcpyDeviceToHost, s.stream));
   G6_CHECK(hipStreamSynchronize(s.stream));
   // candidates: sum over rows of the stencil residents (counted on host from
   // the cell occupancy; cheap and exact)
@@ -352,7 +332,7 @@ int pairlist_once(System &s, const double *com_h, int64_t cap, int64_t *npair0,
   G6_CHECK(hipFree(com));
   G6_CHECK(hipFree(ref));
   // the MD list in the system buffers is no longer valid
-  G6_CHECK(hipMemcpy(s.flags + 1, &one, sizeof(int), hipMemcpyHostToDevice));
+  G6_CHECK(hipMemcpy(s.lflags + 1, &one, sizeof(int), hipMemcpyHostToDevice));
   G6_CHECK(hipMemset(s.cell_count, 0, sizeof(int) * (g.ncell + 1)));
   return rc;
 }

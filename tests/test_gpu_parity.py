@@ -164,24 +164,44 @@ def test_md_trajectory_vs_reference_driver(golden, name):
     assert np.abs(dx - g["dxyz_final"].transpose(1, 0, 2)).max() < 1e-12
 
 
-def test_graph_replay_equals_eager():
-    """hipGraph-captured steps give the same trajectory as eager launches."""
+@pytest.mark.parametrize("prec", [g6.FP64, g6.FP32])
+def test_graph_replay_equals_eager_bitwise(prec):
+    """hipGraph-captured steps give the same trajectory as eager launches,
+    bit for bit: the MD path has no floating-point atomics."""
     n, L = 512, 24.83
     com, q = lattice_start(n, L, 11)
-    a = g6.System(g6.TIP4P, n, L, 9.0, 0.5, 0.002)
-    b = g6.System(g6.TIP4P, n, L, 9.0, 0.5, 0.002)
+    a = g6.System(g6.TIP4P, n, L, 9.0, 0.5, 0.002, prec=prec, skin=1.0)
+    b = g6.System(g6.TIP4P, n, L, 9.0, 0.5, 0.002, prec=prec, skin=1.0)
     for s in (a, b):
         s.upload(com, q)
+        s.set_nose(300.0, 7.5e5)
         s.vreset(300.0, 7)
-    a.step(40)            # two graph replays of 20 steps
-    for _ in range(40):
+    a.step(100)           # two graph replays of 50 steps
+    for _ in range(100):
         b.step(1)         # eager launches
-    ca, qa, _, _ = a.download()
-    cb, qb, _, _ = b.download()
-    # atomics make the FP64 sums order-dependent: agreement to round-off growth
-    assert np.abs(ca - cb).max() < 1e-8 and np.abs(qa - qb).max() < 1e-8
-    ta, tb = a.thermo(), b.thermo()
-    assert abs(ta[0] - tb[0]) < 1e-8 * abs(tb[0])
+    for x, y in zip(a.download(), b.download()):
+        assert np.array_equal(x, y)
+    assert np.array_equal(a.thermo(), b.thermo())
+    assert np.array_equal(a.nose(), b.nose())
+
+
+def test_skin_list_gives_the_reference_interacting_set():
+    """A Verlet skin changes when rows are rebuilt, not which pairs interact:
+    50 FP64 steps with skin 0 and skin 1.5 A agree to round-off."""
+    n, L = 1024, 31.289
+    com, q = lattice_start(n, L, 12)
+    out = []
+    for skin in (0.0, 1.5):
+        s = g6.System(g6.TIP4P, n, L, 9.0, 0.5, 0.002, skin=skin)
+        s.upload(com, q)
+        s.vreset(300.0, 9)
+        pairs = []
+        for _ in range(50):
+            s.step(1)
+            pairs.append(s.thermo()[14])
+        out.append((s.download(), pairs))
+    assert out[0][1] == out[1][1]          # same interacting pairs every step
+    assert np.abs(out[0][0][0] - out[1][0][0]).max() < 1e-9
 
 
 def test_capacity_error_is_reported():
