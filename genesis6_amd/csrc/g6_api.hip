@@ -210,6 +210,9 @@ static int alloc_all(System &s) {
   AL(m.nbr_code, sizeof(int) * (size_t)n * m.maxnb);
   AL(m.nbr_cnt, sizeof(int) * n);
   AL(m.blockpart, sizeof(double) * 16 * s.nblk_md);
+  AL(m.blockek, sizeof(double) * 4 * ((n + 63) / 64));
+  AL(m.frc, sizeof(double) * 6 * n);
+  AL(m.epvir, sizeof(double) * 16);
   AL(m.done, sizeof(int) * 4);
   AL(m.nose, sizeof(double) * 9);
   AL(m.thermo, sizeof(double) * G6_THERMO_LEN);
@@ -238,6 +241,7 @@ static int alloc_all(System &s) {
   G6_CHECK(hipMemset(m.quat, 0, sizeof(double) * 20 * n));
   G6_CHECK(hipMemset(m.nbr_cnt, 0, sizeof(int) * n));
   G6_CHECK(hipMemset(m.com_ref, 0, sizeof(double) * 3 * n));
+  G6_CHECK(hipMemset(m.epvir, 0, sizeof(double) * 16));
   G6_CHECK(hipMemset(m.done, 0, sizeof(int) * 4));
   G6_CHECK(hipMemset(m.nose, 0, sizeof(double) * 9));
   G6_CHECK(hipMemset(m.thermo, 0, sizeof(double) * G6_THERMO_LEN));
@@ -257,6 +261,7 @@ static void free_all(System &s) {
   const MdBuf &m = s.md;
   void *ps[] = {m.com, m.com_p, m.dx, m.w, m.quat, m.tmat, m.intra, m.intraf, m.com_ref,
                 m.cell, m.cellcnt, m.cellres, m.nbr_j, m.nbr_code, m.nbr_cnt, m.blockpart,
+                m.blockek, m.frc, m.epvir,
                 m.done, m.nose, m.thermo, m.flags, s.cell, s.cell_count, s.cell_start,
                 s.cell_fill, s.sorted_tmp, s.sorted, s.cnt, s.off, s.pi, s.pj, s.scode,
                 s.com_ref, s.lflags};
@@ -267,9 +272,14 @@ static void free_all(System &s) {
   if (s.stream) (void)hipStreamDestroy(s.stream);
 }
 
-static void launch_step(System &s) {
-  md_launch_pre(s, true);
-  md_launch_main(s, nullptr);
+// k steps: predict, forces; (correct + predict, forces) x (k-1); correct.
+// Each call is self-contained, so a captured graph of k steps can be replayed.
+static void launch_steps(System &s, int k) {
+  for (int q = 0; q < k; q++) {
+    md_launch_pre(s, q > 0, true);
+    md_launch_main(s, nullptr);
+  }
+  md_launch_pre(s, true, false);
 }
 
 // steps per captured hipGraph (G6_GRAPH_STEPS; 0 disables graphs)
@@ -501,7 +511,7 @@ int g6_step(g6_handle h, int nsteps) {
       }
       hipGraph_t gr;
       G6_CHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-      for (int k = 0; k < G; k++) launch_step(*s);
+      launch_steps(*s, G);
       G6_CHECK(hipStreamEndCapture(s->stream, &gr));
       G6_CHECK(hipGraphInstantiate(&s->graph, gr, nullptr, nullptr, 0));
       G6_CHECK(hipGraphDestroy(gr));
@@ -510,7 +520,7 @@ int g6_step(g6_handle h, int nsteps) {
     for (int k = 0; k + G <= nsteps; k += G) G6_CHECK(hipGraphLaunch(s->graph, s->stream));
   }
   const int rest = G > 0 ? nsteps % G : nsteps;
-  for (int k = 0; k < rest; k++) launch_step(*s);
+  if (rest > 0) launch_steps(*s, rest);
   G6_CHECK(hipGetLastError());
   s->steps_done += nsteps;
   return check_flags(*s);
@@ -521,19 +531,24 @@ int g6_step_timed(g6_handle h, int nsteps, double *t) {
   if (!s || nsteps < 1 || !t) return G6_EARG;
   double acc[5] = {0, 0, 0, 0, 0};
   for (int k = 0; k < nsteps; k++) {
+    // one step: predict, forces, correct (the corrector and the next
+    // predictor share a kernel in g6_step; here they are timed apart)
     G6_CHECK(hipEventRecord(s->ev[0], s->stream));
-    md_launch_pre(*s, true);
+    md_launch_pre(*s, false, true);
     G6_CHECK(hipEventRecord(s->ev[1], s->stream));
     md_launch_main(*s, nullptr);
     G6_CHECK(hipEventRecord(s->ev[2], s->stream));
-    G6_CHECK(hipEventSynchronize(s->ev[2]));
-    float a, b, e;
+    md_launch_pre(*s, true, false);
+    G6_CHECK(hipEventRecord(s->ev[3], s->stream));
+    G6_CHECK(hipEventSynchronize(s->ev[3]));
+    float a, b, c, e;
     G6_CHECK(hipEventElapsedTime(&a, s->ev[0], s->ev[1]));
     G6_CHECK(hipEventElapsedTime(&b, s->ev[1], s->ev[2]));
-    G6_CHECK(hipEventElapsedTime(&e, s->ev[0], s->ev[2]));
-    acc[0] += b;   // force kernel (with the fused row rebuild and corrector)
-    acc[1] += 0;   // the list build is fused into the force kernel
-    acc[2] += a;   // predictor / sites / binning
+    G6_CHECK(hipEventElapsedTime(&c, s->ev[2], s->ev[3]));
+    G6_CHECK(hipEventElapsedTime(&e, s->ev[0], s->ev[3]));
+    acc[0] += b;   // force kernel (rows rebuilt in it when due)
+    acc[1] += 0;   // no separate list build
+    acc[2] += a + c;
     acc[3] += e;
     double th[G6_THERMO_LEN];
     G6_CHECK(hipMemcpy(th, s->thermo, sizeof(th), hipMemcpyDeviceToHost));
@@ -554,7 +569,7 @@ int g6_eval_forces(g6_handle h, double *force, double *torque, double *thermo) {
   // the thermostat is not advanced
   int one = 1;
   G6_CHECK(hipMemcpyAsync(s->md.flags + 1, &one, sizeof(int), hipMemcpyHostToDevice, s->stream));
-  md_launch_pre(*s, false);
+  md_launch_pre(*s, false, false);
   md_launch_main(*s, out);
   G6_CHECK(hipGetLastError());
   G6_CHECK(hipMemcpyAsync(force, out, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s->stream));

@@ -68,8 +68,11 @@ struct MdBuf {
   int maxnb;
   int nfull;
   int full[27];  // unique neighbour-cell offsets (packed ox | oy<<8 | oz<<16)
-  double *blockpart;
-  int *done;
+  double *frc;        // [6][n] force and torque (lab) of the last k_md_main
+  double *blockpart;  // [nblk][16] k_md_main block partials
+  double *blockek;    // [nblk_pre][4] k_md_pre kinetic partials
+  double *epvir;      // [16] ep, vir[9], pairs, list pairs of the last k_md_main
+  int *done;          // [0] k_md_main, [1] k_md_pre completion counters
   double *nose, *thermo;
   int *flags;  // [0] error code, [1] rebuild rows this step
   double half_skin2;
@@ -111,7 +114,7 @@ struct System {
 bool make_model_table(int model, int prec, ModelTable *mt);
 
 // MD launchers (stream-ordered, graph-capturable: no memset, no host sync)
-void md_launch_pre(System &s, bool predict);
+void md_launch_pre(System &s, bool correct, bool predict);
 void md_launch_main(System &s, double *out);
 // host-array entry points of the module seam
 int pairlist_once(System &s, const double *com_h, int64_t cap, int64_t *npair0,

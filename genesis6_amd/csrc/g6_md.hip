@@ -1,9 +1,12 @@
 // The MD step in two kernels (the path g6_step runs):
 //
-//   k_md_pre   one thread per molecule: Nose_Predict (block 0), rigid_predict,
-//              rigid_setmatrix + site offsets, Address_Assign binning into
-//              fixed-capacity cell lists, and the Verlet-skin test that
-//              decides whether the neighbour lists are rebuilt this step.
+//   k_md_pre   one thread per molecule, state held in registers:
+//              rigid_correct with the forces of the previous k_md_main and the
+//              kinetic partials (the last block forms the thermo record and
+//              applies Nose_Correct), then rigid_predict (the last block then
+//              applies Nose_Predict), rigid_setmatrix + site offsets,
+//              Address_Assign binning into fixed-capacity cell lists, and the
+//              Verlet-skin test that decides whether rows are rebuilt.
 //   k_md_main  one wavefront per molecule i:
 //              (1) if a rebuild is due, i's own neighbour row: all j != i in
 //                  the 27 (or fewer, for ndiv < 3) surrounding cells with the
@@ -13,15 +16,22 @@
 //                  Interaction_Compress2 test, so the interacting set is the
 //                  reference's - via pair_sites (g6_pair.h);
 //              (3) wave reduction of i's force and lab torque (the work of
-//                  rigid_collectforce) and, on lane 0, rigid_correct for i;
-//              (4) block partials of energy, virial and kinetic terms; the
-//                  last block to finish reduces them in a fixed order and
-//                  applies Nose_Correct (NoseModule.F90:43-72).
+//                  rigid_collectforce), stored for the corrector;
+//              (4) block partials of energy and virial; the last block to
+//                  finish reduces them in a fixed order.
+// A step is k_md_pre(correct previous, predict) + k_md_main; a run of steps
+// ends with a correct-only k_md_pre, so every g6_step call leaves the state
+// of the reference's loop end (after Nose_Correct, Mixture.F90:2282).
 // Each unordered pair is evaluated in both rows, so no molecule's force
 // needs an atomic and the FP64 results are deterministic; energies and the
 // virial are halved accordingly.  Positions read by other waves come from
 // com_p (the predicted COMs), never from the state being corrected.
 #include "g6_pair.h"
+
+// Everything written in this file is FP64 bookkeeping in the reference's
+// operation order: no FMA contraction.  The FP32 pair arithmetic lives in
+// g6_pair.h, outside this pragma, and may contract.
+#pragma clang fp contract(off)
 
 namespace g6 {
 
@@ -43,165 +53,131 @@ __device__ inline void wsync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// ---- kernel A --------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_md_pre(ModelTable mt, Geom g, MdBuf b,
-                                               int nose_active, int predict) {
-#pragma clang fp contract(off)
-  const int n = g.n;
-  if (predict && nose_active && blockIdx.x == 0 && threadIdx.x == 0) {
-    double *z = b.nose;
-    z[3] = z[3] + z[4] + z[5] + z[6] + z[7];
-    z[4] = z[4] + 2.0 * z[5] + 3.0 * z[6] + 4.0 * z[7];
-    z[5] = z[5] + 3.0 * z[6] + 6.0 * z[7];
-    z[6] = z[6] + 4.0 * z[7];
-  }
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double *com = b.com, *dx = b.dx, *w = b.w, *quat = b.quat;
-  if (predict) {
-#define DX(s, a) dx[((s) * 3 + (a)) * n + i]
-#define W(s, a) w[((s) * 3 + (a)) * n + i]
-#define Q(s, a) quat[((s) * 4 + (a)) * n + i]
+// ---- per-molecule state in registers --------------------------------------
+struct Mol {
+  double com[3], dx[5][3], w[5][3], q[5][4];
+};
+
+__device__ inline void load_mol(const MdBuf &b, int n, int i, Mol &m) {
+#pragma unroll
+  for (int a = 0; a < 3; a++) m.com[a] = b.com[a * n + i];
+#pragma unroll
+  for (int s = 0; s < 5; s++)
+#pragma unroll
     for (int a = 0; a < 3; a++) {
-      com[a * n + i] = com[a * n + i] + DX(0, a) + DX(1, a) + DX(2, a) + DX(3, a) + DX(4, a);
-      DX(0, a) = DX(0, a) + 2.0 * DX(1, a) + 3.0 * DX(2, a) + 4.0 * DX(3, a) + 5.0 * DX(4, a);
-      DX(1, a) = DX(1, a) + 3.0 * DX(2, a) + 6.0 * DX(3, a) + 10.0 * DX(4, a);
-      DX(2, a) = DX(2, a) + 4.0 * DX(3, a) + 10.0 * DX(4, a);
-      DX(3, a) = DX(3, a) + 5.0 * DX(4, a);
+      m.dx[s][a] = b.dx[(s * 3 + a) * n + i];
+      m.w[s][a] = b.w[(s * 3 + a) * n + i];
     }
-    for (int a = 0; a < 3; a++) {
-      W(0, a) = W(0, a) + W(1, a) + W(2, a) + W(3, a) + W(4, a);
-      W(1, a) = W(1, a) + 2.0 * W(2, a) + 3.0 * W(3, a) + 4.0 * W(4, a);
-      W(2, a) = W(2, a) + 3.0 * W(3, a) + 6.0 * W(4, a);
-      W(3, a) = W(3, a) + 4.0 * W(4, a);
-    }
-    for (int a = 0; a < 4; a++) {
-      Q(0, a) = Q(0, a) + Q(1, a) + Q(2, a) + Q(3, a) + Q(4, a);
-      Q(1, a) = Q(1, a) + 2.0 * Q(2, a) + 3.0 * Q(3, a) + 4.0 * Q(4, a);
-    }
-    const double re = 1.0 / sqrt(Q(0, 0) * Q(0, 0) + Q(0, 1) * Q(0, 1) +
-                                 Q(0, 2) * Q(0, 2) + Q(0, 3) * Q(0, 3));
-    for (int a = 0; a < 4; a++) Q(0, a) = Q(0, a) * re;
-#undef DX
-#undef W
-#undef Q
-  }
-  double p[3];
-  for (int a = 0; a < 3; a++) {
-    p[a] = com[a * n + i];
-    b.com_p[a * n + i] = p[a];
-  }
-  const double qa = quat[i], qb = quat[n + i], qc = quat[2 * n + i], qd = quat[3 * n + i];
-  double t[9];
-  t[0] = (qa * qa + qb * qb - (qc * qc + qd * qd));
-  t[1] = -2.0 * (qa * qd + qb * qc);
-  t[2] = 2.0 * (qb * qd - qa * qc);
-  t[3] = 2.0 * (qa * qd - qb * qc);
-  t[4] = qa * qa + qc * qc - (qb * qb + qd * qd);
-  t[5] = -2.0 * (qa * qb + qc * qd);
-  t[6] = 2.0 * (qa * qc + qb * qd);
-  t[7] = 2.0 * (qa * qb - qc * qd);
-  t[8] = qa * qa + qd * qd - (qb * qb + qc * qc);
-  for (int a = 0; a < 9; a++) b.tmat[a * n + i] = t[a];
-  for (int s = 0; s < mt.nsite; s++) {
-    const double r0 = t[0] * mt.bx[s] + t[1] * mt.by[s] + t[2] * mt.bz[s];
-    const double r1 = t[3] * mt.bx[s] + t[4] * mt.by[s] + t[5] * mt.bz[s];
-    const double r2 = t[6] * mt.bx[s] + t[7] * mt.by[s] + t[8] * mt.bz[s];
-    b.intra[(s * 3 + 0) * n + i] = r0;
-    b.intra[(s * 3 + 1) * n + i] = r1;
-    b.intra[(s * 3 + 2) * n + i] = r2;
-    b.intraf[(s * 3 + 0) * n + i] = (float)r0;
-    b.intraf[(s * 3 + 1) * n + i] = (float)r1;
-    b.intraf[(s * 3 + 2) * n + i] = (float)r2;
-  }
-  int jc[3];
-  for (int a = 0; a < 3; a++) {
-    double d = p[a];
-    const double o = round(d * g.invL) * g.L;
-    d = d - o;
-    int j = (int)round(d * g.cellinv);
-    if (j < 0) j += g.ndiv;
-    j = ((j % g.ndiv) + g.ndiv) % g.ndiv;
-    jc[a] = j;
-  }
-  const int c = (jc[2] * g.ndiv + jc[1]) * g.ndiv + jc[0];
-  b.cell[i] = c;
-  const int slot = atomicAdd(&b.cellcnt[c], 1);
-  if (slot < b.cell_cap) b.cellres[(size_t)c * b.cell_cap + slot] = i;
-  else b.flags[0] = 3;  // cell capacity exceeded
-  if (b.half_skin2 > 0.0) {
-    const double e0 = p[0] - b.com_ref[i], e1 = p[1] - b.com_ref[n + i],
-                 e2 = p[2] - b.com_ref[2 * n + i];
-    if (e0 * e0 + e1 * e1 + e2 * e2 > b.half_skin2) b.flags[1] = 1;
-  }
+#pragma unroll
+  for (int s = 0; s < 5; s++)
+#pragma unroll
+    for (int a = 0; a < 4; a++) m.q[s][a] = b.quat[(s * 4 + a) * n + i];
 }
 
-// ---- kernel B --------------------------------------------------------------
-__device__ inline void correct_one(const ModelTable &mt, int n, int i, double dt,
-                                   double zeta0, const double *F, const double *tl,
-                                   const double *tmat, double *com, double *dx, double *w,
-                                   double *quat, double *ek) {
-#pragma clang fp contract(off)
-  double t[9];
-  for (int a = 0; a < 9; a++) t[a] = tmat[a * n + i];
-  const double tq[3] = {t[0] * tl[0] + t[3] * tl[1] + t[6] * tl[2],
-                        t[1] * tl[0] + t[4] * tl[1] + t[7] * tl[2],
-                        t[2] * tl[0] + t[5] * tl[1] + t[8] * tl[2]};
+__device__ inline void store_mol(const MdBuf &b, int n, int i, const Mol &m) {
+#pragma unroll
+  for (int a = 0; a < 3; a++) b.com[a * n + i] = m.com[a];
+#pragma unroll
+  for (int s = 0; s < 5; s++)
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      b.dx[(s * 3 + a) * n + i] = m.dx[s][a];
+      b.w[(s * 3 + a) * n + i] = m.w[s][a];
+    }
+#pragma unroll
+  for (int s = 0; s < 5; s++)
+#pragma unroll
+    for (int a = 0; a < 4; a++) b.quat[(s * 4 + a) * n + i] = m.q[s][a];
+}
+
+// rigid_predict (RigidModule.F90:480-559)
+__device__ inline void predict_mol(Mol &m) {
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    m.com[a] = m.com[a] + m.dx[0][a] + m.dx[1][a] + m.dx[2][a] + m.dx[3][a] + m.dx[4][a];
+    m.dx[0][a] = m.dx[0][a] + 2.0 * m.dx[1][a] + 3.0 * m.dx[2][a] + 4.0 * m.dx[3][a] +
+                 5.0 * m.dx[4][a];
+    m.dx[1][a] = m.dx[1][a] + 3.0 * m.dx[2][a] + 6.0 * m.dx[3][a] + 10.0 * m.dx[4][a];
+    m.dx[2][a] = m.dx[2][a] + 4.0 * m.dx[3][a] + 10.0 * m.dx[4][a];
+    m.dx[3][a] = m.dx[3][a] + 5.0 * m.dx[4][a];
+  }
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    m.w[0][a] = m.w[0][a] + m.w[1][a] + m.w[2][a] + m.w[3][a] + m.w[4][a];
+    m.w[1][a] = m.w[1][a] + 2.0 * m.w[2][a] + 3.0 * m.w[3][a] + 4.0 * m.w[4][a];
+    m.w[2][a] = m.w[2][a] + 3.0 * m.w[3][a] + 6.0 * m.w[4][a];
+    m.w[3][a] = m.w[3][a] + 4.0 * m.w[4][a];
+  }
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    m.q[0][a] = m.q[0][a] + m.q[1][a] + m.q[2][a] + m.q[3][a] + m.q[4][a];
+    m.q[1][a] = m.q[1][a] + 2.0 * m.q[2][a] + 3.0 * m.q[3][a] + 4.0 * m.q[4][a];
+  }
+  const double re = 1.0 / sqrt(m.q[0][0] * m.q[0][0] + m.q[0][1] * m.q[0][1] +
+                               m.q[0][2] * m.q[0][2] + m.q[0][3] * m.q[0][3]);
+#pragma unroll
+  for (int a = 0; a < 4; a++) m.q[0][a] = m.q[0][a] * re;
+}
+
+// rigid_correct (RigidModule.F90:1255-1487) with the Nose drag; F in the lab
+// frame, tq the body-frame torque; writes the Rigid_Ek terms to ek
+__device__ inline void correct_mol(const ModelTable &mt, double dt, double zeta0,
+                                   const double *F, const double *tq, Mol &m, double *ek) {
   const double nc = zeta0 * dt, ext = nc * 0.5, exr = nc;
   const double tf = dt * dt * 0.5 * (1.0 / mt.mass);
-#define DX(s, a) dx[((s) * 3 + (a)) * n + i]
-#define W(s, a) w[((s) * 3 + (a)) * n + i]
-#define Q(s, a) quat[((s) * 4 + (a)) * n + i]
+#pragma unroll
   for (int a = 0; a < 3; a++) {
     double nf = F[a] * tf;
-    nf = nf - DX(0, a) * ext;
-    const double d = nf - DX(1, a);
-    com[a * n + i] = com[a * n + i] + kG5_0 * d;
-    DX(0, a) = DX(0, a) + kG5_1 * d;
-    DX(1, a) = nf;
-    DX(2, a) = DX(2, a) + kG5_3 * d;
-    DX(3, a) = DX(3, a) + kG5_4 * d;
-    DX(4, a) = DX(4, a) + kG5_5 * d;
+    nf = nf - m.dx[0][a] * ext;
+    const double d = nf - m.dx[1][a];
+    m.com[a] = m.com[a] + kG5_0 * d;
+    m.dx[0][a] = m.dx[0][a] + kG5_1 * d;
+    m.dx[1][a] = nf;
+    m.dx[2][a] = m.dx[2][a] + kG5_3 * d;
+    m.dx[3][a] = m.dx[3][a] + kG5_4 * d;
+    m.dx[4][a] = m.dx[4][a] + kG5_5 * d;
   }
-  const double wx = W(0, 0), wy = W(0, 1), wz = W(0, 2);
-  double qa = Q(0, 0), qb = Q(0, 1), qc = Q(0, 2), qd = Q(0, 3);
+  const double wx = m.w[0][0], wy = m.w[0][1], wz = m.w[0][2];
+  double qa = m.q[0][0], qb = m.q[0][1], qc = m.q[0][2], qd = m.q[0][3];
   const double nwa = (-wx * qb + wy * qc - wz * qd) * dt * 0.5;
   const double nwb = (wx * qa - wy * qd - wz * qc) * dt * 0.5;
   const double nwc = (-wx * qd - wy * qa + wz * qb) * dt * 0.5;
   const double nwd = (wx * qc + wy * qb + wz * qa) * dt * 0.5;
-  const double dwa = nwa - Q(1, 0), dwb = nwb - Q(1, 1), dwc = nwc - Q(1, 2), dwd = nwd - Q(1, 3);
-  Q(0, 0) = Q(0, 0) + kG4_0 * dwa;
-  Q(0, 1) = Q(0, 1) + kG4_0 * dwb;
-  Q(0, 2) = Q(0, 2) + kG4_0 * dwc;
-  Q(0, 3) = Q(0, 3) + kG4_0 * dwd;
+  const double dwa = nwa - m.q[1][0], dwb = nwb - m.q[1][1], dwc = nwc - m.q[1][2],
+               dwd = nwd - m.q[1][3];
+  m.q[0][0] = m.q[0][0] + kG4_0 * dwa;
+  m.q[0][1] = m.q[0][1] + kG4_0 * dwb;
+  m.q[0][2] = m.q[0][2] + kG4_0 * dwc;
+  m.q[0][3] = m.q[0][3] + kG4_0 * dwd;
   double nf[3];
   nf[0] = (tq[0] + (mt.Iyy - mt.Izz) * wy * wz) * dt * (1.0 / mt.Ixx);
   nf[1] = (tq[1] + (mt.Izz - mt.Ixx) * wz * wx) * dt * (1.0 / mt.Iyy);
   nf[2] = (tq[2] + (mt.Ixx - mt.Iyy) * wx * wy) * dt * (1.0 / mt.Izz);
+#pragma unroll
   for (int a = 0; a < 3; a++) {
-    nf[a] = nf[a] - W(0, a) * exr;
-    const double d = nf[a] - W(1, a);
-    W(0, a) = W(0, a) + kG4_0 * d;
-    W(1, a) = nf[a];
-    W(2, a) = W(2, a) + kG4_2 * d;
-    W(3, a) = W(3, a) + kG4_3 * d;
-    W(4, a) = W(4, a) + kG4_4 * d;
+    nf[a] = nf[a] - m.w[0][a] * exr;
+    const double d = nf[a] - m.w[1][a];
+    m.w[0][a] = m.w[0][a] + kG4_0 * d;
+    m.w[1][a] = nf[a];
+    m.w[2][a] = m.w[2][a] + kG4_2 * d;
+    m.w[3][a] = m.w[3][a] + kG4_3 * d;
+    m.w[4][a] = m.w[4][a] + kG4_4 * d;
   }
-  const double w1x = W(0, 0), w1y = W(0, 1), w1z = W(0, 2);
-  const double w2x = W(1, 0), w2y = W(1, 1), w2z = W(1, 2);
-  const double w3x = W(2, 0), w3y = W(2, 1), w3z = W(2, 2);
-  const double w4x = W(3, 0), w4y = W(3, 1), w4z = W(3, 2);
-  qa = Q(0, 0); qb = Q(0, 1); qc = Q(0, 2); qd = Q(0, 3);
+  const double w1x = m.w[0][0], w1y = m.w[0][1], w1z = m.w[0][2];
+  const double w2x = m.w[1][0], w2y = m.w[1][1], w2z = m.w[1][2];
+  const double w3x = m.w[2][0], w3y = m.w[2][1], w3z = m.w[2][2];
+  const double w4x = m.w[3][0], w4y = m.w[3][1], w4z = m.w[3][2];
+  qa = m.q[0][0]; qb = m.q[0][1]; qc = m.q[0][2]; qd = m.q[0][3];
   const double pc = (-w1x * qd - w1y * qa + w1z * qb) * dt * 0.5;
   const double pb = (w1x * qa - w1y * qd - w1z * qc) * dt * 0.5;
   const double pd = (w1x * qc + w1y * qb + w1z * qa) * dt * 0.5;
   const double pa = (-w1x * qb + w1y * qc - w1z * qd) * dt * 0.5;
-  Q(1, 0) = pa; Q(1, 1) = pb; Q(1, 2) = pc; Q(1, 3) = pd;
+  m.q[1][0] = pa; m.q[1][1] = pb; m.q[1][2] = pc; m.q[1][3] = pd;
   const double rc_ = (-w2x * qd - w2y * qa + w2z * qb - w1x * pd - w1y * pa + w1z * pb) * dt * 0.25;
   const double rb = (w2x * qa - w2y * qd - w2z * qc + w1x * pa - w1y * pd - w1z * pc) * dt * 0.25;
   const double rd = (w2x * qc + w2y * qb + w2z * qa + w1x * pc + w1y * pb + w1z * pa) * dt * 0.25;
   const double ra = (-w2x * qb + w2y * qc - w2z * qd - w1x * pb + w1y * pc - w1z * pd) * dt * 0.25;
-  Q(2, 0) = ra; Q(2, 1) = rb; Q(2, 2) = rc_; Q(2, 3) = rd;
+  m.q[2][0] = ra; m.q[2][1] = rb; m.q[2][2] = rc_; m.q[2][3] = rd;
   const double sc = (-w3x * qd - w3y * qa + w3z * qb - w1x * rd - w1y * ra + w1z * rb - w2x * pd -
                      w2y * pa + w2z * pb) * dt * (1.0 / 6.0);
   const double sb = (w3x * qa - w3y * qd - w3z * qc + w1x * ra - w1y * rd - w1z * rc_ + w2x * pa -
@@ -210,28 +186,175 @@ __device__ inline void correct_one(const ModelTable &mt, int n, int i, double dt
                      w2y * pb + w2z * pa) * dt * (1.0 / 6.0);
   const double sa = (-w3x * qb + w3y * qc - w3z * qd - w1x * rb + w1y * rc_ - w1z * rd - w2x * pb +
                      w2y * pc - w2z * pd) * dt * (1.0 / 6.0);
-  Q(3, 0) = sa; Q(3, 1) = sb; Q(3, 2) = sc; Q(3, 3) = sd;
-  Q(4, 2) = (-w4x * qd - w4y * qa + w4z * qb - w3x * pd - w3y * pa + w3z * pb - w2x * rd -
-             w2y * ra + w2z * rb - w1x * sd - w1y * sa + w1z * sb) * dt * (1.0 / 8.0);
-  Q(4, 1) = (w4x * qa - w4y * qd - w4z * qc + w3x * pa - w3y * pd - w3z * pc + w2x * ra -
-             w2y * rd - w2z * rc_ + w1x * sa - w1y * sd - w1z * sc) * dt * (1.0 / 8.0);
-  Q(4, 3) = (w4x * qc + w4y * qb + w4z * qa + w3x * pc + w3y * pb + w3z * pa + w2x * rc_ +
-             w2y * rb + w2z * ra + w1x * sc + w1y * sb + w1z * sa) * dt * (1.0 / 8.0);
-  Q(4, 0) = (-w4x * qb + w4y * qc - w4z * qd - w3x * pb + w3y * pc - w3z * pd - w2x * rb +
-             w2y * rc_ - w2z * rd - w1x * sb + w1y * sc - w1z * sd) * dt * (1.0 / 8.0);
-  ek[0] = DX(0, 0) * DX(0, 0) + DX(0, 1) * DX(0, 1) + DX(0, 2) * DX(0, 2);
-  ek[1] = W(0, 0) * W(0, 0);
-  ek[2] = W(0, 1) * W(0, 1);
-  ek[3] = W(0, 2) * W(0, 2);
-#undef DX
-#undef W
-#undef Q
+  m.q[3][0] = sa; m.q[3][1] = sb; m.q[3][2] = sc; m.q[3][3] = sd;
+  m.q[4][2] = (-w4x * qd - w4y * qa + w4z * qb - w3x * pd - w3y * pa + w3z * pb - w2x * rd -
+               w2y * ra + w2z * rb - w1x * sd - w1y * sa + w1z * sb) * dt * (1.0 / 8.0);
+  m.q[4][1] = (w4x * qa - w4y * qd - w4z * qc + w3x * pa - w3y * pd - w3z * pc + w2x * ra -
+               w2y * rd - w2z * rc_ + w1x * sa - w1y * sd - w1z * sc) * dt * (1.0 / 8.0);
+  m.q[4][3] = (w4x * qc + w4y * qb + w4z * qa + w3x * pc + w3y * pb + w3z * pa + w2x * rc_ +
+               w2y * rb + w2z * ra + w1x * sc + w1y * sb + w1z * sa) * dt * (1.0 / 8.0);
+  m.q[4][0] = (-w4x * qb + w4y * qc - w4z * qd - w3x * pb + w3y * pc - w3z * pd - w2x * rb +
+               w2y * rc_ - w2z * rd - w1x * sb + w1y * sc - w1z * sd) * dt * (1.0 / 8.0);
+  ek[0] = m.dx[0][0] * m.dx[0][0] + m.dx[0][1] * m.dx[0][1] + m.dx[0][2] * m.dx[0][2];
+  ek[1] = m.w[0][0] * m.w[0][0];
+  ek[2] = m.w[0][1] * m.w[0][1];
+  ek[3] = m.w[0][2] * m.w[0][2];
 }
 
-// neighbour-row build of molecule i into nbr[i*maxnb ...], in cell-stencil
-// order; returns the row length (also stored).
+// ---- kernel A --------------------------------------------------------------
+// One thread per molecule.  do_correct: rigid_correct with the forces of the
+// previous k_md_main (b.frc), kinetic partials, and in the last block the
+// step's thermo record and Nose_Correct.  do_predict: rigid_predict (then
+// Nose_Predict in the last block), sites, binning, skin test.
+__global__ __launch_bounds__(64) void k_md_pre(ModelTable mt, Geom g, MdBuf b, double dt,
+                                               int nose_active, int do_correct,
+                                               int do_predict) {
+  const int n = g.n;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double ek[4] = {0, 0, 0, 0};
+  if (i < n) {
+    Mol m;
+    load_mol(b, n, i, m);
+    if (do_correct) {
+      double t[9], F[3], tl[3];
+#pragma unroll
+      for (int a = 0; a < 9; a++) t[a] = b.tmat[a * n + i];
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        F[a] = b.frc[a * n + i];
+        tl[a] = b.frc[(3 + a) * n + i];
+      }
+      // rigid_collectforce's body-frame torque (RigidModule.F90:301-306)
+      const double tq[3] = {t[0] * tl[0] + t[3] * tl[1] + t[6] * tl[2],
+                            t[1] * tl[0] + t[4] * tl[1] + t[7] * tl[2],
+                            t[2] * tl[0] + t[5] * tl[1] + t[8] * tl[2]};
+      correct_mol(mt, dt, nose_active ? b.nose[3] : 0.0, F, tq, m, ek);
+    }
+    if (do_predict) {
+      predict_mol(m);
+      store_mol(b, n, i, m);
+#pragma unroll
+      for (int a = 0; a < 3; a++) b.com_p[a * n + i] = m.com[a];
+      const double qa = m.q[0][0], qb = m.q[0][1], qc = m.q[0][2], qd = m.q[0][3];
+      double t[9];
+      t[0] = (qa * qa + qb * qb - (qc * qc + qd * qd));
+      t[1] = -2.0 * (qa * qd + qb * qc);
+      t[2] = 2.0 * (qb * qd - qa * qc);
+      t[3] = 2.0 * (qa * qd - qb * qc);
+      t[4] = qa * qa + qc * qc - (qb * qb + qd * qd);
+      t[5] = -2.0 * (qa * qb + qc * qd);
+      t[6] = 2.0 * (qa * qc + qb * qd);
+      t[7] = 2.0 * (qa * qb - qc * qd);
+      t[8] = qa * qa + qd * qd - (qb * qb + qc * qc);
+#pragma unroll
+      for (int a = 0; a < 9; a++) b.tmat[a * n + i] = t[a];
+      for (int s = 0; s < mt.nsite; s++) {
+        const double r0 = t[0] * mt.bx[s] + t[1] * mt.by[s] + t[2] * mt.bz[s];
+        const double r1 = t[3] * mt.bx[s] + t[4] * mt.by[s] + t[5] * mt.bz[s];
+        const double r2 = t[6] * mt.bx[s] + t[7] * mt.by[s] + t[8] * mt.bz[s];
+        b.intra[(s * 3 + 0) * n + i] = r0;
+        b.intra[(s * 3 + 1) * n + i] = r1;
+        b.intra[(s * 3 + 2) * n + i] = r2;
+        b.intraf[(s * 3 + 0) * n + i] = (float)r0;
+        b.intraf[(s * 3 + 1) * n + i] = (float)r1;
+        b.intraf[(s * 3 + 2) * n + i] = (float)r2;
+      }
+      int jc[3];
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        double d = m.com[a];
+        const double o = round(d * g.invL) * g.L;
+        d = d - o;
+        int j = (int)round(d * g.cellinv);
+        if (j < 0) j += g.ndiv;
+        j = ((j % g.ndiv) + g.ndiv) % g.ndiv;  // acts only for ndiv = 1
+        jc[a] = j;
+      }
+      const int c = (jc[2] * g.ndiv + jc[1]) * g.ndiv + jc[0];
+      b.cell[i] = c;
+      const int slot = atomicAdd(&b.cellcnt[c], 1);
+      if (slot < b.cell_cap) b.cellres[(size_t)c * b.cell_cap + slot] = i;
+      else b.flags[0] = 3;  // cell capacity exceeded
+      if (b.half_skin2 > 0.0) {
+        const double e0 = m.com[0] - b.com_ref[i], e1 = m.com[1] - b.com_ref[n + i],
+                     e2 = m.com[2] - b.com_ref[2 * n + i];
+        if (e0 * e0 + e1 * e1 + e2 * e2 > b.half_skin2) b.flags[1] = 1;
+      }
+    } else if (do_correct) {
+      store_mol(b, n, i, m);
+    }
+  }
+  if (!do_correct && !(do_predict && nose_active)) return;
+  // kinetic partials of this block, then the last block closes the step
+  __shared__ int s_last;
+#pragma unroll
+  for (int q = 0; q < 4; q++) ek[q] = wsum(ek[q]);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) b.blockek[(size_t)blockIdx.x * 4 + q] = ek[q];
+  }
+  __threadfence();
+  if (threadIdx.x == 0) s_last = (atomicAdd(b.done + 1, 1) == (int)gridDim.x - 1);
+  __syncthreads();
+  if (!s_last) return;
+  __threadfence();
+  double a2[4] = {0, 0, 0, 0};
+  for (int k = threadIdx.x; k < (int)gridDim.x; k += 64)
+#pragma unroll
+    for (int q = 0; q < 4; q++) a2[q] += b.blockek[(size_t)k * 4 + q];
+#pragma unroll
+  for (int q = 0; q < 4; q++) a2[q] = wsum(a2[q]);
+  if (threadIdx.x == 0) {
+    double *z = b.nose;
+    if (do_correct) {
+      double *th = b.thermo;
+      const double ep = b.epvir[0];
+      const double ekt = a2[0] * mt.mass * 0.5 * (1.0 / (dt * dt));
+      const double ekr = (a2[1] * mt.Ixx + a2[2] * mt.Iyy + a2[3] * mt.Izz) * 0.5;
+      const double dof = 6.0 * n;
+      const double temp = 10.0 * 2.0 * (ekt + ekr) / (G6_RGAS_MD * dof);
+      double h = (ep + ekt + ekr) * 10.0;
+      if (nose_active) {
+        const double zetaf = (temp - z[0]) / z[1] - z[2] * z[3];
+        const double dz2 = zetaf - z[4];
+        z[3] = z[3] + kG4_0 * dz2;
+        z[4] = zetaf;
+        z[5] = z[5] + kG4_2 * dz2;
+        z[6] = z[6] + kG4_3 * dz2;
+        z[7] = z[7] + kG4_4 * dz2;
+        z[8] = z[8] + z[3];
+        h = h + (dof * z[0] * (z[8] - 0.5 * z[3]) * dt) * G6_RGAS_MD +
+            (z[3] * z[3] * z[1] * dt * dof * 0.5) * G6_RGAS_MD;
+      }
+      th[0] = ep;
+      th[1] = ekt;
+      th[2] = ekr;
+      th[3] = temp;
+      for (int q = 0; q < 9; q++) th[4 + q] = b.epvir[1 + q];
+      th[13] = h;
+      th[14] = b.epvir[10];
+      th[15] = b.epvir[11];
+    }
+    if (do_predict && nose_active) {
+      z[3] = z[3] + z[4] + z[5] + z[6] + z[7];
+      z[4] = z[4] + 2.0 * z[5] + 3.0 * z[6] + 4.0 * z[7];
+      z[5] = z[5] + 3.0 * z[6] + 6.0 * z[7];
+      z[6] = z[6] + 4.0 * z[7];
+    }
+    b.done[1] = 0;
+  }
+}
+
+// ---- kernel B --------------------------------------------------------------
+
+constexpr int kRowLds = 512;  // rows up to this length are sorted in LDS
+
+// neighbour-row build of molecule i into nbr[i*maxnb ...], sorted by j so
+// that the force sums do not depend on the order in which the binning
+// atomics filled the cells; returns the row length (also stored).
 __device__ inline int build_row(const Geom &g, const MdBuf &b, int i, int lane,
-                                int wid, int (*s_beg)[32], int (*s_pre)[33]) {
+                                int wid, int (*s_beg)[32], int (*s_pre)[33],
+                                int (*s_j)[kRowLds], int (*s_c)[kRowLds]) {
   const int n = g.n;
   const int c = b.cell[i];
   const int nd = g.ndiv;
@@ -270,11 +393,10 @@ __device__ inline int build_row(const Geom &g, const MdBuf &b, int i, int lane,
       if (j != i) {
         double d0 = xi - b.com_p[j], d1 = yi - b.com_p[n + j], d2 = zi - b.com_p[2 * n + j];
         const double k0 = round(d0 * g.invL), k1 = round(d1 * g.invL), k2 = round(d2 * g.invL);
-        d0 = __dsub_rn(d0, __dmul_rn(k0, g.L));
-        d1 = __dsub_rn(d1, __dmul_rn(k1, g.L));
-        d2 = __dsub_rn(d2, __dmul_rn(k2, g.L));
-        const double dd = __dadd_rn(__dadd_rn(__dmul_rn(d0, d0), __dmul_rn(d1, d1)),
-                                    __dmul_rn(d2, d2));
+        d0 = d0 - k0 * g.L;
+        d1 = d1 - k1 * g.L;
+        d2 = d2 - k2 * g.L;
+        const double dd = d0 * d0 + d1 * d1 + d2 * d2;
         pass = dd < g.rr;
         const int f0 = (k0 == 0.0 && signbit(k0)) ? 0 : (int)k0 + 512;
         const int f1 = (k1 == 0.0 && signbit(k1)) ? 0 : (int)k1 + 512;
@@ -289,6 +411,10 @@ __device__ inline int build_row(const Geom &g, const MdBuf &b, int i, int lane,
       if (pos < b.maxnb) {
         nj[pos] = j;
         nc[pos] = code;
+        if (pos < kRowLds) {
+          s_j[wid][pos] = j;
+          s_c[wid][pos] = code;
+        }
       } else {
         b.flags[0] = 4;  // neighbour-row capacity exceeded
       }
@@ -297,6 +423,17 @@ __device__ inline int build_row(const Geom &g, const MdBuf &b, int i, int lane,
   }
   wsync();
   const int m = min(base, b.maxnb);
+  if (m <= kRowLds) {
+    // rank sort by j (each j occurs once in a row)
+    for (int t = lane; t < m; t += 64) {
+      const int key = s_j[wid][t];
+      int rank = 0;
+      for (int u = 0; u < m; u++) rank += s_j[wid][u] < key;
+      nj[rank] = key;
+      nc[rank] = s_c[wid][t];
+    }
+  }
+  wsync();
   if (lane == 0) {
     b.nbr_cnt[i] = m;
     b.com_ref[i] = xi;
@@ -307,24 +444,24 @@ __device__ inline int build_row(const Geom &g, const MdBuf &b, int i, int lane,
 }
 
 template <int M, typename R>
-__global__ __launch_bounds__(256) void k_md_main(ModelTable mt, Geom g, MdBuf b,
-                                                 int nose_active, double dt, double *out) {
+__global__ __launch_bounds__(256) void k_md_main(ModelTable mt, Geom g, MdBuf b, double *out) {
   using T = Traits<M>;
   __shared__ int s_beg[4][32], s_pre[4][33];
-  __shared__ double s_red[4][16];
+  __shared__ int s_j[4][kRowLds], s_c[4][kRowLds];
+  __shared__ double s_red[4][13];
   __shared__ int s_last;
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int n = g.n;
   const int i = blockIdx.x * 4 + wid;
   const bool active = i < n;
-  double acc[16];  // ep, vir[9], npair, ek[4], list pairs
+  double acc[13];  // ep, vir[9], interacting pairs, list pairs (each halved), spare
 #pragma unroll
-  for (int q = 0; q < 16; q++) acc[q] = 0.0;
+  for (int q = 0; q < 13; q++) acc[q] = 0.0;
   if (active) {
     int m;
     if (b.flags[1]) {
-      m = build_row(g, b, i, lane, wid, s_beg, s_pre);
-      acc[15] = 0.5 * m;
+      m = build_row(g, b, i, lane, wid, s_beg, s_pre, s_j, s_c);
+      acc[11] = 0.5 * m;
     } else {
       m = b.nbr_cnt[i];
     }
@@ -346,11 +483,10 @@ __global__ __launch_bounds__(256) void k_md_main(ModelTable mt, Geom g, MdBuf b,
       const double ox = fa == 0 ? -0.0 : (double)(fa - 512) * g.L;
       const double oy = fb == 0 ? -0.0 : (double)(fb - 512) * g.L;
       const double oz = fc == 0 ? -0.0 : (double)(fc - 512) * g.L;
-      const double d0 = __dsub_rn(__dsub_rn(xi, b.com_p[j]), ox);
-      const double d1 = __dsub_rn(__dsub_rn(yi, b.com_p[n + j]), oy);
-      const double d2 = __dsub_rn(__dsub_rn(zi, b.com_p[2 * n + j]), oz);
-      const double dd64 = __dadd_rn(__dadd_rn(__dmul_rn(d0, d0), __dmul_rn(d1, d1)),
-                                    __dmul_rn(d2, d2));
+      const double d0 = (xi - b.com_p[j]) - ox;
+      const double d1 = (yi - b.com_p[n + j]) - oy;
+      const double d2 = (zi - b.com_p[2 * n + j]) - oz;
+      const double dd64 = d0 * d0 + d1 * d1 + d2 * d2;
       if (!(dd64 < g.rc2)) continue;
       R rj[T::NS][3];
 #pragma unroll
@@ -388,7 +524,7 @@ __global__ __launch_bounds__(256) void k_md_main(ModelTable mt, Geom g, MdBuf b,
     acc[10] = 0.5 * wsum(npi);
     if (lane == 0) {
       if (out) {
-        // eval mode: molecule force (lab) and body torque, state untouched
+        // molecule force (lab) and body torque, as rigid_collectforce
         double t[9];
         for (int a = 0; a < 9; a++) t[a] = b.tmat[a * n + i];
         out[i] = F[0];
@@ -398,86 +534,68 @@ __global__ __launch_bounds__(256) void k_md_main(ModelTable mt, Geom g, MdBuf b,
         out[4 * n + i] = t[1] * Ti[0] + t[4] * Ti[1] + t[7] * Ti[2];
         out[5 * n + i] = t[2] * Ti[0] + t[5] * Ti[1] + t[8] * Ti[2];
       } else {
-        double ek[4];
-        correct_one(mt, n, i, dt, nose_active ? b.nose[3] : 0.0, F, Ti, b.tmat, b.com, b.dx,
-                    b.w, b.quat, ek);
-        for (int q = 0; q < 4; q++) acc[11 + q] = ek[q];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+          b.frc[a * n + i] = F[a];
+          b.frc[(3 + a) * n + i] = Ti[a];
+        }
       }
     }
   }
   // block partials: lane 0 of each wave holds the wave's values
   if (lane == 0)
-    for (int q = 0; q < 16; q++) s_red[wid][q] = acc[q];
+    for (int q = 0; q < 12; q++) s_red[wid][q] = acc[q];
   __syncthreads();
-  if (threadIdx.x < 16) {
+  if (threadIdx.x < 12) {
     const int q = threadIdx.x;
     b.blockpart[(size_t)blockIdx.x * 16 + q] = s_red[0][q] + s_red[1][q] + s_red[2][q] + s_red[3][q];
   }
-  // last block to finish closes the step
+  // the last block to finish reduces the partials in a fixed order
   __threadfence();
   __syncthreads();
   if (threadIdx.x == 0) s_last = (atomicAdd(b.done, 1) == (int)gridDim.x - 1);
   __syncthreads();
   if (!s_last) return;
   __threadfence();
-  __shared__ double red[16][256];
-  double a2[16];
-  for (int q = 0; q < 16; q++) a2[q] = 0.0;
+  double a2[12];
+#pragma unroll
+  for (int q = 0; q < 12; q++) a2[q] = 0.0;
   for (int k = threadIdx.x; k < (int)gridDim.x; k += 256)
-    for (int q = 0; q < 16; q++) a2[q] += b.blockpart[(size_t)k * 16 + q];
-  for (int q = 0; q < 16; q++) red[q][threadIdx.x] = a2[q];
+#pragma unroll
+    for (int q = 0; q < 12; q++) a2[q] += b.blockpart[(size_t)k * 16 + q];
+#pragma unroll
+  for (int q = 0; q < 12; q++) a2[q] = wsum(a2[q]);
+  __syncthreads();
+  if (lane == 0)
+    for (int q = 0; q < 12; q++) s_red[wid][q] = a2[q];
   // the cell lists are consumed: clear the counts for the next step
   for (int c = threadIdx.x; c <= g.ncell; c += 256) b.cellcnt[c] = 0;
   __syncthreads();
-  for (int d = 128; d >= 1; d >>= 1) {
-    if ((int)threadIdx.x < d)
-      for (int q = 0; q < 16; q++) red[q][threadIdx.x] += red[q][threadIdx.x + d];
-    __syncthreads();
-  }
   if (threadIdx.x == 0) {
-#pragma clang fp contract(off)
-    double *th = b.thermo;
-    const double ep = red[0][0];
-    const double ekt = red[11][0] * mt.mass * 0.5 * (1.0 / (dt * dt));
-    const double ekr = (red[12][0] * mt.Ixx + red[13][0] * mt.Iyy + red[14][0] * mt.Izz) * 0.5;
-    const double dof = 6.0 * n;
-    const double temp = 10.0 * 2.0 * (ekt + ekr) / (G6_RGAS_MD * dof);
-    double h = (ep + ekt + ekr) * 10.0;
-    if (nose_active && !out) {
-      double *z = b.nose;
-      const double zetaf = (temp - z[0]) / z[1] - z[2] * z[3];
-      const double dz2 = zetaf - z[4];
-      z[3] = z[3] + kG4_0 * dz2;
-      z[4] = zetaf;
-      z[5] = z[5] + kG4_2 * dz2;
-      z[6] = z[6] + kG4_3 * dz2;
-      z[7] = z[7] + kG4_4 * dz2;
-      z[8] = z[8] + z[3];
-      h = h + (dof * z[0] * (z[8] - 0.5 * z[3]) * dt) * G6_RGAS_MD +
-          (z[3] * z[3] * z[1] * dt * dof * 0.5) * G6_RGAS_MD;
+    for (int q = 0; q < 11; q++)
+      b.epvir[q] = s_red[0][q] + s_red[1][q] + s_red[2][q] + s_red[3][q];
+    if (b.flags[1]) b.epvir[11] = s_red[0][11] + s_red[1][11] + s_red[2][11] + s_red[3][11];
+    if (out) {
+      // eval mode: the thermo record carries this state's energy terms
+      for (int q = 0; q < 9; q++) b.thermo[4 + q] = b.epvir[1 + q];
+      b.thermo[0] = b.epvir[0];
+      b.thermo[14] = b.epvir[10];
+      b.thermo[15] = b.epvir[11];
     }
-    th[0] = ep;
-    th[1] = ekt;
-    th[2] = ekr;
-    th[3] = temp;
-    for (int q = 0; q < 9; q++) th[4 + q] = red[1 + q][0];
-    th[13] = h;
-    th[14] = red[10][0];
-    if (b.flags[1]) th[15] = red[15][0];
     if (b.half_skin2 > 0.0) b.flags[1] = 0;
     *b.done = 0;
   }
 }
 
-void md_launch_pre(System &s, bool predict) {
-  k_md_pre<<<(s.n + 63) / 64, 64, 0, s.stream>>>(s.mt, s.geo_md, s.md, s.nose_active ? 1 : 0,
+void md_launch_pre(System &s, bool correct, bool predict) {
+  k_md_pre<<<(s.n + 63) / 64, 64, 0, s.stream>>>(s.mt, s.geo_md, s.md, s.dt,
+                                                  s.nose_active ? 1 : 0, correct ? 1 : 0,
                                                   predict ? 1 : 0);
 }
 
 template <int M, typename R>
 static void md_main_t(System &s, double *out) {
-  k_md_main<M, R><<<s.nblk_md, 256, 0, s.stream>>>(s.mt, s.geo_md, s.md, s.nose_active ? 1 : 0,
-                                                   s.dt, out);
+  k_md_main<M, R><<<s.nblk_md, 256, 0, s.stream>>>(s.mt, s.geo_md, s.md, out);
 }
 
 void md_launch_main(System &s, double *out) {
