@@ -276,10 +276,10 @@ static void free_all(System &s) {
 // Each call is self-contained, so a captured graph of k steps can be replayed.
 static void launch_steps(System &s, int k) {
   for (int q = 0; q < k; q++) {
-    md_launch_pre(s, q > 0, true);
+    md_launch_pre(s, q > 0, true, true);
     md_launch_main(s, nullptr);
   }
-  md_launch_pre(s, true, false);
+  md_launch_pre(s, true, false, false);
 }
 
 // steps per captured hipGraph (G6_GRAPH_STEPS; 0 disables graphs)
@@ -534,11 +534,11 @@ int g6_step_timed(g6_handle h, int nsteps, double *t) {
     // one step: predict, forces, correct (the corrector and the next
     // predictor share a kernel in g6_step; here they are timed apart)
     G6_CHECK(hipEventRecord(s->ev[0], s->stream));
-    md_launch_pre(*s, false, true);
+    md_launch_pre(*s, false, true, true);
     G6_CHECK(hipEventRecord(s->ev[1], s->stream));
     md_launch_main(*s, nullptr);
     G6_CHECK(hipEventRecord(s->ev[2], s->stream));
-    md_launch_pre(*s, true, false);
+    md_launch_pre(*s, true, false, false);
     G6_CHECK(hipEventRecord(s->ev[3], s->stream));
     G6_CHECK(hipEventSynchronize(s->ev[3]));
     float a, b, c, e;
@@ -569,7 +569,7 @@ int g6_eval_forces(g6_handle h, double *force, double *torque, double *thermo) {
   // the thermostat is not advanced
   int one = 1;
   G6_CHECK(hipMemcpyAsync(s->md.flags + 1, &one, sizeof(int), hipMemcpyHostToDevice, s->stream));
-  md_launch_pre(*s, false, false);
+  md_launch_pre(*s, false, false, true);
   md_launch_main(*s, out);
   G6_CHECK(hipGetLastError());
   G6_CHECK(hipMemcpyAsync(force, out, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s->stream));

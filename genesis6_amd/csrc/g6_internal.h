@@ -114,7 +114,7 @@ struct System {
 bool make_model_table(int model, int prec, ModelTable *mt);
 
 // MD launchers (stream-ordered, graph-capturable: no memset, no host sync)
-void md_launch_pre(System &s, bool correct, bool predict);
+void md_launch_pre(System &s, bool correct, bool predict, bool sites);
 void md_launch_main(System &s, double *out);
 // host-array entry points of the module seam
 int pairlist_once(System &s, const double *com_h, int64_t cap, int64_t *npair0,

@@ -205,10 +205,11 @@ __device__ inline void correct_mol(const ModelTable &mt, double dt, double zeta0
 // One thread per molecule.  do_correct: rigid_correct with the forces of the
 // previous k_md_main (b.frc), kinetic partials, and in the last block the
 // step's thermo record and Nose_Correct.  do_predict: rigid_predict (then
-// Nose_Predict in the last block), sites, binning, skin test.
+// Nose_Predict in the last block).  do_sites: predicted COMs, rotation
+// matrices, site offsets, binning and the skin test for k_md_main.
 __global__ __launch_bounds__(64) void k_md_pre(ModelTable mt, Geom g, MdBuf b, double dt,
                                                int nose_active, int do_correct,
-                                               int do_predict) {
+                                               int do_predict, int do_sites) {
   const int n = g.n;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double ek[4] = {0, 0, 0, 0};
@@ -230,9 +231,9 @@ __global__ __launch_bounds__(64) void k_md_pre(ModelTable mt, Geom g, MdBuf b, d
                             t[2] * tl[0] + t[5] * tl[1] + t[8] * tl[2]};
       correct_mol(mt, dt, nose_active ? b.nose[3] : 0.0, F, tq, m, ek);
     }
-    if (do_predict) {
-      predict_mol(m);
-      store_mol(b, n, i, m);
+    if (do_predict) predict_mol(m);
+    if (do_predict || do_correct) store_mol(b, n, i, m);
+    if (do_sites) {
 #pragma unroll
       for (int a = 0; a < 3; a++) b.com_p[a * n + i] = m.com[a];
       const double qa = m.q[0][0], qb = m.q[0][1], qc = m.q[0][2], qd = m.q[0][3];
@@ -280,8 +281,6 @@ __global__ __launch_bounds__(64) void k_md_pre(ModelTable mt, Geom g, MdBuf b, d
                      e2 = m.com[2] - b.com_ref[2 * n + i];
         if (e0 * e0 + e1 * e1 + e2 * e2 > b.half_skin2) b.flags[1] = 1;
       }
-    } else if (do_correct) {
-      store_mol(b, n, i, m);
     }
   }
   if (!do_correct && !(do_predict && nose_active)) return;
@@ -587,10 +586,10 @@ __global__ __launch_bounds__(256) void k_md_main(ModelTable mt, Geom g, MdBuf b,
   }
 }
 
-void md_launch_pre(System &s, bool correct, bool predict) {
+void md_launch_pre(System &s, bool correct, bool predict, bool sites) {
   k_md_pre<<<(s.n + 63) / 64, 64, 0, s.stream>>>(s.mt, s.geo_md, s.md, s.dt,
                                                   s.nose_active ? 1 : 0, correct ? 1 : 0,
-                                                  predict ? 1 : 0);
+                                                  predict ? 1 : 0, sites ? 1 : 0);
 }
 
 template <int M, typename R>
