@@ -6,6 +6,9 @@ FP32 pair arithmetic with FP64 accumulation; synthetic seeded lattice start
 with uniform random orientations and Maxwell-Boltzmann velocities.  One
 "step" is one full MD step (predict, cell grid + pair list, forces,
 collect, correct, thermostat), all resident in HBM and replayed as hipGraphs.
+Neighbour rows carry a 1.0 A Verlet skin by default and are rebuilt when a
+molecule has moved half of it; the FP64 rc test on every listed pair keeps
+the interacting set the reference's (rows rebuilt every step) exactly.
 
 Prints ONE JSON line (rank 0, N=1).  The FP64 GPU path on the same workload
 is timed too and reported beside it.
@@ -71,7 +74,7 @@ def cpu_baseline(com, q, budget_s):
 def traffic_from_profile():
     """HBM bytes per force-kernel launch from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if present."""
-    p = os.path.join(ROOT, "profiles", "force_rows_tip4p4096_traffic.json")
+    p = os.path.join(ROOT, "profiles", "md_main_tip4p4096_traffic.json")
     try:
         return json.load(open(p))["bytes_per_launch"]
     except Exception:
@@ -83,7 +86,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--skin", type=float, default=float(os.environ.get("G6_SKIN", "0")))
+    ap.add_argument("--skin", type=float, default=1.0,
+                    help="Verlet skin in A (0: rows rebuilt every step, as the reference)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
@@ -148,10 +152,10 @@ def main():
         "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": FP32_PEAK / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / FP32_PEAK,
                      "traffic": traffic_from_profile(),
-                     "kernel": "k_force_rows<TIP4P,float>",
+                     "kernel": "k_md_main<TIP4P,float>",
                      "kernel_ms": t[0], "flop_per_pair": FLOP_PER_PAIR,
                      "pairs_per_launch": float(npair),
-                     "pairlist_ms": t[1], "per_molecule_kernels_ms": t[2],
+                     "per_molecule_kernels_ms": t[2],
                      "eager_step_ms": t[3]},
         "cpu_baseline": cpu,
     }

@@ -204,16 +204,14 @@ static int alloc_all(System &s) {
   AL(m.intraf, sizeof(float) * 3 * ns * n);
   AL(m.com_ref, sizeof(double) * 3 * n);
   AL(m.cell, sizeof(int) * n);
-  AL(m.cellcnt, sizeof(int) * (ncm + 1));
-  AL(m.cellres, sizeof(int) * (size_t)ncm * m.cell_cap);
+  AL(m.cellcnt, sizeof(int) * 2 * (ncm + 1));
+  AL(m.cellres, sizeof(int) * 2 * (size_t)ncm * m.cell_cap);
   AL(m.nbr_j, sizeof(int) * (size_t)n * m.maxnb);
   AL(m.nbr_code, sizeof(int) * (size_t)n * m.maxnb);
   AL(m.nbr_cnt, sizeof(int) * n);
   AL(m.blockpart, sizeof(double) * 16 * s.nblk_md);
   AL(m.blockek, sizeof(double) * 4 * ((n + 63) / 64));
   AL(m.frc, sizeof(double) * 6 * n);
-  AL(m.epvir, sizeof(double) * 16);
-  AL(m.done, sizeof(int) * 4);
   AL(m.nose, sizeof(double) * 9);
   AL(m.thermo, sizeof(double) * G6_THERMO_LEN);
   AL(m.flags, sizeof(int) * 8);
@@ -241,8 +239,6 @@ static int alloc_all(System &s) {
   G6_CHECK(hipMemset(m.quat, 0, sizeof(double) * 20 * n));
   G6_CHECK(hipMemset(m.nbr_cnt, 0, sizeof(int) * n));
   G6_CHECK(hipMemset(m.com_ref, 0, sizeof(double) * 3 * n));
-  G6_CHECK(hipMemset(m.epvir, 0, sizeof(double) * 16));
-  G6_CHECK(hipMemset(m.done, 0, sizeof(int) * 4));
   G6_CHECK(hipMemset(m.nose, 0, sizeof(double) * 9));
   G6_CHECK(hipMemset(m.thermo, 0, sizeof(double) * G6_THERMO_LEN));
   return 0;
@@ -250,10 +246,9 @@ static int alloc_all(System &s) {
 
 // neighbour rows and cell lists ready for a fresh first step
 static int reset_step_state(System &s) {
-  int fl[8] = {0, 1, 0, 0, 0, 0, 0, 0};
+  int fl[8] = {0, 0, 1, 1, 0, 0, 0, 0};
   G6_CHECK(hipMemcpy(s.md.flags, fl, sizeof(fl), hipMemcpyHostToDevice));
-  G6_CHECK(hipMemset(s.md.cellcnt, 0, sizeof(int) * (s.geo_md.ncell + 1)));
-  G6_CHECK(hipMemset(s.md.done, 0, sizeof(int) * 4));
+  G6_CHECK(hipMemset(s.md.cellcnt, 0, sizeof(int) * 2 * (s.geo_md.ncell + 1)));
   return 0;
 }
 
@@ -261,8 +256,7 @@ static void free_all(System &s) {
   const MdBuf &m = s.md;
   void *ps[] = {m.com, m.com_p, m.dx, m.w, m.quat, m.tmat, m.intra, m.intraf, m.com_ref,
                 m.cell, m.cellcnt, m.cellres, m.nbr_j, m.nbr_code, m.nbr_cnt, m.blockpart,
-                m.blockek, m.frc, m.epvir,
-                m.done, m.nose, m.thermo, m.flags, s.cell, s.cell_count, s.cell_start,
+                m.blockek, m.frc, m.nose, m.thermo, m.flags, s.cell, s.cell_count, s.cell_start,
                 s.cell_fill, s.sorted_tmp, s.sorted, s.cnt, s.off, s.pi, s.pj, s.scode,
                 s.com_ref, s.lflags};
   for (void *p : ps)
@@ -276,10 +270,13 @@ static void free_all(System &s) {
 // Each call is self-contained, so a captured graph of k steps can be replayed.
 static void launch_steps(System &s, int k) {
   for (int q = 0; q < k; q++) {
-    md_launch_pre(s, q > 0, true, true);
-    md_launch_main(s, nullptr);
+    const int p = q & 1;
+    md_launch_pre(s, q > 0, true, true, p);
+    md_launch_close(s, (q > 0 ? 1 : 0) | 2 | 4, p);
+    md_launch_main(s, nullptr, p);
   }
-  md_launch_pre(s, true, false, false);
+  md_launch_pre(s, true, false, false, 0);
+  md_launch_close(s, 1 | 8, 0);
 }
 
 // steps per captured hipGraph (G6_GRAPH_STEPS; 0 disables graphs)
@@ -534,11 +531,13 @@ int g6_step_timed(g6_handle h, int nsteps, double *t) {
     // one step: predict, forces, correct (the corrector and the next
     // predictor share a kernel in g6_step; here they are timed apart)
     G6_CHECK(hipEventRecord(s->ev[0], s->stream));
-    md_launch_pre(*s, false, true, true);
+    md_launch_pre(*s, false, true, true, 0);
+    md_launch_close(*s, 2 | 4, 0);
     G6_CHECK(hipEventRecord(s->ev[1], s->stream));
-    md_launch_main(*s, nullptr);
+    md_launch_main(*s, nullptr, 0);
     G6_CHECK(hipEventRecord(s->ev[2], s->stream));
-    md_launch_pre(*s, true, false, false);
+    md_launch_pre(*s, true, false, false, 0);
+    md_launch_close(*s, 1 | 8, 0);
     G6_CHECK(hipEventRecord(s->ev[3], s->stream));
     G6_CHECK(hipEventSynchronize(s->ev[3]));
     float a, b, c, e;
@@ -568,9 +567,10 @@ int g6_eval_forces(g6_handle h, double *force, double *torque, double *thermo) {
   // rows are rebuilt from the current positions; the state is not moved and
   // the thermostat is not advanced
   int one = 1;
-  G6_CHECK(hipMemcpyAsync(s->md.flags + 1, &one, sizeof(int), hipMemcpyHostToDevice, s->stream));
-  md_launch_pre(*s, false, false, true);
-  md_launch_main(*s, out);
+  G6_CHECK(hipMemcpyAsync(s->md.flags + 2, &one, sizeof(int), hipMemcpyHostToDevice, s->stream));
+  md_launch_pre(*s, false, false, true, 0);
+  md_launch_main(*s, out, 0);
+  md_launch_close(*s, 16 | 8, 0);
   G6_CHECK(hipGetLastError());
   G6_CHECK(hipMemcpyAsync(force, out, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s->stream));
   G6_CHECK(hipMemcpyAsync(torque, out + 3 * n, sizeof(double) * 3 * n, hipMemcpyDeviceToHost,

@@ -8,15 +8,15 @@
 //   com_p[3][n]       predicted COMs, read by all waves of the force kernel
 //   tmat[9][n]        rotation matrices (rigid_setmatrix, RigidModule.F90:62-83)
 //   intra[ns][3][n]   site offsets from the COM, FP64; intraf the FP32 copy
-//   cellcnt[ncell+1], cellres[ncell][cell_cap]: fixed-capacity cell lists of
-//     the MD grid (cell edge >= rc + skin)
+//   cellcnt[2][ncell+1], cellres[2][ncell][cell_cap]: fixed-capacity cell
+//     lists of the MD grid (cell edge >= rc + skin), one per step parity
 //   nbr_j/nbr_code[n][maxnb], nbr_cnt[n]: full (both-direction) Verlet rows
 //     within rc + skin; nbr_code packs the image shift ox = kx*L etc. as
 //     10-bit fields (kx+512) | (ky+512)<<10 | (kz+512)<<20 (COMs are
 //     unwrapped, so |k| can exceed 1); field value 0 encodes the -0.0 shift
 //     that dnint gives for small negative differences
-//   blockpart[nblk][16], done: per-block energy/virial/kinetic partials and
-//     the completion counter of the last-block reduction
+//   blockpart[nblk_md][16], blockek[nblk_pre][4]: per-block energy/virial and
+//     kinetic partials, reduced in a fixed order by one block (k_md_close)
 // Reference-list entry point (g6_pairlist.hip): cell residents in molecule
 //   order and the half-shell candidate rows of Grid_NeighborList
 //   (GridModule.F90:557-752) with counts cnt[2n], offsets off[2n+1], and the
@@ -71,10 +71,8 @@ struct MdBuf {
   double *frc;        // [6][n] force and torque (lab) of the last k_md_main
   double *blockpart;  // [nblk][16] k_md_main block partials
   double *blockek;    // [nblk_pre][4] k_md_pre kinetic partials
-  double *epvir;      // [16] ep, vir[9], pairs, list pairs of the last k_md_main
-  int *done;          // [0] k_md_main, [1] k_md_pre completion counters
   double *nose, *thermo;
-  int *flags;  // [0] error code, [1] rebuild rows this step
+  int *flags;  // [0] error code, [2 + parity] rebuild rows this step
   double half_skin2;
 };
 
@@ -114,8 +112,9 @@ struct System {
 bool make_model_table(int model, int prec, ModelTable *mt);
 
 // MD launchers (stream-ordered, graph-capturable: no memset, no host sync)
-void md_launch_pre(System &s, bool correct, bool predict, bool sites);
-void md_launch_main(System &s, double *out);
+void md_launch_pre(System &s, bool correct, bool predict, bool sites, int parity);
+void md_launch_close(System &s, int mode, int parity);
+void md_launch_main(System &s, double *out, int parity);
 // host-array entry points of the module seam
 int pairlist_once(System &s, const double *com_h, int64_t cap, int64_t *npair0,
                   int64_t *npair, int32_t *pi, int32_t *pj, double *ox,

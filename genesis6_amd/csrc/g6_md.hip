@@ -1,10 +1,9 @@
 // The MD step in two kernels (the path g6_step runs):
 //
 //   k_md_pre   one thread per molecule, state held in registers:
-//              rigid_correct with the forces of the previous k_md_main and the
-//              kinetic partials (the last block forms the thermo record and
-//              applies Nose_Correct), then rigid_predict (the last block then
-//              applies Nose_Predict), rigid_setmatrix + site offsets,
+//              rigid_correct with the forces of the previous k_md_main and
+//              block kinetic partials, then rigid_predict, rigid_setmatrix +
+//              site offsets,
 //              Address_Assign binning into fixed-capacity cell lists, and the
 //              Verlet-skin test that decides whether rows are rebuilt.
 //   k_md_main  one wavefront per molecule i:
@@ -17,11 +16,15 @@
 //                  reference's - via pair_sites (g6_pair.h);
 //              (3) wave reduction of i's force and lab torque (the work of
 //                  rigid_collectforce), stored for the corrector;
-//              (4) block partials of energy and virial; the last block to
-//                  finish reduces them in a fixed order.
-// A step is k_md_pre(correct previous, predict) + k_md_main; a run of steps
-// ends with a correct-only k_md_pre, so every g6_step call leaves the state
-// of the reference's loop end (after Nose_Correct, Mixture.F90:2282).
+//              (4) block partials of energy and virial.
+//   k_md_close one block: reduces the partials in a fixed order into the
+//              thermo record, Nose_Correct / Nose_Predict, and clears the
+//              cell lists of the other step parity.
+// A step is k_md_pre(correct previous, predict) + k_md_close + k_md_main; a
+// run of steps ends with a correct-only k_md_pre + k_md_close, so every
+// g6_step call leaves the state of the reference's loop end (after
+// Nose_Correct, Mixture.F90:2282).  Cell lists and rebuild flags alternate
+// between two parities so that no kernel needs a grid-wide barrier.
 // Each unordered pair is evaluated in both rows, so no molecule's force
 // needs an atomic and the FP64 results are deterministic; energies and the
 // virial are halved accordingly.  Positions read by other waves come from
@@ -203,13 +206,12 @@ __device__ inline void correct_mol(const ModelTable &mt, double dt, double zeta0
 
 // ---- kernel A --------------------------------------------------------------
 // One thread per molecule.  do_correct: rigid_correct with the forces of the
-// previous k_md_main (b.frc), kinetic partials, and in the last block the
-// step's thermo record and Nose_Correct.  do_predict: rigid_predict (then
-// Nose_Predict in the last block).  do_sites: predicted COMs, rotation
-// matrices, site offsets, binning and the skin test for k_md_main.
+// previous k_md_main (b.frc) and the block's kinetic partials.  do_predict:
+// rigid_predict.  do_sites: predicted COMs, rotation matrices, site offsets,
+// binning into the cell lists of this step's parity and the skin test.
 __global__ __launch_bounds__(64) void k_md_pre(ModelTable mt, Geom g, MdBuf b, double dt,
                                                int nose_active, int do_correct,
-                                               int do_predict, int do_sites) {
+                                               int do_predict, int do_sites, int parity) {
   const int n = g.n;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double ek[4] = {0, 0, 0, 0};
@@ -273,74 +275,107 @@ __global__ __launch_bounds__(64) void k_md_pre(ModelTable mt, Geom g, MdBuf b, d
       }
       const int c = (jc[2] * g.ndiv + jc[1]) * g.ndiv + jc[0];
       b.cell[i] = c;
-      const int slot = atomicAdd(&b.cellcnt[c], 1);
-      if (slot < b.cell_cap) b.cellres[(size_t)c * b.cell_cap + slot] = i;
-      else b.flags[0] = 3;  // cell capacity exceeded
+      const int slot = atomicAdd(&b.cellcnt[parity * (g.ncell + 1) + c], 1);
+      if (slot < b.cell_cap)
+        b.cellres[((size_t)parity * g.ncell + c) * b.cell_cap + slot] = i;
+      else
+        b.flags[0] = 3;  // cell capacity exceeded
       if (b.half_skin2 > 0.0) {
         const double e0 = m.com[0] - b.com_ref[i], e1 = m.com[1] - b.com_ref[n + i],
                      e2 = m.com[2] - b.com_ref[2 * n + i];
-        if (e0 * e0 + e1 * e1 + e2 * e2 > b.half_skin2) b.flags[1] = 1;
+        if (e0 * e0 + e1 * e1 + e2 * e2 > b.half_skin2) b.flags[2 + parity] = 1;
+      } else if (i == 0) {
+        b.flags[2 + parity] = 1;  // no skin: the reference rebuilds every step
       }
     }
   }
-  if (!do_correct && !(do_predict && nose_active)) return;
-  // kinetic partials of this block, then the last block closes the step
-  __shared__ int s_last;
+  // kinetic partials of this block (one wave), reduced by k_md_close
+  if (do_correct) {
 #pragma unroll
-  for (int q = 0; q < 4; q++) ek[q] = wsum(ek[q]);
-  if (threadIdx.x == 0) {
+    for (int q = 0; q < 4; q++) ek[q] = wsum(ek[q]);
+    if (threadIdx.x == 0) {
 #pragma unroll
-    for (int q = 0; q < 4; q++) b.blockek[(size_t)blockIdx.x * 4 + q] = ek[q];
+      for (int q = 0; q < 4; q++) b.blockek[(size_t)blockIdx.x * 4 + q] = ek[q];
+    }
   }
-  __threadfence();
-  if (threadIdx.x == 0) s_last = (atomicAdd(b.done + 1, 1) == (int)gridDim.x - 1);
+}
+
+// ---- kernel R: one block closing the step ------------------------------------
+// mode bits: 1 the kinetic partials of k_md_pre + the pair partials of the
+// previous k_md_main form the thermo record, then Nose_Correct; 2 Nose_Predict;
+// 4 clear the cell counts and rebuild flag of parity 1-p (8: both parities);
+// 16 force evaluation: the pair partials alone go to the thermo record.
+__global__ __launch_bounds__(256) void k_md_close(ModelTable mt, MdBuf b, int n, double dt,
+                                                  int nblk_pre, int nblk_main, int ncell,
+                                                  int nose_active, int mode, int parity) {
+  __shared__ double s_red[4][16];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  double a2[16];
+#pragma unroll
+  for (int q = 0; q < 16; q++) a2[q] = 0.0;
+  if (mode & 17) {
+    for (int k = threadIdx.x; k < nblk_main; k += 256)
+#pragma unroll
+      for (int q = 0; q < 12; q++) a2[q] += b.blockpart[(size_t)k * 16 + q];
+  }
+  if (mode & 1) {
+    for (int k = threadIdx.x; k < nblk_pre; k += 256)
+#pragma unroll
+      for (int q = 0; q < 4; q++) a2[12 + q] += b.blockek[(size_t)k * 4 + q];
+  }
+#pragma unroll
+  for (int q = 0; q < 16; q++) a2[q] = wsum(a2[q]);
+  if (lane == 0)
+#pragma unroll
+    for (int q = 0; q < 16; q++) s_red[wid][q] = a2[q];
+  if (mode & 12) {
+    const int p0 = (mode & 8) ? 0 : 1 - parity, p1 = (mode & 8) ? 1 : 1 - parity;
+    for (int p = p0; p <= p1; p++) {
+      int *cnt = b.cellcnt + p * (ncell + 1);
+      for (int c = threadIdx.x; c <= ncell; c += 256) cnt[c] = 0;
+      if (threadIdx.x == 0 && b.half_skin2 > 0.0) b.flags[2 + p] = 0;
+    }
+  }
   __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  double a2[4] = {0, 0, 0, 0};
-  for (int k = threadIdx.x; k < (int)gridDim.x; k += 64)
-#pragma unroll
-    for (int q = 0; q < 4; q++) a2[q] += b.blockek[(size_t)k * 4 + q];
-#pragma unroll
-  for (int q = 0; q < 4; q++) a2[q] = wsum(a2[q]);
-  if (threadIdx.x == 0) {
-    double *z = b.nose;
-    if (do_correct) {
-      double *th = b.thermo;
-      const double ep = b.epvir[0];
-      const double ekt = a2[0] * mt.mass * 0.5 * (1.0 / (dt * dt));
-      const double ekr = (a2[1] * mt.Ixx + a2[2] * mt.Iyy + a2[3] * mt.Izz) * 0.5;
-      const double dof = 6.0 * n;
-      const double temp = 10.0 * 2.0 * (ekt + ekr) / (G6_RGAS_MD * dof);
-      double h = (ep + ekt + ekr) * 10.0;
-      if (nose_active) {
-        const double zetaf = (temp - z[0]) / z[1] - z[2] * z[3];
-        const double dz2 = zetaf - z[4];
-        z[3] = z[3] + kG4_0 * dz2;
-        z[4] = zetaf;
-        z[5] = z[5] + kG4_2 * dz2;
-        z[6] = z[6] + kG4_3 * dz2;
-        z[7] = z[7] + kG4_4 * dz2;
-        z[8] = z[8] + z[3];
-        h = h + (dof * z[0] * (z[8] - 0.5 * z[3]) * dt) * G6_RGAS_MD +
-            (z[3] * z[3] * z[1] * dt * dof * 0.5) * G6_RGAS_MD;
-      }
-      th[0] = ep;
-      th[1] = ekt;
-      th[2] = ekr;
-      th[3] = temp;
-      for (int q = 0; q < 9; q++) th[4 + q] = b.epvir[1 + q];
-      th[13] = h;
-      th[14] = b.epvir[10];
-      th[15] = b.epvir[11];
+  if (threadIdx.x != 0) return;
+  double r[16];
+  for (int q = 0; q < 16; q++) r[q] = s_red[0][q] + s_red[1][q] + s_red[2][q] + s_red[3][q];
+  double *z = b.nose, *th = b.thermo;
+  if (mode & 17) {
+    th[0] = r[0];
+    for (int q = 0; q < 9; q++) th[4 + q] = r[1 + q];
+    th[14] = r[10];
+    if (r[11] > 0) th[15] = r[11];
+  }
+  if (mode & 1) {
+    const double ep = r[0];
+    const double ekt = r[12] * mt.mass * 0.5 * (1.0 / (dt * dt));
+    const double ekr = (r[13] * mt.Ixx + r[14] * mt.Iyy + r[15] * mt.Izz) * 0.5;
+    const double dof = 6.0 * n;
+    const double temp = 10.0 * 2.0 * (ekt + ekr) / (G6_RGAS_MD * dof);
+    double h = (ep + ekt + ekr) * 10.0;
+    if (nose_active) {
+      const double zetaf = (temp - z[0]) / z[1] - z[2] * z[3];
+      const double dz2 = zetaf - z[4];
+      z[3] = z[3] + kG4_0 * dz2;
+      z[4] = zetaf;
+      z[5] = z[5] + kG4_2 * dz2;
+      z[6] = z[6] + kG4_3 * dz2;
+      z[7] = z[7] + kG4_4 * dz2;
+      z[8] = z[8] + z[3];
+      h = h + (dof * z[0] * (z[8] - 0.5 * z[3]) * dt) * G6_RGAS_MD +
+          (z[3] * z[3] * z[1] * dt * dof * 0.5) * G6_RGAS_MD;
     }
-    if (do_predict && nose_active) {
-      z[3] = z[3] + z[4] + z[5] + z[6] + z[7];
-      z[4] = z[4] + 2.0 * z[5] + 3.0 * z[6] + 4.0 * z[7];
-      z[5] = z[5] + 3.0 * z[6] + 6.0 * z[7];
-      z[6] = z[6] + 4.0 * z[7];
-    }
-    b.done[1] = 0;
+    th[1] = ekt;
+    th[2] = ekr;
+    th[3] = temp;
+    th[13] = h;
+  }
+  if ((mode & 2) && nose_active) {
+    z[3] = z[3] + z[4] + z[5] + z[6] + z[7];
+    z[4] = z[4] + 2.0 * z[5] + 3.0 * z[6] + 4.0 * z[7];
+    z[5] = z[5] + 3.0 * z[6] + 6.0 * z[7];
+    z[6] = z[6] + 4.0 * z[7];
   }
 }
 
@@ -351,15 +386,15 @@ constexpr int kRowLds = 512;  // rows up to this length are sorted in LDS
 // neighbour-row build of molecule i into nbr[i*maxnb ...], sorted by j so
 // that the force sums do not depend on the order in which the binning
 // atomics filled the cells; returns the row length (also stored).
-__device__ inline int build_row(const Geom &g, const MdBuf &b, int i, int lane,
+__device__ inline int build_row(const Geom &g, const MdBuf &b, int parity, int i, int lane,
                                 int wid, int (*s_beg)[32], int (*s_pre)[33],
                                 int (*s_j)[kRowLds], int (*s_c)[kRowLds]) {
   const int n = g.n;
   const int c = b.cell[i];
   const int nd = g.ndiv;
   const int cx = c % nd, cy = (c / nd) % nd, cz = c / (nd * nd);
-  const int *cnt = b.cellcnt;
-  const int *res = b.cellres;
+  const int *cnt = b.cellcnt + parity * (g.ncell + 1);
+  const int *res = b.cellres + (size_t)parity * g.ncell * b.cell_cap;
   int len = 0;
   if (lane < b.nfull) {
     const int off = b.full[lane];
@@ -443,12 +478,12 @@ __device__ inline int build_row(const Geom &g, const MdBuf &b, int i, int lane,
 }
 
 template <int M, typename R>
-__global__ __launch_bounds__(256) void k_md_main(ModelTable mt, Geom g, MdBuf b, double *out) {
+__global__ __launch_bounds__(256) void k_md_main(ModelTable mt, Geom g, MdBuf b, double *out,
+                                                 int parity) {
   using T = Traits<M>;
   __shared__ int s_beg[4][32], s_pre[4][33];
   __shared__ int s_j[4][kRowLds], s_c[4][kRowLds];
   __shared__ double s_red[4][13];
-  __shared__ int s_last;
   const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int n = g.n;
   const int i = blockIdx.x * 4 + wid;
@@ -458,8 +493,8 @@ __global__ __launch_bounds__(256) void k_md_main(ModelTable mt, Geom g, MdBuf b,
   for (int q = 0; q < 13; q++) acc[q] = 0.0;
   if (active) {
     int m;
-    if (b.flags[1]) {
-      m = build_row(g, b, i, lane, wid, s_beg, s_pre, s_j, s_c);
+    if (b.flags[2 + parity]) {
+      m = build_row(g, b, parity, i, lane, wid, s_beg, s_pre, s_j, s_c);
       acc[11] = 0.5 * m;
     } else {
       m = b.nbr_cnt[i];
@@ -549,63 +584,33 @@ __global__ __launch_bounds__(256) void k_md_main(ModelTable mt, Geom g, MdBuf b,
     const int q = threadIdx.x;
     b.blockpart[(size_t)blockIdx.x * 16 + q] = s_red[0][q] + s_red[1][q] + s_red[2][q] + s_red[3][q];
   }
-  // the last block to finish reduces the partials in a fixed order
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) s_last = (atomicAdd(b.done, 1) == (int)gridDim.x - 1);
-  __syncthreads();
-  if (!s_last) return;
-  __threadfence();
-  double a2[12];
-#pragma unroll
-  for (int q = 0; q < 12; q++) a2[q] = 0.0;
-  for (int k = threadIdx.x; k < (int)gridDim.x; k += 256)
-#pragma unroll
-    for (int q = 0; q < 12; q++) a2[q] += b.blockpart[(size_t)k * 16 + q];
-#pragma unroll
-  for (int q = 0; q < 12; q++) a2[q] = wsum(a2[q]);
-  __syncthreads();
-  if (lane == 0)
-    for (int q = 0; q < 12; q++) s_red[wid][q] = a2[q];
-  // the cell lists are consumed: clear the counts for the next step
-  for (int c = threadIdx.x; c <= g.ncell; c += 256) b.cellcnt[c] = 0;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int q = 0; q < 11; q++)
-      b.epvir[q] = s_red[0][q] + s_red[1][q] + s_red[2][q] + s_red[3][q];
-    if (b.flags[1]) b.epvir[11] = s_red[0][11] + s_red[1][11] + s_red[2][11] + s_red[3][11];
-    if (out) {
-      // eval mode: the thermo record carries this state's energy terms
-      for (int q = 0; q < 9; q++) b.thermo[4 + q] = b.epvir[1 + q];
-      b.thermo[0] = b.epvir[0];
-      b.thermo[14] = b.epvir[10];
-      b.thermo[15] = b.epvir[11];
-    }
-    if (b.half_skin2 > 0.0) b.flags[1] = 0;
-    *b.done = 0;
-  }
 }
 
-void md_launch_pre(System &s, bool correct, bool predict, bool sites) {
+void md_launch_pre(System &s, bool correct, bool predict, bool sites, int parity) {
   k_md_pre<<<(s.n + 63) / 64, 64, 0, s.stream>>>(s.mt, s.geo_md, s.md, s.dt,
                                                   s.nose_active ? 1 : 0, correct ? 1 : 0,
-                                                  predict ? 1 : 0, sites ? 1 : 0);
+                                                  predict ? 1 : 0, sites ? 1 : 0, parity);
+}
+
+void md_launch_close(System &s, int mode, int parity) {
+  k_md_close<<<1, 256, 0, s.stream>>>(s.mt, s.md, s.n, s.dt, (s.n + 63) / 64, s.nblk_md,
+                                      s.geo_md.ncell, s.nose_active ? 1 : 0, mode, parity);
 }
 
 template <int M, typename R>
-static void md_main_t(System &s, double *out) {
-  k_md_main<M, R><<<s.nblk_md, 256, 0, s.stream>>>(s.mt, s.geo_md, s.md, out);
+static void md_main_t(System &s, double *out, int parity) {
+  k_md_main<M, R><<<s.nblk_md, 256, 0, s.stream>>>(s.mt, s.geo_md, s.md, out, parity);
 }
 
-void md_launch_main(System &s, double *out) {
+void md_launch_main(System &s, double *out, int parity) {
   if (s.prec == G6_FP32) {
-    if (s.model == G6_TIP4P) md_main_t<G6_TIP4P, float>(s, out);
-    else if (s.model == G6_TIP5P) md_main_t<G6_TIP5P, float>(s, out);
-    else md_main_t<G6_ST2, float>(s, out);
+    if (s.model == G6_TIP4P) md_main_t<G6_TIP4P, float>(s, out, parity);
+    else if (s.model == G6_TIP5P) md_main_t<G6_TIP5P, float>(s, out, parity);
+    else md_main_t<G6_ST2, float>(s, out, parity);
   } else {
-    if (s.model == G6_TIP4P) md_main_t<G6_TIP4P, double>(s, out);
-    else if (s.model == G6_TIP5P) md_main_t<G6_TIP5P, double>(s, out);
-    else md_main_t<G6_ST2, double>(s, out);
+    if (s.model == G6_TIP4P) md_main_t<G6_TIP4P, double>(s, out, parity);
+    else if (s.model == G6_TIP5P) md_main_t<G6_TIP5P, double>(s, out, parity);
+    else md_main_t<G6_ST2, double>(s, out, parity);
   }
 }
 
