@@ -195,16 +195,13 @@ static int alloc_all(System &s) {
 #define AL(ptr, bytes) G6_CHECK(A((void **)&(ptr), (bytes)))
   MdBuf &m = s.md;
   AL(m.com, sizeof(double) * 3 * n);
-  AL(m.com_p, sizeof(double) * 3 * n);
   AL(m.dx, sizeof(double) * 15 * n);
   AL(m.w, sizeof(double) * 15 * n);
   AL(m.quat, sizeof(double) * 20 * n);
   AL(m.tmat, sizeof(double) * 9 * n);
-  AL(m.intra, sizeof(double) * 3 * ns * n);
-  AL(m.intraf, sizeof(float) * 3 * ns * n);
+  AL(m.rec, sizeof(double) * (size_t)m.rs * n);
   AL(m.com_ref, sizeof(double) * 3 * n);
   AL(m.cell, sizeof(int) * n);
-  AL(m.cellcnt, sizeof(int) * 2 * (ncm + 1));
   AL(m.cellres, sizeof(int) * 2 * (size_t)ncm * m.cell_cap);
   AL(m.nbr_j, sizeof(int) * (size_t)n * m.maxnb);
   AL(m.nbr_code, sizeof(int) * (size_t)n * m.maxnb);
@@ -248,14 +245,14 @@ static int alloc_all(System &s) {
 static int reset_step_state(System &s) {
   int fl[8] = {0, 0, 1, 1, 0, 0, 0, 0};
   G6_CHECK(hipMemcpy(s.md.flags, fl, sizeof(fl), hipMemcpyHostToDevice));
-  G6_CHECK(hipMemset(s.md.cellcnt, 0, sizeof(int) * 2 * (s.geo_md.ncell + 1)));
+  G6_CHECK(hipMemset(s.md.cellres, 0, sizeof(int) * 2 * (size_t)s.geo_md.ncell * s.md.cell_cap));
   return 0;
 }
 
 static void free_all(System &s) {
   const MdBuf &m = s.md;
-  void *ps[] = {m.com, m.com_p, m.dx, m.w, m.quat, m.tmat, m.intra, m.intraf, m.com_ref,
-                m.cell, m.cellcnt, m.cellres, m.nbr_j, m.nbr_code, m.nbr_cnt, m.blockpart,
+  void *ps[] = {m.com, m.dx, m.w, m.quat, m.tmat, m.rec, m.com_ref,
+                m.cell, m.cellres, m.nbr_j, m.nbr_code, m.nbr_cnt, m.blockpart,
                 m.blockek, m.frc, m.nose, m.thermo, m.flags, s.cell, s.cell_count, s.cell_start,
                 s.cell_fill, s.sorted_tmp, s.sorted, s.cnt, s.off, s.pi, s.pj, s.scode,
                 s.com_ref, s.lflags};
@@ -345,30 +342,21 @@ int g6_init(g6_handle *h, int model, int n, double box_l, double rc, double tape
   const double capd = (double)n * (per * 1.6 + 64.0);
   const double all = 0.5 * (double)n * (double)(n - 1);
   s->pcap = (int)fmin(fmin(capd, all + 64.0), 2.0e9);
-  // the MD grid: cells no smaller than rc + skin, unique 27-neighbourhood
+  // the MD grid: cells no smaller than (rc + skin)/2, searched with a 5^3
+  // stencil pruned per molecule (all cells once when ndiv < 5)
   {
     Geom &m = s->geo_md;
     m = g;
-    m.ndiv = (int)(box_l / fmax(voxa, rl));
+    m.ndiv = (int)(box_l / (0.5 * rl));
     if (m.ndiv < 1) m.ndiv = 1;
     m.ncell = m.ndiv * m.ndiv * m.ndiv;
     m.cellinv = 1.0 / (box_l / m.ndiv);
     MdBuf &b = s->md;
-    b.nfull = 0;
-    for (int dz = -1; dz <= 1; dz++)
-      for (int dy = -1; dy <= 1; dy++)
-        for (int dx = -1; dx <= 1; dx++) {
-          const int ox = (dx + m.ndiv) % m.ndiv, oy = (dy + m.ndiv) % m.ndiv,
-                    oz = (dz + m.ndiv) % m.ndiv;
-          const int packed = ox | (oy << 8) | (oz << 16);
-          bool seen = false;
-          for (int k = 0; k < b.nfull; k++) seen = seen || b.full[k] == packed;
-          if (!seen) b.full[b.nfull++] = packed;
-        }
+    b.rs = prec == G6_FP32 ? kRecF : kRecD;
     const double rho = n / (box_l * box_l * box_l);
     const double percell = (double)n / m.ncell;
-    b.cell_cap = ((int)(2.0 * percell + 32.0) + 31) / 32 * 32;
-    if (b.cell_cap > n) b.cell_cap = n;
+    b.cell_cap = ((int)(2.0 * percell + 33.0) + 31) / 32 * 32;
+    if (b.cell_cap > n + 1) b.cell_cap = n + 1;
     const double nb = 4.0 / 3.0 * kPI * rl * rl * rl * rho;
     b.maxnb = ((int)(1.5 * nb + 64.0) + 63) / 64 * 64;
     if (b.maxnb > n) b.maxnb = n;

@@ -5,11 +5,11 @@
 // MD path (g6_md.hip), structure MdBuf:
 //   rigid state, FP64, component-major:  com[3][n], dx[5][3][n], w[5][3][n],
 //     quat[5][4][n]  (the Nordsieck arrays of RigidModule.F90:14-33 sRigid1)
-//   com_p[3][n]       predicted COMs, read by all waves of the force kernel
 //   tmat[9][n]        rotation matrices (rigid_setmatrix, RigidModule.F90:62-83)
-//   intra[ns][3][n]   site offsets from the COM, FP64; intraf the FP32 copy
-//   cellcnt[2][ncell+1], cellres[2][ncell][cell_cap]: fixed-capacity cell
-//     lists of the MD grid (cell edge >= rc + skin), one per step parity
+//   rec[n][rs]        per-molecule record of the predicted COM and site offsets
+//   cellres[2][ncell][cell_cap]: fixed-capacity cell lists of the MD grid
+//     (cell edge >= (rc + skin)/2), one per step parity; slot 0 holds the
+//     count so that no two cells share a counter's cache line
 //   nbr_j/nbr_code[n][maxnb], nbr_cnt[n]: full (both-direction) Verlet rows
 //     within rc + skin; nbr_code packs the image shift ox = kx*L etc. as
 //     10-bit fields (kx+512) | (ky+512)<<10 | (kz+512)<<20 (COMs are
@@ -58,16 +58,21 @@ struct Geom {
   double rin, rout, c1, c2; // switch: S between rin and rout
 };
 
+// Per-molecule record strides (in doubles) read by k_md_main: the predicted
+// COM in FP64 at [0..2], then the site offsets of the sites before the CoM
+// site, as floats from [4] (FP32 mode, 128 B) or doubles from [4] (FP64).
+constexpr int kRecF = 16, kRecD = 24;
+
 // Device pointers and sizes of the MD path, passed to kernels by value.
 struct MdBuf {
-  double *com, *com_p, *dx, *w, *quat, *tmat, *intra, *com_ref;
-  float *intraf;
-  int *cell, *cellcnt, *cellres;
+  double *com, *dx, *w, *quat, *tmat, *com_ref;
+  double *rec;  // [n][rs] records
+  int rs;
+  int *cell;     // [n] MD-grid cell of each molecule
+  int *cellres;  // [2][ncell][cell_cap]: slot 0 the count, residents from slot 1
   int cell_cap;
   int *nbr_j, *nbr_code, *nbr_cnt;
   int maxnb;
-  int nfull;
-  int full[27];  // unique neighbour-cell offsets (packed ox | oy<<8 | oz<<16)
   double *frc;        // [6][n] force and torque (lab) of the last k_md_main
   double *blockpart;  // [nblk][16] k_md_main block partials
   double *blockek;    // [nblk_pre][4] k_md_pre kinetic partials

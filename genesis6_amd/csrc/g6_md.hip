@@ -1,34 +1,36 @@
-// The MD step in two kernels (the path g6_step runs):
+// The MD step (the path g6_step runs), three kernels per step:
 //
 //   k_md_pre   one thread per molecule, state held in registers:
 //              rigid_correct with the forces of the previous k_md_main and
-//              block kinetic partials, then rigid_predict, rigid_setmatrix +
-//              site offsets,
-//              Address_Assign binning into fixed-capacity cell lists, and the
-//              Verlet-skin test that decides whether rows are rebuilt.
+//              block kinetic partials, then rigid_predict, rigid_setmatrix,
+//              the per-molecule record read by k_md_main (predicted COM in
+//              FP64, site offsets in the pair precision), Address_Assign
+//              binning into fixed-capacity cell lists of this step's parity,
+//              and the Verlet-skin test that decides whether rows are rebuilt.
+//   k_md_close one block: reduces the block partials in a fixed order into
+//              the thermo record, Nose_Correct / Nose_Predict, and clears the
+//              cell lists of the other parity.
 //   k_md_main  one wavefront per molecule i:
-//              (1) if a rebuild is due, i's own neighbour row: all j != i in
-//                  the 27 (or fewer, for ndiv < 3) surrounding cells with the
-//                  reference's FP64 minimum-image test dd < (rc+skin)^2;
-//              (2) the pair forces of i with every listed j whose FP64
-//                  COM distance passes dd < rc^2 - the reference's
+//              (1) if a rebuild is due, i's own neighbour row: every j != i
+//                  of the cells whose box lies within rc + skin of i (a 5^3
+//                  stencil of cells of edge >= (rc+skin)/2) that passes the
+//                  reference's FP64 minimum-image test dd < (rc+skin)^2,
+//                  sorted by j;
+//              (2) the pair forces of i with every listed j whose FP64 COM
+//                  distance passes dd < rc^2 - the reference's
 //                  Interaction_Compress2 test, so the interacting set is the
 //                  reference's - via pair_sites (g6_pair.h);
 //              (3) wave reduction of i's force and lab torque (the work of
 //                  rigid_collectforce), stored for the corrector;
 //              (4) block partials of energy and virial.
-//   k_md_close one block: reduces the partials in a fixed order into the
-//              thermo record, Nose_Correct / Nose_Predict, and clears the
-//              cell lists of the other step parity.
 // A step is k_md_pre(correct previous, predict) + k_md_close + k_md_main; a
 // run of steps ends with a correct-only k_md_pre + k_md_close, so every
 // g6_step call leaves the state of the reference's loop end (after
 // Nose_Correct, Mixture.F90:2282).  Cell lists and rebuild flags alternate
 // between two parities so that no kernel needs a grid-wide barrier.
 // Each unordered pair is evaluated in both rows, so no molecule's force
-// needs an atomic and the FP64 results are deterministic; energies and the
-// virial are halved accordingly.  Positions read by other waves come from
-// com_p (the predicted COMs), never from the state being corrected.
+// needs an atomic, rows are sorted and every reduction runs in a fixed
+// order: results are deterministic.  Energies and the virial are halved.
 #include "g6_pair.h"
 
 // Everything written in this file is FP64 bookkeeping in the reference's
@@ -44,10 +46,35 @@ constexpr double kG4_0 = 251.0 / 720.0, kG4_2 = 11.0 / 12.0, kG4_3 = 1.0 / 3.0,
                  kG4_4 = 1.0 / 24.0;
 #define G6_RGAS_MD (6.0225e23 * 1.380066e-23)
 
-__device__ inline double wsum(double v) {
-#pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+// DPP move of a double across lanes (both halves); rows outside RM read 0.
+template <int CTRL, int RM = 0xF>
+__device__ inline double dpp_d(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, RM, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, RM, 0xF, false);
+  return __builtin_bit_cast(double,
+                            ((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo);
+}
+
+// Sum over the 64 lanes of a wave in a fixed order; the total is in lane 63.
+// Steps: xor 1, xor 2 (quad_perm), half-row mirror, row mirror, then
+// row_bcast:15 into rows 1 and 3 and row_bcast:31 into rows 2 and 3.
+// Every lane of the wave must be active.
+__device__ inline double wsum63(double v) {
+  v += dpp_d<0xB1>(v);
+  v += dpp_d<0x4E>(v);
+  v += dpp_d<0x141>(v);
+  v += dpp_d<0x140>(v);
+  v += dpp_d<0x142, 0xA>(v);
+  v += dpp_d<0x143, 0xC>(v);
   return v;
+}
+
+__device__ inline double lane63(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const unsigned lo = __builtin_amdgcn_readlane((int)(unsigned)u, 63);
+  const unsigned hi = __builtin_amdgcn_readlane((int)(unsigned)(u >> 32), 63);
+  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
 __device__ inline void wsync() {
@@ -204,11 +231,12 @@ __device__ inline void correct_mol(const ModelTable &mt, double dt, double zeta0
   ek[3] = m.w[0][2] * m.w[0][2];
 }
 
+
 // ---- kernel A --------------------------------------------------------------
 // One thread per molecule.  do_correct: rigid_correct with the forces of the
 // previous k_md_main (b.frc) and the block's kinetic partials.  do_predict:
-// rigid_predict.  do_sites: predicted COMs, rotation matrices, site offsets,
-// binning into the cell lists of this step's parity and the skin test.
+// rigid_predict.  do_sites: rotation matrix, record, binning into the cell
+// lists of this step's parity and the skin test.
 __global__ __launch_bounds__(64) void k_md_pre(ModelTable mt, Geom g, MdBuf b, double dt,
                                                int nose_active, int do_correct,
                                                int do_predict, int do_sites, int parity) {
@@ -236,8 +264,6 @@ __global__ __launch_bounds__(64) void k_md_pre(ModelTable mt, Geom g, MdBuf b, d
     if (do_predict) predict_mol(m);
     if (do_predict || do_correct) store_mol(b, n, i, m);
     if (do_sites) {
-#pragma unroll
-      for (int a = 0; a < 3; a++) b.com_p[a * n + i] = m.com[a];
       const double qa = m.q[0][0], qb = m.q[0][1], qc = m.q[0][2], qd = m.q[0][3];
       double t[9];
       t[0] = (qa * qa + qb * qb - (qc * qc + qd * qd));
@@ -251,16 +277,26 @@ __global__ __launch_bounds__(64) void k_md_pre(ModelTable mt, Geom g, MdBuf b, d
       t[8] = qa * qa + qd * qd - (qb * qb + qc * qc);
 #pragma unroll
       for (int a = 0; a < 9; a++) b.tmat[a * n + i] = t[a];
-      for (int s = 0; s < mt.nsite; s++) {
+      // record: COM (FP64), then the offsets of the sites before the CoM
+      // site (whose offset is zero) in the pair precision
+      double *rec = b.rec + (size_t)i * b.rs;
+      rec[0] = m.com[0];
+      rec[1] = m.com[1];
+      rec[2] = m.com[2];
+      float *recf = reinterpret_cast<float *>(rec + 4);
+      for (int s = 0; s < mt.nsite - 1; s++) {
         const double r0 = t[0] * mt.bx[s] + t[1] * mt.by[s] + t[2] * mt.bz[s];
         const double r1 = t[3] * mt.bx[s] + t[4] * mt.by[s] + t[5] * mt.bz[s];
         const double r2 = t[6] * mt.bx[s] + t[7] * mt.by[s] + t[8] * mt.bz[s];
-        b.intra[(s * 3 + 0) * n + i] = r0;
-        b.intra[(s * 3 + 1) * n + i] = r1;
-        b.intra[(s * 3 + 2) * n + i] = r2;
-        b.intraf[(s * 3 + 0) * n + i] = (float)r0;
-        b.intraf[(s * 3 + 1) * n + i] = (float)r1;
-        b.intraf[(s * 3 + 2) * n + i] = (float)r2;
+        if (b.rs == kRecF) {
+          recf[s * 3 + 0] = (float)r0;
+          recf[s * 3 + 1] = (float)r1;
+          recf[s * 3 + 2] = (float)r2;
+        } else {
+          rec[4 + s * 3 + 0] = r0;
+          rec[4 + s * 3 + 1] = r1;
+          rec[4 + s * 3 + 2] = r2;
+        }
       }
       int jc[3];
 #pragma unroll
@@ -270,14 +306,16 @@ __global__ __launch_bounds__(64) void k_md_pre(ModelTable mt, Geom g, MdBuf b, d
         d = d - o;
         int j = (int)round(d * g.cellinv);
         if (j < 0) j += g.ndiv;
-        j = ((j % g.ndiv) + g.ndiv) % g.ndiv;  // acts only for ndiv = 1
+        j = ((j % g.ndiv) + g.ndiv) % g.ndiv;
         jc[a] = j;
       }
       const int c = (jc[2] * g.ndiv + jc[1]) * g.ndiv + jc[0];
       b.cell[i] = c;
-      const int slot = atomicAdd(&b.cellcnt[parity * (g.ncell + 1) + c], 1);
+      // slot 0 of a cell row is its count; residents follow
+      int *cr = b.cellres + ((size_t)parity * g.ncell + c) * b.cell_cap;
+      const int slot = atomicAdd(cr, 1) + 1;
       if (slot < b.cell_cap)
-        b.cellres[((size_t)parity * g.ncell + c) * b.cell_cap + slot] = i;
+        cr[slot] = i;
       else
         b.flags[0] = 3;  // cell capacity exceeded
       if (b.half_skin2 > 0.0) {
@@ -292,8 +330,8 @@ __global__ __launch_bounds__(64) void k_md_pre(ModelTable mt, Geom g, MdBuf b, d
   // kinetic partials of this block (one wave), reduced by k_md_close
   if (do_correct) {
 #pragma unroll
-    for (int q = 0; q < 4; q++) ek[q] = wsum(ek[q]);
-    if (threadIdx.x == 0) {
+    for (int q = 0; q < 4; q++) ek[q] = wsum63(ek[q]);
+    if (threadIdx.x == 63) {
 #pragma unroll
       for (int q = 0; q < 4; q++) b.blockek[(size_t)blockIdx.x * 4 + q] = ek[q];
     }
@@ -309,7 +347,7 @@ __global__ __launch_bounds__(256) void k_md_close(ModelTable mt, MdBuf b, int n,
                                                   int nblk_pre, int nblk_main, int ncell,
                                                   int nose_active, int mode, int parity) {
   __shared__ double s_red[4][16];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int wid = threadIdx.x >> 6;
   double a2[16];
 #pragma unroll
   for (int q = 0; q < 16; q++) a2[q] = 0.0;
@@ -324,15 +362,15 @@ __global__ __launch_bounds__(256) void k_md_close(ModelTable mt, MdBuf b, int n,
       for (int q = 0; q < 4; q++) a2[12 + q] += b.blockek[(size_t)k * 4 + q];
   }
 #pragma unroll
-  for (int q = 0; q < 16; q++) a2[q] = wsum(a2[q]);
-  if (lane == 0)
+  for (int q = 0; q < 16; q++) a2[q] = wsum63(a2[q]);
+  if ((threadIdx.x & 63) == 63)
 #pragma unroll
     for (int q = 0; q < 16; q++) s_red[wid][q] = a2[q];
   if (mode & 12) {
     const int p0 = (mode & 8) ? 0 : 1 - parity, p1 = (mode & 8) ? 1 : 1 - parity;
     for (int p = p0; p <= p1; p++) {
-      int *cnt = b.cellcnt + p * (ncell + 1);
-      for (int c = threadIdx.x; c <= ncell; c += 256) cnt[c] = 0;
+      int *cr = b.cellres + (size_t)p * ncell * b.cell_cap;
+      for (int c = threadIdx.x; c < ncell; c += 256) cr[(size_t)c * b.cell_cap] = 0;
       if (threadIdx.x == 0 && b.half_skin2 > 0.0) b.flags[2 + p] = 0;
     }
   }
@@ -381,51 +419,97 @@ __global__ __launch_bounds__(256) void k_md_close(ModelTable mt, MdBuf b, int n,
 
 // ---- kernel B --------------------------------------------------------------
 
-constexpr int kRowLds = 512;  // rows up to this length are sorted in LDS
+// Cell stencil of the MD grid: per axis na offsets from lo; with ndiv >= 5
+// the 5 offsets -2..2 are distinct cells and are pruned by their distance
+// from the molecule, below that every cell is taken once (no pruning).
+struct Stencil {
+  int na, lo, nst;
+  bool prune;
+};
+__device__ inline Stencil stencil_of(int nd) {
+  Stencil st;
+  st.prune = nd >= 5;
+  st.na = st.prune ? 5 : nd;
+  st.lo = st.prune ? -2 : -((nd - 1) / 2);
+  st.nst = st.na * st.na * st.na;
+  return st;
+}
+
+// Distance in cell units from a point at u (cell units from the centre of
+// its cell, |u| <= 1/2) to the cell at offset o, which spans [o-1/2, o+1/2];
+// shrunk by 1e-9 so that rounding in the binning never drops a cell.
+__device__ inline double cell_gap(int o, double u) {
+  const double a = (double)o - 0.5 - u, c = u - (double)o - 0.5;
+  return fmax(0.0, fmax(a, c) - 1e-9);
+}
+
+__device__ inline double frac_in_cell(const Geom &g, double x) {
+  const double d = x - round(x * g.invL) * g.L;
+  const double t = d * g.cellinv;
+  return t - round(t);
+}
 
 // neighbour-row build of molecule i into nbr[i*maxnb ...], sorted by j so
 // that the force sums do not depend on the order in which the binning
 // atomics filled the cells; returns the row length (also stored).
 __device__ inline int build_row(const Geom &g, const MdBuf &b, int parity, int i, int lane,
-                                int wid, int (*s_beg)[32], int (*s_pre)[33],
-                                int (*s_j)[kRowLds], int (*s_c)[kRowLds]) {
+                                int (&s_beg)[128], int (&s_pre)[132], int *s_j, int *s_c) {
   const int n = g.n;
-  const int c = b.cell[i];
   const int nd = g.ndiv;
+  const int c = b.cell[i];
   const int cx = c % nd, cy = (c / nd) % nd, cz = c / (nd * nd);
-  const int *cnt = b.cellcnt + parity * (g.ncell + 1);
+  const Stencil st = stencil_of(nd);
+  const double *reci = b.rec + (size_t)i * b.rs;
+  const double xi = reci[0], yi = reci[1], zi = reci[2];
+  const double ux = frac_in_cell(g, xi), uy = frac_in_cell(g, yi), uz = frac_in_cell(g, zi);
+  const double lim = g.rr * (g.cellinv * g.cellinv);  // (rc+skin)^2 in cell units
   const int *res = b.cellres + (size_t)parity * g.ncell * b.cell_cap;
-  int len = 0;
-  if (lane < b.nfull) {
-    const int off = b.full[lane];
-    const int ox = off & 0xff, oy = (off >> 8) & 0xff, oz = (off >> 16) & 0xff;
-    const int c2 = (((cz + oz) % nd) * nd + (cy + oy) % nd) * nd + (cx + ox) % nd;
-    s_beg[wid][lane] = c2 * b.cell_cap;
-    len = min(cnt[c2], b.cell_cap);
+  int carry = 0;
+  for (int e0 = 0; e0 < st.nst; e0 += 64) {
+    const int e = e0 + lane;
+    int len = 0;
+    if (e < st.nst) {
+      const int ox = st.lo + e % st.na, oy = st.lo + (e / st.na) % st.na,
+                oz = st.lo + e / (st.na * st.na);
+      bool keep = true;
+      if (st.prune) {
+        const double gx = cell_gap(ox, ux), gy = cell_gap(oy, uy), gz = cell_gap(oz, uz);
+        keep = gx * gx + gy * gy + gz * gz <= lim;
+      }
+      const int c2 = ((((cz + oz) % nd + nd) % nd) * nd + ((cy + oy) % nd + nd) % nd) * nd +
+                     ((cx + ox) % nd + nd) % nd;
+      s_beg[e] = c2 * b.cell_cap + 1;
+      if (keep) len = min(res[(size_t)c2 * b.cell_cap], b.cell_cap - 1);
+    }
+    int inc = len;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int v = __shfl_up(inc, d, 64);
+      if (lane >= d) inc += v;
+    }
+    if (e < st.nst) s_pre[e + 1] = carry + inc;
+    carry += __shfl(inc, 63, 64);
   }
-  int inc = len;
-  for (int d = 1; d < 32; d <<= 1) {
-    const int v = __shfl_up(inc, d, 64);
-    if (lane >= d) inc += v;
-  }
-  if (lane < b.nfull) s_pre[wid][lane + 1] = inc;
-  if (lane == 0) s_pre[wid][0] = 0;
+  if (lane == 0) s_pre[0] = 0;
   wsync();
-  const int total = s_pre[wid][b.nfull];
-  const double xi = b.com_p[i], yi = b.com_p[n + i], zi = b.com_p[2 * n + i];
+  const int total = s_pre[st.nst];
   int base = 0;
-  int *nj = b.nbr_j + (size_t)i * b.maxnb;
-  int *nc = b.nbr_code + (size_t)i * b.maxnb;
+  const int cap = b.maxnb;
   for (int t0 = 0; t0 < total; t0 += 64) {
     const int t = t0 + lane;
     bool pass = false;
     int j = 0, code = 0;
     if (t < total) {
-      int s = 0;
-      while (s + 1 < b.nfull && s_pre[wid][s + 1] <= t) s++;
-      j = res[s_beg[wid][s] + (t - s_pre[wid][s])];
+      int lo = 0, hi = st.nst;  // s_pre[lo] <= t < s_pre[hi]
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (s_pre[mid] <= t) lo = mid;
+        else hi = mid;
+      }
+      j = res[s_beg[lo] + (t - s_pre[lo])];
       if (j != i) {
-        double d0 = xi - b.com_p[j], d1 = yi - b.com_p[n + j], d2 = zi - b.com_p[2 * n + j];
+        const double *recj = b.rec + (size_t)j * b.rs;
+        double d0 = xi - recj[0], d1 = yi - recj[1], d2 = zi - recj[2];
         const double k0 = round(d0 * g.invL), k1 = round(d1 * g.invL), k2 = round(d2 * g.invL);
         d0 = d0 - k0 * g.L;
         d1 = d1 - k1 * g.L;
@@ -442,13 +526,9 @@ __device__ inline int build_row(const Geom &g, const MdBuf &b, int parity, int i
     const unsigned long long mask = __ballot(pass);
     const int pos = base + __popcll(mask & ((1ull << lane) - 1ull));
     if (pass) {
-      if (pos < b.maxnb) {
-        nj[pos] = j;
-        nc[pos] = code;
-        if (pos < kRowLds) {
-          s_j[wid][pos] = j;
-          s_c[wid][pos] = code;
-        }
+      if (pos < cap) {
+        s_j[pos] = j;
+        s_c[pos] = code;
       } else {
         b.flags[0] = 4;  // neighbour-row capacity exceeded
       }
@@ -456,18 +536,18 @@ __device__ inline int build_row(const Geom &g, const MdBuf &b, int parity, int i
     base += __popcll(mask);
   }
   wsync();
-  const int m = min(base, b.maxnb);
-  if (m <= kRowLds) {
-    // rank sort by j (each j occurs once in a row)
-    for (int t = lane; t < m; t += 64) {
-      const int key = s_j[wid][t];
-      int rank = 0;
-      for (int u = 0; u < m; u++) rank += s_j[wid][u] < key;
-      nj[rank] = key;
-      nc[rank] = s_c[wid][t];
-    }
+  const int m = min(base, cap);
+  // rank sort by j (each j occurs once in a row)
+  int *nj = b.nbr_j + (size_t)i * cap;
+  int *nc = b.nbr_code + (size_t)i * cap;
+  for (int t = lane; t < m; t += 64) {
+    const int key = s_j[t];
+    int rank = 0;
+    for (int u = 0; u < m; u++) rank += s_j[u] < key;
+    nj[rank] = key;
+    nc[rank] = s_c[t];
   }
-  wsync();
+  wsync();  // the force loop reads entries other lanes wrote
   if (lane == 0) {
     b.nbr_cnt[i] = m;
     b.com_ref[i] = xi;
@@ -477,86 +557,144 @@ __device__ inline int build_row(const Geom &g, const MdBuf &b, int parity, int i
   return m;
 }
 
+// site offsets of the sites before the CoM site from a record
+template <int M, typename R>
+struct RecSites;
+template <int M>
+struct RecSites<M, float> {
+  static constexpr int NV = (Traits<M>::COM * 3 + 3) / 4;  // float4 loads
+  __device__ static void load(const double *rec, float (&r)[Traits<M>::NS][3]) {
+    const float4 *p = reinterpret_cast<const float4 *>(rec + 4);
+    float f[NV * 4];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      const float4 q = p[v];
+      f[4 * v] = q.x; f[4 * v + 1] = q.y; f[4 * v + 2] = q.z; f[4 * v + 3] = q.w;
+    }
+#pragma unroll
+    for (int s = 0; s < Traits<M>::COM; s++)
+#pragma unroll
+      for (int a = 0; a < 3; a++) r[s][a] = f[s * 3 + a];
+#pragma unroll
+    for (int a = 0; a < 3; a++) r[Traits<M>::COM][a] = 0.0f;
+  }
+};
+template <int M>
+struct RecSites<M, double> {
+  static constexpr int NV = (Traits<M>::COM * 3 + 1) / 2;  // double2 loads
+  __device__ static void load(const double *rec, double (&r)[Traits<M>::NS][3]) {
+    const double2 *p = reinterpret_cast<const double2 *>(rec + 4);
+    double f[NV * 2];
+#pragma unroll
+    for (int v = 0; v < NV; v++) {
+      const double2 q = p[v];
+      f[2 * v] = q.x; f[2 * v + 1] = q.y;
+    }
+#pragma unroll
+    for (int s = 0; s < Traits<M>::COM; s++)
+#pragma unroll
+      for (int a = 0; a < 3; a++) r[s][a] = f[s * 3 + a];
+#pragma unroll
+    for (int a = 0; a < 3; a++) r[Traits<M>::COM][a] = 0.0;
+  }
+};
+
 template <int M, typename R>
 __global__ __launch_bounds__(256) void k_md_main(ModelTable mt, Geom g, MdBuf b, double *out,
                                                  int parity) {
   using T = Traits<M>;
-  __shared__ int s_beg[4][32], s_pre[4][33];
-  __shared__ int s_j[4][kRowLds], s_c[4][kRowLds];
-  __shared__ double s_red[4][13];
-  const int wid = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  extern __shared__ int s_row[];  // [4][2][maxnb]: row sort buffers
+  __shared__ int s_beg[4][128], s_pre[4][132];
+  __shared__ double s_red[4][12];
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int n = g.n;
   const int i = blockIdx.x * 4 + wid;
-  const bool active = i < n;
-  double acc[13];  // ep, vir[9], interacting pairs, list pairs (each halved), spare
+  double acc[12];  // ep, vir[9], interacting pairs, list pairs (each halved)
 #pragma unroll
-  for (int q = 0; q < 13; q++) acc[q] = 0.0;
-  if (active) {
+  for (int q = 0; q < 12; q++) acc[q] = 0.0;
+  if (i < n) {
     int m;
     if (b.flags[2 + parity]) {
-      m = build_row(g, b, parity, i, lane, wid, s_beg, s_pre, s_j, s_c);
+      int *sj = s_row + wid * 2 * b.maxnb;
+      m = build_row(g, b, parity, i, lane, s_beg[wid], s_pre[wid], sj, sj + b.maxnb);
       acc[11] = 0.5 * m;
     } else {
       m = b.nbr_cnt[i];
     }
-    const R *intra = reinterpret_cast<const R *>(sizeof(R) == 4 ? (const void *)b.intraf
-                                                                 : (const void *)b.intra);
+    const double *reci = b.rec + (size_t)i * b.rs;
     R ri[T::NS][3];
-#pragma unroll
-    for (int s = 0; s < T::NS; s++)
-#pragma unroll
-      for (int a = 0; a < 3; a++) ri[s][a] = intra[(s * 3 + a) * n + i];
-    const double xi = b.com_p[i], yi = b.com_p[n + i], zi = b.com_p[2 * n + i];
+    RecSites<M, R>::load(reci, ri);
+    const double xi = reci[0], yi = reci[1], zi = reci[2];
     const int *nj = b.nbr_j + (size_t)i * b.maxnb;
     const int *nc = b.nbr_code + (size_t)i * b.maxnb;
     double F[3] = {0, 0, 0}, Ti[3] = {0, 0, 0}, ep = 0, v[9] = {0}, npi = 0;
-    for (int k = lane; k < m; k += 64) {
-      const int j = nj[k];
-      const int code = nc[k];
-      const int fa = code & 1023, fb = (code >> 10) & 1023, fc = (code >> 20) & 1023;
-      const double ox = fa == 0 ? -0.0 : (double)(fa - 512) * g.L;
-      const double oy = fb == 0 ? -0.0 : (double)(fb - 512) * g.L;
-      const double oz = fc == 0 ? -0.0 : (double)(fc - 512) * g.L;
-      const double d0 = (xi - b.com_p[j]) - ox;
-      const double d1 = (yi - b.com_p[n + j]) - oy;
-      const double d2 = (zi - b.com_p[2 * n + j]) - oz;
-      const double dd64 = d0 * d0 + d1 * d1 + d2 * d2;
-      if (!(dd64 < g.rc2)) continue;
+    // two-stage software pipeline: the record of the next entry and the
+    // index of the one after are in flight while a pair is evaluated
+    int k = lane;
+    int jA = k < m ? nj[k] : i, cA = k < m ? nc[k] : 0;
+    int jB = k + 64 < m ? nj[k + 64] : i, cB = k + 64 < m ? nc[k + 64] : 0;
+    const double *rA = b.rec + (size_t)jA * b.rs;
+    double xA = rA[0], yA = rA[1], zA = rA[2];
+    R sA[T::NS][3];
+    RecSites<M, R>::load(rA, sA);
+    for (; k < m; k += 64) {
+      const int code = cA;
+      const double xj = xA, yj = yA, zj = zA;
       R rj[T::NS][3];
 #pragma unroll
       for (int s = 0; s < T::NS; s++)
 #pragma unroll
-        for (int a = 0; a < 3; a++) rj[s][a] = intra[(s * 3 + a) * n + j];
-      R hh, dh;
-      cutoff_sw<R>(g, R(dd64), hh, dh);
-      R fi[T::NS][3], fj[T::NS][3];
-      const R e = pair_sites<M, R>(mt, hh, dh, R(d0), R(d1), R(d2), ri, rj, fi, fj);
-      R Fx = 0, Fy = 0, Fz = 0, tix = 0, tiy = 0, tiz = 0;
-#pragma unroll
-      for (int s = 0; s < T::NS; s++) {
-        Fx += fi[s][0]; Fy += fi[s][1]; Fz += fi[s][2];
-        tix += ri[s][1] * fi[s][2] - ri[s][2] * fi[s][1];
-        tiy += ri[s][2] * fi[s][0] - ri[s][0] * fi[s][2];
-        tiz += ri[s][0] * fi[s][1] - ri[s][1] * fi[s][0];
+        for (int a = 0; a < 3; a++) rj[s][a] = sA[s][a];
+      if (k + 64 < m) {
+        const double *rn = b.rec + (size_t)jB * b.rs;
+        xA = rn[0]; yA = rn[1]; zA = rn[2];
+        RecSites<M, R>::load(rn, sA);
+        cA = cB;
+        if (k + 128 < m) {
+          jB = nj[k + 128];
+          cB = nc[k + 128];
+        }
       }
-      F[0] += (double)Fx; F[1] += (double)Fy; F[2] += (double)Fz;
-      Ti[0] += (double)tix; Ti[1] += (double)tiy; Ti[2] += (double)tiz;
-      ep += (double)e;
-      npi += 1.0;
-      v[0] += d0 * (double)Fx; v[1] += d0 * (double)Fy; v[2] += d0 * (double)Fz;
-      v[3] += d1 * (double)Fx; v[4] += d1 * (double)Fy; v[5] += d1 * (double)Fz;
-      v[6] += d2 * (double)Fx; v[7] += d2 * (double)Fy; v[8] += d2 * (double)Fz;
+      const int fa = code & 1023, fb = (code >> 10) & 1023, fc = (code >> 20) & 1023;
+      const double ox = fa == 0 ? -0.0 : (double)(fa - 512) * g.L;
+      const double oy = fb == 0 ? -0.0 : (double)(fb - 512) * g.L;
+      const double oz = fc == 0 ? -0.0 : (double)(fc - 512) * g.L;
+      const double d0 = (xi - xj) - ox;
+      const double d1 = (yi - yj) - oy;
+      const double d2 = (zi - zj) - oz;
+      const double dd64 = d0 * d0 + d1 * d1 + d2 * d2;
+      if (dd64 < g.rc2) {
+        R hh, dh;
+        cutoff_sw<R>(g, R(dd64), hh, dh);
+        R fi[T::NS][3], fj[T::NS][3];
+        const R e = pair_sites<M, R>(mt, hh, dh, R(d0), R(d1), R(d2), ri, rj, fi, fj);
+        R Fx = 0, Fy = 0, Fz = 0, tix = 0, tiy = 0, tiz = 0;
+#pragma unroll
+        for (int s = 0; s < T::NS; s++) {
+          Fx += fi[s][0]; Fy += fi[s][1]; Fz += fi[s][2];
+          tix += ri[s][1] * fi[s][2] - ri[s][2] * fi[s][1];
+          tiy += ri[s][2] * fi[s][0] - ri[s][0] * fi[s][2];
+          tiz += ri[s][0] * fi[s][1] - ri[s][1] * fi[s][0];
+        }
+        F[0] += (double)Fx; F[1] += (double)Fy; F[2] += (double)Fz;
+        Ti[0] += (double)tix; Ti[1] += (double)tiy; Ti[2] += (double)tiz;
+        ep += (double)e;
+        npi += 1.0;
+        v[0] += d0 * (double)Fx; v[1] += d0 * (double)Fy; v[2] += d0 * (double)Fz;
+        v[3] += d1 * (double)Fx; v[4] += d1 * (double)Fy; v[5] += d1 * (double)Fz;
+        v[6] += d2 * (double)Fx; v[7] += d2 * (double)Fy; v[8] += d2 * (double)Fz;
+      }
     }
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-      F[a] = wsum(F[a]);
-      Ti[a] = wsum(Ti[a]);
+      F[a] = wsum63(F[a]);
+      Ti[a] = wsum63(Ti[a]);
     }
-    acc[0] = 0.5 * wsum(ep);
+    acc[0] = 0.5 * wsum63(ep);
 #pragma unroll
-    for (int a = 0; a < 9; a++) acc[1 + a] = 0.5 * wsum(v[a]);
-    acc[10] = 0.5 * wsum(npi);
-    if (lane == 0) {
+    for (int a = 0; a < 9; a++) acc[1 + a] = 0.5 * wsum63(v[a]);
+    acc[10] = 0.5 * wsum63(npi);
+    if (lane == 63) {
       if (out) {
         // molecule force (lab) and body torque, as rigid_collectforce
         double t[9];
@@ -576,8 +714,9 @@ __global__ __launch_bounds__(256) void k_md_main(ModelTable mt, Geom g, MdBuf b,
       }
     }
   }
-  // block partials: lane 0 of each wave holds the wave's values
-  if (lane == 0)
+  // block partials: lane 63 of each wave holds the wave's values
+  if (lane == 63)
+#pragma unroll
     for (int q = 0; q < 12; q++) s_red[wid][q] = acc[q];
   __syncthreads();
   if (threadIdx.x < 12) {
@@ -599,7 +738,8 @@ void md_launch_close(System &s, int mode, int parity) {
 
 template <int M, typename R>
 static void md_main_t(System &s, double *out, int parity) {
-  k_md_main<M, R><<<s.nblk_md, 256, 0, s.stream>>>(s.mt, s.geo_md, s.md, out, parity);
+  const size_t lds = sizeof(int) * 4 * 2 * (size_t)s.md.maxnb;
+  k_md_main<M, R><<<s.nblk_md, 256, lds, s.stream>>>(s.mt, s.geo_md, s.md, out, parity);
 }
 
 void md_launch_main(System &s, double *out, int parity) {
