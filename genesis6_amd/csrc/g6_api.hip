@@ -206,7 +206,7 @@ static int alloc_all(System &s) {
   AL(m.nbr_j, sizeof(int) * (size_t)n * m.maxnb);
   AL(m.nbr_code, sizeof(int) * (size_t)n * m.maxnb);
   AL(m.nbr_cnt, sizeof(int) * n);
-  AL(m.blockpart, sizeof(double) * 16 * s.nblk_md);
+  AL(m.blockpart, sizeof(double) * 2 * 16 * s.nblk_md);
   AL(m.blockek, sizeof(double) * 4 * ((n + 63) / 64));
   AL(m.frc, sizeof(double) * 6 * n);
   AL(m.nose, sizeof(double) * 9);
@@ -269,11 +269,11 @@ static void launch_steps(System &s, int k) {
   for (int q = 0; q < k; q++) {
     const int p = q & 1;
     md_launch_pre(s, q > 0, true, true, p);
-    md_launch_close(s, (q > 0 ? 1 : 0) | 2 | 4, p);
-    md_launch_main(s, nullptr, p);
+    if (q == 0) md_launch_close(s, 2 | 4, p, 1 - p);
+    md_launch_main(s, nullptr, p, q > 0 ? (1 | 2 | 4) : 0);
   }
   md_launch_pre(s, true, false, false, 0);
-  md_launch_close(s, 1 | 8, 0);
+  md_launch_close(s, 1 | 8, 0, (k - 1) & 1);
 }
 
 // steps per captured hipGraph (G6_GRAPH_STEPS; 0 disables graphs)
@@ -520,12 +520,12 @@ int g6_step_timed(g6_handle h, int nsteps, double *t) {
     // predictor share a kernel in g6_step; here they are timed apart)
     G6_CHECK(hipEventRecord(s->ev[0], s->stream));
     md_launch_pre(*s, false, true, true, 0);
-    md_launch_close(*s, 2 | 4, 0);
+    md_launch_close(*s, 2 | 4, 0, 1);
     G6_CHECK(hipEventRecord(s->ev[1], s->stream));
-    md_launch_main(*s, nullptr, 0);
+    md_launch_main(*s, nullptr, 0, 0);
     G6_CHECK(hipEventRecord(s->ev[2], s->stream));
     md_launch_pre(*s, true, false, false, 0);
-    md_launch_close(*s, 1 | 8, 0);
+    md_launch_close(*s, 1 | 8, 0, 0);
     G6_CHECK(hipEventRecord(s->ev[3], s->stream));
     G6_CHECK(hipEventSynchronize(s->ev[3]));
     float a, b, c, e;
@@ -557,8 +557,8 @@ int g6_eval_forces(g6_handle h, double *force, double *torque, double *thermo) {
   int one = 1;
   G6_CHECK(hipMemcpyAsync(s->md.flags + 2, &one, sizeof(int), hipMemcpyHostToDevice, s->stream));
   md_launch_pre(*s, false, false, true, 0);
-  md_launch_main(*s, out, 0);
-  md_launch_close(*s, 16 | 8, 0);
+  md_launch_main(*s, out, 0, 0);
+  md_launch_close(*s, 16 | 8, 0, 0);
   G6_CHECK(hipGetLastError());
   G6_CHECK(hipMemcpyAsync(force, out, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s->stream));
   G6_CHECK(hipMemcpyAsync(torque, out + 3 * n, sizeof(double) * 3 * n, hipMemcpyDeviceToHost,

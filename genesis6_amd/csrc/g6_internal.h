@@ -74,7 +74,7 @@ struct MdBuf {
   int *nbr_j, *nbr_code, *nbr_cnt;
   int maxnb;
   double *frc;        // [6][n] force and torque (lab) of the last k_md_main
-  double *blockpart;  // [nblk][16] k_md_main block partials
+  double *blockpart;  // [2][nblk][16] k_md_main block partials, by parity
   double *blockek;    // [nblk_pre][4] k_md_pre kinetic partials
   double *nose, *thermo;
   int *flags;  // [0] error code, [2 + parity] rebuild rows this step
@@ -117,9 +117,18 @@ struct System {
 bool make_model_table(int model, int prec, ModelTable *mt);
 
 // MD launchers (stream-ordered, graph-capturable: no memset, no host sync)
+// Arguments of the step-close work (k_md_close, or block 0 of k_md_main).
+struct CloseArgs {
+  int mode, nose_active, nblk_pre, nblk_main;
+  const double *bpart;  // pair partials of the k_md_main being closed
+  double dt;
+};
 void md_launch_pre(System &s, bool correct, bool predict, bool sites, int parity);
-void md_launch_close(System &s, int mode, int parity);
-void md_launch_main(System &s, double *out, int parity);
+// close of the step whose k_md_main wrote the partials of src_parity
+void md_launch_close(System &s, int mode, int parity, int src_parity);
+// k_md_main of this parity; close_mode != 0 closes the previous step
+// (partials of parity 1 - parity) in block 0
+void md_launch_main(System &s, double *out, int parity, int close_mode);
 // host-array entry points of the module seam
 int pairlist_once(System &s, const double *com_h, int64_t cap, int64_t *npair0,
                   int64_t *npair, int32_t *pi, int32_t *pj, double *ox,

@@ -1,4 +1,4 @@
-// The MD step (the path g6_step runs), three kernels per step:
+// The MD step (the path g6_step runs), two kernels per step:
 //
 //   k_md_pre   one thread per molecule, state held in registers:
 //              rigid_correct with the forces of the previous k_md_main and
@@ -7,9 +7,10 @@
 //              FP64, site offsets in the pair precision), Address_Assign
 //              binning into fixed-capacity cell lists of this step's parity,
 //              and the Verlet-skin test that decides whether rows are rebuilt.
-//   k_md_close one block: reduces the block partials in a fixed order into
+//   close      one block: reduces the block partials in a fixed order into
 //              the thermo record, Nose_Correct / Nose_Predict, and clears the
-//              cell lists of the other parity.
+//              cell lists of the other parity (block 0 of k_md_main, or
+//              k_md_close).  Pair partials alternate between two parities.
 //   k_md_main  one wavefront per molecule i:
 //              (1) if a rebuild is due, i's own neighbour row: every j != i
 //                  of the cells whose box lies within rc + skin of i (a 5^3
@@ -23,8 +24,10 @@
 //              (3) wave reduction of i's force and lab torque (the work of
 //                  rigid_collectforce), stored for the corrector;
 //              (4) block partials of energy and virial.
-// A step is k_md_pre(correct previous, predict) + k_md_close + k_md_main; a
-// run of steps ends with a correct-only k_md_pre + k_md_close, so every
+// A step is k_md_pre(correct previous, predict) + k_md_main, whose block 0
+// first does the close work of the previous step (k_md_close runs on its own
+// only around the first and last step of a run); a run of steps ends with a
+// correct-only k_md_pre + k_md_close, so every
 // g6_step call leaves the state of the reference's loop end (after
 // Nose_Correct, Mixture.F90:2282).  Cell lists and rebuild flags alternate
 // between two parities so that no kernel needs a grid-wide barrier.
@@ -68,13 +71,6 @@ __device__ inline double wsum63(double v) {
   v += dpp_d<0x142, 0xA>(v);
   v += dpp_d<0x143, 0xC>(v);
   return v;
-}
-
-__device__ inline double lane63(double v) {
-  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
-  const unsigned lo = __builtin_amdgcn_readlane((int)(unsigned)u, 63);
-  const unsigned hi = __builtin_amdgcn_readlane((int)(unsigned)(u >> 32), 63);
-  return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
 __device__ inline void wsync() {
@@ -338,25 +334,29 @@ __global__ __launch_bounds__(64) void k_md_pre(ModelTable mt, Geom g, MdBuf b, d
   }
 }
 
-// ---- kernel R: one block closing the step ------------------------------------
+// ---- step close: one block of 256 threads ---------------------------------
 // mode bits: 1 the kinetic partials of k_md_pre + the pair partials of the
-// previous k_md_main form the thermo record, then Nose_Correct; 2 Nose_Predict;
-// 4 clear the cell counts and rebuild flag of parity 1-p (8: both parities);
-// 16 force evaluation: the pair partials alone go to the thermo record.
-__global__ __launch_bounds__(256) void k_md_close(ModelTable mt, MdBuf b, int n, double dt,
-                                                  int nblk_pre, int nblk_main, int ncell,
-                                                  int nose_active, int mode, int parity) {
-  __shared__ double s_red[4][16];
+// previous k_md_main (bpart) form the thermo record, then Nose_Correct;
+// 2 Nose_Predict; 4 clear the cell counts and rebuild flag of parity 1-p
+// (8: both parities); 16 force evaluation: the pair partials alone go to the
+// thermo record.  Runs as its own kernel (k_md_close) or at the start of
+// block 0 of k_md_main, which neither reads nor writes what it touches.
+__device__ inline void close_body(const ModelTable &mt, const MdBuf &b, int n, double dt,
+                                  int nblk_pre, const double *bpart, int nblk_main, int ncell,
+                                  int nose_active, int mode, int parity,
+                                  double (&s_red)[4][16]) {
   const int wid = threadIdx.x >> 6;
   double a2[16];
 #pragma unroll
   for (int q = 0; q < 16; q++) a2[q] = 0.0;
   if (mode & 17) {
+#pragma unroll 4
     for (int k = threadIdx.x; k < nblk_main; k += 256)
 #pragma unroll
-      for (int q = 0; q < 12; q++) a2[q] += b.blockpart[(size_t)k * 16 + q];
+      for (int q = 0; q < 12; q++) a2[q] += bpart[(size_t)k * 16 + q];
   }
   if (mode & 1) {
+#pragma unroll 2
     for (int k = threadIdx.x; k < nblk_pre; k += 256)
 #pragma unroll
       for (int q = 0; q < 4; q++) a2[12 + q] += b.blockek[(size_t)k * 4 + q];
@@ -375,46 +375,55 @@ __global__ __launch_bounds__(256) void k_md_close(ModelTable mt, MdBuf b, int n,
     }
   }
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  double r[16];
-  for (int q = 0; q < 16; q++) r[q] = s_red[0][q] + s_red[1][q] + s_red[2][q] + s_red[3][q];
-  double *z = b.nose, *th = b.thermo;
-  if (mode & 17) {
-    th[0] = r[0];
-    for (int q = 0; q < 9; q++) th[4 + q] = r[1 + q];
-    th[14] = r[10];
-    if (r[11] > 0) th[15] = r[11];
-  }
-  if (mode & 1) {
-    const double ep = r[0];
-    const double ekt = r[12] * mt.mass * 0.5 * (1.0 / (dt * dt));
-    const double ekr = (r[13] * mt.Ixx + r[14] * mt.Iyy + r[15] * mt.Izz) * 0.5;
-    const double dof = 6.0 * n;
-    const double temp = 10.0 * 2.0 * (ekt + ekr) / (G6_RGAS_MD * dof);
-    double h = (ep + ekt + ekr) * 10.0;
-    if (nose_active) {
-      const double zetaf = (temp - z[0]) / z[1] - z[2] * z[3];
-      const double dz2 = zetaf - z[4];
-      z[3] = z[3] + kG4_0 * dz2;
-      z[4] = zetaf;
-      z[5] = z[5] + kG4_2 * dz2;
-      z[6] = z[6] + kG4_3 * dz2;
-      z[7] = z[7] + kG4_4 * dz2;
-      z[8] = z[8] + z[3];
-      h = h + (dof * z[0] * (z[8] - 0.5 * z[3]) * dt) * G6_RGAS_MD +
-          (z[3] * z[3] * z[1] * dt * dof * 0.5) * G6_RGAS_MD;
+  if (threadIdx.x == 0) {
+    double r[16];
+    for (int q = 0; q < 16; q++) r[q] = s_red[0][q] + s_red[1][q] + s_red[2][q] + s_red[3][q];
+    double *z = b.nose, *th = b.thermo;
+    if (mode & 17) {
+      th[0] = r[0];
+      for (int q = 0; q < 9; q++) th[4 + q] = r[1 + q];
+      th[14] = r[10];
+      if (r[11] > 0) th[15] = r[11];
     }
-    th[1] = ekt;
-    th[2] = ekr;
-    th[3] = temp;
-    th[13] = h;
+    if (mode & 1) {
+      const double ep = r[0];
+      const double ekt = r[12] * mt.mass * 0.5 * (1.0 / (dt * dt));
+      const double ekr = (r[13] * mt.Ixx + r[14] * mt.Iyy + r[15] * mt.Izz) * 0.5;
+      const double dof = 6.0 * n;
+      const double temp = 10.0 * 2.0 * (ekt + ekr) / (G6_RGAS_MD * dof);
+      double h = (ep + ekt + ekr) * 10.0;
+      if (nose_active) {
+        const double zetaf = (temp - z[0]) / z[1] - z[2] * z[3];
+        const double dz2 = zetaf - z[4];
+        z[3] = z[3] + kG4_0 * dz2;
+        z[4] = zetaf;
+        z[5] = z[5] + kG4_2 * dz2;
+        z[6] = z[6] + kG4_3 * dz2;
+        z[7] = z[7] + kG4_4 * dz2;
+        z[8] = z[8] + z[3];
+        h = h + (dof * z[0] * (z[8] - 0.5 * z[3]) * dt) * G6_RGAS_MD +
+            (z[3] * z[3] * z[1] * dt * dof * 0.5) * G6_RGAS_MD;
+      }
+      th[1] = ekt;
+      th[2] = ekr;
+      th[3] = temp;
+      th[13] = h;
+    }
+    if ((mode & 2) && nose_active) {
+      z[3] = z[3] + z[4] + z[5] + z[6] + z[7];
+      z[4] = z[4] + 2.0 * z[5] + 3.0 * z[6] + 4.0 * z[7];
+      z[5] = z[5] + 3.0 * z[6] + 6.0 * z[7];
+      z[6] = z[6] + 4.0 * z[7];
+    }
   }
-  if ((mode & 2) && nose_active) {
-    z[3] = z[3] + z[4] + z[5] + z[6] + z[7];
-    z[4] = z[4] + 2.0 * z[5] + 3.0 * z[6] + 4.0 * z[7];
-    z[5] = z[5] + 3.0 * z[6] + 6.0 * z[7];
-    z[6] = z[6] + 4.0 * z[7];
-  }
+}
+
+__global__ __launch_bounds__(256) void k_md_close(ModelTable mt, MdBuf b, int n, double dt,
+                                                  int nblk_pre, const double *bpart,
+                                                  int nblk_main, int ncell, int nose_active,
+                                                  int mode, int parity) {
+  __shared__ double s_red[4][16];
+  close_body(mt, b, n, dt, nblk_pre, bpart, nblk_main, ncell, nose_active, mode, parity, s_red);
 }
 
 // ---- kernel B --------------------------------------------------------------
@@ -601,11 +610,15 @@ struct RecSites<M, double> {
 
 template <int M, typename R>
 __global__ __launch_bounds__(256) void k_md_main(ModelTable mt, Geom g, MdBuf b, double *out,
-                                                 int parity) {
+                                                 int parity, CloseArgs ca) {
   using T = Traits<M>;
   extern __shared__ int s_row[];  // [4][2][maxnb]: row sort buffers
   __shared__ int s_beg[4][128], s_pre[4][132];
   __shared__ double s_red[4][12];
+  __shared__ double s_close[4][16];
+  if (ca.mode && blockIdx.x == 0)
+    close_body(mt, b, g.n, ca.dt, ca.nblk_pre, ca.bpart, ca.nblk_main, g.ncell, ca.nose_active,
+               ca.mode, parity, s_close);
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int n = g.n;
   const int i = blockIdx.x * 4 + wid;
@@ -721,7 +734,8 @@ __global__ __launch_bounds__(256) void k_md_main(ModelTable mt, Geom g, MdBuf b,
   __syncthreads();
   if (threadIdx.x < 12) {
     const int q = threadIdx.x;
-    b.blockpart[(size_t)blockIdx.x * 16 + q] = s_red[0][q] + s_red[1][q] + s_red[2][q] + s_red[3][q];
+    b.blockpart[((size_t)parity * gridDim.x + blockIdx.x) * 16 + q] =
+        s_red[0][q] + s_red[1][q] + s_red[2][q] + s_red[3][q];
   }
 }
 
@@ -731,26 +745,39 @@ void md_launch_pre(System &s, bool correct, bool predict, bool sites, int parity
                                                   predict ? 1 : 0, sites ? 1 : 0, parity);
 }
 
-void md_launch_close(System &s, int mode, int parity) {
-  k_md_close<<<1, 256, 0, s.stream>>>(s.mt, s.md, s.n, s.dt, (s.n + 63) / 64, s.nblk_md,
-                                      s.geo_md.ncell, s.nose_active ? 1 : 0, mode, parity);
+static CloseArgs close_args(System &s, int mode, int src_parity) {
+  CloseArgs ca;
+  ca.mode = mode;
+  ca.nose_active = s.nose_active ? 1 : 0;
+  ca.nblk_pre = (s.n + 63) / 64;
+  ca.nblk_main = s.nblk_md;
+  ca.bpart = s.md.blockpart + (size_t)src_parity * s.nblk_md * 16;
+  ca.dt = s.dt;
+  return ca;
+}
+
+void md_launch_close(System &s, int mode, int parity, int src_parity) {
+  const CloseArgs ca = close_args(s, mode, src_parity);
+  k_md_close<<<1, 256, 0, s.stream>>>(s.mt, s.md, s.n, s.dt, ca.nblk_pre, ca.bpart, ca.nblk_main,
+                                      s.geo_md.ncell, ca.nose_active, mode, parity);
 }
 
 template <int M, typename R>
-static void md_main_t(System &s, double *out, int parity) {
+static void md_main_t(System &s, double *out, int parity, const CloseArgs &ca) {
   const size_t lds = sizeof(int) * 4 * 2 * (size_t)s.md.maxnb;
-  k_md_main<M, R><<<s.nblk_md, 256, lds, s.stream>>>(s.mt, s.geo_md, s.md, out, parity);
+  k_md_main<M, R><<<s.nblk_md, 256, lds, s.stream>>>(s.mt, s.geo_md, s.md, out, parity, ca);
 }
 
-void md_launch_main(System &s, double *out, int parity) {
+void md_launch_main(System &s, double *out, int parity, int close_mode) {
+  const CloseArgs ca = close_args(s, close_mode, 1 - parity);
   if (s.prec == G6_FP32) {
-    if (s.model == G6_TIP4P) md_main_t<G6_TIP4P, float>(s, out, parity);
-    else if (s.model == G6_TIP5P) md_main_t<G6_TIP5P, float>(s, out, parity);
-    else md_main_t<G6_ST2, float>(s, out, parity);
+    if (s.model == G6_TIP4P) md_main_t<G6_TIP4P, float>(s, out, parity, ca);
+    else if (s.model == G6_TIP5P) md_main_t<G6_TIP5P, float>(s, out, parity, ca);
+    else md_main_t<G6_ST2, float>(s, out, parity, ca);
   } else {
-    if (s.model == G6_TIP4P) md_main_t<G6_TIP4P, double>(s, out, parity);
-    else if (s.model == G6_TIP5P) md_main_t<G6_TIP5P, double>(s, out, parity);
-    else md_main_t<G6_ST2, double>(s, out, parity);
+    if (s.model == G6_TIP4P) md_main_t<G6_TIP4P, double>(s, out, parity, ca);
+    else if (s.model == G6_TIP5P) md_main_t<G6_TIP5P, double>(s, out, parity, ca);
+    else md_main_t<G6_ST2, double>(s, out, parity, ca);
   }
 }
 
