@@ -91,7 +91,20 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run five device-to-device copies of this many MiB and exit "
+                         "(a plain copy kernel of known size, which calibrates the "
+                         "FETCH_SIZE / WRITE_SIZE counters of rocprofv3 --pmc)")
     a = ap.parse_args()
+    if a.copy_probe_mib > 0:
+        import torch
+        x = torch.ones(a.copy_probe_mib << 18, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print("copy probe: 5 x %d MiB device-to-device" % a.copy_probe_mib, file=sys.stderr)
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the system is not sharded across GPUs "
               "(see DESIGN.md, 'More than one GPU'); run with --gpus 1", file=sys.stderr)

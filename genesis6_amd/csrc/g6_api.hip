@@ -361,6 +361,8 @@ int g6_init(g6_handle *h, int model, int n, double box_l, double rc, double tape
     b.maxnb = ((int)(1.5 * nb + 64.0) + 63) / 64 * 64;
     if (b.maxnb > n) b.maxnb = n;
     b.half_skin2 = skin > 0 ? 0.25 * skin * skin : 0.0;
+    s->pkf = make_pair_k<float>(s->mt, m);
+    s->pkd = make_pair_k<double>(s->mt, m);
     s->nblk_md = (n + 3) / 4;
   }
   int rcode = alloc_all(*s);

@@ -13,8 +13,9 @@
 //   nbr_j/nbr_code[n][maxnb], nbr_cnt[n]: full (both-direction) Verlet rows
 //     within rc + skin; nbr_code packs the image shift ox = kx*L etc. as
 //     10-bit fields (kx+512) | (ky+512)<<10 | (kz+512)<<20 (COMs are
-//     unwrapped, so |k| can exceed 1); field value 0 encodes the -0.0 shift
-//     that dnint gives for small negative differences
+//     unwrapped, so |k| can exceed 1).  The -0.0 shift that dnint gives for
+//     small negative differences is stored as 0: x - (-0.0) and x - 0.0 are
+//     equal for every x but a zero x, whose sign no force term can see
 //   blockpart[nblk_md][16], blockek[nblk_pre][4]: per-block energy/virial and
 //     kinetic partials, reduced in a fixed order by one block (k_md_close)
 // Reference-list entry point (g6_pairlist.hip): cell residents in molecule
@@ -45,6 +46,16 @@ struct ModelTable {
   double bx[kMaxSite], by[kMaxSite], bz[kMaxSite];  // body-frame sites
   double Ixx, Iyy, Izz, mass;
   double rl, ru;  // ST2 short-range switch on the COM distance
+};
+
+// Pair constants of the MD force kernel in its arithmetic type R, rounded
+// once on the host (kernel arguments, so they sit in scalar registers).
+template <typename R>
+struct PairK {
+  R qq[kMaxCoul];
+  R lja, ljb, lja12, ljb6;  // e = a/r^12 + b/r^6; force factors 12a, 6b
+  R rin, rout, rin2, rout2, c1, c2;  // switch S between rin and rout
+  R rl, ru, rl2, ru2, w3inv;         // ST2 switch on the COM distance
 };
 
 // Grid geometry and cutoff constants of one system.
@@ -105,6 +116,8 @@ struct System {
   int *flags = nullptr;   // = md.flags
   int *lflags = nullptr;  // g6_pairlist: [0] error code, [1] build gate
   int nblk_md = 0;
+  PairK<float> pkf{};
+  PairK<double> pkd{};
   hipStream_t stream = nullptr;
   hipGraphExec_t graph = nullptr;
   int graph_steps = 0;
@@ -115,6 +128,28 @@ struct System {
 
 // model tables (g6_api.hip)
 bool make_model_table(int model, int prec, ModelTable *mt);
+template <typename R>
+PairK<R> make_pair_k(const ModelTable &mt, const Geom &g) {
+  PairK<R> k{};
+  for (int t = 0; t < kMaxCoul; t++) k.qq[t] = R(mt.qq[t]);
+  k.lja = R(mt.lj_a);
+  k.ljb = R(mt.lj_b);
+  k.lja12 = R(mt.lj_a * 12.0);
+  k.ljb6 = R(mt.lj_b * 6.0);
+  k.rin = R(g.rin);
+  k.rout = R(g.rout);
+  k.rin2 = R(g.rin * g.rin);
+  k.rout2 = R(g.rout * g.rout);
+  k.c1 = R(g.c1);
+  k.c2 = R(g.c2);
+  k.rl = R(mt.rl);
+  k.ru = R(mt.ru);
+  k.rl2 = k.rl * k.rl;
+  k.ru2 = k.ru * k.ru;
+  const R w = k.ru - k.rl;
+  k.w3inv = w > R(0) ? R(1) / (w * w * w) : R(0);
+  return k;
+}
 
 // MD launchers (stream-ordered, graph-capturable: no memset, no host sync)
 // Arguments of the step-close work (k_md_close, or block 0 of k_md_main).

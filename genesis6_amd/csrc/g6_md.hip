@@ -49,9 +49,19 @@ constexpr double kG4_0 = 251.0 / 720.0, kG4_2 = 11.0 / 12.0, kG4_3 = 1.0 / 3.0,
                  kG4_4 = 1.0 / 24.0;
 #define G6_RGAS_MD (6.0225e23 * 1.380066e-23)
 
-// DPP move of a double across lanes (both halves); rows outside RM read 0.
-template <int CTRL, int RM = 0xF>
+// DPP move of a double across lanes (both halves); every row written.
+template <int CTRL>
 __device__ inline double dpp_d(double v) {
+  const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)u, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, false);
+  return __builtin_bit_cast(double,
+                            ((unsigned long long)(unsigned)hi << 32) | (unsigned long long)(unsigned)lo);
+}
+
+// As dpp_d for the rows selected by RM; the other rows read 0.
+template <int CTRL, int RM>
+__device__ inline double dpp_dm(double v) {
   const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
   const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, RM, 0xF, false);
   const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, RM, 0xF, false);
@@ -68,8 +78,8 @@ __device__ inline double wsum63(double v) {
   v += dpp_d<0x4E>(v);
   v += dpp_d<0x141>(v);
   v += dpp_d<0x140>(v);
-  v += dpp_d<0x142, 0xA>(v);
-  v += dpp_d<0x143, 0xC>(v);
+  v += dpp_dm<0x142, 0xA>(v);
+  v += dpp_dm<0x143, 0xC>(v);
   return v;
 }
 
@@ -525,10 +535,7 @@ __device__ inline int build_row(const Geom &g, const MdBuf &b, int parity, int i
         d2 = d2 - k2 * g.L;
         const double dd = d0 * d0 + d1 * d1 + d2 * d2;
         pass = dd < g.rr;
-        const int f0 = (k0 == 0.0 && signbit(k0)) ? 0 : (int)k0 + 512;
-        const int f1 = (k1 == 0.0 && signbit(k1)) ? 0 : (int)k1 + 512;
-        const int f2 = (k2 == 0.0 && signbit(k2)) ? 0 : (int)k2 + 512;
-        code = f0 | (f1 << 10) | (f2 << 20);
+        code = ((int)k0 + 512) | (((int)k1 + 512) << 10) | (((int)k2 + 512) << 20);
         if (pass && (fabs(k0) > 511.0 || fabs(k1) > 511.0 || fabs(k2) > 511.0)) b.flags[0] = 2;
       }
     }
@@ -608,13 +615,24 @@ struct RecSites<M, double> {
   }
 };
 
+// Sum over the 16 lanes of a DPP row in a fixed order; every lane of the
+// row receives the row total.
+__device__ inline double rowsum16(double v) {
+  v += dpp_d<0xB1>(v);
+  v += dpp_d<0x4E>(v);
+  v += dpp_d<0x141>(v);
+  v += dpp_d<0x140>(v);
+  return v;
+}
+
 template <int M, typename R>
-__global__ __launch_bounds__(256) void k_md_main(ModelTable mt, Geom g, MdBuf b, double *out,
-                                                 int parity, CloseArgs ca) {
+__global__ __launch_bounds__(256) void k_md_main(ModelTable mt, Geom g, MdBuf b, PairK<R> pk,
+                                                 double *out, int parity, CloseArgs ca) {
   using T = Traits<M>;
   extern __shared__ int s_row[];  // [4][2][maxnb]: row sort buffers
   __shared__ int s_beg[4][128], s_pre[4][132];
-  __shared__ double s_red[4][12];
+  __shared__ double s_acc[10][256];  // per-lane energy and virial partials
+  __shared__ double s_cnt[4][2];
   __shared__ double s_close[4][16];
   if (ca.mode && blockIdx.x == 0)
     close_body(mt, b, g.n, ca.dt, ca.nblk_pre, ca.bpart, ca.nblk_main, g.ncell, ca.nose_active,
@@ -622,15 +640,14 @@ __global__ __launch_bounds__(256) void k_md_main(ModelTable mt, Geom g, MdBuf b,
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int n = g.n;
   const int i = blockIdx.x * 4 + wid;
-  double acc[12];  // ep, vir[9], interacting pairs, list pairs (each halved)
-#pragma unroll
-  for (int q = 0; q < 12; q++) acc[q] = 0.0;
+  double ep = 0, v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  int npair = 0, nlist = 0;  // wave-uniform counts
   if (i < n) {
     int m;
     if (b.flags[2 + parity]) {
       int *sj = s_row + wid * 2 * b.maxnb;
       m = build_row(g, b, parity, i, lane, s_beg[wid], s_pre[wid], sj, sj + b.maxnb);
-      acc[11] = 0.5 * m;
+      nlist = m;
     } else {
       m = b.nbr_cnt[i];
     }
@@ -640,62 +657,37 @@ __global__ __launch_bounds__(256) void k_md_main(ModelTable mt, Geom g, MdBuf b,
     const double xi = reci[0], yi = reci[1], zi = reci[2];
     const int *nj = b.nbr_j + (size_t)i * b.maxnb;
     const int *nc = b.nbr_code + (size_t)i * b.maxnb;
-    double F[3] = {0, 0, 0}, Ti[3] = {0, 0, 0}, ep = 0, v[9] = {0}, npi = 0;
-    // two-stage software pipeline: the record of the next entry and the
-    // index of the one after are in flight while a pair is evaluated
-    int k = lane;
-    int jA = k < m ? nj[k] : i, cA = k < m ? nc[k] : 0;
-    int jB = k + 64 < m ? nj[k + 64] : i, cB = k + 64 < m ? nc[k + 64] : 0;
-    const double *rA = b.rec + (size_t)jA * b.rs;
-    double xA = rA[0], yA = rA[1], zA = rA[2];
-    R sA[T::NS][3];
-    RecSites<M, R>::load(rA, sA);
-    for (; k < m; k += 64) {
-      const int code = cA;
-      const double xj = xA, yj = yA, zj = zA;
-      R rj[T::NS][3];
-#pragma unroll
-      for (int s = 0; s < T::NS; s++)
-#pragma unroll
-        for (int a = 0; a < 3; a++) rj[s][a] = sA[s][a];
+    double F[3] = {0, 0, 0}, Ti[3] = {0, 0, 0};
+    // the row entry of the next pass is fetched while this one is evaluated
+    int jn = lane < m ? nj[lane] : i, cn = lane < m ? nc[lane] : 0;
+    for (int k = lane; k < m; k += 64) {
+      const int j = jn, code = cn;
       if (k + 64 < m) {
-        const double *rn = b.rec + (size_t)jB * b.rs;
-        xA = rn[0]; yA = rn[1]; zA = rn[2];
-        RecSites<M, R>::load(rn, sA);
-        cA = cB;
-        if (k + 128 < m) {
-          jB = nj[k + 128];
-          cB = nc[k + 128];
-        }
+        jn = nj[k + 64];
+        cn = nc[k + 64];
       }
-      const int fa = code & 1023, fb = (code >> 10) & 1023, fc = (code >> 20) & 1023;
-      const double ox = fa == 0 ? -0.0 : (double)(fa - 512) * g.L;
-      const double oy = fb == 0 ? -0.0 : (double)(fb - 512) * g.L;
-      const double oz = fc == 0 ? -0.0 : (double)(fc - 512) * g.L;
+      const double *recj = b.rec + (size_t)j * b.rs;
+      const double xj = recj[0], yj = recj[1], zj = recj[2];
+      R rj[T::NS][3];
+      RecSites<M, R>::load(recj, rj);
+      const double ox = (double)((code & 1023) - 512) * g.L;
+      const double oy = (double)(((code >> 10) & 1023) - 512) * g.L;
+      const double oz = (double)(((code >> 20) & 1023) - 512) * g.L;
       const double d0 = (xi - xj) - ox;
       const double d1 = (yi - yj) - oy;
       const double d2 = (zi - zj) - oz;
       const double dd64 = d0 * d0 + d1 * d1 + d2 * d2;
-      if (dd64 < g.rc2) {
-        R hh, dh;
-        cutoff_sw<R>(g, R(dd64), hh, dh);
-        R fi[T::NS][3], fj[T::NS][3];
-        const R e = pair_sites<M, R>(mt, hh, dh, R(d0), R(d1), R(d2), ri, rj, fi, fj);
-        R Fx = 0, Fy = 0, Fz = 0, tix = 0, tiy = 0, tiz = 0;
-#pragma unroll
-        for (int s = 0; s < T::NS; s++) {
-          Fx += fi[s][0]; Fy += fi[s][1]; Fz += fi[s][2];
-          tix += ri[s][1] * fi[s][2] - ri[s][2] * fi[s][1];
-          tiy += ri[s][2] * fi[s][0] - ri[s][0] * fi[s][2];
-          tiz += ri[s][0] * fi[s][1] - ri[s][1] * fi[s][0];
-        }
-        F[0] += (double)Fx; F[1] += (double)Fy; F[2] += (double)Fz;
-        Ti[0] += (double)tix; Ti[1] += (double)tiy; Ti[2] += (double)tiz;
+      const bool pass = dd64 < g.rc2;
+      npair += __popcll(__ballot(pass));
+      if (pass) {
+        R Fp[3], Tp[3];
+        const R e = pair_on_i<M, R>(pk, R(dd64), R(d0), R(d1), R(d2), ri, rj, Fp, Tp);
+        F[0] += (double)Fp[0]; F[1] += (double)Fp[1]; F[2] += (double)Fp[2];
+        Ti[0] += (double)Tp[0]; Ti[1] += (double)Tp[1]; Ti[2] += (double)Tp[2];
         ep += (double)e;
-        npi += 1.0;
-        v[0] += d0 * (double)Fx; v[1] += d0 * (double)Fy; v[2] += d0 * (double)Fz;
-        v[3] += d1 * (double)Fx; v[4] += d1 * (double)Fy; v[5] += d1 * (double)Fz;
-        v[6] += d2 * (double)Fx; v[7] += d2 * (double)Fy; v[8] += d2 * (double)Fz;
+        v[0] += d0 * (double)Fp[0]; v[1] += d0 * (double)Fp[1]; v[2] += d0 * (double)Fp[2];
+        v[3] += d1 * (double)Fp[0]; v[4] += d1 * (double)Fp[1]; v[5] += d1 * (double)Fp[2];
+        v[6] += d2 * (double)Fp[0]; v[7] += d2 * (double)Fp[1]; v[8] += d2 * (double)Fp[2];
       }
     }
 #pragma unroll
@@ -703,10 +695,6 @@ __global__ __launch_bounds__(256) void k_md_main(ModelTable mt, Geom g, MdBuf b,
       F[a] = wsum63(F[a]);
       Ti[a] = wsum63(Ti[a]);
     }
-    acc[0] = 0.5 * wsum63(ep);
-#pragma unroll
-    for (int a = 0; a < 9; a++) acc[1 + a] = 0.5 * wsum63(v[a]);
-    acc[10] = 0.5 * wsum63(npi);
     if (lane == 63) {
       if (out) {
         // molecule force (lab) and body torque, as rigid_collectforce
@@ -727,15 +715,28 @@ __global__ __launch_bounds__(256) void k_md_main(ModelTable mt, Geom g, MdBuf b,
       }
     }
   }
-  // block partials: lane 63 of each wave holds the wave's values
-  if (lane == 63)
+  // block partials in a fixed order: the 256 lanes' energy and virial go
+  // through LDS, 16 lanes per row sum 16 entries each, then the row sums
+  s_acc[0][threadIdx.x] = ep;
 #pragma unroll
-    for (int q = 0; q < 12; q++) s_red[wid][q] = acc[q];
+  for (int q = 0; q < 9; q++) s_acc[1 + q][threadIdx.x] = v[q];
+  if (lane == 0) {
+    s_cnt[wid][0] = (double)npair;
+    s_cnt[wid][1] = (double)nlist;
+  }
   __syncthreads();
-  if (threadIdx.x < 12) {
-    const int q = threadIdx.x;
-    b.blockpart[((size_t)parity * gridDim.x + blockIdx.x) * 16 + q] =
-        s_red[0][q] + s_red[1][q] + s_red[2][q] + s_red[3][q];
+  if (threadIdx.x < 160) {
+    const int q = threadIdx.x >> 4, seg = threadIdx.x & 15;
+    double r = 0.0;
+#pragma unroll
+    for (int u = 0; u < 16; u++) r += s_acc[q][seg * 16 + u];
+    r = rowsum16(r);
+    if (seg == 0)
+      b.blockpart[((size_t)parity * gridDim.x + blockIdx.x) * 16 + q] = 0.5 * r;
+  } else if (threadIdx.x < 162) {
+    const int q = threadIdx.x - 160;
+    b.blockpart[((size_t)parity * gridDim.x + blockIdx.x) * 16 + 10 + q] =
+        0.5 * (s_cnt[0][q] + s_cnt[1][q] + s_cnt[2][q] + s_cnt[3][q]);
   }
 }
 
@@ -762,10 +763,18 @@ void md_launch_close(System &s, int mode, int parity, int src_parity) {
                                       s.geo_md.ncell, ca.nose_active, mode, parity);
 }
 
+template <typename R>
+static const PairK<R> &pair_k(const System &s);
+template <>
+const PairK<float> &pair_k<float>(const System &s) { return s.pkf; }
+template <>
+const PairK<double> &pair_k<double>(const System &s) { return s.pkd; }
+
 template <int M, typename R>
 static void md_main_t(System &s, double *out, int parity, const CloseArgs &ca) {
   const size_t lds = sizeof(int) * 4 * 2 * (size_t)s.md.maxnb;
-  k_md_main<M, R><<<s.nblk_md, 256, lds, s.stream>>>(s.mt, s.geo_md, s.md, out, parity, ca);
+  k_md_main<M, R><<<s.nblk_md, 256, lds, s.stream>>>(s.mt, s.geo_md, s.md, pair_k<R>(s), out,
+                                                     parity, ca);
 }
 
 void md_launch_main(System &s, double *out, int parity, int close_mode) {

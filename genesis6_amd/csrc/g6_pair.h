@@ -161,6 +161,108 @@ __device__ inline R pair_sites(const ModelTable &mt, R hh, R dh, R d0, R d1,
   return ep1;
 }
 
+// Force and torque on molecule i only (the MD kernel evaluates each pair
+// from both sides) from the pair constants k: the same terms as pair_sites,
+// with the switch factors applied to the summed site forces and torques
+// instead of to each site force.  Site 0 carries the LJ term only and the
+// sites after it up to the CoM site the Coulomb terms only, in all three
+// models.  dd = |d|^2 of the COM separation d.  Returns ep1.
+template <int M, typename R>
+__device__ inline R pair_on_i(const PairK<R> &k, R dd, R d0, R d1, R d2,
+                              const R (&ri)[Traits<M>::NS][3],
+                              const R (&rj)[Traits<M>::NS][3], R (&F)[3], R (&Tq)[3]) {
+  using T = Traits<M>;
+  constexpr int NQ = T::COM;  // sites 0..NQ-1 carry offsets
+  // switch on the COM distance (cutoff_func, CutoffModule.F90:38-59)
+  R hh = R(0), dh = R(0);
+  if (dd < k.rout2) {
+    if (dd > k.rin2) {
+      const R dr = sqrt(dd);
+      const R u = dr - k.rout, l = dr - k.rin;
+      const R uu = u * u, ll = l * l;
+      hh = k.c1 * uu * u * (R(10) * ll - R(5) * u * l + uu);
+      dh = -(k.c2 * uu * ll / dr);
+    } else {
+      hh = R(1);
+    }
+  }
+  // site positions of i relative to j's COM
+  R pi[NQ][3];
+#pragma unroll
+  for (int s = 0; s < NQ; s++) {
+    pi[s][0] = d0 + ri[s][0];
+    pi[s][1] = d1 + ri[s][1];
+    pi[s][2] = d2 + ri[s][2];
+  }
+  R fs[NQ][3];
+#pragma unroll
+  for (int s = 0; s < NQ; s++) fs[s][0] = fs[s][1] = fs[s][2] = R(0);
+  R epc = R(0);
+#pragma unroll
+  for (int t = 0; t < T::NC; t++) {
+    const int a = T::ci(t), b = T::cj(t);
+    const R dx = pi[a][0] - rj[b][0];
+    const R dy = pi[a][1] - rj[b][1];
+    const R dz = pi[a][2] - rj[b][2];
+    R inv2, inv1;
+    inv_r<R>(dx * dx + dy * dy + dz * dz, inv2, inv1);
+    const R e0 = k.qq[t] * inv1;
+    const R f0 = e0 * inv2;
+    epc += e0;
+    fs[a][0] += dx * f0; fs[a][1] += dy * f0; fs[a][2] += dz * f0;
+  }
+  R eplj;
+  {
+    const R dx = pi[0][0] - rj[0][0];
+    const R dy = pi[0][1] - rj[0][1];
+    const R dz = pi[0][2] - rj[0][2];
+    R inv2, inv1;
+    inv_r<R>(dx * dx + dy * dy + dz * dz, inv2, inv1);
+    const R d6 = inv2 * inv2 * inv2, d12 = d6 * d6;
+    eplj = k.lja * d12 + k.ljb * d6;
+    const R f0 = inv2 * (k.lja12 * d12 + k.ljb6 * d6);
+    fs[0][0] = dx * f0; fs[0][1] = dy * f0; fs[0][2] = dz * f0;
+  }
+  // Coulomb sums (sites 1..NQ-1) and the LJ site, unscaled
+  R fc[3] = {R(0), R(0), R(0)}, tc[3] = {R(0), R(0), R(0)};
+#pragma unroll
+  for (int s = 1; s < NQ; s++) {
+    fc[0] += fs[s][0]; fc[1] += fs[s][1]; fc[2] += fs[s][2];
+    tc[0] += ri[s][1] * fs[s][2] - ri[s][2] * fs[s][1];
+    tc[1] += ri[s][2] * fs[s][0] - ri[s][0] * fs[s][2];
+    tc[2] += ri[s][0] * fs[s][1] - ri[s][1] * fs[s][0];
+  }
+  const R tl[3] = {ri[0][1] * fs[0][2] - ri[0][2] * fs[0][1],
+                   ri[0][2] * fs[0][0] - ri[0][0] * fs[0][2],
+                   ri[0][0] * fs[0][1] - ri[0][1] * fs[0][0]};
+  R hc = hh, cc, ep1;
+  if constexpr (T::ST2SW) {
+    R s12 = R(1), ds12 = R(0);
+    if (dd < k.ru2) {
+      if (dd < k.rl2) {
+        s12 = R(0);
+      } else {
+        const R r12 = sqrt(dd);
+        s12 = (r12 - k.rl) * (r12 - k.rl) * (R(3) * k.ru - k.rl - R(2) * r12) * k.w3inv;
+        ds12 = -(R(6) * (r12 - k.rl) * (k.ru - r12) * k.w3inv) / r12;
+      }
+    }
+    hc = hh * s12;
+    cc = dh * eplj + (dh * s12 + hh * ds12) * epc;
+    ep1 = epc * hc + eplj * hh;
+  } else {
+    cc = dh * (epc + eplj);
+    ep1 = (epc + eplj) * hh;
+  }
+  F[0] = hc * fc[0] + hh * fs[0][0] + cc * d0;
+  F[1] = hc * fc[1] + hh * fs[0][1] + cc * d1;
+  F[2] = hc * fc[2] + hh * fs[0][2] + cc * d2;
+  Tq[0] = hc * tc[0] + hh * tl[0];
+  Tq[1] = hc * tc[1] + hh * tl[1];
+  Tq[2] = hc * tc[2] + hh * tl[2];
+  return ep1;
+}
+
 // Energy only (PairPotential path): sum of site terms times S (and s12 for
 // the ST2 Coulomb part, as force_st2's ep1).
 template <int M, typename R>
