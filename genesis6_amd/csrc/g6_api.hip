@@ -23,6 +23,14 @@ static double ipow(double x, int k) {
   return r;
 }
 
+// the reciprocals rigid_correct multiplies by (1/m, 1/I), rounded once
+static void set_inverses(ModelTable *m) {
+  m->inv_mass = 1.0 / m->mass;
+  m->inv_ixx = 1.0 / m->Ixx;
+  m->inv_iyy = 1.0 / m->Iyy;
+  m->inv_izz = 1.0 / m->Izz;
+}
+
 bool make_model_table(int model, int prec, ModelTable *m) {
   (void)prec;
   memset(m, 0, sizeof(*m));
@@ -55,6 +63,7 @@ bool make_model_table(int model, int prec, ModelTable *m) {
     }
     m->lj_a = 600000.0 * kCA * 100.0;
     m->lj_b = -610.0 * kCA * 100.0;
+    set_inverses(m);
     return true;
   }
   if (model == G6_TIP5P || model == G6_ST2) {
@@ -119,6 +128,7 @@ bool make_model_table(int model, int prec, ModelTable *m) {
       m->rl = 2.0160;
       m->ru = 3.1287;
     }
+    set_inverses(m);
     return true;
   }
   return false;
@@ -207,7 +217,7 @@ static int alloc_all(System &s) {
   AL(m.nbr_code, sizeof(int) * (size_t)n * m.maxnb);
   AL(m.nbr_cnt, sizeof(int) * n);
   AL(m.blockpart, sizeof(double) * 2 * 16 * s.nblk_md);
-  AL(m.blockek, sizeof(double) * 4 * ((n + 63) / 64));
+  AL(m.blockek, sizeof(double) * 4 * (size_t)pre_waves(n));
   AL(m.frc, sizeof(double) * 6 * n);
   AL(m.nose, sizeof(double) * 9);
   AL(m.thermo, sizeof(double) * G6_THERMO_LEN);

@@ -16,8 +16,9 @@
 //     unwrapped, so |k| can exceed 1).  The -0.0 shift that dnint gives for
 //     small negative differences is stored as 0: x - (-0.0) and x - 0.0 are
 //     equal for every x but a zero x, whose sign no force term can see
-//   blockpart[nblk_md][16], blockek[nblk_pre][4]: per-block energy/virial and
-//     kinetic partials, reduced in a fixed order by one block (k_md_close)
+//   blockpart[2][nblk_md][16], blockek[pre waves][4]: per-block energy/virial
+//     and per-wave kinetic partials, reduced in a fixed order by the step
+//     close (one block)
 // Reference-list entry point (g6_pairlist.hip): cell residents in molecule
 //   order and the half-shell candidate rows of Grid_NeighborList
 //   (GridModule.F90:557-752) with counts cnt[2n], offsets off[2n+1], and the
@@ -45,6 +46,7 @@ struct ModelTable {
   double lj_a, lj_b;    // e = a/r^12 + b/r^6
   double bx[kMaxSite], by[kMaxSite], bz[kMaxSite];  // body-frame sites
   double Ixx, Iyy, Izz, mass;
+  double inv_ixx, inv_iyy, inv_izz, inv_mass;  // 1/I and 1/m, rounded on the host
   double rl, ru;  // ST2 short-range switch on the COM distance
 };
 
@@ -86,7 +88,7 @@ struct MdBuf {
   int maxnb;
   double *frc;        // [6][n] force and torque (lab) of the last k_md_main
   double *blockpart;  // [2][nblk][16] k_md_main block partials, by parity
-  double *blockek;    // [nblk_pre][4] k_md_pre kinetic partials
+  double *blockek;    // [k_md_pre waves][4] kinetic partials
   double *nose, *thermo;
   int *flags;  // [0] error code, [2 + parity] rebuild rows this step
   double half_skin2;
@@ -158,6 +160,9 @@ struct CloseArgs {
   const double *bpart;  // pair partials of the k_md_main being closed
   double dt;
 };
+// waves of k_md_pre (64 molecules per block of four waves); each writes
+// one kinetic partial
+inline int pre_waves(int n) { return (n + 63) / 64 * 4; }
 void md_launch_pre(System &s, bool correct, bool predict, bool sites, int parity);
 // close of the step whose k_md_main wrote the partials of src_parity
 void md_launch_close(System &s, int mode, int parity, int src_parity);

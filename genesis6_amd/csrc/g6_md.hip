@@ -89,225 +89,286 @@ __device__ inline void wsync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// ---- per-molecule state in registers --------------------------------------
-struct Mol {
-  double com[3], dx[5][3], w[5][3], q[5][4];
-};
-
-__device__ inline void load_mol(const MdBuf &b, int n, int i, Mol &m) {
-#pragma unroll
-  for (int a = 0; a < 3; a++) m.com[a] = b.com[a * n + i];
-#pragma unroll
-  for (int s = 0; s < 5; s++)
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-      m.dx[s][a] = b.dx[(s * 3 + a) * n + i];
-      m.w[s][a] = b.w[(s * 3 + a) * n + i];
-    }
-#pragma unroll
-  for (int s = 0; s < 5; s++)
-#pragma unroll
-    for (int a = 0; a < 4; a++) m.q[s][a] = b.quat[(s * 4 + a) * n + i];
+// rigid_predict of quaternion component A (levels 0 and 1)
+template <int A>
+__device__ inline void predict_qc(double (&S)[20]) {
+  S[A] = S[A] + S[4 + A] + S[8 + A] + S[12 + A] + S[16 + A];
+  S[4 + A] = S[4 + A] + 2.0 * S[8 + A] + 3.0 * S[12 + A] + 4.0 * S[16 + A];
 }
 
-__device__ inline void store_mol(const MdBuf &b, int n, int i, const Mol &m) {
+// components C0 and C0+1 of the five quaternion levels back to HBM
+template <int C0>
+__device__ inline void store_qc(const MdBuf &b, int n, int i, const double (&S)[20]) {
 #pragma unroll
-  for (int a = 0; a < 3; a++) b.com[a * n + i] = m.com[a];
-#pragma unroll
-  for (int s = 0; s < 5; s++)
-#pragma unroll
-    for (int a = 0; a < 3; a++) {
-      b.dx[(s * 3 + a) * n + i] = m.dx[s][a];
-      b.w[(s * 3 + a) * n + i] = m.w[s][a];
-    }
-#pragma unroll
-  for (int s = 0; s < 5; s++)
-#pragma unroll
-    for (int a = 0; a < 4; a++) b.quat[(s * 4 + a) * n + i] = m.q[s][a];
+  for (int s = 0; s < 5; s++) {
+    b.quat[(s * 4 + C0) * n + i] = S[s * 4 + C0];
+    b.quat[(s * 4 + C0 + 1) * n + i] = S[s * 4 + C0 + 1];
+  }
 }
-
-// rigid_predict (RigidModule.F90:480-559)
-__device__ inline void predict_mol(Mol &m) {
-#pragma unroll
-  for (int a = 0; a < 3; a++) {
-    m.com[a] = m.com[a] + m.dx[0][a] + m.dx[1][a] + m.dx[2][a] + m.dx[3][a] + m.dx[4][a];
-    m.dx[0][a] = m.dx[0][a] + 2.0 * m.dx[1][a] + 3.0 * m.dx[2][a] + 4.0 * m.dx[3][a] +
-                 5.0 * m.dx[4][a];
-    m.dx[1][a] = m.dx[1][a] + 3.0 * m.dx[2][a] + 6.0 * m.dx[3][a] + 10.0 * m.dx[4][a];
-    m.dx[2][a] = m.dx[2][a] + 4.0 * m.dx[3][a] + 10.0 * m.dx[4][a];
-    m.dx[3][a] = m.dx[3][a] + 5.0 * m.dx[4][a];
-  }
-#pragma unroll
-  for (int a = 0; a < 3; a++) {
-    m.w[0][a] = m.w[0][a] + m.w[1][a] + m.w[2][a] + m.w[3][a] + m.w[4][a];
-    m.w[1][a] = m.w[1][a] + 2.0 * m.w[2][a] + 3.0 * m.w[3][a] + 4.0 * m.w[4][a];
-    m.w[2][a] = m.w[2][a] + 3.0 * m.w[3][a] + 6.0 * m.w[4][a];
-    m.w[3][a] = m.w[3][a] + 4.0 * m.w[4][a];
-  }
-#pragma unroll
-  for (int a = 0; a < 4; a++) {
-    m.q[0][a] = m.q[0][a] + m.q[1][a] + m.q[2][a] + m.q[3][a] + m.q[4][a];
-    m.q[1][a] = m.q[1][a] + 2.0 * m.q[2][a] + 3.0 * m.q[3][a] + 4.0 * m.q[4][a];
-  }
-  const double re = 1.0 / sqrt(m.q[0][0] * m.q[0][0] + m.q[0][1] * m.q[0][1] +
-                               m.q[0][2] * m.q[0][2] + m.q[0][3] * m.q[0][3]);
-#pragma unroll
-  for (int a = 0; a < 4; a++) m.q[0][a] = m.q[0][a] * re;
-}
-
-// rigid_correct (RigidModule.F90:1255-1487) with the Nose drag; F in the lab
-// frame, tq the body-frame torque; writes the Rigid_Ek terms to ek
-__device__ inline void correct_mol(const ModelTable &mt, double dt, double zeta0,
-                                   const double *F, const double *tq, Mol &m, double *ek) {
-  const double nc = zeta0 * dt, ext = nc * 0.5, exr = nc;
-  const double tf = dt * dt * 0.5 * (1.0 / mt.mass);
-#pragma unroll
-  for (int a = 0; a < 3; a++) {
-    double nf = F[a] * tf;
-    nf = nf - m.dx[0][a] * ext;
-    const double d = nf - m.dx[1][a];
-    m.com[a] = m.com[a] + kG5_0 * d;
-    m.dx[0][a] = m.dx[0][a] + kG5_1 * d;
-    m.dx[1][a] = nf;
-    m.dx[2][a] = m.dx[2][a] + kG5_3 * d;
-    m.dx[3][a] = m.dx[3][a] + kG5_4 * d;
-    m.dx[4][a] = m.dx[4][a] + kG5_5 * d;
-  }
-  const double wx = m.w[0][0], wy = m.w[0][1], wz = m.w[0][2];
-  double qa = m.q[0][0], qb = m.q[0][1], qc = m.q[0][2], qd = m.q[0][3];
-  const double nwa = (-wx * qb + wy * qc - wz * qd) * dt * 0.5;
-  const double nwb = (wx * qa - wy * qd - wz * qc) * dt * 0.5;
-  const double nwc = (-wx * qd - wy * qa + wz * qb) * dt * 0.5;
-  const double nwd = (wx * qc + wy * qb + wz * qa) * dt * 0.5;
-  const double dwa = nwa - m.q[1][0], dwb = nwb - m.q[1][1], dwc = nwc - m.q[1][2],
-               dwd = nwd - m.q[1][3];
-  m.q[0][0] = m.q[0][0] + kG4_0 * dwa;
-  m.q[0][1] = m.q[0][1] + kG4_0 * dwb;
-  m.q[0][2] = m.q[0][2] + kG4_0 * dwc;
-  m.q[0][3] = m.q[0][3] + kG4_0 * dwd;
-  double nf[3];
-  nf[0] = (tq[0] + (mt.Iyy - mt.Izz) * wy * wz) * dt * (1.0 / mt.Ixx);
-  nf[1] = (tq[1] + (mt.Izz - mt.Ixx) * wz * wx) * dt * (1.0 / mt.Iyy);
-  nf[2] = (tq[2] + (mt.Ixx - mt.Iyy) * wx * wy) * dt * (1.0 / mt.Izz);
-#pragma unroll
-  for (int a = 0; a < 3; a++) {
-    nf[a] = nf[a] - m.w[0][a] * exr;
-    const double d = nf[a] - m.w[1][a];
-    m.w[0][a] = m.w[0][a] + kG4_0 * d;
-    m.w[1][a] = nf[a];
-    m.w[2][a] = m.w[2][a] + kG4_2 * d;
-    m.w[3][a] = m.w[3][a] + kG4_3 * d;
-    m.w[4][a] = m.w[4][a] + kG4_4 * d;
-  }
-  const double w1x = m.w[0][0], w1y = m.w[0][1], w1z = m.w[0][2];
-  const double w2x = m.w[1][0], w2y = m.w[1][1], w2z = m.w[1][2];
-  const double w3x = m.w[2][0], w3y = m.w[2][1], w3z = m.w[2][2];
-  const double w4x = m.w[3][0], w4y = m.w[3][1], w4z = m.w[3][2];
-  qa = m.q[0][0]; qb = m.q[0][1]; qc = m.q[0][2]; qd = m.q[0][3];
-  const double pc = (-w1x * qd - w1y * qa + w1z * qb) * dt * 0.5;
-  const double pb = (w1x * qa - w1y * qd - w1z * qc) * dt * 0.5;
-  const double pd = (w1x * qc + w1y * qb + w1z * qa) * dt * 0.5;
-  const double pa = (-w1x * qb + w1y * qc - w1z * qd) * dt * 0.5;
-  m.q[1][0] = pa; m.q[1][1] = pb; m.q[1][2] = pc; m.q[1][3] = pd;
-  const double rc_ = (-w2x * qd - w2y * qa + w2z * qb - w1x * pd - w1y * pa + w1z * pb) * dt * 0.25;
-  const double rb = (w2x * qa - w2y * qd - w2z * qc + w1x * pa - w1y * pd - w1z * pc) * dt * 0.25;
-  const double rd = (w2x * qc + w2y * qb + w2z * qa + w1x * pc + w1y * pb + w1z * pa) * dt * 0.25;
-  const double ra = (-w2x * qb + w2y * qc - w2z * qd - w1x * pb + w1y * pc - w1z * pd) * dt * 0.25;
-  m.q[2][0] = ra; m.q[2][1] = rb; m.q[2][2] = rc_; m.q[2][3] = rd;
-  const double sc = (-w3x * qd - w3y * qa + w3z * qb - w1x * rd - w1y * ra + w1z * rb - w2x * pd -
-                     w2y * pa + w2z * pb) * dt * (1.0 / 6.0);
-  const double sb = (w3x * qa - w3y * qd - w3z * qc + w1x * ra - w1y * rd - w1z * rc_ + w2x * pa -
-                     w2y * pd - w2z * pc) * dt * (1.0 / 6.0);
-  const double sd = (w3x * qc + w3y * qb + w3z * qa + w1x * rc_ + w1y * rb + w1z * ra + w2x * pc +
-                     w2y * pb + w2z * pa) * dt * (1.0 / 6.0);
-  const double sa = (-w3x * qb + w3y * qc - w3z * qd - w1x * rb + w1y * rc_ - w1z * rd - w2x * pb +
-                     w2y * pc - w2z * pd) * dt * (1.0 / 6.0);
-  m.q[3][0] = sa; m.q[3][1] = sb; m.q[3][2] = sc; m.q[3][3] = sd;
-  m.q[4][2] = (-w4x * qd - w4y * qa + w4z * qb - w3x * pd - w3y * pa + w3z * pb - w2x * rd -
-               w2y * ra + w2z * rb - w1x * sd - w1y * sa + w1z * sb) * dt * (1.0 / 8.0);
-  m.q[4][1] = (w4x * qa - w4y * qd - w4z * qc + w3x * pa - w3y * pd - w3z * pc + w2x * ra -
-               w2y * rd - w2z * rc_ + w1x * sa - w1y * sd - w1z * sc) * dt * (1.0 / 8.0);
-  m.q[4][3] = (w4x * qc + w4y * qb + w4z * qa + w3x * pc + w3y * pb + w3z * pa + w2x * rc_ +
-               w2y * rb + w2z * ra + w1x * sc + w1y * sb + w1z * sa) * dt * (1.0 / 8.0);
-  m.q[4][0] = (-w4x * qb + w4y * qc - w4z * qd - w3x * pb + w3y * pc - w3z * pd - w2x * rb +
-               w2y * rc_ - w2z * rd - w1x * sb + w1y * sc - w1z * sd) * dt * (1.0 / 8.0);
-  ek[0] = m.dx[0][0] * m.dx[0][0] + m.dx[0][1] * m.dx[0][1] + m.dx[0][2] * m.dx[0][2];
-  ek[1] = m.w[0][0] * m.w[0][0];
-  ek[2] = m.w[0][1] * m.w[0][1];
-  ek[3] = m.w[0][2] * m.w[0][2];
-}
-
 
 // ---- kernel A --------------------------------------------------------------
-// One thread per molecule.  do_correct: rigid_correct with the forces of the
-// previous k_md_main (b.frc) and the block's kinetic partials.  do_predict:
-// rigid_predict.  do_sites: rotation matrix, record, binning into the cell
-// lists of this step's parity and the skin test.
-__global__ __launch_bounds__(64) void k_md_pre(ModelTable mt, Geom g, MdBuf b, double dt,
-                                               int nose_active, int do_correct,
-                                               int do_predict, int do_sites, int parity) {
+// A block updates 64 molecules (one per lane) with one role per wave:
+//   wave 0  translation: com, dx[5][3]; record COM, binning, skin test
+//   wave 1  angular velocity w[5][3]
+//   wave 2  quaternion components a, b of every derivative; wave 3 c, d
+// Every component is computed with the operation order of rigid_correct
+// (RigidModule.F90:1255-1487, with the Nose drag) and rigid_predict
+// (RigidModule.F90:480-559); values cross waves only through LDS, so the
+// split changes no bit of the result.  The roles are wave-uniform, so no
+// wave executes another role's code.
+__global__ __launch_bounds__(256) void k_md_pre(ModelTable mt, Geom g, MdBuf b, double dt,
+                                                int nose_active, int do_correct, int do_predict,
+                                                int do_sites, int parity) {
+  __shared__ double s_w0[3][64], s_W[12][64];
+  __shared__ double s_q0[4][64], s_p[4][64], s_r[4][64], s_s[4][64], s_qn[4][64];
   const int n = g.n;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int role = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 64 + lane;
+  const bool ok = i < n;
+  const bool q2 = role == 2;  // wave 2: components a, b; wave 3: c, d
   double ek[4] = {0, 0, 0, 0};
-  if (i < n) {
-    Mol m;
-    load_mol(b, n, i, m);
-    if (do_correct) {
-      double t[9], F[3], tl[3];
+  double S[20];  // wave 0: com, dx; wave 1: w; waves 2-3: q (all four)
 #pragma unroll
-      for (int a = 0; a < 9; a++) t[a] = b.tmat[a * n + i];
+  for (int k = 0; k < 20; k++) S[k] = 0.0;
+  const double zeta0 = nose_active ? b.nose[3] : 0.0;
+  const double nc = zeta0 * dt, ext = nc * 0.5, exr = nc;
+  // every global load is issued here, ahead of the first barrier
+  double F[3] = {0, 0, 0}, t[9], tl[3] = {0, 0, 0}, cref[3] = {0, 0, 0};
+#pragma unroll
+  for (int a = 0; a < 9; a++) t[a] = 0.0;
+  if (ok) {
+    if (role == 0) {
+#pragma unroll
+      for (int a = 0; a < 3; a++) S[a] = b.com[a * n + i];
+#pragma unroll
+      for (int k = 0; k < 15; k++) S[3 + k] = b.dx[k * n + i];
+      if (do_correct)
+#pragma unroll
+        for (int a = 0; a < 3; a++) F[a] = b.frc[a * n + i];
+      if (do_sites && b.half_skin2 > 0.0)
+#pragma unroll
+        for (int a = 0; a < 3; a++) cref[a] = b.com_ref[a * n + i];
+    } else if (role == 1) {
+#pragma unroll
+      for (int k = 0; k < 15; k++) S[k] = b.w[k * n + i];
+      if (do_correct) {
+#pragma unroll
+        for (int a = 0; a < 9; a++) t[a] = b.tmat[a * n + i];
+#pragma unroll
+        for (int a = 0; a < 3; a++) tl[a] = b.frc[(3 + a) * n + i];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 20; k++) S[k] = b.quat[k * n + i];
+    }
+  }
+  // ---- corrector ----
+  if (do_correct) {
+    if (role == 1) {
+#pragma unroll
+      for (int a = 0; a < 3; a++) s_w0[a][lane] = S[a];  // the old w, for nw
+    }
+    __syncthreads();
+    if (role == 0) {
+      if (ok) {
+        const double tf = dt * dt * 0.5 * mt.inv_mass;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+          double nf = F[a] * tf;
+          nf = nf - S[3 + a] * ext;
+          const double d = nf - S[6 + a];
+          S[a] = S[a] + kG5_0 * d;
+          S[3 + a] = S[3 + a] + kG5_1 * d;
+          S[6 + a] = nf;
+          S[9 + a] = S[9 + a] + kG5_3 * d;
+          S[12 + a] = S[12 + a] + kG5_4 * d;
+          S[15 + a] = S[15 + a] + kG5_5 * d;
+        }
+        ek[0] = S[3] * S[3] + S[4] * S[4] + S[5] * S[5];
+      }
+    } else if (role == 1) {
+      if (ok) {
+        // rigid_collectforce's body-frame torque (RigidModule.F90:301-306)
+        const double tq0 = t[0] * tl[0] + t[3] * tl[1] + t[6] * tl[2];
+        const double tq1 = t[1] * tl[0] + t[4] * tl[1] + t[7] * tl[2];
+        const double tq2 = t[2] * tl[0] + t[5] * tl[1] + t[8] * tl[2];
+        const double wx = S[0], wy = S[1], wz = S[2];
+        double nf[3];
+        nf[0] = (tq0 + (mt.Iyy - mt.Izz) * wy * wz) * dt * mt.inv_ixx;
+        nf[1] = (tq1 + (mt.Izz - mt.Ixx) * wz * wx) * dt * mt.inv_iyy;
+        nf[2] = (tq2 + (mt.Ixx - mt.Iyy) * wx * wy) * dt * mt.inv_izz;
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+          nf[a] = nf[a] - S[a] * exr;
+          const double d = nf[a] - S[3 + a];
+          S[a] = S[a] + kG4_0 * d;
+          S[3 + a] = nf[a];
+          S[6 + a] = S[6 + a] + kG4_2 * d;
+          S[9 + a] = S[9 + a] + kG4_3 * d;
+          S[12 + a] = S[12 + a] + kG4_4 * d;
+        }
+        ek[1] = S[0] * S[0];
+        ek[2] = S[1] * S[1];
+        ek[3] = S[2] * S[2];
+      }
+#pragma unroll
+      for (int k = 0; k < 12; k++) s_W[k][lane] = S[k];
+    } else {
+      const double wx = s_w0[0][lane], wy = s_w0[1][lane], wz = s_w0[2][lane];
+      const double qa = S[0], qb = S[1], qc = S[2], qd = S[3];
+      if (q2) {
+        const double nwa = (-wx * qb + wy * qc - wz * qd) * dt * 0.5;
+        const double nwb = (wx * qa - wy * qd - wz * qc) * dt * 0.5;
+        S[0] = S[0] + kG4_0 * (nwa - S[4]);
+        S[1] = S[1] + kG4_0 * (nwb - S[5]);
+      } else {
+        const double nwc = (-wx * qd - wy * qa + wz * qb) * dt * 0.5;
+        const double nwd = (wx * qc + wy * qb + wz * qa) * dt * 0.5;
+        S[2] = S[2] + kG4_0 * (nwc - S[6]);
+        S[3] = S[3] + kG4_0 * (nwd - S[7]);
+      }
+      if (q2) {
+        s_q0[0][lane] = S[0];
+        s_q0[1][lane] = S[1];
+      } else {
+        s_q0[2][lane] = S[2];
+        s_q0[3][lane] = S[3];
+      }
+    }
+    __syncthreads();
+    // quaternion derivatives from the corrected w: own components, then
+    // the partner's through LDS
+    double w1x = 0, w1y = 0, w1z = 0, w2x = 0, w2y = 0, w2z = 0, w3x = 0, w3y = 0, w3z = 0,
+           w4x = 0, w4y = 0, w4z = 0, qa = 0, qb = 0, qc = 0, qd = 0;
+    if (role >= 2) {
+      w1x = s_W[0][lane]; w1y = s_W[1][lane]; w1z = s_W[2][lane];
+      w2x = s_W[3][lane]; w2y = s_W[4][lane]; w2z = s_W[5][lane];
+      w3x = s_W[6][lane]; w3y = s_W[7][lane]; w3z = s_W[8][lane];
+      w4x = s_W[9][lane]; w4y = s_W[10][lane]; w4z = s_W[11][lane];
+      S[0] = s_q0[0][lane]; S[1] = s_q0[1][lane]; S[2] = s_q0[2][lane]; S[3] = s_q0[3][lane];
+      qa = S[0]; qb = S[1]; qc = S[2]; qd = S[3];
+      if (q2) {
+        s_p[0][lane] = (-w1x * qb + w1y * qc - w1z * qd) * dt * 0.5;
+        s_p[1][lane] = (w1x * qa - w1y * qd - w1z * qc) * dt * 0.5;
+      } else {
+        s_p[2][lane] = (-w1x * qd - w1y * qa + w1z * qb) * dt * 0.5;
+        s_p[3][lane] = (w1x * qc + w1y * qb + w1z * qa) * dt * 0.5;
+      }
+    }
+    __syncthreads();
+    double pa = 0, pb = 0, pc = 0, pd = 0;
+    if (role >= 2) {
+      pa = s_p[0][lane]; pb = s_p[1][lane]; pc = s_p[2][lane]; pd = s_p[3][lane];
+      if (q2) {
+        s_r[0][lane] = (-w2x * qb + w2y * qc - w2z * qd - w1x * pb + w1y * pc - w1z * pd) * dt * 0.25;
+        s_r[1][lane] = (w2x * qa - w2y * qd - w2z * qc + w1x * pa - w1y * pd - w1z * pc) * dt * 0.25;
+      } else {
+        s_r[2][lane] = (-w2x * qd - w2y * qa + w2z * qb - w1x * pd - w1y * pa + w1z * pb) * dt * 0.25;
+        s_r[3][lane] = (w2x * qc + w2y * qb + w2z * qa + w1x * pc + w1y * pb + w1z * pa) * dt * 0.25;
+      }
+    }
+    __syncthreads();
+    double ra = 0, rb = 0, rc_ = 0, rd = 0;
+    if (role >= 2) {
+      ra = s_r[0][lane]; rb = s_r[1][lane]; rc_ = s_r[2][lane]; rd = s_r[3][lane];
+      if (q2) {
+        s_s[0][lane] = (-w3x * qb + w3y * qc - w3z * qd - w1x * rb + w1y * rc_ - w1z * rd - w2x * pb +
+                        w2y * pc - w2z * pd) * dt * (1.0 / 6.0);
+        s_s[1][lane] = (w3x * qa - w3y * qd - w3z * qc + w1x * ra - w1y * rd - w1z * rc_ + w2x * pa -
+                        w2y * pd - w2z * pc) * dt * (1.0 / 6.0);
+      } else {
+        s_s[2][lane] = (-w3x * qd - w3y * qa + w3z * qb - w1x * rd - w1y * ra + w1z * rb - w2x * pd -
+                        w2y * pa + w2z * pb) * dt * (1.0 / 6.0);
+        s_s[3][lane] = (w3x * qc + w3y * qb + w3z * qa + w1x * rc_ + w1y * rb + w1z * ra + w2x * pc +
+                        w2y * pb + w2z * pa) * dt * (1.0 / 6.0);
+      }
+    }
+    __syncthreads();
+    if (role >= 2) {
+      const double sa = s_s[0][lane], sb = s_s[1][lane], sc = s_s[2][lane], sd = s_s[3][lane];
+      S[4] = pa; S[5] = pb; S[6] = pc; S[7] = pd;
+      S[8] = ra; S[9] = rb; S[10] = rc_; S[11] = rd;
+      S[12] = sa; S[13] = sb; S[14] = sc; S[15] = sd;
+      if (q2) {
+        S[16] = (-w4x * qb + w4y * qc - w4z * qd - w3x * pb + w3y * pc - w3z * pd - w2x * rb +
+                 w2y * rc_ - w2z * rd - w1x * sb + w1y * sc - w1z * sd) * dt * (1.0 / 8.0);
+        S[17] = (w4x * qa - w4y * qd - w4z * qc + w3x * pa - w3y * pd - w3z * pc + w2x * ra -
+                 w2y * rd - w2z * rc_ + w1x * sa - w1y * sd - w1z * sc) * dt * (1.0 / 8.0);
+      } else {
+        S[18] = (-w4x * qd - w4y * qa + w4z * qb - w3x * pd - w3y * pa + w3z * pb - w2x * rd -
+                 w2y * ra + w2z * rb - w1x * sd - w1y * sa + w1z * sb) * dt * (1.0 / 8.0);
+        S[19] = (w4x * qc + w4y * qb + w4z * qa + w3x * pc + w3y * pb + w3z * pa + w2x * rc_ +
+                 w2y * rb + w2z * ra + w1x * sc + w1y * sb + w1z * sa) * dt * (1.0 / 8.0);
+      }
+    }
+  }
+  // ---- predictor ----
+  if (do_predict) {
+    if (role == 0) {
 #pragma unroll
       for (int a = 0; a < 3; a++) {
-        F[a] = b.frc[a * n + i];
-        tl[a] = b.frc[(3 + a) * n + i];
+        S[a] = S[a] + S[3 + a] + S[6 + a] + S[9 + a] + S[12 + a] + S[15 + a];
+        S[3 + a] = S[3 + a] + 2.0 * S[6 + a] + 3.0 * S[9 + a] + 4.0 * S[12 + a] + 5.0 * S[15 + a];
+        S[6 + a] = S[6 + a] + 3.0 * S[9 + a] + 6.0 * S[12 + a] + 10.0 * S[15 + a];
+        S[9 + a] = S[9 + a] + 4.0 * S[12 + a] + 10.0 * S[15 + a];
+        S[12 + a] = S[12 + a] + 5.0 * S[15 + a];
       }
-      // rigid_collectforce's body-frame torque (RigidModule.F90:301-306)
-      const double tq[3] = {t[0] * tl[0] + t[3] * tl[1] + t[6] * tl[2],
-                            t[1] * tl[0] + t[4] * tl[1] + t[7] * tl[2],
-                            t[2] * tl[0] + t[5] * tl[1] + t[8] * tl[2]};
-      correct_mol(mt, dt, nose_active ? b.nose[3] : 0.0, F, tq, m, ek);
-    }
-    if (do_predict) predict_mol(m);
-    if (do_predict || do_correct) store_mol(b, n, i, m);
-    if (do_sites) {
-      const double qa = m.q[0][0], qb = m.q[0][1], qc = m.q[0][2], qd = m.q[0][3];
-      double t[9];
-      t[0] = (qa * qa + qb * qb - (qc * qc + qd * qd));
-      t[1] = -2.0 * (qa * qd + qb * qc);
-      t[2] = 2.0 * (qb * qd - qa * qc);
-      t[3] = 2.0 * (qa * qd - qb * qc);
-      t[4] = qa * qa + qc * qc - (qb * qb + qd * qd);
-      t[5] = -2.0 * (qa * qb + qc * qd);
-      t[6] = 2.0 * (qa * qc + qb * qd);
-      t[7] = 2.0 * (qa * qb - qc * qd);
-      t[8] = qa * qa + qd * qd - (qb * qb + qc * qc);
+    } else if (role == 1) {
 #pragma unroll
-      for (int a = 0; a < 9; a++) b.tmat[a * n + i] = t[a];
-      // record: COM (FP64), then the offsets of the sites before the CoM
-      // site (whose offset is zero) in the pair precision
-      double *rec = b.rec + (size_t)i * b.rs;
-      rec[0] = m.com[0];
-      rec[1] = m.com[1];
-      rec[2] = m.com[2];
-      float *recf = reinterpret_cast<float *>(rec + 4);
-      for (int s = 0; s < mt.nsite - 1; s++) {
-        const double r0 = t[0] * mt.bx[s] + t[1] * mt.by[s] + t[2] * mt.bz[s];
-        const double r1 = t[3] * mt.bx[s] + t[4] * mt.by[s] + t[5] * mt.bz[s];
-        const double r2 = t[6] * mt.bx[s] + t[7] * mt.by[s] + t[8] * mt.bz[s];
-        if (b.rs == kRecF) {
-          recf[s * 3 + 0] = (float)r0;
-          recf[s * 3 + 1] = (float)r1;
-          recf[s * 3 + 2] = (float)r2;
-        } else {
-          rec[4 + s * 3 + 0] = r0;
-          rec[4 + s * 3 + 1] = r1;
-          rec[4 + s * 3 + 2] = r2;
-        }
+      for (int a = 0; a < 3; a++) {
+        S[a] = S[a] + S[3 + a] + S[6 + a] + S[9 + a] + S[12 + a];
+        S[3 + a] = S[3 + a] + 2.0 * S[6 + a] + 3.0 * S[9 + a] + 4.0 * S[12 + a];
+        S[6 + a] = S[6 + a] + 3.0 * S[9 + a] + 6.0 * S[12 + a];
+        S[9 + a] = S[9 + a] + 4.0 * S[12 + a];
       }
+    } else {
+      if (q2) {
+        predict_qc<0>(S);
+        predict_qc<1>(S);
+        s_qn[0][lane] = S[0];
+        s_qn[1][lane] = S[1];
+      } else {
+        predict_qc<2>(S);
+        predict_qc<3>(S);
+        s_qn[2][lane] = S[2];
+        s_qn[3][lane] = S[3];
+      }
+    }
+    __syncthreads();
+    if (role >= 2) {
+      // all four predicted components, normalised in the order of rigid_predict
+      S[0] = s_qn[0][lane]; S[1] = s_qn[1][lane]; S[2] = s_qn[2][lane]; S[3] = s_qn[3][lane];
+      const double re = 1.0 / sqrt(S[0] * S[0] + S[1] * S[1] + S[2] * S[2] + S[3] * S[3]);
+#pragma unroll
+      for (int a = 0; a < 4; a++) S[a] = S[a] * re;
+    }
+  }
+  if (ok && (do_predict || do_correct)) {
+    if (role == 0) {
+#pragma unroll
+      for (int a = 0; a < 3; a++) b.com[a * n + i] = S[a];
+#pragma unroll
+      for (int k = 0; k < 15; k++) b.dx[k * n + i] = S[3 + k];
+    } else if (role == 1) {
+#pragma unroll
+      for (int k = 0; k < 15; k++) b.w[k * n + i] = S[k];
+    } else if (q2) {
+      store_qc<0>(b, n, i, S);
+    } else {
+      store_qc<2>(b, n, i, S);
+    }
+  }
+  if (do_sites && ok) {
+    if (role == 0) {
+      double *rec = b.rec + (size_t)i * b.rs;
+      rec[0] = S[0];
+      rec[1] = S[1];
+      rec[2] = S[2];
       int jc[3];
 #pragma unroll
       for (int a = 0; a < 3; a++) {
-        double d = m.com[a];
+        double d = S[a];
         const double o = round(d * g.invL) * g.L;
         d = d - o;
         int j = (int)round(d * g.cellinv);
@@ -325,22 +386,59 @@ __global__ __launch_bounds__(64) void k_md_pre(ModelTable mt, Geom g, MdBuf b, d
       else
         b.flags[0] = 3;  // cell capacity exceeded
       if (b.half_skin2 > 0.0) {
-        const double e0 = m.com[0] - b.com_ref[i], e1 = m.com[1] - b.com_ref[n + i],
-                     e2 = m.com[2] - b.com_ref[2 * n + i];
+        const double e0 = S[0] - cref[0], e1 = S[1] - cref[1], e2 = S[2] - cref[2];
         if (e0 * e0 + e1 * e1 + e2 * e2 > b.half_skin2) b.flags[2 + parity] = 1;
       } else if (i == 0) {
         b.flags[2 + parity] = 1;  // no skin: the reference rebuilds every step
       }
+    } else if (role >= 2) {
+      // rigid_setmatrix and the site offsets; the two waves split the stores
+      const double qa = S[0], qb = S[1], qc = S[2], qd = S[3];
+      t[0] = (qa * qa + qb * qb - (qc * qc + qd * qd));
+      t[1] = -2.0 * (qa * qd + qb * qc);
+      t[2] = 2.0 * (qb * qd - qa * qc);
+      t[3] = 2.0 * (qa * qd - qb * qc);
+      t[4] = qa * qa + qc * qc - (qb * qb + qd * qd);
+      t[5] = -2.0 * (qa * qb + qc * qd);
+      t[6] = 2.0 * (qa * qc + qb * qd);
+      t[7] = 2.0 * (qa * qb - qc * qd);
+      t[8] = qa * qa + qd * qd - (qb * qb + qc * qc);
+      if (q2) {
+#pragma unroll
+        for (int a = 0; a < 5; a++) b.tmat[a * n + i] = t[a];
+      } else {
+#pragma unroll
+        for (int a = 5; a < 9; a++) b.tmat[a * n + i] = t[a];
+      }
+      double *rec = b.rec + (size_t)i * b.rs;
+      float *recf = reinterpret_cast<float *>(rec + 4);
+      const int nq = mt.nsite - 1, half = (nq + 1) / 2;
+      const int s0 = q2 ? 0 : half, s1 = q2 ? half : nq;
+      for (int s = s0; s < s1; s++) {
+        const double r0 = t[0] * mt.bx[s] + t[1] * mt.by[s] + t[2] * mt.bz[s];
+        const double r1 = t[3] * mt.bx[s] + t[4] * mt.by[s] + t[5] * mt.bz[s];
+        const double r2 = t[6] * mt.bx[s] + t[7] * mt.by[s] + t[8] * mt.bz[s];
+        if (b.rs == kRecF) {
+          recf[s * 3 + 0] = (float)r0;
+          recf[s * 3 + 1] = (float)r1;
+          recf[s * 3 + 2] = (float)r2;
+        } else {
+          rec[4 + s * 3 + 0] = r0;
+          rec[4 + s * 3 + 1] = r1;
+          rec[4 + s * 3 + 2] = r2;
+        }
+      }
     }
   }
-  // kinetic partials of this block (one wave), reduced by k_md_close
-  if (do_correct) {
+  // kinetic partials of this wave, reduced by the step close
+  if (do_correct && role < 2) {
 #pragma unroll
     for (int q = 0; q < 4; q++) ek[q] = wsum63(ek[q]);
-    if (threadIdx.x == 63) {
+  }
+  if (lane == 63) {
+    const size_t wv = (size_t)blockIdx.x * 4 + role;
 #pragma unroll
-      for (int q = 0; q < 4; q++) b.blockek[(size_t)blockIdx.x * 4 + q] = ek[q];
-    }
+    for (int q = 0; q < 4; q++) b.blockek[wv * 4 + q] = ek[q];
   }
 }
 
@@ -741,7 +839,7 @@ __global__ __launch_bounds__(256) void k_md_main(ModelTable mt, Geom g, MdBuf b,
 }
 
 void md_launch_pre(System &s, bool correct, bool predict, bool sites, int parity) {
-  k_md_pre<<<(s.n + 63) / 64, 64, 0, s.stream>>>(s.mt, s.geo_md, s.md, s.dt,
+  k_md_pre<<<pre_waves(s.n) / 4, 256, 0, s.stream>>>(s.mt, s.geo_md, s.md, s.dt,
                                                   s.nose_active ? 1 : 0, correct ? 1 : 0,
                                                   predict ? 1 : 0, sites ? 1 : 0, parity);
 }
@@ -750,7 +848,7 @@ static CloseArgs close_args(System &s, int mode, int src_parity) {
   CloseArgs ca;
   ca.mode = mode;
   ca.nose_active = s.nose_active ? 1 : 0;
-  ca.nblk_pre = (s.n + 63) / 64;
+  ca.nblk_pre = pre_waves(s.n);
   ca.nblk_main = s.nblk_md;
   ca.bpart = s.md.blockpart + (size_t)src_parity * s.nblk_md * 16;
   ca.dt = s.dt;
