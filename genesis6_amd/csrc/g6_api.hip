@@ -374,6 +374,12 @@ int g6_init(g6_handle *h, int model, int n, double box_l, double rc, double tape
     s->pkf = make_pair_k<float>(s->mt, m);
     s->pkd = make_pair_k<double>(s->mt, m);
     s->nblk_md = (n + 3) / 4;
+    // the force kernel keeps four rows in LDS (64 KiB of dynamic LDS at most)
+    const int rb = main_region_bytes(b.maxnb, prec == G6_FP32 ? 4 : 8);
+    if (4 * rb > 60 * 1024) {
+      delete s;
+      return G6_EARG;
+    }
   }
   int rcode = alloc_all(*s);
   if (rcode == 0 && hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)

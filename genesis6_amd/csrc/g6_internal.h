@@ -160,6 +160,18 @@ struct CloseArgs {
   const double *bpart;  // pair partials of the k_md_main being closed
   double dt;
 };
+// Bytes of the per-wave LDS region of k_md_main: the row sort buffers while
+// a row is rebuilt, then the compacted interacting entries (index and the
+// COM separation in the pair precision), then the lane partials of the
+// energy and virial.
+inline int main_region_bytes(int maxnb, int rsize) {
+  auto up16 = [](int x) { return (x + 15) / 16 * 16; };
+  const int sort = 8 * maxnb;
+  const int comp = up16(4 * maxnb) + 3 * up16(rsize * maxnb);
+  const int acc = 10 * 64 * 8;
+  const int mx = sort > comp ? sort : comp;
+  return up16(mx > acc ? mx : acc);
+}
 // waves of k_md_pre (64 molecules per block of four waves); each writes
 // one kinetic partial
 inline int pre_waves(int n) { return (n + 63) / 64 * 4; }

@@ -725,11 +725,11 @@ __device__ inline double rowsum16(double v) {
 
 template <int M, typename R>
 __global__ __launch_bounds__(256) void k_md_main(ModelTable mt, Geom g, MdBuf b, PairK<R> pk,
-                                                 double *out, int parity, CloseArgs ca) {
+                                                 double *out, int parity, CloseArgs ca,
+                                                 int reg_bytes) {
   using T = Traits<M>;
-  extern __shared__ int s_row[];  // [4][2][maxnb]: row sort buffers
+  extern __shared__ __align__(16) unsigned char s_dyn[];  // [4][reg_bytes]
   __shared__ int s_beg[4][128], s_pre[4][132];
-  __shared__ double s_acc[10][256];  // per-lane energy and virial partials
   __shared__ double s_cnt[4][2];
   __shared__ double s_close[4][16];
   if (ca.mode && blockIdx.x == 0)
@@ -738,55 +738,114 @@ __global__ __launch_bounds__(256) void k_md_main(ModelTable mt, Geom g, MdBuf b,
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int n = g.n;
   const int i = blockIdx.x * 4 + wid;
+  const int cap = b.maxnb;
+  unsigned char *reg = s_dyn + (size_t)wid * reg_bytes;
   double ep = 0, v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   int npair = 0, nlist = 0;  // wave-uniform counts
   if (i < n) {
+    const int *nj = b.nbr_j + (size_t)i * cap;
+    const int *nc = b.nbr_code + (size_t)i * cap;
+    constexpr int kIdle = 512 | (512 << 10) | (512 << 20);  // zero shift
+    // the first round of row entries is fetched with the row length (or
+    // right after a rebuild)
+    int jj[4], cc[4];
     int m;
     if (b.flags[2 + parity]) {
-      int *sj = s_row + wid * 2 * b.maxnb;
-      m = build_row(g, b, parity, i, lane, s_beg[wid], s_pre[wid], sj, sj + b.maxnb);
+      int *sj = reinterpret_cast<int *>(reg);
+      m = build_row(g, b, parity, i, lane, s_beg[wid], s_pre[wid], sj, sj + cap);
       nlist = m;
     } else {
       m = b.nbr_cnt[i];
     }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int k = 64 * u + lane;
+      jj[u] = k < cap ? nj[k] : i;
+      cc[u] = k < cap ? nc[k] : kIdle;
+    }
+    // entries at or past the row length hold stale or never-written values:
+    // they are replaced before any of them forms an address
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      if (64 * u + lane >= m) {
+        jj[u] = i;
+        cc[u] = kIdle;
+      }
+    }
     const double *reci = b.rec + (size_t)i * b.rs;
+    const double xi = reci[0], yi = reci[1], zi = reci[2];
+    // pass 1: the reference's FP64 interaction test on every row entry; the
+    // entries that pass are compacted in row order into LDS
+    int *cj = reinterpret_cast<int *>(reg);
+    R *dv = reinterpret_cast<R *>(reg + (4 * cap + 15) / 16 * 16);
+    const int dstride = (int)(((size_t)cap * sizeof(R) + 15) / 16 * 16 / sizeof(R));
+    wsync();  // the sort buffers of build_row share this region
+    int np = 0;
+    // four entries per lane per round: the index loads, then the record
+    // loads, are in flight together
+    for (int k0 = 0; k0 < m; k0 += 256) {
+      if (k0 > 0) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          const int k = k0 + 64 * u + lane;
+          jj[u] = k < m ? nj[k] : i;
+          cc[u] = k < m ? nc[k] : kIdle;
+        }
+      }
+      double xj[4], yj[4], zj[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const double *recj = b.rec + (size_t)jj[u] * b.rs;
+        xj[u] = recj[0];
+        yj[u] = recj[1];
+        zj[u] = recj[2];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int code = cc[u];
+        const double ox = (double)((code & 1023) - 512) * g.L;
+        const double oy = (double)(((code >> 10) & 1023) - 512) * g.L;
+        const double oz = (double)(((code >> 20) & 1023) - 512) * g.L;
+        const double d0 = (xi - xj[u]) - ox;
+        const double d1 = (yi - yj[u]) - oy;
+        const double d2 = (zi - zj[u]) - oz;
+        const bool pass = k0 + 64 * u + lane < m && d0 * d0 + d1 * d1 + d2 * d2 < g.rc2;
+        const unsigned long long mask = __ballot(pass);
+        if (pass) {
+          const int pos = np + __popcll(mask & ((1ull << lane) - 1ull));
+          cj[pos] = jj[u];
+          dv[pos] = R(d0);
+          dv[dstride + pos] = R(d1);
+          dv[2 * dstride + pos] = R(d2);
+        }
+        np += __popcll(mask);
+      }
+    }
+    npair = np;
+    wsync();
+    // pass 2: the interacting pairs, densely, two per lane per round
     R ri[T::NS][3];
     RecSites<M, R>::load(reci, ri);
-    const double xi = reci[0], yi = reci[1], zi = reci[2];
-    const int *nj = b.nbr_j + (size_t)i * b.maxnb;
-    const int *nc = b.nbr_code + (size_t)i * b.maxnb;
     double F[3] = {0, 0, 0}, Ti[3] = {0, 0, 0};
-    // the row entry of the next pass is fetched while this one is evaluated
-    int jn = lane < m ? nj[lane] : i, cn = lane < m ? nc[lane] : 0;
-    for (int k = lane; k < m; k += 64) {
-      const int j = jn, code = cn;
-      if (k + 64 < m) {
-        jn = nj[k + 64];
-        cn = nc[k + 64];
-      }
-      const double *recj = b.rec + (size_t)j * b.rs;
-      const double xj = recj[0], yj = recj[1], zj = recj[2];
-      R rj[T::NS][3];
-      RecSites<M, R>::load(recj, rj);
-      const double ox = (double)((code & 1023) - 512) * g.L;
-      const double oy = (double)(((code >> 10) & 1023) - 512) * g.L;
-      const double oz = (double)(((code >> 20) & 1023) - 512) * g.L;
-      const double d0 = (xi - xj) - ox;
-      const double d1 = (yi - yj) - oy;
-      const double d2 = (zi - zj) - oz;
-      const double dd64 = d0 * d0 + d1 * d1 + d2 * d2;
-      const bool pass = dd64 < g.rc2;
-      npair += __popcll(__ballot(pass));
-      if (pass) {
-        R Fp[3], Tp[3];
-        const R e = pair_on_i<M, R>(pk, R(dd64), R(d0), R(d1), R(d2), ri, rj, Fp, Tp);
-        F[0] += (double)Fp[0]; F[1] += (double)Fp[1]; F[2] += (double)Fp[2];
-        Ti[0] += (double)Tp[0]; Ti[1] += (double)Tp[1]; Ti[2] += (double)Tp[2];
-        ep += (double)e;
-        v[0] += d0 * (double)Fp[0]; v[1] += d0 * (double)Fp[1]; v[2] += d0 * (double)Fp[2];
-        v[3] += d1 * (double)Fp[0]; v[4] += d1 * (double)Fp[1]; v[5] += d1 * (double)Fp[2];
-        v[6] += d2 * (double)Fp[0]; v[7] += d2 * (double)Fp[1]; v[8] += d2 * (double)Fp[2];
-      }
+    auto eval = [&](int j, R d0, R d1, R d2, const R (&rj)[T::NS][3]) {
+      R Fp[3], Tp[3];
+      const R e = pair_on_i<M, R>(pk, d0 * d0 + d1 * d1 + d2 * d2, d0, d1, d2, ri, rj, Fp, Tp);
+      F[0] += (double)Fp[0]; F[1] += (double)Fp[1]; F[2] += (double)Fp[2];
+      Ti[0] += (double)Tp[0]; Ti[1] += (double)Tp[1]; Ti[2] += (double)Tp[2];
+      ep += (double)e;
+      const double e0 = (double)d0, e1 = (double)d1, e2 = (double)d2;
+      v[0] += e0 * (double)Fp[0]; v[1] += e0 * (double)Fp[1]; v[2] += e0 * (double)Fp[2];
+      v[3] += e1 * (double)Fp[0]; v[4] += e1 * (double)Fp[1]; v[5] += e1 * (double)Fp[2];
+      v[6] += e2 * (double)Fp[0]; v[7] += e2 * (double)Fp[1]; v[8] += e2 * (double)Fp[2];
+    };
+    for (int k0 = 0; k0 < np; k0 += 128) {
+      const int ka = k0 + lane, kb = k0 + 64 + lane;
+      const int ja = ka < np ? cj[ka] : i, jb = kb < np ? cj[kb] : i;
+      R ra[T::NS][3], rb[T::NS][3];
+      RecSites<M, R>::load(b.rec + (size_t)ja * b.rs, ra);
+      RecSites<M, R>::load(b.rec + (size_t)jb * b.rs, rb);
+      if (ka < np) eval(ja, dv[ka], dv[dstride + ka], dv[2 * dstride + ka], ra);
+      if (kb < np) eval(jb, dv[kb], dv[dstride + kb], dv[2 * dstride + kb], rb);
     }
 #pragma unroll
     for (int a = 0; a < 3; a++) {
@@ -812,22 +871,30 @@ __global__ __launch_bounds__(256) void k_md_main(ModelTable mt, Geom g, MdBuf b,
         }
       }
     }
+    wsync();  // the compacted entries are dead; the region takes the partials
   }
-  // block partials in a fixed order: the 256 lanes' energy and virial go
-  // through LDS, 16 lanes per row sum 16 entries each, then the row sums
-  s_acc[0][threadIdx.x] = ep;
+  // block partials in a fixed order: each wave's lane partials of energy
+  // and virial go to its LDS region, 16 lanes per row then sum 16 entries
+  // each, and the row sums are combined
+  double *acc = reinterpret_cast<double *>(reg);  // [10][64] of this wave
+  acc[lane] = ep;
 #pragma unroll
-  for (int q = 0; q < 9; q++) s_acc[1 + q][threadIdx.x] = v[q];
+  for (int q = 0; q < 9; q++) acc[(1 + q) * 64 + lane] = v[q];
   if (lane == 0) {
     s_cnt[wid][0] = (double)npair;
     s_cnt[wid][1] = (double)nlist;
   }
   __syncthreads();
   if (threadIdx.x < 160) {
+    // thread (q, seg) sums lanes seg, seg+16, ... of the 256: for each u the
+    // 16 threads of a DPP row read 16 consecutive doubles (no bank conflict)
     const int q = threadIdx.x >> 4, seg = threadIdx.x & 15;
     double r = 0.0;
 #pragma unroll
-    for (int u = 0; u < 16; u++) r += s_acc[q][seg * 16 + u];
+    for (int u = 0; u < 16; u++) {
+      const int L = seg + 16 * u;
+      r += reinterpret_cast<const double *>(s_dyn + (size_t)(L >> 6) * reg_bytes)[q * 64 + (L & 63)];
+    }
     r = rowsum16(r);
     if (seg == 0)
       b.blockpart[((size_t)parity * gridDim.x + blockIdx.x) * 16 + q] = 0.5 * r;
@@ -870,9 +937,9 @@ const PairK<double> &pair_k<double>(const System &s) { return s.pkd; }
 
 template <int M, typename R>
 static void md_main_t(System &s, double *out, int parity, const CloseArgs &ca) {
-  const size_t lds = sizeof(int) * 4 * 2 * (size_t)s.md.maxnb;
-  k_md_main<M, R><<<s.nblk_md, 256, lds, s.stream>>>(s.mt, s.geo_md, s.md, pair_k<R>(s), out,
-                                                     parity, ca);
+  const int rb = main_region_bytes(s.md.maxnb, (int)sizeof(R));
+  k_md_main<M, R><<<s.nblk_md, 256, 4 * (size_t)rb, s.stream>>>(s.mt, s.geo_md, s.md,
+                                                               pair_k<R>(s), out, parity, ca, rb);
 }
 
 void md_launch_main(System &s, double *out, int parity, int close_mode) {
