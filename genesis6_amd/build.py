@@ -1,4 +1,5 @@
-"""Builds the HIP library genesis6_amd/lib/libg6gpu.so for gfx950 (in-tree).
+"""Builds the HIP library genesis6_amd/lib/libg6gpu.so for gfx950 and the
+Fortran MD driver genesis6_amd/lib/g6md linked against it (in-tree).
 
     python -m genesis6_amd.build
 """
@@ -12,6 +13,9 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libg6gpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+FLANG = os.environ.get("FLANG", "/opt/rocm/bin/amdflang")
+FSRC = os.path.join(HERE, "fortran")
+DRIVER = os.path.join(LIBDIR, "g6md")
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",
@@ -36,5 +40,31 @@ def build(verbose=False):
     return LIB
 
 
+def build_fortran(verbose=False):
+    """The ISO_C_BINDING module and the deck-driven MD driver (flang)."""
+    if not os.path.exists(FLANG):
+        raise RuntimeError("flang not found at %s" % FLANG)
+    objdir = os.path.join(LIBDIR, "fortran_obj")
+    os.makedirs(objdir, exist_ok=True)
+    objs = []
+    for src in ("g6gpu.f90", "g6md.f90"):
+        obj = os.path.join(objdir, src.replace(".f90", ".o"))
+        cmd = [FLANG, "-O2", "-J", objdir, "-c", os.path.join(FSRC, src), "-o", obj]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stderr)
+            raise RuntimeError("flang failed on %s" % src)
+        objs.append(obj)
+    cmd = [FLANG, "-O2", "-o", DRIVER + ".tmp"] + objs + [
+        "-L" + LIBDIR, "-lg6gpu", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stderr)
+        raise RuntimeError("flang link of g6md failed")
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_fortran(verbose=True))

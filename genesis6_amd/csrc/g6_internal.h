@@ -16,7 +16,7 @@
 //     unwrapped, so |k| can exceed 1).  The -0.0 shift that dnint gives for
 //     small negative differences is stored as 0: x - (-0.0) and x - 0.0 are
 //     equal for every x but a zero x, whose sign no force term can see
-//   blockpart[2][nblk_md][16], blockek[pre waves][4]: per-block energy/virial
+//   blockpart[2][nblk_md][16], blockek[pre waves][10]: per-block energy/virial
 //     and per-wave kinetic partials, reduced in a fixed order by the step
 //     close (one block)
 // Reference-list entry point (g6_pairlist.hip): cell residents in molecule
@@ -88,7 +88,7 @@ struct MdBuf {
   int maxnb;
   double *frc;        // [6][n] force and torque (lab) of the last k_md_main
   double *blockpart;  // [2][nblk][16] k_md_main block partials, by parity
-  double *blockek;    // [k_md_pre waves][4] kinetic partials
+  double *blockek;    // [k_md_pre waves][kEkLen] kinetic partials
   double *nose, *thermo;
   int *flags;  // [0] error code, [2 + parity] rebuild rows this step
   double half_skin2;
@@ -172,6 +172,9 @@ inline int main_region_bytes(int maxnb, int rsize) {
   const int mx = sort > comp ? sort : comp;
   return up16(mx > acc ? mx : acc);
 }
+// kinetic partials per k_md_pre wave: sum |dx(1)|^2, sum w_x^2, w_y^2, w_z^2,
+// and the upper triangle xx xy xz yy yz zz of sum dx(1) dx(1)^T
+constexpr int kEkLen = 10;
 // waves of k_md_pre (64 molecules per block of four waves); each writes
 // one kinetic partial
 inline int pre_waves(int n) { return (n + 63) / 64 * 4; }

@@ -1,14 +1,16 @@
 // The MD step (the path g6_step runs), two kernels per step:
 //
-//   k_md_pre   one thread per molecule, state held in registers:
-//              rigid_correct with the forces of the previous k_md_main and
-//              block kinetic partials, then rigid_predict, rigid_setmatrix,
-//              the per-molecule record read by k_md_main (predicted COM in
-//              FP64, site offsets in the pair precision), Address_Assign
-//              binning into fixed-capacity cell lists of this step's parity,
-//              and the Verlet-skin test that decides whether rows are rebuilt.
+//   k_md_pre   64 molecules per block, one Gear role per wave, state held
+//              in registers: rigid_correct with the forces of the previous
+//              k_md_main, kinetic and kinetic-tensor partials, rigid_predict,
+//              rigid_setmatrix, the per-molecule record read by k_md_main
+//              (predicted COM in FP64, site offsets in the pair precision),
+//              Address_Assign binning into fixed-capacity cell lists of this
+//              step's parity, and the Verlet-skin test that decides whether
+//              rows are rebuilt.
 //   close      one block: reduces the block partials in a fixed order into
-//              the thermo record, Nose_Correct / Nose_Predict, and clears the
+//              the thermo record (energies, temperature, pressure, conserved
+//              H), Nose_Correct / Nose_Predict, and clears the
 //              cell lists of the other parity (block 0 of k_md_main, or
 //              k_md_close).  Pair partials alternate between two parities.
 //   k_md_main  one wavefront per molecule i:
@@ -127,7 +129,9 @@ __global__ __launch_bounds__(256) void k_md_pre(ModelTable mt, Geom g, MdBuf b, 
   const int i = blockIdx.x * 64 + lane;
   const bool ok = i < n;
   const bool q2 = role == 2;  // wave 2: components a, b; wave 3: c, d
-  double ek[4] = {0, 0, 0, 0};
+  double ek[kEkLen];  // Ek terms and the kinetic tensor
+#pragma unroll
+  for (int q = 0; q < kEkLen; q++) ek[q] = 0.0;
   double S[20];  // wave 0: com, dx; wave 1: w; waves 2-3: q (all four)
 #pragma unroll
   for (int k = 0; k < 20; k++) S[k] = 0.0;
@@ -186,6 +190,13 @@ __global__ __launch_bounds__(256) void k_md_pre(ModelTable mt, Geom g, MdBuf b, 
           S[15 + a] = S[15 + a] + kG5_5 * d;
         }
         ek[0] = S[3] * S[3] + S[4] * S[4] + S[5] * S[5];
+        // Rigid_PressureTensor sums (RigidModule.F90:600-619), upper triangle
+        ek[4] = S[3] * S[3];
+        ek[5] = S[3] * S[4];
+        ek[6] = S[3] * S[5];
+        ek[7] = S[4] * S[4];
+        ek[8] = S[4] * S[5];
+        ek[9] = S[5] * S[5];
       }
     } else if (role == 1) {
       if (ok) {
@@ -431,14 +442,19 @@ __global__ __launch_bounds__(256) void k_md_pre(ModelTable mt, Geom g, MdBuf b, 
     }
   }
   // kinetic partials of this wave, reduced by the step close
-  if (do_correct && role < 2) {
+  // each wave reduces the partials its role produced (the rest are zero)
+  if (do_correct && role == 0) {
+    ek[0] = wsum63(ek[0]);
 #pragma unroll
-    for (int q = 0; q < 4; q++) ek[q] = wsum63(ek[q]);
+    for (int q = 4; q < kEkLen; q++) ek[q] = wsum63(ek[q]);
+  } else if (do_correct && role == 1) {
+#pragma unroll
+    for (int q = 1; q < 4; q++) ek[q] = wsum63(ek[q]);
   }
   if (lane == 63) {
     const size_t wv = (size_t)blockIdx.x * 4 + role;
 #pragma unroll
-    for (int q = 0; q < 4; q++) b.blockek[wv * 4 + q] = ek[q];
+    for (int q = 0; q < kEkLen; q++) b.blockek[wv * kEkLen + q] = ek[q];
   }
 }
 
@@ -451,12 +467,13 @@ __global__ __launch_bounds__(256) void k_md_pre(ModelTable mt, Geom g, MdBuf b, 
 // block 0 of k_md_main, which neither reads nor writes what it touches.
 __device__ inline void close_body(const ModelTable &mt, const MdBuf &b, int n, double dt,
                                   int nblk_pre, const double *bpart, int nblk_main, int ncell,
-                                  int nose_active, int mode, int parity,
-                                  double (&s_red)[4][16]) {
+                                  int nose_active, int mode, int parity, double box_l,
+                                  double (&s_red)[4][12 + kEkLen]) {
+  constexpr int NR = 12 + kEkLen;
   const int wid = threadIdx.x >> 6;
-  double a2[16];
+  double a2[NR];
 #pragma unroll
-  for (int q = 0; q < 16; q++) a2[q] = 0.0;
+  for (int q = 0; q < NR; q++) a2[q] = 0.0;
   if (mode & 17) {
 #pragma unroll 4
     for (int k = threadIdx.x; k < nblk_main; k += 256)
@@ -467,13 +484,13 @@ __device__ inline void close_body(const ModelTable &mt, const MdBuf &b, int n, d
 #pragma unroll 2
     for (int k = threadIdx.x; k < nblk_pre; k += 256)
 #pragma unroll
-      for (int q = 0; q < 4; q++) a2[12 + q] += b.blockek[(size_t)k * 4 + q];
+      for (int q = 0; q < kEkLen; q++) a2[12 + q] += b.blockek[(size_t)k * kEkLen + q];
   }
 #pragma unroll
-  for (int q = 0; q < 16; q++) a2[q] = wsum63(a2[q]);
+  for (int q = 0; q < NR; q++) a2[q] = wsum63(a2[q]);
   if ((threadIdx.x & 63) == 63)
 #pragma unroll
-    for (int q = 0; q < 16; q++) s_red[wid][q] = a2[q];
+    for (int q = 0; q < NR; q++) s_red[wid][q] = a2[q];
   if (mode & 12) {
     const int p0 = (mode & 8) ? 0 : 1 - parity, p1 = (mode & 8) ? 1 : 1 - parity;
     for (int p = p0; p <= p1; p++) {
@@ -484,8 +501,8 @@ __device__ inline void close_body(const ModelTable &mt, const MdBuf &b, int n, d
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double r[16];
-    for (int q = 0; q < 16; q++) r[q] = s_red[0][q] + s_red[1][q] + s_red[2][q] + s_red[3][q];
+    double r[NR];
+    for (int q = 0; q < NR; q++) r[q] = s_red[0][q] + s_red[1][q] + s_red[2][q] + s_red[3][q];
     double *z = b.nose, *th = b.thermo;
     if (mode & 17) {
       th[0] = r[0];
@@ -500,6 +517,7 @@ __device__ inline void close_body(const ModelTable &mt, const MdBuf &b, int n, d
       const double dof = 6.0 * n;
       const double temp = 10.0 * 2.0 * (ekt + ekr) / (G6_RGAS_MD * dof);
       double h = (ep + ekt + ekr) * 10.0;
+      double nose_ep = 0.0, nose_ek = 0.0;
       if (nose_active) {
         const double zetaf = (temp - z[0]) / z[1] - z[2] * z[3];
         const double dz2 = zetaf - z[4];
@@ -509,13 +527,25 @@ __device__ inline void close_body(const ModelTable &mt, const MdBuf &b, int n, d
         z[6] = z[6] + kG4_3 * dz2;
         z[7] = z[7] + kG4_4 * dz2;
         z[8] = z[8] + z[3];
-        h = h + (dof * z[0] * (z[8] - 0.5 * z[3]) * dt) * G6_RGAS_MD +
-            (z[3] * z[3] * z[1] * dt * dof * 0.5) * G6_RGAS_MD;
+        nose_ek = (z[3] * z[3] * z[1] * dt * dof * 0.5) * G6_RGAS_MD;
+        nose_ep = (dof * z[0] * (z[8] - 0.5 * z[3]) * dt) * G6_RGAS_MD;
+        h = h + nose_ep + nose_ek;
       }
       th[1] = ekt;
       th[2] = ekr;
       th[3] = temp;
       th[13] = h;
+      // kinetic tensor (RigidModule.F90:617) and pressure (Mixture.F90:2243-2255)
+      const double km = mt.mass * (1.0 / (dt * dt));
+      const double kt[9] = {r[16] * km, r[17] * km, r[18] * km, r[17] * km, r[19] * km,
+                            r[20] * km, r[18] * km, r[20] * km, r[21] * km};
+      for (int q = 0; q < 9; q++) th[16 + q] = kt[q];
+      const double vp = (kt[0] + th[4]) + (kt[4] + th[8]) + (kt[8] + th[12]);
+      const double vol = box_l * box_l * box_l;
+      th[25] = (vp / 3.0) * 10.0 * (1.0 / 101326.0) / (vol * 6.0225e23 * (1e-10 * 1e-10 * 1e-10));
+      th[26] = vol;
+      th[27] = nose_ep;
+      th[28] = nose_ek;
     }
     if ((mode & 2) && nose_active) {
       z[3] = z[3] + z[4] + z[5] + z[6] + z[7];
@@ -529,9 +559,10 @@ __device__ inline void close_body(const ModelTable &mt, const MdBuf &b, int n, d
 __global__ __launch_bounds__(256) void k_md_close(ModelTable mt, MdBuf b, int n, double dt,
                                                   int nblk_pre, const double *bpart,
                                                   int nblk_main, int ncell, int nose_active,
-                                                  int mode, int parity) {
-  __shared__ double s_red[4][16];
-  close_body(mt, b, n, dt, nblk_pre, bpart, nblk_main, ncell, nose_active, mode, parity, s_red);
+                                                  int mode, int parity, double box_l) {
+  __shared__ double s_red[4][12 + kEkLen];
+  close_body(mt, b, n, dt, nblk_pre, bpart, nblk_main, ncell, nose_active, mode, parity, box_l,
+             s_red);
 }
 
 // ---- kernel B --------------------------------------------------------------
@@ -731,10 +762,10 @@ __global__ __launch_bounds__(256) void k_md_main(ModelTable mt, Geom g, MdBuf b,
   extern __shared__ __align__(16) unsigned char s_dyn[];  // [4][reg_bytes]
   __shared__ int s_beg[4][128], s_pre[4][132];
   __shared__ double s_cnt[4][2];
-  __shared__ double s_close[4][16];
+  __shared__ double s_close[4][12 + kEkLen];
   if (ca.mode && blockIdx.x == 0)
     close_body(mt, b, g.n, ca.dt, ca.nblk_pre, ca.bpart, ca.nblk_main, g.ncell, ca.nose_active,
-               ca.mode, parity, s_close);
+               ca.mode, parity, g.L, s_close);
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int n = g.n;
   const int i = blockIdx.x * 4 + wid;
@@ -925,7 +956,7 @@ static CloseArgs close_args(System &s, int mode, int src_parity) {
 void md_launch_close(System &s, int mode, int parity, int src_parity) {
   const CloseArgs ca = close_args(s, mode, src_parity);
   k_md_close<<<1, 256, 0, s.stream>>>(s.mt, s.md, s.n, s.dt, ca.nblk_pre, ca.bpart, ca.nblk_main,
-                                      s.geo_md.ncell, ca.nose_active, mode, parity);
+                                      s.geo_md.ncell, ca.nose_active, mode, parity, s.L);
 }
 
 template <typename R>

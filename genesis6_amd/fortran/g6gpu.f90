@@ -1,0 +1,175 @@
+! ISO_C_BINDING interface to libg6gpu (include/g6gpu.h): one interface per
+! entry point, argument for argument.  Arrays are passed as assumed-size
+! dummies, so a caller hands over its own contiguous arrays (component-major
+! layouts such as com(n,3), quat(n,4,5), dx(n,3,5) are the C layouts).
+! Every function returns 0 or a negative G6_E* code.
+module g6gpu
+  use, intrinsic :: iso_c_binding
+  implicit none
+  integer(c_int), parameter :: G6_TIP4P = 0, G6_TIP5P = 1, G6_ST2 = 2
+  integer(c_int), parameter :: G6_FP64 = 0, G6_FP32 = 1
+  integer(c_int), parameter :: G6_EHIP = -1, G6_EARG = -2, G6_ECAPACITY = -3, &
+       G6_ENODEVICE = -4
+  integer, parameter :: G6_THERMO_LEN = 32
+
+  interface
+     integer(c_int) function g6_init(h, model, n, box_l, rc, taper, voxa, dt, prec, skin) &
+          bind(C, name="g6_init")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), intent(out) :: h
+       integer(c_int), value :: model, n, prec
+       real(c_double), value :: box_l, rc, taper, voxa, dt, skin
+     end function g6_init
+
+     integer(c_int) function g6_finalize(h) bind(C, name="g6_finalize")
+       import :: c_int, c_int64_t
+       integer(c_int64_t), value :: h
+     end function g6_finalize
+
+     integer(c_int) function g6_upload_state(h, com, quat, dx, w) bind(C, name="g6_upload_state")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), value :: h
+       real(c_double), intent(in) :: com(*), quat(*), dx(*), w(*)
+     end function g6_upload_state
+
+     integer(c_int) function g6_download_state(h, com, quat, dx, w) &
+          bind(C, name="g6_download_state")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), value :: h
+       real(c_double), intent(out) :: com(*), quat(*), dx(*), w(*)
+     end function g6_download_state
+
+     integer(c_int) function g6_set_nose(h, active, nose) bind(C, name="g6_set_nose")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), value :: h
+       integer(c_int), value :: active
+       real(c_double), intent(in) :: nose(9)
+     end function g6_set_nose
+
+     integer(c_int) function g6_get_nose(h, nose) bind(C, name="g6_get_nose")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), value :: h
+       real(c_double), intent(out) :: nose(9)
+     end function g6_get_nose
+
+     integer(c_int) function g6_vreset(h, temp, seed) bind(C, name="g6_vreset")
+       import :: c_int, c_int64_t, c_double, c_int32_t
+       integer(c_int64_t), value :: h
+       real(c_double), value :: temp
+       integer(c_int32_t), value :: seed   ! uint32_t on the C side
+     end function g6_vreset
+
+     integer(c_int) function g6_step(h, nsteps) bind(C, name="g6_step")
+       import :: c_int, c_int64_t
+       integer(c_int64_t), value :: h
+       integer(c_int), value :: nsteps
+     end function g6_step
+
+     integer(c_int) function g6_get_thermo(h, thermo) bind(C, name="g6_get_thermo")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), value :: h
+       real(c_double), intent(out) :: thermo(*)
+     end function g6_get_thermo
+
+     integer(c_int) function g6_eval_forces(h, force, torque, thermo) &
+          bind(C, name="g6_eval_forces")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), value :: h
+       real(c_double), intent(out) :: force(*), torque(*), thermo(*)
+     end function g6_eval_forces
+
+     integer(c_int) function g6_pairlist(h, com, cap, npair0, npair, pair_i, pair_j, ox, oy, oz) &
+          bind(C, name="g6_pairlist")
+       import :: c_int, c_int64_t, c_int32_t, c_double
+       integer(c_int64_t), value :: h, cap
+       real(c_double), intent(in) :: com(*)
+       integer(c_int64_t), intent(out) :: npair0, npair
+       integer(c_int32_t), intent(out) :: pair_i(*), pair_j(*)
+       real(c_double), intent(out) :: ox(*), oy(*), oz(*)
+     end function g6_pairlist
+
+     integer(c_int) function g6_cells(h, com, cell, ndiv) bind(C, name="g6_cells")
+       import :: c_int, c_int64_t, c_int32_t, c_double
+       integer(c_int64_t), value :: h
+       real(c_double), intent(in) :: com(*)
+       integer(c_int32_t), intent(out) :: cell(*), ndiv
+     end function g6_cells
+
+     integer(c_int) function g6_cutoff(h, npair, pair_i, pair_j, com, ox, oy, oz, eratio, fratio) &
+          bind(C, name="g6_cutoff")
+       import :: c_int, c_int64_t, c_int32_t, c_double
+       integer(c_int64_t), value :: h, npair
+       integer(c_int32_t), intent(in) :: pair_i(*), pair_j(*)
+       real(c_double), intent(in) :: com(*), ox(*), oy(*), oz(*)
+       real(c_double), intent(out) :: eratio(*), fratio(*)
+     end function g6_cutoff
+
+     integer(c_int) function g6_force(h, npair, pair_i, pair_j, ox, oy, oz, eratio, fratio, &
+          x, y, z, fx, fy, fz, ep, vrmat) bind(C, name="g6_force")
+       import :: c_int, c_int64_t, c_int32_t, c_double
+       integer(c_int64_t), value :: h, npair
+       integer(c_int32_t), intent(in) :: pair_i(*), pair_j(*)
+       real(c_double), intent(in) :: ox(*), oy(*), oz(*), eratio(*), fratio(*), x(*), y(*), z(*)
+       real(c_double), intent(inout) :: fx(*), fy(*), fz(*)
+       real(c_double), intent(out) :: ep(*), vrmat(9)
+     end function g6_force
+
+     integer(c_int) function g6_pair_energy(h, npair, pair_i, pair_j, ox, oy, oz, eratio, &
+          x, y, z, ep) bind(C, name="g6_pair_energy")
+       import :: c_int, c_int64_t, c_int32_t, c_double
+       integer(c_int64_t), value :: h, npair
+       integer(c_int32_t), intent(in) :: pair_i(*), pair_j(*)
+       real(c_double), intent(in) :: ox(*), oy(*), oz(*), eratio(*), x(*), y(*), z(*)
+       real(c_double), intent(out) :: ep(*)
+     end function g6_pair_energy
+
+     integer(c_int) function g6_site_positions(h, com, quat1, x, y, z) &
+          bind(C, name="g6_site_positions")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), value :: h
+       real(c_double), intent(in) :: com(*), quat1(*)
+       real(c_double), intent(out) :: x(*), y(*), z(*)
+     end function g6_site_positions
+
+     integer(c_int) function g6_collect(h, quat1, fx, fy, fz, force, torque) &
+          bind(C, name="g6_collect")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), value :: h
+       real(c_double), intent(in) :: quat1(*), fx(*), fy(*), fz(*)
+       real(c_double), intent(out) :: force(*), torque(*)
+     end function g6_collect
+
+     integer(c_int) function g6_lattice_start(n, box_l, seed, com, quat1) &
+          bind(C, name="g6_lattice_start")
+       import :: c_int, c_int32_t, c_double
+       integer(c_int), value :: n
+       real(c_double), value :: box_l
+       integer(c_int32_t), value :: seed
+       real(c_double), intent(out) :: com(*), quat1(*)
+     end function g6_lattice_start
+
+     integer(c_int) function g6_step_timed(h, nsteps, t) bind(C, name="g6_step_timed")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), value :: h
+       integer(c_int), value :: nsteps
+       real(c_double), intent(out) :: t(5)
+     end function g6_step_timed
+
+     integer(c_int) function g6_model_nsite(model) bind(C, name="g6_model_nsite")
+       import :: c_int
+       integer(c_int), value :: model
+     end function g6_model_nsite
+  end interface
+
+contains
+
+  ! stops with the reference's behaviour on an error (Error.F90:12-21: die)
+  subroutine g6_check(rc, what)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: what
+    if (rc /= 0) then
+       write(0, '(a,a,a,i0)') 'g6gpu: ', what, ' failed with code ', rc
+       error stop 1
+    end if
+  end subroutine g6_check
+end module g6gpu

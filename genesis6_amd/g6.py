@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "lib", "libg6gpu.so")
 TIP4P, TIP5P, ST2 = 0, 1, 2
 FP64, FP32 = 0, 1
-THERMO_LEN = 16
+THERMO_LEN = 32
 ERRORS = {-1: "HIP runtime error", -2: "invalid argument",
           -3: "pair capacity exceeded", -4: "no HIP device"}
 

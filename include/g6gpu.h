@@ -38,12 +38,17 @@ enum {
   G6_ECAPACITY = -3,  /* pair capacity exceeded: nothing truncated        */
   G6_ENODEVICE = -4   /* no HIP device                                    */
 };
-/* thermo record of the last step, written by g6_get_thermo:
+/* thermo record of the last step, written by g6_get_thermo (internal
+ * energy units unless stated):
  * [0] ep (sum of pair energies), [1] ekt, [2] ekr, [3] temperature (K),
- * [4..12] virial vrmat(3,3) row-major, [13] conserved H (J/mol,
+ * [4..12] pair virial vrmat(3,3) row-major, [13] conserved H (J/mol,
  * Mixture.F90:2431-2468), [14] interacting pairs (dd < rc^2),
- * [15] pairs in the current list (differs from [14] only when skin > 0) */
-#define G6_THERMO_LEN 16
+ * [15] pairs in the current list (differs from [14] only when skin > 0),
+ * [16..24] kinetic tensor of Rigid_PressureTensor (RigidModule.F90:600-619),
+ * [25] pressure (atm, Mixture.F90:2243-2255), [26] volume (A^3),
+ * [27] Nose potential term nose_ep, [28] Nose kinetic term nose_ek (J/mol,
+ * Mixture.F90:2431-2434), [29..31] zero */
+#define G6_THERMO_LEN 32
 
 /* ---- system lifecycle (Mixture.F90:1141-1347 setup) -------------------- */
 /* model: G6_*; n molecules in a cubic box of edge box_l (@BXLA); cutoff rc

@@ -19,3 +19,27 @@ def lattice_start(n, L, seed):
                   np.sqrt(u[:, 0]) * np.sin(2 * np.pi * u[:, 2]),
                   np.sqrt(u[:, 0]) * np.cos(2 * np.pi * u[:, 2])], 1)
     return com, q
+
+
+def mixture_deck(com, q, L, rc, taper, dt, nsteps, seed, nose=None, model="TIP4P", nlog=1):
+    """A Mixture input deck (the reference's tags and payload layout,
+    Mixture.F90:600-1140): @DTPS, @MDLP, @NLOG, @RCOA, @BXLA, @VOXA, @VSET,
+    optional @NOS2 (T, q, dump), @ID08, @NX4A with x y z a b c d per line."""
+    n = len(com)
+    dt, rc, taper, L = float(dt), float(rc), float(taper), float(L)
+    deck = ["@DTPS", repr(dt), "@MDLP", str(int(nsteps)), "@NLOG", str(int(nlog)), "@RCOA",
+            "%r %r" % (rc, taper), "@BXLA", repr(L), "@VOXA", "%r %r %r" % (rc, rc, rc),
+            "@VSET", "300.0 %d" % int(seed)]
+    if nose is not None:
+        deck += ["@NOS2", "%r %r %r 0 0 0 0 0 0" % tuple(float(v) for v in nose)]
+    deck += ["@ID08", model, "@NX4A", str(n)]
+    deck += [" ".join("%.17e" % v for v in list(com[i]) + list(q[i])) for i in range(n)]
+    return "\n".join(deck) + "\n"
+
+
+def parse_logd(text):
+    """The @LOGD rows of a Mixture-format log as an array (one row per line
+    after each @LOGD tag)."""
+    lines = text.split("\n")
+    rows = [lines[i + 1] for i, l in enumerate(lines[:-1]) if l.strip() == "@LOGD"]
+    return np.array([[float(v) for v in r.split()] for r in rows])

@@ -18,7 +18,7 @@ import tempfile
 import numpy as np
 
 sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
-from configs import lattice_start  # noqa: E402
+from configs import lattice_start, mixture_deck, parse_logd  # noqa: E402
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
@@ -90,18 +90,10 @@ def fortran_records(path):
 def mixture(com, q, L, rc, taper, dt, nsteps, seed, nose=None):
     """Runs the reference MD driver; returns @LOGD rows and the final @WTG2."""
     n = len(com)
-    deck = ["@DTPS", repr(dt), "@MDLP", str(nsteps), "@NLOG", "1", "@RCOA",
-            "%r %r" % (rc, taper), "@BXLA", repr(L), "@VOXA", "%r %r %r" % (rc, rc, rc),
-            "@VSET", "300.0 %d" % seed]
-    if nose is not None:
-        deck += ["@NOS2", "%r %r %r 0 0 0 0 0 0" % nose]
-    deck += ["@ID08", "TIP4P", "@NX4A", str(n)]
-    deck += [" ".join("%.17e" % v for v in list(com[i]) + list(q[i])) for i in range(n)]
+    deck = mixture_deck(com, q, L, rc, taper, dt, nsteps, seed, nose)
     with tempfile.TemporaryDirectory() as d:
-        out = run(os.path.join(REF, "Mixture"), d, "\n".join(deck) + "\n")
-        lines = out.split("\n")
-        rows = [lines[i + 1] for i, l in enumerate(lines[:-1]) if l.strip() == "@LOGD"]
-        logd = np.array([[float(v) for v in r.split()] for r in rows])
+        out = run(os.path.join(REF, "Mixture"), d, deck)
+        logd = parse_logd(out)
         recs = fortran_records(os.path.join(d, "fort.13"))
         k = [i for i, r in enumerate(recs) if r[:5] == b"@WTG2"][0]
         nn = struct.unpack("<i", recs[k + 1])[0]

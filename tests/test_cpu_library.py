@@ -50,3 +50,30 @@ def test_no_device_is_an_error_not_a_fallback():
         pass
     with pytest.raises(g6.G6Error):
         g6.System(g6.TIP4P, 216, 18.626, 8.5, 0.5, 0.002)
+
+
+def test_fortran_interface_binds_every_declared_symbol():
+    """The ISO_C_BINDING module declares one interface per C entry point,
+    under the C name."""
+    src = open(os.path.join(ROOT, "genesis6_amd", "fortran", "g6gpu.f90")).read()
+    bound = sorted(set(re.findall(r'bind\(C,\s*name="(g6_\w+)"\)', src)))
+    assert bound == declared_functions()
+
+
+def test_fortran_driver_reports_a_missing_device():
+    """g6md is built and reads a deck; without a device it stops with the
+    library's error code instead of running anything on the host."""
+    import subprocess
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a device is present")
+    except ImportError:
+        pass
+    exe = os.path.join(ROOT, "genesis6_amd", "lib", "g6md")
+    if not os.path.exists(exe):
+        pytest.skip("g6md not built (python -m genesis6_amd.build)")
+    deck = "@BXLA\n18.626\n@ID08\nTIP4P\n@NX4A\n2\n0 0 0 1 0 0 0\n3 0 0 1 0 0 0\n"
+    r = subprocess.run([exe], input=deck, capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0
+    assert "g6_init failed with code -4" in r.stderr
