@@ -107,7 +107,7 @@ def main():
         return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the system is not sharded across GPUs "
-              "(see DESIGN.md, 'More than one GPU'); run with --gpus 1", file=sys.stderr)
+              "(the single-system step runs on one device); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     from genesis6_amd import g6
     print("bench: TIP4P-%d FP32-pair, %d warmup + %d steps" % (N, a.warmup, a.steps),

@@ -19,7 +19,7 @@ DRIVER = os.path.join(LIBDIR, "g6md")
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",
-    # packed FP32 instructions are kept out of every kernel on this pool
+    # no packed FP32 instructions in any kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics", "-Wno-unused-command-line-argument",
 ]

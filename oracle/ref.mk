@@ -2,7 +2,8 @@
 # /root/reference, into oracle/_ref/ (objects + .mod files), then links the
 # fixture dumper refdump.F90 (our code); `make -f oracle/ref.mk mixture`
 # also links the reference MD driver Mixture.
-# Only ever run in the build container; nothing here travels to the GPU box.
+# Builds the reference from its own sources on the host that holds them; nothing
+# built here is loaded by the GPU tests or the benchmark.
 #
 #   make -f oracle/ref.mk            (from the repository root)
 #

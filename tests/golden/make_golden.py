@@ -1,10 +1,10 @@
 """Writes the golden fixtures under tests/golden/ by running the reference.
 
-Runs ONLY in the build container, where /root/reference exists and
+Runs only where /root/reference exists and
 `make -f oracle/ref.mk` has built oracle/_ref/refdump (our dumper, linked
 against the reference's own unmodified modules) and oracle/_ref/Mixture (the
 reference MD driver).  The fixtures are data: inputs plus the reference's
-outputs.  Nothing here runs on the GPU box.
+outputs.  The GPU tests read only the fixtures it writes.
 
     python tests/golden/make_golden.py
 """
