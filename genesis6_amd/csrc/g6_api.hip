@@ -63,6 +63,12 @@ bool make_model_table(int model, int prec, ModelTable *m) {
     }
     m->lj_a = 600000.0 * kCA * 100.0;
     m->lj_b = -610.0 * kCA * 100.0;
+    // tip4p_setinteraction (TIP4PModule.F90:43-68): charges WM, WH; O LJ
+    // EPSWO, SIGWO (TIP4PConst.F90:7-17)
+    m->std_q[1] = -1.04;
+    m->std_q[2] = m->std_q[3] = 0.52;
+    m->std_eps[0] = 0.15504166666666666666666 * kCA;
+    m->std_sig[0] = 3.15357794197649532464;
     set_inverses(m);
     return true;
   }
@@ -95,6 +101,11 @@ bool make_model_table(int model, int prec, ModelTable *m) {
     m->mass = 2.0 * hmass + omass;
     m->ncoul = 16;
     if (!st2) {
+      // tip5p_setinteraction (TIP5PModule.F90:57-83)
+      m->std_q[1] = m->std_q[2] = 0.2410;
+      m->std_q[3] = m->std_q[4] = -0.2410;
+      m->std_eps[0] = 0.6694;
+      m->std_sig[0] = 3.12;
       // StdForce coefficients (StdInteraction.F90:206-221), the live TIP5P path
       const double q[6] = {0.0, 0.2410, 0.2410, -0.2410, -0.2410, 0.0};
       const double KX2X = 1.0 / 1e-3, J2I = 1.0 / 10.0;
@@ -199,6 +210,8 @@ static System *get(g6_handle h) {
   return g_sys[h - 1];
 }
 
+System *system_of(g6_handle h) { return get(h); }
+
 static int alloc_all(System &s) {
   const int n = s.n, ns = s.ns, nc = s.geo.ncell, ncm = s.geo_md.ncell;
   auto A = [](void **p, size_t b) { return hipMalloc(p, b > 0 ? b : 8); };
@@ -268,6 +281,7 @@ static void free_all(System &s) {
                 s.com_ref, s.lflags};
   for (void *p : ps)
     if (p) (void)hipFree(p);
+  mc_free(s);
   if (s.graph) (void)hipGraphExecDestroy(s.graph);
   for (auto &e : s.ev) (void)hipEventDestroy(e);
   if (s.stream) (void)hipStreamDestroy(s.stream);

@@ -27,6 +27,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "../../include/g6gpu.h"
 
 namespace g6 {
@@ -48,6 +50,21 @@ struct ModelTable {
   double Ixx, Iyy, Izz, mass;
   double inv_ixx, inv_iyy, inv_izz, inv_mass;  // 1/I and 1/m, rounded on the host
   double rl, ru;  // ST2 short-range switch on the COM distance
+  // StdInteraction site parameters (si%param: charge, LJ eps, LJ sigma) of
+  // TIP4P_SetInteraction / TIP5P_SetInteraction, for the energy-only path
+  double std_q[kMaxSite], std_eps[kMaxSite], std_sig[kMaxSite];
+};
+
+// StdPairPotential site terms (StdInteraction.F90:633-762), site a of i
+// outer and b of j inner; kind 1 LJ+Coulomb, 2 Coulomb, 3 LJ.
+struct StdTerm {
+  int a, b, kind;
+  double qq, aa, bb;
+};
+constexpr int kMaxStd = kMaxSite * kMaxSite;
+struct StdTable {
+  int nt;
+  StdTerm t[kMaxStd];
 };
 
 // Pair constants of the MD force kernel in its arithmetic type R, rounded
@@ -94,6 +111,33 @@ struct MdBuf {
   double half_skin2;
 };
 
+// Monte Carlo state (g6_mc.hip): the mc2 single-molecule trial loop on a
+// configuration of its own, FP64 throughout.
+//   com[3][n], quat[4][n] (order-1 quaternions), sx/sy/sz[n*ns] site
+//   positions, pinfo[n][n] PairInfo pair energies (symmetric), delta[n] and
+//   rtr[n] step sizes, u[ucap] the Random_GetNext stream of this call and
+//   z[ucap] the Box-Muller factor sqrt(-2 ln u[k]) sin(2 pi u[k+1]) of each
+//   position, pos/nacc the stream position and accept count, epsum.
+struct McBuf {
+  double *com = nullptr, *quat = nullptr, *sx = nullptr, *sy = nullptr, *sz = nullptr;
+  double *pinfo = nullptr, *delta = nullptr, *rtr = nullptr;
+  double *u = nullptr, *z = nullptr, *epsum = nullptr, *part = nullptr;
+  int *pos = nullptr;
+  long long *nacc = nullptr;
+  int ucap = 0;
+};
+
+// Host side of the MC random stream: the reference's MT19937 (mtrnd.f) and
+// the Random_GetNext values generated but not yet consumed by the device.
+struct McHost {
+  uint32_t mt[624];
+  int mti = 625;
+  std::vector<double> pend;
+  void seed(uint32_t s);
+  uint32_t next();
+  double getnext();  // 1 - the 31-bit uniform of mtrndv (RandomModule.F90:123-130)
+};
+
 struct System {
   int model = 0, n = 0, ns = 0, prec = 0;
   double L = 0, rc = 0, taper = 0, req = 0, dt = 0, skin = 0;
@@ -101,6 +145,11 @@ struct System {
   Geom geo{};     // the reference grid (ndiv = int(L/voxa)), entry points
   Geom geo_md{};  // the MD grid (cell edge >= rc + skin)
   MdBuf md{};
+  McBuf mc{};
+  bool mc_ready = false;
+  McHost mch;
+  double mc_beta = 0;  // 1/T of the MC run
+  StdTable std{};
   int pcap = 0;
   bool nose_active = false;
   double nose_h[9] = {0};
@@ -130,6 +179,9 @@ struct System {
 
 // model tables (g6_api.hip)
 bool make_model_table(int model, int prec, ModelTable *mt);
+void make_std_table(const ModelTable &mt, StdTable *t);
+void mc_free(System &s);
+System *system_of(g6_handle h);  // nullptr for an unknown handle
 template <typename R>
 PairK<R> make_pair_k(const ModelTable &mt, const Geom &g) {
   PairK<R> k{};

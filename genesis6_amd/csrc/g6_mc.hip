@@ -1,0 +1,626 @@
+// Monte Carlo on the GPU: the PairInfo pair-energy matrix (PairInfo.F90:12-46,
+// built as mc2.F90:303-340 builds it) and the single-molecule trial loop of
+// mc2 (mc2.F90:540-862, with DEBUG20050322a as the reference compiles it).
+//
+// The trial loop is sequential by definition, so it runs as one persistent
+// block of 512 threads.  Per trial every thread derives the same proposal
+// from the shared random stream (no broadcast needed), each thread tests the
+// molecules it owns against the moved target (the reference's compressed
+// pair list, radius rc + dt, energies by StdPairPotential), one block
+// reduction forms dE in a fixed order, and every thread takes the same
+// Metropolis decision.  Accepted moves update the target and its PairInfo row
+// and column in place.
+//
+// Random numbers: the host generates the Random_GetNext stream (MT19937, the
+// reference's mtrnd) and, for every position k, the Box-Muller factor
+// sqrt(-2 ln u[k]) sin(2 pi u[k+1]) with the host libm, as Random_Normal
+// evaluates it (RandomModule.F90:28-36).  The kernel picks the factors at the
+// positions a trial consumes, so no device transcendental enters a proposal;
+// only the Metropolis ratio exp() is evaluated on the device.
+#include <math.h>
+
+#include <vector>
+
+#include "g6_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace g6 {
+
+constexpr double kI2J = 10.0, kJ2K = 0.120273;  // PhysConst.F90:48, 56
+// INCREMENT=1.05 and the 0.05 floor are default-real literals in the
+// reference (mc2.F90:61, 811): their single-precision values
+constexpr double kIncrement = (double)1.05f, kDeltaMin = (double)0.05f;
+constexpr int kMcThreads = 512;
+constexpr int kMcMaxPer = 4;  // molecules per thread: n <= 2048
+
+void make_std_table(const ModelTable &m, StdTable *t) {
+  // StdInteraction.F90:201-221: qq = q_a q_b COEFF KX2X J2I scale,
+  // aa = eps_a eps_b, LJ from sqrt(aa) and the mean sigma to the 6th power
+  // COEFF = CA*332.17752, KX2X = 1/X2KX, J2I = 1/I2J (PhysConst.F90:31-50)
+  const double coeff = 4.184 * 332.17752, kx2x = 1.0 / 1e-3, j2i = 1.0 / 10.0, scale = 1.0;
+  t->nt = 0;
+  for (int a = 0; a < m.nsite; a++)
+    for (int b = 0; b < m.nsite; b++) {
+      const double qq = m.std_q[a] * m.std_q[b] * coeff * kx2x * j2i * scale;
+      const double aa = m.std_eps[a] * m.std_eps[b];
+      if (qq == 0.0 && aa == 0.0) continue;
+      StdTerm s{a, b, 0, qq, 0.0, 0.0};
+      if (aa != 0.0) {
+        const double ljeps = sqrt(aa);
+        double sg = (m.std_sig[a] + m.std_sig[b]) * 0.5;
+        sg = sg * sg * sg * sg * sg * sg;
+        s.aa = 4.0 * ljeps * sg * sg * kx2x * j2i * scale;
+        s.bb = -4.0 * ljeps * sg * kx2x * j2i * scale;
+      }
+      s.kind = qq != 0.0 ? (aa != 0.0 ? 1 : 2) : 3;
+      t->t[t->nt++] = s;
+    }
+}
+
+// The site pairs of StdPairPotential's term list for the two MC models, in
+// the order make_std_table produces them (checked at g6_mc_begin), so that
+// the trial kernel indexes sites with constants.
+template <int M>
+struct StdPattern;
+template <>
+struct StdPattern<G6_TIP4P> {  // O-O LJ, then M,H,H x M,H,H Coulomb
+  static constexpr int NT = 10, NS = 5;
+  __device__ __host__ static constexpr int a(int u) { return u == 0 ? 0 : 1 + (u - 1) / 3; }
+  __device__ __host__ static constexpr int b(int u) { return u == 0 ? 0 : 1 + (u - 1) % 3; }
+};
+template <>
+struct StdPattern<G6_TIP5P> {  // O-O LJ, then H,H,L,L x H,H,L,L Coulomb
+  static constexpr int NT = 17, NS = 6;
+  __device__ __host__ static constexpr int a(int u) { return u == 0 ? 0 : 1 + (u - 1) / 4; }
+  __device__ __host__ static constexpr int b(int u) { return u == 0 ? 0 : 1 + (u - 1) % 4; }
+};
+
+template <int M>
+static bool std_table_matches(const StdTable &t) {
+  if (t.nt != StdPattern<M>::NT) return false;
+  for (int u = 0; u < t.nt; u++)
+    if (t.t[u].a != StdPattern<M>::a(u) || t.t[u].b != StdPattern<M>::b(u)) return false;
+  return true;
+}
+
+// Everything a kernel needs about the system, by value.
+struct McArgs {
+  int n, ns;
+  double L, invL, rc, rin, c1;  // cutoff: S between rin and rc
+  double beta;
+  double bx[kMaxSite], by[kMaxSite], bz[kMaxSite];
+  McBuf b;
+};
+
+// CutOff_Dimmer2's S(r) on the COM distance squared (CutoffModule.F90:220-278)
+__device__ inline double mc_switch(const McArgs &a, double dd) {
+  double hh = 0.0;
+  if (dd < a.rc * a.rc) {
+    if (dd > a.rin * a.rin) {
+      const double dr = sqrt(dd), u = dr - a.rc, l = dr - a.rin, uu = u * u, ll = l * l;
+      hh = a.c1 * uu * u * (10.0 * ll - 5.0 * u * l + uu);
+    } else {
+      hh = 1.0;
+    }
+  }
+  return hh;
+}
+
+// StdPairPotential of one pair: site coordinates of i (xi..) and j (xj..),
+// the image shift o subtracted from every site difference, times S.
+__device__ inline double std_energy(const StdTable &st, const double *xi, const double *yi,
+                                   const double *zi, const double *xj, const double *yj,
+                                   const double *zj, double ox, double oy, double oz, double er) {
+  double ep = 0.0;
+  for (int u = 0; u < st.nt; u++) {
+    const StdTerm s = st.t[u];
+    double dx = xi[s.a] - xj[s.b], dy = yi[s.a] - yj[s.b], dz = zi[s.a] - zj[s.b];
+    dx = dx - ox;
+    dy = dy - oy;
+    dz = dz - oz;
+    const double dd = 1.0 / (dx * dx + dy * dy + dz * dz);
+    double e0;
+    if (s.kind == 1) {
+      e0 = s.qq * sqrt(dd);
+      const double d6 = dd * dd * dd, d12 = d6 * d6;
+      e0 = e0 + s.aa * d12 + s.bb * d6;
+    } else if (s.kind == 2) {
+      e0 = s.qq * sqrt(dd);
+    } else {
+      const double d6 = dd * dd * dd, d12 = d6 * d6;
+      e0 = s.aa * d12 + s.bb * d6;
+    }
+    ep = ep + e0;
+  }
+  return ep * er;
+}
+
+// std_energy with the term list of model M: j's sites from memory, the
+// other molecule's sites in registers (constant indices)
+template <int M>
+__device__ inline double std_energy_t(const StdTable &st, const double *xj, const double *yj,
+                                     const double *zj, const double (&xt)[kMaxSite],
+                                     const double (&yt)[kMaxSite], const double (&zt)[kMaxSite],
+                                     double ox, double oy, double oz, double er) {
+  using P = StdPattern<M>;
+  double ep = 0.0;
+#pragma unroll
+  for (int u = 0; u < P::NT; u++) {
+    const StdTerm s = st.t[u];
+    double dx = xj[P::a(u)] - xt[P::b(u)], dy = yj[P::a(u)] - yt[P::b(u)],
+           dz = zj[P::a(u)] - zt[P::b(u)];
+    dx = dx - ox;
+    dy = dy - oy;
+    dz = dz - oz;
+    const double dd = 1.0 / (dx * dx + dy * dy + dz * dz);
+    double e0;
+    if (u == 0) {  // the O-O term: LJ only in both models
+      const double d6 = dd * dd * dd, d12 = d6 * d6;
+      e0 = s.aa * d12 + s.bb * d6;
+    } else {
+      e0 = s.qq * sqrt(dd);
+    }
+    ep = ep + e0;
+  }
+  return ep * er;
+}
+
+// rigid_setmatrix (RigidModule.F90:62-83) and the site positions com + T b
+// (rigid_setsiteposition0, RigidModule.F90:88-144)
+__device__ inline void place_sites(const McArgs &a, double qa, double qb, double qc, double qd,
+                                   double cx, double cy, double cz, double *x, double *y,
+                                   double *z) {
+  const double t0 = (qa * qa + qb * qb - (qc * qc + qd * qd));
+  const double t1 = -2.0 * (qa * qd + qb * qc);
+  const double t2 = 2.0 * (qb * qd - qa * qc);
+  const double t3 = 2.0 * (qa * qd - qb * qc);
+  const double t4 = qa * qa + qc * qc - (qb * qb + qd * qd);
+  const double t5 = -2.0 * (qa * qb + qc * qd);
+  const double t6 = 2.0 * (qa * qc + qb * qd);
+  const double t7 = 2.0 * (qa * qb - qc * qd);
+  const double t8 = qa * qa + qd * qd - (qb * qb + qc * qc);
+  for (int s = 0; s < a.ns; s++) {
+    const double r0 = t0 * a.bx[s] + t1 * a.by[s] + t2 * a.bz[s];
+    const double r1 = t3 * a.bx[s] + t4 * a.by[s] + t5 * a.bz[s];
+    const double r2 = t6 * a.bx[s] + t7 * a.by[s] + t8 * a.bz[s];
+    x[s] = cx + r0;
+    y[s] = cy + r1;
+    z[s] = cz + r2;
+  }
+}
+
+// place_sites with a constant site count (the trial kernel's registers)
+template <int NS>
+__device__ inline void place_sites_t(const McArgs &a, double qa, double qb, double qc, double qd,
+                                     double cx, double cy, double cz, double (&x)[kMaxSite],
+                                     double (&y)[kMaxSite], double (&z)[kMaxSite]) {
+  const double t0 = (qa * qa + qb * qb - (qc * qc + qd * qd));
+  const double t1 = -2.0 * (qa * qd + qb * qc);
+  const double t2 = 2.0 * (qb * qd - qa * qc);
+  const double t3 = 2.0 * (qa * qd - qb * qc);
+  const double t4 = qa * qa + qc * qc - (qb * qb + qd * qd);
+  const double t5 = -2.0 * (qa * qb + qc * qd);
+  const double t6 = 2.0 * (qa * qc + qb * qd);
+  const double t7 = 2.0 * (qa * qb - qc * qd);
+  const double t8 = qa * qa + qd * qd - (qb * qb + qc * qc);
+#pragma unroll
+  for (int s = 0; s < NS; s++) {
+    const double r0 = t0 * a.bx[s] + t1 * a.by[s] + t2 * a.bz[s];
+    const double r1 = t3 * a.bx[s] + t4 * a.by[s] + t5 * a.bz[s];
+    const double r2 = t6 * a.bx[s] + t7 * a.by[s] + t8 * a.bz[s];
+    x[s] = cx + r0;
+    y[s] = cy + r1;
+    z[s] = cz + r2;
+  }
+}
+
+__global__ void k_mc_sites(McArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int n = a.n;
+  place_sites(a, a.b.quat[i], a.b.quat[n + i], a.b.quat[2 * n + i], a.b.quat[3 * n + i],
+              a.b.com[i], a.b.com[n + i], a.b.com[2 * n + i], a.b.sx + (size_t)i * a.ns,
+              a.b.sy + (size_t)i * a.ns, a.b.sz + (size_t)i * a.ns);
+}
+
+// PairInfo: one wave per molecule i computes pinfo(i,j) for j > i in the
+// orientation of the all-pairs list (i < j, d = com_i - com_j), the
+// compression test dd < rc^2 with the dnint shift, S and StdPairPotential,
+// and writes both (i,j) and (j,i); pinfo(i,i) = 0.  part[i] = the row sum.
+__global__ __launch_bounds__(64) void k_mc_pairinfo(McArgs a, StdTable st) {
+  const int i = blockIdx.x, lane = threadIdx.x;
+  const int n = a.n, ns = a.ns;
+  const double xi = a.b.com[i], yi = a.b.com[n + i], zi = a.b.com[2 * n + i];
+  const double *sxi = a.b.sx + (size_t)i * ns, *syi = a.b.sy + (size_t)i * ns,
+               *szi = a.b.sz + (size_t)i * ns;
+  double part = 0.0;
+  for (int j = lane; j < n; j += 64) {
+    double e = 0.0;
+    if (j > i) {
+      double d0 = xi - a.b.com[j], d1 = yi - a.b.com[n + j], d2 = zi - a.b.com[2 * n + j];
+      const double k0 = round(d0 * a.invL) * a.L, k1 = round(d1 * a.invL) * a.L,
+                   k2 = round(d2 * a.invL) * a.L;
+      d0 = d0 - k0;
+      d1 = d1 - k1;
+      d2 = d2 - k2;
+      const double dd = d0 * d0 + d1 * d1 + d2 * d2;
+      if (dd < a.rc * a.rc) {
+        const double er = mc_switch(a, dd);
+        e = std_energy(st, sxi, syi, szi, a.b.sx + (size_t)j * ns, a.b.sy + (size_t)j * ns,
+                       a.b.sz + (size_t)j * ns, k0, k1, k2, er);
+      }
+      a.b.pinfo[(size_t)i * n + j] = e;
+      a.b.pinfo[(size_t)j * n + i] = e;
+      part = part + e;
+    } else if (j == i) {
+      a.b.pinfo[(size_t)i * n + i] = 0.0;
+    }
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) part += __shfl_xor(part, d, 64);
+  if (lane == 0) a.b.part[i] = part;
+}
+
+// mc2's trial loop, ntrial trials, one block.
+template <int M>
+__global__ __launch_bounds__(kMcThreads) void k_mc_trials(McArgs a, StdTable st, int ntrial) {
+  __shared__ double s_red[kMcThreads / 64];
+  const int n = a.n, ns = a.ns;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const double *u = a.b.u, *zf = a.b.z;
+  // the COMs of this thread's molecules stay in registers between trials
+  double cx[kMcMaxPer], cy[kMcMaxPer], cz[kMcMaxPer];
+#pragma unroll
+  for (int m = 0; m < kMcMaxPer; m++) {
+    const int j = tid + m * kMcThreads;
+    cx[m] = j < n ? a.b.com[j] : 0.0;
+    cy[m] = j < n ? a.b.com[n + j] : 0.0;
+    cz[m] = j < n ? a.b.com[2 * n + j] : 0.0;
+  }
+  int pos = 0;
+  long long nacc = 0;
+  double epsum = *a.b.epsum;
+  for (int trial = 0; trial < ntrial; trial++) {
+    // target = random_getnext*nmol + 1 (1-based); u = 1 exactly would name
+    // molecule n+1, which the reference does not guard: clamped here
+    int target = (int)(u[pos] * (double)n + 1.0) - 1;
+    if (target >= n) target = n - 1;
+    const double dt = a.b.delta[target];
+    const double dr = a.b.delta[target] * a.b.rtr[target];
+    const double rotx = zf[pos + 1] * sqrt(dr * dr) + 0.0;
+    const double roty = zf[pos + 3] * sqrt(dr * dr) + 0.0;
+    const double rotz = zf[pos + 5] * sqrt(dr * dr) + 0.0;
+    const double avgrot = (fabs(rotx) + fabs(roty) + fabs(rotz)) / 3.0;
+    // qinfrotator (QuatModule.F90:203-211) + qnormalize (215-223)
+    double qa = 1.0, qb = rotx, qc = roty, qd = rotz;
+    {
+      const double r = 1.0 / sqrt(qa * qa + qb * qb + qc * qc + qd * qd);
+      qa = qa * r;
+      qb = qb * r;
+      qc = qc * r;
+      qd = qd * r;
+    }
+    // qadd (QuatModule.F90:108-133): new = rotator x old, a >= 0
+    {
+      const double a2 = a.b.quat[target], b2 = a.b.quat[n + target],
+                   c2 = a.b.quat[2 * n + target], d2 = a.b.quat[3 * n + target];
+      const double a3 = qa * a2 - qb * b2 - qc * c2 - qd * d2;
+      const double b3 = qa * b2 + qb * a2 + qc * d2 - qd * c2;
+      const double c3 = qa * c2 + qc * a2 - qb * d2 + qd * b2;
+      const double d3 = qa * d2 + qd * a2 + qb * c2 - qc * b2;
+      qa = a3;
+      qb = b3;
+      qc = c3;
+      qd = d3;
+      if (qa < 0) {
+        qa = -qa;
+        qb = -qb;
+        qc = -qc;
+        qd = -qd;
+      }
+    }
+    const double deltax = zf[pos + 7] * sqrt(dt * dt) + 0.0;
+    const double deltay = zf[pos + 9] * sqrt(dt * dt) + 0.0;
+    const double deltaz = zf[pos + 11] * sqrt(dt * dt) + 0.0;
+    const double avgtrans = (fabs(deltax) + fabs(deltay) + fabs(deltaz)) / 3.0;
+    pos += 13;
+    const double tx = deltax + a.b.com[target], ty = deltay + a.b.com[n + target],
+                 tz = deltaz + a.b.com[2 * n + target];
+    double sxt[kMaxSite], syt[kMaxSite], szt[kMaxSite];
+    place_sites_t<StdPattern<M>::NS>(a, qa, qb, qc, qd, tx, ty, tz, sxt, syt, szt);
+    // the compressed list of the moved target (radius rc + dt) and dE
+    const double rr = (a.rc + dt) * (a.rc + dt);
+    double enew[kMcMaxPer], local = 0.0;
+#pragma unroll
+    for (int m = 0; m < kMcMaxPer; m++) {
+      const int j = tid + m * kMcThreads;
+      enew[m] = 0.0;
+      if (j < n) {
+        if (j != target) {
+          double d0 = cx[m] - tx, d1 = cy[m] - ty, d2 = cz[m] - tz;
+          const double k0 = round(d0 * a.invL) * a.L, k1 = round(d1 * a.invL) * a.L,
+                       k2 = round(d2 * a.invL) * a.L;
+          d0 = d0 - k0;
+          d1 = d1 - k1;
+          d2 = d2 - k2;
+          const double dd = d0 * d0 + d1 * d1 + d2 * d2;
+          if (dd < rr) {
+            const double er = mc_switch(a, dd);
+            enew[m] = std_energy_t<M>(st, a.b.sx + (size_t)j * ns, a.b.sy + (size_t)j * ns,
+                                      a.b.sz + (size_t)j * ns, sxt, syt, szt, k0, k1, k2, er);
+          }
+        }
+        local = local + (enew[m] - a.b.pinfo[(size_t)target * n + j]);
+      }
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) local += __shfl_xor(local, d, 64);
+    if (lane == 0) s_red[wid] = local;
+    __syncthreads();
+    double de = 0.0;
+#pragma unroll
+    for (int w = 0; w < kMcThreads / 64; w++) de = de + s_red[w];
+    bool accept;
+    if (de < 0.0) {
+      accept = true;
+    } else {
+      const double ratio = exp(-(de * kI2J * kJ2K) * a.beta);
+      accept = u[pos] < ratio;
+      pos += 1;
+    }
+    if (accept) {
+      nacc += 1;
+      epsum = epsum + de;
+#pragma unroll
+      for (int m = 0; m < kMcMaxPer; m++) {
+        const int j = tid + m * kMcThreads;
+        if (j < n) {
+          a.b.pinfo[(size_t)j * n + target] = enew[m];
+          a.b.pinfo[(size_t)target * n + j] = enew[m];
+          if (j == target) {
+            cx[m] = tx;
+            cy[m] = ty;
+            cz[m] = tz;
+          }
+        }
+      }
+      if (tid == 0) {
+        a.b.com[target] = tx;
+        a.b.com[n + target] = ty;
+        a.b.com[2 * n + target] = tz;
+        a.b.quat[target] = qa;
+        a.b.quat[n + target] = qb;
+        a.b.quat[2 * n + target] = qc;
+        a.b.quat[3 * n + target] = qd;
+#pragma unroll
+        for (int s = 0; s < StdPattern<M>::NS; s++) {
+          a.b.sx[(size_t)target * ns + s] = sxt[s];
+          a.b.sy[(size_t)target * ns + s] = syt[s];
+          a.b.sz[(size_t)target * ns + s] = szt[s];
+        }
+        a.b.delta[target] = a.b.delta[target] * kIncrement;
+        if (avgtrans * a.b.rtr[target] < avgrot)
+          a.b.rtr[target] = a.b.rtr[target] * kIncrement;
+        else
+          a.b.rtr[target] = a.b.rtr[target] / kIncrement;
+      }
+    } else if (tid == 0) {
+      a.b.delta[target] = a.b.delta[target] / kIncrement;
+      if (a.b.delta[target] < kDeltaMin) a.b.delta[target] = kDeltaMin;
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    *a.b.pos = pos;
+    *a.b.nacc = *a.b.nacc + nacc;
+    *a.b.epsum = epsum;
+  }
+}
+
+static McArgs mc_args(const System &s) {
+  McArgs a{};
+  a.n = s.n;
+  a.ns = s.ns;
+  a.L = s.L;
+  a.invL = 1.0 / s.L;
+  a.rc = s.rc;
+  a.rin = s.rc - s.taper;
+  a.c1 = s.taper > 0 ? s.geo.c1 : 0.0;
+  a.beta = s.mc_beta;
+  for (int k = 0; k < kMaxSite; k++) {
+    a.bx[k] = s.mt.bx[k];
+    a.by[k] = s.mt.by[k];
+    a.bz[k] = s.mt.bz[k];
+  }
+  a.b = s.mc;
+  return a;
+}
+
+void mc_free(System &s) {
+  McBuf &m = s.mc;
+  void *ps[] = {m.com, m.quat, m.sx, m.sy, m.sz, m.pinfo, m.delta, m.rtr, m.u, m.z,
+                m.epsum, m.part, m.pos, m.nacc};
+  for (void *p : ps)
+    if (p) (void)hipFree(p);
+  m = McBuf{};
+  s.mc_ready = false;
+}
+
+void McHost::seed(uint32_t s) {
+  mt[0] = s;
+  for (int i = 1; i < 624; i++) mt[i] = 1812433253u * (mt[i - 1] ^ (mt[i - 1] >> 30)) + (uint32_t)i;
+  mti = 624;
+}
+
+uint32_t McHost::next() {
+  if (mti >= 624) {
+    for (int k = 0; k < 624; k++) {
+      const uint32_t y = (mt[k] & 0x80000000u) | (mt[(k + 1) % 624] & 0x7fffffffu);
+      mt[k] = mt[(k + 397) % 624] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+    }
+    mti = 0;
+  }
+  uint32_t y = mt[mti++];
+  y ^= y >> 11;
+  y ^= (y << 7) & 0x9d2c5680u;
+  y ^= (y << 15) & 0xefc60000u;
+  y ^= y >> 18;
+  return y;
+}
+
+double McHost::getnext() { return 1.0 - (double)(next() >> 1) * (1.0 / 2147483648.0); }
+
+}  // namespace g6
+
+using namespace g6;
+
+extern "C" {
+
+int g6_mc_begin(g6_handle h, double temp, uint32_t seed, const double *delta,
+                const double *rtratio) {
+  System *s = system_of(h);
+  if (!s || !(temp > 0) || !delta || !rtratio) return G6_EARG;
+  if (s->model == G6_ST2) return G6_EARG;  // ST2's energy path is its own kernel
+  const int n = s->n, ns = s->ns;
+  if (n > kMcThreads * kMcMaxPer) return G6_EARG;
+  mc_free(*s);
+  McBuf &m = s->mc;
+  auto A = [](void **p, size_t b) { return hipMalloc(p, b > 0 ? b : 8); };
+#define AL(ptr, bytes) G6_CHECK(A((void **)&(ptr), (bytes)))
+  AL(m.com, sizeof(double) * 3 * n);
+  AL(m.quat, sizeof(double) * 4 * n);
+  AL(m.sx, sizeof(double) * (size_t)n * ns);
+  AL(m.sy, sizeof(double) * (size_t)n * ns);
+  AL(m.sz, sizeof(double) * (size_t)n * ns);
+  AL(m.pinfo, sizeof(double) * (size_t)n * n);
+  AL(m.delta, sizeof(double) * n);
+  AL(m.rtr, sizeof(double) * n);
+  AL(m.epsum, sizeof(double));
+  AL(m.part, sizeof(double) * n);
+  AL(m.pos, sizeof(int));
+  AL(m.nacc, sizeof(long long));
+#undef AL
+  // the configuration of the MD state: COM and the order-1 quaternion
+  G6_CHECK(hipMemcpy(m.com, s->md.com, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice));
+  G6_CHECK(hipMemcpy(m.quat, s->md.quat, sizeof(double) * 4 * n, hipMemcpyDeviceToDevice));
+  G6_CHECK(hipMemcpy(m.delta, delta, sizeof(double) * n, hipMemcpyHostToDevice));
+  G6_CHECK(hipMemcpy(m.rtr, rtratio, sizeof(double) * n, hipMemcpyHostToDevice));
+  G6_CHECK(hipMemset(m.nacc, 0, sizeof(long long)));
+  make_std_table(s->mt, &s->std);
+  if (s->model == G6_TIP4P ? !std_table_matches<G6_TIP4P>(s->std)
+                           : !std_table_matches<G6_TIP5P>(s->std))
+    return G6_EARG;
+  s->mc_beta = 1.0 / temp;  // @TEMP: beta = 1/temp (mc2util.F90)
+  McHost *hst = &s->mch;
+  hst->seed(seed);
+  hst->pend.clear();
+  s->mc_ready = true;
+  McArgs a = mc_args(*s);
+  k_mc_sites<<<(n + 255) / 256, 256, 0, s->stream>>>(a);
+  k_mc_pairinfo<<<n, 64, 0, s->stream>>>(a, s->std);
+  G6_CHECK(hipGetLastError());
+  std::vector<double> part(n);
+  G6_CHECK(hipMemcpyAsync(part.data(), m.part, sizeof(double) * n, hipMemcpyDeviceToHost,
+                          s->stream));
+  G6_CHECK(hipStreamSynchronize(s->stream));
+  double e = 0.0;
+  for (int i = 0; i < n; i++) e = e + part[i];
+  G6_CHECK(hipMemcpy(m.epsum, &e, sizeof(double), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int g6_mc_trials(g6_handle h, int ntrial, long long *naccept) {
+  System *s = system_of(h);
+  if (!s || !s->mc_ready || ntrial < 0) return G6_EARG;
+  if (ntrial == 0) return 0;
+  McBuf &m = s->mc;
+  McHost *hst = &s->mch;
+  // a trial consumes 13 or 14 numbers; the factor at k also reads k + 1
+  const size_t need = (size_t)ntrial * 14 + 2;
+  while (hst->pend.size() < need) hst->pend.push_back(hst->getnext());
+  if ((int)need > m.ucap) {
+    if (m.u) (void)hipFree(m.u);
+    if (m.z) (void)hipFree(m.z);
+    m.u = m.z = nullptr;
+    G6_CHECK(hipMalloc(&m.u, sizeof(double) * need));
+    G6_CHECK(hipMalloc(&m.z, sizeof(double) * need));
+    m.ucap = (int)need;
+  }
+  std::vector<double> z(need, 0.0);
+  const double pi = 3.14159265358979323846;  // PhysConst.F90 PI
+  for (size_t k = 0; k + 1 < need; k++) {
+    const double r = sqrt(-2.0 * log(hst->pend[k]));
+    const double theta = 2.0 * pi * hst->pend[k + 1];
+    z[k] = r * sin(theta);
+  }
+  G6_CHECK(hipMemcpy(m.u, hst->pend.data(), sizeof(double) * need, hipMemcpyHostToDevice));
+  G6_CHECK(hipMemcpy(m.z, z.data(), sizeof(double) * need, hipMemcpyHostToDevice));
+  McArgs a = mc_args(*s);
+  if (s->model == G6_TIP4P)
+    k_mc_trials<G6_TIP4P><<<1, kMcThreads, 0, s->stream>>>(a, s->std, ntrial);
+  else
+    k_mc_trials<G6_TIP5P><<<1, kMcThreads, 0, s->stream>>>(a, s->std, ntrial);
+  G6_CHECK(hipGetLastError());
+  int used = 0;
+  long long nacc = 0;
+  G6_CHECK(hipMemcpyAsync(&used, m.pos, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+  G6_CHECK(hipMemcpyAsync(&nacc, m.nacc, sizeof(long long), hipMemcpyDeviceToHost, s->stream));
+  G6_CHECK(hipStreamSynchronize(s->stream));
+  if (used < 0 || (size_t)used > need) return G6_EHIP;
+  hst->pend.erase(hst->pend.begin(), hst->pend.begin() + used);
+  if (naccept) *naccept = nacc;
+  return 0;
+}
+
+int g6_mc_get(g6_handle h, double *com, double *quat1, double *delta, double *rtratio,
+              double *epsum) {
+  System *s = system_of(h);
+  if (!s || !s->mc_ready) return G6_EARG;
+  const int n = s->n;
+  G6_CHECK(hipStreamSynchronize(s->stream));
+  if (com) G6_CHECK(hipMemcpy(com, s->mc.com, sizeof(double) * 3 * n, hipMemcpyDeviceToHost));
+  if (quat1) G6_CHECK(hipMemcpy(quat1, s->mc.quat, sizeof(double) * 4 * n, hipMemcpyDeviceToHost));
+  if (delta) G6_CHECK(hipMemcpy(delta, s->mc.delta, sizeof(double) * n, hipMemcpyDeviceToHost));
+  if (rtratio) G6_CHECK(hipMemcpy(rtratio, s->mc.rtr, sizeof(double) * n, hipMemcpyDeviceToHost));
+  if (epsum) G6_CHECK(hipMemcpy(epsum, s->mc.epsum, sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int g6_mc_pairinfo(g6_handle h, double *ep, double *epsum) {
+  System *s = system_of(h);
+  if (!s || !s->mc_ready || !ep) return G6_EARG;
+  const int n = s->n;
+  G6_CHECK(hipStreamSynchronize(s->stream));
+  G6_CHECK(hipMemcpy(ep, s->mc.pinfo, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToHost));
+  if (epsum) G6_CHECK(hipMemcpy(epsum, s->mc.epsum, sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+// the total energy of the current MC configuration recomputed from scratch
+// (a fresh PairInfo), without touching the bookkeeping
+int g6_mc_energy(g6_handle h, double *epsum) {
+  System *s = system_of(h);
+  if (!s || !s->mc_ready || !epsum) return G6_EARG;
+  const int n = s->n;
+  double *keep = nullptr;
+  G6_CHECK(hipMalloc(&keep, sizeof(double) * (size_t)n * n));
+  G6_CHECK(hipMemcpyAsync(keep, s->mc.pinfo, sizeof(double) * (size_t)n * n,
+                          hipMemcpyDeviceToDevice, s->stream));
+  McArgs a = mc_args(*s);
+  k_mc_pairinfo<<<n, 64, 0, s->stream>>>(a, s->std);
+  G6_CHECK(hipGetLastError());
+  std::vector<double> part(n);
+  G6_CHECK(hipMemcpyAsync(part.data(), s->mc.part, sizeof(double) * n, hipMemcpyDeviceToHost,
+                          s->stream));
+  G6_CHECK(hipMemcpyAsync(s->mc.pinfo, keep, sizeof(double) * (size_t)n * n,
+                          hipMemcpyDeviceToDevice, s->stream));
+  G6_CHECK(hipStreamSynchronize(s->stream));
+  G6_CHECK(hipFree(keep));
+  double e = 0.0;
+  for (int i = 0; i < n; i++) e = e + part[i];
+  *epsum = e;
+  return 0;
+}
+
+}  // extern "C"

@@ -155,6 +155,39 @@ module g6gpu
        real(c_double), intent(out) :: t(5)
      end function g6_step_timed
 
+     integer(c_int) function g6_mc_begin(h, temp, seed, delta, rtratio) bind(C, name="g6_mc_begin")
+       import :: c_int, c_int64_t, c_int32_t, c_double
+       integer(c_int64_t), value :: h
+       real(c_double), value :: temp
+       integer(c_int32_t), value :: seed
+       real(c_double), intent(in) :: delta(*), rtratio(*)
+     end function g6_mc_begin
+
+     integer(c_int) function g6_mc_trials(h, ntrial, naccept) bind(C, name="g6_mc_trials")
+       import :: c_int, c_int64_t, c_long_long
+       integer(c_int64_t), value :: h
+       integer(c_int), value :: ntrial
+       integer(c_long_long), intent(out) :: naccept
+     end function g6_mc_trials
+
+     integer(c_int) function g6_mc_get(h, com, quat1, delta, rtratio, epsum) bind(C, name="g6_mc_get")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), value :: h
+       real(c_double), intent(out) :: com(*), quat1(*), delta(*), rtratio(*), epsum
+     end function g6_mc_get
+
+     integer(c_int) function g6_mc_pairinfo(h, ep, epsum) bind(C, name="g6_mc_pairinfo")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), value :: h
+       real(c_double), intent(out) :: ep(*), epsum
+     end function g6_mc_pairinfo
+
+     integer(c_int) function g6_mc_energy(h, epsum) bind(C, name="g6_mc_energy")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), value :: h
+       real(c_double), intent(out) :: epsum
+     end function g6_mc_energy
+
      integer(c_int) function g6_model_nsite(model) bind(C, name="g6_model_nsite")
        import :: c_int
        integer(c_int), value :: model

@@ -58,6 +58,11 @@ def lib():
         L.g6_collect.argtypes = [_H, _dp, _dp, _dp, _dp, _dp, _dp]
         L.g6_lattice_start.argtypes = [C.c_int, C.c_double, C.c_uint32, _dp, _dp]
         L.g6_model_nsite.argtypes = [C.c_int]
+        L.g6_mc_begin.argtypes = [_H, C.c_double, C.c_uint32, _dp, _dp]
+        L.g6_mc_trials.argtypes = [_H, C.c_int, C.POINTER(C.c_longlong)]
+        L.g6_mc_get.argtypes = [_H, _dp, _dp, _dp, _dp, _dp]
+        L.g6_mc_pairinfo.argtypes = [_H, _dp, _dp]
+        L.g6_mc_energy.argtypes = [_H, _dp]
         _lib = L
     return _lib
 
@@ -142,6 +147,36 @@ class System:
         t = np.zeros(5)
         check(lib().g6_step_timed(self.h, nsteps, t), "g6_step_timed")
         return t
+
+    # ---- Monte Carlo (mc2 trial loop, PairInfo) ----
+    def mc_begin(self, temp, seed, delta, rtratio):
+        n = self.n
+        d = np.ascontiguousarray(np.broadcast_to(np.asarray(delta, float), (n,)))
+        r = np.ascontiguousarray(np.broadcast_to(np.asarray(rtratio, float), (n,)))
+        check(lib().g6_mc_begin(self.h, temp, seed, d, r), "g6_mc_begin")
+
+    def mc_trials(self, ntrial):
+        acc = C.c_longlong(0)
+        check(lib().g6_mc_trials(self.h, ntrial, C.byref(acc)), "g6_mc_trials")
+        return acc.value
+
+    def mc_state(self):
+        n = self.n
+        com = np.zeros(3 * n); q = np.zeros(4 * n); d = np.zeros(n); r = np.zeros(n)
+        e = np.zeros(1)
+        check(lib().g6_mc_get(self.h, com, q, d, r, e), "g6_mc_get")
+        return com.reshape(3, n).T.copy(), q.reshape(4, n).T.copy(), d, r, float(e[0])
+
+    def mc_pairinfo(self):
+        n = self.n
+        ep = np.zeros(n * n); e = np.zeros(1)
+        check(lib().g6_mc_pairinfo(self.h, ep, e), "g6_mc_pairinfo")
+        return ep.reshape(n, n), float(e[0])
+
+    def mc_energy(self):
+        e = np.zeros(1)
+        check(lib().g6_mc_energy(self.h, e), "g6_mc_energy")
+        return float(e[0])
 
     def eval_forces(self):
         """Molecule forces (n,3), body torques (n,3) and the thermo record of

@@ -131,6 +131,32 @@ int g6_collect(g6_handle h, const double *quat1, const double *fx,
                const double *fy, const double *fz, double *force,
                double *torque);
 
+/* ---- Monte Carlo: PairInfo and the mc2 trial loop ------------------- */
+/* Starts an MC run on the handle's current configuration (COMs and order-1
+ * quaternions of the last g6_upload_state): site positions, the PairInfo
+ * matrix of every pair within rc (pairenergy, PairInfo.F90:12-46, built as
+ * mc2.F90:303-340 builds it, StdPairPotential energies times S), its sum,
+ * per-molecule step sizes delta[n] and rotation ratios rtratio[n] (@DELT),
+ * beta = 1/temp (@TEMP) and the random stream seeded with seed (@SEED).
+ * TIP4P and TIP5P (the StdPairPotential models); n <= 2048. */
+int g6_mc_begin(g6_handle h, double temp, uint32_t seed, const double *delta,
+                const double *rtratio);
+/* ntrial single-molecule trials of mc2 (mc2.F90:540-862): target from the
+ * stream, Gaussian rotation (qinfrotator + qadd) and displacement with
+ * widths delta*rtratio and delta, dE of the target against every molecule
+ * within rc + delta, Metropolis at beta, PairInfo and step-size updates.
+ * *naccept = accepted trials since g6_mc_begin. */
+int g6_mc_trials(g6_handle h, int ntrial, long long *naccept);
+/* The MC configuration: com[3n], quat1[4n], delta[n], rtratio[n] and the
+ * running energy sum (any pointer may be NULL). */
+int g6_mc_get(g6_handle h, double *com, double *quat1, double *delta,
+              double *rtratio, double *epsum);
+/* The PairInfo matrix ep[n*n] (ep[i*n+j], symmetric) and the running sum. */
+int g6_mc_pairinfo(g6_handle h, double *ep, double *epsum);
+/* The total energy of the current MC configuration from a fresh PairInfo
+ * (the bookkeeping is left as it is). */
+int g6_mc_energy(g6_handle h, double *epsum);
+
 /* ---- synthetic start (no reference counterpart: the reference reads
  * @NX4A files) --------------------------------------------------------- */
 /* Simple-cubic lattice, m = ceil(n^(1/3)) per edge, occupied sites chosen by

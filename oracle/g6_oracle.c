@@ -921,3 +921,189 @@ void g6o_vreset(int model, int n, double dt, double temp, uint32_t seed,
     }
   }
 }
+
+/* ---- Monte Carlo: PairInfo and the mc2 trial loop ----------------------- */
+/* StdPairPotential of one pair from site arrays (StdInteraction.F90:633-762):
+ * site a of molecule i (xi..) with site b of molecule j, shift o, times er */
+static double std_pair(const stdterm *t, int nt, int ns, const double *x, const double *y,
+                       const double *z, int i, const double *xt, const double *yt,
+                       const double *zt, double ox, double oy, double oz, double er) {
+  double ep = 0.0;
+  for (int u = 0; u < nt; u++) {
+    const stdterm s = t[u];
+    double dx = x[i * ns + s.a] - xt[s.b], dy = y[i * ns + s.a] - yt[s.b],
+           dz = z[i * ns + s.a] - zt[s.b];
+    dx = dx - ox; dy = dy - oy; dz = dz - oz;
+    double dd = 1.0 / (dx * dx + dy * dy + dz * dz), e0;
+    if (s.kind == 1) {
+      double ri = sqrt(dd);
+      e0 = s.qq * ri;
+      double d6 = dd * dd * dd, d12 = d6 * d6;
+      e0 = e0 + s.aa * d12 + s.bb * d6;
+    } else if (s.kind == 2) {
+      e0 = s.qq * sqrt(dd);
+    } else {
+      double d6 = dd * dd * dd, d12 = d6 * d6;
+      e0 = s.aa * d12 + s.bb * d6;
+    }
+    ep = ep + e0;
+  }
+  return ep * er;
+}
+
+/* PairInfo as mc2 builds it (mc2.F90:303-340): interaction_alltoall
+ * (InteractionModule.F90:157-196), Interaction_Compress with rc, CutOff_Dimmer2
+ * and pairenergy (PairInfo.F90:12-46).  com[n][3]; quat[4][n] (order 1);
+ * ep[n*n] symmetric, zero off the list; returns epsum in list order. */
+double g6o_mc_pairinfo(int model, int n, double L, double rc, double taper,
+                       const double *com, const double *quat, double *ep) {
+  g6o_model m;
+  g6o_model_get(model, &m);
+  const int ns = m.nsite;
+  double *x = malloc(sizeof(double) * n * ns), *y = malloc(sizeof(double) * n * ns),
+         *z = malloc(sizeof(double) * n * ns), *t = malloc(sizeof(double) * 9 * n),
+         *intra = malloc(sizeof(double) * 3 * ns * n);
+  g6o_setsites(model, n, com, quat, x, y, z, t, intra);
+  stdterm st[G6O_MAXSITE * G6O_MAXSITE];
+  const int nt = std_terms(&m, st);
+  for (long k = 0; k < (long)n * n; k++) ep[k] = 0.0;
+  double epsum = 0.0;
+  for (int i = 0; i < n; i++)
+    for (int j = i + 1; j < n; j++) {
+      double d[3], o[3];
+      for (int a = 0; a < 3; a++) {
+        d[a] = com[3 * i + a] - com[3 * j + a];
+        o[a] = round(d[a] * (1.0 / L)) * L;
+        d[a] = d[a] - o[a];
+      }
+      const double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      if (!(dd < rc * rc)) continue;
+      double hh, dh;
+      g6o_cutoff_func(rc - taper, rc, dd, &hh, &dh);
+      const double e = std_pair(st, nt, ns, x, y, z, i, x + j * ns, y + j * ns, z + j * ns,
+                                o[0], o[1], o[2], hh);
+      ep[(long)i * n + j] = e;
+      ep[(long)j * n + i] = e;
+      epsum = epsum + e;
+    }
+  free(x); free(y); free(z); free(t); free(intra);
+  return epsum;
+}
+
+/* The mc2 single-molecule trial loop (mc2.F90:540-862, DEBUG20050322a) on
+ * the Random_GetNext stream u starting at *pos; the two draws of each
+ * random_normal are taken in argument order.  com[n][3], quat[4][n],
+ * delta[n], rtr[n], ep[n*n] (PairInfo), *epsum are updated in place;
+ * returns the number of accepted trials. */
+long g6o_mc_trials(int model, int n, double L, double rc, double taper, double temp,
+                   double *com, double *quat, double *delta, double *rtr, double *ep,
+                   double *epsum, const double *u, long *pos, int ntrial) {
+  const double INCREMENT = (double)1.05f, I2J_ = 10.0, J2K_ = 0.120273, beta = 1.0 / temp;
+  g6o_model m;
+  g6o_model_get(model, &m);
+  const int ns = m.nsite;
+  double *x = malloc(sizeof(double) * n * ns), *y = malloc(sizeof(double) * n * ns),
+         *z = malloc(sizeof(double) * n * ns), *t = malloc(sizeof(double) * 9 * n),
+         *intra = malloc(sizeof(double) * 3 * ns * n), *enew = malloc(sizeof(double) * n);
+  char *inl = malloc(n);
+  g6o_setsites(model, n, com, quat, x, y, z, t, intra);
+  stdterm st[G6O_MAXSITE * G6O_MAXSITE];
+  const int nt = std_terms(&m, st);
+  long p = *pos, nacc = 0;
+  for (int trial = 0; trial < ntrial; trial++) {
+    int target = (int)(u[p++] * n + 1) - 1;
+    if (target >= n) target = n - 1;
+    const double dt = delta[target], dr = delta[target] * rtr[target];
+    const double rotx = random_normal(0.0, dr * dr, u[p], u[p + 1]);
+    const double roty = random_normal(0.0, dr * dr, u[p + 2], u[p + 3]);
+    const double rotz = random_normal(0.0, dr * dr, u[p + 4], u[p + 5]);
+    p += 6;
+    const double avgrot = (fabs(rotx) + fabs(roty) + fabs(rotz)) / 3.0;
+    double qa = 1.0, qb = rotx, qc = roty, qd = rotz; /* qinfrotator + qnormalize */
+    {
+      const double r = 1.0 / sqrt(qa * qa + qb * qb + qc * qc + qd * qd);
+      qa = qa * r; qb = qb * r; qc = qc * r; qd = qd * r;
+    }
+    { /* qadd (QuatModule.F90:108-133) */
+      const double a2 = quat[target], b2 = quat[n + target], c2 = quat[2 * n + target],
+                   d2 = quat[3 * n + target];
+      const double a3 = qa * a2 - qb * b2 - qc * c2 - qd * d2;
+      const double b3 = qa * b2 + qb * a2 + qc * d2 - qd * c2;
+      const double c3 = qa * c2 + qc * a2 - qb * d2 + qd * b2;
+      const double d3 = qa * d2 + qd * a2 + qb * c2 - qc * b2;
+      qa = a3; qb = b3; qc = c3; qd = d3;
+      if (qa < 0) { qa = -qa; qb = -qb; qc = -qc; qd = -qd; }
+    }
+    const double deltax = random_normal(0.0, dt * dt, u[p], u[p + 1]);
+    const double dum1 = u[p + 2], dum2 = u[p + 3];
+    const double deltay = random_normal(0.0, dt * dt, dum1, dum2);
+    const double deltaz = random_normal(0.0, dt * dt, u[p + 4], u[p + 5]);
+    p += 6;
+    const double avgtrans = (fabs(deltax) + fabs(deltay) + fabs(deltaz)) / 3.0;
+    const double ct[3] = {deltax + com[3 * target], deltay + com[3 * target + 1],
+                          deltaz + com[3 * target + 2]};
+    double qt[4] = {qa, qb, qc, qd}, xt[G6O_MAXSITE], yt[G6O_MAXSITE], zt[G6O_MAXSITE], tt[9],
+           it[3 * G6O_MAXSITE];
+    g6o_setsites(model, 1, ct, qt, xt, yt, zt, tt, it);
+    /* the compressed list (i, moved target) within rc + dt, ascending i */
+    const double rr = (rc + dt) * (rc + dt);
+    for (int i = 0; i < n; i++) {
+      inl[i] = 0;
+      enew[i] = 0.0;
+      const double *ci = (i == target) ? ct : com + 3 * i;
+      double d[3], o[3];
+      for (int a = 0; a < 3; a++) {
+        d[a] = ci[a] - ct[a];
+        o[a] = round(d[a] * (1.0 / L)) * L;
+        d[a] = d[a] - o[a];
+      }
+      const double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      if (!(dd < rr)) continue;
+      inl[i] = 1;
+      if (i == target) continue;
+      double hh, dh;
+      g6o_cutoff_func(rc - taper, rc, dd, &hh, &dh);
+      enew[i] = std_pair(st, nt, ns, x, y, z, i, xt, yt, zt, o[0], o[1], o[2], hh);
+    }
+    double deltae = 0.0;
+    for (int i = 0; i < n; i++) deltae = deltae - ep[(long)i * n + target];
+    for (int i = 0; i < n; i++)
+      if (inl[i] && i != target) deltae = deltae + enew[i];
+    int accept = 0;
+    if (deltae < 0.0) {
+      accept = 1;
+    } else {
+      const double ratio = exp(-(deltae * I2J_ * J2K_) * beta);
+      if (u[p++] < ratio) accept = 1;
+    }
+    if (accept) {
+      nacc++;
+      for (int a = 0; a < 3; a++) com[3 * target + a] = ct[a];
+      for (int a = 0; a < 4; a++) quat[a * n + target] = qt[a];
+      for (int s = 0; s < ns; s++) {
+        x[target * ns + s] = xt[s]; y[target * ns + s] = yt[s]; z[target * ns + s] = zt[s];
+      }
+      *epsum = *epsum + deltae;
+      for (int i = 0; i < n; i++) {
+        ep[(long)i * n + target] = 0.0;
+        ep[(long)target * n + i] = 0.0;
+      }
+      for (int i = 0; i < n; i++)
+        if (inl[i] && i != target) {
+          ep[(long)i * n + target] = enew[i];
+          ep[(long)target * n + i] = enew[i];
+        }
+      delta[target] = delta[target] * INCREMENT;
+      if (avgtrans * rtr[target] < avgrot)
+        rtr[target] = rtr[target] * INCREMENT;
+      else
+        rtr[target] = rtr[target] / INCREMENT;
+    } else {
+      delta[target] = delta[target] / INCREMENT;
+      if (delta[target] < (double)0.05f) delta[target] = (double)0.05f;
+    }
+  }
+  *pos = p;
+  free(x); free(y); free(z); free(t); free(intra); free(enew); free(inl);
+  return nacc;
+}

@@ -134,6 +134,14 @@ double g6o_random_getnext(void);
 void g6o_vreset(int model, int n, double dt, double temp, uint32_t seed,
                 double *dx, double *w);
 
+/* Monte Carlo (mc2.F90:303-340, 540-862; PairInfo.F90:12-46).
+ * com[n][3], quat[4][n] (order-1 quaternions), ep[n*n] the PairInfo matrix. */
+double g6o_mc_pairinfo(int model, int n, double L, double rc, double taper,
+                       const double *com, const double *quat, double *ep);
+long g6o_mc_trials(int model, int n, double L, double rc, double taper, double temp,
+                   double *com, double *quat, double *delta, double *rtr, double *ep,
+                   double *epsum, const double *u, long *pos, int ntrial);
+
 #ifdef __cplusplus
 }
 #endif

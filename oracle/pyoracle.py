@@ -101,6 +101,13 @@ def _declare(L):
     L.g6o_random_getnext.restype = C.c_double
     L.g6o_vreset.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_uint32,
                              _dp, _dp]
+    L.g6o_mc_pairinfo.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                  _dp, _dp, _dp]
+    L.g6o_mc_pairinfo.restype = C.c_double
+    L.g6o_mc_trials.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                C.c_double, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                C.POINTER(C.c_long), C.c_int]
+    L.g6o_mc_trials.restype = C.c_long
 
 
 def model(mid):
@@ -262,3 +269,55 @@ class MDRun:
 def mt_stream(seed, count):
     lib().g6o_mt_seed(seed)
     return np.array([lib().g6o_mt_next() for _ in range(count)], dtype=np.uint64)
+
+
+def getnext_stream(seed, count):
+    """The Random_GetNext values (RandomModule.F90:123-130) of an MT19937
+    seeded with seed."""
+    lib().g6o_mt_seed(seed)
+    f = lib().g6o_random_getnext
+    return np.array([f() for _ in range(count)], dtype=np.float64)
+
+
+def mc_pairinfo(mid, com, quat, L, rc, taper):
+    """PairInfo matrix (n, n) and its sum; com (n,3), quat (4,n) order 1."""
+    com = np.ascontiguousarray(com, np.float64)
+    n = com.shape[0]
+    ep = np.zeros(n * n)
+    e = lib().g6o_mc_pairinfo(mid, n, L, rc, taper, com, np.ascontiguousarray(quat, np.float64),
+                              ep)
+    return ep.reshape(n, n), e
+
+
+class MCRun:
+    """The mc2 trial loop on the oracle: state (com (n,3), quat (4,n), delta,
+    rtratio, PairInfo, epsum) and the Random_GetNext stream of a seed."""
+
+    def __init__(self, mid, com, quat, L, rc, taper, temp, seed, delta, rtratio):
+        self.mid, self.L, self.rc, self.taper, self.temp = mid, L, rc, taper, temp
+        self.com = np.array(com, np.float64)
+        self.quat = np.array(quat, np.float64)
+        n = self.n = self.com.shape[0]
+        self.delta = np.array(np.broadcast_to(np.asarray(delta, float), (n,)))
+        self.rtr = np.array(np.broadcast_to(np.asarray(rtratio, float), (n,)))
+        self.ep, e = mc_pairinfo(mid, self.com, self.quat, L, rc, taper)
+        self.ep = np.ascontiguousarray(self.ep.ravel())
+        self.epsum = np.array([e])
+        self.seed = seed
+        self.stream = np.zeros(0)
+        self.pos = 0
+        self.nacc = 0
+
+    def _ensure(self, count):
+        if len(self.stream) < count:
+            self.stream = getnext_stream(self.seed, max(count, 2 * len(self.stream)))
+
+    def trials(self, ntrial):
+        self._ensure(self.pos + 14 * ntrial + 2)
+        p = C.c_long(self.pos)
+        acc = lib().g6o_mc_trials(self.mid, self.n, self.L, self.rc, self.taper, self.temp,
+                                  self.com, self.quat, self.delta, self.rtr, self.ep,
+                                  self.epsum, self.stream, C.byref(p), ntrial)
+        self.pos = p.value
+        self.nacc += acc
+        return self.nacc

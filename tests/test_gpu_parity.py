@@ -236,3 +236,41 @@ def test_fortran_driver_reproduces_the_reference_log(golden, name):
         scale = np.abs(ref[:, col]).max() + 1e-300
         err = np.abs(got[:, col] - ref[:, col]).max() / scale
         assert err <= tol, (col, err)
+
+
+@pytest.mark.parametrize("mid", [0, 1])
+def test_mc_pairinfo_matches_oracle(mid):
+    """PairInfo on the GPU equals the oracle's matrix bit for bit."""
+    n, L = 216, 18.626
+    com, q = lattice_start(n, L, 21 + mid)
+    s = g6.System(mid, n, L, 8.5, 0.5, 0.002)
+    s.upload(com, q)
+    s.mc_begin(250.0, 99, 0.3, 1.0)
+    P, e = s.mc_pairinfo()
+    Pr, er = O.mc_pairinfo(mid, com, q.T.copy(), L, 8.5, 0.5)
+    np.testing.assert_array_equal(P, Pr)
+    assert abs(e - er) <= 1e-12 * abs(er)
+
+
+def test_mc_trials_match_oracle():
+    """2000 mc2 trials on the GPU take the oracle's decisions: same accept
+    count, the same final configuration bit for bit, the same PairInfo; the
+    running energy agrees with a fresh recompute."""
+    n, L = 512, 25.088
+    com, q = lattice_start(n, L, 31)
+    s = g6.System(g6.TIP4P, n, L, 9.0, 0.5, 0.002)
+    s.upload(com, q)
+    s.mc_begin(250.0, 4242, 0.3, 1.0)
+    r = O.MCRun(0, com, q.T.copy(), L, 9.0, 0.5, 250.0, 4242, 0.3, 1.0)
+    for chunk in (700, 1300):
+        acc = s.mc_trials(chunk)
+        assert acc == r.trials(chunk), chunk
+    c, qq, d, rt, e = s.mc_state()
+    np.testing.assert_array_equal(c, r.com)
+    np.testing.assert_array_equal(qq.T, r.quat)
+    np.testing.assert_array_equal(d, r.delta)
+    np.testing.assert_array_equal(rt, r.rtr)
+    assert abs(e - r.epsum[0]) <= 1e-10 * abs(e)
+    P, _ = s.mc_pairinfo()
+    np.testing.assert_array_equal(P, r.ep.reshape(n, n))
+    assert abs(s.mc_energy() - e) <= 1e-10 * abs(e)

@@ -171,3 +171,35 @@ def test_lj_coulomb_dimer_closed_form():
     r = np.linalg.norm(body[0] - (body[0] + com[1]))
     e += K[3] / r ** 12 + K[4] / r ** 6
     assert abs(ep[0] - e) < 1e-12 * abs(e)
+
+
+@pytest.mark.parametrize("name,mid", [("tip4p_216", 0), ("tip5p_216", 1)])
+def test_pairinfo_matches_reference_pair_energies(golden, name, mid):
+    """PairInfo (pairenergy over the all-pairs list within rc) holds, for
+    every pair of the reference's own list, the reference's StdForce pair
+    energy; pairs off the list are zero."""
+    g = golden(name)
+    L, rc, taper = float(g["L"]), float(g["rc"]), float(g["taper"])
+    P, e = O.mc_pairinfo(mid, g["com"], g["quat"].T.copy(), L, rc, taper)
+    pi, pj, ref = g["pi"], g["pj"], g["ep_std"]
+    got = P[pi, pj]
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert (P != 0).sum() == 2 * len(pi)
+    assert np.array_equal(P, P.T)
+    assert abs(e - ref.sum()) <= 1e-11 * np.abs(ref).sum()
+
+
+def test_mc_energy_bookkeeping_and_quaternions():
+    """After 1500 mc2 trials the running energy (sum of accepted dE) equals a
+    fresh PairInfo of the final configuration, and every quaternion is still
+    a unit quaternion (qadd of unit quaternions)."""
+    from tests.golden.configs import lattice_start
+    com, q = lattice_start(216, 18.626, 7)
+    r = O.MCRun(0, com, q.T.copy(), 18.626, 8.5, 0.5, 250.0, 4242, 0.3, 1.0)
+    nacc = r.trials(1500)
+    assert 100 < nacc < 1500
+    P, e = O.mc_pairinfo(0, r.com, r.quat, 18.626, 8.5, 0.5)
+    assert abs(r.epsum[0] - e) <= 1e-12 * abs(e)
+    assert np.abs(P - r.ep.reshape(216, 216)).max() <= 1e-9
+    assert np.abs((r.quat ** 2).sum(0) - 1.0).max() < 1e-14
+    assert r.delta.min() >= np.float32(0.05)
