@@ -262,35 +262,62 @@ __global__ __launch_bounds__(64) void k_mc_pairinfo(McArgs a, StdTable st) {
   if (lane == 0) a.b.part[i] = part;
 }
 
-// mc2's trial loop, ntrial trials, one block.
+// mc2's trial loop, ntrial trials, one block.  The configuration (COM,
+// quaternion, step sizes) lives in LDS for the whole call, and the random
+// stream is staged into LDS kMcChunk trials at a time; the only global
+// traffic per trial is the PairInfo row of the target (issued first) and,
+// on acceptance, the PairInfo and configuration write-back.
+constexpr int kMcChunk = 192;                 // trials per staged chunk
+constexpr int kMcStage = kMcChunk * 14 + 2;   // stream entries per chunk
 template <int M>
 __global__ __launch_bounds__(kMcThreads) void k_mc_trials(McArgs a, StdTable st, int ntrial) {
+  constexpr int NS = StdPattern<M>::NS;
+  extern __shared__ __align__(16) double s_mem[];
+  const int n = a.n;
+  double *l_com = s_mem;                      // [3][n]
+  double *l_q = l_com + 3 * n;                // [4][n]
+  double *l_delta = l_q + 4 * n;              // [n]
+  double *l_rtr = l_delta + n;                // [n]
+  double *l_u = l_rtr + n;                    // [kMcStage]
+  double *l_z = l_u + kMcStage;               // [kMcStage]
   __shared__ double s_red[kMcThreads / 64];
-  const int n = a.n, ns = a.ns;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const double *u = a.b.u, *zf = a.b.z;
-  // the COMs of this thread's molecules stay in registers between trials
-  double cx[kMcMaxPer], cy[kMcMaxPer], cz[kMcMaxPer];
-#pragma unroll
-  for (int m = 0; m < kMcMaxPer; m++) {
-    const int j = tid + m * kMcThreads;
-    cx[m] = j < n ? a.b.com[j] : 0.0;
-    cy[m] = j < n ? a.b.com[n + j] : 0.0;
-    cz[m] = j < n ? a.b.com[2 * n + j] : 0.0;
+  for (int k = tid; k < 3 * n; k += kMcThreads) l_com[k] = a.b.com[k];
+  for (int k = tid; k < 4 * n; k += kMcThreads) l_q[k] = a.b.quat[k];
+  for (int k = tid; k < n; k += kMcThreads) {
+    l_delta[k] = a.b.delta[k];
+    l_rtr[k] = a.b.rtr[k];
   }
-  int pos = 0;
+  int pos = 0;    // position in the call's stream
+  int base = -1;  // stream position of l_u[0]
   long long nacc = 0;
   double epsum = *a.b.epsum;
   for (int trial = 0; trial < ntrial; trial++) {
+    if (trial % kMcChunk == 0) {  // stage the next chunk (a trial reads <= 14)
+      __syncthreads();
+      base = pos;
+      for (int k = tid; k < kMcStage; k += kMcThreads) {
+        l_u[k] = a.b.u[base + k];
+        l_z[k] = a.b.z[base + k];
+      }
+      __syncthreads();
+    }
+    const int p = pos - base;
     // target = random_getnext*nmol + 1 (1-based); u = 1 exactly would name
     // molecule n+1, which the reference does not guard: clamped here
-    int target = (int)(u[pos] * (double)n + 1.0) - 1;
+    int target = (int)(l_u[p] * (double)n + 1.0) - 1;
     if (target >= n) target = n - 1;
-    const double dt = a.b.delta[target];
-    const double dr = a.b.delta[target] * a.b.rtr[target];
-    const double rotx = zf[pos + 1] * sqrt(dr * dr) + 0.0;
-    const double roty = zf[pos + 3] * sqrt(dr * dr) + 0.0;
-    const double rotz = zf[pos + 5] * sqrt(dr * dr) + 0.0;
+    double pold[kMcMaxPer];
+#pragma unroll
+    for (int m = 0; m < kMcMaxPer; m++) {
+      const int j = tid + m * kMcThreads;
+      pold[m] = j < n ? a.b.pinfo[(size_t)target * n + j] : 0.0;
+    }
+    const double dt = l_delta[target];
+    const double dr = l_delta[target] * l_rtr[target];
+    const double rotx = l_z[p + 1] * sqrt(dr * dr) + 0.0;
+    const double roty = l_z[p + 3] * sqrt(dr * dr) + 0.0;
+    const double rotz = l_z[p + 5] * sqrt(dr * dr) + 0.0;
     const double avgrot = (fabs(rotx) + fabs(roty) + fabs(rotz)) / 3.0;
     // qinfrotator (QuatModule.F90:203-211) + qnormalize (215-223)
     double qa = 1.0, qb = rotx, qc = roty, qd = rotz;
@@ -303,8 +330,8 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_trials(McArgs a, StdTable st,
     }
     // qadd (QuatModule.F90:108-133): new = rotator x old, a >= 0
     {
-      const double a2 = a.b.quat[target], b2 = a.b.quat[n + target],
-                   c2 = a.b.quat[2 * n + target], d2 = a.b.quat[3 * n + target];
+      const double a2 = l_q[target], b2 = l_q[n + target], c2 = l_q[2 * n + target],
+                   d2 = l_q[3 * n + target];
       const double a3 = qa * a2 - qb * b2 - qc * c2 - qd * d2;
       const double b3 = qa * b2 + qb * a2 + qc * d2 - qd * c2;
       const double c3 = qa * c2 + qc * a2 - qb * d2 + qd * b2;
@@ -320,16 +347,17 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_trials(McArgs a, StdTable st,
         qd = -qd;
       }
     }
-    const double deltax = zf[pos + 7] * sqrt(dt * dt) + 0.0;
-    const double deltay = zf[pos + 9] * sqrt(dt * dt) + 0.0;
-    const double deltaz = zf[pos + 11] * sqrt(dt * dt) + 0.0;
+    const double deltax = l_z[p + 7] * sqrt(dt * dt) + 0.0;
+    const double deltay = l_z[p + 9] * sqrt(dt * dt) + 0.0;
+    const double deltaz = l_z[p + 11] * sqrt(dt * dt) + 0.0;
     const double avgtrans = (fabs(deltax) + fabs(deltay) + fabs(deltaz)) / 3.0;
     pos += 13;
-    const double tx = deltax + a.b.com[target], ty = deltay + a.b.com[n + target],
-                 tz = deltaz + a.b.com[2 * n + target];
+    const double tx = deltax + l_com[target], ty = deltay + l_com[n + target],
+                 tz = deltaz + l_com[2 * n + target];
     double sxt[kMaxSite], syt[kMaxSite], szt[kMaxSite];
-    place_sites_t<StdPattern<M>::NS>(a, qa, qb, qc, qd, tx, ty, tz, sxt, syt, szt);
-    // the compressed list of the moved target (radius rc + dt) and dE
+    place_sites_t<NS>(a, qa, qb, qc, qd, tx, ty, tz, sxt, syt, szt);
+    // the compressed list of the moved target (radius rc + dt) and dE; the
+    // sites of a listed molecule are placed again from its COM and quaternion
     const double rr = (a.rc + dt) * (a.rc + dt);
     double enew[kMcMaxPer], local = 0.0;
 #pragma unroll
@@ -338,7 +366,7 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_trials(McArgs a, StdTable st,
       enew[m] = 0.0;
       if (j < n) {
         if (j != target) {
-          double d0 = cx[m] - tx, d1 = cy[m] - ty, d2 = cz[m] - tz;
+          double d0 = l_com[j] - tx, d1 = l_com[n + j] - ty, d2 = l_com[2 * n + j] - tz;
           const double k0 = round(d0 * a.invL) * a.L, k1 = round(d1 * a.invL) * a.L,
                        k2 = round(d2 * a.invL) * a.L;
           d0 = d0 - k0;
@@ -347,11 +375,13 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_trials(McArgs a, StdTable st,
           const double dd = d0 * d0 + d1 * d1 + d2 * d2;
           if (dd < rr) {
             const double er = mc_switch(a, dd);
-            enew[m] = std_energy_t<M>(st, a.b.sx + (size_t)j * ns, a.b.sy + (size_t)j * ns,
-                                      a.b.sz + (size_t)j * ns, sxt, syt, szt, k0, k1, k2, er);
+            double xj[kMaxSite], yj[kMaxSite], zj[kMaxSite];
+            place_sites_t<NS>(a, l_q[j], l_q[n + j], l_q[2 * n + j], l_q[3 * n + j], l_com[j],
+                              l_com[n + j], l_com[2 * n + j], xj, yj, zj);
+            enew[m] = std_energy_t<M>(st, xj, yj, zj, sxt, syt, szt, k0, k1, k2, er);
           }
         }
-        local = local + (enew[m] - a.b.pinfo[(size_t)target * n + j]);
+        local = local + (enew[m] - pold[m]);
       }
     }
 #pragma unroll
@@ -366,7 +396,7 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_trials(McArgs a, StdTable st,
       accept = true;
     } else {
       const double ratio = exp(-(de * kI2J * kJ2K) * a.beta);
-      accept = u[pos] < ratio;
+      accept = l_u[pos - base] < ratio;
       pos += 1;
     }
     if (accept) {
@@ -378,38 +408,40 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_trials(McArgs a, StdTable st,
         if (j < n) {
           a.b.pinfo[(size_t)j * n + target] = enew[m];
           a.b.pinfo[(size_t)target * n + j] = enew[m];
-          if (j == target) {
-            cx[m] = tx;
-            cy[m] = ty;
-            cz[m] = tz;
-          }
         }
       }
       if (tid == 0) {
-        a.b.com[target] = tx;
-        a.b.com[n + target] = ty;
-        a.b.com[2 * n + target] = tz;
-        a.b.quat[target] = qa;
-        a.b.quat[n + target] = qb;
-        a.b.quat[2 * n + target] = qc;
-        a.b.quat[3 * n + target] = qd;
-#pragma unroll
-        for (int s = 0; s < StdPattern<M>::NS; s++) {
-          a.b.sx[(size_t)target * ns + s] = sxt[s];
-          a.b.sy[(size_t)target * ns + s] = syt[s];
-          a.b.sz[(size_t)target * ns + s] = szt[s];
-        }
-        a.b.delta[target] = a.b.delta[target] * kIncrement;
-        if (avgtrans * a.b.rtr[target] < avgrot)
-          a.b.rtr[target] = a.b.rtr[target] * kIncrement;
+        l_com[target] = tx;
+        l_com[n + target] = ty;
+        l_com[2 * n + target] = tz;
+        l_q[target] = qa;
+        l_q[n + target] = qb;
+        l_q[2 * n + target] = qc;
+        l_q[3 * n + target] = qd;
+        l_delta[target] = l_delta[target] * kIncrement;
+        if (avgtrans * l_rtr[target] < avgrot)
+          l_rtr[target] = l_rtr[target] * kIncrement;
         else
-          a.b.rtr[target] = a.b.rtr[target] / kIncrement;
+          l_rtr[target] = l_rtr[target] / kIncrement;
+      } else if (tid == 64) {  // the site positions the host may read back
+#pragma unroll
+        for (int s = 0; s < NS; s++) {
+          a.b.sx[(size_t)target * a.ns + s] = sxt[s];
+          a.b.sy[(size_t)target * a.ns + s] = syt[s];
+          a.b.sz[(size_t)target * a.ns + s] = szt[s];
+        }
       }
     } else if (tid == 0) {
-      a.b.delta[target] = a.b.delta[target] / kIncrement;
-      if (a.b.delta[target] < kDeltaMin) a.b.delta[target] = kDeltaMin;
+      l_delta[target] = l_delta[target] / kIncrement;
+      if (l_delta[target] < kDeltaMin) l_delta[target] = kDeltaMin;
     }
     __syncthreads();
+  }
+  for (int k = tid; k < 3 * n; k += kMcThreads) a.b.com[k] = l_com[k];
+  for (int k = tid; k < 4 * n; k += kMcThreads) a.b.quat[k] = l_q[k];
+  for (int k = tid; k < n; k += kMcThreads) {
+    a.b.delta[k] = l_delta[k];
+    a.b.rtr[k] = l_rtr[k];
   }
   if (tid == 0) {
     *a.b.pos = pos;
@@ -417,6 +449,9 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_trials(McArgs a, StdTable st,
     *a.b.epsum = epsum;
   }
 }
+
+// LDS bytes of k_mc_trials for n molecules
+static size_t mc_lds(int n) { return sizeof(double) * ((size_t)9 * n + 2 * kMcStage); }
 
 static McArgs mc_args(const System &s) {
   McArgs a{};
@@ -483,7 +518,7 @@ int g6_mc_begin(g6_handle h, double temp, uint32_t seed, const double *delta,
   if (!s || !(temp > 0) || !delta || !rtratio) return G6_EARG;
   if (s->model == G6_ST2) return G6_EARG;  // ST2's energy path is its own kernel
   const int n = s->n, ns = s->ns;
-  if (n > kMcThreads * kMcMaxPer) return G6_EARG;
+  if (n > kMcThreads * kMcMaxPer || mc_lds(n) > 150 * 1024) return G6_EARG;
   mc_free(*s);
   McBuf &m = s->mc;
   auto A = [](void **p, size_t b) { return hipMalloc(p, b > 0 ? b : 8); };
@@ -536,8 +571,10 @@ int g6_mc_trials(g6_handle h, int ntrial, long long *naccept) {
   if (ntrial == 0) return 0;
   McBuf &m = s->mc;
   McHost *hst = &s->mch;
-  // a trial consumes 13 or 14 numbers; the factor at k also reads k + 1
-  const size_t need = (size_t)ntrial * 14 + 2;
+  // a trial consumes 13 or 14 numbers; the factor at k also reads k + 1;
+  // the kernel stages whole chunks, so the last one may read kMcStage past
+  // the position it starts at
+  const size_t need = (size_t)ntrial * 14 + 2 + kMcStage;
   while (hst->pend.size() < need) hst->pend.push_back(hst->getnext());
   if ((int)need > m.ucap) {
     if (m.u) (void)hipFree(m.u);
@@ -557,10 +594,16 @@ int g6_mc_trials(g6_handle h, int ntrial, long long *naccept) {
   G6_CHECK(hipMemcpy(m.u, hst->pend.data(), sizeof(double) * need, hipMemcpyHostToDevice));
   G6_CHECK(hipMemcpy(m.z, z.data(), sizeof(double) * need, hipMemcpyHostToDevice));
   McArgs a = mc_args(*s);
-  if (s->model == G6_TIP4P)
-    k_mc_trials<G6_TIP4P><<<1, kMcThreads, 0, s->stream>>>(a, s->std, ntrial);
-  else
-    k_mc_trials<G6_TIP5P><<<1, kMcThreads, 0, s->stream>>>(a, s->std, ntrial);
+  const size_t lds = mc_lds(s->n);
+  if (s->model == G6_TIP4P) {
+    G6_CHECK(hipFuncSetAttribute((const void *)k_mc_trials<G6_TIP4P>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    k_mc_trials<G6_TIP4P><<<1, kMcThreads, lds, s->stream>>>(a, s->std, ntrial);
+  } else {
+    G6_CHECK(hipFuncSetAttribute((const void *)k_mc_trials<G6_TIP5P>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    k_mc_trials<G6_TIP5P><<<1, kMcThreads, lds, s->stream>>>(a, s->std, ntrial);
+  }
   G6_CHECK(hipGetLastError());
   int used = 0;
   long long nacc = 0;

@@ -138,7 +138,8 @@ int g6_collect(g6_handle h, const double *quat1, const double *fx,
  * mc2.F90:303-340 builds it, StdPairPotential energies times S), its sum,
  * per-molecule step sizes delta[n] and rotation ratios rtratio[n] (@DELT),
  * beta = 1/temp (@TEMP) and the random stream seeded with seed (@SEED).
- * TIP4P and TIP5P (the StdPairPotential models); n <= 2048. */
+ * TIP4P and TIP5P (the StdPairPotential models); n <= 1500 (the trial
+ * loop keeps the configuration in LDS). */
 int g6_mc_begin(g6_handle h, double temp, uint32_t seed, const double *delta,
                 const double *rtratio);
 /* ntrial single-molecule trials of mc2 (mc2.F90:540-862): target from the
