@@ -71,6 +71,86 @@ def cpu_baseline(com, q, budget_s):
     return N * k / el, k, el
 
 
+# Replica ladders: 4 ladders x 16 rungs, T_k = 220 (320/220)^(k/15) K; per
+# ladder (N, rho, L): TIP4P, rc 9.0 A switched over 0.5 A, FP32 pairs, Nose
+# Q = T tau^2/dt^2 (tau 0.1 ps); one exchange of 30 trials per ladder every
+# 500 steps over the adjacent-rung chain.
+LADDERS = [(256, 0.92, 20.266), (512, 0.97, 25.088), (1024, 1.00, 31.289),
+           (2048, 1.05, 38.786)]
+RUNGS, RX_EVERY, RX_TRIALS = 16, 500, 30
+
+
+def ladder_run(intervals, skin):
+    """All 64 replicas stepped by one ladder (one launch per MD kernel for
+    every replica, one graph per 500-step interval) with the reference's
+    replica exchange between intervals.  Returns molecule-steps/s over the
+    timed intervals, exchanges included."""
+    from genesis6_amd import g6
+    temps = [220.0 * (320.0 / 220.0) ** (k / 15.0) for k in range(RUNGS)]
+    systems, groups = [], []
+    for li, (n, rho, Lb) in enumerate(LADDERS):
+        rung = []
+        for k, T in enumerate(temps):
+            seed = 20261202 + 4 + 100 * li + k
+            s = g6.System(g6.TIP4P, n, Lb, RC, TAPER, DT, prec=g6.FP32, skin=skin)
+            com, q = g6.lattice_start(n, Lb, seed)
+            s.upload(com, q)
+            s.set_nose(T, T * 2500.0)
+            s.vreset(T, seed)
+            rung.append(s)
+        systems += rung
+        bonds = g6.rxpr_bonds(RUNGS, [(k, k + 1) for k in range(RUNGS - 1)])
+        groups.append(g6.ReplicaExchange(RUNGS, bonds, RX_TRIALS, 20261202 + 4 + 100 * li + 99,
+                                         systems=rung))
+    lad = g6.Ladder(systems)
+    nmol = sum(s.n for s in systems)
+
+    def interval():
+        lad.step(RX_EVERY)
+        th = lad.thermo()
+        acc = 0
+        for li, grp in enumerate(groups):
+            acc += grp.exchange(th[li * RUNGS:(li + 1) * RUNGS])
+        return acc, th
+
+    interval()                                   # warm-up: graph capture
+    t0 = time.perf_counter()
+    acc = 0
+    for _ in range(intervals):
+        a, th = interval()
+        acc += a
+    el = time.perf_counter() - t0
+    steps = intervals * RX_EVERY
+    out = dict(value=nmol * steps / el, unit="molecule-steps/s", replicas=len(systems),
+               molecules=nmol, steps=steps, ms_per_step=el / steps * 1e3,
+               exchange_trials=intervals * RX_TRIALS * len(groups), accepted=acc,
+               pairs_per_step=float(th[:, 14].sum()),
+               mean_temperature_K=float(th[:, 3].mean()))
+    lad.close()
+    for g in groups:
+        g.close()
+    for s in systems:
+        s.close()
+    return out
+
+
+def tip5p_run(steps, skin):
+    """65,536 TIP5P molecules at 1.0 g/cm3 (L = 125.157 A), Nose NVT 275 K,
+    rc 9.0 A switched over 0.5 A, FP32 pairs."""
+    from genesis6_amd import g6
+    n, Lb, T = 65536, 125.157, 275.0
+    s = g6.System(g6.TIP5P, n, Lb, RC, TAPER, DT, prec=g6.FP32, skin=skin)
+    com, q = g6.lattice_start(n, Lb, 20261202 + 3)
+    s.upload(com, q)
+    s.set_nose(T, T * 2500.0)
+    s.vreset(T, 20261202 + 3)
+    el, th = timed_run(s, steps, 50)
+    s.close()
+    return dict(value=n * steps / el, unit="molecule-steps/s", molecules=n, steps=steps,
+                ms_per_step=el / steps * 1e3, pairs_per_step=float(th[14]),
+                temperature_K=float(th[3]))
+
+
 def traffic_from_profile():
     """HBM bytes per force-kernel launch from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if present."""
@@ -91,6 +171,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--ladder-intervals", type=int, default=4,
+                    help="timed 500-step intervals of the 64-replica ladder (0: skip)")
+    ap.add_argument("--tip5p-steps", type=int, default=500,
+                    help="timed steps of the TIP5P-65536 system (0: skip)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run five device-to-device copies of this many MiB and exit "
                          "(a plain copy kernel of known size, which calibrates the "
@@ -133,6 +217,17 @@ def main():
         fp64 = dict(value=N * k64 / el64, steps=k64, force_kernel_us=t64[0] * 1e3)
         s64.close()
         print("bench: FP64 %.4g molecule-steps/s" % fp64["value"], file=sys.stderr, flush=True)
+    ladder = None
+    if a.ladder_intervals > 0:
+        ladder = ladder_run(a.ladder_intervals, a.skin)
+        print("bench: ladder %.4g molecule-steps/s (%d replicas, %.3f ms/step)"
+              % (ladder["value"], ladder["replicas"], ladder["ms_per_step"]),
+              file=sys.stderr, flush=True)
+    tip5p = None
+    if a.tip5p_steps > 0:
+        tip5p = tip5p_run(a.tip5p_steps, a.skin)
+        print("bench: TIP5P-65536 %.4g molecule-steps/s" % tip5p["value"], file=sys.stderr,
+              flush=True)
     cpu = None
     if not a.no_cpu:
         v, k, e = cpu_baseline(com, q, a.cpu_seconds)
@@ -171,6 +266,8 @@ def main():
                      "per_molecule_kernels_ms": t[2],
                      "eager_step_ms": t[3]},
         "cpu_baseline": cpu,
+        "replica_ladders": ladder,
+        "tip5p_65536": tip5p,
     }
     print(json.dumps(out))
 

@@ -111,6 +111,15 @@ struct MdBuf {
   double half_skin2;
 };
 
+// Descriptor of one system of a ladder, in device memory: the ladder
+// kernels (k_md_*_lad) read it per block instead of taking it by value.
+struct MdRep {
+  Geom g;  // the MD grid of the system
+  MdBuf b;
+  double dt;
+  int nose_active, nblk_main;
+};
+
 // Monte Carlo state (g6_mc.hip): the mc2 single-molecule trial loop on a
 // configuration of its own, FP64 throughout.
 //   com[3][n], quat[4][n] (order-1 quaternions), sx/sy/sz[n*ns] site
@@ -236,6 +245,37 @@ void md_launch_close(System &s, int mode, int parity, int src_parity);
 // k_md_main of this parity; close_mode != 0 closes the previous step
 // (partials of parity 1 - parity) in block 0
 void md_launch_main(System &s, double *out, int parity, int close_mode);
+// A ladder: several systems of one model and precision stepped together,
+// each MD kernel launched once for all of them (g6_ladder.hip).  The block
+// maps interleave the systems over the eight XCDs: workgroup b runs on XCD
+// b mod 8, and every block of a system gets the same b mod 8, so that a
+// system's records and rows stay in one XCD's L2.
+struct Ladder {
+  std::vector<System *> sys;
+  int model = 0, prec = 0;
+  ModelTable mt{};
+  PairK<float> pkf{};
+  PairK<double> pkd{};
+  MdRep *reps = nullptr;                         // [nsys], device
+  int2 *map_pre = nullptr, *map_main = nullptr;  // [grid], device
+  int grid_pre = 0, grid_main = 0, reg_bytes = 0;
+  double *th_dev = nullptr;  // [nsys][G6_THERMO_LEN] gathered thermo records
+  int *fl_dev = nullptr;     // [nsys] gathered error codes
+  double *th_host = nullptr; // pinned copies of the two
+  int *fl_host = nullptr;
+  hipStream_t stream = nullptr;
+  hipGraphExec_t graph = nullptr;
+  int graph_steps = 0;
+};
+void md_launch_pre_lad(Ladder &l, bool correct, bool predict, bool sites, int parity);
+void md_launch_close_lad(Ladder &l, int mode, int parity, int src_parity);
+void md_launch_main_lad(Ladder &l, int parity, int close_mode);
+void md_launch_gather_lad(Ladder &l);  // thermo records and error codes
+// Set a system's thermostat to temp after a replica exchange
+// (Mixture.F90:1630-1653): velocities scaled by sqrt(temp / old temp),
+// zeta0..zeta4 cleared.  Enqueued on the system's stream.
+int md_set_temperature(System &s, double temp);
+
 // host-array entry points of the module seam
 int pairlist_once(System &s, const double *com_h, int64_t cap, int64_t *npair0,
                   int64_t *npair, int32_t *pi, int32_t *pj, double *ox,

@@ -118,15 +118,16 @@ __device__ inline void store_qc(const MdBuf &b, int n, int i, const double (&S)[
 // (RigidModule.F90:480-559); values cross waves only through LDS, so the
 // split changes no bit of the result.  The roles are wave-uniform, so no
 // wave executes another role's code.
-__global__ __launch_bounds__(256) void k_md_pre(ModelTable mt, Geom g, MdBuf b, double dt,
-                                                int nose_active, int do_correct, int do_predict,
-                                                int do_sites, int parity) {
+// blk is the block's index within its system (blockIdx.x for one system).
+__device__ __forceinline__ void md_pre_body(const ModelTable &mt, const Geom &g, const MdBuf &b,
+                                            double dt, int nose_active, int do_correct,
+                                            int do_predict, int do_sites, int parity, int blk) {
   __shared__ double s_w0[3][64], s_W[12][64];
   __shared__ double s_q0[4][64], s_p[4][64], s_r[4][64], s_s[4][64], s_qn[4][64];
   const int n = g.n;
   const int role = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
-  const int i = blockIdx.x * 64 + lane;
+  const int i = blk * 64 + lane;
   const bool ok = i < n;
   const bool q2 = role == 2;  // wave 2: components a, b; wave 3: c, d
   double ek[kEkLen];  // Ek terms and the kinetic tensor
@@ -452,10 +453,30 @@ __global__ __launch_bounds__(256) void k_md_pre(ModelTable mt, Geom g, MdBuf b, 
     for (int q = 1; q < 4; q++) ek[q] = wsum63(ek[q]);
   }
   if (lane == 63) {
-    const size_t wv = (size_t)blockIdx.x * 4 + role;
+    const size_t wv = (size_t)blk * 4 + role;
 #pragma unroll
     for (int q = 0; q < kEkLen; q++) b.blockek[wv * kEkLen + q] = ek[q];
   }
+}
+
+__global__ __launch_bounds__(256) void k_md_pre(ModelTable mt, Geom g, MdBuf b, double dt,
+                                                int nose_active, int do_correct, int do_predict,
+                                                int do_sites, int parity) {
+  md_pre_body(mt, g, b, dt, nose_active, do_correct, do_predict, do_sites, parity, blockIdx.x);
+}
+
+// Ladder form: one launch over the blocks of every system of a ladder;
+// map[blockIdx.x] names the system and its block (system < 0: an idle
+// block that pads the XCD interleave).
+__global__ __launch_bounds__(256) void k_md_pre_lad(ModelTable mt, const MdRep *__restrict__ reps,
+                                                    const int2 *__restrict__ map, int do_correct,
+                                                    int do_predict, int do_sites, int parity) {
+  const int2 m = map[blockIdx.x];
+  if (m.x < 0) return;
+  const Geom g = reps[m.x].g;
+  const MdBuf b = reps[m.x].b;
+  md_pre_body(mt, g, b, reps[m.x].dt, reps[m.x].nose_active, do_correct, do_predict, do_sites,
+              parity, m.y);
 }
 
 // ---- step close: one block of 256 threads ---------------------------------
@@ -546,6 +567,7 @@ __device__ inline void close_body(const ModelTable &mt, const MdBuf &b, int n, d
       th[26] = vol;
       th[27] = nose_ep;
       th[28] = nose_ek;
+      th[29] = nose_active ? z[0] : 0.0;
     }
     if ((mode & 2) && nose_active) {
       z[3] = z[3] + z[4] + z[5] + z[6] + z[7];
@@ -563,6 +585,17 @@ __global__ __launch_bounds__(256) void k_md_close(ModelTable mt, MdBuf b, int n,
   __shared__ double s_red[4][12 + kEkLen];
   close_body(mt, b, n, dt, nblk_pre, bpart, nblk_main, ncell, nose_active, mode, parity, box_l,
              s_red);
+}
+
+// Ladder form: block r closes system r.
+__global__ __launch_bounds__(256) void k_md_close_lad(ModelTable mt, const MdRep *__restrict__ reps,
+                                                      int mode, int parity, int src_parity) {
+  __shared__ double s_red[4][12 + kEkLen];
+  const MdRep &r = reps[blockIdx.x];
+  const MdBuf b = r.b;
+  const int n = r.g.n, nm = r.nblk_main;
+  close_body(mt, b, n, r.dt, (n + 63) / 64 * 4, b.blockpart + (size_t)src_parity * nm * 16, nm,
+             r.g.ncell, r.nose_active, mode, parity, r.g.L, s_red);
 }
 
 // ---- kernel B --------------------------------------------------------------
@@ -754,21 +787,24 @@ __device__ inline double rowsum16(double v) {
   return v;
 }
 
+// blk is the block's index within its system and nblk the system's block
+// count (blockIdx.x and gridDim.x for one system).
 template <int M, typename R>
-__global__ __launch_bounds__(256) void k_md_main(ModelTable mt, Geom g, MdBuf b, PairK<R> pk,
-                                                 double *out, int parity, CloseArgs ca,
-                                                 int reg_bytes) {
+__device__ __forceinline__ void md_main_body(const ModelTable &mt, const Geom &g, const MdBuf &b,
+                                             const PairK<R> &pk, double *out, int parity,
+                                             const CloseArgs &ca, int reg_bytes, int blk,
+                                             int nblk) {
   using T = Traits<M>;
   extern __shared__ __align__(16) unsigned char s_dyn[];  // [4][reg_bytes]
   __shared__ int s_beg[4][128], s_pre[4][132];
   __shared__ double s_cnt[4][2];
   __shared__ double s_close[4][12 + kEkLen];
-  if (ca.mode && blockIdx.x == 0)
+  if (ca.mode && blk == 0)
     close_body(mt, b, g.n, ca.dt, ca.nblk_pre, ca.bpart, ca.nblk_main, g.ncell, ca.nose_active,
                ca.mode, parity, g.L, s_close);
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int n = g.n;
-  const int i = blockIdx.x * 4 + wid;
+  const int i = blk * 4 + wid;
   const int cap = b.maxnb;
   unsigned char *reg = s_dyn + (size_t)wid * reg_bytes;
   double ep = 0, v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -928,12 +964,40 @@ __global__ __launch_bounds__(256) void k_md_main(ModelTable mt, Geom g, MdBuf b,
     }
     r = rowsum16(r);
     if (seg == 0)
-      b.blockpart[((size_t)parity * gridDim.x + blockIdx.x) * 16 + q] = 0.5 * r;
+      b.blockpart[((size_t)parity * nblk + blk) * 16 + q] = 0.5 * r;
   } else if (threadIdx.x < 162) {
     const int q = threadIdx.x - 160;
-    b.blockpart[((size_t)parity * gridDim.x + blockIdx.x) * 16 + 10 + q] =
+    b.blockpart[((size_t)parity * nblk + blk) * 16 + 10 + q] =
         0.5 * (s_cnt[0][q] + s_cnt[1][q] + s_cnt[2][q] + s_cnt[3][q]);
   }
+}
+
+template <int M, typename R>
+__global__ __launch_bounds__(256) void k_md_main(ModelTable mt, Geom g, MdBuf b, PairK<R> pk,
+                                                 double *out, int parity, CloseArgs ca,
+                                                 int reg_bytes) {
+  md_main_body<M, R>(mt, g, b, pk, out, parity, ca, reg_bytes, blockIdx.x, gridDim.x);
+}
+
+// Ladder form (map as k_md_pre_lad); block 0 of each system closes that
+// system's previous step when close_mode != 0.
+template <int M, typename R>
+__global__ __launch_bounds__(256) void k_md_main_lad(ModelTable mt, const MdRep *__restrict__ reps,
+                                                     const int2 *__restrict__ map, PairK<R> pk,
+                                                     int parity, int close_mode, int reg_bytes) {
+  const int2 m = map[blockIdx.x];
+  if (m.x < 0) return;
+  const Geom g = reps[m.x].g;
+  const MdBuf b = reps[m.x].b;
+  const int nm = reps[m.x].nblk_main;
+  CloseArgs ca;
+  ca.mode = close_mode;
+  ca.nose_active = reps[m.x].nose_active;
+  ca.nblk_pre = (g.n + 63) / 64 * 4;
+  ca.nblk_main = nm;
+  ca.bpart = b.blockpart + (size_t)(1 - parity) * nm * 16;
+  ca.dt = reps[m.x].dt;
+  md_main_body<M, R>(mt, g, b, pk, nullptr, parity, ca, reg_bytes, m.y, nm);
 }
 
 void md_launch_pre(System &s, bool correct, bool predict, bool sites, int parity) {
@@ -983,6 +1047,33 @@ void md_launch_main(System &s, double *out, int parity, int close_mode) {
     if (s.model == G6_TIP4P) md_main_t<G6_TIP4P, double>(s, out, parity, ca);
     else if (s.model == G6_TIP5P) md_main_t<G6_TIP5P, double>(s, out, parity, ca);
     else md_main_t<G6_ST2, double>(s, out, parity, ca);
+  }
+}
+
+void md_launch_pre_lad(Ladder &l, bool correct, bool predict, bool sites, int parity) {
+  k_md_pre_lad<<<l.grid_pre, 256, 0, l.stream>>>(l.mt, l.reps, l.map_pre, correct ? 1 : 0,
+                                                 predict ? 1 : 0, sites ? 1 : 0, parity);
+}
+
+void md_launch_close_lad(Ladder &l, int mode, int parity, int src_parity) {
+  k_md_close_lad<<<(int)l.sys.size(), 256, 0, l.stream>>>(l.mt, l.reps, mode, parity, src_parity);
+}
+
+template <int M, typename R>
+static void md_main_lad_t(Ladder &l, const PairK<R> &pk, int parity, int close_mode) {
+  k_md_main_lad<M, R><<<l.grid_main, 256, 4 * (size_t)l.reg_bytes, l.stream>>>(
+      l.mt, l.reps, l.map_main, pk, parity, close_mode, l.reg_bytes);
+}
+
+void md_launch_main_lad(Ladder &l, int parity, int close_mode) {
+  if (l.prec == G6_FP32) {
+    if (l.model == G6_TIP4P) md_main_lad_t<G6_TIP4P, float>(l, l.pkf, parity, close_mode);
+    else if (l.model == G6_TIP5P) md_main_lad_t<G6_TIP5P, float>(l, l.pkf, parity, close_mode);
+    else md_main_lad_t<G6_ST2, float>(l, l.pkf, parity, close_mode);
+  } else {
+    if (l.model == G6_TIP4P) md_main_lad_t<G6_TIP4P, double>(l, l.pkd, parity, close_mode);
+    else if (l.model == G6_TIP5P) md_main_lad_t<G6_TIP5P, double>(l, l.pkd, parity, close_mode);
+    else md_main_lad_t<G6_ST2, double>(l, l.pkd, parity, close_mode);
   }
 }
 

@@ -188,6 +188,74 @@ module g6gpu
        real(c_double), intent(out) :: epsum
      end function g6_mc_energy
 
+     ! replica ladders and exchange groups (the replica branch of Mixture.F90)
+     integer(c_int) function g6_ladder_create(hl, nsys, hs) bind(C, name="g6_ladder_create")
+       import :: c_int, c_int64_t
+       integer(c_int64_t), intent(out) :: hl
+       integer(c_int), value :: nsys
+       integer(c_int64_t), intent(in) :: hs(*)
+     end function g6_ladder_create
+
+     integer(c_int) function g6_ladder_free(hl) bind(C, name="g6_ladder_free")
+       import :: c_int, c_int64_t
+       integer(c_int64_t), value :: hl
+     end function g6_ladder_free
+
+     integer(c_int) function g6_ladder_step(hl, nsteps) bind(C, name="g6_ladder_step")
+       import :: c_int, c_int64_t
+       integer(c_int64_t), value :: hl
+       integer(c_int), value :: nsteps
+     end function g6_ladder_step
+
+     integer(c_int) function g6_ladder_thermo(hl, thermo) bind(C, name="g6_ladder_thermo")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), value :: hl
+       real(c_double), intent(out) :: thermo(*)
+     end function g6_ladder_thermo
+
+     integer(c_int) function g6_set_temperature(h, temp) bind(C, name="g6_set_temperature")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), value :: h
+       real(c_double), value :: temp
+     end function g6_set_temperature
+
+     integer(c_int) function g6_rx_create(hx, nrep, hs, nbond, a, b, nexchange, seed) &
+          bind(C, name="g6_rx_create")
+       import :: c_int, c_int64_t, c_int32_t
+       integer(c_int64_t), intent(out) :: hx
+       integer(c_int), value :: nrep, nbond, nexchange
+       integer(c_int64_t), intent(in) :: hs(*)
+       integer(c_int32_t), intent(in) :: a(*), b(*)
+       integer(c_int32_t), value :: seed
+     end function g6_rx_create
+
+     integer(c_int) function g6_rx_free(hx) bind(C, name="g6_rx_free")
+       import :: c_int, c_int64_t
+       integer(c_int64_t), value :: hx
+     end function g6_rx_free
+
+     integer(c_int) function g6_rx_trials(hx, energy, beta, naccept) bind(C, name="g6_rx_trials")
+       import :: c_int, c_int64_t, c_int32_t, c_double
+       integer(c_int64_t), value :: hx
+       real(c_double), intent(in) :: energy(*)
+       real(c_double), intent(inout) :: beta(*)
+       integer(c_int32_t), intent(out) :: naccept
+     end function g6_rx_trials
+
+     integer(c_int) function g6_rx_nodes(hx, node_in_charge) bind(C, name="g6_rx_nodes")
+       import :: c_int, c_int64_t, c_int32_t
+       integer(c_int64_t), value :: hx
+       integer(c_int32_t), intent(out) :: node_in_charge(*)
+     end function g6_rx_nodes
+
+     ! thermo: c_loc of the records (node order) or c_null_ptr
+     integer(c_int) function g6_rx_exchange(hx, thermo, naccept) bind(C, name="g6_rx_exchange")
+       import :: c_int, c_int64_t, c_int32_t, c_ptr
+       integer(c_int64_t), value :: hx
+       type(c_ptr), value :: thermo
+       integer(c_int32_t), intent(out) :: naccept
+     end function g6_rx_exchange
+
      integer(c_int) function g6_model_nsite(model) bind(C, name="g6_model_nsite")
        import :: c_int
        integer(c_int), value :: model

@@ -22,6 +22,7 @@ _lib = None
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _i64p = C.POINTER(C.c_int64)
+_i64a = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
 _H = C.c_int64
 
 
@@ -63,6 +64,17 @@ def lib():
         L.g6_mc_get.argtypes = [_H, _dp, _dp, _dp, _dp, _dp]
         L.g6_mc_pairinfo.argtypes = [_H, _dp, _dp]
         L.g6_mc_energy.argtypes = [_H, _dp]
+        L.g6_ladder_create.argtypes = [C.POINTER(_H), C.c_int, _i64a]
+        L.g6_ladder_free.argtypes = [_H]
+        L.g6_ladder_step.argtypes = [_H, C.c_int]
+        L.g6_ladder_thermo.argtypes = [_H, _dp]
+        L.g6_set_temperature.argtypes = [_H, C.c_double]
+        L.g6_rx_create.argtypes = [C.POINTER(_H), C.c_int, C.c_void_p, C.c_int, _ip, _ip,
+                                   C.c_int, C.c_uint32]
+        L.g6_rx_free.argtypes = [_H]
+        L.g6_rx_trials.argtypes = [_H, _dp, _dp, C.POINTER(C.c_int32)]
+        L.g6_rx_nodes.argtypes = [_H, _ip]
+        L.g6_rx_exchange.argtypes = [_H, C.c_void_p, C.POINTER(C.c_int32)]
         _lib = L
     return _lib
 
@@ -142,6 +154,11 @@ class System:
 
     def step(self, nsteps=1):
         check(lib().g6_step(self.h, nsteps), "g6_step")
+
+    def set_temperature(self, temp):
+        """Thermostat retarget of a replica exchange: velocities scaled by
+        sqrt(temp / nose temp), nose temp = temp, zeta0..zeta4 = 0."""
+        check(lib().g6_set_temperature(self.h, temp), "g6_set_temperature")
 
     def step_timed(self, nsteps):
         t = np.zeros(5)
@@ -249,3 +266,108 @@ class System:
         check(lib().g6_collect(self.h, _c(quat1.T).ravel(), _c(forces[:, 0]), _c(forces[:, 1]),
                                _c(forces[:, 2]), F, T), "g6_collect")
         return F.reshape(3, n).T.copy(), T.reshape(3, n).T.copy()
+
+
+class Ladder:
+    """Several systems of one model and precision stepped together: each MD
+    kernel is launched once for all of them (a g6_ladder)."""
+
+    def __init__(self, systems):
+        self.systems = list(systems)
+        hs = np.array([s.h for s in self.systems], dtype=np.int64)
+        h = _H(0)
+        check(lib().g6_ladder_create(C.byref(h), len(hs), hs), "g6_ladder_create")
+        self.h = h.value
+
+    def step(self, nsteps):
+        check(lib().g6_ladder_step(self.h, nsteps), "g6_ladder_step")
+
+    def thermo(self):
+        """Thermo records of the last g6_ladder_step, (nsys, THERMO_LEN)."""
+        t = np.zeros(len(self.systems) * THERMO_LEN)
+        check(lib().g6_ladder_thermo(self.h, t), "g6_ladder_thermo")
+        return t.reshape(len(self.systems), THERMO_LEN)
+
+    def close(self):
+        if self.h:
+            lib().g6_ladder_free(self.h)
+            self.h = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def rxpr_bonds(nnode, links):
+    """Directed exchange bonds (a, b), 1-based, in the order graph2pair
+    (Graph.F90:34-52) lists the graph that graph_read (Graph.F90:112-127)
+    builds from an @RXPR deck of 0-based links (i, j); links=None gives
+    graph_perfect (Graph.F90:93-110), the default without @RXPR."""
+    nei = [[] for _ in range(nnode)]
+    if links is None:
+        for i in range(nnode):
+            for j in range(i + 1, nnode):
+                nei[i].append(j)
+                nei[j].append(i)
+    else:
+        for i, j in links:
+            nei[i].append(j)
+            nei[j].append(i)
+    a = [i + 1 for i in range(nnode) for _ in nei[i]]
+    b = [j + 1 for i in range(nnode) for j in nei[i]]
+    return np.array(a, np.int32), np.array(b, np.int32)
+
+
+class ReplicaExchange:
+    """An exchange group (a g6_rx): the node-0 state of the reference's
+    replica exchange, i.e. its MT19937 stream and node_in_charge."""
+
+    def __init__(self, nrep, bonds, nexchange, seed, systems=None):
+        self.nrep = nrep
+        self.systems = list(systems) if systems is not None else None
+        a, b = (_c(x, np.int32) for x in bonds)
+        hs = None
+        if self.systems is not None:
+            self._hs = np.array([s.h for s in self.systems], dtype=np.int64)
+            hs = self._hs.ctypes.data_as(C.c_void_p)
+        h = _H(0)
+        check(lib().g6_rx_create(C.byref(h), nrep, hs, len(a), a, b, nexchange, seed),
+              "g6_rx_create")
+        self.h = h.value
+
+    def trials(self, energy, beta):
+        """One exchange loop on given energies and betas (node order);
+        returns the new betas and the number of accepted swaps."""
+        beta = _c(beta).copy()
+        acc = C.c_int32(0)
+        check(lib().g6_rx_trials(self.h, _c(energy), beta, C.byref(acc)), "g6_rx_trials")
+        return beta, acc.value
+
+    def nodes(self):
+        nic = np.zeros(self.nrep, np.int32)
+        check(lib().g6_rx_nodes(self.h, nic), "g6_rx_nodes")
+        return nic
+
+    def exchange(self, thermo=None):
+        """The exchange of Mixture.F90:1570-1653 between the group's
+        systems; thermo (nrep, THERMO_LEN) from Ladder.thermo or None."""
+        ptr = None
+        if thermo is not None:
+            thermo = _c(thermo)
+            ptr = thermo.ctypes.data_as(C.c_void_p)
+        acc = C.c_int32(0)
+        check(lib().g6_rx_exchange(self.h, ptr, C.byref(acc)), "g6_rx_exchange")
+        return acc.value
+
+    def close(self):
+        if self.h:
+            lib().g6_rx_free(self.h)
+            self.h = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -47,7 +47,8 @@ enum {
  * [16..24] kinetic tensor of Rigid_PressureTensor (RigidModule.F90:600-619),
  * [25] pressure (atm, Mixture.F90:2243-2255), [26] volume (A^3),
  * [27] Nose potential term nose_ep, [28] Nose kinetic term nose_ek (J/mol,
- * Mixture.F90:2431-2434), [29..31] zero */
+ * Mixture.F90:2431-2434), [29] Nose target temperature nose%temp (K; 0
+ * without a thermostat), [30..31] zero */
 #define G6_THERMO_LEN 32
 
 /* ---- system lifecycle (Mixture.F90:1141-1347 setup) -------------------- */
@@ -157,6 +158,56 @@ int g6_mc_pairinfo(g6_handle h, double *ep, double *epsum);
 /* The total energy of the current MC configuration from a fresh PairInfo
  * (the bookkeeping is left as it is). */
 int g6_mc_energy(g6_handle h, double *epsum);
+
+/* ---- replica ladders (MDReplica.F90, the replica branch of Mixture.F90)
+ * The reference runs one process per replica (fakempi) and exchanges
+ * temperatures every @INNR steps.  Here the replicas are systems of one
+ * process: a ladder steps all of them together, each MD kernel launched
+ * once for every system, and an exchange group applies the reference's
+ * swap rule between them. */
+typedef int64_t g6_ladder;
+typedef int64_t g6_rx;
+
+/* A ladder over nsys distinct systems (g6_init handles) of one model and
+ * precision with equal rc and taper.  The systems keep their own state and
+ * stay usable on their own between ladder calls. */
+int g6_ladder_create(g6_ladder *hl, int nsys, const g6_handle *hs);
+int g6_ladder_free(g6_ladder hl);
+/* nsteps MD steps of every system: what g6_step does to each, with the
+ * same results bit for bit (the inner loop of Mixture.F90:1833-2286 on
+ * every node), launched as one captured hipGraph. */
+int g6_ladder_step(g6_ladder hl, int nsteps);
+/* the thermo records of the last step: thermo[nsys * G6_THERMO_LEN], in
+ * the order of hs */
+int g6_ladder_thermo(g6_ladder hl, double *thermo);
+
+/* Thermostat retarget after an exchange (Mixture.F90:1630-1653 with
+ * Rigid_ScaleVelocity, RigidModule.F90:621-633): dx(1) and w(1) scaled by
+ * sqrt(temp / nose%temp), nose%temp = temp, zeta0..zeta4 = 0.  Requires an
+ * active Nose thermostat. */
+int g6_set_temperature(g6_handle h, double temp);
+
+/* An exchange group of nrep replicas (node k = hs[k-1]; hs may be NULL for
+ * the host rule alone): directed bonds a[k] -> b[k], 1-based, in the order
+ * graph2pair gives (Graph.F90:34-52; @RXPR chain: 1->2, 2->1, 2->3, ...);
+ * nexchange trials per exchange (@NRXG); MT19937 seed of the group's
+ * stream (@SEED).  node_in_charge starts as the identity (Mixture.F90:1483). */
+int g6_rx_create(g6_rx *hx, int nrep, const g6_handle *hs, int nbond, const int32_t *a,
+                 const int32_t *b, int nexchange, uint32_t seed);
+int g6_rx_free(g6_rx hx);
+/* The exchange loop of Mixture.F90:1605-1622 with replica_exchange_trial
+ * (Replica.F90:7-73, no umbrella): energy[nrep] (internal units) and
+ * beta[nrep] (1/K) indexed by node; beta is updated in place and
+ * node_in_charge kept by the group.  naccept = accepted swaps. */
+int g6_rx_trials(g6_rx hx, const double *energy, double *beta, int32_t *naccept);
+/* node_in_charge[nrep] (1-based, as the reference) */
+int g6_rx_nodes(g6_rx hx, int32_t *node_in_charge);
+/* One exchange of Mixture.F90:1570-1653 between the group's systems:
+ * energies thermo[0] and temperatures thermo[29] of each replica's record
+ * (thermo[nrep * G6_THERMO_LEN] in node order, e.g. from g6_ladder_thermo;
+ * NULL reads each system's own), beta = 1/temp, the trials, then
+ * g6_set_temperature(1/beta) on every replica whose beta changed. */
+int g6_rx_exchange(g6_rx hx, const double *thermo, int32_t *naccept);
 
 /* ---- synthetic start (no reference counterpart: the reference reads
  * @NX4A files) --------------------------------------------------------- */

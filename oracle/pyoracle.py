@@ -5,6 +5,7 @@ cpu_baseline leg of bench.py, never by the product package.  Every wrapped
 function restates a reference routine; the citations live in g6_oracle.h.
 """
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -261,6 +262,17 @@ class MDRun:
     def hamiltonian(self):
         return lib().g6o_md_hamiltonian(C.byref(self.s))
 
+    def set_temperature(self, temp):
+        """After an accepted exchange (Mixture.F90:1630-1653): velocities
+        times dsqrt(temp / nose%temp) (Rigid_ScaleVelocity,
+        RigidModule.F90:621-633), nose%temp = temp, zeta0..zeta4 = 0."""
+        ratio = math.sqrt(temp / self.s.nose[0])
+        self.dx[0] *= ratio
+        self.w[0] *= ratio
+        self.s.nose[0] = temp
+        for z in range(3, 8):
+            self.s.nose[z] = 0.0
+
     @property
     def nose(self):
         return np.array(self.s.nose[:])
@@ -321,3 +333,45 @@ class MCRun:
         self.pos = p.value
         self.nacc += acc
         return self.nacc
+
+
+class RxRun:
+    """Node 0 of the reference's replica-exchange MD: the exchange loop of
+    Mixture.F90:1605-1622 with replica_exchange_trial (Replica.F90:7-73,
+    numbrella = 0), its Random_GetNext stream (@SEED) and node_in_charge
+    (identity at start, Mixture.F90:1483).  Bonds a, b are 1-based, in
+    graph2pair order.  A short loop per exchange: plain Python."""
+
+    I2J, J2K = 10.0, 0.120273  # PhysConst.F90:48, 56
+
+    def __init__(self, nrep, a, b, nexchange, seed):
+        self.a, self.b = [int(x) for x in a], [int(x) for x in b]
+        self.nexchange, self.seed = nexchange, seed
+        self.nic = list(range(1, nrep + 1))
+        self.u, self.pos = getnext_stream(seed, 1024), 0
+
+    def _getnext(self):
+        if self.pos >= len(self.u):
+            self.u = getnext_stream(self.seed, 2 * len(self.u))
+        v = float(self.u[self.pos])
+        self.pos += 1
+        return v
+
+    def trials(self, energy, beta):
+        beta = [float(x) for x in beta]
+        nb, acc = len(self.a), 0
+        for _ in range(self.nexchange):
+            # k = random_getnext * nbond + 1 (Fortran truncation); a draw of
+            # exactly 1 would index past the last bond: clamped
+            k = min(int(self._getnext() * nb + 1), nb)
+            j, jj = self.a[k - 1], self.b[k - 1]
+            n0, n1 = self.nic[j - 1], self.nic[jj - 1]
+            a0, a1 = float(energy[n0 - 1]), float(energy[n1 - 1])
+            acc0 = -(a0 * beta[n0 - 1] + a1 * beta[n1 - 1]) * self.I2J * self.J2K
+            acc1 = -(a0 * beta[n1 - 1] + a1 * beta[n0 - 1]) * self.I2J * self.J2K
+            accept = acc0 < acc1 or self._getnext() < math.exp(acc1 - acc0)
+            if accept:
+                beta[n0 - 1], beta[n1 - 1] = beta[n1 - 1], beta[n0 - 1]
+                self.nic[j - 1], self.nic[jj - 1] = n1, n0
+                acc += 1
+        return np.array(beta), acc

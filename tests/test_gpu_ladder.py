@@ -1,0 +1,126 @@
+"""Replica ladders on the GPU: one launch per MD kernel for several systems
+gives each system exactly what g6_step gives it alone; the thermostat
+retarget of an exchange (Mixture.F90:1630-1653) matches the oracle; a full
+exchange between ladder replicas makes the oracle's decisions."""
+import numpy as np
+import pytest
+
+from genesis6_amd import g6
+from oracle import pyoracle as O
+from tests.golden.configs import lattice_start
+
+pytestmark = pytest.mark.gpu
+
+RC, TAPER, DT = 7.0, 0.5, 0.002
+
+
+def box(n, rho):
+    """Cubic box edge (A) for n waters at rho g/cm3 (M = 18.015 g/mol)."""
+    return (n * 18.015 / (6.0221e23 * rho) * 1e24) ** (1.0 / 3.0)
+
+
+def make(n, rho, seed, temp, prec, skin, nose=True):
+    L = box(n, rho)
+    print("system n=%d L=%.4f T=%g prec=%d skin=%g nose=%d" % (n, L, temp, prec, skin, nose),
+          flush=True)
+    s = g6.System(g6.TIP4P, n, L, RC, TAPER, DT, prec=prec, skin=skin)
+    com, q = lattice_start(n, L, seed)
+    s.upload(com, q)
+    if nose:
+        s.set_nose(temp, temp * 2500.0)
+    s.vreset(temp, seed)
+    return s
+
+
+SPECS = [(128, 0.92, 11, 250.0, 0.5, True), (250, 1.0, 12, 280.0, 1.0, True),
+         (300, 1.05, 13, 300.0, 0.0, False), (128, 1.0, 14, 320.0, 0.5, True)]
+
+
+@pytest.mark.parametrize("prec", [g6.FP32, g6.FP64])
+def test_ladder_steps_equal_single_system_steps_bitwise(prec):
+    lad_sys = [make(n, r, sd, T, prec, sk, ns) for n, r, sd, T, sk, ns in SPECS]
+    solo = [make(n, r, sd, T, prec, sk, ns) for n, r, sd, T, sk, ns in SPECS]
+    lad = g6.Ladder(lad_sys)
+    for k in (7, 60, 1):
+        print("steps", k, flush=True)
+        lad.step(k)
+        for s in solo:
+            s.step(k)
+        th = lad.thermo()
+        for r, (a, b) in enumerate(zip(lad_sys, solo)):
+            for x, y in zip(a.download(), b.download()):
+                assert np.array_equal(x, y), (k, r)
+            assert np.array_equal(th[r], b.thermo()), (k, r)
+            assert np.array_equal(a.nose(), b.nose()), (k, r)
+    # the systems stay usable on their own after ladder steps
+    lad_sys[1].step(3)
+    solo[1].step(3)
+    for x, y in zip(lad_sys[1].download(), solo[1].download()):
+        assert np.array_equal(x, y)
+
+
+def test_set_temperature_matches_oracle():
+    n, L = 128, box(128, 1.0)
+    com, q = lattice_start(n, L, 21)
+    s = g6.System(g6.TIP4P, n, L, RC, TAPER, DT, prec=g6.FP64)
+    s.upload(com, q)
+    s.set_nose(260.0, 260.0 * 2500.0)
+    s.vreset(260.0, 21)
+    md = O.MDRun(O.TIP4P, com, q.T.copy(), L, RC, TAPER, DT, fast=True,
+                 nose=(260.0, 260.0 * 2500.0, 0.0))
+    md.vreset(260.0, 21)
+    s.step(4)
+    md.step(4)
+    _, _, dx0, w0 = s.download()
+    z0 = s.nose()
+    s.set_temperature(281.5)
+    md.set_temperature(281.5)
+    _, _, dx1, w1 = s.download()
+    z1 = s.nose()
+    ratio = np.sqrt(281.5 / z0[0])
+    assert np.array_equal(dx1[0], dx0[0] * ratio)
+    assert np.array_equal(w1[0], w0[0] * ratio)
+    assert np.array_equal(dx1[1:], dx0[1:]) and np.array_equal(w1[1:], w0[1:])
+    assert z1[0] == 281.5 and not z1[3:8].any()
+    assert z1[1] == z0[1] and z1[2] == z0[2] and z1[8] == z0[8]
+    # the retargeted run continues as the oracle's
+    s.step(5)
+    md.step(5)
+    c, _, _, _ = s.download()
+    assert np.abs(c - md.com).max() < 1e-10
+    assert abs(s.thermo()[3] - md.s.temperature) < 1e-9 * md.s.temperature
+
+
+def test_ladder_exchange_makes_the_oracle_decisions():
+    temps = [250.0, 251.0, 252.0, 253.5]
+    sys_ = [make(128, 1.0, 30 + k, T, g6.FP32, 0.5) for k, T in enumerate(temps)]
+    lad = g6.Ladder(sys_)
+    a, b = g6.rxpr_bonds(4, [(0, 1), (1, 2), (2, 3)])
+    rx = g6.ReplicaExchange(4, (a, b), 30, 777, systems=sys_)
+    orc = O.RxRun(4, a, b, 30, 777)
+    total = 0
+    for rnd in range(3):
+        print("exchange round", rnd, flush=True)
+        lad.step(20)
+        th = lad.thermo()
+        before = [s.download() for s in sys_]
+        tnow = th[:, 29].copy()
+        assert np.array_equal(tnow, np.array([s.nose()[0] for s in sys_]))
+        beta_o, acc_o = orc.trials(th[:, 0], 1.0 / tnow)
+        acc = rx.exchange(th)
+        assert acc == acc_o
+        total += acc
+        assert list(rx.nodes()) == orc.nic
+        for r, s in enumerate(sys_):
+            z = s.nose()
+            _, _, dx, w = s.download()
+            if beta_o[r] != 1.0 / tnow[r]:
+                tn = 1.0 / beta_o[r]
+                assert z[0] == tn and not z[3:8].any()
+                ratio = np.sqrt(tn / tnow[r])
+                assert np.array_equal(dx[0], before[r][2][0] * ratio)
+                assert np.array_equal(w[0], before[r][3][0] * ratio)
+            else:
+                assert z[0] == tnow[r]
+                assert np.array_equal(dx, before[r][2]) and np.array_equal(w, before[r][3])
+    assert total > 0  # the rule was exercised on accepted swaps
