@@ -256,6 +256,56 @@ module g6gpu
        integer(c_int32_t), intent(out) :: naccept
      end function g6_rx_exchange
 
+     ! the learned energy surrogate (arrays float32: real(c_float))
+     integer(c_int) function g6_mlp_create(hm) bind(C, name="g6_mlp_create")
+       import :: c_int, c_int64_t
+       integer(c_int64_t), intent(out) :: hm
+     end function g6_mlp_create
+
+     integer(c_int) function g6_mlp_free(hm) bind(C, name="g6_mlp_free")
+       import :: c_int, c_int64_t
+       integer(c_int64_t), value :: hm
+     end function g6_mlp_free
+
+     ! params, norm: c_loc of real(c_float) arrays or c_null_ptr
+     integer(c_int) function g6_mlp_set_params(hm, params, norm) bind(C, name="g6_mlp_set_params")
+       import :: c_int, c_int64_t, c_ptr
+       integer(c_int64_t), value :: hm
+       type(c_ptr), value :: params, norm
+     end function g6_mlp_set_params
+
+     integer(c_int) function g6_mlp_get_params(hm, params, norm) bind(C, name="g6_mlp_get_params")
+       import :: c_int, c_int64_t, c_ptr
+       integer(c_int64_t), value :: hm
+       type(c_ptr), value :: params, norm
+     end function g6_mlp_get_params
+
+     integer(c_int) function g6_mlp_forward(hm, m, x, y) bind(C, name="g6_mlp_forward")
+       import :: c_int, c_int64_t, c_float
+       integer(c_int64_t), value :: hm, m
+       real(c_float), intent(in) :: x(*)
+       real(c_float), intent(out) :: y(*)
+     end function g6_mlp_forward
+
+     integer(c_int) function g6_mlp_loss_grad(hm, m, x, t, loss, grad) &
+          bind(C, name="g6_mlp_loss_grad")
+       import :: c_int, c_int64_t, c_float
+       integer(c_int64_t), value :: hm, m
+       real(c_float), intent(in) :: x(*), t(*)
+       real(c_float), intent(out) :: loss, grad(*)
+     end function g6_mlp_loss_grad
+
+     integer(c_int) function g6_mlp_train(hm, m, x, t, epochs, batch, lr, seed, loss_per_epoch) &
+          bind(C, name="g6_mlp_train")
+       import :: c_int, c_int64_t, c_float, c_int32_t
+       integer(c_int64_t), value :: hm, m
+       real(c_float), intent(in) :: x(*), t(*)
+       integer(c_int), value :: epochs, batch
+       real(c_float), value :: lr
+       integer(c_int32_t), value :: seed
+       real(c_float), intent(out) :: loss_per_epoch(*)
+     end function g6_mlp_train
+
      integer(c_int) function g6_model_nsite(model) bind(C, name="g6_model_nsite")
        import :: c_int
        integer(c_int), value :: model

@@ -23,6 +23,7 @@ _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 _i64p = C.POINTER(C.c_int64)
 _i64a = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _H = C.c_int64
 
 
@@ -75,6 +76,14 @@ def lib():
         L.g6_rx_trials.argtypes = [_H, _dp, _dp, C.POINTER(C.c_int32)]
         L.g6_rx_nodes.argtypes = [_H, _ip]
         L.g6_rx_exchange.argtypes = [_H, C.c_void_p, C.POINTER(C.c_int32)]
+        L.g6_mlp_create.argtypes = [C.POINTER(_H)]
+        L.g6_mlp_free.argtypes = [_H]
+        L.g6_mlp_set_params.argtypes = [_H, C.c_void_p, C.c_void_p]
+        L.g6_mlp_get_params.argtypes = [_H, C.c_void_p, C.c_void_p]
+        L.g6_mlp_forward.argtypes = [_H, C.c_int64, _fp, _fp]
+        L.g6_mlp_loss_grad.argtypes = [_H, C.c_int64, _fp, _fp, C.POINTER(C.c_float), _fp]
+        L.g6_mlp_train.argtypes = [_H, C.c_int64, _fp, _fp, C.c_int, C.c_int, C.c_float,
+                                   C.c_uint32, _fp]
         _lib = L
     return _lib
 
@@ -371,3 +380,73 @@ class ReplicaExchange:
             self.close()
         except Exception:
             pass
+
+
+MLP_NIN, MLP_NPARAM, MLP_NNORM = 32, 6337, 66
+
+
+def _vp(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class Mlp:
+    """The per-molecule energy network on the GPU (a g6_mlp)."""
+
+    def __init__(self, params=None, norm=None):
+        h = _H(0)
+        check(lib().g6_mlp_create(C.byref(h)), "g6_mlp_create")
+        self.h = h.value
+        if params is not None or norm is not None:
+            self.set(params, norm)
+
+    def set(self, params=None, norm=None):
+        p = None if params is None else _c(params, np.float32)
+        n = None if norm is None else _c(norm, np.float32)
+        if p is not None and p.size != MLP_NPARAM or n is not None and n.size != MLP_NNORM:
+            raise G6Error("g6_mlp_set_params: wrong sizes")
+        check(lib().g6_mlp_set_params(self.h, _vp(p), _vp(n)), "g6_mlp_set_params")
+
+    def get(self):
+        p = np.zeros(MLP_NPARAM, np.float32); n = np.zeros(MLP_NNORM, np.float32)
+        check(lib().g6_mlp_get_params(self.h, _vp(p), _vp(n)), "g6_mlp_get_params")
+        return p, n
+
+    def forward(self, x):
+        x = _c(x, np.float32).reshape(-1, MLP_NIN)
+        y = np.zeros(len(x), np.float32)
+        check(lib().g6_mlp_forward(self.h, len(x), x, y), "g6_mlp_forward")
+        return y
+
+    def loss_grad(self, x, t):
+        x = _c(x, np.float32).reshape(-1, MLP_NIN)
+        g = np.zeros(MLP_NPARAM, np.float32); loss = C.c_float(0)
+        check(lib().g6_mlp_loss_grad(self.h, len(x), x, _c(t, np.float32), C.byref(loss), g),
+              "g6_mlp_loss_grad")
+        return loss.value, g
+
+    def train(self, x, t, epochs, batch, lr, seed):
+        x = _c(x, np.float32).reshape(-1, MLP_NIN)
+        out = np.zeros(epochs, np.float32)
+        check(lib().g6_mlp_train(self.h, len(x), x, _c(t, np.float32), epochs, batch, lr, seed,
+                                 out), "g6_mlp_train")
+        return out
+
+    def close(self):
+        if self.h:
+            lib().g6_mlp_free(self.h)
+            self.h = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def mlp_init_params(seed):
+    """Glorot-uniform weights and zero biases for a new network."""
+    rng = np.random.default_rng(seed)
+    a1 = np.sqrt(6.0 / (MLP_NIN + 64)); a2 = np.sqrt(6.0 / 128.0); a3 = np.sqrt(6.0 / 65.0)
+    return np.concatenate([rng.uniform(-a1, a1, 64 * MLP_NIN), np.zeros(64),
+                           rng.uniform(-a2, a2, 64 * 64), np.zeros(64),
+                           rng.uniform(-a3, a3, 64), [0.0]]).astype(np.float32)

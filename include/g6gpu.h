@@ -209,6 +209,33 @@ int g6_rx_nodes(g6_rx hx, int32_t *node_in_charge);
  * g6_set_temperature(1/beta) on every replica whose beta changed. */
 int g6_rx_exchange(g6_rx hx, const double *thermo, int32_t *naccept);
 
+/* ---- learned energy surrogate (no reference counterpart) ---------------
+ * A per-molecule MLP 32 -> 64 -> 64 -> 1 with SiLU: BF16 weights on the
+ * matrix cores, FP32 accumulation, FP32 master weights.  params[6337] =
+ * W1[64][32] (output-major), b1[64], W2[64][64], b2[64], w3[64], b3;
+ * norm[66] = feature mean[32], feature 1/std[32], y_shift, y_scale: the
+ * network sees (x - mean) / std and its output o gives y_scale*o + y_shift. */
+typedef int64_t g6_mlp;
+#define G6_MLP_NIN 32
+#define G6_MLP_NPARAM 6337
+#define G6_MLP_NNORM 66
+/* zero weights, identity normalisation */
+int g6_mlp_create(g6_mlp *hm);
+int g6_mlp_free(g6_mlp hm);
+/* either pointer may be NULL (left unchanged); new params restart Adam */
+int g6_mlp_set_params(g6_mlp hm, const float *params, const float *norm);
+int g6_mlp_get_params(g6_mlp hm, float *params, float *norm);
+/* y[m] for feature rows x[m][32] */
+int g6_mlp_forward(g6_mlp hm, int64_t m, const float *x, float *y);
+/* loss = mean over rows of (o - (t - y_shift)/y_scale)^2 and its gradient
+ * grad[6337] with respect to params */
+int g6_mlp_loss_grad(g6_mlp hm, int64_t m, const float *x, const float *t, float *loss,
+                     float *grad);
+/* Adam (0.9, 0.999, 1e-8) on minibatches of a shuffled copy of the rows
+ * (MT19937 seed), data resident on the device; loss_per_epoch[epochs] */
+int g6_mlp_train(g6_mlp hm, int64_t m, const float *x, const float *t, int epochs, int batch,
+                 float lr, uint32_t seed, float *loss_per_epoch);
+
 /* ---- synthetic start (no reference counterpart: the reference reads
  * @NX4A files) --------------------------------------------------------- */
 /* Simple-cubic lattice, m = ceil(n^(1/3)) per edge, occupied sites chosen by
