@@ -256,6 +256,40 @@ module g6gpu
        integer(c_int32_t), intent(out) :: naccept
      end function g6_rx_exchange
 
+     ! NPT Monte Carlo with collective moves
+     integer(c_int) function g6_npt_begin(h, temp, press_bar, seed, disp_max, rot_max_deg, &
+          dlnv_max, p_volume, voxa) bind(C, name="g6_npt_begin")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), value :: h, seed
+       real(c_double), value :: temp, press_bar, disp_max, rot_max_deg, dlnv_max, p_volume, voxa
+     end function g6_npt_begin
+
+     integer(c_int) function g6_npt_run(h, count, hm, k, stats) bind(C, name="g6_npt_run")
+       import :: c_int, c_int64_t, c_long_long
+       integer(c_int64_t), value :: h, hm
+       integer(c_int), value :: count, k
+       integer(c_long_long), intent(out) :: stats(*)
+     end function g6_npt_run
+
+     integer(c_int) function g6_npt_get(h, com, quat1, box_l, energy) bind(C, name="g6_npt_get")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), value :: h
+       real(c_double), intent(out) :: com(*), quat1(*), box_l, energy
+     end function g6_npt_get
+
+     integer(c_int) function g6_npt_energy(h, energy) bind(C, name="g6_npt_energy")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), value :: h
+       real(c_double), intent(out) :: energy
+     end function g6_npt_energy
+
+     integer(c_int) function g6_npt_features(h, x, e) bind(C, name="g6_npt_features")
+       import :: c_int, c_int64_t, c_double, c_float
+       integer(c_int64_t), value :: h
+       real(c_float), intent(out) :: x(*)
+       real(c_double), intent(out) :: e(*)
+     end function g6_npt_features
+
      ! the learned energy surrogate (arrays float32: real(c_float))
      integer(c_int) function g6_mlp_create(hm) bind(C, name="g6_mlp_create")
        import :: c_int, c_int64_t

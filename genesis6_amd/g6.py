@@ -76,6 +76,12 @@ def lib():
         L.g6_rx_trials.argtypes = [_H, _dp, _dp, C.POINTER(C.c_int32)]
         L.g6_rx_nodes.argtypes = [_H, _ip]
         L.g6_rx_exchange.argtypes = [_H, C.c_void_p, C.POINTER(C.c_int32)]
+        L.g6_npt_begin.argtypes = [_H, C.c_double, C.c_double, C.c_uint64, C.c_double,
+                                   C.c_double, C.c_double, C.c_double, C.c_double]
+        L.g6_npt_run.argtypes = [_H, C.c_int, _H, C.c_int, _i64a]
+        L.g6_npt_get.argtypes = [_H, _dp, _dp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.g6_npt_energy.argtypes = [_H, C.POINTER(C.c_double)]
+        L.g6_npt_features.argtypes = [_H, _fp, _dp]
         L.g6_mlp_create.argtypes = [C.POINTER(_H)]
         L.g6_mlp_free.argtypes = [_H]
         L.g6_mlp_set_params.argtypes = [_H, C.c_void_p, C.c_void_p]
@@ -203,6 +209,39 @@ class System:
         e = np.zeros(1)
         check(lib().g6_mc_energy(self.h, e), "g6_mc_energy")
         return float(e[0])
+
+    # ---- NPT Monte Carlo with collective moves ----
+    def npt_begin(self, temp, press_bar, seed, disp_max=0.15, rot_max_deg=8.0, dlnv_max=0.005,
+                  p_volume=0.1, voxa=None):
+        check(lib().g6_npt_begin(self.h, temp, press_bar, seed, disp_max, rot_max_deg, dlnv_max,
+                                 p_volume, voxa if voxa else self.rc), "g6_npt_begin")
+
+    def npt_run(self, count, mlp=None, K=16):
+        """count plain moves (mlp None) or delayed-acceptance rounds of K
+        proposals; returns the cumulative counters (dict)."""
+        st = np.zeros(9, np.int64)
+        check(lib().g6_npt_run(self.h, count, mlp.h if mlp is not None else 0, K, st),
+              "g6_npt_run")
+        keys = ["proposals", "accepted", "volume_proposals", "volume_accepted",
+                "surrogate_passes", "exact_evaluations", "rounds", "void", "next_index"]
+        return dict(zip(keys, (int(v) for v in st)))
+
+    def npt_state(self):
+        n = self.n
+        com = np.zeros(3 * n); q = np.zeros(4 * n); L = C.c_double(0); U = C.c_double(0)
+        check(lib().g6_npt_get(self.h, com, q, C.byref(L), C.byref(U)), "g6_npt_get")
+        return com.reshape(3, n).T.copy(), q.reshape(4, n).copy(), L.value, U.value
+
+    def npt_energy(self):
+        U = C.c_double(0)
+        check(lib().g6_npt_energy(self.h, C.byref(U)), "g6_npt_energy")
+        return U.value
+
+    def npt_features(self):
+        n = self.n
+        x = np.zeros(32 * n, np.float32); e = np.zeros(n)
+        check(lib().g6_npt_features(self.h, x, e), "g6_npt_features")
+        return x.reshape(n, 32), e
 
     def eval_forces(self):
         """Molecule forces (n,3), body torques (n,3) and the thermo record of

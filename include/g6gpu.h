@@ -209,6 +209,36 @@ int g6_rx_nodes(g6_rx hx, int32_t *node_in_charge);
  * g6_set_temperature(1/beta) on every replica whose beta changed. */
 int g6_rx_exchange(g6_rx hx, const double *thermo, int32_t *naccept);
 
+/* ---- NPT Monte Carlo with collective moves (no reference counterpart: the
+ * reference's MC has single-molecule NVT moves; the energy is PairInfo,
+ * PairInfo.F90:12-46, as g6_mc_pairinfo) ------------------------------- */
+/* Starts an NPT run on the handle's current configuration (COMs and order-1
+ * quaternions of the MD state, box L of g6_init): temp (K), press_bar,
+ * seed of the proposal stream (Philox4x32-10), block moves of every molecule
+ * of one Address_Assign cell (ndiv = int(L/voxa)) displaced within a cube
+ * of half-diagonal disp_max (A) and turned by at most rot_max_deg, volume
+ * moves ln V +- dlnv_max (<= 0.03) with probability p_volume.  n <= 4096,
+ * TIP4P or TIP5P, rc < L (1 - dlnv_max) / 2. */
+int g6_npt_begin(g6_handle h, double temp, double press_bar, uint64_t seed, double disp_max,
+                 double rot_max_deg, double dlnv_max, double p_volume, double voxa);
+/* hm = 0: count plain moves, each decided on the exact FP64 energy.
+ * hm = a g6_mlp: count delayed-acceptance rounds of K (1..64) proposals
+ * screened by the network (per-molecule energies of 32 features within
+ * 6 A), the first that passes decided on the exact energy.
+ * stats[9] (cumulative): proposals, accepted, volume proposals, volume
+ * accepted, surrogate passes, exact evaluations, rounds, void proposals
+ * (a block of more than 96 molecules: rejected), next proposal index.
+ * G6_ECAPACITY if a proposal was void or a molecule had more than 128
+ * neighbours within 6 A. */
+int g6_npt_run(g6_handle h, int count, int64_t hm, int K, long long *stats);
+/* com[3n], quat1[4n], box edge, PairInfo energy sum (internal units) */
+int g6_npt_get(g6_handle h, double *com, double *quat1, double *box_l, double *energy);
+/* the PairInfo sum recomputed from scratch on the current configuration */
+int g6_npt_energy(g6_handle h, double *energy);
+/* features x[n][32] (float) of the current configuration and the exact
+ * per-molecule energies e[n] = (1/2) sum_j pinfo(i, j) */
+int g6_npt_features(g6_handle h, float *x, double *e);
+
 /* ---- learned energy surrogate (no reference counterpart) ---------------
  * A per-molecule MLP 32 -> 64 -> 64 -> 1 with SiLU: BF16 weights on the
  * matrix cores, FP32 accumulation, FP32 master weights.  params[6337] =

@@ -1107,3 +1107,264 @@ long g6o_mc_trials(int model, int n, double L, double rc, double taper, double t
   free(x); free(y); free(z); free(t); free(intra); free(enew); free(inl);
   return nacc;
 }
+
+/* ---- NPT Monte Carlo with collective moves ------------------------------
+ * No reference counterpart: the reference has single-molecule NVT moves
+ * only.  The energy is PairInfo (as g6o_mc_pairinfo); the moves, the
+ * proposal stream and the acceptance rule are the ones the device path
+ * defines (genesis6_amd/csrc/g6_npt.hip), restated here serially. */
+
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) */
+void g6o_philox10(uint32_t *c, uint32_t k0, uint32_t k1) {
+  for (int r = 0; r < 10; r++) {
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c[0], p1 = (uint64_t)0xCD9E8D57u * c[2];
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0, n1 = (uint32_t)p1;
+    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1, n3 = (uint32_t)p0;
+    c[0] = n0; c[1] = n1; c[2] = n2; c[3] = n3;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+static double npt_u01(uint32_t x) { return ((double)x + 0.5) / 4294967296.0; }
+
+typedef struct {
+  int type;
+  double a, x, uacc, u1;
+} npt_prop;
+
+typedef struct {
+  uint32_t k0, k1;
+  double pvol, dlnv, dmax, rot_e;
+} npt_cfg;
+
+/* degree-5 Taylor polynomial of exp (the volume factor of a proposal) */
+static double npt_exp_small(double y) {
+  return 1.0 + y * (1.0 + y * (0.5 + y * (1.0 / 6.0 + y * (1.0 / 24.0 + y * (1.0 / 120.0)))));
+}
+
+static npt_prop npt_make_prop(const npt_cfg *g, unsigned long long idx) {
+  uint32_t c[4] = {(uint32_t)idx, (uint32_t)(idx >> 32), 0u, 0x4E505400u};
+  g6o_philox10(c, g->k0, g->k1);
+  npt_prop p;
+  p.type = npt_u01(c[0]) < g->pvol;
+  p.uacc = npt_u01(c[2]);
+  p.u1 = npt_u01(c[3]);
+  if (p.type) {
+    p.x = g->dlnv * (2.0 * npt_u01(c[1]) - 1.0);
+    p.a = npt_exp_small(p.x / 3.0);
+  } else {
+    p.x = 0.0;
+    p.a = npt_u01(c[1]);
+  }
+  return p;
+}
+
+/* new COM and quaternion of a block member in slot k */
+static void npt_member(const npt_cfg *g, unsigned long long idx, int k, const double *c0,
+                       const double *q0, double *c1, double *q1) {
+  uint32_t c[4] = {(uint32_t)idx, (uint32_t)(idx >> 32), (uint32_t)(1 + 2 * k), 0x4E505400u};
+  uint32_t e[4] = {(uint32_t)idx, (uint32_t)(idx >> 32), (uint32_t)(2 + 2 * k), 0x4E505400u};
+  g6o_philox10(c, g->k0, g->k1);
+  g6o_philox10(e, g->k0, g->k1);
+  for (int a = 0; a < 3; a++) c1[a] = c0[a] + g->dmax * (2.0 * npt_u01(c[a]) - 1.0);
+  const double e0 = g->rot_e * (2.0 * npt_u01(c[3]) - 1.0);
+  const double e1 = g->rot_e * (2.0 * npt_u01(e[0]) - 1.0);
+  const double e2 = g->rot_e * (2.0 * npt_u01(e[1]) - 1.0);
+  const double r = 1.0 / sqrt(1.0 + e0 * e0 + e1 * e1 + e2 * e2);
+  const double w0 = r, x0 = e0 * r, y0 = e1 * r, z0 = e2 * r;
+  /* (w0, x0, y0, z0) (x) q0, then normalised */
+  double w = w0 * q0[0] - x0 * q0[1] - y0 * q0[2] - z0 * q0[3];
+  double x = w0 * q0[1] + x0 * q0[0] + y0 * q0[3] - z0 * q0[2];
+  double y = w0 * q0[2] - x0 * q0[3] + y0 * q0[0] + z0 * q0[1];
+  double z = w0 * q0[3] + x0 * q0[2] - y0 * q0[1] + z0 * q0[0];
+  const double rn = 1.0 / sqrt(w * w + x * x + y * y + z * z);
+  q1[0] = w * rn; q1[1] = x * rn; q1[2] = y * rn; q1[3] = z * rn;
+}
+
+/* Address_Assign cell (dnint binning of the wrapped COM) */
+static int npt_cell(const double *c, double L, int nd) {
+  const double invL = 1.0 / L, cellinv = 1.0 / (L / nd);
+  int jc[3];
+  for (int a = 0; a < 3; a++) {
+    double d = c[a];
+    d = d - round(d * invL) * L;
+    int j = (int)round(d * cellinv);
+    if (j < 0) j += nd;
+    j = ((j % nd) + nd) % nd;
+    jc[a] = j;
+  }
+  return (jc[2] * nd + jc[1]) * nd + jc[0];
+}
+
+/* sites of one molecule (rigid_setmatrix + site placement) */
+static void npt_place(const g6o_model *m, const double *q, const double *c, double *x, double *y,
+                      double *z) {
+  const double qa = q[0], qb = q[1], qc = q[2], qd = q[3];
+  const double t0 = (qa * qa + qb * qb - (qc * qc + qd * qd)), t1 = -2.0 * (qa * qd + qb * qc),
+               t2 = 2.0 * (qb * qd - qa * qc), t3 = 2.0 * (qa * qd - qb * qc),
+               t4 = qa * qa + qc * qc - (qb * qb + qd * qd), t5 = -2.0 * (qa * qb + qc * qd),
+               t6 = 2.0 * (qa * qc + qb * qd), t7 = 2.0 * (qa * qb - qc * qd),
+               t8 = qa * qa + qd * qd - (qb * qb + qc * qc);
+  for (int s = 0; s < m->nsite; s++) {
+    x[s] = c[0] + (t0 * m->bx[s] + t1 * m->by[s] + t2 * m->bz[s]);
+    y[s] = c[1] + (t3 * m->bx[s] + t4 * m->by[s] + t5 * m->bz[s]);
+    z[s] = c[2] + (t6 * m->bx[s] + t7 * m->by[s] + t8 * m->bz[s]);
+  }
+}
+
+/* PairInfo pair energy, lower index first (sites x..[i*ns + s], COMs com[3i + a]) */
+static double npt_pair(const g6o_model *m, const stdterm *st, int nt, const double *com,
+                       const double *x, const double *y, const double *z, int i, int j,
+                       double L, double rc, double taper) {
+  const int lo = i < j ? i : j, hi = i < j ? j : i, ns = m->nsite;
+  double d[3], o[3];
+  for (int a = 0; a < 3; a++) {
+    d[a] = com[3 * lo + a] - com[3 * hi + a];
+    o[a] = round(d[a] * (1.0 / L)) * L;
+    d[a] = d[a] - o[a];
+  }
+  const double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  if (!(dd < rc * rc)) return 0.0;
+  double hh, dh;
+  g6o_cutoff_func(rc - taper, rc, dd, &hh, &dh);
+  return std_pair(st, nt, ns, x, y, z, lo, x + hi * ns, y + hi * ns, z + hi * ns, o[0], o[1],
+                  o[2], hh);
+}
+
+/* nmoves plain NPT moves from proposal index *index.  com[n][3], quat[4][n]
+ * (order 1), *L and *U (the PairInfo sum) are updated; accepted[nmoves];
+ * counts[5] += proposals, accepted, volume proposals, volume accepted,
+ * void block moves (more than 96 molecules). */
+void g6o_npt_run(int model, int n, double rc, double taper, double voxa, double temp,
+                 double press_bar, uint64_t seed, double disp_max, double rot_max_deg,
+                 double dlnv_max, double p_volume, double *L, double *com, double *quat,
+                 double *U, long long *index, int nmoves, long long *counts, int *accepted) {
+  g6o_model m;
+  g6o_model_get(model, &m);
+  const int ns = m.nsite;
+  stdterm st[G6O_MAXSITE * G6O_MAXSITE];
+  const int nt = std_terms(&m, st);
+  npt_cfg g;
+  g.k0 = (uint32_t)seed;
+  g.k1 = (uint32_t)(seed >> 32);
+  g.pvol = p_volume;
+  g.dlnv = dlnv_max;
+  g.dmax = disp_max / sqrt(3.0);
+  g.rot_e = tan(0.5 * rot_max_deg * 3.14159265358979323846 / 180.0) / sqrt(3.0);
+  const double beta = 1.0 / temp, press_k = press_bar * 1e5 * 1e-30 / 1.380066e-23;
+  double *x = malloc(sizeof(double) * n * ns), *y = malloc(sizeof(double) * n * ns),
+         *z = malloc(sizeof(double) * n * ns);
+  double *x2 = malloc(sizeof(double) * n * ns), *y2 = malloc(sizeof(double) * n * ns),
+         *z2 = malloc(sizeof(double) * n * ns);
+  double *com2 = malloc(sizeof(double) * 3 * n), *quat2 = malloc(sizeof(double) * 4 * n);
+  double *ep = malloc(sizeof(double) * n * n), *ep2 = malloc(sizeof(double) * n * n);
+  int *slot = malloc(sizeof(int) * n), *mem = malloc(sizeof(int) * n);
+  double q[4], c[3];
+  for (int i = 0; i < n; i++) {
+    for (int a = 0; a < 4; a++) q[a] = quat[a * n + i];
+    npt_place(&m, q, com + 3 * i, x + i * ns, y + i * ns, z + i * ns);
+  }
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < n; j++)
+      ep[(long)i * n + j] = i == j ? 0.0 : npt_pair(&m, st, nt, com, x, y, z, i, j, *L, rc, taper);
+  for (int mv = 0; mv < nmoves; mv++) {
+    const unsigned long long idx = (unsigned long long)(*index)++;
+    const npt_prop p = npt_make_prop(&g, idx);
+    int acc = 0;
+    counts[0]++;
+    if (p.type) {
+      counts[2]++;
+      const double Lp = *L * p.a;
+      for (int k = 0; k < 3 * n; k++) com2[k] = com[k] * p.a;
+      for (int i = 0; i < n; i++) {
+        for (int a = 0; a < 4; a++) q[a] = quat[a * n + i];
+        npt_place(&m, q, com2 + 3 * i, x2 + i * ns, y2 + i * ns, z2 + i * ns);
+      }
+      double up = 0.0;
+      for (int i = 0; i < n; i++)
+        for (int j = 0; j < n; j++) {
+          const double e = i == j ? 0.0 : npt_pair(&m, st, nt, com2, x2, y2, z2, i, j, Lp, rc, taper);
+          ep2[(long)i * n + j] = e;
+          if (j > i) up = up + e;
+        }
+      const double du = up - *U, dv = Lp * Lp * Lp - *L * *L * *L;
+      const double arg = -(du * I2J * 0.120273 + press_k * dv) * beta + (double)(n + 1) * p.x;
+      acc = arg >= 0.0 || p.uacc < exp(arg);
+      if (acc) {
+        counts[3]++;
+        *L = Lp;
+        *U = up;
+        memcpy(com, com2, sizeof(double) * 3 * n);
+        memcpy(x, x2, sizeof(double) * n * ns);
+        memcpy(y, y2, sizeof(double) * n * ns);
+        memcpy(z, z2, sizeof(double) * n * ns);
+        memcpy(ep, ep2, sizeof(double) * n * n);
+      }
+    } else {
+      int nd = (int)(*L / voxa);
+      if (nd < 1) nd = 1;
+      const int ncell = nd * nd * nd;
+      int cc = (int)(p.a * (double)ncell);
+      if (cc > ncell - 1) cc = ncell - 1;
+      int cnt = 0;
+      for (int i = 0; i < n; i++) {
+        slot[i] = -1;
+        if (npt_cell(com + 3 * i, *L, nd) == cc) {
+          slot[i] = cnt;
+          mem[cnt++] = i;
+        }
+      }
+      if (cnt > 96) {
+        counts[4]++;
+      } else {
+        memcpy(com2, com, sizeof(double) * 3 * n);
+        memcpy(quat2, quat, sizeof(double) * 4 * n);
+        memcpy(x2, x, sizeof(double) * n * ns);
+        memcpy(y2, y, sizeof(double) * n * ns);
+        memcpy(z2, z, sizeof(double) * n * ns);
+        for (int k = 0; k < cnt; k++) {
+          const int i = mem[k];
+          double q1[4];
+          for (int a = 0; a < 4; a++) q[a] = quat[a * n + i];
+          npt_member(&g, idx, k, com + 3 * i, q, c, q1);
+          for (int a = 0; a < 3; a++) com2[3 * i + a] = c[a];
+          for (int a = 0; a < 4; a++) quat2[a * n + i] = q1[a];
+          npt_place(&m, q1, c, x2 + i * ns, y2 + i * ns, z2 + i * ns);
+        }
+        double du = 0.0;
+        for (int k = 0; k < cnt; k++) {
+          const int i = mem[k];
+          for (int j = 0; j < n; j++) {
+            if (j == i) continue;
+            const double e = npt_pair(&m, st, nt, com2, x2, y2, z2, i, j, *L, rc, taper);
+            ep2[(long)k * n + j] = e;
+            if (slot[j] < 0 || j > i) du = du + (e - ep[(long)i * n + j]);
+          }
+        }
+        const double arg = -(du * I2J * 0.120273) * beta;
+        acc = arg >= 0.0 || p.uacc < exp(arg);
+        if (acc) {
+          *U = *U + du;
+          memcpy(com, com2, sizeof(double) * 3 * n);
+          memcpy(quat, quat2, sizeof(double) * 4 * n);
+          memcpy(x, x2, sizeof(double) * n * ns);
+          memcpy(y, y2, sizeof(double) * n * ns);
+          memcpy(z, z2, sizeof(double) * n * ns);
+          for (int k = 0; k < cnt; k++) {
+            const int i = mem[k];
+            for (int j = 0; j < n; j++) {
+              if (j == i) continue;
+              ep[(long)i * n + j] = ep2[(long)k * n + j];
+              ep[(long)j * n + i] = ep2[(long)k * n + j];
+            }
+          }
+        }
+      }
+    }
+    counts[1] += acc;
+    accepted[mv] = acc;
+  }
+  free(x); free(y); free(z); free(x2); free(y2); free(z2);
+  free(com2); free(quat2); free(ep); free(ep2); free(slot); free(mem);
+}

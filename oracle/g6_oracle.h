@@ -142,7 +142,16 @@ long g6o_mc_trials(int model, int n, double L, double rc, double taper, double t
                    double *com, double *quat, double *delta, double *rtr, double *ep,
                    double *epsum, const double *u, long *pos, int ntrial);
 
+/* NPT Monte Carlo with collective moves (no reference counterpart; the
+ * device path's definition, restated serially) */
+void g6o_philox10(uint32_t *c, uint32_t k0, uint32_t k1);
+void g6o_npt_run(int model, int n, double rc, double taper, double voxa, double temp,
+                 double press_bar, uint64_t seed, double disp_max, double rot_max_deg,
+                 double dlnv_max, double p_volume, double *L, double *com, double *quat,
+                 double *U, long long *index, int nmoves, long long *counts, int *accepted);
+
 #ifdef __cplusplus
 }
 #endif
+
 #endif

@@ -109,6 +109,13 @@ def _declare(L):
                                 C.c_double, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                 C.POINTER(C.c_long), C.c_int]
     L.g6o_mc_trials.restype = C.c_long
+    L.g6o_philox10.argtypes = [C.POINTER(C.c_uint32), C.c_uint32, C.c_uint32]
+    L.g6o_npt_run.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                              C.c_double, C.c_double, C.c_uint64, C.c_double, C.c_double,
+                              C.c_double, C.c_double, C.POINTER(C.c_double), _dp, _dp,
+                              C.POINTER(C.c_double), C.POINTER(C.c_longlong), C.c_int,
+                              np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"),
+                              _ip]
 
 
 def model(mid):
@@ -375,3 +382,41 @@ class RxRun:
                 self.nic[j - 1], self.nic[jj - 1] = n1, n0
                 acc += 1
         return np.array(beta), acc
+
+
+def philox10(ctr, key):
+    """Philox4x32-10 of a 4-word counter and a 2-word key."""
+    c = (C.c_uint32 * 4)(*[int(v) & 0xFFFFFFFF for v in ctr])
+    lib().g6o_philox10(c, int(key[0]) & 0xFFFFFFFF, int(key[1]) & 0xFFFFFFFF)
+    return [int(v) for v in c]
+
+
+class NptRun:
+    """The NPT chain of plain moves (volume and block) on the oracle, from a
+    configuration com (n,3), quat (4,n) of order 1 in a box L."""
+
+    def __init__(self, mid, com, quat, L, rc, taper, voxa, temp, press_bar, seed, disp_max,
+                 rot_max_deg, dlnv_max, p_volume, energy=None):
+        self.mid, self.rc, self.taper, self.voxa = mid, rc, taper, voxa
+        self.temp, self.press, self.seed = temp, press_bar, seed
+        self.disp, self.rot, self.dlnv, self.pvol = disp_max, rot_max_deg, dlnv_max, p_volume
+        self.com = np.array(com, np.float64)
+        self.quat = np.array(quat, np.float64)
+        self.L = float(L)
+        if energy is None:
+            _, energy = mc_pairinfo(mid, self.com, self.quat, self.L, rc, taper)
+        self.U = float(energy)
+        self.index = 0
+        self.counts = np.zeros(5, np.int64)
+
+    def run(self, nmoves):
+        n = len(self.com)
+        com = np.ascontiguousarray(self.com)
+        quat = np.ascontiguousarray(self.quat)
+        Lc, Uc, ic = C.c_double(self.L), C.c_double(self.U), C.c_longlong(self.index)
+        acc = np.zeros(nmoves, np.int32)
+        lib().g6o_npt_run(self.mid, n, self.rc, self.taper, self.voxa, self.temp, self.press,
+                          self.seed, self.disp, self.rot, self.dlnv, self.pvol, C.byref(Lc),
+                          com, quat, C.byref(Uc), C.byref(ic), nmoves, self.counts, acc)
+        self.com, self.quat, self.L, self.U, self.index = com, quat, Lc.value, Uc.value, ic.value
+        return acc

@@ -1,0 +1,95 @@
+"""NPT Monte Carlo on the GPU against the oracle: the same proposals,
+decisions and configurations bit for bit, the energy bookkeeping, the
+surrogate's features, and delayed acceptance reduced to plain MC."""
+import numpy as np
+import pytest
+
+from genesis6_amd import g6
+from oracle import npt_oracle as NO
+from oracle import pyoracle as O
+from tests.golden.configs import lattice_start
+
+pytestmark = pytest.mark.gpu
+
+RC, TAPER, VOXA = 7.0, 0.5, 5.0
+
+
+def box(n, rho):
+    return (n * 18.015 / (6.0221e23 * rho) * 1e24) ** (1.0 / 3.0)
+
+
+def system(n, rho, seed, steps=0):
+    L = box(n, rho)
+    s = g6.System(g6.TIP4P, n, L, RC, TAPER, 0.002, voxa=VOXA, prec=g6.FP64)
+    com, q = lattice_start(n, L, seed)
+    s.upload(com, q)
+    if steps:
+        s.set_nose(250.0, 250.0 * 2500.0)
+        s.vreset(250.0, seed)
+        s.step(steps)
+    c, q5, _, _ = s.download()
+    return s, c, q5[0].copy(), L
+
+
+@pytest.mark.parametrize("pvol", [0.0, 0.3])
+def test_plain_moves_match_oracle(pvol):
+    n = 128
+    s, com, q, L = system(n, 1.0, 41, steps=40)
+    print("npt n=%d L=%.4f pvol=%g" % (n, L, pvol), flush=True)
+    args = (250.0, 1.0, 1234567, 0.15, 8.0, 0.005, pvol, VOXA)
+    s.npt_begin(*args)
+    _, _, L0, U0 = s.npt_state()
+    orc = O.NptRun(O.TIP4P, com, q, L, RC, TAPER, VOXA, 250.0, 1.0, 1234567, 0.15, 8.0,
+                   0.005, pvol, energy=U0)
+    total = 0
+    for rnd in range(3):
+        st = s.npt_run(150)
+        total += int(orc.run(150).sum())
+        c, qq, Lg, Ug = s.npt_state()
+        print("round", rnd, st, flush=True)
+        assert st["accepted"] == total == orc.counts[1]
+        assert st["proposals"] == orc.counts[0] and st["volume_proposals"] == orc.counts[2]
+        assert st["volume_accepted"] == orc.counts[3]
+        assert Lg == orc.L
+        assert np.array_equal(c, orc.com) and np.array_equal(qq, orc.quat)
+        assert abs(Ug - orc.U) < 1e-10 * abs(orc.U)
+    assert st["accepted"] > 0
+    # the incremental energy against a fresh PairInfo sum
+    fresh = s.npt_energy()
+    assert abs(Ug - fresh) < 1e-10 * abs(fresh)
+
+
+def test_features_and_molecule_energies_match_oracle():
+    n = 128
+    s, com, q, L = system(n, 1.0, 43, steps=30)
+    s.npt_begin(250.0, 1.0, 5, 0.15, 8.0, 0.005, 0.0, VOXA)
+    x, e = s.npt_features()
+    # site positions of the oracle's placement
+    sites, _, _ = O.setsites(O.TIP4P, com, q)
+    xs, ys, zs = (sites[:, k].reshape(n, 5) for k in range(3))
+    Os = np.stack([xs[:, 0], ys[:, 0], zs[:, 0]], 1)
+    Hs = np.stack([np.stack([xs[:, h], ys[:, h], zs[:, h]], 1) for h in (2, 3)], 1)
+    xo = NO.features(Os, Hs, L)
+    assert np.abs(x - xo).max() < 1e-4 * max(1.0, np.abs(xo).max())
+    ep, _ = O.mc_pairinfo(O.TIP4P, com, q, L, RC, TAPER)
+    eo = 0.5 * ep.sum(1)
+    assert np.abs(e - eo).max() < 1e-10 * np.abs(eo).max()
+
+
+def test_delayed_acceptance_with_a_zero_network_is_plain_mc():
+    """With the network's output identically zero every block proposal
+    passes the screen, so K = 1 rounds make exactly the plain chain's
+    decisions on the same proposals (block moves only)."""
+    n = 128
+    s1, _, _, _ = system(n, 1.0, 47, steps=20)
+    s2, _, _, _ = system(n, 1.0, 47, steps=20)
+    for s in (s1, s2):
+        s.npt_begin(250.0, 1.0, 99, 0.15, 8.0, 0.005, 0.0, VOXA)
+    net = g6.Mlp(np.zeros(g6.MLP_NPARAM, np.float32))
+    a = s1.npt_run(200)
+    b = s2.npt_run(200, mlp=net, K=1)
+    print(a, b, flush=True)
+    assert b["rounds"] == 200 and b["surrogate_passes"] == 200 and b["exact_evaluations"] == 200
+    assert a["accepted"] == b["accepted"] and a["next_index"] == b["next_index"]
+    for u, v in zip(s1.npt_state(), s2.npt_state()):
+        assert np.array_equal(u, v)
