@@ -48,6 +48,12 @@ constexpr int kNptMaxBlock = 96;   // molecules of one block move
 constexpr int kNptMaxK = 64;       // proposals per delayed-acceptance round
 constexpr int kDescMaxNb = 128;    // neighbours within the descriptor radius
 constexpr float kDescRc = 6.0f;
+// The neighbour list of a round: O-O pairs within 6.5 A of the current
+// configuration.  A proposal moves an O by less than 0.25 A (block: 0.15 A
+// plus the turn of the 0.07 A O offset; volume: 0.03% of 13 A), so every
+// pair within 6 A after a proposal is on the list.
+constexpr int kNbl = 96;
+constexpr double kNblR = 6.5;
 constexpr int kCnt = 12;
 // counters: 0 proposals, 1 accepted, 2 volume proposals, 3 volume accepted,
 // 4 surrogate passes, 5 exact evaluations, 6 rounds, 7 void proposals
@@ -76,6 +82,9 @@ struct NptArgs {
   double *des;          // [K] surrogate energy changes
   int *vflag;           // [K] void proposals
   int *derr;            // neighbour overflow of the descriptors
+  int *nbl, *nbn;       // [n][kNbl] O-O neighbours within kNblR of the current
+                        // configuration (ascending j), [n] their counts
+  int *arow, *naff;     // [K][n] molecules whose features a proposal changes, [K] counts
 };
 
 // ---- the proposal stream ---------------------------------------------------
@@ -486,8 +495,8 @@ __global__ __launch_bounds__(256) void k_npt_commit(NptArgs A, int da) {
       }
     }
     if (da) {
-      const float *y = A.ys + (size_t)A.ist[2] * n;
-      for (size_t t = t0; t < (size_t)n; t += stride) A.ycur[t] = y[t];
+      const size_t k = (size_t)A.ist[2];
+      for (size_t t = t0; t < (size_t)A.naff[k]; t += stride) A.ycur[A.arow[k * n + t]] = A.ys[k * n + t];
     }
   }
   if (t0 == 0) A.cnt[8] += da ? A.K : 1;
@@ -510,17 +519,28 @@ __device__ inline void put_sites(const NptArgs &A, int k, int i, const double *x
   h[5] = (float)z[A.h1];
 }
 
-// configuration of proposal k = blockIdx.x of the round (k_da_prep) or the
-// current one (k = 0, current != 0)
+// the affected rows of a proposal: every molecule, in index order
+__device__ inline void all_rows(const NptArgs &A, int k) {
+  for (int i = threadIdx.x; i < A.n; i += blockDim.x) A.arow[(size_t)k * A.n + i] = i;
+  if (threadIdx.x == 0) A.naff[k] = A.n;
+}
+
+// configuration of proposal k = blockIdx.x of the round, or the current one
+// (k = 0, current != 0): FP32 O and H sites, box edge, and the molecules
+// whose features it changes (a block: its members and their listed
+// neighbours; a volume move: every molecule), in index order
 __global__ __launch_bounds__(1024) void k_da_prep(NptArgs A, int current) {
   __shared__ int s_mem[kNptMaxBlock];
   __shared__ short s_slot[4096];
+  __shared__ unsigned char s_mark[4096];
+  __shared__ int s_wc[16], s_run;
   const int k = blockIdx.x, n = A.n, ns = A.ns, tid = threadIdx.x;
   const double L = A.st[0];
   if (current) {
     for (int i = tid; i < n; i += blockDim.x)
       put_sites(A, 0, i, A.sx + (size_t)i * ns, A.sy + (size_t)i * ns, A.sz + (size_t)i * ns);
     if (tid == 0) A.fL[0] = (float)L;
+    all_rows(A, 0);
     return;
   }
   const unsigned long long idx = (unsigned long long)A.cnt[8] + k;
@@ -536,6 +556,7 @@ __global__ __launch_bounds__(1024) void k_da_prep(NptArgs A, int current) {
       A.fL[k] = (float)(L * p.a);
       A.vflag[k] = 0;
     }
+    all_rows(A, k);
     return;
   }
   const int cnt = block_members(A, p, L, s_mem, s_slot, kNptMaxBlock);
@@ -543,19 +564,81 @@ __global__ __launch_bounds__(1024) void k_da_prep(NptArgs A, int current) {
     A.fL[k] = (float)L;
     A.vflag[k] = cnt > kNptMaxBlock ? 1 : 0;
   }
-  if (cnt > kNptMaxBlock) return;
+  if (cnt > kNptMaxBlock) {
+    if (tid == 0) A.naff[k] = 0;
+    return;
+  }
+  for (int i = tid; i < n; i += blockDim.x) s_mark[i] = 0;
+  __syncthreads();
+  for (int t = tid; t < cnt * (kNbl + 1); t += blockDim.x) {
+    const int m = s_mem[t / (kNbl + 1)], e = t % (kNbl + 1);
+    if (e == kNbl) s_mark[m] = 1;
+    else if (e < A.nbn[m]) s_mark[A.nbl[(size_t)m * kNbl + e]] = 1;
+  }
   for (int i = tid; i < n; i += blockDim.x) {
-    const int s = s_slot[i];
-    if (s < 0) {
+    const int sl = s_slot[i];
+    if (sl < 0) {
       put_sites(A, k, i, A.sx + (size_t)i * ns, A.sy + (size_t)i * ns, A.sz + (size_t)i * ns);
     } else {
       double d[3], qr[4], q[4];
-      member_move(A, idx, s, d, qr);
+      member_move(A, idx, sl, d, qr);
       turn(qr, A.quat[i], A.quat[n + i], A.quat[2 * n + i], A.quat[3 * n + i], q);
       place_sites(A, q[0], q[1], q[2], q[3], A.com[i] + d[0], A.com[n + i] + d[1],
                   A.com[2 * n + i] + d[2], X, Y, Z);
       put_sites(A, k, i, X, Y, Z);
     }
+  }
+  // compaction of the marks in index order
+  const int lane = tid & 63, wid = tid >> 6, nw = blockDim.x >> 6;
+  if (tid == 0) s_run = 0;
+  __syncthreads();
+  for (int base = 0; base < n; base += blockDim.x) {
+    const int i = base + tid;
+    const bool in = i < n && s_mark[i];
+    const unsigned long long mk = __ballot(in);
+    if (lane == 0) s_wc[wid] = __popcll(mk);
+    __syncthreads();
+    int off = s_run, tot = 0;
+    for (int w = 0; w < nw; w++) {
+      if (w < wid) off += s_wc[w];
+      tot += s_wc[w];
+    }
+    if (in) A.arow[(size_t)k * n + off + __popcll(mk & ((1ull << lane) - 1ull))] = i;
+    __syncthreads();
+    if (tid == 0) s_run = s_run + tot;
+    __syncthreads();
+  }
+  if (tid == 0) A.naff[k] = s_run;
+}
+
+// O-O neighbours of every molecule within kNblR in the current
+// configuration (FP64 minimum image), ascending j; one wave per molecule
+__global__ __launch_bounds__(256) void k_nbl(NptArgs A) {
+  const int n = A.n, ns = A.ns, lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= n) return;
+  const double L = A.st[0], invL = 1.0 / L;
+  const double ox = A.sx[(size_t)i * ns], oy = A.sy[(size_t)i * ns], oz = A.sz[(size_t)i * ns];
+  int base = 0;
+  for (int j0 = 0; j0 < n; j0 += 64) {
+    const int j = j0 + lane;
+    bool in = false;
+    if (j < n && j != i) {
+      double dx = A.sx[(size_t)j * ns] - ox, dy = A.sy[(size_t)j * ns] - oy,
+             dz = A.sz[(size_t)j * ns] - oz;
+      dx = dx - round(dx * invL) * L;
+      dy = dy - round(dy * invL) * L;
+      dz = dz - round(dz * invL) * L;
+      in = dx * dx + dy * dy + dz * dz < kNblR * kNblR;
+    }
+    const unsigned long long m = __ballot(in);
+    const int pos = base + __popcll(m & ((1ull << lane) - 1ull));
+    if (in && pos < kNbl) A.nbl[(size_t)i * kNbl + pos] = j;
+    base += __popcll(m);
+  }
+  if (lane == 0) {
+    A.nbn[i] = base < kNbl ? base : kNbl;
+    if (base > kNbl) *A.derr = 1;
   }
 }
 
@@ -574,32 +657,27 @@ __device__ inline float wave_sumf(float v) {
   return v;
 }
 
-__global__ __launch_bounds__(256) void k_desc(const float *__restrict__ fo,
-                                              const float *__restrict__ fh,
-                                              const float *__restrict__ fL, int n, long long nrow,
-                                              float *__restrict__ desc, int *derr) {
-  __shared__ float4 s_nb[4][kDescMaxNb];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+__global__ __launch_bounds__(256) void k_desc(NptArgs A, int nconf) {
+  __shared__ float4 s_nb[4][kNbl];
+  __shared__ int s_j[4][kNbl];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, n = A.n;
   const long long row = (long long)blockIdx.x * 4 + wid;
-  if (row >= nrow) return;
-  const int c = (int)(row / n), i = (int)(row % n);
-  const float *O = fo + (size_t)c * n * 3, *H = fh + (size_t)c * n * 6;
-  const float L = fL[c], iL = 1.0f / L, rc2 = kDescRc * kDescRc, pr = 3.14159265358979f / kDescRc;
+  if (row >= (long long)nconf * n) return;
+  const int c = (int)(row / n), r = (int)(row % n);
+  if (r >= A.naff[c]) return;
+  const int i = A.arow[(size_t)c * n + r];
+  const float *O = A.fo + (size_t)c * n * 3, *H = A.fh + (size_t)c * n * 6;
+  const float L = A.fL[c], iL = 1.0f / L, rc2 = kDescRc * kDescRc, pr = 3.14159265358979f / kDescRc;
   const float ox = O[i * 3], oy = O[i * 3 + 1], oz = O[i * 3 + 2];
-  float hx[2], hy[2], hz[2];
-  for (int h = 0; h < 2; h++) {
-    hx[h] = H[i * 6 + 3 * h];
-    hy[h] = H[i * 6 + 3 * h + 1];
-    hz[h] = H[i * 6 + 3 * h + 2];
-  }
-  float g[32];
-#pragma unroll
-  for (int k = 0; k < 32; k++) g[k] = 0.0f;
-  int base = 0;
-  for (int j0 = 0; j0 < n; j0 += 64) {
-    const int j = j0 + lane;
+  // the listed neighbours within Rc, compacted in list order (ascending j)
+  const int nl = A.nbn[i];
+  int cnt = 0;
+  for (int e0 = 0; e0 < nl; e0 += 64) {
+    const int e = e0 + lane;
     float dx = 0, dy = 0, dz = 0, r2 = 1e30f;
-    if (j < n && j != i) {
+    int j = 0;
+    if (e < nl) {
+      j = A.nbl[(size_t)i * kNbl + e];
       dx = O[j * 3] - ox;
       dy = O[j * 3 + 1] - oy;
       dz = O[j * 3 + 2] - oz;
@@ -609,64 +687,82 @@ __global__ __launch_bounds__(256) void k_desc(const float *__restrict__ fo,
       r2 = dx * dx + dy * dy + dz * dz;
     }
     const bool in = r2 < rc2;
-    float fc = 0.0f;
-    if (in) {
-      const float r = sqrtf(r2);
-      fc = 0.5f * (cosf(pr * r) + 1.0f);
-#pragma unroll
-      for (int k = 0; k < 8; k++) {
-        const float t = r - (2.4f + 0.5f * k);
-        g[k] += expf(-4.0f * t * t) * fc;
-      }
-      // the image of j next to i: O_j' = O_i + d; H_j' = H_j + (O_j' - O_j)
-      const float sx = ox + dx - O[j * 3], sy = oy + dy - O[j * 3 + 1], sz = oz + dz - O[j * 3 + 2];
-      const float mu[4] = {1.6f, 1.9f, 2.2f, 2.6f};
-      for (int h = 0; h < 2; h++) {
-        const float ax = ox + dx - hx[h], ay = oy + dy - hy[h], az = oz + dz - hz[h];
-        const float ra = sqrtf(ax * ax + ay * ay + az * az);
-        const float bx = H[j * 6 + 3 * h] + sx - ox, by = H[j * 6 + 3 * h + 1] + sy - oy,
-                    bz = H[j * 6 + 3 * h + 2] + sz - oz;
-        const float rb = sqrtf(bx * bx + by * by + bz * bz);
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-          const float ta = ra - mu[k], tb = rb - mu[k];
-          g[8 + k] += expf(-10.0f * ta * ta) * fc;
-          g[12 + k] += expf(-10.0f * tb * tb) * fc;
-        }
-      }
-    }
     const unsigned long long m = __ballot(in);
-    const int pos = base + __popcll(m & ((1ull << lane) - 1ull));
-    if (in && pos < kDescMaxNb) s_nb[wid][pos] = make_float4(dx, dy, dz, fc);
-    base += __popcll(m);
-  }
-  if (base > kDescMaxNb) {
-    if (lane == 0) *derr = 1;
-    base = kDescMaxNb;
+    const int pos = cnt + __popcll(m & ((1ull << lane) - 1ull));
+    if (in) {
+      const float rr = sqrtf(r2);
+      s_nb[wid][pos] = make_float4(dx, dy, dz, 0.5f * (cosf(pr * rr) + 1.0f));
+      s_j[wid][pos] = j;
+    }
+    cnt += __popcll(m);
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  for (int a = lane; a < base; a += 64) {
-    const float4 va = s_nb[wid][a];
-    const float ra2 = va.x * va.x + va.y * va.y + va.z * va.z, ra = sqrtf(ra2);
-    for (int b = a + 1; b < base; b++) {
-      const float4 vb = s_nb[wid][b];
-      const float rb2 = vb.x * vb.x + vb.y * vb.y + vb.z * vb.z;
-      const float cs = (va.x * vb.x + va.y * vb.y + va.z * vb.z) / (ra * sqrtf(rb2));
-      const float f = va.w * vb.w, e2 = ra2 + rb2;
+  float g[32];
 #pragma unroll
-      for (int e = 0; e < 2; e++) {
-        const float ex = expf(-(e ? 0.08f : 0.02f) * e2) * f;
+  for (int k = 0; k < 32; k++) g[k] = 0.0f;
+  float hx[2], hy[2], hz[2];
 #pragma unroll
-        for (int l = 0; l < 2; l++) {
-          const float p1 = l ? 1.0f - cs : 1.0f + cs, p2 = p1 * p1, p4 = p2 * p2, p8 = p4 * p4;
-          float *q = g + 16 + 8 * e + 4 * l;
-          q[0] += p1 * ex;
-          q[1] += p2 * ex;
-          q[2] += p4 * ex;
-          q[3] += p8 * ex;
-        }
+  for (int h = 0; h < 2; h++) {
+    hx[h] = H[i * 6 + 3 * h];
+    hy[h] = H[i * 6 + 3 * h + 1];
+    hz[h] = H[i * 6 + 3 * h + 2];
+  }
+  // radial and hydrogen-bond terms, one compacted neighbour per lane
+  for (int a = lane; a < cnt; a += 64) {
+    const float4 v = s_nb[wid][a];
+    const int j = s_j[wid][a];
+    const float rr = sqrtf(v.x * v.x + v.y * v.y + v.z * v.z), fc = v.w;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const float t = rr - (2.4f + 0.5f * k);
+      g[k] += expf(-4.0f * t * t) * fc;
+    }
+    // the image of j next to i: O_j' = O_i + d, H_j' = H_j + (O_j' - O_j)
+    const float sx = ox + v.x - O[j * 3], sy = oy + v.y - O[j * 3 + 1], sz = oz + v.z - O[j * 3 + 2];
+    const float mu[4] = {1.6f, 1.9f, 2.2f, 2.6f};
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const float ax = ox + v.x - hx[h], ay = oy + v.y - hy[h], az = oz + v.z - hz[h];
+      const float ra = sqrtf(ax * ax + ay * ay + az * az);
+      const float bx = H[j * 6 + 3 * h] + sx - ox, by = H[j * 6 + 3 * h + 1] + sy - oy,
+                  bz = H[j * 6 + 3 * h + 2] + sz - oz;
+      const float rb = sqrtf(bx * bx + by * by + bz * bz);
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const float ta = ra - mu[k], tb = rb - mu[k];
+        g[8 + k] += expf(-10.0f * ta * ta) * fc;
+        g[12 + k] += expf(-10.0f * tb * tb) * fc;
+      }
+    }
+  }
+  // angular terms over the pairs a < b of the compacted list, pair p on lane
+  // p mod 64 (row a of the triangle holds cnt - 1 - a pairs)
+  const int npair = cnt * (cnt - 1) / 2;
+  int a = 0, off = 0;  // row a starts at pair index off
+  for (int p = lane; p < npair; p += 64) {
+    while (p >= off + (cnt - 1 - a)) {
+      off += cnt - 1 - a;
+      a++;
+    }
+    const int b = a + 1 + (p - off);
+    const float4 va = s_nb[wid][a], vb = s_nb[wid][b];
+    const float ra2 = va.x * va.x + va.y * va.y + va.z * va.z;
+    const float rb2 = vb.x * vb.x + vb.y * vb.y + vb.z * vb.z;
+    const float cs = (va.x * vb.x + va.y * vb.y + va.z * vb.z) / (sqrtf(ra2) * sqrtf(rb2));
+    const float f = va.w * vb.w, e2 = ra2 + rb2;
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+      const float ex = expf(-(e ? 0.08f : 0.02f) * e2) * f;
+#pragma unroll
+      for (int l = 0; l < 2; l++) {
+        const float p1 = l ? 1.0f - cs : 1.0f + cs, p2 = p1 * p1, p4 = p2 * p2, p8 = p4 * p4;
+        float *q = g + 16 + 8 * e + 4 * l;
+        q[0] += p1 * ex;
+        q[1] += p2 * ex;
+        q[2] += p4 * ex;
+        q[3] += p8 * ex;
       }
     }
   }
@@ -678,7 +774,7 @@ __global__ __launch_bounds__(256) void k_desc(const float *__restrict__ fo,
     g[k] = v;
   }
   if (lane == 0) {
-    float *out = desc + (size_t)row * 32;
+    float *out = A.desc + (size_t)row * 32;
 #pragma unroll
     for (int k = 0; k < 32; k++) out[k] = g[k];
   }
@@ -691,8 +787,9 @@ __global__ __launch_bounds__(256) void k_da_select(NptArgs A) {
   for (int k = 0; k < A.K; k++) {
     __shared__ double s[256];
     double a = 0.0;
-    for (int i = threadIdx.x; i < n; i += 256)
-      a = a + ((double)A.ys[(size_t)k * n + i] - (double)A.ycur[i]);
+    const int m = A.naff[k];
+    for (int r = threadIdx.x; r < m; r += 256)
+      a = a + ((double)A.ys[(size_t)k * n + r] - (double)A.ycur[A.arow[(size_t)k * n + r]]);
     s[threadIdx.x] = a;
     __syncthreads();
     for (int h = 128; h >= 1; h >>= 1) {
@@ -859,6 +956,8 @@ static int npt_alloc_k(Npt &p, int K) {
   NAL(a.ys, (size_t)K * n);
   NAL(a.des, K);
   NAL(a.vflag, K);
+  NAL(a.arow, (size_t)K * n);
+  NAL(a.naff, K);
   p.kcap = K;
   return 0;
 }
@@ -875,21 +974,27 @@ static void launch_move(System &s, Npt &p, int da) {
 template <int M>
 static void launch_round(System &s, Npt &p, Mlp &net) {
   NptArgs &a = p.a;
+  k_nbl<<<(a.n + 3) / 4, 256, 0, s.stream>>>(a);
   k_da_prep<<<a.K, 1024, 0, s.stream>>>(a, 0);
   const long long rows = (long long)a.K * a.n;
-  k_desc<<<(unsigned)((rows + 3) / 4), 256, 0, s.stream>>>(a.fo, a.fh, a.fL, a.n, rows, a.desc,
-                                                          a.derr);
+  k_desc<<<(unsigned)((rows + 3) / 4), 256, 0, s.stream>>>(a, a.K);
   mlp_launch_forward(net, rows, a.desc, nullptr, a.ys, s.stream);
   k_da_select<<<1, 256, 0, s.stream>>>(a);
   launch_move<M>(s, p, 1);
 }
 
+// features of the current configuration into desc rows 0..n-1
+static void launch_features(System &s, Npt &p) {
+  NptArgs &a = p.a;
+  k_nbl<<<(a.n + 3) / 4, 256, 0, s.stream>>>(a);
+  k_da_prep<<<1, 1024, 0, s.stream>>>(a, 1);
+  k_desc<<<(a.n + 3) / 4, 256, 0, s.stream>>>(a, 1);
+}
+
 // surrogate energies of the current configuration into ycur
 static void launch_current(System &s, Npt &p, Mlp &net) {
-  NptArgs &a = p.a;
-  k_da_prep<<<1, 1024, 0, s.stream>>>(a, 1);
-  k_desc<<<(a.n + 3) / 4, 256, 0, s.stream>>>(a.fo, a.fh, a.fL, a.n, a.n, a.desc, a.derr);
-  mlp_launch_forward(net, a.n, a.desc, nullptr, a.ycur, s.stream);
+  launch_features(s, p);
+  mlp_launch_forward(net, p.a.n, p.a.desc, nullptr, p.a.ycur, s.stream);
 }
 
 static int npt_run(System &s, Npt &p, int count, Mlp *net, int K) {
@@ -976,7 +1081,9 @@ int g6_npt_begin(g6_handle h, double temp, double press_bar, uint64_t seed, doub
   if (s->rc > 0.5 * s->L * (1.0 - dlnv_max)) return G6_EARG;  // one image within rc
   Npt &p = *npt_of(s, true);
   G6_CHECK(hipStreamSynchronize(s->stream));
-  npt_release(p);
+  // a second start on the same handle keeps the device buffers
+  const bool reuse = p.ready;
+  NptArgs keep = p.a;
   p.a = npt_args_base(*s);
   NptArgs &a = p.a;
   const size_t n = s->n, ns = s->ns;
@@ -990,34 +1097,54 @@ int g6_npt_begin(g6_handle h, double temp, double press_bar, uint64_t seed, doub
   a.rot_e = tan(0.5 * rot_max_deg * 3.14159265358979323846 / 180.0) / sqrt(3.0);
   a.key0 = (uint32_t)seed;
   a.key1 = (uint32_t)(seed >> 32);
-  NAL(a.com, 3 * n);
-  NAL(a.quat, 4 * n);
-  NAL(a.sx, n * ns);
-  NAL(a.sy, n * ns);
-  NAL(a.sz, n * ns);
-  NAL(a.pinfo, n * n);
-  NAL(a.com2, 3 * n);
-  NAL(a.sx2, n * ns);
-  NAL(a.sy2, n * ns);
-  NAL(a.sz2, n * ns);
-  NAL(a.pinfo2, n * n);
-  NAL(a.bc, 3 * kNptMaxBlock);
-  NAL(a.bq, 4 * kNptMaxBlock);
-  NAL(a.bsx, kNptMaxBlock * ns);
-  NAL(a.bsy, kNptMaxBlock * ns);
-  NAL(a.bsz, kNptMaxBlock * ns);
-  NAL(a.rows, kNptMaxBlock * n);
-  NAL(a.members, kNptMaxBlock);
-  NAL(a.slot, n);
-  NAL(a.part, n);
-  NAL(a.st, 8);
-  NAL(a.ist, 8);
-  NAL(a.cnt, kCnt);
-  NAL(a.ycur, n);
-  NAL(a.derr, 1);
-  const int rc = npt_alloc_k(p, 1);
-  if (rc) return rc;
-  a.K = 1;
+  if (reuse) {
+    const NptArgs fresh = a;
+    a = keep;  // every device pointer and the surrogate buffers
+    a.voxa = fresh.voxa;
+    a.beta = fresh.beta;
+    a.press_k = fresh.press_k;
+    a.pvol = fresh.pvol;
+    a.dlnv = fresh.dlnv;
+    a.dmax = fresh.dmax;
+    a.rot_e = fresh.rot_e;
+    a.key0 = fresh.key0;
+    a.key1 = fresh.key1;
+    a.K = 1;
+    if (p.graph) (void)hipGraphExecDestroy(p.graph);  // captured with the old arguments
+    p.graph = nullptr;
+  }
+  if (!reuse) {
+    NAL(a.com, 3 * n);
+    NAL(a.quat, 4 * n);
+    NAL(a.sx, n * ns);
+    NAL(a.sy, n * ns);
+    NAL(a.sz, n * ns);
+    NAL(a.pinfo, n * n);
+    NAL(a.com2, 3 * n);
+    NAL(a.sx2, n * ns);
+    NAL(a.sy2, n * ns);
+    NAL(a.sz2, n * ns);
+    NAL(a.pinfo2, n * n);
+    NAL(a.bc, 3 * kNptMaxBlock);
+    NAL(a.bq, 4 * kNptMaxBlock);
+    NAL(a.bsx, kNptMaxBlock * ns);
+    NAL(a.bsy, kNptMaxBlock * ns);
+    NAL(a.bsz, kNptMaxBlock * ns);
+    NAL(a.rows, kNptMaxBlock * n);
+    NAL(a.members, kNptMaxBlock);
+    NAL(a.slot, n);
+    NAL(a.part, n);
+    NAL(a.st, 8);
+    NAL(a.ist, 8);
+    NAL(a.cnt, kCnt);
+    NAL(a.ycur, n);
+    NAL(a.derr, 1);
+    NAL(a.nbl, n * kNbl);
+    NAL(a.nbn, n);
+    const int rc = npt_alloc_k(p, 1);
+    if (rc) return rc;
+    a.K = 1;
+  }
   make_std_table(s->mt, &p.st);
   if (s->model == G6_TIP4P ? !std_table_matches<G6_TIP4P>(p.st)
                            : !std_table_matches<G6_TIP5P>(p.st))
@@ -1103,8 +1230,7 @@ int g6_npt_features(g6_handle h, float *x, double *e) {
   if (!p || !p->ready) return G6_EARG;
   NptArgs &a = p->a;
   const size_t n = s->n;
-  k_da_prep<<<1, 1024, 0, s->stream>>>(a, 1);
-  k_desc<<<(s->n + 3) / 4, 256, 0, s->stream>>>(a.fo, a.fh, a.fL, s->n, s->n, a.desc, a.derr);
+  launch_features(*s, *p);
   double *de = nullptr;
   G6_CHECK(hipMalloc(&de, sizeof(double) * n));
   k_npt_mol_energy<<<(s->n + 255) / 256, 256, 0, s->stream>>>(a, de);
