@@ -151,6 +151,71 @@ def tip5p_run(steps, skin):
                 temperature_K=float(th[3]))
 
 
+def npt_run(configs, epochs, moves, rounds, K, p_volume, log):
+    """1024 TIP4P molecules, NPT Monte Carlo at 250 K and 1 bar (L = 31.289 A
+    at the start, rc 9.0 A switched over 0.5 A, FP64 PairInfo energies):
+    volume moves ln V +- 0.5%, block moves of every molecule of one
+    Address_Assign cell (ndiv 3) displaced <= 0.15 A and turned <= 8 deg.
+    Plain MC decides every proposal on the exact energy; delayed acceptance
+    screens rounds of K proposals with the surrogate, trained here on the
+    GPU on `configs` MD configurations at 240-300 K and five densities.
+    Both chains start from the same equilibrated state.  Returns the
+    exact-accepted moves per second of both."""
+    from genesis6_amd import g6, surrogate as S
+    n, L, T, P = 1024, 31.289, 250.0, 1.0
+    temps = [240.0, 250.0, 260.0, 270.0, 280.0, 290.0, 300.0]
+    scales = (0.97, 0.985, 1.0, 1.015, 1.03)
+    per = max(1, -(-configs // (len(temps) * len(scales))))
+    t0 = time.perf_counter()
+    X, E = S.md_dataset(n, L, temps, per, 20, 20261207, scales=scales)
+    t_data = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    net, losses = S.train(X, E, epochs=epochs, batch=4096, lr=2e-3)
+    t_train = time.perf_counter() - t0
+    rows = len(X)
+    del X, E
+    log("npt: %d configurations (%.1f s), trained %d epochs (%.1f s), loss %.3f -> %.3f"
+        % (rows // n, t_data, epochs, t_train, losses[0], losses[-1]))
+    s = g6.System(g6.TIP4P, n, L, RC, TAPER, DT, prec=g6.FP32, skin=1.0)
+    com, q = g6.lattice_start(n, L, 20261202 + 5)
+    s.upload(com, q)
+    s.set_nose(T, T * 2500.0)
+    s.vreset(T, 20261202 + 5)
+    s.step(3000)
+
+    def start():
+        s.npt_begin(T, P, 20261207, 0.15, 8.0, 0.005, p_volume, 9.0)
+        s.npt_run(2000)  # the same equilibration for both chains
+        return s.npt_run(0)
+
+    a = start()
+    t0 = time.perf_counter()
+    b = s.npt_run(moves)
+    el_p = time.perf_counter() - t0
+    plain = {k: b[k] - a[k] for k in b}
+    a = start()
+    s.npt_run(2, mlp=net, K=K)   # graph capture
+    a = s.npt_run(0)
+    t0 = time.perf_counter()
+    b = s.npt_run(rounds, mlp=net, K=K)
+    el_d = time.perf_counter() - t0
+    da = {k: b[k] - a[k] for k in b}
+    _, _, Lf, Uf = s.npt_state()
+    s.close()
+    net.close()
+    vp = plain["accepted"] / el_p
+    vd = da["accepted"] / el_d
+    return {"metric": "exact-accepted MC moves/s", "unit": "moves/s",
+            "plain": vp, "delayed_acceptance": vd, "speedup": vd / vp,
+            "K": K, "p_volume": p_volume,
+            "plain_counts": dict(plain, seconds=el_p),
+            "delayed_acceptance_counts": dict(da, seconds=el_d),
+            "training": {"configurations": rows // n, "rows": rows, "epochs": epochs,
+                         "loss_first_epoch": float(losses[0]), "loss_last_epoch": float(losses[-1]),
+                         "data_seconds": t_data, "train_seconds": t_train},
+            "final_box_A": Lf}
+
+
 def traffic_from_profile():
     """HBM bytes per force-kernel launch from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if present."""
@@ -175,6 +240,15 @@ def main():
                     help="timed 500-step intervals of the 64-replica ladder (0: skip)")
     ap.add_argument("--tip5p-steps", type=int, default=500,
                     help="timed steps of the TIP5P-65536 system (0: skip)")
+    ap.add_argument("--npt-configs", type=int, default=20000,
+                    help="MD configurations the surrogate is trained on (0: skip NPT MC)")
+    ap.add_argument("--npt-epochs", type=int, default=10)
+    ap.add_argument("--npt-moves", type=int, default=20000, help="timed plain MC moves")
+    ap.add_argument("--npt-rounds", type=int, default=2000,
+                    help="timed delayed-acceptance rounds")
+    ap.add_argument("--npt-k", type=int, default=32, help="proposals per round")
+    ap.add_argument("--npt-pvol", type=float, default=1.0 / 28.0,
+                    help="volume-move probability (1/28: one per sweep of the 27 cell blocks)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run five device-to-device copies of this many MiB and exit "
                          "(a plain copy kernel of known size, which calibrates the "
@@ -228,6 +302,13 @@ def main():
         tip5p = tip5p_run(a.tip5p_steps, a.skin)
         print("bench: TIP5P-65536 %.4g molecule-steps/s" % tip5p["value"], file=sys.stderr,
               flush=True)
+    npt = None
+    if a.npt_configs > 0:
+        npt = npt_run(a.npt_configs, a.npt_epochs, a.npt_moves, a.npt_rounds, a.npt_k,
+                      a.npt_pvol, lambda m: print("bench: " + m, file=sys.stderr, flush=True))
+        print("bench: NPT MC exact-accepted moves/s plain %.0f, delayed acceptance %.0f (%.2fx)"
+              % (npt["plain"], npt["delayed_acceptance"], npt["speedup"]), file=sys.stderr,
+              flush=True)
     cpu = None
     if not a.no_cpu:
         v, k, e = cpu_baseline(com, q, a.cpu_seconds)
@@ -266,6 +347,7 @@ def main():
                      "per_molecule_kernels_ms": t[2],
                      "eager_step_ms": t[3]},
         "cpu_baseline": cpu,
+        "npt_mc": npt,
         "replica_ladders": ladder,
         "tip5p_65536": tip5p,
     }

@@ -416,6 +416,7 @@ int g6_finalize(g6_handle h) {
   System *s = get(h);
   if (!s) return G6_EARG;
   (void)hipStreamSynchronize(s->stream);
+  npt_forget(s);
   free_all(*s);
   delete s;
   std::lock_guard<std::mutex> lk(g_mu);

@@ -190,6 +190,7 @@ struct System {
 bool make_model_table(int model, int prec, ModelTable *mt);
 void make_std_table(const ModelTable &mt, StdTable *t);
 void mc_free(System &s);
+void npt_forget(System *s);  // frees the NPT Monte Carlo state of s (g6_npt.hip)
 System *system_of(g6_handle h);  // nullptr for an unknown handle
 template <typename R>
 PairK<R> make_pair_k(const ModelTable &mt, const Geom &g) {

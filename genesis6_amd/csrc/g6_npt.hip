@@ -921,6 +921,26 @@ static Npt *npt_of(System *s, bool create) {
   return p;
 }
 
+static void npt_release(Npt &p);
+
+// the NPT state of a system that is being finalised
+void npt_forget(System *s) {
+  Npt *p = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_nmu);
+    for (size_t k = 0; k < g_npt.size(); k++)
+      if (g_npt[k].first == s) {
+        p = g_npt[k].second;
+        g_npt.erase(g_npt.begin() + k);
+        break;
+      }
+  }
+  if (p) {
+    npt_release(*p);
+    delete p;
+  }
+}
+
 static void npt_release(Npt &p) {
   if (p.graph) (void)hipGraphExecDestroy(p.graph);
   p.graph = nullptr;
@@ -1082,7 +1102,8 @@ int g6_npt_begin(g6_handle h, double temp, double press_bar, uint64_t seed, doub
   Npt &p = *npt_of(s, true);
   G6_CHECK(hipStreamSynchronize(s->stream));
   // a second start on the same handle keeps the device buffers
-  const bool reuse = p.ready;
+  const bool reuse = p.ready && p.a.n == s->n && p.a.ns == s->ns && p.a.model == s->model;
+  if (p.ready && !reuse) npt_release(p);
   NptArgs keep = p.a;
   p.a = npt_args_base(*s);
   NptArgs &a = p.a;

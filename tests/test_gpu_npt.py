@@ -93,3 +93,17 @@ def test_delayed_acceptance_with_a_zero_network_is_plain_mc():
     assert a["accepted"] == b["accepted"] and a["next_index"] == b["next_index"]
     for u, v in zip(s1.npt_state(), s2.npt_state()):
         assert np.array_equal(u, v)
+
+
+def test_npt_state_is_freed_with_its_system():
+    """A finalised system takes its NPT state with it: a new system (which
+    may reuse the old one's host address) starts from its own configuration."""
+    for n in (200, 100, 150):
+        s, com, q, L = system(n, 1.0, 50 + n)
+        s.npt_begin(250.0, 1.0, 5, 0.15, 8.0, 0.005, 0.2, VOXA)
+        c, qq, Lg, U = s.npt_state()
+        assert c.shape == (n, 3) and np.array_equal(c, com) and Lg == L
+        _, fresh = O.mc_pairinfo(O.TIP4P, com, q, L, RC, TAPER)
+        assert abs(U - fresh) < 1e-10 * abs(fresh)
+        s.npt_run(50)
+        s.close()
