@@ -82,11 +82,13 @@ static void launch_steps_lad(Ladder &l, int k) {
 // Block maps: systems go to the XCD lane with the fewest molecules so far
 // (largest first); lane x's blocks take workgroup slots x, x + 8, x + 16, ...
 // and short lanes are padded with idle blocks.
-static void build_map(const std::vector<int> &lane_of, const std::vector<int> &nblk,
+// first: the block index each system's list starts from (-1 adds the close
+// block of k_md_main_lad)
+static void build_map(const std::vector<int> &lane_of, const std::vector<int> &nblk, int first,
                       std::vector<int2> &map) {
   std::vector<std::vector<int2>> lanes(8);
   for (size_t r = 0; r < nblk.size(); r++)
-    for (int b = 0; b < nblk[r]; b++) lanes[lane_of[r]].push_back(make_int2((int)r, b));
+    for (int b = first; b < nblk[r]; b++) lanes[lane_of[r]].push_back(make_int2((int)r, b));
   size_t len = 0;
   for (auto &v : lanes) len = std::max(len, v.size());
   map.assign(8 * len, make_int2(-1, 0));
@@ -167,8 +169,8 @@ static int ladder_setup(Ladder &l) {
     load[best] += l.sys[r]->n;
   }
   std::vector<int2> mp, mm;
-  build_map(lane_of, np, mp);
-  build_map(lane_of, nm, mm);
+  build_map(lane_of, np, 0, mp);
+  build_map(lane_of, nm, -1, mm);
   l.grid_pre = (int)mp.size();
   l.grid_main = (int)mm.size();
   int rb = 0;

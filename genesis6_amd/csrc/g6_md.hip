@@ -798,10 +798,6 @@ __device__ __forceinline__ void md_main_body(const ModelTable &mt, const Geom &g
   extern __shared__ __align__(16) unsigned char s_dyn[];  // [4][reg_bytes]
   __shared__ int s_beg[4][128], s_pre[4][132];
   __shared__ double s_cnt[4][2];
-  __shared__ double s_close[4][12 + kEkLen];
-  if (ca.mode && blk == 0)
-    close_body(mt, b, g.n, ca.dt, ca.nblk_pre, ca.bpart, ca.nblk_main, g.ncell, ca.nose_active,
-               ca.mode, parity, g.L, s_close);
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int n = g.n;
   const int i = blk * 4 + wid;
@@ -972,17 +968,33 @@ __device__ __forceinline__ void md_main_body(const ModelTable &mt, const Geom &g
   }
 }
 
+// FP32 TIP4P/TIP5P rows fit in 128 VGPRs: four waves per SIMD, so a 4096-molecule
+// grid (1024 blocks) is resident at once on 256 CUs
 template <int M, typename R>
-__global__ __launch_bounds__(256) void k_md_main(ModelTable mt, Geom g, MdBuf b, PairK<R> pk,
+constexpr int main_waves_per_eu() { return (M != G6_ST2 && sizeof(R) == 4) ? 4 : 1; }
+
+template <int M, typename R>
+__global__ __launch_bounds__(256, (main_waves_per_eu<M, R>())) void k_md_main(ModelTable mt, Geom g, MdBuf b, PairK<R> pk,
                                                  double *out, int parity, CloseArgs ca,
                                                  int reg_bytes) {
-  md_main_body<M, R>(mt, g, b, pk, out, parity, ca, reg_bytes, blockIdx.x, gridDim.x);
+  if (ca.mode) {
+    // block 0 closes the previous step while the other blocks run the rows
+    if (blockIdx.x == 0) {
+      __shared__ double s_close[4][12 + kEkLen];
+      close_body(mt, b, g.n, ca.dt, ca.nblk_pre, ca.bpart, ca.nblk_main, g.ncell, ca.nose_active,
+                 ca.mode, parity, g.L, s_close);
+      return;
+    }
+    md_main_body<M, R>(mt, g, b, pk, out, parity, ca, reg_bytes, blockIdx.x - 1, gridDim.x - 1);
+  } else {
+    md_main_body<M, R>(mt, g, b, pk, out, parity, ca, reg_bytes, blockIdx.x, gridDim.x);
+  }
 }
 
-// Ladder form (map as k_md_pre_lad); block 0 of each system closes that
-// system's previous step when close_mode != 0.
+// Ladder form (map as k_md_pre_lad); the block of index -1 of each system
+// closes that system's previous step when close_mode != 0.
 template <int M, typename R>
-__global__ __launch_bounds__(256) void k_md_main_lad(ModelTable mt, const MdRep *__restrict__ reps,
+__global__ __launch_bounds__(256, (main_waves_per_eu<M, R>())) void k_md_main_lad(ModelTable mt, const MdRep *__restrict__ reps,
                                                      const int2 *__restrict__ map, PairK<R> pk,
                                                      int parity, int close_mode, int reg_bytes) {
   const int2 m = map[blockIdx.x];
@@ -997,6 +1009,14 @@ __global__ __launch_bounds__(256) void k_md_main_lad(ModelTable mt, const MdRep 
   ca.nblk_main = nm;
   ca.bpart = b.blockpart + (size_t)(1 - parity) * nm * 16;
   ca.dt = reps[m.x].dt;
+  if (m.y < 0) {
+    if (close_mode) {
+      __shared__ double s_close[4][12 + kEkLen];
+      close_body(mt, b, g.n, ca.dt, ca.nblk_pre, ca.bpart, ca.nblk_main, g.ncell, ca.nose_active,
+                 ca.mode, parity, g.L, s_close);
+    }
+    return;
+  }
   md_main_body<M, R>(mt, g, b, pk, nullptr, parity, ca, reg_bytes, m.y, nm);
 }
 
@@ -1033,7 +1053,7 @@ const PairK<double> &pair_k<double>(const System &s) { return s.pkd; }
 template <int M, typename R>
 static void md_main_t(System &s, double *out, int parity, const CloseArgs &ca) {
   const int rb = main_region_bytes(s.md.maxnb, (int)sizeof(R));
-  k_md_main<M, R><<<s.nblk_md, 256, 4 * (size_t)rb, s.stream>>>(s.mt, s.geo_md, s.md,
+  k_md_main<M, R><<<s.nblk_md + (ca.mode ? 1 : 0), 256, 4 * (size_t)rb, s.stream>>>(s.mt, s.geo_md, s.md,
                                                                pair_k<R>(s), out, parity, ca, rb);
 }
 
