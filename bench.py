@@ -80,29 +80,46 @@ LADDERS = [(256, 0.92, 20.266), (512, 0.97, 25.088), (1024, 1.00, 31.289),
 RUNGS, RX_EVERY, RX_TRIALS = 16, 500, 30
 
 
-def ladder_run(intervals, skin):
-    """All 64 replicas stepped by one ladder (one launch per MD kernel for
-    every replica, one graph per 500-step interval) with the reference's
-    replica exchange between intervals.  Returns molecule-steps/s over the
-    timed intervals, exchanges included."""
+TEMPS = [220.0 * (320.0 / 220.0) ** (k / 15.0) for k in range(RUNGS)]
+
+
+def ladder_systems(skin):
+    """The 64 replicas (4 ladders x 16 rungs) from their seeded starts."""
     from genesis6_amd import g6
-    temps = [220.0 * (320.0 / 220.0) ** (k / 15.0) for k in range(RUNGS)]
-    systems, groups = [], []
+    systems = []
     for li, (n, rho, Lb) in enumerate(LADDERS):
-        rung = []
-        for k, T in enumerate(temps):
+        for k, T in enumerate(TEMPS):
             seed = 20261202 + 4 + 100 * li + k
             s = g6.System(g6.TIP4P, n, Lb, RC, TAPER, DT, prec=g6.FP32, skin=skin)
             com, q = g6.lattice_start(n, Lb, seed)
             s.upload(com, q)
             s.set_nose(T, T * 2500.0)
             s.vreset(T, seed)
-            rung.append(s)
-        systems += rung
+            systems.append(s)
+    return systems
+
+
+def ladder_run(intervals, skin, tune=False, log=None):
+    """All 64 replicas stepped by one ladder (one launch per MD kernel for
+    every replica, one graph per 500-step interval) with the reference's
+    replica exchange between intervals.  With tune, the autotuner first
+    picks the skin and block assignment.  Returns molecule-steps/s over the
+    timed intervals, exchanges included."""
+    from genesis6_amd import g6
+    from genesis6_amd.autotune import tune_ladder
+    best, table = None, None
+    mode = g6.Ladder.MAP_FLAT
+    if tune:
+        best, table = tune_ladder(ladder_systems, log=log)
+        skin, mode = best["skin"], best["map_mode"]
+    systems = ladder_systems(skin)
+    groups = []
+    for li in range(len(LADDERS)):
         bonds = g6.rxpr_bonds(RUNGS, [(k, k + 1) for k in range(RUNGS - 1)])
         groups.append(g6.ReplicaExchange(RUNGS, bonds, RX_TRIALS, 20261202 + 4 + 100 * li + 99,
-                                         systems=rung))
+                                         systems=systems[li * RUNGS:(li + 1) * RUNGS]))
     lad = g6.Ladder(systems)
+    lad.set_map(mode)
     nmol = sum(s.n for s in systems)
 
     def interval():
@@ -123,6 +140,8 @@ def ladder_run(intervals, skin):
     steps = intervals * RX_EVERY
     out = dict(value=nmol * steps / el, unit="molecule-steps/s", replicas=len(systems),
                molecules=nmol, steps=steps, ms_per_step=el / steps * 1e3,
+               skin_A=skin, block_map="xcd" if mode == g6.Ladder.MAP_XCD else "flat",
+               autotune=table,
                exchange_trials=intervals * RX_TRIALS * len(groups), accepted=acc,
                pairs_per_step=float(th[:, 14].sum()),
                mean_temperature_K=float(th[:, 3].mean()))
@@ -238,6 +257,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--ladder-intervals", type=int, default=4,
                     help="timed 500-step intervals of the 64-replica ladder (0: skip)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="skip the ladder autotuner (skin and block assignment)")
     ap.add_argument("--tip5p-steps", type=int, default=500,
                     help="timed steps of the TIP5P-65536 system (0: skip)")
     ap.add_argument("--npt-configs", type=int, default=20000,
@@ -293,7 +314,8 @@ def main():
         print("bench: FP64 %.4g molecule-steps/s" % fp64["value"], file=sys.stderr, flush=True)
     ladder = None
     if a.ladder_intervals > 0:
-        ladder = ladder_run(a.ladder_intervals, a.skin)
+        ladder = ladder_run(a.ladder_intervals, a.skin, tune=not a.no_tune,
+                            log=lambda m: print("bench: " + m, file=sys.stderr, flush=True))
         print("bench: ladder %.4g molecule-steps/s (%d replicas, %.3f ms/step)"
               % (ladder["value"], ladder["replicas"], ladder["ms_per_step"]),
               file=sys.stderr, flush=True)

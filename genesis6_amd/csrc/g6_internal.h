@@ -248,9 +248,9 @@ void md_launch_close(System &s, int mode, int parity, int src_parity);
 void md_launch_main(System &s, double *out, int parity, int close_mode);
 // A ladder: several systems of one model and precision stepped together,
 // each MD kernel launched once for all of them (g6_ladder.hip).  The block
-// maps interleave the systems over the eight XCDs: workgroup b runs on XCD
-// b mod 8, and every block of a system gets the same b mod 8, so that a
-// system's records and rows stay in one XCD's L2.
+// maps list every block of every system: one after another (flat, the
+// default), or interleaved over the eight XCDs so that every block of a
+// system runs on one XCD (workgroup b runs on XCD b mod 8).
 struct Ladder {
   std::vector<System *> sys;
   int model = 0, prec = 0;
@@ -267,6 +267,7 @@ struct Ladder {
   hipStream_t stream = nullptr;
   hipGraphExec_t graph = nullptr;
   int graph_steps = 0;
+  int map_mode = G6_LADDER_MAP_FLAT;
 };
 void md_launch_pre_lad(Ladder &l, bool correct, bool predict, bool sites, int parity);
 void md_launch_close_lad(Ladder &l, int mode, int parity, int src_parity);

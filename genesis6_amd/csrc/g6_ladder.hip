@@ -8,9 +8,10 @@
 //
 // Here the replicas are systems of one process and one device.  A ladder
 // steps all of them with one launch per MD kernel (k_md_*_lad, g6_md.hip):
-// the block maps below list every block of every system, interleaved over
-// the eight XCDs so that each system stays on one XCD, and the launches of a
-// run of steps are captured into one hipGraph.  An exchange group holds the
+// the block maps below list every block of every system (one after another,
+// or interleaved over the eight XCDs so that each system stays on one XCD;
+// the autotuner times both), and the launches of a run of steps are
+// captured into one hipGraph.  An exchange group holds the
 // node-0 state (MT19937 stream, node_in_charge) and applies the swap rule on
 // the host: it reads 64 numbers per exchange, so it does not belong on the
 // device.
@@ -150,7 +151,15 @@ static std::vector<MdRep> descriptors(const Ladder &l) {
   return d;
 }
 
-static int ladder_setup(Ladder &l) {
+// contiguous map: the systems' blocks one after another (a system spread
+// over every XCD)
+static void build_map_flat(const std::vector<int> &nblk, int first, std::vector<int2> &map) {
+  map.clear();
+  for (size_t r = 0; r < nblk.size(); r++)
+    for (int b = first; b < nblk[r]; b++) map.push_back(make_int2((int)r, b));
+}
+
+static int ladder_maps(Ladder &l) {
   const size_t ns = l.sys.size();
   std::vector<int> order(ns), lane_of(ns), np(ns), nm(ns);
   for (size_t r = 0; r < ns; r++) {
@@ -158,37 +167,54 @@ static int ladder_setup(Ladder &l) {
     np[r] = pre_waves(l.sys[r]->n) / 4;
     nm[r] = l.sys[r]->nblk_md;
   }
-  std::stable_sort(order.begin(), order.end(),
-                   [&](int x, int y) { return l.sys[x]->n > l.sys[y]->n; });
-  long long load[8] = {0};
-  for (int r : order) {
-    int best = 0;
-    for (int x = 1; x < 8; x++)
-      if (load[x] < load[best]) best = x;
-    lane_of[r] = best;
-    load[best] += l.sys[r]->n;
-  }
   std::vector<int2> mp, mm;
-  build_map(lane_of, np, 0, mp);
-  build_map(lane_of, nm, -1, mm);
+  if (l.map_mode == G6_LADDER_MAP_XCD) {
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int x, int y) { return l.sys[x]->n > l.sys[y]->n; });
+    long long load[8] = {0};
+    for (int r : order) {
+      int best = 0;
+      for (int x = 1; x < 8; x++)
+        if (load[x] < load[best]) best = x;
+      lane_of[r] = best;
+      load[best] += l.sys[r]->n;
+    }
+    build_map(lane_of, np, 0, mp);
+    build_map(lane_of, nm, -1, mm);
+  } else {
+    build_map_flat(np, 0, mp);
+    build_map_flat(nm, -1, mm);
+  }
+  if (l.map_pre) (void)hipFree(l.map_pre);
+  if (l.map_main) (void)hipFree(l.map_main);
+  l.map_pre = l.map_main = nullptr;
   l.grid_pre = (int)mp.size();
   l.grid_main = (int)mm.size();
+  G6_CHECK(hipMalloc(&l.map_pre, sizeof(int2) * mp.size()));
+  G6_CHECK(hipMalloc(&l.map_main, sizeof(int2) * mm.size()));
+  G6_CHECK(hipMemcpy(l.map_pre, mp.data(), sizeof(int2) * mp.size(), hipMemcpyHostToDevice));
+  G6_CHECK(hipMemcpy(l.map_main, mm.data(), sizeof(int2) * mm.size(), hipMemcpyHostToDevice));
+  if (l.graph) (void)hipGraphExecDestroy(l.graph);  // captured with the old maps
+  l.graph = nullptr;
+  return 0;
+}
+
+static int ladder_setup(Ladder &l) {
+  const size_t ns = l.sys.size();
+  const int rm = ladder_maps(l);
+  if (rm) return rm;
   int rb = 0;
   for (System *s : l.sys)
     rb = std::max(rb, main_region_bytes(s->md.maxnb, l.prec == G6_FP32 ? 4 : 8));
   l.reg_bytes = rb;
   const std::vector<MdRep> d = descriptors(l);
   G6_CHECK(hipMalloc(&l.reps, sizeof(MdRep) * ns));
-  G6_CHECK(hipMalloc(&l.map_pre, sizeof(int2) * mp.size()));
-  G6_CHECK(hipMalloc(&l.map_main, sizeof(int2) * mm.size()));
   G6_CHECK(hipMalloc(&l.th_dev, sizeof(double) * G6_THERMO_LEN * ns));
   G6_CHECK(hipMalloc(&l.fl_dev, sizeof(int) * ns));
   G6_CHECK(hipHostMalloc(&l.th_host, sizeof(double) * G6_THERMO_LEN * ns, hipHostMallocDefault));
   G6_CHECK(hipHostMalloc(&l.fl_host, sizeof(int) * ns, hipHostMallocDefault));
   memset(l.th_host, 0, sizeof(double) * G6_THERMO_LEN * ns);
   G6_CHECK(hipMemcpy(l.reps, d.data(), sizeof(MdRep) * ns, hipMemcpyHostToDevice));
-  G6_CHECK(hipMemcpy(l.map_pre, mp.data(), sizeof(int2) * mp.size(), hipMemcpyHostToDevice));
-  G6_CHECK(hipMemcpy(l.map_main, mm.data(), sizeof(int2) * mm.size(), hipMemcpyHostToDevice));
   G6_CHECK(hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking));
   return 0;
 }
@@ -325,6 +351,14 @@ int g6_ladder_step(g6_ladder hl, int nsteps) {
     else if (f == 2 && rc == 0) rc = G6_EARG;
   }
   return rc;
+}
+
+int g6_ladder_set_map(g6_ladder hl, int mode) {
+  Ladder *l = lget(hl);
+  if (!l || (mode != G6_LADDER_MAP_XCD && mode != G6_LADDER_MAP_FLAT)) return G6_EARG;
+  G6_CHECK(hipStreamSynchronize(l->stream));
+  l->map_mode = mode;
+  return ladder_maps(*l);
 }
 
 int g6_ladder_thermo(g6_ladder hl, double *thermo) {

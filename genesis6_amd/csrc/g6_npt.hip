@@ -32,6 +32,7 @@
 // passing one are discarded unseen (they were drawn independently of
 // everything before them, so discarding them changes nothing).
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <mutex>
@@ -45,6 +46,7 @@ namespace g6 {
 
 constexpr double kI2Jn = 10.0, kJ2Kn = 0.120273;  // PhysConst.F90:48, 56
 constexpr int kNptMaxBlock = 96;   // molecules of one block move
+constexpr int kNptSub = 8;         // waves per changed row of a block move
 constexpr int kNptMaxK = 64;       // proposals per delayed-acceptance round
 constexpr int kDescMaxNb = 128;    // neighbours within the descriptor radius
 constexpr float kDescRc = 6.0f;
@@ -293,7 +295,46 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
-// one wave per changed PairInfo row
+// Volume move: row w of the scaled configuration (j > w), written to both
+// halves of the proposed matrix, row sum into part[w]
+template <int M>
+__device__ inline void volume_row(const NptArgs &A, const StdTable &st, int w, int lane) {
+  constexpr int NS = StdPattern<M>::NS;
+  const int n = A.n;
+  double XI[kMaxSite], YI[kMaxSite], ZI[kMaxSite], XJ[kMaxSite], YJ[kMaxSite], ZJ[kMaxSite];
+  const double L = A.st[2], invL = 1.0 / L;
+  const double xi = A.com2[w], yi = A.com2[n + w], zi = A.com2[2 * n + w];
+  load_sites<NS>(A.sx2 + (size_t)w * NS, A.sy2 + (size_t)w * NS, A.sz2 + (size_t)w * NS, XI, YI,
+                 ZI);
+  double acc = 0.0;
+  for (int j = lane; j < n; j += 64) {
+    double e = 0.0;
+    if (j > w) {
+      double d0 = xi - A.com2[j], d1 = yi - A.com2[n + j], d2 = zi - A.com2[2 * n + j];
+      const double k0 = round(d0 * invL) * L, k1 = round(d1 * invL) * L, k2 = round(d2 * invL) * L;
+      d0 = d0 - k0;
+      d1 = d1 - k1;
+      d2 = d2 - k2;
+      const double dd = d0 * d0 + d1 * d1 + d2 * d2;
+      if (dd < A.rc * A.rc) {
+        load_sites<NS>(A.sx2 + (size_t)j * NS, A.sy2 + (size_t)j * NS, A.sz2 + (size_t)j * NS, XJ,
+                       YJ, ZJ);
+        e = std_energy_rr<M>(st, XI, YI, ZI, XJ, YJ, ZJ, k0, k1, k2, mc_switch(A, dd));
+      }
+      A.pinfo2[(size_t)w * n + j] = e;
+      A.pinfo2[(size_t)j * n + w] = e;
+      acc = acc + e;
+    } else if (j == w) {
+      A.pinfo2[(size_t)w * n + w] = 0.0;
+    }
+  }
+  acc = wave_sum(acc);
+  if (lane == 0) A.part[w] = acc;
+}
+
+// The changed PairInfo rows.  Volume: wave w takes rows w and n-1-w (equal
+// work per wave).  Block: kNptSub waves per member row, each a contiguous
+// range of j; partial dU of wave (member m, range r) into part[m*kNptSub+r].
 template <int M>
 __global__ __launch_bounds__(256) void k_npt_eval(NptArgs A, StdTable st, int da) {
   constexpr int NS = StdPattern<M>::NS;
@@ -301,50 +342,27 @@ __global__ __launch_bounds__(256) void k_npt_eval(NptArgs A, StdTable st, int da
   if (idx < 0) return;
   const int n = A.n, lane = threadIdx.x & 63;
   const int w = blockIdx.x * 4 + (threadIdx.x >> 6);
-  double acc = 0.0;
-  double XI[kMaxSite], YI[kMaxSite], ZI[kMaxSite], XJ[kMaxSite], YJ[kMaxSite], ZJ[kMaxSite];
-  if (A.ist[4]) {  // volume: row w of the scaled configuration, j > w
-    if (w >= n) return;
-    const double L = A.st[2], invL = 1.0 / L;
-    const double xi = A.com2[w], yi = A.com2[n + w], zi = A.com2[2 * n + w];
-    load_sites<NS>(A.sx2 + (size_t)w * NS, A.sy2 + (size_t)w * NS, A.sz2 + (size_t)w * NS, XI, YI,
-                   ZI);
-    for (int j = lane; j < n; j += 64) {
-      double e = 0.0;
-      if (j > w) {
-        double d0 = xi - A.com2[j], d1 = yi - A.com2[n + j], d2 = zi - A.com2[2 * n + j];
-        const double k0 = round(d0 * invL) * L, k1 = round(d1 * invL) * L,
-                     k2 = round(d2 * invL) * L;
-        d0 = d0 - k0;
-        d1 = d1 - k1;
-        d2 = d2 - k2;
-        const double dd = d0 * d0 + d1 * d1 + d2 * d2;
-        if (dd < A.rc * A.rc) {
-          load_sites<NS>(A.sx2 + (size_t)j * NS, A.sy2 + (size_t)j * NS, A.sz2 + (size_t)j * NS,
-                         XJ, YJ, ZJ);
-          e = std_energy_rr<M>(st, XI, YI, ZI, XJ, YJ, ZJ, k0, k1, k2, mc_switch(A, dd));
-        }
-        A.pinfo2[(size_t)w * n + j] = e;
-        A.pinfo2[(size_t)j * n + w] = e;
-        acc = acc + e;
-      } else if (j == w) {
-        A.pinfo2[(size_t)w * n + w] = 0.0;
-      }
-    }
-    acc = wave_sum(acc);
-    if (lane == 0) A.part[w] = acc;
+  if (A.ist[4]) {
+    const int h = (n + 1) / 2;
+    if (w >= h) return;
+    volume_row<M>(A, st, w, lane);
+    if (n - 1 - w != w) volume_row<M>(A, st, n - 1 - w, lane);
     return;
   }
-  // block: row of member slot w against every molecule
-  if (A.ist[3] || w >= A.ist[0]) return;
-  const int i = A.members[w];
+  double XI[kMaxSite], YI[kMaxSite], ZI[kMaxSite], XJ[kMaxSite], YJ[kMaxSite], ZJ[kMaxSite];
+  const int mslot = w / kNptSub, sub = w % kNptSub;
+  if (A.ist[3] || mslot >= A.ist[0]) return;
+  const int i = A.members[mslot];
+  const int chunk = ((n + kNptSub - 1) / kNptSub + 63) / 64 * 64;
+  const int jlo = sub * chunk, jhi = min(n, jlo + chunk);
   const double L = A.st[0], invL = 1.0 / L;
-  const double xi = A.bc[w * 3], yi = A.bc[w * 3 + 1], zi = A.bc[w * 3 + 2];
-  load_sites<NS>(A.bsx + (size_t)w * NS, A.bsy + (size_t)w * NS, A.bsz + (size_t)w * NS, XI, YI,
-                 ZI);
-  for (int j = lane; j < n; j += 64) {
+  const double xi = A.bc[mslot * 3], yi = A.bc[mslot * 3 + 1], zi = A.bc[mslot * 3 + 2];
+  load_sites<NS>(A.bsx + (size_t)mslot * NS, A.bsy + (size_t)mslot * NS,
+                 A.bsz + (size_t)mslot * NS, XI, YI, ZI);
+  double acc = 0.0;
+  for (int j = jlo + lane; j < jhi; j += 64) {
     if (j == i) {
-      A.rows[(size_t)w * n + j] = 0.0;
+      A.rows[(size_t)mslot * n + j] = 0.0;
       continue;
     }
     const int sj = A.slot[j];
@@ -378,7 +396,7 @@ __global__ __launch_bounds__(256) void k_npt_eval(NptArgs A, StdTable st, int da
       e = i < j ? std_energy_rr<M>(st, XI, YI, ZI, XJ, YJ, ZJ, k0, k1, k2, er)
                 : std_energy_rr<M>(st, XJ, YJ, ZJ, XI, YI, ZI, k0, k1, k2, er);
     }
-    A.rows[(size_t)w * n + j] = e;
+    A.rows[(size_t)mslot * n + j] = e;
     if (sj < 0 || j > i) acc = acc + (e - A.pinfo[(size_t)i * n + j]);
   }
   acc = wave_sum(acc);
@@ -418,7 +436,7 @@ __global__ __launch_bounds__(256) void k_npt_decide(NptArgs A, int da) {
     if (vol) {
       du = up - U;
     } else if (!bad) {
-      const int cnt = A.ist[0];
+      const int cnt = A.ist[0] * kNptSub;
       for (int w = 0; w < cnt; w++) du = du + A.part[w];
     }
     double arg;
@@ -780,26 +798,28 @@ __global__ __launch_bounds__(256) void k_desc(NptArgs A, int nconf) {
   }
 }
 
-// the round's screening: dU_s of every proposal (fixed-order sums), then the
-// surrogate Metropolis test in proposal order; the first pass is selected
-__global__ __launch_bounds__(256) void k_da_select(NptArgs A) {
-  const int n = A.n;
-  for (int k = 0; k < A.K; k++) {
-    __shared__ double s[256];
-    double a = 0.0;
-    const int m = A.naff[k];
-    for (int r = threadIdx.x; r < m; r += 256)
-      a = a + ((double)A.ys[(size_t)k * n + r] - (double)A.ycur[A.arow[(size_t)k * n + r]]);
-    s[threadIdx.x] = a;
-    __syncthreads();
-    for (int h = 128; h >= 1; h >>= 1) {
-      if (threadIdx.x < h) s[threadIdx.x] = s[threadIdx.x] + s[threadIdx.x + h];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) A.des[k] = s[0];
+// dU_s of proposal k = blockIdx.x: the fixed-order sum over its affected
+// rows of (new - cached) surrogate energies
+__global__ __launch_bounds__(256) void k_da_sum(NptArgs A) {
+  __shared__ double s[256];
+  const int n = A.n, k = blockIdx.x, m = A.naff[k];
+  double a = 0.0;
+  for (int r = threadIdx.x; r < m; r += 256)
+    a = a + ((double)A.ys[(size_t)k * n + r] - (double)A.ycur[A.arow[(size_t)k * n + r]]);
+  s[threadIdx.x] = a;
+  __syncthreads();
+  for (int h = 128; h >= 1; h >>= 1) {
+    if (threadIdx.x < h) s[threadIdx.x] = s[threadIdx.x] + s[threadIdx.x + h];
     __syncthreads();
   }
+  if (threadIdx.x == 0) A.des[k] = s[0];
+}
+
+// the surrogate's Metropolis test in proposal order; the first pass is the
+// round's selection (one thread)
+__global__ void k_da_select(NptArgs A) {
   if (threadIdx.x != 0) return;
+  const int n = A.n;
   const double L = A.st[0];
   int sel = -1, seen = 0, vols = 0, voids = 0;
   for (int k = 0; k < A.K; k++) {
@@ -891,8 +911,8 @@ __global__ __launch_bounds__(256) void k_npt_total(NptArgs A, double *out) {
 __global__ void k_npt_mol_energy(NptArgs A, double *e) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.n) return;
-  double s = 0.0;
-  for (int j = 0; j < A.n; j++) s = s + A.pinfo[(size_t)i * A.n + j];
+  double s = 0.0;  // column i of the symmetric matrix: coalesced, j in order
+  for (int j = 0; j < A.n; j++) s = s + A.pinfo[(size_t)j * A.n + i];
   e[i] = 0.5 * s;
 }
 
@@ -986,7 +1006,8 @@ template <int M>
 static void launch_move(System &s, Npt &p, int da) {
   NptArgs &a = p.a;
   k_npt_prep<<<1, 1024, 0, s.stream>>>(a, da);
-  k_npt_eval<M><<<(a.n + 3) / 4, 256, 0, s.stream>>>(a, p.st, da);
+  const int waves = a.n > kNptMaxBlock * kNptSub ? a.n : kNptMaxBlock * kNptSub;
+  k_npt_eval<M><<<(waves + 3) / 4, 256, 0, s.stream>>>(a, p.st, da);
   k_npt_decide<<<1, 256, 0, s.stream>>>(a, da);
   k_npt_commit<<<256, 256, 0, s.stream>>>(a, da);
 }
@@ -999,7 +1020,8 @@ static void launch_round(System &s, Npt &p, Mlp &net) {
   const long long rows = (long long)a.K * a.n;
   k_desc<<<(unsigned)((rows + 3) / 4), 256, 0, s.stream>>>(a, a.K);
   mlp_launch_forward(net, rows, a.desc, nullptr, a.ys, s.stream);
-  k_da_select<<<1, 256, 0, s.stream>>>(a);
+  k_da_sum<<<a.K, 256, 0, s.stream>>>(a);
+  k_da_select<<<1, 64, 0, s.stream>>>(a);
   launch_move<M>(s, p, 1);
 }
 
@@ -1028,7 +1050,9 @@ static int npt_run(System &s, Npt &p, int count, Mlp *net, int K) {
   } else {
     a.K = 1;
   }
-  const int G = count < 200 ? count : 200;
+  // moves per captured graph (G6_GRAPH_STEPS = 0: eager launches)
+  const char *env = getenv("G6_GRAPH_STEPS");
+  const int G = env && atoi(env) <= 0 ? 0 : (count < 200 ? count : 200);
   auto body = [&](int m) {
     for (int q = 0; q < m; q++) {
       if (!da) {
@@ -1058,8 +1082,9 @@ static int npt_run(System &s, Npt &p, int count, Mlp *net, int K) {
       p.graph_mlp = mid;
     }
     for (int k = 0; k + G <= count; k += G) G6_CHECK(hipGraphLaunch(p.graph, s.stream));
-    const int rest = count % G;
-    body(rest);
+    body(count % G);
+  } else {
+    body(count);
   }
   G6_CHECK(hipGetLastError());
   G6_CHECK(hipStreamSynchronize(s.stream));
@@ -1154,7 +1179,7 @@ int g6_npt_begin(g6_handle h, double temp, double press_bar, uint64_t seed, doub
     NAL(a.rows, kNptMaxBlock * n);
     NAL(a.members, kNptMaxBlock);
     NAL(a.slot, n);
-    NAL(a.part, n);
+    NAL(a.part, n > (size_t)kNptMaxBlock * kNptSub ? n : (size_t)kNptMaxBlock * kNptSub);
     NAL(a.st, 8);
     NAL(a.ist, 8);
     NAL(a.cnt, kCnt);

@@ -11,6 +11,7 @@ module g6gpu
   integer(c_int), parameter :: G6_EHIP = -1, G6_EARG = -2, G6_ECAPACITY = -3, &
        G6_ENODEVICE = -4
   integer, parameter :: G6_THERMO_LEN = 32
+  integer(c_int), parameter :: G6_LADDER_MAP_XCD = 0, G6_LADDER_MAP_FLAT = 1
 
   interface
      integer(c_int) function g6_init(h, model, n, box_l, rc, taper, voxa, dt, prec, skin) &
@@ -206,6 +207,12 @@ module g6gpu
        integer(c_int64_t), value :: hl
        integer(c_int), value :: nsteps
      end function g6_ladder_step
+
+     integer(c_int) function g6_ladder_set_map(hl, mode) bind(C, name="g6_ladder_set_map")
+       import :: c_int, c_int64_t
+       integer(c_int64_t), value :: hl
+       integer(c_int), value :: mode
+     end function g6_ladder_set_map
 
      integer(c_int) function g6_ladder_thermo(hl, thermo) bind(C, name="g6_ladder_thermo")
        import :: c_int, c_int64_t, c_double

@@ -69,6 +69,7 @@ def lib():
         L.g6_ladder_free.argtypes = [_H]
         L.g6_ladder_step.argtypes = [_H, C.c_int]
         L.g6_ladder_thermo.argtypes = [_H, _dp]
+        L.g6_ladder_set_map.argtypes = [_H, C.c_int]
         L.g6_set_temperature.argtypes = [_H, C.c_double]
         L.g6_rx_create.argtypes = [C.POINTER(_H), C.c_int, C.c_void_p, C.c_int, _ip, _ip,
                                    C.c_int, C.c_uint32]
@@ -329,6 +330,11 @@ class Ladder:
 
     def step(self, nsteps):
         check(lib().g6_ladder_step(self.h, nsteps), "g6_ladder_step")
+
+    MAP_XCD, MAP_FLAT = 0, 1
+
+    def set_map(self, mode):
+        check(lib().g6_ladder_set_map(self.h, mode), "g6_ladder_set_map")
 
     def thermo(self):
         """Thermo records of the last g6_ladder_step, (nsys, THERMO_LEN)."""

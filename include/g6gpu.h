@@ -177,6 +177,13 @@ int g6_ladder_free(g6_ladder hl);
  * same results bit for bit (the inner loop of Mixture.F90:1833-2286 on
  * every node), launched as one captured hipGraph. */
 int g6_ladder_step(g6_ladder hl, int nsteps);
+/* Block assignment of the ladder's launches (the autotuner's knob; results
+ * do not depend on it): G6_LADDER_MAP_FLAT lays the systems' blocks one
+ * after another (the default); G6_LADDER_MAP_XCD gives every block of a
+ * system the same XCD (workgroup b runs on XCD b mod 8; systems balanced
+ * over the eight, largest first). */
+enum { G6_LADDER_MAP_XCD = 0, G6_LADDER_MAP_FLAT = 1 };
+int g6_ladder_set_map(g6_ladder hl, int mode);
 /* the thermo records of the last step: thermo[nsys * G6_THERMO_LEN], in
  * the order of hs */
 int g6_ladder_thermo(g6_ladder hl, double *thermo);

@@ -124,3 +124,28 @@ def test_ladder_exchange_makes_the_oracle_decisions():
                 assert z[0] == tnow[r]
                 assert np.array_equal(dx, before[r][2]) and np.array_equal(w, before[r][3])
     assert total > 0  # the rule was exercised on accepted swaps
+
+
+def test_block_assignment_changes_no_result():
+    """The autotuner's block-assignment knob: the flat map gives the same
+    states as the XCD map, bit for bit."""
+    a = [make(n, r, sd, T, g6.FP32, sk, ns) for n, r, sd, T, sk, ns in SPECS]
+    b = [make(n, r, sd, T, g6.FP32, sk, ns) for n, r, sd, T, sk, ns in SPECS]
+    la, lb = g6.Ladder(a), g6.Ladder(b)
+    lb.set_map(g6.Ladder.MAP_FLAT)
+    la.step(25)
+    lb.step(25)
+    for x, y in zip(a, b):
+        for u, v in zip(x.download(), y.download()):
+            assert np.array_equal(u, v)
+    assert np.array_equal(la.thermo(), lb.thermo())
+
+
+def test_autotuner_returns_a_measured_candidate():
+    from genesis6_amd.autotune import tune_ladder
+    specs = SPECS[:2]
+    best, table = tune_ladder(lambda skin: [make(n, r, sd, T, g6.FP32, skin, ns)
+                                            for n, r, sd, T, _, ns in specs],
+                              skins=(0.5, 1.0), steps=20, warmup=5)
+    assert len(table) == 4 and best in table
+    assert all(r["ms_per_step"] > 0 for r in table)
