@@ -32,9 +32,11 @@ struct Mlp {
 };
 
 // y[r] = energy of feature row x[r*32..] (r < m), device pointers; rows
-// listed by idx when idx != nullptr.  Stream-ordered, no host sync.
+// listed by idx when idx != nullptr.  With segcnt, the rows form segments of
+// seg rows of which the first segcnt[k] are live; 16-row tiles wholly past
+// them are not computed.  Stream-ordered, no host sync.
 void mlp_launch_forward(const Mlp &net, long long m, const float *x, const int *idx, float *y,
-                        hipStream_t s);
+                        hipStream_t s, int seg = 0, const int *segcnt = nullptr);
 // gradient of the mean squared error in normalised units over the rows
 // (idx or 0..m-1), into net.grad (loss in net.grad[kMlpParams])
 int mlp_launch_grad(Mlp &net, long long m, const float *x, const float *t, const int *idx,

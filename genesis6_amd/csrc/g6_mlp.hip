@@ -67,7 +67,8 @@ __global__ __launch_bounds__(256) void k_mlp(const float *__restrict__ P,
                                              const float *__restrict__ x,
                                              const float *__restrict__ t,
                                              const int *__restrict__ idx, float *__restrict__ y,
-                                             float *__restrict__ part) {
+                                             float *__restrict__ part, int seg,
+                                             const int *__restrict__ segcnt) {
   __shared__ uint16_t s_w1[kMlpHid * kMlpIn], s_w2[kMlpHid * kMlpHid];
   __shared__ float s_b1[kMlpHid], s_b2[kMlpHid], s_w3[kMlpHid], s_mean[kMlpIn], s_istd[kMlpIn];
   __shared__ uint16_t s_xn[4][16 * kMlpIn], s_h1[4][16 * kMlpHid];
@@ -105,6 +106,9 @@ __global__ __launch_bounds__(256) void k_mlp(const float *__restrict__ P,
   for (long long tile = (long long)blockIdx.x * 4 + wid; tile < ntile;
        tile += (long long)gridDim.x * 4) {
     const long long row0 = tile * 16;
+    // rows in segments of seg with segcnt[k] live rows each: a tile wholly
+    // past its segment's live rows is skipped
+    if (segcnt && row0 / seg == (row0 + 15) / seg && row0 % seg >= segcnt[row0 / seg]) continue;
     wsync_m();  // the previous tile's LDS reads are done
     {
       // lane: row r16, features 8q..8q+7, normalised, to BF16
@@ -323,11 +327,11 @@ __global__ void k_mlp_cast(const float *__restrict__ p, uint16_t *__restrict__ w
 constexpr int kMlpGradBlocks = 240;
 
 void mlp_launch_forward(const Mlp &net, long long m, const float *x, const int *idx, float *y,
-                        hipStream_t s) {
+                        hipStream_t s, int seg, const int *segcnt) {
   if (m <= 0) return;
   const long long blocks = ((m + 15) / 16 + 3) / 4;
   k_mlp<false><<<(unsigned)blocks, 256, 0, s>>>(net.p, net.wb, net.norm, m, x, nullptr, idx, y,
-                                                nullptr);
+                                                nullptr, seg, segcnt);
 }
 
 int mlp_launch_grad(Mlp &net, long long m, const float *x, const float *t, const int *idx,
@@ -336,7 +340,7 @@ int mlp_launch_grad(Mlp &net, long long m, const float *x, const float *t, const
   long long blocks = ((m + 15) / 16 + 3) / 4;
   if (blocks > net.part_blocks) blocks = net.part_blocks;
   k_mlp<true><<<(unsigned)blocks, 256, 0, s>>>(net.p, net.wb, net.norm, m, x, t, idx, nullptr,
-                                               net.part);
+                                               net.part, 0, nullptr);
   k_mlp_reduce<<<(kMlpParams + 1 + 255) / 256, 256, 0, s>>>(net.part, (int)blocks, net.grad, m);
   return 0;
 }

@@ -427,18 +427,13 @@ __global__ __launch_bounds__(256) void k_npt_decide(NptArgs A, int da) {
   const Prop p = make_prop(A, (unsigned long long)idx);
   const int n = A.n, ns = A.ns, tid = threadIdx.x;
   const bool vol = A.ist[4] != 0;
-  double up = 0.0;
-  if (vol) up = tree_sum256(A.part, n);
+  const bool bad = !vol && A.ist[3];
+  // volume: the new energy (row sums); block: dU (member-range partials)
+  const double sum = tree_sum256(A.part, vol ? n : (bad ? 0 : A.ist[0] * kNptSub));
+  const double up = sum;
   if (tid == 0) {
     const double L = A.st[0], U = A.st[1];
-    const bool bad = !vol && A.ist[3];
-    double du = 0.0;
-    if (vol) {
-      du = up - U;
-    } else if (!bad) {
-      const int cnt = A.ist[0] * kNptSub;
-      for (int w = 0; w < cnt; w++) du = du + A.part[w];
-    }
+    const double du = vol ? up - U : (bad ? 0.0 : sum);
     double arg;
     if (da) {
       arg = -((du - A.st[4]) * kI2Jn * kJ2Kn) * A.beta;
@@ -517,7 +512,10 @@ __global__ __launch_bounds__(256) void k_npt_commit(NptArgs A, int da) {
       for (size_t t = t0; t < (size_t)A.naff[k]; t += stride) A.ycur[A.arow[k * n + t]] = A.ys[k * n + t];
     }
   }
-  if (t0 == 0) A.cnt[8] += da ? A.K : 1;
+  if (t0 == 0) {
+    A.cnt[8] += da ? A.K : 1;
+    if (go) A.ist[5] = 1;  // the configuration changed: the next round rebuilds the list
+  }
 }
 
 // ---- surrogate screening ----------------------------------------------------
@@ -561,6 +559,7 @@ __global__ __launch_bounds__(1024) void k_da_prep(NptArgs A, int current) {
     all_rows(A, 0);
     return;
   }
+  if (k == 0 && tid == 0) A.ist[5] = 0;  // k_nbl of this round has run
   const unsigned long long idx = (unsigned long long)A.cnt[8] + k;
   const Prop p = make_prop(A, idx);
   double X[kMaxSite], Y[kMaxSite], Z[kMaxSite];
@@ -631,10 +630,10 @@ __global__ __launch_bounds__(1024) void k_da_prep(NptArgs A, int current) {
 
 // O-O neighbours of every molecule within kNblR in the current
 // configuration (FP64 minimum image), ascending j; one wave per molecule
-__global__ __launch_bounds__(256) void k_nbl(NptArgs A) {
+__global__ __launch_bounds__(256) void k_nbl(NptArgs A, int always) {
   const int n = A.n, ns = A.ns, lane = threadIdx.x & 63;
   const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (i >= n) return;
+  if (i >= n || !(always || A.ist[5])) return;
   const double L = A.st[0], invL = 1.0 / L;
   const double ox = A.sx[(size_t)i * ns], oy = A.sy[(size_t)i * ns], oz = A.sz[(size_t)i * ns];
   int base = 0;
@@ -798,8 +797,9 @@ __global__ __launch_bounds__(256) void k_desc(NptArgs A, int nconf) {
   }
 }
 
-// dU_s of proposal k = blockIdx.x: the fixed-order sum over its affected
-// rows of (new - cached) surrogate energies
+// Proposal k = blockIdx.x of the round: dU_s, the fixed-order sum over its
+// affected rows of (new - cached) surrogate energies, and the surrogate's
+// Metropolis test.  vflag[k]: bit 0 void, bit 1 volume, bit 2 passed.
 __global__ __launch_bounds__(256) void k_da_sum(NptArgs A) {
   __shared__ double s[256];
   const int n = A.n, k = blockIdx.x, m = A.naff[k];
@@ -812,33 +812,36 @@ __global__ __launch_bounds__(256) void k_da_sum(NptArgs A) {
     if (threadIdx.x < h) s[threadIdx.x] = s[threadIdx.x] + s[threadIdx.x + h];
     __syncthreads();
   }
-  if (threadIdx.x == 0) A.des[k] = s[0];
-}
-
-// the surrogate's Metropolis test in proposal order; the first pass is the
-// round's selection (one thread)
-__global__ void k_da_select(NptArgs A) {
   if (threadIdx.x != 0) return;
-  const int n = A.n;
+  const double des = s[0];
+  A.des[k] = des;
+  const Prop p = make_prop(A, (unsigned long long)A.cnt[8] + k);
   const double L = A.st[0];
-  int sel = -1, seen = 0, vols = 0, voids = 0;
-  for (int k = 0; k < A.K; k++) {
-    const Prop p = make_prop(A, (unsigned long long)A.cnt[8] + k);
-    seen++;
-    vols += p.type;
-    if (A.vflag[k]) {
-      voids++;
-      continue;
-    }
+  int f = (A.vflag[k] & 1) | (p.type ? 2 : 0);
+  if (!(f & 1)) {
     double arg;
     if (p.type) {
       const double Lp = L * p.a;
       const double dv = Lp * Lp * Lp - L * L * L;
-      arg = -(A.des[k] * kI2Jn * kJ2Kn + A.press_k * dv) * A.beta + (double)(n + 1) * p.x;
+      arg = -(des * kI2Jn * kJ2Kn + A.press_k * dv) * A.beta + (double)(n + 1) * p.x;
     } else {
-      arg = -(A.des[k] * kI2Jn * kJ2Kn) * A.beta;
+      arg = -(des * kI2Jn * kJ2Kn) * A.beta;
     }
-    if (arg >= 0.0 || p.u1 < exp(arg)) {
+    if (arg >= 0.0 || p.u1 < exp(arg)) f |= 4;
+  }
+  A.vflag[k] = f;
+}
+
+// the round's selection: the first proposal that passed (one thread)
+__global__ void k_da_select(NptArgs A) {
+  if (threadIdx.x != 0) return;
+  int sel = -1, seen = 0, vols = 0, voids = 0;
+  for (int k = 0; k < A.K; k++) {
+    const int f = A.vflag[k];
+    seen++;
+    vols += (f >> 1) & 1;
+    voids += f & 1;
+    if (f & 4) {
       sel = k;
       break;
     }
@@ -907,13 +910,15 @@ __global__ __launch_bounds__(256) void k_npt_total(NptArgs A, double *out) {
   if (threadIdx.x == 0) *out = u;
 }
 
-// e_i = (1/2) sum_j pinfo(i, j) in j order
-__global__ void k_npt_mol_energy(NptArgs A, double *e) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// e_i = (1/2) sum_j pinfo(i, j): one wave per molecule, lane-strided sums
+// and a fixed tree
+__global__ __launch_bounds__(256) void k_npt_mol_energy(NptArgs A, double *e) {
+  const int lane = threadIdx.x & 63, i = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (i >= A.n) return;
-  double s = 0.0;  // column i of the symmetric matrix: coalesced, j in order
-  for (int j = 0; j < A.n; j++) s = s + A.pinfo[(size_t)j * A.n + i];
-  e[i] = 0.5 * s;
+  double s = 0.0;
+  for (int j = lane; j < A.n; j += 64) s = s + A.pinfo[(size_t)i * A.n + j];
+  s = wave_sum(s);
+  if (lane == 0) e[i] = 0.5 * s;
 }
 
 struct Npt {
@@ -1015,11 +1020,11 @@ static void launch_move(System &s, Npt &p, int da) {
 template <int M>
 static void launch_round(System &s, Npt &p, Mlp &net) {
   NptArgs &a = p.a;
-  k_nbl<<<(a.n + 3) / 4, 256, 0, s.stream>>>(a);
+  k_nbl<<<(a.n + 3) / 4, 256, 0, s.stream>>>(a, 0);
   k_da_prep<<<a.K, 1024, 0, s.stream>>>(a, 0);
   const long long rows = (long long)a.K * a.n;
   k_desc<<<(unsigned)((rows + 3) / 4), 256, 0, s.stream>>>(a, a.K);
-  mlp_launch_forward(net, rows, a.desc, nullptr, a.ys, s.stream);
+  mlp_launch_forward(net, rows, a.desc, nullptr, a.ys, s.stream, a.n, a.naff);
   k_da_sum<<<a.K, 256, 0, s.stream>>>(a);
   k_da_select<<<1, 64, 0, s.stream>>>(a);
   launch_move<M>(s, p, 1);
@@ -1028,7 +1033,7 @@ static void launch_round(System &s, Npt &p, Mlp &net) {
 // features of the current configuration into desc rows 0..n-1
 static void launch_features(System &s, Npt &p) {
   NptArgs &a = p.a;
-  k_nbl<<<(a.n + 3) / 4, 256, 0, s.stream>>>(a);
+  k_nbl<<<(a.n + 3) / 4, 256, 0, s.stream>>>(a, 1);
   k_da_prep<<<1, 1024, 0, s.stream>>>(a, 1);
   k_desc<<<(a.n + 3) / 4, 256, 0, s.stream>>>(a, 1);
 }
@@ -1198,7 +1203,7 @@ int g6_npt_begin(g6_handle h, double temp, double press_bar, uint64_t seed, doub
   G6_CHECK(hipMemcpy(a.com, s->md.com, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice));
   G6_CHECK(hipMemcpy(a.quat, s->md.quat, sizeof(double) * 4 * n, hipMemcpyDeviceToDevice));
   double st0[8] = {s->L, 0, 0, 0, 0, 0, 0, 0};
-  int ist0[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int ist0[8] = {0, 0, 0, 0, 0, 1, 0, 0};  // [5]: the neighbour list is stale
   long long c0[kCnt] = {0};
   G6_CHECK(hipMemcpy(a.st, st0, sizeof(st0), hipMemcpyHostToDevice));
   G6_CHECK(hipMemcpy(a.ist, ist0, sizeof(ist0), hipMemcpyHostToDevice));
@@ -1279,7 +1284,7 @@ int g6_npt_features(g6_handle h, float *x, double *e) {
   launch_features(*s, *p);
   double *de = nullptr;
   G6_CHECK(hipMalloc(&de, sizeof(double) * n));
-  k_npt_mol_energy<<<(s->n + 255) / 256, 256, 0, s->stream>>>(a, de);
+  k_npt_mol_energy<<<(s->n + 3) / 4, 256, 0, s->stream>>>(a, de);
   G6_CHECK(hipGetLastError());
   if (x) G6_CHECK(hipMemcpyAsync(x, a.desc, sizeof(float) * 32 * n, hipMemcpyDeviceToHost, s->stream));
   if (e) G6_CHECK(hipMemcpyAsync(e, de, sizeof(double) * n, hipMemcpyDeviceToHost, s->stream));
