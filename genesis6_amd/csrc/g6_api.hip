@@ -229,7 +229,7 @@ static int alloc_all(System &s) {
   AL(m.nbr_j, sizeof(int) * (size_t)n * m.maxnb);
   AL(m.nbr_code, sizeof(int) * (size_t)n * m.maxnb);
   AL(m.nbr_cnt, sizeof(int) * n);
-  AL(m.blockpart, sizeof(double) * 2 * 16 * s.nblk_md);
+  AL(m.blockpart, sizeof(double) * 2 * kBP * s.nblk_md);
   AL(m.blockek, sizeof(double) * kEkLen * (size_t)pre_waves(n));
   AL(m.frc, sizeof(double) * 6 * n);
   AL(m.nose, sizeof(double) * 9);

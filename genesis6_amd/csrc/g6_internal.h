@@ -234,9 +234,15 @@ inline int main_region_bytes(int maxnb, int rsize) {
   const int mx = sort > comp ? sort : comp;
   return up16(mx > acc ? mx : acc);
 }
+// doubles per block partial of k_md_main: the energy and the pair virial
+// [0..9] (each a compensated block sum, rounded once), pair counts [10..11]
+constexpr int kBP = 16;
 // kinetic partials per k_md_pre wave: sum |dx(1)|^2, sum w_x^2, w_y^2, w_z^2,
 // and the upper triangle xx xy xz yy yz zz of sum dx(1) dx(1)^T
 constexpr int kEkLen = 10;
+// reduction slots of the step close per wave: 12 pair quantities and kEkLen
+// kinetic ones
+constexpr int kRed = 12 + kEkLen;
 // waves of k_md_pre (64 molecules per block of four waves); each writes
 // one kinetic partial
 inline int pre_waves(int n) { return (n + 63) / 64 * 4; }

@@ -85,6 +85,29 @@ __device__ inline double wsum63(double v) {
   return v;
 }
 
+// Compensated sums (Knuth's TwoSum): the pair (s, c) holds s + c, the
+// rounding error of every addition carried in c.
+__device__ inline void two_sum_acc(double &s, double &c, double x) {
+  const double t = s + x;
+  const double bp = t - s;
+  c = c + ((s - (t - bp)) + (x - bp));
+  s = t;
+}
+__device__ inline void dd_add(double &s, double &c, double so, double co) {
+  const double t = s + so;
+  const double bp = t - s;
+  const double e = (s - (t - bp)) + (so - bp);
+  c = (c + co) + e;
+  s = t;
+}
+// rowsum16 on compensated pairs (the same lane pattern)
+__device__ inline void rowsum16_dd(double &s, double &c) {
+  dd_add(s, c, dpp_d<0xB1>(s), dpp_d<0xB1>(c));
+  dd_add(s, c, dpp_d<0x4E>(s), dpp_d<0x4E>(c));
+  dd_add(s, c, dpp_d<0x141>(s), dpp_d<0x141>(c));
+  dd_add(s, c, dpp_d<0x140>(s), dpp_d<0x140>(c));
+}
+
 __device__ inline void wsync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -489,29 +512,44 @@ __global__ __launch_bounds__(256) void k_md_pre_lad(ModelTable mt, const MdRep *
 __device__ inline void close_body(const ModelTable &mt, const MdBuf &b, int n, double dt,
                                   int nblk_pre, const double *bpart, int nblk_main, int ncell,
                                   int nose_active, int mode, int parity, double box_l,
-                                  double (&s_red)[4][12 + kEkLen]) {
+                                  double (&s_red)[4][kRed]) {
   constexpr int NR = 12 + kEkLen;
   const int wid = threadIdx.x >> 6;
-  double a2[NR];
+  {
+    // the pair partials of k_md_main (each rounded once from its compensated
+    // block sum), summed over blocks in a fixed order
+    double ps[12];
 #pragma unroll
-  for (int q = 0; q < NR; q++) a2[q] = 0.0;
-  if (mode & 17) {
+    for (int q = 0; q < 12; q++) ps[q] = 0.0;
+    if (mode & 17) {
 #pragma unroll 4
-    for (int k = threadIdx.x; k < nblk_main; k += 256)
+      for (int k = threadIdx.x; k < nblk_main; k += 256)
 #pragma unroll
-      for (int q = 0; q < 12; q++) a2[q] += bpart[(size_t)k * 16 + q];
+        for (int q = 0; q < 12; q++) ps[q] += bpart[(size_t)k * kBP + q];
+    }
+#pragma unroll
+    for (int q = 0; q < 12; q++) ps[q] = wsum63(ps[q]);
+    if ((threadIdx.x & 63) == 63)
+#pragma unroll
+      for (int q = 0; q < 12; q++) s_red[wid][q] = ps[q];
   }
-  if (mode & 1) {
-#pragma unroll 2
-    for (int k = threadIdx.x; k < nblk_pre; k += 256)
+  {
+    // the kinetic partials of k_md_pre
+    double ek[kEkLen];
 #pragma unroll
-      for (int q = 0; q < kEkLen; q++) a2[12 + q] += b.blockek[(size_t)k * kEkLen + q];
+    for (int q = 0; q < kEkLen; q++) ek[q] = 0.0;
+    if (mode & 1) {
+#pragma unroll 1
+      for (int k = threadIdx.x; k < nblk_pre; k += 256)
+#pragma unroll
+        for (int q = 0; q < kEkLen; q++) ek[q] += b.blockek[(size_t)k * kEkLen + q];
+    }
+#pragma unroll
+    for (int q = 0; q < kEkLen; q++) ek[q] = wsum63(ek[q]);
+    if ((threadIdx.x & 63) == 63)
+#pragma unroll
+      for (int q = 0; q < kEkLen; q++) s_red[wid][12 + q] = ek[q];
   }
-#pragma unroll
-  for (int q = 0; q < NR; q++) a2[q] = wsum63(a2[q]);
-  if ((threadIdx.x & 63) == 63)
-#pragma unroll
-    for (int q = 0; q < NR; q++) s_red[wid][q] = a2[q];
   if (mode & 12) {
     const int p0 = (mode & 8) ? 0 : 1 - parity, p1 = (mode & 8) ? 1 : 1 - parity;
     for (int p = p0; p <= p1; p++) {
@@ -521,9 +559,14 @@ __device__ inline void close_body(const ModelTable &mt, const MdBuf &b, int n, d
     }
   }
   __syncthreads();
+  // totals over the four waves, one quantity per thread, into row 0
+  if (threadIdx.x < NR) {
+    const int q = threadIdx.x;
+    s_red[0][q] = s_red[0][q] + s_red[1][q] + s_red[2][q] + s_red[3][q];
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
-    double r[NR];
-    for (int q = 0; q < NR; q++) r[q] = s_red[0][q] + s_red[1][q] + s_red[2][q] + s_red[3][q];
+    const double *r = s_red[0];
     double *z = b.nose, *th = b.thermo;
     if (mode & 17) {
       th[0] = r[0];
@@ -582,7 +625,7 @@ __global__ __launch_bounds__(256) void k_md_close(ModelTable mt, MdBuf b, int n,
                                                   int nblk_pre, const double *bpart,
                                                   int nblk_main, int ncell, int nose_active,
                                                   int mode, int parity, double box_l) {
-  __shared__ double s_red[4][12 + kEkLen];
+  __shared__ double s_red[4][kRed];
   close_body(mt, b, n, dt, nblk_pre, bpart, nblk_main, ncell, nose_active, mode, parity, box_l,
              s_red);
 }
@@ -590,11 +633,11 @@ __global__ __launch_bounds__(256) void k_md_close(ModelTable mt, MdBuf b, int n,
 // Ladder form: block r closes system r.
 __global__ __launch_bounds__(256) void k_md_close_lad(ModelTable mt, const MdRep *__restrict__ reps,
                                                       int mode, int parity, int src_parity) {
-  __shared__ double s_red[4][12 + kEkLen];
+  __shared__ double s_red[4][kRed];
   const MdRep &r = reps[blockIdx.x];
   const MdBuf b = r.b;
   const int n = r.g.n, nm = r.nblk_main;
-  close_body(mt, b, n, r.dt, (n + 63) / 64 * 4, b.blockpart + (size_t)src_parity * nm * 16, nm,
+  close_body(mt, b, n, r.dt, (n + 63) / 64 * 4, b.blockpart + (size_t)src_parity * nm * kBP, nm,
              r.g.ncell, r.nose_active, mode, parity, r.g.L, s_red);
 }
 
@@ -951,19 +994,20 @@ __device__ __forceinline__ void md_main_body(const ModelTable &mt, const Geom &g
   if (threadIdx.x < 160) {
     // thread (q, seg) sums lanes seg, seg+16, ... of the 256: for each u the
     // 16 threads of a DPP row read 16 consecutive doubles (no bank conflict)
+    // (compensated: TwoSum over the 16 terms and the row)
     const int q = threadIdx.x >> 4, seg = threadIdx.x & 15;
-    double r = 0.0;
+    double r = 0.0, rc = 0.0;
 #pragma unroll
     for (int u = 0; u < 16; u++) {
       const int L = seg + 16 * u;
-      r += reinterpret_cast<const double *>(s_dyn + (size_t)(L >> 6) * reg_bytes)[q * 64 + (L & 63)];
+      two_sum_acc(r, rc,
+                  reinterpret_cast<const double *>(s_dyn + (size_t)(L >> 6) * reg_bytes)[q * 64 + (L & 63)]);
     }
-    r = rowsum16(r);
-    if (seg == 0)
-      b.blockpart[((size_t)parity * nblk + blk) * 16 + q] = 0.5 * r;
+    rowsum16_dd(r, rc);
+    if (seg == 0) b.blockpart[((size_t)parity * nblk + blk) * kBP + q] = 0.5 * (r + rc);
   } else if (threadIdx.x < 162) {
     const int q = threadIdx.x - 160;
-    b.blockpart[((size_t)parity * nblk + blk) * 16 + 10 + q] =
+    b.blockpart[((size_t)parity * nblk + blk) * kBP + 10 + q] =
         0.5 * (s_cnt[0][q] + s_cnt[1][q] + s_cnt[2][q] + s_cnt[3][q]);
   }
 }
@@ -980,7 +1024,7 @@ __global__ __launch_bounds__(256, (main_waves_per_eu<M, R>())) void k_md_main(Mo
   if (ca.mode) {
     // block 0 closes the previous step while the other blocks run the rows
     if (blockIdx.x == 0) {
-      __shared__ double s_close[4][12 + kEkLen];
+      __shared__ double s_close[4][kRed];
       close_body(mt, b, g.n, ca.dt, ca.nblk_pre, ca.bpart, ca.nblk_main, g.ncell, ca.nose_active,
                  ca.mode, parity, g.L, s_close);
       return;
@@ -1007,11 +1051,11 @@ __global__ __launch_bounds__(256, (main_waves_per_eu<M, R>())) void k_md_main_la
   ca.nose_active = reps[m.x].nose_active;
   ca.nblk_pre = (g.n + 63) / 64 * 4;
   ca.nblk_main = nm;
-  ca.bpart = b.blockpart + (size_t)(1 - parity) * nm * 16;
+  ca.bpart = b.blockpart + (size_t)(1 - parity) * nm * kBP;
   ca.dt = reps[m.x].dt;
   if (m.y < 0) {
     if (close_mode) {
-      __shared__ double s_close[4][12 + kEkLen];
+      __shared__ double s_close[4][kRed];
       close_body(mt, b, g.n, ca.dt, ca.nblk_pre, ca.bpart, ca.nblk_main, g.ncell, ca.nose_active,
                  ca.mode, parity, g.L, s_close);
     }
@@ -1032,7 +1076,7 @@ static CloseArgs close_args(System &s, int mode, int src_parity) {
   ca.nose_active = s.nose_active ? 1 : 0;
   ca.nblk_pre = pre_waves(s.n);
   ca.nblk_main = s.nblk_md;
-  ca.bpart = s.md.blockpart + (size_t)src_parity * s.nblk_md * 16;
+  ca.bpart = s.md.blockpart + (size_t)src_parity * s.nblk_md * kBP;
   ca.dt = s.dt;
   return ca;
 }
