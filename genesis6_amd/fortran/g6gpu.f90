@@ -12,6 +12,9 @@ module g6gpu
        G6_ENODEVICE = -4
   integer, parameter :: G6_THERMO_LEN = 32
   integer(c_int), parameter :: G6_LADDER_MAP_XCD = 0, G6_LADDER_MAP_FLAT = 1
+  integer(c_int), parameter :: G6_ORDER_NONE = 0, G6_ORDER_BY_DISTANCE = 1, &
+       G6_ORDER_BY_LABEL = 2
+  integer, parameter :: G6_NEAR = 151
 
   interface
      integer(c_int) function g6_init(h, model, n, box_l, rc, taper, voxa, dt, prec, skin) &
@@ -139,6 +142,45 @@ module g6gpu
        real(c_double), intent(in) :: quat1(*), fx(*), fy(*), fz(*)
        real(c_double), intent(out) :: force(*), torque(*)
      end function g6_collect
+
+     integer(c_int) function g6_neighbors(h, nsite, npair, pair_i, pair_j, ox, oy, oz, &
+          coord, maxnei, order, radius, nnear, near, dx, dy, dz, dd) &
+          bind(C, name="g6_neighbors")
+       import :: c_int, c_int64_t, c_int32_t, c_double
+       integer(c_int64_t), value :: h, npair
+       integer(c_int), value :: nsite, maxnei, order
+       real(c_double), value :: radius
+       integer(c_int32_t), intent(in) :: pair_i(*), pair_j(*)
+       real(c_double), intent(in) :: ox(*), oy(*), oz(*), coord(*)
+       integer(c_int32_t), intent(out) :: nnear(*), near(*)
+       real(c_double), intent(out) :: dx(*), dy(*), dz(*), dd(*)
+     end function g6_neighbors
+
+     integer(c_int) function g6_neighbors_compress(h, nrow, nnear_in, near_in, dx_in, &
+          dy_in, dz_in, dd_in, maxnei, order, radius, nnear, near, dx, dy, dz, dd) &
+          bind(C, name="g6_neighbors_compress")
+       import :: c_int, c_int64_t, c_int32_t, c_double
+       integer(c_int64_t), value :: h
+       integer(c_int), value :: nrow, maxnei, order
+       real(c_double), value :: radius
+       integer(c_int32_t), intent(in) :: nnear_in(*), near_in(*)
+       real(c_double), intent(in) :: dx_in(*), dy_in(*), dz_in(*), dd_in(*)
+       integer(c_int32_t), intent(out) :: nnear(*), near(*)
+       real(c_double), intent(out) :: dx(*), dy(*), dz(*), dd(*)
+     end function g6_neighbors_compress
+
+     integer(c_int) function g6_neighbors_displacement(h, nsite, nnear_in, near_in, dx_in, &
+          dy_in, dz_in, dd_in, order, target, ddx, ddy, ddz, nmodified, index, nnear, near, &
+          dx, dy, dz, dd) bind(C, name="g6_neighbors_displacement")
+       import :: c_int, c_int64_t, c_int32_t, c_double
+       integer(c_int64_t), value :: h
+       integer(c_int), value :: nsite, order, target
+       real(c_double), value :: ddx, ddy, ddz
+       integer(c_int32_t), intent(in) :: nnear_in(*), near_in(*)
+       real(c_double), intent(in) :: dx_in(*), dy_in(*), dz_in(*), dd_in(*)
+       integer(c_int32_t), intent(out) :: nmodified, index(*), nnear(*), near(*)
+       real(c_double), intent(out) :: dx(*), dy(*), dz(*), dd(*)
+     end function g6_neighbors_displacement
 
      integer(c_int) function g6_lattice_start(n, box_l, seed, com, quat1) &
           bind(C, name="g6_lattice_start")

@@ -15,6 +15,8 @@ LIB = os.path.join(HERE, "lib", "libg6gpu.so")
 TIP4P, TIP5P, ST2 = 0, 1, 2
 FP64, FP32 = 0, 1
 THERMO_LEN = 32
+ORDER_NONE, ORDER_BY_DISTANCE, ORDER_BY_LABEL = 0, 1, 2
+NEAR = 151          # entries of one neighbour array (MAXNEAR + 1)
 ERRORS = {-1: "HIP runtime error", -2: "invalid argument",
           -3: "pair capacity exceeded", -4: "no HIP device"}
 
@@ -58,6 +60,13 @@ def lib():
                                      _dp, _dp]
         L.g6_site_positions.argtypes = [_H, _dp, _dp, _dp, _dp, _dp]
         L.g6_collect.argtypes = [_H, _dp, _dp, _dp, _dp, _dp, _dp]
+        tab = [_ip, _ip, _dp, _dp, _dp, _dp]
+        L.g6_neighbors.argtypes = [_H, C.c_int, C.c_int64, _ip, _ip, _dp, _dp, _dp, _dp,
+                                   C.c_int, C.c_int, C.c_double] + tab
+        L.g6_neighbors_compress.argtypes = [_H, C.c_int] + tab + [
+            C.c_int, C.c_int, C.c_double] + tab
+        L.g6_neighbors_displacement.argtypes = [_H, C.c_int] + tab + [
+            C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _ip, _ip] + tab
         L.g6_lattice_start.argtypes = [C.c_int, C.c_double, C.c_uint32, _dp, _dp]
         L.g6_model_nsite.argtypes = [C.c_int]
         L.g6_mc_begin.argtypes = [_H, C.c_double, C.c_uint32, _dp, _dp]
@@ -315,6 +324,67 @@ class System:
         check(lib().g6_collect(self.h, _c(quat1.T).ravel(), _c(forces[:, 0]), _c(forces[:, 1]),
                                _c(forces[:, 2]), F, T), "g6_collect")
         return F.reshape(3, n).T.copy(), T.reshape(3, n).T.copy()
+
+    # ---- neighbour arrays (Neighbor.F90) ----
+    def neighbors(self, nsite, pi, pj, shifts, coord, maxnei, order, radius=None):
+        """neighbors_all2 over a pair list: a NeighborTable of nsite arrays."""
+        k = len(pi)
+        out = NeighborTable.empty(nsite)
+        sh = _c(shifts).reshape(-1, 3)
+        check(lib().g6_neighbors(self.h, nsite, k, _c(pi, np.int32), _c(pj, np.int32),
+                                 _c(sh[:, 0]), _c(sh[:, 1]), _c(sh[:, 2]),
+                                 _c(np.asarray(coord).T).ravel(), maxnei, order,
+                                 radius if radius else 0.0, *out.arrays()), "g6_neighbors")
+        return out
+
+    def neighbors_compress(self, tab, maxnei, order, radius=None):
+        """neighbor1_compress2 of every array of tab."""
+        out = NeighborTable.empty(len(tab.nnear))
+        check(lib().g6_neighbors_compress(self.h, len(tab.nnear), *tab.arrays(), maxnei, order,
+                                          radius if radius else 0.0, *out.arrays()),
+              "g6_neighbors_compress")
+        return out
+
+    def neighbors_displacement(self, tab, order, target, delta):
+        """neighbors_displacement: (molecule indices, NeighborTable) of the
+        arrays that molecule `target` moving by delta changes."""
+        out = NeighborTable.empty(NEAR + 1)
+        nmod = np.zeros(1, np.int32)
+        idx = np.zeros(NEAR + 1, np.int32)
+        check(lib().g6_neighbors_displacement(self.h, len(tab.nnear), *tab.arrays(), order,
+                                              target, float(delta[0]), float(delta[1]),
+                                              float(delta[2]), nmod, idx, *out.arrays()),
+              "g6_neighbors_displacement")
+        m = int(nmod[0])
+        return idx[:m].copy(), out.rows(m)
+
+
+class NeighborTable:
+    """A table of neighbour arrays: nnear (rows,), near/dx/dy/dz/dd
+    (rows, NEAR); near holds 0-based molecule indices."""
+
+    def __init__(self, nnear, near, dx, dy, dz, dd):
+        self.nnear, self.near = nnear, near
+        self.dx, self.dy, self.dz, self.dd = dx, dy, dz, dd
+
+    @classmethod
+    def empty(cls, rows):
+        return cls(np.zeros(rows, np.int32), np.zeros((rows, NEAR), np.int32),
+                   *[np.zeros((rows, NEAR)) for _ in range(4)])
+
+    def arrays(self):
+        return [_c(self.nnear, np.int32), _c(self.near, np.int32), _c(self.dx), _c(self.dy),
+                _c(self.dz), _c(self.dd)]
+
+    def rows(self, m):
+        return NeighborTable(*[a[:m].copy() for a in (self.nnear, self.near, self.dx, self.dy,
+                                                      self.dz, self.dd)])
+
+    def entries(self, r):
+        """(near, dx, dy, dz, dd) of the filled part of array r."""
+        c = int(self.nnear[r])
+        return (self.near[r, :c], self.dx[r, :c], self.dy[r, :c], self.dz[r, :c],
+                self.dd[r, :c])
 
 
 class Ladder:

@@ -132,6 +132,49 @@ int g6_collect(g6_handle h, const double *quat1, const double *fx,
                const double *fy, const double *fz, double *force,
                double *torque);
 
+/* ---- neighbour arrays of the order parameters (Neighbor.F90) ---------- */
+/* orders of neighbors_module (Neighbor.F90:5) */
+enum { G6_ORDER_NONE = 0, G6_ORDER_BY_DISTANCE = 1, G6_ORDER_BY_LABEL = 2 };
+/* entries per array: MAXNEAR + 1 (Neighbor.F90:4, 8-13) */
+#define G6_NEAR 151
+/* A table of arrays is nnear[rows] and near, dx, dy, dz, dd[rows * G6_NEAR]
+ * (Fortran near(G6_NEAR, rows)); near holds 0-based molecule indices and
+ * (dx, dy, dz) the vector from the array's molecule to that neighbour.
+ * radius <= 0 stands for the absent optional argument.  Each array is the
+ * reference's: appended (ORDER_NONE) or sorted by distance or label with
+ * ties in insertion order, cut to maxnei (0: no limit). */
+/* neighbors_all2 (Neighbor.F90:91-159) over a pair list of nsite molecules
+ * with centres coord[nsite*3] (Fortran coord(nsite,3)); nnear/near/dx/dy/
+ * dz/dd have nsite rows.  G6_ECAPACITY when an array would need more than
+ * G6_NEAR entries (the reference then writes past its arrays).  Uses the
+ * handle's device and stream only. */
+int g6_neighbors(g6_handle h, int nsite, int64_t npair, const int32_t *pair_i,
+                 const int32_t *pair_j, const double *ox, const double *oy,
+                 const double *oz, const double *coord, int maxnei, int order,
+                 double radius, int32_t *nnear, int32_t *near, double *dx,
+                 double *dy, double *dz, double *dd);
+/* neighbor1_compress2 (Neighbor.F90:195-215) of each of nrow arrays: one
+ * call replaces the reference's loop over molecules. */
+int g6_neighbors_compress(g6_handle h, int nrow, const int32_t *nnear_in,
+                          const int32_t *near_in, const double *dx_in,
+                          const double *dy_in, const double *dz_in,
+                          const double *dd_in, int maxnei, int order,
+                          double radius, int32_t *nnear, int32_t *near,
+                          double *dx, double *dy, double *dz, double *dd);
+/* neighbors_displacement (Neighbor.F90:401-457): molecule `target` of a
+ * table of nsite arrays built with `order` moves by (ddx, ddy, ddz); writes
+ * the *nmodified updated arrays (the moved molecule's first, then those of
+ * its neighbours that list it, in its array's order) and their molecule
+ * indices index[]; the output table needs nnear_in[target] + 1 <= 152 rows. */
+int g6_neighbors_displacement(g6_handle h, int nsite, const int32_t *nnear_in,
+                              const int32_t *near_in, const double *dx_in,
+                              const double *dy_in, const double *dz_in,
+                              const double *dd_in, int order, int target,
+                              double ddx, double ddy, double ddz,
+                              int32_t *nmodified, int32_t *index, int32_t *nnear,
+                              int32_t *near, double *dx, double *dy, double *dz,
+                              double *dd);
+
 /* ---- Monte Carlo: PairInfo and the mc2 trial loop ------------------- */
 /* Starts an MC run on the handle's current configuration (COMs and order-1
  * quaternions of the last g6_upload_state): site positions, the PairInfo

@@ -1368,3 +1368,159 @@ void g6o_npt_run(int model, int n, double rc, double taper, double voxa, double 
   free(x); free(y); free(z); free(x2); free(y2); free(z2);
   free(com2); free(quat2); free(ep); free(ep2); free(slot); free(mem);
 }
+
+/* ------------------------------------------------------------------------
+ * Neighbour arrays (Neighbor.F90).
+ * ---------------------------------------------------------------------- */
+
+/* insert (Neighbor.F90:20-88).  ORDER_NONE appends; otherwise the new entry
+ * goes before the first entry with a larger key (distance or label), the
+ * array growing by one up to `limit` (0: no limit) and losing its last entry
+ * when full; an entry whose place is past the limit is dropped. */
+static int nb_insert(g6o_nbtab t, int row, int order, int label, double dx, double dy,
+                     double dz, double dd, int limit) {
+  int *nn = &t.nnear[row];
+  const long b = (long)row * G6O_NEAR;
+  int pos, last;
+  if (order == 0) {
+    if (*nn >= G6O_NEAR) return 1;
+    pos = (*nn)++;
+    t.near[b + pos] = label;
+    t.dx[b + pos] = dx;
+    t.dy[b + pos] = dy;
+    t.dz[b + pos] = dz;
+    t.dd[b + pos] = dd;
+    return 0;
+  }
+  for (pos = 0; pos < *nn; pos++) {
+    if (order == 1 ? dd < t.dd[b + pos] : label < t.near[b + pos]) break;
+  }
+  if (limit != 0 && pos >= limit) return 0;
+  last = *nn;          /* index the shifted tail ends at */
+  if (limit != 0 && last + 1 > limit) last = limit - 1;
+  else {
+    if (*nn >= G6O_NEAR) return 1;
+    (*nn)++;
+  }
+  for (int j = last; j > pos; j--) {
+    t.near[b + j] = t.near[b + j - 1];
+    t.dx[b + j] = t.dx[b + j - 1];
+    t.dy[b + j] = t.dy[b + j - 1];
+    t.dz[b + j] = t.dz[b + j - 1];
+    t.dd[b + j] = t.dd[b + j - 1];
+  }
+  t.near[b + pos] = label;
+  t.dx[b + pos] = dx;
+  t.dy[b + pos] = dy;
+  t.dz[b + pos] = dz;
+  t.dd[b + pos] = dd;
+  return 0;
+}
+
+int g6o_neighbors_all2(int nsite, long npair, const int *pi, const int *pj, const double *ox,
+                       const double *oy, const double *oz, const double *coord, int maxnei,
+                       int order, double radius, g6o_nbtab out) {
+  int err = 0;
+  for (int i = 0; i < nsite; i++) out.nnear[i] = 0;
+  for (long k = 0; k < npair; k++) {
+    const int i = pi[k], j = pj[k];
+    double dx = coord[3 * i] - coord[3 * j];
+    double dy = coord[3 * i + 1] - coord[3 * j + 1];
+    double dz = coord[3 * i + 2] - coord[3 * j + 2];
+    dx -= ox[k];
+    dy -= oy[k];
+    dz -= oz[k];
+    const double dd = dx * dx + dy * dy + dz * dz;
+    if (radius > 0 && radius * radius < dd) continue;
+    err |= nb_insert(out, i, order, j, -dx, -dy, -dz, dd, maxnei);
+    err |= nb_insert(out, j, order, i, dx, dy, dz, dd, maxnei);
+  }
+  return err;
+}
+
+int g6o_neighbors_compress(int nrow, g6o_nbtab in, int maxnei, int order, double radius,
+                           g6o_nbtab out) {
+  int err = 0;
+  for (int r = 0; r < nrow; r++) {
+    const long b = (long)r * G6O_NEAR;
+    out.nnear[r] = 0;
+    for (int t = 0; t < in.nnear[r]; t++) {
+      if (radius > 0 && radius * radius < in.dd[b + t]) continue;
+      err |= nb_insert(out, r, order, in.near[b + t], in.dx[b + t], in.dy[b + t],
+                       in.dz[b + t], in.dd[b + t], maxnei);
+    }
+  }
+  return err;
+}
+
+/* neighbors_sort (Neighbor.F90:279-333): insertion sort, an entry moving in
+ * front of the first entry with a strictly larger key (stable) */
+static void nb_sort(g6o_nbtab t, int row, int order) {
+  const long b = (long)row * G6O_NEAR;
+  const int n = t.nnear[row];
+  if (order == 0) return;
+  for (int i = 1; i < n; i++) {
+    int j;
+    for (j = 0; j < i; j++)
+      if (order == 1 ? t.dd[b + i] < t.dd[b + j] : t.near[b + i] < t.near[b + j]) break;
+    if (j == i) continue;
+    const int lab = t.near[b + i];
+    const double x = t.dx[b + i], y = t.dy[b + i], z = t.dz[b + i], d = t.dd[b + i];
+    for (int k = i; k > j; k--) {
+      t.near[b + k] = t.near[b + k - 1];
+      t.dx[b + k] = t.dx[b + k - 1];
+      t.dy[b + k] = t.dy[b + k - 1];
+      t.dz[b + k] = t.dz[b + k - 1];
+      t.dd[b + k] = t.dd[b + k - 1];
+    }
+    t.near[b + j] = lab;
+    t.dx[b + j] = x;
+    t.dy[b + j] = y;
+    t.dz[b + j] = z;
+    t.dd[b + j] = d;
+  }
+}
+
+static void nb_copy_row(g6o_nbtab to, int rt, g6o_nbtab from, int rf) {
+  const long bt = (long)rt * G6O_NEAR, bf = (long)rf * G6O_NEAR;
+  to.nnear[rt] = from.nnear[rf];
+  memcpy(to.near + bt, from.near + bf, sizeof(int) * G6O_NEAR);
+  memcpy(to.dx + bt, from.dx + bf, sizeof(double) * G6O_NEAR);
+  memcpy(to.dy + bt, from.dy + bf, sizeof(double) * G6O_NEAR);
+  memcpy(to.dz + bt, from.dz + bf, sizeof(double) * G6O_NEAR);
+  memcpy(to.dd + bt, from.dd + bf, sizeof(double) * G6O_NEAR);
+}
+
+int g6o_neighbors_displacement(g6o_nbtab in, int order, int target, double ddx, double ddy,
+                               double ddz, int *nmod, int *idx, g6o_nbtab out) {
+  /* the moved molecule: every relative vector changes */
+  nb_copy_row(out, 0, in, target);
+  idx[0] = target;
+  for (int t = 0; t < out.nnear[0]; t++) {
+    out.dx[t] -= ddx;
+    out.dy[t] -= ddy;
+    out.dz[t] -= ddz;
+    out.dd[t] = out.dx[t] * out.dx[t] + out.dy[t] * out.dy[t] + out.dz[t] * out.dz[t];
+  }
+  /* its neighbours that list it: the first such entry moves by +delta */
+  int m = 1;
+  for (int t = 0; t < out.nnear[0]; t++) {
+    const int k = out.near[t];
+    const long b = (long)k * G6O_NEAR;
+    int p;
+    for (p = 0; p < in.nnear[k]; p++)
+      if (in.near[b + p] == target) break;
+    if (p == in.nnear[k]) continue;
+    nb_copy_row(out, m, in, k);
+    idx[m] = k;
+    const long o = (long)m * G6O_NEAR + p;
+    out.dx[o] += ddx;
+    out.dy[o] += ddy;
+    out.dz[o] += ddz;
+    out.dd[o] = out.dx[o] * out.dx[o] + out.dy[o] * out.dy[o] + out.dz[o] * out.dz[o];
+    m++;
+  }
+  *nmod = m;
+  for (int r = 0; r < m; r++) nb_sort(out, r, order);
+  return 0;
+}

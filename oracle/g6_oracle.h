@@ -150,6 +150,28 @@ void g6o_npt_run(int model, int n, double rc, double taper, double voxa, double 
                  double dlnv_max, double p_volume, double *L, double *com, double *quat,
                  double *U, long long *index, int nmoves, long long *counts, int *accepted);
 
+/* Neighbour arrays (Neighbor.F90).  A table of n arrays: nnear[n] and
+ * near/dx/dy/dz/dd[n][G6O_NEAR] (G6O_NEAR = MAXNEAR+1 = 151 entries per
+ * array), 0-based labels.  radius <= 0 means the argument is absent.
+ * Return 0, or 1 when an array would exceed G6O_NEAR entries (the
+ * reference then writes past its arrays). */
+#define G6O_NEAR 151
+typedef struct {
+  int *nnear, *near;
+  double *dx, *dy, *dz, *dd;
+} g6o_nbtab;
+/* neighbors_all2 (Neighbor.F90:91-159) over a pair list */
+int g6o_neighbors_all2(int nsite, long npair, const int *pi, const int *pj, const double *ox,
+                       const double *oy, const double *oz, const double *coord, int maxnei,
+                       int order, double radius, g6o_nbtab out);
+/* neighbor1_compress2 (Neighbor.F90:195-215) applied to arrays 0..nrow-1 */
+int g6o_neighbors_compress(int nrow, g6o_nbtab in, int maxnei, int order, double radius,
+                           g6o_nbtab out);
+/* neighbors_displacement (Neighbor.F90:401-457): *nmod arrays and their
+ * molecule indices idx[] */
+int g6o_neighbors_displacement(g6o_nbtab in, int order, int target, double ddx, double ddy,
+                               double ddz, int *nmod, int *idx, g6o_nbtab out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -59,6 +59,71 @@ class MD(C.Structure):
                 ("npair", C.c_long)]
 
 
+NEAR = 151   # G6O_NEAR
+
+
+class NbTab(C.Structure):
+    _fields_ = [("nnear", C.POINTER(C.c_int)), ("near", C.POINTER(C.c_int))] + [
+        (k, C.POINTER(C.c_double)) for k in ("dx", "dy", "dz", "dd")]
+
+
+class Neighbors:
+    """A table of neighbour arrays (numpy): nnear (rows,), near/dx/dy/dz/dd
+    (rows, NEAR), 0-based labels."""
+
+    def __init__(self, rows):
+        self.nnear = np.zeros(rows, np.int32)
+        self.near = np.zeros((rows, NEAR), np.int32)
+        self.dx, self.dy, self.dz, self.dd = [np.zeros((rows, NEAR)) for _ in range(4)]
+
+    def c(self):
+        ip, dp = C.POINTER(C.c_int), C.POINTER(C.c_double)
+        return NbTab(self.nnear.ctypes.data_as(ip), self.near.ctypes.data_as(ip),
+                     *[a.ctypes.data_as(dp) for a in (self.dx, self.dy, self.dz, self.dd)])
+
+    @classmethod
+    def of(cls, nnear, near, dx, dy, dz, dd):
+        t = cls(len(nnear))
+        t.nnear[:] = nnear
+        for name, a in (("near", near), ("dx", dx), ("dy", dy), ("dz", dz), ("dd", dd)):
+            getattr(t, name)[:] = a
+        return t
+
+
+def neighbors_all2(nsite, pi, pj, shifts, coord, maxnei, order, radius=None):
+    """neighbors_all2 (Neighbor.F90:91-159); returns (Neighbors, overflow)."""
+    t = Neighbors(nsite)
+    sh = np.ascontiguousarray(shifts, np.float64).reshape(-1, 3)
+    err = lib().g6o_neighbors_all2(
+        nsite, len(pi), np.ascontiguousarray(pi, np.int32), np.ascontiguousarray(pj, np.int32),
+        np.ascontiguousarray(sh[:, 0]), np.ascontiguousarray(sh[:, 1]),
+        np.ascontiguousarray(sh[:, 2]), np.ascontiguousarray(coord, np.float64).ravel(),
+        maxnei, order, radius if radius else 0.0, t.c())
+    return t, bool(err)
+
+
+def neighbors_compress(tab, maxnei, order, radius=None):
+    """neighbor1_compress2 (Neighbor.F90:195-215) of every array of tab."""
+    t = Neighbors(len(tab.nnear))
+    lib().g6o_neighbors_compress(len(tab.nnear), tab.c(), maxnei, order,
+                                 radius if radius else 0.0, t.c())
+    return t
+
+
+def neighbors_displacement(tab, order, target, delta):
+    """neighbors_displacement (Neighbor.F90:401-457): (indices, Neighbors)."""
+    t = Neighbors(NEAR + 1)
+    idx = np.zeros(NEAR + 1, np.int32)
+    nm = C.c_int(0)
+    lib().g6o_neighbors_displacement(tab.c(), order, target, float(delta[0]), float(delta[1]),
+                                     float(delta[2]), C.byref(nm), idx, t.c())
+    m = nm.value
+    out = Neighbors(m)
+    for name in ("nnear", "near", "dx", "dy", "dz", "dd"):
+        getattr(out, name)[:] = getattr(t, name)[:m]
+    return idx[:m].copy(), out
+
+
 def _declare(L):
     L.g6o_model_get.argtypes = [C.c_int, C.POINTER(Model)]
     L.g6o_constants.argtypes = [_dp]
@@ -116,6 +181,11 @@ def _declare(L):
                               C.POINTER(C.c_double), C.POINTER(C.c_longlong), C.c_int,
                               np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"),
                               _ip]
+    L.g6o_neighbors_all2.argtypes = [C.c_int, C.c_long, _ip, _ip, _dp, _dp, _dp, _dp,
+                                     C.c_int, C.c_int, C.c_double, NbTab]
+    L.g6o_neighbors_compress.argtypes = [C.c_int, NbTab, C.c_int, C.c_int, C.c_double, NbTab]
+    L.g6o_neighbors_displacement.argtypes = [NbTab, C.c_int, C.c_int, C.c_double, C.c_double,
+                                             C.c_double, C.POINTER(C.c_int), _ip, NbTab]
 
 
 def model(mid):

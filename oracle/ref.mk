@@ -27,7 +27,7 @@ OBJS = Umbrella Error PhysConst CommonModule RandomModule Graph Replica MDReplic
 
 OBJFILES = $(addprefix $(OUT)/,$(addsuffix .o,$(OBJS))) $(OUT)/mtrnd.o
 
-all: $(OUT)/refdump
+all: $(OUT)/refdump $(OUT)/refneigh
 
 # The reference MD driver (about 300 MB of static arrays); built only on
 # demand by tests/golden/make_golden.py, which removes it afterwards.
@@ -51,6 +51,9 @@ $(OUT)/%.o: $(REF)/%.c | $(OUT)
 
 $(OUT)/refdump: oracle/refdump.F90 $(OUT)/objs.stamp $(COBJS)
 	$(REFFC) $(FFLAGS) oracle/refdump.F90 $(OBJFILES) $(COBJS) -o $@
+
+$(OUT)/refneigh: oracle/refneigh.F90 $(OUT)/objs.stamp $(COBJS)
+	$(REFFC) $(FFLAGS) oracle/refneigh.F90 $(OBJFILES) $(COBJS) -o $@
 
 # The reference MD driver itself.
 $(OUT)/Mixture: $(OUT)/objs.stamp $(COBJS)
