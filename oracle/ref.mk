@@ -33,6 +33,12 @@ all: $(OUT)/refdump $(OUT)/refneigh
 # demand by tests/golden/make_golden.py, which removes it afterwards.
 mixture: $(OUT)/Mixture
 
+# The reference MC program mc2 (about 280 MB of static arrays; its link line
+# is Makefile.rules:32); built on demand by tests/golden/make_mc2.py, which
+# removes it afterwards.
+MC2OBJS = Tetrahedrality Set DistribModule Triangle q6 Q6Local mc2util
+mc2: $(OUT)/mc2
+
 $(OUT):
 	mkdir -p $(OUT)
 
@@ -54,6 +60,17 @@ $(OUT)/refdump: oracle/refdump.F90 $(OUT)/objs.stamp $(COBJS)
 
 $(OUT)/refneigh: oracle/refneigh.F90 $(OUT)/objs.stamp $(COBJS)
 	$(REFFC) $(FFLAGS) oracle/refneigh.F90 $(OBJFILES) $(COBJS) -o $@
+
+$(OUT)/mc2.stamp: $(OUT)/objs.stamp
+	set -e; for o in $(MC2OBJS); do \
+	  $(REFFC) $(FFLAGS) -c $(REF)/$$o.F90 -o $(OUT)/$$o.o; done
+	touch $@
+
+$(OUT)/mc2: $(OUT)/mc2.stamp $(COBJS) $(OUT)/om.o
+	$(REFFC) $(FFLAGS) -c $(REF)/mc2.F90 -o $(OUT)/mc2.o
+	$(REFFC) $(FFLAGS) $(OUT)/mc2.o $(OBJFILES) $(addprefix $(OUT)/,$(addsuffix .o,$(MC2OBJS))) \
+	  $(OUT)/om.o $(COBJS) -o $@
+	rm -f $(OUT)/mc2.o
 
 # The reference MD driver itself.
 $(OUT)/Mixture: $(OUT)/objs.stamp $(COBJS)
