@@ -65,7 +65,18 @@ def main():
                         "com_%d" % c: com, "quat_%d" % c: quat})
             print("case", c, "energies", eps)
     finally:
+        # the binary and mc2's own module objects (DistribModule.o alone
+        # holds 50 MB of static arrays) are not kept
         os.remove(EXE)
+        for m in ("Tetrahedrality", "Set", "DistribModule", "Triangle", "q6", "Q6Local",
+                  "mc2util"):
+            for ext in (".o",):
+                f = os.path.join(ROOT, "oracle", "_ref", m + ext)
+                if os.path.exists(f):
+                    os.remove(f)
+        stamp = os.path.join(ROOT, "oracle", "_ref", "mc2.stamp")
+        if os.path.exists(stamp):
+            os.remove(stamp)
     np.savez_compressed(os.path.join(HERE, "mc2_tip4p_216.npz"), **out)
 
 

@@ -203,3 +203,19 @@ def test_mc_energy_bookkeeping_and_quaternions():
     assert np.abs(P - r.ep.reshape(216, 216)).max() <= 1e-9
     assert np.abs((r.quat ** 2).sum(0) - 1.0).max() < 1e-14
     assert r.delta.min() >= np.float32(0.05)
+
+
+def test_nve_1000_steps_match_reference_driver(golden):
+    """1000 NVE steps of 216 TIP4P from an equilibrated start: the oracle's
+    conserved H equals the reference driver's at every step (the reference
+    itself drifts upward by about 25 kJ/mol per ps here)."""
+    g = golden("md_tip4p_216_nve1000")
+    md = O.MDRun(O.TIP4P, g["com"], g["quat"].T.copy(), float(g["L"]), float(g["rc"]),
+                 float(g["taper"]), float(g["dt"]))
+    md.vreset(300.0, int(g["seed"]))
+    H = []
+    for _ in range(int(g["nsteps"])):
+        md.step()
+        H.append(md.hamiltonian())
+    ref = g["logd"][:, 12]
+    assert np.abs(np.array(H) - ref).max() <= 1e-12 * np.abs(ref).max()
