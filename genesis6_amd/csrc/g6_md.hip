@@ -936,13 +936,16 @@ __device__ __forceinline__ void md_main_body(const ModelTable &mt, const Geom &g
     auto eval = [&](int j, R d0, R d1, R d2, const R (&rj)[T::NS][3]) {
       R Fp[3], Tp[3];
       const R e = pair_on_i<M, R>(pk, d0 * d0 + d1 * d1 + d2 * d2, d0, d1, d2, ri, rj, Fp, Tp);
-      F[0] += (double)Fp[0]; F[1] += (double)Fp[1]; F[2] += (double)Fp[2];
+      F[0] += (double)Fp[0]; F[1] += (double)Fp[1]; F[2] += (double)Fp[2];  // FP64 sums
       Ti[0] += (double)Tp[0]; Ti[1] += (double)Tp[1]; Ti[2] += (double)Tp[2];
       ep += (double)e;
-      const double e0 = (double)d0, e1 = (double)d1, e2 = (double)d2;
-      v[0] += e0 * (double)Fp[0]; v[1] += e0 * (double)Fp[1]; v[2] += e0 * (double)Fp[2];
-      v[3] += e1 * (double)Fp[0]; v[4] += e1 * (double)Fp[1]; v[5] += e1 * (double)Fp[2];
-      v[6] += e2 * (double)Fp[0]; v[7] += e2 * (double)Fp[1]; v[8] += e2 * (double)Fp[2];
+      // pair virial d (x) F, fused multiply-adds in FP64
+      const double dd3[3] = {(double)d0, (double)d1, (double)d2};
+      const double ff3[3] = {(double)Fp[0], (double)Fp[1], (double)Fp[2]};
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) v[3 * a + c] = __builtin_fma(dd3[a], ff3[c], v[3 * a + c]);
     };
     for (int k0 = 0; k0 < np; k0 += 128) {
       const int ka = k0 + lane, kb = k0 + 64 + lane;

@@ -8,6 +8,15 @@
 ! @ID08 model | @NX4A n, then n lines x y z a b c d.
 ! Command line: g6md [fp64|fp32] [skin]   (default fp64, skin 0: the
 ! reference's list rebuilt every step).
+!
+! Restart files are Mixture's: a binary fort.12 in the working directory,
+! if present, is read first (the records Mixture.F90:400-577 reads: @RCOA
+! @BOX3 @BOOK @VOXA @NLOG @DTPS @ASTP @NOS2 @TEST @MDLP @NCMP @TIE4 @FIXC
+! @ID08 @WTG2), the text deck on standard input may then override any tag,
+! and the run continues from the step count @ASTP with the full Gear state
+! of @WTG2.  At the end the state is written to fort.13 with the records
+! and layout Mixture writes there (Mixture.F90:2530-2598,
+! RigidModule.F90:801-828): angular velocities in external units (w dt).
 program g6md
   use, intrinsic :: iso_c_binding
   use g6gpu
@@ -16,6 +25,8 @@ program g6md
   real(c_double) :: dt = 0.002d0, rc = 8.5d0, taper = 0.5d0, box = 0d0, voxa(3) = 0d0
   real(c_double) :: vtemp = -1d0, skin = 0d0, nose(9), th(G6_THERMO_LEN)
   integer :: nsteps = 0, nlog = 0, ios, i, done, chunk, target
+  integer :: nstep0 = 0
+  logical :: restart, fixc = .false.
   integer(c_int) :: model = -1, prec = G6_FP64, n = 0
   integer(c_int32_t) :: vseed = 0
   integer(c_int64_t) :: h
@@ -31,6 +42,9 @@ program g6md
      call get_command_argument(2, arg)
      read(arg, *) skin
   end if
+
+  inquire(file='fort.12', exist=restart)
+  if (restart) call read_restart()
 
   do
      read(5, '(a)', iostat=ios) line
@@ -70,6 +84,7 @@ program g6md
         end select
      case ('@NX4A')
         read(5, *) n
+        if (allocated(com)) deallocate(com, quat, dx, w)
         allocate(com(n, 3), quat(n, 4, 5), dx(n, 3, 5), w(n, 3, 5))
         quat = 0d0
         dx = 0d0
@@ -87,7 +102,7 @@ program g6md
      write(0, '(a)') 'g6md: the deck needs @ID08, @NX4A and @BXLA'
      error stop 2
   end if
-  if (voxa(1) <= 0d0) voxa(1) = rc
+  if (voxa(1) <= 0d0) voxa = rc
 
   call g6_check(g6_init(h, model, n, box, rc, taper, voxa(1), dt, prec, skin), 'g6_init')
   call g6_check(g6_upload_state(h, com, quat, dx, w), 'g6_upload_state')
@@ -97,30 +112,160 @@ program g6md
   end if
   if (vtemp >= 0d0) call g6_check(g6_vreset(h, vtemp, vseed), 'g6_vreset')
 
-  ! Mixture counts nstep from 0 and logs when mod(nstep, nlog) = 0 after
-  ! the step numbered nstep + 1 (Mixture.F90:360, 2429-2490, 2524): the
-  ! log lines come after steps 1, nlog + 1, 2 nlog + 1, ...
+  ! Mixture runs @MDLP more steps from the count nstep (0, or @ASTP of a
+  ! restart) and logs when mod(nstep, nlog) = 0 after the step that starts
+  ! at nstep, before nstep is advanced (Mixture.F90:1513, 2429-2490, 2524)
   done = 0
   do while (done < nsteps)
      chunk = nsteps - done
      if (nlog > 0) then
-        target = (done / nlog) * nlog + 1
-        if (target <= done) target = target + nlog
+        ! the next step k (1-based in this run) with mod(nstep0 + k - 1, nlog) = 0
+        target = done + 1 + modulo(-(nstep0 + done), nlog)
         chunk = min(chunk, target - done)
      end if
      call g6_check(g6_step(h, int(chunk, c_int)), 'g6_step')
      done = done + chunk
      if (nlog > 0) then
-        if (mod(done - 1, nlog) == 0) then
+        if (modulo(nstep0 + done - 1, nlog) == 0) then
            call g6_check(g6_get_thermo(h, th), 'g6_get_thermo')
-           call write_logd(done - 1, th)
+           call write_logd(nstep0 + done - 1, th)
         end if
      end if
   end do
   call g6_check(g6_download_state(h, com, quat, dx, w), 'g6_download_state')
+  if (has_nose) call g6_check(g6_get_nose(h, nose), 'g6_get_nose')
   call g6_check(g6_finalize(h), 'g6_finalize')
+  call write_restart()
 
 contains
+
+  ! the binary restart of a previous run (Mixture's fort.13 format)
+  subroutine read_restart()
+    character(len=5) :: btag
+    character(len=8) :: id
+    integer :: ib, nt, nb, m, c, k
+    real(8) :: d3(3), dbl, d9(9)
+    open(12, file='fort.12', form='unformatted', access='sequential', status='old')
+    do
+       read(12, iostat=ios) btag
+       if (ios /= 0) exit
+       select case (btag)
+       case ('@RCOA')
+          read(12) rc, taper             ! co%out, co%out - co%in
+       case ('@BOX3')
+          read(12) d3
+          box = d3(1)
+       case ('@BXLA')
+          read(12) box
+       case ('@BOOK')
+          read(12) ib, dbl               ! interval, margin: not used here
+       case ('@VOXA')
+          read(12) voxa
+       case ('@NLOG')
+          read(12) nlog
+       case ('@DTPS')
+          read(12) dt
+       case ('@ASTP')
+          read(12) nstep0
+       case ('@NOS2')
+          read(12) d9
+          nose = d9
+          has_nose = .true.
+       case ('@TEST')
+       case ('@MDLP')
+          read(12) nsteps
+       case ('@NCMP')
+          read(12) nt
+          if (nt /= 1) then
+             write(0, '(a)') 'g6md: restart with more than one component'
+             error stop 2
+          end if
+       case ('@TIE4')
+          read(12) nt, nb
+          if (nt /= 0) then
+             write(0, '(a)') 'g6md: restart with tied molecules (@TIE4) is not on the GPU path'
+             error stop 2
+          end if
+       case ('@FIXC')
+          read(12) fixc
+       case ('@ID08')
+          read(12) id
+          select case (id)
+          case ('TIP4P   ')
+             model = G6_TIP4P
+          case ('TIP5P   ')
+             model = G6_TIP5P
+          case ('ST2_____')
+             model = G6_ST2
+          case default
+             write(0, '(a,a)') 'g6md: @ID08 model not on the GPU path: ', id
+             error stop 2
+          end select
+       case ('@WTG2')
+          read(12) m
+          if (allocated(com)) deallocate(com, quat, dx, w)
+          n = m
+          allocate(com(n, 3), quat(n, 4, 5), dx(n, 3, 5), w(n, 3, 5))
+          read(12) (com(:, c), c = 1, 3), ((dx(:, c, k), k = 1, 5), c = 1, 3), &
+               ((quat(:, c, k), k = 1, 5), c = 1, 4), ((w(:, c, k), k = 1, 5), c = 1, 3)
+          w = w / dt                     ! rigid_toInternalUnit (RigidModule.F90:1639-1652)
+       case default
+          write(0, '(a,a)') 'g6md: restart record not handled: ', btag
+          error stop 2
+       end select
+    end do
+    close(12)
+  end subroutine read_restart
+
+  ! the binary restart of this run, records and order as Mixture writes them
+  subroutine write_restart()
+    character(len=8) :: id
+    integer :: c, k
+    select case (model)
+    case (G6_TIP4P)
+       id = 'TIP4P   '
+    case (G6_TIP5P)
+       id = 'TIP5P   '
+    case default
+       id = 'ST2_____'
+    end select
+    open(13, file='fort.13', form='unformatted', access='sequential', status='replace')
+    write(13) '@RCOA'
+    write(13) rc, taper
+    write(13) '@BOX3'
+    write(13) box, box, box
+    write(13) '@BOOK'
+    write(13) 1, skin
+    write(13) '@VOXA'
+    write(13) voxa
+    write(13) '@NLOG'
+    write(13) nlog
+    write(13) '@DTPS'
+    write(13) dt
+    write(13) '@ASTP'
+    write(13) nstep0 + nsteps
+    if (has_nose) then
+       nose(2) = nose(2) / dt          ! back to the @NOS2 mass
+       write(13) '@NOS2'
+       write(13) nose
+    end if
+    write(13) '@TEST'
+    write(13) '@MDLP'
+    write(13) nsteps
+    write(13) '@NCMP'
+    write(13) 1
+    write(13) '@TIE4'
+    write(13) 0, 2
+    write(13) '@FIXC'
+    write(13) fixc
+    write(13) '@ID08'
+    write(13) id
+    write(13) '@WTG2'
+    write(13) n
+    write(13) (com(:, c), c = 1, 3), ((dx(:, c, k), k = 1, 5), c = 1, 3), &
+         ((quat(:, c, k), k = 1, 5), c = 1, 4), ((w(:, c, k) * dt, k = 1, 5), c = 1, 3)
+    close(13)
+  end subroutine write_restart
 
   ! @LOGD columns of Mixture.F90:2453-2471 for one component and no Andersen
   ! barostat: T, P, V, ep, ek, ep/N, ek/N (kJ/mol), nose_ep, nose_ek, 0, 0,

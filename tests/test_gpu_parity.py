@@ -6,6 +6,7 @@ force error, both against the FP64 oracle.
 """
 import json
 import os
+import tempfile
 
 import numpy as np
 import pytest
@@ -225,7 +226,9 @@ def test_fortran_driver_reproduces_the_reference_log(golden, name):
                         float(g["dt"]), int(g["nsteps"]), int(g["seed"]), nose)
     exe = os.path.join(os.path.dirname(GOLD), "..", "genesis6_amd", "lib", "g6md")
     print("g6md deck", name, flush=True)
-    r = subprocess.run([exe, "fp64"], input=deck, capture_output=True, text=True, timeout=120)
+    with tempfile.TemporaryDirectory() as d:   # g6md writes its restart file fort.13
+        r = subprocess.run([os.path.abspath(exe), "fp64"], input=deck, capture_output=True,
+                           text=True, timeout=120, cwd=d)
     assert r.returncode == 0, r.stderr
     got = parse_logd(r.stdout)
     ref = g["logd"]
