@@ -278,7 +278,7 @@ static void free_all(System &s) {
                 m.cell, m.cellres, m.nbr_j, m.nbr_code, m.nbr_cnt, m.blockpart,
                 m.blockek, m.frc, m.nose, m.thermo, m.flags, s.cell, s.cell_count, s.cell_start,
                 s.cell_fill, s.sorted_tmp, s.sorted, s.cnt, s.off, s.pi, s.pj, s.scode,
-                s.com_ref, s.lflags};
+                s.com_ref, s.lflags, m.box, m.anders};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   mc_free(s);
@@ -289,7 +289,24 @@ static void free_all(System &s) {
 
 // k steps: predict, forces; (correct + predict, forces) x (k-1); correct.
 // Each call is self-contained, so a captured graph of k steps can be replayed.
+// With the Andersen barostat the box of step t follows from the pressure of
+// step t-1, which needs every molecule's corrector: correct, close (thermo,
+// Nose and Andersen correct and predict, the new box), predict and bin in
+// that box, forces; four kernels per step.
 static void launch_steps(System &s, int k) {
+  if (s.andersen) {
+    for (int q = 0; q < k; q++) {
+      const int p = q & 1;
+      if (q > 0) md_launch_pre(s, true, false, false, 0);
+      // clears the cells and rebuild flag of parity p before they are filled
+      md_launch_close(s, (q > 0 ? 1 : 0) | 2 | 4, 1 - p, 1 - p);
+      md_launch_pre(s, false, true, true, p);
+      md_launch_main(s, nullptr, p, 0);
+    }
+    md_launch_pre(s, true, false, false, 0);
+    md_launch_close(s, 1 | 8, 0, (k - 1) & 1);
+    return;
+  }
   for (int q = 0; q < k; q++) {
     const int p = q & 1;
     md_launch_pre(s, q > 0, true, true, p);
@@ -512,7 +529,7 @@ static int check_flags(System &s) {
   int fl[2] = {0, 0};
   G6_CHECK(hipMemcpyAsync(fl, s.md.flags, sizeof(fl), hipMemcpyDeviceToHost, s.stream));
   G6_CHECK(hipStreamSynchronize(s.stream));
-  if (fl[0] == 1 || fl[0] == 3 || fl[0] == 4) return G6_ECAPACITY;
+  if (fl[0] == 1 || fl[0] == 3 || fl[0] == 4 || fl[0] == 5) return G6_ECAPACITY;
   if (fl[0] == 2) return G6_EARG;
   return 0;
 }
@@ -546,7 +563,7 @@ int g6_step(g6_handle h, int nsteps) {
 
 int g6_step_timed(g6_handle h, int nsteps, double *t) {
   System *s = get(h);
-  if (!s || nsteps < 1 || !t) return G6_EARG;
+  if (!s || nsteps < 1 || !t || s->andersen) return G6_EARG;
   double acc[5] = {0, 0, 0, 0, 0};
   for (int k = 0; k < nsteps; k++) {
     // one step: predict, forces, correct (the corrector and the next
@@ -601,6 +618,81 @@ int g6_eval_forces(g6_handle h, double *force, double *torque, double *thermo) {
   if (thermo) G6_CHECK(hipMemcpy(thermo, s->thermo, sizeof(double) * G6_THERMO_LEN,
                                  hipMemcpyDeviceToHost));
   return check_flags(*s);
+}
+
+int g6_set_andersen(g6_handle h, int mode, double pex, double mass, const double *v) {
+  System *s = get(h);
+  if (!s) return G6_EARG;
+  if (mode == 0) {
+    if (s->andersen) {
+      G6_CHECK(hipStreamSynchronize(s->stream));
+      (void)hipFree(s->md.box);
+      (void)hipFree(s->md.anders);
+      s->md.box = s->md.anders = nullptr;
+      s->andersen = false;
+    }
+  } else {
+    // orthorhombic (the cube follows the volume); orz changes one edge only
+    if (mode != 1 || !(mass > 0)) return G6_EARG;
+    const double dof = 6.0 * s->n;
+    const double rl = s->rc + s->skin;
+    if (!s->andersen) {
+      G6_CHECK(hipMalloc((void **)&s->md.box, sizeof(double) * 8));
+      G6_CHECK(hipMalloc((void **)&s->md.anders, sizeof(double) * 8));
+      s->andersen = true;
+    }
+    // the box is the handle's (a restart's @BOX3); v continues a saved
+    // barostat state (@PCN5), NULL starts one
+    const double L = s->L;
+    if (v && !(v[0] > 0)) return G6_EARG;
+    const double nd = s->geo_md.ndiv;
+    const double bx[8] = {L, 1.0 / L, 1.0 / (L / nd), L, nd, nd * 0.5 * rl, 0, 0};
+    // internal mass: per-molecule mass * dof / dt^2 (Mixture.F90:1224-1228);
+    // a fresh start takes v0 = the box volume (Andersen_Initialize)
+    const double an[8] = {pex, mass * dof / (s->dt * s->dt), v ? v[0] : (L * L) * L,
+                          v ? v[1] : 0.0, v ? v[2] : 0.0, v ? v[3] : 0.0, v ? v[4] : 0.0,
+                          v ? v[5] : 0.0};
+    if (L < bx[5]) return G6_EARG;
+    G6_CHECK(hipStreamSynchronize(s->stream));
+    G6_CHECK(hipMemcpy(s->md.box, bx, sizeof(bx), hipMemcpyHostToDevice));
+    G6_CHECK(hipMemcpy(s->md.anders, an, sizeof(an), hipMemcpyHostToDevice));
+    // rows of the new box
+    int one[2] = {1, 1};
+    G6_CHECK(hipMemcpy(s->md.flags + 2, one, sizeof(one), hipMemcpyHostToDevice));
+  }
+  if (s->graph) {  // the captured launches embed the step sequence
+    (void)hipGraphExecDestroy(s->graph);
+    s->graph = nullptr;
+  }
+  return 0;
+}
+
+int g6_get_andersen(g6_handle h, double *st) {
+  System *s = get(h);
+  if (!s || !st) return G6_EARG;
+  if (!s->andersen) {
+    for (int q = 0; q < 8; q++) st[q] = 0.0;
+    return 0;
+  }
+  double an[8];
+  G6_CHECK(hipStreamSynchronize(s->stream));
+  G6_CHECK(hipMemcpy(an, s->md.anders, sizeof(an), hipMemcpyDeviceToHost));
+  st[0] = an[0];
+  st[1] = an[1] * (s->dt * s->dt) / (6.0 * s->n);  // back to the @PCN4 mass
+  for (int q = 2; q < 8; q++) st[q] = an[q];
+  return 0;
+}
+
+int g6_get_box(g6_handle h, double *box_l) {
+  System *s = get(h);
+  if (!s || !box_l) return G6_EARG;
+  if (!s->andersen) {
+    *box_l = s->L;
+    return 0;
+  }
+  G6_CHECK(hipStreamSynchronize(s->stream));
+  G6_CHECK(hipMemcpy(box_l, s->md.box, sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
 }
 
 int g6_get_thermo(g6_handle h, double *thermo) {

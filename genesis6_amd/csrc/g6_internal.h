@@ -109,6 +109,11 @@ struct MdBuf {
   double *nose, *thermo;
   int *flags;  // [0] error code, [2 + parity] rebuild rows this step
   double half_skin2;
+  // Andersen constant-pressure MD (both nullptr for a fixed box).  box: [0]
+  // the current edge L, [1] 1/L, [2] 1/(L/ndiv) of the MD grid, [3] L at the
+  // last row build, [4] ndiv, [5] the smallest edge the grid allows.
+  // anders: pex (atm), mass (internal units), v0..v5 (AndersenModule.F90:5-8)
+  double *box, *anders;
 };
 
 // Descriptor of one system of a ladder, in device memory: the ladder
@@ -162,6 +167,7 @@ struct System {
   int pcap = 0;
   bool nose_active = false;
   double nose_h[9] = {0};
+  bool andersen = false;  // md.box and md.anders are allocated
   // device buffers
   double *com = nullptr, *dx = nullptr, *w = nullptr, *quat = nullptr;
   double *tmat = nullptr, *intra = nullptr;

@@ -266,7 +266,7 @@ int g6_ladder_create(g6_ladder *hl, int nsys, const g6_handle *hs) {
     System *s = system_of(hs[r]);
     bool dup = false;
     for (System *t : l->sys) dup = dup || t == s;
-    if (!s || dup) {
+    if (!s || dup || s->andersen) {  // ladders step fixed boxes
       delete l;
       return G6_EARG;
     }

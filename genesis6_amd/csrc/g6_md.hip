@@ -131,6 +131,16 @@ __device__ inline void store_qc(const MdBuf &b, int n, int i, const double (&S)[
   }
 }
 
+// The box of this step: the fixed box of the grid, or the Andersen
+// barostat's current box in device memory (written by the step close).
+struct BoxNow {
+  double L, invL, cellinv;
+};
+__device__ inline BoxNow box_now(const Geom &g, const MdBuf &b) {
+  if (b.box) return BoxNow{b.box[0], b.box[1], b.box[2]};
+  return BoxNow{g.L, g.invL, g.cellinv};
+}
+
 // ---- kernel A --------------------------------------------------------------
 // A block updates 64 molecules (one per lane) with one role per wave:
 //   wave 0  translation: com, dx[5][3]; record COM, binning, skin test
@@ -160,7 +170,11 @@ __device__ __forceinline__ void md_pre_body(const ModelTable &mt, const Geom &g,
 #pragma unroll
   for (int k = 0; k < 20; k++) S[k] = 0.0;
   const double zeta0 = nose_active ? b.nose[3] : 0.0;
-  const double nc = zeta0 * dt, ext = nc * 0.5, exr = nc;
+  // translational drag: Nose, plus the Andersen dilation v1 / (2*3*v0)
+  // (RigidModule.F90:1275-1305, orthorhombic)
+  const double acoeff = b.anders ? b.anders[3] / (2.0 * 3.0 * b.anders[2]) : 0.0;
+  const double nc = zeta0 * dt, ext = nc * 0.5 + acoeff, exr = nc;
+  const BoxNow bx = box_now(g, b);
   // every global load is issued here, ahead of the first barrier
   double F[3] = {0, 0, 0}, t[9], tl[3] = {0, 0, 0}, cref[3] = {0, 0, 0};
 #pragma unroll
@@ -404,9 +418,9 @@ __device__ __forceinline__ void md_pre_body(const ModelTable &mt, const Geom &g,
 #pragma unroll
       for (int a = 0; a < 3; a++) {
         double d = S[a];
-        const double o = round(d * g.invL) * g.L;
+        const double o = round(d * bx.invL) * bx.L;
         d = d - o;
-        int j = (int)round(d * g.cellinv);
+        int j = (int)round(d * bx.cellinv);
         if (j < 0) j += g.ndiv;
         j = ((j % g.ndiv) + g.ndiv) % g.ndiv;
         jc[a] = j;
@@ -421,11 +435,19 @@ __device__ __forceinline__ void md_pre_body(const ModelTable &mt, const Geom &g,
       else
         b.flags[0] = 3;  // cell capacity exceeded
       if (b.half_skin2 > 0.0) {
+        // a pair of the rows moves by at most twice the largest displacement
+        // plus sqrt(3)|L - L_ref| (image shifts of one box per axis)
+        double h2 = b.half_skin2;
+        if (b.box) {
+          const double h = sqrt(b.half_skin2) - 0.8660254037844386 * fabs(bx.L - b.box[3]);
+          h2 = h > 0.0 ? h * h : -1.0;
+        }
         const double e0 = S[0] - cref[0], e1 = S[1] - cref[1], e2 = S[2] - cref[2];
-        if (e0 * e0 + e1 * e1 + e2 * e2 > b.half_skin2) b.flags[2 + parity] = 1;
+        if (e0 * e0 + e1 * e1 + e2 * e2 > h2) b.flags[2 + parity] = 1;
       } else if (i == 0) {
         b.flags[2 + parity] = 1;  // no skin: the reference rebuilds every step
       }
+      if (b.box && i == 0 && bx.L < b.box[5]) b.flags[0] = 5;  // cells below (rc+skin)/2
     } else if (role >= 2) {
       // rigid_setmatrix and the site offsets; the two waves split the stores
       const double qa = S[0], qb = S[1], qc = S[2], qd = S[3];
@@ -500,6 +522,17 @@ __global__ __launch_bounds__(256) void k_md_pre_lad(ModelTable mt, const MdRep *
   const MdBuf b = reps[m.x].b;
   md_pre_body(mt, g, b, reps[m.x].dt, reps[m.x].nose_active, do_correct, do_predict, do_sites,
               parity, m.y);
+}
+
+// Andersen_Rescale, orthorhombic (AndersenModule.F90:65-82, BoxModule.F90:
+// 176-183): the cube takes the volume v0; box[2] is the MD grid's 1/cell.
+__device__ inline void andersen_rescale(const double *a, double *box) {
+  double L = box[0];
+  const double r2 = pow(a[2] / ((L * L) * L), 1.0 / 3.0);
+  L = L * r2;
+  box[0] = L;
+  box[1] = 1.0 / L;
+  box[2] = 1.0 / (L / box[4]);
 }
 
 // ---- step close: one block of 256 threads ---------------------------------
@@ -599,24 +632,61 @@ __device__ inline void close_body(const ModelTable &mt, const MdBuf &b, int n, d
       th[2] = ekr;
       th[3] = temp;
       th[13] = h;
-      // kinetic tensor (RigidModule.F90:617) and pressure (Mixture.F90:2243-2255)
-      const double km = mt.mass * (1.0 / (dt * dt));
-      const double kt[9] = {r[16] * km, r[17] * km, r[18] * km, r[17] * km, r[19] * km,
-                            r[20] * km, r[18] * km, r[20] * km, r[21] * km};
+      // kinetic tensor (RigidModule.F90:617: sums * mass * (1/dt^2)) and
+      // pressure (Mixture.F90:2236-2255) in the box of this step
+      const double kd = 1.0 / (dt * dt);
+      const double kt[9] = {r[16] * mt.mass * kd, r[17] * mt.mass * kd, r[18] * mt.mass * kd,
+                            r[17] * mt.mass * kd, r[19] * mt.mass * kd, r[20] * mt.mass * kd,
+                            r[18] * mt.mass * kd, r[20] * mt.mass * kd, r[21] * mt.mass * kd};
       for (int q = 0; q < 9; q++) th[16 + q] = kt[q];
       const double vp = (kt[0] + th[4]) + (kt[4] + th[8]) + (kt[8] + th[12]);
-      const double vol = box_l * box_l * box_l;
-      th[25] = (vp / 3.0) * 10.0 * (1.0 / 101326.0) / (vol * 6.0225e23 * (1e-10 * 1e-10 * 1e-10));
-      th[26] = vol;
+      const double lb = b.box ? b.box[0] : box_l;
+      const double vol = (lb * lb) * lb;
+      const double press =
+          (vp / 3.0) * 10.0 * (1.0 / 101326.0) / (vol * 6.0225e23 * (1e-10 * 1e-10 * 1e-10));
+      th[25] = press;
+      double and_ep = 0.0, and_ek = 0.0, volc = vol;
+      if (b.anders) {
+        // Andersen_Correct (AndersenModule.F90:95-110) with this pressure
+        double *a = b.anders;
+        const double volp2 = (press - a[0]) / a[1] * 0.5;
+        const double cvol = volp2 - a[4];
+        a[2] = a[2] + kG5_0 * cvol;
+        a[3] = a[3] + kG5_1 * cvol;
+        a[4] = volp2;
+        a[5] = a[5] + kG5_3 * cvol;
+        a[6] = a[6] + kG5_4 * cvol;
+        a[7] = a[7] + kG5_5 * cvol;
+        andersen_rescale(a, b.box);
+        const double lc = b.box[0];
+        volc = (lc * lc) * lc;
+        // barostat energies of the log line (Mixture.F90:2436-2439), J/mol
+        and_ep = a[0] * a[2] * 101326.0 * 6.0225e23 * (1e-10 * 1e-10 * 1e-10);
+        and_ek = a[3] * a[3] * a[1] * 0.5 * 101326.0 * 6.0225e23 * (1e-10 * 1e-10 * 1e-10);
+        th[13] = h + and_ep + and_ek;
+      }
+      th[26] = volc;
       th[27] = nose_ep;
       th[28] = nose_ek;
       th[29] = nose_active ? z[0] : 0.0;
+      th[30] = and_ep;
+      th[31] = and_ek;
     }
     if ((mode & 2) && nose_active) {
       z[3] = z[3] + z[4] + z[5] + z[6] + z[7];
       z[4] = z[4] + 2.0 * z[5] + 3.0 * z[6] + 4.0 * z[7];
       z[5] = z[5] + 3.0 * z[6] + 6.0 * z[7];
       z[6] = z[6] + 4.0 * z[7];
+    }
+    if ((mode & 2) && b.anders) {
+      // Andersen_Predict (AndersenModule.F90:84-93)
+      double *a = b.anders;
+      a[2] = a[2] + a[3] + a[4] + a[5] + a[6] + a[7];
+      a[3] = a[3] + 2.0 * a[4] + 3.0 * a[5] + 4.0 * a[6] + 5.0 * a[7];
+      a[4] = a[4] + 3.0 * a[5] + 6.0 * a[6] + 10.0 * a[7];
+      a[5] = a[5] + 4.0 * a[6] + 10.0 * a[7];
+      a[6] = a[6] + 5.0 * a[7];
+      andersen_rescale(a, b.box);
     }
   }
 }
@@ -667,17 +737,18 @@ __device__ inline double cell_gap(int o, double u) {
   return fmax(0.0, fmax(a, c) - 1e-9);
 }
 
-__device__ inline double frac_in_cell(const Geom &g, double x) {
-  const double d = x - round(x * g.invL) * g.L;
-  const double t = d * g.cellinv;
+__device__ inline double frac_in_cell(const BoxNow &bx, double x) {
+  const double d = x - round(x * bx.invL) * bx.L;
+  const double t = d * bx.cellinv;
   return t - round(t);
 }
 
 // neighbour-row build of molecule i into nbr[i*maxnb ...], sorted by j so
 // that the force sums do not depend on the order in which the binning
 // atomics filled the cells; returns the row length (also stored).
-__device__ inline int build_row(const Geom &g, const MdBuf &b, int parity, int i, int lane,
-                                int (&s_beg)[128], int (&s_pre)[132], int *s_j, int *s_c) {
+__device__ inline int build_row(const Geom &g, const BoxNow &bx, const MdBuf &b, int parity,
+                                int i, int lane, int (&s_beg)[128], int (&s_pre)[132], int *s_j,
+                                int *s_c) {
   const int n = g.n;
   const int nd = g.ndiv;
   const int c = b.cell[i];
@@ -685,8 +756,8 @@ __device__ inline int build_row(const Geom &g, const MdBuf &b, int parity, int i
   const Stencil st = stencil_of(nd);
   const double *reci = b.rec + (size_t)i * b.rs;
   const double xi = reci[0], yi = reci[1], zi = reci[2];
-  const double ux = frac_in_cell(g, xi), uy = frac_in_cell(g, yi), uz = frac_in_cell(g, zi);
-  const double lim = g.rr * (g.cellinv * g.cellinv);  // (rc+skin)^2 in cell units
+  const double ux = frac_in_cell(bx, xi), uy = frac_in_cell(bx, yi), uz = frac_in_cell(bx, zi);
+  const double lim = g.rr * (bx.cellinv * bx.cellinv);  // (rc+skin)^2 in cell units
   const int *res = b.cellres + (size_t)parity * g.ncell * b.cell_cap;
   int carry = 0;
   for (int e0 = 0; e0 < st.nst; e0 += 64) {
@@ -734,10 +805,11 @@ __device__ inline int build_row(const Geom &g, const MdBuf &b, int parity, int i
       if (j != i) {
         const double *recj = b.rec + (size_t)j * b.rs;
         double d0 = xi - recj[0], d1 = yi - recj[1], d2 = zi - recj[2];
-        const double k0 = round(d0 * g.invL), k1 = round(d1 * g.invL), k2 = round(d2 * g.invL);
-        d0 = d0 - k0 * g.L;
-        d1 = d1 - k1 * g.L;
-        d2 = d2 - k2 * g.L;
+        const double k0 = round(d0 * bx.invL), k1 = round(d1 * bx.invL),
+                     k2 = round(d2 * bx.invL);
+        d0 = d0 - k0 * bx.L;
+        d1 = d1 - k1 * bx.L;
+        d2 = d2 - k2 * bx.L;
         const double dd = d0 * d0 + d1 * d1 + d2 * d2;
         pass = dd < g.rr;
         code = ((int)k0 + 512) | (((int)k1 + 512) << 10) | (((int)k2 + 512) << 20);
@@ -774,6 +846,7 @@ __device__ inline int build_row(const Geom &g, const MdBuf &b, int parity, int i
     b.com_ref[i] = xi;
     b.com_ref[n + i] = yi;
     b.com_ref[2 * n + i] = zi;
+    if (b.box && i == 0) b.box[3] = bx.L;  // the box the rows were built in
   }
   return m;
 }
@@ -845,6 +918,7 @@ __device__ __forceinline__ void md_main_body(const ModelTable &mt, const Geom &g
   const int n = g.n;
   const int i = blk * 4 + wid;
   const int cap = b.maxnb;
+  const BoxNow bx = box_now(g, b);
   unsigned char *reg = s_dyn + (size_t)wid * reg_bytes;
   double ep = 0, v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   int npair = 0, nlist = 0;  // wave-uniform counts
@@ -858,7 +932,7 @@ __device__ __forceinline__ void md_main_body(const ModelTable &mt, const Geom &g
     int m;
     if (b.flags[2 + parity]) {
       int *sj = reinterpret_cast<int *>(reg);
-      m = build_row(g, b, parity, i, lane, s_beg[wid], s_pre[wid], sj, sj + cap);
+      m = build_row(g, bx, b, parity, i, lane, s_beg[wid], s_pre[wid], sj, sj + cap);
       nlist = m;
     } else {
       m = b.nbr_cnt[i];
@@ -909,9 +983,9 @@ __device__ __forceinline__ void md_main_body(const ModelTable &mt, const Geom &g
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         const int code = cc[u];
-        const double ox = (double)((code & 1023) - 512) * g.L;
-        const double oy = (double)(((code >> 10) & 1023) - 512) * g.L;
-        const double oz = (double)(((code >> 20) & 1023) - 512) * g.L;
+        const double ox = (double)((code & 1023) - 512) * bx.L;
+        const double oy = (double)(((code >> 10) & 1023) - 512) * bx.L;
+        const double oz = (double)(((code >> 20) & 1023) - 512) * bx.L;
         const double d0 = (xi - xj[u]) - ox;
         const double d1 = (yi - yj[u]) - oy;
         const double d2 = (zi - zj[u]) - oz;

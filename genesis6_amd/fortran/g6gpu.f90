@@ -182,6 +182,27 @@ module g6gpu
        real(c_double), intent(out) :: dx(*), dy(*), dz(*), dd(*)
      end function g6_neighbors_displacement
 
+     integer(c_int) function g6_set_andersen(h, mode, pex, mass, v) &
+          bind(C, name="g6_set_andersen")
+       import :: c_int, c_int64_t, c_double, c_ptr
+       integer(c_int64_t), value :: h
+       integer(c_int), value :: mode
+       real(c_double), value :: pex, mass
+       type(c_ptr), value :: v               ! c_null_ptr, or c_loc of v0..v5
+     end function g6_set_andersen
+
+     integer(c_int) function g6_get_andersen(h, state8) bind(C, name="g6_get_andersen")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), value :: h
+       real(c_double), intent(out) :: state8(8)
+     end function g6_get_andersen
+
+     integer(c_int) function g6_get_box(h, box_l) bind(C, name="g6_get_box")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), value :: h
+       real(c_double), intent(out) :: box_l
+     end function g6_get_box
+
      integer(c_int) function g6_lattice_start(n, box_l, seed, com, quat1) &
           bind(C, name="g6_lattice_start")
        import :: c_int, c_int32_t, c_double

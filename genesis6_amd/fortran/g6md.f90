@@ -5,14 +5,15 @@
 !
 ! Tags read: @DTPS dt | @MDLP steps | @NLOG interval | @RCOA rc taper |
 ! @BXLA L | @VOXA vx vy vz | @VSET T seed | @NOS2 T q dump z0..z4 zsum |
+! @PCN4 mode pex mass (Andersen barostat, mode 1) |
 ! @ID08 model | @NX4A n, then n lines x y z a b c d.
 ! Command line: g6md [fp64|fp32] [skin]   (default fp64, skin 0: the
 ! reference's list rebuilt every step).
 !
 ! Restart files are Mixture's: a binary fort.12 in the working directory,
 ! if present, is read first (the records Mixture.F90:400-577 reads: @RCOA
-! @BOX3 @BOOK @VOXA @NLOG @DTPS @ASTP @NOS2 @TEST @MDLP @NCMP @TIE4 @FIXC
-! @ID08 @WTG2), the text deck on standard input may then override any tag,
+! @BOX3 @BOOK @VOXA @NLOG @DTPS @ASTP @NOS2 @PCN5 @TEST @MDLP @NCMP @TIE4
+! @FIXC @ID08 @WTG2), the text deck on standard input may then override any tag,
 ! and the run continues from the step count @ASTP with the full Gear state
 ! of @WTG2.  At the end the state is written to fort.13 with the records
 ! and layout Mixture writes there (Mixture.F90:2530-2598,
@@ -27,6 +28,10 @@ program g6md
   integer :: nsteps = 0, nlog = 0, ios, i, done, chunk, target
   integer :: nstep0 = 0
   logical :: restart, fixc = .false.
+  integer(c_int) :: amode = 0
+  real(c_double) :: apex = 0d0, amass = 0d0, astate(8)
+  real(c_double), target :: av(6)
+  logical :: has_av = .false.
   integer(c_int) :: model = -1, prec = G6_FP64, n = 0
   integer(c_int32_t) :: vseed = 0
   integer(c_int64_t) :: h
@@ -69,6 +74,9 @@ program g6md
      case ('@NOS2')
         read(5, *) nose
         has_nose = .true.
+     case ('@PCN4')
+        read(5, *) amode, apex, amass
+        has_av = .false.
      case ('@ID08')
         read(5, '(a)') line
         select case (trim(adjustl(line)))
@@ -111,6 +119,13 @@ program g6md
      call g6_check(g6_set_nose(h, 1_c_int, nose), 'g6_set_nose')
   end if
   if (vtemp >= 0d0) call g6_check(g6_vreset(h, vtemp, vseed), 'g6_vreset')
+  if (amode /= 0) then
+     if (has_av) then
+        call g6_check(g6_set_andersen(h, amode, apex, amass, c_loc(av)), 'g6_set_andersen')
+     else
+        call g6_check(g6_set_andersen(h, amode, apex, amass, c_null_ptr), 'g6_set_andersen')
+     end if
+  end if
 
   ! Mixture runs @MDLP more steps from the count nstep (0, or @ASTP of a
   ! restart) and logs when mod(nstep, nlog) = 0 after the step that starts
@@ -134,6 +149,8 @@ program g6md
   end do
   call g6_check(g6_download_state(h, com, quat, dx, w), 'g6_download_state')
   if (has_nose) call g6_check(g6_get_nose(h, nose), 'g6_get_nose')
+  if (amode /= 0) call g6_check(g6_get_andersen(h, astate), 'g6_get_andersen')
+  call g6_check(g6_get_box(h, box), 'g6_get_box')
   call g6_check(g6_finalize(h), 'g6_finalize')
   call write_restart()
 
@@ -171,6 +188,9 @@ contains
           read(12) d9
           nose = d9
           has_nose = .true.
+       case ('@PCN5')
+          read(12) amode, apex, amass, av
+          has_av = .true.
        case ('@TEST')
        case ('@MDLP')
           read(12) nsteps
@@ -249,6 +269,10 @@ contains
        write(13) '@NOS2'
        write(13) nose
     end if
+    if (amode /= 0) then               ! Andersen_WriteBinaryPCN5
+       write(13) '@PCN5'
+       write(13) amode, astate
+    end if
     write(13) '@TEST'
     write(13) '@MDLP'
     write(13) nsteps
@@ -267,9 +291,9 @@ contains
     close(13)
   end subroutine write_restart
 
-  ! @LOGD columns of Mixture.F90:2453-2471 for one component and no Andersen
-  ! barostat: T, P, V, ep, ek, ep/N, ek/N (kJ/mol), nose_ep, nose_ek, 0, 0,
-  ! H, exsum (0), H, emonsum (0)
+  ! @LOGD columns of Mixture.F90:2453-2471 for one component: T, P, V, ep,
+  ! ek, ep/N, ek/N (kJ/mol), nose_ep, nose_ek, andersen_ep, andersen_ek, H,
+  ! exsum (0), H, emonsum (0)
   subroutine write_logd(step, t)
     integer, intent(in) :: step
     real(c_double), intent(in) :: t(G6_THERMO_LEN)
@@ -279,6 +303,6 @@ contains
     write(6, '(a)') '@LOGD'
     write(6, '(i10,30(1x,E24.17))') step, t(4), t(26), t(27), ep * I2J * X2KX, &
          ek * I2J * X2KX, ep * I2J / dble(n) * X2KX, ek * I2J / dble(n) * X2KX, &
-         t(28) * X2KX, t(29) * X2KX, 0d0, 0d0, t(14), 0d0, t(14), 0d0
+         t(28) * X2KX, t(29) * X2KX, t(31) * X2KX, t(32) * X2KX, t(14), 0d0, t(14), 0d0
   end subroutine write_logd
 end program g6md

@@ -60,6 +60,9 @@ def lib():
                                      _dp, _dp]
         L.g6_site_positions.argtypes = [_H, _dp, _dp, _dp, _dp, _dp]
         L.g6_collect.argtypes = [_H, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.g6_set_andersen.argtypes = [_H, C.c_int, C.c_double, C.c_double, C.c_void_p]
+        L.g6_get_andersen.argtypes = [_H, _dp]
+        L.g6_get_box.argtypes = [_H, C.POINTER(C.c_double)]
         tab = [_ip, _ip, _dp, _dp, _dp, _dp]
         L.g6_neighbors.argtypes = [_H, C.c_int, C.c_int64, _ip, _ip, _dp, _dp, _dp, _dp,
                                    C.c_int, C.c_int, C.c_double] + tab
@@ -184,6 +187,26 @@ class System:
         """Thermostat retarget of a replica exchange: velocities scaled by
         sqrt(temp / nose temp), nose temp = temp, zeta0..zeta4 = 0."""
         check(lib().g6_set_temperature(self.h, temp), "g6_set_temperature")
+
+    def set_andersen(self, pex, mass, v=None, mode=1):
+        """Andersen barostat (@PCN4 mode pex mass; v = v0..v5 of a saved
+        state, or None to start one); mode 0 switches it off."""
+        vp = None
+        if v is not None:
+            self._av = np.ascontiguousarray(v, np.float64)
+            vp = self._av.ctypes.data_as(C.c_void_p)
+        check(lib().g6_set_andersen(self.h, mode, pex, mass, vp), "g6_set_andersen")
+
+    def andersen_state(self):
+        """pex, mass (per molecule), v0..v5."""
+        st = np.zeros(8)
+        check(lib().g6_get_andersen(self.h, st), "g6_get_andersen")
+        return st
+
+    def box(self):
+        L = C.c_double(0)
+        check(lib().g6_get_box(self.h, C.byref(L)), "g6_get_box")
+        return L.value
 
     def step_timed(self, nsteps):
         t = np.zeros(5)

@@ -48,7 +48,8 @@ enum {
  * [25] pressure (atm, Mixture.F90:2243-2255), [26] volume (A^3),
  * [27] Nose potential term nose_ep, [28] Nose kinetic term nose_ek (J/mol,
  * Mixture.F90:2431-2434), [29] Nose target temperature nose%temp (K; 0
- * without a thermostat), [30..31] zero */
+ * without a thermostat), [30] Andersen andersen_ep and [31] andersen_ek
+ * (J/mol, Mixture.F90:2436-2439; 0 without a barostat) */
 #define G6_THERMO_LEN 32
 
 /* ---- system lifecycle (Mixture.F90:1141-1347 setup) -------------------- */
@@ -174,6 +175,25 @@ int g6_neighbors_displacement(g6_handle h, int nsite, const int32_t *nnear_in,
                               int32_t *nmodified, int32_t *index, int32_t *nnear,
                               int32_t *near, double *dx, double *dy, double *dz,
                               double *dd);
+
+/* ---- constant-pressure MD: the Andersen barostat (AndersenModule.F90) ---
+ * mode 1 (orthorhombic: the cubic box follows the volume, the reference's
+ * only mode for a cube) with external pressure pex (atm) and mass per
+ * molecule as @PCN4 (Mixture.F90:1207-1231); mode 0 switches it off.
+ * v: NULL starts the barostat (Andersen_Initialize: v0 = the box volume),
+ * or v[0..5] = v0..v5 continue a saved state (@PCN5).  g6_step then runs
+ * Andersen_Predict before the forces and Andersen_Correct with the step's
+ * pressure after the corrector (Mixture.F90:1843-1844, 2283-2286); the box
+ * edge changes every step.  The thermo record adds the barostat energies
+ * ([30], [31]) to H ([13]); [25] is the pressure in the step's box, [26]
+ * the corrected volume.  The module entry points (g6_pairlist, g6_force,
+ * ...) keep the box of g6_init.  Not for handles in a ladder. */
+int g6_set_andersen(g6_handle h, int mode, double pex, double mass,
+                    const double *v);
+/* pex, mass (per molecule, as @PCN4), v0..v5: the @PCN5 payload */
+int g6_get_andersen(g6_handle h, double *state8);
+/* the current box edge (A) */
+int g6_get_box(g6_handle h, double *box_l);
 
 /* ---- Monte Carlo: PairInfo and the mc2 trial loop ------------------- */
 /* Starts an MC run on the handle's current configuration (COMs and order-1

@@ -694,11 +694,20 @@ void g6o_predict(int n, double *com, double *dx, double *w, double *quat) {
 void g6o_correct(int model, int n, double dt, double zeta0, const double *force,
                  const double *torque, double *com, double *dx, double *w,
                  double *quat) {
+  g6o_correct_npt(model, n, dt, zeta0, 0.0, force, torque, com, dx, w, quat);
+}
+
+/* the same with the Andersen dilation term acoeff = v1 / (2*3*v0) added to
+ * the translational drag (RigidModule.F90:1290-1305, orthorhombic) */
+void g6o_correct_npt(int model, int n, double dt, double zeta0, double acoeff,
+                     const double *force, const double *torque, double *com, double *dx,
+                     double *w, double *quat) {
   g6o_model m;
   g6o_model_get(model, &m);
   const double massi = 1.0 / m.mass, Ixxi = 1.0 / m.Ixx, Iyyi = 1.0 / m.Iyy,
                Izzi = 1.0 / m.Izz;
-  const double nc = zeta0 * dt, ext = nc * 0.5, exr = nc;
+  const double nc = zeta0 * dt, exr = nc;
+  const double ext = nc * 0.5 + acoeff;
   const double h = dt * 0.5, tf = dt * dt * 0.5 * massi;
   for (int i = 0; i < n; i++) {
     for (int a = 0; a < 3; a++) {
@@ -812,12 +821,43 @@ void g6o_nose_correct(double *s, double temp) {
 }
 
 /* ---- MD step in Mixture.F90 order (1789-2286) --------------------------- */
+/* Andersen_Rescale, orthorhombic (AndersenModule.F90:65-82, BoxModule.F90:
+ * 176-183): the cube takes the volume v0 */
+static void andersen_rescale(const double *a, double *L) {
+  const double r2 = pow(a[2] / ((*L * *L) * *L), 1.0 / 3.0);
+  *L = *L * r2;
+}
+
+/* Andersen_Predict (AndersenModule.F90:84-93); a = pex, mass, v0..v5 */
+void g6o_andersen_predict(double *a, double *L) {
+  a[2] = a[2] + a[3] + a[4] + a[5] + a[6] + a[7];
+  a[3] = a[3] + 2.0 * a[4] + 3.0 * a[5] + 4.0 * a[6] + 5.0 * a[7];
+  a[4] = a[4] + 3.0 * a[5] + 6.0 * a[6] + 10.0 * a[7];
+  a[5] = a[5] + 4.0 * a[6] + 10.0 * a[7];
+  a[6] = a[6] + 5.0 * a[7];
+  andersen_rescale(a, L);
+}
+
+/* Andersen_Correct (AndersenModule.F90:95-110), pin in atm */
+void g6o_andersen_correct(double *a, double *L, double pin) {
+  const double volp2 = (pin - a[0]) / a[1] * 0.5;
+  const double cvol = volp2 - a[4];
+  a[2] = a[2] + Gear5_0 * cvol;
+  a[3] = a[3] + Gear5_1 * cvol;
+  a[4] = volp2;
+  a[5] = a[5] + Gear5_3 * cvol;
+  a[6] = a[6] + Gear5_4 * cvol;
+  a[7] = a[7] + Gear5_5 * cvol;
+  andersen_rescale(a, L);
+}
+
 long g6o_md_step(g6o_md *s) {
   g6o_model m;
   g6o_model_get(s->model, &m);
   const int n = s->n, ns = m.nsite;
   g6o_predict(n, s->com, s->dx, s->w, s->quat);
   if (s->nose_active) g6o_nose_predict(s->nose);
+  if (s->andersen) g6o_andersen_predict(s->anders, &s->L);
   double *x = malloc(sizeof(double) * n * ns * 6), *y = x + n * ns, *z = y + n * ns;
   double *fx = z + n * ns, *fy = fx + n * ns, *fz = fy + n * ns;
   double *tmat = malloc(sizeof(double) * 9 * n), *intra = malloc(sizeof(double) * 3 * ns * n);
@@ -843,12 +883,31 @@ long g6o_md_step(g6o_md *s) {
     s->ep = ep;
     double *F = malloc(sizeof(double) * 6 * n), *T = F + 3 * n;
     g6o_collect(s->model, n, fx, fy, fz, tmat, intra, F, T);
-    g6o_correct(s->model, n, s->dt, s->nose_active ? s->nose[3] : 0.0, F, T, s->com,
-                s->dx, s->w, s->quat);
+    const double acoeff = s->andersen ? s->anders[3] / (2.0 * 3.0 * s->anders[2]) : 0.0;
+    g6o_correct_npt(s->model, n, s->dt, s->nose_active ? s->nose[3] : 0.0, acoeff, F, T,
+                    s->com, s->dx, s->w, s->quat);
     free(F);
     g6o_ek(s->model, n, s->dt, s->dx, s->w, &s->ekt, &s->ekr);
     s->temperature = I2J * 2.0 * (s->ekt + s->ekr) / (rgas() * (double)(6 * n));
+    /* Rigid_PressureTensor (RigidModule.F90:600-619) + the pair virial, and
+     * the pressure in atm (Mixture.F90:2236-2255) */
+    double kt[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < n; i++)
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) kt[3 * a + b] = kt[3 * a + b] + s->dx[a * n + i] * s->dx[b * n + i];
+    for (int q = 0; q < 9; q++) kt[q] = kt[q] * m.mass * (1.0 / (s->dt * s->dt));
+    const double vp = (kt[0] + s->vir[0]) + (kt[4] + s->vir[4]) + (kt[8] + s->vir[8]);
+    s->pressure = (vp / 3.0) * I2J * (1.0 / 101326.0) /
+                  (((s->L * s->L) * s->L) * 6.0225e23 * (1e-10 * 1e-10 * 1e-10));
     if (s->nose_active) g6o_nose_correct(s->nose, s->temperature);
+    if (s->andersen) g6o_andersen_correct(s->anders, &s->L, s->pressure);
+    s->volume = (s->L * s->L) * s->L;
+    s->andersen_ep = s->andersen ? s->anders[0] * s->anders[2] * 101326.0 * 6.0225e23 *
+                                       (1e-10 * 1e-10 * 1e-10)
+                                 : 0.0;
+    s->andersen_ek = s->andersen ? s->anders[3] * s->anders[3] * s->anders[1] * 0.5 *
+                                       101326.0 * 6.0225e23 * (1e-10 * 1e-10 * 1e-10)
+                                 : 0.0;
   }
   s->npair = np;
   free(x); free(tmat); free(intra); free(pi); free(pj); free(o);
@@ -864,6 +923,7 @@ double g6o_md_hamiltonian(const g6o_md *s) {
     const double nose_ep = (dof * z[0] * (z[8] - 0.5 * z[3]) * s->dt) * rgas();
     h = h + nose_ep + nose_ek;
   }
+  if (s->andersen) h = h + s->andersen_ep + s->andersen_ek;
   return h;
 }
 

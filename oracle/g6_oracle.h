@@ -108,6 +108,11 @@ void g6o_correct(int model, int n, double dt, double zeta0, const double *force,
 void g6o_ek(int model, int n, double dt, const double *dx, const double *w,
             double *ekt, double *ekr);
 /* Nose state: [temp, q(internal, x dt), dump, zeta0..zeta4, zetasum]      */
+void g6o_correct_npt(int model, int n, double dt, double zeta0, double acoeff,
+                     const double *force, const double *torque, double *com, double *dx,
+                     double *w, double *quat);
+void g6o_andersen_predict(double *a, double *L);
+void g6o_andersen_correct(double *a, double *L, double pin);
 void g6o_nose_predict(double *nose);
 void g6o_nose_correct(double *nose, double temperature);
 
@@ -119,6 +124,11 @@ typedef struct {
   double *com, *dx, *w, *quat;      /* state, as above                    */
   double ep, ekt, ekr, temperature, vir[9];
   long npair;
+  /* Andersen barostat, orthorhombic (cube): anders = pex (atm), mass
+   * (internal: @PCN4 mass * dof / dt^2, Mixture.F90:1207-1231), v0..v5 */
+  int andersen;
+  double anders[8];
+  double pressure, volume, andersen_ep, andersen_ek; /* atm, A^3, J/mol */
 } g6o_md;
 /* one step; returns npair or -1 */
 long g6o_md_step(g6o_md *s);

@@ -56,7 +56,9 @@ class MD(C.Structure):
                 ("w", C.POINTER(C.c_double)), ("quat", C.POINTER(C.c_double)),
                 ("ep", C.c_double), ("ekt", C.c_double), ("ekr", C.c_double),
                 ("temperature", C.c_double), ("vir", C.c_double * 9),
-                ("npair", C.c_long)]
+                ("npair", C.c_long), ("andersen", C.c_int), ("anders", C.c_double * 8),
+                ("pressure", C.c_double), ("volume", C.c_double),
+                ("andersen_ep", C.c_double), ("andersen_ek", C.c_double)]
 
 
 NEAR = 151   # G6O_NEAR
@@ -323,6 +325,16 @@ class MDRun:
         self.s.dx = self.dx.ctypes.data_as(P)
         self.s.w = self.w.ctypes.data_as(P)
         self.s.quat = self.quat.ctypes.data_as(P)
+
+    def set_andersen(self, pex, mass):
+        """Orthorhombic Andersen barostat from @PCN4 (mode 1, pex atm, mass
+        per molecule): internal mass = mass * dof / dt^2 (Mixture.F90:1224-1228),
+        v0 = the box volume (Andersen_Initialize)."""
+        dof = 6.0 * self.n
+        self.s.andersen = 1
+        a = [pex, mass * dof / self.s.dt ** 2, (self.s.L * self.s.L) * self.s.L, 0, 0, 0, 0, 0]
+        for k, v in enumerate(a):
+            self.s.anders[k] = v
 
     def vreset(self, temp, seed):
         dx1 = np.zeros(3 * self.n); w1 = np.zeros(3 * self.n)

@@ -219,3 +219,23 @@ def test_nve_1000_steps_match_reference_driver(golden):
         H.append(md.hamiltonian())
     ref = g["logd"][:, 12]
     assert np.abs(np.array(H) - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("name,nose", [("npt", (300.0, 7.5e5, 0.0)), ("nph", None)])
+def test_andersen_npt_matches_reference_driver(golden, name, nose):
+    """50 steps with the Andersen barostat (@PCN4 1 1.0 1e-4), with and
+    without the Nose thermostat: the oracle's temperature, pressure, volume
+    and barostat energies equal the reference's @LOGD at every step."""
+    g = golden("md_tip4p_216_nve1000")
+    f = golden("md_tip4p_216_npt")
+    md = O.MDRun(O.TIP4P, g["com"], g["quat"].T.copy(), float(g["L"]), float(g["rc"]),
+                 float(g["taper"]), float(g["dt"]), nose=nose)
+    md.set_andersen(1.0, 1e-4)
+    md.vreset(300.0, 4246)
+    ref = f["logd_" + name]
+    for k in range(len(ref)):
+        md.step()
+        r = ref[k]
+        assert md.s.temperature == r[1] and md.s.pressure == r[2] and md.s.volume == r[3], k
+        assert md.s.andersen_ep * 1e-3 == r[10] and md.s.andersen_ek * 1e-3 == r[11], k
+        assert abs(md.hamiltonian() - r[12]) <= 1e-12 * abs(r[12])
