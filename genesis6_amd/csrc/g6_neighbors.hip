@@ -277,6 +277,208 @@ __global__ __launch_bounds__(64) void k_nb_disp(
   if (lane == 0) nnear[o] = cnt;
 }
 
+// ---- umbrella order parameters of mc2 (Q6Local.F90, Triangle.F90) -------
+constexpr int kMaxBond = 10;  // MAXBOND (Q6Local.F90:15): bonds within 3 A per molecule
+constexpr double kPiT = 3.1415926535897932;  // PItanaka (PhysConst.F90:18)
+
+// plgndr (q6.F90): associated Legendre function, upward recurrence in l
+__device__ inline double q6_plgndr(int l, int m, double x) {
+  double pmm = 1.0;
+  if (m > 0) {
+    const double somx2 = sqrt((1.0 - x) * (1.0 + x));
+    double fact = 1.0;
+    for (int i = 1; i <= m; i++) {
+      pmm = -pmm * fact * somx2;
+      fact = fact + 2.0;
+    }
+  }
+  if (l == m) return pmm;
+  double pmmp1 = x * (2 * m + 1) * pmm;
+  if (l == m + 1) return pmmp1;
+  double pll = 0.0;
+  for (int ll = m + 2; ll <= l; ll++) {
+    pll = (x * (2 * ll - 1) * pmmp1 - (ll + m - 1) * pmm) / (ll - m);
+    pmm = pmmp1;
+    pmmp1 = pll;
+  }
+  return pll;
+}
+
+// qlm(6, m, x, y, z) of q6.F90 for m = 0..6 (q[2m], q[2m+1]); m < 0 is
+// (-1)^m conjg of these.  As the reference evaluates it: w = cmplx(x, y)
+// (rounded to single precision) divided by |(x, y)| as a complex division,
+// w**m by repeated squaring, the prefactor times plgndr before the power.
+__device__ inline void q6_ylm6(double x, double y, double z, double (&q)[14]) {
+  const double prod[13] = {1.0, 1.0, 2.0, 6.0, 24.0, 120.0, 720.0, 5040.0, 40320.0,
+                           362880.0, 3628800.0, 39916800.0, 479001600.0};
+  const double r = sqrt(x * x + y * y);
+  const double fa = (double)(float)x, fb = (double)(float)y, den = r * r + 0.0 * 0.0;
+  const double wa = (fa * r + fb * 0.0) / den, wb = (fb * r - fa * 0.0) / den;
+  for (int m = 0; m <= 6; m++) {
+    const double c = sqrt((2.0 * 6 + 1.0) * prod[6 - m] / (4.0 * kPiT * prod[6 + m])) *
+                     q6_plgndr(6, m, z);
+    double pa = 1.0, pb = 0.0, xa = wa, xb = wb;
+    int k = m;
+    while (k > 0) {
+      if (k & 1) {
+        const double ta = pa * xa - pb * xb, tb = pa * xb + pb * xa;
+        pa = ta;
+        pb = tb;
+      }
+      k >>= 1;
+      if (k > 0) {
+        const double ta = xa * xa - xb * xb, tb = xa * xb + xb * xa;
+        xa = ta;
+        xb = tb;
+      }
+    }
+    q[2 * m] = c * pa;
+    q[2 * m + 1] = c * pb;
+  }
+}
+
+// qlm of every bond (i, jj) within 3 A: [i][jj][m = -6..6][re, im]
+__global__ void k_q6_qlm(int n, const int *nn3, const double *dx, const double *dy,
+                         const double *dz, const double *dd, double *qlm, int *err) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = t / kMaxBond, jj = t % kMaxBond;
+  if (i >= n) return;
+  const int c = nn3[i];
+  if (c > kMaxBond) {
+    if (jj == 0) atomicOr(err, 1);  // the reference's bond arrays end at MAXBOND
+    return;
+  }
+  if (jj >= c) return;
+  const size_t e = (size_t)i * kRow + jj;
+  const double pp = 1.0 / sqrt(dd[e]);
+  const double px = dx[e] * pp, py = dy[e] * pp, pz = dz[e] * pp;
+  double q[14];
+  q6_ylm6(px, py, pz, q);
+  double *o = qlm + ((size_t)i * kMaxBond + jj) * 26;
+  for (int m = -6; m <= 6; m++) {
+    const int am = m < 0 ? -m : m;
+    double re = q[2 * am], im = q[2 * am + 1];
+    if (m < 0) {
+      const double sg = (am & 1) ? -1.0 : 1.0;
+      re = sg * re;
+      im = sg * -im;
+    }
+    o[2 * (m + 6)] = re;
+    o[2 * (m + 6) + 1] = im;
+  }
+}
+
+// label in a sorted array (binary search)
+__device__ inline bool nb_has(const int *a, int c, int key) {
+  int lo = 0, hi = c;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (a[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo < c && a[lo] == key;
+}
+
+// q6 of molecule k (one wave): the sum of qlm over the bonds (i, j) with i
+// and j both in neis(k) = nei5(k) + k, in the order of i then of the bond,
+// which is the order the reference adds them to qlmsum(k); lane c < 26 holds
+// one component.  q6 = sqrt(sum_m |qlmsum|^2 * 4 pi / 13) / nbond
+// (Q6Local.F90:101-231).
+__global__ __launch_bounds__(64) void k_q6_sum(int n, const int *nn3, const int *near3,
+                                                const int *nn5, const int *near5,
+                                                const double *qlm, double *q6, int *nbond) {
+  __shared__ double s_a[13];
+  const int k = blockIdx.x, lane = threadIdx.x;
+  const int c5 = nn5[k];
+  const int *a5 = near5 + (size_t)k * kRow;
+  double acc = 0.0;
+  int nb = 0;
+  // neis(k) in label order: nei5(k) with k inserted
+  bool kdone = false;
+  for (int p = 0; p <= c5; p++) {
+    int i;
+    if (p < c5 && (kdone || a5[p] < k)) {
+      i = a5[p];
+    } else if (!kdone) {
+      i = k;
+      kdone = true;
+      p--;  // a5[p] still to come
+    } else {
+      break;
+    }
+    const int c3 = nn3[i];
+    for (int jj = 0; jj < c3; jj++) {
+      const int j = near3[(size_t)i * kRow + jj];
+      if (j != k && !nb_has(a5, c5, j)) continue;
+      nb++;
+      if (lane < 26) acc = acc + qlm[((size_t)i * kMaxBond + jj) * 26 + lane];
+    }
+  }
+  // |qlmsum(m)| for each m, then the sum over m in order
+  const double re = __shfl(acc, (2 * lane) & 63, 64), im = __shfl(acc, (2 * lane + 1) & 63, 64);
+  if (lane < 13) {
+    const double a = hypot(re, im);
+    s_a[lane] = a * a;
+  }
+  __syncthreads();
+  if (lane == 0) {
+    double nq = 0.0;
+    for (int m = 0; m < 13; m++) nq = nq + s_a[m];
+    q6[k] = sqrt(nq * 4.0 * kPiT / (2.0 * 6 + 1.0)) / (double)nb;
+    nbond[k] = nb;
+  }
+}
+
+// the sum of q6 in molecule order and op = sum / n
+__global__ void k_q6_total(int n, const double *q6, double *out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double s = 0.0;
+  for (int i = 0; i < n; i++) s = s + q6[i];
+  out[0] = s;
+  out[1] = s / n;
+}
+
+// the O-O-O angles of each molecule's (up to) four nearest neighbours
+// within 5 A, binned as histogram_accumulate: 21 bins of 0.1 from -1
+// (Triangle.F90:53-116, DistribModule.F90:149-158); counts are integers
+__global__ void k_tri_hist(int n, const int *nn, const double *dx, const double *dy,
+                           const double *dz, const double *dd, unsigned int *cnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const size_t e = (size_t)i * kRow;
+  const int c = nn[i];
+  for (int j = 0; j < c; j++)
+    for (int k = j + 1; k < c; k++) {
+      double cs = dx[e + j] * dx[e + k] + dy[e + j] * dy[e + k] + dz[e + j] * dz[e + k];
+      cs = cs / sqrt(dd[e + j] * dd[e + k]);
+      const int bin = (int)round((cs - (-1.0)) / 0.1) + 1;
+      if (1 <= bin && bin <= 21) atomicAdd(&cnt[bin - 1], 1u);
+    }
+}
+
+// kullback_div of the histogram against the reference histogram
+// (DistribModule.F90:304-326), and its log for the logarithmic form
+__global__ void k_tri_op(const unsigned int *cnt, const double *ref, int logarithmic,
+                         double *out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double h[21], s1 = 0.0, s2 = 0.0, div = 0.0;
+  for (int b = 0; b < 21; b++) {
+    h[b] = (double)cnt[b];
+    out[b] = h[b];
+  }
+  for (int b = 0; b < 21; b++) {
+    s1 = s1 + h[b];
+    s2 = s2 + ref[b];
+  }
+  for (int b = 0; b < 21; b++) {
+    const double p = h[b] / s1;
+    double q = ref[b] / s2;
+    if (q == 0) q = p;
+    if (p != 0) div = div + p * log(p / q);
+  }
+  out[21] = logarithmic ? log(div) : div;
+}
+
 // device buffers of one call, released together
 struct NbMem {
   std::vector<void *> keep;
@@ -339,6 +541,95 @@ static bool nb_order_ok(int order) {
   return order == kOrderNone || order == kOrderDist || order == kOrderLabel;
 }
 
+// the arguments of a pair-list entry point: labels in range, no self pairs
+static int nb_check_pairs(int nsite, int64_t npair, const int32_t *pi, const int32_t *pj,
+                          const double *ox, const double *oy, const double *oz,
+                          const double *coord) {
+  if (nsite <= 0 || npair < 0 || !coord) return G6_EARG;
+  if (npair > 0 && (!pi || !pj || !ox || !oy || !oz)) return G6_EARG;
+  if (npair > ((int64_t)1 << 30)) return G6_EARG;
+  for (int64_t k = 0; k < npair; k++)
+    if (pi[k] < 0 || pi[k] >= nsite || pj[k] < 0 || pj[k] >= nsite || pi[k] == pj[k])
+      return G6_EARG;
+  return 0;
+}
+
+// a pair list and the molecule centres on the device
+struct NbPairs {
+  int n = 0;
+  int64_t np = 0;
+  int *pi = nullptr, *pj = nullptr;
+  double *ox = nullptr, *oy = nullptr, *oz = nullptr, *x = nullptr, *y = nullptr,
+         *z = nullptr;
+  int upload(NbMem &mem, int nsite, int64_t npair, const int32_t *hpi, const int32_t *hpj,
+             const double *hox, const double *hoy, const double *hoz, const double *coord,
+             hipStream_t st) {
+    n = nsite;
+    np = npair;
+    const size_t p = (size_t)npair, m = (size_t)nsite;
+    pi = mem.get<int>(p);
+    pj = mem.get<int>(p);
+    ox = mem.get<double>(p);
+    oy = mem.get<double>(p);
+    oz = mem.get<double>(p);
+    x = mem.get<double>(m);
+    y = mem.get<double>(m);
+    z = mem.get<double>(m);
+    if (mem.bad) return G6_EHIP;
+    // coord(nsite, 3): component-major, as com
+    const bool ok = mem.put(x, coord, m, st) && mem.put(y, coord + m, m, st) &&
+                    mem.put(z, coord + 2 * m, m, st) && mem.put(pi, hpi, p, st) &&
+                    mem.put(pj, hpj, p, st) && mem.put(ox, hox, p, st) &&
+                    mem.put(oy, hoy, p, st) && mem.put(oz, hoz, p, st);
+    return ok ? 0 : G6_EHIP;
+  }
+};
+
+// neighbors_all2 into a device table: 0, G6_ECAPACITY (an array past 151
+// entries; the table is still written) or G6_EHIP
+static int nb_all2_dev(NbMem &mem, hipStream_t st, const NbPairs &d, int maxnei, int order,
+                       double radius, NbTab &out) {
+  const size_t np = (size_t)d.np, n = (size_t)d.n;
+  int *cnt = mem.get<int>(n), *off = mem.get<int>(n + 1), *fill = mem.get<int>(n);
+  int *derr = mem.get<int>(1);
+  long long *ent = mem.get<long long>(2 * np);
+  if (mem.bad) return G6_EHIP;
+  bool ok = hipMemsetAsync(cnt, 0, sizeof(int) * n, st) == hipSuccess &&
+            hipMemsetAsync(fill, 0, sizeof(int) * n, st) == hipSuccess &&
+            hipMemsetAsync(derr, 0, sizeof(int), st) == hipSuccess;
+  if (!ok) return G6_EHIP;
+  const double r2 = radius > 0 ? radius * radius : -1.0;
+  const unsigned gp = (unsigned)((np + 255) / 256);
+  if (np > 0)
+    k_nb_count<<<gp, 256, 0, st>>>(d.np, d.pi, d.pj, d.ox, d.oy, d.oz, d.x, d.y, d.z, r2, cnt);
+  std::vector<int> hc(n), ho(n + 1);
+  ok = mem.take(hc.data(), cnt, n, st) && hipStreamSynchronize(st) == hipSuccess;
+  if (!ok) return G6_EHIP;
+  ho[0] = 0;
+  for (size_t m = 0; m < n; m++) ho[m + 1] = ho[m] + hc[m];
+  ok = mem.put(off, ho.data(), n + 1, st);
+  if (ok && np > 0)
+    k_nb_scatter<<<gp, 256, 0, st>>>(d.np, d.pi, d.pj, d.ox, d.oy, d.oz, d.x, d.y, d.z, r2, off,
+                                     fill, ent);
+  if (ok)
+    k_nb_all2<<<d.n, 64, 0, st>>>(d.n, d.pi, d.pj, d.ox, d.oy, d.oz, d.x, d.y, d.z, off, ent,
+                                  maxnei, order, out.nn, out.near, out.dx, out.dy, out.dz,
+                                  out.dd, derr);
+  int herr = 0;
+  ok = ok && hipGetLastError() == hipSuccess && mem.take(&herr, derr, 1, st) &&
+       hipStreamSynchronize(st) == hipSuccess;
+  if (!ok) return G6_EHIP;
+  return herr ? G6_ECAPACITY : 0;
+}
+
+// neighbor1_compress2 of every array of a device table
+static void nb_compress_dev(hipStream_t st, int nrow, const NbTab &in, int maxnei, int order,
+                            double radius, NbTab &out) {
+  const double r2 = radius > 0 ? radius * radius : -1.0;
+  k_nb_compress<<<nrow, 64, 0, st>>>(in.nn, in.near, in.dx, in.dy, in.dz, in.dd, maxnei, order,
+                                     r2, out.nn, out.near, out.dx, out.dy, out.dz, out.dd);
+}
+
 }  // namespace g6
 
 using namespace g6;
@@ -350,57 +641,22 @@ int g6_neighbors(g6_handle h, int nsite, int64_t npair, const int32_t *pair_i,
                  const double *coord, int maxnei, int order, double radius, int32_t *nnear,
                  int32_t *near, double *dx, double *dy, double *dz, double *dd) {
   System *s = system_of(h);
-  if (!s || nsite <= 0 || npair < 0 || !coord || !nnear || !near || !dx || !dy || !dz ||
-      !dd || maxnei < 0 || !nb_order_ok(order))
+  if (!s || !nnear || !near || !dx || !dy || !dz || !dd || maxnei < 0 || !nb_order_ok(order))
     return G6_EARG;
-  if (npair > 0 && (!pair_i || !pair_j || !ox || !oy || !oz)) return G6_EARG;
-  if (npair > ((int64_t)1 << 30)) return G6_EARG;
-  for (int64_t k = 0; k < npair; k++)
-    if (pair_i[k] < 0 || pair_i[k] >= nsite || pair_j[k] < 0 || pair_j[k] >= nsite ||
-        pair_i[k] == pair_j[k])
-      return G6_EARG;
-  const size_t np = (size_t)npair, n = (size_t)nsite;
+  int rc = nb_check_pairs(nsite, npair, pair_i, pair_j, ox, oy, oz, coord);
+  if (rc) return rc;
   hipStream_t st = s->stream;
   NbMem mem;
-  int *dpi = mem.get<int>(np), *dpj = mem.get<int>(np);
-  double *dox = mem.get<double>(np), *doy = mem.get<double>(np), *doz = mem.get<double>(np);
-  double *x = mem.get<double>(n), *y = mem.get<double>(n), *z = mem.get<double>(n);
-  int *cnt = mem.get<int>(n), *off = mem.get<int>(n + 1), *fill = mem.get<int>(n);
-  int *derr = mem.get<int>(1);
-  long long *ent = mem.get<long long>(2 * np);
+  NbPairs d;
   NbTab out;
-  out.alloc(mem, n);
-  if (mem.bad) return G6_EHIP;
-  // coord(nsite, 3): component-major, as com
-  bool ok = mem.put(x, coord, n, st) && mem.put(y, coord + n, n, st) &&
-            mem.put(z, coord + 2 * n, n, st) && mem.put(dpi, pair_i, np, st) &&
-            mem.put(dpj, pair_j, np, st) && mem.put(dox, ox, np, st) &&
-            mem.put(doy, oy, np, st) && mem.put(doz, oz, np, st) &&
-            hipMemsetAsync(cnt, 0, sizeof(int) * n, st) == hipSuccess &&
-            hipMemsetAsync(fill, 0, sizeof(int) * n, st) == hipSuccess &&
-            hipMemsetAsync(derr, 0, sizeof(int), st) == hipSuccess;
+  out.alloc(mem, (size_t)nsite);
+  if ((rc = d.upload(mem, nsite, npair, pair_i, pair_j, ox, oy, oz, coord, st))) return rc;
+  const int ec = nb_all2_dev(mem, st, d, maxnei, order, radius, out);
+  if (ec == G6_EHIP) return ec;
+  const bool ok = out.download(mem, (size_t)nsite, nnear, near, dx, dy, dz, dd, st) &&
+                  hipStreamSynchronize(st) == hipSuccess;
   if (!ok) return G6_EHIP;
-  const double r2 = radius > 0 ? radius * radius : -1.0;
-  const unsigned gp = (unsigned)((np + 255) / 256);
-  if (np > 0) k_nb_count<<<gp, 256, 0, st>>>(npair, dpi, dpj, dox, doy, doz, x, y, z, r2, cnt);
-  std::vector<int> hc(n), ho(n + 1);
-  ok = mem.take(hc.data(), cnt, n, st) && hipStreamSynchronize(st) == hipSuccess;
-  if (!ok) return G6_EHIP;
-  ho[0] = 0;
-  for (size_t m = 0; m < n; m++) ho[m + 1] = ho[m] + hc[m];
-  ok = mem.put(off, ho.data(), n + 1, st);
-  if (ok && np > 0)
-    k_nb_scatter<<<gp, 256, 0, st>>>(npair, dpi, dpj, dox, doy, doz, x, y, z, r2, off, fill, ent);
-  if (ok)
-    k_nb_all2<<<nsite, 64, 0, st>>>(nsite, dpi, dpj, dox, doy, doz, x, y, z, off, ent, maxnei,
-                                    order, out.nn, out.near, out.dx, out.dy, out.dz, out.dd,
-                                    derr);
-  int herr = 0;
-  ok = ok && hipGetLastError() == hipSuccess && mem.take(&herr, derr, 1, st) &&
-       out.download(mem, n, nnear, near, dx, dy, dz, dd, st) &&
-       hipStreamSynchronize(st) == hipSuccess;
-  if (!ok) return G6_EHIP;
-  return herr ? G6_ECAPACITY : 0;
+  return ec;
 }
 
 int g6_neighbors_compress(g6_handle h, int nrow, const int32_t *nnear_in,
@@ -422,11 +678,7 @@ int g6_neighbors_compress(g6_handle h, int nrow, const int32_t *nnear_in,
   out.alloc(mem, n);
   if (mem.bad) return G6_EHIP;
   bool ok = in.upload(mem, n, nnear_in, near_in, dx_in, dy_in, dz_in, dd_in, st);
-  const double r2 = radius > 0 ? radius * radius : -1.0;
-  if (ok)
-    k_nb_compress<<<nrow, 64, 0, st>>>(in.nn, in.near, in.dx, in.dy, in.dz, in.dd, maxnei,
-                                       order, r2, out.nn, out.near, out.dx, out.dy, out.dz,
-                                       out.dd);
+  if (ok) nb_compress_dev(st, nrow, in, maxnei, order, radius, out);
   ok = ok && hipGetLastError() == hipSuccess &&
        out.download(mem, n, nnear, near, dx, dy, dz, dd, st) &&
        hipStreamSynchronize(st) == hipSuccess;
@@ -498,6 +750,89 @@ int g6_neighbors_displacement(g6_handle h, int nsite, const int32_t *nnear_in,
     if (hpos[j] >= 0) index[m++] = rows[1 + j];
   *nmodified = nm;
   return m == nm ? 0 : G6_EHIP;
+}
+
+int g6_q6local(g6_handle h, int nsite, int64_t npair, const int32_t *pair_i,
+               const int32_t *pair_j, const double *ox, const double *oy, const double *oz,
+               const double *coord, double *q6, int32_t *nbond, double *sumq6, double *op) {
+  System *s = system_of(h);
+  if (!s || !q6 || !nbond || !sumq6 || !op) return G6_EARG;
+  int rc = nb_check_pairs(nsite, npair, pair_i, pair_j, ox, oy, oz, coord);
+  if (rc) return rc;
+  hipStream_t st = s->stream;
+  const size_t n = (size_t)nsite;
+  NbMem mem;
+  NbPairs d;
+  NbTab t6, t3, t5;
+  t6.alloc(mem, n);
+  t3.alloc(mem, n);
+  t5.alloc(mem, n);
+  double *qlm = mem.get<double>(n * kMaxBond * 26), *dq6 = mem.get<double>(n),
+         *tot = mem.get<double>(2);
+  int *dnb = mem.get<int>(n), *derr = mem.get<int>(1);
+  if (mem.bad) return G6_EHIP;
+  if ((rc = d.upload(mem, nsite, npair, pair_i, pair_j, ox, oy, oz, coord, st))) return rc;
+  // nei6 by label within 6 A, then nei3 and nei5 from it (Q6Local.F90:123-130)
+  rc = nb_all2_dev(mem, st, d, 0, kOrderLabel, 6.0, t6);
+  if (rc) return rc;
+  nb_compress_dev(st, nsite, t6, 0, kOrderLabel, 3.0, t3);
+  nb_compress_dev(st, nsite, t6, 0, kOrderLabel, 5.0, t5);
+  if (hipMemsetAsync(derr, 0, sizeof(int), st) != hipSuccess) return G6_EHIP;
+  k_q6_qlm<<<(unsigned)((n * kMaxBond + 255) / 256), 256, 0, st>>>(nsite, t3.nn, t3.dx, t3.dy,
+                                                                   t3.dz, t3.dd, qlm, derr);
+  int herr = 0;
+  bool ok = hipGetLastError() == hipSuccess && mem.take(&herr, derr, 1, st) &&
+            hipStreamSynchronize(st) == hipSuccess;
+  if (!ok) return G6_EHIP;
+  if (herr) return G6_ECAPACITY;  // more than MAXBOND bonds within 3 A
+  k_q6_sum<<<nsite, 64, 0, st>>>(nsite, t3.nn, t3.near, t5.nn, t5.near, qlm, dq6, dnb);
+  k_q6_total<<<1, 64, 0, st>>>(nsite, dq6, tot);
+  double ht[2];
+  ok = hipGetLastError() == hipSuccess && mem.take(q6, dq6, n, st) &&
+       mem.take(nbond, dnb, n, st) && mem.take(ht, tot, 2, st) &&
+       hipStreamSynchronize(st) == hipSuccess;
+  if (!ok) return G6_EHIP;
+  *sumq6 = ht[0];
+  *op = ht[1];
+  return 0;
+}
+
+int g6_triangle(g6_handle h, int nsite, int64_t npair, const int32_t *pair_i,
+                const int32_t *pair_j, const double *ox, const double *oy, const double *oz,
+                const double *coord, const double *refhist, int logarithmic, double *hist,
+                double *op) {
+  System *s = system_of(h);
+  if (!s || !refhist || !hist || !op) return G6_EARG;
+  int rc = nb_check_pairs(nsite, npair, pair_i, pair_j, ox, oy, oz, coord);
+  if (rc) return rc;
+  hipStream_t st = s->stream;
+  const size_t n = (size_t)nsite;
+  NbMem mem;
+  NbPairs d;
+  NbTab t2, t4;
+  t2.alloc(mem, n);
+  t4.alloc(mem, n);
+  unsigned int *cnt = mem.get<unsigned int>(21);
+  double *dref = mem.get<double>(21), *out = mem.get<double>(22);
+  if (mem.bad) return G6_EHIP;
+  if ((rc = d.upload(mem, nsite, npair, pair_i, pair_j, ox, oy, oz, coord, st))) return rc;
+  // nei2 by distance within 5 A, nei = its four nearest (Triangle.F90:81-87)
+  rc = nb_all2_dev(mem, st, d, 0, kOrderDist, 5.0, t2);
+  if (rc) return rc;
+  nb_compress_dev(st, nsite, t2, 4, kOrderDist, 0.0, t4);
+  bool ok = mem.put(dref, refhist, 21, st) &&
+            hipMemsetAsync(cnt, 0, sizeof(unsigned int) * 21, st) == hipSuccess;
+  if (!ok) return G6_EHIP;
+  k_tri_hist<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(nsite, t4.nn, t4.dx, t4.dy, t4.dz,
+                                                           t4.dd, cnt);
+  k_tri_op<<<1, 64, 0, st>>>(cnt, dref, logarithmic, out);
+  double ho[22];
+  ok = hipGetLastError() == hipSuccess && mem.take(ho, out, 22, st) &&
+       hipStreamSynchronize(st) == hipSuccess;
+  if (!ok) return G6_EHIP;
+  for (int b = 0; b < 21; b++) hist[b] = ho[b];
+  *op = ho[21];
+  return 0;
 }
 
 }  // extern "C"

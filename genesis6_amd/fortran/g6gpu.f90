@@ -182,6 +182,27 @@ module g6gpu
        real(c_double), intent(out) :: dx(*), dy(*), dz(*), dd(*)
      end function g6_neighbors_displacement
 
+     integer(c_int) function g6_q6local(h, nsite, npair, pair_i, pair_j, ox, oy, oz, coord, &
+          q6, nbond, sumq6, op) bind(C, name="g6_q6local")
+       import :: c_int, c_int64_t, c_int32_t, c_double
+       integer(c_int64_t), value :: h, npair
+       integer(c_int), value :: nsite
+       integer(c_int32_t), intent(in) :: pair_i(*), pair_j(*)
+       real(c_double), intent(in) :: ox(*), oy(*), oz(*), coord(*)
+       real(c_double), intent(out) :: q6(*), sumq6, op
+       integer(c_int32_t), intent(out) :: nbond(*)
+     end function g6_q6local
+
+     integer(c_int) function g6_triangle(h, nsite, npair, pair_i, pair_j, ox, oy, oz, coord, &
+          refhist, logarithmic, hist, op) bind(C, name="g6_triangle")
+       import :: c_int, c_int64_t, c_int32_t, c_double
+       integer(c_int64_t), value :: h, npair
+       integer(c_int), value :: nsite, logarithmic
+       integer(c_int32_t), intent(in) :: pair_i(*), pair_j(*)
+       real(c_double), intent(in) :: ox(*), oy(*), oz(*), coord(*), refhist(21)
+       real(c_double), intent(out) :: hist(21), op
+     end function g6_triangle
+
      integer(c_int) function g6_set_andersen(h, mode, pex, mass, v) &
           bind(C, name="g6_set_andersen")
        import :: c_int, c_int64_t, c_double, c_ptr

@@ -60,6 +60,10 @@ def lib():
                                      _dp, _dp]
         L.g6_site_positions.argtypes = [_H, _dp, _dp, _dp, _dp, _dp]
         L.g6_collect.argtypes = [_H, _dp, _dp, _dp, _dp, _dp, _dp]
+        L.g6_q6local.argtypes = [_H, C.c_int, C.c_int64, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _ip,
+                                 C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.g6_triangle.argtypes = [_H, C.c_int, C.c_int64, _ip, _ip, _dp, _dp, _dp, _dp, _dp,
+                                  C.c_int, _dp, C.POINTER(C.c_double)]
         L.g6_set_andersen.argtypes = [_H, C.c_int, C.c_double, C.c_double, C.c_void_p]
         L.g6_get_andersen.argtypes = [_H, _dp]
         L.g6_get_box.argtypes = [_H, C.POINTER(C.c_double)]
@@ -380,6 +384,33 @@ class System:
               "g6_neighbors_displacement")
         m = int(nmod[0])
         return idx[:m].copy(), out.rows(m)
+
+
+def _pairs(pi, pj, shifts, coord):
+    sh = _c(shifts).reshape(-1, 3)
+    return [len(pi), _c(pi, np.int32), _c(pj, np.int32), _c(sh[:, 0]), _c(sh[:, 1]),
+            _c(sh[:, 2]), _c(np.asarray(coord).T).ravel()]
+
+
+def q6local(s, pi, pj, shifts, coord):
+    """Q6 local bond order of mc2's umbrella (sq6system_prepareall) on the
+    device of System s: (q6[n], nbond[n], sumq6, op)."""
+    n = len(coord)
+    q6 = np.zeros(n); nb = np.zeros(n, np.int32)
+    sm, op = C.c_double(0), C.c_double(0)
+    check(lib().g6_q6local(s.h, n, *_pairs(pi, pj, shifts, coord), q6, nb, C.byref(sm),
+                           C.byref(op)), "g6_q6local")
+    return q6, nb, sm.value, op.value
+
+
+def triangle(s, pi, pj, shifts, coord, refhist, logarithmic=False):
+    """The triangle order parameter (sTriangle_prepareall): (hist[21], op)."""
+    n = len(coord)
+    h = np.zeros(21)
+    op = C.c_double(0)
+    check(lib().g6_triangle(s.h, n, *_pairs(pi, pj, shifts, coord), _c(refhist),
+                            int(logarithmic), h, C.byref(op)), "g6_triangle")
+    return h, op.value
 
 
 class NeighborTable:

@@ -176,6 +176,25 @@ int g6_neighbors_displacement(g6_handle h, int nsite, const int32_t *nnear_in,
                               int32_t *near, double *dx, double *dy, double *dz,
                               double *dd);
 
+/* ---- umbrella order parameters of mc2 (Q6Local.F90, Triangle.F90) -------
+ * On a pair list and molecule centres as g6_neighbors.  Q6 local bond order,
+ * sq6system_prepareall (Q6Local.F90:101-231): q6[nsite], the bond count
+ * nbond[nsite] of each molecule, their sum and op = sum / nsite;
+ * G6_ECAPACITY when a molecule has more than MAXBOND = 10 neighbours within
+ * 3 A (the reference's arrays end there). */
+int g6_q6local(g6_handle h, int nsite, int64_t npair, const int32_t *pair_i,
+               const int32_t *pair_j, const double *ox, const double *oy,
+               const double *oz, const double *coord, double *q6, int32_t *nbond,
+               double *sumq6, double *op);
+/* sTriangle_prepareall (Triangle.F90:53-116): the histogram (21 bins of 0.1
+ * from -1) of the O-O-O angles of each molecule's four nearest neighbours
+ * within 5 A, and its Kullback-Leibler divergence from refhist[21]
+ * (DistribModule.F90:304-326); logarithmic: its log (LOGTRIAN). */
+int g6_triangle(g6_handle h, int nsite, int64_t npair, const int32_t *pair_i,
+                const int32_t *pair_j, const double *ox, const double *oy,
+                const double *oz, const double *coord, const double *refhist,
+                int logarithmic, double *hist, double *op);
+
 /* ---- constant-pressure MD: the Andersen barostat (AndersenModule.F90) ---
  * mode 1 (orthorhombic: the cubic box follows the volume, the reference's
  * only mode for a cube) with external pressure pex (atm) and mass per

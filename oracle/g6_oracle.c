@@ -1584,3 +1584,211 @@ int g6o_neighbors_displacement(g6o_nbtab in, int order, int target, double ddx, 
   for (int r = 0; r < m; r++) nb_sort(out, r, order);
   return 0;
 }
+
+/* ------------------------------------------------------------------------
+ * Umbrella order parameters of mc2 (Q6Local.F90, Triangle.F90, q6.F90,
+ * DistribModule.F90), on the neighbour arrays above.
+ * ---------------------------------------------------------------------- */
+
+/* plgndr (q6.F90): the associated Legendre function by the upward
+ * recurrence in l */
+static double q6_plgndr(int l, int m, double x) {
+  double pmm = 1.0;
+  if (m > 0) {
+    const double somx2 = sqrt((1.0 - x) * (1.0 + x));
+    double fact = 1.0;
+    for (int i = 1; i <= m; i++) {
+      pmm = -pmm * fact * somx2;
+      fact = fact + 2.0;
+    }
+  }
+  if (l == m) return pmm;
+  double pmmp1 = x * (2 * m + 1) * pmm;
+  if (l == m + 1) return pmmp1;
+  double pll = 0.0;
+  for (int ll = m + 2; ll <= l; ll++) {
+    pll = (x * (2 * ll - 1) * pmmp1 - (ll + m - 1) * pmm) / (ll - m);
+    pmm = pmmp1;
+    pmmp1 = pll;
+  }
+  return pll;
+}
+
+/* complex integer power by repeated squaring (the exponent's bits from the
+ * lowest), the Fortran runtime's z**m */
+static void q6_cpow(double a, double b, int m, double *ra, double *rb) {
+  double pa = 1.0, pb = 0.0, xa = a, xb = b;
+  int k = m;
+  while (k > 0) {
+    if (k & 1) {
+      const double ta = pa * xa - pb * xb, tb = pa * xb + pb * xa;
+      pa = ta;
+      pb = tb;
+    }
+    k >>= 1;
+    if (k > 0) {
+      const double ta = xa * xa - xb * xb, tb = xa * xb + xb * xa;
+      xa = ta;
+      xb = tb;
+    }
+  }
+  *ra = pa;
+  *rb = pb;
+}
+
+/* qlm(6, m, x, y, z) of q6.F90 with ylm2/ylm: w = cmplx(x, y) / |(x, y)|,
+ * where cmplx without a kind rounds x and y to single precision and the
+ * division is the complex one, (a c + b d, b c - a d) / (c^2 + d^2) with d = 0
+ * (checked bit for bit against the reference's qlm built with flang) */
+void g6o_q6_qlm(int m, double x, double y, double z, double *re, double *im) {
+  static const double prod[13] = {1.0, 1.0, 2.0, 6.0, 24.0, 120.0, 720.0, 5040.0, 40320.0,
+                                  362880.0, 3628800.0, 39916800.0, 479001600.0};
+  const int l = 6;
+  const double r = sqrt(x * x + y * y);
+  const double fa = (double)(float)x, fb = (double)(float)y, den = r * r + 0.0 * 0.0;
+  const double wa = (fa * r + fb * 0.0) / den, wb = (fb * r - fa * 0.0) / den;
+  const int am = m < 0 ? -m : m;
+  const double c = sqrt((2.0 * l + 1.0) * prod[l - am] / (4.0 * PI_T * prod[l + am])) *
+                   q6_plgndr(l, am, z);
+  double pa, pb;
+  q6_cpow(wa, wb, am, &pa, &pb);
+  double va = c * pa, vb = c * pb;
+  if (m < 0) {  /* (-1)**m * conjg(ylm(l, -m)) */
+    const double s = (am & 1) ? -1.0 : 1.0;
+    va = s * va;
+    vb = s * -vb;
+  }
+  *re = va;
+  *im = vb;
+}
+
+static int tab_alloc(g6o_nbtab *t, int rows) {
+  t->nnear = calloc(rows, sizeof(int));
+  t->near = calloc((size_t)rows * G6O_NEAR, sizeof(int));
+  t->dx = calloc((size_t)rows * G6O_NEAR * 4, sizeof(double));
+  t->dy = t->dx + (size_t)rows * G6O_NEAR;
+  t->dz = t->dy + (size_t)rows * G6O_NEAR;
+  t->dd = t->dz + (size_t)rows * G6O_NEAR;
+  return t->nnear && t->near && t->dx;
+}
+static void tab_free(g6o_nbtab *t) {
+  free(t->nnear);
+  free(t->near);
+  free(t->dx);
+}
+
+/* membership in neis(i) = nei5(i) + i: the label-sorted array or i itself */
+static int q6_in(const g6o_nbtab *t5, int i, int k) {
+  if (k == i) return 1;
+  const int *a = t5->near + (long)i * G6O_NEAR;
+  for (int p = 0; p < t5->nnear[i]; p++)
+    if (a[p] == k) return 1;
+  return 0;
+}
+
+/* sq6system_prepareall (Q6Local.F90:101-231): q6[n], nbond[n], the sum and
+ * op = sum / n; returns 1 when a molecule has more than MAXBOND = 10 bonds
+ * within 3 A (the reference's arrays end there) or an array overflows */
+int g6o_q6local(int nsite, long npair, const int *pi, const int *pj, const double *ox,
+                const double *oy, const double *oz, const double *coord, double *q6,
+                int *nbond, double *sumq6, double *op) {
+  g6o_nbtab t6, t3, t5;
+  int err = 0;
+  if (!tab_alloc(&t6, nsite) || !tab_alloc(&t3, nsite) || !tab_alloc(&t5, nsite)) return 1;
+  err |= g6o_neighbors_all2(nsite, npair, pi, pj, ox, oy, oz, coord, 0, 2, 6.0, t6);
+  err |= g6o_neighbors_compress(nsite, t6, 0, 2, 3.0, t3);
+  err |= g6o_neighbors_compress(nsite, t6, 0, 2, 5.0, t5);
+  for (int i = 0; i < nsite; i++) err |= t3.nnear[i] > 10;
+  double *qlm = calloc((size_t)nsite * 10 * 26, sizeof(double));
+  double *qs = calloc((size_t)nsite * 26, sizeof(double));
+  if (!err) {
+    for (int i = 0; i < nsite; i++)
+      for (int jj = 0; jj < t3.nnear[i]; jj++) {
+        const long e = (long)i * G6O_NEAR + jj;
+        const double pp = 1.0 / sqrt(t3.dd[e]);
+        const double px = t3.dx[e] * pp, py = t3.dy[e] * pp, pz = t3.dz[e] * pp;
+        for (int m = -6; m <= 6; m++)
+          g6o_q6_qlm(m, px, py, pz, &qlm[((size_t)i * 10 + jj) * 26 + 2 * (m + 6)],
+                     &qlm[((size_t)i * 10 + jj) * 26 + 2 * (m + 6) + 1]);
+      }
+    for (int k = 0; k < nsite; k++) nbond[k] = 0;
+    /* neib(jj, i) = neis(i) & neis(j); qlmsum(k) over the bonds whose neib
+     * holds k, in the order i, jj */
+    for (int i = 0; i < nsite; i++)
+      for (int jj = 0; jj < t3.nnear[i]; jj++) {
+        const int j = t3.near[(long)i * G6O_NEAR + jj];
+        for (int kk = 0; kk <= t5.nnear[i]; kk++) {
+          const int k = kk < t5.nnear[i] ? t5.near[(long)i * G6O_NEAR + kk] : i;
+          if (!q6_in(&t5, j, k)) continue;
+          nbond[k]++;
+          for (int c = 0; c < 26; c++)
+            qs[(size_t)k * 26 + c] = qs[(size_t)k * 26 + c] + qlm[((size_t)i * 10 + jj) * 26 + c];
+        }
+      }
+    double sum = 0.0;
+    for (int i = 0; i < nsite; i++) {
+      double nq = 0.0;
+      for (int m = 0; m < 13; m++) {
+        const double a = hypot(qs[(size_t)i * 26 + 2 * m], qs[(size_t)i * 26 + 2 * m + 1]);
+        nq = nq + a * a;
+      }
+      q6[i] = sqrt(nq * 4.0 * PI_T / (2.0 * 6 + 1.0)) / nbond[i];
+      sum = sum + q6[i];
+    }
+    *sumq6 = sum;
+    *op = sum / nsite;
+  }
+  free(qlm);
+  free(qs);
+  tab_free(&t6);
+  tab_free(&t3);
+  tab_free(&t5);
+  return err;
+}
+
+/* kullback_div (DistribModule.F90:304-326) */
+static double kullback_div(int nbin, const double *h1, const double *h2) {
+  double s1 = 0.0, s2 = 0.0, div = 0.0;
+  for (int i = 0; i < nbin; i++) {
+    s1 = s1 + h1[i];
+    s2 = s2 + h2[i];
+  }
+  for (int i = 0; i < nbin; i++) {
+    const double p = h1[i] / s1;
+    double q = h2[i] / s2;
+    if (q == 0) q = p;
+    if (p != 0) div = div + p * log(p / q);
+  }
+  return div;
+}
+
+/* sTriangle_prepareall (Triangle.F90:53-116): the O-O-O angles of each
+ * molecule's four nearest neighbours within 5 A in a histogram of 21 bins
+ * of 0.1 from -1 (histogram_accumulate, DistribModule.F90:149-158), and its
+ * Kullback-Leibler divergence from refhist (log of it if logarithmic) */
+int g6o_triangle(int nsite, long npair, const int *pi, const int *pj, const double *ox,
+                 const double *oy, const double *oz, const double *coord, const double *refhist,
+                 int logarithmic, double *hist, double *op) {
+  g6o_nbtab t2, t4;
+  int err = 0;
+  if (!tab_alloc(&t2, nsite) || !tab_alloc(&t4, nsite)) return 1;
+  err |= g6o_neighbors_all2(nsite, npair, pi, pj, ox, oy, oz, coord, 0, 1, 5.0, t2);
+  err |= g6o_neighbors_compress(nsite, t2, 4, 1, 0.0, t4);
+  for (int b = 0; b < 21; b++) hist[b] = 0.0;
+  for (int i = 0; i < nsite; i++) {
+    const long e = (long)i * G6O_NEAR;
+    for (int j = 0; j < t4.nnear[i]; j++)
+      for (int k = j + 1; k < t4.nnear[i]; k++) {
+        double c = t4.dx[e + j] * t4.dx[e + k] + t4.dy[e + j] * t4.dy[e + k] +
+                   t4.dz[e + j] * t4.dz[e + k];
+        c = c / sqrt(t4.dd[e + j] * t4.dd[e + k]);
+        const int bin = (int)round((c - (-1.0)) / 0.1) + 1;
+        if (1 <= bin && bin <= 21) hist[bin - 1] = hist[bin - 1] + 1.0;
+      }
+  }
+  *op = kullback_div(21, hist, refhist);
+  if (logarithmic) *op = log(*op);
+  tab_free(&t2);
+  tab_free(&t4);
+  return err;
+}

@@ -182,6 +182,17 @@ int g6o_neighbors_compress(int nrow, g6o_nbtab in, int maxnei, int order, double
 int g6o_neighbors_displacement(g6o_nbtab in, int order, int target, double ddx, double ddy,
                                double ddz, int *nmod, int *idx, g6o_nbtab out);
 
+/* Umbrella order parameters of mc2 on the neighbour arrays: Q6 local bond
+ * order (Q6Local.F90:101-231, q6.F90) and the triangle histogram
+ * (Triangle.F90:53-116, DistribModule.F90:149-158, 304-326) */
+void g6o_q6_qlm(int m, double x, double y, double z, double *re, double *im);
+int g6o_q6local(int nsite, long npair, const int *pi, const int *pj, const double *ox,
+                const double *oy, const double *oz, const double *coord, double *q6,
+                int *nbond, double *sumq6, double *op);
+int g6o_triangle(int nsite, long npair, const int *pi, const int *pj, const double *ox,
+                 const double *oy, const double *oz, const double *coord, const double *refhist,
+                 int logarithmic, double *hist, double *op);
+
 #ifdef __cplusplus
 }
 #endif

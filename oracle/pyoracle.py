@@ -126,6 +126,36 @@ def neighbors_displacement(tab, order, target, delta):
     return idx[:m].copy(), out
 
 
+def q6local(com, pi, pj, shifts):
+    """sq6system_prepareall: (q6[n], nbond[n], sumq6, op, overflow)."""
+    n = len(com)
+    sh = np.ascontiguousarray(shifts, np.float64).reshape(-1, 3)
+    q6 = np.zeros(n); nb = np.zeros(n, np.int32)
+    s, op = C.c_double(0), C.c_double(0)
+    err = lib().g6o_q6local(n, len(pi), np.ascontiguousarray(pi, np.int32),
+                            np.ascontiguousarray(pj, np.int32), np.ascontiguousarray(sh[:, 0]),
+                            np.ascontiguousarray(sh[:, 1]), np.ascontiguousarray(sh[:, 2]),
+                            np.ascontiguousarray(com, np.float64).ravel(), q6, nb, C.byref(s),
+                            C.byref(op))
+    return q6, nb, s.value, op.value, bool(err)
+
+
+def triangle(com, pi, pj, shifts, refhist, logarithmic=False):
+    """sTriangle_prepareall: (hist[21], op)."""
+    n = len(com)
+    sh = np.ascontiguousarray(shifts, np.float64).reshape(-1, 3)
+    h = np.zeros(21)
+    op = C.c_double(0)
+    err = lib().g6o_triangle(n, len(pi), np.ascontiguousarray(pi, np.int32),
+                             np.ascontiguousarray(pj, np.int32), np.ascontiguousarray(sh[:, 0]),
+                             np.ascontiguousarray(sh[:, 1]), np.ascontiguousarray(sh[:, 2]),
+                             np.ascontiguousarray(com, np.float64).ravel(),
+                             np.ascontiguousarray(refhist, np.float64), int(logarithmic), h,
+                             C.byref(op))
+    assert not err
+    return h, op.value
+
+
 def _declare(L):
     L.g6o_model_get.argtypes = [C.c_int, C.POINTER(Model)]
     L.g6o_constants.argtypes = [_dp]
@@ -183,6 +213,12 @@ def _declare(L):
                               C.POINTER(C.c_double), C.POINTER(C.c_longlong), C.c_int,
                               np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS"),
                               _ip]
+    L.g6o_q6local.argtypes = [C.c_int, C.c_long, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _ip,
+                              C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.g6o_triangle.argtypes = [C.c_int, C.c_long, _ip, _ip, _dp, _dp, _dp, _dp, _dp, C.c_int,
+                               _dp, C.POINTER(C.c_double)]
+    L.g6o_q6_qlm.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double,
+                             C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.g6o_neighbors_all2.argtypes = [C.c_int, C.c_long, _ip, _ip, _dp, _dp, _dp, _dp,
                                      C.c_int, C.c_int, C.c_double, NbTab]
     L.g6o_neighbors_compress.argtypes = [C.c_int, NbTab, C.c_int, C.c_int, C.c_double, NbTab]

@@ -66,6 +66,12 @@ $(OUT)/mc2.stamp: $(OUT)/objs.stamp
 	  $(REFFC) $(FFLAGS) -c $(REF)/$$o.F90 -o $(OUT)/$$o.o; done
 	touch $@
 
+# the order-parameter dumper (needs mc2's modules); built on demand by
+# tests/golden/make_order.py, which removes it afterwards
+$(OUT)/refop: oracle/refop.F90 $(OUT)/mc2.stamp $(COBJS)
+	$(REFFC) $(FFLAGS) oracle/refop.F90 $(OBJFILES) $(addprefix $(OUT)/,$(addsuffix .o,$(MC2OBJS))) \
+	  $(COBJS) -o $@
+
 $(OUT)/mc2: $(OUT)/mc2.stamp $(COBJS) $(OUT)/om.o
 	$(REFFC) $(FFLAGS) -c $(REF)/mc2.F90 -o $(OUT)/mc2.o
 	$(REFFC) $(FFLAGS) $(OUT)/mc2.o $(OBJFILES) $(addprefix $(OUT)/,$(addsuffix .o,$(MC2OBJS))) \
