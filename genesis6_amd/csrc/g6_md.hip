@@ -136,8 +136,11 @@ __device__ inline void store_qc(const MdBuf &b, int n, int i, const double (&S)[
 struct BoxNow {
   double L, invL, cellinv;
 };
+// NPT: the barostat is on (kernels are instantiated both ways, so the fixed
+// box costs nothing)
+template <bool NPT>
 __device__ inline BoxNow box_now(const Geom &g, const MdBuf &b) {
-  if (b.box) return BoxNow{b.box[0], b.box[1], b.box[2]};
+  if (NPT) return BoxNow{b.box[0], b.box[1], b.box[2]};
   return BoxNow{g.L, g.invL, g.cellinv};
 }
 
@@ -152,6 +155,7 @@ __device__ inline BoxNow box_now(const Geom &g, const MdBuf &b) {
 // split changes no bit of the result.  The roles are wave-uniform, so no
 // wave executes another role's code.
 // blk is the block's index within its system (blockIdx.x for one system).
+template <bool NPT>
 __device__ __forceinline__ void md_pre_body(const ModelTable &mt, const Geom &g, const MdBuf &b,
                                             double dt, int nose_active, int do_correct,
                                             int do_predict, int do_sites, int parity, int blk) {
@@ -172,9 +176,9 @@ __device__ __forceinline__ void md_pre_body(const ModelTable &mt, const Geom &g,
   const double zeta0 = nose_active ? b.nose[3] : 0.0;
   // translational drag: Nose, plus the Andersen dilation v1 / (2*3*v0)
   // (RigidModule.F90:1275-1305, orthorhombic)
-  const double acoeff = b.anders ? b.anders[3] / (2.0 * 3.0 * b.anders[2]) : 0.0;
+  const double acoeff = NPT ? b.anders[3] / (2.0 * 3.0 * b.anders[2]) : 0.0;
   const double nc = zeta0 * dt, ext = nc * 0.5 + acoeff, exr = nc;
-  const BoxNow bx = box_now(g, b);
+  const BoxNow bx = box_now<NPT>(g, b);
   // every global load is issued here, ahead of the first barrier
   double F[3] = {0, 0, 0}, t[9], tl[3] = {0, 0, 0}, cref[3] = {0, 0, 0};
 #pragma unroll
@@ -438,7 +442,7 @@ __device__ __forceinline__ void md_pre_body(const ModelTable &mt, const Geom &g,
         // a pair of the rows moves by at most twice the largest displacement
         // plus sqrt(3)|L - L_ref| (image shifts of one box per axis)
         double h2 = b.half_skin2;
-        if (b.box) {
+        if (NPT) {
           const double h = sqrt(b.half_skin2) - 0.8660254037844386 * fabs(bx.L - b.box[3]);
           h2 = h > 0.0 ? h * h : -1.0;
         }
@@ -447,7 +451,7 @@ __device__ __forceinline__ void md_pre_body(const ModelTable &mt, const Geom &g,
       } else if (i == 0) {
         b.flags[2 + parity] = 1;  // no skin: the reference rebuilds every step
       }
-      if (b.box && i == 0 && bx.L < b.box[5]) b.flags[0] = 5;  // cells below (rc+skin)/2
+      if (NPT && i == 0 && bx.L < b.box[5]) b.flags[0] = 5;  // cells below (rc+skin)/2
     } else if (role >= 2) {
       // rigid_setmatrix and the site offsets; the two waves split the stores
       const double qa = S[0], qb = S[1], qc = S[2], qd = S[3];
@@ -504,10 +508,12 @@ __device__ __forceinline__ void md_pre_body(const ModelTable &mt, const Geom &g,
   }
 }
 
+template <bool NPT>
 __global__ __launch_bounds__(256) void k_md_pre(ModelTable mt, Geom g, MdBuf b, double dt,
                                                 int nose_active, int do_correct, int do_predict,
                                                 int do_sites, int parity) {
-  md_pre_body(mt, g, b, dt, nose_active, do_correct, do_predict, do_sites, parity, blockIdx.x);
+  md_pre_body<NPT>(mt, g, b, dt, nose_active, do_correct, do_predict, do_sites, parity,
+                   blockIdx.x);
 }
 
 // Ladder form: one launch over the blocks of every system of a ladder;
@@ -520,7 +526,8 @@ __global__ __launch_bounds__(256) void k_md_pre_lad(ModelTable mt, const MdRep *
   if (m.x < 0) return;
   const Geom g = reps[m.x].g;
   const MdBuf b = reps[m.x].b;
-  md_pre_body(mt, g, b, reps[m.x].dt, reps[m.x].nose_active, do_correct, do_predict, do_sites,
+  md_pre_body<false>(mt, g, b, reps[m.x].dt, reps[m.x].nose_active, do_correct, do_predict,
+                     do_sites,
               parity, m.y);
 }
 
@@ -542,6 +549,7 @@ __device__ inline void andersen_rescale(const double *a, double *box) {
 // (8: both parities); 16 force evaluation: the pair partials alone go to the
 // thermo record.  Runs as its own kernel (k_md_close) or at the start of
 // block 0 of k_md_main, which neither reads nor writes what it touches.
+template <bool NPT>
 __device__ inline void close_body(const ModelTable &mt, const MdBuf &b, int n, double dt,
                                   int nblk_pre, const double *bpart, int nblk_main, int ncell,
                                   int nose_active, int mode, int parity, double box_l,
@@ -640,13 +648,13 @@ __device__ inline void close_body(const ModelTable &mt, const MdBuf &b, int n, d
                             r[18] * mt.mass * kd, r[20] * mt.mass * kd, r[21] * mt.mass * kd};
       for (int q = 0; q < 9; q++) th[16 + q] = kt[q];
       const double vp = (kt[0] + th[4]) + (kt[4] + th[8]) + (kt[8] + th[12]);
-      const double lb = b.box ? b.box[0] : box_l;
+      const double lb = NPT ? b.box[0] : box_l;
       const double vol = (lb * lb) * lb;
       const double press =
           (vp / 3.0) * 10.0 * (1.0 / 101326.0) / (vol * 6.0225e23 * (1e-10 * 1e-10 * 1e-10));
       th[25] = press;
       double and_ep = 0.0, and_ek = 0.0, volc = vol;
-      if (b.anders) {
+      if (NPT) {
         // Andersen_Correct (AndersenModule.F90:95-110) with this pressure
         double *a = b.anders;
         const double volp2 = (press - a[0]) / a[1] * 0.5;
@@ -678,7 +686,7 @@ __device__ inline void close_body(const ModelTable &mt, const MdBuf &b, int n, d
       z[5] = z[5] + 3.0 * z[6] + 6.0 * z[7];
       z[6] = z[6] + 4.0 * z[7];
     }
-    if ((mode & 2) && b.anders) {
+    if (NPT && (mode & 2)) {
       // Andersen_Predict (AndersenModule.F90:84-93)
       double *a = b.anders;
       a[2] = a[2] + a[3] + a[4] + a[5] + a[6] + a[7];
@@ -691,13 +699,14 @@ __device__ inline void close_body(const ModelTable &mt, const MdBuf &b, int n, d
   }
 }
 
+template <bool NPT>
 __global__ __launch_bounds__(256) void k_md_close(ModelTable mt, MdBuf b, int n, double dt,
                                                   int nblk_pre, const double *bpart,
                                                   int nblk_main, int ncell, int nose_active,
                                                   int mode, int parity, double box_l) {
   __shared__ double s_red[4][kRed];
-  close_body(mt, b, n, dt, nblk_pre, bpart, nblk_main, ncell, nose_active, mode, parity, box_l,
-             s_red);
+  close_body<NPT>(mt, b, n, dt, nblk_pre, bpart, nblk_main, ncell, nose_active, mode, parity,
+                  box_l, s_red);
 }
 
 // Ladder form: block r closes system r.
@@ -707,7 +716,7 @@ __global__ __launch_bounds__(256) void k_md_close_lad(ModelTable mt, const MdRep
   const MdRep &r = reps[blockIdx.x];
   const MdBuf b = r.b;
   const int n = r.g.n, nm = r.nblk_main;
-  close_body(mt, b, n, r.dt, (n + 63) / 64 * 4, b.blockpart + (size_t)src_parity * nm * kBP, nm,
+  close_body<false>(mt, b, n, r.dt, (n + 63) / 64 * 4, b.blockpart + (size_t)src_parity * nm * kBP, nm,
              r.g.ncell, r.nose_active, mode, parity, r.g.L, s_red);
 }
 
@@ -746,6 +755,7 @@ __device__ inline double frac_in_cell(const BoxNow &bx, double x) {
 // neighbour-row build of molecule i into nbr[i*maxnb ...], sorted by j so
 // that the force sums do not depend on the order in which the binning
 // atomics filled the cells; returns the row length (also stored).
+template <bool NPT>
 __device__ inline int build_row(const Geom &g, const BoxNow &bx, const MdBuf &b, int parity,
                                 int i, int lane, int (&s_beg)[128], int (&s_pre)[132], int *s_j,
                                 int *s_c) {
@@ -846,7 +856,7 @@ __device__ inline int build_row(const Geom &g, const BoxNow &bx, const MdBuf &b,
     b.com_ref[i] = xi;
     b.com_ref[n + i] = yi;
     b.com_ref[2 * n + i] = zi;
-    if (b.box && i == 0) b.box[3] = bx.L;  // the box the rows were built in
+    if (NPT && i == 0) b.box[3] = bx.L;  // the box the rows were built in
   }
   return m;
 }
@@ -905,7 +915,7 @@ __device__ inline double rowsum16(double v) {
 
 // blk is the block's index within its system and nblk the system's block
 // count (blockIdx.x and gridDim.x for one system).
-template <int M, typename R>
+template <int M, typename R, bool NPT>
 __device__ __forceinline__ void md_main_body(const ModelTable &mt, const Geom &g, const MdBuf &b,
                                              const PairK<R> &pk, double *out, int parity,
                                              const CloseArgs &ca, int reg_bytes, int blk,
@@ -918,7 +928,7 @@ __device__ __forceinline__ void md_main_body(const ModelTable &mt, const Geom &g
   const int n = g.n;
   const int i = blk * 4 + wid;
   const int cap = b.maxnb;
-  const BoxNow bx = box_now(g, b);
+  const BoxNow bx = box_now<NPT>(g, b);
   unsigned char *reg = s_dyn + (size_t)wid * reg_bytes;
   double ep = 0, v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   int npair = 0, nlist = 0;  // wave-uniform counts
@@ -932,7 +942,7 @@ __device__ __forceinline__ void md_main_body(const ModelTable &mt, const Geom &g
     int m;
     if (b.flags[2 + parity]) {
       int *sj = reinterpret_cast<int *>(reg);
-      m = build_row(g, bx, b, parity, i, lane, s_beg[wid], s_pre[wid], sj, sj + cap);
+      m = build_row<NPT>(g, bx, b, parity, i, lane, s_beg[wid], s_pre[wid], sj, sj + cap);
       nlist = m;
     } else {
       m = b.nbr_cnt[i];
@@ -1094,7 +1104,7 @@ __device__ __forceinline__ void md_main_body(const ModelTable &mt, const Geom &g
 template <int M, typename R>
 constexpr int main_waves_per_eu() { return (M != G6_ST2 && sizeof(R) == 4) ? 4 : 1; }
 
-template <int M, typename R>
+template <int M, typename R, bool NPT>
 __global__ __launch_bounds__(256, (main_waves_per_eu<M, R>())) void k_md_main(ModelTable mt, Geom g, MdBuf b, PairK<R> pk,
                                                  double *out, int parity, CloseArgs ca,
                                                  int reg_bytes) {
@@ -1102,13 +1112,14 @@ __global__ __launch_bounds__(256, (main_waves_per_eu<M, R>())) void k_md_main(Mo
     // block 0 closes the previous step while the other blocks run the rows
     if (blockIdx.x == 0) {
       __shared__ double s_close[4][kRed];
-      close_body(mt, b, g.n, ca.dt, ca.nblk_pre, ca.bpart, ca.nblk_main, g.ncell, ca.nose_active,
-                 ca.mode, parity, g.L, s_close);
+      close_body<false>(mt, b, g.n, ca.dt, ca.nblk_pre, ca.bpart, ca.nblk_main, g.ncell,
+                        ca.nose_active, ca.mode, parity, g.L, s_close);
       return;
     }
-    md_main_body<M, R>(mt, g, b, pk, out, parity, ca, reg_bytes, blockIdx.x - 1, gridDim.x - 1);
+    md_main_body<M, R, NPT>(mt, g, b, pk, out, parity, ca, reg_bytes, blockIdx.x - 1,
+                            gridDim.x - 1);
   } else {
-    md_main_body<M, R>(mt, g, b, pk, out, parity, ca, reg_bytes, blockIdx.x, gridDim.x);
+    md_main_body<M, R, NPT>(mt, g, b, pk, out, parity, ca, reg_bytes, blockIdx.x, gridDim.x);
   }
 }
 
@@ -1133,18 +1144,23 @@ __global__ __launch_bounds__(256, (main_waves_per_eu<M, R>())) void k_md_main_la
   if (m.y < 0) {
     if (close_mode) {
       __shared__ double s_close[4][kRed];
-      close_body(mt, b, g.n, ca.dt, ca.nblk_pre, ca.bpart, ca.nblk_main, g.ncell, ca.nose_active,
-                 ca.mode, parity, g.L, s_close);
+      close_body<false>(mt, b, g.n, ca.dt, ca.nblk_pre, ca.bpart, ca.nblk_main, g.ncell,
+                        ca.nose_active, ca.mode, parity, g.L, s_close);
     }
     return;
   }
-  md_main_body<M, R>(mt, g, b, pk, nullptr, parity, ca, reg_bytes, m.y, nm);
+  md_main_body<M, R, false>(mt, g, b, pk, nullptr, parity, ca, reg_bytes, m.y, nm);
 }
 
 void md_launch_pre(System &s, bool correct, bool predict, bool sites, int parity) {
-  k_md_pre<<<pre_waves(s.n) / 4, 256, 0, s.stream>>>(s.mt, s.geo_md, s.md, s.dt,
-                                                  s.nose_active ? 1 : 0, correct ? 1 : 0,
-                                                  predict ? 1 : 0, sites ? 1 : 0, parity);
+  if (s.andersen)
+    k_md_pre<true><<<pre_waves(s.n) / 4, 256, 0, s.stream>>>(
+        s.mt, s.geo_md, s.md, s.dt, s.nose_active ? 1 : 0, correct ? 1 : 0, predict ? 1 : 0,
+        sites ? 1 : 0, parity);
+  else
+    k_md_pre<false><<<pre_waves(s.n) / 4, 256, 0, s.stream>>>(
+        s.mt, s.geo_md, s.md, s.dt, s.nose_active ? 1 : 0, correct ? 1 : 0, predict ? 1 : 0,
+        sites ? 1 : 0, parity);
 }
 
 static CloseArgs close_args(System &s, int mode, int src_parity) {
@@ -1160,8 +1176,14 @@ static CloseArgs close_args(System &s, int mode, int src_parity) {
 
 void md_launch_close(System &s, int mode, int parity, int src_parity) {
   const CloseArgs ca = close_args(s, mode, src_parity);
-  k_md_close<<<1, 256, 0, s.stream>>>(s.mt, s.md, s.n, s.dt, ca.nblk_pre, ca.bpart, ca.nblk_main,
-                                      s.geo_md.ncell, ca.nose_active, mode, parity, s.L);
+  if (s.andersen)
+    k_md_close<true><<<1, 256, 0, s.stream>>>(s.mt, s.md, s.n, s.dt, ca.nblk_pre, ca.bpart,
+                                              ca.nblk_main, s.geo_md.ncell, ca.nose_active, mode,
+                                              parity, s.L);
+  else
+    k_md_close<false><<<1, 256, 0, s.stream>>>(s.mt, s.md, s.n, s.dt, ca.nblk_pre, ca.bpart,
+                                               ca.nblk_main, s.geo_md.ncell, ca.nose_active,
+                                               mode, parity, s.L);
 }
 
 template <typename R>
@@ -1174,8 +1196,15 @@ const PairK<double> &pair_k<double>(const System &s) { return s.pkd; }
 template <int M, typename R>
 static void md_main_t(System &s, double *out, int parity, const CloseArgs &ca) {
   const int rb = main_region_bytes(s.md.maxnb, (int)sizeof(R));
-  k_md_main<M, R><<<s.nblk_md + (ca.mode ? 1 : 0), 256, 4 * (size_t)rb, s.stream>>>(s.mt, s.geo_md, s.md,
-                                                               pair_k<R>(s), out, parity, ca, rb);
+  const unsigned grid = s.nblk_md + (ca.mode ? 1 : 0);
+  if (s.andersen)
+    k_md_main<M, R, true><<<grid, 256, 4 * (size_t)rb, s.stream>>>(s.mt, s.geo_md, s.md,
+                                                                   pair_k<R>(s), out, parity, ca,
+                                                                   rb);
+  else
+    k_md_main<M, R, false><<<grid, 256, 4 * (size_t)rb, s.stream>>>(s.mt, s.geo_md, s.md,
+                                                                    pair_k<R>(s), out, parity, ca,
+                                                                    rb);
 }
 
 void md_launch_main(System &s, double *out, int parity, int close_mode) {
