@@ -596,6 +596,42 @@ int g6_step_timed(g6_handle h, int nsteps, double *t) {
   return check_flags(*s);
 }
 
+int g6_time_force(g6_handle h, int nlaunch, double *ms) {
+  System *s = get(h);
+  if (!s || nlaunch < 1 || nlaunch > 10000 || !ms || s->andersen) return G6_EARG;
+  // the force kernel alone, nlaunch times back to back in one graph, on the
+  // state a g6_step call leaves: its records and rows are consistent, the
+  // rebuild flags clear; each launch writes the same forces and partials
+  // (eager launches when G6_GRAPH_STEPS=0, as for kernel tracing)
+  hipGraph_t gr;
+  hipGraphExec_t ge = nullptr;
+  const bool graph = graph_steps() > 0;
+  G6_CHECK(hipStreamSynchronize(s->stream));
+  if (graph) {
+    G6_CHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < nlaunch; k++) md_launch_main(*s, nullptr, 0, 0);
+    G6_CHECK(hipStreamEndCapture(s->stream, &gr));
+    G6_CHECK(hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0));
+    G6_CHECK(hipGraphDestroy(gr));
+    G6_CHECK(hipGraphLaunch(ge, s->stream));  // warm
+  } else {
+    md_launch_main(*s, nullptr, 0, 0);
+  }
+  G6_CHECK(hipEventRecord(s->ev[4], s->stream));
+  if (graph) {
+    G6_CHECK(hipGraphLaunch(ge, s->stream));
+  } else {
+    for (int k = 0; k < nlaunch; k++) md_launch_main(*s, nullptr, 0, 0);
+  }
+  G6_CHECK(hipEventRecord(s->ev[5], s->stream));
+  G6_CHECK(hipEventSynchronize(s->ev[5]));
+  float e = 0;
+  G6_CHECK(hipEventElapsedTime(&e, s->ev[4], s->ev[5]));
+  if (ge) G6_CHECK(hipGraphExecDestroy(ge));
+  *ms = e / nlaunch;
+  return check_flags(*s);
+}
+
 int g6_eval_forces(g6_handle h, double *force, double *torque, double *thermo) {
   System *s = get(h);
   if (!s || !force || !torque) return G6_EARG;

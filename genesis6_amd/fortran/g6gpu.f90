@@ -240,6 +240,13 @@ module g6gpu
        real(c_double), intent(out) :: t(5)
      end function g6_step_timed
 
+     integer(c_int) function g6_time_force(h, nlaunch, ms) bind(C, name="g6_time_force")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), value :: h
+       integer(c_int), value :: nlaunch
+       real(c_double), intent(out) :: ms
+     end function g6_time_force
+
      integer(c_int) function g6_mc_begin(h, temp, seed, delta, rtratio) bind(C, name="g6_mc_begin")
        import :: c_int, c_int64_t, c_int32_t, c_double
        integer(c_int64_t), value :: h

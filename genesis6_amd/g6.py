@@ -64,6 +64,7 @@ def lib():
                                  C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.g6_triangle.argtypes = [_H, C.c_int, C.c_int64, _ip, _ip, _dp, _dp, _dp, _dp, _dp,
                                   C.c_int, _dp, C.POINTER(C.c_double)]
+        L.g6_time_force.argtypes = [_H, C.c_int, C.POINTER(C.c_double)]
         L.g6_set_andersen.argtypes = [_H, C.c_int, C.c_double, C.c_double, C.c_void_p]
         L.g6_get_andersen.argtypes = [_H, _dp]
         L.g6_get_box.argtypes = [_H, C.POINTER(C.c_double)]
@@ -211,6 +212,12 @@ class System:
         L = C.c_double(0)
         check(lib().g6_get_box(self.h, C.byref(L)), "g6_get_box")
         return L.value
+
+    def time_force(self, nlaunch=200):
+        """Mean ms per launch of the force kernel alone (g6_time_force)."""
+        ms = C.c_double(0)
+        check(lib().g6_time_force(self.h, nlaunch, C.byref(ms)), "g6_time_force")
+        return ms.value
 
     def step_timed(self, nsteps):
         t = np.zeros(5)

@@ -369,6 +369,11 @@ int g6_lattice_start(int n, double box_l, uint32_t seed, double *com,
  * whole step; t[4] = mean interacting pairs per step.  Same results as
  * g6_step. */
 int g6_step_timed(g6_handle h, int nsteps, double *t);
+/* The force kernel alone: nlaunch launches back to back in one hipGraph on
+ * the state a g6_step call leaves (same forces each time), timed with HIP
+ * events on the library stream; *ms = mean milliseconds per launch.  The
+ * state is not moved.  Not with the barostat. */
+int g6_time_force(g6_handle h, int nlaunch, double *ms);
 int g6_model_nsite(int model);
 
 #ifdef __cplusplus
