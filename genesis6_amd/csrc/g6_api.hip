@@ -142,6 +142,48 @@ bool make_model_table(int model, int prec, ModelTable *m) {
     set_inverses(m);
     return true;
   }
+  if (model == G6_SPCE) {
+    // Rigid_SPCE_Constructor (SPCEModule.F90:48-80): O, H, H, CoM
+    const double theta = 109.5 * kPI / 180.0, l1 = 1.0, hmass = 1.0, omass = 16.0;
+    m->nsite = 4;
+    m->by[1] = l1 * sin(0.5 * theta);
+    m->bz[1] = l1 * cos(0.5 * theta);
+    m->by[2] = -l1 * sin(0.5 * theta);
+    m->bz[2] = l1 * cos(0.5 * theta);
+    const double zshift = m->bz[1] * 2.0 * hmass / (2.0 * hmass + omass);
+    for (int i = 0; i < 3; i++) m->bz[i] = m->bz[i] - zshift;
+    m->Ixx = omass * (m->by[0] * m->by[0] + m->bz[0] * m->bz[0]) +
+             hmass * (m->by[1] * m->by[1] + m->bz[1] * m->bz[1]) * 2.0;
+    m->Iyy = omass * (m->bz[0] * m->bz[0] + m->bx[0] * m->bx[0]) +
+             hmass * (m->bz[1] * m->bz[1] + m->bx[1] * m->bx[1]) * 2.0;
+    m->Izz = omass * (m->bx[0] * m->bx[0] + m->by[0] * m->by[0]) +
+             hmass * (m->bx[1] * m->bx[1] + m->by[1] * m->by[1]) * 2.0;
+    m->mass = 2.0 * hmass + omass;
+    // spce_setinteraction (SPCEModule.F90:25-46): the O site carries the
+    // charge and the LJ term; eps from K by J2K (PhysConst.F90:56)
+    const double q[3] = {-0.8476, 0.4238, 0.4238};
+    const double eps = 78.208 / 0.120273 * 1e-3, sig = 3.166;
+    m->std_q[0] = q[0];
+    m->std_q[1] = m->std_q[2] = q[1];
+    m->std_eps[0] = eps;
+    m->std_sig[0] = sig;
+    // StdForce coefficients (StdInteraction.F90:206-221): 9 Coulomb terms
+    // over O, H, H and the O-O LJ term
+    const double KX2X = 1.0 / 1e-3, J2I = 1.0 / 10.0;
+    m->ncoul = 9;
+    for (int t = 0; t < 9; t++) {
+      const int a = t / 3, b = t % 3;
+      m->ci[t] = a;
+      m->cj[t] = b;
+      m->qq[t] = q[a] * q[b] * coeff() * KX2X * J2I * 1.0;
+    }
+    const double ljeps = sqrt(eps * eps);
+    const double s6 = ipow((sig + sig) * 0.5, 6);
+    m->lj_a = 4.0 * ljeps * s6 * s6 * KX2X * J2I * 1.0;
+    m->lj_b = -4.0 * ljeps * s6 * KX2X * J2I * 1.0;
+    set_inverses(m);
+    return true;
+  }
   return false;
 }
 

@@ -396,9 +396,9 @@ int g6_mc_begin(g6_handle h, double temp, uint32_t seed, const double *delta,
   G6_CHECK(hipMemcpy(m.rtr, rtratio, sizeof(double) * n, hipMemcpyHostToDevice));
   G6_CHECK(hipMemset(m.nacc, 0, sizeof(long long)));
   make_std_table(s->mt, &s->std);
-  if (s->model == G6_TIP4P ? !std_table_matches<G6_TIP4P>(s->std)
-                           : !std_table_matches<G6_TIP5P>(s->std))
-    return G6_EARG;
+  bool match = false;
+  std_model(s->model, [&](auto m) { match = std_table_matches<decltype(m)::value>(s->std); });
+  if (!match) return G6_EARG;
   s->mc_beta = 1.0 / temp;  // @TEMP: beta = 1/temp (mc2util.F90)
   McHost *hst = &s->mch;
   hst->seed(seed);
@@ -448,15 +448,14 @@ int g6_mc_trials(g6_handle h, int ntrial, long long *naccept) {
   G6_CHECK(hipMemcpy(m.z, z.data(), sizeof(double) * need, hipMemcpyHostToDevice));
   McArgs a = mc_args(*s);
   const size_t lds = mc_lds(s->n);
-  if (s->model == G6_TIP4P) {
-    G6_CHECK(hipFuncSetAttribute((const void *)k_mc_trials<G6_TIP4P>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    k_mc_trials<G6_TIP4P><<<1, kMcThreads, lds, s->stream>>>(a, s->std, ntrial);
-  } else {
-    G6_CHECK(hipFuncSetAttribute((const void *)k_mc_trials<G6_TIP5P>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    k_mc_trials<G6_TIP5P><<<1, kMcThreads, lds, s->stream>>>(a, s->std, ntrial);
-  }
+  hipError_t attr = hipSuccess;
+  std_model(s->model, [&](auto m) {
+    constexpr int M = decltype(m)::value;
+    attr = hipFuncSetAttribute((const void *)k_mc_trials<M>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (attr == hipSuccess) k_mc_trials<M><<<1, kMcThreads, lds, s->stream>>>(a, s->std, ntrial);
+  });
+  G6_CHECK(attr);
   G6_CHECK(hipGetLastError());
   int used = 0;
   long long nacc = 0;

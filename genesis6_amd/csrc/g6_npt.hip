@@ -1060,13 +1060,11 @@ static int npt_run(System &s, Npt &p, int count, Mlp *net, int K) {
   const int G = env && atoi(env) <= 0 ? 0 : (count < 200 ? count : 200);
   auto body = [&](int m) {
     for (int q = 0; q < m; q++) {
-      if (!da) {
-        if (s.model == G6_TIP4P) launch_move<G6_TIP4P>(s, p, 0);
-        else launch_move<G6_TIP5P>(s, p, 0);
-      } else {
-        if (s.model == G6_TIP4P) launch_round<G6_TIP4P>(s, p, *net);
-        else launch_round<G6_TIP5P>(s, p, *net);
-      }
+      std_model(s.model, [&](auto mm) {
+        constexpr int M = decltype(mm)::value;
+        if (!da) launch_move<M>(s, p, 0);
+        else launch_round<M>(s, p, *net);
+      });
     }
   };
   if (G > 0) {
@@ -1197,9 +1195,9 @@ int g6_npt_begin(g6_handle h, double temp, double press_bar, uint64_t seed, doub
     a.K = 1;
   }
   make_std_table(s->mt, &p.st);
-  if (s->model == G6_TIP4P ? !std_table_matches<G6_TIP4P>(p.st)
-                           : !std_table_matches<G6_TIP5P>(p.st))
-    return G6_EARG;
+  bool match = false;
+  std_model(s->model, [&](auto m) { match = std_table_matches<decltype(m)::value>(p.st); });
+  if (!match) return G6_EARG;
   G6_CHECK(hipMemcpy(a.com, s->md.com, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice));
   G6_CHECK(hipMemcpy(a.quat, s->md.quat, sizeof(double) * 4 * n, hipMemcpyDeviceToDevice));
   double st0[8] = {s->L, 0, 0, 0, 0, 0, 0, 0};
@@ -1210,8 +1208,9 @@ int g6_npt_begin(g6_handle h, double temp, double press_bar, uint64_t seed, doub
   G6_CHECK(hipMemcpy(a.cnt, c0, sizeof(c0), hipMemcpyHostToDevice));
   G6_CHECK(hipMemset(a.derr, 0, sizeof(int)));
   k_npt_sites<<<(s->n + 255) / 256, 256, 0, s->stream>>>(a);
-  if (s->model == G6_TIP4P) k_npt_pairinfo<G6_TIP4P><<<(s->n + 3) / 4, 256, 0, s->stream>>>(a, p.st, a.pinfo);
-  else k_npt_pairinfo<G6_TIP5P><<<(s->n + 3) / 4, 256, 0, s->stream>>>(a, p.st, a.pinfo);
+  std_model(s->model, [&](auto m) {
+    k_npt_pairinfo<decltype(m)::value><<<(s->n + 3) / 4, 256, 0, s->stream>>>(a, p.st, a.pinfo);
+  });
   k_npt_total<<<1, 256, 0, s->stream>>>(a, a.st + 1);
   G6_CHECK(hipGetLastError());
   G6_CHECK(hipStreamSynchronize(s->stream));
@@ -1265,8 +1264,9 @@ int g6_npt_energy(g6_handle h, double *energy) {
   double *tmp = nullptr;
   G6_CHECK(hipMalloc(&tmp, sizeof(double)));
   // a fresh PairInfo of the current configuration (into the proposal matrix)
-  if (s->model == G6_TIP4P) k_npt_pairinfo<G6_TIP4P><<<(s->n + 3) / 4, 256, 0, s->stream>>>(a, p->st, a.pinfo2);
-  else k_npt_pairinfo<G6_TIP5P><<<(s->n + 3) / 4, 256, 0, s->stream>>>(a, p->st, a.pinfo2);
+  std_model(s->model, [&](auto m) {
+    k_npt_pairinfo<decltype(m)::value><<<(s->n + 3) / 4, 256, 0, s->stream>>>(a, p->st, a.pinfo2);
+  });
   k_npt_total<<<1, 256, 0, s->stream>>>(a, tmp);
   G6_CHECK(hipGetLastError());
   G6_CHECK(hipMemcpyAsync(energy, tmp, sizeof(double), hipMemcpyDeviceToHost, s->stream));

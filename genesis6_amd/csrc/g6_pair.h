@@ -7,6 +7,7 @@
 //   TIP5P  StdForce, StdInteraction.F90:201-576 (16 Coulomb + O-O LJ)
 //   ST2    force_st2, ST2Module.F90:187-738 (16 Coulomb scaled by S*s12 with
 //          the Stillinger-Rahman switch s12 on the COM distance, LJ by S)
+//   SPC/E  StdForce (9 Coulomb over O, H, H + O-O LJ)
 // Cutoff ratios hh = S(r), dh = -S'(r)/r on the COM distance are inputs
 // (cutoff_func, CutoffModule.F90:38-59; cutoff_sw below).
 // Inputs are the COM separation d00 (shift already removed) and the site
@@ -45,6 +46,13 @@ struct Traits<G6_ST2> {
   static constexpr bool ST2SW = true;
   __device__ static constexpr int ci(int t) { return 1 + t / 4; }
   __device__ static constexpr int cj(int t) { return 1 + t % 4; }
+};
+template <>
+struct Traits<G6_SPCE> {  // the O site carries both the charge and the LJ term
+  static constexpr int NS = 4, COM = 3, NC = 9;
+  static constexpr bool ST2SW = false;
+  __device__ static constexpr int ci(int t) { return t / 3; }
+  __device__ static constexpr int cj(int t) { return t % 3; }
 };
 
 __device__ inline float g6_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
@@ -164,9 +172,10 @@ __device__ inline R pair_sites(const ModelTable &mt, R hh, R dh, R d0, R d1,
 // Force and torque on molecule i only (the MD kernel evaluates each pair
 // from both sides) from the pair constants k: the same terms as pair_sites,
 // with the switch factors applied to the summed site forces and torques
-// instead of to each site force.  Site 0 carries the LJ term only and the
-// sites after it up to the CoM site the Coulomb terms only, in all three
-// models.  dd = |d|^2 of the COM separation d.  Returns ep1.
+// instead of to each site force.  Site 0 carries the LJ term; the Coulomb
+// terms sit on the sites from T::ci(0) up to the CoM site (from site 1 in
+// TIP4P, TIP5P and ST2, from the O site in SPC/E).  dd = |d|^2 of the COM
+// separation d.  Returns ep1.
 template <int M, typename R>
 __device__ inline R pair_on_i(const PairK<R> &k, R dd, R d0, R d1, R d2,
                               const R (&ri)[Traits<M>::NS][3],
@@ -211,7 +220,7 @@ __device__ inline R pair_on_i(const PairK<R> &k, R dd, R d0, R d1, R d2,
     epc += e0;
     fs[a][0] += dx * f0; fs[a][1] += dy * f0; fs[a][2] += dz * f0;
   }
-  R eplj;
+  R eplj, fl[3];
   {
     const R dx = pi[0][0] - rj[0][0];
     const R dy = pi[0][1] - rj[0][1];
@@ -221,20 +230,20 @@ __device__ inline R pair_on_i(const PairK<R> &k, R dd, R d0, R d1, R d2,
     const R d6 = inv2 * inv2 * inv2, d12 = d6 * d6;
     eplj = k.lja * d12 + k.ljb * d6;
     const R f0 = inv2 * (k.lja12 * d12 + k.ljb6 * d6);
-    fs[0][0] = dx * f0; fs[0][1] = dy * f0; fs[0][2] = dz * f0;
+    fl[0] = dx * f0; fl[1] = dy * f0; fl[2] = dz * f0;
   }
-  // Coulomb sums (sites 1..NQ-1) and the LJ site, unscaled
+  // Coulomb sums over the charged sites and the LJ site, unscaled
   R fc[3] = {R(0), R(0), R(0)}, tc[3] = {R(0), R(0), R(0)};
 #pragma unroll
-  for (int s = 1; s < NQ; s++) {
+  for (int s = T::ci(0); s < NQ; s++) {
     fc[0] += fs[s][0]; fc[1] += fs[s][1]; fc[2] += fs[s][2];
     tc[0] += ri[s][1] * fs[s][2] - ri[s][2] * fs[s][1];
     tc[1] += ri[s][2] * fs[s][0] - ri[s][0] * fs[s][2];
     tc[2] += ri[s][0] * fs[s][1] - ri[s][1] * fs[s][0];
   }
-  const R tl[3] = {ri[0][1] * fs[0][2] - ri[0][2] * fs[0][1],
-                   ri[0][2] * fs[0][0] - ri[0][0] * fs[0][2],
-                   ri[0][0] * fs[0][1] - ri[0][1] * fs[0][0]};
+  const R tl[3] = {ri[0][1] * fl[2] - ri[0][2] * fl[1],
+                   ri[0][2] * fl[0] - ri[0][0] * fl[2],
+                   ri[0][0] * fl[1] - ri[0][1] * fl[0]};
   R hc = hh, cc, ep1;
   if constexpr (T::ST2SW) {
     R s12 = R(1), ds12 = R(0);
@@ -254,9 +263,9 @@ __device__ inline R pair_on_i(const PairK<R> &k, R dd, R d0, R d1, R d2,
     cc = dh * (epc + eplj);
     ep1 = (epc + eplj) * hh;
   }
-  F[0] = hc * fc[0] + hh * fs[0][0] + cc * d0;
-  F[1] = hc * fc[1] + hh * fs[0][1] + cc * d1;
-  F[2] = hc * fc[2] + hh * fs[0][2] + cc * d2;
+  F[0] = hc * fc[0] + hh * fl[0] + cc * d0;
+  F[1] = hc * fc[1] + hh * fl[1] + cc * d1;
+  F[2] = hc * fc[2] + hh * fl[2] + cc * d2;
   Tq[0] = hc * tc[0] + hh * tl[0];
   Tq[1] = hc * tc[1] + hh * tl[1];
   Tq[2] = hc * tc[2] + hh * tl[2];

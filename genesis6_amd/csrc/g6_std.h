@@ -4,6 +4,8 @@
 // site placement (RigidModule.F90:62-144).  The argument type A of the
 // templates provides ns, bx/by/bz (body-frame sites) and rc, rin, c1.
 #pragma once
+#include <type_traits>
+
 #include "g6_internal.h"
 
 // FP64 energies in the reference's operation order: no FMA contraction
@@ -12,7 +14,7 @@
 
 namespace g6 {
 
-// The site pairs of StdPairPotential's term list for the two MC models, in
+// The site pairs of StdPairPotential's term list for the MC models, in
 // the order make_std_table produces them (checked at g6_mc_begin), so that
 // the trial kernel indexes sites with constants.
 template <int M>
@@ -20,15 +22,34 @@ struct StdPattern;
 template <>
 struct StdPattern<G6_TIP4P> {  // O-O LJ, then M,H,H x M,H,H Coulomb
   static constexpr int NT = 10, NS = 5;
+  static constexpr bool Q0 = false;  // the O-O term is LJ only
   __device__ __host__ static constexpr int a(int u) { return u == 0 ? 0 : 1 + (u - 1) / 3; }
   __device__ __host__ static constexpr int b(int u) { return u == 0 ? 0 : 1 + (u - 1) % 3; }
 };
 template <>
 struct StdPattern<G6_TIP5P> {  // O-O LJ, then H,H,L,L x H,H,L,L Coulomb
   static constexpr int NT = 17, NS = 6;
+  static constexpr bool Q0 = false;
   __device__ __host__ static constexpr int a(int u) { return u == 0 ? 0 : 1 + (u - 1) / 4; }
   __device__ __host__ static constexpr int b(int u) { return u == 0 ? 0 : 1 + (u - 1) % 4; }
 };
+
+template <>
+struct StdPattern<G6_SPCE> {  // O-O LJ + Coulomb, then the rest of O,H,H x O,H,H Coulomb
+  static constexpr int NT = 9, NS = 4;
+  static constexpr bool Q0 = true;
+  __device__ __host__ static constexpr int a(int u) { return u / 3; }
+  __device__ __host__ static constexpr int b(int u) { return u % 3; }
+};
+
+// Calls f(std::integral_constant<int, M>) for the StdPairPotential model M
+// (the MC paths take TIP4P, TIP5P and SPC/E).
+template <class F>
+inline void std_model(int model, F &&f) {
+  if (model == G6_TIP4P) f(std::integral_constant<int, G6_TIP4P>());
+  else if (model == G6_TIP5P) f(std::integral_constant<int, G6_TIP5P>());
+  else f(std::integral_constant<int, G6_SPCE>());
+}
 
 template <int M>
 static bool std_table_matches(const StdTable &t) {
@@ -101,7 +122,11 @@ __device__ inline double std_energy_t(const StdTable &st, const double *xj, cons
     dz = dz - oz;
     const double dd = 1.0 / (dx * dx + dy * dy + dz * dz);
     double e0;
-    if (u == 0) {  // the O-O term: LJ only in both models
+    if (u == 0 && P::Q0) {  // the O-O term of SPC/E: Coulomb + LJ (kind 1)
+      e0 = s.qq * sqrt(dd);
+      const double d6 = dd * dd * dd, d12 = d6 * d6;
+      e0 = e0 + s.aa * d12 + s.bb * d6;
+    } else if (u == 0) {  // the O-O term of TIP4P, TIP5P: LJ only
       const double d6 = dd * dd * dd, d12 = d6 * d6;
       e0 = s.aa * d12 + s.bb * d6;
     } else {
@@ -132,7 +157,11 @@ __device__ inline double std_energy_rr(const StdTable &st, const double (&xa)[kM
     dz = dz - oz;
     const double dd = 1.0 / (dx * dx + dy * dy + dz * dz);
     double e0;
-    if (u == 0) {
+    if (u == 0 && P::Q0) {  // the O-O term of SPC/E: Coulomb + LJ (kind 1)
+      e0 = s.qq * sqrt(dd);
+      const double d6 = dd * dd * dd, d12 = d6 * d6;
+      e0 = e0 + s.aa * d12 + s.bb * d6;
+    } else if (u == 0) {  // the O-O term of TIP4P, TIP5P: LJ only
       const double d6 = dd * dd * dd, d12 = d6 * d6;
       e0 = s.aa * d12 + s.bb * d6;
     } else {

@@ -86,6 +86,8 @@ program g6md
            model = G6_TIP5P
         case ('ST2', 'ST2_____')
            model = G6_ST2
+        case ('SPC_E')
+           model = G6_SPCE
         case default
            write(0, '(a,a)') 'g6md: @ID08 model not on the GPU path: ', trim(line)
            error stop 2
@@ -217,6 +219,8 @@ contains
              model = G6_TIP5P
           case ('ST2_____')
              model = G6_ST2
+          case ('SPC_E   ')
+             model = G6_SPCE
           case default
              write(0, '(a,a)') 'g6md: @ID08 model not on the GPU path: ', id
              error stop 2
@@ -246,6 +250,8 @@ contains
        id = 'TIP4P   '
     case (G6_TIP5P)
        id = 'TIP5P   '
+    case (G6_SPCE)
+       id = 'SPC_E   '
     case default
        id = 'ST2_____'
     end select

@@ -118,6 +118,31 @@ int g6o_model_get(int model, g6o_model *m) {
     m->sig[0] = 3.100;
     return m->nsite;
   }
+  if (model == G6O_SPCE) {
+    /* Rigid_SPCE_Constructor, SPCEModule.F90:48-80: O, H, H, CoM */
+    const double theta = 109.5 * PI_T / 180.0, l1 = 1.0, hmass = 1.0, omass = 16.0;
+    m->nsite = 4;
+    m->by[1] = l1 * sin(0.5 * theta);
+    m->bz[1] = l1 * cos(0.5 * theta);
+    m->by[2] = -l1 * sin(0.5 * theta);
+    m->bz[2] = l1 * cos(0.5 * theta);
+    double zshift = m->bz[1] * 2.0 * hmass / (2.0 * hmass + omass);
+    for (int i = 0; i < 3; i++) m->bz[i] = m->bz[i] - zshift;
+    m->Ixx = omass * (m->by[0] * m->by[0] + m->bz[0] * m->bz[0]) +
+             hmass * (m->by[1] * m->by[1] + m->bz[1] * m->bz[1]) * 2.0;
+    m->Iyy = omass * (m->bz[0] * m->bz[0] + m->bx[0] * m->bx[0]) +
+             hmass * (m->bz[1] * m->bz[1] + m->bx[1] * m->bx[1]) * 2.0;
+    m->Izz = omass * (m->bx[0] * m->bx[0] + m->by[0] * m->by[0]) +
+             hmass * (m->bx[1] * m->bx[1] + m->by[1] * m->by[1]) * 2.0;
+    m->mass = 2.0 * hmass + omass;
+    /* spce_setinteraction, SPCEModule.F90:25-46: O carries the charge and
+     * the LJ site; eps in K converted by J2K (PhysConst.F90:56) */
+    m->q[0] = -0.8476;
+    m->q[1] = m->q[2] = +0.4238;
+    m->eps[0] = 78.208 / 0.120273 * 1e-3;
+    m->sig[0] = 3.166;
+    return m->nsite;
+  }
   return 0;
 }
 

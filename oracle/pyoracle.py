@@ -12,8 +12,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
-TIP4P, TIP5P, ST2 = 0, 1, 2
-MODEL_IDS = {TIP4P: "TIP4P   ", TIP5P: "TIP5P   ", ST2: "ST2_____"}
+TIP4P, TIP5P, ST2, SPCE = 0, 1, 2, 3
+MODEL_IDS = {TIP4P: "TIP4P   ", TIP5P: "TIP5P   ", ST2: "ST2_____", SPCE: "SPC_E   "}
 
 _lib = None
 

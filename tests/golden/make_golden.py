@@ -23,7 +23,7 @@ from configs import lattice_start, mixture_deck, parse_logd  # noqa: E402
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 REF = os.path.join(ROOT, "oracle", "_ref")
-IDS = {0: "TIP4P   ", 1: "TIP5P   ", 2: "ST2_____"}
+IDS = {0: "TIP4P   ", 1: "TIP5P   ", 2: "ST2_____", 3: "SPC_E   "}
 
 
 def run(cmd, cwd, stdin_text):
@@ -87,10 +87,10 @@ def fortran_records(path):
     return recs
 
 
-def mixture(com, q, L, rc, taper, dt, nsteps, seed, nose=None):
+def mixture(com, q, L, rc, taper, dt, nsteps, seed, nose=None, model="TIP4P"):
     """Runs the reference MD driver; returns @LOGD rows and the final @WTG2."""
     n = len(com)
-    deck = mixture_deck(com, q, L, rc, taper, dt, nsteps, seed, nose)
+    deck = mixture_deck(com, q, L, rc, taper, dt, nsteps, seed, nose, model=model)
     with tempfile.TemporaryDirectory() as d:
         out = run(os.path.join(REF, "Mixture"), d, deck)
         logd = parse_logd(out)

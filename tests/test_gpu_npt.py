@@ -18,9 +18,9 @@ def box(n, rho):
     return (n * 18.015 / (6.0221e23 * rho) * 1e24) ** (1.0 / 3.0)
 
 
-def system(n, rho, seed, steps=0):
+def system(n, rho, seed, steps=0, mid=g6.TIP4P):
     L = box(n, rho)
-    s = g6.System(g6.TIP4P, n, L, RC, TAPER, 0.002, voxa=VOXA, prec=g6.FP64)
+    s = g6.System(mid, n, L, RC, TAPER, 0.002, voxa=VOXA, prec=g6.FP64)
     com, q = lattice_start(n, L, seed)
     s.upload(com, q)
     if steps:
@@ -31,15 +31,15 @@ def system(n, rho, seed, steps=0):
     return s, c, q5[0].copy(), L
 
 
-@pytest.mark.parametrize("pvol", [0.0, 0.3])
-def test_plain_moves_match_oracle(pvol):
+@pytest.mark.parametrize("mid,pvol", [(g6.TIP4P, 0.0), (g6.TIP4P, 0.3), (g6.SPCE, 0.3)])
+def test_plain_moves_match_oracle(mid, pvol):
     n = 128
-    s, com, q, L = system(n, 1.0, 41, steps=40)
-    print("npt n=%d L=%.4f pvol=%g" % (n, L, pvol), flush=True)
+    s, com, q, L = system(n, 1.0, 41, steps=40, mid=mid)
+    print("npt model=%d n=%d L=%.4f pvol=%g" % (mid, n, L, pvol), flush=True)
     args = (250.0, 1.0, 1234567, 0.15, 8.0, 0.005, pvol, VOXA)
     s.npt_begin(*args)
     _, _, L0, U0 = s.npt_state()
-    orc = O.NptRun(O.TIP4P, com, q, L, RC, TAPER, VOXA, 250.0, 1.0, 1234567, 0.15, 8.0,
+    orc = O.NptRun(mid, com, q, L, RC, TAPER, VOXA, 250.0, 1.0, 1234567, 0.15, 8.0,
                    0.005, pvol, energy=U0)
     total = 0
     for rnd in range(3):

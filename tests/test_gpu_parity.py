@@ -17,7 +17,7 @@ from tests.golden.configs import lattice_start
 
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
-CASES = ["tip4p_216", "tip5p_216", "st2_216", "tip4p_125_ndiv3"]
+CASES = ["tip4p_216", "tip5p_216", "st2_216", "tip4p_125_ndiv3", "spce_216"]
 
 
 def relrms(a, b):
@@ -142,27 +142,42 @@ def test_md_kernels_forces_c2_size(skin):
         assert relrms(th[4:13].reshape(3, 3), vr) <= tol
 
 
-@pytest.mark.parametrize("name", ["md_tip4p_216_nve", "md_tip4p_216_nvt"])
-def test_md_trajectory_vs_reference_driver(golden, name):
+# (fixture, model, ensemble): the TIP4P pair of fixtures and the SPC/E one,
+# whose arrays carry an "nve_" / "nvt_" prefix
+TRAJ = [("md_tip4p_216_nve", g6.TIP4P, ""), ("md_tip4p_216_nvt", g6.TIP4P, ""),
+        ("md_spce_216", g6.SPCE, "nve"), ("md_spce_216", g6.SPCE, "nvt")]
+
+
+def traj_case(g, ens):
+    """(prefix, Nose parameters or None) of one trajectory fixture."""
+    if ens:
+        return ens + "_", (tuple(g["nose"]) if ens == "nvt" else None)
+    return "", (tuple(g["nose"]) if g["nose"][1] != 0 else None)
+
+
+@pytest.mark.parametrize("name,mid,ens", TRAJ)
+def test_md_trajectory_vs_reference_driver(golden, name, mid, ens):
     """20 FP64 steps against the reference Mixture run (the same MB start
     from MT19937): temperature and energy per step, final Gear state."""
     g = golden(name)
+    pre, nose = traj_case(g, ens)
     n = len(g["com"])
-    s = g6.System(g6.TIP4P, n, float(g["L"]), float(g["rc"]), float(g["taper"]), float(g["dt"]))
+    print("trajectory", name, ens, flush=True)
+    s = g6.System(mid, n, float(g["L"]), float(g["rc"]), float(g["taper"]), float(g["dt"]))
     s.upload(g["com"], g["quat"])
-    if g["nose"][1] != 0:
-        s.set_nose(*g["nose"])
+    if nose is not None:
+        s.set_nose(*nose)
     s.vreset(300.0, int(g["seed"]))
     for k in range(int(g["nsteps"])):
         s.step(1)
         th = s.thermo()
-        row = g["logd"][k]
+        row = g[pre + "logd"][k]
         assert abs(th[3] - row[1]) <= 1e-10 * row[1], (k, th[3], row[1])
         assert abs(th[0] * 1e-2 - row[4]) <= 1e-9 * abs(row[4]) + 1e-9, (k, th[0] * 1e-2, row[4])
     com, quat, dx, w = s.download()
-    assert np.abs(com.T - g["com_final"]).max() < 1e-9
-    assert np.abs(quat - g["quat_final"].transpose(1, 0, 2)).max() < 1e-9
-    assert np.abs(dx - g["dxyz_final"].transpose(1, 0, 2)).max() < 1e-12
+    assert np.abs(com.T - g[pre + "com_final"]).max() < 1e-9
+    assert np.abs(quat - g[pre + "quat_final"].transpose(1, 0, 2)).max() < 1e-9
+    assert np.abs(dx - g[pre + "dxyz_final"].transpose(1, 0, 2)).max() < 1e-12
 
 
 @pytest.mark.parametrize("prec", [g6.FP64, g6.FP32])
@@ -213,17 +228,18 @@ def test_capacity_error_is_reported():
         s.pairlist(com, cap=10)
 
 
-@pytest.mark.parametrize("name", ["md_tip4p_216_nve", "md_tip4p_216_nvt"])
-def test_fortran_driver_reproduces_the_reference_log(golden, name):
+@pytest.mark.parametrize("name,mid,ens", TRAJ)
+def test_fortran_driver_reproduces_the_reference_log(golden, name, mid, ens):
     """The Fortran driver g6md reads the same Mixture deck as the reference
     and writes @LOGD rows that match the reference's own log: step number,
     temperature, pressure, volume, energies and the conserved quantity."""
     import subprocess
     from tests.golden.configs import mixture_deck, parse_logd
     g = golden(name)
-    nose = tuple(g["nose"]) if g["nose"][1] != 0 else None
+    pre, nose = traj_case(g, ens)
     deck = mixture_deck(g["com"], g["quat"], float(g["L"]), float(g["rc"]), float(g["taper"]),
-                        float(g["dt"]), int(g["nsteps"]), int(g["seed"]), nose)
+                        float(g["dt"]), int(g["nsteps"]), int(g["seed"]), nose,
+                        model="SPC_E" if mid == g6.SPCE else "TIP4P")
     exe = os.path.join(os.path.dirname(GOLD), "..", "genesis6_amd", "lib", "g6md")
     print("g6md deck", name, flush=True)
     with tempfile.TemporaryDirectory() as d:   # g6md writes its restart file fort.13
@@ -231,7 +247,7 @@ def test_fortran_driver_reproduces_the_reference_log(golden, name):
                            text=True, timeout=120, cwd=d)
     assert r.returncode == 0, r.stderr
     got = parse_logd(r.stdout)
-    ref = g["logd"]
+    ref = g[pre + "logd"]
     assert got.shape == ref.shape
     np.testing.assert_array_equal(got[:, 0], ref[:, 0])
     for col, tol in [(1, 1e-10), (2, 1e-6), (3, 1e-14), (4, 1e-9), (5, 1e-10), (8, 1e-8),
@@ -241,7 +257,7 @@ def test_fortran_driver_reproduces_the_reference_log(golden, name):
         assert err <= tol, (col, err)
 
 
-@pytest.mark.parametrize("mid", [0, 1])
+@pytest.mark.parametrize("mid", [0, 1, 3])
 def test_mc_pairinfo_matches_oracle(mid):
     """PairInfo on the GPU equals the oracle's matrix bit for bit."""
     n, L = 216, 18.626
@@ -255,16 +271,18 @@ def test_mc_pairinfo_matches_oracle(mid):
     assert abs(e - er) <= 1e-12 * abs(er)
 
 
-def test_mc_trials_match_oracle():
+@pytest.mark.parametrize("mid", [g6.TIP4P, g6.SPCE])
+def test_mc_trials_match_oracle(mid):
     """2000 mc2 trials on the GPU take the oracle's decisions: same accept
     count, the same final configuration bit for bit, the same PairInfo; the
     running energy agrees with a fresh recompute."""
     n, L = 512, 25.088
     com, q = lattice_start(n, L, 31)
-    s = g6.System(g6.TIP4P, n, L, 9.0, 0.5, 0.002)
+    print("mc trials model", mid, flush=True)
+    s = g6.System(mid, n, L, 9.0, 0.5, 0.002)
     s.upload(com, q)
     s.mc_begin(250.0, 4242, 0.3, 1.0)
-    r = O.MCRun(0, com, q.T.copy(), L, 9.0, 0.5, 250.0, 4242, 0.3, 1.0)
+    r = O.MCRun(mid, com, q.T.copy(), L, 9.0, 0.5, 250.0, 4242, 0.3, 1.0)
     for chunk in (700, 1300):
         acc = s.mc_trials(chunk)
         assert acc == r.trials(chunk), chunk

@@ -10,7 +10,7 @@ from oracle import pyoracle as O
 from tests.golden.configs import lattice_start
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
-CASES = ["tip4p_216", "tip5p_216", "st2_216", "tip4p_125_ndiv3"]
+CASES = ["tip4p_216", "tip5p_216", "st2_216", "tip4p_125_ndiv3", "spce_216"]
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -70,24 +70,35 @@ def test_pairlist_digests_larger_grids():
         assert sha(O.address_assign(com, d["L"], d["ndiv"])) == d["sha_cell"]
 
 
-@pytest.mark.parametrize("name", ["md_tip4p_216_nve", "md_tip4p_216_nvt"])
-def test_md_trajectory_matches_reference_driver(golden, name):
-    g = golden(name)
-    nose = None if g["nose"][1] == 0 else tuple(g["nose"])
-    md = O.MDRun(O.TIP4P, g["com"], g["quat"].T.copy(), float(g["L"]), float(g["rc"]),
+def md_trajectory(g, mid, nose, pre=""):
+    md = O.MDRun(mid, g["com"], g["quat"].T.copy(), float(g["L"]), float(g["rc"]),
                  float(g["taper"]), float(g["dt"]), fast=False, nose=nose)
     md.vreset(300.0, int(g["seed"]))
     for s in range(int(g["nsteps"])):
         md.step()
-        row = g["logd"][s]
+        row = g[pre + "logd"][s]
         assert md.s.temperature == row[1]                 # @LOGD column 2
         assert md.s.ep * 10 * 1e-3 == row[4]              # column 5, kJ/mol
         assert abs(md.hamiltonian() - row[12]) <= 1e-12 * abs(row[12])
-    n = md.n
-    assert np.array_equal(md.com.T, g["com_final"])
-    assert np.array_equal(md.dx, g["dxyz_final"].transpose(1, 0, 2))
-    assert np.array_equal(md.quat, g["quat_final"].transpose(1, 0, 2))
-    assert np.array_equal(md.w * float(g["dt"]), g["w_final_ext"].transpose(1, 0, 2))
+    assert np.array_equal(md.com.T, g[pre + "com_final"])
+    assert np.array_equal(md.dx, g[pre + "dxyz_final"].transpose(1, 0, 2))
+    assert np.array_equal(md.quat, g[pre + "quat_final"].transpose(1, 0, 2))
+    assert np.array_equal(md.w * float(g["dt"]), g[pre + "w_final_ext"].transpose(1, 0, 2))
+
+
+@pytest.mark.parametrize("name", ["md_tip4p_216_nve", "md_tip4p_216_nvt"])
+def test_md_trajectory_matches_reference_driver(golden, name):
+    g = golden(name)
+    nose = None if g["nose"][1] == 0 else tuple(g["nose"])
+    md_trajectory(g, O.TIP4P, nose)
+
+
+@pytest.mark.parametrize("ens", ["nve", "nvt"])
+def test_spce_md_trajectory_matches_reference_driver(golden, ens):
+    """SPC/E (StdForce with the O site charged and LJ) through the reference
+    driver: the oracle's step reproduces its log and final state bit for bit."""
+    g = golden("md_spce_216")
+    md_trajectory(g, O.SPCE, tuple(g["nose"]) if ens == "nvt" else None, ens + "_")
 
 
 def test_mt19937_known_answers():
@@ -115,7 +126,7 @@ def test_cutoff_closed_form():
     assert abs(f(rin + 1e-7)[0] - 1) < 1e-12 and abs(f(rout - 1e-7)[0]) < 1e-12
 
 
-@pytest.mark.parametrize("mid", [O.TIP4P, O.TIP5P, O.ST2])
+@pytest.mark.parametrize("mid", [O.TIP4P, O.TIP5P, O.ST2, O.SPCE])
 def test_force_is_minus_gradient(mid):
     """F = -dE/dx by central differences on one molecule's COM; sum F = 0."""
     L, rc, taper = 14.0, 6.5, 1.0
@@ -124,7 +135,7 @@ def test_force_is_minus_gradient(mid):
         _, pi, pj, sh = O.pairlist(c, L, rc, alltoall=True)
         er, fr = O.cutoff(pi, pj, c, sh, rc - taper, rc)
         sites, tm, intra = O.setsites(mid, c, q.T.copy())
-        kind = "fast" if mid != O.TIP5P else "std"
+        kind = "fast" if mid in (O.TIP4P, O.ST2) else "std"
         F, ep, _ = O.force(mid, kind, pi, pj, sh, er, fr, sites)
         Fm, _ = O.collect(mid, F, tm, intra)
         return ep.sum(), Fm
@@ -173,7 +184,7 @@ def test_lj_coulomb_dimer_closed_form():
     assert abs(ep[0] - e) < 1e-12 * abs(e)
 
 
-@pytest.mark.parametrize("name,mid", [("tip4p_216", 0), ("tip5p_216", 1)])
+@pytest.mark.parametrize("name,mid", [("tip4p_216", 0), ("tip5p_216", 1), ("spce_216", 3)])
 def test_pairinfo_matches_reference_pair_energies(golden, name, mid):
     """PairInfo (pairenergy over the all-pairs list within rc) holds, for
     every pair of the reference's own list, the reference's StdForce pair
