@@ -26,7 +26,7 @@ program g6md
   real(c_double) :: dt = 0.002d0, rc = 8.5d0, taper = 0.5d0, box = 0d0, voxa(3) = 0d0
   real(c_double) :: vtemp = -1d0, skin = 0d0, nose(9), th(G6_THERMO_LEN)
   integer :: nsteps = 0, nlog = 0, ios, i, done, chunk, target
-  integer :: nstep0 = 0
+  integer :: nstep0 = 0, snpo = 0, snpv = 0
   logical :: restart, fixc = .false.
   integer(c_int) :: amode = 0
   real(c_double) :: apex = 0d0, amass = 0d0, astate(8)
@@ -63,6 +63,10 @@ program g6md
         read(5, *) nsteps
      case ('@NLOG')
         read(5, *) nlog
+     case ('@SNPO')
+        read(5, *) snpo
+     case ('@SNPV')
+        read(5, *) snpv
      case ('@RCOA')
         read(5, *) rc, taper
      case ('@BXLA')
@@ -130,23 +134,28 @@ program g6md
   end if
 
   ! Mixture runs @MDLP more steps from the count nstep (0, or @ASTP of a
-  ! restart) and logs when mod(nstep, nlog) = 0 after the step that starts
-  ! at nstep, before nstep is advanced (Mixture.F90:1513, 2429-2490, 2524)
+  ! restart) and, after the step that starts at nstep and before nstep is
+  ! advanced, writes the coordinate snapshot when mod(nstep, snpo) = 0, the
+  ! coordinate and velocity snapshot when mod(nstep, snpv) = 0 and the log
+  ! row when mod(nstep, nlog) = 0, in that order (Mixture.F90:1513,
+  ! 2338-2385, 2429-2490, 2524)
   done = 0
   do while (done < nsteps)
      chunk = nsteps - done
-     if (nlog > 0) then
-        ! the next step k (1-based in this run) with mod(nstep0 + k - 1, nlog) = 0
-        target = done + 1 + modulo(-(nstep0 + done), nlog)
-        chunk = min(chunk, target - done)
-     end if
+     call next_event(nlog)
+     call next_event(snpo)
+     call next_event(snpv)
      call g6_check(g6_step(h, int(chunk, c_int)), 'g6_step')
      done = done + chunk
-     if (nlog > 0) then
-        if (modulo(nstep0 + done - 1, nlog) == 0) then
-           call g6_check(g6_get_thermo(h, th), 'g6_get_thermo')
-           call write_logd(nstep0 + done - 1, th)
-        end if
+     if (due(snpo) .or. due(snpv)) then
+        call g6_check(g6_download_state(h, com, quat, dx, w), 'g6_download_state')
+        call g6_check(g6_get_box(h, box), 'g6_get_box')
+        if (due(snpo)) call write_snapshot(nstep0 + done - 1, .false.)
+        if (due(snpv)) call write_snapshot(nstep0 + done - 1, .true.)
+     end if
+     if (due(nlog)) then
+        call g6_check(g6_get_thermo(h, th), 'g6_get_thermo')
+        call write_logd(nstep0 + done - 1, th)
      end if
   end do
   call g6_check(g6_download_state(h, com, quat, dx, w), 'g6_download_state')
@@ -242,19 +251,24 @@ contains
   end subroutine read_restart
 
   ! the binary restart of this run, records and order as Mixture writes them
+  ! the reference's 8-character id of the model
+  character(len=8) function model_id()
+    select case (model)
+    case (G6_TIP4P)
+       model_id = 'TIP4P   '
+    case (G6_TIP5P)
+       model_id = 'TIP5P   '
+    case (G6_SPCE)
+       model_id = 'SPC_E   '
+    case default
+       model_id = 'ST2_____'
+    end select
+  end function model_id
+
   subroutine write_restart()
     character(len=8) :: id
     integer :: c, k
-    select case (model)
-    case (G6_TIP4P)
-       id = 'TIP4P   '
-    case (G6_TIP5P)
-       id = 'TIP5P   '
-    case (G6_SPCE)
-       id = 'SPC_E   '
-    case default
-       id = 'ST2_____'
-    end select
+    id = model_id()
     open(13, file='fort.13', form='unformatted', access='sequential', status='replace')
     write(13) '@RCOA'
     write(13) rc, taper
@@ -296,6 +310,60 @@ contains
          ((quat(:, c, k), k = 1, 5), c = 1, 4), ((w(:, c, k) * dt, k = 1, 5), c = 1, 3)
     close(13)
   end subroutine write_restart
+
+  ! shortens chunk so that the run stops after the next step k (1-based in
+  ! this run) with mod(nstep0 + k - 1, every) = 0
+  subroutine next_event(every)
+    integer, intent(in) :: every
+    if (every > 0) then
+       target = done + 1 + modulo(-(nstep0 + done), every)
+       chunk = min(chunk, target - done)
+    end if
+  end subroutine next_event
+
+  ! whether the step just finished is one of every's output steps
+  logical function due(every)
+    integer, intent(in) :: every
+    due = .false.
+    if (every > 0) due = modulo(nstep0 + done - 1, every) == 0
+  end function due
+
+  ! the snapshot blocks of Mixture.F90:2338-2385 for one rigid component:
+  ! @ASTP, @BOX3 (Box_WriteBOX3, BoxModule.F90:127-135), @NCMP, then
+  ! @FIXC + Rigid_WriteNX4A (RigidModule.F90:865-884) or Rigid_WriteWTG3
+  ! (RigidModule.F90:830-863: velocity dx/dt in A/ps, angular velocity in
+  ! internal units)
+  subroutine write_snapshot(step, velocities)
+    integer, intent(in) :: step
+    logical, intent(in) :: velocities
+    integer :: j
+    write(6, '("@ASTP")')
+    write(6, *) step
+    write(6, '("@BOX3")')
+    write(6, *) box, box, box
+    write(6, '("@NCMP")')
+    write(6, *) 1
+    if (.not. velocities) then
+       write(6, '("@FIXC")')
+       write(6, *) fixc
+    end if
+    write(6, '("@ID08")')
+    write(6, '(a8)') model_id()
+    if (velocities) then
+       write(6, '("@WTG3")')
+       write(6, *) n
+       do j = 1, n
+          write(6, '(13(e17.10,1x))') com(j, 1), com(j, 2), com(j, 3), quat(j, 1, 1), &
+               quat(j, 2, 1), quat(j, 3, 1), quat(j, 4, 1), dx(j, 1, 1) / dt, dx(j, 2, 1) / dt, &
+               dx(j, 3, 1) / dt, w(j, 1, 1), w(j, 2, 1), w(j, 3, 1)
+       end do
+    else
+       write(6, '("@NX4A")')
+       write(6, *) n
+       write(6, '(7(e17.10,1x))') (com(j, 1), com(j, 2), com(j, 3), quat(j, 1, 1), &
+            quat(j, 2, 1), quat(j, 3, 1), quat(j, 4, 1), j = 1, n)
+    end if
+  end subroutine write_snapshot
 
   ! @LOGD columns of Mixture.F90:2453-2471 for one component: T, P, V, ep,
   ! ek, ep/N, ek/N (kJ/mol), nose_ep, nose_ek, andersen_ep, andersen_ek, H,

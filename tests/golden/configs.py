@@ -21,15 +21,21 @@ def lattice_start(n, L, seed):
     return com, q
 
 
-def mixture_deck(com, q, L, rc, taper, dt, nsteps, seed, nose=None, model="TIP4P", nlog=1):
+def mixture_deck(com, q, L, rc, taper, dt, nsteps, seed, nose=None, model="TIP4P", nlog=1,
+                 snpo=0, snpv=0):
     """A Mixture input deck (the reference's tags and payload layout,
-    Mixture.F90:600-1140): @DTPS, @MDLP, @NLOG, @RCOA, @BXLA, @VOXA, @VSET,
-    optional @NOS2 (T, q, dump), @ID08, @NX4A with x y z a b c d per line."""
+    Mixture.F90:597-1140): @DTPS, @MDLP, @NLOG, @RCOA, @BXLA, @VOXA, @VSET,
+    optional @SNPO / @SNPV (snapshot intervals), optional @NOS2 (T, q,
+    dump), @ID08, @NX4A with x y z a b c d per line."""
     n = len(com)
     dt, rc, taper, L = float(dt), float(rc), float(taper), float(L)
     deck = ["@DTPS", repr(dt), "@MDLP", str(int(nsteps)), "@NLOG", str(int(nlog)), "@RCOA",
             "%r %r" % (rc, taper), "@BXLA", repr(L), "@VOXA", "%r %r %r" % (rc, rc, rc),
             "@VSET", "300.0 %d" % int(seed)]
+    if snpo:
+        deck += ["@SNPO", str(int(snpo))]
+    if snpv:
+        deck += ["@SNPV", str(int(snpv))]
     if nose is not None:
         deck += ["@NOS2", "%r %r %r 0 0 0 0 0 0" % tuple(float(v) for v in nose)]
     deck += ["@ID08", model, "@NX4A", str(n)]
@@ -43,3 +49,46 @@ def parse_logd(text):
     lines = text.split("\n")
     rows = [lines[i + 1] for i, l in enumerate(lines[:-1]) if l.strip() == "@LOGD"]
     return np.array([[float(v) for v in r.split()] for r in rows])
+
+
+def parse_snapshots(text):
+    """The snapshot blocks of a Mixture-format log, in order: for each
+    @ASTP block a dict with step, box (3), ncmp, fixc (or None), id, kind
+    ("NX4A" or "WTG3") and rows (n x 7 or n x 13)."""
+    toks = text.split("\n")
+    out, i = [], 0
+    while i < len(toks):
+        if toks[i].strip() != "@ASTP":
+            i += 1
+            continue
+        b = {"step": int(toks[i + 1]), "fixc": None}
+        i += 2
+        while True:
+            tag = toks[i].strip()
+            if tag == "@BOX3":
+                b["box"] = [float(v) for v in toks[i + 1].split()]
+                i += 2
+            elif tag == "@NCMP":
+                b["ncmp"] = int(toks[i + 1])
+                i += 2
+            elif tag == "@FIXC":
+                b["fixc"] = toks[i + 1].strip()
+                i += 2
+            elif tag == "@ID08":
+                b["id"] = toks[i + 1][:8]
+                i += 2
+            elif tag in ("@NX4A", "@WTG3"):
+                b["kind"] = tag[1:]
+                n = int(toks[i + 1])
+                i += 2
+                width = 7 if tag == "@NX4A" else 13
+                vals = []
+                while len(vals) < n * width:
+                    vals += [float(v) for v in toks[i].split()]
+                    i += 1
+                b["rows"] = np.array(vals).reshape(n, width)
+                break
+            else:
+                raise ValueError("unexpected line in a snapshot block: %r" % toks[i])
+        out.append(b)
+    return out

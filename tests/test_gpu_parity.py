@@ -257,6 +257,40 @@ def test_fortran_driver_reproduces_the_reference_log(golden, name, mid, ens):
         assert err <= tol, (col, err)
 
 
+def test_fortran_driver_snapshots_like_the_reference(golden):
+    """@SNPO / @SNPV: the Fortran driver writes the coordinate and the
+    coordinate + velocity snapshots of the reference's log at the same steps,
+    in the same order among the @LOGD rows, with the same records and values
+    (11 significant digits as printed; the trajectories agree far below)."""
+    import subprocess
+    from tests.golden.configs import mixture_deck, parse_snapshots
+    g = golden("md_tip4p_216_nvt")
+    ref = golden("snapshots_tip4p_216")
+    deck = mixture_deck(g["com"], g["quat"], float(g["L"]), float(g["rc"]), float(g["taper"]),
+                        float(g["dt"]), int(g["nsteps"]), int(g["seed"]), tuple(g["nose"]),
+                        snpo=int(ref["snpo"]), snpv=int(ref["snpv"]))
+    exe = os.path.join(os.path.dirname(GOLD), "..", "genesis6_amd", "lib", "g6md")
+    print("g6md snapshots", flush=True)
+    with tempfile.TemporaryDirectory() as d:
+        r = subprocess.run([os.path.abspath(exe), "fp64"], input=deck, capture_output=True,
+                           text=True, timeout=120, cwd=d)
+    assert r.returncode == 0, r.stderr
+    seq = "".join({"@LOGD": "L", "@NX4A": "S", "@WTG3": "V"}[l.strip()]
+                  for l in r.stdout.split("\n") if l.strip() in ("@LOGD", "@NX4A", "@WTG3"))
+    assert seq == str(ref["sequence"])
+    snaps = parse_snapshots(r.stdout)
+    assert len(snaps) == len(ref["step"])
+    for k, b in enumerate(snaps):
+        assert b["kind"] == ref["kind"][k] and b["step"] == ref["step"][k]
+        assert b["ncmp"] == ref["ncmp"][k] and (b["fixc"] or "") == ref["fixc"][k]
+        assert b["id"] == ref["ident"][k]
+        assert np.array_equal(b["box"], ref["box"][k])
+        rows = ref["b%d_rows" % k]
+        assert b["rows"].shape == rows.shape
+        err = np.abs(b["rows"] - rows) / (np.abs(rows).max(0) + 1e-300)
+        assert err.max() <= 1e-9, (k, err.max(0))
+
+
 @pytest.mark.parametrize("mid", [0, 1, 3])
 def test_mc_pairinfo_matches_oracle(mid):
     """PairInfo on the GPU equals the oracle's matrix bit for bit."""
