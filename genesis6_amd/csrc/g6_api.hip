@@ -184,6 +184,77 @@ bool make_model_table(int model, int prec, ModelTable *m) {
     set_inverses(m);
     return true;
   }
+  if (model == G6_NVDE) {
+    // Rigid_NvdE_Constructor (NvdEModule.F90:90-148): O, H, H, M, L, L, CoM;
+    // the x shift is comz/mass as the constructor computes it
+    const double oh = 0.980, om = 0.230, ol = 0.8892;
+    const double hoh = 108.00 * kPI / 180.0, lol = 111.0 * kPI / 180.0;
+    const double ms[7] = {16.0, 1.0, 1.0, 0.0, 0.0, 0.0, 0.0};
+    double x[7] = {0, 0, 0, 0, ol * sin(0.5 * lol), -ol * sin(0.5 * lol), 0};
+    double y[7] = {0, oh * sin(0.5 * hoh), -oh * sin(0.5 * hoh), 0, 0, 0, 0};
+    double z[7] = {0, oh * cos(0.5 * hoh), oh * cos(0.5 * hoh), om, -ol * cos(0.5 * lol),
+                   -ol * cos(0.5 * lol), 0};
+    double cx = 0, cy = 0, cz = 0, mass = 0;
+    for (int i = 0; i < 6; i++) {
+      cx = cx + x[i] * ms[i];
+      cy = cy + y[i] * ms[i];
+      cz = cz + z[i] * ms[i];
+      mass = mass + ms[i];
+    }
+    const double mi = 1.0 / mass;
+    cx = cz * mi;
+    cy = cy * mi;
+    cz = cz * mi;
+    for (int i = 0; i < 6; i++) {
+      x[i] = x[i] - cx;
+      y[i] = y[i] - cy;
+      z[i] = z[i] - cz;
+    }
+    m->nsite = 7;
+    m->Ixx = m->Iyy = m->Izz = 0.0;
+    for (int i = 0; i < 7; i++) {
+      m->bx[i] = x[i];
+      m->by[i] = y[i];
+      m->bz[i] = z[i];
+      m->Ixx = m->Ixx + ms[i] * (y[i] * y[i] + z[i] * z[i]);
+      m->Iyy = m->Iyy + ms[i] * (z[i] * z[i] + x[i] * x[i]);
+      m->Izz = m->Izz + ms[i] * (x[i] * x[i] + y[i] * y[i]);
+    }
+    m->mass = mass;
+    // nvde_setinteraction (NvdEModule.F90:58-88), NvdEModule.F90:10-21
+    const double rgas = kNA * kKB;
+    const double q[6] = {0.0, 0.477, 0.477, -0.866, -0.044, -0.044};
+    const double eps[3] = {85.9766 * rgas * 1e-3, 13.8817 * rgas * 1e-3, 13.8817 * rgas * 1e-3};
+    const double sig[3] = {3.115, 0.673, 0.673};
+    for (int i = 0; i < 6; i++) m->std_q[i] = q[i];
+    for (int i = 0; i < 3; i++) {
+      m->std_eps[i] = eps[i];
+      m->std_sig[i] = sig[i];
+    }
+    // StdForce coefficients (StdInteraction.F90:206-221): 25 Coulomb terms
+    // over H, H, M, L, L; LJ on O-O and on the O-H, H-O, H-H pairs
+    const double KX2X = 1.0 / 1e-3, J2I = 1.0 / 10.0;
+    m->ncoul = 25;
+    for (int t = 0; t < 25; t++) {
+      const int a = 1 + t / 5, b = 1 + t % 5;
+      m->ci[t] = a;
+      m->cj[t] = b;
+      m->qq[t] = q[a] * q[b] * coeff() * KX2X * J2I * 1.0;
+    }
+    auto lj = [&](int a, int b, double *A, double *B) {
+      const double ljeps = sqrt(eps[a] * eps[b]);
+      const double s6 = ipow((sig[a] + sig[b]) * 0.5, 6);
+      *A = 4.0 * ljeps * s6 * s6 * KX2X * J2I * 1.0;
+      *B = -4.0 * ljeps * s6 * KX2X * J2I * 1.0;
+    };
+    lj(0, 0, &m->lj_a, &m->lj_b);
+    // the pairs of Traits<G6_NVDE>::xa/xb: (0,1) (0,2) (1,0) (2,0) (1,1) (1,2) (2,1) (2,2)
+    const int la[8] = {0, 0, 1, 2, 1, 1, 2, 2}, lb[8] = {1, 2, 0, 0, 1, 2, 1, 2};
+    m->nlx = 8;
+    for (int t = 0; t < 8; t++) lj(la[t], lb[t], &m->lx_a[t], &m->lx_b[t]);
+    set_inverses(m);
+    return true;
+  }
   return false;
 }
 

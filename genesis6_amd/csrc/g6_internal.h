@@ -33,8 +33,9 @@
 
 namespace g6 {
 
-constexpr int kMaxSite = 6;
-constexpr int kMaxCoul = 16;
+constexpr int kMaxSite = 7;
+constexpr int kMaxCoul = 25;
+constexpr int kMaxLJ = 8;  // LJ site pairs besides O-O
 
 // Interaction table of one rigid model, computed on the host with the
 // reference's coefficient arithmetic and passed to kernels by value.
@@ -44,8 +45,11 @@ struct ModelTable {
   int ncoul;     // Coulomb site pairs
   int ci[kMaxCoul], cj[kMaxCoul];
   double qq[kMaxCoul];  // prefactor, internal units
-  int lj_site;          // LJ site (O = 0), O-O only for all three models
+  int lj_site;          // the O-O LJ term (site 0 with site 0)
   double lj_a, lj_b;    // e = a/r^12 + b/r^6
+  // further LJ site pairs (NvdE: O-H, H-O, H-H), coefficients as lj_a, lj_b
+  int nlx;
+  double lx_a[kMaxLJ], lx_b[kMaxLJ];
   double bx[kMaxSite], by[kMaxSite], bz[kMaxSite];  // body-frame sites
   double Ixx, Iyy, Izz, mass;
   double inv_ixx, inv_iyy, inv_izz, inv_mass;  // 1/I and 1/m, rounded on the host
@@ -73,6 +77,7 @@ template <typename R>
 struct PairK {
   R qq[kMaxCoul];
   R lja, ljb, lja12, ljb6;  // e = a/r^12 + b/r^6; force factors 12a, 6b
+  R xa[kMaxLJ], xb[kMaxLJ], xa12[kMaxLJ], xb6[kMaxLJ];  // the further LJ terms
   R rin, rout, rin2, rout2, c1, c2;  // switch S between rin and rout
   R rl, ru, rl2, ru2, w3inv;         // ST2 switch on the COM distance
 };
@@ -206,6 +211,12 @@ PairK<R> make_pair_k(const ModelTable &mt, const Geom &g) {
   k.ljb = R(mt.lj_b);
   k.lja12 = R(mt.lj_a * 12.0);
   k.ljb6 = R(mt.lj_b * 6.0);
+  for (int t = 0; t < kMaxLJ; t++) {
+    k.xa[t] = R(mt.lx_a[t]);
+    k.xb[t] = R(mt.lx_b[t]);
+    k.xa12[t] = R(mt.lx_a[t] * 12.0);
+    k.xb6[t] = R(mt.lx_b[t] * 6.0);
+  }
   k.rin = R(g.rin);
   k.rout = R(g.rout);
   k.rin2 = R(g.rin * g.rin);

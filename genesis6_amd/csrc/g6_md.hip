@@ -1102,7 +1102,9 @@ __device__ __forceinline__ void md_main_body(const ModelTable &mt, const Geom &g
 // FP32 TIP4P/TIP5P/SPC/E rows fit in 128 VGPRs: four waves per SIMD, so a 4096-molecule
 // grid (1024 blocks) is resident at once on 256 CUs
 template <int M, typename R>
-constexpr int main_waves_per_eu() { return (M != G6_ST2 && sizeof(R) == 4) ? 4 : 1; }
+constexpr int main_waves_per_eu() {
+  return (M != G6_ST2 && M != G6_NVDE && sizeof(R) == 4) ? 4 : 1;
+}
 
 template <int M, typename R, bool NPT>
 __global__ __launch_bounds__(256, (main_waves_per_eu<M, R>())) void k_md_main(ModelTable mt, Geom g, MdBuf b, PairK<R> pk,
@@ -1213,11 +1215,13 @@ void md_launch_main(System &s, double *out, int parity, int close_mode) {
     if (s.model == G6_TIP4P) md_main_t<G6_TIP4P, float>(s, out, parity, ca);
     else if (s.model == G6_TIP5P) md_main_t<G6_TIP5P, float>(s, out, parity, ca);
     else if (s.model == G6_SPCE) md_main_t<G6_SPCE, float>(s, out, parity, ca);
+    else if (s.model == G6_NVDE) md_main_t<G6_NVDE, float>(s, out, parity, ca);
     else md_main_t<G6_ST2, float>(s, out, parity, ca);
   } else {
     if (s.model == G6_TIP4P) md_main_t<G6_TIP4P, double>(s, out, parity, ca);
     else if (s.model == G6_TIP5P) md_main_t<G6_TIP5P, double>(s, out, parity, ca);
     else if (s.model == G6_SPCE) md_main_t<G6_SPCE, double>(s, out, parity, ca);
+    else if (s.model == G6_NVDE) md_main_t<G6_NVDE, double>(s, out, parity, ca);
     else md_main_t<G6_ST2, double>(s, out, parity, ca);
   }
 }
@@ -1242,11 +1246,13 @@ void md_launch_main_lad(Ladder &l, int parity, int close_mode) {
     if (l.model == G6_TIP4P) md_main_lad_t<G6_TIP4P, float>(l, l.pkf, parity, close_mode);
     else if (l.model == G6_TIP5P) md_main_lad_t<G6_TIP5P, float>(l, l.pkf, parity, close_mode);
     else if (l.model == G6_SPCE) md_main_lad_t<G6_SPCE, float>(l, l.pkf, parity, close_mode);
+    else if (l.model == G6_NVDE) md_main_lad_t<G6_NVDE, float>(l, l.pkf, parity, close_mode);
     else md_main_lad_t<G6_ST2, float>(l, l.pkf, parity, close_mode);
   } else {
     if (l.model == G6_TIP4P) md_main_lad_t<G6_TIP4P, double>(l, l.pkd, parity, close_mode);
     else if (l.model == G6_TIP5P) md_main_lad_t<G6_TIP5P, double>(l, l.pkd, parity, close_mode);
     else if (l.model == G6_SPCE) md_main_lad_t<G6_SPCE, double>(l, l.pkd, parity, close_mode);
+    else if (l.model == G6_NVDE) md_main_lad_t<G6_NVDE, double>(l, l.pkd, parity, close_mode);
     else md_main_lad_t<G6_ST2, double>(l, l.pkd, parity, close_mode);
   }
 }

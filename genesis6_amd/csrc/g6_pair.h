@@ -8,6 +8,7 @@
 //   ST2    force_st2, ST2Module.F90:187-738 (16 Coulomb scaled by S*s12 with
 //          the Stillinger-Rahman switch s12 on the COM distance, LJ by S)
 //   SPC/E  StdForce (9 Coulomb over O, H, H + O-O LJ)
+//   NvdE   StdForce (25 Coulomb over H, H, M, L, L + LJ on O-O, O-H, H-H)
 // Cutoff ratios hh = S(r), dh = -S'(r)/r on the COM distance are inputs
 // (cutoff_func, CutoffModule.F90:38-59; cutoff_sw below).
 // Inputs are the COM separation d00 (shift already removed) and the site
@@ -22,7 +23,7 @@ template <int M>
 struct Traits;
 template <>
 struct Traits<G6_TIP4P> {
-  static constexpr int NS = 5, COM = 4, NC = 9;
+  static constexpr int NS = 5, COM = 4, NC = 9, NLX = 0;
   static constexpr bool ST2SW = false;
   __device__ static constexpr int ci(int t) {
     constexpr int a[9] = {1, 1, 1, 2, 3, 2, 2, 3, 3};
@@ -35,24 +36,40 @@ struct Traits<G6_TIP4P> {
 };
 template <>
 struct Traits<G6_TIP5P> {
-  static constexpr int NS = 6, COM = 5, NC = 16;
+  static constexpr int NS = 6, COM = 5, NC = 16, NLX = 0;
   static constexpr bool ST2SW = false;
   __device__ static constexpr int ci(int t) { return 1 + t / 4; }
   __device__ static constexpr int cj(int t) { return 1 + t % 4; }
 };
 template <>
 struct Traits<G6_ST2> {
-  static constexpr int NS = 6, COM = 5, NC = 16;
+  static constexpr int NS = 6, COM = 5, NC = 16, NLX = 0;
   static constexpr bool ST2SW = true;
   __device__ static constexpr int ci(int t) { return 1 + t / 4; }
   __device__ static constexpr int cj(int t) { return 1 + t % 4; }
 };
 template <>
 struct Traits<G6_SPCE> {  // the O site carries both the charge and the LJ term
-  static constexpr int NS = 4, COM = 3, NC = 9;
+  static constexpr int NS = 4, COM = 3, NC = 9, NLX = 0;
   static constexpr bool ST2SW = false;
   __device__ static constexpr int ci(int t) { return t / 3; }
   __device__ static constexpr int cj(int t) { return t % 3; }
+};
+template <>
+struct Traits<G6_NVDE> {  // O, H, H, M, L, L, CoM: LJ on O and H, charges on H, M, L
+  static constexpr int NS = 7, COM = 6, NC = 25, NLX = 8;
+  static constexpr bool ST2SW = false;
+  __device__ static constexpr int ci(int t) { return 1 + t / 5; }
+  __device__ static constexpr int cj(int t) { return 1 + t % 5; }
+  // the further LJ pairs in the order of ModelTable::lx_a
+  __device__ static constexpr int xa(int t) {
+    constexpr int a[8] = {0, 0, 1, 2, 1, 1, 2, 2};
+    return a[t];
+  }
+  __device__ static constexpr int xb(int t) {
+    constexpr int b[8] = {1, 2, 0, 0, 1, 2, 1, 2};
+    return b[t];
+  }
 };
 
 __device__ inline float g6_rsqrt(float x) { return __builtin_amdgcn_rsqf(x); }
@@ -128,6 +145,23 @@ __device__ inline R pair_sites(const ModelTable &mt, R hh, R dh, R d0, R d1,
     const R f0 = inv2 * (R(mt.lj_a * 12.0) * d12 + R(mt.lj_b * 6.0) * d6);
     fi[0][0] += dx * f0; fi[0][1] += dy * f0; fi[0][2] += dz * f0;
     fj[0][0] += dx * f0; fj[0][1] += dy * f0; fj[0][2] += dz * f0;
+  }
+  if constexpr (T::NLX > 0) {
+    static_assert(!T::ST2SW, "further LJ terms are scaled like the O-O term");
+#pragma unroll
+    for (int t = 0; t < T::NLX; t++) {
+      const int a = T::xa(t), b = T::xb(t);
+      const R dx = d0 + (ri[a][0] - rj[b][0]);
+      const R dy = d1 + (ri[a][1] - rj[b][1]);
+      const R dz = d2 + (ri[a][2] - rj[b][2]);
+      R inv2, inv1;
+      inv_r<R>(dx * dx + dy * dy + dz * dz, inv2, inv1);
+      const R d6 = inv2 * inv2 * inv2, d12 = d6 * d6;
+      eplj += R(mt.lx_a[t]) * d12 + R(mt.lx_b[t]) * d6;
+      const R f0 = inv2 * (R(mt.lx_a[t] * 12.0) * d12 + R(mt.lx_b[t] * 6.0) * d6);
+      fi[a][0] += dx * f0; fi[a][1] += dy * f0; fi[a][2] += dz * f0;
+      fj[b][0] += dx * f0; fj[b][1] += dy * f0; fj[b][2] += dz * f0;
+    }
   }
   R ep1;
   if constexpr (T::ST2SW) {
@@ -232,6 +266,28 @@ __device__ inline R pair_on_i(const PairK<R> &k, R dd, R d0, R d1, R d2,
     const R f0 = inv2 * (k.lja12 * d12 + k.ljb6 * d6);
     fl[0] = dx * f0; fl[1] = dy * f0; fl[2] = dz * f0;
   }
+  if constexpr (T::NLX > 0) {
+    // further LJ terms: on site 0 with the O-O term, on the other sites with
+    // the Coulomb sums (both scaled by S alone outside ST2)
+    static_assert(!T::ST2SW, "further LJ terms are scaled like the O-O term");
+#pragma unroll
+    for (int t = 0; t < T::NLX; t++) {
+      const int a = T::xa(t), b = T::xb(t);
+      const R dx = pi[a][0] - rj[b][0];
+      const R dy = pi[a][1] - rj[b][1];
+      const R dz = pi[a][2] - rj[b][2];
+      R inv2, inv1;
+      inv_r<R>(dx * dx + dy * dy + dz * dz, inv2, inv1);
+      const R d6 = inv2 * inv2 * inv2, d12 = d6 * d6;
+      eplj += k.xa[t] * d12 + k.xb[t] * d6;
+      const R f0 = inv2 * (k.xa12[t] * d12 + k.xb6[t] * d6);
+      if (a == 0) {
+        fl[0] += dx * f0; fl[1] += dy * f0; fl[2] += dz * f0;
+      } else {
+        fs[a][0] += dx * f0; fs[a][1] += dy * f0; fs[a][2] += dz * f0;
+      }
+    }
+  }
   // Coulomb sums over the charged sites and the LJ site, unscaled
   R fc[3] = {R(0), R(0), R(0)}, tc[3] = {R(0), R(0), R(0)};
 #pragma unroll
@@ -299,6 +355,19 @@ __device__ inline R pair_energy(const ModelTable &mt, R hh, R d0, R d1,
     inv_r<R>(dx * dx + dy * dy + dz * dz, inv2, inv1);
     const R d6 = inv2 * inv2 * inv2;
     eplj = R(mt.lj_a) * d6 * d6 + R(mt.lj_b) * d6;
+  }
+  if constexpr (T::NLX > 0) {
+#pragma unroll
+    for (int t = 0; t < T::NLX; t++) {
+      const int a = T::xa(t), b = T::xb(t);
+      const R dx = d0 + (ri[a][0] - rj[b][0]);
+      const R dy = d1 + (ri[a][1] - rj[b][1]);
+      const R dz = d2 + (ri[a][2] - rj[b][2]);
+      R inv2, inv1;
+      inv_r<R>(dx * dx + dy * dy + dz * dz, inv2, inv1);
+      const R d6 = inv2 * inv2 * inv2;
+      eplj += R(mt.lx_a[t]) * d6 * d6 + R(mt.lx_b[t]) * d6;
+    }
   }
   if constexpr (T::ST2SW) {
     const R dd = d0 * d0 + d1 * d1 + d2 * d2;

@@ -22,40 +22,71 @@ struct StdPattern;
 template <>
 struct StdPattern<G6_TIP4P> {  // O-O LJ, then M,H,H x M,H,H Coulomb
   static constexpr int NT = 10, NS = 5;
-  static constexpr bool Q0 = false;  // the O-O term is LJ only
   __device__ __host__ static constexpr int a(int u) { return u == 0 ? 0 : 1 + (u - 1) / 3; }
   __device__ __host__ static constexpr int b(int u) { return u == 0 ? 0 : 1 + (u - 1) % 3; }
+  __device__ __host__ static constexpr int kind(int u) { return u == 0 ? 3 : 2; }
 };
 template <>
 struct StdPattern<G6_TIP5P> {  // O-O LJ, then H,H,L,L x H,H,L,L Coulomb
   static constexpr int NT = 17, NS = 6;
-  static constexpr bool Q0 = false;
   __device__ __host__ static constexpr int a(int u) { return u == 0 ? 0 : 1 + (u - 1) / 4; }
   __device__ __host__ static constexpr int b(int u) { return u == 0 ? 0 : 1 + (u - 1) % 4; }
+  __device__ __host__ static constexpr int kind(int u) { return u == 0 ? 3 : 2; }
 };
-
 template <>
 struct StdPattern<G6_SPCE> {  // O-O LJ + Coulomb, then the rest of O,H,H x O,H,H Coulomb
   static constexpr int NT = 9, NS = 4;
-  static constexpr bool Q0 = true;
   __device__ __host__ static constexpr int a(int u) { return u / 3; }
   __device__ __host__ static constexpr int b(int u) { return u % 3; }
+  __device__ __host__ static constexpr int kind(int u) { return u == 0 ? 1 : 2; }
+};
+// NvdE (O, H, H, M, L, L, CoM): every site pair with a charge product or an
+// LJ eps product, site a of i outer and b of j inner
+struct NvdeTerms {
+  int a[30], b[30], k[30];
+};
+__device__ __host__ constexpr NvdeTerms nvde_terms() {
+  NvdeTerms t{};
+  const double q[6] = {0.0, 1.0, 1.0, 1.0, 1.0, 1.0};  // charged sites
+  const double e[6] = {1.0, 1.0, 1.0, 0.0, 0.0, 0.0};  // LJ sites
+  int u = 0;
+  for (int a = 0; a < 6; a++)
+    for (int b = 0; b < 6; b++) {
+      const bool qq = q[a] * q[b] != 0.0, lj = e[a] * e[b] != 0.0;
+      if (qq || lj) {
+        t.a[u] = a;
+        t.b[u] = b;
+        t.k[u] = qq ? (lj ? 1 : 2) : 3;
+        u++;
+      }
+    }
+  return t;
+}
+template <>
+struct StdPattern<G6_NVDE> {
+  static constexpr int NT = 30, NS = 7;
+  __device__ __host__ static constexpr int a(int u) { return nvde_terms().a[u]; }
+  __device__ __host__ static constexpr int b(int u) { return nvde_terms().b[u]; }
+  __device__ __host__ static constexpr int kind(int u) { return nvde_terms().k[u]; }
 };
 
 // Calls f(std::integral_constant<int, M>) for the StdPairPotential model M
-// (the MC paths take TIP4P, TIP5P and SPC/E).
+// (the MC paths take TIP4P, TIP5P, SPC/E and NvdE).
 template <class F>
 inline void std_model(int model, F &&f) {
   if (model == G6_TIP4P) f(std::integral_constant<int, G6_TIP4P>());
   else if (model == G6_TIP5P) f(std::integral_constant<int, G6_TIP5P>());
-  else f(std::integral_constant<int, G6_SPCE>());
+  else if (model == G6_SPCE) f(std::integral_constant<int, G6_SPCE>());
+  else f(std::integral_constant<int, G6_NVDE>());
 }
 
 template <int M>
 static bool std_table_matches(const StdTable &t) {
   if (t.nt != StdPattern<M>::NT) return false;
   for (int u = 0; u < t.nt; u++)
-    if (t.t[u].a != StdPattern<M>::a(u) || t.t[u].b != StdPattern<M>::b(u)) return false;
+    if (t.t[u].a != StdPattern<M>::a(u) || t.t[u].b != StdPattern<M>::b(u) ||
+        t.t[u].kind != StdPattern<M>::kind(u))
+      return false;
   return true;
 }
 
@@ -122,11 +153,11 @@ __device__ inline double std_energy_t(const StdTable &st, const double *xj, cons
     dz = dz - oz;
     const double dd = 1.0 / (dx * dx + dy * dy + dz * dz);
     double e0;
-    if (u == 0 && P::Q0) {  // the O-O term of SPC/E: Coulomb + LJ (kind 1)
+    if (P::kind(u) == 1) {  // Coulomb + LJ
       e0 = s.qq * sqrt(dd);
       const double d6 = dd * dd * dd, d12 = d6 * d6;
       e0 = e0 + s.aa * d12 + s.bb * d6;
-    } else if (u == 0) {  // the O-O term of TIP4P, TIP5P: LJ only
+    } else if (P::kind(u) == 3) {  // LJ only
       const double d6 = dd * dd * dd, d12 = d6 * d6;
       e0 = s.aa * d12 + s.bb * d6;
     } else {
@@ -157,11 +188,11 @@ __device__ inline double std_energy_rr(const StdTable &st, const double (&xa)[kM
     dz = dz - oz;
     const double dd = 1.0 / (dx * dx + dy * dy + dz * dz);
     double e0;
-    if (u == 0 && P::Q0) {  // the O-O term of SPC/E: Coulomb + LJ (kind 1)
+    if (P::kind(u) == 1) {  // Coulomb + LJ
       e0 = s.qq * sqrt(dd);
       const double d6 = dd * dd * dd, d12 = d6 * d6;
       e0 = e0 + s.aa * d12 + s.bb * d6;
-    } else if (u == 0) {  // the O-O term of TIP4P, TIP5P: LJ only
+    } else if (P::kind(u) == 3) {  // LJ only
       const double d6 = dd * dd * dd, d12 = d6 * d6;
       e0 = s.aa * d12 + s.bb * d6;
     } else {

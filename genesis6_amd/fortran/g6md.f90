@@ -92,6 +92,8 @@ program g6md
            model = G6_ST2
         case ('SPC_E')
            model = G6_SPCE
+        case ('NVDE____')
+           model = G6_NVDE
         case default
            write(0, '(a,a)') 'g6md: @ID08 model not on the GPU path: ', trim(line)
            error stop 2
@@ -230,6 +232,8 @@ contains
              model = G6_ST2
           case ('SPC_E   ')
              model = G6_SPCE
+          case ('NVDE____')
+             model = G6_NVDE
           case default
              write(0, '(a,a)') 'g6md: @ID08 model not on the GPU path: ', id
              error stop 2
@@ -260,6 +264,8 @@ contains
        model_id = 'TIP5P   '
     case (G6_SPCE)
        model_id = 'SPC_E   '
+    case (G6_NVDE)
+       model_id = 'NVDE____'
     case default
        model_id = 'ST2_____'
     end select

@@ -25,9 +25,9 @@ extern "C" {
 
 typedef int64_t g6_handle;
 
-/* models: the reference ids "TIP4P   ", "TIP5P   ", "ST2_____", "SPC_E   "
- * (RigidSetup.F90:33-62) */
-enum { G6_TIP4P = 0, G6_TIP5P = 1, G6_ST2 = 2, G6_SPCE = 3 };
+/* models: the reference ids "TIP4P   ", "TIP5P   ", "ST2_____", "SPC_E   ",
+ * "NVDE____" (RigidSetup.F90:33-62) */
+enum { G6_TIP4P = 0, G6_TIP5P = 1, G6_ST2 = 2, G6_SPCE = 3, G6_NVDE = 4 };
 /* pair arithmetic precision; accumulation of forces, torques, energy and
  * virial is FP64 in both */
 enum { G6_FP64 = 0, G6_FP32 = 1 };

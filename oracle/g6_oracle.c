@@ -143,6 +143,58 @@ int g6o_model_get(int model, g6o_model *m) {
     m->sig[0] = 3.166;
     return m->nsite;
   }
+  if (model == G6O_NVDE) {
+    /* Rigid_NvdE_Constructor, NvdEModule.F90:90-148: O, H, H, M, L, L, CoM.
+     * The constructor shifts x by comz/mass (NvdEModule.F90:125), kept. */
+    const double oh = 0.980, om = 0.230, ol = 0.8892;
+    const double hoh = 108.00 * PI_T / 180.0, lol = 111.0 * PI_T / 180.0;
+    const double ms[7] = {16.0, 1.0, 1.0, 0.0, 0.0, 0.0, 0.0};
+    double x[7] = {0, 0, 0, 0, 0, 0, 0}, y[7] = {0, 0, 0, 0, 0, 0, 0}, z[7] = {0, 0, 0, 0, 0, 0, 0};
+    y[1] = oh * sin(0.5 * hoh);
+    z[1] = oh * cos(0.5 * hoh);
+    y[2] = -oh * sin(0.5 * hoh);
+    z[2] = oh * cos(0.5 * hoh);
+    z[3] = om;
+    x[4] = ol * sin(0.5 * lol);
+    z[4] = -ol * cos(0.5 * lol);
+    x[5] = -ol * sin(0.5 * lol);
+    z[5] = -ol * cos(0.5 * lol);
+    double cx = 0, cy = 0, cz = 0, mass = 0;
+    for (int i = 0; i < 6; i++) {
+      cx = cx + x[i] * ms[i];
+      cy = cy + y[i] * ms[i];
+      cz = cz + z[i] * ms[i];
+      mass = mass + ms[i];
+    }
+    const double mi = 1.0 / mass;
+    cx = cz * mi;
+    cy = cy * mi;
+    cz = cz * mi;
+    for (int i = 0; i < 6; i++) {
+      x[i] = x[i] - cx;
+      y[i] = y[i] - cy;
+      z[i] = z[i] - cz;
+    }
+    m->nsite = 7;
+    for (int i = 0; i < 7; i++) {
+      m->bx[i] = x[i];
+      m->by[i] = y[i];
+      m->bz[i] = z[i];
+      m->Ixx = m->Ixx + ms[i] * (y[i] * y[i] + z[i] * z[i]);
+      m->Iyy = m->Iyy + ms[i] * (z[i] * z[i] + x[i] * x[i]);
+      m->Izz = m->Izz + ms[i] * (x[i] * x[i] + y[i] * y[i]);
+    }
+    m->mass = mass;
+    /* nvde_setinteraction, NvdEModule.F90:58-88 (parameters :10-21) */
+    m->q[1] = m->q[2] = 0.477;
+    m->q[3] = -0.866;
+    m->q[4] = m->q[5] = -0.044;
+    m->eps[0] = 85.9766 * rgas() * 1e-3;
+    m->eps[1] = m->eps[2] = 13.8817 * rgas() * 1e-3;
+    m->sig[0] = 3.115;
+    m->sig[1] = m->sig[2] = 0.673;
+    return m->nsite;
+  }
   return 0;
 }
 

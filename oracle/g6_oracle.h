@@ -15,14 +15,15 @@
 extern "C" {
 #endif
 
-enum { G6O_TIP4P = 0, G6O_TIP5P = 1, G6O_ST2 = 2, G6O_SPCE = 3 };
-#define G6O_MAXSITE 6
+enum { G6O_TIP4P = 0, G6O_TIP5P = 1, G6O_ST2 = 2, G6O_SPCE = 3, G6O_NVDE = 4 };
+#define G6O_MAXSITE 7
 
 /* Rigid-model tables: site charges/LJ, body-frame geometry, inertia, mass.
  * TIP4P: TIP4PModule.F90:43-97, TIP4PConst.F90:4-22
  * TIP5P: TIP5PModule.F90:39-83, RigidModule.F90:429-477
  * ST2:   ST2Module.F90:10-100
- * SPC/E: SPCEModule.F90:7-82 (StdForce, Interaction2.F90:67)              */
+ * SPC/E: SPCEModule.F90:7-82 (StdForce, Interaction2.F90:67)
+ * NvdE:  NvdEModule.F90:10-148 (StdForce)                                 */
 typedef struct {
   int nsite;                                  /* including the CoM site   */
   double q[G6O_MAXSITE], eps[G6O_MAXSITE], sig[G6O_MAXSITE];

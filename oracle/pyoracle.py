@@ -12,8 +12,10 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
-TIP4P, TIP5P, ST2, SPCE = 0, 1, 2, 3
-MODEL_IDS = {TIP4P: "TIP4P   ", TIP5P: "TIP5P   ", ST2: "ST2_____", SPCE: "SPC_E   "}
+TIP4P, TIP5P, ST2, SPCE, NVDE = 0, 1, 2, 3, 4
+MODEL_IDS = {TIP4P: "TIP4P   ", TIP5P: "TIP5P   ", ST2: "ST2_____", SPCE: "SPC_E   ",
+             NVDE: "NVDE____"}
+MAXSITE = 7         # G6O_MAXSITE
 
 _lib = None
 
@@ -42,7 +44,7 @@ _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 
 class Model(C.Structure):
     _fields_ = [("nsite", C.c_int)] + [
-        (k, C.c_double * 6) for k in ("q", "eps", "sig", "bx", "by", "bz")] + [
+        (k, C.c_double * MAXSITE) for k in ("q", "eps", "sig", "bx", "by", "bz")] + [
         ("Ixx", C.c_double), ("Iyy", C.c_double), ("Izz", C.c_double),
         ("mass", C.c_double)]
 

@@ -17,7 +17,7 @@ from tests.golden.configs import lattice_start
 
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
-CASES = ["tip4p_216", "tip5p_216", "st2_216", "tip4p_125_ndiv3", "spce_216"]
+CASES = ["tip4p_216", "tip5p_216", "st2_216", "tip4p_125_ndiv3", "spce_216", "nvde_216"]
 
 
 def relrms(a, b):
@@ -142,10 +142,12 @@ def test_md_kernels_forces_c2_size(skin):
         assert relrms(th[4:13].reshape(3, 3), vr) <= tol
 
 
-# (fixture, model, ensemble): the TIP4P pair of fixtures and the SPC/E one,
-# whose arrays carry an "nve_" / "nvt_" prefix
+# (fixture, model, ensemble): the TIP4P pair of fixtures and the SPC/E and
+# NvdE ones, whose arrays carry an "nve_" / "nvt_" prefix
 TRAJ = [("md_tip4p_216_nve", g6.TIP4P, ""), ("md_tip4p_216_nvt", g6.TIP4P, ""),
-        ("md_spce_216", g6.SPCE, "nve"), ("md_spce_216", g6.SPCE, "nvt")]
+        ("md_spce_216", g6.SPCE, "nve"), ("md_spce_216", g6.SPCE, "nvt"),
+        ("md_nvde_216", g6.NVDE, "nve"), ("md_nvde_216", g6.NVDE, "nvt")]
+DECK_ID = {g6.TIP4P: "TIP4P", g6.SPCE: "SPC_E", g6.NVDE: "NVDE____"}
 
 
 def traj_case(g, ens):
@@ -239,7 +241,7 @@ def test_fortran_driver_reproduces_the_reference_log(golden, name, mid, ens):
     pre, nose = traj_case(g, ens)
     deck = mixture_deck(g["com"], g["quat"], float(g["L"]), float(g["rc"]), float(g["taper"]),
                         float(g["dt"]), int(g["nsteps"]), int(g["seed"]), nose,
-                        model="SPC_E" if mid == g6.SPCE else "TIP4P")
+                        model=DECK_ID[mid])
     exe = os.path.join(os.path.dirname(GOLD), "..", "genesis6_amd", "lib", "g6md")
     print("g6md deck", name, flush=True)
     with tempfile.TemporaryDirectory() as d:   # g6md writes its restart file fort.13
@@ -291,7 +293,7 @@ def test_fortran_driver_snapshots_like_the_reference(golden):
         assert err.max() <= 1e-9, (k, err.max(0))
 
 
-@pytest.mark.parametrize("mid", [0, 1, 3])
+@pytest.mark.parametrize("mid", [0, 1, 3, 4])
 def test_mc_pairinfo_matches_oracle(mid):
     """PairInfo on the GPU equals the oracle's matrix bit for bit."""
     n, L = 216, 18.626
@@ -305,7 +307,7 @@ def test_mc_pairinfo_matches_oracle(mid):
     assert abs(e - er) <= 1e-12 * abs(er)
 
 
-@pytest.mark.parametrize("mid", [g6.TIP4P, g6.SPCE])
+@pytest.mark.parametrize("mid", [g6.TIP4P, g6.SPCE, g6.NVDE])
 def test_mc_trials_match_oracle(mid):
     """2000 mc2 trials on the GPU take the oracle's decisions: same accept
     count, the same final configuration bit for bit, the same PairInfo; the
