@@ -31,7 +31,22 @@ static void set_inverses(ModelTable *m) {
   m->inv_izz = 1.0 / m->Izz;
 }
 
-bool make_model_table(int model, int prec, ModelTable *m) {
+// StdForce's pair coefficients (StdInteraction.F90:206-221) from the site
+// parameters std_q, std_eps, std_sig for the Coulomb site pairs ci/cj and
+// the O-O LJ term, in place of a custom kernel's constants
+static void std_coefficients(ModelTable *m) {
+  const double KX2X = 1.0 / 1e-3, J2I = 1.0 / 10.0;
+  for (int t = 0; t < m->ncoul; t++)
+    m->qq[t] = m->std_q[m->ci[t]] * m->std_q[m->cj[t]] * coeff() * KX2X * J2I * 1.0;
+  const double eps = m->std_eps[0], sig = m->std_sig[0];
+  const double ljeps = sqrt(eps * eps);
+  const double s6 = ipow((sig + sig) * 0.5, 6);
+  m->lj_a = 4.0 * ljeps * s6 * s6 * KX2X * J2I * 1.0;
+  m->lj_b = -4.0 * ljeps * s6 * KX2X * J2I * 1.0;
+  m->rl = m->ru = 0.0;
+}
+
+bool make_model_table(int model, int prec, ModelTable *m, bool fast) {
   (void)prec;
   memset(m, 0, sizeof(*m));
   m->model = model;
@@ -69,6 +84,7 @@ bool make_model_table(int model, int prec, ModelTable *m) {
     m->std_q[2] = m->std_q[3] = 0.52;
     m->std_eps[0] = 0.15504166666666666666666 * kCA;
     m->std_sig[0] = 3.15357794197649532464;
+    if (!fast) std_coefficients(m);  // StdForce (StdInteraction.F90:206-221)
     set_inverses(m);
     return true;
   }
@@ -138,6 +154,12 @@ bool make_model_table(int model, int prec, ModelTable *m) {
       m->lj_b = -4.0 * eps * ipow(sig, 6) * 100.0;
       m->rl = 2.0160;
       m->ru = 3.1287;
+      // st2_setinteraction (ST2Module.F90:32-58): the StdForce parameters
+      m->std_q[1] = m->std_q[2] = qh;
+      m->std_q[3] = m->std_q[4] = qm;
+      m->std_eps[0] = eps;
+      m->std_sig[0] = sig;
+      if (!fast) std_coefficients(m);  // StdForce: no Stillinger-Rahman switch
     }
     set_inverses(m);
     return true;
@@ -461,6 +483,8 @@ int g6_init(g6_handle *h, int model, int n, double box_l, double rc, double tape
     return G6_EARG;
   }
   s->model = model;
+  s->fast = 1;
+  s->kmodel = kernel_model(model, true);
   s->n = n;
   s->ns = s->mt.nsite;
   s->prec = prec;
@@ -588,6 +612,24 @@ int g6_set_nose(g6_handle h, int active, const double *nose) {
   G6_CHECK(hipStreamSynchronize(s->stream));
   G6_CHECK(hipMemcpy(s->nose, z, sizeof(z), hipMemcpyHostToDevice));
   if (s->graph) {  // the captured launches embed the Nose flag
+    (void)hipGraphExecDestroy(s->graph);
+    s->graph = nullptr;
+  }
+  return 0;
+}
+
+int g6_set_fast(g6_handle h, int fast) {
+  System *s = get(h);
+  if (!s) return G6_EARG;
+  ModelTable m;
+  if (!make_model_table(s->model, s->prec, &m, fast != 0)) return G6_EARG;
+  G6_CHECK(hipStreamSynchronize(s->stream));
+  s->mt = m;
+  s->fast = fast != 0;
+  s->kmodel = kernel_model(s->model, s->fast);
+  s->pkf = make_pair_k<float>(s->mt, s->geo_md);
+  s->pkd = make_pair_k<double>(s->mt, s->geo_md);
+  if (s->graph) {  // the captured launches embed the pair constants
     (void)hipGraphExecDestroy(s->graph);
     s->graph = nullptr;
   }

@@ -159,6 +159,10 @@ struct McHost {
 
 struct System {
   int model = 0, n = 0, ns = 0, prec = 0;
+  // fast: the reference's @FAST (custom kernels for TIP4P and ST2); kmodel:
+  // the pair-kernel layout the MD and force paths instantiate (ST2 without
+  // fast runs StdForce, whose site terms are those of the TIP5P layout)
+  int fast = 1, kmodel = 0;
   double L = 0, rc = 0, taper = 0, req = 0, dt = 0, skin = 0;
   ModelTable mt{};
   Geom geo{};     // the reference grid (ndiv = int(L/voxa)), entry points
@@ -198,7 +202,9 @@ struct System {
 };
 
 // model tables (g6_api.hip)
-bool make_model_table(int model, int prec, ModelTable *mt);
+bool make_model_table(int model, int prec, ModelTable *mt, bool fast = true);
+// the pair-kernel layout of a model with or without the custom kernels
+inline int kernel_model(int model, bool fast) { return model == G6_ST2 && !fast ? G6_TIP5P : model; }
 void make_std_table(const ModelTable &mt, StdTable *t);
 void mc_free(System &s);
 void npt_forget(System *s);  // frees the NPT Monte Carlo state of s (g6_npt.hip)

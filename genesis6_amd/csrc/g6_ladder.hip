@@ -271,12 +271,12 @@ int g6_ladder_create(g6_ladder *hl, int nsys, const g6_handle *hs) {
       return G6_EARG;
     }
     if (r == 0) {
-      l->model = s->model;
+      l->model = s->kmodel;  // the pair-kernel layout
       l->prec = s->prec;
       l->mt = s->mt;
       l->pkf = s->pkf;
       l->pkd = s->pkd;
-    } else if (s->model != l->model || s->prec != l->prec ||
+    } else if (s->kmodel != l->model || s->prec != l->prec ||
                memcmp(&s->pkf, &l->pkf, sizeof(l->pkf)) != 0 ||
                memcmp(&s->pkd, &l->pkd, sizeof(l->pkd)) != 0) {
       delete l;  // one kernel instance serves every system

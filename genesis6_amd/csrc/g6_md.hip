@@ -1212,16 +1212,16 @@ static void md_main_t(System &s, double *out, int parity, const CloseArgs &ca) {
 void md_launch_main(System &s, double *out, int parity, int close_mode) {
   const CloseArgs ca = close_args(s, close_mode, 1 - parity);
   if (s.prec == G6_FP32) {
-    if (s.model == G6_TIP4P) md_main_t<G6_TIP4P, float>(s, out, parity, ca);
-    else if (s.model == G6_TIP5P) md_main_t<G6_TIP5P, float>(s, out, parity, ca);
-    else if (s.model == G6_SPCE) md_main_t<G6_SPCE, float>(s, out, parity, ca);
-    else if (s.model == G6_NVDE) md_main_t<G6_NVDE, float>(s, out, parity, ca);
+    if (s.kmodel == G6_TIP4P) md_main_t<G6_TIP4P, float>(s, out, parity, ca);
+    else if (s.kmodel == G6_TIP5P) md_main_t<G6_TIP5P, float>(s, out, parity, ca);
+    else if (s.kmodel == G6_SPCE) md_main_t<G6_SPCE, float>(s, out, parity, ca);
+    else if (s.kmodel == G6_NVDE) md_main_t<G6_NVDE, float>(s, out, parity, ca);
     else md_main_t<G6_ST2, float>(s, out, parity, ca);
   } else {
-    if (s.model == G6_TIP4P) md_main_t<G6_TIP4P, double>(s, out, parity, ca);
-    else if (s.model == G6_TIP5P) md_main_t<G6_TIP5P, double>(s, out, parity, ca);
-    else if (s.model == G6_SPCE) md_main_t<G6_SPCE, double>(s, out, parity, ca);
-    else if (s.model == G6_NVDE) md_main_t<G6_NVDE, double>(s, out, parity, ca);
+    if (s.kmodel == G6_TIP4P) md_main_t<G6_TIP4P, double>(s, out, parity, ca);
+    else if (s.kmodel == G6_TIP5P) md_main_t<G6_TIP5P, double>(s, out, parity, ca);
+    else if (s.kmodel == G6_SPCE) md_main_t<G6_SPCE, double>(s, out, parity, ca);
+    else if (s.kmodel == G6_NVDE) md_main_t<G6_NVDE, double>(s, out, parity, ca);
     else md_main_t<G6_ST2, double>(s, out, parity, ca);
   }
 }

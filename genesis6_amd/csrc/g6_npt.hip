@@ -1125,7 +1125,7 @@ int g6_npt_begin(g6_handle h, double temp, double press_bar, uint64_t seed, doub
       rot_max_deg >= 90.0 || !(dlnv_max >= 0) || dlnv_max > 0.03 || !(p_volume >= 0) ||
       p_volume > 1 || !(voxa > 0))
     return G6_EARG;
-  if (s->model == G6_ST2 || s->n > 4096 || s->n < 2) return G6_EARG;
+  if (s->n > 4096 || s->n < 2) return G6_EARG;
   if (s->rc > 0.5 * s->L * (1.0 - dlnv_max)) return G6_EARG;  // one image within rc
   Npt &p = *npt_of(s, true);
   G6_CHECK(hipStreamSynchronize(s->stream));

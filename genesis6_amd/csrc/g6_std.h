@@ -70,12 +70,13 @@ struct StdPattern<G6_NVDE> {
   __device__ __host__ static constexpr int kind(int u) { return nvde_terms().k[u]; }
 };
 
-// Calls f(std::integral_constant<int, M>) for the StdPairPotential model M
-// (the MC paths take TIP4P, TIP5P, SPC/E and NvdE).
+// Calls f(std::integral_constant<int, M>) for the StdPairPotential term
+// pattern M of a model: ST2's StdPairPotential terms (no Stillinger-Rahman
+// switch, as mc2 evaluates them) have the TIP5P pattern.
 template <class F>
 inline void std_model(int model, F &&f) {
   if (model == G6_TIP4P) f(std::integral_constant<int, G6_TIP4P>());
-  else if (model == G6_TIP5P) f(std::integral_constant<int, G6_TIP5P>());
+  else if (model == G6_TIP5P || model == G6_ST2) f(std::integral_constant<int, G6_TIP5P>());
   else if (model == G6_SPCE) f(std::integral_constant<int, G6_SPCE>());
   else f(std::integral_constant<int, G6_NVDE>());
 }

@@ -44,6 +44,12 @@ module g6gpu
        real(c_double), intent(out) :: com(*), quat(*), dx(*), w(*)
      end function g6_download_state
 
+     integer(c_int) function g6_set_fast(h, fast) bind(C, name="g6_set_fast")
+       import :: c_int, c_int64_t
+       integer(c_int64_t), value :: h
+       integer(c_int), value :: fast
+     end function g6_set_fast
+
      integer(c_int) function g6_set_nose(h, active, nose) bind(C, name="g6_set_nose")
        import :: c_int, c_int64_t, c_double
        integer(c_int64_t), value :: h

@@ -32,7 +32,7 @@ program g6md
   real(c_double) :: apex = 0d0, amass = 0d0, astate(8)
   real(c_double), target :: av(6)
   logical :: has_av = .false.
-  integer(c_int) :: model = -1, prec = G6_FP64, n = 0
+  integer(c_int) :: model = -1, prec = G6_FP64, n = 0, fast = 0
   integer(c_int32_t) :: vseed = 0
   integer(c_int64_t) :: h
   logical :: has_nose = .false.
@@ -63,6 +63,10 @@ program g6md
         read(5, *) nsteps
      case ('@NLOG')
         read(5, *) nlog
+     case ('@FAST')
+        read(5, *) i
+        fast = 0
+        if (i /= 0) fast = 1
      case ('@SNPO')
         read(5, *) snpo
      case ('@SNPV')
@@ -121,6 +125,8 @@ program g6md
   if (voxa(1) <= 0d0) voxa = rc
 
   call g6_check(g6_init(h, model, n, box, rc, taper, voxa(1), dt, prec, skin), 'g6_init')
+  ! @FAST (default off, Mixture.F90:357, 732-736): StdForce or the custom kernels
+  call g6_check(g6_set_fast(h, fast), 'g6_set_fast')
   call g6_check(g6_upload_state(h, com, quat, dx, w), 'g6_upload_state')
   if (has_nose) then
      nose(2) = nose(2) * dt        ! @NOS2 mass, times dt (Mixture.F90:1206)

@@ -171,6 +171,10 @@ class System:
         return (com.reshape(3, n).T.copy(), q.reshape(5, 4, n), dx.reshape(5, 3, n),
                 w.reshape(5, 3, n))
 
+    def set_fast(self, fast):
+        """@FAST: True for the custom TIP4P/ST2 kernels, False for StdForce."""
+        check(lib().g6_set_fast(self.h, 1 if fast else 0), "g6_set_fast")
+
     def set_nose(self, temp, q_nos2, dump=0.0, zeta=(0, 0, 0, 0, 0), zetasum=0.0,
                  active=True):
         """@NOS2 semantics: q_nos2 is the tag's mass; internally q*dt."""

@@ -71,6 +71,15 @@ int g6_upload_state(g6_handle h, const double *com, const double *quat,
 int g6_download_state(g6_handle h, double *com, double *quat, double *dx,
                       double *w);
 
+/* The reference driver's @FAST switch (Mixture.F90:732-736, the custom
+ * argument of force, Interaction2.F90:19-67).  fast = 0: every pair through
+ * StdForce's coefficients (the reference's default), which for ST2 means
+ * no Stillinger-Rahman switch; fast != 0: the custom TIP4P and ST2 kernels
+ * (interaction_force_tip4p, force_st2).  Other models have one path.  The
+ * library starts with fast = 1; the Monte Carlo paths always use
+ * StdPairPotential, as mc2 does (mc2.F90:338). */
+int g6_set_fast(g6_handle h, int fast);
+
 /* Nose thermostat (NoseModule.F90:14-21).  nose[9] = temp (K), q (internal,
  * i.e. the @NOS2 value times dt, Mixture.F90:1206), dump, zeta0..zeta4,
  * zetasum.  active = 0 switches it off (NVE). */

@@ -22,11 +22,12 @@ def lattice_start(n, L, seed):
 
 
 def mixture_deck(com, q, L, rc, taper, dt, nsteps, seed, nose=None, model="TIP4P", nlog=1,
-                 snpo=0, snpv=0):
+                 snpo=0, snpv=0, fast=None):
     """A Mixture input deck (the reference's tags and payload layout,
     Mixture.F90:597-1140): @DTPS, @MDLP, @NLOG, @RCOA, @BXLA, @VOXA, @VSET,
-    optional @SNPO / @SNPV (snapshot intervals), optional @NOS2 (T, q,
-    dump), @ID08, @NX4A with x y z a b c d per line."""
+    optional @SNPO / @SNPV (snapshot intervals), optional @FAST (custom
+    kernels), optional @NOS2 (T, q, dump), @ID08, @NX4A with x y z a b c d
+    per line."""
     n = len(com)
     dt, rc, taper, L = float(dt), float(rc), float(taper), float(L)
     deck = ["@DTPS", repr(dt), "@MDLP", str(int(nsteps)), "@NLOG", str(int(nlog)), "@RCOA",
@@ -36,6 +37,8 @@ def mixture_deck(com, q, L, rc, taper, dt, nsteps, seed, nose=None, model="TIP4P
         deck += ["@SNPO", str(int(snpo))]
     if snpv:
         deck += ["@SNPV", str(int(snpv))]
+    if fast is not None:
+        deck += ["@FAST", "1" if fast else "0"]
     if nose is not None:
         deck += ["@NOS2", "%r %r %r 0 0 0 0 0 0" % tuple(float(v) for v in nose)]
     deck += ["@ID08", model, "@NX4A", str(n)]

@@ -87,10 +87,10 @@ def fortran_records(path):
     return recs
 
 
-def mixture(com, q, L, rc, taper, dt, nsteps, seed, nose=None, model="TIP4P"):
+def mixture(com, q, L, rc, taper, dt, nsteps, seed, nose=None, model="TIP4P", fast=None):
     """Runs the reference MD driver; returns @LOGD rows and the final @WTG2."""
     n = len(com)
-    deck = mixture_deck(com, q, L, rc, taper, dt, nsteps, seed, nose, model=model)
+    deck = mixture_deck(com, q, L, rc, taper, dt, nsteps, seed, nose, model=model, fast=fast)
     with tempfile.TemporaryDirectory() as d:
         out = run(os.path.join(REF, "Mixture"), d, deck)
         logd = parse_logd(out)

@@ -122,6 +122,29 @@ def test_md_kernels_forces(golden, name, prec, tol):
     assert int(th[14]) == len(g["pi"])
 
 
+@pytest.mark.parametrize("prec,tol", [(g6.FP64, 1e-12), (g6.FP32, 1e-5)])
+def test_st2_without_fast_is_stdforce(golden, prec, tol):
+    """g6_set_fast(0): ST2 through StdForce (no Stillinger-Rahman switch), as
+    the reference driver runs it without @FAST: the force seam and the MD
+    kernels against the reference's StdForce output."""
+    g = golden("st2_216")
+    s = system_for(g, prec)
+    s.set_fast(False)
+    F, ep, vr = s.force(g["pi"], g["pj"], g["shift"], g["eratio"], g["fratio"], g["sites"])
+    assert relrms(F, g["force_std"]) <= tol, relrms(F, g["force_std"])
+    assert relrms(ep, g["ep_std"]) <= tol and relrms(vr, g["vir_std"]) <= tol
+    e = s.pair_energy(g["pi"], g["pj"], g["shift"], g["eratio"], g["sites"])
+    assert relrms(e, g["ep_std"]) <= tol
+    s.upload(g["com"], g["quat"])
+    Fm, T, th = s.eval_forces()
+    assert relrms(Fm, g["mol_force"]) <= tol and relrms(T, g["mol_torque"]) <= tol
+    assert abs(th[0] - g["ep_std"].sum()) <= tol * np.abs(g["ep_std"]).sum()
+    # and back: the custom kernel again
+    s.set_fast(True)
+    F2, _, _ = s.force(g["pi"], g["pj"], g["shift"], g["eratio"], g["fratio"], g["sites"])
+    assert relrms(F2, g["force_fast"]) <= tol
+
+
 @pytest.mark.parametrize("skin", [0.0, 1.0])
 def test_md_kernels_forces_c2_size(skin):
     """4096 TIP4P at C2 density: FP64 and FP32 hot path vs the FP64 oracle."""
@@ -146,15 +169,19 @@ def test_md_kernels_forces_c2_size(skin):
 # NvdE ones, whose arrays carry an "nve_" / "nvt_" prefix
 TRAJ = [("md_tip4p_216_nve", g6.TIP4P, ""), ("md_tip4p_216_nvt", g6.TIP4P, ""),
         ("md_spce_216", g6.SPCE, "nve"), ("md_spce_216", g6.SPCE, "nvt"),
-        ("md_nvde_216", g6.NVDE, "nve"), ("md_nvde_216", g6.NVDE, "nvt")]
-DECK_ID = {g6.TIP4P: "TIP4P", g6.SPCE: "SPC_E", g6.NVDE: "NVDE____"}
+        ("md_nvde_216", g6.NVDE, "nve"), ("md_nvde_216", g6.NVDE, "nvt"),
+        ("md_st2_216", g6.ST2, "std"), ("md_st2_216", g6.ST2, "fast")]
+DECK_ID = {g6.TIP4P: "TIP4P", g6.SPCE: "SPC_E", g6.NVDE: "NVDE____", g6.ST2: "ST2_____"}
 
 
 def traj_case(g, ens):
-    """(prefix, Nose parameters or None) of one trajectory fixture."""
+    """(prefix, Nose parameters or None, @FAST) of one trajectory fixture:
+    the reference driver runs without @FAST unless the case is "fast"."""
+    if ens in ("std", "fast"):
+        return ens + "_", None, ens == "fast"
     if ens:
-        return ens + "_", (tuple(g["nose"]) if ens == "nvt" else None)
-    return "", (tuple(g["nose"]) if g["nose"][1] != 0 else None)
+        return ens + "_", (tuple(g["nose"]) if ens == "nvt" else None), False
+    return "", (tuple(g["nose"]) if g["nose"][1] != 0 else None), False
 
 
 @pytest.mark.parametrize("name,mid,ens", TRAJ)
@@ -162,10 +189,11 @@ def test_md_trajectory_vs_reference_driver(golden, name, mid, ens):
     """20 FP64 steps against the reference Mixture run (the same MB start
     from MT19937): temperature and energy per step, final Gear state."""
     g = golden(name)
-    pre, nose = traj_case(g, ens)
+    pre, nose, fast = traj_case(g, ens)
     n = len(g["com"])
     print("trajectory", name, ens, flush=True)
     s = g6.System(mid, n, float(g["L"]), float(g["rc"]), float(g["taper"]), float(g["dt"]))
+    s.set_fast(fast)
     s.upload(g["com"], g["quat"])
     if nose is not None:
         s.set_nose(*nose)
@@ -238,10 +266,10 @@ def test_fortran_driver_reproduces_the_reference_log(golden, name, mid, ens):
     import subprocess
     from tests.golden.configs import mixture_deck, parse_logd
     g = golden(name)
-    pre, nose = traj_case(g, ens)
+    pre, nose, fast = traj_case(g, ens)
     deck = mixture_deck(g["com"], g["quat"], float(g["L"]), float(g["rc"]), float(g["taper"]),
                         float(g["dt"]), int(g["nsteps"]), int(g["seed"]), nose,
-                        model=DECK_ID[mid])
+                        model=DECK_ID[mid], fast=True if fast else None)
     exe = os.path.join(os.path.dirname(GOLD), "..", "genesis6_amd", "lib", "g6md")
     print("g6md deck", name, flush=True)
     with tempfile.TemporaryDirectory() as d:   # g6md writes its restart file fort.13
@@ -293,7 +321,7 @@ def test_fortran_driver_snapshots_like_the_reference(golden):
         assert err.max() <= 1e-9, (k, err.max(0))
 
 
-@pytest.mark.parametrize("mid", [0, 1, 3, 4])
+@pytest.mark.parametrize("mid", [0, 1, 2, 3, 4])
 def test_mc_pairinfo_matches_oracle(mid):
     """PairInfo on the GPU equals the oracle's matrix bit for bit."""
     n, L = 216, 18.626
@@ -307,7 +335,7 @@ def test_mc_pairinfo_matches_oracle(mid):
     assert abs(e - er) <= 1e-12 * abs(er)
 
 
-@pytest.mark.parametrize("mid", [g6.TIP4P, g6.SPCE, g6.NVDE])
+@pytest.mark.parametrize("mid", [g6.TIP4P, g6.ST2, g6.SPCE, g6.NVDE])
 def test_mc_trials_match_oracle(mid):
     """2000 mc2 trials on the GPU take the oracle's decisions: same accept
     count, the same final configuration bit for bit, the same PairInfo; the
