@@ -170,8 +170,10 @@ def test_md_kernels_forces_c2_size(skin):
 TRAJ = [("md_tip4p_216_nve", g6.TIP4P, ""), ("md_tip4p_216_nvt", g6.TIP4P, ""),
         ("md_spce_216", g6.SPCE, "nve"), ("md_spce_216", g6.SPCE, "nvt"),
         ("md_nvde_216", g6.NVDE, "nve"), ("md_nvde_216", g6.NVDE, "nvt"),
-        ("md_st2_216", g6.ST2, "std"), ("md_st2_216", g6.ST2, "fast")]
-DECK_ID = {g6.TIP4P: "TIP4P", g6.SPCE: "SPC_E", g6.NVDE: "NVDE____", g6.ST2: "ST2_____"}
+        ("md_st2_216", g6.ST2, "std"), ("md_st2_216", g6.ST2, "fast"),
+        ("md_tip5p_216", g6.TIP5P, "nve"), ("md_tip5p_216", g6.TIP5P, "nvt")]
+DECK_ID = {g6.TIP4P: "TIP4P", g6.SPCE: "SPC_E", g6.NVDE: "NVDE____", g6.ST2: "ST2_____",
+           g6.TIP5P: "TIP5P"}
 
 
 def traj_case(g, ens):

@@ -107,11 +107,13 @@ def test_md_trajectory_matches_reference_driver(golden, name):
     md_trajectory(g, O.TIP4P, nose)
 
 
-@pytest.mark.parametrize("name,mid", [("md_spce_216", O.SPCE), ("md_nvde_216", O.NVDE)])
+@pytest.mark.parametrize("name,mid", [("md_spce_216", O.SPCE), ("md_nvde_216", O.NVDE),
+                                      ("md_tip5p_216", O.TIP5P)])
 @pytest.mark.parametrize("ens", ["nve", "nvt"])
 def test_std_model_md_trajectory_matches_reference_driver(golden, name, mid, ens):
-    """SPC/E (the O site charged and LJ) and NvdE (LJ on O and H, charges on
-    H, M and the lone pairs) through the reference driver's StdForce path:
+    """TIP5P, SPC/E (the O site charged and LJ) and NvdE (LJ on O and H,
+    charges on H, M and the lone pairs) through the reference driver's
+    StdForce path:
     the oracle's step reproduces its log and final state bit for bit."""
     g = golden(name)
     md_trajectory(g, mid, tuple(g["nose"]) if ens == "nvt" else None, ens + "_")
