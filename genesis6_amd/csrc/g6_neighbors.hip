@@ -835,4 +835,45 @@ int g6_triangle(g6_handle h, int nsite, int64_t npair, const int32_t *pair_i,
   return 0;
 }
 
+
+// The order parameter after a trial displacement of molecule target by
+// (dx, dy, dz), as sq6system_Differentiate / sTriangle_Differentiate return
+// it: the full evaluation of the displaced centres on the same pair list.
+static int displaced(int nsite, const double *coord, int target, double dx, double dy, double dz,
+                     std::vector<double> &out) {
+  if (!coord || target < 0 || target >= nsite || !(fabs(dx) < 1.0) || !(fabs(dy) < 1.0) ||
+      !(fabs(dz) < 1.0))
+    return G6_EARG;
+  out.assign(coord, coord + (size_t)nsite * 3);  // coord[3][nsite]
+  out[(size_t)target] += dx;
+  out[(size_t)nsite + target] += dy;
+  out[(size_t)2 * nsite + target] += dz;
+  return 0;
+}
+
+int g6_q6local_displacement(g6_handle h, int nsite, int64_t npair, const int32_t *pair_i,
+                            const int32_t *pair_j, const double *ox, const double *oy,
+                            const double *oz, const double *coord, int target, double dx,
+                            double dy, double dz, double *q6, int32_t *nbond, double *sumq6,
+                            double *op) {
+  if (nsite < 1) return G6_EARG;
+  std::vector<double> c;
+  const int rc = displaced(nsite, coord, target, dx, dy, dz, c);
+  if (rc) return rc;
+  return g6_q6local(h, nsite, npair, pair_i, pair_j, ox, oy, oz, c.data(), q6, nbond, sumq6, op);
+}
+
+int g6_triangle_displacement(g6_handle h, int nsite, int64_t npair, const int32_t *pair_i,
+                             const int32_t *pair_j, const double *ox, const double *oy,
+                             const double *oz, const double *coord, int target, double dx,
+                             double dy, double dz, const double *refhist, int logarithmic,
+                             double *hist, double *op) {
+  if (nsite < 1) return G6_EARG;
+  std::vector<double> c;
+  const int rc = displaced(nsite, coord, target, dx, dy, dz, c);
+  if (rc) return rc;
+  return g6_triangle(h, nsite, npair, pair_i, pair_j, ox, oy, oz, c.data(), refhist, logarithmic,
+                     hist, op);
+}
+
 }  // extern "C"

@@ -210,6 +210,33 @@ module g6gpu
        real(c_double), intent(out) :: hist(21), op
      end function g6_triangle
 
+     ! the order parameters after a trial displacement of molecule target
+     ! (0-based), as sq6system_Differentiate / sTriangle_Differentiate
+     integer(c_int) function g6_q6local_displacement(h, nsite, npair, pair_i, pair_j, ox, oy, &
+          oz, coord, target, dx, dy, dz, q6, nbond, sumq6, op) &
+          bind(C, name="g6_q6local_displacement")
+       import :: c_int, c_int64_t, c_int32_t, c_double
+       integer(c_int64_t), value :: h, npair
+       integer(c_int), value :: nsite, target
+       integer(c_int32_t), intent(in) :: pair_i(*), pair_j(*)
+       real(c_double), intent(in) :: ox(*), oy(*), oz(*), coord(*)
+       real(c_double), value :: dx, dy, dz
+       real(c_double), intent(out) :: q6(*), sumq6, op
+       integer(c_int32_t), intent(out) :: nbond(*)
+     end function g6_q6local_displacement
+
+     integer(c_int) function g6_triangle_displacement(h, nsite, npair, pair_i, pair_j, ox, oy, &
+          oz, coord, target, dx, dy, dz, refhist, logarithmic, hist, op) &
+          bind(C, name="g6_triangle_displacement")
+       import :: c_int, c_int64_t, c_int32_t, c_double
+       integer(c_int64_t), value :: h, npair
+       integer(c_int), value :: nsite, target, logarithmic
+       integer(c_int32_t), intent(in) :: pair_i(*), pair_j(*)
+       real(c_double), intent(in) :: ox(*), oy(*), oz(*), coord(*), refhist(21)
+       real(c_double), value :: dx, dy, dz
+       real(c_double), intent(out) :: hist(21), op
+     end function g6_triangle_displacement
+
      integer(c_int) function g6_set_andersen(h, mode, pex, mass, v) &
           bind(C, name="g6_set_andersen")
        import :: c_int, c_int64_t, c_double, c_ptr

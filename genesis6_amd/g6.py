@@ -64,6 +64,14 @@ def lib():
                                  C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.g6_triangle.argtypes = [_H, C.c_int, C.c_int64, _ip, _ip, _dp, _dp, _dp, _dp, _dp,
                                   C.c_int, _dp, C.POINTER(C.c_double)]
+        _d = C.c_double
+        L.g6_q6local_displacement.argtypes = [_H, C.c_int, C.c_int64, _ip, _ip, _dp, _dp, _dp,
+                                              _dp, C.c_int, _d, _d, _d, _dp, _ip,
+                                              C.POINTER(_d), C.POINTER(_d)]
+        L.g6_triangle_displacement.argtypes = [_H, C.c_int, C.c_int64, _ip, _ip, _dp, _dp, _dp,
+                                               _dp, C.c_int, _d, _d, _d, _dp, C.c_int, _dp,
+                                               C.POINTER(_d)]
+        L.g6_set_fast.argtypes = [_H, C.c_int]
         L.g6_time_force.argtypes = [_H, C.c_int, C.POINTER(C.c_double)]
         L.g6_set_andersen.argtypes = [_H, C.c_int, C.c_double, C.c_double, C.c_void_p]
         L.g6_get_andersen.argtypes = [_H, _dp]
@@ -421,6 +429,30 @@ def triangle(s, pi, pj, shifts, coord, refhist, logarithmic=False):
     op = C.c_double(0)
     check(lib().g6_triangle(s.h, n, *_pairs(pi, pj, shifts, coord), _c(refhist),
                             int(logarithmic), h, C.byref(op)), "g6_triangle")
+    return h, op.value
+
+
+def q6local_displacement(s, pi, pj, shifts, coord, target, delta):
+    """sq6system_Differentiate's order parameter after moving molecule
+    target (0-based) by delta (3,): (q6[n], nbond[n], sumq6, op)."""
+    n = len(coord)
+    q6 = np.zeros(n); nb = np.zeros(n, np.int32)
+    sm, op = C.c_double(0), C.c_double(0)
+    check(lib().g6_q6local_displacement(s.h, n, *_pairs(pi, pj, shifts, coord), int(target),
+                                        *[float(v) for v in delta], q6, nb, C.byref(sm),
+                                        C.byref(op)), "g6_q6local_displacement")
+    return q6, nb, sm.value, op.value
+
+
+def triangle_displacement(s, pi, pj, shifts, coord, target, delta, refhist, logarithmic=False):
+    """sTriangle_Differentiate's order parameter after moving molecule
+    target by delta: (hist[21], op)."""
+    h = np.zeros(21)
+    op = C.c_double(0)
+    check(lib().g6_triangle_displacement(s.h, len(coord), *_pairs(pi, pj, shifts, coord),
+                                         int(target), *[float(v) for v in delta], _c(refhist),
+                                         int(logarithmic), h, C.byref(op)),
+          "g6_triangle_displacement")
     return h, op.value
 
 

@@ -203,6 +203,28 @@ int g6_triangle(g6_handle h, int nsite, int64_t npair, const int32_t *pair_i,
                 const int32_t *pair_j, const double *ox, const double *oy,
                 const double *oz, const double *coord, const double *refhist,
                 int logarithmic, double *hist, double *op);
+/* The order parameters after a trial displacement of molecule target
+ * (0-based) by (dx, dy, dz), each component below 1 A, on the same pair list
+ * and image shifts: what sq6system_Differentiate (Q6Local.F90:236-465) and
+ * sTriangle_Differentiate (Triangle.F90:125-249) return to mc2's umbrella
+ * step (mc2.F90:690-692), evaluated in full on the displaced centres.  The
+ * reference updates its sums incrementally and agrees with this except when
+ * two molecules bonded to each other (within 3 A) both enter or leave the
+ * target's 5 A shell in one move: its Q6 update then adds their bond to the
+ * target's sum from both sides, and this returns its own full evaluation
+ * (sq6system_prepareall) of the displaced configuration instead. */
+int g6_q6local_displacement(g6_handle h, int nsite, int64_t npair,
+                            const int32_t *pair_i, const int32_t *pair_j,
+                            const double *ox, const double *oy, const double *oz,
+                            const double *coord, int target, double dx, double dy,
+                            double dz, double *q6, int32_t *nbond, double *sumq6,
+                            double *op);
+int g6_triangle_displacement(g6_handle h, int nsite, int64_t npair,
+                             const int32_t *pair_i, const int32_t *pair_j,
+                             const double *ox, const double *oy, const double *oz,
+                             const double *coord, int target, double dx, double dy,
+                             double dz, const double *refhist, int logarithmic,
+                             double *hist, double *op);
 
 /* ---- constant-pressure MD: the Andersen barostat (AndersenModule.F90) ---
  * mode 1 (orthorhombic: the cubic box follows the volume, the reference's

@@ -14,6 +14,10 @@
 !   'Q6' / per molecule: q6 nbond / sumq6 op       (sq6system_prepareall)
 !   'TRI' / 21 bins / op                            (sTriangle_prepareall)
 !   'LOGTRI' / 21 bins / op                         (the logarithmic form)
+! then, if stdin holds them, ncase lines: target dx dy dz (1-based target);
+! for each case the order parameters after that trial displacement:
+!   'DQ6' op   (sq6system_Differentiate, then sq6system_Reject)
+!   'DTRI' op  (sTriangle_Differentiate of the linear form, then Reject)
 program refop
   use interaction_module
   use vector_module
@@ -21,12 +25,13 @@ program refop
   use triangle
   use distrib_module
   implicit none
-  integer :: nsite, npair, i, k
-  real(kind=8) :: op
+  integer :: nsite, npair, i, k, ncase, c, target, ios
+  real(kind=8) :: op, ddx, ddy, ddz
   type(sInteraction) :: ww
   type(vector3), allocatable :: coord(:)
   type(sq6system), pointer :: q6s
-  type(sTriangle), pointer :: tri
+  type(sTriangle), pointer :: tri, tri1
+  type(sq6diff) :: q6d
   type(sHistogram), pointer :: refhist
   character(len=*), parameter :: f2 = '(es26.17e3,1x,i6)', f1 = '(es26.17e3)'
 
@@ -61,6 +66,7 @@ program refop
      write(*,f1) tri%hist%bins(i)
   enddo
   write(*,f1) op
+  tri1 => tri
 
   call new(tri, .true.)
   call sTriangle_prepareall(tri, nsite, ww, coord, refhist, op)
@@ -69,4 +75,18 @@ program refop
      write(*,f1) tri%hist%bins(i)
   enddo
   write(*,f1) op
+
+  read(*,*,iostat=ios) ncase
+  if (ios /= 0) ncase = 0
+  do c = 1, ncase
+     read(*,*) target, ddx, ddy, ddz
+     call sq6system_Differentiate(q6s, target, ddx, ddy, ddz, nsite, q6d, op)
+     write(*,'(a)') 'DQ6'
+     write(*,f1) op
+     call sq6system_Reject(q6s, q6d, target, nsite)
+     call sTriangle_Differentiate(tri1, target, ddx, ddy, ddz, refhist, op)
+     write(*,'(a)') 'DTRI'
+     write(*,f1) op
+     call sTriangle_Reject(tri1)
+  enddo
 end program refop

@@ -11,7 +11,9 @@ oracle), their all-pairs list within 8.5 A from the oracle, and the
 reference histogram of the triangle order parameter in the reference's
 mc2.input (@HIST).  Outputs: q6 and the bond count of every molecule, the
 sum and the order parameter; the triangle histogram and both forms of its
-order parameter.
+order parameter; and for eight seeded trial displacements of one molecule
+(up to 0.3 A per axis) the order parameters that the reference's incremental
+sq6system_Differentiate and sTriangle_Differentiate return.
 
     python tests/golden/make_order.py
 """
@@ -37,10 +39,15 @@ REFHIST = [2.18, 20.292, 65.672, 138.316, 245.649, 374.39, 478.799, 515.596, 465
 def run(com, L, rc):
     _, pi, pj, sh = po.pairlist(com, L, rc)
     n = len(com)
+    rng = np.random.default_rng(n)
+    targets = rng.choice(n, 8, replace=False).astype(np.int32)
+    delta = rng.uniform(-0.3, 0.3, (8, 3))
     lines = ["%d %d" % (n, len(pi))]
     lines += ["%d %d %.17e %.17e %.17e" % (pi[k] + 1, pj[k] + 1, *sh[k]) for k in range(len(pi))]
     lines += ["%.17e %.17e %.17e" % tuple(c) for c in com]
     lines += ["%.17e" % v for v in REFHIST]
+    lines += ["%d" % len(targets)]
+    lines += ["%d %.17e %.17e %.17e" % (targets[c] + 1, *delta[c]) for c in range(len(targets))]
     r = subprocess.run("ulimit -s unlimited; " + EXE, shell=True, input="\n".join(lines) + "\n",
                        capture_output=True, text=True, timeout=600)
     if r.returncode != 0:
@@ -56,6 +63,11 @@ def run(com, L, rc):
         k = t.index(tag)
         out["hist_" + tag.lower()] = np.array([float(x) for x in t[k + 1:k + 22]])
         out["op_" + tag.lower()] = float(t[k + 22])
+    out["disp_target"], out["disp_delta"] = targets, delta
+    for tag in ("DQ6", "DTRI"):
+        out["disp_op_" + tag[1:].lower()] = np.array(
+            [float(t[k + 1]) for k in range(len(t)) if t[k] == tag])
+        assert len(out["disp_op_" + tag[1:].lower()]) == len(targets)
     return out
 
 

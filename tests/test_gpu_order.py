@@ -18,6 +18,36 @@ def system(n, L):
 
 
 @pytest.mark.parametrize("name", ["order_tip4p_216", "order_tip4p_1024"])
+def test_trial_displacements_match_reference(golden, name):
+    """g6_q6local_displacement / g6_triangle_displacement against what the
+    reference's sq6system_Differentiate and sTriangle_Differentiate return for
+    eight trial moves, and against the oracle's full evaluation of the
+    displaced centres; in the one move where two bonded neighbours enter the
+    target's shell together the reference's incremental Q6 counts their bond
+    twice, and the GPU gives the full evaluation there."""
+    from tests.test_order_oracle import double_counted_cases
+    g = golden(name)
+    deviating = set(double_counted_cases(g))
+    s = system(len(g["com"]), float(g["L"]))
+    try:
+        for c, t in enumerate(g["disp_target"]):
+            print(name, "move", c, "target", int(t), flush=True)
+            d = g["disp_delta"][c]
+            _, _, _, op = g6.q6local_displacement(s, g["pi"], g["pj"], g["shift"], g["com"], t, d)
+            _, o = g6.triangle_displacement(s, g["pi"], g["pj"], g["shift"], g["com"], t, d,
+                                            g["refhist"])
+            com = g["com"].copy()
+            com[t] += d
+            _, _, _, op_o, _ = po.q6local(com, g["pi"], g["pj"], g["shift"])
+            assert abs(op - op_o) <= 1e-14 * abs(op_o)
+            if c not in deviating:
+                assert abs(op - g["disp_op_q6"][c]) <= 1e-13 * abs(g["disp_op_q6"][c])
+            assert abs(o - g["disp_op_tri"][c]) <= 1e-14 * abs(g["disp_op_tri"][c])
+    finally:
+        s.close()
+
+
+@pytest.mark.parametrize("name", ["order_tip4p_216", "order_tip4p_1024"])
 def test_order_parameters_match_reference(golden, name):
     g = golden(name)
     print(name, "pairs", len(g["pi"]), flush=True)
