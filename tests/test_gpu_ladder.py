@@ -19,11 +19,11 @@ def box(n, rho):
     return (n * 18.015 / (6.0221e23 * rho) * 1e24) ** (1.0 / 3.0)
 
 
-def make(n, rho, seed, temp, prec, skin, nose=True):
+def make(n, rho, seed, temp, prec, skin, nose=True, mid=g6.TIP4P):
     L = box(n, rho)
-    print("system n=%d L=%.4f T=%g prec=%d skin=%g nose=%d" % (n, L, temp, prec, skin, nose),
-          flush=True)
-    s = g6.System(g6.TIP4P, n, L, RC, TAPER, DT, prec=prec, skin=skin)
+    print("system model=%d n=%d L=%.4f T=%g prec=%d skin=%g nose=%d"
+          % (mid, n, L, temp, prec, skin, nose), flush=True)
+    s = g6.System(mid, n, L, RC, TAPER, DT, prec=prec, skin=skin)
     com, q = lattice_start(n, L, seed)
     s.upload(com, q)
     if nose:
@@ -36,10 +36,11 @@ SPECS = [(128, 0.92, 11, 250.0, 0.5, True), (250, 1.0, 12, 280.0, 1.0, True),
          (300, 1.05, 13, 300.0, 0.0, False), (128, 1.0, 14, 320.0, 0.5, True)]
 
 
-@pytest.mark.parametrize("prec", [g6.FP32, g6.FP64])
-def test_ladder_steps_equal_single_system_steps_bitwise(prec):
-    lad_sys = [make(n, r, sd, T, prec, sk, ns) for n, r, sd, T, sk, ns in SPECS]
-    solo = [make(n, r, sd, T, prec, sk, ns) for n, r, sd, T, sk, ns in SPECS]
+@pytest.mark.parametrize("prec,mid", [(g6.FP32, g6.TIP4P), (g6.FP64, g6.TIP4P),
+                                      (g6.FP32, g6.NVDE), (g6.FP32, g6.SPCE)])
+def test_ladder_steps_equal_single_system_steps_bitwise(prec, mid):
+    lad_sys = [make(n, r, sd, T, prec, sk, ns, mid) for n, r, sd, T, sk, ns in SPECS]
+    solo = [make(n, r, sd, T, prec, sk, ns, mid) for n, r, sd, T, sk, ns in SPECS]
     lad = g6.Ladder(lad_sys)
     for k in (7, 60, 1):
         print("steps", k, flush=True)

@@ -11,6 +11,7 @@ import numpy as np
 import pytest
 
 from genesis6_amd import g6
+from oracle import pyoracle as O
 from tests.golden.configs import parse_logd
 from tests.test_gpu_restart import same_restart
 
@@ -44,6 +45,37 @@ def test_andersen_log_matches_reference_driver(golden, name):
         assert err <= tol, (col, err)
     if name == "npt":
         same_restart(f13, f["fort13_npt"].tobytes(), 216)
+
+
+@pytest.mark.parametrize("mid", [g6.SPCE, g6.NVDE, g6.TIP5P])
+def test_andersen_other_models_follow_the_oracle(golden, mid):
+    """The barostat with the other rigid models (the Andersen step does not
+    depend on the model): 30 FP64 steps of @PCN4 1 1.0 1e-4 with the Nose
+    thermostat against the oracle's step (pinned bit for bit to the
+    reference driver for TIP4P): T, P, V and H at every step."""
+    g = golden("md_tip4p_216_nve1000")
+    L, rc, taper, dt = float(g["L"]), float(g["rc"]), float(g["taper"]), float(g["dt"])
+    print("andersen model", mid, flush=True)
+    s = g6.System(mid, 216, L, rc, taper, dt, prec=g6.FP64)
+    md = O.MDRun(mid, g["com"], g["quat"].T.copy(), L, rc, taper, dt, nose=(300.0, 7.5e5, 0.0))
+    try:
+        s.upload(g["com"], g["quat"])
+        s.set_nose(300.0, 7.5e5)
+        s.vreset(300.0, 4246)
+        s.set_andersen(1.0, 1e-4)
+        md.set_andersen(1.0, 1e-4)
+        md.vreset(300.0, 4246)
+        for k in range(30):
+            s.step(1)
+            md.step()
+            th = s.thermo()
+            assert abs(th[3] - md.s.temperature) <= 1e-11 * md.s.temperature, k
+            assert abs(th[25] - md.s.pressure) <= 1e-8 * max(1.0, abs(md.s.pressure)), k
+            assert abs(th[26] - md.s.volume) <= 1e-13 * md.s.volume, k
+            assert abs(th[13] - md.hamiltonian()) <= 1e-11 * abs(md.hamiltonian()), k
+        assert abs(s.box() - md.s.L) <= 1e-13 * md.s.L
+    finally:
+        s.close()
 
 
 def test_skin_keeps_the_interacting_set_while_the_box_moves(golden):
