@@ -171,6 +171,27 @@ def tip5p_run(steps, skin):
                 temperature_K=float(th[3]))
 
 
+def models_run(steps, skin):
+    """The other rigid models on the headline workload (4096 molecules at
+    1.0 g/cm3, L = 49.669 A, Nose NVT 300 K, rc 9.0 A switched over 0.5 A,
+    FP32 pairs): SPC/E, NvdE and ST2 (its custom kernel), molecule-steps/s."""
+    from genesis6_amd import g6
+    n, Lb, T = 4096, 49.669, 300.0
+    out = {}
+    for name, mid in (("spce", g6.SPCE), ("nvde", g6.NVDE), ("st2", g6.ST2)):
+        s = g6.System(mid, n, Lb, RC, TAPER, DT, prec=g6.FP32, skin=skin)
+        com, q = g6.lattice_start(n, Lb, 20261204)
+        s.upload(com, q)
+        s.set_nose(T, T * 2500.0)
+        s.vreset(T, 20261204)
+        el, th = timed_run(s, steps, 50)
+        s.close()
+        out[name] = dict(value=n * steps / el, unit="molecule-steps/s", steps=steps,
+                         ms_per_step=el / steps * 1e3, pairs_per_step=float(th[14]),
+                         temperature_K=float(th[3]))
+    return out
+
+
 def npt_run(configs, epochs, moves, rounds, K, p_volume, log):
     """1024 TIP4P molecules, NPT Monte Carlo at 250 K and 1 bar (L = 31.289 A
     at the start, rc 9.0 A switched over 0.5 A, FP64 PairInfo energies):
@@ -263,6 +284,8 @@ def main():
                     help="skip the ladder autotuner (skin and block assignment)")
     ap.add_argument("--tip5p-steps", type=int, default=500,
                     help="timed steps of the TIP5P-65536 system (0: skip)")
+    ap.add_argument("--model-steps", type=int, default=2000,
+                    help="timed steps of SPC/E, NvdE and ST2 at 4096 molecules (0: skip)")
     ap.add_argument("--npt-configs", type=int, default=20000,
                     help="MD configurations the surrogate is trained on (0: skip NPT MC)")
     ap.add_argument("--npt-epochs", type=int, default=10)
@@ -335,6 +358,12 @@ def main():
         tip5p = tip5p_run(a.tip5p_steps, a.skin)
         print("bench: TIP5P-65536 %.4g molecule-steps/s" % tip5p["value"], file=sys.stderr,
               flush=True)
+    models = None
+    if a.model_steps > 0:
+        models = models_run(a.model_steps, a.skin)
+        print("bench: 4096 molecules, " + ", ".join(
+            "%s %.4g" % (k, v["value"]) for k, v in models.items()) + " molecule-steps/s",
+            file=sys.stderr, flush=True)
     npt = None
     if a.npt_configs > 0:
         npt = npt_run(a.npt_configs, a.npt_epochs, a.npt_moves, a.npt_rounds, a.npt_k,
@@ -384,6 +413,7 @@ def main():
         "npt_mc": npt,
         "replica_ladders": ladder,
         "tip5p_65536": tip5p,
+        "other_models_4096": models,
     }
     print(json.dumps(out))
 
