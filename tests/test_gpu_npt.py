@@ -32,7 +32,7 @@ def system(n, rho, seed, steps=0, mid=g6.TIP4P):
 
 
 @pytest.mark.parametrize("mid,pvol", [(g6.TIP4P, 0.0), (g6.TIP4P, 0.3), (g6.SPCE, 0.3),
-                                      (g6.NVDE, 0.3)])
+                                      (g6.NVDE, 0.3), (g6.ST2, 0.3)])
 def test_plain_moves_match_oracle(mid, pvol):
     n = 128
     s, com, q, L = system(n, 1.0, 41, steps=40, mid=mid)
