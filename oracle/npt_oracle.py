@@ -10,6 +10,12 @@ molecule i with neighbours j whose O lies within Rc = 6 A of i's O
   16-31 2^(1-z) sum_{j<k} (1 + l cos t)^z exp(-eta (r_j^2 + r_k^2)) fc_j fc_k,
         index 16 + 8 e + 4 l + z for eta = 0.02, 0.08; l = +1, -1; z = 1, 2, 4, 8
 with fc(r) = (cos(pi r / Rc) + 1) / 2.
+
+Pinned by tests/test_features_oracle.py against what the geometry fixes:
+hand-computed dimer and trimer values for every block (radial, both H-bond
+blocks, all sixteen angular indices at cos t = 0, -1, 1/2), invariance under
+translation, rotation, relabelling and periodic images, and continuity to
+zero at Rc.
 """
 import numpy as np
 
