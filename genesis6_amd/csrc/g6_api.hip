@@ -347,6 +347,120 @@ static System *get(g6_handle h) {
 
 System *system_of(g6_handle h) { return get(h); }
 
+static int env_int(const char *name, int dflt) {
+  const char *e = getenv(name);
+  return e ? atoi(e) : dflt;
+}
+
+// The MD grid for the rows of both directions: cells of edge >= (rc+skin)/2.
+static void set_row_grid(System &s) {
+  Geom &m = s.geo_md;
+  const double rl = s.rc + s.skin;
+  m.ndiv = (int)(s.L / (0.5 * rl));
+  if (m.ndiv < 1) m.ndiv = 1;
+  m.ncell = m.ndiv * m.ndiv * m.ndiv;
+  m.cellinv = 1.0 / (s.L / m.ndiv);
+  const double percell = (double)s.n / m.ncell;
+  s.md.cell_cap = ((int)(2.0 * percell + 33.0) + 31) / 32 * 32;
+  if (s.md.cell_cap > s.n + 1) s.md.cell_cap = s.n + 1;
+  // G6_CELL_CAP (at g6_init) lowers the capacity, to exercise its error path
+  const int cc = env_int("G6_CELL_CAP", 0);
+  if (cc > 1 && cc < s.md.cell_cap) s.md.cell_cap = cc;
+  s.nblk_md = (s.n + 3) / 4;
+  s.md.part = nullptr;
+  s.tiles = false;
+}
+
+// The MD grid of the tile kernel: cells of edge >= rc + skin, one workgroup
+// per (cell, slice).
+static void set_tile_grid(System &s) {
+  Geom &m = s.geo_md;
+  m.ndiv = s.tile_ndiv;
+  m.ncell = m.ndiv * m.ndiv * m.ndiv;
+  m.cellinv = 1.0 / (s.L / m.ndiv);
+  s.md.cell_cap = s.tile_cap;
+  s.nblk_md = m.ncell * s.md.slices;
+  s.tiles = true;
+}
+
+
+// Whether a system runs the tile kernel, and its sizes.  The kernel needs
+// FP32 pairs, at least 3 cells of edge >= rc + skin per axis (27 distinct
+// stencil cells), slots that fit 11 bits and the LDS of the device.
+// G6_TILES=0 keeps the rows of both directions; G6_TILE_SLICES and
+// G6_TILE_WAVES override the workgroup shape.
+static void tile_plan(System &s, double nb_full) {
+  s.tile_ok = false;
+  if (s.prec != G6_FP32 || env_int("G6_TILES", 1) == 0) return;
+  const double rl = s.rc + s.skin;
+  const int nd = (int)(s.L / rl);
+  if (nd < 3) return;
+  const int ncell = nd * nd * nd;
+  // below one tile per CU the rows of both directions are faster (4096
+  // TIP4P at rc 9 A: 64 tiles, 11.7 us against 17.8 us per force evaluation)
+  if (ncell < env_int("G6_TILE_MIN", 256)) return;
+  const double percell = (double)s.n / ncell;
+  int cap = ((int)(2.0 * percell + 33.0) + 31) / 32 * 32;
+  if (cap > s.n + 1) cap = s.n + 1;
+  if (cap > 1024) return;  // the sort scratch of a wave holds 1024 ids
+  // neighbourhood slots: at least the 14 cells' mean plus 15 % and 48 (a
+  // liquid's count over 14 cells varies by a few per cent), and as many more
+  // as two workgroups per CU leave room for (a lattice start can fill some
+  // cells far above the mean); a neighbourhood beyond them is a capacity
+  // error, on which g6_step falls back to the rows
+  int nbmax = ((int)(1.15 * 14.0 * percell + 48.0) + 15) / 16 * 16;
+  // four to eight waves: the step close in block 0 and the block sums need
+  // 256 threads
+  // Waves per workgroup: a multiple of the CU's four SIMDs, so that every
+  // wave gets the same share of its SIMD.  Measured on 65,536 TIP5P at
+  // rc 9 A, skin 1 A (1728 tiles): 8 waves, records gathered from HBM, two
+  // workgroups per CU 135 us; 4 waves 174 us; 8 waves with the sites staged
+  // in LDS (one workgroup per CU) 180 us; the rows of both directions 165 us
+  int waves = env_int("G6_TILE_WAVES", 8);
+  if (waves < 4 || waves > 8) waves = 8;
+  int slices = env_int("G6_TILE_SLICES", 0);
+  if (slices < 1) {
+    slices = (256 + ncell - 1) / ncell;
+    if (slices > 4) slices = 4;
+  }
+  int lds_max = 0;
+  if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, 0) != hipSuccess)
+    return;
+  const int nq = s.mt.nsite - 1;
+  // LDS besides the dynamic part: the kernel's static arrays and the close
+  const int fixed = 2048;
+  const bool stage = env_int("G6_TILE_STAGE", 0) != 0;
+  {
+    // the most slots that two workgroups per CU leave room for
+    int room = std::min(14 * (cap - 1), (1 << 11) - 1) / 16 * 16;
+    while (room > nbmax && tile_layout(nq, room, cap, waves, stage).bytes + fixed > lds_max / 2)
+      room -= 16;
+    if (room > nbmax) nbmax = room;
+  }
+  if (nbmax > 14 * (cap - 1)) nbmax = 14 * (cap - 1);
+  if (nbmax > (1 << 11) - 1) nbmax = (1 << 11) - 1 - 15;
+  // G6_TILE_SLOTS (at g6_init) lowers the slots, to exercise the fallback
+  const int ts = env_int("G6_TILE_SLOTS", 0);
+  if (ts >= 16 && ts < nbmax) nbmax = ts / 16 * 16;
+  TileLayout lay = tile_layout(nq, nbmax, cap, waves, stage);
+  while (lay.bytes + fixed > lds_max && waves > 4) {
+    waves -= 1;
+    lay = tile_layout(nq, nbmax, cap, waves, stage);
+  }
+  if (lay.bytes + fixed > lds_max) return;
+  s.tile_ndiv = nd;
+  s.tile_cap = cap;
+  s.tile_waves = waves;
+  s.tile_lay = lay;
+  s.md.nbmax = nbmax;
+  s.md.slices = slices;
+  s.md.nds = 14 * slices;
+  int mr = ((int)(nb_full + 64.0) + 63) / 64 * 64;
+  if (mr > s.n) mr = s.n;
+  s.md.maxrow = mr;
+  s.tile_ok = true;
+}
+
 static int alloc_all(System &s) {
   const int n = s.n, ns = s.ns, nc = s.geo.ncell, ncm = s.geo_md.ncell;
   auto A = [](void **p, size_t b) { return hipMalloc(p, b > 0 ? b : 8); };
@@ -360,11 +474,24 @@ static int alloc_all(System &s) {
   AL(m.rec, sizeof(double) * (size_t)m.rs * n);
   AL(m.com_ref, sizeof(double) * 3 * n);
   AL(m.cell, sizeof(int) * n);
-  AL(m.cellres, sizeof(int) * 2 * (size_t)ncm * m.cell_cap);
+  // cell lists sized for the row grid and, when the system qualifies, the tile grid
+  size_t ncap = (size_t)ncm * m.cell_cap;
+  if (s.tile_ok) {
+    const size_t nt = (size_t)s.tile_ndiv * s.tile_ndiv * s.tile_ndiv * s.tile_cap;
+    if (nt > ncap) ncap = nt;
+  }
+  AL(m.cellres, sizeof(int) * 2 * ncap);
   AL(m.nbr_j, sizeof(int) * (size_t)n * m.maxnb);
   AL(m.nbr_code, sizeof(int) * (size_t)n * m.maxnb);
   AL(m.nbr_cnt, sizeof(int) * n);
-  AL(m.blockpart, sizeof(double) * 2 * kBP * s.nblk_md);
+  {
+    size_t nb = (size_t)s.nblk_md;
+    if (s.tile_ok) {
+      const size_t nt = (size_t)s.tile_ndiv * s.tile_ndiv * s.tile_ndiv * s.md.slices;
+      if (nt > nb) nb = nt;
+    }
+    AL(m.blockpart, sizeof(double) * 2 * kBP * nb);
+  }
   AL(m.blockek, sizeof(double) * kEkLen * (size_t)pre_waves(n));
   AL(m.frc, sizeof(double) * 6 * n);
   AL(m.nose, sizeof(double) * 9);
@@ -399,6 +526,23 @@ static int alloc_all(System &s) {
   return 0;
 }
 
+// the tile kernel's buffers (the system qualified in tile_plan)
+static int tile_alloc(System &s) {
+  MdBuf &m = s.md;
+  const size_t nt = (size_t)s.tile_ndiv * s.tile_ndiv * s.tile_ndiv;
+  auto A = [](void **p, size_t b) { return hipMalloc(p, b > 0 ? b : 8); };
+  G6_CHECK(A((void **)&m.part, sizeof(float) * 6 * (size_t)m.nds * s.n));
+  G6_CHECK(A((void **)&m.nb_id, sizeof(int) * nt * m.nbmax));
+  G6_CHECK(A((void **)&m.nb_base, sizeof(int) * 16 * nt));
+  G6_CHECK(A((void **)&m.trow, sizeof(unsigned) * (size_t)m.maxrow * s.n));
+  G6_CHECK(A((void **)&m.trow_cnt, sizeof(int) * s.n));
+  G6_CHECK(A((void **)&m.tcnt, sizeof(int) * nt * s.tile_cap));
+  G6_CHECK(hipMemset(m.part, 0, sizeof(float) * 6 * (size_t)m.nds * s.n));
+  if (s.tile_lay.bytes > 64 * 1024) G6_CHECK(tile_set_lds_limit(s.tile_lay.bytes) == 0 ? hipSuccess : hipErrorInvalidValue);
+  s.tile_part = m.part;
+  return 0;
+}
+
 // neighbour rows and cell lists ready for a fresh first step
 static int reset_step_state(System &s) {
   int fl[8] = {0, 0, 1, 1, 0, 0, 0, 0};
@@ -407,13 +551,26 @@ static int reset_step_state(System &s) {
   return 0;
 }
 
+// Back to the rows of both directions (ladders and the barostat use them).
+int use_row_grid(System &s) {
+  if (!s.tiles) return 0;
+  G6_CHECK(hipStreamSynchronize(s.stream));
+  set_row_grid(s);
+  if (s.graph) {
+    (void)hipGraphExecDestroy(s.graph);
+    s.graph = nullptr;
+  }
+  return reset_step_state(s);
+}
+
 static void free_all(System &s) {
   const MdBuf &m = s.md;
   void *ps[] = {m.com, m.dx, m.w, m.quat, m.tmat, m.rec, m.com_ref,
                 m.cell, m.cellres, m.nbr_j, m.nbr_code, m.nbr_cnt, m.blockpart,
                 m.blockek, m.frc, m.nose, m.thermo, m.flags, s.cell, s.cell_count, s.cell_start,
                 s.cell_fill, s.sorted_tmp, s.sorted, s.cnt, s.off, s.pi, s.pj, s.scode,
-                s.com_ref, s.lflags, m.box, m.anders};
+                s.com_ref, s.lflags, m.box, m.anders, s.tile_part, m.nb_id, m.nb_base,
+                m.trow, m.trow_cnt, m.tcnt, s.snap};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   mc_free(s);
@@ -521,35 +678,34 @@ int g6_init(g6_handle *h, int model, int n, double box_l, double rc, double tape
   const double all = 0.5 * (double)n * (double)(n - 1);
   s->pcap = (int)fmin(fmin(capd, all + 64.0), 2.0e9);
   // the MD grid: cells no smaller than (rc + skin)/2, searched with a 5^3
-  // stencil pruned per molecule (all cells once when ndiv < 5)
+  // stencil pruned per molecule (all cells once when ndiv < 5); tile systems
+  // switch to cells no smaller than rc + skin below
   {
     Geom &m = s->geo_md;
     m = g;
-    m.ndiv = (int)(box_l / (0.5 * rl));
-    if (m.ndiv < 1) m.ndiv = 1;
-    m.ncell = m.ndiv * m.ndiv * m.ndiv;
-    m.cellinv = 1.0 / (box_l / m.ndiv);
     MdBuf &b = s->md;
     b.rs = prec == G6_FP32 ? kRecF : kRecD;
+    set_row_grid(*s);
     const double rho = n / (box_l * box_l * box_l);
-    const double percell = (double)n / m.ncell;
-    b.cell_cap = ((int)(2.0 * percell + 33.0) + 31) / 32 * 32;
-    if (b.cell_cap > n + 1) b.cell_cap = n + 1;
     const double nb = 4.0 / 3.0 * kPI * rl * rl * rl * rho;
     b.maxnb = ((int)(1.5 * nb + 64.0) + 63) / 64 * 64;
     if (b.maxnb > n) b.maxnb = n;
     b.half_skin2 = skin > 0 ? 0.25 * skin * skin : 0.0;
     s->pkf = make_pair_k<float>(s->mt, m);
     s->pkd = make_pair_k<double>(s->mt, m);
-    s->nblk_md = (n + 3) / 4;
     // the force kernel keeps four rows in LDS (64 KiB of dynamic LDS at most)
     const int rb = main_region_bytes(b.maxnb, prec == G6_FP32 ? 4 : 8);
     if (4 * rb > 60 * 1024) {
       delete s;
       return G6_EARG;
     }
+    tile_plan(*s, nb);
   }
   int rcode = alloc_all(*s);
+  if (rcode == 0 && s->tile_ok) {
+    rcode = tile_alloc(*s);
+    if (rcode == 0) set_tile_grid(*s);
+  }
   if (rcode == 0 && hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
     rcode = G6_EHIP;
   for (int k = 0; rcode == 0 && k < 8; k++)
@@ -689,9 +845,53 @@ static int check_flags(System &s) {
   return 0;
 }
 
+// Device copy of the state a g6_step call starts from (Nordsieck arrays,
+// the Nose and barostat variables), so that a call that runs out of a
+// capacity leaves the state it was given.
+static int snap_state(System &s, bool restore) {
+  const size_t n = (size_t)s.n;
+  if (!s.snap) G6_CHECK(hipMalloc((void **)&s.snap, sizeof(double) * (53 * n + 25)));
+  struct Part {
+    double *d;
+    size_t len;
+  } parts[] = {{s.md.com, 3 * n}, {s.md.dx, 15 * n}, {s.md.w, 15 * n},   {s.md.quat, 20 * n},
+               {s.md.nose, 9},    {s.md.box, 8},      {s.md.anders, 8}};
+  double *p = s.snap;
+  for (const Part &q : parts) {
+    if (!q.d) continue;
+    G6_CHECK(hipMemcpyAsync(restore ? q.d : p, restore ? p : q.d, sizeof(double) * q.len,
+                            hipMemcpyDeviceToDevice, s.stream));
+    p += q.len;
+  }
+  return 0;
+}
+
+static int step_once(System *s, int nsteps);
+
 int g6_step(g6_handle h, int nsteps) {
   System *s = get(h);
   if (!s || nsteps < 0) return G6_EARG;
+  const int sv = snap_state(*s, false);
+  if (sv != 0) return sv;
+  const int rc = step_once(s, nsteps);
+  if (rc == 0 || rc == G6_EHIP) return rc;
+  // a capacity ran out (or an image shift left its code): back to the state
+  // of the call's start; a tile system reruns the call on the rows of both
+  // directions, whose capacities are its own, and keeps them
+  const int rs = snap_state(*s, true);
+  if (rs != 0) return rs;
+  G6_CHECK(hipStreamSynchronize(s->stream));
+  s->steps_done -= nsteps;
+  if (s->tiles) {
+    const int rg = use_row_grid(*s);
+    if (rg != 0) return rg;
+    return g6_step(h, nsteps);
+  }
+  const int rr = reset_step_state(*s);
+  return rr != 0 ? rr : rc;
+}
+
+static int step_once(System *s, int nsteps) {
   const int G = graph_steps();
   if (G > 0 && nsteps >= G) {
     if (!s->graph || s->graph_steps != G) {
@@ -787,6 +987,17 @@ int g6_time_force(g6_handle h, int nlaunch, double *ms) {
   return check_flags(*s);
 }
 
+int g6_md_kernel(g6_handle h, int32_t *info) {
+  System *s = get(h);
+  if (!s || !info) return G6_EARG;
+  info[0] = s->tiles ? 1 : 0;
+  info[1] = s->tiles ? s->tile_ndiv : 0;
+  info[2] = s->tiles ? s->md.slices : 0;
+  info[3] = s->tiles ? s->tile_waves : 0;
+  info[4] = s->tiles ? s->md.nbmax : 0;
+  return 0;
+}
+
 int g6_eval_forces(g6_handle h, double *force, double *torque, double *thermo) {
   System *s = get(h);
   if (!s || !force || !torque) return G6_EARG;
@@ -806,9 +1017,16 @@ int g6_eval_forces(g6_handle h, double *force, double *torque, double *thermo) {
                           s->stream));
   G6_CHECK(hipStreamSynchronize(s->stream));
   G6_CHECK(hipFree(out));
+  const int rc = check_flags(*s);
+  if (rc != 0 && rc != G6_EHIP && s->tiles) {
+    // a tile neighbourhood or row ran out of room: the rows of both
+    // directions from here on
+    const int rg = use_row_grid(*s);
+    return rg != 0 ? rg : g6_eval_forces(h, force, torque, thermo);
+  }
   if (thermo) G6_CHECK(hipMemcpy(thermo, s->thermo, sizeof(double) * G6_THERMO_LEN,
                                  hipMemcpyDeviceToHost));
-  return check_flags(*s);
+  return rc;
 }
 
 int g6_set_andersen(g6_handle h, int mode, double pex, double mass, const double *v) {
@@ -824,18 +1042,19 @@ int g6_set_andersen(g6_handle h, int mode, double pex, double mass, const double
     }
   } else {
     // orthorhombic (the cube follows the volume); orz changes one edge only
+    // every argument is checked before the handle changes: a failed call
+    // leaves the fixed-box path as it was
     if (mode != 1 || !(mass > 0)) return G6_EARG;
+    if (v && !(v[0] > 0)) return G6_EARG;
     const double dof = 6.0 * s->n;
     const double rl = s->rc + s->skin;
-    if (!s->andersen) {
-      G6_CHECK(hipMalloc((void **)&s->md.box, sizeof(double) * 8));
-      G6_CHECK(hipMalloc((void **)&s->md.anders, sizeof(double) * 8));
-      s->andersen = true;
-    }
+    // the barostat runs on the rows of both directions (its grid has cells
+    // of edge >= (rc + skin)/2 in the largest box it allows)
+    const int rg = use_row_grid(*s);
+    if (rg != 0) return rg;
     // the box is the handle's (a restart's @BOX3); v continues a saved
     // barostat state (@PCN5), NULL starts one
     const double L = s->L;
-    if (v && !(v[0] > 0)) return G6_EARG;
     const double nd = s->geo_md.ndiv;
     const double bx[8] = {L, 1.0 / L, 1.0 / (L / nd), L, nd, nd * 0.5 * rl, 0, 0};
     // internal mass: per-molecule mass * dof / dt^2 (Mixture.F90:1224-1228);
@@ -844,6 +1063,17 @@ int g6_set_andersen(g6_handle h, int mode, double pex, double mass, const double
                           v ? v[1] : 0.0, v ? v[2] : 0.0, v ? v[3] : 0.0, v ? v[4] : 0.0,
                           v ? v[5] : 0.0};
     if (L < bx[5]) return G6_EARG;
+    if (!s->andersen) {
+      double *box = nullptr, *anders = nullptr;
+      if (hipMalloc((void **)&box, sizeof(double) * 8) != hipSuccess) return G6_EHIP;
+      if (hipMalloc((void **)&anders, sizeof(double) * 8) != hipSuccess) {
+        (void)hipFree(box);
+        return G6_EHIP;
+      }
+      s->md.box = box;
+      s->md.anders = anders;
+      s->andersen = true;
+    }
     G6_CHECK(hipStreamSynchronize(s->stream));
     G6_CHECK(hipMemcpy(s->md.box, bx, sizeof(bx), hipMemcpyHostToDevice));
     G6_CHECK(hipMemcpy(s->md.anders, an, sizeof(an), hipMemcpyHostToDevice));

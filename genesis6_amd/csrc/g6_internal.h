@@ -119,6 +119,20 @@ struct MdBuf {
   // last row build, [4] ndiv, [5] the smallest edge the grid allows.
   // anders: pex (atm), mass (internal units), v0..v5 (AndersenModule.F90:5-8)
   double *box, *anders;
+  // Cell tiles (g6_tile.hip), FP32 fixed-box systems whose grid has cells of
+  // edge >= rc + skin; part == nullptr selects the rows of both directions.
+  //   part[n][nds][6]    force and lab torque partials: each tile writes one
+  //                      slot for every molecule of its neighbourhood
+  //   nb_id[ncell][nbmax] molecule ids of a tile's neighbourhood (its own cell,
+  //                      then the 13 forward cells, each sorted by id), and
+  //   nb_base[ncell][16] the first slot of each of the 14 cells ([14] total),
+  //                      both frozen at the last row build
+  //   trow[n][maxrow]    half rows: slot | image code << 11; trow_cnt[n]
+  //   tcnt[ncell][cell_cap] the row length of each tile member
+  float *part;
+  int nds, slices, nbmax, maxrow;
+  int *nb_id, *nb_base, *trow_cnt, *tcnt;
+  unsigned *trow;
 };
 
 // Descriptor of one system of a ladder, in device memory: the ladder
@@ -157,6 +171,15 @@ struct McHost {
   double getnext();  // 1 - the 31-bit uniform of mtrndv (RandomModule.F90:123-130)
 };
 
+// LDS layout of the tile kernel (byte offsets into its dynamic LDS): per slot
+// the FP64 COM components, six 64-bit accumulators, the molecule id and
+// (stage != 0) the FP32 site offsets; then per wave a ring of pair entries and
+// a buffer for the next member's row.
+struct TileLayout {
+  int nbmax, cx, cy, cz, acc, id, site, ring, rowbuf, order, bytes, stage;
+};
+TileLayout tile_layout(int nq, int nbmax, int cap, int waves, bool stage);
+
 struct System {
   int model = 0, n = 0, ns = 0, prec = 0;
   // fast: the reference's @FAST (custom kernels for TIP4P and ST2); kmodel:
@@ -177,6 +200,13 @@ struct System {
   bool nose_active = false;
   double nose_h[9] = {0};
   bool andersen = false;  // md.box and md.anders are allocated
+  bool tiles = false;     // the MD path runs the cell-tile force kernel
+  bool tile_ok = false;   // tile buffers allocated (the system qualifies)
+  int tile_waves = 8;     // waves per tile workgroup
+  int tile_ndiv = 0, tile_cap = 0;  // the tile grid and its cell capacity
+  TileLayout tile_lay{};
+  float *tile_part = nullptr;  // md.part while the tiles run; kept for freeing
+  double *snap = nullptr;      // the state at the start of a g6_step call
   // device buffers
   double *com = nullptr, *dx = nullptr, *w = nullptr, *quat = nullptr;
   double *tmat = nullptr, *intra = nullptr;
@@ -275,6 +305,13 @@ void md_launch_close(System &s, int mode, int parity, int src_parity);
 // k_md_main of this parity; close_mode != 0 closes the previous step
 // (partials of parity 1 - parity) in block 0
 void md_launch_main(System &s, double *out, int parity, int close_mode);
+// the cell-tile force kernel (g6_tile.hip) and the reduction of its partials
+// into force and body torque (out[6][n])
+void md_launch_tile(System &s, int parity, const CloseArgs &ca);
+void md_launch_part_reduce(System &s, double *out);
+int tile_set_lds_limit(int bytes);
+// switch a tile system back to the rows of both directions (g6_api.hip)
+int use_row_grid(System &s);
 // A ladder: several systems of one model and precision stepped together,
 // each MD kernel launched once for all of them (g6_ladder.hip).  The block
 // maps list every block of every system: one after another (flat, the

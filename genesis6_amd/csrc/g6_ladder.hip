@@ -284,6 +284,14 @@ int g6_ladder_create(g6_ladder *hl, int nsys, const g6_handle *hs) {
     }
     l->sys.push_back(s);
   }
+  // the ladder kernels run the rows of both directions
+  for (System *s : l->sys) {
+    const int rg = use_row_grid(*s);
+    if (rg != 0) {
+      delete l;
+      return rg;
+    }
+  }
   const int rc = ladder_setup(*l);
   if (rc != 0) {
     ladder_release(*l);

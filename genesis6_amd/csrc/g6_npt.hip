@@ -155,13 +155,25 @@ __device__ inline void member_move(const NptArgs &A, unsigned long long idx, int
   qr[3] = e2 * r;
 }
 
-// Address_Assign cell of molecule i (the MD binning: dnint on the wrapped
-// COM), grid ndiv^3 of box L
-__device__ inline int cell_of(const double *com, int n, int i, double L, double invL, int nd,
-                              double cellinv) {
+// Offset of the block grid of proposal idx, uniform in one cell per axis.
+// Cells are those of Address_Assign (GridModule.F90:503-552) shifted by it;
+// a random shift per move lets block moves carry molecules across every
+// fixed plane while each move keeps its members in their cell.
+__device__ inline void block_offset(const NptArgs &A, unsigned long long idx, double L, int nd,
+                                    double o[3]) {
+  uint32_t c[4] = {(uint32_t)idx, (uint32_t)(idx >> 32), 0xFFFFu, kTag};
+  philox10(c, A.key0, A.key1);
+  const double cell = L / nd;
+  for (int a = 0; a < 3; a++) o[a] = u01(c[a]) * cell;
+}
+
+// Address_Assign cell of a point (GridModule.F90:503-552)
+__device__ inline int cell_at(double x, double y, double z, double L, int nd) {
+  const double invL = 1.0 / L, cellinv = 1.0 / (L / nd);
+  const double c[3] = {x, y, z};
   int jc[3];
   for (int a = 0; a < 3; a++) {
-    double d = com[a * n + i];
+    double d = c[a];
     const double o = round(d * invL) * L;
     d = d - o;
     int j = (int)round(d * cellinv);
@@ -172,25 +184,33 @@ __device__ inline int cell_of(const double *com, int n, int i, double L, double 
   return (jc[2] * nd + jc[1]) * nd + jc[0];
 }
 
-// members of the block of proposal p in index order (one block of threads);
-// slot_out[i] = member slot or -1; returns the member count
-template <typename SlotT>
-__device__ int block_members(const NptArgs &A, const Prop &p, double L, int *members_out,
-                             SlotT *slot_out, int cap) {
-  __shared__ int s_wc[32], s_run;
-  const int n = A.n, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  const int nw = blockDim.x >> 6;
-  int nd = (int)(L / A.voxa);
+// the cell a block move picks (uniform over the voxa grid of box L)
+__device__ inline int block_cell(const NptArgs &A, const Prop &p, double L, int &nd) {
+  nd = (int)(L / A.voxa);
   if (nd < 1) nd = 1;
   const int ncell = nd * nd * nd;
   int c = (int)(p.a * (double)ncell);
   if (c > ncell - 1) c = ncell - 1;
-  const double invL = 1.0 / L, cellinv = 1.0 / (L / nd);
+  return c;
+}
+
+// members of the block of proposal p (cells shifted by o) in index order
+// (one block of threads); slot_out[i] = member slot or -1; returns the
+// member count
+template <typename SlotT>
+__device__ int block_members(const NptArgs &A, const Prop &p, double L, const double o[3],
+                             int *members_out, SlotT *slot_out, int cap) {
+  __shared__ int s_wc[32], s_run;
+  const int n = A.n, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int nw = blockDim.x >> 6;
+  int nd;
+  const int c = block_cell(A, p, L, nd);
   if (tid == 0) s_run = 0;
   __syncthreads();
   for (int base = 0; base < n; base += blockDim.x) {
     const int i = base + tid;
-    const bool in = i < n && cell_of(A.com, n, i, L, invL, nd, cellinv) == c;
+    const bool in = i < n && cell_at(A.com[i] - o[0], A.com[n + i] - o[1],
+                                     A.com[2 * n + i] - o[2], L, nd) == c;
     const unsigned long long m = __ballot(in);
     if (lane == 0) s_wc[wid] = __popcll(m);
     __syncthreads();
@@ -242,7 +262,8 @@ __global__ __launch_bounds__(1024) void k_npt_prep(NptArgs A, int da) {
     if (tid == 0) {
       A.st[2] = L * p.a;
       A.ist[0] = 0;
-      A.ist[3] = 0;
+      // one minimum image per pair needs rc < L'/2: a smaller box is void
+      A.ist[3] = 2.0 * A.rc >= L * p.a ? 1 : 0;
       A.ist[4] = 1;
     }
     for (int i = tid; i < n; i += blockDim.x) {
@@ -255,25 +276,33 @@ __global__ __launch_bounds__(1024) void k_npt_prep(NptArgs A, int da) {
     }
     return;
   }
-  const int cnt = block_members(A, p, L, A.members, A.slot, kNptMaxBlock);
-  if (tid == 0) {
-    A.ist[0] = cnt;
-    A.ist[3] = cnt > kNptMaxBlock ? 1 : 0;
-    A.ist[4] = 0;
-  }
-  if (cnt > kNptMaxBlock) return;
-  for (int k = tid; k < cnt; k += blockDim.x) {
+  int nd;
+  const int cell = block_cell(A, p, L, nd);
+  double o[3];
+  block_offset(A, (unsigned long long)idx, L, nd, o);
+  const int cnt = block_members(A, p, L, o, A.members, A.slot, kNptMaxBlock);
+  // a member moved out of the cell makes the move void: the reverse move
+  // would pick another member set, so q(x'->x) = 0 while q(x->x') > 0
+  int out = 0;
+  for (int k = tid; cnt <= kNptMaxBlock && k < cnt; k += blockDim.x) {
     const int i = A.members[k];
     double d[3], qr[4], q[4];
     member_move(A, (unsigned long long)idx, k, d, qr);
     turn(qr, A.quat[i], A.quat[n + i], A.quat[2 * n + i], A.quat[3 * n + i], q);
     const double cx = A.com[i] + d[0], cy = A.com[n + i] + d[1], cz = A.com[2 * n + i] + d[2];
+    if (cell_at(cx - o[0], cy - o[1], cz - o[2], L, nd) != cell) out = 1;
     A.bc[k * 3] = cx;
     A.bc[k * 3 + 1] = cy;
     A.bc[k * 3 + 2] = cz;
     for (int a = 0; a < 4; a++) A.bq[k * 4 + a] = q[a];
     place_sites(A, q[0], q[1], q[2], q[3], cx, cy, cz, A.bsx + (size_t)k * ns,
                 A.bsy + (size_t)k * ns, A.bsz + (size_t)k * ns);
+  }
+  const int any_out = __syncthreads_or(out);
+  if (tid == 0) {
+    A.ist[0] = cnt;
+    A.ist[3] = (cnt > kNptMaxBlock || any_out) ? 1 : 0;
+    A.ist[4] = 0;
   }
 }
 
@@ -427,7 +456,7 @@ __global__ __launch_bounds__(256) void k_npt_decide(NptArgs A, int da) {
   const Prop p = make_prop(A, (unsigned long long)idx);
   const int n = A.n, ns = A.ns, tid = threadIdx.x;
   const bool vol = A.ist[4] != 0;
-  const bool bad = !vol && A.ist[3];
+  const bool bad = A.ist[3] != 0;  // void: rejected without a test
   // volume: the new energy (row sums); block: dU (member-range partials)
   const double sum = tree_sum256(A.part, vol ? n : (bad ? 0 : A.ist[0] * kNptSub));
   const double up = sum;
@@ -571,17 +600,33 @@ __global__ __launch_bounds__(1024) void k_da_prep(NptArgs A, int current) {
     }
     if (tid == 0) {
       A.fL[k] = (float)(L * p.a);
-      A.vflag[k] = 0;
+      A.vflag[k] = 2.0 * A.rc >= L * p.a ? 1 : 0;  // void as in k_npt_prep
     }
     all_rows(A, k);
     return;
   }
-  const int cnt = block_members(A, p, L, s_mem, s_slot, kNptMaxBlock);
+  int nd;
+  const int cell = block_cell(A, p, L, nd);
+  double o[3];
+  block_offset(A, idx, L, nd, o);
+  const int cnt = block_members(A, p, L, o, s_mem, s_slot, kNptMaxBlock);
+  // void as in k_npt_prep: too many members, or a member leaving the cell
+  int out = 0;
+  for (int m = tid; cnt <= kNptMaxBlock && m < cnt; m += blockDim.x) {
+    const int i = s_mem[m];
+    double d[3], qr[4];
+    member_move(A, idx, m, d, qr);
+    if (cell_at(A.com[i] + d[0] - o[0], A.com[n + i] + d[1] - o[1],
+                A.com[2 * n + i] + d[2] - o[2], L, nd) != cell)
+      out = 1;
+  }
+  const int any_out = __syncthreads_or(out);
+  const bool bad = cnt > kNptMaxBlock || any_out;
   if (tid == 0) {
     A.fL[k] = (float)L;
-    A.vflag[k] = cnt > kNptMaxBlock ? 1 : 0;
+    A.vflag[k] = bad ? 1 : 0;
   }
-  if (cnt > kNptMaxBlock) {
+  if (bad) {
     if (tid == 0) A.naff[k] = 0;
     return;
   }
@@ -931,6 +976,7 @@ struct Npt {
   int graph_moves = 0, graph_mode = -1, graph_k = 0;
   int64_t graph_mlp = 0;
   std::vector<void *> allocs;
+  double o_move = 0;  // largest O displacement of a block move (A)
 };
 
 static std::mutex g_nmu;
@@ -1146,6 +1192,10 @@ int g6_npt_begin(g6_handle h, double temp, double press_bar, uint64_t seed, doub
   a.rot_e = tan(0.5 * rot_max_deg * 3.14159265358979323846 / 180.0) / sqrt(3.0);
   a.key0 = (uint32_t)seed;
   a.key1 = (uint32_t)(seed >> 32);
+  // an O moves by the displacement plus its turn about the COM, 2 rho sin(theta/2)
+  const double rho = sqrt(s->mt.bx[0] * s->mt.bx[0] + s->mt.by[0] * s->mt.by[0] +
+                          s->mt.bz[0] * s->mt.bz[0]);
+  p.o_move = disp_max + 2.0 * rho * sin(0.5 * rot_max_deg * 3.14159265358979323846 / 180.0);
   if (reuse) {
     const NptArgs fresh = a;
     a = keep;  // every device pointer and the surrogate buffers
@@ -1226,6 +1276,13 @@ int g6_npt_run(g6_handle h, int count, int64_t hm, int K, long long *stats) {
   if (hm) {
     net = mlp_of(hm);
     if (!net || K < 1 || K > kNptMaxK) return G6_EARG;
+    // The surrogate is a function of the configuration only while every O-O
+    // pair within kDescRc after a proposal is on the round's list of pairs
+    // within kNblR before it: a pair closes by at most twice the largest O
+    // move of a block, and a volume move scales it by exp(-dlnv/3) at least.
+    if (2.0 * p->o_move >= kNblR - (double)kDescRc ||
+        kNblR * exp(-p->a.dlnv / 3.0) <= (double)kDescRc)
+      return G6_EARG;
     G6_CHECK(hipStreamSynchronize(net->stream));
     p->mlp = hm;
   }

@@ -216,15 +216,23 @@ __device__ inline R pair_on_i(const PairK<R> &k, R dd, R d0, R d1, R d2,
                               const R (&rj)[Traits<M>::NS][3], R (&F)[3], R (&Tq)[3]) {
   using T = Traits<M>;
   constexpr int NQ = T::COM;  // sites 0..NQ-1 carry offsets
-  // switch on the COM distance (cutoff_func, CutoffModule.F90:38-59)
+  // switch on the COM distance (cutoff_func, CutoffModule.F90:38-59); FP32
+  // pairs take r and 1/r from one reciprocal square root
   R hh = R(0), dh = R(0);
   if (dd < k.rout2) {
     if (dd > k.rin2) {
-      const R dr = sqrt(dd);
+      R dr, rinv;
+      if constexpr (sizeof(R) == 4) {
+        rinv = g6_rsqrt(dd);
+        dr = dd * rinv;
+      } else {
+        dr = sqrt(dd);
+        rinv = R(1) / dr;
+      }
       const R u = dr - k.rout, l = dr - k.rin;
       const R uu = u * u, ll = l * l;
       hh = k.c1 * uu * u * (R(10) * ll - R(5) * u * l + uu);
-      dh = -(k.c2 * uu * ll / dr);
+      dh = -(k.c2 * uu * ll * rinv);
     } else {
       hh = R(1);
     }
@@ -307,9 +315,16 @@ __device__ inline R pair_on_i(const PairK<R> &k, R dd, R d0, R d1, R d2,
       if (dd < k.rl2) {
         s12 = R(0);
       } else {
-        const R r12 = sqrt(dd);
+        R r12, rinv;
+        if constexpr (sizeof(R) == 4) {
+          rinv = g6_rsqrt(dd);
+          r12 = dd * rinv;
+        } else {
+          r12 = sqrt(dd);
+          rinv = R(1) / r12;
+        }
         s12 = (r12 - k.rl) * (r12 - k.rl) * (R(3) * k.ru - k.rl - R(2) * r12) * k.w3inv;
-        ds12 = -(R(6) * (r12 - k.rl) * (k.ru - r12) * k.w3inv) / r12;
+        ds12 = -(R(6) * (r12 - k.rl) * (k.ru - r12) * k.w3inv) * rinv;
       }
     }
     hc = hh * s12;

@@ -281,6 +281,12 @@ module g6gpu
        real(c_double), intent(out) :: ms
      end function g6_time_force
 
+     integer(c_int) function g6_md_kernel(h, info) bind(C, name="g6_md_kernel")
+       import :: c_int, c_int64_t, c_int32_t
+       integer(c_int64_t), value :: h
+       integer(c_int32_t), intent(out) :: info(5)
+     end function g6_md_kernel
+
      integer(c_int) function g6_mc_begin(h, temp, seed, delta, rtratio) bind(C, name="g6_mc_begin")
        import :: c_int, c_int64_t, c_int32_t, c_double
        integer(c_int64_t), value :: h

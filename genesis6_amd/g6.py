@@ -73,6 +73,7 @@ def lib():
                                                C.POINTER(_d)]
         L.g6_set_fast.argtypes = [_H, C.c_int]
         L.g6_time_force.argtypes = [_H, C.c_int, C.POINTER(C.c_double)]
+        L.g6_md_kernel.argtypes = [_H, _ip]
         L.g6_set_andersen.argtypes = [_H, C.c_int, C.c_double, C.c_double, C.c_void_p]
         L.g6_get_andersen.argtypes = [_H, _dp]
         L.g6_get_box.argtypes = [_H, C.POINTER(C.c_double)]
@@ -224,6 +225,13 @@ class System:
         L = C.c_double(0)
         check(lib().g6_get_box(self.h, C.byref(L)), "g6_get_box")
         return L.value
+
+    def md_kernel(self):
+        """(tiles, cells per axis, workgroups per tile, waves, slots) of the
+        MD force kernel (g6_md_kernel)."""
+        info = np.zeros(5, np.int32)
+        check(lib().g6_md_kernel(self.h, info), "g6_md_kernel")
+        return tuple(int(x) for x in info)
 
     def time_force(self, nlaunch=200):
         """Mean ms per launch of the force kernel alone (g6_time_force)."""

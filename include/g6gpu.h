@@ -405,6 +405,15 @@ int g6_step_timed(g6_handle h, int nsteps, double *t);
  * events on the library stream; *ms = mean milliseconds per launch.  The
  * state is not moved.  Not with the barostat. */
 int g6_time_force(g6_handle h, int nlaunch, double *ms);
+/* The force kernel a system's MD path runs (no counterpart in the reference,
+ * whose force routine has one form): info[0] = 1 for the cell-tile kernel
+ * that evaluates each pair once, 0 for the rows of both directions;
+ * info[1] = tile cells per axis, info[2] = workgroups per tile, info[3] =
+ * waves per workgroup, info[4] = neighbourhood slots per tile (zeros for the
+ * rows).  FP32 fixed-box systems with at least 256 cells of edge rc + skin
+ * take the tiles; G6_TILES=0 in the environment at g6_init keeps the rows and
+ * G6_TILE_MIN sets the cell threshold. */
+int g6_md_kernel(g6_handle h, int32_t *info);
 int g6_model_nsite(int model);
 
 #ifdef __cplusplus

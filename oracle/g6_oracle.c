@@ -1298,6 +1298,16 @@ static npt_prop npt_make_prop(const npt_cfg *g, unsigned long long idx) {
 }
 
 /* new COM and quaternion of a block member in slot k */
+/* offset of the block grid of proposal idx: uniform in one cell per axis
+ * (Philox slot 0xFFFF); a block takes the molecules of one cell of the
+ * Address_Assign grid shifted by it */
+static void npt_offset(const npt_cfg *g, unsigned long long idx, double L, int nd, double *o) {
+  uint32_t c[4] = {(uint32_t)idx, (uint32_t)(idx >> 32), 0xFFFFu, 0x4E505400u};
+  g6o_philox10(c, g->k0, g->k1);
+  const double cell = L / nd;
+  for (int a = 0; a < 3; a++) o[a] = npt_u01(c[a]) * cell;
+}
+
 static void npt_member(const npt_cfg *g, unsigned long long idx, int k, const double *c0,
                        const double *q0, double *c1, double *q1) {
   uint32_t c[4] = {(uint32_t)idx, (uint32_t)(idx >> 32), (uint32_t)(1 + 2 * k), 0x4E505400u};
@@ -1372,7 +1382,8 @@ static double npt_pair(const g6o_model *m, const stdterm *st, int nt, const doub
 /* nmoves plain NPT moves from proposal index *index.  com[n][3], quat[4][n]
  * (order 1), *L and *U (the PairInfo sum) are updated; accepted[nmoves];
  * counts[5] += proposals, accepted, volume proposals, volume accepted,
- * void block moves (more than 96 molecules). */
+ * void proposals (rejected untested: a block of more than 96 molecules, a
+ * block move taking a member out of its cell, a box edge below 2 rc). */
 void g6o_npt_run(int model, int n, double rc, double taper, double voxa, double temp,
                  double press_bar, uint64_t seed, double disp_max, double rot_max_deg,
                  double dlnv_max, double p_volume, double *L, double *com, double *quat,
@@ -1410,7 +1421,11 @@ void g6o_npt_run(int model, int n, double rc, double taper, double voxa, double 
     const npt_prop p = npt_make_prop(&g, idx);
     int acc = 0;
     counts[0]++;
-    if (p.type) {
+    if (p.type && 2.0 * rc >= *L * p.a) {
+      /* void: one minimum image per pair needs rc < L'/2 */
+      counts[2]++;
+      counts[4]++;
+    } else if (p.type) {
       counts[2]++;
       const double Lp = *L * p.a;
       for (int k = 0; k < 3 * n; k++) com2[k] = com[k] * p.a;
@@ -1444,15 +1459,32 @@ void g6o_npt_run(int model, int n, double rc, double taper, double voxa, double 
       const int ncell = nd * nd * nd;
       int cc = (int)(p.a * (double)ncell);
       if (cc > ncell - 1) cc = ncell - 1;
+      double off[3];
+      npt_offset(&g, idx, *L, nd, off);
       int cnt = 0;
       for (int i = 0; i < n; i++) {
         slot[i] = -1;
-        if (npt_cell(com + 3 * i, *L, nd) == cc) {
+        const double sh[3] = {com[3 * i] - off[0], com[3 * i + 1] - off[1], com[3 * i + 2] - off[2]};
+        if (npt_cell(sh, *L, nd) == cc) {
           slot[i] = cnt;
           mem[cnt++] = i;
         }
       }
-      if (cnt > 96) {
+      /* void (rejected, counted): more than 96 members, or a member that the
+       * move takes out of the cell, for which the reverse move would pick
+       * another member set */
+      int out = 0;
+      if (cnt <= 96) {
+        for (int k = 0; k < cnt; k++) {
+          double q1[4], c1[3];
+          const int i = mem[k];
+          for (int a = 0; a < 4; a++) q[a] = quat[a * n + i];
+          npt_member(&g, idx, k, com + 3 * i, q, c1, q1);
+          for (int a = 0; a < 3; a++) c1[a] -= off[a];
+          if (npt_cell(c1, *L, nd) != cc) out = 1;
+        }
+      }
+      if (cnt > 96 || out) {
         counts[4]++;
       } else {
         memcpy(com2, com, sizeof(double) * 3 * n);

@@ -49,3 +49,30 @@ def test_incremental_energy_equals_a_fresh_sum():
     _, fresh = O.mc_pairinfo(O.TIP4P, run.com, run.quat, run.L, 5.5, 0.5)
     assert acc.sum() > 0 and run.counts[2] > 0
     assert abs(run.U - fresh) < 1e-9 * abs(fresh)
+
+
+def test_block_moves_alone_sample_the_ideal_gas():
+    """Detailed balance and ergodicity of the block move.  Without
+    interactions every molecule position is equally likely, so the fraction
+    of molecule pairs sharing a cell of any fixed grid is 1/ncell.  A block
+    move whose member may leave the cell has no reverse move (it picks a
+    different member set) and drifts above it (0.141 for 1/8 here); a block
+    grid that never shifts keeps the start's occupancy for ever.  The chain
+    starts with every molecule in one cell."""
+    n, L, voxa, nd = 16, 8.0, 4.0, 2
+    rng = np.random.default_rng(3)
+    com = rng.uniform(-1.5, 1.5, (n, 3))
+    quat = np.zeros((4, n)); quat[0] = 1.0
+    run = O.NptRun(O.TIP4P, com, quat, L, 0.01, 0.005, voxa, 300.0, 1.0, 7, 1.6, 5.0,
+                   0.0, 0.0, energy=0.0)
+    frac = []
+    for _ in range(1500):
+        run.run(50)
+        c = run.com - L * np.round(run.com / L)
+        j = np.round(c / (L / nd)).astype(int) % nd
+        cnt = np.bincount((j[:, 2] * nd + j[:, 1]) * nd + j[:, 0], minlength=nd ** 3)
+        frac.append((cnt * (cnt - 1)).sum() / (n * (n - 1)))
+    frac = np.array(frac[200:])
+    err = frac.reshape(13, -1).mean(1).std() / np.sqrt(13)  # batch means
+    assert run.counts[1] > 0 and run.counts[4] > 0  # accepted and void block moves
+    assert abs(frac.mean() - 1.0 / nd ** 3) < max(4 * err, 0.006), (frac.mean(), err)
