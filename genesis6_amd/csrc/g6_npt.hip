@@ -1294,7 +1294,10 @@ int g6_npt_run(g6_handle h, int count, int64_t hm, int K, long long *stats) {
   G6_CHECK(hipMemcpy(&derr, p->a.derr, sizeof(int), hipMemcpyDeviceToHost));
   if (stats)
     for (int k = 0; k < 9; k++) stats[k] = c[k];
-  if (derr || c[7] > 0) return G6_ECAPACITY;
+  // void proposals (a block of more than kNptMaxBlock members, a member
+  // leaving its cell, a box below 2 rc) are rejections: the reverse of a void
+  // move is void too.  A descriptor neighbour list that overflowed is an error.
+  if (derr) return G6_ECAPACITY;
   return 0;
 }
 

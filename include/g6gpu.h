@@ -335,21 +335,28 @@ int g6_rx_exchange(g6_rx hx, const double *thermo, int32_t *naccept);
 /* Starts an NPT run on the handle's current configuration (COMs and order-1
  * quaternions of the MD state, box L of g6_init): temp (K), press_bar,
  * seed of the proposal stream (Philox4x32-10), block moves of every molecule
- * of one Address_Assign cell (ndiv = int(L/voxa)) displaced within a cube
- * of half-diagonal disp_max (A) and turned by at most rot_max_deg, volume
- * moves ln V +- dlnv_max (<= 0.03) with probability p_volume.  n <= 4096,
- * TIP4P or TIP5P, rc < L (1 - dlnv_max) / 2. */
+ * of one cell of the Address_Assign grid (ndiv = int(L/voxa)) shifted by a
+ * random offset per move, displaced within a cube of half-diagonal disp_max
+ * (A) and turned by at most rot_max_deg, volume moves ln V +- dlnv_max
+ * (<= 0.03) with probability p_volume.  A block move that takes a member out
+ * of the shifted cell, a block of more than 96 molecules and a volume move
+ * to a box edge below 2 rc are void: rejected untested (the reverse of a
+ * void move is void, so detailed balance holds).  n <= 4096, TIP4P or TIP5P,
+ * rc < L (1 - dlnv_max) / 2 at the start. */
 int g6_npt_begin(g6_handle h, double temp, double press_bar, uint64_t seed, double disp_max,
                  double rot_max_deg, double dlnv_max, double p_volume, double voxa);
 /* hm = 0: count plain moves, each decided on the exact FP64 energy.
  * hm = a g6_mlp: count delayed-acceptance rounds of K (1..64) proposals
  * screened by the network (per-molecule energies of 32 features within
- * 6 A), the first that passes decided on the exact energy.
+ * 6 A), the first that passes decided on the exact energy.  The surrogate
+ * is a function of the configuration only while no O-O pair can come within
+ * 6 A from beyond the round's 6.5 A list: G6_EARG if twice the largest O
+ * move of a block (disp_max plus the turn of the O about the COM) reaches
+ * 0.5 A.
  * stats[9] (cumulative): proposals, accepted, volume proposals, volume
- * accepted, surrogate passes, exact evaluations, rounds, void proposals
- * (a block of more than 96 molecules: rejected), next proposal index.
- * G6_ECAPACITY if a proposal was void or a molecule had more than 128
- * neighbours within 6 A. */
+ * accepted, surrogate passes, exact evaluations, rounds, void proposals,
+ * next proposal index.  G6_ECAPACITY if a molecule had more than 96
+ * neighbours within 6.5 A for the descriptors. */
 int g6_npt_run(g6_handle h, int count, int64_t hm, int K, long long *stats);
 /* com[3n], quat1[4n], box edge, PairInfo energy sum (internal units) */
 int g6_npt_get(g6_handle h, double *com, double *quat1, double *box_l, double *energy);
