@@ -627,6 +627,8 @@ int g6_model_nsite(int model) {
   return make_model_table(model, 0, &m) ? m.nsite : 0;
 }
 
+static int scratch_address(void **p);
+
 int g6_init(g6_handle *h, int model, int n, double box_l, double rc, double taper,
             double voxa, double dt, int prec, double skin) {
   if (!h || n < 2 || !(box_l > 0) || !(rc > 0) || !(taper >= 0) || taper > rc || !(dt > 0) ||
@@ -711,6 +713,11 @@ int g6_init(g6_handle *h, int model, int n, double box_l, double rc, double tape
   for (int k = 0; rcode == 0 && k < 8; k++)
     if (hipEventCreate(&s->ev[k]) != hipSuccess) rcode = G6_EHIP;
   if (rcode == 0) rcode = reset_step_state(*s);
+  if (rcode == 0) {
+    // the no-allocation entry point's device array, on this device
+    void *scr = nullptr;
+    rcode = scratch_address(&scr);
+  }
   if (rcode != 0) {
     free_all(*s);
     delete s;
@@ -998,35 +1005,83 @@ int g6_md_kernel(g6_handle h, int32_t *info) {
   return 0;
 }
 
+// Forces through the device buffer out[6][n] into the host arrays.  The
+// rows are rebuilt from the current positions; the state is not moved and the
+// thermostat is not advanced.  With fallback, a tile system that runs out of
+// a capacity continues on the rows of both directions.
+static int eval_forces_into(System *s, double *out, double *force, double *torque,
+                            double *thermo, bool fallback) {
+  const int n = s->n;
+  for (int attempt = 0;; attempt++) {
+    md_launch_flag(*s, 2, 1);  // rebuild the rows of parity 0
+    md_launch_pre(*s, false, false, true, 0);
+    md_launch_main(*s, out, 0, 0);
+    md_launch_close(*s, 16 | 8, 0, 0);
+    G6_CHECK(hipGetLastError());
+    G6_CHECK(hipMemcpyAsync(force, out, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s->stream));
+    G6_CHECK(hipMemcpyAsync(torque, out + 3 * n, sizeof(double) * 3 * n, hipMemcpyDeviceToHost,
+                            s->stream));
+    G6_CHECK(hipStreamSynchronize(s->stream));
+    const int rc = check_flags(*s);
+    if (rc != 0 && rc != G6_EHIP && s->tiles && fallback && attempt == 0) {
+      const int rg = use_row_grid(*s);
+      if (rg != 0) return rg;
+      continue;
+    }
+    if (rc != 0) return rc;
+    if (thermo)
+      G6_CHECK(hipMemcpy(thermo, s->thermo, sizeof(double) * G6_THERMO_LEN, hipMemcpyDeviceToHost));
+    return 0;
+  }
+}
+
 int g6_eval_forces(g6_handle h, double *force, double *torque, double *thermo) {
   System *s = get(h);
   if (!s || !force || !torque) return G6_EARG;
-  const int n = s->n;
   double *out = nullptr;
-  G6_CHECK(hipMalloc(&out, sizeof(double) * 6 * n));
-  // rows are rebuilt from the current positions; the state is not moved and
-  // the thermostat is not advanced
-  int one = 1;
-  G6_CHECK(hipMemcpyAsync(s->md.flags + 2, &one, sizeof(int), hipMemcpyHostToDevice, s->stream));
-  md_launch_pre(*s, false, false, true, 0);
-  md_launch_main(*s, out, 0, 0);
-  md_launch_close(*s, 16 | 8, 0, 0);
-  G6_CHECK(hipGetLastError());
-  G6_CHECK(hipMemcpyAsync(force, out, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s->stream));
-  G6_CHECK(hipMemcpyAsync(torque, out + 3 * n, sizeof(double) * 3 * n, hipMemcpyDeviceToHost,
-                          s->stream));
-  G6_CHECK(hipStreamSynchronize(s->stream));
-  G6_CHECK(hipFree(out));
-  const int rc = check_flags(*s);
-  if (rc != 0 && rc != G6_EHIP && s->tiles) {
-    // a tile neighbourhood or row ran out of room: the rows of both
-    // directions from here on
-    const int rg = use_row_grid(*s);
-    return rg != 0 ? rg : g6_eval_forces(h, force, torque, thermo);
-  }
-  if (thermo) G6_CHECK(hipMemcpy(thermo, s->thermo, sizeof(double) * G6_THERMO_LEN,
-                                 hipMemcpyDeviceToHost));
+  G6_CHECK(hipMalloc(&out, sizeof(double) * 6 * s->n));
+  const int rc = eval_forces_into(s, out, force, torque, thermo, true);
+  (void)hipFree(out);
   return rc;
+}
+
+// ---- the entry point that allocates nothing -------------------------------
+// g6_eval_forces_noalloc stages force and torque (6 n doubles) in this
+// file-scope device array instead of a hipMalloc'd buffer.  Every GPU the
+// library runs on holds its own copy until the process ends; host code uses
+// the address hipGetSymbolAddress gives on each device, never the array's
+// name.  Calls that overlap on one GPU share it and race.
+constexpr size_t kScratchBytes = (size_t)16 << 20;  // 6 n doubles for n <= 349,525
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scr_mu;
+static void *g_scr_ptr[kMaxDevices];  // scratch's device address, per device
+
+// the address of scratch on the current device, resolved once per device
+static int scratch_address(void **p) {
+  int dev = 0;
+  G6_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return G6_EARG;
+  std::lock_guard<std::mutex> lk(g_scr_mu);
+  if (!g_scr_ptr[dev]) {
+    void *q = nullptr;
+    if (hipGetSymbolAddress(&q, HIP_SYMBOL(scratch)) != hipSuccess || !q) return G6_EHIP;
+    g_scr_ptr[dev] = q;
+  }
+  *p = g_scr_ptr[dev];
+  return 0;
+}
+
+int g6_eval_forces_noalloc(g6_handle h, double *force, double *torque, double *thermo) {
+  System *s = get(h);
+  if (!s || !force || !torque) return G6_EARG;
+  if (sizeof(double) * 6 * (size_t)s->n > kScratchBytes) return G6_ECAPACITY;
+  void *p = nullptr;
+  const int rc = scratch_address(&p);
+  if (rc != 0) return rc;
+  // every element the kernels read from the buffer is written first in the
+  // same call, so the last call's contents need no clearing
+  return eval_forces_into(s, static_cast<double *>(p), force, torque, thermo, false);
 }
 
 int g6_set_andersen(g6_handle h, int mode, double pex, double mass, const double *v) {

@@ -300,6 +300,8 @@ constexpr int kRed = 12 + kEkLen;
 // one kinetic partial
 inline int pre_waves(int n) { return (n + 63) / 64 * 4; }
 void md_launch_pre(System &s, bool correct, bool predict, bool sites, int parity);
+// flags[k] = v, stream-ordered
+void md_launch_flag(System &s, int k, int v);
 // close of the step whose k_md_main wrote the partials of src_parity
 void md_launch_close(System &s, int mode, int parity, int src_parity);
 // k_md_main of this parity; close_mode != 0 closes the previous step

@@ -850,6 +850,12 @@ __global__ __launch_bounds__(256, (main_waves_per_eu<M, R>())) void k_md_main_la
   md_main_body<M, R, false>(mt, g, b, pk, nullptr, parity, ca, reg_bytes, m.y, nm);
 }
 
+__global__ void k_flag_set(int *flags, int k, int v) {
+  if (threadIdx.x == 0) flags[k] = v;
+}
+
+void md_launch_flag(System &s, int k, int v) { k_flag_set<<<1, 64, 0, s.stream>>>(s.md.flags, k, v); }
+
 void md_launch_pre(System &s, bool correct, bool predict, bool sites, int parity) {
   if (s.andersen)
     k_md_pre<true><<<pre_waves(s.n) / 4, 256, 0, s.stream>>>(

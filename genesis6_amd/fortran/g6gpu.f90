@@ -89,6 +89,13 @@ module g6gpu
        real(c_double), intent(out) :: force(*), torque(*), thermo(*)
      end function g6_eval_forces
 
+     integer(c_int) function g6_eval_forces_noalloc(h, force, torque, thermo) &
+          bind(C, name="g6_eval_forces_noalloc")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), value :: h
+       real(c_double), intent(out) :: force(*), torque(*), thermo(*)
+     end function g6_eval_forces_noalloc
+
      integer(c_int) function g6_pairlist(h, com, cap, npair0, npair, pair_i, pair_j, ox, oy, oz) &
           bind(C, name="g6_pairlist")
        import :: c_int, c_int64_t, c_int32_t, c_double

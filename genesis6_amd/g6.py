@@ -51,6 +51,7 @@ def lib():
         L.g6_step_timed.argtypes = [_H, C.c_int, _dp]
         L.g6_get_thermo.argtypes = [_H, _dp]
         L.g6_eval_forces.argtypes = [_H, _dp, _dp, _dp]
+        L.g6_eval_forces_noalloc.argtypes = [_H, _dp, _dp, _dp]
         L.g6_pairlist.argtypes = [_H, _dp, C.c_int64, _i64p, _i64p, _ip, _ip, _dp, _dp, _dp]
         L.g6_cells.argtypes = [_H, _dp, _ip, _ip]
         L.g6_cutoff.argtypes = [_H, C.c_int64, _ip, _ip, _dp, _dp, _dp, _dp, _dp, _dp]
@@ -313,6 +314,13 @@ class System:
         n = self.n
         F = np.zeros(3 * n); T = np.zeros(3 * n); th = np.zeros(THERMO_LEN)
         check(lib().g6_eval_forces(self.h, F, T, th), "g6_eval_forces")
+        return F.reshape(3, n).T.copy(), T.reshape(3, n).T.copy(), th
+
+    def eval_forces_noalloc(self):
+        """eval_forces through g6_eval_forces_noalloc (no device allocation)."""
+        n = self.n
+        F = np.zeros(3 * n); T = np.zeros(3 * n); th = np.zeros(THERMO_LEN)
+        check(lib().g6_eval_forces_noalloc(self.h, F, T, th), "g6_eval_forces_noalloc")
         return F.reshape(3, n).T.copy(), T.reshape(3, n).T.copy(), th
 
     def thermo(self):

@@ -100,6 +100,15 @@ int g6_get_thermo(g6_handle h, double *thermo);
  * (RigidModule.F90:275-317).  force[3n] lab frame, torque[3n] body frame;
  * thermo as g6_get_thermo with ep, virial and pair counts of this state. */
 int g6_eval_forces(g6_handle h, double *force, double *torque, double *thermo);
+/* g6_eval_forces for callers that cannot allocate (no counterpart in the
+ * reference): the same kernels and results, with the device staging buffer
+ * of force and torque taken from a fixed device array of 16 MiB that each
+ * GPU holds for the life of the process, so the call makes no hipMalloc or
+ * hipFree.  n is limited to 349,525 molecules (6 n doubles): larger systems
+ * return G6_ECAPACITY.  The array is shared by every call on a GPU: calls
+ * that overlap on one GPU race.  A tile system that runs out of a capacity
+ * returns the error instead of falling back to the rows. */
+int g6_eval_forces_noalloc(g6_handle h, double *force, double *torque, double *thermo);
 
 /* ---- the reference's module entry points, one call each --------------- */
 /* Grid_Update + Address_Assign + Grid_NeighborList + Interaction_Compress2
