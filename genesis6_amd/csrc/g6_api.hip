@@ -556,10 +556,7 @@ int use_row_grid(System &s) {
   if (!s.tiles) return 0;
   G6_CHECK(hipStreamSynchronize(s.stream));
   set_row_grid(s);
-  if (s.graph) {
-    (void)hipGraphExecDestroy(s.graph);
-    s.graph = nullptr;
-  }
+  drop_graphs(s);
   return reset_step_state(s);
 }
 
@@ -574,7 +571,7 @@ static void free_all(System &s) {
   for (void *p : ps)
     if (p) (void)hipFree(p);
   mc_free(s);
-  if (s.graph) (void)hipGraphExecDestroy(s.graph);
+  drop_graphs(s);
   for (auto &e : s.ev) (void)hipEventDestroy(e);
   if (s.stream) (void)hipStreamDestroy(s.stream);
 }
@@ -607,6 +604,16 @@ static void launch_steps(System &s, int k) {
   }
   md_launch_pre(s, true, false, false, 0);
   md_launch_close(s, 1 | 8, 0, (k - 1) & 1);
+}
+
+void drop_graphs(System &s) {
+  if (s.graph) (void)hipGraphExecDestroy(s.graph);
+  s.graph = nullptr;
+  for (int q = 0; q < 4; q++) {
+    if (s.graph_k[q]) (void)hipGraphExecDestroy(s.graph_k[q]);
+    s.graph_k[q] = nullptr;
+    s.graph_k_steps[q] = 0;
+  }
 }
 
 // steps per captured hipGraph (G6_GRAPH_STEPS; 0 disables graphs)
@@ -774,10 +781,7 @@ int g6_set_nose(g6_handle h, int active, const double *nose) {
   if (nose) memcpy(z, nose, sizeof(z));
   G6_CHECK(hipStreamSynchronize(s->stream));
   G6_CHECK(hipMemcpy(s->nose, z, sizeof(z), hipMemcpyHostToDevice));
-  if (s->graph) {  // the captured launches embed the Nose flag
-    (void)hipGraphExecDestroy(s->graph);
-    s->graph = nullptr;
-  }
+  drop_graphs(*s);  // the captured launches embed the Nose flag
   return 0;
 }
 
@@ -792,10 +796,7 @@ int g6_set_fast(g6_handle h, int fast) {
   s->kmodel = kernel_model(s->model, s->fast);
   s->pkf = make_pair_k<float>(s->mt, s->geo_md);
   s->pkd = make_pair_k<double>(s->mt, s->geo_md);
-  if (s->graph) {  // the captured launches embed the pair constants
-    (void)hipGraphExecDestroy(s->graph);
-    s->graph = nullptr;
-  }
+  drop_graphs(*s);  // the captured launches embed the pair constants
   return 0;
 }
 
@@ -898,26 +899,51 @@ int g6_step(g6_handle h, int nsteps) {
   return rr != 0 ? rr : rc;
 }
 
+static int capture_steps(System &s, int k, hipGraphExec_t *g) {
+  hipGraph_t gr;
+  G6_CHECK(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
+  launch_steps(s, k);
+  G6_CHECK(hipStreamEndCapture(s.stream, &gr));
+  const hipError_t e = hipGraphInstantiate(g, gr, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(gr);
+  G6_CHECK(e);
+  return 0;
+}
+
+// G-step graphs replayed, then a graph of the remaining count (cached by
+// count, four at most), so that every call replays graphs
 static int step_once(System *s, int nsteps) {
   const int G = graph_steps();
-  if (G > 0 && nsteps >= G) {
-    if (!s->graph || s->graph_steps != G) {
-      if (s->graph) {
-        (void)hipGraphExecDestroy(s->graph);
+  if (G == 0) {
+    launch_steps(*s, nsteps);
+  } else {
+    if (nsteps >= G) {
+      if (!s->graph || s->graph_steps != G) {
+        if (s->graph) (void)hipGraphExecDestroy(s->graph);
         s->graph = nullptr;
+        const int rc = capture_steps(*s, G, &s->graph);
+        if (rc != 0) return rc;
+        s->graph_steps = G;
       }
-      hipGraph_t gr;
-      G6_CHECK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-      launch_steps(*s, G);
-      G6_CHECK(hipStreamEndCapture(s->stream, &gr));
-      G6_CHECK(hipGraphInstantiate(&s->graph, gr, nullptr, nullptr, 0));
-      G6_CHECK(hipGraphDestroy(gr));
-      s->graph_steps = G;
+      for (int k = 0; k + G <= nsteps; k += G) G6_CHECK(hipGraphLaunch(s->graph, s->stream));
     }
-    for (int k = 0; k + G <= nsteps; k += G) G6_CHECK(hipGraphLaunch(s->graph, s->stream));
+    const int rest = nsteps % G;
+    if (rest > 0) {
+      int slot = -1;
+      for (int q = 0; q < 4; q++)
+        if (s->graph_k[q] && s->graph_k_steps[q] == rest) slot = q;
+      if (slot < 0) {
+        slot = s->graph_k_next;
+        s->graph_k_next = (slot + 1) % 4;
+        if (s->graph_k[slot]) (void)hipGraphExecDestroy(s->graph_k[slot]);
+        s->graph_k[slot] = nullptr;
+        const int rc = capture_steps(*s, rest, &s->graph_k[slot]);
+        if (rc != 0) return rc;
+        s->graph_k_steps[slot] = rest;
+      }
+      G6_CHECK(hipGraphLaunch(s->graph_k[slot], s->stream));
+    }
   }
-  const int rest = G > 0 ? nsteps % G : nsteps;
-  if (rest > 0) launch_steps(*s, rest);
   G6_CHECK(hipGetLastError());
   s->steps_done += nsteps;
   return check_flags(*s);
@@ -1136,10 +1162,7 @@ int g6_set_andersen(g6_handle h, int mode, double pex, double mass, const double
     int one[2] = {1, 1};
     G6_CHECK(hipMemcpy(s->md.flags + 2, one, sizeof(one), hipMemcpyHostToDevice));
   }
-  if (s->graph) {  // the captured launches embed the step sequence
-    (void)hipGraphExecDestroy(s->graph);
-    s->graph = nullptr;
-  }
+  drop_graphs(*s);  // the captured launches embed the step sequence
   return 0;
 }
 

@@ -224,8 +224,12 @@ struct System {
   PairK<float> pkf{};
   PairK<double> pkd{};
   hipStream_t stream = nullptr;
-  hipGraphExec_t graph = nullptr;
+  hipGraphExec_t graph = nullptr;  // graph_steps steps
   int graph_steps = 0;
+  // graphs of shorter runs, by step count (g6_step of any count replays)
+  hipGraphExec_t graph_k[4] = {nullptr, nullptr, nullptr, nullptr};
+  int graph_k_steps[4] = {0, 0, 0, 0};
+  int graph_k_next = 0;
   hipEvent_t ev[8];
   double t_force = 0, t_list = 0, t_integ = 0, t_step = 0;
   int steps_done = 0;
@@ -314,6 +318,8 @@ void md_launch_part_reduce(System &s, double *out);
 int tile_set_lds_limit(int bytes);
 // switch a tile system back to the rows of both directions (g6_api.hip)
 int use_row_grid(System &s);
+// destroys the captured step graphs (their launches embed the arguments)
+void drop_graphs(System &s);
 // A ladder: several systems of one model and precision stepped together,
 // each MD kernel launched once for all of them (g6_ladder.hip).  The block
 // maps list every block of every system: one after another (flat, the

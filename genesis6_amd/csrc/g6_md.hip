@@ -45,6 +45,9 @@
 
 namespace g6 {
 
+__device__ inline float fma_r(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ inline double fma_r(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 // force (C0 = 0) or lab torque (C0 = 3) of molecule i from the tile
 // kernel's partials, summed in a fixed order (as k_part_reduce)
 template <int C0>
@@ -712,20 +715,24 @@ __device__ __forceinline__ void md_main_body(const ModelTable &mt, const Geom &g
     // pass 2: the interacting pairs, densely, two per lane per round
     R ri[T::NS][3];
     RecSites<M, R>::load(reci, ri);
-    double F[3] = {0, 0, 0}, Ti[3] = {0, 0, 0};
+    // each lane's own pairs (two or three in a liquid) are summed in the pair
+    // precision; the lane totals then go to FP64 for the wave and block sums
+    R Fr[3] = {R(0), R(0), R(0)}, Tr[3] = {R(0), R(0), R(0)}, er = R(0);
+    R vr[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) vr[q] = R(0);
     auto eval = [&](int j, R d0, R d1, R d2, const R (&rj)[T::NS][3]) {
       R Fp[3], Tp[3];
       const R e = pair_on_i<M, R>(pk, d0 * d0 + d1 * d1 + d2 * d2, d0, d1, d2, ri, rj, Fp, Tp);
-      F[0] += (double)Fp[0]; F[1] += (double)Fp[1]; F[2] += (double)Fp[2];  // FP64 sums
-      Ti[0] += (double)Tp[0]; Ti[1] += (double)Tp[1]; Ti[2] += (double)Tp[2];
-      ep += (double)e;
-      // pair virial d (x) F, fused multiply-adds in FP64
-      const double dd3[3] = {(double)d0, (double)d1, (double)d2};
-      const double ff3[3] = {(double)Fp[0], (double)Fp[1], (double)Fp[2]};
+      Fr[0] += Fp[0]; Fr[1] += Fp[1]; Fr[2] += Fp[2];
+      Tr[0] += Tp[0]; Tr[1] += Tp[1]; Tr[2] += Tp[2];
+      er += e;
+      // pair virial d (x) F, fused multiply-adds
+      const R dd3[3] = {d0, d1, d2};
 #pragma unroll
       for (int a = 0; a < 3; a++)
 #pragma unroll
-        for (int c = 0; c < 3; c++) v[3 * a + c] = __builtin_fma(dd3[a], ff3[c], v[3 * a + c]);
+        for (int c = 0; c < 3; c++) vr[3 * a + c] = fma_r(dd3[a], Fp[c], vr[3 * a + c]);
     };
     for (int k0 = 0; k0 < np; k0 += 128) {
       const int ka = k0 + lane, kb = k0 + 64 + lane;
@@ -736,10 +743,14 @@ __device__ __forceinline__ void md_main_body(const ModelTable &mt, const Geom &g
       if (ka < np) eval(ja, dv[ka], dv[dstride + ka], dv[2 * dstride + ka], ra);
       if (kb < np) eval(jb, dv[kb], dv[dstride + kb], dv[2 * dstride + kb], rb);
     }
+    double F[3], Ti[3];
+    ep = (double)er;
+#pragma unroll
+    for (int q = 0; q < 9; q++) v[q] = (double)vr[q];
 #pragma unroll
     for (int a = 0; a < 3; a++) {
-      F[a] = wsum63(F[a]);
-      Ti[a] = wsum63(Ti[a]);
+      F[a] = wsum63((double)Fr[a]);
+      Ti[a] = wsum63((double)Tr[a]);
     }
     if (lane == 63) {
       if (out) {
@@ -795,10 +806,14 @@ __device__ __forceinline__ void md_main_body(const ModelTable &mt, const Geom &g
   }
 }
 
-// FP32 TIP4P/TIP5P/SPC/E rows fit in 128 VGPRs: four waves per SIMD, so a 4096-molecule
-// grid (1024 blocks) is resident at once on 256 CUs
+// Waves per SIMD the row kernel is compiled for (register budget)
 template <int M, typename R>
 constexpr int main_waves_per_eu() {
+  // FP32 TIP4P and SPC/E rows fit in 96 VGPRs: five waves per SIMD, so that a
+  // 4096-molecule grid plus the step-close block (1025 blocks) is resident
+  // at once on 256 CUs (four waves per SIMD leave the last block waiting for
+  // a slot, which set the kernel's end)
+  if (sizeof(R) == 4 && (M == G6_TIP4P || M == G6_SPCE)) return 5;
   return (M != G6_ST2 && M != G6_NVDE && sizeof(R) == 4) ? 4 : 1;
 }
 

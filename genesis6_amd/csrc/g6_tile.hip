@@ -258,9 +258,11 @@ __device__ __forceinline__ void tile_body(const Geom &g, const MdBuf &b, const P
 
   // ---- 3. rows, interaction test, dense pair rounds ----
   const double Lb = g.L, invL = g.invL, rr = g.rr, rc2 = g.rc2;
-  double ep = 0.0, v[9];
+  // each lane's energy and virial over its own pairs (a few per step) in
+  // FP32; the lane totals go to FP64 for the compensated block sums
+  float ep = 0.0f, v[9];
 #pragma unroll
-  for (int q = 0; q < 9; q++) v[q] = 0.0;
+  for (int q = 0; q < 9; q++) v[q] = 0.0f;
   int npair = 0, nlist = 0;  // wave-uniform
   int wp = 0, rp = 0;        // ring write / read positions (wave-uniform)
   const unsigned long long lt = (1ull << lane) - 1ull;
@@ -318,13 +320,12 @@ __device__ __forceinline__ void tile_body(const Geom &g, const MdBuf &b, const P
         atomicAdd(&s_acc[a * nbmax + js], fix64(-F[a]));
         atomicAdd(&s_acc[(3 + a) * nbmax + js], fix64(tj[a]));
       }
-      ep += (double)e1;
-      const double dd3[3] = {(double)d0, (double)d1, (double)d2};
-      const double ff3[3] = {(double)F[0], (double)F[1], (double)F[2]};
+      ep += e1;
+      const float dd3[3] = {d0, d1, d2};
 #pragma unroll
       for (int a = 0; a < 3; a++)
 #pragma unroll
-        for (int c = 0; c < 3; c++) v[3 * a + c] = __builtin_fma(dd3[a], ff3[c], v[3 * a + c]);
+        for (int c = 0; c < 3; c++) v[3 * a + c] = __builtin_fmaf(dd3[a], F[c], v[3 * a + c]);
     }
     rp += 64;
   };
@@ -472,9 +473,9 @@ __device__ __forceinline__ void tile_body(const Geom &g, const MdBuf &b, const P
   // the 16 segments of q, one DPP row, are combined with compensated adds
   const int nl = 64 * W;
   double *red = reinterpret_cast<double *>(s_dyn + lay.acc);  // [10][64 W]
-  red[tid] = ep;
+  red[tid] = (double)ep;
 #pragma unroll
-  for (int q = 0; q < 9; q++) red[(1 + q) * nl + tid] = v[q];
+  for (int q = 0; q < 9; q++) red[(1 + q) * nl + tid] = (double)v[q];
   __syncthreads();
   if (tid < 160) {
     const int q = tid >> 4, seg = tid & 15;
