@@ -3,21 +3,27 @@
 Workload (the headline configuration): 4096 TIP4P molecules at 1.0 g/cm3
 (L = 49.669 A), Nose NVT at 300 K, cutoff 9.0 A switched over 0.5 A, dt 2 fs,
 FP32 pair arithmetic with FP64 accumulation; synthetic seeded lattice start
-with uniform random orientations and Maxwell-Boltzmann velocities.  One
-"step" is one full MD step (predict, cell grid + pair list, forces,
-collect, correct, thermostat), all resident in HBM and replayed as hipGraphs.
-Neighbour rows carry a 1.8 A Verlet skin by default (the fastest of 0.7-3.0 A
-measured on this workload) and are rebuilt when a molecule has moved half of
-it; the FP64 rc test on every listed pair keeps
-the interacting set the reference's (rows rebuilt every step) exactly.
+with uniform random orientations and Maxwell-Boltzmann velocities, brought to
+a 300 K liquid by an untimed equilibration (velocity redraws at 300 K, then
+Nose NVT alone).  The FP64 GPU path starts from the same equilibrated state
+and is timed with the same step counts and the same launch mode (every
+g6_step call replays captured hipGraphs).  One "step" is one full MD step
+(predict, cell grid + pair rows, forces, collect, correct, thermostat), all
+resident in HBM.  Neighbour rows carry a 1.8 A Verlet skin by default and are
+rebuilt when a molecule has moved half of it; the FP64 rc test on every
+listed pair keeps the interacting set the reference's (rows rebuilt every
+step) exactly.
 
-Prints ONE JSON line (rank 0, N=1).  The FP64 GPU path on the same workload
-is timed too and reported beside it.
+Prints ONE JSON line (rank 0, N=1).  --only c2|c3|c4|c5 runs one workload
+(for profiling); --eager launches every kernel without graphs (the mode
+kernel traces are taken in).
 """
 import argparse
 import json
 import os
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -28,38 +34,165 @@ sys.path.insert(0, ROOT)
 N, L, RC, TAPER, DT, TEMP = 4096, 49.669, 9.0, 0.5, 0.002, 300.0
 NOSE_Q = 7.5e5          # @NOS2 mass: T0*tau^2/dt^2 with tau = 0.1 ps
 SEED = 20261204
-FLOP_PER_PAIR = 257     # TIP4P: 9 Coulomb x 21 + LJ 28 + 40 per molecule pair
-FP32_PEAK = 157.3e12    # MI355X vector FP32 (vendor)
-HBM_PEAK = 8.0e12
+# algorithmic FLOP per interacting molecule pair (FMA = 2, rsqrt/rcp = 1):
+# Coulomb site pair 21, LJ site pair 28, 40 per molecule pair
+FLOP_PER_PAIR = {"tip4p": 9 * 21 + 28 + 40, "tip5p": 16 * 21 + 28 + 40}
+FP32_PEAK = 157.3e12    # MI355X vector FP32 (vendor, packed)
 
 
-def make_system(prec, skin):
+def log(msg):
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
+def new_system(mid, n, Lb, prec, skin, seed, temp, q_nos2):
     from genesis6_amd import g6
-    com, q = g6.lattice_start(N, L, SEED)
-    s = g6.System(g6.TIP4P, N, L, RC, TAPER, DT, prec=prec, skin=skin)
+    com, q = g6.lattice_start(n, Lb, seed)
+    s = g6.System(mid, n, Lb, RC, TAPER, DT, prec=prec, skin=skin)
     s.upload(com, q)
-    s.set_nose(TEMP, NOSE_Q)
-    s.vreset(TEMP, SEED)
-    return s, com, q
+    s.set_nose(temp, q_nos2)
+    s.vreset(temp, seed)
+    return s
 
 
-def timed_run(s, steps, warmup):
+def equilibrate(s, temp, seed, redraws, every, settle):
+    """Untimed: `redraws` rounds of a fresh Maxwell-Boltzmann draw at temp
+    followed by `every` Nose NVT steps (they carry the lattice's excess
+    potential energy off), then `settle` steps of Nose NVT alone."""
+    for r in range(redraws):
+        s.vreset(temp, seed + 1000 + r)
+        s.step(every)
+    s.step(settle)
+    return s.thermo().copy()
+
+
+def state_of(s):
+    com, q, dx, w = s.download()
+    return dict(com=com, quat=q, dx=dx, w=w, nose=s.nose())
+
+
+def load_state(s, st):
+    s.upload(st["com"], None, st["dx"], st["w"], quat=st["quat"])
+    z = st["nose"]
+    s.set_nose(z[0], z[1] / s.dt, z[2], tuple(z[3:8]), z[8])
+
+
+def timed_leg(s, steps, warmup):
+    """A step(steps) call to capture its graphs (untimed), `warmup` untimed
+    steps, then exactly `steps` steps between two stream synchronizations."""
+    s.step(steps)
     s.step(warmup)
     s.thermo()                       # synchronizes the stream
     t0 = time.perf_counter()
     s.step(steps)
     th = s.thermo()                  # synchronizes the stream
-    t1 = time.perf_counter()
-    return t1 - t0, th
+    return time.perf_counter() - t0, th.copy()
 
 
-def cpu_baseline(com, q, budget_s):
-    """The FP64 oracle (serial C restatement of the reference step) on the
-    same start, on one host core, for a bounded number of steps."""
+def roofline(flop_per_pair, pairs, kernel_ms, kernel, traffic, **extra):
+    achieved = flop_per_pair * pairs / (kernel_ms * 1e-3)
+    out = {"bound": "valu", "achieved": achieved / 1e12, "peak": FP32_PEAK / 1e12,
+           "unit": "TFLOP/s", "frac": achieved / FP32_PEAK, "traffic": traffic,
+           "kernel": kernel, "kernel_ms": kernel_ms, "flop_per_pair": flop_per_pair,
+           "pairs_per_launch": float(pairs)}
+    out.update(extra)
+    return out
+
+
+def traffic_of(name):
+    """HBM bytes per force-kernel launch from a committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if present."""
+    try:
+        return json.load(open(os.path.join(ROOT, "profiles", name)))["bytes_per_launch"]
+    except (OSError, ValueError, KeyError):
+        return None
+
+
+def c2_run(a):
+    """The headline: FP32 and FP64 legs on the equilibrated 300 K state."""
+    from genesis6_amd import g6
+    log("c2: TIP4P-%d, equilibration" % N)
+    s = new_system(g6.TIP4P, N, L, g6.FP32, a.skin, SEED, TEMP, NOSE_Q)
+    t0 = time.perf_counter()
+    th_eq = equilibrate(s, TEMP, SEED, 10, 2000, 10000)
+    log("c2: equilibrated in %.1f s: T %.1f K, U %.1f, %d pairs"
+        % (time.perf_counter() - t0, th_eq[3], th_eq[0], th_eq[14]))
+    st = state_of(s)
+    legs = {}
+    for name, prec in (("fp32", g6.FP32), ("fp64", g6.FP64)):
+        if name == "fp64" and a.no_fp64:
+            continue
+        if name == "fp64":
+            s = g6.System(g6.TIP4P, N, L, RC, TAPER, DT, prec=g6.FP64, skin=a.skin)
+        load_state(s, st)
+        el, th = timed_leg(s, a.steps, a.warmup)
+        leg = dict(value=N * a.steps / el, steps=a.steps, warmup=a.warmup,
+                   ms_per_step=el / a.steps * 1e3, temperature_K=float(th[3]),
+                   interacting_pairs=float(th[14]))
+        # a long run of the same leg (10,000 steps by default)
+        if a.long_steps > 0:
+            load_state(s, st)
+            el_l, _ = timed_leg(s, a.long_steps, a.warmup)
+            leg["long_run"] = dict(steps=a.long_steps, value=N * a.long_steps / el_l,
+                                   ms_per_step=el_l / a.long_steps * 1e3)
+        # the force kernel: back-to-back launches on the equilibrated state
+        # (events on the library stream) and its share of eager steps
+        load_state(s, st)
+        s.step(1)
+        leg["force_kernel_ms"] = s.time_force(200)
+        t = s.step_timed(50)
+        leg["eager_step"] = dict(force_kernel_ms=t[0], per_molecule_kernels_ms=t[2],
+                                 step_ms=t[3], pairs=t[4])
+        legs[name] = leg
+        log("c2 %s: %.4g molecule-steps/s (%.4f ms/step), force kernel %.2f us, T %.1f K"
+            % (name, leg["value"], leg["ms_per_step"], leg["force_kernel_ms"] * 1e3,
+               leg["temperature_K"]))
+        s.close()
+    f32 = legs["fp32"]
+    rl = roofline(FLOP_PER_PAIR["tip4p"], f32["eager_step"]["pairs"], f32["force_kernel_ms"],
+                  "k_md_main<TIP4P,float>", traffic_of("md_main_tip4p4096_traffic.json"),
+                  kernel_in_eager_step_ms=f32["eager_step"]["force_kernel_ms"])
+    return legs, rl, st, th_eq
+
+
+def tip5p_run(a):
+    """65,536 TIP5P molecules at 1.0 g/cm3 (L = 125.157 A), Nose NVT 275 K,
+    rc 9.0 A switched over 0.5 A, FP32 pairs (the cell-tile force kernel),
+    timed from an equilibrated start."""
+    from genesis6_amd import g6
+    n, Lb, T = 65536, 125.157, 275.0
+    s = new_system(g6.TIP5P, n, Lb, g6.FP32, a.tip5p_skin, 20261202 + 3, T, T * 2500.0)
+    equilibrate(s, T, 20261202 + 3, 4, 250, 500)
+    el, th = timed_leg(s, a.tip5p_steps, 50)
+    kern = s.md_kernel()
+    s.step(1)
+    kms = s.time_force(50)
+    pairs = float(s.thermo()[14])
+    s.close()
+    name = "k_md_tile<TIP5P>" if kern[0] else "k_md_main<TIP5P,float>"
+    out = dict(value=n * a.tip5p_steps / el, unit="molecule-steps/s", molecules=n,
+               steps=a.tip5p_steps, ms_per_step=el / a.tip5p_steps * 1e3,
+               pair_list_skin_A=a.tip5p_skin,
+               pairs_per_step=float(th[14]), temperature_K=float(th[3]),
+               force_kernel={"tiles": bool(kern[0]), "cells_per_axis": kern[1],
+                             "workgroups_per_tile": kern[2], "waves": kern[3],
+                             "slots": kern[4], "left_tiles_on_code": kern[5]})
+    out["roofline"] = roofline(FLOP_PER_PAIR["tip5p"], pairs, kms, name,
+                               traffic_of("md_tile_tip5p65536_traffic.json"))
+    return out
+
+
+def cpu_child(path, budget):
+    """One process of the all-cores CPU baseline: the FP64 oracle from the
+    saved state for `budget` seconds; prints molecule-steps/s."""
     from oracle import pyoracle as O
-    md = O.MDRun(O.TIP4P, com, q.T.copy(), L, RC, TAPER, DT, fast=True,
+    z = np.load(path)
+    md = O.MDRun(O.TIP4P, z["com"], z["quat"][0], L, RC, TAPER, DT, fast=True,
                  nose=(TEMP, NOSE_Q, 0.0))
-    md.vreset(TEMP, SEED)
+    md.quat[:] = z["quat"]
+    md.dx[:] = z["dx"]
+    md.w[:] = z["w"]
+    for i in range(9):
+        md.s.nose[i] = z["nose"][i]
     md.step(1)
     t0 = time.perf_counter()
     k = 0
@@ -67,9 +200,40 @@ def cpu_baseline(com, q, budget_s):
         md.step(1)
         k += 1
         el = time.perf_counter() - t0
-        if el >= budget_s:
+        if el >= budget:
             break
     return N * k / el, k, el
+
+
+def cpu_baseline(st, budget, procs):
+    """The FP64 oracle (serial C restatement of the reference step, custom
+    TIP4P kernel) on the equilibrated C2 state: one process in this one,
+    then `procs` concurrent child processes (independent replicas, the
+    reference's replica-parallel mode)."""
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "state.npz")
+        np.savez(path, com=st["com"], quat=st["quat"], dx=st["dx"], w=st["w"], nose=st["nose"])
+        v1, k1, e1 = cpu_child(path, budget)
+        code = ("import sys; sys.path.insert(0, %r); import bench; "
+                "print(bench.cpu_child(%r, %r)[0])" % (ROOT, path, budget))
+        ps = [subprocess.Popen([sys.executable, "-c", code], stdout=subprocess.PIPE, text=True)
+              for _ in range(procs)]
+        rates = []
+        for p in ps:
+            out, _ = p.communicate()
+            if p.returncode != 0:
+                raise RuntimeError("CPU baseline child failed")
+            rates.append(float(out.strip().splitlines()[-1]))
+    return v1, k1, e1, sum(rates)
+
+
+def reference_calibration():
+    """The oracle / flang-built reference rate ratio measured where the
+    reference sources are (committed by oracle/ref_timing.py)."""
+    try:
+        return json.load(open(os.path.join(ROOT, "profiles", "cpu_reference_timing.json")))
+    except (OSError, ValueError):
+        return None
 
 
 # Replica ladders: 4 ladders x 16 rungs, T_k = 220 (320/220)^(k/15) K; per
@@ -139,6 +303,7 @@ def ladder_run(intervals, skin, tune=False, log=None):
         acc += a
     el = time.perf_counter() - t0
     steps = intervals * RX_EVERY
+    kms = lad.time_force(100)
     out = dict(value=nmol * steps / el, unit="molecule-steps/s", replicas=len(systems),
                molecules=nmol, steps=steps, ms_per_step=el / steps * 1e3,
                skin_A=skin, block_map="xcd" if mode == g6.Ladder.MAP_XCD else "flat",
@@ -146,6 +311,9 @@ def ladder_run(intervals, skin, tune=False, log=None):
                exchange_trials=intervals * RX_TRIALS * len(groups), accepted=acc,
                pairs_per_step=float(th[:, 14].sum()),
                mean_temperature_K=float(th[:, 3].mean()))
+    out["roofline"] = roofline(FLOP_PER_PAIR["tip4p"], th[:, 14].sum(), kms,
+                               "k_md_main_lad<TIP4P,float>",
+                               traffic_of("md_main_lad_tip4p64x960_traffic.json"))
     lad.close()
     for g in groups:
         g.close()
@@ -154,37 +322,18 @@ def ladder_run(intervals, skin, tune=False, log=None):
     return out
 
 
-def tip5p_run(steps, skin):
-    """65,536 TIP5P molecules at 1.0 g/cm3 (L = 125.157 A), Nose NVT 275 K,
-    rc 9.0 A switched over 0.5 A, FP32 pairs."""
-    from genesis6_amd import g6
-    n, Lb, T = 65536, 125.157, 275.0
-    s = g6.System(g6.TIP5P, n, Lb, RC, TAPER, DT, prec=g6.FP32, skin=skin)
-    com, q = g6.lattice_start(n, Lb, 20261202 + 3)
-    s.upload(com, q)
-    s.set_nose(T, T * 2500.0)
-    s.vreset(T, 20261202 + 3)
-    el, th = timed_run(s, steps, 50)
-    s.close()
-    return dict(value=n * steps / el, unit="molecule-steps/s", molecules=n, steps=steps,
-                ms_per_step=el / steps * 1e3, pairs_per_step=float(th[14]),
-                temperature_K=float(th[3]))
-
-
 def models_run(steps, skin):
     """The other rigid models on the headline workload (4096 molecules at
     1.0 g/cm3, L = 49.669 A, Nose NVT 300 K, rc 9.0 A switched over 0.5 A,
-    FP32 pairs): SPC/E, NvdE and ST2 (its custom kernel), molecule-steps/s."""
+    FP32 pairs): SPC/E, NvdE and ST2 (its custom kernel), molecule-steps/s,
+    each timed from an equilibrated start."""
     from genesis6_amd import g6
     n, Lb, T = 4096, 49.669, 300.0
     out = {}
     for name, mid in (("spce", g6.SPCE), ("nvde", g6.NVDE), ("st2", g6.ST2)):
-        s = g6.System(mid, n, Lb, RC, TAPER, DT, prec=g6.FP32, skin=skin)
-        com, q = g6.lattice_start(n, Lb, 20261204)
-        s.upload(com, q)
-        s.set_nose(T, T * 2500.0)
-        s.vreset(T, 20261204)
-        el, th = timed_run(s, steps, 50)
+        s = new_system(mid, n, Lb, g6.FP32, skin, 20261204, T, T * 2500.0)
+        equilibrate(s, T, 20261204, 5, 1000, 2000)
+        el, th = timed_leg(s, steps, 50)
         s.close()
         out[name] = dict(value=n * steps / el, unit="molecule-steps/s", steps=steps,
                          ms_per_step=el / steps * 1e3, pairs_per_step=float(th[14]),
@@ -257,25 +406,25 @@ def npt_run(configs, epochs, moves, rounds, K, p_volume, log):
             "final_box_A": Lf}
 
 
-def traffic_from_profile():
-    """HBM bytes per force-kernel launch from the committed rocprofv3 --pmc
-    summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if present."""
-    p = os.path.join(ROOT, "profiles", "md_main_tip4p4096_traffic.json")
-    try:
-        return json.load(open(p))["bytes_per_launch"]
-    except Exception:
-        return None
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--long-steps", type=int, default=10000,
+                    help="a second timing of each TIP4P-4096 leg over this many steps "
+                         "(0: skip)")
     ap.add_argument("--skin", type=float, default=1.8,
                     help="Verlet skin in A (0: rows rebuilt every step, as the reference); "
                          "1.8 A measured fastest for TIP4P-4096 among 0.7-3.0 A")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--only", choices=["c2", "c3", "c4", "c5", "models"], default=None,
+                    help="run one workload only (profiling)")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch every kernel eagerly (no hipGraphs), as kernel traces are taken")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="seconds of each CPU-baseline process")
+    ap.add_argument("--cpu-procs", type=int, default=0,
+                    help="concurrent CPU-baseline processes (0: the usable cores, at most 16)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--ladder-intervals", type=int, default=4,
@@ -284,6 +433,9 @@ def main():
                     help="skip the ladder autotuner (skin and block assignment)")
     ap.add_argument("--tip5p-steps", type=int, default=500,
                     help="timed steps of the TIP5P-65536 system (0: skip)")
+    ap.add_argument("--tip5p-skin", type=float, default=1.0,
+                    help="Verlet skin of the TIP5P-65536 system in A (1.0-1.4 A measured "
+                         "fastest for its cell tiles among 0.6-1.8 A)")
     ap.add_argument("--model-steps", type=int, default=2000,
                     help="timed steps of SPC/E, NvdE and ST2 at 4096 molecules (0: skip)")
     ap.add_argument("--npt-configs", type=int, default=20000,
@@ -300,6 +452,8 @@ def main():
                          "(a plain copy kernel of known size, which calibrates the "
                          "FETCH_SIZE / WRITE_SIZE counters of rocprofv3 --pmc)")
     a = ap.parse_args()
+    if a.eager:
+        os.environ["G6_GRAPH_STEPS"] = "0"
     if a.copy_probe_mib > 0:
         import torch
         x = torch.ones(a.copy_probe_mib << 18, dtype=torch.float32, device="cuda")
@@ -307,108 +461,97 @@ def main():
         for _ in range(5):
             y.copy_(x)
         torch.cuda.synchronize()
-        print("copy probe: 5 x %d MiB device-to-device" % a.copy_probe_mib, file=sys.stderr)
+        log("copy probe: 5 x %d MiB device-to-device" % a.copy_probe_mib)
         return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the system is not sharded across GPUs "
               "(the single-system step runs on one device); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-    from genesis6_amd import g6
-    print("bench: TIP4P-%d FP32-pair, %d warmup + %d steps" % (N, a.warmup, a.steps),
-          file=sys.stderr, flush=True)
-    s, com, q = make_system(g6.FP32, a.skin)
-    el, th = timed_run(s, a.steps, a.warmup)
-    value = N * a.steps / el
-    # per-kernel timing with HIP events on the library stream: each phase of
-    # eager steps bracketed (the force kernel with its dispatch: what the
-    # roofline uses), and the force kernel alone, back to back in one graph
-    # on warm caches (a lower bound: in a step its inputs come fresh from the
-    # Gear kernel)
-    nt = min(200, max(20, a.steps // 50))
-    t = s.step_timed(nt)
-    npair = t[4]
-    s.step(10)
-    t_b2b_ms = s.time_force(200)
-    t_force_ms = t[0]
-    t_force_s = t_force_ms * 1e-3
-    achieved = npair * FLOP_PER_PAIR / t_force_s
-    print("bench: FP32 %.4g molecule-steps/s, T=%.1f K, force kernel %.1f us "
-          "(%.1f us back to back), npair %.0f"
-          % (value, th[3], t_force_ms * 1e3, t_b2b_ms * 1e3, npair), file=sys.stderr,
-          flush=True)
-    s.close()
-    fp64 = None
-    if not a.no_fp64:
-        s64, _, _ = make_system(g6.FP64, a.skin)
-        k64 = max(100, a.steps // 5)
-        el64, _ = timed_run(s64, k64, a.warmup)
-        s64.step(10)
-        fp64 = dict(value=N * k64 / el64, steps=k64, force_kernel_us=s64.time_force(100) * 1e3)
-        s64.close()
-        print("bench: FP64 %.4g molecule-steps/s" % fp64["value"], file=sys.stderr, flush=True)
-    ladder = None
-    if a.ladder_intervals > 0:
-        ladder = ladder_run(a.ladder_intervals, a.skin, tune=not a.no_tune,
-                            log=lambda m: print("bench: " + m, file=sys.stderr, flush=True))
-        print("bench: ladder %.4g molecule-steps/s (%d replicas, %.3f ms/step)"
-              % (ladder["value"], ladder["replicas"], ladder["ms_per_step"]),
-              file=sys.stderr, flush=True)
-    tip5p = None
-    if a.tip5p_steps > 0:
-        tip5p = tip5p_run(a.tip5p_steps, a.skin)
-        print("bench: TIP5P-65536 %.4g molecule-steps/s" % tip5p["value"], file=sys.stderr,
-              flush=True)
-    models = None
-    if a.model_steps > 0:
+    only = a.only
+    if only not in (None, "c2"):
+        out = {"only": only}
+        if only == "c3":
+            out["tip5p_65536"] = tip5p_run(a)
+        elif only == "c4":
+            out["replica_ladders"] = ladder_run(a.ladder_intervals or 1, a.skin,
+                                                tune=not a.no_tune, log=log)
+        elif only == "c5":
+            out["npt_mc"] = npt_run(a.npt_configs, a.npt_epochs, a.npt_moves, a.npt_rounds,
+                                    a.npt_k, a.npt_pvol, log)
+        else:
+            out["other_models_4096"] = models_run(a.model_steps, a.skin)
+        print(json.dumps(out))
+        return
+    legs, rl, st, th_eq = c2_run(a)
+    f32 = legs["fp32"]
+    fp64 = legs.get("fp64")
+    ladder = tip5p = models = npt = None
+    if only is None and a.ladder_intervals > 0:
+        ladder = ladder_run(a.ladder_intervals, a.skin, tune=not a.no_tune, log=log)
+        log("ladder %.4g molecule-steps/s (%d replicas, %.3f ms/step), force kernel %.1f%% of "
+            "FP32 peak" % (ladder["value"], ladder["replicas"], ladder["ms_per_step"],
+                           100 * ladder["roofline"]["frac"]))
+    if only is None and a.tip5p_steps > 0:
+        tip5p = tip5p_run(a)
+        log("TIP5P-65536 %.4g molecule-steps/s, force kernel %.1f%% of FP32 peak"
+            % (tip5p["value"], 100 * tip5p["roofline"]["frac"]))
+    if only is None and a.model_steps > 0:
         models = models_run(a.model_steps, a.skin)
-        print("bench: 4096 molecules, " + ", ".join(
-            "%s %.4g" % (k, v["value"]) for k, v in models.items()) + " molecule-steps/s",
-            file=sys.stderr, flush=True)
-    npt = None
-    if a.npt_configs > 0:
+        log("4096 molecules, " + ", ".join(
+            "%s %.4g" % (k, v["value"]) for k, v in models.items()) + " molecule-steps/s")
+    if only is None and a.npt_configs > 0:
         npt = npt_run(a.npt_configs, a.npt_epochs, a.npt_moves, a.npt_rounds, a.npt_k,
-                      a.npt_pvol, lambda m: print("bench: " + m, file=sys.stderr, flush=True))
-        print("bench: NPT MC exact-accepted moves/s plain %.0f, delayed acceptance %.0f (%.2fx)"
-              % (npt["plain"], npt["delayed_acceptance"], npt["speedup"]), file=sys.stderr,
-              flush=True)
+                      a.npt_pvol, log)
+        log("NPT MC exact-accepted moves/s plain %.0f, delayed acceptance %.0f (%.2fx)"
+            % (npt["plain"], npt["delayed_acceptance"], npt["speedup"]))
     cpu = None
     if not a.no_cpu:
-        v, k, e = cpu_baseline(com, q, a.cpu_seconds)
-        cpu = {"value": v, "unit": "molecule-steps/s", "cores": 1, "kind": "port",
-               "sample": "TIP4P-4096 Nose NVT, %d MD steps (%.1f s) from the same start, "
-                         "FP64 oracle (serial C restatement of the reference step, "
-                         "custom TIP4P kernel)" % (k, e)}
+        usable = len(os.sched_getaffinity(0))
+        procs = a.cpu_procs or min(16, usable)
+        v1, k1, e1, vall = cpu_baseline(st, a.cpu_seconds, procs)
+        cal = reference_calibration()
+        ratio = None
+        if cal and "C2_tip4p_4096_nvt" in cal:
+            ratio = cal["C2_tip4p_4096_nvt"]["oracle_over_reference"]
+        cpu = {"value": vall, "unit": "molecule-steps/s", "cores": procs, "kind": "port",
+               "sample": "TIP4P-4096 Nose NVT from the equilibrated 300 K state, FP64 oracle "
+                         "(serial C restatement of the reference step, custom TIP4P kernel): "
+                         "%d concurrent processes (independent replicas) for %.0f s each; "
+                         "one process alone: %.4g molecule-steps/s (%d steps in %.1f s)"
+                         % (procs, a.cpu_seconds, v1, k1, e1),
+               "one_core": v1, "nproc": os.cpu_count(), "usable_cores": usable,
+               "oracle_over_reference": ratio,
+               "reference_equivalent_one_core": (v1 / ratio) if ratio else None,
+               "reference_timing_here": cal}
+        log("CPU oracle %.4g molecule-steps/s on %d processes, %.4g on one"
+            % (vall, procs, v1))
     out = {
         "metric": "TIP4P-4096 molecule-steps/s",
-        "value": value,
+        "value": f32["value"],
         "unit": "molecule-steps/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": el / a.steps * 1e3,
+        "ms_per_step": f32["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic: seeded lattice start, uniform random orientations, "
-                "Maxwell-Boltzmann velocities (MT19937 seed %d)" % SEED,
+                "Maxwell-Boltzmann velocities (MT19937 seed %d), equilibrated untimed to a "
+                "300 K liquid (10 x 2000 steps with velocity redraws, 10000 Nose steps)" % SEED,
         "config": {"workload": "TIP4P rigid water, 4096 molecules, 1.0 g/cm3, L=49.669 A, "
                                "Nose NVT 300 K, rc 9.0 A switched over 0.5 A, dt 2 fs, "
                                "FP32 pair arithmetic / FP64 accumulation",
                    "n_molecules": N, "pair_list_skin_A": a.skin,
-                   "final_temperature_K": float(th[3]),
-                   "interacting_pairs_per_step": float(npair)},
+                   "launch": "eager" if a.eager else "hipGraph replay",
+                   "equilibrated_temperature_K": float(th_eq[3]),
+                   "final_temperature_K": f32["temperature_K"],
+                   "interacting_pairs_per_step": f32["interacting_pairs"]},
+        "fp32_gpu": f32,
         "fp64_gpu": fp64,
-        "speedup_vs_fp64_gpu": (value / fp64["value"]) if fp64 else None,
-        "roofline": {"bound": "valu", "achieved": achieved / 1e12, "peak": FP32_PEAK / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK,
-                     "traffic": traffic_from_profile(),
-                     "kernel": "k_md_main<TIP4P,float>",
-                     "kernel_ms": t_force_ms, "kernel_back_to_back_ms": t_b2b_ms,
-                     "flop_per_pair": FLOP_PER_PAIR,
-                     "pairs_per_launch": float(npair),
-                     "per_molecule_kernels_ms": t[2],
-                     "eager_step_ms": t[3]},
+        "speedup_vs_fp64_gpu": (f32["value"] / fp64["value"]) if fp64 else None,
+        "roofline": rl,
         "cpu_baseline": cpu,
         "npt_mc": npt,
         "replica_ladders": ladder,

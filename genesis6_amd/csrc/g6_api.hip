@@ -551,6 +551,26 @@ static int reset_step_state(System &s) {
   return 0;
 }
 
+// Back to the tiles after a capacity fallback (a lattice start can fill
+// some tile neighbourhoods far above a liquid's counts for a few steps).
+static int use_tile_grid(System &s) {
+  if (s.tiles || !s.tile_ok) return 0;
+  G6_CHECK(hipStreamSynchronize(s.stream));
+  s.md.part = s.tile_part;
+  set_tile_grid(s);
+  drop_graphs(s);
+  return reset_step_state(s);
+}
+
+// a capacity error moved a tile system to the rows: when to try the tiles
+// again (G6_TILE_RETRY steps later, then twice as long after each failure)
+static void note_tile_exit(System &s) {
+  s.tile_exit = s.last_flag;
+  if (s.tile_retry_gap == 0) s.tile_retry_gap = std::max(1, env_int("G6_TILE_RETRY", 1000));
+  s.tile_retry_at = s.steps_done + s.tile_retry_gap;
+  s.tile_retry_gap = (int)std::min(2LL * s.tile_retry_gap, 1LL << 24);
+}
+
 // Back to the rows of both directions (ladders and the barostat use them).
 int use_row_grid(System &s) {
   if (!s.tiles) return 0;
@@ -848,6 +868,7 @@ static int check_flags(System &s) {
   int fl[2] = {0, 0};
   G6_CHECK(hipMemcpyAsync(fl, s.md.flags, sizeof(fl), hipMemcpyDeviceToHost, s.stream));
   G6_CHECK(hipStreamSynchronize(s.stream));
+  s.last_flag = fl[0];
   if (fl[0] == 1 || fl[0] == 3 || fl[0] == 4 || fl[0] == 5) return G6_ECAPACITY;
   if (fl[0] == 2) return G6_EARG;
   return 0;
@@ -879,6 +900,11 @@ static int step_once(System *s, int nsteps);
 int g6_step(g6_handle h, int nsteps) {
   System *s = get(h);
   if (!s || nsteps < 0) return G6_EARG;
+  if (!s->tiles && s->tile_ok && s->tile_exit != 0 && !s->andersen &&
+      s->steps_done >= s->tile_retry_at) {
+    const int rt = use_tile_grid(*s);
+    if (rt != 0) return rt;
+  }
   const int sv = snap_state(*s, false);
   if (sv != 0) return sv;
   const int rc = step_once(s, nsteps);
@@ -891,6 +917,7 @@ int g6_step(g6_handle h, int nsteps) {
   G6_CHECK(hipStreamSynchronize(s->stream));
   s->steps_done -= nsteps;
   if (s->tiles) {
+    note_tile_exit(*s);
     const int rg = use_row_grid(*s);
     if (rg != 0) return rg;
     return g6_step(h, nsteps);
@@ -1028,6 +1055,7 @@ int g6_md_kernel(g6_handle h, int32_t *info) {
   info[2] = s->tiles ? s->md.slices : 0;
   info[3] = s->tiles ? s->tile_waves : 0;
   info[4] = s->tiles ? s->md.nbmax : 0;
+  info[5] = s->tile_exit;
   return 0;
 }
 
@@ -1050,6 +1078,7 @@ static int eval_forces_into(System *s, double *out, double *force, double *torqu
     G6_CHECK(hipStreamSynchronize(s->stream));
     const int rc = check_flags(*s);
     if (rc != 0 && rc != G6_EHIP && s->tiles && fallback && attempt == 0) {
+      note_tile_exit(*s);
       const int rg = use_row_grid(*s);
       if (rg != 0) return rg;
       continue;
@@ -1133,6 +1162,7 @@ int g6_set_andersen(g6_handle h, int mode, double pex, double mass, const double
     // of edge >= (rc + skin)/2 in the largest box it allows)
     const int rg = use_row_grid(*s);
     if (rg != 0) return rg;
+    s->tile_ok = false;  // the tile grid was planned for the fixed box
     // the box is the handle's (a restart's @BOX3); v continues a saved
     // barostat state (@PCN5), NULL starts one
     const double L = s->L;

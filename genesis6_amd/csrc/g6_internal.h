@@ -201,6 +201,12 @@ struct System {
   double nose_h[9] = {0};
   bool andersen = false;  // md.box and md.anders are allocated
   bool tiles = false;     // the MD path runs the cell-tile force kernel
+  int tile_exit = 0;      // flag code of the capacity error that left the tiles
+  int last_flag = 0;      // flag code the last check of the step flags read
+  // a tile system moved to the rows by a capacity error tries the tiles
+  // again at the first g6_step call from step tile_retry_at on; the gap
+  // doubles after every failed try
+  int tile_retry_at = 0, tile_retry_gap = 0;
   bool tile_ok = false;   // tile buffers allocated (the system qualifies)
   int tile_waves = 8;     // waves per tile workgroup
   int tile_ndiv = 0, tile_cap = 0;  // the tile grid and its cell capacity

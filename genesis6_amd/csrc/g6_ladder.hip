@@ -291,6 +291,7 @@ int g6_ladder_create(g6_ladder *hl, int nsys, const g6_handle *hs) {
       delete l;
       return rg;
     }
+    s->tile_ok = false;  // a ladder's systems keep the rows
   }
   const int rc = ladder_setup(*l);
   if (rc != 0) {
@@ -359,6 +360,60 @@ int g6_ladder_step(g6_ladder hl, int nsteps) {
     else if (f == 2 && rc == 0) rc = G6_EARG;
   }
   return rc;
+}
+
+int g6_ladder_time_force(g6_ladder hl, int nlaunch, double *ms) {
+  Ladder *l = lget(hl);
+  if (!l || nlaunch < 1 || nlaunch > 10000 || !ms) return G6_EARG;
+  // the ladder's force kernel alone (every system's blocks in one launch),
+  // nlaunch times back to back on the state a g6_ladder_step call leaves;
+  // one graph, or eager launches when G6_GRAPH_STEPS=0
+  for (System *s : l->sys) G6_CHECK(hipStreamSynchronize(s->stream));
+  G6_CHECK(hipStreamSynchronize(l->stream));
+  hipEvent_t e0, e1;
+  G6_CHECK(hipEventCreate(&e0));
+  const hipError_t ec = hipEventCreate(&e1);
+  if (ec != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return G6_EHIP;
+  }
+  hipGraphExec_t ge = nullptr;
+  hipError_t err = hipSuccess;
+  const bool graph = ladder_steps_setting() > 0;
+  if (graph) {
+    hipGraph_t gr;
+    err = hipStreamBeginCapture(l->stream, hipStreamCaptureModeThreadLocal);
+    if (err == hipSuccess) {
+      for (int k = 0; k < nlaunch; k++) md_launch_main_lad(*l, 0, 0);
+      err = hipStreamEndCapture(l->stream, &gr);
+      if (err == hipSuccess) {
+        err = hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+      }
+    }
+    if (err == hipSuccess) err = hipGraphLaunch(ge, l->stream);  // warm
+  } else {
+    md_launch_main_lad(*l, 0, 0);
+  }
+  if (err == hipSuccess) err = hipEventRecord(e0, l->stream);
+  if (err == hipSuccess) {
+    if (graph) {
+      err = hipGraphLaunch(ge, l->stream);
+    } else {
+      for (int k = 0; k < nlaunch; k++) md_launch_main_lad(*l, 0, 0);
+      err = hipGetLastError();
+    }
+  }
+  if (err == hipSuccess) err = hipEventRecord(e1, l->stream);
+  if (err == hipSuccess) err = hipEventSynchronize(e1);
+  float e = 0;
+  if (err == hipSuccess) err = hipEventElapsedTime(&e, e0, e1);
+  if (ge) (void)hipGraphExecDestroy(ge);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (err != hipSuccess) return G6_EHIP;
+  *ms = e / nlaunch;
+  return 0;
 }
 
 int g6_ladder_set_map(g6_ladder hl, int mode) {

@@ -356,7 +356,7 @@ __device__ __forceinline__ void md_pre_body(const ModelTable &mt, const Geom &g,
       if (slot < b.cell_cap)
         cr[slot] = i;
       else
-        b.flags[0] = 3;  // cell capacity exceeded
+        raise_flag(b.flags, 3);  // cell capacity exceeded
       if (b.half_skin2 > 0.0) {
         // a pair of the rows moves by at most twice the largest displacement
         // plus sqrt(3)|L - L_ref| (image shifts of one box per axis)
@@ -370,7 +370,7 @@ __device__ __forceinline__ void md_pre_body(const ModelTable &mt, const Geom &g,
       } else if (i == 0) {
         b.flags[2 + parity] = 1;  // no skin: the reference rebuilds every step
       }
-      if (NPT && i == 0 && bx.L < b.box[5]) b.flags[0] = 5;  // cells below (rc+skin)/2
+      if (NPT && i == 0 && bx.L < b.box[5]) raise_flag(b.flags, 5);  // cells below (rc+skin)/2
     } else if (role >= 2) {
       // rigid_setmatrix and the site offsets; the two waves split the stores
       const double qa = S[0], qb = S[1], qc = S[2], qd = S[3];
@@ -574,7 +574,7 @@ __device__ inline int build_row(const Geom &g, const BoxNow &bx, const MdBuf &b,
         const double dd = d0 * d0 + d1 * d1 + d2 * d2;
         pass = dd < g.rr;
         code = ((int)k0 + 512) | (((int)k1 + 512) << 10) | (((int)k2 + 512) << 20);
-        if (pass && (fabs(k0) > 511.0 || fabs(k1) > 511.0 || fabs(k2) > 511.0)) b.flags[0] = 2;
+        if (pass && (fabs(k0) > 511.0 || fabs(k1) > 511.0 || fabs(k2) > 511.0)) raise_flag(b.flags, 2);
       }
     }
     const unsigned long long mask = __ballot(pass);
@@ -584,7 +584,7 @@ __device__ inline int build_row(const Geom &g, const BoxNow &bx, const MdBuf &b,
         s_j[pos] = j;
         s_c[pos] = code;
       } else {
-        b.flags[0] = 4;  // neighbour-row capacity exceeded
+        raise_flag(b.flags, 4);  // neighbour-row capacity exceeded
       }
     }
     base += __popcll(mask);

@@ -10,6 +10,11 @@
 
 namespace g6 {
 
+// The step's error code: the first one raised stays (a later symptom of the
+// same failure, e.g. a runaway molecule's image shift, does not replace it).
+__device__ inline void raise_flag(int *flags, int code) { atomicCAS(flags, 0, code); }
+
+
 constexpr double kG5_0 = 3.0 / 16.0, kG5_1 = 251.0 / 360.0, kG5_3 = 11.0 / 18.0,
                  kG5_4 = 1.0 / 6.0, kG5_5 = 1.0 / 60.0;
 constexpr double kG4_0 = 251.0 / 720.0, kG4_2 = 11.0 / 12.0, kG4_3 = 1.0 / 3.0,

@@ -152,7 +152,7 @@ __device__ __forceinline__ void tile_body(const Geom &g, const MdBuf &b, const P
         s_base[d] = acc;
         acc += s_ccnt[d];
       }
-      if (acc > nbmax) b.flags[0] = 4;  // neighbourhood beyond the LDS slots
+      if (acc > nbmax) raise_flag(b.flags, 4);  // neighbourhood beyond the LDS slots
       s_base[14] = min(acc, nbmax);
     }
     __syncthreads();
@@ -382,7 +382,7 @@ __device__ __forceinline__ void tile_body(const Geom &g, const MdBuf &b, const P
           const double dd = d0 * d0 + d1 * d1 + d2 * d2;
           keep = dd < rr;
           if (keep && (fabs(k0) > 63.0 || fabs(k1) > 63.0 || fabs(k2) > 63.0)) {
-            b.flags[0] = 2;
+            raise_flag(b.flags, 2);
             keep = false;
           }
           const unsigned code = (unsigned)(((int)k0 + 64) | (((int)k1 + 64) << 7) |
@@ -394,7 +394,7 @@ __device__ __forceinline__ void tile_body(const Geom &g, const MdBuf &b, const P
         if (keep) {
           const int p = base + __popcll(km & lt);
           if (p < b.maxrow) row[p] = ent;
-          else b.flags[0] = 4;  // row capacity exceeded
+          else raise_flag(b.flags, 4);  // row capacity exceeded
         }
         base += __popcll(km);
       } else if (c < ncand) {

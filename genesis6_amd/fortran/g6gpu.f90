@@ -291,7 +291,7 @@ module g6gpu
      integer(c_int) function g6_md_kernel(h, info) bind(C, name="g6_md_kernel")
        import :: c_int, c_int64_t, c_int32_t
        integer(c_int64_t), value :: h
-       integer(c_int32_t), intent(out) :: info(5)
+       integer(c_int32_t), intent(out) :: info(6)
      end function g6_md_kernel
 
      integer(c_int) function g6_mc_begin(h, temp, seed, delta, rtratio) bind(C, name="g6_mc_begin")
@@ -351,6 +351,14 @@ module g6gpu
        integer(c_int64_t), value :: hl
        integer(c_int), value :: mode
      end function g6_ladder_set_map
+
+     integer(c_int) function g6_ladder_time_force(hl, nlaunch, ms) &
+          bind(C, name="g6_ladder_time_force")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), value :: hl
+       integer(c_int), value :: nlaunch
+       real(c_double), intent(out) :: ms
+     end function g6_ladder_time_force
 
      integer(c_int) function g6_ladder_thermo(hl, thermo) bind(C, name="g6_ladder_thermo")
        import :: c_int, c_int64_t, c_double

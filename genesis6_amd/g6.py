@@ -74,6 +74,7 @@ def lib():
                                                C.POINTER(_d)]
         L.g6_set_fast.argtypes = [_H, C.c_int]
         L.g6_time_force.argtypes = [_H, C.c_int, C.POINTER(C.c_double)]
+        L.g6_ladder_time_force.argtypes = [_H, C.c_int, C.POINTER(C.c_double)]
         L.g6_md_kernel.argtypes = [_H, _ip]
         L.g6_set_andersen.argtypes = [_H, C.c_int, C.c_double, C.c_double, C.c_void_p]
         L.g6_get_andersen.argtypes = [_H, _dp]
@@ -228,9 +229,10 @@ class System:
         return L.value
 
     def md_kernel(self):
-        """(tiles, cells per axis, workgroups per tile, waves, slots) of the
-        MD force kernel (g6_md_kernel)."""
-        info = np.zeros(5, np.int32)
+        """(tiles, cells per axis, workgroups per tile, waves, slots, capacity
+        code that moved the system to the rows) of the MD force kernel
+        (g6_md_kernel)."""
+        info = np.zeros(6, np.int32)
         check(lib().g6_md_kernel(self.h, info), "g6_md_kernel")
         return tuple(int(x) for x in info)
 
@@ -518,6 +520,13 @@ class Ladder:
 
     def set_map(self, mode):
         check(lib().g6_ladder_set_map(self.h, mode), "g6_ladder_set_map")
+
+    def time_force(self, nlaunch=100):
+        """Mean ms per launch of the ladder's force kernel alone
+        (g6_ladder_time_force)."""
+        ms = C.c_double()
+        check(lib().g6_ladder_time_force(self.h, nlaunch, C.byref(ms)), "g6_ladder_time_force")
+        return ms.value
 
     def thermo(self):
         """Thermo records of the last g6_ladder_step, (nsys, THERMO_LEN)."""

@@ -292,7 +292,8 @@ typedef int64_t g6_rx;
 
 /* A ladder over nsys distinct systems (g6_init handles) of one model and
  * precision with equal rc and taper.  The systems keep their own state and
- * stay usable on their own between ladder calls. */
+ * stay usable on their own between ladder calls (on the rows of both
+ * directions from here on). */
 int g6_ladder_create(g6_ladder *hl, int nsys, const g6_handle *hs);
 int g6_ladder_free(g6_ladder hl);
 /* nsteps MD steps of every system: what g6_step does to each, with the
@@ -306,6 +307,12 @@ int g6_ladder_step(g6_ladder hl, int nsteps);
  * over the eight, largest first). */
 enum { G6_LADDER_MAP_XCD = 0, G6_LADDER_MAP_FLAT = 1 };
 int g6_ladder_set_map(g6_ladder hl, int mode);
+/* The ladder's force kernel alone (one launch covers every system), nlaunch
+ * launches back to back on the state a g6_ladder_step call leaves, timed
+ * with HIP events on the ladder's stream; *ms = mean milliseconds per
+ * launch.  The state is not moved.  No reference counterpart: the
+ * measurement hook of the ladder's roofline. */
+int g6_ladder_time_force(g6_ladder hl, int nlaunch, double *ms);
 /* the thermo records of the last step: thermo[nsys * G6_THERMO_LEN], in
  * the order of hs */
 int g6_ladder_thermo(g6_ladder hl, double *thermo);
@@ -416,19 +423,25 @@ int g6_lattice_start(int n, double box_l, uint32_t seed, double *com,
  * whole step; t[4] = mean interacting pairs per step.  Same results as
  * g6_step. */
 int g6_step_timed(g6_handle h, int nsteps, double *t);
-/* The force kernel alone: nlaunch launches back to back in one hipGraph on
- * the state a g6_step call leaves (same forces each time), timed with HIP
- * events on the library stream; *ms = mean milliseconds per launch.  The
- * state is not moved.  Not with the barostat. */
+/* The force kernel alone: nlaunch launches back to back in one hipGraph
+ * (eager launches when G6_GRAPH_STEPS=0) on the state a g6_step call leaves
+ * (same forces each time), timed with HIP events on the library stream;
+ * *ms = mean milliseconds per launch.  The state is not moved.  Not with
+ * the barostat. */
 int g6_time_force(g6_handle h, int nlaunch, double *ms);
 /* The force kernel a system's MD path runs (no counterpart in the reference,
  * whose force routine has one form): info[0] = 1 for the cell-tile kernel
  * that evaluates each pair once, 0 for the rows of both directions;
  * info[1] = tile cells per axis, info[2] = workgroups per tile, info[3] =
  * waves per workgroup, info[4] = neighbourhood slots per tile (zeros for the
- * rows).  FP32 fixed-box systems with at least 256 cells of edge rc + skin
- * take the tiles; G6_TILES=0 in the environment at g6_init keeps the rows and
- * G6_TILE_MIN sets the cell threshold. */
+ * rows), info[5] = the capacity code that moved a tile system to the rows
+ * (3 cell residents, 4 neighbourhood slots or row entries, 2 image shift; 0
+ * when none did).  FP32 fixed-box systems with at least 256 cells of edge
+ * rc + skin take the tiles; G6_TILES=0 in the environment at g6_init keeps
+ * the rows and G6_TILE_MIN sets the cell threshold.  A system that a
+ * capacity error moved to the rows tries the tiles again at the first
+ * g6_step call 1000 steps later (G6_TILE_RETRY), the gap doubling after
+ * each failed try; systems in a ladder or under the barostat keep the rows. */
 int g6_md_kernel(g6_handle h, int32_t *info);
 int g6_model_nsite(int model);
 
