@@ -150,3 +150,70 @@ def test_autotuner_returns_a_measured_candidate():
                               skins=(0.5, 1.0), steps=20, warmup=5)
     assert len(table) == 4 and best in table
     assert all(r["ms_per_step"] > 0 for r in table)
+
+
+def test_c4_ladders_full_shape():
+    """The 64-replica workload: 4 TIP4P ladders (N 256, 512, 1024, 2048 at
+    0.92, 0.97, 1.00, 1.05 g/cm3) of 16 rungs 220-320 K (geometric), rc 9 A
+    switched over 0.5 A, FP32 pairs, 1 A skin.  Two 500-step intervals of
+    one ladder of all 64 equal 64 systems stepped alone, bit for bit, and
+    the exchange between them (30 trials per ladder over the adjacent-rung
+    chain) makes the oracle's decisions; the solo systems get the same
+    thermostat retargets."""
+    rc9 = 9.0
+    specs = [(256, 0.92, 20.266), (512, 0.97, 25.088), (1024, 1.00, 31.289),
+             (2048, 1.05, 38.786)]
+    temps = [220.0 * (320.0 / 220.0) ** (k / 15.0) for k in range(16)]
+
+    def one(n, Lb, T, seed):
+        s = g6.System(g6.TIP4P, n, Lb, rc9, TAPER, DT, prec=g6.FP32, skin=1.0)
+        com, q = lattice_start(n, Lb, seed)
+        s.upload(com, q)
+        s.set_nose(T, T * 2500.0)
+        s.vreset(T, seed)
+        return s
+
+    lad_sys, solo = [], []
+    for li, (n, rho, Lb) in enumerate(specs):
+        assert abs(box(n, rho) - Lb) < 2e-3
+        for k, T in enumerate(temps):
+            seed = 20261206 + 100 * li + k
+            lad_sys.append(one(n, Lb, T, seed))
+            solo.append(one(n, Lb, T, seed))
+    print("c4: 64 + 64 systems", flush=True)
+    lad = g6.Ladder(lad_sys)
+    groups, orcs = [], []
+    for li in range(4):
+        a, b = g6.rxpr_bonds(16, [(k, k + 1) for k in range(15)])
+        groups.append(g6.ReplicaExchange(16, (a, b), 30, 4242 + li,
+                                         systems=lad_sys[16 * li:16 * li + 16]))
+        orcs.append(O.RxRun(16, a, b, 30, 4242 + li))
+    accepted = 0
+    for interval in range(2):
+        print("c4 interval", interval, flush=True)
+        lad.step(500)
+        for s in solo:
+            s.step(500)
+        th = lad.thermo()
+        for r, (x, y) in enumerate(zip(lad_sys, solo)):
+            for u, v in zip(x.download(), y.download()):
+                assert np.array_equal(u, v), (interval, r)
+            assert np.array_equal(th[r], y.thermo()), (interval, r)
+        for li in range(4):
+            sl = slice(16 * li, 16 * li + 16)
+            tnow = th[sl, 29].copy()
+            beta_o, acc_o = orcs[li].trials(th[sl, 0], 1.0 / tnow)
+            acc = groups[li].exchange(th[sl])
+            assert acc == acc_o and list(groups[li].nodes()) == orcs[li].nic
+            accepted += acc
+            for r in range(16):
+                if beta_o[r] != 1.0 / tnow[r]:
+                    solo[16 * li + r].set_temperature(1.0 / beta_o[r])
+                assert lad_sys[16 * li + r].nose()[0] == solo[16 * li + r].nose()[0]
+    print("c4 accepted swaps", accepted, flush=True)
+    assert accepted > 0
+    lad.close()
+    for g in groups:
+        g.close()
+    for s in lad_sys + solo:
+        s.close()

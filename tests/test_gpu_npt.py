@@ -111,3 +111,81 @@ def test_npt_state_is_freed_with_its_system():
         assert abs(U - fresh) < 1e-10 * abs(fresh)
         s.npt_run(50)
         s.close()
+
+
+def c5_start(seed):
+    """1024 TIP4P at 1.0 g/cm3 (L = 31.289 A), rc 9 A switched over 0.5 A,
+    after 200 FP64 Nose steps at 250 K from the seeded lattice."""
+    n, L = 1024, 31.289
+    s = g6.System(g6.TIP4P, n, L, 9.0, TAPER, 0.002, voxa=9.0, prec=g6.FP64)
+    com, q = lattice_start(n, L, seed)
+    s.upload(com, q)
+    s.set_nose(250.0, 250.0 * 2500.0)
+    s.vreset(250.0, seed)
+    s.step(200)
+    c, q5, _, _ = s.download()
+    return s, c, q5[0].copy(), L
+
+
+def test_c5_plain_chain_matches_oracle():
+    """The NPT workload at full size (1024 TIP4P, 250 K, 1 bar, rc 9 A, block
+    cells of the 9 A Address_Assign grid, volume moves with probability
+    1/28): 300 plain moves equal the oracle's chain bit for bit."""
+    s, com, q, L = c5_start(61)
+    args = (250.0, 1.0, 20261207, 0.15, 8.0, 0.005, 1.0 / 28.0, 9.0)
+    s.npt_begin(*args)
+    _, _, L0, U0 = s.npt_state()
+    orc = O.NptRun(g6.TIP4P, com, q, L, 9.0, TAPER, 9.0, 250.0, 1.0, 20261207, 0.15, 8.0,
+                   0.005, 1.0 / 28.0, energy=U0)
+    for rnd in range(3):
+        print("c5 plain round", rnd, flush=True)
+        st = s.npt_run(100)
+        orc.run(100)
+        c, qq, Lg, Ug = s.npt_state()
+        assert st["proposals"] == orc.counts[0] and st["accepted"] == orc.counts[1]
+        assert st["volume_proposals"] == orc.counts[2]
+        assert st["volume_accepted"] == orc.counts[3]
+        assert Lg == orc.L
+        assert np.array_equal(c, orc.com) and np.array_equal(qq, orc.quat)
+        assert abs(Ug - orc.U) < 1e-10 * abs(orc.U)
+    print("c5 plain", st, flush=True)
+    assert st["accepted"] > 0 and st["volume_proposals"] > 0
+    fresh = s.npt_energy()
+    assert abs(Ug - fresh) < 1e-10 * abs(fresh)
+    s.close()
+
+
+def test_c5_zero_network_rounds_of_32_match_oracle():
+    """Delayed acceptance at the workload's K = 32 with a network whose output
+    is identically zero (block moves only, displaced <= 0.05 A and turned
+    <= 2 deg so that some are accepted): every non-void proposal passes the
+    screen, so round r decides the first non-void proposal of indices
+    32 r .. 32 r + 31 on the exact energy.  The oracle's plain chain run on
+    exactly those proposals makes the same decisions, bit for bit."""
+    s, com, q, L = c5_start(62)
+    args = (250.0, 1.0, 99, 0.05, 2.0, 0.005, 0.0, 9.0)
+    s.npt_begin(*args)
+    _, _, _, U0 = s.npt_state()
+    net = g6.Mlp(np.zeros(g6.MLP_NPARAM, np.float32))
+    orc = O.NptRun(g6.TIP4P, com, q, L, 9.0, TAPER, 9.0, 250.0, 1.0, 99, 0.05, 2.0, 0.005,
+                   0.0, energy=U0)
+    R = 100
+    b = s.npt_run(R, mlp=net, K=32)
+    print("c5 zero network", b, flush=True)
+    assert b["rounds"] == R and b["next_index"] == 32 * R
+    decided = 0
+    for r in range(R):
+        orc.index = 32 * r
+        while orc.index < 32 * (r + 1):
+            void = orc.counts[4]
+            orc.run(1)
+            if orc.counts[4] == void:
+                decided += 1
+                break
+    assert b["surrogate_passes"] == b["exact_evaluations"] == decided
+    assert b["accepted"] == orc.counts[1] and b["accepted"] > 0
+    c, qq, Lg, Ug = s.npt_state()
+    assert np.array_equal(c, orc.com) and np.array_equal(qq, orc.quat) and Lg == orc.L
+    assert abs(Ug - orc.U) < 1e-10 * abs(orc.U)
+    net.close()
+    s.close()
