@@ -341,7 +341,77 @@ def models_run(steps, skin):
     return out
 
 
-def npt_run(configs, epochs, moves, rounds, K, p_volume, log):
+def by_move_type(plain, da):
+    """Stage-1 (surrogate) passes and stage-2 (exact) acceptances of the
+    delayed-acceptance chain by move type, beside the plain chain's
+    acceptance of each type."""
+    pv, pb = plain["volume_proposals"], plain["proposals"] - plain["volume_proposals"]
+    dv, db = da["volume_proposals"], da["proposals"] - da["volume_proposals"]
+    s1v = da["volume_surrogate_passes"]
+    s1b = da["surrogate_passes"] - s1v
+    av, ab = da["volume_accepted"], da["accepted"] - da["volume_accepted"]
+    frac = lambda a, b: a / b if b else None
+    return {
+        "plain": {"volume_acceptance": frac(plain["volume_accepted"], pv),
+                  "block_acceptance": frac(plain["accepted"] - plain["volume_accepted"], pb)},
+        "delayed_acceptance": {
+            "volume": {"proposals": dv, "stage1_passes": s1v, "stage2_accepts": av,
+                       "stage1_pass_rate": frac(s1v, dv), "stage2_accept_rate": frac(av, s1v),
+                       "acceptance": frac(av, dv)},
+            "block": {"proposals": db, "stage1_passes": s1b, "stage2_accepts": ab,
+                      "stage1_pass_rate": frac(s1b, db), "stage2_accept_rate": frac(ab, s1b),
+                      "acceptance": frac(ab, db)}}}
+
+
+def npt_ensemble(net, moves, rounds, K, p_volume, chains, log):
+    """<V> and <U> of the plain and delayed-acceptance chains: `chains`
+    independent pairs of chains, each pair from its own MD start (3000 Nose
+    steps at 250 K from its own lattice seed) and its own proposal stream,
+    both chains of a pair from the same 2000-move plain equilibration; the
+    box volume and energy are read 20 times along each chain.  Errors are the
+    standard errors over the independent chains."""
+    from genesis6_amd import g6
+    n, L, T, P = 1024, 31.289, 250.0, 1.0
+    means = {"plain": [], "delayed_acceptance": []}
+    for c in range(chains):
+        s = g6.System(g6.TIP4P, n, L, RC, TAPER, DT, prec=g6.FP32, skin=1.0)
+        com, q = g6.lattice_start(n, L, 20261300 + c)
+        s.upload(com, q)
+        s.set_nose(T, T * 2500.0)
+        s.vreset(T, 20261300 + c)
+        s.step(3000)
+        for kind, count in (("plain", moves), ("delayed_acceptance", rounds)):
+            s.npt_begin(T, P, 20261400 + c, 0.15, 8.0, 0.005, p_volume, 9.0)
+            s.npt_run(2000)
+            vs, us = [], []
+            for _ in range(20):
+                if kind == "plain":
+                    s.npt_run(count // 20)
+                else:
+                    s.npt_run(count // 20, mlp=net, K=K)
+                _, _, Lc, Uc = s.npt_state()
+                vs.append(Lc ** 3)
+                us.append(Uc)
+            means[kind].append((float(np.mean(vs)), float(np.mean(us))))
+        s.close()
+    out = {"chains": chains, "unit_volume": "A^3", "unit_energy": "internal (10 J/mol)"}
+    for kind, m in means.items():
+        a = np.array(m)
+        se = a.std(0, ddof=1) / np.sqrt(len(a)) if len(a) > 1 else np.zeros(2)
+        out[kind] = {"V": float(a[:, 0].mean()), "V_err": float(se[0]),
+                     "U": float(a[:, 1].mean()), "U_err": float(se[1]),
+                     "per_chain": m}
+    p, d = out["plain"], out["delayed_acceptance"]
+    for q in ("V", "U"):
+        err = np.hypot(p[q + "_err"], d[q + "_err"])
+        out["z_" + q] = float((d[q] - p[q]) / err) if err > 0 else None
+    log("npt ensemble: <V> plain %.0f +- %.0f, DA %.0f +- %.0f; <U> plain %.0f +- %.0f, "
+        "DA %.0f +- %.0f" % (p["V"], p["V_err"], d["V"], d["V_err"], p["U"], p["U_err"],
+                             d["U"], d["U_err"]))
+    return out
+
+
+def npt_run(configs, epochs, moves, rounds, K, p_volume, log, chains=4):
     """1024 TIP4P molecules, NPT Monte Carlo at 250 K and 1 bar (L = 31.289 A
     at the start, rc 9.0 A switched over 0.5 A, FP64 PairInfo energies):
     volume moves ln V +- 0.5%, block moves of every molecule of one
@@ -392,6 +462,7 @@ def npt_run(configs, epochs, moves, rounds, K, p_volume, log):
     da = {k: b[k] - a[k] for k in b}
     _, _, Lf, Uf = s.npt_state()
     s.close()
+    ens = npt_ensemble(net, moves, rounds, K, p_volume, chains, log)
     net.close()
     vp = plain["accepted"] / el_p
     vd = da["accepted"] / el_d
@@ -400,6 +471,8 @@ def npt_run(configs, epochs, moves, rounds, K, p_volume, log):
             "K": K, "p_volume": p_volume,
             "plain_counts": dict(plain, seconds=el_p),
             "delayed_acceptance_counts": dict(da, seconds=el_d),
+            "by_move_type": by_move_type(plain, da),
+            "ensemble": ens,
             "training": {"configurations": rows // n, "rows": rows, "epochs": epochs,
                          "loss_first_epoch": float(losses[0]), "loss_last_epoch": float(losses[-1]),
                          "data_seconds": t_data, "train_seconds": t_train},

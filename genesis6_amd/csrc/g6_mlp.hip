@@ -1,14 +1,20 @@
 // The energy surrogate's network on the matrix cores: inference, the
 // gradient of the training loss, and Adam, all hand-written.
 //
-// A wave owns a tile of 16 feature rows.  Layer products run as
-// v_mfma_f32_16x16x16_bf16 (A: lane l holds row l%16, k = 4(l/16)..+3;
-// B: k = 4(l/16)..+3 of column l%16; D: rows 4(l/16)..+3 of column l%16),
-// activations cross from the D layout to the A layout through LDS as BF16.
-// The backward pass reuses the D layout directly as the transposed A
-// operand of the weight gradients (dW = dZ^T H contracts over the tile's
-// rows).  Gradient partials are summed in a fixed order (waves of a block,
-// then blocks), so a gradient is bitwise reproducible.
+// A wave owns a tile of 16 feature rows.  The forward layer products run as
+// gfx950's v_mfma_f32_16x16x32_bf16 (A: lane l holds row l%16, k =
+// 8(l/16)..+7; B: k = 8(l/16)..+7 of column l%16; D: rows 4(l/16)..+3 of
+// column l%16).  Their activation operands are split into two BF16 terms,
+// hi = bf16(v) and lo = bf16(v - hi) (16 significant bits), each multiplied
+// by the BF16 weights: a single BF16 rounding (8 bits) would make the
+// network's output step by the rounding of its inputs, and a volume move's
+// energy change (every feature moved by ~0.2 %) would be lost in those
+// steps.  Activations cross from the D layout to the A layout through LDS.
+// The backward pass runs v_mfma_f32_16x16x16_bf16 on the hi terms and reuses
+// the D layout directly as the transposed A operand of the weight gradients
+// (dW = dZ^T H contracts over the tile's 16 rows).  Gradient partials are
+// summed in a fixed order (waves of a block, then blocks), so a gradient is
+// bitwise reproducible.
 #include <math.h>
 #include <string.h>
 
@@ -22,6 +28,8 @@ namespace g6 {
 
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8b __attribute__((ext_vector_type(8)));
+typedef unsigned short v8u __attribute__((ext_vector_type(8)));
 
 // FP32 -> BF16, round to nearest even (finite inputs)
 __device__ inline uint16_t f2bf(float f) {
@@ -48,6 +56,14 @@ __device__ inline v4s lds4(const uint16_t *p) { return pack4(p[0], p[1], p[2], p
 __device__ inline v4f mfma(v4s a, v4s b, v4f c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
+// eight consecutive BF16 values (16-byte aligned) as an MFMA operand
+__device__ inline v8b lds8(const uint16_t *p) {
+  return __builtin_bit_cast(v8b, *reinterpret_cast<const v8u *>(p));
+}
+__device__ inline v4f mfma32(v8b a, v8b b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+__device__ inline float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 __device__ inline void wsync_m() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -69,9 +85,10 @@ __global__ __launch_bounds__(256) void k_mlp(const float *__restrict__ P,
                                              const int *__restrict__ idx, float *__restrict__ y,
                                              float *__restrict__ part, int seg,
                                              const int *__restrict__ segcnt) {
-  __shared__ uint16_t s_w1[kMlpHid * kMlpIn], s_w2[kMlpHid * kMlpHid];
+  __shared__ __align__(16) uint16_t s_w1[kMlpHid * kMlpIn], s_w2[kMlpHid * kMlpHid];
   __shared__ float s_b1[kMlpHid], s_b2[kMlpHid], s_w3[kMlpHid], s_mean[kMlpIn], s_istd[kMlpIn];
-  __shared__ uint16_t s_xn[4][16 * kMlpIn], s_h1[4][16 * kMlpHid];
+  __shared__ __align__(16) uint16_t s_xn[4][16 * kMlpIn], s_h1[4][16 * kMlpHid];
+  __shared__ __align__(16) uint16_t s_h1lo[4][16 * kMlpHid];
   __shared__ uint16_t s_dz[GRAD ? 4 : 1][GRAD ? 16 * kMlpHid : 1];
   __shared__ float s_acc[GRAD ? kMlpParams + 1 : 1];
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
@@ -102,7 +119,7 @@ __global__ __launch_bounds__(256) void k_mlp(const float *__restrict__ P,
     for (int k = 0; k < 2; k++) gW1[c][k] = v4f{0, 0, 0, 0};
   }
   const long long ntile = (m + 15) / 16;
-  uint16_t *xn = s_xn[wid], *h1 = s_h1[wid];
+  uint16_t *xn = s_xn[wid], *h1 = s_h1[wid], *h1lo = s_h1lo[wid];
   for (long long tile = (long long)blockIdx.x * 4 + wid; tile < ntile;
        tile += (long long)gridDim.x * 4) {
     const long long row0 = tile * 16;
@@ -110,48 +127,61 @@ __global__ __launch_bounds__(256) void k_mlp(const float *__restrict__ P,
     // past its segment's live rows is skipped
     if (segcnt && row0 / seg == (row0 + 15) / seg && row0 % seg >= segcnt[row0 / seg]) continue;
     wsync_m();  // the previous tile's LDS reads are done
+    // layer 1: z1 = W1 xn + b1; the lane's A operand is its own row r16,
+    // features 8q..8q+7, normalised in FP32 and split into hi + lo
+    v4f z1[4];
     {
-      // lane: row r16, features 8q..8q+7, normalised, to BF16
       const long long r = row0 + r16;
       const bool ok = r < m;
       const long long src = ok ? (idx ? (long long)idx[r] : r) : 0;
+      v8u ahi, alo;
 #pragma unroll
       for (int j = 0; j < 8; j++) {
         const int f = 8 * q + j;
         const float v = ok ? (x[src * kMlpIn + f] - s_mean[f]) * s_istd[f] : 0.0f;
-        xn[r16 * kMlpIn + f] = f2bf(v);
+        const uint16_t h = f2bf(v);
+        ahi[j] = h;
+        alo[j] = f2bf(v - bf2f(h));
+        if (GRAD) xn[r16 * kMlpIn + f] = h;  // the weight gradient's operand
       }
-    }
-    wsync_m();
-    // layer 1: z1 = W1 xn + b1
-    v4f z1[4];
-    {
-      const v4s a0 = lds4(xn + r16 * kMlpIn + 4 * q), a1 = lds4(xn + r16 * kMlpIn + 16 + 4 * q);
+      const v8b a_hi = __builtin_bit_cast(v8b, ahi), a_lo = __builtin_bit_cast(v8b, alo);
 #pragma unroll
       for (int ct = 0; ct < 4; ct++) {
-        const uint16_t *w = s_w1 + (16 * ct + r16) * kMlpIn + 4 * q;
+        const v8b w = lds8(s_w1 + (16 * ct + r16) * kMlpIn + 8 * q);
         v4f acc = {0, 0, 0, 0};
-        acc = mfma(a0, lds4(w), acc);
-        acc = mfma(a1, lds4(w + 16), acc);
+        acc = mfma32(a_hi, w, acc);
+        acc = mfma32(a_lo, w, acc);
         const float bb = s_b1[16 * ct + r16];
         z1[ct] = v4f{acc.x + bb, acc.y + bb, acc.z + bb, acc.w + bb};
 #pragma unroll
-        for (int i = 0; i < 4; i++) h1[(4 * q + i) * kMlpHid + 16 * ct + r16] = f2bf(silu(z1[ct][i]));
+        for (int i = 0; i < 4; i++) {
+          const float hv = silu(z1[ct][i]);
+          const uint16_t h = f2bf(hv);
+          h1[(4 * q + i) * kMlpHid + 16 * ct + r16] = h;
+          h1lo[(4 * q + i) * kMlpHid + 16 * ct + r16] = f2bf(hv - bf2f(h));
+        }
       }
     }
     wsync_m();
-    // layer 2: z2 = W2 h1 + b2
+    // layer 2: z2 = W2 (h1_hi + h1_lo) + b2, k = 32 kb + 8q..+7
     v4f z2[4];
     {
-      v4s a2[4];
+      v8b ah[2], al[2];
 #pragma unroll
-      for (int kb = 0; kb < 4; kb++) a2[kb] = lds4(h1 + r16 * kMlpHid + 16 * kb + 4 * q);
+      for (int kb = 0; kb < 2; kb++) {
+        ah[kb] = lds8(h1 + r16 * kMlpHid + 32 * kb + 8 * q);
+        al[kb] = lds8(h1lo + r16 * kMlpHid + 32 * kb + 8 * q);
+      }
 #pragma unroll
       for (int ct = 0; ct < 4; ct++) {
-        const uint16_t *w = s_w2 + (16 * ct + r16) * kMlpHid + 4 * q;
+        const uint16_t *w = s_w2 + (16 * ct + r16) * kMlpHid + 8 * q;
         v4f acc = {0, 0, 0, 0};
 #pragma unroll
-        for (int kb = 0; kb < 4; kb++) acc = mfma(a2[kb], lds4(w + 16 * kb), acc);
+        for (int kb = 0; kb < 2; kb++) {
+          const v8b wb = lds8(w + 32 * kb);
+          acc = mfma32(ah[kb], wb, acc);
+          acc = mfma32(al[kb], wb, acc);
+        }
         const float bb = s_b2[16 * ct + r16];
         z2[ct] = v4f{acc.x + bb, acc.y + bb, acc.z + bb, acc.w + bb};
       }
@@ -295,8 +325,17 @@ __global__ void k_mlp_reduce(const float *__restrict__ part, int nb, float *__re
                              long long m) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k > kMlpParams) return;
+  // block order; sixteen loads in flight at a time
   float s = 0.0f;
-  for (int b = 0; b < nb; b++) s += part[(size_t)b * (kMlpParams + 1) + k];
+  int b = 0;
+  for (; b + 16 <= nb; b += 16) {
+    float v[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) v[u] = part[(size_t)(b + u) * (kMlpParams + 1) + k];
+#pragma unroll
+    for (int u = 0; u < 16; u++) s += v[u];
+  }
+  for (; b < nb; b++) s += part[(size_t)b * (kMlpParams + 1) + k];
   grad[k] = k == kMlpParams ? s / (float)m : s;
 }
 

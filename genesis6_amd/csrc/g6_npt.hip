@@ -895,6 +895,7 @@ __global__ void k_da_select(NptArgs A) {
   A.cnt[2] += vols;
   A.cnt[7] += voids;
   A.cnt[4] += sel >= 0 ? 1 : 0;
+  A.cnt[9] += (sel >= 0 && (A.vflag[sel] & 2)) ? 1 : 0;  // volume moves among the passes
   A.cnt[6] += 1;
   A.ist[2] = sel;
   A.ist[1] = 0;
@@ -1293,7 +1294,7 @@ int g6_npt_run(g6_handle h, int count, int64_t hm, int K, long long *stats) {
   G6_CHECK(hipMemcpy(c, p->a.cnt, sizeof(c), hipMemcpyDeviceToHost));
   G6_CHECK(hipMemcpy(&derr, p->a.derr, sizeof(int), hipMemcpyDeviceToHost));
   if (stats)
-    for (int k = 0; k < 9; k++) stats[k] = c[k];
+    for (int k = 0; k < 10; k++) stats[k] = c[k];
   // void proposals (a block of more than kNptMaxBlock members, a member
   // leaving its cell, a box below 2 rc) are rejections: the reverse of a void
   // move is void too.  A descriptor neighbour list that overflowed is an error.

@@ -286,11 +286,12 @@ class System:
     def npt_run(self, count, mlp=None, K=16):
         """count plain moves (mlp None) or delayed-acceptance rounds of K
         proposals; returns the cumulative counters (dict)."""
-        st = np.zeros(9, np.int64)
+        st = np.zeros(10, np.int64)
         check(lib().g6_npt_run(self.h, count, mlp.h if mlp is not None else 0, K, st),
               "g6_npt_run")
         keys = ["proposals", "accepted", "volume_proposals", "volume_accepted",
-                "surrogate_passes", "exact_evaluations", "rounds", "void", "next_index"]
+                "surrogate_passes", "exact_evaluations", "rounds", "void", "next_index",
+                "volume_surrogate_passes"]
         return dict(zip(keys, (int(v) for v in st)))
 
     def npt_state(self):

@@ -369,9 +369,9 @@ int g6_npt_begin(g6_handle h, double temp, double press_bar, uint64_t seed, doub
  * 6 A from beyond the round's 6.5 A list: G6_EARG if twice the largest O
  * move of a block (disp_max plus the turn of the O about the COM) reaches
  * 0.5 A.
- * stats[9] (cumulative): proposals, accepted, volume proposals, volume
+ * stats[10] (cumulative): proposals, accepted, volume proposals, volume
  * accepted, surrogate passes, exact evaluations, rounds, void proposals,
- * next proposal index.  G6_ECAPACITY if a molecule had more than 96
+ * next proposal index, volume moves among the surrogate passes.  G6_ECAPACITY if a molecule had more than 96
  * neighbours within 6.5 A for the descriptors. */
 int g6_npt_run(g6_handle h, int count, int64_t hm, int K, long long *stats);
 /* com[3n], quat1[4n], box edge, PairInfo energy sum (internal units) */

@@ -3,11 +3,12 @@
 Test infrastructure only: imported by tests/ and nothing in the product.
 The network has no counterpart in the reference: this module restates the
 definition the device kernels implement (an MLP 32 -> 64 -> 64 -> 1 with
-SiLU, BF16-rounded operands of every matrix product, the loss
-mean((o - (t - y_shift)/y_scale)^2)), so that the kernels' forward pass and
-gradient can be compared with it.  With quant=False every BF16 rounding is
-skipped and the arithmetic is plain float64: that variant is pinned by
-central finite differences of its own loss.
+SiLU; BF16 weights; forward activations as two BF16 terms hi = bf16(v),
+lo = bf16(v - hi); the backward products on BF16 operands (the hi terms);
+the loss mean((o - (t - y_shift)/y_scale)^2)), so that the kernels' forward
+pass and gradient can be compared with it.  With quant=False every BF16
+rounding is skipped and the arithmetic is plain float64: that variant is
+pinned by central finite differences of its own loss.
 
 Parameter layout: W1[64][32], b1[64], W2[64][64], b2[64], w3[64], b3.
 Normalisation: mean[32], inv_std[32], y_shift, y_scale.
@@ -55,6 +56,15 @@ def _q(a, quant):
     return bf16(a) if quant else np.asarray(a, np.float64)
 
 
+def split(a):
+    """(hi, lo) = (bf16(v), bf16(v - hi)) of float32 values v: the two-term
+    BF16 form of a forward activation (the difference is exact in FP32)."""
+    v = np.asarray(a, np.float32)
+    hi = bf16(v)
+    lo = bf16((v.astype(np.float64) - hi).astype(np.float32))
+    return hi, lo
+
+
 def forward(p, norm, x, quant=True, keep=False):
     """Energies y[m] = y_scale * o + y_shift of feature rows x[m][32]."""
     W1, b1, W2, b2, w3, b3 = unpack(p)
@@ -62,12 +72,16 @@ def forward(p, norm, x, quant=True, keep=False):
     mean, istd, ysh, ysc = norm[:NIN], norm[NIN:2 * NIN], norm[2 * NIN], norm[2 * NIN + 1]
     x = np.asarray(x, np.float64)
     if quant:  # the kernel normalises in FP32
-        xn = bf16((x.astype(np.float32) - mean.astype(np.float32)) * istd.astype(np.float32))
+        xh, xl = split((x.astype(np.float32) - mean.astype(np.float32)) * istd.astype(np.float32))
+        z1 = (xh + xl) @ bf16(W1).T + b1
+        hh, hl = split(silu(z1))
+        z2 = (hh + hl) @ bf16(W2).T + b2
+        xn, h1 = xh, hh       # the backward's operands
     else:
         xn = (x - mean) * istd
-    z1 = xn @ _q(W1, quant).T + b1
-    h1 = _q(silu(z1), quant)
-    z2 = h1 @ _q(W2, quant).T + b2
+        z1 = xn @ W1.T + b1
+        h1 = silu(z1)
+        z2 = h1 @ W2.T + b2
     o = silu(z2) @ w3 + b3
     y = ysc * o + ysh
     if keep:

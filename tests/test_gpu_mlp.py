@@ -62,3 +62,16 @@ def test_training_reduces_the_loss():
     l1, _ = net.loss_grad(x, t)
     print("loss %.4f -> %.4f" % (l0, l1), flush=True)
     assert losses[-1] < losses[0] and l1 < 0.1 * l0
+
+
+def test_forward_resolves_small_input_changes():
+    """The kernel's two-term BF16 activations: the output change under a
+    0.1 % scaling of every feature (a volume move's size) follows the float64
+    network's to a few per cent."""
+    p, norm, x, _ = _case(9, 4000)
+    x2 = (x * np.float32(1.001)).astype(np.float32)
+    net = g6.Mlp(p, norm)
+    d = net.forward(x2).astype(np.float64) - net.forward(x)
+    d64 = (M.forward(p.astype(np.float64), norm, x2, quant=False)
+           - M.forward(p.astype(np.float64), norm, x, quant=False))
+    assert rel(d, d64) < 0.05

@@ -52,3 +52,26 @@ def test_quantised_oracle_stays_close_to_float64():
     l1, g1 = M.loss_grad(p, norm, x, t, quant=True)
     assert abs(l1 - l0) < 0.02 * l0
     assert np.linalg.norm(g1 - g0) < 0.05 * np.linalg.norm(g0)
+
+
+def test_split_forward_resolves_small_input_changes():
+    """The two-term BF16 activations keep 16 significant bits: the oracle's
+    output change under a 0.1 % scaling of every feature follows the float64
+    network's to a few per cent, where single BF16 operands (8 bits) lose
+    it in rounding steps."""
+    p, norm, x, _ = _case(5, 4000)
+    x2 = (x * np.float32(1.001)).astype(np.float32)
+    d64 = M.forward(p, norm, x2, quant=False) - M.forward(p, norm, x, quant=False)
+    dq = M.forward(p, norm, x2) - M.forward(p, norm, x)
+    assert np.linalg.norm(dq - d64) < 0.05 * np.linalg.norm(d64)
+    # single BF16 operands for comparison
+    W1, b1, W2, b2, w3, b3 = M.unpack(p)
+    nm = np.asarray(norm, np.float64)
+
+    def one_term(xx):
+        xn = M.bf16((xx - nm[:M.NIN].astype(np.float32)) * nm[M.NIN:2 * M.NIN].astype(np.float32))
+        h1 = M.bf16(M.silu(xn @ M.bf16(W1).T + b1))
+        return M.silu(h1 @ M.bf16(W2).T + b2) @ w3 + b3
+
+    d1 = one_term(x2) - one_term(x)
+    assert np.linalg.norm(d1 * nm[-1] - d64) > 0.3 * np.linalg.norm(d64)
