@@ -38,6 +38,7 @@ SEED = 20261204
 # Coulomb site pair 21, LJ site pair 28, 40 per molecule pair
 FLOP_PER_PAIR = {"tip4p": 9 * 21 + 28 + 40, "tip5p": 16 * 21 + 28 + 40}
 FP32_PEAK = 157.3e12    # MI355X vector FP32 (vendor, packed)
+FP32_PEAK_NONPACKED = 78.6e12  # without packed FP32 instructions (not used by these kernels)
 
 
 def log(msg):
@@ -93,7 +94,8 @@ def roofline(flop_per_pair, pairs, kernel_ms, kernel, traffic, **extra):
     out = {"bound": "valu", "achieved": achieved / 1e12, "peak": FP32_PEAK / 1e12,
            "unit": "TFLOP/s", "frac": achieved / FP32_PEAK, "traffic": traffic,
            "kernel": kernel, "kernel_ms": kernel_ms, "flop_per_pair": flop_per_pair,
-           "pairs_per_launch": float(pairs)}
+           "pairs_per_launch": float(pairs),
+           "frac_of_nonpacked_peak": achieved / FP32_PEAK_NONPACKED}
     out.update(extra)
     return out
 

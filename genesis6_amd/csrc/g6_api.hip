@@ -448,6 +448,7 @@ static void tile_plan(System &s, double nb_full) {
     lay = tile_layout(nq, nbmax, cap, waves, stage);
   }
   if (lay.bytes + fixed > lds_max) return;
+  lay.xcd = env_int("G6_TILE_XCD", 1) != 0 ? 1 : 0;
   s.tile_ndiv = nd;
   s.tile_cap = cap;
   s.tile_waves = waves;

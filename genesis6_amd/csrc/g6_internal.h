@@ -177,6 +177,7 @@ struct McHost {
 // a buffer for the next member's row.
 struct TileLayout {
   int nbmax, cx, cy, cz, acc, id, site, ring, rowbuf, order, bytes, stage;
+  int xcd;  // workgroups of one XCD take a contiguous run of tiles
 };
 TileLayout tile_layout(int nq, int nbmax, int cap, int waves, bool stage);
 

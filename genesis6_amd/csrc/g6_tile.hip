@@ -109,7 +109,16 @@ __device__ inline void sites_lds(const float *s_site, int nbmax, int q,
 
 template <int M, bool STAGE>
 __device__ __forceinline__ void tile_body(const Geom &g, const MdBuf &b, const PairK<float> &pk,
-                                          int parity, const TileLayout &lay, int blk, int nblk) {
+                                          int parity, const TileLayout &lay, int blk_hw, int nblk) {
+  // Workgroups are dealt to the eight XCDs round-robin by index.  With xcd,
+  // the workgroups of one XCD (index mod 8) take one contiguous run of
+  // tiles, a slab of the box in z-major order, so the records its tiles
+  // share stay in that XCD's L2.
+  int blk = blk_hw;
+  if (lay.xcd) {
+    const int q = nblk >> 3, r = nblk & 7, x = blk_hw & 7, p = blk_hw >> 3;
+    blk = x < r ? x * (q + 1) + p : r * (q + 1) + (x - r) * q + p;
+  }
   using T = Traits<M>;
   constexpr int NQ = T::COM;
   extern __shared__ __align__(16) unsigned char s_dyn[];
