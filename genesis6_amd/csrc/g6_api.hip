@@ -552,6 +552,19 @@ static int reset_step_state(System &s) {
   return 0;
 }
 
+int seam_reserve(System &s, size_t bytes, char **base) {
+  if (bytes > s.seam_cap) {
+    G6_CHECK(hipStreamSynchronize(s.stream));
+    if (s.seam) (void)hipFree(s.seam);
+    s.seam = nullptr;
+    s.seam_cap = 0;
+    G6_CHECK(hipMalloc((void **)&s.seam, bytes));
+    s.seam_cap = bytes;
+  }
+  *base = s.seam;
+  return 0;
+}
+
 // Back to the tiles after a capacity fallback (a lattice start can fill
 // some tile neighbourhoods far above a liquid's counts for a few steps).
 static int use_tile_grid(System &s) {
@@ -588,7 +601,7 @@ static void free_all(System &s) {
                 m.blockek, m.frc, m.nose, m.thermo, m.flags, s.cell, s.cell_count, s.cell_start,
                 s.cell_fill, s.sorted_tmp, s.sorted, s.cnt, s.off, s.pi, s.pj, s.scode,
                 s.com_ref, s.lflags, m.box, m.anders, s.tile_part, m.nb_id, m.nb_base,
-                m.trow, m.trow_cnt, m.tcnt, s.snap};
+                m.trow, m.trow_cnt, m.tcnt, s.snap, s.seam};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   mc_free(s);

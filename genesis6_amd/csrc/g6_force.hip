@@ -111,15 +111,19 @@ int force_pairs(System &s, int64_t np, const int32_t *pi_h, const int32_t *pj_h,
   const int nsites = s.n * s.ns;
   const size_t pb = sizeof(double) * (size_t)(np > 0 ? np : 1);
   const size_t sb = sizeof(double) * (size_t)nsites;
-  int *pi = nullptr, *pj = nullptr;
+  const size_t ib = sizeof(int) * (size_t)(np > 0 ? np : 1);
+  char *base = nullptr;
+  const int rs = seam_reserve(s, 2 * seam_piece(ib) + 6 * seam_piece(pb) + 6 * seam_piece(sb) +
+                                     seam_piece(sizeof(double) * 9),
+                              &base);
+  if (rs != 0) return rs;
+  Carve cv{base};
+  int *pi = cv.take<int>(np > 0 ? np : 1), *pj = cv.take<int>(np > 0 ? np : 1);
   double *d[12];
-  double *vir = nullptr;
-  G6_CHECK(hipMalloc(&pi, sizeof(int) * (size_t)(np > 0 ? np : 1)));
-  G6_CHECK(hipMalloc(&pj, sizeof(int) * (size_t)(np > 0 ? np : 1)));
-  for (int a = 0; a < 5; a++) G6_CHECK(hipMalloc(&d[a], pb));
-  for (int a = 5; a < 11; a++) G6_CHECK(hipMalloc(&d[a], sb));
-  G6_CHECK(hipMalloc(&d[11], pb));
-  G6_CHECK(hipMalloc(&vir, sizeof(double) * 9));
+  for (int a = 0; a < 5; a++) d[a] = cv.take<double>(pb / sizeof(double));
+  for (int a = 5; a < 11; a++) d[a] = cv.take<double>(sb / sizeof(double));
+  d[11] = cv.take<double>(pb / sizeof(double));
+  double *vir = cv.take<double>(9);
   hipStream_t st = s.stream;
   if (np > 0) {
     G6_CHECK(hipMemcpyAsync(pi, pi_h, sizeof(int) * np, hipMemcpyHostToDevice, st));
@@ -164,10 +168,6 @@ int force_pairs(System &s, int64_t np, const int32_t *pi_h, const int32_t *pj_h,
     G6_CHECK(hipMemcpyAsync(vrmat, vir, sizeof(double) * 9, hipMemcpyDeviceToHost, st));
   }
   G6_CHECK(hipStreamSynchronize(st));
-  G6_CHECK(hipFree(pi));
-  G6_CHECK(hipFree(pj));
-  for (int a = 0; a < 12; a++) G6_CHECK(hipFree(d[a]));
-  G6_CHECK(hipFree(vir));
   return 0;
 }
 

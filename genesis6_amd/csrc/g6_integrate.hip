@@ -104,10 +104,14 @@ int cutoff_once(System &s, int64_t np, const int32_t *pi_h, const int32_t *pj_h,
   const int n = s.n;
   for (int64_t k = 0; k < np; k++)  // indices come from the caller: check them
     if (pi_h[k] < 0 || pi_h[k] >= n || pj_h[k] < 0 || pj_h[k] >= n) return G6_EARG;
-  int *ij = nullptr;
-  double *d = nullptr;
-  G6_CHECK(hipMalloc(&ij, sizeof(int) * 2 * np));
-  G6_CHECK(hipMalloc(&d, sizeof(double) * (3 * n + 5 * np)));
+  char *base = nullptr;
+  const int rs = seam_reserve(s, seam_piece(sizeof(int) * 2 * np) +
+                                     seam_piece(sizeof(double) * (3 * n + 5 * np)),
+                              &base);
+  if (rs != 0) return rs;
+  Carve cv{base};
+  int *ij = cv.take<int>(2 * (size_t)np);
+  double *d = cv.take<double>(3 * (size_t)n + 5 * (size_t)np);
   double *com = d, *o = d + 3 * n, *er = o + 3 * np, *fr = er + np;
   G6_CHECK(hipMemcpyAsync(ij, pi_h, sizeof(int) * np, hipMemcpyHostToDevice, s.stream));
   G6_CHECK(hipMemcpyAsync(ij + np, pj_h, sizeof(int) * np, hipMemcpyHostToDevice, s.stream));
@@ -121,16 +125,16 @@ int cutoff_once(System &s, int64_t np, const int32_t *pi_h, const int32_t *pj_h,
   G6_CHECK(hipMemcpyAsync(er_h, er, sizeof(double) * np, hipMemcpyDeviceToHost, s.stream));
   G6_CHECK(hipMemcpyAsync(fr_h, fr, sizeof(double) * np, hipMemcpyDeviceToHost, s.stream));
   G6_CHECK(hipStreamSynchronize(s.stream));
-  G6_CHECK(hipFree(ij));
-  G6_CHECK(hipFree(d));
   return 0;
 }
 
 int sites_once(System &s, const double *com_h, const double *q_h, double *x, double *y,
                double *z) {
   const int n = s.n, ns = s.ns;
-  double *d = nullptr;
-  G6_CHECK(hipMalloc(&d, sizeof(double) * (7 * n + 3 * n * ns)));
+  char *base = nullptr;
+  const int rs = seam_reserve(s, seam_piece(sizeof(double) * (7 * n + 3 * n * ns)), &base);
+  if (rs != 0) return rs;
+  double *d = reinterpret_cast<double *>(base);
   double *com = d, *q = d + 3 * n, *o = d + 7 * n;
   G6_CHECK(hipMemcpyAsync(com, com_h, sizeof(double) * 3 * n, hipMemcpyHostToDevice, s.stream));
   G6_CHECK(hipMemcpyAsync(q, q_h, sizeof(double) * 4 * n, hipMemcpyHostToDevice, s.stream));
@@ -141,15 +145,16 @@ int sites_once(System &s, const double *com_h, const double *q_h, double *x, dou
   G6_CHECK(hipMemcpyAsync(z, o + 2 * n * ns, sizeof(double) * n * ns, hipMemcpyDeviceToHost,
                           s.stream));
   G6_CHECK(hipStreamSynchronize(s.stream));
-  G6_CHECK(hipFree(d));
   return 0;
 }
 
 int collect_once(System &s, const double *q_h, const double *fx, const double *fy,
                  const double *fz, double *force, double *torque) {
   const int n = s.n, ns = s.ns;
-  double *d = nullptr;
-  G6_CHECK(hipMalloc(&d, sizeof(double) * (10 * n + 3 * n * ns)));
+  char *base = nullptr;
+  const int rs = seam_reserve(s, seam_piece(sizeof(double) * (10 * n + 3 * n * ns)), &base);
+  if (rs != 0) return rs;
+  double *d = reinterpret_cast<double *>(base);
   double *q = d, *F = d + 4 * n, *T = d + 7 * n, *f = d + 10 * n;
   G6_CHECK(hipMemcpyAsync(q, q_h, sizeof(double) * 4 * n, hipMemcpyHostToDevice, s.stream));
   G6_CHECK(hipMemcpyAsync(f, fx, sizeof(double) * n * ns, hipMemcpyHostToDevice, s.stream));
@@ -163,7 +168,6 @@ int collect_once(System &s, const double *q_h, const double *fx, const double *f
   G6_CHECK(hipMemcpyAsync(force, F, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s.stream));
   G6_CHECK(hipMemcpyAsync(torque, T, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s.stream));
   G6_CHECK(hipStreamSynchronize(s.stream));
-  G6_CHECK(hipFree(d));
   return 0;
 }
 

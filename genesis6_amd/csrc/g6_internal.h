@@ -208,6 +208,9 @@ struct System {
   // again at the first g6_step call from step tile_retry_at on; the gap
   // doubles after every failed try
   int tile_retry_at = 0, tile_retry_gap = 0;
+  // device scratch of the host-array entry points, grown on demand
+  char *seam = nullptr;
+  size_t seam_cap = 0;
   bool tile_ok = false;   // tile buffers allocated (the system qualifies)
   int tile_waves = 8;     // waves per tile workgroup
   int tile_ndiv = 0, tile_cap = 0;  // the tile grid and its cell capacity
@@ -327,6 +330,22 @@ int tile_set_lds_limit(int bytes);
 int use_row_grid(System &s);
 // destroys the captured step graphs (their launches embed the arguments)
 void drop_graphs(System &s);
+
+// The host-array entry points stage their arrays in one device buffer per
+// handle (grown on demand, freed by g6_finalize), so that no call allocates
+// in the steady state and no error path leaks.  seam_reserve waits for the
+// handle's stream before it replaces a smaller buffer.
+int seam_reserve(System &s, size_t bytes, char **base);
+inline size_t seam_piece(size_t bytes) { return ((bytes ? bytes : 1) + 255) / 256 * 256; }
+struct Carve {
+  char *p;
+  template <class T>
+  T *take(size_t count) {
+    T *r = reinterpret_cast<T *>(p);
+    p += seam_piece(sizeof(T) * count);
+    return r;
+  }
+};
 // A ladder: several systems of one model and precision stepped together,
 // each MD kernel launched once for all of them (g6_ladder.hip).  The block
 // maps list every block of every system: one after another (flat, the

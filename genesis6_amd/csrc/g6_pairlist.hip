@@ -219,19 +219,21 @@ __global__ void k_bin_only(Geom g, const double *com, int *cell, int *count) {
 
 int cells_once(System &s, const double *com_h, int32_t *cell_h) {
   const Geom &g = s.geo;
-  double *com = nullptr;
-  int *cell = nullptr, *count = nullptr;
-  G6_CHECK(hipMalloc(&com, sizeof(double) * 3 * g.n));
-  G6_CHECK(hipMalloc(&cell, sizeof(int) * (g.n + g.ncell + 1)));
-  count = cell + g.n;
+  char *base = nullptr;
+  const int rs = seam_reserve(s, seam_piece(sizeof(double) * 3 * g.n) +
+                                     seam_piece(sizeof(int) * (g.n + g.ncell + 1)),
+                              &base);
+  if (rs != 0) return rs;
+  Carve cv{base};
+  double *com = cv.take<double>(3 * (size_t)g.n);
+  int *cell = cv.take<int>((size_t)g.n + g.ncell + 1);
+  int *count = cell + g.n;
   G6_CHECK(hipMemcpyAsync(com, com_h, sizeof(double) * 3 * g.n, hipMemcpyHostToDevice, s.stream));
   G6_CHECK(hipMemsetAsync(count, 0, sizeof(int) * (g.ncell + 1), s.stream));
   k_bin_only<<<(g.n + 255) / 256, 256, 0, s.stream>>>(g, com, cell, count);
   G6_CHECK(hipGetLastError());
   G6_CHECK(hipMemcpyAsync(cell_h, cell, sizeof(int) * g.n, hipMemcpyDeviceToHost, s.stream));
   G6_CHECK(hipStreamSynchronize(s.stream));
-  G6_CHECK(hipFree(com));
-  G6_CHECK(hipFree(cell));
   return 0;
 }
 
@@ -251,9 +253,11 @@ int pairlist_once(System &s, const double *com_h, int64_t cap, int64_t *npair0,
   const Geom &g = s.geo;
   const int n = g.n;
   // a private list build on the caller's COMs, into the system's buffers
-  double *com = nullptr, *ref = nullptr;
-  G6_CHECK(hipMalloc(&com, sizeof(double) * 3 * n));
-  G6_CHECK(hipMalloc(&ref, sizeof(double) * 3 * n));
+  char *base = nullptr;
+  const int rs = seam_reserve(s, 2 * seam_piece(sizeof(double) * 3 * n), &base);
+  if (rs != 0) return rs;
+  Carve cv{base};
+  double *com = cv.take<double>(3 * (size_t)n), *ref = cv.take<double>(3 * (size_t)n);
   G6_CHECK(hipMemcpyAsync(com, com_h, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
                           s.stream));
   G6_CHECK(hipMemsetAsync(s.cell_count, 0, sizeof(int) * (g.ncell + 1), s.stream));
@@ -315,8 +319,11 @@ int pairlist_once(System &s, const double *com_h, int64_t cap, int64_t *npair0,
   if (flag == 2) rc = G6_EARG;
   if (total > cap || total > s.pcap) rc = G6_ECAPACITY;
   if (rc == 0 && total > 0) {
-    double *o = nullptr;
-    G6_CHECK(hipMalloc(&o, sizeof(double) * 3 * (size_t)total));
+    // com and ref are dead here (the stream has been synchronized)
+    char *ob = nullptr;
+    const int ro = seam_reserve(s, seam_piece(sizeof(double) * 3 * (size_t)total), &ob);
+    if (ro != 0) return ro;
+    double *o = reinterpret_cast<double *>(ob);
     k_unpack_shifts<<<(total + 255) / 256, 256, 0, s.stream>>>(
         s.scode, total, g.L, o, o + total, o + 2 * (size_t)total);
     G6_CHECK(hipMemcpyAsync(pi_h, s.pi, sizeof(int) * total, hipMemcpyDeviceToHost, s.stream));
@@ -327,10 +334,7 @@ int pairlist_once(System &s, const double *com_h, int64_t cap, int64_t *npair0,
     G6_CHECK(hipMemcpyAsync(oz_h, o + 2 * (size_t)total, sizeof(double) * total,
                             hipMemcpyDeviceToHost, s.stream));
     G6_CHECK(hipStreamSynchronize(s.stream));
-    G6_CHECK(hipFree(o));
   }
-  G6_CHECK(hipFree(com));
-  G6_CHECK(hipFree(ref));
   // the MD list in the system buffers is no longer valid
   G6_CHECK(hipMemcpy(s.lflags + 1, &one, sizeof(int), hipMemcpyHostToDevice));
   G6_CHECK(hipMemset(s.cell_count, 0, sizeof(int) * (g.ncell + 1)));

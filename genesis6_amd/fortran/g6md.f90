@@ -4,11 +4,24 @@
 ! reference runs here unchanged and the two logs can be compared line by line.
 !
 ! Tags read: @DTPS dt | @MDLP steps | @NLOG interval | @RCOA rc taper |
-! @BXLA L | @VOXA vx vy vz | @VSET T seed | @NOS2 T q dump z0..z4 zsum |
-! @PCN4 mode pex mass (Andersen barostat, mode 1) |
+! @BXLA L | @BOX3 Lx Ly Lz (a cube: the three equal) | @VOXA vx vy vz |
+! @VOXN nx ny nz | @BOOK interval margin | @VSET T seed |
+! @NOS2 T q dump z0..z4 zsum | @NOSE T q dump z0..z4 (the older form) |
+! @PCN4 mode pex mass (Andersen barostat, mode 1) | @PCN2 (refused, as
+! the reference refuses it) | @FAST | @SNPO | @SNPV |
 ! @ID08 model | @NX4A n, then n lines x y z a b c d.
 ! Command line: g6md [fp64|fp32] [skin]   (default fp64, skin 0: the
-! reference's list rebuilt every step).
+! reference's list rebuilt every step).  Without a skin on the command line,
+! @BOOK interval margin (interval > 0) makes margin the skin: the rows then
+! hold every pair within rc + margin and are rebuilt whenever a molecule has
+! moved margin/2, so the interacting set is every pair within rc at every
+! step: the run is the reference's unbooked run (the reference evaluates a
+! list booked every interval steps, and its run departs from its unbooked
+! one).
+! @VOXN fixes the reference's cell counts; the GPU MD path's interacting set
+! is every pair within rc whatever the grid (the reference's fixed-grid
+! stencil covers them all only when the cells allow it, e.g. one cell), and
+! the cell edge L/nx is passed on as the requested edge.
 !
 ! Restart files are Mixture's: a binary fort.12 in the working directory,
 ! if present, is read first (the records Mixture.F90:400-577 reads: @RCOA
@@ -25,6 +38,9 @@ program g6md
   character(len=256) :: line, tag, arg
   real(c_double) :: dt = 0.002d0, rc = 8.5d0, taper = 0.5d0, box = 0d0, voxa(3) = 0d0
   real(c_double) :: vtemp = -1d0, skin = 0d0, nose(9), th(G6_THERMO_LEN)
+  real(c_double) :: book_margin = 0d0, box3(3)
+  integer :: book_interval = 0, voxn(3) = 0
+  logical :: skin_given = .false.
   integer :: nsteps = 0, nlog = 0, ios, i, done, chunk, target
   integer :: nstep0 = 0, snpo = 0, snpv = 0
   logical :: restart, fixc = .false.
@@ -46,6 +62,7 @@ program g6md
   if (command_argument_count() >= 2) then
      call get_command_argument(2, arg)
      read(arg, *) skin
+     skin_given = .true.
   end if
 
   inquire(file='fort.12', exist=restart)
@@ -75,8 +92,34 @@ program g6md
         read(5, *) rc, taper
      case ('@BXLA')
         read(5, *) box
+     case ('@BOX3')
+        ! Box_ReadBOX3 (BoxModule.F90:109-116): three edges
+        read(5, *) box3
+        if (box3(2) /= box3(1) .or. box3(3) /= box3(1)) then
+           write(0, '(a)') 'g6md: @BOX3 with unequal edges: the GPU path runs cubic boxes'
+           error stop 2
+        end if
+        box = box3(1)
      case ('@VOXA')
         read(5, *) voxa
+        voxn = 0
+     case ('@VOXN')
+        ! Grid_ReadVOXN (GridModule.F90:139-159): fixed cell counts
+        read(5, *) voxn
+     case ('@BOOK')
+        ! Book_ReadBOOK (BookModule.F90:48-64): interval and margin
+        read(5, *) book_interval, book_margin
+     case ('@NOSE')
+        ! Nose_ReadNOSE (NoseModule.F90:90-103): no zetasum, and q in the
+        ! older unit, times 21.8852721616497249235**2
+        read(5, *) nose(1:8)
+        nose(9) = 0d0
+        nose(2) = nose(2) * 21.8852721616497249235d0**2
+        has_nose = .true.
+     case ('@PCN2')
+        ! Andersen_ReadPCN2 (AndersenModule.F90:118-132) stops the run
+        write(0, '(a)') '@PCN2 is not supported any more. Please use @PCN4 instead.'
+        error stop 2
      case ('@VSET')
         read(5, *) vtemp, vseed
      case ('@NOS2')
@@ -122,7 +165,9 @@ program g6md
      write(0, '(a)') 'g6md: the deck needs @ID08, @NX4A and @BXLA'
      error stop 2
   end if
+  if (voxn(1) > 0) voxa = box / voxn(1)
   if (voxa(1) <= 0d0) voxa = rc
+  if (.not. skin_given .and. book_interval > 0) skin = book_margin
 
   call g6_check(g6_init(h, model, n, box, rc, taper, voxa(1), dt, prec, skin), 'g6_init')
   ! @FAST (default off, Mixture.F90:357, 732-736): StdForce or the custom kernels
@@ -194,7 +239,15 @@ contains
        case ('@BXLA')
           read(12) box
        case ('@BOOK')
-          read(12) ib, dbl               ! interval, margin: not used here
+          read(12) book_interval, book_margin
+       case ('@VOXN')
+          read(12) voxn
+       case ('@NOSE')
+          read(12) d9(1:8)               ! Nose_ReadBinaryNOSE (NoseModule.F90:105-118)
+          nose(1:8) = d9(1:8)
+          nose(9) = 0d0
+          nose(2) = nose(2) * 21.8852721616497249235d0**2
+          has_nose = .true.
        case ('@VOXA')
           read(12) voxa
        case ('@NLOG')

@@ -95,3 +95,34 @@ def parse_snapshots(text):
                 raise ValueError("unexpected line in a snapshot block: %r" % toks[i])
         out.append(b)
     return out
+
+
+def tag_decks(g):
+    """Variants of the 20-step Nose NVT deck of md_tip4p_216_nvt.npz that use
+    the other forms of the reference's tags (Mixture.F90:712-717 hands each
+    tag to its module's loader): @NOSE (NoseModule.F90:90-103), @VOXN
+    (GridModule.F90:139-159), @BOX3 (BoxModule.F90:109-116) and @BOOK
+    (BookModule.F90:48-64).  Returns {name: deck text}."""
+    L = float(g["L"])
+    base = mixture_deck(g["com"], g["quat"], L, float(g["rc"]), float(g["taper"]),
+                        float(g["dt"]), int(g["nsteps"]), int(g["seed"]), tuple(g["nose"]))
+    lines = base.split("\n")
+
+    def swap(tag, new_tag, payload):
+        out = list(lines)
+        k = out.index(tag)
+        out[k], out[k + 1] = new_tag, payload
+        return "\n".join(out)
+
+    T, Q, dump = (float(v) for v in g["nose"])
+    return {
+        # the older thermostat tag: no zetasum, the mass in the older unit
+        "nose": swap("@NOS2", "@NOSE", "%r 1500.0 %r 0 0 0 0 0" % (T, dump)),
+        # a fixed grid of one cell: the reference's fixed-grid stencil then
+        # covers every pair within rc (with more cells of this small box it
+        # does not, and its interacting set is not the full one)
+        "voxn": swap("@VOXA", "@VOXN", "1 1 1"),
+        "box3": swap("@BXLA", "@BOX3", "%r %r %r" % (L, L, L)),
+        # a booked list rebuilt every 10 steps with a 1 A margin
+        "book": base.replace("@ID08", "@BOOK\n10 1.0\n@ID08", 1),
+    }

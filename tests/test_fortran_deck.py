@@ -1,0 +1,42 @@
+"""The Fortran MD driver's deck reader, on the CPU: tags it refuses stop the
+run before any device work, as the reference's loaders stop it."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from tests.golden.configs import mixture_deck
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "genesis6_amd", "lib", "g6md")
+
+
+def run(deck):
+    if not os.path.exists(EXE):
+        pytest.skip("g6md is not built")
+    with tempfile.TemporaryDirectory() as d:
+        return subprocess.run([EXE, "fp64"], input=deck, capture_output=True, text=True,
+                              timeout=60, cwd=d)
+
+
+def small_deck():
+    com = np.array([[0.0, 0.0, 0.0], [3.0, 0.0, 0.0]])
+    q = np.array([[1.0, 0.0, 0.0, 0.0], [1.0, 0.0, 0.0, 0.0]])
+    return mixture_deck(com, q, 18.0, 8.5, 0.5, 0.002, 1, 1)
+
+
+def test_pcn2_is_refused_as_the_reference_refuses_it():
+    """Andersen_ReadPCN2 (AndersenModule.F90:118-132) stops with a message
+    that points to @PCN4."""
+    deck = small_deck().replace("@ID08", "@PCN2\n1 1.0 1e-4 0.0\n@ID08", 1)
+    r = run(deck)
+    assert r.returncode != 0
+    assert "@PCN2 is not supported any more. Please use @PCN4 instead." in r.stderr
+
+
+def test_non_cubic_box3_is_refused():
+    deck = small_deck().replace("@BXLA\n18.0", "@BOX3\n18.0 18.0 19.0", 1)
+    r = run(deck)
+    assert r.returncode != 0 and "cubic" in r.stderr

@@ -361,3 +361,33 @@ def test_mc_trials_match_oracle(mid):
     P, _ = s.mc_pairinfo()
     np.testing.assert_array_equal(P, r.ep.reshape(n, n))
     assert abs(s.mc_energy() - e) <= 1e-10 * abs(e)
+
+
+@pytest.mark.parametrize("name", ["nose", "voxn", "box3", "book"])
+def test_fortran_driver_reads_the_other_tag_forms(golden, name):
+    """g6md on the decks that use @NOSE (the older thermostat tag: no zetasum,
+    the mass in the older unit), @VOXN and @BOX3 writes the @LOGD rows that
+    the reference driver writes for the same decks.  With @BOOK 10 1.0 the
+    reference evaluates a list booked every 10 steps, and its run departs
+    from its own unbooked run (T by 2e-5 relative within 20 steps); g6md
+    takes the margin as its skin and keeps the exact interacting set, so its
+    @BOOK run is the reference's unbooked run."""
+    import subprocess
+    from tests.golden.configs import parse_logd, tag_decks
+    g = golden("md_tip4p_216_nvt")
+    ref = g["logd"] if name == "book" else golden("decktags_tip4p_216")[name]
+    deck = tag_decks(g)[name]
+    exe = os.path.join(os.path.dirname(GOLD), "..", "genesis6_amd", "lib", "g6md")
+    print("g6md tag deck", name, flush=True)
+    with tempfile.TemporaryDirectory() as d:
+        r = subprocess.run([os.path.abspath(exe), "fp64"], input=deck, capture_output=True,
+                           text=True, timeout=120, cwd=d)
+    assert r.returncode == 0, r.stderr
+    got = parse_logd(r.stdout)
+    assert got.shape == ref.shape
+    np.testing.assert_array_equal(got[:, 0], ref[:, 0])
+    for col, tol in [(1, 1e-10), (2, 1e-6), (3, 1e-14), (4, 1e-9), (5, 1e-10), (8, 1e-8),
+                     (9, 1e-8), (12, 1e-10)]:
+        scale = np.abs(ref[:, col]).max() + 1e-300
+        err = np.abs(got[:, col] - ref[:, col]).max() / scale
+        assert err <= tol, (col, err)
