@@ -16,6 +16,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FLANG", "/opt/rocm/bin/amdflang")
 FSRC = os.path.join(HERE, "fortran")
 DRIVER = os.path.join(LIBDIR, "g6md")
+MC_DRIVER = os.path.join(LIBDIR, "g6mc")
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",
@@ -46,22 +47,24 @@ def build_fortran(verbose=False):
         raise RuntimeError("flang not found at %s" % FLANG)
     objdir = os.path.join(LIBDIR, "fortran_obj")
     os.makedirs(objdir, exist_ok=True)
-    objs = []
-    for src in ("g6gpu.f90", "g6md.f90"):
+    objs = {}
+    for src in ("g6gpu.f90", "g6md.f90", "g6mc.f90"):
         obj = os.path.join(objdir, src.replace(".f90", ".o"))
         cmd = [FLANG, "-O2", "-J", objdir, "-c", os.path.join(FSRC, src), "-o", obj]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             sys.stderr.write(r.stderr)
             raise RuntimeError("flang failed on %s" % src)
-        objs.append(obj)
-    cmd = [FLANG, "-O2", "-o", DRIVER + ".tmp"] + objs + [
-        "-L" + LIBDIR, "-lg6gpu", "-Wl,-rpath,$ORIGIN"]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        sys.stderr.write(r.stderr)
-        raise RuntimeError("flang link of g6md failed")
-    os.replace(DRIVER + ".tmp", DRIVER)
+        objs[src] = obj
+    # the MD driver (Mixture decks) and the MC driver (mc2 inputs)
+    for prog, exe in (("g6md.f90", DRIVER), ("g6mc.f90", MC_DRIVER)):
+        cmd = [FLANG, "-O2", "-o", exe + ".tmp", objs["g6gpu.f90"], objs[prog],
+               "-L" + LIBDIR, "-lg6gpu", "-Wl,-rpath,$ORIGIN"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stderr)
+            raise RuntimeError("flang link of %s failed" % os.path.basename(exe))
+        os.replace(exe + ".tmp", exe)
     return DRIVER
 
 

@@ -126,3 +126,29 @@ def tag_decks(g):
         # a booked list rebuilt every 10 steps with a 1 A margin
         "book": base.replace("@ID08", "@BOOK\n10 1.0\n@ID08", 1),
     }
+
+
+def mc2_files(com, q, L, rc, taper, seed, temp, sweeps):
+    """The input of the reference's mc2 for one node (mc2util.F90:119-300):
+    (config file text, node input text).  @MCLP = @INNR = @SNPO = sweeps,
+    @NLOG 1, no grid and no bookkeeping (mc2 then uses the all-pairs list)."""
+    n = len(com)
+    deck = ["@SEED", str(seed), "@MCLP", str(sweeps), "@INNR", str(sweeps), "@NLOG", "1",
+            "@SNPO", str(sweeps), "@TEMP", repr(temp), "@RCOA", "%r %r" % (rc, taper),
+            "@BXLA", repr(L), "@NCMP", "1", "@ID08", "TIP4P", "@NX4A", str(n)]
+    deck += [" ".join("%.17e" % v for v in list(com[i]) + list(q[i])) for i in range(n)]
+    return "@MPIN\n1\nnode0.in\n", "\n".join(deck) + "\n"
+
+
+def parse_mc2_out(text, n):
+    """(energies of the @MC2O lines, delta, rtratio, final com, final quat)
+    of an mc2 output file."""
+    lines = text.split("\n")
+    eps = [float(lines[i + 1].split()[1]) for i, l in enumerate(lines) if l.strip() == "@MC2O"]
+    k = lines.index("@DELT")
+    assert int(lines[k + 1]) == n
+    dr = np.array([[float(v) for v in lines[k + 2 + i].split()] for i in range(n)])
+    k = max(i for i, l in enumerate(lines) if l.strip() == "@NX4A")
+    assert int(lines[k + 1]) == n
+    cq = np.array([[float(v) for v in lines[k + 2 + i].split()] for i in range(n)])
+    return np.array(eps), dr[:, 0], dr[:, 1], cq[:, :3], cq[:, 3:]

@@ -15,38 +15,31 @@ import os
 import subprocess
 import tempfile
 
+import sys
+
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+from configs import mc2_files, parse_mc2_out  # noqa: E402
 ROOT = os.path.dirname(os.path.dirname(HERE))
 EXE = os.path.join(ROOT, "oracle", "_ref", "mc2")
 CASES = [(4321, 300.0, 3), (97, 230.0, 4)]
 
 
 def run_mc2(com, q, L, rc, taper, seed, temp, sweeps):
-    n = len(com)
-    deck = ["@SEED", str(seed), "@MCLP", str(sweeps), "@INNR", str(sweeps), "@NLOG", "1",
-            "@SNPO", str(sweeps), "@TEMP", repr(temp), "@RCOA", "%r %r" % (rc, taper),
-            "@BXLA", repr(L), "@NCMP", "1", "@ID08", "TIP4P", "@NX4A", str(n)]
-    deck += [" ".join("%.17e" % v for v in list(com[i]) + list(q[i])) for i in range(n)]
+    cfg, deck = mc2_files(com, q, L, rc, taper, seed, temp, sweeps)
     with tempfile.TemporaryDirectory() as d:
         with open(os.path.join(d, "mc2.cfg"), "w") as f:
-            f.write("@MPIN\n1\nnode0.in\n")
+            f.write(cfg)
         with open(os.path.join(d, "node0.in"), "w") as f:
-            f.write("\n".join(deck) + "\n")
+            f.write(deck)
         r = subprocess.run("ulimit -s unlimited; %s mc2.cfg" % EXE, shell=True, cwd=d,
                            capture_output=True, text=True, timeout=600)
         if r.returncode != 0:
             raise RuntimeError(r.stderr[-2000:])
-        lines = open(os.path.join(d, "node0.in.out")).read().split("\n")
-    eps = [float(lines[i + 1].split()[1]) for i, l in enumerate(lines) if l.strip() == "@MC2O"]
-    k = lines.index("@DELT")
-    assert int(lines[k + 1]) == n
-    dr = np.array([[float(v) for v in lines[k + 2 + i].split()] for i in range(n)])
-    k = max(i for i, l in enumerate(lines) if l.strip() == "@NX4A")
-    assert int(lines[k + 1]) == n
-    cq = np.array([[float(v) for v in lines[k + 2 + i].split()] for i in range(n)])
-    return np.array(eps), dr[:, 0], dr[:, 1], cq[:, :3], cq[:, 3:]
+        text = open(os.path.join(d, "node0.in.out")).read()
+    return parse_mc2_out(text, len(com))
 
 
 def main():

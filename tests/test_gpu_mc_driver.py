@@ -1,0 +1,46 @@
+"""The Fortran MC driver g6mc reads the reference mc2's own input files
+(configuration file with @MPIN, node input) and writes mc2's output file;
+on TIP4P-216 it writes what mc2 itself wrote for the same input: the energy
+of every @MC2O line, the final step sizes and rotation ratios (@DELT) and
+the final configuration (@NX4A)."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from tests.golden.configs import mc2_files, parse_mc2_out
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "genesis6_amd", "lib", "g6mc")
+
+
+@pytest.mark.parametrize("case", [0, 1])
+def test_g6mc_writes_what_mc2_writes(golden, case):
+    g = golden("tip4p_216")
+    ref = golden("mc2_tip4p_216")
+    seed, temp, sweeps = ref["cases"][case]
+    cfg, deck = mc2_files(g["com"], g["quat"], float(g["L"]), float(g["rc"]), float(g["taper"]),
+                          int(seed), float(temp), int(sweeps))
+    print("g6mc case", case, flush=True)
+    n = len(g["com"])
+    with tempfile.TemporaryDirectory() as d:
+        with open(os.path.join(d, "mc2.cfg"), "w") as f:
+            f.write(cfg)
+        with open(os.path.join(d, "node0.in"), "w") as f:
+            f.write(deck)
+        r = subprocess.run([EXE, "mc2.cfg"], cwd=d, capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr
+        text = open(os.path.join(d, "node0.in.out")).read()
+    eps, delta, rtr, com, quat = parse_mc2_out(text, n)
+    e_ref = ref["epsum_%d" % case]
+    assert len(eps) == len(e_ref)
+    assert np.abs(eps - e_ref).max() <= 1e-13 * np.abs(e_ref).max()
+    # every decision the same: step sizes and ratios change by 1.05 on each
+    assert np.array_equal(delta, ref["delta_%d" % case])
+    assert np.array_equal(rtr, ref["rtratio_%d" % case])
+    # the configuration to the ten digits both print
+    assert np.array_equal(com, ref["com_%d" % case]) and np.array_equal(quat, ref["quat_%d" % case])
