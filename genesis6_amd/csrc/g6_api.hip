@@ -277,6 +277,58 @@ bool make_model_table(int model, int prec, ModelTable *m, bool fast) {
     set_inverses(m);
     return true;
   }
+  if (model == G6_OPLSMEOH) {
+    // Rigid_OPLSMeOH_Constructor (RigidModule.F90:391-426): Me, O, H, CoM in
+    // the plane z = 0, the coordinates as given (no shift)
+    const double ms[4] = {15.0, 16.0, 1.0, 0.0};
+    const double x[4] = {0.0133, -0.0634, 0.8152, 0.0};
+    const double y[4] = {0.7666, -0.6559, -1.0042, 0.0};
+    m->nsite = 4;
+    m->Ixx = m->Iyy = m->Izz = m->mass = 0.0;
+    for (int i = 0; i < 4; i++) {
+      m->bx[i] = x[i];
+      m->by[i] = y[i];
+      m->bz[i] = 0.0;
+      m->Ixx = m->Ixx + ms[i] * (y[i] * y[i] + 0.0 * 0.0);
+      m->Iyy = m->Iyy + ms[i] * (0.0 * 0.0 + x[i] * x[i]);
+      m->Izz = m->Izz + ms[i] * (x[i] * x[i] + y[i] * y[i]);
+      m->mass = m->mass + ms[i];
+    }
+    // oplsmeoh_setinteraction (OPLSMeOHModule.F90:82-101), the parameters of
+    // OPLSMeOHModule.F90:41-49 (eps in kcal/mol times CA = 4.184,
+    // PhysConst.F90:29-31)
+    const double CA = 4.184;
+    const double q[3] = {0.265, -0.700, 0.435};
+    const double eps[2] = {0.207 * CA, 0.170 * CA}, sig[2] = {3.775, 3.070};
+    for (int i = 0; i < 3; i++) m->std_q[i] = q[i];
+    for (int i = 0; i < 2; i++) {
+      m->std_eps[i] = eps[i];
+      m->std_sig[i] = sig[i];
+    }
+    // StdForce coefficients (StdInteraction.F90:206-221): 9 Coulomb terms
+    // over Me, O, H; LJ on Me-Me and on the Me-O, O-Me, O-O pairs
+    const double KX2X = 1.0 / 1e-3, J2I = 1.0 / 10.0;
+    m->ncoul = 9;
+    for (int t = 0; t < 9; t++) {
+      const int a = t / 3, b = t % 3;
+      m->ci[t] = a;
+      m->cj[t] = b;
+      m->qq[t] = q[a] * q[b] * coeff() * KX2X * J2I * 1.0;
+    }
+    auto lj = [&](int a, int b, double *A, double *B) {
+      const double ljeps = sqrt(eps[a] * eps[b]);
+      const double s6 = ipow((sig[a] + sig[b]) * 0.5, 6);
+      *A = 4.0 * ljeps * s6 * s6 * KX2X * J2I * 1.0;
+      *B = -4.0 * ljeps * s6 * KX2X * J2I * 1.0;
+    };
+    lj(0, 0, &m->lj_a, &m->lj_b);
+    // the pairs of Traits<G6_OPLSMEOH>::xa/xb: (0,1) (1,0) (1,1)
+    const int la[3] = {0, 1, 1}, lb[3] = {1, 0, 1};
+    m->nlx = 3;
+    for (int t = 0; t < 3; t++) lj(la[t], lb[t], &m->lx_a[t], &m->lx_b[t]);
+    set_inverses(m);
+    return true;
+  }
   return false;
 }
 
@@ -883,7 +935,7 @@ static int check_flags(System &s) {
   G6_CHECK(hipMemcpyAsync(fl, s.md.flags, sizeof(fl), hipMemcpyDeviceToHost, s.stream));
   G6_CHECK(hipStreamSynchronize(s.stream));
   s.last_flag = fl[0];
-  if (fl[0] == 1 || fl[0] == 3 || fl[0] == 4 || fl[0] == 5) return G6_ECAPACITY;
+  if (fl[0] == 1 || fl[0] == 3 || fl[0] == 4 || fl[0] == 5 || fl[0] == 6) return G6_ECAPACITY;
   if (fl[0] == 2) return G6_EARG;
   return 0;
 }

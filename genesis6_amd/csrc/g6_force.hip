@@ -150,12 +150,16 @@ int force_pairs(System &s, int64_t np, const int32_t *pi_h, const int32_t *pj_h,
       else if (s.kmodel == G6_TIP5P) launch_pairs_t<G6_TIP5P, float>(s, np, d, vir, pi, pj, eo);
       else if (s.kmodel == G6_SPCE) launch_pairs_t<G6_SPCE, float>(s, np, d, vir, pi, pj, eo);
       else if (s.kmodel == G6_NVDE) launch_pairs_t<G6_NVDE, float>(s, np, d, vir, pi, pj, eo);
+      else if (s.kmodel == G6_OPLSMEOH)
+        launch_pairs_t<G6_OPLSMEOH, float>(s, np, d, vir, pi, pj, eo);
       else launch_pairs_t<G6_ST2, float>(s, np, d, vir, pi, pj, eo);
     } else {
       if (s.kmodel == G6_TIP4P) launch_pairs_t<G6_TIP4P, double>(s, np, d, vir, pi, pj, eo);
       else if (s.kmodel == G6_TIP5P) launch_pairs_t<G6_TIP5P, double>(s, np, d, vir, pi, pj, eo);
       else if (s.kmodel == G6_SPCE) launch_pairs_t<G6_SPCE, double>(s, np, d, vir, pi, pj, eo);
       else if (s.kmodel == G6_NVDE) launch_pairs_t<G6_NVDE, double>(s, np, d, vir, pi, pj, eo);
+      else if (s.kmodel == G6_OPLSMEOH)
+        launch_pairs_t<G6_OPLSMEOH, double>(s, np, d, vir, pi, pj, eo);
       else launch_pairs_t<G6_ST2, double>(s, np, d, vir, pi, pj, eo);
     }
     G6_CHECK(hipGetLastError());

@@ -813,7 +813,7 @@ constexpr int main_waves_per_eu() {
   // 4096-molecule grid plus the step-close block (1025 blocks) is resident
   // at once on 256 CUs (four waves per SIMD leave the last block waiting for
   // a slot, which set the kernel's end)
-  if (sizeof(R) == 4 && (M == G6_TIP4P || M == G6_SPCE)) return 5;
+  if (sizeof(R) == 4 && (M == G6_TIP4P || M == G6_SPCE || M == G6_OPLSMEOH)) return 5;
   return (M != G6_ST2 && M != G6_NVDE && sizeof(R) == 4) ? 4 : 1;
 }
 
@@ -938,12 +938,14 @@ void md_launch_main(System &s, double *out, int parity, int close_mode) {
     else if (s.kmodel == G6_TIP5P) md_main_t<G6_TIP5P, float>(s, out, parity, ca);
     else if (s.kmodel == G6_SPCE) md_main_t<G6_SPCE, float>(s, out, parity, ca);
     else if (s.kmodel == G6_NVDE) md_main_t<G6_NVDE, float>(s, out, parity, ca);
+    else if (s.kmodel == G6_OPLSMEOH) md_main_t<G6_OPLSMEOH, float>(s, out, parity, ca);
     else md_main_t<G6_ST2, float>(s, out, parity, ca);
   } else {
     if (s.kmodel == G6_TIP4P) md_main_t<G6_TIP4P, double>(s, out, parity, ca);
     else if (s.kmodel == G6_TIP5P) md_main_t<G6_TIP5P, double>(s, out, parity, ca);
     else if (s.kmodel == G6_SPCE) md_main_t<G6_SPCE, double>(s, out, parity, ca);
     else if (s.kmodel == G6_NVDE) md_main_t<G6_NVDE, double>(s, out, parity, ca);
+    else if (s.kmodel == G6_OPLSMEOH) md_main_t<G6_OPLSMEOH, double>(s, out, parity, ca);
     else md_main_t<G6_ST2, double>(s, out, parity, ca);
   }
 }
@@ -969,12 +971,16 @@ void md_launch_main_lad(Ladder &l, int parity, int close_mode) {
     else if (l.model == G6_TIP5P) md_main_lad_t<G6_TIP5P, float>(l, l.pkf, parity, close_mode);
     else if (l.model == G6_SPCE) md_main_lad_t<G6_SPCE, float>(l, l.pkf, parity, close_mode);
     else if (l.model == G6_NVDE) md_main_lad_t<G6_NVDE, float>(l, l.pkf, parity, close_mode);
+    else if (l.model == G6_OPLSMEOH)
+      md_main_lad_t<G6_OPLSMEOH, float>(l, l.pkf, parity, close_mode);
     else md_main_lad_t<G6_ST2, float>(l, l.pkf, parity, close_mode);
   } else {
     if (l.model == G6_TIP4P) md_main_lad_t<G6_TIP4P, double>(l, l.pkd, parity, close_mode);
     else if (l.model == G6_TIP5P) md_main_lad_t<G6_TIP5P, double>(l, l.pkd, parity, close_mode);
     else if (l.model == G6_SPCE) md_main_lad_t<G6_SPCE, double>(l, l.pkd, parity, close_mode);
     else if (l.model == G6_NVDE) md_main_lad_t<G6_NVDE, double>(l, l.pkd, parity, close_mode);
+    else if (l.model == G6_OPLSMEOH)
+      md_main_lad_t<G6_OPLSMEOH, double>(l, l.pkd, parity, close_mode);
     else md_main_lad_t<G6_ST2, double>(l, l.pkd, parity, close_mode);
   }
 }

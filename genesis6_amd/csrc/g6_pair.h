@@ -9,6 +9,8 @@
 //          the Stillinger-Rahman switch s12 on the COM distance, LJ by S)
 //   SPC/E  StdForce (9 Coulomb over O, H, H + O-O LJ)
 //   NvdE   StdForce (25 Coulomb over H, H, M, L, L + LJ on O-O, O-H, H-H)
+//   OPLS methanol  StdForce (9 Coulomb over Me, O, H + LJ on Me-Me, Me-O,
+//          O-Me, O-O)
 // Cutoff ratios hh = S(r), dh = -S'(r)/r on the COM distance are inputs
 // (cutoff_func, CutoffModule.F90:38-59; cutoff_sw below).
 // Inputs are the COM separation d00 (shift already removed) and the site
@@ -68,6 +70,23 @@ struct Traits<G6_NVDE> {  // O, H, H, M, L, L, CoM: LJ on O and H, charges on H,
   }
   __device__ static constexpr int xb(int t) {
     constexpr int b[8] = {1, 2, 0, 0, 1, 2, 1, 2};
+    return b[t];
+  }
+};
+
+template <>
+struct Traits<G6_OPLSMEOH> {  // Me, O, H, CoM: LJ and charges on Me and O, a charge on H
+  static constexpr int NS = 4, COM = 3, NC = 9, NLX = 3;
+  static constexpr bool ST2SW = false;
+  __device__ static constexpr int ci(int t) { return t / 3; }
+  __device__ static constexpr int cj(int t) { return t % 3; }
+  // the further LJ pairs in the order of ModelTable::lx_a: Me-O, O-Me, O-O
+  __device__ static constexpr int xa(int t) {
+    constexpr int a[3] = {0, 1, 1};
+    return a[t];
+  }
+  __device__ static constexpr int xb(int t) {
+    constexpr int b[3] = {1, 0, 1};
     return b[t];
   }
 };

@@ -70,6 +70,14 @@ struct StdPattern<G6_NVDE> {
   __device__ __host__ static constexpr int kind(int u) { return nvde_terms().k[u]; }
 };
 
+template <>
+struct StdPattern<G6_OPLSMEOH> {  // Me, O, H x Me, O, H: LJ + Coulomb on Me/O pairs
+  static constexpr int NT = 9, NS = 4;
+  __device__ __host__ static constexpr int a(int u) { return u / 3; }
+  __device__ __host__ static constexpr int b(int u) { return u % 3; }
+  __device__ __host__ static constexpr int kind(int u) { return (u / 3 < 2 && u % 3 < 2) ? 1 : 2; }
+};
+
 // Calls f(std::integral_constant<int, M>) for the StdPairPotential term
 // pattern M of a model: ST2's StdPairPotential terms (no Stillinger-Rahman
 // switch, as mc2 evaluates them) have the TIP5P pattern.
@@ -78,6 +86,7 @@ inline void std_model(int model, F &&f) {
   if (model == G6_TIP4P) f(std::integral_constant<int, G6_TIP4P>());
   else if (model == G6_TIP5P || model == G6_ST2) f(std::integral_constant<int, G6_TIP5P>());
   else if (model == G6_SPCE) f(std::integral_constant<int, G6_SPCE>());
+  else if (model == G6_OPLSMEOH) f(std::integral_constant<int, G6_OPLSMEOH>());
   else f(std::integral_constant<int, G6_NVDE>());
 }
 

@@ -324,6 +324,14 @@ __device__ __forceinline__ void tile_body(const Geom &g, const MdBuf &b, const P
       const float tj[3] = {-(Tq[0] + (d1 * F[2] - d2 * F[1])),
                            -(Tq[1] + (d2 * F[0] - d0 * F[2])),
                            -(Tq[2] + (d0 * F[1] - d1 * F[0]))};
+      // fix64 rounds exactly below 2^27: every pair value (and so each
+      // quad's sum of four) stays below 2^25, or the step is a capacity
+      // error (overlapping molecules of an unphysical configuration)
+      float mx = 0.0f;
+#pragma unroll
+      for (int a = 0; a < 3; a++)
+        mx = fmaxf(mx, fmaxf(fabsf(F[a]), fmaxf(fabsf(Tq[a]), fabsf(tj[a]))));
+      if (!(mx < 33554432.0f)) raise_flag(b.flags, 6);
 #pragma unroll
       for (int a = 0; a < 3; a++) {
         atomicAdd(&s_acc[a * nbmax + js], fix64(-F[a]));
@@ -582,6 +590,7 @@ void md_launch_tile(System &s, int parity, const CloseArgs &ca) {
   else if (s.kmodel == G6_TIP5P) tile_t<G6_TIP5P>(s, parity, ca);
   else if (s.kmodel == G6_SPCE) tile_t<G6_SPCE>(s, parity, ca);
   else if (s.kmodel == G6_NVDE) tile_t<G6_NVDE>(s, parity, ca);
+  else if (s.kmodel == G6_OPLSMEOH) tile_t<G6_OPLSMEOH>(s, parity, ca);
   else tile_t<G6_ST2>(s, parity, ca);
 }
 
@@ -596,6 +605,7 @@ int tile_set_lds_limit(int bytes) {
       (const void *)k_md_tile<G6_TIP5P, true>, (const void *)k_md_tile<G6_TIP5P, false>,
       (const void *)k_md_tile<G6_SPCE, true>,  (const void *)k_md_tile<G6_SPCE, false>,
       (const void *)k_md_tile<G6_NVDE, true>,  (const void *)k_md_tile<G6_NVDE, false>,
+      (const void *)k_md_tile<G6_OPLSMEOH, true>, (const void *)k_md_tile<G6_OPLSMEOH, false>,
       (const void *)k_md_tile<G6_ST2, true>,   (const void *)k_md_tile<G6_ST2, false>};
   for (const void *f : fns)
     if (hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) != hipSuccess)

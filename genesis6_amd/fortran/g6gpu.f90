@@ -7,7 +7,7 @@ module g6gpu
   use, intrinsic :: iso_c_binding
   implicit none
   integer(c_int), parameter :: G6_TIP4P = 0, G6_TIP5P = 1, G6_ST2 = 2, G6_SPCE = 3, &
-       G6_NVDE = 4
+       G6_NVDE = 4, G6_OPLSMEOH = 5
   integer(c_int), parameter :: G6_FP64 = 0, G6_FP32 = 1
   integer(c_int), parameter :: G6_EHIP = -1, G6_EARG = -2, G6_ECAPACITY = -3, &
        G6_ENODEVICE = -4

@@ -194,6 +194,8 @@ contains
        model = G6_TIP5P
     case ('NVDE____')
        model = G6_NVDE
+    case ('OPLSMEOH')
+       model = G6_OPLSMEOH
     case default
        write(0, '(a,a)') 'g6mc: @ID08 model not on the GPU MC path: ', lastid
        error stop 2

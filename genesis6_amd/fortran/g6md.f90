@@ -141,6 +141,8 @@ program g6md
            model = G6_SPCE
         case ('NVDE____')
            model = G6_NVDE
+        case ('OPLSMEOH')
+           model = G6_OPLSMEOH
         case default
            write(0, '(a,a)') 'g6md: @ID08 model not on the GPU path: ', trim(line)
            error stop 2
@@ -293,6 +295,8 @@ contains
              model = G6_SPCE
           case ('NVDE____')
              model = G6_NVDE
+          case ('OPLSMEOH')
+             model = G6_OPLSMEOH
           case default
              write(0, '(a,a)') 'g6md: @ID08 model not on the GPU path: ', id
              error stop 2
@@ -325,6 +329,8 @@ contains
        model_id = 'SPC_E   '
     case (G6_NVDE)
        model_id = 'NVDE____'
+    case (G6_OPLSMEOH)
+       model_id = 'OPLSMEOH'
     case default
        model_id = 'ST2_____'
     end select

@@ -12,7 +12,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "lib", "libg6gpu.so")
-TIP4P, TIP5P, ST2, SPCE, NVDE = 0, 1, 2, 3, 4
+TIP4P, TIP5P, ST2, SPCE, NVDE, OPLSMEOH = 0, 1, 2, 3, 4, 5
 FP64, FP32 = 0, 1
 THERMO_LEN = 32
 ORDER_NONE, ORDER_BY_DISTANCE, ORDER_BY_LABEL = 0, 1, 2

@@ -27,7 +27,7 @@ typedef int64_t g6_handle;
 
 /* models: the reference ids "TIP4P   ", "TIP5P   ", "ST2_____", "SPC_E   ",
  * "NVDE____" (RigidSetup.F90:33-62) */
-enum { G6_TIP4P = 0, G6_TIP5P = 1, G6_ST2 = 2, G6_SPCE = 3, G6_NVDE = 4 };
+enum { G6_TIP4P = 0, G6_TIP5P = 1, G6_ST2 = 2, G6_SPCE = 3, G6_NVDE = 4, G6_OPLSMEOH = 5 };
 /* pair arithmetic precision; accumulation of forces, torques, energy and
  * virial is FP64 in both */
 enum { G6_FP64 = 0, G6_FP32 = 1 };
@@ -435,8 +435,8 @@ int g6_time_force(g6_handle h, int nlaunch, double *ms);
  * info[1] = tile cells per axis, info[2] = workgroups per tile, info[3] =
  * waves per workgroup, info[4] = neighbourhood slots per tile (zeros for the
  * rows), info[5] = the capacity code that moved a tile system to the rows
- * (3 cell residents, 4 neighbourhood slots or row entries, 2 image shift; 0
- * when none did).  FP32 fixed-box systems with at least 256 cells of edge
+ * (3 cell residents, 4 neighbourhood slots or row entries, 2 image shift, 6
+ * a pair force beyond the fixed-point sums' range; 0 when none did).  FP32 fixed-box systems with at least 256 cells of edge
  * rc + skin take the tiles; G6_TILES=0 in the environment at g6_init keeps
  * the rows and G6_TILE_MIN sets the cell threshold.  A system that a
  * capacity error moved to the rows tries the tiles again at the first

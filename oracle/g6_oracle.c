@@ -195,6 +195,33 @@ int g6o_model_get(int model, g6o_model *m) {
     m->sig[1] = m->sig[2] = 0.673;
     return m->nsite;
   }
+  if (model == G6O_OPLSMEOH) {
+    /* Rigid_OPLSMeOH_Constructor, RigidModule.F90:391-426: Me, O, H, CoM in
+     * the plane z = 0, as given */
+    const double ms[4] = {15.0, 16.0, 1.0, 0.0};
+    const double x[4] = {0.0133, -0.0634, 0.8152, 0.0};
+    const double y[4] = {0.7666, -0.6559, -1.0042, 0.0};
+    m->nsite = 4;
+    for (int i = 0; i < 4; i++) {
+      m->bx[i] = x[i];
+      m->by[i] = y[i];
+      m->bz[i] = 0.0;
+      m->Ixx = m->Ixx + ms[i] * (m->by[i] * m->by[i] + m->bz[i] * m->bz[i]);
+      m->Iyy = m->Iyy + ms[i] * (m->bz[i] * m->bz[i] + m->bx[i] * m->bx[i]);
+      m->Izz = m->Izz + ms[i] * (m->bx[i] * m->bx[i] + m->by[i] * m->by[i]);
+      m->mass = m->mass + ms[i];
+    }
+    /* oplsmeoh_setinteraction, OPLSMeOHModule.F90:82-101 (parameters :41-49,
+     * eps in kcal/mol times CA, PhysConst.F90:29-31) */
+    m->q[0] = 0.265;
+    m->q[1] = -0.700;
+    m->q[2] = 0.435;
+    m->eps[0] = 0.207 * CA_;
+    m->eps[1] = 0.170 * CA_;
+    m->sig[0] = 3.775;
+    m->sig[1] = 3.070;
+    return m->nsite;
+  }
   return 0;
 }
 

@@ -15,7 +15,7 @@
 extern "C" {
 #endif
 
-enum { G6O_TIP4P = 0, G6O_TIP5P = 1, G6O_ST2 = 2, G6O_SPCE = 3, G6O_NVDE = 4 };
+enum { G6O_TIP4P = 0, G6O_TIP5P = 1, G6O_ST2 = 2, G6O_SPCE = 3, G6O_NVDE = 4, G6O_OPLSMEOH = 5 };
 #define G6O_MAXSITE 7
 
 /* Rigid-model tables: site charges/LJ, body-frame geometry, inertia, mass.

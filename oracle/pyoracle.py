@@ -12,9 +12,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
-TIP4P, TIP5P, ST2, SPCE, NVDE = 0, 1, 2, 3, 4
+TIP4P, TIP5P, ST2, SPCE, NVDE, OPLSMEOH = 0, 1, 2, 3, 4, 5
 MODEL_IDS = {TIP4P: "TIP4P   ", TIP5P: "TIP5P   ", ST2: "ST2_____", SPCE: "SPC_E   ",
-             NVDE: "NVDE____"}
+             NVDE: "NVDE____", OPLSMEOH: "OPLSMEOH"}
 MAXSITE = 7         # G6O_MAXSITE
 
 _lib = None

@@ -23,7 +23,8 @@ from configs import lattice_start, mixture_deck, parse_logd  # noqa: E402
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 REF = os.path.join(ROOT, "oracle", "_ref")
-IDS = {0: "TIP4P   ", 1: "TIP5P   ", 2: "ST2_____", 3: "SPC_E   ", 4: "NVDE____"}
+IDS = {0: "TIP4P   ", 1: "TIP5P   ", 2: "ST2_____", 3: "SPC_E   ", 4: "NVDE____",
+       5: "OPLSMEOH"}
 
 
 def run(cmd, cwd, stdin_text):

@@ -17,7 +17,8 @@ from tests.golden.configs import lattice_start
 
 pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
-CASES = ["tip4p_216", "tip5p_216", "st2_216", "tip4p_125_ndiv3", "spce_216", "nvde_216"]
+CASES = ["tip4p_216", "tip5p_216", "st2_216", "tip4p_125_ndiv3", "spce_216", "nvde_216",
+         "meoh_216"]
 
 
 def relrms(a, b):
@@ -171,9 +172,10 @@ TRAJ = [("md_tip4p_216_nve", g6.TIP4P, ""), ("md_tip4p_216_nvt", g6.TIP4P, ""),
         ("md_spce_216", g6.SPCE, "nve"), ("md_spce_216", g6.SPCE, "nvt"),
         ("md_nvde_216", g6.NVDE, "nve"), ("md_nvde_216", g6.NVDE, "nvt"),
         ("md_st2_216", g6.ST2, "std"), ("md_st2_216", g6.ST2, "fast"),
-        ("md_tip5p_216", g6.TIP5P, "nve"), ("md_tip5p_216", g6.TIP5P, "nvt")]
+        ("md_tip5p_216", g6.TIP5P, "nve"), ("md_tip5p_216", g6.TIP5P, "nvt"),
+        ("md_meoh_216", g6.OPLSMEOH, "nve"), ("md_meoh_216", g6.OPLSMEOH, "nvt")]
 DECK_ID = {g6.TIP4P: "TIP4P", g6.SPCE: "SPC_E", g6.NVDE: "NVDE____", g6.ST2: "ST2_____",
-           g6.TIP5P: "TIP5P"}
+           g6.TIP5P: "TIP5P", g6.OPLSMEOH: "OPLSMEOH"}
 
 
 def traj_case(g, ens):
@@ -323,7 +325,7 @@ def test_fortran_driver_snapshots_like_the_reference(golden):
         assert err.max() <= 1e-9, (k, err.max(0))
 
 
-@pytest.mark.parametrize("mid", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("mid", [0, 1, 2, 3, 4, 5])
 def test_mc_pairinfo_matches_oracle(mid):
     """PairInfo on the GPU equals the oracle's matrix bit for bit."""
     n, L = 216, 18.626
@@ -337,7 +339,7 @@ def test_mc_pairinfo_matches_oracle(mid):
     assert abs(e - er) <= 1e-12 * abs(er)
 
 
-@pytest.mark.parametrize("mid", [g6.TIP4P, g6.ST2, g6.SPCE, g6.NVDE])
+@pytest.mark.parametrize("mid", [g6.TIP4P, g6.ST2, g6.SPCE, g6.NVDE, g6.OPLSMEOH])
 def test_mc_trials_match_oracle(mid):
     """2000 mc2 trials on the GPU take the oracle's decisions: same accept
     count, the same final configuration bit for bit, the same PairInfo; the

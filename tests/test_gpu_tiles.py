@@ -60,13 +60,15 @@ def check_forces(s, Fm, Tm, ep, vr, npair, tol=1e-5):
 
 # (model, oracle kernel, @FAST)
 MODELS = [(g6.TIP4P, "fast", True), (g6.TIP5P, "std", True), (g6.SPCE, "std", True),
-          (g6.NVDE, "std", True), (g6.ST2, "fast", True), (g6.ST2, "std", False)]
+          (g6.NVDE, "std", True), (g6.ST2, "fast", True), (g6.ST2, "std", False),
+          (g6.OPLSMEOH, "std", True)]
 
 
 @pytest.mark.parametrize("mid,kind,fast", MODELS)
 @pytest.mark.parametrize("skin", [0.0, 0.8])
 def test_tile_forces_every_model(mid, kind, fast, skin):
-    n, L = 1024, 31.289
+    # water at 1.0 g/cm3; methanol at 0.79 g/cm3 (M 32.04)
+    n, L = 1024, (41.008 if mid == g6.OPLSMEOH else 31.289)
     print("tiles model=%d fast=%d skin=%g" % (mid, fast, skin), flush=True)
     com, q = lattice_start(n, L, 20261202 + 7)
     ref = oracle_forces(mid, kind, com, q, L)
@@ -74,7 +76,7 @@ def test_tile_forces_every_model(mid, kind, fast, skin):
         s = g6.System(mid, n, L, RC, TAPER, DT, prec=g6.FP32, skin=skin)
     if not fast:
         s.set_fast(False)
-    assert s.md_kernel()[0] == 1 and s.md_kernel()[1] == 3
+    assert s.md_kernel()[0] == 1 and s.md_kernel()[1] == int(L / (RC + skin))
     s.upload(com, q)
     check_forces(s, *ref)
     s.close()

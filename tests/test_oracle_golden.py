@@ -10,7 +10,8 @@ from oracle import pyoracle as O
 from tests.golden.configs import lattice_start
 
 GOLD = os.path.join(os.path.dirname(__file__), "golden")
-CASES = ["tip4p_216", "tip5p_216", "st2_216", "tip4p_125_ndiv3", "spce_216", "nvde_216"]
+CASES = ["tip4p_216", "tip5p_216", "st2_216", "tip4p_125_ndiv3", "spce_216", "nvde_216",
+         "meoh_216"]
 
 
 @pytest.mark.parametrize("name", CASES)
@@ -108,12 +109,13 @@ def test_md_trajectory_matches_reference_driver(golden, name):
 
 
 @pytest.mark.parametrize("name,mid", [("md_spce_216", O.SPCE), ("md_nvde_216", O.NVDE),
-                                      ("md_tip5p_216", O.TIP5P)])
+                                      ("md_tip5p_216", O.TIP5P),
+                                      ("md_meoh_216", O.OPLSMEOH)])
 @pytest.mark.parametrize("ens", ["nve", "nvt"])
 def test_std_model_md_trajectory_matches_reference_driver(golden, name, mid, ens):
-    """TIP5P, SPC/E (the O site charged and LJ) and NvdE (LJ on O and H,
-    charges on H, M and the lone pairs) through the reference driver's
-    StdForce path:
+    """TIP5P, SPC/E (the O site charged and LJ), NvdE (LJ on O and H,
+    charges on H, M and the lone pairs) and OPLS methanol (LJ and charges on
+    Me and O, a charge on H) through the reference driver's StdForce path:
     the oracle's step reproduces its log and final state bit for bit."""
     g = golden(name)
     md_trajectory(g, mid, tuple(g["nose"]) if ens == "nvt" else None, ens + "_")
@@ -144,7 +146,7 @@ def test_cutoff_closed_form():
     assert abs(f(rin + 1e-7)[0] - 1) < 1e-12 and abs(f(rout - 1e-7)[0]) < 1e-12
 
 
-@pytest.mark.parametrize("mid", [O.TIP4P, O.TIP5P, O.ST2, O.SPCE, O.NVDE])
+@pytest.mark.parametrize("mid", [O.TIP4P, O.TIP5P, O.ST2, O.SPCE, O.NVDE, O.OPLSMEOH])
 def test_force_is_minus_gradient(mid):
     """F = -dE/dx by central differences on one molecule's COM; sum F = 0."""
     L, rc, taper = 14.0, 6.5, 1.0
@@ -203,7 +205,7 @@ def test_lj_coulomb_dimer_closed_form():
 
 
 @pytest.mark.parametrize("name,mid", [("tip4p_216", 0), ("tip5p_216", 1), ("spce_216", 3),
-                                      ("nvde_216", 4)])
+                                      ("nvde_216", 4), ("meoh_216", 5)])
 def test_pairinfo_matches_reference_pair_energies(golden, name, mid):
     """PairInfo (pairenergy over the all-pairs list within rc) holds, for
     every pair of the reference's own list, the reference's StdForce pair
