@@ -267,6 +267,8 @@ __device__ __forceinline__ void tile_body(const Geom &g, const MdBuf &b, const P
 
   // ---- 3. rows, interaction test, dense pair rounds ----
   const double Lb = g.L, invL = g.invL, rr = g.rr, rc2 = g.rc2;
+  const double fmax = 33554432.0 / (1.0 + sqrt(rc2));
+  const float m2max = (float)(fmax * fmax);
   // each lane's energy and virial over its own pairs (a few per step) in
   // FP32; the lane totals go to FP64 for the compensated block sums
   float ep = 0.0f, v[9];
@@ -324,14 +326,14 @@ __device__ __forceinline__ void tile_body(const Geom &g, const MdBuf &b, const P
       const float tj[3] = {-(Tq[0] + (d1 * F[2] - d2 * F[1])),
                            -(Tq[1] + (d2 * F[0] - d0 * F[2])),
                            -(Tq[2] + (d0 * F[1] - d1 * F[0]))};
-      // fix64 rounds exactly below 2^27: every pair value (and so each
-      // quad's sum of four) stays below 2^25, or the step is a capacity
-      // error (overlapping molecules of an unphysical configuration)
-      float mx = 0.0f;
-#pragma unroll
-      for (int a = 0; a < 3; a++)
-        mx = fmaxf(mx, fmaxf(fabsf(F[a]), fmaxf(fabsf(Tq[a]), fabsf(tj[a]))));
-      if (!(mx < 33554432.0f)) raise_flag(b.flags, 6);
+      // fix64 rounds exactly below 2^27.  With |F|, |tau_i| below
+      // 2^25 / (1 + rc) (the norm test) and |d| < rc, every summed value
+      // (tau_j, and each quad's sum of four) stays below 2^27; a larger pair
+      // force (molecules overlapping in an unphysical configuration) is a
+      // capacity error
+      const float m2 = F[0] * F[0] + F[1] * F[1] + F[2] * F[2] + Tq[0] * Tq[0] +
+                       Tq[1] * Tq[1] + Tq[2] * Tq[2];
+      if (!(m2 < m2max)) raise_flag(b.flags, 6);
 #pragma unroll
       for (int a = 0; a < 3; a++) {
         atomicAdd(&s_acc[a * nbmax + js], fix64(-F[a]));
