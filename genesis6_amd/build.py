@@ -17,6 +17,7 @@ FLANG = os.environ.get("FLANG", "/opt/rocm/bin/amdflang")
 FSRC = os.path.join(HERE, "fortran")
 DRIVER = os.path.join(LIBDIR, "g6md")
 MC_DRIVER = os.path.join(LIBDIR, "g6mc")
+RX_DRIVER = os.path.join(LIBDIR, "g6rx")
 
 FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",
@@ -48,7 +49,7 @@ def build_fortran(verbose=False):
     objdir = os.path.join(LIBDIR, "fortran_obj")
     os.makedirs(objdir, exist_ok=True)
     objs = {}
-    for src in ("g6gpu.f90", "g6md.f90", "g6mc.f90"):
+    for src in ("g6gpu.f90", "g6md.f90", "g6mc.f90", "g6rx.f90"):
         obj = os.path.join(objdir, src.replace(".f90", ".o"))
         cmd = [FLANG, "-O2", "-J", objdir, "-c", os.path.join(FSRC, src), "-o", obj]
         r = subprocess.run(cmd, capture_output=True, text=True)
@@ -56,8 +57,9 @@ def build_fortran(verbose=False):
             sys.stderr.write(r.stderr)
             raise RuntimeError("flang failed on %s" % src)
         objs[src] = obj
-    # the MD driver (Mixture decks) and the MC driver (mc2 inputs)
-    for prog, exe in (("g6md.f90", DRIVER), ("g6mc.f90", MC_DRIVER)):
+    # the MD driver (Mixture decks), the MC driver (mc2 inputs) and the
+    # replica-exchange MD driver
+    for prog, exe in (("g6md.f90", DRIVER), ("g6mc.f90", MC_DRIVER), ("g6rx.f90", RX_DRIVER)):
         cmd = [FLANG, "-O2", "-o", exe + ".tmp", objs["g6gpu.f90"], objs[prog],
                "-L" + LIBDIR, "-lg6gpu", "-Wl,-rpath,$ORIGIN"]
         r = subprocess.run(cmd, capture_output=True, text=True)
