@@ -151,6 +151,21 @@ struct MdRep {
 //   rtr[n] step sizes, u[ucap] the Random_GetNext stream of this call and
 //   z[ucap] the Box-Muller factor sqrt(-2 ln u[k]) sin(2 pi u[k+1]) of each
 //   position, pos/nacc the stream position and accept count, epsum.
+// Umbrellas of the MC trial loop: potentials coeffi (op - center)^2 on the
+// triangle histogram's divergence (kinds 1 and 4, log for 4) or the energy
+// (kind 3).  The triangle tables hold, per molecule, the molecules within
+// 5 A at the start of the call (kTriNear entries at most): label and the
+// vector to it with its squared length (dx, dy, dz, dd).
+constexpr int kUmbMax = 4;
+constexpr int kTriNear = 64;
+struct McUmb {
+  int numb = 0;
+  int kind[kUmbMax] = {0, 0, 0, 0};
+  double center[kUmbMax] = {0, 0, 0, 0}, coeffi[kUmbMax] = {0, 0, 0, 0};
+  double refhist[21] = {};
+  int tri = 0;  // any triangle kind
+};
+
 struct McBuf {
   double *com = nullptr, *quat = nullptr, *sx = nullptr, *sy = nullptr, *sz = nullptr;
   double *pinfo = nullptr, *delta = nullptr, *rtr = nullptr;
@@ -158,6 +173,10 @@ struct McBuf {
   int *pos = nullptr;
   long long *nacc = nullptr;
   int ucap = 0;
+  // umbrella state: triangle tables [n][kTriNear], histogram, the order
+  // parameters after the last call, the capacity flag of the tables
+  int *tri_nn = nullptr, *tri_near = nullptr, *tri_hist = nullptr, *uflag = nullptr;
+  double *tri_v = nullptr, *uop = nullptr;
 };
 
 // Host side of the MC random stream: the reference's MT19937 (mtrnd.f) and
@@ -196,6 +215,7 @@ struct System {
   bool mc_ready = false;
   McHost mch;
   double mc_beta = 0;  // 1/T of the MC run
+  McUmb mc_umb;
   StdTable std{};
   int pcap = 0;
   bool nose_active = false;

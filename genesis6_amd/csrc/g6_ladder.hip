@@ -219,9 +219,19 @@ static int ladder_setup(Ladder &l) {
   return 0;
 }
 
-// replica_exchange_trial (Replica.F90:7-73) without umbrellas, inside the
-// exchange loop of Mixture.F90:1605-1622
-static int rx_trials(Rx &x, const double *energy, double *beta) {
+static double rx_umbrella(double coeffi, double center, double op) {
+  return coeffi * ((op - center) * (op - center));  // Umbrella.F90:11-16
+}
+
+// replica_exchange_trial (Replica.F90:7-73) inside the exchange loop of
+// Mixture.F90:1605-1622 (numb = 0) or of mc2.F90:973-996: with numb
+// umbrellas per node, each state's energy adds its umbrellas at its order
+// parameters (op, center, coeffi indexed (node-1)*numb + k), the swapped
+// state the other node's umbrellas, and an accepted swap exchanges the
+// nodes' umbrellas with their betas
+static int rx_trials(Rx &x, const double *energy, double *beta, int numb = 0,
+                     const double *op = nullptr, double *center = nullptr,
+                     double *coeffi = nullptr) {
   int acc = 0;
   const int nbond = (int)x.a.size();
   for (int t = 0; t < x.nexchange; t++) {
@@ -231,8 +241,19 @@ static int rx_trials(Rx &x, const double *energy, double *beta) {
     if (k > nbond) k = nbond;
     const int j = x.a[k - 1], jj = x.b[k - 1];
     const int n0 = x.nic[j - 1], n1 = x.nic[jj - 1];
-    const double a0 = energy[n0 - 1], a1 = energy[n1 - 1];
+    const int n00 = (n0 - 1) * numb, n11 = (n1 - 1) * numb;
+    double a0 = energy[n0 - 1], a1 = energy[n1 - 1];
+    for (int u = 0; u < numb; u++) {
+      a0 = a0 + rx_umbrella(coeffi[n00 + u], center[n00 + u], op[n00 + u]);
+      a1 = a1 + rx_umbrella(coeffi[n11 + u], center[n11 + u], op[n11 + u]);
+    }
     const double acc0 = -(a0 * beta[n0 - 1] + a1 * beta[n1 - 1]) * kI2J_rx * kJ2K_rx;
+    a0 = energy[n0 - 1];
+    a1 = energy[n1 - 1];
+    for (int u = 0; u < numb; u++) {
+      a0 = a0 + rx_umbrella(coeffi[n11 + u], center[n11 + u], op[n00 + u]);
+      a1 = a1 + rx_umbrella(coeffi[n00 + u], center[n00 + u], op[n11 + u]);
+    }
     const double acc1 = -(a0 * beta[n1 - 1] + a1 * beta[n0 - 1]) * kI2J_rx * kJ2K_rx;
     bool accept = false;
     if (acc0 < acc1) {
@@ -242,6 +263,10 @@ static int rx_trials(Rx &x, const double *energy, double *beta) {
       if (x.rand.getnext() < ratio) accept = true;
     }
     if (accept) {
+      for (int u = 0; u < numb; u++) {
+        std::swap(center[n00 + u], center[n11 + u]);
+        std::swap(coeffi[n00 + u], coeffi[n11 + u]);
+      }
       const double d = beta[n0 - 1];
       beta[n0 - 1] = beta[n1 - 1];
       beta[n1 - 1] = d;
@@ -488,6 +513,17 @@ int g6_rx_trials(g6_rx hx, const double *energy, double *beta, int32_t *naccept)
   Rx *x = xget(hx);
   if (!x || !energy || !beta) return G6_EARG;
   const int acc = rx_trials(*x, energy, beta);
+  if (naccept) *naccept = acc;
+  return 0;
+}
+
+int g6_rx_trials_umbrella(g6_rx hx, const double *energy, double *beta, int numbrella,
+                          const double *op, double *center, double *coeffi,
+                          int32_t *naccept) {
+  Rx *x = xget(hx);
+  if (!x || !energy || !beta || numbrella < 0) return G6_EARG;
+  if (numbrella > 0 && (!op || !center || !coeffi)) return G6_EARG;
+  const int acc = rx_trials(*x, energy, beta, numbrella, op, center, coeffi);
   if (naccept) *naccept = acc;
   return 0;
 }

@@ -66,6 +66,7 @@ struct McArgs {
   double beta;
   double bx[kMaxSite], by[kMaxSite], bz[kMaxSite];
   McBuf b;
+  McUmb u;
 };
 
 __global__ void k_mc_sites(McArgs a) {
@@ -115,6 +116,141 @@ __global__ __launch_bounds__(64) void k_mc_pairinfo(McArgs a, StdTable st) {
   if (lane == 0) a.b.part[i] = part;
 }
 
+// ---- umbrellas (mc2.F90:509-521, 684-771; Triangle.F90; Umbrella.F90) ----
+//
+// The triangle order parameter is the Kullback-Leibler divergence of the
+// histogram of O-O-O angle cosines (21 bins of 0.1 from -1), taken at every
+// molecule between the vectors to its four nearest neighbours, from a
+// reference histogram.  mc2 keeps, for one loop, each molecule's neighbours
+// within 5 A at the loop's start (nei2) and moves only their vectors as
+// molecules move (neighbors_displacement): the membership is fixed for the
+// loop.  The tables here hold the same entries and vectors, unsorted; the
+// nearest four are selected by distance where needed.  A trial changes the
+// angles of the target and of the molecules in its table only (the vector
+// to the target), so the new histogram is the old one plus, over those
+// rows, the new row's angle bins minus the old row's: integer counts, the
+// same numbers as the reference's set-wise update.
+
+// keeps the four smallest keys (ascending; a key equal to a kept one goes
+// behind it, as the reference's insertion in front of the first strictly
+// larger key, and a full list drops it)
+__device__ inline void top4_insert(double (&d)[4], int (&e)[4], double key, int idx) {
+  if (!(key < d[3])) return;
+  if (key < d[2]) {
+    d[3] = d[2];
+    e[3] = e[2];
+    if (key < d[1]) {
+      d[2] = d[1];
+      e[2] = e[1];
+      if (key < d[0]) {
+        d[1] = d[0];
+        e[1] = e[0];
+        d[0] = key;
+        e[0] = idx;
+      } else {
+        d[1] = key;
+        e[1] = idx;
+      }
+    } else {
+      d[2] = key;
+      e[2] = idx;
+    }
+  } else {
+    d[3] = key;
+    e[3] = idx;
+  }
+}
+
+// bin (0..20, or -1 outside) of the angle between two table vectors:
+// costh = v_j.v_k / sqrt(dd_j dd_k), bin = nint((costh + 1) / 0.1)
+// (Triangle.F90:95-103, DistribModule.F90:149-158)
+__device__ inline int tri_bin(const double4 &a, const double4 &b) {
+  double c = a.x * b.x + a.y * b.y + a.z * b.z;
+  c = c / sqrt(a.w * b.w);
+  const double r = round((c - (-1.0)) / 0.1);
+  return (r >= 0.0 && r <= 20.0) ? (int)r : -1;
+}
+
+// the table entry moved by the trial: the target's own vectors lose the
+// displacement, the others' vectors to the target gain it
+// (neighbors_displacement, Neighbor.F90:401-457)
+__device__ inline double4 tri_moved(double4 v, double sgn_dx, double sgn_dy, double sgn_dz) {
+  v.x = v.x + sgn_dx;
+  v.y = v.y + sgn_dy;
+  v.z = v.z + sgn_dz;
+  v.w = v.x * v.x + v.y * v.y + v.z * v.z;
+  return v;
+}
+
+// the six (or fewer) angle bins of a row's nearest four, added to h with
+// weight w (LDS integer atomics: exact, order free)
+__device__ inline void tri_row_add(int *h, const double4 (&v)[4], int k, int w) {
+  for (int p = 0; p < k; p++)
+    for (int q = p + 1; q < k; q++) {
+      const int bin = tri_bin(v[p], v[q]);
+      if (bin >= 0) atomicAdd(&h[bin], w);
+    }
+}
+
+// sTriangle_prepareall (Triangle.F90:53-116) for molecule i: its entries
+// within 5 A (the loop's pair list within rc, neighbors_all2 radius 5: kept
+// when dd <= 25), the vector v = c_j - c_i less the nearest image shift, as
+// the reference forms it from the pair (i, j) in either orientation, and
+// its nearest four's angle bins added to the histogram
+__global__ __launch_bounds__(256) void k_mc_umb_prepare(McArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int n = a.n;
+  const double xi = a.b.com[i], yi = a.b.com[n + i], zi = a.b.com[2 * n + i];
+  const double rr = a.rc * a.rc;
+  int cnt = 0;
+  double dk[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
+  int ek[4] = {-1, -1, -1, -1};
+  double4 *row = reinterpret_cast<double4 *>(a.b.tri_v) + (size_t)i * kTriNear;
+  for (int j = 0; j < n; j++) {
+    if (j == i) continue;
+    double d0 = a.b.com[j] - xi, d1 = a.b.com[n + j] - yi, d2 = a.b.com[2 * n + j] - zi;
+    d0 = d0 - round(d0 * a.invL) * a.L;
+    d1 = d1 - round(d1 * a.invL) * a.L;
+    d2 = d2 - round(d2 * a.invL) * a.L;
+    const double dd = d0 * d0 + d1 * d1 + d2 * d2;
+    if (!(dd < rr) || 25.0 < dd) continue;
+    if (cnt < kTriNear) {
+      a.b.tri_near[(size_t)i * kTriNear + cnt] = j;
+      row[cnt] = make_double4(d0, d1, d2, dd);
+      top4_insert(dk, ek, dd, cnt);
+    }
+    cnt++;
+  }
+  if (cnt > kTriNear) atomicCAS(a.b.uflag, 0, 1);
+  const int m = cnt < kTriNear ? cnt : kTriNear;
+  a.b.tri_nn[i] = m;
+  const int k = m < 4 ? m : 4;
+  double4 v[4];
+  for (int p = 0; p < k; p++) v[p] = row[ek[p]];
+  tri_row_add(a.b.tri_hist, v, k, 1);
+}
+
+// kullback_div (DistribModule.F90:304-326) of h (counts) from ref
+__device__ inline double kl_div(const int *h, const double *ref) {
+  double s1 = 0.0, s2 = 0.0, div = 0.0;
+  for (int b = 0; b < 21; b++) {
+    s1 = s1 + (double)h[b];
+    s2 = s2 + ref[b];
+  }
+  for (int b = 0; b < 21; b++) {
+    const double p = (double)h[b] / s1;
+    double q = ref[b] / s2;
+    if (q == 0) q = p;
+    if (p != 0) div = div + p * log(p / q);
+  }
+  return div;
+}
+
+__device__ inline double umbrella_potential(double coeffi, double center, double op) {
+  return coeffi * ((op - center) * (op - center));
+}
+
 // mc2's trial loop, ntrial trials, one block.  The configuration (COM,
 // quaternion, step sizes) lives in LDS for the whole call, and the random
 // stream is staged into LDS kMcChunk trials at a time; the only global
@@ -134,7 +270,13 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_trials(McArgs a, StdTable st,
   double *l_u = l_rtr + n;                    // [kMcStage]
   double *l_z = l_u + kMcStage;               // [kMcStage]
   __shared__ double s_red[kMcThreads / 64];
+  // umbrella state: histogram, the trial's change, reference, order
+  // parameters now and after the trial, the trial's total energy change
+  __shared__ int s_hist[21], s_dh[21];
+  __shared__ double s_ref[21], s_op[kUmbMax], s_newop[kUmbMax], s_des;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int numb = a.u.numb;
+  if (numb > 0 && *a.b.uflag != 0) return;  // the tables did not fit
   for (int k = tid; k < 3 * n; k += kMcThreads) l_com[k] = a.b.com[k];
   for (int k = tid; k < 4 * n; k += kMcThreads) l_q[k] = a.b.quat[k];
   for (int k = tid; k < n; k += kMcThreads) {
@@ -145,6 +287,19 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_trials(McArgs a, StdTable st,
   int base = -1;  // stream position of l_u[0]
   long long nacc = 0;
   double epsum = *a.b.epsum;
+  if (numb > 0) {
+    if (tid < 21) {
+      s_hist[tid] = a.u.tri ? a.b.tri_hist[tid] : 0;
+      s_dh[tid] = 0;
+      s_ref[tid] = a.u.refhist[tid];
+    }
+    __syncthreads();
+    if (tid == 0) {  // prepareall's op (mc2.F90:509-521)
+      const double div = a.u.tri ? kl_div(s_hist, s_ref) : 0.0;
+      for (int k = 0; k < numb; k++)
+        s_op[k] = a.u.kind[k] == 3 ? epsum : (a.u.kind[k] == 4 ? log(div) : div);
+    }
+  }
   for (int trial = 0; trial < ntrial; trial++) {
     if (trial % kMcChunk == 0) {  // stage the next chunk (a trial reads <= 14)
       __syncthreads();
@@ -240,10 +395,74 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_trials(McArgs a, StdTable st,
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) local += __shfl_xor(local, d, 64);
     if (lane == 0) s_red[wid] = local;
+    // the triangle rows the trial changes: the target (tid 0) and the
+    // molecules of its table (tid 1..nt); tslot = the row's entry of the
+    // target, whose vector the move changes
+    int trow = -1, tslot = -1;
+    if (a.u.tri) {
+      const int nt = a.b.tri_nn[target];
+      if (tid <= nt) {
+        trow = tid == 0 ? target : a.b.tri_near[(size_t)target * kTriNear + tid - 1];
+        const int m = a.b.tri_nn[trow];
+        const double4 *row = reinterpret_cast<const double4 *>(a.b.tri_v) + (size_t)trow * kTriNear;
+        const int *lab = a.b.tri_near + (size_t)trow * kTriNear;
+        double dold[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
+        double dnew[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
+        int eold[4] = {-1, -1, -1, -1}, enew4[4] = {-1, -1, -1, -1};
+        for (int e = 0; e < m; e++) {
+          const double4 v = row[e];
+          top4_insert(dold, eold, v.w, e);
+          double nd = v.w;
+          if (tid == 0) {
+            nd = tri_moved(v, -deltax, -deltay, -deltaz).w;
+          } else if (lab[e] == target) {
+            nd = tri_moved(v, deltax, deltay, deltaz).w;
+            tslot = e;
+          }
+          top4_insert(dnew, enew4, nd, e);
+        }
+        const int k = m < 4 ? m : 4;
+        double4 v[4];
+        for (int p = 0; p < k; p++) v[p] = row[eold[p]];
+        tri_row_add(s_dh, v, k, -1);
+        for (int p = 0; p < k; p++) {
+          v[p] = row[enew4[p]];
+          if (tid == 0)
+            v[p] = tri_moved(v[p], -deltax, -deltay, -deltaz);
+          else if (enew4[p] == tslot)
+            v[p] = tri_moved(v[p], deltax, deltay, deltaz);
+        }
+        tri_row_add(s_dh, v, k, 1);
+      }
+    }
     __syncthreads();
     double de = 0.0;
 #pragma unroll
     for (int w = 0; w < kMcThreads / 64; w++) de = de + s_red[w];
+    const double deltae = de;
+    if (numb > 0) {
+      // deltaesum = deltae + sum(U(newop) - U(op)) (mc2.F90:684-706),
+      // formed once and shared
+      if (tid == 0) {
+        double div = 0.0;
+        if (a.u.tri) {
+          int h[21];
+          for (int b = 0; b < 21; b++) h[b] = s_hist[b] + s_dh[b];
+          div = kl_div(h, s_ref);
+        }
+        double dope = 0.0;
+        for (int k = 0; k < numb; k++) {
+          const double nop =
+              a.u.kind[k] == 3 ? epsum + deltae : (a.u.kind[k] == 4 ? log(div) : div);
+          s_newop[k] = nop;
+          dope = dope + umbrella_potential(a.u.coeffi[k], a.u.center[k], nop) -
+                 umbrella_potential(a.u.coeffi[k], a.u.center[k], s_op[k]);
+        }
+        s_des = deltae + dope;
+      }
+      __syncthreads();
+      de = s_des;
+    }
     bool accept;
     if (de < 0.0) {
       accept = true;
@@ -252,9 +471,28 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_trials(McArgs a, StdTable st,
       accept = l_u[pos - base] < ratio;
       pos += 1;
     }
+    if (numb > 0) {
+      if (accept) {
+        if (tid < numb) s_op[tid] = s_newop[tid];
+        if (tid < 21) s_hist[tid] = s_hist[tid] + s_dh[tid];
+        // sTriangle_Integrate: the moved vectors into the tables
+        if (trow >= 0) {
+          double4 *row = reinterpret_cast<double4 *>(a.b.tri_v);
+          if (tid > 0) {
+            const size_t et = (size_t)target * kTriNear + tid - 1;
+            row[et] = tri_moved(row[et], -deltax, -deltay, -deltaz);
+            if (tslot >= 0) {
+              const size_t es = (size_t)trow * kTriNear + tslot;
+              row[es] = tri_moved(row[es], deltax, deltay, deltaz);
+            }
+          }
+        }
+      }
+      if (tid < 21) s_dh[tid] = 0;
+    }
     if (accept) {
       nacc += 1;
-      epsum = epsum + de;
+      epsum = epsum + deltae;
 #pragma unroll
       for (int m = 0; m < kMcMaxPer; m++) {
         const int j = tid + m * kMcThreads;
@@ -300,6 +538,7 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_trials(McArgs a, StdTable st,
     *a.b.pos = pos;
     *a.b.nacc = *a.b.nacc + nacc;
     *a.b.epsum = epsum;
+    for (int k = 0; k < numb; k++) a.b.uop[k] = s_op[k];
   }
 }
 
@@ -322,16 +561,19 @@ static McArgs mc_args(const System &s) {
     a.bz[k] = s.mt.bz[k];
   }
   a.b = s.mc;
+  a.u = s.mc_umb;
   return a;
 }
 
 void mc_free(System &s) {
   McBuf &m = s.mc;
   void *ps[] = {m.com, m.quat, m.sx, m.sy, m.sz, m.pinfo, m.delta, m.rtr, m.u, m.z,
-                m.epsum, m.part, m.pos, m.nacc};
+                m.epsum, m.part, m.pos, m.nacc, m.tri_nn, m.tri_near, m.tri_hist,
+                m.uflag, m.tri_v, m.uop};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   m = McBuf{};
+  s.mc_umb = McUmb{};
   s.mc_ready = false;
 }
 
@@ -446,6 +688,15 @@ int g6_mc_trials(g6_handle h, int ntrial, long long *naccept) {
   G6_CHECK(hipMemcpy(m.u, hst->pend.data(), sizeof(double) * need, hipMemcpyHostToDevice));
   G6_CHECK(hipMemcpy(m.z, z.data(), sizeof(double) * need, hipMemcpyHostToDevice));
   McArgs a = mc_args(*s);
+  if (s->mc_umb.numb > 0) {
+    // the order parameters of the loop's start (mc2.F90:486-521)
+    G6_CHECK(hipMemsetAsync(m.uflag, 0, sizeof(int), s->stream));
+    if (s->mc_umb.tri) {
+      G6_CHECK(hipMemsetAsync(m.tri_hist, 0, sizeof(int) * 21, s->stream));
+      k_mc_umb_prepare<<<(s->n + 255) / 256, 256, 0, s->stream>>>(a);
+      G6_CHECK(hipGetLastError());
+    }
+  }
   const size_t lds = mc_lds(s->n);
   hipError_t attr = hipSuccess;
   std_model(s->model, [&](auto m) {
@@ -456,14 +707,73 @@ int g6_mc_trials(g6_handle h, int ntrial, long long *naccept) {
   });
   G6_CHECK(attr);
   G6_CHECK(hipGetLastError());
-  int used = 0;
+  int used = 0, uflag = 0;
   long long nacc = 0;
+  if (s->mc_umb.numb > 0)
+    G6_CHECK(hipMemcpyAsync(&uflag, m.uflag, sizeof(int), hipMemcpyDeviceToHost, s->stream));
   G6_CHECK(hipMemcpyAsync(&used, m.pos, sizeof(int), hipMemcpyDeviceToHost, s->stream));
   G6_CHECK(hipMemcpyAsync(&nacc, m.nacc, sizeof(long long), hipMemcpyDeviceToHost, s->stream));
   G6_CHECK(hipStreamSynchronize(s->stream));
+  if (uflag) return G6_ECAPACITY;  // nothing ran: the stream is kept whole
   if (used < 0 || (size_t)used > need) return G6_EHIP;
   hst->pend.erase(hst->pend.begin(), hst->pend.begin() + used);
   if (naccept) *naccept = nacc;
+  return 0;
+}
+
+int g6_mc_set_umbrella(g6_handle h, int numbrella, const int32_t *kind, const double *center,
+                       const double *coeffi, const double *refhist) {
+  System *s = system_of(h);
+  if (!s || !s->mc_ready || numbrella < 0 || numbrella > kUmbMax) return G6_EARG;
+  if (numbrella > 0 && (!kind || !center || !coeffi)) return G6_EARG;
+  McUmb u;
+  u.numb = numbrella;
+  for (int k = 0; k < numbrella; k++) {
+    const int t = kind[k];
+    if (t != G6_UMB_TRIANGLE && t != G6_UMB_EPOP && t != G6_UMB_LOGTRIAN) return G6_EARG;
+    u.kind[k] = t;
+    u.center[k] = center[k];
+    u.coeffi[k] = coeffi[k];
+    if (t != G6_UMB_EPOP) u.tri = 1;
+  }
+  if (u.tri) {
+    if (!refhist) return G6_EARG;
+    for (int b = 0; b < 21; b++) u.refhist[b] = refhist[b];
+  }
+  McBuf &m = s->mc;
+  const int n = s->n;
+  if (!m.uop) {
+    G6_CHECK(hipMalloc(&m.uop, sizeof(double) * kUmbMax));
+    G6_CHECK(hipMalloc(&m.uflag, sizeof(int)));
+    G6_CHECK(hipMemset(m.uop, 0, sizeof(double) * kUmbMax));
+  }
+  if (u.tri && !m.tri_v) {
+    G6_CHECK(hipMalloc(&m.tri_nn, sizeof(int) * n));
+    G6_CHECK(hipMalloc(&m.tri_near, sizeof(int) * (size_t)n * kTriNear));
+    G6_CHECK(hipMalloc(&m.tri_hist, sizeof(int) * 21));
+    G6_CHECK(hipMalloc(&m.tri_v, sizeof(double) * 4 * (size_t)n * kTriNear));
+  }
+  s->mc_umb = u;
+  return 0;
+}
+
+int g6_mc_get_umbrella(g6_handle h, double *op, double *center, double *coeffi) {
+  System *s = system_of(h);
+  if (!s || !s->mc_ready) return G6_EARG;
+  const McUmb &u = s->mc_umb;
+  G6_CHECK(hipStreamSynchronize(s->stream));
+  if (op && u.numb > 0) G6_CHECK(hipMemcpy(op, s->mc.uop, sizeof(double) * u.numb, hipMemcpyDeviceToHost));
+  for (int k = 0; k < u.numb; k++) {
+    if (center) center[k] = u.center[k];
+    if (coeffi) coeffi[k] = u.coeffi[k];
+  }
+  return 0;
+}
+
+int g6_mc_set_temperature(g6_handle h, double temp) {
+  System *s = system_of(h);
+  if (!s || !s->mc_ready || !(temp > 0)) return G6_EARG;
+  s->mc_beta = 1.0 / temp;
   return 0;
 }
 

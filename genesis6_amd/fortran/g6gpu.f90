@@ -13,6 +13,7 @@ module g6gpu
        G6_ENODEVICE = -4
   integer, parameter :: G6_THERMO_LEN = 32
   integer(c_int), parameter :: G6_LADDER_MAP_XCD = 0, G6_LADDER_MAP_FLAT = 1
+  integer(c_int), parameter :: G6_UMB_TRIANGLE = 1, G6_UMB_EPOP = 3, G6_UMB_LOGTRIAN = 4
   integer(c_int), parameter :: G6_ORDER_NONE = 0, G6_ORDER_BY_DISTANCE = 1, &
        G6_ORDER_BY_LABEL = 2
   integer, parameter :: G6_NEAR = 151
@@ -327,6 +328,29 @@ module g6gpu
        real(c_double), intent(out) :: epsum
      end function g6_mc_energy
 
+     ! umbrellas of the trial loop (@GEUM): kinds G6_UMB_*, refhist(21)
+     integer(c_int) function g6_mc_set_umbrella(h, numbrella, kind, center, coeffi, refhist) &
+          bind(C, name="g6_mc_set_umbrella")
+       import :: c_int, c_int32_t, c_int64_t, c_double
+       integer(c_int64_t), value :: h
+       integer(c_int), value :: numbrella
+       integer(c_int32_t), intent(in) :: kind(*)
+       real(c_double), intent(in) :: center(*), coeffi(*), refhist(*)
+     end function g6_mc_set_umbrella
+
+     integer(c_int) function g6_mc_get_umbrella(h, op, center, coeffi) &
+          bind(C, name="g6_mc_get_umbrella")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), value :: h
+       real(c_double), intent(out) :: op(*), center(*), coeffi(*)
+     end function g6_mc_get_umbrella
+
+     integer(c_int) function g6_mc_set_temperature(h, temp) bind(C, name="g6_mc_set_temperature")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), value :: h
+       real(c_double), value :: temp
+     end function g6_mc_set_temperature
+
      ! replica ladders and exchange groups (the replica branch of Mixture.F90)
      integer(c_int) function g6_ladder_create(hl, nsys, hs) bind(C, name="g6_ladder_create")
        import :: c_int, c_int64_t
@@ -394,6 +418,16 @@ module g6gpu
        real(c_double), intent(inout) :: beta(*)
        integer(c_int32_t), intent(out) :: naccept
      end function g6_rx_trials
+
+     integer(c_int) function g6_rx_trials_umbrella(hx, energy, beta, numbrella, op, center, &
+          coeffi, naccept) bind(C, name="g6_rx_trials_umbrella")
+       import :: c_int, c_int32_t, c_int64_t, c_double
+       integer(c_int64_t), value :: hx
+       real(c_double), intent(in) :: energy(*), op(*)
+       real(c_double), intent(inout) :: beta(*), center(*), coeffi(*)
+       integer(c_int), value :: numbrella
+       integer(c_int32_t), intent(out) :: naccept
+     end function g6_rx_trials_umbrella
 
      integer(c_int) function g6_rx_nodes(hx, node_in_charge) bind(C, name="g6_rx_nodes")
        import :: c_int, c_int64_t, c_int32_t

@@ -9,16 +9,21 @@
 ! @INNR inner loops | @NLOG interval | @SNPO interval | @TEMP T |
 ! @DELT n, then n lines delta rtratio | @RCOA rc taper | @BXLA L |
 ! @BOX3 L L L | @VOXA | @VOXN | @BOOK | @NCMP 1 | @ID08 model |
-! @NX4A n, then n lines x y z a b c d.  Reading stops after the @NX4A of
-! the last component, as ReadParams stops.  Tags it does not know are passed
-! over, as ReadParams passes them over.  The umbrella tags (@HIST, @REPQ,
-! @GEUM), a saved random stream (@MTRN) and more than one node stop the run:
-! those paths are not on the GPU.
+! @NX4A n, then n lines x y z a b c d | the umbrellas: @HIST (nbin 21,
+! width, minimum, then the 21 counts of the reference histogram), @REPQ
+! center coeffi (a TRIANGLE umbrella), @GEUM kind, then center coeffi
+! (kinds TRIANGLE, LOGTRIAN, EPOP____; at most 4).  Reading stops after the
+! @NX4A of the last component, as ReadParams stops.  Tags it does not know
+! are passed over, as ReadParams passes them over.  A Q6LOCAL_ umbrella, a
+! saved random stream (@MTRN) and more than one node stop the run: those
+! paths are not on the GPU.
 !
 ! The run is mc2's main loop (mc2.F90:379-899): loop = 0, 1, ...; each loop
 ! is 2N single-molecule trials (g6_mc_trials, the decisions of mc2's trial
-! loop), then the configuration block when mod(loop, snpo) = 0 and the @MC2O
-! line (loop, energy in internal units, T) when mod(loop, nlog) = 0.  The
+! loop, with the umbrellas' change in every trial's energy difference), then
+! the configuration block when mod(loop, snpo) = 0 and the @MC2O line (loop,
+! energy in internal units, 1/beta, then op, center and coeffi of each
+! umbrella, mc2.F90:876-884) when mod(loop, nlog) = 0.  The
 ! output file is the node's input file name with ".out" appended; at the end
 ! it gets mc2's @CONT block: @DELT, @TEMP, @BOX3 and the final @NX4A
 ! (mc2.F90:1009-1025), without the saved random stream (@MTRN).
@@ -41,6 +46,10 @@ program g6mc
   integer(c_long_long) :: nacc
   real(c_double), allocatable :: com(:,:), quat(:,:,:), dx(:,:,:), w(:,:,:)
   real(c_double), allocatable :: delta(:), rtratio(:), com1(:,:), q1(:,:)
+  integer(c_int32_t) :: ukind(4) = 0
+  real(c_double) :: ucenter(4) = 0d0, ucoeffi(4) = 0d0, uop(4) = 0d0, refhist(21) = 0d0
+  integer :: numb = 0
+  logical :: have_hist = .false.
   character(len=100), parameter :: fmt0 = '(99(e25.17))', fmt1 = '(i11,1x,99(e25.17))'
   integer, parameter :: OUT = 13
 
@@ -73,6 +82,14 @@ program g6mc
   call g6_check(g6_init(h, model, n, box, rc, taper, voxa(1), 0.002d0, G6_FP64, 0d0), 'g6_init')
   call g6_check(g6_upload_state(h, com, quat, dx, w), 'g6_upload_state')
   call g6_check(g6_mc_begin(h, temp, seed, delta, rtratio), 'g6_mc_begin')
+  if (numb > 0) then
+     if (any(ukind(1:numb) /= G6_UMB_EPOP) .and. .not. have_hist) then
+        write(0, '(a)') 'g6mc: a triangle umbrella needs @HIST before it'
+        error stop 2
+     end if
+     call g6_check(g6_mc_set_umbrella(h, int(numb, c_int), ukind, ucenter, ucoeffi, refhist), &
+          'g6_mc_set_umbrella')
+  end if
   allocate(com1(n, 3), q1(n, 4))
 
   open(OUT, file=trim(infile) // '.out', status='replace')
@@ -84,6 +101,8 @@ program g6mc
      do inner = 1, ninnerloop
         call g6_check(g6_mc_trials(h, int(2 * n, c_int), nacc), 'g6_mc_trials')
         call g6_check(g6_mc_get(h, com1, q1, delta, rtratio, epsum), 'g6_mc_get')
+        if (numb > 0) call g6_check(g6_mc_get_umbrella(h, uop, ucenter, ucoeffi), &
+             'g6_mc_get_umbrella')
         if (snpo > 0) then
            if (mod(loop, snpo) == 0) then
               write(OUT, '("@NCMP")')
@@ -94,7 +113,8 @@ program g6mc
         if (nlog > 0) then
            if (mod(loop, nlog) == 0) then
               write(OUT, '("@MC2O")')
-              write(OUT, fmt1) loop, epsum, temp
+              write(OUT, fmt1) loop, epsum, 1d0 / (1d0 / temp), &
+                   (uop(i), ucenter(i), ucoeffi(i), i = 1, numb)
            end if
         end if
         loop = loop + 1
@@ -118,6 +138,8 @@ contains
   subroutine read_params(path)
     character(len=*), intent(in) :: path
     integer :: u, k
+    real(c_double) :: w, wmin
+    character(len=8) :: uid
     open(newunit=u, file=path, status='old', action='read')
     do
        read(u, '(a5)', iostat=ios) tag
@@ -175,7 +197,32 @@ contains
           read(u, *) voxn
        case ('@BOOK')
           read(u, *) book_interval, book_margin
-       case ('@HIST', '@REPQ', '@GEUM', '@MTRN', '@RXPR', '@NRXG')
+       case ('@HIST')                          ! histogram_load
+          read(u, *) k, w, wmin
+          if (k /= 21) then
+             write(0, '(a)') 'g6mc: @HIST needs the 21 bins of the triangle histogram'
+             error stop 2
+          end if
+          do k = 1, 21
+             read(u, *) refhist(k)
+          end do
+          have_hist = .true.
+       case ('@REPQ')                          ! a TRIANGLE umbrella
+          call add_umbrella(G6_UMB_TRIANGLE, u)
+       case ('@GEUM')
+          read(u, *) uid
+          select case (uid)
+          case ('TRIANGLE')
+             call add_umbrella(G6_UMB_TRIANGLE, u)
+          case ('LOGTRIAN')
+             call add_umbrella(G6_UMB_LOGTRIAN, u)
+          case ('EPOP____')
+             call add_umbrella(G6_UMB_EPOP, u)
+          case default
+             write(0, '(a,a)') 'g6mc: umbrella not on the GPU path: ', uid
+             error stop 2
+          end select
+       case ('@MTRN', '@RXPR', '@NRXG')
           write(0, '(a,a)') 'g6mc: not on the GPU path: ', tag
           error stop 2
        case default
@@ -183,6 +230,19 @@ contains
     end do
     close(u)
   end subroutine read_params
+
+  ! umbrella_ReadParam (Umbrella.F90:18-23): center coeffi
+  subroutine add_umbrella(kind, u)
+    integer(c_int), intent(in) :: kind
+    integer, intent(in) :: u
+    if (numb == 4) then
+       write(0, '(a)') 'g6mc: at most 4 umbrellas on the GPU path'
+       error stop 2
+    end if
+    numb = numb + 1
+    ukind(numb) = kind
+    read(u, *) ucenter(numb), ucoeffi(numb)
+  end subroutine add_umbrella
 
   subroutine read_nx4a(u)
     integer, intent(in) :: u

@@ -93,6 +93,9 @@ def lib():
         L.g6_mc_get.argtypes = [_H, _dp, _dp, _dp, _dp, _dp]
         L.g6_mc_pairinfo.argtypes = [_H, _dp, _dp]
         L.g6_mc_energy.argtypes = [_H, _dp]
+        L.g6_mc_set_umbrella.argtypes = [_H, C.c_int, _ip, _dp, _dp, _dp]
+        L.g6_mc_get_umbrella.argtypes = [_H, _dp, _dp, _dp]
+        L.g6_mc_set_temperature.argtypes = [_H, C.c_double]
         L.g6_ladder_create.argtypes = [C.POINTER(_H), C.c_int, _i64a]
         L.g6_ladder_free.argtypes = [_H]
         L.g6_ladder_step.argtypes = [_H, C.c_int]
@@ -103,6 +106,8 @@ def lib():
                                    C.c_int, C.c_uint32]
         L.g6_rx_free.argtypes = [_H]
         L.g6_rx_trials.argtypes = [_H, _dp, _dp, C.POINTER(C.c_int32)]
+        L.g6_rx_trials_umbrella.argtypes = [_H, _dp, _dp, C.c_int, _dp, _dp, _dp,
+                                            C.POINTER(C.c_int32)]
         L.g6_rx_nodes.argtypes = [_H, _ip]
         L.g6_rx_exchange.argtypes = [_H, C.c_void_p, C.POINTER(C.c_int32)]
         L.g6_npt_begin.argtypes = [_H, C.c_double, C.c_double, C.c_uint64, C.c_double,
@@ -271,6 +276,29 @@ class System:
         ep = np.zeros(n * n); e = np.zeros(1)
         check(lib().g6_mc_pairinfo(self.h, ep, e), "g6_mc_pairinfo")
         return ep.reshape(n, n), float(e[0])
+
+    # umbrella kinds of @GEUM (mc2util.F90:174-191) on the device path
+    UMB = {"TRIANGLE": 1, "EPOP____": 3, "LOGTRIAN": 4}
+
+    def mc_set_umbrella(self, kinds, center, coeffi, refhist=None):
+        """Umbrellas coeffi (op - center)^2 on the order parameters named by
+        kinds (TRIANGLE, LOGTRIAN, EPOP____); each mc_trials call is then
+        one mc2 loop."""
+        k = np.array([self.UMB[x] for x in kinds], np.int32)
+        ref = _c(np.zeros(21) if refhist is None else refhist)
+        check(lib().g6_mc_set_umbrella(self.h, len(k), k, _c(center), _c(coeffi), ref),
+              "g6_mc_set_umbrella")
+
+    def mc_umbrella(self):
+        """(op, center, coeffi) per umbrella: op after the last mc_trials."""
+        nb = 4
+        op, c, k = np.zeros(nb), np.full(nb, np.nan), np.full(nb, np.nan)
+        check(lib().g6_mc_get_umbrella(self.h, op, c, k), "g6_mc_get_umbrella")
+        m = int(np.sum(~np.isnan(c)))
+        return op[:m], c[:m], k[:m]
+
+    def mc_set_temperature(self, temp):
+        check(lib().g6_mc_set_temperature(self.h, temp), "g6_mc_set_temperature")
 
     def mc_energy(self):
         e = np.zeros(1)
@@ -591,6 +619,19 @@ class ReplicaExchange:
         acc = C.c_int32(0)
         check(lib().g6_rx_trials(self.h, _c(energy), beta, C.byref(acc)), "g6_rx_trials")
         return beta, acc.value
+
+    def trials_umbrella(self, energy, beta, op, center, coeffi):
+        """The exchange loop weighted by umbrellas: op, center, coeffi are
+        (nrep, numbrella) in node order; returns the new betas, centers,
+        coefficients and the number of accepted swaps."""
+        beta = _c(beta).copy()
+        op = _c(op)
+        nb = op.shape[1] if op.ndim == 2 else 0
+        c, k = _c(center).copy(), _c(coeffi).copy()
+        acc = C.c_int32(0)
+        check(lib().g6_rx_trials_umbrella(self.h, _c(energy), beta, nb, op.ravel(), c.ravel(),
+                                          k.ravel(), C.byref(acc)), "g6_rx_trials_umbrella")
+        return beta, c, k, acc.value
 
     def nodes(self):
         nic = np.zeros(self.nrep, np.int32)

@@ -261,8 +261,9 @@ int g6_get_box(g6_handle h, double *box_l);
  * mc2.F90:303-340 builds it, StdPairPotential energies times S), its sum,
  * per-molecule step sizes delta[n] and rotation ratios rtratio[n] (@DELT),
  * beta = 1/temp (@TEMP) and the random stream seeded with seed (@SEED).
- * TIP4P and TIP5P (the StdPairPotential models); n <= 1500 (the trial
- * loop keeps the configuration in LDS). */
+ * The models whose pair energy is StdPairPotential's (TIP4P, TIP5P, ST2 as
+ * TIP5P, SPC/E, NvdE, OPLS methanol); n <= 1500 (the trial loop keeps the
+ * configuration in LDS).  Clears any umbrellas. */
 int g6_mc_begin(g6_handle h, double temp, uint32_t seed, const double *delta,
                 const double *rtratio);
 /* ntrial single-molecule trials of mc2 (mc2.F90:540-862): target from the
@@ -280,6 +281,34 @@ int g6_mc_pairinfo(g6_handle h, double *ep, double *epsum);
 /* The total energy of the current MC configuration from a fresh PairInfo
  * (the bookkeeping is left as it is). */
 int g6_mc_energy(g6_handle h, double *epsum);
+
+/* Umbrella sampling in the trial loop: mc2 built with OP (mc2.F90:18).
+ * Kinds of @GEUM (mc2util.F90:174-191): the triangle order parameter
+ * (sTriangle, Triangle.F90:53-280): the Kullback-Leibler divergence
+ * (DistribModule.F90:304-326) from refhist[21] of the histogram of O-O-O
+ * angle cosines (21 bins of 0.1 from -1) between each molecule's four
+ * nearest neighbours among those within 5 A at the loop's start, or its
+ * log (LOGTRIAN); the energy (EPOP____).  Q6LOCAL_ is not on this path
+ * (G6_EARG). */
+enum { G6_UMB_TRIANGLE = 1, G6_UMB_EPOP = 3, G6_UMB_LOGTRIAN = 4 };
+/* numbrella (<= 4; 0 removes them) potentials coeffi (op - center)^2
+ * (Umbrella.F90:11-16), refhist[21] for the triangle kinds (@HIST, read
+ * as mc2util.F90:156-159).  With umbrellas set, every g6_mc_trials call is
+ * one mc2 loop: it first prepares the order parameters on the current
+ * configuration (the 5 A tables from the pairs within rc, mc2.F90:486-521;
+ * op = the energy for EPOP____), then each trial adds
+ * sum(U(newop) - U(op)) to dE before the Metropolis test (mc2.F90:684-706)
+ * and an accepted trial moves the tables (sTriangle_Integrate).  A
+ * molecule with more than 64 molecules within 5 A stops the call before
+ * any trial: G6_ECAPACITY, the state unchanged.  Requires g6_mc_begin. */
+int g6_mc_set_umbrella(g6_handle h, int numbrella, const int32_t *kind, const double *center,
+                       const double *coeffi, const double *refhist);
+/* op[numbrella]: each order parameter after the last g6_mc_trials call
+ * (what mc2's @MC2O line prints, mc2.F90:876-884; 0 before the first);
+ * center and coeffi as set (any pointer may be NULL). */
+int g6_mc_get_umbrella(g6_handle h, double *op, double *center, double *coeffi);
+/* beta = 1/temp for the following trials (an exchange's new temperature) */
+int g6_mc_set_temperature(g6_handle h, double temp);
 
 /* ---- replica ladders (MDReplica.F90, the replica branch of Mixture.F90)
  * The reference runs one process per replica (fakempi) and exchanges
@@ -336,6 +365,19 @@ int g6_rx_free(g6_rx hx);
  * beta[nrep] (1/K) indexed by node; beta is updated in place and
  * node_in_charge kept by the group.  naccept = accepted swaps. */
 int g6_rx_trials(g6_rx hx, const double *energy, double *beta, int32_t *naccept);
+/* The same trials weighted by umbrellas (Replica.F90:30-49), as mc2's
+ * replica exchange of MC nodes runs them (mc2.F90:973-996): op, center and
+ * coeffi hold numbrella entries per node, node k at (k-1)*numbrella.  A
+ * state's weight adds coeffi (op - center)^2 of each umbrella to the
+ * node's energy; the swapped state evaluates each node's order parameters
+ * under the other node's umbrellas; an accepted swap exchanges the two
+ * nodes' centers and coefficients (updated in place) with their betas.
+ * The arguments mean what Replica.F90 names them; mc2.F90:981 passes its
+ * beta array in the energy position and its energies in the beta position,
+ * and a caller reproducing mc2's exchanges passes them the same way. */
+int g6_rx_trials_umbrella(g6_rx hx, const double *energy, double *beta, int numbrella,
+                          const double *op, double *center, double *coeffi,
+                          int32_t *naccept);
 /* node_in_charge[nrep] (1-based, as the reference) */
 int g6_rx_nodes(g6_rx hx, int32_t *node_in_charge);
 /* One exchange of Mixture.F90:1570-1653 between the group's systems:

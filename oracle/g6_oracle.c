@@ -1159,9 +1159,15 @@ double g6o_mc_pairinfo(int model, int n, double L, double rc, double taper,
  * random_normal are taken in argument order.  com[n][3], quat[4][n],
  * delta[n], rtr[n], ep[n*n] (PairInfo), *epsum are updated in place;
  * returns the number of accepted trials. */
-long g6o_mc_trials(int model, int n, double L, double rc, double taper, double temp,
-                   double *com, double *quat, double *delta, double *rtr, double *ep,
-                   double *epsum, const double *u, long *pos, int ntrial) {
+typedef struct umbctx umbctx;
+static int umb_prepare(umbctx *U, int n, double L, double rc, const double *com, double epsum);
+static double umb_trial(umbctx *U, int target, double dx, double dy, double dz, double epsum,
+                        double deltae);
+static void umb_accept(umbctx *U);
+
+static long mc_trials(int model, int n, double L, double rc, double taper, double temp,
+                      double *com, double *quat, double *delta, double *rtr, double *ep,
+                      double *epsum, const double *u, long *pos, int ntrial, umbctx *U) {
   const double INCREMENT = (double)1.05f, I2J_ = 10.0, J2K_ = 0.120273, beta = 1.0 / temp;
   g6o_model m;
   g6o_model_get(model, &m);
@@ -1174,6 +1180,7 @@ long g6o_mc_trials(int model, int n, double L, double rc, double taper, double t
   stdterm st[G6O_MAXSITE * G6O_MAXSITE];
   const int nt = std_terms(&m, st);
   long p = *pos, nacc = 0;
+  if (U && umb_prepare(U, n, L, rc, com, *epsum)) ntrial = -1;
   for (int trial = 0; trial < ntrial; trial++) {
     int target = (int)(u[p++] * n + 1) - 1;
     if (target >= n) target = n - 1;
@@ -1233,15 +1240,19 @@ long g6o_mc_trials(int model, int n, double L, double rc, double taper, double t
     for (int i = 0; i < n; i++) deltae = deltae - ep[(long)i * n + target];
     for (int i = 0; i < n; i++)
       if (inl[i] && i != target) deltae = deltae + enew[i];
+    /* deltaesum = deltae + the umbrellas' change (mc2.F90:684-706) */
+    const double deltaesum = U ? umb_trial(U, target, deltax, deltay, deltaz, *epsum, deltae)
+                               : deltae;
     int accept = 0;
-    if (deltae < 0.0) {
+    if (deltaesum < 0.0) {
       accept = 1;
     } else {
-      const double ratio = exp(-(deltae * I2J_ * J2K_) * beta);
+      const double ratio = exp(-(deltaesum * I2J_ * J2K_) * beta);
       if (u[p++] < ratio) accept = 1;
     }
     if (accept) {
       nacc++;
+      if (U) umb_accept(U);
       for (int a = 0; a < 3; a++) com[3 * target + a] = ct[a];
       for (int a = 0; a < 4; a++) quat[a * n + target] = qt[a];
       for (int s = 0; s < ns; s++) {
@@ -1269,7 +1280,14 @@ long g6o_mc_trials(int model, int n, double L, double rc, double taper, double t
   }
   *pos = p;
   free(x); free(y); free(z); free(t); free(intra); free(enew); free(inl);
-  return nacc;
+  return ntrial < 0 ? -1 : nacc;
+}
+
+long g6o_mc_trials(int model, int n, double L, double rc, double taper, double temp,
+                   double *com, double *quat, double *delta, double *rtr, double *ep,
+                   double *epsum, const double *u, long *pos, int ntrial) {
+  return mc_trials(model, n, L, rc, taper, temp, com, quat, delta, rtr, ep, epsum, u, pos,
+                   ntrial, NULL);
 }
 
 /* ---- NPT Monte Carlo with collective moves ------------------------------
@@ -1927,4 +1945,213 @@ int g6o_triangle(int nsite, long npair, const int *pi, const int *pj, const doub
   tab_free(&t2);
   tab_free(&t4);
   return err;
+}
+
+/* ---- mc2's trial loop with umbrellas ------------------------------------
+ * Umbrella potentials coeffi (op - center)^2 (Umbrella.F90:11-16) on the
+ * order parameters of @GEUM (mc2util.F90:174-191): the triangle histogram's
+ * divergence (TRIANGLE, LOGTRIAN) and the energy (EPOP____).  Each call is
+ * one loop of mc2 (mc2.F90:407-899): the pair list of the loop (all pairs
+ * within rc, mc2.F90:486-496), prepareall (mc2.F90:509-521), then every
+ * trial adds sum(U(newop) - U(op)) to deltae (mc2.F90:684-706) and an
+ * accepted trial integrates the tables (mc2.F90:754-771). */
+struct umbctx {
+  int numb;
+  const int *type;
+  const double *center, *coeffi, *refhist;
+  double *op;
+  double newop[G6O_MAXUMB];
+  /* sTriangle per umbrella: nei2 (5 A shell by distance), nei (nearest 4),
+   * hist; the trial's modified arrays and histogram */
+  g6o_nbtab nei2[G6O_MAXUMB], nei[G6O_MAXUMB], new2, new4;
+  double hist[G6O_MAXUMB][21], newhist[G6O_MAXUMB][21];
+  int nmod, idx[G6O_NEAR + 1], n;
+  char *inset;
+};
+
+static void hist_add(double *h, const g6o_nbtab *t, int row, double amount) {
+  const long e = (long)row * G6O_NEAR;
+  for (int j = 0; j < t->nnear[row]; j++)
+    for (int k = j + 1; k < t->nnear[row]; k++) {
+      double c = t->dx[e + j] * t->dx[e + k] + t->dy[e + j] * t->dy[e + k] +
+                 t->dz[e + j] * t->dz[e + k];
+      c = c / sqrt(t->dd[e + j] * t->dd[e + k]);
+      const int bin = (int)round((c - (-1.0)) / 0.1) + 1;
+      if (1 <= bin && bin <= 21) h[bin - 1] = h[bin - 1] + amount;
+    }
+}
+
+static int umb_tri(const umbctx *U, int b) { return U->type[b] == 1 || U->type[b] == 4; }
+
+/* prepareall: sTriangle_prepareall (Triangle.F90:53-116) on the loop's pair
+ * list; op = epsum for EPOP____ (mc2.F90:519-520) */
+static int umb_prepare(umbctx *U, int n, double L, double rc, const double *com, double epsum) {
+  int err = 0;
+  long np = 0;
+  int *pi = malloc(sizeof(int) * (size_t)n * (n - 1) / 2 + 4),
+      *pj = malloc(sizeof(int) * (size_t)n * (n - 1) / 2 + 4);
+  double *o = malloc(sizeof(double) * 3 * ((size_t)n * (n - 1) / 2 + 1));
+  /* the all-pairs list compressed to rc (Interaction_Compress), i < j */
+  for (int i = 0; i < n; i++)
+    for (int j = i + 1; j < n; j++) {
+      double d[3], s[3];
+      for (int a = 0; a < 3; a++) {
+        d[a] = com[3 * i + a] - com[3 * j + a];
+        s[a] = round(d[a] * (1.0 / L)) * L;
+        d[a] = d[a] - s[a];
+      }
+      if (!(d[0] * d[0] + d[1] * d[1] + d[2] * d[2] < rc * rc)) continue;
+      pi[np] = i;
+      pj[np] = j;
+      o[3 * np] = s[0];
+      o[3 * np + 1] = s[1];
+      o[3 * np + 2] = s[2];
+      np++;
+    }
+  double *ox = malloc(sizeof(double) * (np + 1)), *oy = malloc(sizeof(double) * (np + 1)),
+         *oz = malloc(sizeof(double) * (np + 1));
+  for (long k = 0; k < np; k++) {
+    ox[k] = o[3 * k];
+    oy[k] = o[3 * k + 1];
+    oz[k] = o[3 * k + 2];
+  }
+  U->n = n;
+  for (int b = 0; b < U->numb; b++) {
+    if (U->type[b] == 3) {
+      U->op[b] = epsum;
+    } else if (umb_tri(U, b)) {
+      err |= g6o_neighbors_all2(n, np, pi, pj, ox, oy, oz, com, 0, 1, 5.0, U->nei2[b]);
+      err |= g6o_neighbors_compress(n, U->nei2[b], 4, 1, 0.0, U->nei[b]);
+      for (int k = 0; k < 21; k++) U->hist[b][k] = 0.0;
+      for (int i = 0; i < n; i++) hist_add(U->hist[b], &U->nei[b], i, 1.0);
+      U->op[b] = kullback_div(21, U->hist[b], U->refhist);
+      if (U->type[b] == 4) U->op[b] = log(U->op[b]);
+    } else {
+      err = 1;
+    }
+  }
+  free(pi); free(pj); free(o); free(ox); free(oy); free(oz);
+  return err;
+}
+
+static int row_has(const g6o_nbtab *t, int row, int label) {
+  for (int k = 0; k < t->nnear[row]; k++)
+    if (t->near[(long)row * G6O_NEAR + k] == label) return 1;
+  return 0;
+}
+
+/* sTriangle_Differentiate (Triangle.F90:125-249): the modified arrays, the
+ * set of molecules whose angles may change, the histogram with the set's
+ * old angles taken out (partial_accumulate) and the new ones put in
+ * (partial_accumulate_indexed, DistribModule.F90:231-300) */
+static double tri_differentiate(umbctx *U, int b, int target, double dx, double dy, double dz) {
+  g6o_nbtab *n2 = &U->nei2[b], *n4 = &U->nei[b];
+  g6o_neighbors_displacement(*n2, 1, target, dx, dy, dz, &U->nmod, U->idx, U->new2);
+  g6o_neighbors_compress(U->nmod, U->new2, 4, 1, 0.0, U->new4);
+  int list[2 * G6O_NEAR + 8], ns = 0;
+#define JOIN(v)                \
+  do {                         \
+    const int v_ = (v);        \
+    if (!U->inset[v_]) {       \
+      U->inset[v_] = 1;        \
+      list[ns++] = v_;         \
+    }                          \
+  } while (0)
+  JOIN(target);
+  for (int k = 0; k < n4->nnear[target]; k++) JOIN(n4->near[(long)target * G6O_NEAR + k]);
+  for (int k = 0; k < n2->nnear[target]; k++) {
+    const int i = n2->near[(long)target * G6O_NEAR + k];
+    if (row_has(n4, i, target)) JOIN(i);
+  }
+  for (int k = 0; k < U->new4.nnear[0]; k++) JOIN(U->new4.near[k]);
+  for (int r = 1; r < U->nmod; r++)
+    if (row_has(&U->new4, r, target)) JOIN(U->idx[r]);
+#undef JOIN
+  memcpy(U->newhist[b], U->hist[b], sizeof(double) * 21);
+  for (int k = 0; k < ns; k++) hist_add(U->newhist[b], n4, list[k], -1.0);
+  for (int r = 0; r < U->nmod; r++)
+    if (U->inset[U->idx[r]]) hist_add(U->newhist[b], &U->new4, r, 1.0);
+  for (int k = 0; k < ns; k++) U->inset[list[k]] = 0;
+  double op = kullback_div(21, U->newhist[b], U->refhist);
+  if (U->type[b] == 4) op = log(op);
+  return op;
+}
+
+static void copy_row(g6o_nbtab *dst, int drow, const g6o_nbtab *src, int srow) {
+  const long d = (long)drow * G6O_NEAR, s = (long)srow * G6O_NEAR;
+  const int m = src->nnear[srow];
+  dst->nnear[drow] = m;
+  memcpy(dst->near + d, src->near + s, sizeof(int) * m);
+  memcpy(dst->dx + d, src->dx + s, sizeof(double) * m);
+  memcpy(dst->dy + d, src->dy + s, sizeof(double) * m);
+  memcpy(dst->dz + d, src->dz + s, sizeof(double) * m);
+  memcpy(dst->dd + d, src->dd + s, sizeof(double) * m);
+}
+
+static double umbrella_potential(double coeffi, double center, double op) {
+  return coeffi * ((op - center) * (op - center));
+}
+
+/* every umbrella's newop, then deltaesum = deltae + deltaope
+ * (mc2.F90:684-706); the triangle umbrellas keep their modified arrays
+ * until the decision.  Umbrellas of one kind on one configuration evolve
+ * alike, but each keeps its own tables as the reference's sTriangle does;
+ * the arrays of the trial are those of the last triangle umbrella, which
+ * are the same for all of them. */
+static double umb_trial(umbctx *U, int target, double dx, double dy, double dz, double epsum,
+                        double deltae) {
+  for (int b = 0; b < U->numb; b++)
+    U->newop[b] = U->type[b] == 3 ? epsum + deltae : tri_differentiate(U, b, target, dx, dy, dz);
+  double deltaope = 0.0;
+  for (int b = 0; b < U->numb; b++)
+    deltaope = deltaope + umbrella_potential(U->coeffi[b], U->center[b], U->newop[b]) -
+               umbrella_potential(U->coeffi[b], U->center[b], U->op[b]);
+  return deltae + deltaope;
+}
+
+/* sTriangle_Integrate (Triangle.F90:251-280) and op = newop */
+static void umb_accept(umbctx *U) {
+  for (int b = 0; b < U->numb; b++) {
+    if (umb_tri(U, b)) {
+      for (int r = 0; r < U->nmod; r++) {
+        copy_row(&U->nei2[b], U->idx[r], &U->new2, r);
+        copy_row(&U->nei[b], U->idx[r], &U->new4, r);
+      }
+      memcpy(U->hist[b], U->newhist[b], sizeof(double) * 21);
+    }
+    U->op[b] = U->newop[b];
+  }
+}
+
+long g6o_mc_trials_umbrella(int model, int n, double L, double rc, double taper, double temp,
+                            double *com, double *quat, double *delta, double *rtr, double *ep,
+                            double *epsum, const double *u, long *pos, int ntrial, int numb,
+                            const int *type, const double *center, const double *coeffi,
+                            const double *refhist, double *op) {
+  if (numb < 0 || numb > G6O_MAXUMB) return -1;
+  umbctx U;
+  memset(&U, 0, sizeof(U));
+  U.numb = numb;
+  U.type = type;
+  U.center = center;
+  U.coeffi = coeffi;
+  U.refhist = refhist;
+  U.op = op;
+  int ok = tab_alloc(&U.new2, G6O_NEAR + 1) && tab_alloc(&U.new4, G6O_NEAR + 1);
+  for (int b = 0; b < numb; b++)
+    if (umb_tri(&U, b)) ok = ok && tab_alloc(&U.nei2[b], n) && tab_alloc(&U.nei[b], n);
+  U.inset = calloc(n, 1);
+  long acc = -1;
+  if (ok && U.inset)
+    acc = mc_trials(model, n, L, rc, taper, temp, com, quat, delta, rtr, ep, epsum, u, pos,
+                    ntrial, &U);
+  for (int b = 0; b < numb; b++)
+    if (U.nei2[b].nnear) {
+      tab_free(&U.nei2[b]);
+      tab_free(&U.nei[b]);
+    }
+  tab_free(&U.new2);
+  tab_free(&U.new4);
+  free(U.inset);
+  return acc;
 }

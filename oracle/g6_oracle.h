@@ -154,6 +154,20 @@ long g6o_mc_trials(int model, int n, double L, double rc, double taper, double t
                    double *com, double *quat, double *delta, double *rtr, double *ep,
                    double *epsum, const double *u, long *pos, int ntrial);
 
+/* The trial loop with umbrella potentials coeffi (op - center)^2
+ * (Umbrella.F90:11-16) added to every trial's energy change (mc2.F90:684-706):
+ * type 1 TRIANGLE, 4 LOGTRIAN (sTriangle, Triangle.F90:53-280, on refhist[21]),
+ * 3 EPOP____ (op = the energy).  One call is one mc2 loop: the order
+ * parameters are prepared on the loop's pair list first (mc2.F90:486-521).
+ * op[numb] returns each order parameter after the last trial.  Returns the
+ * accepted count, or -1 (an unknown type, or a neighbour array overflow). */
+#define G6O_MAXUMB 10
+long g6o_mc_trials_umbrella(int model, int n, double L, double rc, double taper, double temp,
+                            double *com, double *quat, double *delta, double *rtr, double *ep,
+                            double *epsum, const double *u, long *pos, int ntrial, int numb,
+                            const int *type, const double *center, const double *coeffi,
+                            const double *refhist, double *op);
+
 /* NPT Monte Carlo with collective moves (no reference counterpart; the
  * device path's definition, restated serially) */
 void g6o_philox10(uint32_t *c, uint32_t k0, uint32_t k1);

@@ -208,6 +208,11 @@ def _declare(L):
                                 C.c_double, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                 C.POINTER(C.c_long), C.c_int]
     L.g6o_mc_trials.restype = C.c_long
+    L.g6o_mc_trials_umbrella.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                         C.c_double, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                         C.POINTER(C.c_long), C.c_int, C.c_int, _ip, _dp, _dp,
+                                         _dp, _dp]
+    L.g6o_mc_trials_umbrella.restype = C.c_long
     L.g6o_philox10.argtypes = [C.POINTER(C.c_uint32), C.c_uint32, C.c_uint32]
     L.g6o_npt_run.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                               C.c_double, C.c_double, C.c_uint64, C.c_double, C.c_double,
@@ -451,12 +456,32 @@ class MCRun:
         if len(self.stream) < count:
             self.stream = getnext_stream(self.seed, max(count, 2 * len(self.stream)))
 
+    # umbrella kinds of @GEUM (mc2util.F90:174-191)
+    UMB = {"TRIANGLE": 1, "EPOP____": 3, "LOGTRIAN": 4}
+
+    def set_umbrella(self, kinds, center, coeffi, refhist=None):
+        """Umbrellas coeffi (op - center)^2 on the order parameters named by
+        kinds (TRIANGLE, LOGTRIAN, EPOP____); refhist[21] for the triangle
+        kinds.  Each trials() call is then one mc2 loop."""
+        self.umb = np.array([self.UMB[k] for k in kinds], np.int32)
+        self.center = np.array(center, np.float64)
+        self.coeffi = np.array(coeffi, np.float64)
+        self.refhist = np.zeros(21) if refhist is None else np.array(refhist, np.float64)
+        self.op = np.zeros(len(kinds))
+
     def trials(self, ntrial):
         self._ensure(self.pos + 14 * ntrial + 2)
         p = C.c_long(self.pos)
-        acc = lib().g6o_mc_trials(self.mid, self.n, self.L, self.rc, self.taper, self.temp,
-                                  self.com, self.quat, self.delta, self.rtr, self.ep,
-                                  self.epsum, self.stream, C.byref(p), ntrial)
+        if getattr(self, "umb", None) is not None and len(self.umb):
+            acc = lib().g6o_mc_trials_umbrella(
+                self.mid, self.n, self.L, self.rc, self.taper, self.temp, self.com, self.quat,
+                self.delta, self.rtr, self.ep, self.epsum, self.stream, C.byref(p), ntrial,
+                len(self.umb), self.umb, self.center, self.coeffi, self.refhist, self.op)
+            assert acc >= 0, "umbrella setup failed"
+        else:
+            acc = lib().g6o_mc_trials(self.mid, self.n, self.L, self.rc, self.taper, self.temp,
+                                      self.com, self.quat, self.delta, self.rtr, self.ep,
+                                      self.epsum, self.stream, C.byref(p), ntrial)
         self.pos = p.value
         self.nacc += acc
         return self.nacc
@@ -484,8 +509,17 @@ class RxRun:
         self.pos += 1
         return v
 
-    def trials(self, energy, beta):
+    def trials(self, energy, beta, op=None, center=None, coeffi=None):
+        """One exchange loop; with op, center, coeffi ((nrep, numbrella),
+        node order) the umbrella-weighted form (Replica.F90:30-49): returns
+        (beta, acc), or (beta, center, coeffi, acc) with umbrellas."""
         beta = [float(x) for x in beta]
+        umb = op is not None
+        if umb:
+            op = [list(map(float, r)) for r in op]
+            center = [list(map(float, r)) for r in center]
+            coeffi = [list(map(float, r)) for r in coeffi]
+        U = lambda k, c, o: k * ((o - c) * (o - c))  # Umbrella.F90:11-16
         nb, acc = len(self.a), 0
         for _ in range(self.nexchange):
             # k = random_getnext * nbond + 1 (Fortran truncation); a draw of
@@ -494,13 +528,27 @@ class RxRun:
             j, jj = self.a[k - 1], self.b[k - 1]
             n0, n1 = self.nic[j - 1], self.nic[jj - 1]
             a0, a1 = float(energy[n0 - 1]), float(energy[n1 - 1])
+            if umb:
+                for u in range(len(op[0])):
+                    a0 = a0 + U(coeffi[n0 - 1][u], center[n0 - 1][u], op[n0 - 1][u])
+                    a1 = a1 + U(coeffi[n1 - 1][u], center[n1 - 1][u], op[n1 - 1][u])
             acc0 = -(a0 * beta[n0 - 1] + a1 * beta[n1 - 1]) * self.I2J * self.J2K
+            a0, a1 = float(energy[n0 - 1]), float(energy[n1 - 1])
+            if umb:
+                for u in range(len(op[0])):
+                    a0 = a0 + U(coeffi[n1 - 1][u], center[n1 - 1][u], op[n0 - 1][u])
+                    a1 = a1 + U(coeffi[n0 - 1][u], center[n0 - 1][u], op[n1 - 1][u])
             acc1 = -(a0 * beta[n1 - 1] + a1 * beta[n0 - 1]) * self.I2J * self.J2K
             accept = acc0 < acc1 or self._getnext() < math.exp(acc1 - acc0)
             if accept:
                 beta[n0 - 1], beta[n1 - 1] = beta[n1 - 1], beta[n0 - 1]
+                if umb:
+                    center[n0 - 1], center[n1 - 1] = center[n1 - 1], center[n0 - 1]
+                    coeffi[n0 - 1], coeffi[n1 - 1] = coeffi[n1 - 1], coeffi[n0 - 1]
                 self.nic[j - 1], self.nic[jj - 1] = n1, n0
                 acc += 1
+        if umb:
+            return np.array(beta), np.array(center), np.array(coeffi), acc
         return np.array(beta), acc
 
 

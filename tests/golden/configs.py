@@ -128,16 +128,31 @@ def tag_decks(g):
     }
 
 
-def mc2_files(com, q, L, rc, taper, seed, temp, sweeps):
+def mc2_files(com, q, L, rc, taper, seed, temp, sweeps, umbrellas=(), refhist=None):
     """The input of the reference's mc2 for one node (mc2util.F90:119-300):
     (config file text, node input text).  @MCLP = @INNR = @SNPO = sweeps,
-    @NLOG 1, no grid and no bookkeeping (mc2 then uses the all-pairs list)."""
+    @NLOG 1, no grid and no bookkeeping (mc2 then uses the all-pairs list).
+    umbrellas: (kind, center, coeffi) per @GEUM (mc2util.F90:174-191), after
+    an @HIST of refhist[21] (bins of 0.1 from -1) when refhist is given."""
     n = len(com)
     deck = ["@SEED", str(seed), "@MCLP", str(sweeps), "@INNR", str(sweeps), "@NLOG", "1",
             "@SNPO", str(sweeps), "@TEMP", repr(temp), "@RCOA", "%r %r" % (rc, taper),
-            "@BXLA", repr(L), "@NCMP", "1", "@ID08", "TIP4P", "@NX4A", str(n)]
+            "@BXLA", repr(L)]
+    if refhist is not None:
+        deck += ["@HIST", "21 0.1 -1.0"] + ["%r" % float(v) for v in refhist]
+    for kind, center, coeffi in umbrellas:
+        deck += ["@GEUM", kind, "%r %r" % (float(center), float(coeffi))]
+    deck += ["@NCMP", "1", "@ID08", "TIP4P", "@NX4A", str(n)]
     deck += [" ".join("%.17e" % v for v in list(com[i]) + list(q[i])) for i in range(n)]
     return "@MPIN\n1\nnode0.in\n", "\n".join(deck) + "\n"
+
+
+def parse_mc2o_rows(text):
+    """The @MC2O lines of an mc2 output file: loop, energy, 1/beta, then
+    (op, center, coeffi) per umbrella, as float rows."""
+    lines = text.split("\n")
+    return np.array([[float(v) for v in lines[i + 1].split()] for i, l in enumerate(lines)
+                     if l.strip() == "@MC2O"])
 
 
 def parse_mc2_out(text, n):
