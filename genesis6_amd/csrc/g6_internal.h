@@ -164,6 +164,25 @@ struct McUmb {
   double center[kUmbMax] = {0, 0, 0, 0}, coeffi[kUmbMax] = {0, 0, 0, 0};
   double refhist[21] = {};
   int tri = 0;  // any triangle kind
+  int q6 = 0;   // any Q6LOCAL_ (kind 2)
+};
+
+// The Q6LOCAL_ umbrella's state (the reference's sq6system for one loop):
+// rows within 6 A by label (dx, dy, dz, dd per entry; the entries within 3
+// and 5 A are nei3 and nei5), the ordered sets neis(i) and neib(jj, i) of
+// every bond, qlm of every bond, qlmsum and nbond of every molecule, each
+// molecule's bond count, q6 of the trial, and the order parameter prepared
+// at the loop's start.
+constexpr int kQ6Near = 64;  // entries of a 6 A row
+constexpr int kQ6Set = 128;  // elements of a set
+constexpr int kQ6Rec = 32;   // doubles of a qlm record (26 used)
+struct Q6Buf {
+  int *nn6 = nullptr, *lab6 = nullptr;
+  double *v6 = nullptr;
+  int *neis_n = nullptr, *neis = nullptr;
+  int *neib_n = nullptr, *neib = nullptr;
+  double *qlm = nullptr, *qsum = nullptr, *dq6 = nullptr, *op = nullptr;
+  int *nbond = nullptr, *nb3 = nullptr;
 };
 
 struct McBuf {
@@ -177,6 +196,7 @@ struct McBuf {
   // parameters after the last call, the capacity flag of the tables
   int *tri_nn = nullptr, *tri_near = nullptr, *tri_hist = nullptr, *uflag = nullptr;
   double *tri_v = nullptr, *uop = nullptr;
+  Q6Buf q6;
 };
 
 // Host side of the MC random stream: the reference's MT19937 (mtrnd.f) and

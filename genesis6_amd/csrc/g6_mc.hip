@@ -22,6 +22,7 @@
 #include <vector>
 
 #include "g6_internal.h"
+#include "g6_mc_q6.h"
 #include "g6_std.h"
 
 #pragma clang fp contract(off)
@@ -231,6 +232,129 @@ __global__ __launch_bounds__(256) void k_mc_umb_prepare(McArgs a) {
   tri_row_add(a.b.tri_hist, v, k, 1);
 }
 
+// sq6system_prepareall (Q6Local.F90:101-231) for molecule i: its row of the
+// molecules within 6 A (the loop's pairs within rc, neighbors_all2 radius 6,
+// by label), neis(i) = those within 5 A then i, the qlm of its bonds (within
+// 3 A) and its bond count.  Flag 2: a row past 64 entries or more than
+// MAXBOND bonds.
+__global__ __launch_bounds__(256) void k_q6u_rows(McArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int n = a.n;
+  const Q6Buf &q = a.b.q6;
+  const double xi = a.b.com[i], yi = a.b.com[n + i], zi = a.b.com[2 * n + i];
+  const double rr = a.rc * a.rc;
+  double4 *row = reinterpret_cast<double4 *>(q.v6) + (size_t)i * kQ6Near;
+  int cnt = 0;
+  for (int j = 0; j < n; j++) {
+    if (j == i) continue;
+    double d0 = a.b.com[j] - xi, d1 = a.b.com[n + j] - yi, d2 = a.b.com[2 * n + j] - zi;
+    d0 = d0 - round(d0 * a.invL) * a.L;
+    d1 = d1 - round(d1 * a.invL) * a.L;
+    d2 = d2 - round(d2 * a.invL) * a.L;
+    const double dd = d0 * d0 + d1 * d1 + d2 * d2;
+    if (!(dd < rr) || 36.0 < dd) continue;
+    if (cnt < kQ6Near) {
+      q.lab6[(size_t)i * kQ6Near + cnt] = j;
+      row[cnt] = make_double4(d0, d1, d2, dd);
+    }
+    cnt++;
+  }
+  const int m = cnt < kQ6Near ? cnt : kQ6Near;
+  q.nn6[i] = m;
+  int ns = 0, nb = 0;
+  int *neis = q.neis + (size_t)i * kQ6Set;
+  for (int e = 0; e < m; e++) {
+    const double4 v = row[e];
+    if (q6_is_shell(v.w)) neis[ns++] = q.lab6[(size_t)i * kQ6Near + e];
+    if (q6_is_bond(v.w)) {
+      if (nb < kMaxBond) q6_record(v, q6_rec(q, i, nb));
+      nb++;
+    }
+  }
+  neis[ns++] = i;
+  q.neis_n[i] = ns;
+  q.nb3[i] = nb;
+  if (cnt > kQ6Near || nb > kMaxBond) atomicCAS(a.b.uflag, 0, 2);
+}
+
+// neib(jj, i) = neis(i) & neis(j), in neis(i)'s order, for bond jj of i
+__global__ __launch_bounds__(256) void k_q6u_neib(McArgs a) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = t / kMaxBond, jj = t % kMaxBond;
+  if (i >= a.n) return;
+  const Q6Buf &q = a.b.q6;
+  if (jj >= q.nb3[i] || q.nb3[i] > kMaxBond) return;
+  const double4 *row = reinterpret_cast<const double4 *>(q.v6) + (size_t)i * kQ6Near;
+  int j = -1, b = 0;
+  for (int e = 0; e < q.nn6[i]; e++)
+    if (q6_is_bond(row[e].w)) {
+      if (b == jj) {
+        j = q.lab6[(size_t)i * kQ6Near + e];
+        break;
+      }
+      b++;
+    }
+  if (j < 0) return;
+  const int *si = q.neis + (size_t)i * kQ6Set, *sj = q.neis + (size_t)j * kQ6Set;
+  const int ni = q.neis_n[i], nj = q.neis_n[j];
+  int *out = q6_neib(q, i, jj), c = 0;
+  for (int p = 0; p < ni; p++) {
+    bool in = false;
+    for (int r = 0; r < nj; r++) in = in || sj[r] == si[p];
+    if (in) out[c++] = si[p];
+  }
+  *q6_neib_n(q, i, jj) = c;
+}
+
+// qlmsum(k) and nbond(k): the bonds (i, jj) whose set holds k, added in the
+// reference's order (i ascending, then jj); such an i is k or a molecule of
+// k's 5 A shell.  One wave per molecule, lane c < 26 one component; q6(k).
+__global__ __launch_bounds__(64) void k_q6u_sum(McArgs a) {
+  const int k = blockIdx.x, lane = threadIdx.x;
+  const Q6Buf &q = a.b.q6;
+  const WRow rk = wr_load(q, k, lane);
+  const unsigned long long sh = __ballot(lane < rk.n && q6_is_shell(rk.v.w));
+  const int ns = __popcll(sh);
+  double acc = 0.0;
+  int nb = 0;
+  bool kdone = false;
+  for (int p = 0; p <= ns; p++) {
+    int i;
+    const int lab = p < ns ? __shfl(rk.lab, q6_select_bit(sh, p), 64) : 0x7fffffff;
+    if (!kdone && k < lab) {
+      i = k;
+      kdone = true;
+      p--;
+    } else if (p < ns) {
+      i = lab;
+    } else {
+      break;
+    }
+    const int nbi = q.nb3[i] < kMaxBond ? q.nb3[i] : kMaxBond;
+    for (int jj = 0; jj < nbi; jj++) {
+      const WSet s = ws_load(q6_neib(q, i, jj), q6_neib_n(q, i, jj), lane);
+      if (!ws_has(s, k, lane)) continue;
+      nb++;
+      if (lane < 26) acc = acc + q6_rec(q, i, jj)[lane];
+    }
+  }
+  if (lane < 26) q.qsum[(size_t)k * kQ6Rec + lane] = acc;
+  if (lane == 0) q.nbond[k] = nb;
+}
+
+// q6 of every molecule and the order parameter sum / n
+__global__ __launch_bounds__(256) void k_q6u_q6(McArgs a) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < a.n) a.b.q6.dq6[k] = q6_of(a.b.q6, k);
+}
+__global__ void k_q6u_total(McArgs a) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  double s = 0.0;
+  for (int k = 0; k < a.n; k++) s = s + a.b.q6.dq6[k];
+  *a.b.q6.op = s / a.n;
+}
+
 // kullback_div (DistribModule.F90:304-326) of h (counts) from ref
 __device__ inline double kl_div(const int *h, const double *ref) {
   double s1 = 0.0, s2 = 0.0, div = 0.0;
@@ -258,7 +382,7 @@ __device__ inline double umbrella_potential(double coeffi, double center, double
 // on acceptance, the PairInfo and configuration write-back.
 constexpr int kMcChunk = 192;                 // trials per staged chunk
 constexpr int kMcStage = kMcChunk * 14 + 2;   // stream entries per chunk
-template <int M>
+template <int M, bool UQ6>
 __global__ __launch_bounds__(kMcThreads) void k_mc_trials(McArgs a, StdTable st, int ntrial) {
   constexpr int NS = StdPattern<M>::NS;
   extern __shared__ __align__(16) double s_mem[];
@@ -272,7 +396,7 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_trials(McArgs a, StdTable st,
   __shared__ double s_red[kMcThreads / 64];
   // umbrella state: histogram, the trial's change, reference, order
   // parameters now and after the trial, the trial's total energy change
-  __shared__ int s_hist[21], s_dh[21];
+  __shared__ int s_hist[21], s_dh[21], s_abort;
   __shared__ double s_ref[21], s_op[kUmbMax], s_newop[kUmbMax], s_des;
   const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
   const int numb = a.u.numb;
@@ -293,13 +417,17 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_trials(McArgs a, StdTable st,
       s_dh[tid] = 0;
       s_ref[tid] = a.u.refhist[tid];
     }
+    if (tid == 0) s_abort = 0;
     __syncthreads();
     if (tid == 0) {  // prepareall's op (mc2.F90:509-521)
       const double div = a.u.tri ? kl_div(s_hist, s_ref) : 0.0;
-      for (int k = 0; k < numb; k++)
-        s_op[k] = a.u.kind[k] == 3 ? epsum : (a.u.kind[k] == 4 ? log(div) : div);
+      for (int k = 0; k < numb; k++) {
+        const int kind = a.u.kind[k];
+        s_op[k] = kind == 3 ? epsum : kind == 4 ? log(div) : kind == 2 ? *a.b.q6.op : div;
+      }
     }
   }
+  bool aborted = false;
   for (int trial = 0; trial < ntrial; trial++) {
     if (trial % kMcChunk == 0) {  // stage the next chunk (a trial reads <= 14)
       __syncthreads();
@@ -311,6 +439,7 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_trials(McArgs a, StdTable st,
       __syncthreads();
     }
     const int p = pos - base;
+    const int trial_pos = pos;
     // target = random_getnext*nmol + 1 (1-based); u = 1 exactly would name
     // molecule n+1, which the reference does not guard: clamped here
     int target = (int)(l_u[p] * (double)n + 1.0) - 1;
@@ -435,7 +564,35 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_trials(McArgs a, StdTable st,
         tri_row_add(s_dh, v, k, 1);
       }
     }
+    // the Q6 umbrella's Differentiate on wave 0: the target's row before
+    // and after the move; a move that would give a molecule more than
+    // MAXBOND bonds stops the call before it changes anything
+    [[maybe_unused]] Q6Trial qt;
+    if constexpr (UQ6) {
+      if (wid == 0) {
+        const Q6Buf &q = a.b.q6;
+        qt.old_t = wr_load(q, target, lane);
+        qt.old_b = wr_bonds(qt.old_t, lane);
+        qt.new_t = qt.old_t;
+        if (lane < qt.new_t.n) qt.new_t.v = tri_moved(qt.old_t.v, -deltax, -deltay, -deltaz);
+        qt.new_b = wr_bonds(qt.new_t, lane);
+        if (!q6_bonds_fit(q, qt, lane)) {
+          if (lane == 0) s_abort = 1;
+        } else {
+          q6_differentiate(q, qt, target, lane);
+        }
+      }
+    }
     __syncthreads();
+    if constexpr (UQ6) {
+      if (s_abort) {  // uniform: every thread leaves before the decision
+        pos = trial_pos;
+        aborted = true;
+        break;
+      }
+      for (int k = tid; k < n; k += kMcThreads) a.b.q6.dq6[k] = q6_of(a.b.q6, k);
+      __syncthreads();
+    }
     double de = 0.0;
 #pragma unroll
     for (int w = 0; w < kMcThreads / 64; w++) de = de + s_red[w];
@@ -450,10 +607,19 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_trials(McArgs a, StdTable st,
           for (int b = 0; b < 21; b++) h[b] = s_hist[b] + s_dh[b];
           div = kl_div(h, s_ref);
         }
+        double q6op = 0.0;
+        if constexpr (UQ6) {  // sumq6 / nmol (Q6Local.F90:429-440)
+          double sq = 0.0;
+          for (int k = 0; k < n; k++) sq = sq + a.b.q6.dq6[k];
+          q6op = sq / n;
+        }
         double dope = 0.0;
         for (int k = 0; k < numb; k++) {
-          const double nop =
-              a.u.kind[k] == 3 ? epsum + deltae : (a.u.kind[k] == 4 ? log(div) : div);
+          const int kind = a.u.kind[k];
+          const double nop = kind == 3 ? epsum + deltae
+                             : kind == 4 ? log(div)
+                             : kind == 2 ? q6op
+                                         : div;
           s_newop[k] = nop;
           dope = dope + umbrella_potential(a.u.coeffi[k], a.u.center[k], nop) -
                  umbrella_potential(a.u.coeffi[k], a.u.center[k], s_op[k]);
@@ -489,6 +655,15 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_trials(McArgs a, StdTable st,
         }
       }
       if (tid < 21) s_dh[tid] = 0;
+    }
+    if constexpr (UQ6) {  // sq6system_Integrate or _Reject
+      if (wid == 0) {
+        int ovf = 0;
+        if (accept)
+          q6_integrate(a.b.q6, qt, target, deltax, deltay, deltaz, lane, ovf);
+        else
+          q6_reject(a.b.q6, qt, target, lane);
+      }
     }
     if (accept) {
       nacc += 1;
@@ -539,6 +714,7 @@ __global__ __launch_bounds__(kMcThreads) void k_mc_trials(McArgs a, StdTable st,
     *a.b.nacc = *a.b.nacc + nacc;
     *a.b.epsum = epsum;
     for (int k = 0; k < numb; k++) a.b.uop[k] = s_op[k];
+    if (aborted) *a.b.uflag = 3;
   }
 }
 
@@ -567,9 +743,11 @@ static McArgs mc_args(const System &s) {
 
 void mc_free(System &s) {
   McBuf &m = s.mc;
+  Q6Buf &q = m.q6;
   void *ps[] = {m.com, m.quat, m.sx, m.sy, m.sz, m.pinfo, m.delta, m.rtr, m.u, m.z,
                 m.epsum, m.part, m.pos, m.nacc, m.tri_nn, m.tri_near, m.tri_hist,
-                m.uflag, m.tri_v, m.uop};
+                m.uflag, m.tri_v, m.uop, q.nn6, q.lab6, q.v6, q.neis_n, q.neis, q.neib_n,
+                q.neib, q.qlm, q.qsum, q.dq6, q.op, q.nbond, q.nb3};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   m = McBuf{};
@@ -696,14 +874,29 @@ int g6_mc_trials(g6_handle h, int ntrial, long long *naccept) {
       k_mc_umb_prepare<<<(s->n + 255) / 256, 256, 0, s->stream>>>(a);
       G6_CHECK(hipGetLastError());
     }
+    if (s->mc_umb.q6) {
+      const int n = s->n;
+      k_q6u_rows<<<(n + 255) / 256, 256, 0, s->stream>>>(a);
+      k_q6u_neib<<<(n * kMaxBond + 255) / 256, 256, 0, s->stream>>>(a);
+      k_q6u_sum<<<n, 64, 0, s->stream>>>(a);
+      k_q6u_q6<<<(n + 255) / 256, 256, 0, s->stream>>>(a);
+      k_q6u_total<<<1, 1, 0, s->stream>>>(a);
+      G6_CHECK(hipGetLastError());
+    }
   }
   const size_t lds = mc_lds(s->n);
   hipError_t attr = hipSuccess;
+  auto launch = [&](auto kern) {
+    attr = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)lds);
+    if (attr == hipSuccess) kern<<<1, kMcThreads, lds, s->stream>>>(a, s->std, ntrial);
+  };
   std_model(s->model, [&](auto m) {
     constexpr int M = decltype(m)::value;
-    attr = hipFuncSetAttribute((const void *)k_mc_trials<M>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (attr == hipSuccess) k_mc_trials<M><<<1, kMcThreads, lds, s->stream>>>(a, s->std, ntrial);
+    if (s->mc_umb.q6)
+      launch(k_mc_trials<M, true>);
+    else
+      launch(k_mc_trials<M, false>);
   });
   G6_CHECK(attr);
   G6_CHECK(hipGetLastError());
@@ -714,11 +907,14 @@ int g6_mc_trials(g6_handle h, int ntrial, long long *naccept) {
   G6_CHECK(hipMemcpyAsync(&used, m.pos, sizeof(int), hipMemcpyDeviceToHost, s->stream));
   G6_CHECK(hipMemcpyAsync(&nacc, m.nacc, sizeof(long long), hipMemcpyDeviceToHost, s->stream));
   G6_CHECK(hipStreamSynchronize(s->stream));
-  if (uflag) return G6_ECAPACITY;  // nothing ran: the stream is kept whole
+  // flags 1, 2: the loop's tables did not fit and nothing ran (the stream is
+  // kept whole); 3: a move would have given a molecule more than MAXBOND
+  // bonds, and the trials before it stand
+  if (uflag == 1 || uflag == 2) return G6_ECAPACITY;
   if (used < 0 || (size_t)used > need) return G6_EHIP;
   hst->pend.erase(hst->pend.begin(), hst->pend.begin() + used);
   if (naccept) *naccept = nacc;
-  return 0;
+  return uflag ? G6_ECAPACITY : 0;
 }
 
 int g6_mc_set_umbrella(g6_handle h, int numbrella, const int32_t *kind, const double *center,
@@ -730,11 +926,13 @@ int g6_mc_set_umbrella(g6_handle h, int numbrella, const int32_t *kind, const do
   u.numb = numbrella;
   for (int k = 0; k < numbrella; k++) {
     const int t = kind[k];
-    if (t != G6_UMB_TRIANGLE && t != G6_UMB_EPOP && t != G6_UMB_LOGTRIAN) return G6_EARG;
+    if (t != G6_UMB_TRIANGLE && t != G6_UMB_EPOP && t != G6_UMB_LOGTRIAN && t != G6_UMB_Q6LOCAL)
+      return G6_EARG;
     u.kind[k] = t;
     u.center[k] = center[k];
     u.coeffi[k] = coeffi[k];
-    if (t != G6_UMB_EPOP) u.tri = 1;
+    if (t == G6_UMB_TRIANGLE || t == G6_UMB_LOGTRIAN) u.tri = 1;
+    if (t == G6_UMB_Q6LOCAL) u.q6 = 1;
   }
   if (u.tri) {
     if (!refhist) return G6_EARG;
@@ -746,6 +944,23 @@ int g6_mc_set_umbrella(g6_handle h, int numbrella, const int32_t *kind, const do
     G6_CHECK(hipMalloc(&m.uop, sizeof(double) * kUmbMax));
     G6_CHECK(hipMalloc(&m.uflag, sizeof(int)));
     G6_CHECK(hipMemset(m.uop, 0, sizeof(double) * kUmbMax));
+  }
+  if (u.q6 && !m.q6.v6) {
+    Q6Buf &q = m.q6;
+    const size_t nb = (size_t)n * kMaxBond;
+    G6_CHECK(hipMalloc(&q.nn6, sizeof(int) * n));
+    G6_CHECK(hipMalloc(&q.lab6, sizeof(int) * (size_t)n * kQ6Near));
+    G6_CHECK(hipMalloc(&q.v6, sizeof(double) * 4 * (size_t)n * kQ6Near));
+    G6_CHECK(hipMalloc(&q.neis_n, sizeof(int) * n));
+    G6_CHECK(hipMalloc(&q.neis, sizeof(int) * (size_t)n * kQ6Set));
+    G6_CHECK(hipMalloc(&q.neib_n, sizeof(int) * nb));
+    G6_CHECK(hipMalloc(&q.neib, sizeof(int) * nb * kQ6Set));
+    G6_CHECK(hipMalloc(&q.qlm, sizeof(double) * nb * kQ6Rec));
+    G6_CHECK(hipMalloc(&q.qsum, sizeof(double) * (size_t)n * kQ6Rec));
+    G6_CHECK(hipMalloc(&q.dq6, sizeof(double) * n));
+    G6_CHECK(hipMalloc(&q.op, sizeof(double)));
+    G6_CHECK(hipMalloc(&q.nbond, sizeof(int) * n));
+    G6_CHECK(hipMalloc(&q.nb3, sizeof(int) * n));
   }
   if (u.tri && !m.tri_v) {
     G6_CHECK(hipMalloc(&m.tri_nn, sizeof(int) * n));

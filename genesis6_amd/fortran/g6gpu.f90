@@ -13,7 +13,8 @@ module g6gpu
        G6_ENODEVICE = -4
   integer, parameter :: G6_THERMO_LEN = 32
   integer(c_int), parameter :: G6_LADDER_MAP_XCD = 0, G6_LADDER_MAP_FLAT = 1
-  integer(c_int), parameter :: G6_UMB_TRIANGLE = 1, G6_UMB_EPOP = 3, G6_UMB_LOGTRIAN = 4
+  integer(c_int), parameter :: G6_UMB_TRIANGLE = 1, G6_UMB_Q6LOCAL = 2, G6_UMB_EPOP = 3, &
+       G6_UMB_LOGTRIAN = 4
   integer(c_int), parameter :: G6_ORDER_NONE = 0, G6_ORDER_BY_DISTANCE = 1, &
        G6_ORDER_BY_LABEL = 2
   integer, parameter :: G6_NEAR = 151

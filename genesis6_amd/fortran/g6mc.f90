@@ -12,11 +12,11 @@
 ! @NX4A n, then n lines x y z a b c d | the umbrellas: @HIST (nbin 21,
 ! width, minimum, then the 21 counts of the reference histogram), @REPQ
 ! center coeffi (a TRIANGLE umbrella), @GEUM kind, then center coeffi
-! (kinds TRIANGLE, LOGTRIAN, EPOP____; at most 4).  Reading stops after the
-! @NX4A of the last component, as ReadParams stops.  Tags it does not know
-! are passed over, as ReadParams passes them over.  A Q6LOCAL_ umbrella, a
-! saved random stream (@MTRN) and more than one node stop the run: those
-! paths are not on the GPU.
+! (kinds TRIANGLE, LOGTRIAN, Q6LOCAL_, EPOP____; at most 4).  Reading stops
+! after the @NX4A of the last component, as ReadParams stops.  Tags it does
+! not know are passed over, as ReadParams passes them over.  A saved random
+! stream (@MTRN) and more than one node stop the run: those paths are not
+! on the GPU.
 !
 ! The run is mc2's main loop (mc2.F90:379-899): loop = 0, 1, ...; each loop
 ! is 2N single-molecule trials (g6_mc_trials, the decisions of mc2's trial
@@ -83,7 +83,8 @@ program g6mc
   call g6_check(g6_upload_state(h, com, quat, dx, w), 'g6_upload_state')
   call g6_check(g6_mc_begin(h, temp, seed, delta, rtratio), 'g6_mc_begin')
   if (numb > 0) then
-     if (any(ukind(1:numb) /= G6_UMB_EPOP) .and. .not. have_hist) then
+     if (any(ukind(1:numb) == G6_UMB_TRIANGLE .or. ukind(1:numb) == G6_UMB_LOGTRIAN) &
+          .and. .not. have_hist) then
         write(0, '(a)') 'g6mc: a triangle umbrella needs @HIST before it'
         error stop 2
      end if
@@ -216,6 +217,8 @@ contains
              call add_umbrella(G6_UMB_TRIANGLE, u)
           case ('LOGTRIAN')
              call add_umbrella(G6_UMB_LOGTRIAN, u)
+          case ('Q6LOCAL_')
+             call add_umbrella(G6_UMB_Q6LOCAL, u)
           case ('EPOP____')
              call add_umbrella(G6_UMB_EPOP, u)
           case default

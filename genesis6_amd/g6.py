@@ -278,12 +278,12 @@ class System:
         return ep.reshape(n, n), float(e[0])
 
     # umbrella kinds of @GEUM (mc2util.F90:174-191) on the device path
-    UMB = {"TRIANGLE": 1, "EPOP____": 3, "LOGTRIAN": 4}
+    UMB = {"TRIANGLE": 1, "Q6LOCAL_": 2, "EPOP____": 3, "LOGTRIAN": 4}
 
     def mc_set_umbrella(self, kinds, center, coeffi, refhist=None):
         """Umbrellas coeffi (op - center)^2 on the order parameters named by
-        kinds (TRIANGLE, LOGTRIAN, EPOP____); each mc_trials call is then
-        one mc2 loop."""
+        kinds (TRIANGLE, LOGTRIAN, Q6LOCAL_, EPOP____); each mc_trials call
+        is then one mc2 loop."""
         k = np.array([self.UMB[x] for x in kinds], np.int32)
         ref = _c(np.zeros(21) if refhist is None else refhist)
         check(lib().g6_mc_set_umbrella(self.h, len(k), k, _c(center), _c(coeffi), ref),

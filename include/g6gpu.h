@@ -288,9 +288,13 @@ int g6_mc_energy(g6_handle h, double *epsum);
  * (DistribModule.F90:304-326) from refhist[21] of the histogram of O-O-O
  * angle cosines (21 bins of 0.1 from -1) between each molecule's four
  * nearest neighbours among those within 5 A at the loop's start, or its
- * log (LOGTRIAN); the energy (EPOP____).  Q6LOCAL_ is not on this path
- * (G6_EARG). */
-enum { G6_UMB_TRIANGLE = 1, G6_UMB_EPOP = 3, G6_UMB_LOGTRIAN = 4 };
+ * log (LOGTRIAN); the local bond order Q6 (Q6LOCAL_, sq6system,
+ * Q6Local.F90:42-750): the mean over molecules of q6 from the bonds within
+ * 3 A inside each molecule's 5 A shell, its sums updated over the trials of
+ * a loop as the reference updates them (rejected trials included, so the
+ * value is the reference's to the last bits of its sums); the energy
+ * (EPOP____). */
+enum { G6_UMB_TRIANGLE = 1, G6_UMB_Q6LOCAL = 2, G6_UMB_EPOP = 3, G6_UMB_LOGTRIAN = 4 };
 /* numbrella (<= 4; 0 removes them) potentials coeffi (op - center)^2
  * (Umbrella.F90:11-16), refhist[21] for the triangle kinds (@HIST, read
  * as mc2util.F90:156-159).  With umbrellas set, every g6_mc_trials call is
@@ -298,9 +302,13 @@ enum { G6_UMB_TRIANGLE = 1, G6_UMB_EPOP = 3, G6_UMB_LOGTRIAN = 4 };
  * configuration (the 5 A tables from the pairs within rc, mc2.F90:486-521;
  * op = the energy for EPOP____), then each trial adds
  * sum(U(newop) - U(op)) to dE before the Metropolis test (mc2.F90:684-706)
- * and an accepted trial moves the tables (sTriangle_Integrate).  A
- * molecule with more than 64 molecules within 5 A stops the call before
- * any trial: G6_ECAPACITY, the state unchanged.  Requires g6_mc_begin. */
+ * and an accepted trial moves the tables (sTriangle_Integrate,
+ * sq6system_Integrate / _Reject).  A molecule with more than 64 molecules
+ * within 5 A (triangle) or 6 A (Q6), or with more than MAXBOND = 10 bonds,
+ * stops the call before any trial: G6_ECAPACITY, the state unchanged.  A
+ * trial that would give a molecule an eleventh bond (the reference's arrays
+ * end there) stops the call before that trial: G6_ECAPACITY, the trials
+ * before it kept.  Requires g6_mc_begin. */
 int g6_mc_set_umbrella(g6_handle h, int numbrella, const int32_t *kind, const double *center,
                        const double *coeffi, const double *refhist);
 /* op[numbrella]: each order parameter after the last g6_mc_trials call

@@ -1164,6 +1164,7 @@ static int umb_prepare(umbctx *U, int n, double L, double rc, const double *com,
 static double umb_trial(umbctx *U, int target, double dx, double dy, double dz, double epsum,
                         double deltae);
 static void umb_accept(umbctx *U);
+static void umb_reject(umbctx *U);
 
 static long mc_trials(int model, int n, double L, double rc, double taper, double temp,
                       double *com, double *quat, double *delta, double *rtr, double *ep,
@@ -1274,6 +1275,7 @@ static long mc_trials(int model, int n, double L, double rc, double taper, doubl
       else
         rtr[target] = rtr[target] / INCREMENT;
     } else {
+      if (U) umb_reject(U);
       delta[target] = delta[target] / INCREMENT;
       if (delta[target] < (double)0.05f) delta[target] = (double)0.05f;
     }
@@ -1947,6 +1949,325 @@ int g6o_triangle(int nsite, long npair, const int *pi, const int *pj, const doub
   return err;
 }
 
+/* ---- Q6 local bond order with its incremental trial update -------------
+ * sq6system (Q6Local.F90:42-61) and its trial steps: prepareall (101-231),
+ * Differentiate (236-465), Integrate (467-700), Reject (702-750), restated
+ * with the same order of every floating-point operation on qlmsum, so that
+ * a chain of trials, accepted or rejected, leaves the sums the reference
+ * leaves.  Sets (Set.F90) keep their element order: join appends, drop
+ * moves the last element into the hole, intersection and difference keep
+ * the first set's order. */
+#define Q6SET (G6O_NEAR + 1)
+#define Q6MAXBOND 10
+typedef struct {
+  int n;
+  g6o_nbtab nei6, nei3, nei5;
+  int *neis, *neis_n;       /* [n][Q6SET] */
+  int *neib, *neib_n;       /* [n][Q6MAXBOND][Q6SET] */
+  double *qlm;              /* [n][Q6MAXBOND][26] */
+  double *qlmsum;           /* [n][26] */
+  int *nbond;
+  double *q6, sumq6;
+  /* sQ6diff (Q6Local.F90:47-57) */
+  g6o_nbtab new6, new3, new5;
+  int nmod, idx[G6O_NEAR + 1];
+  int dneis[Q6SET], dneis_n, plus[Q6SET], plus_n, minus[Q6SET], minus_n;
+  int dneib[Q6MAXBOND][Q6SET], dneib_n[Q6MAXBOND];
+  double dqlm[Q6MAXBOND][26];
+  double *dq6, dsumq6;
+  int err; /* a trial gave a molecule more than MAXBOND bonds */
+} q6st;
+
+static int set_has(const int *a, int na, int x) {
+  for (int i = 0; i < na; i++)
+    if (a[i] == x) return 1;
+  return 0;
+}
+static void set_join(int *a, int *na, int x) {
+  if (!set_has(a, *na, x)) a[(*na)++] = x;
+}
+static void set_drop(int *a, int *na, int x) {
+  for (int i = 0; i < *na; i++)
+    if (a[i] == x) {
+      a[i] = a[*na - 1];
+      (*na)--;
+      return;
+    }
+}
+/* intersection (a & b) and difference (a - b), in a's order */
+static void set_and(const int *a, int na, const int *b, int nb, int *r, int *nr) {
+  *nr = 0;
+  for (int i = 0; i < na; i++)
+    if (set_has(b, nb, a[i])) r[(*nr)++] = a[i];
+}
+static void set_minus(const int *a, int na, const int *b, int nb, int *r, int *nr) {
+  *nr = 0;
+  for (int i = 0; i < na; i++)
+    if (!set_has(b, nb, a[i])) r[(*nr)++] = a[i];
+}
+static int row_lookup(const g6o_nbtab *t, int row, int label) { /* sneighbor1_lookup */
+  for (int k = 0; k < t->nnear[row]; k++)
+    if (t->near[(long)row * G6O_NEAR + k] == label) return k;
+  return -1;
+}
+#define NEIS(q, i) ((q)->neis + (size_t)(i) * Q6SET)
+#define NEIB(q, i, jj) ((q)->neib + ((size_t)(i) * Q6MAXBOND + (jj)) * Q6SET)
+#define NEIBN(q, i, jj) ((q)->neib_n[(size_t)(i) * Q6MAXBOND + (jj)])
+#define QLM(q, i, jj) ((q)->qlm + ((size_t)(i) * Q6MAXBOND + (jj)) * 26)
+#define QSUM(q, k) ((q)->qlmsum + (size_t)(k) * 26)
+
+static void q6_bond_qlm(const g6o_nbtab *t, int row, int jj, double *out) {
+  const long e = (long)row * G6O_NEAR + jj;
+  const double pp = 1.0 / sqrt(t->dd[e]);
+  const double px = t->dx[e] * pp, py = t->dy[e] * pp, pz = t->dz[e] * pp;
+  for (int m = -6; m <= 6; m++) g6o_q6_qlm(m, px, py, pz, &out[2 * (m + 6)], &out[2 * (m + 6) + 1]);
+}
+
+static double q6_of(const double *qs, int nbond) {
+  double nq = 0.0;
+  for (int m = 0; m < 13; m++) {
+    const double a = hypot(qs[2 * m], qs[2 * m + 1]);
+    nq = nq + a * a;
+  }
+  return sqrt(nq * 4.0 * PI_T / (2.0 * 6 + 1.0)) / nbond;
+}
+
+static int q6st_alloc(q6st *q, int n) {
+  memset(q, 0, sizeof(*q));
+  q->n = n;
+  int ok = tab_alloc(&q->nei6, n) && tab_alloc(&q->nei3, n) && tab_alloc(&q->nei5, n) &&
+           tab_alloc(&q->new6, G6O_NEAR + 1) && tab_alloc(&q->new3, G6O_NEAR + 1) &&
+           tab_alloc(&q->new5, G6O_NEAR + 1);
+  q->neis = malloc(sizeof(int) * (size_t)n * Q6SET);
+  q->neis_n = calloc(n, sizeof(int));
+  q->neib = malloc(sizeof(int) * (size_t)n * Q6MAXBOND * Q6SET);
+  q->neib_n = calloc((size_t)n * Q6MAXBOND, sizeof(int));
+  q->qlm = calloc((size_t)n * Q6MAXBOND * 26, sizeof(double));
+  q->qlmsum = calloc((size_t)n * 26, sizeof(double));
+  q->nbond = calloc(n, sizeof(int));
+  q->q6 = calloc(n, sizeof(double));
+  q->dq6 = calloc(n, sizeof(double));
+  return ok && q->neis && q->neis_n && q->neib && q->neib_n && q->qlm && q->qlmsum &&
+         q->nbond && q->q6 && q->dq6;
+}
+static void q6st_free(q6st *q) {
+  if (!q->n) return;
+  tab_free(&q->nei6); tab_free(&q->nei3); tab_free(&q->nei5);
+  tab_free(&q->new6); tab_free(&q->new3); tab_free(&q->new5);
+  free(q->neis); free(q->neis_n); free(q->neib); free(q->neib_n); free(q->qlm);
+  free(q->qlmsum); free(q->nbond); free(q->q6); free(q->dq6);
+}
+
+/* sq6system_prepareall on a pair list; 1 when a molecule has more than
+ * MAXBOND bonds or an array overflows */
+static int q6_prepare(q6st *q, long np, const int *pi, const int *pj, const double *ox,
+                      const double *oy, const double *oz, const double *com, double *op) {
+  const int n = q->n;
+  int err = 0;
+  err |= g6o_neighbors_all2(n, np, pi, pj, ox, oy, oz, com, 0, 2, 6.0, q->nei6);
+  err |= g6o_neighbors_compress(n, q->nei6, 0, 2, 3.0, q->nei3);
+  err |= g6o_neighbors_compress(n, q->nei6, 0, 2, 5.0, q->nei5);
+  if (err) return 1;
+  for (int i = 0; i < n; i++) {
+    if (q->nei3.nnear[i] > Q6MAXBOND) return 1;
+    q->neis_n[i] = 0;
+    for (int k = 0; k < q->nei5.nnear[i]; k++) NEIS(q, i)[q->neis_n[i]++] = q->nei5.near[(long)i * G6O_NEAR + k];
+    set_join(NEIS(q, i), &q->neis_n[i], i);
+  }
+  for (int i = 0; i < n; i++)
+    for (int jj = 0; jj < q->nei3.nnear[i]; jj++) {
+      const int j = q->nei3.near[(long)i * G6O_NEAR + jj];
+      set_and(NEIS(q, i), q->neis_n[i], NEIS(q, j), q->neis_n[j], NEIB(q, i, jj), &NEIBN(q, i, jj));
+      q6_bond_qlm(&q->nei3, i, jj, QLM(q, i, jj));
+    }
+  memset(q->qlmsum, 0, sizeof(double) * (size_t)n * 26);
+  memset(q->nbond, 0, sizeof(int) * n);
+  for (int i = 0; i < n; i++)
+    for (int jj = 0; jj < q->nei3.nnear[i]; jj++)
+      for (int kk = 0; kk < NEIBN(q, i, jj); kk++) {
+        const int k = NEIB(q, i, jj)[kk];
+        q->nbond[k] += 1;
+        for (int c = 0; c < 26; c++) QSUM(q, k)[c] = QSUM(q, k)[c] + QLM(q, i, jj)[c];
+      }
+  q->sumq6 = 0.0;
+  for (int i = 0; i < n; i++) {
+    q->q6[i] = q6_of(QSUM(q, i), q->nbond[i]);
+    q->sumq6 = q->sumq6 + q->q6[i];
+  }
+  *op = q->sumq6 / n;
+  return 0;
+}
+
+/* qlmsum(:, k) += w * 2 * v (w = +1 or -1), nbond(k) += 2 w */
+static void q6_bump(q6st *q, int k, const double *v, int w) {
+  q->nbond[k] += 2 * w;
+  for (int c = 0; c < 26; c++)
+    QSUM(q, k)[c] = w > 0 ? QSUM(q, k)[c] + v[c] * 2 : QSUM(q, k)[c] - v[c] * 2;
+}
+
+/* sq6system_Differentiate: changes qlmsum and nbond in place (Integrate
+ * keeps, Reject undoes them) and returns the new op */
+static double q6_differentiate(q6st *q, int target, double dx, double dy, double dz) {
+  const int n = q->n;
+  for (int ii = 0; ii < q->nei3.nnear[target]; ii++)
+    for (int kk = 0; kk < NEIBN(q, target, ii); kk++)
+      q6_bump(q, NEIB(q, target, ii)[kk], QLM(q, target, ii), -1);
+  g6o_neighbors_displacement(q->nei6, 2, target, dx, dy, dz, &q->nmod, q->idx, q->new6);
+  g6o_neighbors_compress(q->nmod, q->new6, 0, 2, 3.0, q->new3);
+  g6o_neighbors_compress(q->nmod, q->new6, 0, 2, 5.0, q->new5);
+  for (int r = 0; r < q->nmod; r++) q->err |= q->new3.nnear[r] > Q6MAXBOND;
+  q->dneis_n = 0;
+  for (int k = 0; k < q->new5.nnear[0]; k++) q->dneis[q->dneis_n++] = q->new5.near[k];
+  set_join(q->dneis, &q->dneis_n, target);
+  set_minus(q->dneis, q->dneis_n, NEIS(q, target), q->neis_n[target], q->plus, &q->plus_n);
+  set_minus(NEIS(q, target), q->neis_n[target], q->dneis, q->dneis_n, q->minus, &q->minus_n);
+  for (int jj = 0; jj < q->new3.nnear[0] && jj < Q6MAXBOND; jj++) {
+    const int j = q->new3.near[jj];
+    set_and(q->dneis, q->dneis_n, NEIS(q, j), q->neis_n[j], q->dneib[jj], &q->dneib_n[jj]);
+    q6_bond_qlm(&q->new3, 0, jj, q->dqlm[jj]);
+  }
+  for (int jj = 0; jj < q->new3.nnear[0] && jj < Q6MAXBOND; jj++)
+    for (int kk = 0; kk < q->dneib_n[jj]; kk++) q6_bump(q, q->dneib[jj][kk], q->dqlm[jj], 1);
+  for (int ii = 0; ii < q->plus_n; ii++) {
+    const int i = q->plus[ii];
+    for (int jj = 0; jj < q->nei3.nnear[i]; jj++)
+      if (set_has(q->dneis, q->dneis_n, q->nei3.near[(long)i * G6O_NEAR + jj]))
+        q6_bump(q, target, QLM(q, i, jj), 1);
+  }
+  for (int ii = 0; ii < q->minus_n; ii++) {
+    const int i = q->minus[ii];
+    for (int jj = 0; jj < q->nei3.nnear[i]; jj++)
+      if (set_has(NEIS(q, target), q->neis_n[target], q->nei3.near[(long)i * G6O_NEAR + jj]))
+        q6_bump(q, target, QLM(q, i, jj), -1);
+  }
+  q->dsumq6 = 0.0;
+  for (int i = 0; i < n; i++) {
+    q->dq6[i] = q6_of(QSUM(q, i), q->nbond[i]);
+    q->dsumq6 = q->dsumq6 + q->dq6[i];
+  }
+  return q->dsumq6 / n;
+}
+
+static void copy_row(g6o_nbtab *dst, int drow, const g6o_nbtab *src, int srow);
+
+/* sq6system_Integrate */
+static void q6_integrate(q6st *q, int target) {
+  const int n = q->n;
+  memcpy(q->q6, q->dq6, sizeof(double) * n);
+  q->sumq6 = q->dsumq6;
+  for (int jj = 0; jj < q->new3.nnear[0] && jj < Q6MAXBOND; jj++)
+    memcpy(QLM(q, target, jj), q->dqlm[jj], sizeof(double) * 26);
+  memcpy(NEIS(q, target), q->dneis, sizeof(int) * q->dneis_n);
+  q->neis_n[target] = q->dneis_n;
+  for (int ii = 0; ii < q->plus_n; ii++) set_join(NEIS(q, q->plus[ii]), &q->neis_n[q->plus[ii]], target);
+  for (int ii = 0; ii < q->minus_n; ii++) set_drop(NEIS(q, q->minus[ii]), &q->neis_n[q->minus[ii]], target);
+  int oldb[Q6SET], newb[Q6SET], lost[Q6SET], found[Q6SET], kept[Q6SET];
+  int no = q->nei3.nnear[target], nn = q->new3.nnear[0], nl, nf, nk;
+  memcpy(oldb, q->nei3.near + (long)target * G6O_NEAR, sizeof(int) * no);
+  memcpy(newb, q->new3.near, sizeof(int) * nn);
+  set_minus(oldb, no, newb, nn, lost, &nl);
+  set_minus(newb, nn, oldb, no, found, &nf);
+  set_and(oldb, no, newb, nn, kept, &nk);
+  for (int ii = 0; ii < q->plus_n; ii++) {
+    const int i = q->plus[ii];
+    for (int jj = 0; jj < q->nei3.nnear[i]; jj++) {
+      const int j = q->nei3.near[(long)i * G6O_NEAR + jj];
+      if (!set_has(q->dneis, q->dneis_n, j)) continue;
+      set_join(NEIB(q, i, jj), &NEIBN(q, i, jj), target);
+      const int kk = row_lookup(&q->nei3, j, i);
+      if (kk >= 0) set_join(NEIB(q, j, kk), &NEIBN(q, j, kk), target);
+    }
+  }
+  for (int ii = 0; ii < q->minus_n; ii++) {
+    const int i = q->minus[ii];
+    for (int jj = 0; jj < q->nei3.nnear[i]; jj++) {
+      const int j = q->nei3.near[(long)i * G6O_NEAR + jj];
+      /* neis(target) is the new set here (assigned above) */
+      if (!set_has(NEIS(q, target), q->neis_n[target], j)) continue;
+      set_drop(NEIB(q, i, jj), &NEIBN(q, i, jj), target);
+      const int kk = row_lookup(&q->nei3, j, i);
+      if (kk >= 0) set_drop(NEIB(q, j, kk), &NEIBN(q, j, kk), target);
+    }
+  }
+  for (int r = 0; r < q->nmod; r++) copy_row(&q->nei6, q->idx[r], &q->new6, r);
+  for (int ii = 0; ii < nl; ii++) { /* lost bonds: the target's qlm drops out */
+    const int i = lost[ii];
+    int jj = 0;
+    for (int kk = 0; kk < q->nei3.nnear[i]; kk++)
+      if (q->nei3.near[(long)i * G6O_NEAR + kk] != target) {
+        memmove(QLM(q, i, jj), QLM(q, i, kk), sizeof(double) * 26);
+        jj++;
+      }
+  }
+  double tmp2[26] = {0};
+  for (int ii = 0; ii < nk; ii++) { /* kept bonds: the target's new qlm */
+    const int i = kept[ii];
+    const int jj = row_lookup(&q->new3, 0, i);
+    if (jj >= 0) memcpy(tmp2, q->dqlm[jj], sizeof(tmp2));
+    const int kk = row_lookup(&q->nei3, i, target);
+    if (kk >= 0) memcpy(QLM(q, i, kk), tmp2, sizeof(tmp2));
+  }
+  for (int ii = 0; ii < q->nmod; ii++) { /* found bonds: room for the target's */
+    const int i = q->idx[ii];
+    if (!set_has(found, nf, i)) continue;
+    double tmp[Q6MAXBOND][26];
+    const int m = q->new3.nnear[ii];
+    for (int c = 0; c + 1 < m && c < Q6MAXBOND; c++) memcpy(tmp[c], QLM(q, i, c), sizeof(tmp[c]));
+    const int jj0 = row_lookup(&q->new3, 0, i);
+    if (jj0 >= 0) memcpy(tmp2, q->dqlm[jj0], sizeof(tmp2));
+    int jj = 0;
+    for (int kk = 0; kk < m && kk < Q6MAXBOND; kk++) {
+      if (q->new3.near[(long)ii * G6O_NEAR + kk] != target) {
+        memcpy(QLM(q, i, kk), tmp[jj], sizeof(tmp2));
+        jj++;
+      } else {
+        memcpy(QLM(q, i, kk), tmp2, sizeof(tmp2));
+      }
+    }
+  }
+  for (int r = 0; r < q->nmod; r++) {
+    copy_row(&q->nei3, q->idx[r], &q->new3, r);
+    copy_row(&q->nei5, q->idx[r], &q->new5, r);
+  }
+  for (int jj = 0; jj < q->nei3.nnear[target]; jj++) {
+    const int j = q->nei3.near[(long)target * G6O_NEAR + jj];
+    set_and(NEIS(q, target), q->neis_n[target], NEIS(q, j), q->neis_n[j], NEIB(q, target, jj),
+            &NEIBN(q, target, jj));
+    for (int kk = 0; kk < q->nei3.nnear[j]; kk++) {
+      const int k = q->nei3.near[(long)j * G6O_NEAR + kk];
+      set_and(NEIS(q, j), q->neis_n[j], NEIS(q, k), q->neis_n[k], NEIB(q, j, kk), &NEIBN(q, j, kk));
+    }
+  }
+  for (int ii = 0; ii < nl; ii++) {
+    const int i = lost[ii];
+    for (int jj = 0; jj < q->nei3.nnear[i]; jj++) {
+      const int j = q->nei3.near[(long)i * G6O_NEAR + jj];
+      set_and(NEIS(q, i), q->neis_n[i], NEIS(q, j), q->neis_n[j], NEIB(q, i, jj), &NEIBN(q, i, jj));
+    }
+  }
+}
+
+/* sq6system_Reject: the sums of Differentiate taken back, in its order */
+static void q6_reject(q6st *q, int target) {
+  for (int jj = 0; jj < q->new3.nnear[0] && jj < Q6MAXBOND; jj++)
+    for (int kk = 0; kk < q->dneib_n[jj]; kk++) q6_bump(q, q->dneib[jj][kk], q->dqlm[jj], -1);
+  for (int ii = 0; ii < q->nei3.nnear[target]; ii++)
+    for (int kk = 0; kk < NEIBN(q, target, ii); kk++)
+      q6_bump(q, NEIB(q, target, ii)[kk], QLM(q, target, ii), 1);
+  for (int ii = 0; ii < q->plus_n; ii++) {
+    const int i = q->plus[ii];
+    for (int jj = 0; jj < q->nei3.nnear[i]; jj++)
+      if (set_has(q->dneis, q->dneis_n, q->nei3.near[(long)i * G6O_NEAR + jj]))
+        q6_bump(q, target, QLM(q, i, jj), -1);
+  }
+  for (int ii = 0; ii < q->minus_n; ii++) {
+    const int i = q->minus[ii];
+    for (int jj = 0; jj < q->nei3.nnear[i]; jj++)
+      if (set_has(NEIS(q, target), q->neis_n[target], q->nei3.near[(long)i * G6O_NEAR + jj]))
+        q6_bump(q, target, QLM(q, i, jj), 1);
+  }
+}
+
 /* ---- mc2's trial loop with umbrellas ------------------------------------
  * Umbrella potentials coeffi (op - center)^2 (Umbrella.F90:11-16) on the
  * order parameters of @GEUM (mc2util.F90:174-191): the triangle histogram's
@@ -1967,6 +2288,8 @@ struct umbctx {
   double hist[G6O_MAXUMB][21], newhist[G6O_MAXUMB][21];
   int nmod, idx[G6O_NEAR + 1], n;
   char *inset;
+  q6st q6[G6O_MAXUMB]; /* sq6system per Q6LOCAL_ umbrella */
+  int target;
 };
 
 static void hist_add(double *h, const g6o_nbtab *t, int row, double amount) {
@@ -2019,6 +2342,8 @@ static int umb_prepare(umbctx *U, int n, double L, double rc, const double *com,
   for (int b = 0; b < U->numb; b++) {
     if (U->type[b] == 3) {
       U->op[b] = epsum;
+    } else if (U->type[b] == 2) {
+      err |= q6_prepare(&U->q6[b], np, pi, pj, ox, oy, oz, com, &U->op[b]);
     } else if (umb_tri(U, b)) {
       err |= g6o_neighbors_all2(n, np, pi, pj, ox, oy, oz, com, 0, 1, 5.0, U->nei2[b]);
       err |= g6o_neighbors_compress(n, U->nei2[b], 4, 1, 0.0, U->nei[b]);
@@ -2100,8 +2425,11 @@ static double umbrella_potential(double coeffi, double center, double op) {
  * are the same for all of them. */
 static double umb_trial(umbctx *U, int target, double dx, double dy, double dz, double epsum,
                         double deltae) {
+  U->target = target;
   for (int b = 0; b < U->numb; b++)
-    U->newop[b] = U->type[b] == 3 ? epsum + deltae : tri_differentiate(U, b, target, dx, dy, dz);
+    U->newop[b] = U->type[b] == 3   ? epsum + deltae
+                  : U->type[b] == 2 ? q6_differentiate(&U->q6[b], target, dx, dy, dz)
+                                    : tri_differentiate(U, b, target, dx, dy, dz);
   double deltaope = 0.0;
   for (int b = 0; b < U->numb; b++)
     deltaope = deltaope + umbrella_potential(U->coeffi[b], U->center[b], U->newop[b]) -
@@ -2118,9 +2446,23 @@ static void umb_accept(umbctx *U) {
         copy_row(&U->nei[b], U->idx[r], &U->new4, r);
       }
       memcpy(U->hist[b], U->newhist[b], sizeof(double) * 21);
+    } else if (U->type[b] == 2) {
+      q6_integrate(&U->q6[b], U->target);
     }
     U->op[b] = U->newop[b];
   }
+}
+
+/* the rejected trial: sq6system_Reject (mc2.F90:820-829) */
+static void umb_reject(umbctx *U) {
+  for (int b = 0; b < U->numb; b++)
+    if (U->type[b] == 2) q6_reject(&U->q6[b], U->target);
+}
+
+static int umb_failed(const umbctx *U) {
+  for (int b = 0; b < U->numb; b++)
+    if (U->type[b] == 2 && U->q6[b].err) return 1;
+  return 0;
 }
 
 long g6o_mc_trials_umbrella(int model, int n, double L, double rc, double taper, double temp,
@@ -2138,18 +2480,23 @@ long g6o_mc_trials_umbrella(int model, int n, double L, double rc, double taper,
   U.refhist = refhist;
   U.op = op;
   int ok = tab_alloc(&U.new2, G6O_NEAR + 1) && tab_alloc(&U.new4, G6O_NEAR + 1);
-  for (int b = 0; b < numb; b++)
+  for (int b = 0; b < numb; b++) {
     if (umb_tri(&U, b)) ok = ok && tab_alloc(&U.nei2[b], n) && tab_alloc(&U.nei[b], n);
+    if (type[b] == 2) ok = ok && q6st_alloc(&U.q6[b], n);
+  }
   U.inset = calloc(n, 1);
   long acc = -1;
   if (ok && U.inset)
     acc = mc_trials(model, n, L, rc, taper, temp, com, quat, delta, rtr, ep, epsum, u, pos,
                     ntrial, &U);
-  for (int b = 0; b < numb; b++)
+  if (umb_failed(&U)) acc = -1;
+  for (int b = 0; b < numb; b++) {
     if (U.nei2[b].nnear) {
       tab_free(&U.nei2[b]);
       tab_free(&U.nei[b]);
     }
+    q6st_free(&U.q6[b]);
+  }
   tab_free(&U.new2);
   tab_free(&U.new4);
   free(U.inset);

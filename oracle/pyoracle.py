@@ -457,7 +457,7 @@ class MCRun:
             self.stream = getnext_stream(self.seed, max(count, 2 * len(self.stream)))
 
     # umbrella kinds of @GEUM (mc2util.F90:174-191)
-    UMB = {"TRIANGLE": 1, "EPOP____": 3, "LOGTRIAN": 4}
+    UMB = {"TRIANGLE": 1, "Q6LOCAL_": 2, "EPOP____": 3, "LOGTRIAN": 4}
 
     def set_umbrella(self, kinds, center, coeffi, refhist=None):
         """Umbrellas coeffi (op - center)^2 on the order parameters named by

@@ -59,10 +59,10 @@ def run_g6mc(cfg, deck):
         return open(os.path.join(d, "node0.in.out")).read()
 
 
-@pytest.mark.parametrize("case", [0, 1])
+@pytest.mark.parametrize("case", [0, 1, 2, 3])
 def test_g6mc_umbrella_deck_writes_what_mc2_writes(golden, case):
-    g = golden("tip4p_216")
     ref = golden("mc2_umbrella_tip4p_216")
+    g = golden(str(ref["start_%d" % case]))
     seed, temp, sweeps = ref["case_%d" % case]
     umb = list(zip([str(k) for k in ref["kinds_%d" % case]], ref["center_%d" % case],
                    ref["coeffi_%d" % case]))

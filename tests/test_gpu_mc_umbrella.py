@@ -12,13 +12,15 @@ from oracle import pyoracle as O
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("case", [0, 1])
+@pytest.mark.parametrize("case", [0, 1, 2, 3])
 def test_gpu_umbrella_mc_equals_reference_mc2(golden, case):
     """Same seed and start as mc2: every decision is mc2's (step sizes and
     rotation ratios bit for bit), and after each sweep of 2N trials the
-    energy and each order parameter equal what mc2 prints."""
-    g = golden("tip4p_216")
+    energy and each order parameter equal what mc2 prints.  Cases 2 and 3
+    carry the Q6 umbrella, whose sums the reference updates through every
+    trial of a sweep, rejected ones included."""
     r = golden("mc2_umbrella_tip4p_216")
+    g = golden(str(r["start_%d" % case]))
     seed, temp, sweeps = r["case_%d" % case]
     n = len(g["com"])
     print("umbrella mc2 case", case, flush=True)
@@ -109,3 +111,37 @@ def test_gpu_umbrella_capacity_error_keeps_the_state():
     np.testing.assert_array_equal(s.mc_state()[0], t.mc_state()[0])
     s.close()
     t.close()
+
+
+def test_gpu_q6_umbrella_mc_equals_oracle_1024(golden):
+    """1024 molecules of a liquid (the centres of order_tip4p_1024.npz, seeded
+    orientations), a Q6 and a triangle umbrella, two loops of 2N trials:
+    the oracle's decisions and order parameters.  The Q6 sums are updated
+    incrementally over 4096 trials on both sides."""
+    g = golden("order_tip4p_1024")
+    r = golden("mc2_umbrella_tip4p_216")
+    com, L = g["com"], float(g["L"])
+    n = len(com)
+    rng = np.random.default_rng(1024)
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1)[:, None]
+    q[q[:, 0] < 0] *= -1
+    kinds = ["Q6LOCAL_", "TRIANGLE"]
+    center, coeffi = [0.3, 0.5], [5.0e6, 3.0e3]
+    print("q6 umbrella mc 1024", flush=True)
+    s = g6.System(g6.TIP4P, n, L, 9.0, 0.5, 0.002)
+    s.upload(com, q)
+    s.mc_begin(300.0, 61, 0.15, 1.0)
+    s.mc_set_umbrella(kinds, center, coeffi, r["refhist"])
+    o = O.MCRun(O.TIP4P, com, q.T.copy(), L, 9.0, 0.5, 300.0, 61, 0.15, 1.0)
+    o.set_umbrella(kinds, center, coeffi, r["refhist"])
+    for loop in range(2):
+        acc = s.mc_trials(2 * n)
+        assert acc == o.trials(2 * n), loop
+        op = s.mc_umbrella()[0]
+        assert np.allclose(op, o.op, rtol=1e-12, atol=0), (loop, op, o.op)
+    c, qq, d, rt, e = s.mc_state()
+    np.testing.assert_array_equal(c, o.com)
+    np.testing.assert_array_equal(d, o.delta)
+    assert abs(e - o.epsum[0]) <= 1e-10 * abs(e)
+    s.close()

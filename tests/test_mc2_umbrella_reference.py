@@ -5,7 +5,9 @@ tests/golden/make_mc2_umbrella.py from the unmodified mc2.F90).
 mc2 adds sum(U(newop) - U(op)) with U = coeffi (op - center)^2 to every
 trial's energy change (mc2.F90:684-706), the order parameters being the
 triangle histogram's divergence from a reference histogram (TRIANGLE,
-LOGTRIAN: Triangle.F90) and the energy (EPOP____).  Same seed and start:
+LOGTRIAN: Triangle.F90), the local bond order Q6 updated incrementally over
+the trials of a loop (Q6LOCAL_: Q6Local.F90, whose sums the oracle must
+reproduce through rejected trials too) and the energy (EPOP____).  Same seed and start:
 every decision must be mc2's, so the step sizes and rotation ratios are
 equal bit for bit, and the order parameters after each sweep equal what
 mc2 prints.  The umbrellas are strong enough to steer the chain: without
@@ -16,7 +18,8 @@ import pytest
 from oracle import pyoracle as po
 
 
-def run_case(g, r, c, coeffi_scale=1.0, refhist=None):
+def run_case(golden, r, c, coeffi_scale=1.0, refhist=None):
+    g = golden(str(r["start_%d" % c]))
     seed, temp, sweeps = r["case_%d" % c]
     n = len(g["com"])
     run = po.MCRun(po.TIP4P, g["com"], g["quat"].T.copy(), float(g["L"]), float(g["rc"]),
@@ -31,11 +34,10 @@ def run_case(g, r, c, coeffi_scale=1.0, refhist=None):
     return run, np.array(rows)
 
 
-@pytest.mark.parametrize("case", [0, 1])
+@pytest.mark.parametrize("case", [0, 1, 2, 3])
 def test_oracle_replays_reference_mc2_umbrella(golden, case):
-    g = golden("tip4p_216")
     r = golden("mc2_umbrella_tip4p_216")
-    run, rows = run_case(g, r, case)
+    run, rows = run_case(golden, r, case)
     want = r["mc2o_%d" % case]
     nu = len(r["kinds_%d" % case])
     assert np.allclose(rows[:, 0], want[:, 1], rtol=1e-13, atol=0)
@@ -48,17 +50,15 @@ def test_oracle_replays_reference_mc2_umbrella(golden, case):
     assert np.allclose(run.quat.T, r["quat_%d" % case], rtol=0, atol=1e-9)
 
 
-@pytest.mark.parametrize("case", [0, 1])
+@pytest.mark.parametrize("case", [0, 1, 2, 3])
 def test_umbrella_steers_the_decisions(golden, case):
-    g = golden("tip4p_216")
     r = golden("mc2_umbrella_tip4p_216")
-    run, _ = run_case(g, r, case, coeffi_scale=0.0)
+    run, _ = run_case(golden, r, case, coeffi_scale=0.0)
     assert not np.array_equal(run.delta, r["delta_%d" % case])
 
 
 def test_reference_histogram_enters(golden):
-    g = golden("tip4p_216")
     r = golden("mc2_umbrella_tip4p_216")
     flat = np.roll(r["refhist"], 1)
-    _, rows = run_case(g, r, 1, refhist=flat)
+    _, rows = run_case(golden, r, 1, refhist=flat)
     assert not np.allclose(rows[:, 1], r["mc2o_1"][:, 3], rtol=1e-6)
