@@ -334,19 +334,24 @@ bool make_model_table(int model, int prec, ModelTable *m, bool fast) {
 
 // half-shell stencil, maxradius = 0 branch (NeighborCell_ListHomo,
 // GridModule.F90:279-374); offsets packed ox | oy<<8 | oz<<16
-static void make_stencil(int nd, Geom *g) {
-  int nmax = 1, nmin = -1;
-  if (nmax > (nd - 1) / 2) nmax = (nd - 1) / 2;
-  if (nmin < -(nd / 2)) nmin = -(nd / 2);
-  std::vector<int> mark((size_t)nd * nd * nd, 0);
-  auto MK = [&](int x, int y, int z) -> int & { return mark[((size_t)z * nd + y) * nd + x]; };
+static void make_stencil(const int (&nd)[3], Geom *g) {
+  int nmax[3], nmin[3];
+  for (int a = 0; a < 3; a++) {
+    nmax[a] = 1;
+    nmin[a] = -1;
+    if (nmax[a] > (nd[a] - 1) / 2) nmax[a] = (nd[a] - 1) / 2;
+    if (nmin[a] < -(nd[a] / 2)) nmin[a] = -(nd[a] / 2);
+  }
+  const int nx = nd[0], ny = nd[1], nz = nd[2];
+  std::vector<int> mark((size_t)nx * ny * nz, 0);
+  auto MK = [&](int x, int y, int z) -> int & { return mark[((size_t)z * ny + y) * nx + x]; };
   g->nst = g->nst0 = 0;
-  for (int jz = nmin; jz <= nmax; jz++) {
-    const int pz = jz < 0 ? jz + nd : jz, mz = pz == 0 ? 0 : nd - pz;
-    for (int jy = nmin; jy <= nmax; jy++) {
-      const int py = jy < 0 ? jy + nd : jy, my = py == 0 ? 0 : nd - py;
-      for (int jx = nmin; jx <= nmax; jx++) {
-        const int px = jx < 0 ? jx + nd : jx, mx = px == 0 ? 0 : nd - px;
+  for (int jz = nmin[2]; jz <= nmax[2]; jz++) {
+    const int pz = jz < 0 ? jz + nz : jz, mz = pz == 0 ? 0 : nz - pz;
+    for (int jy = nmin[1]; jy <= nmax[1]; jy++) {
+      const int py = jy < 0 ? jy + ny : jy, my = py == 0 ? 0 : ny - py;
+      for (int jx = nmin[0]; jx <= nmax[0]; jx++) {
+        const int px = jx < 0 ? jx + nx : jx, mx = px == 0 ? 0 : nx - px;
         if (MK(px, py, pz) == 0) {
           MK(mx, my, mz) += 1;
           MK(px, py, pz) += 1;
@@ -408,10 +413,12 @@ static int env_int(const char *name, int dflt) {
 static void set_row_grid(System &s) {
   Geom &m = s.geo_md;
   const double rl = s.rc + s.skin;
-  m.ndiv = (int)(s.L / (0.5 * rl));
-  if (m.ndiv < 1) m.ndiv = 1;
-  m.ncell = m.ndiv * m.ndiv * m.ndiv;
-  m.cellinv = 1.0 / (s.L / m.ndiv);
+  for (int a = 0; a < 3; a++) {
+    m.nd[a] = (int)(s.Lb[a] / (0.5 * rl));
+    if (m.nd[a] < 1) m.nd[a] = 1;
+    m.cellinv[a] = 1.0 / (s.Lb[a] / m.nd[a]);
+  }
+  m.ncell = m.nd[0] * m.nd[1] * m.nd[2];
   const double percell = (double)s.n / m.ncell;
   s.md.cell_cap = ((int)(2.0 * percell + 33.0) + 31) / 32 * 32;
   if (s.md.cell_cap > s.n + 1) s.md.cell_cap = s.n + 1;
@@ -427,9 +434,11 @@ static void set_row_grid(System &s) {
 // per (cell, slice).
 static void set_tile_grid(System &s) {
   Geom &m = s.geo_md;
-  m.ndiv = s.tile_ndiv;
-  m.ncell = m.ndiv * m.ndiv * m.ndiv;
-  m.cellinv = 1.0 / (s.L / m.ndiv);
+  for (int a = 0; a < 3; a++) {
+    m.nd[a] = s.tile_ndiv;
+    m.cellinv[a] = 1.0 / (s.Lb[a] / m.nd[a]);
+  }
+  m.ncell = m.nd[0] * m.nd[1] * m.nd[2];
   s.md.cell_cap = s.tile_cap;
   s.nblk_md = m.ncell * s.md.slices;
   s.tiles = true;
@@ -443,7 +452,7 @@ static void set_tile_grid(System &s) {
 // G6_TILE_WAVES override the workgroup shape.
 static void tile_plan(System &s, double nb_full) {
   s.tile_ok = false;
-  if (s.prec != G6_FP32 || env_int("G6_TILES", 1) == 0) return;
+  if (s.prec != G6_FP32 || env_int("G6_TILES", 1) == 0 || !is_cubic(s)) return;
   const double rl = s.rc + s.skin;
   const int nd = (int)(s.L / rl);
   if (nd < 3) return;
@@ -722,11 +731,13 @@ int g6_model_nsite(int model) {
 
 static int scratch_address(void **p);
 
-int g6_init(g6_handle *h, int model, int n, double box_l, double rc, double taper,
-            double voxa, double dt, int prec, double skin) {
-  if (!h || n < 2 || !(box_l > 0) || !(rc > 0) || !(taper >= 0) || taper > rc || !(dt > 0) ||
-      (prec != G6_FP64 && prec != G6_FP32) || !(skin >= 0) || !(voxa > 0))
+static int init_system(g6_handle *h, int model, int n, const double *box3, double rc,
+                       double taper, const double *voxa3, double dt, int prec, double skin) {
+  if (!h || n < 2 || !(rc > 0) || !(taper >= 0) || taper > rc || !(dt > 0) ||
+      (prec != G6_FP64 && prec != G6_FP32) || !(skin >= 0))
     return G6_EARG;
+  for (int a = 0; a < 3; a++)
+    if (!(box3[a] > 0) || !(voxa3[a] > 0)) return G6_EARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return G6_ENODEVICE;
   System *s = new System();
@@ -740,26 +751,30 @@ int g6_init(g6_handle *h, int model, int n, double box_l, double rc, double tape
   s->n = n;
   s->ns = s->mt.nsite;
   s->prec = prec;
-  s->L = box_l;
+  for (int a = 0; a < 3; a++) s->Lb[a] = box3[a];
+  s->L = box3[0];
   s->rc = rc;
   s->taper = taper;
-  s->req = voxa;
+  s->req = voxa3[0];
   s->dt = dt;
   s->skin = skin;
   Geom &g = s->geo;
   g.n = n;
-  // Grid_ResetDivision: integer part of L/req (GridModule.F90:497-499)
-  g.ndiv = (int)(box_l / voxa);
-  if (g.ndiv < 1) g.ndiv = 1;
-  if (rc + skin > 0.5 * box_l * (1.0 + 1e-12) && g.ndiv > 2) {
-    delete s;  // the half-shell stencil of 27 cells needs rc + skin <= L/2
-    return G6_EARG;
+  for (int a = 0; a < 3; a++) {
+    // Grid_ResetDivision: integer part of L/req per axis (GridModule.F90:494-499)
+    g.nd[a] = (int)(box3[a] / voxa3[a]);
+    if (g.nd[a] < 1) g.nd[a] = 1;
+    if (rc + skin > 0.5 * box3[a] * (1.0 + 1e-12) && g.nd[a] > 2) {
+      delete s;  // the half-shell stencil of 27 cells needs rc + skin <= L/2
+      return G6_EARG;
+    }
+    g.L[a] = box3[a];
+    g.invL[a] = 1.0 / box3[a];                 // Box_ReadBXLA / Box_ReadBOX3
+    g.cellinv[a] = 1.0 / (box3[a] / g.nd[a]);  // Grid_ResetCellSize + invert
   }
-  g.ncell = g.ndiv * g.ndiv * g.ndiv;
-  make_stencil(g.ndiv, &g);
-  g.L = box_l;
-  g.invL = 1.0 / box_l;                  // Box_ReadBXLA
-  g.cellinv = 1.0 / (box_l / g.ndiv);    // Grid_ResetCellSize + invert
+  g.ncell = g.nd[0] * g.nd[1] * g.nd[2];
+  make_stencil(g.nd, &g);
+  const double vol = (box3[0] * box3[1]) * box3[2];
   const double rl = rc + skin;
   g.rr = rl * rl;
   g.rc2 = rc * rc;
@@ -768,7 +783,7 @@ int g6_init(g6_handle *h, int model, int n, double box_l, double rc, double tape
   g.c1 = taper > 0 ? 1.0 / ipow(g.rin - g.rout, 5) : 0.0;  // cutoff_initialize
   g.c2 = g.c1 * 30.0;
   // pair capacity: 1.6x the ideal-gas half-shell count within rc+skin, +64/molecule
-  const double per = 0.5 * 4.0 / 3.0 * kPI * rl * rl * rl * n / (box_l * box_l * box_l);
+  const double per = 0.5 * 4.0 / 3.0 * kPI * rl * rl * rl * n / vol;
   const double capd = (double)n * (per * 1.6 + 64.0);
   const double all = 0.5 * (double)n * (double)(n - 1);
   s->pcap = (int)fmin(fmin(capd, all + 64.0), 2.0e9);
@@ -781,7 +796,7 @@ int g6_init(g6_handle *h, int model, int n, double box_l, double rc, double tape
     MdBuf &b = s->md;
     b.rs = prec == G6_FP32 ? kRecF : kRecD;
     set_row_grid(*s);
-    const double rho = n / (box_l * box_l * box_l);
+    const double rho = n / vol;
     const double nb = 4.0 / 3.0 * kPI * rl * rl * rl * rho;
     b.maxnb = ((int)(1.5 * nb + 64.0) + 63) / 64 * 64;
     if (b.maxnb > n) b.maxnb = n;
@@ -820,6 +835,18 @@ int g6_init(g6_handle *h, int model, int n, double box_l, double rc, double tape
   g_sys.push_back(s);
   *h = (g6_handle)g_sys.size();
   return 0;
+}
+
+int g6_init(g6_handle *h, int model, int n, double box_l, double rc, double taper,
+            double voxa, double dt, int prec, double skin) {
+  const double box3[3] = {box_l, box_l, box_l}, voxa3[3] = {voxa, voxa, voxa};
+  return init_system(h, model, n, box3, rc, taper, voxa3, dt, prec, skin);
+}
+
+int g6_init_box3(g6_handle *h, int model, int n, const double *box3, double rc, double taper,
+                 const double *voxa3, double dt, int prec, double skin) {
+  if (!box3 || !voxa3) return G6_EARG;
+  return init_system(h, model, n, box3, rc, taper, voxa3, dt, prec, skin);
 }
 
 int g6_finalize(g6_handle h) {
@@ -1217,10 +1244,10 @@ int g6_set_andersen(g6_handle h, int mode, double pex, double mass, const double
       s->andersen = false;
     }
   } else {
-    // orthorhombic (the cube follows the volume); orz changes one edge only
+    // orthorhombic (every edge follows the volume) or orz (the z edge alone);
     // every argument is checked before the handle changes: a failed call
     // leaves the fixed-box path as it was
-    if (mode != 1 || !(mass > 0)) return G6_EARG;
+    if ((mode != 1 && mode != 2) || !(mass > 0)) return G6_EARG;
     if (v && !(v[0] > 0)) return G6_EARG;
     const double dof = 6.0 * s->n;
     const double rl = s->rc + s->skin;
@@ -1231,18 +1258,26 @@ int g6_set_andersen(g6_handle h, int mode, double pex, double mass, const double
     s->tile_ok = false;  // the tile grid was planned for the fixed box
     // the box is the handle's (a restart's @BOX3); v continues a saved
     // barostat state (@PCN5), NULL starts one
-    const double L = s->L;
-    const double nd = s->geo_md.ndiv;
-    const double bx[8] = {L, 1.0 / L, 1.0 / (L / nd), L, nd, nd * 0.5 * rl, 0, 0};
+    double bx[kBoxLen];
+    for (int a = 0; a < 3; a++) {
+      const double L = s->Lb[a], nd = s->geo_md.nd[a];
+      bx[a] = L;
+      bx[3 + a] = 1.0 / L;
+      bx[6 + a] = 1.0 / (L / nd);
+      bx[9 + a] = L;
+      bx[12 + a] = nd;
+      if (L < nd * (0.5 * rl)) return G6_EARG;
+    }
+    bx[15] = 0.5 * rl;
+    bx[16] = mode;
     // internal mass: per-molecule mass * dof / dt^2 (Mixture.F90:1224-1228);
     // a fresh start takes v0 = the box volume (Andersen_Initialize)
-    const double an[8] = {pex, mass * dof / (s->dt * s->dt), v ? v[0] : (L * L) * L,
+    const double an[8] = {pex, mass * dof / (s->dt * s->dt), v ? v[0] : box_volume(s->Lb),
                           v ? v[1] : 0.0, v ? v[2] : 0.0, v ? v[3] : 0.0, v ? v[4] : 0.0,
                           v ? v[5] : 0.0};
-    if (L < bx[5]) return G6_EARG;
     if (!s->andersen) {
       double *box = nullptr, *anders = nullptr;
-      if (hipMalloc((void **)&box, sizeof(double) * 8) != hipSuccess) return G6_EHIP;
+      if (hipMalloc((void **)&box, sizeof(double) * kBoxLen) != hipSuccess) return G6_EHIP;
       if (hipMalloc((void **)&anders, sizeof(double) * 8) != hipSuccess) {
         (void)hipFree(box);
         return G6_EHIP;
@@ -1254,6 +1289,7 @@ int g6_set_andersen(g6_handle h, int mode, double pex, double mass, const double
     G6_CHECK(hipStreamSynchronize(s->stream));
     G6_CHECK(hipMemcpy(s->md.box, bx, sizeof(bx), hipMemcpyHostToDevice));
     G6_CHECK(hipMemcpy(s->md.anders, an, sizeof(an), hipMemcpyHostToDevice));
+    s->andersen_mode = mode;
     // rows of the new box
     int one[2] = {1, 1};
     G6_CHECK(hipMemcpy(s->md.flags + 2, one, sizeof(one), hipMemcpyHostToDevice));
@@ -1278,16 +1314,24 @@ int g6_get_andersen(g6_handle h, double *st) {
   return 0;
 }
 
-int g6_get_box(g6_handle h, double *box_l) {
+int g6_get_box3(g6_handle h, double *edges) {
   System *s = get(h);
-  if (!s || !box_l) return G6_EARG;
+  if (!s || !edges) return G6_EARG;
   if (!s->andersen) {
-    *box_l = s->L;
+    for (int a = 0; a < 3; a++) edges[a] = s->Lb[a];
     return 0;
   }
   G6_CHECK(hipStreamSynchronize(s->stream));
-  G6_CHECK(hipMemcpy(box_l, s->md.box, sizeof(double), hipMemcpyDeviceToHost));
+  G6_CHECK(hipMemcpy(edges, s->md.box, sizeof(double) * 3, hipMemcpyDeviceToHost));
   return 0;
+}
+
+int g6_get_box(g6_handle h, double *box_l) {
+  if (!box_l) return G6_EARG;
+  double e[3];
+  const int rc = g6_get_box3(h, e);
+  if (rc == 0) *box_l = e[0];
+  return rc;
 }
 
 int g6_get_thermo(g6_handle h, double *thermo) {
@@ -1309,7 +1353,7 @@ int g6_pairlist(g6_handle h, const double *com, int64_t cap, int64_t *npair0, in
 int g6_cells(g6_handle h, const double *com, int32_t *cell, int32_t *ndiv) {
   System *s = get(h);
   if (!s || !com || !cell || !ndiv) return G6_EARG;
-  *ndiv = s->geo.ndiv;
+  *ndiv = s->geo.nd[0];
   return cells_once(*s, com, cell);
 }
 

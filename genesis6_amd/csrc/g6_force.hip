@@ -13,15 +13,6 @@ __device__ inline double wave_sum(double v) {
   return v;
 }
 
-__device__ inline void unpack_shift(int code, double L, double &ox, double &oy,
-                                    double &oz) {
-  // field 0 encodes -0.0 (see shift_code in g6_pairlist.hip)
-  const int a = code & 1023, b = (code >> 10) & 1023, c = (code >> 20) & 1023;
-  ox = a == 0 ? -0.0 : (double)(a - 512) * L;
-  oy = b == 0 ? -0.0 : (double)(b - 512) * L;
-  oz = c == 0 ? -0.0 : (double)(c - 512) * L;
-}
-
 // ---- module seam: force() / PairPotential() on host-given arrays ---------
 template <int M, typename R>
 __global__ __launch_bounds__(256) void k_force_pairs(

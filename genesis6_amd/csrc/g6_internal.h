@@ -82,12 +82,17 @@ struct PairK {
   R rl, ru, rl2, ru2, w3inv;         // ST2 switch on the COM distance
 };
 
-// Grid geometry and cutoff constants of one system.
+constexpr int kBoxLen = 17;  // doubles of the barostat's device box
+
+// Grid geometry and cutoff constants of one system.  The box is
+// orthorhombic: edges L[a], 1/L[a] and 1/(L[a]/nd[a]) per axis, as the
+// reference's sBox and sGrid hold them (BoxModule.F90, GridModule.F90:473-501).
 struct Geom {
-  int n, ndiv, ncell;
+  int n, ncell;
+  int nd[3];               // cells per axis
   int nst, nst0;           // half-shell and self-mirror stencil sizes
   int st[27], st0[27];     // stencil offsets packed ox | oy<<8 | oz<<16
-  double L, invL, cellinv; // box edge, 1/L, 1/(L/ndiv) as the reference
+  double L[3], invL[3], cellinv[3];
   double rr;               // pair-list radius squared ((rc+skin)^2)
   double rc2;              // rc^2 (interaction filter)
   double rin, rout, c1, c2; // switch: S between rin and rout
@@ -114,9 +119,12 @@ struct MdBuf {
   double *nose, *thermo;
   int *flags;  // [0] error code, [2 + parity] rebuild rows this step
   double half_skin2;
-  // Andersen constant-pressure MD (both nullptr for a fixed box).  box: [0]
-  // the current edge L, [1] 1/L, [2] 1/(L/ndiv) of the MD grid, [3] L at the
-  // last row build, [4] ndiv, [5] the smallest edge the grid allows.
+  // Andersen constant-pressure MD (both nullptr for a fixed box).  box
+  // (kBoxLen doubles): [0..2] the current edges, [3..5] their inverses,
+  // [6..8] 1/(L/nd) of the MD grid per axis, [9..11] the edges at the last
+  // row build, [12..14] nd per axis, [15] the smallest cell edge the grid
+  // allows, [16] the barostat mode (1 orthorhombic: every edge follows the
+  // volume; 2 orz: the z edge alone, AndersenModule.F90:65-82).
   // anders: pex (atm), mass (internal units), v0..v5 (AndersenModule.F90:5-8)
   double *box, *anders;
   // Cell tiles (g6_tile.hip), FP32 fixed-box systems whose grid has cells of
@@ -226,6 +234,9 @@ struct System {
   // the pair-kernel layout the MD and force paths instantiate (ST2 without
   // fast runs StdForce, whose site terms are those of the TIP5P layout)
   int fast = 1, kmodel = 0;
+  // Lb: the box edges; L = Lb[0], the edge of the paths that take cubic
+  // boxes only (tiles, ladders, Monte Carlo, NPT Monte Carlo)
+  double Lb[3] = {0, 0, 0};
   double L = 0, rc = 0, taper = 0, req = 0, dt = 0, skin = 0;
   ModelTable mt{};
   Geom geo{};     // the reference grid (ndiv = int(L/voxa)), entry points
@@ -241,6 +252,7 @@ struct System {
   bool nose_active = false;
   double nose_h[9] = {0};
   bool andersen = false;  // md.box and md.anders are allocated
+  int andersen_mode = 0;  // 1 orthorhombic, 2 orz
   bool tiles = false;     // the MD path runs the cell-tile force kernel
   int tile_exit = 0;      // flag code of the capacity error that left the tiles
   int last_flag = 0;      // flag code the last check of the step flags read
@@ -443,3 +455,9 @@ int force_pairs(System &s, int64_t np, const int32_t *pi, const int32_t *pj,
     hipError_t e_ = (x);                  \
     if (e_ != hipSuccess) return G6_EHIP; \
   } while (0)
+
+namespace g6 {
+inline bool is_cubic(const System &s) { return s.Lb[0] == s.Lb[1] && s.Lb[0] == s.Lb[2]; }
+// Box_Volume (BoxModule.F90:160-165): x y z
+__device__ __host__ inline double box_volume(const double *L) { return (L[0] * L[1]) * L[2]; }
+}  // namespace g6

@@ -291,7 +291,7 @@ int g6_ladder_create(g6_ladder *hl, int nsys, const g6_handle *hs) {
     System *s = system_of(hs[r]);
     bool dup = false;
     for (System *t : l->sys) dup = dup || t == s;
-    if (!s || dup || s->andersen) {  // ladders step fixed boxes
+    if (!s || dup || s->andersen || !is_cubic(*s)) {  // ladders step fixed cubic boxes
       delete l;
       return G6_EARG;
     }

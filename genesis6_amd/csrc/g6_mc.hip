@@ -789,6 +789,7 @@ int g6_mc_begin(g6_handle h, double temp, uint32_t seed, const double *delta,
                 const double *rtratio) {
   System *s = system_of(h);
   if (!s || !(temp > 0) || !delta || !rtratio) return G6_EARG;
+  if (!is_cubic(*s)) return G6_EARG;  // mc2's trial loop runs cubic boxes here
   const int n = s->n, ns = s->ns;
   if (n > kMcThreads * kMcMaxPer || mc_lds(n) > 150 * 1024) return G6_EARG;
   mc_free(*s);

@@ -88,10 +88,16 @@ __device__ __forceinline__ void md_pre_body(const ModelTable &mt, const Geom &g,
 #pragma unroll
   for (int k = 0; k < 20; k++) S[k] = 0.0;
   const double zeta0 = nose_active ? b.nose[3] : 0.0;
-  // translational drag: Nose, plus the Andersen dilation v1 / (2*3*v0)
-  // (RigidModule.F90:1275-1305, orthorhombic)
-  const double acoeff = NPT ? b.anders[3] / (2.0 * 3.0 * b.anders[2]) : 0.0;
-  const double nc = zeta0 * dt, ext = nc * 0.5 + acoeff, exr = nc;
+  // translational drag: Nose, plus the Andersen dilation
+  // (RigidModule.F90:1275-1305): v1 / (2*3*v0) on every axis in the
+  // orthorhombic mode, v1 / (2*v0) on z alone in the orz mode
+  const bool orz = NPT && b.box[16] == 2.0;
+  const double acoeff = !NPT ? 0.0
+                        : orz ? b.anders[3] / (2.0 * b.anders[2])
+                              : b.anders[3] / (2.0 * 3.0 * b.anders[2]);
+  const double nc = zeta0 * dt, exr = nc;
+  const double ext[3] = {nc * 0.5 + (orz ? 0.0 : acoeff), nc * 0.5 + (orz ? 0.0 : acoeff),
+                         nc * 0.5 + acoeff};
   const BoxNow bx = box_now<NPT>(g, b);
   // every global load is issued here, ahead of the first barrier
   double F[3] = {0, 0, 0}, t[9], tl[3] = {0, 0, 0}, cref[3] = {0, 0, 0};
@@ -141,7 +147,7 @@ __device__ __forceinline__ void md_pre_body(const ModelTable &mt, const Geom &g,
 #pragma unroll
         for (int a = 0; a < 3; a++) {
           double nf = F[a] * tf;
-          nf = nf - S[3 + a] * ext;
+          nf = nf - S[3 + a] * ext[a];
           const double d = nf - S[6 + a];
           S[a] = S[a] + kG5_0 * d;
           S[3 + a] = S[3 + a] + kG5_1 * d;
@@ -341,14 +347,14 @@ __device__ __forceinline__ void md_pre_body(const ModelTable &mt, const Geom &g,
 #pragma unroll
       for (int a = 0; a < 3; a++) {
         double d = S[a];
-        const double o = round(d * bx.invL) * bx.L;
+        const double o = round(d * bx.invL[a]) * bx.L[a];
         d = d - o;
-        int j = (int)round(d * bx.cellinv);
-        if (j < 0) j += g.ndiv;
-        j = ((j % g.ndiv) + g.ndiv) % g.ndiv;
+        int j = (int)round(d * bx.cellinv[a]);
+        if (j < 0) j += g.nd[a];
+        j = ((j % g.nd[a]) + g.nd[a]) % g.nd[a];
         jc[a] = j;
       }
-      const int c = (jc[2] * g.ndiv + jc[1]) * g.ndiv + jc[0];
+      const int c = (jc[2] * g.nd[1] + jc[1]) * g.nd[0] + jc[0];
       b.cell[i] = c;
       // slot 0 of a cell row is its count; residents follow
       int *cr = b.cellres + ((size_t)parity * g.ncell + c) * b.cell_cap;
@@ -359,10 +365,11 @@ __device__ __forceinline__ void md_pre_body(const ModelTable &mt, const Geom &g,
         raise_flag(b.flags, 3);  // cell capacity exceeded
       if (b.half_skin2 > 0.0) {
         // a pair of the rows moves by at most twice the largest displacement
-        // plus sqrt(3)|L - L_ref| (image shifts of one box per axis)
+        // plus |L - L_ref| (image shifts of one box per axis)
         double h2 = b.half_skin2;
         if (NPT) {
-          const double h = sqrt(b.half_skin2) - 0.8660254037844386 * fabs(bx.L - b.box[3]);
+          const double e0 = bx.L[0] - b.box[9], e1 = bx.L[1] - b.box[10], e2 = bx.L[2] - b.box[11];
+          const double h = sqrt(b.half_skin2) - 0.5 * sqrt(e0 * e0 + e1 * e1 + e2 * e2);
           h2 = h > 0.0 ? h * h : -1.0;
         }
         const double e0 = S[0] - cref[0], e1 = S[1] - cref[1], e2 = S[2] - cref[2];
@@ -370,7 +377,10 @@ __device__ __forceinline__ void md_pre_body(const ModelTable &mt, const Geom &g,
       } else if (i == 0) {
         b.flags[2 + parity] = 1;  // no skin: the reference rebuilds every step
       }
-      if (NPT && i == 0 && bx.L < b.box[5]) raise_flag(b.flags, 5);  // cells below (rc+skin)/2
+      if (NPT && i == 0 &&
+          (bx.L[0] < b.box[12] * b.box[15] || bx.L[1] < b.box[13] * b.box[15] ||
+           bx.L[2] < b.box[14] * b.box[15]))
+        raise_flag(b.flags, 5);  // cells below (rc+skin)/2
     } else if (role >= 2) {
       // rigid_setmatrix and the site offsets; the two waves split the stores
       const double qa = S[0], qb = S[1], qc = S[2], qd = S[3];
@@ -454,10 +464,10 @@ template <bool NPT>
 __global__ __launch_bounds__(256) void k_md_close(ModelTable mt, MdBuf b, int n, double dt,
                                                   int nblk_pre, const double *bpart,
                                                   int nblk_main, int ncell, int nose_active,
-                                                  int mode, int parity, double box_l) {
+                                                  int mode, int parity, double box_vol) {
   __shared__ double s_red[4][kRed];
   close_body<NPT>(mt, b, n, dt, nblk_pre, bpart, nblk_main, ncell, nose_active, mode, parity,
-                  box_l, s_red);
+                  box_vol, s_red);
 }
 
 // Ladder form: block r closes system r.
@@ -468,24 +478,31 @@ __global__ __launch_bounds__(256) void k_md_close_lad(ModelTable mt, const MdRep
   const MdBuf b = r.b;
   const int n = r.g.n, nm = r.nblk_main;
   close_body<false>(mt, b, n, r.dt, (n + 63) / 64 * 4, b.blockpart + (size_t)src_parity * nm * kBP, nm,
-             r.g.ncell, r.nose_active, mode, parity, r.g.L, s_red);
+             r.g.ncell, r.nose_active, mode, parity, box_volume(r.g.L), s_red);
 }
 
 // ---- kernel B --------------------------------------------------------------
 
-// Cell stencil of the MD grid: per axis na offsets from lo; with ndiv >= 5
-// the 5 offsets -2..2 are distinct cells and are pruned by their distance
-// from the molecule, below that every cell is taken once (no pruning).
+// Cell stencil of the MD grid: per axis na offsets from lo; on an axis of
+// nd >= 5 cells the 5 offsets -2..2 are distinct cells and are pruned by
+// their distance from the molecule, on a shorter axis every cell is taken
+// once (no pruning).
 struct Stencil {
-  int na, lo, nst;
-  bool prune;
+  int na[3], lo[3], nst;
+  bool prune[3], any;
 };
-__device__ inline Stencil stencil_of(int nd) {
+__device__ inline Stencil stencil_of(const int (&nd)[3]) {
   Stencil st;
-  st.prune = nd >= 5;
-  st.na = st.prune ? 5 : nd;
-  st.lo = st.prune ? -2 : -((nd - 1) / 2);
-  st.nst = st.na * st.na * st.na;
+  st.nst = 1;
+  st.any = false;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    st.prune[a] = nd[a] >= 5;
+    st.na[a] = st.prune[a] ? 5 : nd[a];
+    st.lo[a] = st.prune[a] ? -2 : -((nd[a] - 1) / 2);
+    st.nst *= st.na[a];
+    st.any = st.any || st.prune[a];
+  }
   return st;
 }
 
@@ -497,9 +514,9 @@ __device__ inline double cell_gap(int o, double u) {
   return fmax(0.0, fmax(a, c) - 1e-9);
 }
 
-__device__ inline double frac_in_cell(const BoxNow &bx, double x) {
-  const double d = x - round(x * bx.invL) * bx.L;
-  const double t = d * bx.cellinv;
+__device__ inline double frac_in_cell(const BoxNow &bx, double x, int a) {
+  const double d = x - round(x * bx.invL[a]) * bx.L[a];
+  const double t = d * bx.cellinv[a];
   return t - round(t);
 }
 
@@ -511,29 +528,31 @@ __device__ inline int build_row(const Geom &g, const BoxNow &bx, const MdBuf &b,
                                 int i, int lane, int (&s_beg)[128], int (&s_pre)[132], int *s_j,
                                 int *s_c) {
   const int n = g.n;
-  const int nd = g.ndiv;
+  const int nx = g.nd[0], ny = g.nd[1], nz = g.nd[2];
   const int c = b.cell[i];
-  const int cx = c % nd, cy = (c / nd) % nd, cz = c / (nd * nd);
-  const Stencil st = stencil_of(nd);
+  const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
+  const Stencil st = stencil_of(g.nd);
   const double *reci = b.rec + (size_t)i * b.rs;
   const double xi = reci[0], yi = reci[1], zi = reci[2];
-  const double ux = frac_in_cell(bx, xi), uy = frac_in_cell(bx, yi), uz = frac_in_cell(bx, zi);
-  const double lim = g.rr * (bx.cellinv * bx.cellinv);  // (rc+skin)^2 in cell units
+  const double ux = frac_in_cell(bx, xi, 0), uy = frac_in_cell(bx, yi, 1),
+               uz = frac_in_cell(bx, zi, 2);
   const int *res = b.cellres + (size_t)parity * g.ncell * b.cell_cap;
   int carry = 0;
   for (int e0 = 0; e0 < st.nst; e0 += 64) {
     const int e = e0 + lane;
     int len = 0;
     if (e < st.nst) {
-      const int ox = st.lo + e % st.na, oy = st.lo + (e / st.na) % st.na,
-                oz = st.lo + e / (st.na * st.na);
+      const int ox = st.lo[0] + e % st.na[0], oy = st.lo[1] + (e / st.na[0]) % st.na[1],
+                oz = st.lo[2] + e / (st.na[0] * st.na[1]);
       bool keep = true;
-      if (st.prune) {
-        const double gx = cell_gap(ox, ux), gy = cell_gap(oy, uy), gz = cell_gap(oz, uz);
-        keep = gx * gx + gy * gy + gz * gz <= lim;
+      if (st.any) {  // the gap to the cell in A on the pruned axes
+        const double gx = st.prune[0] ? cell_gap(ox, ux) / bx.cellinv[0] : 0.0;
+        const double gy = st.prune[1] ? cell_gap(oy, uy) / bx.cellinv[1] : 0.0;
+        const double gz = st.prune[2] ? cell_gap(oz, uz) / bx.cellinv[2] : 0.0;
+        keep = gx * gx + gy * gy + gz * gz <= g.rr;
       }
-      const int c2 = ((((cz + oz) % nd + nd) % nd) * nd + ((cy + oy) % nd + nd) % nd) * nd +
-                     ((cx + ox) % nd + nd) % nd;
+      const int c2 = ((((cz + oz) % nz + nz) % nz) * ny + ((cy + oy) % ny + ny) % ny) * nx +
+                     ((cx + ox) % nx + nx) % nx;
       s_beg[e] = c2 * b.cell_cap + 1;
       if (keep) len = min(res[(size_t)c2 * b.cell_cap], b.cell_cap - 1);
     }
@@ -566,11 +585,11 @@ __device__ inline int build_row(const Geom &g, const BoxNow &bx, const MdBuf &b,
       if (j != i) {
         const double *recj = b.rec + (size_t)j * b.rs;
         double d0 = xi - recj[0], d1 = yi - recj[1], d2 = zi - recj[2];
-        const double k0 = round(d0 * bx.invL), k1 = round(d1 * bx.invL),
-                     k2 = round(d2 * bx.invL);
-        d0 = d0 - k0 * bx.L;
-        d1 = d1 - k1 * bx.L;
-        d2 = d2 - k2 * bx.L;
+        const double k0 = round(d0 * bx.invL[0]), k1 = round(d1 * bx.invL[1]),
+                     k2 = round(d2 * bx.invL[2]);
+        d0 = d0 - k0 * bx.L[0];
+        d1 = d1 - k1 * bx.L[1];
+        d2 = d2 - k2 * bx.L[2];
         const double dd = d0 * d0 + d1 * d1 + d2 * d2;
         pass = dd < g.rr;
         code = ((int)k0 + 512) | (((int)k1 + 512) << 10) | (((int)k2 + 512) << 20);
@@ -607,7 +626,11 @@ __device__ inline int build_row(const Geom &g, const BoxNow &bx, const MdBuf &b,
     b.com_ref[i] = xi;
     b.com_ref[n + i] = yi;
     b.com_ref[2 * n + i] = zi;
-    if (NPT && i == 0) b.box[3] = bx.L;  // the box the rows were built in
+    if (NPT && i == 0) {  // the box the rows were built in
+      b.box[9] = bx.L[0];
+      b.box[10] = bx.L[1];
+      b.box[11] = bx.L[2];
+    }
   }
   return m;
 }
@@ -692,9 +715,9 @@ __device__ __forceinline__ void md_main_body(const ModelTable &mt, const Geom &g
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         const int code = cc[u];
-        const double ox = (double)((code & 1023) - 512) * bx.L;
-        const double oy = (double)(((code >> 10) & 1023) - 512) * bx.L;
-        const double oz = (double)(((code >> 20) & 1023) - 512) * bx.L;
+        const double ox = (double)((code & 1023) - 512) * bx.L[0];
+        const double oy = (double)(((code >> 10) & 1023) - 512) * bx.L[1];
+        const double oz = (double)(((code >> 20) & 1023) - 512) * bx.L[2];
         const double d0 = (xi - xj[u]) - ox;
         const double d1 = (yi - yj[u]) - oy;
         const double d2 = (zi - zj[u]) - oz;
@@ -826,7 +849,7 @@ __global__ __launch_bounds__(256, (main_waves_per_eu<M, R>())) void k_md_main(Mo
     if (blockIdx.x == 0) {
       __shared__ double s_close[4][kRed];
       close_body<false>(mt, b, g.n, ca.dt, ca.nblk_pre, ca.bpart, ca.nblk_main, g.ncell,
-                        ca.nose_active, ca.mode, parity, g.L, s_close);
+                        ca.nose_active, ca.mode, parity, box_volume(g.L), s_close);
       return;
     }
     md_main_body<M, R, NPT>(mt, g, b, pk, out, parity, ca, reg_bytes, blockIdx.x - 1,
@@ -858,7 +881,7 @@ __global__ __launch_bounds__(256, (main_waves_per_eu<M, R>())) void k_md_main_la
     if (close_mode) {
       __shared__ double s_close[4][kRed];
       close_body<false>(mt, b, g.n, ca.dt, ca.nblk_pre, ca.bpart, ca.nblk_main, g.ncell,
-                        ca.nose_active, ca.mode, parity, g.L, s_close);
+                        ca.nose_active, ca.mode, parity, box_volume(g.L), s_close);
     }
     return;
   }
@@ -898,11 +921,11 @@ void md_launch_close(System &s, int mode, int parity, int src_parity) {
   if (s.andersen)
     k_md_close<true><<<1, 256, 0, s.stream>>>(s.mt, s.md, s.n, s.dt, ca.nblk_pre, ca.bpart,
                                               ca.nblk_main, s.geo_md.ncell, ca.nose_active, mode,
-                                              parity, s.L);
+                                              parity, box_volume(s.Lb));
   else
     k_md_close<false><<<1, 256, 0, s.stream>>>(s.mt, s.md, s.n, s.dt, ca.nblk_pre, ca.bpart,
                                                ca.nblk_main, s.geo_md.ncell, ca.nose_active,
-                                               mode, parity, s.L);
+                                               mode, parity, box_volume(s.Lb));
 }
 
 template <typename R>

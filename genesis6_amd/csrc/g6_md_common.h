@@ -104,26 +104,41 @@ __device__ inline void store_qc(const MdBuf &b, int n, int i, const double (&S)[
 // The box of this step: the fixed box of the grid, or the Andersen
 // barostat's current box in device memory (written by the step close).
 struct BoxNow {
-  double L, invL, cellinv;
+  double L[3], invL[3], cellinv[3];
 };
 // NPT: the barostat is on (kernels are instantiated both ways, so the fixed
 // box costs nothing)
 template <bool NPT>
 __device__ inline BoxNow box_now(const Geom &g, const MdBuf &b) {
-  if (NPT) return BoxNow{b.box[0], b.box[1], b.box[2]};
-  return BoxNow{g.L, g.invL, g.cellinv};
+  BoxNow r;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    r.L[a] = NPT ? b.box[a] : g.L[a];
+    r.invL[a] = NPT ? b.box[3 + a] : g.invL[a];
+    r.cellinv[a] = NPT ? b.box[6 + a] : g.cellinv[a];
+  }
+  return r;
 }
 
 
-// Andersen_Rescale, orthorhombic (AndersenModule.F90:65-82, BoxModule.F90:
-// 176-183): the cube takes the volume v0; box[2] is the MD grid's 1/cell.
+// Andersen_Rescale (AndersenModule.F90:65-82, BoxModule.F90:167-183): in
+// the orthorhombic mode every edge scales by (v0 / V)^(1/3), in the orz mode
+// the z edge alone by v0 / V; then the inverses and the MD grid's 1/cell.
 __device__ inline void andersen_rescale(const double *a, double *box) {
-  double L = box[0];
-  const double r2 = pow(a[2] / ((L * L) * L), 1.0 / 3.0);
-  L = L * r2;
-  box[0] = L;
-  box[1] = 1.0 / L;
-  box[2] = 1.0 / (L / box[4]);
+  const double vol = box_volume(box);
+  if (box[16] == 2.0) {
+    box[2] = box[2] * (a[2] / vol);
+  } else {
+    const double r2 = pow(a[2] / vol, 1.0 / 3.0);
+    box[0] = box[0] * r2;
+    box[1] = box[1] * r2;
+    box[2] = box[2] * r2;
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    box[3 + k] = 1.0 / box[k];
+    box[6 + k] = 1.0 / (box[k] / box[12 + k]);
+  }
 }
 
 // ---- step close: one block of 256 threads ---------------------------------
@@ -136,7 +151,7 @@ __device__ inline void andersen_rescale(const double *a, double *box) {
 template <bool NPT>
 __device__ inline void close_body(const ModelTable &mt, const MdBuf &b, int n, double dt,
                                   int nblk_pre, const double *bpart, int nblk_main, int ncell,
-                                  int nose_active, int mode, int parity, double box_l,
+                                  int nose_active, int mode, int parity, double box_vol,
                                   double (&s_red)[4][kRed]) {
   constexpr int NR = 12 + kEkLen;
   const int wid = threadIdx.x >> 6;
@@ -245,8 +260,7 @@ __device__ inline void close_body(const ModelTable &mt, const MdBuf &b, int n, d
                             r[18] * mt.mass * kd, r[20] * mt.mass * kd, r[21] * mt.mass * kd};
       for (int q = 0; q < 9; q++) th[16 + q] = kt[q];
       const double vp = (kt[0] + th[4]) + (kt[4] + th[8]) + (kt[8] + th[12]);
-      const double lb = NPT ? b.box[0] : box_l;
-      const double vol = (lb * lb) * lb;
+      const double vol = NPT ? box_volume(b.box) : box_vol;
       const double press =
           (vp / 3.0) * 10.0 * (1.0 / 101326.0) / (vol * 6.0225e23 * (1e-10 * 1e-10 * 1e-10));
       th[25] = press;
@@ -263,8 +277,7 @@ __device__ inline void close_body(const ModelTable &mt, const MdBuf &b, int n, d
         a[6] = a[6] + kG5_4 * cvol;
         a[7] = a[7] + kG5_5 * cvol;
         andersen_rescale(a, b.box);
-        const double lc = b.box[0];
-        volc = (lc * lc) * lc;
+        volc = box_volume(b.box);
         // barostat energies of the log line (Mixture.F90:2436-2439), J/mol
         and_ep = a[0] * a[2] * 101326.0 * 6.0225e23 * (1e-10 * 1e-10 * 1e-10);
         and_ek = a[3] * a[3] * a[1] * 0.5 * 101326.0 * 6.0225e23 * (1e-10 * 1e-10 * 1e-10);

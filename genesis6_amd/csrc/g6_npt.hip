@@ -1174,6 +1174,7 @@ int g6_npt_begin(g6_handle h, double temp, double press_bar, uint64_t seed, doub
     return G6_EARG;
   if (s->n > 4096 || s->n < 2) return G6_EARG;
   if (s->model == G6_OPLSMEOH) return G6_EARG;  // the surrogate's features are water's
+  if (!is_cubic(*s)) return G6_EARG;            // volume moves scale a cube
   if (s->rc > 0.5 * s->L * (1.0 - dlnv_max)) return G6_EARG;  // one image within rc
   Npt &p = *npt_of(s, true);
   G6_CHECK(hipStreamSynchronize(s->stream));

@@ -95,11 +95,11 @@ __global__ __launch_bounds__(256) void k_sort_cells(const int *start,
   }
 }
 
-__device__ inline int nbr_cell(const Geom &g, int c, int packed) {
-  const int nd = g.ndiv;
-  const int cx = c % nd, cy = (c / nd) % nd, cz = c / (nd * nd);
+__device__ __host__ inline int nbr_cell(const Geom &g, int c, int packed) {
+  const int nx = g.nd[0], ny = g.nd[1], nz = g.nd[2];
+  const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
   const int ox = packed & 0xff, oy = (packed >> 8) & 0xff, oz = (packed >> 16) & 0xff;
-  return (((cz + oz) % nd) * nd + (cy + oy) % nd) * nd + (cx + ox) % nd;
+  return (((cz + oz) % nz) * ny + (cy + oy) % ny) * nx + (cx + ox) % nx;
 }
 
 // Shift code: three 10-bit fields holding k + 512 for ox = k*L, k in
@@ -164,11 +164,11 @@ __global__ __launch_bounds__(256) void k_rows(Geom g, const double *com,
         j = sorted[s_beg[wid][s] + (t - s_pre[wid][s])];
         if (part == 0 || j > i) {
           double d0 = xi - com[j], d1 = yi - com[n + j], d2 = zi - com[2 * n + j];
-          const double k0 = round(d0 * g.invL), k1 = round(d1 * g.invL),
-                       k2 = round(d2 * g.invL);
-          d0 = d0 - k0 * g.L;
-          d1 = d1 - k1 * g.L;
-          d2 = d2 - k2 * g.L;
+          const double k0 = round(d0 * g.invL[0]), k1 = round(d1 * g.invL[1]),
+                       k2 = round(d2 * g.invL[2]);
+          d0 = d0 - k0 * g.L[0];
+          d1 = d1 - k1 * g.L[1];
+          d2 = d2 - k2 * g.L[2];
           const double dd = d0 * d0 + d1 * d1 + d2 * d2;
           pass = dd < g.rr;
           if (MODE == 1 && pass) {
@@ -204,15 +204,15 @@ __global__ void k_bin_only(Geom g, const double *com, int *cell, int *count) {
   int jc[3];
   for (int a = 0; a < 3; a++) {
     double d = com[a * n + i];
-    const double o = round(d * g.invL) * g.L;
+    const double o = round(d * g.invL[a]) * g.L[a];
     d = d - o;
-    int j = (int)round(d * g.cellinv);
-    if (j < 0) j += g.ndiv;
+    int j = (int)round(d * g.cellinv[a]);
+    if (j < 0) j += g.nd[a];
     // the reference indexes out of range here only for ndiv = 1
-    j = ((j % g.ndiv) + g.ndiv) % g.ndiv;
+    j = ((j % g.nd[a]) + g.nd[a]) % g.nd[a];
     jc[a] = j;
   }
-  const int c = (jc[2] * g.ndiv + jc[1]) * g.ndiv + jc[0];
+  const int c = (jc[2] * g.nd[1] + jc[1]) * g.nd[0] + jc[0];
   cell[i] = c;
   atomicAdd(&count[c], 1);
 }
@@ -237,14 +237,14 @@ int cells_once(System &s, const double *com_h, int32_t *cell_h) {
   return 0;
 }
 
-__global__ void k_unpack_shifts(const int *code, int np, double L, double *ox,
-                                double *oy, double *oz) {
+__global__ void k_unpack_shifts(const int *code, int np, double Lx, double Ly, double Lz,
+                                double *ox, double *oy, double *oz) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= np) return;
   const int v = code[k];
-  ox[k] = shift_value(v & 1023, L);
-  oy[k] = shift_value((v >> 10) & 1023, L);
-  oz[k] = shift_value((v >> 20) & 1023, L);
+  ox[k] = shift_value(v & 1023, Lx);
+  oy[k] = shift_value((v >> 10) & 1023, Ly);
+  oz[k] = shift_value((v >> 20) & 1023, Lz);
 }
 
 int pairlist_once(System &s, const double *com_h, int64_t cap, int64_t *npair0,
@@ -295,12 +295,7 @@ int pairlist_once(System &s, const double *com_h, int64_t cap, int64_t *npair0,
     G6_CHECK(hipMemcpy(cl.data(), s.cell, sizeof(int) * n, hipMemcpyDeviceToHost));
     G6_CHECK(hipMemcpy(so.data(), s.sorted, sizeof(int) * n, hipMemcpyDeviceToHost));
     int64_t c0 = 0;
-    auto nb_host = [&](int c, int packed) {
-      const int nd = g.ndiv;
-      const int cx = c % nd, cy = (c / nd) % nd, cz = c / (nd * nd);
-      const int ox = packed & 0xff, oy = (packed >> 8) & 0xff, oz = (packed >> 16) & 0xff;
-      return (((cz + oz) % nd) * nd + (cy + oy) % nd) * nd + (cx + ox) % nd;
-    };
+    auto nb_host = [&](int c, int packed) { return nbr_cell(g, c, packed); };
     for (int r = 0; r < n; r++) {
       const int i = so[r], c = cl[i];
       for (int t = 0; t < g.nst; t++) {
@@ -325,7 +320,7 @@ int pairlist_once(System &s, const double *com_h, int64_t cap, int64_t *npair0,
     if (ro != 0) return ro;
     double *o = reinterpret_cast<double *>(ob);
     k_unpack_shifts<<<(total + 255) / 256, 256, 0, s.stream>>>(
-        s.scode, total, g.L, o, o + total, o + 2 * (size_t)total);
+        s.scode, total, g.L[0], g.L[1], g.L[2], o, o + total, o + 2 * (size_t)total);
     G6_CHECK(hipMemcpyAsync(pi_h, s.pi, sizeof(int) * total, hipMemcpyDeviceToHost, s.stream));
     G6_CHECK(hipMemcpyAsync(pj_h, s.pj, sizeof(int) * total, hipMemcpyDeviceToHost, s.stream));
     G6_CHECK(hipMemcpyAsync(ox_h, o, sizeof(double) * total, hipMemcpyDeviceToHost, s.stream));

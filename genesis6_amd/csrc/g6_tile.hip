@@ -129,7 +129,7 @@ __device__ __forceinline__ void tile_body(const Geom &g, const MdBuf &b, const P
   const int W = blockDim.x >> 6;
   const int S = b.slices;
   const int tile = blk / S, sl = blk - tile * S;
-  const int nd = g.ndiv, n = g.n;
+  const int nd = g.nd[0], n = g.n;  // tiles run cubic boxes
   const int tx = tile % nd, ty = (tile / nd) % nd, tz = tile / (nd * nd);
   const int nbmax = lay.nbmax;
   int *s_id = reinterpret_cast<int *>(s_dyn + lay.id);
@@ -266,7 +266,7 @@ __device__ __forceinline__ void tile_body(const Geom &g, const MdBuf &b, const P
   park();
 
   // ---- 3. rows, interaction test, dense pair rounds ----
-  const double Lb = g.L, invL = g.invL, rr = g.rr, rc2 = g.rc2;
+  const double Lb = g.L[0], invL = g.invL[0], rr = g.rr, rc2 = g.rc2;
   const double fmax = 33554432.0 / (1.0 + sqrt(rc2));
   const float m2max = (float)(fmax * fmax);
   // each lane's energy and virial over its own pairs (a few per step) in
@@ -519,7 +519,7 @@ __global__ __launch_bounds__(512) void k_md_tile(ModelTable mt, Geom g, MdBuf b,
     if (blockIdx.x == 0) {
       __shared__ double s_close[4][kRed];
       close_body<false>(mt, b, g.n, ca.dt, ca.nblk_pre, ca.bpart, ca.nblk_main, g.ncell,
-                        ca.nose_active, ca.mode, parity, g.L, s_close);
+                        ca.nose_active, ca.mode, parity, box_volume(g.L), s_close);
       return;
     }
     tile_body<M, STAGE>(g, b, pk, parity, lay, blockIdx.x - 1, gridDim.x - 1);

@@ -28,6 +28,15 @@ module g6gpu
        real(c_double), value :: box_l, rc, taper, voxa, dt, skin
      end function g6_init
 
+     integer(c_int) function g6_init_box3(h, model, n, box3, rc, taper, voxa3, dt, prec, skin) &
+          bind(C, name="g6_init_box3")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), intent(out) :: h
+       integer(c_int), value :: model, n, prec
+       real(c_double), intent(in) :: box3(3), voxa3(3)
+       real(c_double), value :: rc, taper, dt, skin
+     end function g6_init_box3
+
      integer(c_int) function g6_finalize(h) bind(C, name="g6_finalize")
        import :: c_int, c_int64_t
        integer(c_int64_t), value :: h
@@ -266,6 +275,12 @@ module g6gpu
        integer(c_int64_t), value :: h
        real(c_double), intent(out) :: box_l
      end function g6_get_box
+
+     integer(c_int) function g6_get_box3(h, edges) bind(C, name="g6_get_box3")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), value :: h
+       real(c_double), intent(out) :: edges(3)
+     end function g6_get_box3
 
      integer(c_int) function g6_lattice_start(n, box_l, seed, com, quat1) &
           bind(C, name="g6_lattice_start")

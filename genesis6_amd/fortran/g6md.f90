@@ -4,10 +4,11 @@
 ! reference runs here unchanged and the two logs can be compared line by line.
 !
 ! Tags read: @DTPS dt | @MDLP steps | @NLOG interval | @RCOA rc taper |
-! @BXLA L | @BOX3 Lx Ly Lz (a cube: the three equal) | @VOXA vx vy vz |
+! @BXLA L | @BOX3 Lx Ly Lz (an orthorhombic box) | @VOXA vx vy vz |
 ! @VOXN nx ny nz | @BOOK interval margin | @VSET T seed |
 ! @NOS2 T q dump z0..z4 zsum | @NOSE T q dump z0..z4 (the older form) |
-! @PCN4 mode pex mass (Andersen barostat, mode 1) | @PCN2 (refused, as
+! @PCN4 mode pex mass (Andersen barostat: mode 1 orthorhombic, every edge
+! following the volume; mode 2 orz, the z edge alone) | @PCN2 (refused, as
 ! the reference refuses it) | @FAST | @SNPO | @SNPV |
 ! @ID08 model | @NX4A n, then n lines x y z a b c d.
 ! Command line: g6md [fp64|fp32] [skin]   (default fp64, skin 0: the
@@ -36,9 +37,9 @@ program g6md
   use g6gpu
   implicit none
   character(len=256) :: line, tag, arg
-  real(c_double) :: dt = 0.002d0, rc = 8.5d0, taper = 0.5d0, box = 0d0, voxa(3) = 0d0
+  real(c_double) :: dt = 0.002d0, rc = 8.5d0, taper = 0.5d0, voxa(3) = 0d0
   real(c_double) :: vtemp = -1d0, skin = 0d0, nose(9), th(G6_THERMO_LEN)
-  real(c_double) :: book_margin = 0d0, box3(3)
+  real(c_double) :: book_margin = 0d0, box3(3) = 0d0, bxla
   integer :: book_interval = 0, voxn(3) = 0
   logical :: skin_given = .false.
   integer :: nsteps = 0, nlog = 0, ios, i, done, chunk, target
@@ -91,15 +92,11 @@ program g6md
      case ('@RCOA')
         read(5, *) rc, taper
      case ('@BXLA')
-        read(5, *) box
+        read(5, *) bxla
+        box3 = bxla
      case ('@BOX3')
         ! Box_ReadBOX3 (BoxModule.F90:109-116): three edges
         read(5, *) box3
-        if (box3(2) /= box3(1) .or. box3(3) /= box3(1)) then
-           write(0, '(a)') 'g6md: @BOX3 with unequal edges: the GPU path runs cubic boxes'
-           error stop 2
-        end if
-        box = box3(1)
      case ('@VOXA')
         read(5, *) voxa
         voxn = 0
@@ -163,15 +160,15 @@ program g6md
         error stop 2
      end select
   end do
-  if (model < 0 .or. n < 2 .or. box <= 0d0) then
+  if (model < 0 .or. n < 2 .or. any(box3 <= 0d0)) then
      write(0, '(a)') 'g6md: the deck needs @ID08, @NX4A and @BXLA'
      error stop 2
   end if
-  if (voxn(1) > 0) voxa = box / voxn(1)
+  if (voxn(1) > 0) voxa = box3 / voxn
   if (voxa(1) <= 0d0) voxa = rc
   if (.not. skin_given .and. book_interval > 0) skin = book_margin
 
-  call g6_check(g6_init(h, model, n, box, rc, taper, voxa(1), dt, prec, skin), 'g6_init')
+  call g6_check(g6_init_box3(h, model, n, box3, rc, taper, voxa, dt, prec, skin), 'g6_init_box3')
   ! @FAST (default off, Mixture.F90:357, 732-736): StdForce or the custom kernels
   call g6_check(g6_set_fast(h, fast), 'g6_set_fast')
   call g6_check(g6_upload_state(h, com, quat, dx, w), 'g6_upload_state')
@@ -204,7 +201,7 @@ program g6md
      done = done + chunk
      if (due(snpo) .or. due(snpv)) then
         call g6_check(g6_download_state(h, com, quat, dx, w), 'g6_download_state')
-        call g6_check(g6_get_box(h, box), 'g6_get_box')
+        call g6_check(g6_get_box3(h, box3), 'g6_get_box3')
         if (due(snpo)) call write_snapshot(nstep0 + done - 1, .false.)
         if (due(snpv)) call write_snapshot(nstep0 + done - 1, .true.)
      end if
@@ -216,7 +213,7 @@ program g6md
   call g6_check(g6_download_state(h, com, quat, dx, w), 'g6_download_state')
   if (has_nose) call g6_check(g6_get_nose(h, nose), 'g6_get_nose')
   if (amode /= 0) call g6_check(g6_get_andersen(h, astate), 'g6_get_andersen')
-  call g6_check(g6_get_box(h, box), 'g6_get_box')
+  call g6_check(g6_get_box3(h, box3), 'g6_get_box3')
   call g6_check(g6_finalize(h), 'g6_finalize')
   call write_restart()
 
@@ -227,7 +224,7 @@ contains
     character(len=5) :: btag
     character(len=8) :: id
     integer :: ib, nt, nb, m, c, k
-    real(8) :: d3(3), dbl, d9(9)
+    real(8) :: dbl, d9(9)
     open(12, file='fort.12', form='unformatted', access='sequential', status='old')
     do
        read(12, iostat=ios) btag
@@ -236,10 +233,10 @@ contains
        case ('@RCOA')
           read(12) rc, taper             ! co%out, co%out - co%in
        case ('@BOX3')
-          read(12) d3
-          box = d3(1)
+          read(12) box3
        case ('@BXLA')
-          read(12) box
+          read(12) bxla
+          box3 = bxla
        case ('@BOOK')
           read(12) book_interval, book_margin
        case ('@VOXN')
@@ -344,7 +341,7 @@ contains
     write(13) '@RCOA'
     write(13) rc, taper
     write(13) '@BOX3'
-    write(13) box, box, box
+    write(13) box3
     write(13) '@BOOK'
     write(13) 1, skin
     write(13) '@VOXA'
@@ -411,7 +408,7 @@ contains
     write(6, '("@ASTP")')
     write(6, *) step
     write(6, '("@BOX3")')
-    write(6, *) box, box, box
+    write(6, *) box3(1), box3(2), box3(3)
     write(6, '("@NCMP")')
     write(6, *) 1
     if (.not. velocities) then

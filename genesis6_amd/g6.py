@@ -79,6 +79,9 @@ def lib():
         L.g6_set_andersen.argtypes = [_H, C.c_int, C.c_double, C.c_double, C.c_void_p]
         L.g6_get_andersen.argtypes = [_H, _dp]
         L.g6_get_box.argtypes = [_H, C.POINTER(C.c_double)]
+        L.g6_get_box3.argtypes = [_H, _dp]
+        L.g6_init_box3.argtypes = [C.POINTER(_H), C.c_int, C.c_int, _dp, C.c_double, C.c_double,
+                                   _dp, C.c_double, C.c_int, C.c_double]
         tab = [_ip, _ip, _dp, _dp, _dp, _dp]
         L.g6_neighbors.argtypes = [_H, C.c_int, C.c_int64, _ip, _ip, _dp, _dp, _dp, _dp,
                                    C.c_int, C.c_int, C.c_double] + tab
@@ -148,11 +151,18 @@ class System:
     """One rigid-water system resident on the GPU (a g6_handle)."""
 
     def __init__(self, model, n, L, rc, taper, dt, voxa=None, prec=FP64, skin=0.0):
+        """L: the cubic box edge, or three edges of an orthorhombic box
+        (@BOX3); voxa: the requested cell edge (default rc), or three."""
         self.model, self.n, self.L, self.rc, self.taper, self.dt = model, n, L, rc, taper, dt
         self.nsite = lib().g6_model_nsite(model)
         h = _H(0)
-        check(lib().g6_init(C.byref(h), model, n, L, rc, taper, voxa if voxa else rc, dt,
-                            prec, skin), "g6_init")
+        if np.ndim(L) == 0:
+            check(lib().g6_init(C.byref(h), model, n, float(L), rc, taper, voxa if voxa else rc,
+                                dt, prec, skin), "g6_init")
+        else:
+            v = np.broadcast_to(np.asarray(voxa if voxa else rc, np.float64), (3,))
+            check(lib().g6_init_box3(C.byref(h), model, n, _c(L), rc, taper, _c(v), dt, prec,
+                                     skin), "g6_init_box3")
         self.h = h.value
 
     def close(self):
@@ -232,6 +242,11 @@ class System:
         L = C.c_double(0)
         check(lib().g6_get_box(self.h, C.byref(L)), "g6_get_box")
         return L.value
+
+    def box3(self):
+        e = np.zeros(3)
+        check(lib().g6_get_box3(self.h, e), "g6_get_box3")
+        return e
 
     def md_kernel(self):
         """(tiles, cells per axis, workgroups per tile, waves, slots, capacity

@@ -61,6 +61,14 @@ enum {
  * skin/2 and filters dd < rc^2 in FP64 every step (same interacting set). */
 int g6_init(g6_handle *h, int model, int n, double box_l, double rc,
             double taper, double voxa, double dt, int prec, double skin);
+/* g6_init with an orthorhombic box: edges box3[3] (@BOX3, BoxModule.F90:
+ * 109-116) and a requested cell edge per axis voxa3[3] (@VOXA x y z; the
+ * reference grid divides each axis on its own, GridModule.F90:494-499).  The
+ * MD step, the seam entry points and both barostat modes take such boxes;
+ * the tile kernel, ladders, Monte Carlo and NPT Monte Carlo take cubic boxes
+ * (G6_EARG otherwise; g6_step runs the rows of both directions). */
+int g6_init_box3(g6_handle *h, int model, int n, const double *box3, double rc,
+                 double taper, const double *voxa3, double dt, int prec, double skin);
 int g6_finalize(g6_handle h);
 
 /* Rigid state (RigidModule.F90:14-33 sRigid1, internal units).  com[3n];
@@ -119,7 +127,8 @@ int g6_eval_forces_noalloc(g6_handle h, double *force, double *torque, double *t
 int g6_pairlist(g6_handle h, const double *com, int64_t cap, int64_t *npair0,
                 int64_t *npair, int32_t *pair_i, int32_t *pair_j, double *ox,
                 double *oy, double *oz);
-/* Address_Assign cell of each molecule (0-based mAddr-1) and the grid. */
+/* Address_Assign cell of each molecule (0-based mAddr-1, mAddr = (z ny + y)
+ * nx + x + 1) and the cells per axis of x (*ndiv; every axis of a cube). */
 int g6_cells(g6_handle h, const double *com, int32_t *cell, int32_t *ndiv);
 /* CutOff_Dimmer2 (CutoffModule.F90:220-278) for a given pair list. */
 int g6_cutoff(g6_handle h, int64_t npair, const int32_t *pair_i,
@@ -236,9 +245,11 @@ int g6_triangle_displacement(g6_handle h, int nsite, int64_t npair,
                              double *hist, double *op);
 
 /* ---- constant-pressure MD: the Andersen barostat (AndersenModule.F90) ---
- * mode 1 (orthorhombic: the cubic box follows the volume, the reference's
- * only mode for a cube) with external pressure pex (atm) and mass per
- * molecule as @PCN4 (Mixture.F90:1207-1231); mode 0 switches it off.
+ * mode 1 (orthorhombic: every edge scales by (v0/V)^(1/3)) or mode 2 (orz:
+ * the z edge alone takes the volume, and the dilation drag of the corrector
+ * acts on z alone, RigidModule.F90:1290-1306) with external pressure pex
+ * (atm) and mass per molecule as @PCN4 (Mixture.F90:1207-1231); mode 0
+ * switches it off.
  * v: NULL starts the barostat (Andersen_Initialize: v0 = the box volume),
  * or v[0..5] = v0..v5 continue a saved state (@PCN5).  g6_step then runs
  * Andersen_Predict before the forces and Andersen_Correct with the step's
@@ -251,8 +262,10 @@ int g6_set_andersen(g6_handle h, int mode, double pex, double mass,
                     const double *v);
 /* pex, mass (per molecule, as @PCN4), v0..v5: the @PCN5 payload */
 int g6_get_andersen(g6_handle h, double *state8);
-/* the current box edge (A) */
+/* the current box edge (A; the x edge of an orthorhombic box) */
 int g6_get_box(g6_handle h, double *box_l);
+/* the current box edges edges[3] (A) */
+int g6_get_box3(g6_handle h, double *edges);
 
 /* ---- Monte Carlo: PairInfo and the mc2 trial loop ------------------- */
 /* Starts an MC run on the handle's current configuration (COMs and order-1

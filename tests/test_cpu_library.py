@@ -76,4 +76,4 @@ def test_fortran_driver_reports_a_missing_device():
     deck = "@BXLA\n18.626\n@ID08\nTIP4P\n@NX4A\n2\n0 0 0 1 0 0 0\n3 0 0 1 0 0 0\n"
     r = subprocess.run([exe], input=deck, capture_output=True, text=True, timeout=60)
     assert r.returncode != 0
-    assert "g6_init failed with code -4" in r.stderr
+    assert "g6_init_box3 failed with code -4" in r.stderr
