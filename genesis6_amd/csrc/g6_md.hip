@@ -69,10 +69,11 @@ __device__ inline void part_sum(const MdBuf &b, int i, double (&f)[3]) {
 // split changes no bit of the result.  The roles are wave-uniform, so no
 // wave executes another role's code.
 // blk is the block's index within its system (blockIdx.x for one system).
-template <bool NPT>
+template <int BOX>
 __device__ __forceinline__ void md_pre_body(const ModelTable &mt, const Geom &g, const MdBuf &b,
                                             double dt, int nose_active, int do_correct,
                                             int do_predict, int do_sites, int parity, int blk) {
+  constexpr bool NPT = BOX == kBoxBaro;
   __shared__ double s_w0[3][64], s_W[12][64];
   __shared__ double s_q0[4][64], s_p[4][64], s_r[4][64], s_s[4][64], s_qn[4][64];
   const int n = g.n;
@@ -98,7 +99,7 @@ __device__ __forceinline__ void md_pre_body(const ModelTable &mt, const Geom &g,
   const double nc = zeta0 * dt, exr = nc;
   const double ext[3] = {nc * 0.5 + (orz ? 0.0 : acoeff), nc * 0.5 + (orz ? 0.0 : acoeff),
                          nc * 0.5 + acoeff};
-  const BoxNow bx = box_now<NPT>(g, b);
+  const BoxNow bx = box_now<BOX>(g, b);
   // every global load is issued here, ahead of the first barrier
   double F[3] = {0, 0, 0}, t[9], tl[3] = {0, 0, 0}, cref[3] = {0, 0, 0};
 #pragma unroll
@@ -343,18 +344,19 @@ __device__ __forceinline__ void md_pre_body(const ModelTable &mt, const Geom &g,
       rec[0] = S[0];
       rec[1] = S[1];
       rec[2] = S[2];
-      int jc[3];
+      int jc[3], nd[3];
+      grid_nd<BOX>(g, nd);
 #pragma unroll
       for (int a = 0; a < 3; a++) {
         double d = S[a];
         const double o = round(d * bx.invL[a]) * bx.L[a];
         d = d - o;
         int j = (int)round(d * bx.cellinv[a]);
-        if (j < 0) j += g.nd[a];
-        j = ((j % g.nd[a]) + g.nd[a]) % g.nd[a];
+        if (j < 0) j += nd[a];
+        j = ((j % nd[a]) + nd[a]) % nd[a];
         jc[a] = j;
       }
-      const int c = (jc[2] * g.nd[1] + jc[1]) * g.nd[0] + jc[0];
+      const int c = (jc[2] * nd[1] + jc[1]) * nd[0] + jc[0];
       b.cell[i] = c;
       // slot 0 of a cell row is its count; residents follow
       int *cr = b.cellres + ((size_t)parity * g.ncell + c) * b.cell_cap;
@@ -437,11 +439,11 @@ __device__ __forceinline__ void md_pre_body(const ModelTable &mt, const Geom &g,
   }
 }
 
-template <bool NPT>
+template <int BOX>
 __global__ __launch_bounds__(256) void k_md_pre(ModelTable mt, Geom g, MdBuf b, double dt,
                                                 int nose_active, int do_correct, int do_predict,
                                                 int do_sites, int parity) {
-  md_pre_body<NPT>(mt, g, b, dt, nose_active, do_correct, do_predict, do_sites, parity,
+  md_pre_body<BOX>(mt, g, b, dt, nose_active, do_correct, do_predict, do_sites, parity,
                    blockIdx.x);
 }
 
@@ -455,7 +457,7 @@ __global__ __launch_bounds__(256) void k_md_pre_lad(ModelTable mt, const MdRep *
   if (m.x < 0) return;
   const Geom g = reps[m.x].g;
   const MdBuf b = reps[m.x].b;
-  md_pre_body<false>(mt, g, b, reps[m.x].dt, reps[m.x].nose_active, do_correct, do_predict,
+  md_pre_body<kBoxCube>(mt, g, b, reps[m.x].dt, reps[m.x].nose_active, do_correct, do_predict,
                      do_sites,
               parity, m.y);
 }
@@ -523,15 +525,18 @@ __device__ inline double frac_in_cell(const BoxNow &bx, double x, int a) {
 // neighbour-row build of molecule i into nbr[i*maxnb ...], sorted by j so
 // that the force sums do not depend on the order in which the binning
 // atomics filled the cells; returns the row length (also stored).
-template <bool NPT>
-__device__ inline int build_row(const Geom &g, const BoxNow &bx, const MdBuf &b, int parity,
-                                int i, int lane, int (&s_beg)[128], int (&s_pre)[132], int *s_j,
-                                int *s_c) {
+template <int BOX>
+__device__ inline int build_row(const Geom &g, const MdBuf &b, int parity, int i, int lane,
+                                int (&s_beg)[128], int (&s_pre)[132], int *s_j, int *s_c) {
+  constexpr bool NPT = BOX == kBoxBaro;
+  const BoxNow bx = box_now<BOX>(g, b);
   const int n = g.n;
-  const int nx = g.nd[0], ny = g.nd[1], nz = g.nd[2];
+  int nd[3];
+  grid_nd<BOX>(g, nd);
+  const int nx = nd[0], ny = nd[1], nz = nd[2];
   const int c = b.cell[i];
   const int cx = c % nx, cy = (c / nx) % ny, cz = c / (nx * ny);
-  const Stencil st = stencil_of(g.nd);
+  const Stencil st = stencil_of(nd);
   const double *reci = b.rec + (size_t)i * b.rs;
   const double xi = reci[0], yi = reci[1], zi = reci[2];
   const double ux = frac_in_cell(bx, xi, 0), uy = frac_in_cell(bx, yi, 1),
@@ -545,7 +550,10 @@ __device__ inline int build_row(const Geom &g, const BoxNow &bx, const MdBuf &b,
       const int ox = st.lo[0] + e % st.na[0], oy = st.lo[1] + (e / st.na[0]) % st.na[1],
                 oz = st.lo[2] + e / (st.na[0] * st.na[1]);
       bool keep = true;
-      if (st.any) {  // the gap to the cell in A on the pruned axes
+      if (BOX == kBoxCube && st.any) {  // in cell units of the cube
+        const double gx = cell_gap(ox, ux), gy = cell_gap(oy, uy), gz = cell_gap(oz, uz);
+        keep = gx * gx + gy * gy + gz * gz <= g.rr * (bx.cellinv[0] * bx.cellinv[0]);
+      } else if (st.any) {  // the gap to the cell in A on the pruned axes
         const double gx = st.prune[0] ? cell_gap(ox, ux) / bx.cellinv[0] : 0.0;
         const double gy = st.prune[1] ? cell_gap(oy, uy) / bx.cellinv[1] : 0.0;
         const double gz = st.prune[2] ? cell_gap(oz, uz) / bx.cellinv[2] : 0.0;
@@ -637,7 +645,7 @@ __device__ inline int build_row(const Geom &g, const BoxNow &bx, const MdBuf &b,
 
 // blk is the block's index within its system and nblk the system's block
 // count (blockIdx.x and gridDim.x for one system).
-template <int M, typename R, bool NPT>
+template <int M, typename R, int BOX>
 __device__ __forceinline__ void md_main_body(const ModelTable &mt, const Geom &g, const MdBuf &b,
                                              const PairK<R> &pk, double *out, int parity,
                                              const CloseArgs &ca, int reg_bytes, int blk,
@@ -650,7 +658,7 @@ __device__ __forceinline__ void md_main_body(const ModelTable &mt, const Geom &g
   const int n = g.n;
   const int i = blk * 4 + wid;
   const int cap = b.maxnb;
-  const BoxNow bx = box_now<NPT>(g, b);
+  const BoxNow bl = box_now<BOX>(g, b);
   unsigned char *reg = s_dyn + (size_t)wid * reg_bytes;
   double ep = 0, v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   int npair = 0, nlist = 0;  // wave-uniform counts
@@ -664,7 +672,7 @@ __device__ __forceinline__ void md_main_body(const ModelTable &mt, const Geom &g
     int m;
     if (b.flags[2 + parity]) {
       int *sj = reinterpret_cast<int *>(reg);
-      m = build_row<NPT>(g, bx, b, parity, i, lane, s_beg[wid], s_pre[wid], sj, sj + cap);
+      m = build_row<BOX>(g, b, parity, i, lane, s_beg[wid], s_pre[wid], sj, sj + cap);
       nlist = m;
     } else {
       m = b.nbr_cnt[i];
@@ -715,9 +723,9 @@ __device__ __forceinline__ void md_main_body(const ModelTable &mt, const Geom &g
 #pragma unroll
       for (int u = 0; u < 4; u++) {
         const int code = cc[u];
-        const double ox = (double)((code & 1023) - 512) * bx.L[0];
-        const double oy = (double)(((code >> 10) & 1023) - 512) * bx.L[1];
-        const double oz = (double)(((code >> 20) & 1023) - 512) * bx.L[2];
+        const double ox = (double)((code & 1023) - 512) * bl.L[0];
+        const double oy = (double)(((code >> 10) & 1023) - 512) * bl.L[1];
+        const double oz = (double)(((code >> 20) & 1023) - 512) * bl.L[2];
         const double d0 = (xi - xj[u]) - ox;
         const double d1 = (yi - yj[u]) - oy;
         const double d2 = (zi - zj[u]) - oz;
@@ -840,7 +848,7 @@ constexpr int main_waves_per_eu() {
   return (M != G6_ST2 && M != G6_NVDE && sizeof(R) == 4) ? 4 : 1;
 }
 
-template <int M, typename R, bool NPT>
+template <int M, typename R, int BOX>
 __global__ __launch_bounds__(256, (main_waves_per_eu<M, R>())) void k_md_main(ModelTable mt, Geom g, MdBuf b, PairK<R> pk,
                                                  double *out, int parity, CloseArgs ca,
                                                  int reg_bytes) {
@@ -852,10 +860,10 @@ __global__ __launch_bounds__(256, (main_waves_per_eu<M, R>())) void k_md_main(Mo
                         ca.nose_active, ca.mode, parity, box_volume(g.L), s_close);
       return;
     }
-    md_main_body<M, R, NPT>(mt, g, b, pk, out, parity, ca, reg_bytes, blockIdx.x - 1,
+    md_main_body<M, R, BOX>(mt, g, b, pk, out, parity, ca, reg_bytes, blockIdx.x - 1,
                             gridDim.x - 1);
   } else {
-    md_main_body<M, R, NPT>(mt, g, b, pk, out, parity, ca, reg_bytes, blockIdx.x, gridDim.x);
+    md_main_body<M, R, BOX>(mt, g, b, pk, out, parity, ca, reg_bytes, blockIdx.x, gridDim.x);
   }
 }
 
@@ -885,7 +893,7 @@ __global__ __launch_bounds__(256, (main_waves_per_eu<M, R>())) void k_md_main_la
     }
     return;
   }
-  md_main_body<M, R, false>(mt, g, b, pk, nullptr, parity, ca, reg_bytes, m.y, nm);
+  md_main_body<M, R, kBoxCube>(mt, g, b, pk, nullptr, parity, ca, reg_bytes, m.y, nm);
 }
 
 __global__ void k_flag_set(int *flags, int k, int v) {
@@ -895,14 +903,17 @@ __global__ void k_flag_set(int *flags, int k, int v) {
 void md_launch_flag(System &s, int k, int v) { k_flag_set<<<1, 64, 0, s.stream>>>(s.md.flags, k, v); }
 
 void md_launch_pre(System &s, bool correct, bool predict, bool sites, int parity) {
+  const int nb = pre_waves(s.n) / 4, na = s.nose_active ? 1 : 0;
   if (s.andersen)
-    k_md_pre<true><<<pre_waves(s.n) / 4, 256, 0, s.stream>>>(
-        s.mt, s.geo_md, s.md, s.dt, s.nose_active ? 1 : 0, correct ? 1 : 0, predict ? 1 : 0,
-        sites ? 1 : 0, parity);
+    k_md_pre<kBoxBaro><<<nb, 256, 0, s.stream>>>(s.mt, s.geo_md, s.md, s.dt, na, correct ? 1 : 0,
+                                                 predict ? 1 : 0, sites ? 1 : 0, parity);
+  else if (is_cubic(s))
+    k_md_pre<kBoxCube><<<nb, 256, 0, s.stream>>>(s.mt, s.geo_md, s.md, s.dt, na, correct ? 1 : 0,
+                                                 predict ? 1 : 0, sites ? 1 : 0, parity);
   else
-    k_md_pre<false><<<pre_waves(s.n) / 4, 256, 0, s.stream>>>(
-        s.mt, s.geo_md, s.md, s.dt, s.nose_active ? 1 : 0, correct ? 1 : 0, predict ? 1 : 0,
-        sites ? 1 : 0, parity);
+    k_md_pre<kBoxOrtho><<<nb, 256, 0, s.stream>>>(s.mt, s.geo_md, s.md, s.dt, na,
+                                                  correct ? 1 : 0, predict ? 1 : 0, sites ? 1 : 0,
+                                                  parity);
 }
 
 static CloseArgs close_args(System &s, int mode, int src_parity) {
@@ -939,14 +950,16 @@ template <int M, typename R>
 static void md_main_t(System &s, double *out, int parity, const CloseArgs &ca) {
   const int rb = main_region_bytes(s.md.maxnb, (int)sizeof(R));
   const unsigned grid = s.nblk_md + (ca.mode ? 1 : 0);
+  const size_t lds = 4 * (size_t)rb;
   if (s.andersen)
-    k_md_main<M, R, true><<<grid, 256, 4 * (size_t)rb, s.stream>>>(s.mt, s.geo_md, s.md,
-                                                                   pair_k<R>(s), out, parity, ca,
-                                                                   rb);
+    k_md_main<M, R, kBoxBaro><<<grid, 256, lds, s.stream>>>(s.mt, s.geo_md, s.md, pair_k<R>(s),
+                                                            out, parity, ca, rb);
+  else if (is_cubic(s))
+    k_md_main<M, R, kBoxCube><<<grid, 256, lds, s.stream>>>(s.mt, s.geo_md, s.md, pair_k<R>(s),
+                                                            out, parity, ca, rb);
   else
-    k_md_main<M, R, false><<<grid, 256, 4 * (size_t)rb, s.stream>>>(s.mt, s.geo_md, s.md,
-                                                                    pair_k<R>(s), out, parity, ca,
-                                                                    rb);
+    k_md_main<M, R, kBoxOrtho><<<grid, 256, lds, s.stream>>>(s.mt, s.geo_md, s.md, pair_k<R>(s),
+                                                             out, parity, ca, rb);
 }
 
 void md_launch_main(System &s, double *out, int parity, int close_mode) {

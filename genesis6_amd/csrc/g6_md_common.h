@@ -106,20 +106,28 @@ __device__ inline void store_qc(const MdBuf &b, int n, int i, const double (&S)[
 struct BoxNow {
   double L[3], invL[3], cellinv[3];
 };
-// NPT: the barostat is on (kernels are instantiated both ways, so the fixed
-// box costs nothing)
-template <bool NPT>
+// The kernels are instantiated per kind of box: a fixed cube (one edge for
+// every axis, so the edges take as few registers as a scalar box), a fixed
+// orthorhombic box, and the barostat's box in device memory.
+constexpr int kBoxCube = 0, kBoxOrtho = 1, kBoxBaro = 2;
+template <int BOX>
 __device__ inline BoxNow box_now(const Geom &g, const MdBuf &b) {
   BoxNow r;
 #pragma unroll
   for (int a = 0; a < 3; a++) {
-    r.L[a] = NPT ? b.box[a] : g.L[a];
-    r.invL[a] = NPT ? b.box[3 + a] : g.invL[a];
-    r.cellinv[a] = NPT ? b.box[6 + a] : g.cellinv[a];
+    const int e = BOX == kBoxCube ? 0 : a;
+    r.L[a] = BOX == kBoxBaro ? b.box[a] : g.L[e];
+    r.invL[a] = BOX == kBoxBaro ? b.box[3 + a] : g.invL[e];
+    r.cellinv[a] = BOX == kBoxBaro ? b.box[6 + a] : g.cellinv[e];
   }
   return r;
 }
-
+// the MD grid's cells per axis (one count for a cube)
+template <int BOX>
+__device__ inline void grid_nd(const Geom &g, int (&nd)[3]) {
+#pragma unroll
+  for (int a = 0; a < 3; a++) nd[a] = g.nd[BOX == kBoxCube ? 0 : a];
+}
 
 // Andersen_Rescale (AndersenModule.F90:65-82, BoxModule.F90:167-183): in
 // the orthorhombic mode every edge scales by (v0 / V)^(1/3), in the orz mode
