@@ -922,38 +922,54 @@ int g6_get_nose(g6_handle h, double *nose) {
 }
 
 int g6_vreset(g6_handle h, double temp, uint32_t seed) {
-  // Rigid_VReset2 (RigidModule.F90:1016-1053) with Random_Normal
-  // (RandomModule.F90:28-36): dx(1) = N(0, T*rgas*J2I/m)*dt, w(1) = N(0, T*rgas*J2I/I)
   System *s = get(h);
   if (!s || !(temp >= 0)) return G6_EARG;
-  const int n = s->n;
-  std::vector<double> dx(15 * (size_t)n), w(15 * (size_t)n);
-  G6_CHECK(hipStreamSynchronize(s->stream));
-  G6_CHECK(hipMemcpy(dx.data(), s->md.dx, sizeof(double) * 15 * n, hipMemcpyDeviceToHost));
-  G6_CHECK(hipMemcpy(w.data(), s->md.w, sizeof(double) * 15 * n, hipMemcpyDeviceToHost));
+  return vreset_systems(&s, 1, temp, seed);
+}
+
+}  // extern "C"
+
+namespace g6 {
+
+int vreset_systems(System *const *sys, int nsys, double temp, uint32_t seed) {
+  // Rigid_VReset2 (RigidModule.F90:1016-1053) with Random_Normal
+  // (RandomModule.F90:28-36): dx(1) = N(0, T*rgas*J2I/m)*dt, w(1) = N(0, T*rgas*J2I/I);
+  // the components of a mixture draw in turn from one stream (Mixture.F90:1336-1346)
   MT mt(seed);
-  const double rgas = kNA * kKB, J2I = 1.0 / 10.0;
-  const double massi = 1.0 / s->mt.mass;
-  const double ii[3] = {1.0 / s->mt.Ixx, 1.0 / s->mt.Iyy, 1.0 / s->mt.Izz};
-  auto normal = [](double var, double r1, double r2) {
-    const double r = sqrt(-2.0 * log(r1));
-    const double theta = 2.0 * kPI * r2;
-    return r * sin(theta) * sqrt(var) + 0.0;
-  };
-  for (int i = 0; i < n; i++) {
-    for (int a = 0; a < 3; a++) {
-      const double r1 = mt.getnext(), r2 = mt.getnext();
-      dx[(size_t)a * n + i] = normal(temp * rgas * J2I * massi, r1, r2) * s->dt;
+  for (int c = 0; c < nsys; c++) {
+    System *s = sys[c];
+    const int n = s->n;
+    std::vector<double> dx(15 * (size_t)n), w(15 * (size_t)n);
+    G6_CHECK(hipStreamSynchronize(s->stream));
+    G6_CHECK(hipMemcpy(dx.data(), s->md.dx, sizeof(double) * 15 * n, hipMemcpyDeviceToHost));
+    G6_CHECK(hipMemcpy(w.data(), s->md.w, sizeof(double) * 15 * n, hipMemcpyDeviceToHost));
+    const double rgas = kNA * kKB, J2I = 1.0 / 10.0;
+    const double massi = 1.0 / s->mt.mass;
+    const double ii[3] = {1.0 / s->mt.Ixx, 1.0 / s->mt.Iyy, 1.0 / s->mt.Izz};
+    auto normal = [](double var, double r1, double r2) {
+      const double r = sqrt(-2.0 * log(r1));
+      const double theta = 2.0 * kPI * r2;
+      return r * sin(theta) * sqrt(var) + 0.0;
+    };
+    for (int i = 0; i < n; i++) {
+      for (int a = 0; a < 3; a++) {
+        const double r1 = mt.getnext(), r2 = mt.getnext();
+        dx[(size_t)a * n + i] = normal(temp * rgas * J2I * massi, r1, r2) * s->dt;
+      }
+      for (int a = 0; a < 3; a++) {
+        const double r1 = mt.getnext(), r2 = mt.getnext();
+        w[(size_t)a * n + i] = normal(temp * rgas * J2I * ii[a], r1, r2);
+      }
     }
-    for (int a = 0; a < 3; a++) {
-      const double r1 = mt.getnext(), r2 = mt.getnext();
-      w[(size_t)a * n + i] = normal(temp * rgas * J2I * ii[a], r1, r2);
-    }
+    G6_CHECK(hipMemcpy(s->md.dx, dx.data(), sizeof(double) * 15 * n, hipMemcpyHostToDevice));
+    G6_CHECK(hipMemcpy(s->md.w, w.data(), sizeof(double) * 15 * n, hipMemcpyHostToDevice));
   }
-  G6_CHECK(hipMemcpy(s->md.dx, dx.data(), sizeof(double) * 15 * n, hipMemcpyHostToDevice));
-  G6_CHECK(hipMemcpy(s->md.w, w.data(), sizeof(double) * 15 * n, hipMemcpyHostToDevice));
   return 0;
 }
+
+}  // namespace g6
+
+extern "C" {
 
 // flags[0]: 1 pair capacity, 2 image shift beyond 511 box lengths,
 // 3 cell capacity, 4 neighbour-row capacity

@@ -302,6 +302,9 @@ bool make_model_table(int model, int prec, ModelTable *mt, bool fast = true);
 // the pair-kernel layout of a model with or without the custom kernels
 inline int kernel_model(int model, bool fast) { return model == G6_ST2 && !fast ? G6_TIP5P : model; }
 void make_std_table(const ModelTable &mt, StdTable *t);
+void make_std_table(const ModelTable &mi, const ModelTable &mj, StdTable *t);
+// Rigid_VReset2 of each system in turn on one random stream (g6_api.hip)
+int vreset_systems(System *const *sys, int nsys, double temp, uint32_t seed);
 void mc_free(System &s);
 void npt_forget(System *s);  // frees the NPT Monte Carlo state of s (g6_npt.hip)
 System *system_of(g6_handle h);  // nullptr for an unknown handle

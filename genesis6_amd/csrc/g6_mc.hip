@@ -36,21 +36,24 @@ constexpr double kIncrement = (double)1.05f, kDeltaMin = (double)0.05f;
 constexpr int kMcThreads = 512;
 constexpr int kMcMaxPer = 4;  // molecules per thread: n <= 2048
 
-void make_std_table(const ModelTable &m, StdTable *t) {
+void make_std_table(const ModelTable &m, StdTable *t) { make_std_table(m, m, t); }
+
+void make_std_table(const ModelTable &mi, const ModelTable &mj, StdTable *t) {
   // StdInteraction.F90:201-221: qq = q_a q_b COEFF KX2X J2I scale,
   // aa = eps_a eps_b, LJ from sqrt(aa) and the mean sigma to the 6th power
-  // COEFF = CA*332.17752, KX2X = 1/X2KX, J2I = 1/I2J (PhysConst.F90:31-50)
+  // COEFF = CA*332.17752, KX2X = 1/X2KX, J2I = 1/I2J (PhysConst.F90:31-50);
+  // site a of mi, b of mj (two models: the pairs between two components)
   const double coeff = 4.184 * 332.17752, kx2x = 1.0 / 1e-3, j2i = 1.0 / 10.0, scale = 1.0;
   t->nt = 0;
-  for (int a = 0; a < m.nsite; a++)
-    for (int b = 0; b < m.nsite; b++) {
-      const double qq = m.std_q[a] * m.std_q[b] * coeff * kx2x * j2i * scale;
-      const double aa = m.std_eps[a] * m.std_eps[b];
+  for (int a = 0; a < mi.nsite; a++)
+    for (int b = 0; b < mj.nsite; b++) {
+      const double qq = mi.std_q[a] * mj.std_q[b] * coeff * kx2x * j2i * scale;
+      const double aa = mi.std_eps[a] * mj.std_eps[b];
       if (qq == 0.0 && aa == 0.0) continue;
       StdTerm s{a, b, 0, qq, 0.0, 0.0};
       if (aa != 0.0) {
         const double ljeps = sqrt(aa);
-        double sg = (m.std_sig[a] + m.std_sig[b]) * 0.5;
+        double sg = (mi.std_sig[a] + mj.std_sig[b]) * 0.5;
         sg = sg * sg * sg * sg * sg * sg;
         s.aa = 4.0 * ljeps * sg * sg * kx2x * j2i * scale;
         s.bb = -4.0 * ljeps * sg * kx2x * j2i * scale;

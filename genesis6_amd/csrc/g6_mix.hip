@@ -1,0 +1,504 @@
+// Mixtures of two rigid components (the reference's @NCMP 2, Mixture.F90:
+// 475-499, 1833-2286): each component is a handle of its own (its molecules,
+// model table, Gear state, rows and grid), and a mixture steps both with one
+// Nose thermostat and adds the forces between them.
+//
+// Per step, on the first component's stream: both correctors; one step close
+// over both (the thermo record of the whole system: pair energies and virial
+// of the three pair sets, the kinetic energies with each component's masses
+// and moments, the temperature over all degrees of freedom, Nose_Correct and
+// Nose_Predict, Mixture.F90:2236-2286); both predictors with their binning;
+// the rows of each component (k_md_main, as a single component steps); then
+// k_mix_cross, the pairs between the components.  The second component reads
+// the first's thermostat state, so both correctors take the same zeta.
+//
+// Pairs between components are StdForce's (StdInteraction.F90:63-630, the
+// reference's force for every pair of a mixture without @FAST, Interaction2.
+// F90:19-67): the site-pair terms of the two models' site parameters
+// (Lorentz-Berthelot: sqrt(eps_a eps_b), (sig_a + sig_b)/2, q_a q_b), times
+// the switch S on the COM distance, with the switch's derivative on the COMs.
+// k_mix_cross takes every molecule of one component against every molecule
+// of the other in index order (one thread per molecule, both directions, so
+// every sum has a fixed order and the step is deterministic), and adds each
+// molecule's force and lab torque to its component's force array.
+#include <math.h>
+
+#include <mutex>
+#include <vector>
+
+#include "g6_internal.h"
+#include "g6_md_common.h"
+#include "g6_std.h"
+
+#pragma clang fp contract(off)
+
+namespace g6 {
+
+// site terms between the components: site a of the row component, site b of
+// the other (qq, aa, bb as StdInteraction.F90:201-221 forms them)
+struct CrossArgs {
+  int ni, nj, nsi, nsj;    // molecules and sites (CoM site last) of each side
+  const double *reci;      // records of the row component (FP64: COM, site offsets)
+  const double *recj;
+  int rs;                  // record stride (doubles)
+  double *frc;             // the row component's force and lab torque [6][ni]
+  double *part;            // [nblk][kBP] pair partials of the launch
+  double L[3], invL[3];
+  double rc2, rin, rout, c1, c2;
+  StdTable t;
+};
+
+constexpr int kMixThreads = 128;
+
+// one thread per molecule i of the row component, every molecule j of the
+// other in index order: the COM test dd < rc^2 (minimum image per axis), S
+// and dS on the COM distance, StdForce's site terms; F and tau of i added to
+// its force array; energy, virial and pair count summed per block in a fixed
+// order (halved: every pair is met from both sides)
+__global__ __launch_bounds__(kMixThreads) void k_mix_cross(CrossArgs x) {
+  __shared__ double s_v[12][kMixThreads];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc[12];
+#pragma unroll
+  for (int q = 0; q < 12; q++) acc[q] = 0.0;
+  if (i < x.ni) {
+    const double *ri = x.reci + (size_t)i * x.rs;
+    double si[kMaxSite][3];
+    for (int s = 0; s < kMaxSite; s++) si[s][0] = si[s][1] = si[s][2] = 0.0;
+    for (int s = 0; s + 1 < x.nsi; s++)
+      for (int a = 0; a < 3; a++) si[s][a] = ri[4 + s * 3 + a];
+    double F[3] = {0, 0, 0}, Tq[3] = {0, 0, 0};
+    for (int j = 0; j < x.nj; j++) {
+      const double *rj = x.recj + (size_t)j * x.rs;
+      double d[3];
+      for (int a = 0; a < 3; a++) {
+        d[a] = ri[a] - rj[a];
+        d[a] = d[a] - round(d[a] * x.invL[a]) * x.L[a];
+      }
+      const double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+      if (!(dd < x.rc2)) continue;
+      // cutoff_func (CutoffModule.F90:38-59) on the COM distance
+      double hh = 1.0, dh = 0.0;
+      if (dd > x.rin * x.rin) {
+        const double dr = sqrt(dd), u = dr - x.rout, l = dr - x.rin, uu = u * u, ll = l * l;
+        hh = x.c1 * uu * u * (10.0 * ll - 5.0 * u * l + uu);
+        dh = -(x.c2 * uu * ll / dr);
+      }
+      double fs[kMaxSite][3];
+      for (int s = 0; s < kMaxSite; s++) fs[s][0] = fs[s][1] = fs[s][2] = 0.0;
+      double ep = 0.0;
+      for (int u = 0; u < x.t.nt; u++) {
+        const StdTerm &tm = x.t.t[u];
+        const int a = tm.a, b = tm.b;
+        double dx = d[0] + si[a][0], dy = d[1] + si[a][1], dz = d[2] + si[a][2];
+        if (b + 1 < x.nsj) {
+          dx = dx - rj[4 + b * 3];
+          dy = dy - rj[4 + b * 3 + 1];
+          dz = dz - rj[4 + b * 3 + 2];
+        }
+        const double inv2 = 1.0 / (dx * dx + dy * dy + dz * dz);
+        double e0 = 0.0, f0 = 0.0;
+        if (tm.kind != 3) {
+          const double ec = tm.qq * sqrt(inv2);
+          e0 = ec;
+          f0 = ec * inv2;
+        }
+        if (tm.kind != 2) {
+          const double d6 = inv2 * inv2 * inv2, d12 = d6 * d6;
+          e0 = e0 + tm.aa * d12 + tm.bb * d6;
+          f0 = f0 + inv2 * (12.0 * tm.aa * d12 + 6.0 * tm.bb * d6);
+        }
+        ep = ep + e0;
+        fs[a][0] += dx * f0;
+        fs[a][1] += dy * f0;
+        fs[a][2] += dz * f0;
+      }
+      double fc[3] = {0, 0, 0}, tc[3] = {0, 0, 0};
+      for (int s = 0; s < x.nsi; s++) {
+        fc[0] += fs[s][0];
+        fc[1] += fs[s][1];
+        fc[2] += fs[s][2];
+        tc[0] += si[s][1] * fs[s][2] - si[s][2] * fs[s][1];
+        tc[1] += si[s][2] * fs[s][0] - si[s][0] * fs[s][2];
+        tc[2] += si[s][0] * fs[s][1] - si[s][1] * fs[s][0];
+      }
+      const double cc = dh * ep;
+      double Fp[3];
+      for (int a = 0; a < 3; a++) {
+        Fp[a] = hh * fc[a] + cc * d[a];
+        F[a] += Fp[a];
+        Tq[a] += hh * tc[a];
+      }
+      acc[0] += ep * hh;
+      for (int a = 0; a < 3; a++)
+        for (int c = 0; c < 3; c++) acc[1 + 3 * a + c] = fma(d[a], Fp[c], acc[1 + 3 * a + c]);
+      acc[10] += 1.0;
+    }
+    const int ni = x.ni;
+    for (int a = 0; a < 3; a++) {
+      x.frc[a * ni + i] = x.frc[a * ni + i] + F[a];
+      x.frc[(3 + a) * ni + i] = x.frc[(3 + a) * ni + i] + Tq[a];
+    }
+  }
+  // block sums in thread order (a fixed tree)
+#pragma unroll
+  for (int q = 0; q < 12; q++) s_v[q][threadIdx.x] = acc[q];
+  __syncthreads();
+  for (int w = kMixThreads / 2; w >= 1; w >>= 1) {
+    if ((int)threadIdx.x < w)
+#pragma unroll
+      for (int q = 0; q < 12; q++) s_v[q][threadIdx.x] += s_v[q][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 12) x.part[(size_t)blockIdx.x * kBP + threadIdx.x] = 0.5 * s_v[threadIdx.x][0];
+}
+
+// The step close of a mixture (the single-component step close for two
+// components and the pairs between them).  mode: 1 thermo record and
+// Nose_Correct, 2 Nose_Predict, 4 clear the cells and rebuild flag of parity
+// 1 - parity of both components, 8 of both parities.  The thermo record of
+// the first component takes the whole system: energy and virial summed over
+// the pair sets (first component, second, between) in that order, as
+// Mixture.F90:2085-2090 sums ep(k) and virialp(k); kinetic energies and
+// tensors summed over the components, each with its own mass and moments
+// (Mixture.F90:2180-2212); the temperature over 6 (nA + nB) degrees of
+// freedom.  parts: ep of each set and ek of each component (internal units).
+struct MixClose {
+  ModelTable ma, mb;
+  MdBuf a, b;
+  int na, nb, nblk_pre_a, nblk_pre_b, nblk_main_a, nblk_main_b, nblk_x, ncell_a, ncell_b;
+  const double *bpart_a, *bpart_b, *bpart_x;
+  double *parts;
+  double dt, box_vol;
+  int nose_active;
+};
+
+constexpr int kMixRed = 36 + 2 * kEkLen;
+
+__global__ __launch_bounds__(256) void k_mix_close(MixClose c, int mode, int parity) {
+  __shared__ double s_red[4][kMixRed];
+  const int wid = threadIdx.x >> 6;
+  {
+    // pair partials of each set, then kinetic partials of each component,
+    // per thread in block order, then over the wave and the four waves
+    double v[kMixRed];
+#pragma unroll
+    for (int q = 0; q < kMixRed; q++) v[q] = 0.0;
+    if (mode & 1) {
+      for (int k = threadIdx.x; k < c.nblk_main_a; k += 256)
+#pragma unroll
+        for (int q = 0; q < 12; q++) v[q] += c.bpart_a[(size_t)k * kBP + q];
+      for (int k = threadIdx.x; k < c.nblk_main_b; k += 256)
+#pragma unroll
+        for (int q = 0; q < 12; q++) v[12 + q] += c.bpart_b[(size_t)k * kBP + q];
+      for (int k = threadIdx.x; k < c.nblk_x; k += 256)
+#pragma unroll
+        for (int q = 0; q < 12; q++) v[24 + q] += c.bpart_x[(size_t)k * kBP + q];
+      for (int k = threadIdx.x; k < c.nblk_pre_a; k += 256)
+#pragma unroll
+        for (int q = 0; q < kEkLen; q++) v[36 + q] += c.a.blockek[(size_t)k * kEkLen + q];
+      for (int k = threadIdx.x; k < c.nblk_pre_b; k += 256)
+#pragma unroll
+        for (int q = 0; q < kEkLen; q++) v[36 + kEkLen + q] += c.b.blockek[(size_t)k * kEkLen + q];
+    }
+#pragma unroll
+    for (int q = 0; q < kMixRed; q++) v[q] = wsum63(v[q]);
+    if ((threadIdx.x & 63) == 63)
+#pragma unroll
+      for (int q = 0; q < kMixRed; q++) s_red[wid][q] = v[q];
+  }
+  if (mode & 12) {
+    const int p0 = (mode & 8) ? 0 : 1 - parity, p1 = (mode & 8) ? 1 : 1 - parity;
+    for (int p = p0; p <= p1; p++)
+      for (int u = 0; u < 2; u++) {
+        const MdBuf &m = u ? c.b : c.a;
+        const int ncell = u ? c.ncell_b : c.ncell_a;
+        int *cr = m.cellres + (size_t)p * ncell * m.cell_cap;
+        for (int q = threadIdx.x; q < ncell; q += 256) cr[(size_t)q * m.cell_cap] = 0;
+        if (threadIdx.x == 0 && m.half_skin2 > 0.0) m.flags[2 + p] = 0;
+      }
+  }
+  __syncthreads();
+  if (threadIdx.x < kMixRed) {
+    const int q = threadIdx.x;
+    s_red[0][q] = s_red[0][q] + s_red[1][q] + s_red[2][q] + s_red[3][q];
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const double *ra = s_red[0], *rb = s_red[0] + 12, *rx = s_red[0] + 24;
+  const double *ka = s_red[0] + 36, *kb = s_red[0] + 36 + kEkLen;
+  double *z = c.a.nose, *th = c.a.thermo;
+  const double dt = c.dt;
+  if (mode & 1) {
+    th[0] = (ra[0] + rb[0]) + rx[0];
+    for (int q = 0; q < 9; q++) th[4 + q] = (ra[1 + q] + rb[1 + q]) + rx[1 + q];
+    th[14] = (ra[10] + rb[10]) + rx[10];
+    const double nl = ra[11] + rb[11];
+    if (nl > 0) th[15] = nl;
+    // Rigid_Ek of each component (RigidModule.F90:576-599)
+    const double kd2 = 0.5 * (1.0 / (dt * dt));
+    const double ekta = ka[0] * c.ma.mass * kd2, ektb = kb[0] * c.mb.mass * kd2;
+    const double ekra = (ka[1] * c.ma.Ixx + ka[2] * c.ma.Iyy + ka[3] * c.ma.Izz) * 0.5;
+    const double ekrb = (kb[1] * c.mb.Ixx + kb[2] * c.mb.Iyy + kb[3] * c.mb.Izz) * 0.5;
+    const double ekt = ekta + ektb, ekr = ekra + ekrb;
+    const double dof = 6.0 * (c.na + c.nb);
+    const double temp = 10.0 * 2.0 * (ekt + ekr) / (G6_RGAS_MD * dof);
+    double h = (th[0] + (ekt + ekr)) * 10.0;
+    double nose_ep = 0.0, nose_ek = 0.0;
+    if (c.nose_active) {
+      const double zetaf = (temp - z[0]) / z[1] - z[2] * z[3];
+      const double dz2 = zetaf - z[4];
+      z[3] = z[3] + kG4_0 * dz2;
+      z[4] = zetaf;
+      z[5] = z[5] + kG4_2 * dz2;
+      z[6] = z[6] + kG4_3 * dz2;
+      z[7] = z[7] + kG4_4 * dz2;
+      z[8] = z[8] + z[3];
+      nose_ek = (z[3] * z[3] * z[1] * dt * dof * 0.5) * G6_RGAS_MD;
+      nose_ep = (dof * z[0] * (z[8] - 0.5 * z[3]) * dt) * G6_RGAS_MD;
+      h = h + nose_ep + nose_ek;
+    }
+    th[1] = ekt;
+    th[2] = ekr;
+    th[3] = temp;
+    th[13] = h;
+    // kinetic tensors (RigidModule.F90:617), summed over the components,
+    // and the pressure (Mixture.F90:2206-2220)
+    const double kd = 1.0 / (dt * dt);
+    const int idx[9] = {4, 5, 6, 5, 7, 8, 6, 8, 9};
+    double kt[9];
+    for (int q = 0; q < 9; q++) kt[q] = ka[idx[q]] * c.ma.mass * kd + kb[idx[q]] * c.mb.mass * kd;
+    for (int q = 0; q < 9; q++) th[16 + q] = kt[q];
+    const double vp = (kt[0] + th[4]) + (kt[4] + th[8]) + (kt[8] + th[12]);
+    th[25] = (vp / 3.0) * 10.0 * (1.0 / 101326.0) /
+             (c.box_vol * 6.0225e23 * (1e-10 * 1e-10 * 1e-10));
+    th[26] = c.box_vol;
+    th[27] = nose_ep;
+    th[28] = nose_ek;
+    th[29] = c.nose_active ? z[0] : 0.0;
+    th[30] = 0.0;
+    th[31] = 0.0;
+    c.parts[0] = ra[0];
+    c.parts[1] = rb[0];
+    c.parts[2] = rx[0];
+    c.parts[3] = ekta + ekra;
+    c.parts[4] = ektb + ekrb;
+  }
+  if ((mode & 2) && c.nose_active) {
+    z[3] = z[3] + z[4] + z[5] + z[6] + z[7];
+    z[4] = z[4] + 2.0 * z[5] + 3.0 * z[6] + 4.0 * z[7];
+    z[5] = z[5] + 3.0 * z[6] + 6.0 * z[7];
+    z[6] = z[6] + 4.0 * z[7];
+  }
+}
+
+struct Mix {
+  g6_handle ha = 0, hb = 0;
+  System *a = nullptr, *b = nullptr;
+  StdTable ab{}, ba{};       // site terms: a's sites with b's, b's with a's
+  double *part = nullptr;    // [2][nblk_a + nblk_b][kBP]: cross partials by parity
+  double *parts = nullptr;   // [8]: ep of each pair set, ek of each component
+  int nblk_a = 0, nblk_b = 0;
+};
+
+static std::mutex g_mxmu;
+static std::vector<Mix *> g_mix;
+
+static Mix *mix_of(int64_t h) {
+  std::lock_guard<std::mutex> lk(g_mxmu);
+  if (h <= 0 || h > (int64_t)g_mix.size()) return nullptr;
+  return g_mix[h - 1];
+}
+
+static CrossArgs cross_args(const System &r, const System &o, const StdTable &t, double *part) {
+  CrossArgs x;
+  x.ni = r.n;
+  x.nj = o.n;
+  x.nsi = r.mt.nsite;
+  x.nsj = o.mt.nsite;
+  x.reci = r.md.rec;
+  x.recj = o.md.rec;
+  x.rs = r.md.rs;
+  x.frc = r.md.frc;
+  x.part = part;
+  for (int a = 0; a < 3; a++) {
+    x.L[a] = r.Lb[a];
+    x.invL[a] = 1.0 / r.Lb[a];
+  }
+  x.rc2 = r.geo_md.rc2;
+  x.rin = r.geo_md.rin;
+  x.rout = r.geo_md.rout;
+  x.c1 = r.geo_md.c1;
+  x.c2 = r.geo_md.c2;
+  x.t = t;
+  return x;
+}
+
+static void mix_close(Mix &m, int mode, int parity, int src_parity) {
+  const System &a = *m.a, &b = *m.b;
+  MixClose c;
+  c.ma = a.mt;
+  c.mb = b.mt;
+  c.a = a.md;
+  c.b = b.md;
+  c.na = a.n;
+  c.nb = b.n;
+  c.nblk_pre_a = pre_waves(a.n);
+  c.nblk_pre_b = pre_waves(b.n);
+  c.nblk_main_a = a.nblk_md;
+  c.nblk_main_b = b.nblk_md;
+  c.nblk_x = m.nblk_a + m.nblk_b;
+  c.ncell_a = a.geo_md.ncell;
+  c.ncell_b = b.geo_md.ncell;
+  c.bpart_a = a.md.blockpart + (size_t)src_parity * a.nblk_md * kBP;
+  c.bpart_b = b.md.blockpart + (size_t)src_parity * b.nblk_md * kBP;
+  c.bpart_x = m.part + (size_t)src_parity * (m.nblk_a + m.nblk_b) * kBP;
+  c.parts = m.parts;
+  c.dt = a.dt;
+  c.box_vol = box_volume(a.Lb);
+  c.nose_active = a.nose_active ? 1 : 0;
+  k_mix_close<<<1, 256, 0, m.a->stream>>>(c, mode, parity);
+}
+
+static void mix_cross(Mix &m, int parity) {
+  double *p = m.part + (size_t)parity * (m.nblk_a + m.nblk_b) * kBP;
+  k_mix_cross<<<m.nblk_a, kMixThreads, 0, m.a->stream>>>(cross_args(*m.a, *m.b, m.ab, p));
+  k_mix_cross<<<m.nblk_b, kMixThreads, 0, m.a->stream>>>(
+      cross_args(*m.b, *m.a, m.ba, p + (size_t)m.nblk_a * kBP));
+}
+
+// k mixture steps, every launch on the first component's stream (the phases
+// of the barostat's step: correctors, close, predictors, forces)
+static void mix_steps(Mix &m, int k) {
+  System &a = *m.a, &b = *m.b;
+  for (int q = 0; q < k; q++) {
+    const int p = q & 1;
+    if (q > 0) {
+      md_launch_pre(a, true, false, false, 0);
+      md_launch_pre(b, true, false, false, 0);
+    }
+    // clears the cells and rebuild flags of parity p before they are filled
+    mix_close(m, (q > 0 ? 1 : 0) | 2 | 4, 1 - p, 1 - p);
+    md_launch_pre(a, false, true, true, p);
+    md_launch_pre(b, false, true, true, p);
+    md_launch_main(a, nullptr, p, 0);
+    md_launch_main(b, nullptr, p, 0);
+    mix_cross(m, p);
+  }
+  md_launch_pre(a, true, false, false, 0);
+  md_launch_pre(b, true, false, false, 0);
+  mix_close(m, 1 | 8, 0, (k - 1) & 1);
+}
+
+static int mix_flags(System &s) {
+  int fl = 0;
+  G6_CHECK(hipMemcpyAsync(&fl, s.md.flags, sizeof(int), hipMemcpyDeviceToHost, s.stream));
+  G6_CHECK(hipStreamSynchronize(s.stream));
+  s.last_flag = fl;
+  if (fl == 2) return G6_EARG;
+  return fl ? G6_ECAPACITY : 0;
+}
+
+}  // namespace g6
+
+using namespace g6;
+
+extern "C" {
+
+int g6_mixture_create(g6_mixture *hm, g6_handle ha, g6_handle hb) {
+  if (!hm) return G6_EARG;
+  System *a = system_of(ha), *b = system_of(hb);
+  if (!a || !b || a == b) return G6_EARG;
+  // one box, cutoff, time step and pair-list skin; FP64 rows on a fixed box
+  for (int k = 0; k < 3; k++)
+    if (a->Lb[k] != b->Lb[k]) return G6_EARG;
+  if (a->rc != b->rc || a->taper != b->taper || a->dt != b->dt || a->skin != b->skin ||
+      a->prec != G6_FP64 || b->prec != G6_FP64 || a->andersen || b->andersen || a->tiles ||
+      b->tiles)
+    return G6_EARG;
+  {
+    std::lock_guard<std::mutex> lk(g_mxmu);
+    for (Mix *o : g_mix)
+      if (o && (o->a == a || o->b == a || o->a == b || o->b == b)) return G6_EARG;
+  }
+  Mix *m = new Mix();
+  m->ha = ha;
+  m->hb = hb;
+  m->a = a;
+  m->b = b;
+  make_std_table(a->mt, b->mt, &m->ab);
+  make_std_table(b->mt, a->mt, &m->ba);
+  m->nblk_a = (a->n + kMixThreads - 1) / kMixThreads;
+  m->nblk_b = (b->n + kMixThreads - 1) / kMixThreads;
+  if (hipMalloc(&m->part, sizeof(double) * 2 * (m->nblk_a + m->nblk_b) * kBP) != hipSuccess ||
+      hipMalloc(&m->parts, sizeof(double) * 8) != hipSuccess) {
+    (void)hipFree(m->part);
+    delete m;
+    return G6_EHIP;
+  }
+  G6_CHECK(hipMemset(m->parts, 0, sizeof(double) * 8));
+  std::lock_guard<std::mutex> lk(g_mxmu);
+  g_mix.push_back(m);
+  *hm = (g6_mixture)g_mix.size();
+  return 0;
+}
+
+int g6_mixture_free(g6_mixture hm) {
+  Mix *m = mix_of(hm);
+  if (!m) return G6_EARG;
+  (void)hipStreamSynchronize(m->a->stream);
+  (void)hipFree(m->part);
+  (void)hipFree(m->parts);
+  std::lock_guard<std::mutex> lk(g_mxmu);
+  g_mix[hm - 1] = nullptr;
+  delete m;
+  return 0;
+}
+
+int g6_mixture_vreset(g6_mixture hm, double temp, uint32_t seed) {
+  Mix *m = mix_of(hm);
+  if (!m || !(temp >= 0)) return G6_EARG;
+  System *s[2] = {m->a, m->b};
+  return vreset_systems(s, 2, temp, seed);
+}
+
+int g6_mixture_step(g6_mixture hm, int nsteps) {
+  Mix *m = mix_of(hm);
+  if (!m || nsteps < 0) return G6_EARG;
+  System &a = *m->a, &b = *m->b;
+  if (system_of(m->ha) != &a || system_of(m->hb) != &b || a.andersen || b.andersen || a.tiles ||
+      b.tiles)
+    return G6_EARG;
+  if (nsteps == 0) return 0;
+  G6_CHECK(hipStreamSynchronize(b.stream));
+  // the launches of the second component go to the first's stream and
+  // its correctors read the first's thermostat (kernels take the buffers
+  // by value at launch, so its own are back in place afterwards)
+  const bool keep_na = b.nose_active;
+  hipStream_t keep_st = b.stream;
+  double *keep_nose = b.md.nose;
+  b.nose_active = a.nose_active;
+  b.stream = a.stream;
+  b.md.nose = a.md.nose;
+  mix_steps(*m, nsteps);
+  b.nose_active = keep_na;
+  b.stream = keep_st;
+  b.md.nose = keep_nose;
+  G6_CHECK(hipGetLastError());
+  const int ra = mix_flags(a), rb = mix_flags(b);
+  a.steps_done += nsteps;
+  b.steps_done += nsteps;
+  return ra ? ra : rb;
+}
+
+int g6_mixture_thermo(g6_mixture hm, double *thermo, double *parts) {
+  Mix *m = mix_of(hm);
+  if (!m || !thermo) return G6_EARG;
+  G6_CHECK(hipStreamSynchronize(m->a->stream));
+  G6_CHECK(hipMemcpy(thermo, m->a->md.thermo, sizeof(double) * G6_THERMO_LEN,
+                     hipMemcpyDeviceToHost));
+  if (parts) G6_CHECK(hipMemcpy(parts, m->parts, sizeof(double) * 5, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+}  // extern "C"

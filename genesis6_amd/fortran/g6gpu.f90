@@ -422,6 +422,37 @@ module g6gpu
        integer(c_int32_t), value :: seed
      end function g6_rx_create
 
+     ! mixtures of two rigid components (g6_mixture_* in g6gpu.h)
+     integer(c_int) function g6_mixture_create(hm, ha, hb) bind(C, name="g6_mixture_create")
+       import :: c_int, c_int64_t
+       integer(c_int64_t), intent(out) :: hm
+       integer(c_int64_t), value :: ha, hb
+     end function g6_mixture_create
+
+     integer(c_int) function g6_mixture_free(hm) bind(C, name="g6_mixture_free")
+       import :: c_int, c_int64_t
+       integer(c_int64_t), value :: hm
+     end function g6_mixture_free
+
+     integer(c_int) function g6_mixture_vreset(hm, temp, seed) bind(C, name="g6_mixture_vreset")
+       import :: c_int, c_int64_t, c_double, c_int32_t
+       integer(c_int64_t), value :: hm
+       real(c_double), value :: temp
+       integer(c_int32_t), value :: seed   ! uint32_t on the C side
+     end function g6_mixture_vreset
+
+     integer(c_int) function g6_mixture_step(hm, nsteps) bind(C, name="g6_mixture_step")
+       import :: c_int, c_int64_t
+       integer(c_int64_t), value :: hm
+       integer(c_int), value :: nsteps
+     end function g6_mixture_step
+
+     integer(c_int) function g6_mixture_thermo(hm, thermo, parts) bind(C, name="g6_mixture_thermo")
+       import :: c_int, c_int64_t, c_double
+       integer(c_int64_t), value :: hm
+       real(c_double), intent(out) :: thermo(*), parts(*)
+     end function g6_mixture_thermo
+
      integer(c_int) function g6_rx_free(hx) bind(C, name="g6_rx_free")
        import :: c_int, c_int64_t
        integer(c_int64_t), value :: hx

@@ -10,7 +10,12 @@
 ! @PCN4 mode pex mass (Andersen barostat: mode 1 orthorhombic, every edge
 ! following the volume; mode 2 orz, the z edge alone) | @PCN2 (refused, as
 ! the reference refuses it) | @FAST | @SNPO | @SNPV |
-! @ID08 model | @NX4A n, then n lines x y z a b c d.
+! @ID08 model | @NX4A n, then n lines x y z a b c d | @NCMP 1 or 2.
+! Two @ID08 + @NX4A blocks make a mixture of two rigid components (as
+! Mixture.F90:636-712 adds a component per block): both step together on
+! the GPU (g6_mixture_*), the @LOGD line then carries the pair energy of each
+! pair set and the kinetic energy of each component, and the restart the
+! two components; a mixture takes neither the barostat nor snapshots here.
 ! Command line: g6md [fp64|fp32] [skin]   (default fp64, skin 0: the
 ! reference's list rebuilt every step).  Without a skin on the command line,
 ! @BOOK interval margin (interval > 0) makes margin the skin: the rows then
@@ -51,7 +56,13 @@ program g6md
   logical :: has_av = .false.
   integer(c_int) :: model = -1, prec = G6_FP64, n = 0, fast = 0
   integer(c_int32_t) :: vseed = 0
-  integer(c_int64_t) :: h
+  integer(c_int64_t) :: h, h2 = 0, hm = 0
+  ! the second component of a mixture; idm: the model of the last @ID08
+  integer(c_int) :: model2 = -1, n2 = 0, idm = -1
+  integer :: ncomp = 0, rcomp = 0
+  logical :: mix
+  real(c_double) :: parts(5)
+  real(c_double), allocatable :: com2(:,:), quat2(:,:,:), dx2(:,:,:), w2(:,:,:)
   logical :: has_nose = .false.
   real(c_double), allocatable :: com(:,:), quat(:,:,:), dx(:,:,:), w(:,:,:)
   real(8), parameter :: I2J = 10d0, X2KX = 1d-3
@@ -129,37 +140,53 @@ program g6md
         read(5, '(a)') line
         select case (trim(adjustl(line)))
         case ('TIP4P')
-           model = G6_TIP4P
+           idm = G6_TIP4P
         case ('TIP5P')
-           model = G6_TIP5P
+           idm = G6_TIP5P
         case ('ST2', 'ST2_____')
-           model = G6_ST2
+           idm = G6_ST2
         case ('SPC_E')
-           model = G6_SPCE
+           idm = G6_SPCE
         case ('NVDE____')
-           model = G6_NVDE
+           idm = G6_NVDE
         case ('OPLSMEOH')
-           model = G6_OPLSMEOH
+           idm = G6_OPLSMEOH
         case default
            write(0, '(a,a)') 'g6md: @ID08 model not on the GPU path: ', trim(line)
            error stop 2
         end select
+     case ('@NCMP')
+        read(5, *) i
+        if (i < 1 .or. i > 2) then
+           write(0, '(a)') 'g6md: @NCMP: one or two components on the GPU path'
+           error stop 2
+        end if
      case ('@NX4A')
-        read(5, *) n
-        if (allocated(com)) deallocate(com, quat, dx, w)
-        allocate(com(n, 3), quat(n, 4, 5), dx(n, 3, 5), w(n, 3, 5))
-        quat = 0d0
-        dx = 0d0
-        w = 0d0
-        do i = 1, n
-           read(5, *) com(i, 1), com(i, 2), com(i, 3), quat(i, 1, 1), quat(i, 2, 1), &
-                quat(i, 3, 1), quat(i, 4, 1)
-        end do
+        ncomp = ncomp + 1
+        if (ncomp == 1) then
+           read(5, *) n
+           if (idm >= 0) model = idm
+           call read_nx4a(n, com, quat, dx, w)
+        else if (ncomp == 2 .and. idm >= 0) then
+           read(5, *) n2
+           model2 = idm
+           call read_nx4a(n2, com2, quat2, dx2, w2)
+        else
+           write(0, '(a)') 'g6md: @NX4A: at most two components, each after its @ID08'
+           error stop 2
+        end if
+        idm = -1
      case default
         write(0, '(a,a)') 'g6md: tag not handled: ', trim(tag)
         error stop 2
      end select
   end do
+  if (idm >= 0 .and. ncomp == 0) model = idm
+  mix = ncomp == 2 .or. (ncomp == 0 .and. rcomp == 2)
+  if (mix .and. (amode /= 0 .or. snpo > 0 .or. snpv > 0)) then
+     write(0, '(a)') 'g6md: a mixture takes neither @PCN4 nor snapshots on the GPU path'
+     error stop 2
+  end if
   if (model < 0 .or. n < 2 .or. any(box3 <= 0d0)) then
      write(0, '(a)') 'g6md: the deck needs @ID08, @NX4A and @BXLA'
      error stop 2
@@ -172,11 +199,22 @@ program g6md
   ! @FAST (default off, Mixture.F90:357, 732-736): StdForce or the custom kernels
   call g6_check(g6_set_fast(h, fast), 'g6_set_fast')
   call g6_check(g6_upload_state(h, com, quat, dx, w), 'g6_upload_state')
+  if (mix) then
+     call g6_check(g6_init_box3(h2, model2, n2, box3, rc, taper, voxa, dt, prec, skin), &
+          'g6_init_box3')
+     call g6_check(g6_set_fast(h2, fast), 'g6_set_fast')
+     call g6_check(g6_upload_state(h2, com2, quat2, dx2, w2), 'g6_upload_state')
+  end if
   if (has_nose) then
      nose(2) = nose(2) * dt        ! @NOS2 mass, times dt (Mixture.F90:1206)
      call g6_check(g6_set_nose(h, 1_c_int, nose), 'g6_set_nose')
   end if
-  if (vtemp >= 0d0) call g6_check(g6_vreset(h, vtemp, vseed), 'g6_vreset')
+  if (mix) then
+     call g6_check(g6_mixture_create(hm, h, h2), 'g6_mixture_create')
+     if (vtemp >= 0d0) call g6_check(g6_mixture_vreset(hm, vtemp, vseed), 'g6_mixture_vreset')
+  else
+     if (vtemp >= 0d0) call g6_check(g6_vreset(h, vtemp, vseed), 'g6_vreset')
+  end if
   if (amode /= 0) then
      if (has_av) then
         call g6_check(g6_set_andersen(h, amode, apex, amass, c_loc(av)), 'g6_set_andersen')
@@ -197,7 +235,11 @@ program g6md
      call next_event(nlog)
      call next_event(snpo)
      call next_event(snpv)
-     call g6_check(g6_step(h, int(chunk, c_int)), 'g6_step')
+     if (mix) then
+        call g6_check(g6_mixture_step(hm, int(chunk, c_int)), 'g6_mixture_step')
+     else
+        call g6_check(g6_step(h, int(chunk, c_int)), 'g6_step')
+     end if
      done = done + chunk
      if (due(snpo) .or. due(snpv)) then
         call g6_check(g6_download_state(h, com, quat, dx, w), 'g6_download_state')
@@ -206,14 +248,24 @@ program g6md
         if (due(snpv)) call write_snapshot(nstep0 + done - 1, .true.)
      end if
      if (due(nlog)) then
-        call g6_check(g6_get_thermo(h, th), 'g6_get_thermo')
-        call write_logd(nstep0 + done - 1, th)
+        if (mix) then
+           call g6_check(g6_mixture_thermo(hm, th, parts), 'g6_mixture_thermo')
+           call write_logd2(nstep0 + done - 1, th)
+        else
+           call g6_check(g6_get_thermo(h, th), 'g6_get_thermo')
+           call write_logd(nstep0 + done - 1, th)
+        end if
      end if
   end do
   call g6_check(g6_download_state(h, com, quat, dx, w), 'g6_download_state')
   if (has_nose) call g6_check(g6_get_nose(h, nose), 'g6_get_nose')
   if (amode /= 0) call g6_check(g6_get_andersen(h, astate), 'g6_get_andersen')
   call g6_check(g6_get_box3(h, box3), 'g6_get_box3')
+  if (mix) then
+     call g6_check(g6_download_state(h2, com2, quat2, dx2, w2), 'g6_download_state')
+     call g6_check(g6_mixture_free(hm), 'g6_mixture_free')
+     call g6_check(g6_finalize(h2), 'g6_finalize')
+  end if
   call g6_check(g6_finalize(h), 'g6_finalize')
   call write_restart()
 
@@ -267,8 +319,8 @@ contains
           read(12) nsteps
        case ('@NCMP')
           read(12) nt
-          if (nt /= 1) then
-             write(0, '(a)') 'g6md: restart with more than one component'
+          if (nt < 1 .or. nt > 2) then
+             write(0, '(a)') 'g6md: restart with more than two components'
              error stop 2
           end if
        case ('@TIE4')
@@ -283,29 +335,37 @@ contains
           read(12) id
           select case (id)
           case ('TIP4P   ')
-             model = G6_TIP4P
+             idm = G6_TIP4P
           case ('TIP5P   ')
-             model = G6_TIP5P
+             idm = G6_TIP5P
           case ('ST2_____')
-             model = G6_ST2
+             idm = G6_ST2
           case ('SPC_E   ')
-             model = G6_SPCE
+             idm = G6_SPCE
           case ('NVDE____')
-             model = G6_NVDE
+             idm = G6_NVDE
           case ('OPLSMEOH')
-             model = G6_OPLSMEOH
+             idm = G6_OPLSMEOH
           case default
              write(0, '(a,a)') 'g6md: @ID08 model not on the GPU path: ', id
              error stop 2
           end select
        case ('@WTG2')
           read(12) m
-          if (allocated(com)) deallocate(com, quat, dx, w)
-          n = m
-          allocate(com(n, 3), quat(n, 4, 5), dx(n, 3, 5), w(n, 3, 5))
-          read(12) (com(:, c), c = 1, 3), ((dx(:, c, k), k = 1, 5), c = 1, 3), &
-               ((quat(:, c, k), k = 1, 5), c = 1, 4), ((w(:, c, k), k = 1, 5), c = 1, 3)
-          w = w / dt                     ! rigid_toInternalUnit (RigidModule.F90:1639-1652)
+          rcomp = rcomp + 1
+          if (rcomp == 1) then
+             n = m
+             model = idm
+             call read_wtg2(n, com, quat, dx, w)
+          else if (rcomp == 2) then
+             n2 = m
+             model2 = idm
+             call read_wtg2(n2, com2, quat2, dx2, w2)
+          else
+             write(0, '(a)') 'g6md: restart with more than two components'
+             error stop 2
+          end if
+          idm = -1
        case default
           write(0, '(a,a)') 'g6md: restart record not handled: ', btag
           error stop 2
@@ -316,8 +376,9 @@ contains
 
   ! the binary restart of this run, records and order as Mixture writes them
   ! the reference's 8-character id of the model
-  character(len=8) function model_id()
-    select case (model)
+  character(len=8) function model_id(mdl)
+    integer(c_int), intent(in) :: mdl
+    select case (mdl)
     case (G6_TIP4P)
        model_id = 'TIP4P   '
     case (G6_TIP5P)
@@ -336,7 +397,7 @@ contains
   subroutine write_restart()
     character(len=8) :: id
     integer :: c, k
-    id = model_id()
+    id = model_id(model)
     open(13, file='fort.13', form='unformatted', access='sequential', status='replace')
     write(13) '@RCOA'
     write(13) rc, taper
@@ -364,8 +425,16 @@ contains
     write(13) '@TEST'
     write(13) '@MDLP'
     write(13) nsteps
+    ! @NCMP, one @TIE4 per component, then @FIXC @ID08 @WTG2 per component
+    ! (Mixture.F90:2562-2580)
     write(13) '@NCMP'
-    write(13) 1
+    if (mix) then
+       write(13) 2
+       write(13) '@TIE4'
+       write(13) 0, 2
+    else
+       write(13) 1
+    end if
     write(13) '@TIE4'
     write(13) 0, 2
     write(13) '@FIXC'
@@ -376,6 +445,16 @@ contains
     write(13) n
     write(13) (com(:, c), c = 1, 3), ((dx(:, c, k), k = 1, 5), c = 1, 3), &
          ((quat(:, c, k), k = 1, 5), c = 1, 4), ((w(:, c, k) * dt, k = 1, 5), c = 1, 3)
+    if (mix) then
+       write(13) '@FIXC'
+       write(13) fixc
+       write(13) '@ID08'
+       write(13) model_id(model2)
+       write(13) '@WTG2'
+       write(13) n2
+       write(13) (com2(:, c), c = 1, 3), ((dx2(:, c, k), k = 1, 5), c = 1, 3), &
+            ((quat2(:, c, k), k = 1, 5), c = 1, 4), ((w2(:, c, k) * dt, k = 1, 5), c = 1, 3)
+    end if
     close(13)
   end subroutine write_restart
 
@@ -416,7 +495,7 @@ contains
        write(6, *) fixc
     end if
     write(6, '("@ID08")')
-    write(6, '(a8)') model_id()
+    write(6, '(a8)') model_id(model)
     if (velocities) then
        write(6, '("@WTG3")')
        write(6, *) n
@@ -432,6 +511,52 @@ contains
             quat(j, 2, 1), quat(j, 3, 1), quat(j, 4, 1), j = 1, n)
     end if
   end subroutine write_snapshot
+
+  ! n lines x y z a b c d of an @NX4A block into a component's state
+  ! (Rigid_ReadNX4A: order-1 quaternions, higher orders and velocities zero)
+  subroutine read_nx4a(nn, c, q, d, ww)
+    integer(c_int), intent(in) :: nn
+    real(c_double), allocatable, intent(inout) :: c(:,:), q(:,:,:), d(:,:,:), ww(:,:,:)
+    integer :: j
+    if (allocated(c)) deallocate(c, q, d, ww)
+    allocate(c(nn, 3), q(nn, 4, 5), d(nn, 3, 5), ww(nn, 3, 5))
+    q = 0d0
+    d = 0d0
+    ww = 0d0
+    do j = 1, nn
+       read(5, *) c(j, 1), c(j, 2), c(j, 3), q(j, 1, 1), q(j, 2, 1), q(j, 3, 1), q(j, 4, 1)
+    end do
+  end subroutine read_nx4a
+
+  ! the Gear state of an @WTG2 record of the restart into a component's state
+  subroutine read_wtg2(nn, c, q, d, ww)
+    integer(c_int), intent(in) :: nn
+    real(c_double), allocatable, intent(inout) :: c(:,:), q(:,:,:), d(:,:,:), ww(:,:,:)
+    integer :: a, k
+    if (allocated(c)) deallocate(c, q, d, ww)
+    allocate(c(nn, 3), q(nn, 4, 5), d(nn, 3, 5), ww(nn, 3, 5))
+    read(12) (c(:, a), a = 1, 3), ((d(:, a, k), k = 1, 5), a = 1, 3), &
+         ((q(:, a, k), k = 1, 5), a = 1, 4), ((ww(:, a, k), k = 1, 5), a = 1, 3)
+    ww = ww / dt                     ! rigid_toInternalUnit (RigidModule.F90:1639-1652)
+  end subroutine read_wtg2
+
+  ! @LOGD columns of Mixture.F90:2453-2471 for two components: T, P, V, ep
+  ! of the pair sets 1-1, 2-2, 1-2, ek of each component, ep/N, ek/N over
+  ! all molecules (kJ/mol), nose_ep, nose_ek, andersen_ep, andersen_ek (0),
+  ! H, exsum (0), H, emonsum (0)
+  subroutine write_logd2(step, t)
+    integer, intent(in) :: step
+    real(c_double), intent(in) :: t(G6_THERMO_LEN)
+    real(8) :: ep, ek, nm
+    ep = t(1)
+    ek = t(2) + t(3)
+    nm = dble(n + n2)
+    write(6, '(a)') '@LOGD'
+    write(6, '(i10,30(1x,E24.17))') step, t(4), t(26), t(27), parts(1) * I2J * X2KX, &
+         parts(2) * I2J * X2KX, parts(3) * I2J * X2KX, parts(4) * I2J * X2KX, &
+         parts(5) * I2J * X2KX, ep * I2J / nm * X2KX, ek * I2J / nm * X2KX, &
+         t(28) * X2KX, t(29) * X2KX, t(31) * X2KX, t(32) * X2KX, t(14), 0d0, t(14), 0d0
+  end subroutine write_logd2
 
   ! @LOGD columns of Mixture.F90:2453-2471 for one component: T, P, V, ep,
   ! ek, ep/N, ek/N (kJ/mol), nose_ep, nose_ek, andersen_ep, andersen_ek, H,

@@ -108,6 +108,11 @@ def lib():
         L.g6_rx_create.argtypes = [C.POINTER(_H), C.c_int, C.c_void_p, C.c_int, _ip, _ip,
                                    C.c_int, C.c_uint32]
         L.g6_rx_free.argtypes = [_H]
+        L.g6_mixture_create.argtypes = [C.POINTER(_H), _H, _H]
+        L.g6_mixture_free.argtypes = [_H]
+        L.g6_mixture_vreset.argtypes = [_H, C.c_double, C.c_uint32]
+        L.g6_mixture_step.argtypes = [_H, C.c_int]
+        L.g6_mixture_thermo.argtypes = [_H, _dp, _dp]
         L.g6_rx_trials.argtypes = [_H, _dp, _dp, C.POINTER(C.c_int32)]
         L.g6_rx_trials_umbrella.argtypes = [_H, _dp, _dp, C.c_int, _dp, _dp, _dp,
                                             C.POINTER(C.c_int32)]
@@ -608,6 +613,43 @@ def rxpr_bonds(nnode, links):
     a = [i + 1 for i in range(nnode) for _ in nei[i]]
     b = [j + 1 for i in range(nnode) for j in nei[i]]
     return np.array(a, np.int32), np.array(b, np.int32)
+
+
+class Mixture:
+    """Two rigid components stepped together (a g6_mixture: the reference's
+    @NCMP 2 run, Mixture.F90:1833-2290): the first component's Nose
+    thermostat drives both, and the pairs between them are StdForce's."""
+
+    def __init__(self, first, second):
+        self.first, self.second = first, second
+        h = _H(0)
+        check(lib().g6_mixture_create(C.byref(h), first.h, second.h), "g6_mixture_create")
+        self.h = h.value
+
+    def vreset(self, temp, seed):
+        check(lib().g6_mixture_vreset(self.h, float(temp), int(seed)), "g6_mixture_vreset")
+
+    def step(self, nsteps):
+        check(lib().g6_mixture_step(self.h, int(nsteps)), "g6_mixture_step")
+
+    def thermo(self):
+        """(thermo record of the whole system, parts): parts = ep inside the
+        first, inside the second, between them; ek of each component."""
+        t = np.zeros(THERMO_LEN)
+        p = np.zeros(5)
+        check(lib().g6_mixture_thermo(self.h, t, p), "g6_mixture_thermo")
+        return t, p
+
+    def close(self):
+        if self.h:
+            lib().g6_mixture_free(self.h)
+            self.h = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class ReplicaExchange:

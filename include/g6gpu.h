@@ -331,6 +331,35 @@ int g6_mc_get_umbrella(g6_handle h, double *op, double *center, double *coeffi);
 /* beta = 1/temp for the following trials (an exchange's new temperature) */
 int g6_mc_set_temperature(g6_handle h, double temp);
 
+/* ---- mixtures of two rigid components (@NCMP 2 and two @ID08/@NX4A
+ * blocks in a Mixture deck; Mixture.F90:636-712 reads them, 1833-2290 steps
+ * them).  The components are two handles of g6_init / g6_init_box3 with one
+ * box, rc, taper, dt and skin, both G6_FP64 and neither on the barostat or
+ * the cell tiles.  The pairs inside each component keep that component's
+ * force (g6_set_fast); the pairs between them are StdForce's
+ * (StdInteraction.F90:63-630, Interaction2.F90:19-67 without @FAST) with
+ * the Lorentz-Berthelot site parameters of the two models, every pair
+ * whose COM distance is within rc, switched on the COM distance.  One Nose
+ * thermostat (the first component's, g6_set_nose) over 6 (nA + nB) degrees
+ * of freedom drives both.  A component takes part in one mixture at a time
+ * and should not be stepped on its own while in it. */
+typedef int64_t g6_mixture;
+
+int g6_mixture_create(g6_mixture *hm, g6_handle first, g6_handle second);
+int g6_mixture_free(g6_mixture hm);
+/* Rigid_VReset2 of the first component, then of the second, from one
+ * random stream (Mixture.F90:1336-1346) */
+int g6_mixture_vreset(g6_mixture hm, double temp, uint32_t seed);
+/* nsteps MD steps of both components.  G6_ECAPACITY if a capacity of
+ * either component ran out (its state is then not the reference's). */
+int g6_mixture_step(g6_mixture hm, int nsteps);
+/* thermo record of the whole system (layout of g6_get_thermo; thermo[0]
+ * the total pair energy) and, if parts is non-null, parts[5]: the pair
+ * energy inside the first component, inside the second, between them, and
+ * the kinetic energy of each component (internal units) -- the ep(k) and
+ * ekt + ekr columns of Mixture's @LOGD line (Mixture.F90:2453-2460). */
+int g6_mixture_thermo(g6_mixture hm, double *thermo, double *parts);
+
 /* ---- replica ladders (MDReplica.F90, the replica branch of Mixture.F90)
  * The reference runs one process per replica (fakempi) and exchanges
  * temperatures every @INNR steps.  Here the replicas are systems of one

@@ -1,0 +1,157 @@
+"""Two-component MD on the GPU (g6_mixture_*): 448 TIP4P water and 64 OPLS
+methanol in a 29 A box against the reference MD driver's run of the same
+deck (tests/golden/md_mixture_tip4p_meoh.npz, written by
+tests/golden/make_mixture_md.py): @VSET 300 K drawn over the water, then
+the methanol, from one random stream; one Nose thermostat over both; the
+pairs between the components by StdForce with Lorentz-Berthelot site
+parameters.  Every @LOGD column of the 30 steps is compared: T, P, V, the
+pair energy of each pair set (water-water, methanol-methanol,
+water-methanol), the kinetic energy of each component, the per-molecule
+sums, the thermostat energies and H.  The Fortran driver g6md runs the
+reference's deck itself and continues it from its own restart."""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from genesis6_amd import g6
+from tests.golden.configs import parse_logd
+
+pytestmark = pytest.mark.gpu
+I2J, X2KX = 10.0, 1e-3
+COLS = [(c, 1e-12) for c in (1, 2, 4, 5, 6, 7, 8, 9, 10, 11, 12, 15, 17)] + [(3, 1e-15)]
+
+
+def logd_row(t, p, nmol):
+    ep, ek = t[0], t[1] + t[2]
+    return [t[3], t[25], t[26], p[0] * I2J * X2KX, p[1] * I2J * X2KX, p[2] * I2J * X2KX,
+            p[3] * I2J * X2KX, p[4] * I2J * X2KX, ep * I2J / nmol * X2KX,
+            ek * I2J / nmol * X2KX, t[27] * X2KX, t[28] * X2KX, t[30] * X2KX, t[31] * X2KX,
+            t[13], 0.0, t[13], 0.0]
+
+
+def mixture(f, fast=False):
+    L, rc, taper, dt = float(f["L"]), float(f["rc"]), float(f["taper"]), float(f["dt"])
+    w, m = f["water"], f["meoh"]
+    sw = g6.System(g6.TIP4P, len(w), L, rc, taper, dt, prec=g6.FP64)
+    sm = g6.System(g6.OPLSMEOH, len(m), L, rc, taper, dt, prec=g6.FP64)
+    for s, x in ((sw, w), (sm, m)):
+        s.set_fast(fast)
+        s.upload(x[:, :3], x[:, 3:])
+    sw.set_nose(300.0, 7.5e5)
+    mx = g6.Mixture(sw, sm)
+    mx.vreset(300.0, int(f["seed"]))
+    return sw, sm, mx
+
+
+def test_mixture_log_matches_reference_driver(golden):
+    f = golden("md_mixture_tip4p_meoh")
+    ref = f["logd"]
+    print("mixture 448 TIP4P + 64 OPLSMEOH, %d steps" % len(ref), flush=True)
+    sw, sm, mx = mixture(f)
+    nmol = sw.n + sm.n
+    got = []
+    for k in range(len(ref)):
+        mx.step(1)
+        t, p = mx.thermo()
+        got.append([k] + logd_row(t, p, nmol))
+    got = np.array(got)
+    assert got.shape == ref.shape
+    np.testing.assert_array_equal(got[:, 0], ref[:, 0])
+    # T, P, V, ep of the three sets, ek of both, per molecule, Nose, H: each
+    # within 1e-12 of its column's largest magnitude (the differences are
+    # summation order; measured at about 1e-14 and below)
+    for col, tol in COLS:
+        err = np.abs(got[:, col] - ref[:, col]).max() / (np.abs(ref[:, col]).max() + 1e-300)
+        print("  column %d: %.2e" % (col, err))
+        assert err <= tol, (col, err)
+    for c in (13, 14, 16, 18):
+        assert np.all(ref[:, c] == 0.0)
+    mx.close()
+    sw.close()
+    sm.close()
+
+
+def test_mixture_steps_in_chunks_as_one_by_one(golden):
+    """k steps in one call end where k calls of one step end, bit for bit
+    (the state between calls is the corrected state), and the methanol keeps
+    its own thermostat buffer untouched."""
+    f = golden("md_mixture_tip4p_meoh")
+    ends = []
+    for chunks in ([1] * 6, [4, 2]):
+        sw, sm, mx = mixture(f)
+        nose_m = sm.nose()
+        for c in chunks:
+            mx.step(c)
+        t, p = mx.thermo()
+        ends.append((t.copy(), p.copy(), sw.download()[0], sm.download()[0]))
+        assert np.array_equal(sm.nose(), nose_m)
+        mx.close()
+        sw.close()
+        sm.close()
+    for a, b in zip(ends[0], ends[1]):
+        assert np.array_equal(a, b)
+
+
+def test_mixture_refuses_mismatched_components(golden):
+    f = golden("md_mixture_tip4p_meoh")
+    L, rc, taper, dt = float(f["L"]), float(f["rc"]), float(f["taper"]), float(f["dt"])
+    a = g6.System(g6.TIP4P, 448, L, rc, taper, dt, prec=g6.FP64)
+    for kw in ({"prec": g6.FP32}, {"L": L + 1.0}, {"rc": rc - 1.0}, {"dt": 0.001}):
+        args = {"L": L, "rc": rc, "dt": dt, "prec": g6.FP64}
+        args.update(kw)
+        b = g6.System(g6.OPLSMEOH, 64, args["L"], args["rc"], taper, args["dt"],
+                      prec=args["prec"])
+        with pytest.raises(g6.G6Error):
+            g6.Mixture(a, b)
+        b.close()
+    with pytest.raises(g6.G6Error):
+        g6.Mixture(a, a)
+    a.close()
+
+
+EXE = os.path.abspath(os.path.join(os.path.dirname(__file__), "..", "genesis6_amd", "lib",
+                                   "g6md"))
+
+
+def run_g6md(deck, d, *args):
+    r = subprocess.run([EXE] + list(args), input=deck, capture_output=True, text=True,
+                       timeout=120, cwd=d)
+    assert r.returncode == 0, r.stderr
+    return parse_logd(r.stdout)
+
+
+def close_to(got, ref):
+    assert got.shape == ref.shape
+    np.testing.assert_array_equal(got[:, 0], ref[:, 0])
+    for col, tol in COLS:
+        err = np.abs(got[:, col] - ref[:, col]).max() / (np.abs(ref[:, col]).max() + 1e-300)
+        assert err <= tol, (col, err)
+
+
+def test_g6md_runs_the_two_component_deck(golden):
+    """The reference's own deck (two @ID08/@NX4A blocks) through the Fortran
+    driver: its @LOGD lines carry the three pair energies and two kinetic
+    energies as the reference's do."""
+    f = golden("md_mixture_tip4p_meoh")
+    print("g6md mixture deck", flush=True)
+    with tempfile.TemporaryDirectory() as d:
+        close_to(run_g6md(str(f["deck"]), d, "fp64"), f["logd"])
+
+
+def test_g6md_mixture_restart_continues_the_run(golden):
+    """20 steps, the fort.13 restart (@NCMP 2, a @TIE4 per component, then
+    @FIXC @ID08 @WTG2 per component) read back as fort.12, 10 more steps:
+    the last 10 log lines are the reference's single run's."""
+    f = golden("md_mixture_tip4p_meoh")
+    deck = str(f["deck"]).replace("@MDLP\n30\n", "@MDLP\n20\n", 1)
+    assert "@MDLP\n20\n" in deck
+    print("g6md mixture restart", flush=True)
+    with tempfile.TemporaryDirectory() as d:
+        first = run_g6md(deck, d, "fp64")
+        close_to(first, f["logd"][:20])
+        os.rename(os.path.join(d, "fort.13"), os.path.join(d, "fort.12"))
+        rest = run_g6md("@MDLP\n10\n", d, "fp64")
+    close_to(rest, f["logd"][20:])
