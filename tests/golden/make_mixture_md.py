@@ -7,7 +7,9 @@ box of 29 A; the 64 sites with even lattice indices on all three axes hold
 methanol, the other 448 water; orientations are uniform random quaternions
 (normalised Gaussian 4-vectors, seed 7).  rc 8.5 A switched over 0.5 A,
 @VOXA rc, so the reference grid has 3 x 3 x 3 cells; dt 2 fs;
-@VSET 300 K; the Nose thermostat at 300 K; 30 steps, one @LOGD row each.
+@VSET 300 K; the Nose thermostat at 300 K; 30 steps, one @LOGD row each;
+the same deck once more with @FAST 1 (the reference's custom TIP4P,
+methanol and TIP4P x methanol kernels, Interaction2.F90:19-59).
 The deck is the single-component deck of mixture_deck with a second @ID08
 OPLSMEOH / @NX4A block after the water's (Mixture.F90:636-712 adds a
 component for each @ID08 + @NX4A pair).  The Mixture binary is built on
@@ -62,16 +64,23 @@ def main():
     if built:
         subprocess.run(["make", "-f", os.path.join(ROOT, "oracle", "ref.mk"), "mixture"],
                        cwd=ROOT, check=True, capture_output=True)
+    fast = dk.replace("@ID08", "@FAST\n1\n@ID08", 1)
     try:
         with tempfile.TemporaryDirectory() as d:
             text = run(exe, d, dk)
+        with tempfile.TemporaryDirectory() as d:
+            text_fast = run(exe, d, fast)
     finally:
         if built:
             os.remove(exe)
     logd = parse_logd(text)
     print(logd.shape, "T %.3f -> %.3f" % (logd[0, 1], logd[-1, 1]),
           "ep(ww, mm, wm) %s" % logd[0, 4:7], "H %.6f -> %.6f" % (logd[0, -4], logd[-1, -4]))
+    logd_fast = parse_logd(text_fast)
+    print("@FAST 1: largest relative difference per column from StdForce's run",
+          (np.abs(logd_fast - logd).max(0) / (np.abs(logd).max(0) + 1e-300)).max())
     np.savez_compressed(os.path.join(HERE, "md_mixture_tip4p_meoh.npz"), deck=dk, logd=logd,
+                        deck_fast=fast, logd_fast=logd_fast,
                         water=water, meoh=meoh, L=L, rc=RC, taper=TAPER, dt=DT, seed=SEED,
                         nsteps=NSTEPS)
 

@@ -155,3 +155,25 @@ def test_g6md_mixture_restart_continues_the_run(golden):
         os.rename(os.path.join(d, "fort.13"), os.path.join(d, "fort.12"))
         rest = run_g6md("@MDLP\n10\n", d, "fp64")
     close_to(rest, f["logd"][20:])
+
+
+def test_g6md_mixture_with_fast_kernels(golden):
+    """With @FAST 1 the reference runs its custom TIP4P, methanol and
+    TIP4P x methanol kernels (Interaction2.F90:19-59).  Its two methanol
+    kernels assign each pair's virial to vrmat instead of adding it
+    (OPLSMeOHModule.F90:500-511, 1655-1666), so the reference's pressure then
+    holds the last pair's virial of those two sets alone; every other column
+    (energies, temperature, H: the trajectory) equals the StdForce run's.
+    g6md with @FAST 1 gives the reference's @FAST columns except P, and the
+    pressure of the full virial, the StdForce run's P."""
+    f = golden("md_mixture_tip4p_meoh")
+    ref, ref_fast = f["logd"], f["logd_fast"]
+    print("g6md mixture deck with @FAST 1", flush=True)
+    with tempfile.TemporaryDirectory() as d:
+        got = run_g6md(str(f["deck_fast"]), d, "fp64")
+    close_to(got, ref)
+    pfast = np.abs(ref_fast[:, 2] - ref[:, 2]).max() / np.abs(ref[:, 2]).max()
+    assert pfast > 0.5, pfast            # the reference's @FAST pressure is another number
+    other = [c for c, _ in COLS if c != 2]
+    err = np.abs(got[:, other] - ref_fast[:, other]).max(0) / np.abs(ref_fast[:, other]).max(0)
+    assert err.max() <= 1e-12, err
