@@ -23,6 +23,7 @@
 // molecule's force and lab torque to its component's force array.
 #include <math.h>
 
+#include <initializer_list>
 #include <mutex>
 #include <vector>
 
@@ -38,9 +39,9 @@ namespace g6 {
 // the other (qq, aa, bb as StdInteraction.F90:201-221 forms them)
 struct CrossArgs {
   int ni, nj, nsi, nsj;    // molecules and sites (CoM site last) of each side
-  const double *reci;      // records of the row component (FP64: COM, site offsets)
+  const double *reci;      // records of the row component (COM, site offsets)
   const double *recj;
-  int rs;                  // record stride (doubles)
+  int rs;                  // record stride (doubles): kRecD FP64 offsets, kRecF FP32
   double *frc;             // the row component's force and lab torque [6][ni]
   double *part;            // [nblk][kBP] pair partials of the launch
   double L[3], invL[3];
@@ -49,6 +50,13 @@ struct CrossArgs {
 };
 
 constexpr int kMixThreads = 128;
+
+// site offset q (site * 3 + axis) of a record: FP64 records hold them as
+// doubles after the COM, FP32 records as floats (the pair precision's
+// rounding of the same lab vectors)
+__device__ inline double site_off(const double *rec, int rs, int q) {
+  return rs == kRecF ? (double)reinterpret_cast<const float *>(rec + 4)[q] : rec[4 + q];
+}
 
 // one thread per molecule i of the row component, every molecule j of the
 // other in index order: the COM test dd < rc^2 (minimum image per axis), S
@@ -66,7 +74,7 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_cross(CrossArgs x) {
     double si[kMaxSite][3];
     for (int s = 0; s < kMaxSite; s++) si[s][0] = si[s][1] = si[s][2] = 0.0;
     for (int s = 0; s + 1 < x.nsi; s++)
-      for (int a = 0; a < 3; a++) si[s][a] = ri[4 + s * 3 + a];
+      for (int a = 0; a < 3; a++) si[s][a] = site_off(ri, x.rs, s * 3 + a);
     double F[3] = {0, 0, 0}, Tq[3] = {0, 0, 0};
     for (int j = 0; j < x.nj; j++) {
       const double *rj = x.recj + (size_t)j * x.rs;
@@ -92,9 +100,9 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_cross(CrossArgs x) {
         const int a = tm.a, b = tm.b;
         double dx = d[0] + si[a][0], dy = d[1] + si[a][1], dz = d[2] + si[a][2];
         if (b + 1 < x.nsj) {
-          dx = dx - rj[4 + b * 3];
-          dy = dy - rj[4 + b * 3 + 1];
-          dz = dz - rj[4 + b * 3 + 2];
+          dx = dx - site_off(rj, x.rs, b * 3);
+          dy = dy - site_off(rj, x.rs, b * 3 + 1);
+          dz = dz - site_off(rj, x.rs, b * 3 + 2);
         }
         const double inv2 = 1.0 / (dx * dx + dy * dy + dz * dz);
         double e0 = 0.0, f0 = 0.0;
@@ -409,12 +417,13 @@ int g6_mixture_create(g6_mixture *hm, g6_handle ha, g6_handle hb) {
   if (!hm) return G6_EARG;
   System *a = system_of(ha), *b = system_of(hb);
   if (!a || !b || a == b) return G6_EARG;
-  // one box, cutoff, time step and pair-list skin; FP64 rows on a fixed box
+  // one box, cutoff, time step, pair-list skin and pair precision; a fixed
+  // box; the rows of both directions (a component on the cell tiles moves to
+  // them, as a ladder's systems do)
   for (int k = 0; k < 3; k++)
     if (a->Lb[k] != b->Lb[k]) return G6_EARG;
   if (a->rc != b->rc || a->taper != b->taper || a->dt != b->dt || a->skin != b->skin ||
-      a->prec != G6_FP64 || b->prec != G6_FP64 || a->andersen || b->andersen || a->tiles ||
-      b->tiles)
+      a->prec != b->prec || a->andersen || b->andersen)
     return G6_EARG;
   {
     std::lock_guard<std::mutex> lk(g_mxmu);
@@ -466,10 +475,13 @@ int g6_mixture_step(g6_mixture hm, int nsteps) {
   Mix *m = mix_of(hm);
   if (!m || nsteps < 0) return G6_EARG;
   System &a = *m->a, &b = *m->b;
-  if (system_of(m->ha) != &a || system_of(m->hb) != &b || a.andersen || b.andersen || a.tiles ||
-      b.tiles)
+  if (system_of(m->ha) != &a || system_of(m->hb) != &b || a.andersen || b.andersen)
     return G6_EARG;
   if (nsteps == 0) return 0;
+  for (System *s : {&a, &b}) {
+    const int rg = use_row_grid(*s);
+    if (rg != 0) return rg;
+  }
   G6_CHECK(hipStreamSynchronize(b.stream));
   // the launches of the second component go to the first's stream and
   // its correctors read the first's thermostat (kernels take the buffers

@@ -42,7 +42,7 @@ void mlp_launch_forward(const Mlp &net, long long m, const float *x, const int *
 int mlp_launch_grad(Mlp &net, long long m, const float *x, const float *t, const int *idx,
                     hipStream_t s);
 // one Adam step with the gradient in net.grad; refreshes the BF16 copies
-void mlp_launch_adam(Mlp &net, float lr, hipStream_t s);
+void mlp_launch_adam(Mlp &net, float lr, hipStream_t s, float *loss_out = nullptr);
 Mlp *mlp_of(int64_t h);  // nullptr for an unknown g6_mlp handle
 
 }  // namespace g6

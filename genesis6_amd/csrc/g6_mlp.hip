@@ -339,10 +339,13 @@ __global__ void k_mlp_reduce(const float *__restrict__ part, int nb, float *__re
   grad[k] = k == kMlpParams ? s / (float)m : s;
 }
 
+// Adam step on every parameter; loss_out (if set) receives the batch loss
+// grad[kMlpParams], so a training epoch keeps its losses on the device
 __global__ void k_mlp_adam(float *__restrict__ p, float *__restrict__ mo, float *__restrict__ ve,
                            const float *__restrict__ grad, uint16_t *__restrict__ wb, float lr,
-                           float c1, float c2) {
+                           float c1, float c2, float *__restrict__ loss_out) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k == 0 && loss_out) *loss_out = grad[kMlpParams];
   if (k >= kMlpParams) return;
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   const float gk = grad[k];
@@ -384,12 +387,12 @@ int mlp_launch_grad(Mlp &net, long long m, const float *x, const float *t, const
   return 0;
 }
 
-void mlp_launch_adam(Mlp &net, float lr, hipStream_t s) {
+void mlp_launch_adam(Mlp &net, float lr, hipStream_t s, float *loss_out) {
   net.adam_t++;
   const double c1 = 1.0 / (1.0 - pow(0.9, (double)net.adam_t));
   const double c2 = 1.0 / (1.0 - pow(0.999, (double)net.adam_t));
   k_mlp_adam<<<(kMlpParams + 255) / 256, 256, 0, s>>>(net.p, net.adam_m, net.adam_v, net.grad,
-                                                      net.wb, lr, (float)c1, (float)c2);
+                                                      net.wb, lr, (float)c1, (float)c2, loss_out);
 }
 
 static std::mutex g_mmu;
@@ -550,10 +553,11 @@ int g6_mlp_train(g6_mlp hm, int64_t m, const float *x, const float *t, int epoch
                  float lr, uint32_t seed, float *loss_per_epoch) {
   Mlp *n = mget(hm);
   if (!n || m < 1 || !x || !t || epochs < 1 || batch < 16 || !(lr > 0)) return G6_EARG;
-  DevBuf dx, dt, di;
+  DevBuf dx, dt, di, dl;
   G6_CHECK(hipMalloc(&dx.p, sizeof(float) * kMlpIn * m));
   G6_CHECK(hipMalloc(&dt.p, sizeof(float) * m));
   G6_CHECK(hipMalloc(&di.p, sizeof(int) * m));
+  G6_CHECK(hipMalloc(&dl.p, sizeof(float) * (m / batch > 0 ? m / batch : 1)));
   G6_CHECK(hipMemcpy(dx.p, x, sizeof(float) * kMlpIn * m, hipMemcpyHostToDevice));
   G6_CHECK(hipMemcpy(dt.p, t, sizeof(float) * m, hipMemcpyHostToDevice));
   McHost rng;
@@ -570,15 +574,18 @@ int g6_mlp_train(g6_mlp hm, int64_t m, const float *x, const float *t, int epoch
     G6_CHECK(hipMemcpyAsync(di.p, perm.data(), sizeof(int) * m, hipMemcpyHostToDevice, n->stream));
     const int64_t nb = m / batch;  // full batches only
     lossb.assign(nb > 0 ? nb : 1, 0.0f);
+    // the batches run back to back; each batch loss stays on the device
+    // (written by the Adam step) until the epoch's one copy back
     for (int64_t b = 0; b < nb; b++) {
       const int rc = mlp_launch_grad(*n, batch, (const float *)dx.p, (const float *)dt.p,
                                      (const int *)di.p + b * batch, n->stream);
       if (rc != 0) return rc;
-      G6_CHECK(hipMemcpyAsync(&lossb[b], n->grad + kMlpParams, sizeof(float),
-                              hipMemcpyDeviceToHost, n->stream));
-      mlp_launch_adam(*n, lr, n->stream);
+      mlp_launch_adam(*n, lr, n->stream, (float *)dl.p + b);
     }
     G6_CHECK(hipGetLastError());
+    if (nb > 0)
+      G6_CHECK(hipMemcpyAsync(lossb.data(), dl.p, sizeof(float) * nb, hipMemcpyDeviceToHost,
+                              n->stream));
     G6_CHECK(hipStreamSynchronize(n->stream));
     double s = 0.0;
     for (float v : lossb) s += v;

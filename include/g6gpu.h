@@ -334,12 +334,14 @@ int g6_mc_set_temperature(g6_handle h, double temp);
 /* ---- mixtures of two rigid components (@NCMP 2 and two @ID08/@NX4A
  * blocks in a Mixture deck; Mixture.F90:636-712 reads them, 1833-2290 steps
  * them).  The components are two handles of g6_init / g6_init_box3 with one
- * box, rc, taper, dt and skin, both G6_FP64 and neither on the barostat or
- * the cell tiles.  The pairs inside each component keep that component's
+ * box, rc, taper, dt, skin and precision, neither on the barostat; a
+ * component on the cell tiles moves to the rows.  The pairs inside each
+ * component keep that component's
  * force (g6_set_fast); the pairs between them are StdForce's
  * (StdInteraction.F90:63-630, Interaction2.F90:19-67 without @FAST) with
  * the Lorentz-Berthelot site parameters of the two models, every pair
- * whose COM distance is within rc, switched on the COM distance.  One Nose
+ * whose COM distance is within rc, switched on the COM distance, in FP64
+ * from the site offsets of the records (FP32 ones under G6_FP32).  One Nose
  * thermostat (the first component's, g6_set_nose) over 6 (nA + nB) degrees
  * of freedom drives both.  A component takes part in one mixture at a time
  * and should not be stepped on its own while in it. */

@@ -95,11 +95,35 @@ def test_mixture_steps_in_chunks_as_one_by_one(golden):
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("skin", ["0", "1.0"])
+def test_g6md_fp32_mixture_follows_the_reference(golden, skin):
+    """FP32 pair arithmetic (the pairs between components in FP64 from the
+    FP32 site offsets of the records), with and without a pair-list skin:
+    the temperature, the three pair energies and H follow the reference's
+    FP64 run over the 30 steps."""
+    f = golden("md_mixture_tip4p_meoh")
+    ref = f["logd"]
+    print("g6md mixture deck fp32 skin", skin, flush=True)
+    with tempfile.TemporaryDirectory() as d:
+        got = run_g6md(str(f["deck"]), d, "fp32", skin)
+    assert got.shape == ref.shape
+    # T and H against their own size; each pair set's energy against the
+    # largest pair energy (the methanol-methanol set is a small sum of large
+    # terms of both signs, so its own size is no scale for FP32 rounding)
+    ep_scale = np.abs(ref[:, 4:7]).max()
+    for col in (1, 4, 5, 6, 15):
+        scale = ep_scale if 4 <= col <= 6 else np.abs(ref[:, col]).max()
+        err = np.abs(got[:, col] - ref[:, col]).max() / scale
+        print("  column %d: %.2e" % (col, err))
+        assert err <= 1e-5, (col, err)
+
+
 def test_mixture_refuses_mismatched_components(golden):
     f = golden("md_mixture_tip4p_meoh")
     L, rc, taper, dt = float(f["L"]), float(f["rc"]), float(f["taper"]), float(f["dt"])
     a = g6.System(g6.TIP4P, 448, L, rc, taper, dt, prec=g6.FP64)
     for kw in ({"prec": g6.FP32}, {"L": L + 1.0}, {"rc": rc - 1.0}, {"dt": 0.001}):
+        # precision, box, cutoff, time step: each must be the first's
         args = {"L": L, "rc": rc, "dt": dt, "prec": g6.FP64}
         args.update(kw)
         b = g6.System(g6.OPLSMEOH, 64, args["L"], args["rc"], taper, args["dt"],
