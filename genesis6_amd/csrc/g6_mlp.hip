@@ -359,6 +359,46 @@ __global__ void k_mlp_adam(float *__restrict__ p, float *__restrict__ mo, float 
   else if (k >= kOffW2 && k < kOffB2) wb[kMlpHid * kMlpIn + (k - kOffW2)] = f2bf(pk);
 }
 
+// One training step after k_mlp<true>: each thread sums its parameter's
+// block partials (k_mlp_reduce's order) and applies k_mlp_adam's update to
+// it; the loss entry goes to loss_out.  Equal to k_mlp_reduce followed by
+// k_mlp_adam, with one launch fewer.
+__global__ void k_mlp_reduce_adam(const float *__restrict__ part, int nb, long long m,
+                                  float *__restrict__ p, float *__restrict__ mo,
+                                  float *__restrict__ ve, float *__restrict__ grad,
+                                  uint16_t *__restrict__ wb, float lr, float c1, float c2,
+                                  float *__restrict__ loss_out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > kMlpParams) return;
+  float s = 0.0f;
+  int b = 0;
+  for (; b + 16 <= nb; b += 16) {
+    float v[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) v[u] = part[(size_t)(b + u) * (kMlpParams + 1) + k];
+#pragma unroll
+    for (int u = 0; u < 16; u++) s += v[u];
+  }
+  for (; b < nb; b++) s += part[(size_t)b * (kMlpParams + 1) + k];
+  if (k == kMlpParams) {
+    const float loss = s / (float)m;
+    grad[k] = loss;
+    if (loss_out) *loss_out = loss;
+    return;
+  }
+  grad[k] = s;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  const float gk = s;
+  const float m1 = b1 * mo[k] + (1.0f - b1) * gk;
+  const float v1 = b2 * ve[k] + (1.0f - b2) * gk * gk;
+  mo[k] = m1;
+  ve[k] = v1;
+  const float pk = p[k] - lr * (m1 * c1) / (sqrtf(v1 * c2) + eps);
+  p[k] = pk;
+  if (k < kOffB1) wb[k] = f2bf(pk);
+  else if (k >= kOffW2 && k < kOffB2) wb[kMlpHid * kMlpIn + (k - kOffW2)] = f2bf(pk);
+}
+
 // BF16 copies of W1 and W2 from the master weights
 __global__ void k_mlp_cast(const float *__restrict__ p, uint16_t *__restrict__ wb) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -384,6 +424,23 @@ int mlp_launch_grad(Mlp &net, long long m, const float *x, const float *t, const
   k_mlp<true><<<(unsigned)blocks, 256, 0, s>>>(net.p, net.wb, net.norm, m, x, t, idx, nullptr,
                                                net.part, 0, nullptr);
   k_mlp_reduce<<<(kMlpParams + 1 + 255) / 256, 256, 0, s>>>(net.part, (int)blocks, net.grad, m);
+  return 0;
+}
+
+// gradient of one batch and the Adam step, three launches in two
+static int mlp_launch_train_step(Mlp &net, long long m, const float *x, const float *t,
+                                 const int *idx, float lr, float *loss_out, hipStream_t s) {
+  if (m <= 0) return G6_EARG;
+  long long blocks = ((m + 15) / 16 + 3) / 4;
+  if (blocks > net.part_blocks) blocks = net.part_blocks;
+  k_mlp<true><<<(unsigned)blocks, 256, 0, s>>>(net.p, net.wb, net.norm, m, x, t, idx, nullptr,
+                                               net.part, 0, nullptr);
+  net.adam_t++;
+  const double c1 = 1.0 / (1.0 - pow(0.9, (double)net.adam_t));
+  const double c2 = 1.0 / (1.0 - pow(0.999, (double)net.adam_t));
+  k_mlp_reduce_adam<<<(kMlpParams + 1 + 255) / 256, 256, 0, s>>>(
+      net.part, (int)blocks, m, net.p, net.adam_m, net.adam_v, net.grad, net.wb, lr, (float)c1,
+      (float)c2, loss_out);
   return 0;
 }
 
@@ -553,43 +610,63 @@ int g6_mlp_train(g6_mlp hm, int64_t m, const float *x, const float *t, int epoch
                  float lr, uint32_t seed, float *loss_per_epoch) {
   Mlp *n = mget(hm);
   if (!n || m < 1 || !x || !t || epochs < 1 || batch < 16 || !(lr > 0)) return G6_EARG;
-  DevBuf dx, dt, di, dl;
+  const int64_t nb = m / batch;  // full batches only
+  DevBuf dx, dt, di[2], dl;
   G6_CHECK(hipMalloc(&dx.p, sizeof(float) * kMlpIn * m));
   G6_CHECK(hipMalloc(&dt.p, sizeof(float) * m));
-  G6_CHECK(hipMalloc(&di.p, sizeof(int) * m));
-  G6_CHECK(hipMalloc(&dl.p, sizeof(float) * (m / batch > 0 ? m / batch : 1)));
+  G6_CHECK(hipMalloc(&di[0].p, sizeof(int) * m));
+  G6_CHECK(hipMalloc(&di[1].p, sizeof(int) * m));
+  G6_CHECK(hipMalloc(&dl.p, sizeof(float) * (size_t)epochs * (nb > 0 ? nb : 1)));
   G6_CHECK(hipMemcpy(dx.p, x, sizeof(float) * kMlpIn * m, hipMemcpyHostToDevice));
   G6_CHECK(hipMemcpy(dt.p, t, sizeof(float) * m, hipMemcpyHostToDevice));
   McHost rng;
   rng.seed(seed);
   std::vector<int> perm(m);
   for (int64_t r = 0; r < m; r++) perm[r] = (int)r;
-  std::vector<float> lossb;
-  for (int e = 0; e < epochs; e++) {
-    // Fisher-Yates shuffle from the MT19937 stream
-    for (int64_t r = m - 1; r > 0; r--) {
-      const int64_t k = (int64_t)(rng.next() % (uint32_t)(r + 1));
-      std::swap(perm[r], perm[k]);
+  // Fisher-Yates shuffle from the MT19937 stream, one per epoch in turn.
+  // The draws do not depend on the array, so they are made first and the
+  // swap loop prefetches the element it will swap 32 steps ahead (the array
+  // is far larger than the caches).
+  std::vector<uint32_t> ks(m > 1 ? m - 1 : 1);
+  auto shuffle = [&]() {
+    const int64_t cnt = m - 1;
+    for (int64_t q = 0; q < cnt; q++) ks[q] = rng.next() % (uint32_t)(m - q);
+    for (int64_t q = 0; q < cnt; q++) {
+      if (q + 32 < cnt) __builtin_prefetch(&perm[ks[q + 32]], 1);
+      std::swap(perm[m - 1 - q], perm[ks[q]]);
     }
-    G6_CHECK(hipMemcpyAsync(di.p, perm.data(), sizeof(int) * m, hipMemcpyHostToDevice, n->stream));
-    const int64_t nb = m / batch;  // full batches only
-    lossb.assign(nb > 0 ? nb : 1, 0.0f);
-    // the batches run back to back; each batch loss stays on the device
-    // (written by the Adam step) until the epoch's one copy back
+  };
+  shuffle();
+  G6_CHECK(hipMemcpy(di[0].p, perm.data(), sizeof(int) * m, hipMemcpyHostToDevice));
+  for (int e = 0; e < epochs; e++) {
+    // the epoch's batches run back to back on the device; each batch loss
+    // stays there (written by the Adam step).  Meanwhile the host shuffles
+    // the next epoch's order into the other index buffer.
+    const int *idx = (const int *)di[e & 1].p;
     for (int64_t b = 0; b < nb; b++) {
-      const int rc = mlp_launch_grad(*n, batch, (const float *)dx.p, (const float *)dt.p,
-                                     (const int *)di.p + b * batch, n->stream);
+      const int rc = mlp_launch_train_step(*n, batch, (const float *)dx.p, (const float *)dt.p,
+                                           idx + b * batch, lr, (float *)dl.p + e * nb + b,
+                                           n->stream);
       if (rc != 0) return rc;
-      mlp_launch_adam(*n, lr, n->stream, (float *)dl.p + b);
     }
     G6_CHECK(hipGetLastError());
-    if (nb > 0)
-      G6_CHECK(hipMemcpyAsync(lossb.data(), dl.p, sizeof(float) * nb, hipMemcpyDeviceToHost,
-                              n->stream));
-    G6_CHECK(hipStreamSynchronize(n->stream));
+    if (e + 1 < epochs) {
+      shuffle();
+      G6_CHECK(hipMemcpyAsync(di[(e + 1) & 1].p, perm.data(), sizeof(int) * m,
+                              hipMemcpyHostToDevice, n->stream));
+      G6_CHECK(hipStreamSynchronize(n->stream));  // perm is the next shuffle's input
+    }
+  }
+  std::vector<float> lossb((size_t)epochs * (nb > 0 ? nb : 1), 0.0f);
+  if (nb > 0)
+    G6_CHECK(hipMemcpyAsync(lossb.data(), dl.p, sizeof(float) * epochs * nb,
+                            hipMemcpyDeviceToHost, n->stream));
+  G6_CHECK(hipStreamSynchronize(n->stream));
+  const int64_t per = nb > 0 ? nb : 1;
+  for (int e = 0; e < epochs; e++) {
     double s = 0.0;
-    for (float v : lossb) s += v;
-    if (loss_per_epoch) loss_per_epoch[e] = (float)(s / (double)lossb.size());
+    for (int64_t b = 0; b < per; b++) s += lossb[(size_t)e * per + b];
+    if (loss_per_epoch) loss_per_epoch[e] = (float)(s / (double)per);
   }
   return 0;
 }
