@@ -18,9 +18,10 @@
 // (Lorentz-Berthelot: sqrt(eps_a eps_b), (sig_a + sig_b)/2, q_a q_b), times
 // the switch S on the COM distance, with the switch's derivative on the COMs.
 // k_mix_cross takes every molecule of one component against every molecule
-// of the other in index order (one thread per molecule, both directions, so
-// every sum has a fixed order and the step is deterministic), and adds each
-// molecule's force and lab torque to its component's force array.
+// of the other (one wave per molecule, its lanes over the other component's
+// molecules in index order, both directions, so every sum has a fixed order
+// and the step is deterministic), and adds each molecule's force and lab
+// torque to its component's force array.
 #include <math.h>
 
 #include <initializer_list>
@@ -49,7 +50,7 @@ struct CrossArgs {
   StdTable t;
 };
 
-constexpr int kMixThreads = 128;
+constexpr int kMixThreads = 256;  // four waves, one molecule each
 
 // site offset q (site * 3 + axis) of a record: FP64 records hold them as
 // doubles after the COM, FP32 records as floats (the pair precision's
@@ -58,25 +59,29 @@ __device__ inline double site_off(const double *rec, int rs, int q) {
   return rs == kRecF ? (double)reinterpret_cast<const float *>(rec + 4)[q] : rec[4 + q];
 }
 
-// one thread per molecule i of the row component, every molecule j of the
-// other in index order: the COM test dd < rc^2 (minimum image per axis), S
-// and dS on the COM distance, StdForce's site terms; F and tau of i added to
-// its force array; energy, virial and pair count summed per block in a fixed
-// order (halved: every pair is met from both sides)
+// One wave per molecule i of the row component; lane l takes the molecules
+// j = l, l + 64, ... of the other component in index order: the COM test
+// dd < rc^2 (minimum image per axis), S and dS on the COM distance,
+// StdForce's site terms.  Force, lab torque, energy, virial and pair count
+// are summed per lane, then over the wave in a fixed order (lane 63 holds
+// the totals); i's force and torque are added to its force array, and the
+// block's energy, virial and count (the four waves in order, halved: every
+// pair is met from both sides) go to its partials.
 __global__ __launch_bounds__(kMixThreads) void k_mix_cross(CrossArgs x) {
-  __shared__ double s_v[12][kMixThreads];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double s_v[kMixThreads / 64][12];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int i = blockIdx.x * (kMixThreads / 64) + wid;
   double acc[12];
 #pragma unroll
   for (int q = 0; q < 12; q++) acc[q] = 0.0;
-  if (i < x.ni) {
+  double F[3] = {0, 0, 0}, Tq[3] = {0, 0, 0};
+  if (i < x.ni) {  // uniform over the wave
     const double *ri = x.reci + (size_t)i * x.rs;
     double si[kMaxSite][3];
     for (int s = 0; s < kMaxSite; s++) si[s][0] = si[s][1] = si[s][2] = 0.0;
     for (int s = 0; s + 1 < x.nsi; s++)
       for (int a = 0; a < 3; a++) si[s][a] = site_off(ri, x.rs, s * 3 + a);
-    double F[3] = {0, 0, 0}, Tq[3] = {0, 0, 0};
-    for (int j = 0; j < x.nj; j++) {
+    for (int j = lane; j < x.nj; j += 64) {
       const double *rj = x.recj + (size_t)j * x.rs;
       double d[3];
       for (int a = 0; a < 3; a++) {
@@ -142,23 +147,31 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_cross(CrossArgs x) {
         for (int c = 0; c < 3; c++) acc[1 + 3 * a + c] = fma(d[a], Fp[c], acc[1 + 3 * a + c]);
       acc[10] += 1.0;
     }
-    const int ni = x.ni;
-    for (int a = 0; a < 3; a++) {
-      x.frc[a * ni + i] = x.frc[a * ni + i] + F[a];
-      x.frc[(3 + a) * ni + i] = x.frc[(3 + a) * ni + i] + Tq[a];
+  }
+#pragma unroll
+  for (int q = 0; q < 12; q++) acc[q] = wsum63(acc[q]);
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    F[a] = wsum63(F[a]);
+    Tq[a] = wsum63(Tq[a]);
+  }
+  if (lane == 63) {
+    if (i < x.ni) {
+      const int ni = x.ni;
+      for (int a = 0; a < 3; a++) {
+        x.frc[a * ni + i] = x.frc[a * ni + i] + F[a];
+        x.frc[(3 + a) * ni + i] = x.frc[(3 + a) * ni + i] + Tq[a];
+      }
     }
-  }
-  // block sums in thread order (a fixed tree)
 #pragma unroll
-  for (int q = 0; q < 12; q++) s_v[q][threadIdx.x] = acc[q];
+    for (int q = 0; q < 12; q++) s_v[wid][q] = acc[q];
+  }
   __syncthreads();
-  for (int w = kMixThreads / 2; w >= 1; w >>= 1) {
-    if ((int)threadIdx.x < w)
-#pragma unroll
-      for (int q = 0; q < 12; q++) s_v[q][threadIdx.x] += s_v[q][threadIdx.x + w];
-    __syncthreads();
+  if (threadIdx.x < 12) {
+    double v = 0.0;
+    for (int w = 0; w < kMixThreads / 64; w++) v += s_v[w][threadIdx.x];
+    x.part[(size_t)blockIdx.x * kBP + threadIdx.x] = 0.5 * v;
   }
-  if (threadIdx.x < 12) x.part[(size_t)blockIdx.x * kBP + threadIdx.x] = 0.5 * s_v[threadIdx.x][0];
 }
 
 // The step close of a mixture (the single-component step close for two
@@ -437,8 +450,9 @@ int g6_mixture_create(g6_mixture *hm, g6_handle ha, g6_handle hb) {
   m->b = b;
   make_std_table(a->mt, b->mt, &m->ab);
   make_std_table(b->mt, a->mt, &m->ba);
-  m->nblk_a = (a->n + kMixThreads - 1) / kMixThreads;
-  m->nblk_b = (b->n + kMixThreads - 1) / kMixThreads;
+  constexpr int per = kMixThreads / 64;  // molecules per block (a wave each)
+  m->nblk_a = (a->n + per - 1) / per;
+  m->nblk_b = (b->n + per - 1) / per;
   if (hipMalloc(&m->part, sizeof(double) * 2 * (m->nblk_a + m->nblk_b) * kBP) != hipSuccess ||
       hipMalloc(&m->parts, sizeof(double) * 8) != hipSuccess) {
     (void)hipFree(m->part);
