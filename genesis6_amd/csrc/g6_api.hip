@@ -702,6 +702,7 @@ static void launch_steps(System &s, int k) {
 }
 
 void drop_graphs(System &s) {
+  s.graph_gen++;
   if (s.graph) (void)hipGraphExecDestroy(s.graph);
   s.graph = nullptr;
   for (int q = 0; q < 4; q++) {
@@ -712,7 +713,7 @@ void drop_graphs(System &s) {
 }
 
 // steps per captured hipGraph (G6_GRAPH_STEPS; 0 disables graphs)
-static int graph_steps() {
+int graph_steps() {
   const char *e = getenv("G6_GRAPH_STEPS");
   int v = e ? atoi(e) : 50;
   return v < 0 ? 0 : (v > 1000 ? 1000 : v);

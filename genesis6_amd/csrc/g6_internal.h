@@ -292,6 +292,7 @@ struct System {
   hipGraphExec_t graph_k[4] = {nullptr, nullptr, nullptr, nullptr};
   int graph_k_steps[4] = {0, 0, 0, 0};
   int graph_k_next = 0;
+  int graph_gen = 0;  // drop_graphs calls so far (graphs of others that embed this system)
   hipEvent_t ev[8];
   double t_force = 0, t_list = 0, t_integ = 0, t_step = 0;
   int steps_done = 0;
@@ -385,6 +386,7 @@ int tile_set_lds_limit(int bytes);
 int use_row_grid(System &s);
 // destroys the captured step graphs (their launches embed the arguments)
 void drop_graphs(System &s);
+int graph_steps();  // steps per captured graph (G6_GRAPH_STEPS, default 50; 0: none)
 
 // The host-array entry points stage their arrays in one device buffer per
 // handle (grown on demand, freed by g6_finalize), so that no call allocates

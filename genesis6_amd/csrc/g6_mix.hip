@@ -48,9 +48,11 @@ struct CrossArgs {
   double L[3], invL[3];
   double rc2, rin, rout, c1, c2;
   StdTable t;
+  signed char term[kMaxSite][kMaxSite];  // index into t of site pair (a, b), -1 none
 };
 
 constexpr int kMixThreads = 256;  // four waves, one molecule each
+constexpr int kMixChunk = 256;    // candidates of one COM-test pass (four per lane)
 
 // site offset q (site * 3 + axis) of a record: FP64 records hold them as
 // doubles after the COM, FP32 records as floats (the pair precision's
@@ -59,16 +61,21 @@ __device__ inline double site_off(const double *rec, int rs, int q) {
   return rs == kRecF ? (double)reinterpret_cast<const float *>(rec + 4)[q] : rec[4 + q];
 }
 
-// One wave per molecule i of the row component; lane l takes the molecules
-// j = l, l + 64, ... of the other component in index order: the COM test
-// dd < rc^2 (minimum image per axis), S and dS on the COM distance,
-// StdForce's site terms.  Force, lab torque, energy, virial and pair count
-// are summed per lane, then over the wave in a fixed order (lane 63 holds
-// the totals); i's force and torque are added to its force array, and the
-// block's energy, virial and count (the four waves in order, halved: every
-// pair is met from both sides) go to its partials.
+// One wave per molecule i of the row component.  The other component's
+// molecules are taken in chunks of kMixChunk in index order: the COM test
+// dd < rc^2 (minimum image per axis) on all of them, the pairs that pass
+// compacted into LDS in index order, then evaluated densely over the lanes
+// (S and dS on the COM distance, StdForce's site terms), so a wave runs the
+// site terms once per 64 pairs within rc rather than once per 64
+// candidates.  Force, lab torque, energy, virial and pair count are summed
+// per lane, then over the wave in a fixed order (lane 63 holds the totals);
+// i's force and torque are added to its force array, and the block's
+// energy, virial and count (the four waves in order, halved: every pair is
+// met from both sides) go to its partials.
 __global__ __launch_bounds__(kMixThreads) void k_mix_cross(CrossArgs x) {
   __shared__ double s_v[kMixThreads / 64][12];
+  __shared__ int s_j[kMixThreads / 64][kMixChunk];
+  __shared__ double s_d[kMixThreads / 64][3 * kMixChunk];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int i = blockIdx.x * (kMixThreads / 64) + wid;
   double acc[12];
@@ -76,20 +83,59 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_cross(CrossArgs x) {
   for (int q = 0; q < 12; q++) acc[q] = 0.0;
   double F[3] = {0, 0, 0}, Tq[3] = {0, 0, 0};
   if (i < x.ni) {  // uniform over the wave
+    // site offsets in registers: every site index below is a compile-time
+    // constant (unrolled loops over kMaxSite, the term table uniform); the
+    // CoM site and the sites past a model's count are zero
     const double *ri = x.reci + (size_t)i * x.rs;
     double si[kMaxSite][3];
-    for (int s = 0; s < kMaxSite; s++) si[s][0] = si[s][1] = si[s][2] = 0.0;
-    for (int s = 0; s + 1 < x.nsi; s++)
-      for (int a = 0; a < 3; a++) si[s][a] = site_off(ri, x.rs, s * 3 + a);
-    for (int j = lane; j < x.nj; j += 64) {
-      const double *rj = x.recj + (size_t)j * x.rs;
-      double d[3];
-      for (int a = 0; a < 3; a++) {
-        d[a] = ri[a] - rj[a];
-        d[a] = d[a] - round(d[a] * x.invL[a]) * x.L[a];
+#pragma unroll
+    for (int s = 0; s < kMaxSite; s++)
+#pragma unroll
+      for (int a = 0; a < 3; a++) si[s][a] = s + 1 < x.nsi ? site_off(ri, x.rs, s * 3 + a) : 0.0;
+    const double xi = ri[0], yi = ri[1], zi = ri[2];
+    int *cj = s_j[wid];
+    double *cd = s_d[wid];
+    for (int j0 = 0; j0 < x.nj; j0 += kMixChunk) {
+    // pass 1: the COM test on the chunk's candidates, four per lane with
+    // their loads in flight together; the pairs within rc are compacted in
+    // index order into this wave's LDS rows
+    int np = 0;
+    {
+      double xj[4], yj[4], zj[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int j = j0 + 64 * u + lane;
+        const double *rj = x.recj + (size_t)(j < x.nj ? j : 0) * x.rs;
+        xj[u] = rj[0];
+        yj[u] = rj[1];
+        zj[u] = rj[2];
       }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int j = j0 + 64 * u + lane;
+        double d0 = xi - xj[u], d1 = yi - yj[u], d2 = zi - zj[u];
+        d0 = d0 - round(d0 * x.invL[0]) * x.L[0];
+        d1 = d1 - round(d1 * x.invL[1]) * x.L[1];
+        d2 = d2 - round(d2 * x.invL[2]) * x.L[2];
+        const bool pass = j < x.nj && d0 * d0 + d1 * d1 + d2 * d2 < x.rc2;
+        const unsigned long long mask = __ballot(pass);
+        if (pass) {
+          const int pos = np + __popcll(mask & ((1ull << lane) - 1ull));
+          cj[pos] = j;
+          cd[pos] = d0;
+          cd[kMixChunk + pos] = d1;
+          cd[2 * kMixChunk + pos] = d2;
+        }
+        np += __popcll(mask);
+      }
+    }
+    wsync();
+    // pass 2: the pairs within rc, densely over the lanes
+    for (int k = lane; k < np; k += 64) {
+      const int j = cj[k];
+      const double *rj = x.recj + (size_t)j * x.rs;
+      const double d[3] = {cd[k], cd[kMixChunk + k], cd[2 * kMixChunk + k]};
       const double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-      if (!(dd < x.rc2)) continue;
       // cutoff_func (CutoffModule.F90:38-59) on the COM distance
       double hh = 1.0, dh = 0.0;
       if (dd > x.rin * x.rin) {
@@ -97,18 +143,26 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_cross(CrossArgs x) {
         hh = x.c1 * uu * u * (10.0 * ll - 5.0 * u * l + uu);
         dh = -(x.c2 * uu * ll / dr);
       }
+      double sj[kMaxSite][3];
+#pragma unroll
+      for (int s = 0; s < kMaxSite; s++)
+#pragma unroll
+        for (int a = 0; a < 3; a++) sj[s][a] = s + 1 < x.nsj ? site_off(rj, x.rs, s * 3 + a) : 0.0;
       double fs[kMaxSite][3];
+#pragma unroll
       for (int s = 0; s < kMaxSite; s++) fs[s][0] = fs[s][1] = fs[s][2] = 0.0;
       double ep = 0.0;
-      for (int u = 0; u < x.t.nt; u++) {
+      // the terms in table order (site a of i outer, b of j inner)
+#pragma unroll
+      for (int a = 0; a < kMaxSite; a++)
+#pragma unroll
+      for (int b = 0; b < kMaxSite; b++) {
+        const int u = x.term[a][b];
+        if (u < 0) continue;
         const StdTerm &tm = x.t.t[u];
-        const int a = tm.a, b = tm.b;
-        double dx = d[0] + si[a][0], dy = d[1] + si[a][1], dz = d[2] + si[a][2];
-        if (b + 1 < x.nsj) {
-          dx = dx - site_off(rj, x.rs, b * 3);
-          dy = dy - site_off(rj, x.rs, b * 3 + 1);
-          dz = dz - site_off(rj, x.rs, b * 3 + 2);
-        }
+        const double dx = (d[0] + si[a][0]) - sj[b][0];
+        const double dy = (d[1] + si[a][1]) - sj[b][1];
+        const double dz = (d[2] + si[a][2]) - sj[b][2];
         const double inv2 = 1.0 / (dx * dx + dy * dy + dz * dz);
         double e0 = 0.0, f0 = 0.0;
         if (tm.kind != 3) {
@@ -127,7 +181,8 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_cross(CrossArgs x) {
         fs[a][2] += dz * f0;
       }
       double fc[3] = {0, 0, 0}, tc[3] = {0, 0, 0};
-      for (int s = 0; s < x.nsi; s++) {
+#pragma unroll
+      for (int s = 0; s < kMaxSite; s++) {
         fc[0] += fs[s][0];
         fc[1] += fs[s][1];
         fc[2] += fs[s][2];
@@ -146,6 +201,8 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_cross(CrossArgs x) {
       for (int a = 0; a < 3; a++)
         for (int c = 0; c < 3; c++) acc[1 + 3 * a + c] = fma(d[a], Fp[c], acc[1 + 3 * a + c]);
       acc[10] += 1.0;
+    }
+    wsync();  // the next chunk overwrites the rows
     }
   }
 #pragma unroll
@@ -320,7 +377,20 @@ struct Mix {
   double *part = nullptr;    // [2][nblk_a + nblk_b][kBP]: cross partials by parity
   double *parts = nullptr;   // [8]: ep of each pair set, ek of each component
   int nblk_a = 0, nblk_b = 0;
+  // captured step sequences by step count (four at most), valid while
+  // neither component dropped its own graphs (graph_gen) since the capture
+  hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
+  int graph_k[4] = {0, 0, 0, 0};
+  int graph_next = 0, gen_a = -1, gen_b = -1;
 };
+
+static void mix_drop_graphs(Mix &m) {
+  for (int q = 0; q < 4; q++) {
+    if (m.graph[q]) (void)hipGraphExecDestroy(m.graph[q]);
+    m.graph[q] = nullptr;
+    m.graph_k[q] = 0;
+  }
+}
 
 static std::mutex g_mxmu;
 static std::vector<Mix *> g_mix;
@@ -352,6 +422,9 @@ static CrossArgs cross_args(const System &r, const System &o, const StdTable &t,
   x.c1 = r.geo_md.c1;
   x.c2 = r.geo_md.c2;
   x.t = t;
+  for (int a = 0; a < kMaxSite; a++)
+    for (int b = 0; b < kMaxSite; b++) x.term[a][b] = -1;
+  for (int u = 0; u < t.nt; u++) x.term[t.t[u].a][t.t[u].b] = (signed char)u;
   return x;
 }
 
@@ -470,6 +543,7 @@ int g6_mixture_free(g6_mixture hm) {
   Mix *m = mix_of(hm);
   if (!m) return G6_EARG;
   (void)hipStreamSynchronize(m->a->stream);
+  mix_drop_graphs(*m);
   (void)hipFree(m->part);
   (void)hipFree(m->parts);
   std::lock_guard<std::mutex> lk(g_mxmu);
@@ -497,6 +571,13 @@ int g6_mixture_step(g6_mixture hm, int nsteps) {
     if (rg != 0) return rg;
   }
   G6_CHECK(hipStreamSynchronize(b.stream));
+  if (a.graph_gen != m->gen_a || b.graph_gen != m->gen_b) {
+    // a component changed what its launches embed (thermostat flag, pair
+    // constants, grid)
+    mix_drop_graphs(*m);
+    m->gen_a = a.graph_gen;
+    m->gen_b = b.graph_gen;
+  }
   // the launches of the second component go to the first's stream and
   // its correctors read the first's thermostat (kernels take the buffers
   // by value at launch, so its own are back in place afterwards)
@@ -506,10 +587,43 @@ int g6_mixture_step(g6_mixture hm, int nsteps) {
   b.nose_active = a.nose_active;
   b.stream = a.stream;
   b.md.nose = a.md.nose;
-  mix_steps(*m, nsteps);
+  int rc = 0;
+  const int G = graph_steps();
+  if (G == 0) {
+    mix_steps(*m, nsteps);
+  } else {
+    // G-step graphs, then one of the remaining count; graphs cached by count
+    auto run = [&](int k) -> int {
+      int slot = -1;
+      for (int q = 0; q < 4; q++)
+        if (m->graph[q] && m->graph_k[q] == k) slot = q;
+      if (slot < 0) {
+        slot = m->graph_next;
+        m->graph_next = (slot + 1) % 4;
+        if (m->graph[slot]) (void)hipGraphExecDestroy(m->graph[slot]);
+        m->graph[slot] = nullptr;
+        hipGraph_t gr;
+        if (hipStreamBeginCapture(a.stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+          return G6_EHIP;
+        mix_steps(*m, k);
+        if (hipStreamEndCapture(a.stream, &gr) != hipSuccess) return G6_EHIP;
+        const hipError_t e = hipGraphInstantiate(&m->graph[slot], gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        if (e != hipSuccess) {
+          m->graph[slot] = nullptr;
+          return G6_EHIP;
+        }
+        m->graph_k[slot] = k;
+      }
+      return hipGraphLaunch(m->graph[slot], a.stream) == hipSuccess ? 0 : G6_EHIP;
+    };
+    for (int k = 0; rc == 0 && k + G <= nsteps; k += G) rc = run(G);
+    if (rc == 0 && nsteps % G) rc = run(nsteps % G);
+  }
   b.nose_active = keep_na;
   b.stream = keep_st;
   b.md.nose = keep_nose;
+  if (rc != 0) return rc;
   G6_CHECK(hipGetLastError());
   const int ra = mix_flags(a), rb = mix_flags(b);
   a.steps_done += nsteps;
