@@ -14,7 +14,7 @@ rebuilt when a molecule has moved half of it; the FP64 rc test on every
 listed pair keeps the interacting set the reference's (rows rebuilt every
 step) exactly.
 
-Prints ONE JSON line (rank 0, N=1).  --only c2|c3|c4|c5 runs one workload
+Prints ONE JSON line (rank 0, N=1).  --only c2|c3|c4|c5|models|mix runs one workload
 (for profiling); --eager launches every kernel without graphs (the mode
 kernel traces are taken in).
 """
@@ -343,6 +343,51 @@ def models_run(steps, skin):
     return out
 
 
+def mixture_run(steps, skin):
+    """A two-component mixture (the reference's @NCMP 2 run): 3584 TIP4P
+    water and 512 OPLS methanol on a 16 x 16 x 16 simple cubic lattice of
+    spacing 3.625 A (L = 58.0 A; methanol on the sites with even indices on
+    all three axes), uniform random orientations (seed 7), @VSET 300 K over
+    both from one stream, one Nose thermostat at 300 K, rc 8.5 A switched
+    over 0.5 A, dt 2 fs.  200 untimed steps from the lattice, then `steps`
+    timed steps, for FP32 and FP64 pairs; molecule-steps/s."""
+    from genesis6_amd import g6
+    nside, a = 16, 3.625
+    L = nside * a
+    rng = np.random.default_rng(7)
+    water, meoh = [], []
+    for i in range(nside):
+        for j in range(nside):
+            for k in range(nside):
+                q = rng.standard_normal(4)
+                row = [(i + 0.5) * a, (j + 0.5) * a, (k + 0.5) * a] + list(q / np.linalg.norm(q))
+                (meoh if i % 2 == 0 and j % 2 == 0 and k % 2 == 0 else water).append(row)
+    water, meoh = np.array(water), np.array(meoh)
+    out = {"water": len(water), "methanol": len(meoh), "L_A": L, "pair_list_skin_A": skin}
+    for name, prec in (("fp32", g6.FP32), ("fp64", g6.FP64)):
+        sw = g6.System(g6.TIP4P, len(water), L, 8.5, 0.5, 0.002, prec=prec, skin=skin)
+        sm = g6.System(g6.OPLSMEOH, len(meoh), L, 8.5, 0.5, 0.002, prec=prec, skin=skin)
+        sw.upload(water[:, :3], water[:, 3:])
+        sm.upload(meoh[:, :3], meoh[:, 3:])
+        sw.set_nose(300.0, 7.5e5)
+        mx = g6.Mixture(sw, sm)
+        mx.vreset(300.0, 4246)
+        mx.step(200)
+        mx.thermo()
+        t0 = time.perf_counter()
+        mx.step(steps)
+        th, parts = mx.thermo()
+        el = time.perf_counter() - t0
+        n = len(water) + len(meoh)
+        out[name] = dict(value=n * steps / el, unit="molecule-steps/s", steps=steps,
+                         ms_per_step=el / steps * 1e3, temperature_K=float(th[3]),
+                         pairs_per_step=float(th[14]))
+        mx.close()
+        sw.close()
+        sm.close()
+    return out
+
+
 def by_move_type(plain, da):
     """Stage-1 (surrogate) passes and stage-2 (exact) acceptances of the
     delayed-acceptance chain by move type, beside the plain chain's
@@ -492,7 +537,7 @@ def main():
     ap.add_argument("--skin", type=float, default=1.8,
                     help="Verlet skin in A (0: rows rebuilt every step, as the reference); "
                          "1.8 A measured fastest for TIP4P-4096 among 0.7-3.0 A")
-    ap.add_argument("--only", choices=["c2", "c3", "c4", "c5", "models"], default=None,
+    ap.add_argument("--only", choices=["c2", "c3", "c4", "c5", "models", "mix"], default=None,
                     help="run one workload only (profiling)")
     ap.add_argument("--eager", action="store_true",
                     help="launch every kernel eagerly (no hipGraphs), as kernel traces are taken")
@@ -513,6 +558,8 @@ def main():
                          "fastest for its cell tiles among 0.6-1.8 A)")
     ap.add_argument("--model-steps", type=int, default=2000,
                     help="timed steps of SPC/E, NvdE and ST2 at 4096 molecules (0: skip)")
+    ap.add_argument("--mix-steps", type=int, default=2000,
+                    help="timed steps of the 3584 water + 512 methanol mixture (0: skip)")
     ap.add_argument("--npt-configs", type=int, default=20000,
                     help="MD configurations the surrogate is trained on (0: skip NPT MC)")
     ap.add_argument("--npt-epochs", type=int, default=10)
@@ -553,6 +600,8 @@ def main():
         elif only == "c5":
             out["npt_mc"] = npt_run(a.npt_configs, a.npt_epochs, a.npt_moves, a.npt_rounds,
                                     a.npt_k, a.npt_pvol, log)
+        elif only == "mix":
+            out["mixture_4096"] = mixture_run(a.mix_steps, a.skin)
         else:
             out["other_models_4096"] = models_run(a.model_steps, a.skin)
         print(json.dumps(out))
@@ -560,7 +609,7 @@ def main():
     legs, rl, st, th_eq = c2_run(a)
     f32 = legs["fp32"]
     fp64 = legs.get("fp64")
-    ladder = tip5p = models = npt = None
+    ladder = tip5p = models = npt = mix = None
     if only is None and a.ladder_intervals > 0:
         ladder = ladder_run(a.ladder_intervals, a.skin, tune=not a.no_tune, log=log)
         log("ladder %.4g molecule-steps/s (%d replicas, %.3f ms/step), force kernel %.1f%% of "
@@ -574,6 +623,10 @@ def main():
         models = models_run(a.model_steps, a.skin)
         log("4096 molecules, " + ", ".join(
             "%s %.4g" % (k, v["value"]) for k, v in models.items()) + " molecule-steps/s")
+    if only is None and a.mix_steps > 0:
+        mix = mixture_run(a.mix_steps, a.skin)
+        log("mixture 3584 water + 512 methanol %.4g (FP32) / %.4g (FP64) molecule-steps/s"
+            % (mix["fp32"]["value"], mix["fp64"]["value"]))
     if only is None and a.npt_configs > 0:
         npt = npt_run(a.npt_configs, a.npt_epochs, a.npt_moves, a.npt_rounds, a.npt_k,
                       a.npt_pvol, log)
@@ -632,6 +685,7 @@ def main():
         "replica_ladders": ladder,
         "tip5p_65536": tip5p,
         "other_models_4096": models,
+        "mixture_4096": mix,
     }
     print(json.dumps(out))
 

@@ -3,13 +3,14 @@
 // model table, Gear state, rows and grid), and a mixture steps both with one
 // Nose thermostat and adds the forces between them.
 //
-// Per step, on the first component's stream: both correctors; one step close
-// over both (the thermo record of the whole system: pair energies and virial
-// of the three pair sets, the kinetic energies with each component's masses
-// and moments, the temperature over all degrees of freedom, Nose_Correct and
-// Nose_Predict, Mixture.F90:2236-2286); both predictors with their binning;
-// the rows of each component (k_md_main, as a single component steps); then
-// k_mix_cross, the pairs between the components.  The second component reads
+// Per step, on the first component's stream: each component's corrector and
+// predictor with its binning (one k_md_pre launch each); one step close over
+// both (the thermo record of the whole system: pair energies and virial of
+// the three pair sets, the kinetic energies with each component's masses and
+// moments, the temperature over all degrees of freedom, Nose_Correct and
+// Nose_Predict, Mixture.F90:2236-2286); the rows of each component
+// (k_md_main, as a single component steps); then k_mix_cross, the pairs
+// between the components.  Steps are captured in hipGraphs and replayed.  The second component reads
 // the first's thermostat state, so both correctors take the same zeta.
 //
 // Pairs between components are StdForce's (StdInteraction.F90:63-630, the
@@ -53,6 +54,7 @@ struct CrossArgs {
 
 constexpr int kMixThreads = 256;  // four waves, one molecule each
 constexpr int kMixChunk = 256;    // candidates of one COM-test pass (four per lane)
+constexpr int kMixCap = 512;      // compacted pairs a wave holds before the site terms
 
 // site offset q (site * 3 + axis) of a record: FP64 records hold them as
 // doubles after the COM, FP32 records as floats (the pair precision's
@@ -64,18 +66,18 @@ __device__ inline double site_off(const double *rec, int rs, int q) {
 // One wave per molecule i of the row component.  The other component's
 // molecules are taken in chunks of kMixChunk in index order: the COM test
 // dd < rc^2 (minimum image per axis) on all of them, the pairs that pass
-// compacted into LDS in index order, then evaluated densely over the lanes
-// (S and dS on the COM distance, StdForce's site terms), so a wave runs the
-// site terms once per 64 pairs within rc rather than once per 64
-// candidates.  Force, lab torque, energy, virial and pair count are summed
+// appended to LDS rows in index order; whenever the rows are nearly full,
+// and at the end, the rows are evaluated densely over the lanes (S and dS
+// on the COM distance, StdForce's site terms), so a wave runs the site
+// terms once per 64 pairs within rc rather than once per 64 candidates.  Force, lab torque, energy, virial and pair count are summed
 // per lane, then over the wave in a fixed order (lane 63 holds the totals);
 // i's force and torque are added to its force array, and the block's
 // energy, virial and count (the four waves in order, halved: every pair is
 // met from both sides) go to its partials.
 __global__ __launch_bounds__(kMixThreads) void k_mix_cross(CrossArgs x) {
   __shared__ double s_v[kMixThreads / 64][12];
-  __shared__ int s_j[kMixThreads / 64][kMixChunk];
-  __shared__ double s_d[kMixThreads / 64][3 * kMixChunk];
+  __shared__ int s_j[kMixThreads / 64][kMixCap];
+  __shared__ double s_d[kMixThreads / 64][3 * kMixCap];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int i = blockIdx.x * (kMixThreads / 64) + wid;
   double acc[12];
@@ -95,114 +97,117 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_cross(CrossArgs x) {
     const double xi = ri[0], yi = ri[1], zi = ri[2];
     int *cj = s_j[wid];
     double *cd = s_d[wid];
-    for (int j0 = 0; j0 < x.nj; j0 += kMixChunk) {
-    // pass 1: the COM test on the chunk's candidates, four per lane with
-    // their loads in flight together; the pairs within rc are compacted in
-    // index order into this wave's LDS rows
     int np = 0;
-    {
-      double xj[4], yj[4], zj[4];
+    for (int j0 = 0; j0 < x.nj; j0 += kMixChunk) {
+      // pass 1: the COM test on the chunk's candidates, four per lane with
+      // their loads in flight together; the pairs within rc are appended in
+      // index order to this wave's LDS rows
+      {
+        double xj[4], yj[4], zj[4];
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const int j = j0 + 64 * u + lane;
-        const double *rj = x.recj + (size_t)(j < x.nj ? j : 0) * x.rs;
-        xj[u] = rj[0];
-        yj[u] = rj[1];
-        zj[u] = rj[2];
-      }
-#pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const int j = j0 + 64 * u + lane;
-        double d0 = xi - xj[u], d1 = yi - yj[u], d2 = zi - zj[u];
-        d0 = d0 - round(d0 * x.invL[0]) * x.L[0];
-        d1 = d1 - round(d1 * x.invL[1]) * x.L[1];
-        d2 = d2 - round(d2 * x.invL[2]) * x.L[2];
-        const bool pass = j < x.nj && d0 * d0 + d1 * d1 + d2 * d2 < x.rc2;
-        const unsigned long long mask = __ballot(pass);
-        if (pass) {
-          const int pos = np + __popcll(mask & ((1ull << lane) - 1ull));
-          cj[pos] = j;
-          cd[pos] = d0;
-          cd[kMixChunk + pos] = d1;
-          cd[2 * kMixChunk + pos] = d2;
+        for (int u = 0; u < 4; u++) {
+          const int j = j0 + 64 * u + lane;
+          const double *rj = x.recj + (size_t)(j < x.nj ? j : 0) * x.rs;
+          xj[u] = rj[0];
+          yj[u] = rj[1];
+          zj[u] = rj[2];
         }
-        np += __popcll(mask);
-      }
-    }
-    wsync();
-    // pass 2: the pairs within rc, densely over the lanes
-    for (int k = lane; k < np; k += 64) {
-      const int j = cj[k];
-      const double *rj = x.recj + (size_t)j * x.rs;
-      const double d[3] = {cd[k], cd[kMixChunk + k], cd[2 * kMixChunk + k]};
-      const double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-      // cutoff_func (CutoffModule.F90:38-59) on the COM distance
-      double hh = 1.0, dh = 0.0;
-      if (dd > x.rin * x.rin) {
-        const double dr = sqrt(dd), u = dr - x.rout, l = dr - x.rin, uu = u * u, ll = l * l;
-        hh = x.c1 * uu * u * (10.0 * ll - 5.0 * u * l + uu);
-        dh = -(x.c2 * uu * ll / dr);
-      }
-      double sj[kMaxSite][3];
 #pragma unroll
-      for (int s = 0; s < kMaxSite; s++)
-#pragma unroll
-        for (int a = 0; a < 3; a++) sj[s][a] = s + 1 < x.nsj ? site_off(rj, x.rs, s * 3 + a) : 0.0;
-      double fs[kMaxSite][3];
-#pragma unroll
-      for (int s = 0; s < kMaxSite; s++) fs[s][0] = fs[s][1] = fs[s][2] = 0.0;
-      double ep = 0.0;
-      // the terms in table order (site a of i outer, b of j inner)
-#pragma unroll
-      for (int a = 0; a < kMaxSite; a++)
-#pragma unroll
-      for (int b = 0; b < kMaxSite; b++) {
-        const int u = x.term[a][b];
-        if (u < 0) continue;
-        const StdTerm &tm = x.t.t[u];
-        const double dx = (d[0] + si[a][0]) - sj[b][0];
-        const double dy = (d[1] + si[a][1]) - sj[b][1];
-        const double dz = (d[2] + si[a][2]) - sj[b][2];
-        const double inv2 = 1.0 / (dx * dx + dy * dy + dz * dz);
-        double e0 = 0.0, f0 = 0.0;
-        if (tm.kind != 3) {
-          const double ec = tm.qq * sqrt(inv2);
-          e0 = ec;
-          f0 = ec * inv2;
+        for (int u = 0; u < 4; u++) {
+          const int j = j0 + 64 * u + lane;
+          double d0 = xi - xj[u], d1 = yi - yj[u], d2 = zi - zj[u];
+          d0 = d0 - round(d0 * x.invL[0]) * x.L[0];
+          d1 = d1 - round(d1 * x.invL[1]) * x.L[1];
+          d2 = d2 - round(d2 * x.invL[2]) * x.L[2];
+          const bool pass = j < x.nj && d0 * d0 + d1 * d1 + d2 * d2 < x.rc2;
+          const unsigned long long mask = __ballot(pass);
+          if (pass) {
+            const int pos = np + __popcll(mask & ((1ull << lane) - 1ull));
+            cj[pos] = j;
+            cd[pos] = d0;
+            cd[kMixCap + pos] = d1;
+            cd[2 * kMixCap + pos] = d2;
+          }
+          np += __popcll(mask);
         }
-        if (tm.kind != 2) {
-          const double d6 = inv2 * inv2 * inv2, d12 = d6 * d6;
-          e0 = e0 + tm.aa * d12 + tm.bb * d6;
-          f0 = f0 + inv2 * (12.0 * tm.aa * d12 + 6.0 * tm.bb * d6);
+      }
+      wsync();
+      // pass 2, once the rows cannot take another chunk or the candidates
+      // are done: the pairs held, densely over the lanes (np is uniform)
+      if (np <= kMixCap - kMixChunk && j0 + kMixChunk < x.nj) continue;
+      for (int k = lane; k < np; k += 64) {
+        const int j = cj[k];
+        const double *rj = x.recj + (size_t)j * x.rs;
+        const double d[3] = {cd[k], cd[kMixCap + k], cd[2 * kMixCap + k]};
+        const double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+        // cutoff_func (CutoffModule.F90:38-59) on the COM distance
+        double hh = 1.0, dh = 0.0;
+        if (dd > x.rin * x.rin) {
+          const double dr = sqrt(dd), u = dr - x.rout, l = dr - x.rin, uu = u * u, ll = l * l;
+          hh = x.c1 * uu * u * (10.0 * ll - 5.0 * u * l + uu);
+          dh = -(x.c2 * uu * ll / dr);
         }
-        ep = ep + e0;
-        fs[a][0] += dx * f0;
-        fs[a][1] += dy * f0;
-        fs[a][2] += dz * f0;
-      }
-      double fc[3] = {0, 0, 0}, tc[3] = {0, 0, 0};
+        double sj[kMaxSite][3];
 #pragma unroll
-      for (int s = 0; s < kMaxSite; s++) {
-        fc[0] += fs[s][0];
-        fc[1] += fs[s][1];
-        fc[2] += fs[s][2];
-        tc[0] += si[s][1] * fs[s][2] - si[s][2] * fs[s][1];
-        tc[1] += si[s][2] * fs[s][0] - si[s][0] * fs[s][2];
-        tc[2] += si[s][0] * fs[s][1] - si[s][1] * fs[s][0];
+        for (int s = 0; s < kMaxSite; s++)
+#pragma unroll
+          for (int a = 0; a < 3; a++) sj[s][a] = s + 1 < x.nsj ? site_off(rj, x.rs, s * 3 + a) : 0.0;
+        double fs[kMaxSite][3];
+#pragma unroll
+        for (int s = 0; s < kMaxSite; s++) fs[s][0] = fs[s][1] = fs[s][2] = 0.0;
+        double ep = 0.0;
+        // the terms in table order (site a of i outer, b of j inner)
+#pragma unroll
+        for (int a = 0; a < kMaxSite; a++)
+#pragma unroll
+        for (int b = 0; b < kMaxSite; b++) {
+          const int u = x.term[a][b];
+          if (u < 0) continue;
+          const StdTerm &tm = x.t.t[u];
+          const double dx = (d[0] + si[a][0]) - sj[b][0];
+          const double dy = (d[1] + si[a][1]) - sj[b][1];
+          const double dz = (d[2] + si[a][2]) - sj[b][2];
+          const double inv2 = 1.0 / (dx * dx + dy * dy + dz * dz);
+          double e0 = 0.0, f0 = 0.0;
+          if (tm.kind != 3) {
+            const double ec = tm.qq * sqrt(inv2);
+            e0 = ec;
+            f0 = ec * inv2;
+          }
+          if (tm.kind != 2) {
+            const double d6 = inv2 * inv2 * inv2, d12 = d6 * d6;
+            e0 = e0 + tm.aa * d12 + tm.bb * d6;
+            f0 = f0 + inv2 * (12.0 * tm.aa * d12 + 6.0 * tm.bb * d6);
+          }
+          ep = ep + e0;
+          fs[a][0] += dx * f0;
+          fs[a][1] += dy * f0;
+          fs[a][2] += dz * f0;
+        }
+        double fc[3] = {0, 0, 0}, tc[3] = {0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < kMaxSite; s++) {
+          fc[0] += fs[s][0];
+          fc[1] += fs[s][1];
+          fc[2] += fs[s][2];
+          tc[0] += si[s][1] * fs[s][2] - si[s][2] * fs[s][1];
+          tc[1] += si[s][2] * fs[s][0] - si[s][0] * fs[s][2];
+          tc[2] += si[s][0] * fs[s][1] - si[s][1] * fs[s][0];
+        }
+        const double cc = dh * ep;
+        double Fp[3];
+        for (int a = 0; a < 3; a++) {
+          Fp[a] = hh * fc[a] + cc * d[a];
+          F[a] += Fp[a];
+          Tq[a] += hh * tc[a];
+        }
+        acc[0] += ep * hh;
+        for (int a = 0; a < 3; a++)
+          for (int c = 0; c < 3; c++) acc[1 + 3 * a + c] = fma(d[a], Fp[c], acc[1 + 3 * a + c]);
+        acc[10] += 1.0;
       }
-      const double cc = dh * ep;
-      double Fp[3];
-      for (int a = 0; a < 3; a++) {
-        Fp[a] = hh * fc[a] + cc * d[a];
-        F[a] += Fp[a];
-        Tq[a] += hh * tc[a];
-      }
-      acc[0] += ep * hh;
-      for (int a = 0; a < 3; a++)
-        for (int c = 0; c < 3; c++) acc[1 + 3 * a + c] = fma(d[a], Fp[c], acc[1 + 3 * a + c]);
-      acc[10] += 1.0;
-    }
-    wsync();  // the next chunk overwrites the rows
+      wsync();  // the next chunk overwrites the rows
+      np = 0;
     }
   }
 #pragma unroll
@@ -461,20 +466,20 @@ static void mix_cross(Mix &m, int parity) {
       cross_args(*m.b, *m.a, m.ba, p + (size_t)m.nblk_a * kBP));
 }
 
-// k mixture steps, every launch on the first component's stream (the phases
-// of the barostat's step: correctors, close, predictors, forces)
+// k mixture steps, every launch on the first component's stream (the
+// phases of the single-component step: corrector and predictor, close,
+// forces)
 static void mix_steps(Mix &m, int k) {
   System &a = *m.a, &b = *m.b;
   for (int q = 0; q < k; q++) {
     const int p = q & 1;
-    if (q > 0) {
-      md_launch_pre(a, true, false, false, 0);
-      md_launch_pre(b, true, false, false, 0);
-    }
-    // clears the cells and rebuild flags of parity p before they are filled
-    mix_close(m, (q > 0 ? 1 : 0) | 2 | 4, 1 - p, 1 - p);
-    md_launch_pre(a, false, true, true, p);
-    md_launch_pre(b, false, true, true, p);
+    // correctors (from the second step on) and predictors in one launch
+    // per component, binned into the cells of parity p; the close then takes
+    // the corrected kinetic terms, and clears the cells of parity 1 - p for
+    // the next step (the predictors do not read the thermostat)
+    md_launch_pre(a, q > 0, true, true, p);
+    md_launch_pre(b, q > 0, true, true, p);
+    mix_close(m, (q > 0 ? 1 : 0) | 2 | 4, p, 1 - p);
     md_launch_main(a, nullptr, p, 0);
     md_launch_main(b, nullptr, p, 0);
     mix_cross(m, p);
