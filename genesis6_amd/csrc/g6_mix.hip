@@ -65,19 +65,19 @@ __device__ inline double site_off(const double *rec, int rs, int q) {
 
 // One wave per molecule i of the row component.  The other component's
 // molecules are taken in chunks of kMixChunk in index order: the COM test
-// dd < rc^2 (minimum image per axis) on all of them, the pairs that pass
-// appended to LDS rows in index order; whenever the rows are nearly full,
-// and at the end, the rows are evaluated densely over the lanes (S and dS
-// on the COM distance, StdForce's site terms), so a wave runs the site
-// terms once per 64 pairs within rc rather than once per 64 candidates.  Force, lab torque, energy, virial and pair count are summed
-// per lane, then over the wave in a fixed order (lane 63 holds the totals);
-// i's force and torque are added to its force array, and the block's
-// energy, virial and count (the four waves in order, halved: every pair is
-// met from both sides) go to its partials.
+// dd < rc^2 (minimum image per axis) on all of them, the indices of the
+// pairs that pass appended to this wave's LDS row in index order; whenever
+// the row is nearly full, and at the end, its pairs are evaluated densely
+// over the lanes (the COM vector again, S and dS on the COM distance,
+// StdForce's site terms), so a wave runs the site terms once per 64 pairs
+// within rc rather than once per 64 candidates.  Force, lab torque, energy,
+// virial and pair count are summed per lane, then over the wave in a fixed
+// order (lane 63 holds the totals); i's force and torque are added to its
+// force array, and the block's energy, virial and count (the four waves in
+// order, halved: every pair is met from both sides) go to its partials.
 __global__ __launch_bounds__(kMixThreads) void k_mix_cross(CrossArgs x) {
   __shared__ double s_v[kMixThreads / 64][12];
   __shared__ int s_j[kMixThreads / 64][kMixCap];
-  __shared__ double s_d[kMixThreads / 64][3 * kMixCap];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int i = blockIdx.x * (kMixThreads / 64) + wid;
   double acc[12];
@@ -96,7 +96,6 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_cross(CrossArgs x) {
       for (int a = 0; a < 3; a++) si[s][a] = s + 1 < x.nsi ? site_off(ri, x.rs, s * 3 + a) : 0.0;
     const double xi = ri[0], yi = ri[1], zi = ri[2];
     int *cj = s_j[wid];
-    double *cd = s_d[wid];
     int np = 0;
     for (int j0 = 0; j0 < x.nj; j0 += kMixChunk) {
       // pass 1: the COM test on the chunk's candidates, four per lane with
@@ -121,13 +120,7 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_cross(CrossArgs x) {
           d2 = d2 - round(d2 * x.invL[2]) * x.L[2];
           const bool pass = j < x.nj && d0 * d0 + d1 * d1 + d2 * d2 < x.rc2;
           const unsigned long long mask = __ballot(pass);
-          if (pass) {
-            const int pos = np + __popcll(mask & ((1ull << lane) - 1ull));
-            cj[pos] = j;
-            cd[pos] = d0;
-            cd[kMixCap + pos] = d1;
-            cd[2 * kMixCap + pos] = d2;
-          }
+          if (pass) cj[np + __popcll(mask & ((1ull << lane) - 1ull))] = j;
           np += __popcll(mask);
         }
       }
@@ -138,7 +131,9 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_cross(CrossArgs x) {
       for (int k = lane; k < np; k += 64) {
         const int j = cj[k];
         const double *rj = x.recj + (size_t)j * x.rs;
-        const double d[3] = {cd[k], cd[kMixCap + k], cd[2 * kMixCap + k]};
+        // the COM vector again, by pass 1's expression (the same bits)
+        double d[3] = {xi - rj[0], yi - rj[1], zi - rj[2]};
+        for (int a = 0; a < 3; a++) d[a] = d[a] - round(d[a] * x.invL[a]) * x.L[a];
         const double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
         // cutoff_func (CutoffModule.F90:38-59) on the COM distance
         double hh = 1.0, dh = 0.0;
