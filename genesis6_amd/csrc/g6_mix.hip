@@ -484,6 +484,11 @@ static void mix_steps(Mix &m, int k) {
   mix_close(m, 1 | 8, 0, (k - 1) & 1);
 }
 
+// both components still alive (neither handle finalized since the create)
+static bool mix_live(const Mix &m) {
+  return system_of(m.ha) == m.a && system_of(m.hb) == m.b;
+}
+
 static int mix_flags(System &s) {
   int fl = 0;
   G6_CHECK(hipMemcpyAsync(&fl, s.md.flags, sizeof(int), hipMemcpyDeviceToHost, s.stream));
@@ -542,7 +547,7 @@ int g6_mixture_create(g6_mixture *hm, g6_handle ha, g6_handle hb) {
 int g6_mixture_free(g6_mixture hm) {
   Mix *m = mix_of(hm);
   if (!m) return G6_EARG;
-  (void)hipStreamSynchronize(m->a->stream);
+  if (mix_live(*m)) (void)hipStreamSynchronize(m->a->stream);
   mix_drop_graphs(*m);
   (void)hipFree(m->part);
   (void)hipFree(m->parts);
@@ -554,17 +559,16 @@ int g6_mixture_free(g6_mixture hm) {
 
 int g6_mixture_vreset(g6_mixture hm, double temp, uint32_t seed) {
   Mix *m = mix_of(hm);
-  if (!m || !(temp >= 0)) return G6_EARG;
+  if (!m || !mix_live(*m) || !(temp >= 0)) return G6_EARG;
   System *s[2] = {m->a, m->b};
   return vreset_systems(s, 2, temp, seed);
 }
 
 int g6_mixture_step(g6_mixture hm, int nsteps) {
   Mix *m = mix_of(hm);
-  if (!m || nsteps < 0) return G6_EARG;
+  if (!m || nsteps < 0 || !mix_live(*m)) return G6_EARG;
   System &a = *m->a, &b = *m->b;
-  if (system_of(m->ha) != &a || system_of(m->hb) != &b || a.andersen || b.andersen)
-    return G6_EARG;
+  if (a.andersen || b.andersen) return G6_EARG;
   if (nsteps == 0) return 0;
   for (System *s : {&a, &b}) {
     const int rg = use_row_grid(*s);
@@ -633,7 +637,7 @@ int g6_mixture_step(g6_mixture hm, int nsteps) {
 
 int g6_mixture_thermo(g6_mixture hm, double *thermo, double *parts) {
   Mix *m = mix_of(hm);
-  if (!m || !thermo) return G6_EARG;
+  if (!m || !thermo || !mix_live(*m)) return G6_EARG;
   G6_CHECK(hipStreamSynchronize(m->a->stream));
   G6_CHECK(hipMemcpy(thermo, m->a->md.thermo, sizeof(double) * G6_THERMO_LEN,
                      hipMemcpyDeviceToHost));

@@ -201,3 +201,16 @@ def test_g6md_mixture_with_fast_kernels(golden):
     other = [c for c, _ in COLS if c != 2]
     err = np.abs(got[:, other] - ref_fast[:, other]).max(0) / np.abs(ref_fast[:, other]).max(0)
     assert err.max() <= 1e-12, err
+
+
+def test_mixture_refuses_calls_after_a_component_is_freed(golden):
+    f = golden("md_mixture_tip4p_meoh")
+    sw, sm, mx = mixture(f)
+    mx.step(2)
+    sm.close()
+    for call in (lambda: mx.step(1), mx.thermo, lambda: mx.vreset(300.0, 1)):
+        with pytest.raises(g6.G6Error):
+            call()
+    mx.close()
+    sw.step(1)          # the first component steps on its own again
+    sw.close()
