@@ -43,6 +43,7 @@ struct CrossArgs {
   int ni, nj, nsi, nsj;    // molecules and sites (CoM site last) of each side
   const double *reci;      // records of the row component (COM, site offsets)
   const double *recj;
+  const double *comj;      // the other component's COMs, [3][nj] (the records' values)
   int rs;                  // record stride (doubles): kRecD FP64 offsets, kRecF FP32
   double *frc;             // the row component's force and lab torque [6][ni]
   double *part;            // [nblk][kBP] pair partials of the launch
@@ -105,11 +106,10 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_cross(CrossArgs x) {
         double xj[4], yj[4], zj[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-          const int j = j0 + 64 * u + lane;
-          const double *rj = x.recj + (size_t)(j < x.nj ? j : 0) * x.rs;
-          xj[u] = rj[0];
-          yj[u] = rj[1];
-          zj[u] = rj[2];
+          const int jr = j0 + 64 * u + lane, j = jr < x.nj ? jr : 0;
+          xj[u] = x.comj[j];
+          yj[u] = x.comj[x.nj + j];
+          zj[u] = x.comj[2 * x.nj + j];
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) {
@@ -132,7 +132,7 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_cross(CrossArgs x) {
         const int j = cj[k];
         const double *rj = x.recj + (size_t)j * x.rs;
         // the COM vector again, by pass 1's expression (the same bits)
-        double d[3] = {xi - rj[0], yi - rj[1], zi - rj[2]};
+        double d[3] = {xi - x.comj[j], yi - x.comj[x.nj + j], zi - x.comj[2 * x.nj + j]};
         for (int a = 0; a < 3; a++) d[a] = d[a] - round(d[a] * x.invL[a]) * x.L[a];
         const double dd = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
         // cutoff_func (CutoffModule.F90:38-59) on the COM distance
@@ -409,6 +409,7 @@ static CrossArgs cross_args(const System &r, const System &o, const StdTable &t,
   x.nsj = o.mt.nsite;
   x.reci = r.md.rec;
   x.recj = o.md.rec;
+  x.comj = o.md.com;
   x.rs = r.md.rs;
   x.frc = r.md.frc;
   x.part = part;
