@@ -76,29 +76,45 @@ __device__ inline double site_off(const double *rec, int rs, int q) {
 // order (lane 63 holds the totals); i's force and torque are added to its
 // force array, and the block's energy, virial and count (the four waves in
 // order, halved: every pair is met from both sides) go to its partials.
+// sum over the LPM lanes of a molecule's group in a fixed order (the whole
+// wave for 64; a 16-lane row for 16, every lane of the row receiving it)
+template <int LPM>
+__device__ inline double group_sum(double v) {
+  if constexpr (LPM == 64) return wsum63(v);
+  else return rowsum16(v);
+}
+
+template <int LPM>
 __global__ __launch_bounds__(kMixThreads) void k_mix_cross(CrossArgs x) {
-  __shared__ double s_v[kMixThreads / 64][12];
+  static_assert(LPM == 64 || LPM == 16, "a wave or a 16-lane row per molecule");
+  constexpr int MPW = 64 / LPM;        // molecules per wave
+  constexpr int CAP = kMixCap / MPW;   // LDS row entries per molecule
+  constexpr int CH = 4 * LPM;          // candidates per pass per molecule
+  __shared__ double s_v[kMixThreads / 64 * MPW][12];
   __shared__ int s_j[kMixThreads / 64][kMixCap];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const int i = blockIdx.x * (kMixThreads / 64) + wid;
+  const int grp = lane / LPM, sub = lane % LPM;
+  const int slot = wid * MPW + grp;
+  const int i = blockIdx.x * (kMixThreads / 64 * MPW) + slot;
+  const bool live = i < x.ni;
   double acc[12];
 #pragma unroll
   for (int q = 0; q < 12; q++) acc[q] = 0.0;
   double F[3] = {0, 0, 0}, Tq[3] = {0, 0, 0};
-  if (i < x.ni) {  // uniform over the wave
+  if (__ballot(live) != 0) {  // uniform over the wave; groups past ni find no pairs
     // site offsets in registers: every site index below is a compile-time
     // constant (unrolled loops over kMaxSite, the term table uniform); the
     // CoM site and the sites past a model's count are zero
-    const double *ri = x.reci + (size_t)i * x.rs;
+    const double *ri = x.reci + (size_t)(live ? i : 0) * x.rs;
     double si[kMaxSite][3];
 #pragma unroll
     for (int s = 0; s < kMaxSite; s++)
 #pragma unroll
       for (int a = 0; a < 3; a++) si[s][a] = s + 1 < x.nsi ? site_off(ri, x.rs, s * 3 + a) : 0.0;
     const double xi = ri[0], yi = ri[1], zi = ri[2];
-    int *cj = s_j[wid];
-    int np = 0;
-    for (int j0 = 0; j0 < x.nj; j0 += kMixChunk) {
+    int *cj = s_j[wid] + grp * CAP;
+    int np = 0;  // this group's entries (equal over its lanes)
+    for (int j0 = 0; j0 < x.nj; j0 += CH) {
       // pass 1: the COM test on the chunk's candidates, four per lane with
       // their loads in flight together; the pairs within rc are appended in
       // index order to this wave's LDS rows
@@ -106,29 +122,31 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_cross(CrossArgs x) {
         double xj[4], yj[4], zj[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-          const int jr = j0 + 64 * u + lane, j = jr < x.nj ? jr : 0;
+          const int jr = j0 + LPM * u + sub, j = jr < x.nj ? jr : 0;
           xj[u] = x.comj[j];
           yj[u] = x.comj[x.nj + j];
           zj[u] = x.comj[2 * x.nj + j];
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-          const int j = j0 + 64 * u + lane;
+          const int j = j0 + LPM * u + sub;
           double d0 = xi - xj[u], d1 = yi - yj[u], d2 = zi - zj[u];
           d0 = d0 - round(d0 * x.invL[0]) * x.L[0];
           d1 = d1 - round(d1 * x.invL[1]) * x.L[1];
           d2 = d2 - round(d2 * x.invL[2]) * x.L[2];
-          const bool pass = j < x.nj && d0 * d0 + d1 * d1 + d2 * d2 < x.rc2;
-          const unsigned long long mask = __ballot(pass);
-          if (pass) cj[np + __popcll(mask & ((1ull << lane) - 1ull))] = j;
+          const bool pass = live && j < x.nj && d0 * d0 + d1 * d1 + d2 * d2 < x.rc2;
+          unsigned long long mask = __ballot(pass);
+          if constexpr (LPM < 64) mask = (mask >> (grp * LPM)) & ((1ull << LPM) - 1ull);
+          if (pass) cj[np + __popcll(mask & ((1ull << sub) - 1ull))] = j;
           np += __popcll(mask);
         }
       }
       wsync();
-      // pass 2, once the rows cannot take another chunk or the candidates
-      // are done: the pairs held, densely over the lanes (np is uniform)
-      if (np <= kMixCap - kMixChunk && j0 + kMixChunk < x.nj) continue;
-      for (int k = lane; k < np; k += 64) {
+      // pass 2, once a row cannot take another chunk or the candidates are
+      // done (a decision of the whole wave): the pairs held, densely over
+      // each group's lanes
+      if (__ballot(np > CAP - CH) == 0 && j0 + CH < x.nj) continue;
+      for (int k = sub; k < np; k += LPM) {
         const int j = cj[k];
         const double *rj = x.recj + (size_t)j * x.rs;
         // the COM vector again, by pass 1's expression (the same bits)
@@ -206,14 +224,14 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_cross(CrossArgs x) {
     }
   }
 #pragma unroll
-  for (int q = 0; q < 12; q++) acc[q] = wsum63(acc[q]);
+  for (int q = 0; q < 12; q++) acc[q] = group_sum<LPM>(acc[q]);
 #pragma unroll
   for (int a = 0; a < 3; a++) {
-    F[a] = wsum63(F[a]);
-    Tq[a] = wsum63(Tq[a]);
+    F[a] = group_sum<LPM>(F[a]);
+    Tq[a] = group_sum<LPM>(Tq[a]);
   }
-  if (lane == 63) {
-    if (i < x.ni) {
+  if (sub == LPM - 1) {
+    if (live) {
       const int ni = x.ni;
       for (int a = 0; a < 3; a++) {
         x.frc[a * ni + i] = x.frc[a * ni + i] + F[a];
@@ -221,12 +239,12 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_cross(CrossArgs x) {
       }
     }
 #pragma unroll
-    for (int q = 0; q < 12; q++) s_v[wid][q] = acc[q];
+    for (int q = 0; q < 12; q++) s_v[slot][q] = acc[q];
   }
   __syncthreads();
   if (threadIdx.x < 12) {
     double v = 0.0;
-    for (int w = 0; w < kMixThreads / 64; w++) v += s_v[w][threadIdx.x];
+    for (int w = 0; w < kMixThreads / 64 * MPW; w++) v += s_v[w][threadIdx.x];
     x.part[(size_t)blockIdx.x * kBP + threadIdx.x] = 0.5 * v;
   }
 }
@@ -377,6 +395,7 @@ struct Mix {
   double *part = nullptr;    // [2][nblk_a + nblk_b][kBP]: cross partials by parity
   double *parts = nullptr;   // [8]: ep of each pair set, ek of each component
   int nblk_a = 0, nblk_b = 0;
+  int lpm_a = 64, lpm_b = 64;  // lanes per molecule of each direction
   // captured step sequences by step count (four at most), valid while
   // neither component dropped its own graphs (graph_gen) since the capture
   hipGraphExec_t graph[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -457,9 +476,12 @@ static void mix_close(Mix &m, int mode, int parity, int src_parity) {
 
 static void mix_cross(Mix &m, int parity) {
   double *p = m.part + (size_t)parity * (m.nblk_a + m.nblk_b) * kBP;
-  k_mix_cross<<<m.nblk_a, kMixThreads, 0, m.a->stream>>>(cross_args(*m.a, *m.b, m.ab, p));
-  k_mix_cross<<<m.nblk_b, kMixThreads, 0, m.a->stream>>>(
-      cross_args(*m.b, *m.a, m.ba, p + (size_t)m.nblk_a * kBP));
+  const CrossArgs xa = cross_args(*m.a, *m.b, m.ab, p);
+  const CrossArgs xb = cross_args(*m.b, *m.a, m.ba, p + (size_t)m.nblk_a * kBP);
+  if (m.lpm_a == 16) k_mix_cross<16><<<m.nblk_a, kMixThreads, 0, m.a->stream>>>(xa);
+  else k_mix_cross<64><<<m.nblk_a, kMixThreads, 0, m.a->stream>>>(xa);
+  if (m.lpm_b == 16) k_mix_cross<16><<<m.nblk_b, kMixThreads, 0, m.a->stream>>>(xb);
+  else k_mix_cross<64><<<m.nblk_b, kMixThreads, 0, m.a->stream>>>(xb);
 }
 
 // k mixture steps, every launch on the first component's stream (the
@@ -529,9 +551,15 @@ int g6_mixture_create(g6_mixture *hm, g6_handle ha, g6_handle hb) {
   m->b = b;
   make_std_table(a->mt, b->mt, &m->ab);
   make_std_table(b->mt, a->mt, &m->ba);
-  constexpr int per = kMixThreads / 64;  // molecules per block (a wave each)
-  m->nblk_a = (a->n + per - 1) / per;
-  m->nblk_b = (b->n + per - 1) / per;
+  // a 16-lane row per molecule where a molecule has few partners of the
+  // other component within rc (expected count nj (4/3) pi rc^3 / V below
+  // 24), else a wave
+  const double vol = box_volume(a->Lb), sph = 4.0 / 3.0 * 3.14159265358979323846 * a->rc * a->rc * a->rc;
+  m->lpm_a = b->n * sph / vol < 24.0 ? 16 : 64;
+  m->lpm_b = a->n * sph / vol < 24.0 ? 16 : 64;
+  const int per_a = kMixThreads / 64 * (64 / m->lpm_a), per_b = kMixThreads / 64 * (64 / m->lpm_b);
+  m->nblk_a = (a->n + per_a - 1) / per_a;
+  m->nblk_b = (b->n + per_b - 1) / per_b;
   if (hipMalloc(&m->part, sizeof(double) * 2 * (m->nblk_a + m->nblk_b) * kBP) != hipSuccess ||
       hipMalloc(&m->parts, sizeof(double) * 8) != hipSuccess) {
     (void)hipFree(m->part);
