@@ -24,6 +24,7 @@
 // and the step is deterministic), and adds each molecule's force and lab
 // torque to its component's force array.
 #include <math.h>
+#include <stdlib.h>
 
 #include <initializer_list>
 #include <mutex>
@@ -84,19 +85,29 @@ __device__ inline double group_sum(double v) {
   else return rowsum16(v);
 }
 
+// LPM: lanes per molecule.  16: four molecules per wave (a 16-lane row
+// each); 64: a wave per molecule; 256: a block per molecule, each wave
+// taking a quarter of the candidates (contiguous index ranges) and the four
+// wave totals summed in wave order.
 template <int LPM>
 __global__ __launch_bounds__(kMixThreads) void k_mix_cross(CrossArgs x) {
-  static_assert(LPM == 64 || LPM == 16, "a wave or a 16-lane row per molecule");
-  constexpr int MPW = 64 / LPM;        // molecules per wave
-  constexpr int CAP = kMixCap / MPW;   // LDS row entries per molecule
-  constexpr int CH = 4 * LPM;          // candidates per pass per molecule
-  __shared__ double s_v[kMixThreads / 64 * MPW][12];
+  static_assert(LPM == 16 || LPM == 64 || LPM == 256, "16, 64 or 256 lanes per molecule");
+  constexpr bool SPLIT = LPM == 256;
+  constexpr int GL = SPLIT ? 64 : LPM;  // lanes of a molecule within a wave
+  constexpr int MPW = 64 / GL;          // molecules per wave
+  constexpr int CAP = kMixCap / MPW;    // LDS row entries per group
+  constexpr int CH = 4 * GL;            // candidates per pass per group
+  __shared__ double s_v[kMixThreads / 64 * MPW][12 + 6];
   __shared__ int s_j[kMixThreads / 64][kMixCap];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const int grp = lane / LPM, sub = lane % LPM;
+  const int grp = lane / GL, sub = lane % GL;
   const int slot = wid * MPW + grp;
-  const int i = blockIdx.x * (kMixThreads / 64 * MPW) + slot;
+  const int i = SPLIT ? (int)blockIdx.x : (int)blockIdx.x * (kMixThreads / 64 * MPW) + slot;
   const bool live = i < x.ni;
+  // this group's candidates: all of the other component, or the wave's
+  // quarter of it
+  const int jb = SPLIT ? (int)((long long)x.nj * wid / 4) : 0;
+  const int je = SPLIT ? (int)((long long)x.nj * (wid + 1) / 4) : x.nj;
   double acc[12];
 #pragma unroll
   for (int q = 0; q < 12; q++) acc[q] = 0.0;
@@ -114,7 +125,7 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_cross(CrossArgs x) {
     const double xi = ri[0], yi = ri[1], zi = ri[2];
     int *cj = s_j[wid] + grp * CAP;
     int np = 0;  // this group's entries (equal over its lanes)
-    for (int j0 = 0; j0 < x.nj; j0 += CH) {
+    for (int j0 = jb; j0 < je; j0 += CH) {
       // pass 1: the COM test on the chunk's candidates, four per lane with
       // their loads in flight together; the pairs within rc are appended in
       // index order to this wave's LDS rows
@@ -122,21 +133,21 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_cross(CrossArgs x) {
         double xj[4], yj[4], zj[4];
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-          const int jr = j0 + LPM * u + sub, j = jr < x.nj ? jr : 0;
+          const int jr = j0 + GL * u + sub, j = jr < je ? jr : 0;
           xj[u] = x.comj[j];
           yj[u] = x.comj[x.nj + j];
           zj[u] = x.comj[2 * x.nj + j];
         }
 #pragma unroll
         for (int u = 0; u < 4; u++) {
-          const int j = j0 + LPM * u + sub;
+          const int j = j0 + GL * u + sub;
           double d0 = xi - xj[u], d1 = yi - yj[u], d2 = zi - zj[u];
           d0 = d0 - round(d0 * x.invL[0]) * x.L[0];
           d1 = d1 - round(d1 * x.invL[1]) * x.L[1];
           d2 = d2 - round(d2 * x.invL[2]) * x.L[2];
-          const bool pass = live && j < x.nj && d0 * d0 + d1 * d1 + d2 * d2 < x.rc2;
+          const bool pass = live && j < je && d0 * d0 + d1 * d1 + d2 * d2 < x.rc2;
           unsigned long long mask = __ballot(pass);
-          if constexpr (LPM < 64) mask = (mask >> (grp * LPM)) & ((1ull << LPM) - 1ull);
+          if constexpr (GL < 64) mask = (mask >> (grp * GL)) & ((1ull << GL) - 1ull);
           if (pass) cj[np + __popcll(mask & ((1ull << sub) - 1ull))] = j;
           np += __popcll(mask);
         }
@@ -145,8 +156,8 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_cross(CrossArgs x) {
       // pass 2, once a row cannot take another chunk or the candidates are
       // done (a decision of the whole wave): the pairs held, densely over
       // each group's lanes
-      if (__ballot(np > CAP - CH) == 0 && j0 + CH < x.nj) continue;
-      for (int k = sub; k < np; k += LPM) {
+      if (__ballot(np > CAP - CH) == 0 && j0 + CH < je) continue;
+      for (int k = sub; k < np; k += GL) {
         const int j = cj[k];
         const double *rj = x.recj + (size_t)j * x.rs;
         // the COM vector again, by pass 1's expression (the same bits)
@@ -224,14 +235,14 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_cross(CrossArgs x) {
     }
   }
 #pragma unroll
-  for (int q = 0; q < 12; q++) acc[q] = group_sum<LPM>(acc[q]);
+  for (int q = 0; q < 12; q++) acc[q] = group_sum<GL>(acc[q]);
 #pragma unroll
   for (int a = 0; a < 3; a++) {
-    F[a] = group_sum<LPM>(F[a]);
-    Tq[a] = group_sum<LPM>(Tq[a]);
+    F[a] = group_sum<GL>(F[a]);
+    Tq[a] = group_sum<GL>(Tq[a]);
   }
-  if (sub == LPM - 1) {
-    if (live) {
+  if (sub == GL - 1) {
+    if (live && !SPLIT) {
       const int ni = x.ni;
       for (int a = 0; a < 3; a++) {
         x.frc[a * ni + i] = x.frc[a * ni + i] + F[a];
@@ -240,12 +251,23 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_cross(CrossArgs x) {
     }
 #pragma unroll
     for (int q = 0; q < 12; q++) s_v[slot][q] = acc[q];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      s_v[slot][12 + a] = F[a];
+      s_v[slot][15 + a] = Tq[a];
+    }
   }
   __syncthreads();
   if (threadIdx.x < 12) {
     double v = 0.0;
     for (int w = 0; w < kMixThreads / 64 * MPW; w++) v += s_v[w][threadIdx.x];
     x.part[(size_t)blockIdx.x * kBP + threadIdx.x] = 0.5 * v;
+  } else if (SPLIT && threadIdx.x < 18 && live) {
+    // the molecule's force and torque: the four waves' totals in order
+    const int q = threadIdx.x - 12, ni = x.ni;
+    double v = 0.0;
+    for (int w = 0; w < 4; w++) v += s_v[w][12 + q];
+    x.frc[q * ni + i] = x.frc[q * ni + i] + v;
   }
 }
 
@@ -478,10 +500,13 @@ static void mix_cross(Mix &m, int parity) {
   double *p = m.part + (size_t)parity * (m.nblk_a + m.nblk_b) * kBP;
   const CrossArgs xa = cross_args(*m.a, *m.b, m.ab, p);
   const CrossArgs xb = cross_args(*m.b, *m.a, m.ba, p + (size_t)m.nblk_a * kBP);
-  if (m.lpm_a == 16) k_mix_cross<16><<<m.nblk_a, kMixThreads, 0, m.a->stream>>>(xa);
-  else k_mix_cross<64><<<m.nblk_a, kMixThreads, 0, m.a->stream>>>(xa);
-  if (m.lpm_b == 16) k_mix_cross<16><<<m.nblk_b, kMixThreads, 0, m.a->stream>>>(xb);
-  else k_mix_cross<64><<<m.nblk_b, kMixThreads, 0, m.a->stream>>>(xb);
+  auto launch = [&](int lpm, int nblk, const CrossArgs &x) {
+    if (lpm == 16) k_mix_cross<16><<<nblk, kMixThreads, 0, m.a->stream>>>(x);
+    else if (lpm == 256) k_mix_cross<256><<<nblk, kMixThreads, 0, m.a->stream>>>(x);
+    else k_mix_cross<64><<<nblk, kMixThreads, 0, m.a->stream>>>(x);
+  };
+  launch(m.lpm_a, m.nblk_a, xa);
+  launch(m.lpm_b, m.nblk_b, xb);
 }
 
 // k mixture steps, every launch on the first component's stream (the
@@ -555,9 +580,18 @@ int g6_mixture_create(g6_mixture *hm, g6_handle ha, g6_handle hb) {
   // other component within rc (expected count nj (4/3) pi rc^3 / V below
   // 24), else a wave
   const double vol = box_volume(a->Lb), sph = 4.0 / 3.0 * 3.14159265358979323846 * a->rc * a->rc * a->rc;
-  m->lpm_a = b->n * sph / vol < 24.0 ? 16 : 64;
-  m->lpm_b = a->n * sph / vol < 24.0 ? 16 : 64;
-  const int per_a = kMixThreads / 64 * (64 / m->lpm_a), per_b = kMixThreads / 64 * (64 / m->lpm_b);
+  // (24 or more: a wave; and a block, its waves over quarters of the
+  // candidates, where the other component has 2048 molecules or more)
+  auto lpm = [&](int nj) { return nj * sph / vol < 24.0 ? 16 : nj >= 2048 ? 256 : 64; };
+  m->lpm_a = lpm(b->n);
+  m->lpm_b = lpm(a->n);
+  // G6_MIX_LPM (16, 64 or 256) sets both directions' form, to test each
+  if (const char *e = getenv("G6_MIX_LPM")) {
+    const int f = atoi(e);
+    if (f == 16 || f == 64 || f == 256) m->lpm_a = m->lpm_b = f;
+  }
+  auto per = [](int l) { return l == 256 ? 1 : kMixThreads / 64 * (64 / l); };
+  const int per_a = per(m->lpm_a), per_b = per(m->lpm_b);
   m->nblk_a = (a->n + per_a - 1) / per_a;
   m->nblk_b = (b->n + per_b - 1) / per_b;
   if (hipMalloc(&m->part, sizeof(double) * 2 * (m->nblk_a + m->nblk_b) * kBP) != hipSuccess ||

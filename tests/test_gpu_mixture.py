@@ -214,3 +214,25 @@ def test_mixture_refuses_calls_after_a_component_is_freed(golden):
     mx.close()
     sw.step(1)          # the first component steps on its own again
     sw.close()
+
+
+@pytest.mark.parametrize("lpm", ["16", "64", "256"])
+def test_every_cross_kernel_form_follows_the_reference(golden, monkeypatch, lpm):
+    """The cross kernel's three forms (a 16-lane row, a wave, or a block of
+    four waves per molecule; chosen by partner count and size, set here by
+    G6_MIX_LPM) each give the reference's log."""
+    f = golden("md_mixture_tip4p_meoh")
+    ref = f["logd"]
+    monkeypatch.setenv("G6_MIX_LPM", lpm)
+    sw, sm, mx = mixture(f)
+    monkeypatch.delenv("G6_MIX_LPM")
+    print("mixture, cross kernel form", lpm, flush=True)
+    got = []
+    for k in range(10):
+        mx.step(1)
+        t, p = mx.thermo()
+        got.append([k] + logd_row(t, p, sw.n + sm.n))
+    close_to(np.array(got), ref[:10])
+    mx.close()
+    sw.close()
+    sm.close()
