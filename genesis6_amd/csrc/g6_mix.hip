@@ -19,10 +19,10 @@
 // (Lorentz-Berthelot: sqrt(eps_a eps_b), (sig_a + sig_b)/2, q_a q_b), times
 // the switch S on the COM distance, with the switch's derivative on the COMs.
 // k_mix_cross takes every molecule of one component against every molecule
-// of the other (one wave per molecule, its lanes over the other component's
-// molecules in index order, both directions, so every sum has a fixed order
-// and the step is deterministic), and adds each molecule's force and lab
-// torque to its component's force array.
+// of the other (a 16-lane row, a wave or a block per molecule, its lanes over
+// the other component's molecules in index order; both directions, every sum
+// in a fixed order, so the step is deterministic), and adds each molecule's
+// force and lab torque to its component's force array.
 #include <math.h>
 #include <stdlib.h>
 
@@ -55,7 +55,6 @@ struct CrossArgs {
 };
 
 constexpr int kMixThreads = 256;  // four waves, one molecule each
-constexpr int kMixChunk = 256;    // candidates of one COM-test pass (four per lane)
 constexpr int kMixCap = 512;      // compacted pairs a wave holds before the site terms
 
 // site offset q (site * 3 + axis) of a record: FP64 records hold them as
@@ -65,18 +64,20 @@ __device__ inline double site_off(const double *rec, int rs, int q) {
   return rs == kRecF ? (double)reinterpret_cast<const float *>(rec + 4)[q] : rec[4 + q];
 }
 
-// One wave per molecule i of the row component.  The other component's
-// molecules are taken in chunks of kMixChunk in index order: the COM test
-// dd < rc^2 (minimum image per axis) on all of them, the indices of the
-// pairs that pass appended to this wave's LDS row in index order; whenever
-// the row is nearly full, and at the end, its pairs are evaluated densely
-// over the lanes (the COM vector again, S and dS on the COM distance,
-// StdForce's site terms), so a wave runs the site terms once per 64 pairs
-// within rc rather than once per 64 candidates.  Force, lab torque, energy,
-// virial and pair count are summed per lane, then over the wave in a fixed
-// order (lane 63 holds the totals); i's force and torque are added to its
-// force array, and the block's energy, virial and count (the four waves in
-// order, halved: every pair is met from both sides) go to its partials.
+// For each molecule i of the row component (a group of lanes: see LPM
+// below), the other component's molecules are taken in chunks in index
+// order: the COM test dd < rc^2 (minimum image per axis) on all of them, the
+// indices of the pairs that pass appended to the group's LDS row in index
+// order; whenever a row is nearly full, and at the end, the pairs are
+// evaluated densely over the group's lanes (the COM vector again, S and dS
+// on the COM distance, StdForce's site terms), so the site terms run once
+// per group-width of pairs within rc rather than once per group-width of
+// candidates.  Force, lab torque, energy, virial and pair count are summed
+// per lane, then over the group in a fixed order; i's force and torque are
+// added to its force array, and the block's energy, virial and count (its
+// groups in order, halved: every pair is met from both sides) go to its
+// partials.
+
 // sum over the LPM lanes of a molecule's group in a fixed order (the whole
 // wave for 64; a 16-lane row for 16, every lane of the row receiving it)
 template <int LPM>
