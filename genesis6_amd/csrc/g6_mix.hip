@@ -64,6 +64,158 @@ __device__ inline double site_off(const double *rec, int rs, int q) {
   return rs == kRecF ? (double)reinterpret_cast<const float *>(rec + 4)[q] : rec[4 + q];
 }
 
+// The step close of a mixture (the single-component step close for two
+// components and the pairs between them).  mode: 1 thermo record and
+// Nose_Correct, 2 Nose_Predict, 4 clear the cells and rebuild flag of parity
+// 1 - parity of both components, 8 of both parities.  The thermo record of
+// the first component takes the whole system: energy and virial summed over
+// the pair sets (first component, second, between) in that order, as
+// Mixture.F90:2085-2090 sums ep(k) and virialp(k); kinetic energies and
+// tensors summed over the components, each with its own mass and moments
+// (Mixture.F90:2180-2212); the temperature over 6 (nA + nB) degrees of
+// freedom.  parts: ep of each set and ek of each component (internal units).
+// What the close of each component reads and writes (small enough to ride
+// in the cross kernel's arguments beside its site table).
+struct MixSide {
+  double mass, Ixx, Iyy, Izz;
+  const double *blockek;
+  int *cellres, *flags;
+  int cell_cap;
+  double half_skin2;
+};
+struct MixClose {
+  MixSide a, b;
+  double *nose, *thermo;  // the first component's
+  int na, nb, nblk_pre_a, nblk_pre_b, nblk_main_a, nblk_main_b, nblk_x, ncell_a, ncell_b;
+  const double *bpart_a, *bpart_b, *bpart_x;
+  double *parts;
+  double dt, box_vol;
+  int nose_active;
+};
+
+constexpr int kMixRed = 36 + 2 * kEkLen;
+
+__device__ inline void mix_close_body(const MixClose &c, int mode, int parity) {
+  __shared__ double s_red[4][kMixRed];
+  const int wid = threadIdx.x >> 6;
+  {
+    // pair partials of each set, then kinetic partials of each component,
+    // per thread in block order, then over the wave and the four waves
+    double v[kMixRed];
+#pragma unroll
+    for (int q = 0; q < kMixRed; q++) v[q] = 0.0;
+    if (mode & 1) {
+      for (int k = threadIdx.x; k < c.nblk_main_a; k += 256)
+#pragma unroll
+        for (int q = 0; q < 12; q++) v[q] += c.bpart_a[(size_t)k * kBP + q];
+      for (int k = threadIdx.x; k < c.nblk_main_b; k += 256)
+#pragma unroll
+        for (int q = 0; q < 12; q++) v[12 + q] += c.bpart_b[(size_t)k * kBP + q];
+      for (int k = threadIdx.x; k < c.nblk_x; k += 256)
+#pragma unroll
+        for (int q = 0; q < 12; q++) v[24 + q] += c.bpart_x[(size_t)k * kBP + q];
+      for (int k = threadIdx.x; k < c.nblk_pre_a; k += 256)
+#pragma unroll
+        for (int q = 0; q < kEkLen; q++) v[36 + q] += c.a.blockek[(size_t)k * kEkLen + q];
+      for (int k = threadIdx.x; k < c.nblk_pre_b; k += 256)
+#pragma unroll
+        for (int q = 0; q < kEkLen; q++) v[36 + kEkLen + q] += c.b.blockek[(size_t)k * kEkLen + q];
+    }
+#pragma unroll
+    for (int q = 0; q < kMixRed; q++) v[q] = wsum63(v[q]);
+    if ((threadIdx.x & 63) == 63)
+#pragma unroll
+      for (int q = 0; q < kMixRed; q++) s_red[wid][q] = v[q];
+  }
+  if (mode & 12) {
+    const int p0 = (mode & 8) ? 0 : 1 - parity, p1 = (mode & 8) ? 1 : 1 - parity;
+    for (int p = p0; p <= p1; p++)
+      for (int u = 0; u < 2; u++) {
+        const MixSide &m = u ? c.b : c.a;
+        const int ncell = u ? c.ncell_b : c.ncell_a;
+        int *cr = m.cellres + (size_t)p * ncell * m.cell_cap;
+        for (int q = threadIdx.x; q < ncell; q += 256) cr[(size_t)q * m.cell_cap] = 0;
+        if (threadIdx.x == 0 && m.half_skin2 > 0.0) m.flags[2 + p] = 0;
+      }
+  }
+  __syncthreads();
+  if (threadIdx.x < kMixRed) {
+    const int q = threadIdx.x;
+    s_red[0][q] = s_red[0][q] + s_red[1][q] + s_red[2][q] + s_red[3][q];
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const double *ra = s_red[0], *rb = s_red[0] + 12, *rx = s_red[0] + 24;
+  const double *ka = s_red[0] + 36, *kb = s_red[0] + 36 + kEkLen;
+  double *z = c.nose, *th = c.thermo;
+  const double dt = c.dt;
+  if (mode & 1) {
+    th[0] = (ra[0] + rb[0]) + rx[0];
+    for (int q = 0; q < 9; q++) th[4 + q] = (ra[1 + q] + rb[1 + q]) + rx[1 + q];
+    th[14] = (ra[10] + rb[10]) + rx[10];
+    const double nl = ra[11] + rb[11];
+    if (nl > 0) th[15] = nl;
+    // Rigid_Ek of each component (RigidModule.F90:576-599)
+    const double kd2 = 0.5 * (1.0 / (dt * dt));
+    const double ekta = ka[0] * c.a.mass * kd2, ektb = kb[0] * c.b.mass * kd2;
+    const double ekra = (ka[1] * c.a.Ixx + ka[2] * c.a.Iyy + ka[3] * c.a.Izz) * 0.5;
+    const double ekrb = (kb[1] * c.b.Ixx + kb[2] * c.b.Iyy + kb[3] * c.b.Izz) * 0.5;
+    const double ekt = ekta + ektb, ekr = ekra + ekrb;
+    const double dof = 6.0 * (c.na + c.nb);
+    const double temp = 10.0 * 2.0 * (ekt + ekr) / (G6_RGAS_MD * dof);
+    double h = (th[0] + (ekt + ekr)) * 10.0;
+    double nose_ep = 0.0, nose_ek = 0.0;
+    if (c.nose_active) {
+      const double zetaf = (temp - z[0]) / z[1] - z[2] * z[3];
+      const double dz2 = zetaf - z[4];
+      z[3] = z[3] + kG4_0 * dz2;
+      z[4] = zetaf;
+      z[5] = z[5] + kG4_2 * dz2;
+      z[6] = z[6] + kG4_3 * dz2;
+      z[7] = z[7] + kG4_4 * dz2;
+      z[8] = z[8] + z[3];
+      nose_ek = (z[3] * z[3] * z[1] * dt * dof * 0.5) * G6_RGAS_MD;
+      nose_ep = (dof * z[0] * (z[8] - 0.5 * z[3]) * dt) * G6_RGAS_MD;
+      h = h + nose_ep + nose_ek;
+    }
+    th[1] = ekt;
+    th[2] = ekr;
+    th[3] = temp;
+    th[13] = h;
+    // kinetic tensors (RigidModule.F90:617), summed over the components,
+    // and the pressure (Mixture.F90:2206-2220)
+    const double kd = 1.0 / (dt * dt);
+    const int idx[9] = {4, 5, 6, 5, 7, 8, 6, 8, 9};
+    double kt[9];
+    for (int q = 0; q < 9; q++) kt[q] = ka[idx[q]] * c.a.mass * kd + kb[idx[q]] * c.b.mass * kd;
+    for (int q = 0; q < 9; q++) th[16 + q] = kt[q];
+    const double vp = (kt[0] + th[4]) + (kt[4] + th[8]) + (kt[8] + th[12]);
+    th[25] = (vp / 3.0) * 10.0 * (1.0 / 101326.0) /
+             (c.box_vol * 6.0225e23 * (1e-10 * 1e-10 * 1e-10));
+    th[26] = c.box_vol;
+    th[27] = nose_ep;
+    th[28] = nose_ek;
+    th[29] = c.nose_active ? z[0] : 0.0;
+    th[30] = 0.0;
+    th[31] = 0.0;
+    c.parts[0] = ra[0];
+    c.parts[1] = rb[0];
+    c.parts[2] = rx[0];
+    c.parts[3] = ekta + ekra;
+    c.parts[4] = ektb + ekrb;
+  }
+  if ((mode & 2) && c.nose_active) {
+    z[3] = z[3] + z[4] + z[5] + z[6] + z[7];
+    z[4] = z[4] + 2.0 * z[5] + 3.0 * z[6] + 4.0 * z[7];
+    z[5] = z[5] + 3.0 * z[6] + 6.0 * z[7];
+    z[6] = z[6] + 4.0 * z[7];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_mix_close(MixClose c, int mode, int parity) {
+  mix_close_body(c, mode, parity);
+}
+
 // For each molecule i of the row component (a group of lanes: see LPM
 // below), the other component's molecules are taken in chunks in index
 // order: the COM test dd < rc^2 (minimum image per axis) on all of them, the
@@ -91,8 +243,16 @@ __device__ inline double group_sum(double v) {
 // taking a quarter of the candidates (contiguous index ranges) and the four
 // wave totals summed in wave order.
 template <int LPM>
-__global__ __launch_bounds__(kMixThreads) void k_mix_cross(CrossArgs x) {
+__global__ __launch_bounds__(kMixThreads) void k_mix_cross(CrossArgs x, MixClose c, int close_mode,
+                                                           int close_parity) {
   static_assert(LPM == 16 || LPM == 64 || LPM == 256, "16, 64 or 256 lanes per molecule");
+  static_assert(kMixThreads == 256, "the step close runs on 256 threads");
+  // the last block of a launch with close_mode != 0 closes the step (it
+  // touches nothing the cross blocks read or write)
+  if (close_mode && blockIdx.x == gridDim.x - 1) {
+    mix_close_body(c, close_mode, close_parity);
+    return;
+  }
   constexpr bool SPLIT = LPM == 256;
   constexpr int GL = SPLIT ? 64 : LPM;  // lanes of a molecule within a wave
   constexpr int MPW = 64 / GL;          // molecules per wave
@@ -272,144 +432,6 @@ __global__ __launch_bounds__(kMixThreads) void k_mix_cross(CrossArgs x) {
   }
 }
 
-// The step close of a mixture (the single-component step close for two
-// components and the pairs between them).  mode: 1 thermo record and
-// Nose_Correct, 2 Nose_Predict, 4 clear the cells and rebuild flag of parity
-// 1 - parity of both components, 8 of both parities.  The thermo record of
-// the first component takes the whole system: energy and virial summed over
-// the pair sets (first component, second, between) in that order, as
-// Mixture.F90:2085-2090 sums ep(k) and virialp(k); kinetic energies and
-// tensors summed over the components, each with its own mass and moments
-// (Mixture.F90:2180-2212); the temperature over 6 (nA + nB) degrees of
-// freedom.  parts: ep of each set and ek of each component (internal units).
-struct MixClose {
-  ModelTable ma, mb;
-  MdBuf a, b;
-  int na, nb, nblk_pre_a, nblk_pre_b, nblk_main_a, nblk_main_b, nblk_x, ncell_a, ncell_b;
-  const double *bpart_a, *bpart_b, *bpart_x;
-  double *parts;
-  double dt, box_vol;
-  int nose_active;
-};
-
-constexpr int kMixRed = 36 + 2 * kEkLen;
-
-__global__ __launch_bounds__(256) void k_mix_close(MixClose c, int mode, int parity) {
-  __shared__ double s_red[4][kMixRed];
-  const int wid = threadIdx.x >> 6;
-  {
-    // pair partials of each set, then kinetic partials of each component,
-    // per thread in block order, then over the wave and the four waves
-    double v[kMixRed];
-#pragma unroll
-    for (int q = 0; q < kMixRed; q++) v[q] = 0.0;
-    if (mode & 1) {
-      for (int k = threadIdx.x; k < c.nblk_main_a; k += 256)
-#pragma unroll
-        for (int q = 0; q < 12; q++) v[q] += c.bpart_a[(size_t)k * kBP + q];
-      for (int k = threadIdx.x; k < c.nblk_main_b; k += 256)
-#pragma unroll
-        for (int q = 0; q < 12; q++) v[12 + q] += c.bpart_b[(size_t)k * kBP + q];
-      for (int k = threadIdx.x; k < c.nblk_x; k += 256)
-#pragma unroll
-        for (int q = 0; q < 12; q++) v[24 + q] += c.bpart_x[(size_t)k * kBP + q];
-      for (int k = threadIdx.x; k < c.nblk_pre_a; k += 256)
-#pragma unroll
-        for (int q = 0; q < kEkLen; q++) v[36 + q] += c.a.blockek[(size_t)k * kEkLen + q];
-      for (int k = threadIdx.x; k < c.nblk_pre_b; k += 256)
-#pragma unroll
-        for (int q = 0; q < kEkLen; q++) v[36 + kEkLen + q] += c.b.blockek[(size_t)k * kEkLen + q];
-    }
-#pragma unroll
-    for (int q = 0; q < kMixRed; q++) v[q] = wsum63(v[q]);
-    if ((threadIdx.x & 63) == 63)
-#pragma unroll
-      for (int q = 0; q < kMixRed; q++) s_red[wid][q] = v[q];
-  }
-  if (mode & 12) {
-    const int p0 = (mode & 8) ? 0 : 1 - parity, p1 = (mode & 8) ? 1 : 1 - parity;
-    for (int p = p0; p <= p1; p++)
-      for (int u = 0; u < 2; u++) {
-        const MdBuf &m = u ? c.b : c.a;
-        const int ncell = u ? c.ncell_b : c.ncell_a;
-        int *cr = m.cellres + (size_t)p * ncell * m.cell_cap;
-        for (int q = threadIdx.x; q < ncell; q += 256) cr[(size_t)q * m.cell_cap] = 0;
-        if (threadIdx.x == 0 && m.half_skin2 > 0.0) m.flags[2 + p] = 0;
-      }
-  }
-  __syncthreads();
-  if (threadIdx.x < kMixRed) {
-    const int q = threadIdx.x;
-    s_red[0][q] = s_red[0][q] + s_red[1][q] + s_red[2][q] + s_red[3][q];
-  }
-  __syncthreads();
-  if (threadIdx.x != 0) return;
-  const double *ra = s_red[0], *rb = s_red[0] + 12, *rx = s_red[0] + 24;
-  const double *ka = s_red[0] + 36, *kb = s_red[0] + 36 + kEkLen;
-  double *z = c.a.nose, *th = c.a.thermo;
-  const double dt = c.dt;
-  if (mode & 1) {
-    th[0] = (ra[0] + rb[0]) + rx[0];
-    for (int q = 0; q < 9; q++) th[4 + q] = (ra[1 + q] + rb[1 + q]) + rx[1 + q];
-    th[14] = (ra[10] + rb[10]) + rx[10];
-    const double nl = ra[11] + rb[11];
-    if (nl > 0) th[15] = nl;
-    // Rigid_Ek of each component (RigidModule.F90:576-599)
-    const double kd2 = 0.5 * (1.0 / (dt * dt));
-    const double ekta = ka[0] * c.ma.mass * kd2, ektb = kb[0] * c.mb.mass * kd2;
-    const double ekra = (ka[1] * c.ma.Ixx + ka[2] * c.ma.Iyy + ka[3] * c.ma.Izz) * 0.5;
-    const double ekrb = (kb[1] * c.mb.Ixx + kb[2] * c.mb.Iyy + kb[3] * c.mb.Izz) * 0.5;
-    const double ekt = ekta + ektb, ekr = ekra + ekrb;
-    const double dof = 6.0 * (c.na + c.nb);
-    const double temp = 10.0 * 2.0 * (ekt + ekr) / (G6_RGAS_MD * dof);
-    double h = (th[0] + (ekt + ekr)) * 10.0;
-    double nose_ep = 0.0, nose_ek = 0.0;
-    if (c.nose_active) {
-      const double zetaf = (temp - z[0]) / z[1] - z[2] * z[3];
-      const double dz2 = zetaf - z[4];
-      z[3] = z[3] + kG4_0 * dz2;
-      z[4] = zetaf;
-      z[5] = z[5] + kG4_2 * dz2;
-      z[6] = z[6] + kG4_3 * dz2;
-      z[7] = z[7] + kG4_4 * dz2;
-      z[8] = z[8] + z[3];
-      nose_ek = (z[3] * z[3] * z[1] * dt * dof * 0.5) * G6_RGAS_MD;
-      nose_ep = (dof * z[0] * (z[8] - 0.5 * z[3]) * dt) * G6_RGAS_MD;
-      h = h + nose_ep + nose_ek;
-    }
-    th[1] = ekt;
-    th[2] = ekr;
-    th[3] = temp;
-    th[13] = h;
-    // kinetic tensors (RigidModule.F90:617), summed over the components,
-    // and the pressure (Mixture.F90:2206-2220)
-    const double kd = 1.0 / (dt * dt);
-    const int idx[9] = {4, 5, 6, 5, 7, 8, 6, 8, 9};
-    double kt[9];
-    for (int q = 0; q < 9; q++) kt[q] = ka[idx[q]] * c.ma.mass * kd + kb[idx[q]] * c.mb.mass * kd;
-    for (int q = 0; q < 9; q++) th[16 + q] = kt[q];
-    const double vp = (kt[0] + th[4]) + (kt[4] + th[8]) + (kt[8] + th[12]);
-    th[25] = (vp / 3.0) * 10.0 * (1.0 / 101326.0) /
-             (c.box_vol * 6.0225e23 * (1e-10 * 1e-10 * 1e-10));
-    th[26] = c.box_vol;
-    th[27] = nose_ep;
-    th[28] = nose_ek;
-    th[29] = c.nose_active ? z[0] : 0.0;
-    th[30] = 0.0;
-    th[31] = 0.0;
-    c.parts[0] = ra[0];
-    c.parts[1] = rb[0];
-    c.parts[2] = rx[0];
-    c.parts[3] = ekta + ekra;
-    c.parts[4] = ektb + ekrb;
-  }
-  if ((mode & 2) && c.nose_active) {
-    z[3] = z[3] + z[4] + z[5] + z[6] + z[7];
-    z[4] = z[4] + 2.0 * z[5] + 3.0 * z[6] + 4.0 * z[7];
-    z[5] = z[5] + 3.0 * z[6] + 6.0 * z[7];
-    z[6] = z[6] + 4.0 * z[7];
-  }
-}
 
 struct Mix {
   g6_handle ha = 0, hb = 0;
@@ -471,13 +493,27 @@ static CrossArgs cross_args(const System &r, const System &o, const StdTable &t,
   return x;
 }
 
-static void mix_close(Mix &m, int mode, int parity, int src_parity) {
+static MixSide mix_side(const System &s) {
+  MixSide d;
+  d.mass = s.mt.mass;
+  d.Ixx = s.mt.Ixx;
+  d.Iyy = s.mt.Iyy;
+  d.Izz = s.mt.Izz;
+  d.blockek = s.md.blockek;
+  d.cellres = s.md.cellres;
+  d.flags = s.md.flags;
+  d.cell_cap = s.md.cell_cap;
+  d.half_skin2 = s.md.half_skin2;
+  return d;
+}
+
+static MixClose mix_close_args(const Mix &m, int src_parity) {
   const System &a = *m.a, &b = *m.b;
   MixClose c;
-  c.ma = a.mt;
-  c.mb = b.mt;
-  c.a = a.md;
-  c.b = b.md;
+  c.a = mix_side(a);
+  c.b = mix_side(b);
+  c.nose = a.md.nose;
+  c.thermo = a.md.thermo;
   c.na = a.n;
   c.nb = b.n;
   c.nblk_pre_a = pre_waves(a.n);
@@ -494,20 +530,29 @@ static void mix_close(Mix &m, int mode, int parity, int src_parity) {
   c.dt = a.dt;
   c.box_vol = box_volume(a.Lb);
   c.nose_active = a.nose_active ? 1 : 0;
-  k_mix_close<<<1, 256, 0, m.a->stream>>>(c, mode, parity);
+  return c;
 }
 
-static void mix_cross(Mix &m, int parity) {
+static void mix_close(Mix &m, int mode, int parity, int src_parity) {
+  k_mix_close<<<1, 256, 0, m.a->stream>>>(mix_close_args(m, src_parity), mode, parity);
+}
+
+// the cross pairs of step parity p; the first launch carries one more block
+// that closes the step (mode close_mode, src parity 1 - p) when close_mode
+// is set
+static void mix_cross(Mix &m, int parity, int close_mode) {
   double *p = m.part + (size_t)parity * (m.nblk_a + m.nblk_b) * kBP;
   const CrossArgs xa = cross_args(*m.a, *m.b, m.ab, p);
   const CrossArgs xb = cross_args(*m.b, *m.a, m.ba, p + (size_t)m.nblk_a * kBP);
-  auto launch = [&](int lpm, int nblk, const CrossArgs &x) {
-    if (lpm == 16) k_mix_cross<16><<<nblk, kMixThreads, 0, m.a->stream>>>(x);
-    else if (lpm == 256) k_mix_cross<256><<<nblk, kMixThreads, 0, m.a->stream>>>(x);
-    else k_mix_cross<64><<<nblk, kMixThreads, 0, m.a->stream>>>(x);
+  const MixClose c = mix_close_args(m, 1 - parity);
+  auto launch = [&](int lpm, int nblk, const CrossArgs &x, int mode) {
+    const int g = nblk + (mode ? 1 : 0);
+    if (lpm == 16) k_mix_cross<16><<<g, kMixThreads, 0, m.a->stream>>>(x, c, mode, parity);
+    else if (lpm == 256) k_mix_cross<256><<<g, kMixThreads, 0, m.a->stream>>>(x, c, mode, parity);
+    else k_mix_cross<64><<<g, kMixThreads, 0, m.a->stream>>>(x, c, mode, parity);
   };
-  launch(m.lpm_a, m.nblk_a, xa);
-  launch(m.lpm_b, m.nblk_b, xb);
+  launch(m.lpm_a, m.nblk_a, xa, close_mode);
+  launch(m.lpm_b, m.nblk_b, xb, 0);
 }
 
 // k mixture steps, every launch on the first component's stream (the
@@ -518,15 +563,17 @@ static void mix_steps(Mix &m, int k) {
   for (int q = 0; q < k; q++) {
     const int p = q & 1;
     // correctors (from the second step on) and predictors in one launch
-    // per component, binned into the cells of parity p; the close then takes
-    // the corrected kinetic terms, and clears the cells of parity 1 - p for
-    // the next step (the predictors do not read the thermostat)
+    // per component, binned into the cells of parity p (the predictors do
+    // not read the thermostat, which the close below updates)
     md_launch_pre(a, q > 0, true, true, p);
     md_launch_pre(b, q > 0, true, true, p);
-    mix_close(m, (q > 0 ? 1 : 0) | 2 | 4, p, 1 - p);
     md_launch_main(a, nullptr, p, 0);
     md_launch_main(b, nullptr, p, 0);
-    mix_cross(m, p);
+    // the close rides in the first cross launch: it reads the correctors'
+    // kinetic partials and the previous step's pair partials, updates the
+    // thermostat (read by the next correctors) and clears the cells of
+    // parity 1 - p; the rows and cross pairs of this step use none of these
+    mix_cross(m, p, (q > 0 ? 1 : 0) | 2 | 4);
   }
   md_launch_pre(a, true, false, false, 0);
   md_launch_pre(b, true, false, false, 0);
