@@ -4,13 +4,14 @@
 // Nose thermostat and adds the forces between them.
 //
 // Per step, on the first component's stream: each component's corrector and
-// predictor with its binning (one k_md_pre launch each); one step close over
-// both (the thermo record of the whole system: pair energies and virial of
-// the three pair sets, the kinetic energies with each component's masses and
-// moments, the temperature over all degrees of freedom, Nose_Correct and
-// Nose_Predict, Mixture.F90:2236-2286); the rows of each component
-// (k_md_main, as a single component steps); then k_mix_cross, the pairs
-// between the components.  Steps are captured in hipGraphs and replayed.  The second component reads
+// predictor with its binning (one k_md_pre launch each); the rows of each
+// component (k_md_main, as a single component steps); then k_mix_cross, the
+// pairs between the components, whose first launch carries the step close
+// over both as one more block (the thermo record of the whole system: pair
+// energies and virial of the three pair sets, the kinetic energies with each
+// component's masses and moments, the temperature over all degrees of
+// freedom, Nose_Correct and Nose_Predict, Mixture.F90:2236-2286).  Steps are
+// captured in hipGraphs and replayed.  The second component reads
 // the first's thermostat state, so both correctors take the same zeta.
 //
 // Pairs between components are StdForce's (StdInteraction.F90:63-630, the
