@@ -236,3 +236,50 @@ def test_every_cross_kernel_form_follows_the_reference(golden, monkeypatch, lpm)
     mx.close()
     sw.close()
     sm.close()
+
+
+def lattice_mixture(nside=16, a=3.625, seed=7):
+    """The benchmark's mixture start: water on a simple cubic lattice, OPLS
+    methanol on the sites with even indices on all three axes."""
+    rng = np.random.default_rng(seed)
+    water, meoh = [], []
+    for i in range(nside):
+        for j in range(nside):
+            for k in range(nside):
+                q = rng.standard_normal(4)
+                row = [(i + 0.5) * a, (j + 0.5) * a, (k + 0.5) * a] + list(q / np.linalg.norm(q))
+                (meoh if i % 2 == 0 and j % 2 == 0 and k % 2 == 0 else water).append(row)
+    return np.array(water), np.array(meoh), nside * a
+
+
+def test_cross_forms_agree_at_4096_molecules(monkeypatch):
+    """3584 water + 512 methanol (58 A box): the cross kernel's three forms
+    give the same pair energies and thermo record after 20 steps within
+    1e-12, and FP32 pairs follow FP64 (T within 1e-5)."""
+    water, meoh, L = lattice_mixture()
+    out = {}
+    for name, prec, lpm in (("16", g6.FP64, "16"), ("64", g6.FP64, "64"),
+                            ("256", g6.FP64, "256"), ("fp32", g6.FP32, "")):
+        print("mixture 4096, form", name, flush=True)
+        sw = g6.System(g6.TIP4P, len(water), L, 8.5, 0.5, 0.002, prec=prec, skin=1.0)
+        sm = g6.System(g6.OPLSMEOH, len(meoh), L, 8.5, 0.5, 0.002, prec=prec, skin=1.0)
+        sw.upload(water[:, :3], water[:, 3:])
+        sm.upload(meoh[:, :3], meoh[:, 3:])
+        sw.set_nose(300.0, 7.5e5)
+        if lpm:
+            monkeypatch.setenv("G6_MIX_LPM", lpm)
+        mx = g6.Mixture(sw, sm)
+        monkeypatch.delenv("G6_MIX_LPM", raising=False)
+        mx.vreset(300.0, 4246)
+        mx.step(20)
+        out[name] = mx.thermo()
+        mx.close()
+        sw.close()
+        sm.close()
+    t0, p0 = out["64"]
+    for name in ("16", "256"):
+        t, p = out[name]
+        assert np.abs(p[:3] - p0[:3]).max() <= 1e-12 * np.abs(p0[:3]).max(), (name, p, p0)
+        assert abs(t[3] - t0[3]) <= 1e-12 * t0[3] and abs(t[13] - t0[13]) <= 1e-12 * abs(t0[13])
+    t, _ = out["fp32"]
+    assert abs(t[3] / t0[3] - 1.0) < 1e-5, (t[3], t0[3])
